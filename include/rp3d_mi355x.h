@@ -1,0 +1,195 @@
+/*
+ * rp3d_mi355x — C ABI of the MI355X-native PhysicsWorld::update() step path.
+ *
+ * The ABI is the drop-in boundary: the C++ facade in
+ * include/reactphysics3d_amd/reactphysics3d.h (namespace reactphysics3d,
+ * PhysicsCommon / PhysicsWorld / RigidBody / Collider / shapes / joints) and
+ * the Python tests and bench call only these functions.  Plain pointers,
+ * sizes and POD structs cross it; no C++ or torch types.
+ *
+ * Each entry point names the reference interface it replaces
+ * (reference = ReactPhysics3D 0.10.2, paths relative to its repository).
+ *
+ * Status convention: functions returning int return RP3DG_OK (0) on success
+ * and a negative RP3DG_E* code otherwise; rp3dg_last_error() returns the
+ * message of the last failure on the calling thread.  Object-creating
+ * functions return the new index (>= 0) or a negative code.
+ * Threading: one world / batch handle per thread; handles are not
+ * internally synchronised (the reference's PhysicsWorld is not either).
+ */
+#ifndef RP3D_MI355X_H
+#define RP3D_MI355X_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define RP3DG_OK 0
+#define RP3DG_EINVAL (-1)      /* invalid argument / index */
+#define RP3DG_EHIP (-2)        /* HIP runtime error */
+#define RP3DG_EOVERFLOW (-3)   /* device arena overflow that could not be grown */
+#define RP3DG_ESTATE (-4)      /* call not allowed in the current state */
+#define RP3DG_EUNSUPPORTED (-5)
+
+/* Body types: same order as reactphysics3d::BodyType (include/reactphysics3d/body/RigidBody.h). */
+#define RP3DG_STATIC 0
+#define RP3DG_KINEMATIC 1
+#define RP3DG_DYNAMIC 2
+
+/* Collision shape kinds (reactphysics3d::CollisionShapeName, CollisionShape.h). */
+#define RP3DG_SHAPE_SPHERE 0
+#define RP3DG_SHAPE_CAPSULE 1
+#define RP3DG_SHAPE_BOX 2
+#define RP3DG_SHAPE_CONVEX_MESH 3
+
+/* Joint kinds (reactphysics3d::JointType, constraint/Joint.h). */
+#define RP3DG_JOINT_BALLSOCKET 0
+#define RP3DG_JOINT_SLIDER 1
+#define RP3DG_JOINT_HINGE 2
+#define RP3DG_JOINT_FIXED 3
+
+typedef struct rp3dg_world rp3dg_world;
+typedef struct rp3dg_batch rp3dg_batch;
+
+/* Mirrors PhysicsWorld::WorldSettings (include/reactphysics3d/engine/PhysicsWorld.h:77-137). */
+typedef struct rp3dg_world_settings {
+    float gravity[3];                            /* default (0, -9.81, 0) */
+    float persistent_contact_distance_threshold; /* 0.03 */
+    float default_friction_coefficient;          /* 0.3 */
+    float default_bounciness;                    /* 0.5 */
+    float restitution_velocity_threshold;        /* 0.5 */
+    float cos_angle_similar_contact_manifold;    /* 0.95 */
+    uint32_t default_velocity_solver_iterations; /* 6 */
+    uint32_t default_position_solver_iterations; /* 3 */
+    int32_t is_sleeping_enabled;                 /* device path requires 0 (sleeping not on the device) */
+    int32_t is_gravity_enabled;                  /* 1 */
+} rp3dg_world_settings;
+
+/* ---- world construction (host side, replaces PhysicsCommon / PhysicsWorld factories) ---- */
+
+/* WorldSettings() defaults (PhysicsWorld.h:122-137). */
+void rp3dg_world_settings_default(rp3dg_world_settings* out);
+/* PhysicsCommon::createPhysicsWorld (src/engine/PhysicsCommon.cpp:221-240). */
+rp3dg_world* rp3dg_world_create(const rp3dg_world_settings* settings);
+/* PhysicsCommon::destroyPhysicsWorld (src/engine/PhysicsCommon.cpp:243). */
+void rp3dg_world_destroy(rp3dg_world* world);
+/* PhysicsWorld::setNbIterationsVelocitySolver / PositionSolver (src/engine/PhysicsWorld.cpp:663,957). */
+int rp3dg_world_set_iterations(rp3dg_world* world, uint32_t velocity_iterations, uint32_t position_iterations);
+/* PhysicsWorld::setGravity (src/engine/PhysicsWorld.cpp:969). */
+int rp3dg_world_set_gravity(rp3dg_world* world, const float gravity[3]);
+
+/* PhysicsWorld::createRigidBody (src/engine/PhysicsWorld.cpp:401-442); returns the body index. */
+int32_t rp3dg_body_create(rp3dg_world* world, const float position[3], const float orientation[4]);
+/* RigidBody::setType (src/body/RigidBody.cpp:66-130). */
+int rp3dg_body_set_type(rp3dg_world* world, int32_t body, int32_t type);
+/* RigidBody::setLinearVelocity / setAngularVelocity (src/body/RigidBody.cpp). */
+int rp3dg_body_set_velocity(rp3dg_world* world, int32_t body, const float linear[3], const float angular[3]);
+/* RigidBody::setLinearDamping / setAngularDamping. */
+int rp3dg_body_set_damping(rp3dg_world* world, int32_t body, float linear, float angular);
+/* RigidBody::enableGravity. */
+int rp3dg_body_enable_gravity(rp3dg_world* world, int32_t body, int32_t enabled);
+/* RigidBody::applyWorldForceAtCenterOfMass / applyLocalTorque accumulate until the next update(). */
+int rp3dg_body_apply_force_torque(rp3dg_world* world, int32_t body, const float force[3], const float torque[3]);
+/* RigidBody::setLinearLockAxisFactor / setAngularLockAxisFactor. */
+int rp3dg_body_set_lock_factors(rp3dg_world* world, int32_t body, const float linear[3], const float angular[3]);
+/* RigidBody::updateMassPropertiesFromColliders (src/body/RigidBody.cpp:548-612). */
+int rp3dg_body_update_mass_properties(rp3dg_world* world, int32_t body);
+/* RigidBody::setMass / setLocalInertiaTensor / setLocalCenterOfMass (explicit values). */
+int rp3dg_body_set_mass_properties(rp3dg_world* world, int32_t body, float mass, const float inertia_local[3],
+                                   const float com_local[3]);
+
+/* PhysicsCommon::create{Sphere,Capsule,Box}Shape + RigidBody::addCollider (src/body/RigidBody.cpp:661-721).
+ * params: sphere {radius}; capsule {radius, height}; box {hx, hy, hz}; convex mesh {hull index}.
+ * Returns the collider index. */
+int32_t rp3dg_collider_add(rp3dg_world* world, int32_t body, int32_t shape_type, const float params[4],
+                           const float local_position[3], const float local_orientation[4]);
+/* Collider::getMaterial().set{FrictionCoefficient,Bounciness,MassDensity} (engine/Material.h). */
+int rp3dg_collider_set_material(rp3dg_world* world, int32_t collider, float friction, float bounciness,
+                                float mass_density);
+/* Collider::setCollisionCategoryBits / setCollideWithMaskBits (src/collision/Collider.cpp). */
+int rp3dg_collider_set_filter(rp3dg_world* world, int32_t collider, uint16_t category, uint16_t mask);
+
+/* PhysicsCommon::createConvexMesh(PolygonVertexArray) (src/engine/PhysicsCommon.cpp:600-630):
+ * vertices xyz[nv], faces given as counts[nf] + concatenated vertex indices (CCW seen from outside).
+ * Returns the hull index used as params[0] of RP3DG_SHAPE_CONVEX_MESH. */
+int32_t rp3dg_hull_create(rp3dg_world* world, const float* vertices, int32_t nb_vertices,
+                          const int32_t* face_counts, const int32_t* face_indices, int32_t nb_faces);
+
+/* PhysicsWorld::createJoint (src/engine/PhysicsWorld.cpp:483-600) with anchor / axis given in world space
+ * (the JointInfo "world space" constructors, e.g. BallAndSocketJointInfo(body1, body2, anchor)).
+ * axis is used by hinge and slider; limits[4] = {min, max, motor speed, max motor force/torque}
+ * with flags bit0 limit enabled, bit1 motor enabled.  Returns the joint index. */
+int32_t rp3dg_joint_create(rp3dg_world* world, int32_t type, int32_t body1, int32_t body2,
+                           const float anchor_world[3], const float axis_world[3], int32_t collide_connected,
+                           int32_t flags, const float limits[4]);
+
+/* ---- stepping (replaces PhysicsWorld::update, src/engine/PhysicsWorld.cpp:239-307) ---- */
+
+/* Steps one world on the GPU (uploads it on first use, as a batch of one). */
+int rp3dg_world_update(rp3dg_world* world, float time_step);
+
+/* Many independent worlds stepped together: one launch sequence per step for all of them. */
+rp3dg_batch* rp3dg_batch_create(rp3dg_world* const* worlds, int32_t nb_worlds);
+void rp3dg_batch_destroy(rp3dg_batch* batch);
+int rp3dg_batch_step(rp3dg_batch* batch, float time_step, int32_t nb_steps);
+/* Capture one step into a hipGraph and replay it nb_steps times (no allocation, no host sync inside). */
+int rp3dg_batch_step_graph(rp3dg_batch* batch, float time_step, int32_t nb_steps);
+int rp3dg_batch_synchronize(rp3dg_batch* batch);
+
+/* ---- readback (host mirror of RigidBody getters / contact data) ---- */
+
+typedef struct rp3dg_counts {
+    int64_t nb_bodies, nb_colliders, nb_joints, nb_worlds;
+    int64_t nb_pairs;          /* overlapping pairs after the broadphase of the last step */
+    int64_t nb_manifolds;      /* contact manifolds of the last step */
+    int64_t nb_contact_points; /* contact points of the last step */
+    int64_t nb_colors;         /* solver colours used by the last step */
+    int64_t status;            /* device status word (0 = OK) */
+} rp3dg_counts;
+int rp3dg_batch_get_counts(rp3dg_batch* batch, rp3dg_counts* out);
+
+/* Body state, indices are global over the batch (world order, then body creation order).
+ * position = body origin (Transform position), orientation xyzw, velocities of the centre of mass. */
+int rp3dg_batch_get_bodies(rp3dg_batch* batch, float* position3, float* orientation4, float* linear3,
+                           float* angular3);
+int rp3dg_batch_set_bodies(rp3dg_batch* batch, const float* position3, const float* orientation4,
+                           const float* linear3, const float* angular3);
+/* Overlapping pairs (global collider indices, collider1 = reference's pair collider1), sorted by (min,max). */
+int rp3dg_batch_get_pairs(rp3dg_batch* batch, int32_t* collider1, int32_t* collider2, int64_t capacity);
+
+/* Contacts of the last step, one record per manifold in solver order.  Per manifold m:
+ * colliders[2m..2m+1], nb_points[m], color[m]; per point p (4 slots per manifold):
+ * normal[3], depth, local1[3], local2[3], impulse (accumulated normal impulse after the step). */
+typedef struct rp3dg_contact_view {
+    int32_t* colliders;     /* 2 per manifold */
+    int32_t* nb_points;     /* 1 per manifold */
+    int32_t* color;         /* 1 per manifold */
+    float* normal_depth;    /* 16 per manifold (4 points x {nx,ny,nz,depth}) */
+    float* local1;          /* 12 per manifold */
+    float* local2;          /* 12 per manifold */
+    float* impulse;         /* 4 per manifold */
+    float* friction;        /* 3 per manifold: friction1, friction2, twist impulses */
+} rp3dg_contact_view;
+int rp3dg_batch_get_contacts(rp3dg_batch* batch, rp3dg_contact_view* out, int64_t capacity);
+/* Joint solve order (joint indices in the order the device solved them this step, per type block). */
+int rp3dg_batch_get_joint_order(rp3dg_batch* batch, int32_t* order, int64_t capacity);
+
+/* Per-kernel timing with HIP events on the batch's stream (for the roofline figures).
+ * rp3dg_batch_profile(batch, 1) enables; get returns the mean launch time in ms of each named kernel. */
+int rp3dg_batch_profile(rp3dg_batch* batch, int32_t enable);
+int rp3dg_batch_get_kernel_times(rp3dg_batch* batch, const char* kernel, double* mean_ms, int64_t* launches);
+
+/* Solver convergence mode (iterate to a velocity residual instead of fixed counts). */
+int rp3dg_batch_set_residual_mode(rp3dg_batch* batch, float residual_threshold, int32_t max_iterations);
+int rp3dg_batch_get_iteration_stats(rp3dg_batch* batch, double* mean_iterations, double* max_residual);
+
+const char* rp3dg_last_error(void);
+const char* rp3dg_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* RP3D_MI355X_H */

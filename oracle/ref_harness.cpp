@@ -1,0 +1,376 @@
+// ORACLE / TEST INFRASTRUCTURE — never linked into the product library.
+//
+// C-ABI harness over the *unmodified* ReactPhysics3D 0.10.2 library compiled
+// from /root/reference (see oracle/Makefile).  It builds a reference
+// PhysicsWorld from the same flat scene description the tests give the MI355X
+// library and exposes:
+//   * rp3dref_step           : PhysicsWorld::update(dt) (src/engine/PhysicsWorld.cpp:239-307)
+//   * rp3dref_step_ordered   : the same phases, but the contact solver rows are
+//                              permuted into a caller-given manifold order before
+//                              warm starting (the "coloured-order oracle"): the
+//                              solver then runs exactly the sequence the GPU's
+//                              colour sweeps run.
+//   * dumps of bodies, overlapping pairs and contacts.
+// Compiled with -fno-access-control so that private members can be read without
+// editing any reference file.
+#include <reactphysics3d/reactphysics3d.h>
+
+#include <algorithm>
+#include <cstring>
+#include <map>
+#include <unordered_map>
+#include <vector>
+
+using namespace reactphysics3d;
+
+extern "C" {
+
+struct RefSettings {
+    float gravity[3];
+    int32_t vel_iters, pos_iters;
+    float persistent, restitution_threshold, cos_similar;
+    float default_friction, default_bounciness;
+    int32_t sleeping;
+};
+
+struct RefBody {
+    int32_t type;
+    float pos[3];
+    float quat[4];
+    float lin[3];
+    float ang[3];
+    int32_t gravity;
+    float ldamp, adamp;
+};
+
+struct RefCollider {
+    int32_t body;
+    int32_t shape;    // 0 sphere, 1 capsule, 2 box, 3 convex mesh
+    float params[4];
+    float lpos[3];
+    float lquat[4];
+    float friction, bounciness, density;
+    int32_t category, mask;
+};
+
+struct RefJoint {
+    int32_t type;     // 0 ball-socket, 1 slider, 2 hinge, 3 fixed
+    int32_t body1, body2;
+    float anchor[3];
+    float axis[3];
+    int32_t collide;
+    int32_t flags;    // bit0 limit, bit1 motor
+    float limits[4];  // min, max, motor speed, max motor force/torque
+};
+
+}  // extern "C"
+
+struct RefWorld {
+    PhysicsCommon common;
+    PhysicsWorld* world = nullptr;
+    std::vector<RigidBody*> bodies;
+    std::vector<Collider*> colliders;
+    std::vector<Joint*> joints;
+    std::unordered_map<uint32_t, int> collider_of_entity;
+    std::unordered_map<uint32_t, int> body_of_entity;
+    float last_dt = 0.f;
+};
+
+static Transform make_tf(const float* p, const float* q) {
+    return Transform(Vector3(p[0], p[1], p[2]), Quaternion(q[0], q[1], q[2], q[3]));
+}
+
+extern "C" {
+
+int rp3dref_precision_bytes() { return (int)sizeof(decimal); }
+
+void* rp3dref_create(const RefSettings* s, int nb, const RefBody* bodies, int nc, const RefCollider* cols, int nj,
+                     const RefJoint* joints, const float* hull_vertices, const int32_t* hull_offsets) {
+    (void)hull_vertices;
+    (void)hull_offsets;
+    RefWorld* R = new RefWorld();
+    PhysicsWorld::WorldSettings ws;
+    ws.gravity = Vector3(s->gravity[0], s->gravity[1], s->gravity[2]);
+    ws.persistentContactDistanceThreshold = s->persistent;
+    ws.restitutionVelocityThreshold = s->restitution_threshold;
+    ws.cosAngleSimilarContactManifold = s->cos_similar;
+    ws.defaultFrictionCoefficient = s->default_friction;
+    ws.defaultBounciness = s->default_bounciness;
+    ws.isSleepingEnabled = s->sleeping != 0;
+    ws.defaultVelocitySolverNbIterations = (uint16)s->vel_iters;
+    ws.defaultPositionSolverNbIterations = (uint16)s->pos_iters;
+    R->world = R->common.createPhysicsWorld(ws);
+    // canonical creation order (identical to the device-side scene loader):
+    // body, setType, colliders (+material, filter), mass properties, velocities
+    int ci = 0;
+    for (int b = 0; b < nb; b++) {
+        const RefBody& B = bodies[b];
+        RigidBody* body = R->world->createRigidBody(make_tf(B.pos, B.quat));
+        body->setType(B.type == 0 ? BodyType::STATIC : (B.type == 1 ? BodyType::KINEMATIC : BodyType::DYNAMIC));
+        R->bodies.push_back(body);
+        R->body_of_entity[body->getEntity().id] = b;
+        while (ci < nc && cols[ci].body == b) {
+            const RefCollider& C = cols[ci];
+            CollisionShape* shape = nullptr;
+            if (C.shape == 0) shape = R->common.createSphereShape(C.params[0]);
+            else if (C.shape == 1) shape = R->common.createCapsuleShape(C.params[0], C.params[1]);
+            else if (C.shape == 2) shape = R->common.createBoxShape(Vector3(C.params[0], C.params[1], C.params[2]));
+            else { delete R; return nullptr; }
+            Collider* col = body->addCollider(shape, make_tf(C.lpos, C.lquat));
+            col->getMaterial().setFrictionCoefficient(C.friction);
+            col->getMaterial().setBounciness(C.bounciness);
+            col->getMaterial().setMassDensity(C.density);
+            if (C.category != 0x0001) col->setCollisionCategoryBits((unsigned short)C.category);
+            if (C.mask != 0xFFFF) col->setCollideWithMaskBits((unsigned short)C.mask);
+            R->colliders.push_back(col);
+            R->collider_of_entity[col->getEntity().id] = ci;
+            ci++;
+        }
+        if (B.type == 2) body->updateMassPropertiesFromColliders();
+        if (B.type != 0) {
+            body->setLinearVelocity(Vector3(B.lin[0], B.lin[1], B.lin[2]));
+            body->setAngularVelocity(Vector3(B.ang[0], B.ang[1], B.ang[2]));
+        }
+        if (!B.gravity) body->enableGravity(false);
+        if (B.ldamp != 0.f) body->setLinearDamping(B.ldamp);
+        if (B.adamp != 0.f) body->setAngularDamping(B.adamp);
+    }
+    for (int j = 0; j < nj; j++) {
+        const RefJoint& J = joints[j];
+        RigidBody* b1 = R->bodies[J.body1];
+        RigidBody* b2 = R->bodies[J.body2];
+        const Vector3 anchor(J.anchor[0], J.anchor[1], J.anchor[2]);
+        const Vector3 axis(J.axis[0], J.axis[1], J.axis[2]);
+        Joint* joint = nullptr;
+        if (J.type == 0) {
+            BallAndSocketJointInfo info(b1, b2, anchor);
+            info.isCollisionEnabled = J.collide != 0;
+            joint = R->world->createJoint(info);
+        } else if (J.type == 1) {
+            SliderJointInfo info(b1, b2, anchor, axis);
+            info.isCollisionEnabled = J.collide != 0;
+            if (J.flags & 1) { info.isLimitEnabled = true; info.minTranslationLimit = J.limits[0]; info.maxTranslationLimit = J.limits[1]; }
+            if (J.flags & 2) { info.isMotorEnabled = true; info.motorSpeed = J.limits[2]; info.maxMotorForce = J.limits[3]; }
+            joint = R->world->createJoint(info);
+        } else if (J.type == 2) {
+            HingeJointInfo info(b1, b2, anchor, axis);
+            info.isCollisionEnabled = J.collide != 0;
+            if (J.flags & 1) { info.isLimitEnabled = true; info.minAngleLimit = J.limits[0]; info.maxAngleLimit = J.limits[1]; }
+            if (J.flags & 2) { info.isMotorEnabled = true; info.motorSpeed = J.limits[2]; info.maxMotorTorque = J.limits[3]; }
+            joint = R->world->createJoint(info);
+        } else {
+            FixedJointInfo info(b1, b2, anchor);
+            info.isCollisionEnabled = J.collide != 0;
+            joint = R->world->createJoint(info);
+        }
+        R->joints.push_back(joint);
+    }
+    return R;
+}
+
+void rp3dref_destroy(void* h) {
+    RefWorld* R = (RefWorld*)h;
+    if (!R) return;
+    R->common.destroyPhysicsWorld(R->world);
+    delete R;
+}
+
+int rp3dref_step(void* h, float dt) {
+    RefWorld* R = (RefWorld*)h;
+    R->world->update(dt);
+    R->last_dt = dt;
+    return 0;
+}
+
+static int collider_index(RefWorld* R, Entity e) {
+    auto it = R->collider_of_entity.find(e.id);
+    return it == R->collider_of_entity.end() ? -1 : it->second;
+}
+
+// ContactSolverSystem::init (src/systems/ContactSolverSystem.cpp:66-105) with the rows of each
+// manifold permuted into the order of (order_c1[i], order_c2[i]) before warmStart().
+static int solver_init_ordered(RefWorld* R, float dt, const int32_t* oc1, const int32_t* oc2, int n) {
+    PhysicsWorld& W = *R->world;
+    ContactSolverSystem& CS = W.mContactSolverSystem;
+    CollisionDetectionSystem& CD = W.mCollisionDetection;
+    Array<ContactManifold>* manifolds = CD.mCurrentContactManifolds;
+    Array<ContactPoint>* points = CD.mCurrentContactPoints;
+    CS.mAllContactManifolds = manifolds;
+    CS.mAllContactPoints = points;
+    CS.mTimeStep = dt;
+    const uint32 nbM = (uint32)manifolds->size();
+    const uint32 nbP = (uint32)points->size();
+    CS.mNbContactManifolds = 0;
+    CS.mNbContactPoints = 0;
+    CS.mContactConstraints = nullptr;
+    CS.mContactPoints = nullptr;
+    if (nbM == 0 || nbP == 0) return 0;
+    CS.mContactPoints = static_cast<ContactSolverSystem::ContactPointSolver*>(
+        W.mMemoryManager.allocate(MemoryManager::AllocationType::Frame, sizeof(ContactSolverSystem::ContactPointSolver) * nbP));
+    CS.mContactConstraints = static_cast<ContactSolverSystem::ContactManifoldSolver*>(
+        W.mMemoryManager.allocate(MemoryManager::AllocationType::Frame, sizeof(ContactSolverSystem::ContactManifoldSolver) * nbM));
+    const uint32 nbIslands = W.mIslands.getNbIslands();
+    for (uint32 i = 0; i < nbIslands; i++)
+        if (W.mIslands.nbContactManifolds[i] > 0) CS.initializeForIsland(i);
+    // permutation
+    std::map<std::pair<int, int>, int> pos;
+    for (int i = 0; i < n; i++) pos[{oc1[i], oc2[i]}] = i;
+    const uint32 m = CS.mNbContactManifolds;
+    std::vector<uint32> first(m);
+    uint32 acc = 0;
+    for (uint32 c = 0; c < m; c++) { first[c] = acc; acc += CS.mContactConstraints[c].nbContacts; }
+    std::vector<std::pair<int, uint32>> keyed(m);
+    for (uint32 c = 0; c < m; c++) {
+        const ContactManifold* em = CS.mContactConstraints[c].externalContactManifold;
+        const int a = collider_index(R, em->colliderEntity1), b = collider_index(R, em->colliderEntity2);
+        auto it = pos.find({a, b});
+        if (it == pos.end()) return -1;
+        keyed[c] = {it->second, c};
+    }
+    std::sort(keyed.begin(), keyed.end());
+    std::vector<ContactSolverSystem::ContactManifoldSolver> ms(CS.mContactConstraints, CS.mContactConstraints + m);
+    std::vector<ContactSolverSystem::ContactPointSolver> ps(CS.mContactPoints, CS.mContactPoints + CS.mNbContactPoints);
+    uint32 wp = 0;
+    for (uint32 c = 0; c < m; c++) {
+        const uint32 src = keyed[c].second;
+        CS.mContactConstraints[c] = ms[src];
+        for (int k = 0; k < ms[src].nbContacts; k++) CS.mContactPoints[wp++] = ps[first[src] + k];
+    }
+    CS.warmStart();
+    return 0;
+}
+
+// PhysicsWorld::update (src/engine/PhysicsWorld.cpp:239-307), phase by phase, with the ordered solver init.
+int rp3dref_step_ordered(void* h, float dt, const int32_t* oc1, const int32_t* oc2, int n) {
+    RefWorld* R = (RefWorld*)h;
+    PhysicsWorld& W = *R->world;
+    W.mCollisionDetection.computeCollisionDetection();
+    W.createIslands();
+    W.mCollisionDetection.createContacts();
+    W.mCollisionDetection.reportContactsAndTriggers();
+    W.updateBodiesInverseWorldInertiaTensors();
+    W.enableDisableJoints();
+    W.mDynamicsSystem.integrateRigidBodiesVelocities(dt);
+    // solveContactsAndConstraints (PhysicsWorld.cpp:321-345)
+    if (solver_init_ordered(R, dt, oc1, oc2, n) != 0) return -1;
+    W.mConstraintSolverSystem.initialize(dt);
+    for (uint32 i = 0; i < W.mNbVelocitySolverIterations; i++) {
+        W.mConstraintSolverSystem.solveVelocityConstraints();
+        W.mContactSolverSystem.solve();
+    }
+    W.mContactSolverSystem.storeImpulses();
+    W.mContactSolverSystem.reset();
+    W.mDynamicsSystem.integrateRigidBodiesPositions(dt, W.mContactSolverSystem.isSplitImpulseActive());
+    W.solvePositionCorrection();
+    W.mDynamicsSystem.updateBodiesState();
+    W.mCollisionDetection.updateColliders();
+    if (W.mIsSleepingEnabled) W.updateSleepingBodies(dt);
+    W.mDynamicsSystem.resetBodiesForceAndTorque();
+    W.mIslands.clear();
+    W.mProcessContactPairsOrderIslands.clear(true);
+    W.mMemoryManager.resetFrameAllocator();
+    R->last_dt = dt;
+    return 0;
+}
+
+// Bodies: origin position, orientation (x,y,z,w), centre-of-mass linear velocity, angular velocity.
+int rp3dref_get_bodies(void* h, float* pos, float* q, float* v, float* w) {
+    RefWorld* R = (RefWorld*)h;
+    for (size_t i = 0; i < R->bodies.size(); i++) {
+        const Transform& t = R->bodies[i]->getTransform();
+        const Vector3 lv = R->bodies[i]->getLinearVelocity(), av = R->bodies[i]->getAngularVelocity();
+        if (pos) { pos[3 * i] = (float)t.getPosition().x; pos[3 * i + 1] = (float)t.getPosition().y; pos[3 * i + 2] = (float)t.getPosition().z; }
+        if (q) {
+            q[4 * i] = (float)t.getOrientation().x; q[4 * i + 1] = (float)t.getOrientation().y;
+            q[4 * i + 2] = (float)t.getOrientation().z; q[4 * i + 3] = (float)t.getOrientation().w;
+        }
+        if (v) { v[3 * i] = (float)lv.x; v[3 * i + 1] = (float)lv.y; v[3 * i + 2] = (float)lv.z; }
+        if (w) { w[3 * i] = (float)av.x; w[3 * i + 1] = (float)av.y; w[3 * i + 2] = (float)av.z; }
+    }
+    return (int)R->bodies.size();
+}
+
+// Mass properties as computed by RigidBody::updateMassPropertiesFromColliders.
+int rp3dref_get_mass_properties(void* h, float* mass, float* inertia, float* com_local) {
+    RefWorld* R = (RefWorld*)h;
+    for (size_t i = 0; i < R->bodies.size(); i++) {
+        RigidBody* b = R->bodies[i];
+        if (mass) mass[i] = (float)b->getMass();
+        const Vector3 I = b->getLocalInertiaTensor(), c = b->getLocalCenterOfMass();
+        if (inertia) { inertia[3 * i] = (float)I.x; inertia[3 * i + 1] = (float)I.y; inertia[3 * i + 2] = (float)I.z; }
+        if (com_local) { com_local[3 * i] = (float)c.x; com_local[3 * i + 1] = (float)c.y; com_local[3 * i + 2] = (float)c.z; }
+    }
+    return (int)R->bodies.size();
+}
+
+// Fat AABBs of the broadphase tree (DynamicAABBTree node of every collider).
+int rp3dref_get_fat_aabbs(void* h, float* mn, float* mx) {
+    RefWorld* R = (RefWorld*)h;
+    for (size_t i = 0; i < R->colliders.size(); i++) {
+        const int id = R->colliders[i]->getBroadPhaseId();
+        const AABB& a = R->world->mCollisionDetection.mBroadPhaseSystem.mDynamicAABBTree.getFatAABB(id);
+        mn[3 * i] = (float)a.getMin().x; mn[3 * i + 1] = (float)a.getMin().y; mn[3 * i + 2] = (float)a.getMin().z;
+        mx[3 * i] = (float)a.getMax().x; mx[3 * i + 1] = (float)a.getMax().y; mx[3 * i + 2] = (float)a.getMax().z;
+    }
+    return (int)R->colliders.size();
+}
+
+int rp3dref_get_broadphase_ids(void* h, int32_t* ids) {
+    RefWorld* R = (RefWorld*)h;
+    for (size_t i = 0; i < R->colliders.size(); i++) ids[i] = R->colliders[i]->getBroadPhaseId();
+    return (int)R->colliders.size();
+}
+
+// Convex overlapping pairs: collider1 / collider2 as the reference oriented them.
+int rp3dref_get_pairs(void* h, int32_t* c1, int32_t* c2, int cap) {
+    RefWorld* R = (RefWorld*)h;
+    auto& pairs = R->world->mCollisionDetection.mOverlappingPairs.mConvexPairs;
+    const int n = (int)pairs.size();
+    if (n > cap) return -n;
+    for (int i = 0; i < n; i++) {
+        c1[i] = collider_index(R, pairs[i].collider1);
+        c2[i] = collider_index(R, pairs[i].collider2);
+    }
+    return n;
+}
+
+// Contacts of the last step: per manifold (island order) collider1/2, nb points, friction impulses
+// (friction1, friction2, twist); per point (4 slots) normal+depth, local points, normal impulse.
+int rp3dref_get_contacts(void* h, int32_t* cols, int32_t* npts, float* nd, float* l1, float* l2, float* imp,
+                         float* fric, int cap) {
+    RefWorld* R = (RefWorld*)h;
+    Array<ContactManifold>& M = *R->world->mCollisionDetection.mCurrentContactManifolds;
+    Array<ContactPoint>& P = *R->world->mCollisionDetection.mCurrentContactPoints;
+    const int n = (int)M.size();
+    if (n > cap) return -n;
+    for (int m = 0; m < n; m++) {
+        const ContactManifold& cm = M[m];
+        cols[2 * m] = collider_index(R, cm.colliderEntity1);
+        cols[2 * m + 1] = collider_index(R, cm.colliderEntity2);
+        npts[m] = cm.nbContactPoints;
+        fric[3 * m] = (float)cm.frictionImpulse1;
+        fric[3 * m + 1] = (float)cm.frictionImpulse2;
+        fric[3 * m + 2] = (float)cm.frictionTwistImpulse;
+        for (int k = 0; k < 4; k++) {
+            float* a = nd + 16 * m + 4 * k;
+            float* b = l1 + 12 * m + 3 * k;
+            float* c = l2 + 12 * m + 3 * k;
+            if (k < cm.nbContactPoints) {
+                const ContactPoint& p = P[cm.contactPointsIndex + k];
+                a[0] = (float)p.getNormal().x; a[1] = (float)p.getNormal().y; a[2] = (float)p.getNormal().z;
+                a[3] = (float)p.getPenetrationDepth();
+                b[0] = (float)p.getLocalPointOnShape1().x; b[1] = (float)p.getLocalPointOnShape1().y; b[2] = (float)p.getLocalPointOnShape1().z;
+                c[0] = (float)p.getLocalPointOnShape2().x; c[1] = (float)p.getLocalPointOnShape2().y; c[2] = (float)p.getLocalPointOnShape2().z;
+                imp[4 * m + k] = (float)p.getPenetrationImpulse();
+            } else {
+                a[0] = a[1] = a[2] = a[3] = 0.f;
+                b[0] = b[1] = b[2] = 0.f;
+                c[0] = c[1] = c[2] = 0.f;
+                imp[4 * m + k] = 0.f;
+            }
+        }
+    }
+    return n;
+}
+
+}  // extern "C"

@@ -1,0 +1,80 @@
+// Device batch of worlds (definition of the opaque rp3dg_batch handle).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <map>
+#include <string>
+#include <vector>
+
+#include "rp3d_internal.h"
+
+struct rp3dg_world;
+
+struct rp3dg_batch {
+    struct Pending { const char* name; hipEvent_t a, e; };
+    struct KernelStat { double total_ms = 0.0; int64_t launches = 0; };
+
+    std::vector<rp3dg_world*> worlds;
+    std::vector<int> body_base, collider_base;
+    std::vector<void*> allocs;
+    size_t bytes = 0;
+    hipStream_t stream = nullptr;
+    int nb = 0, nc = 0, nworlds = 0, world_bits = 0;
+    rp3dg::Settings settings{};
+    rp3dg::DevBodies B{};
+    rp3dg::DevColliders C{};
+    rp3dg::DevPairs Pa{}, Pb{};
+    rp3dg::DevSolver S{};
+    bool cur_is_a = true;
+    int pair_cap = 0;
+    int nb_pairs = 0;
+    int last_colors = 0;
+    int color_rounds = 0;
+    int64_t steps_done = 0;
+
+    // hull arena
+    rp3dg::HullHeader* d_hulls = nullptr;
+    float4 *d_hverts = nullptr, *d_hnormals = nullptr, *d_hbounds = nullptr;
+    int4* d_hedges = nullptr;
+    int *d_hfedge = nullptr, *d_hfverts = nullptr, *d_hfvidx = nullptr;
+    uint64_t* d_nocol = nullptr;
+    int nnocol = 0;
+
+    // broadphase scratch
+    uint64_t *bp_keys = nullptr, *bp_ktmp = nullptr, *bp_out = nullptr;
+    int *bp_vals = nullptr, *bp_vtmp = nullptr;
+    int2 *bp_child = nullptr, *bp_range = nullptr;
+    int *bp_parent = nullptr, *bp_flags = nullptr, *bp_root = nullptr, *bp_count = nullptr, *bp_offs = nullptr;
+    float4 *bp_nmin = nullptr, *bp_nmax = nullptr, *bp_partial = nullptr;
+    // narrowphase
+    int *np_lists = nullptr, *np_counts = nullptr;
+    // manifolds / colouring
+    int *m_flag = nullptr, *m_offs = nullptr, *m_list = nullptr;
+    int *adj_cnt = nullptr, *adj_offs = nullptr, *adj_cursor = nullptr, *adj = nullptr;
+    int2* mb = nullptr;
+    int* color = nullptr;
+    uint64_t *ckeys = nullptr, *cktmp = nullptr;
+    int *cvals = nullptr, *cvtmp = nullptr, *cstart = nullptr;
+    // scan / sort
+    int* scan_sums_buf = nullptr;
+    int scan_sums_cap = 0;
+    int *sort_hist = nullptr, *sort_offs = nullptr;
+    int sort_hist_cap = 0;
+
+    rp3dg::DevCounters* counters = nullptr;
+    rp3dg::DevCounters* hcounters = nullptr;
+
+    bool profiling = false;
+    std::vector<Pending> pending;
+    std::map<std::string, KernelStat> stats;
+
+    ~rp3dg_batch();
+    int upload(rp3dg_world* const* worlds, int n);
+    int step(float dt);
+    int scan(const int* in, int n, int* out, hipStream_t s);
+    int sort(uint64_t** keys, int** vals, uint64_t* ktmp, int* vtmp, int n, int bits, hipStream_t s);
+    void timer_begin(const char* name);
+    void timer_end();
+    void timer_flush();
+};

@@ -1,0 +1,781 @@
+// Device batch: uploads one or many host worlds into SoA arenas in HBM and
+// runs the step in the phase order of PhysicsWorld::update()
+// (src/engine/PhysicsWorld.cpp:239-307).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <string>
+#include <vector>
+
+#include "../../include/rp3d_mi355x.h"
+#include "batch.h"
+#include "kcommon.h"
+#include "narrowphase.h"
+#include "solver.h"
+#include "world.h"
+
+namespace rp3dg {
+
+// broadphase.hip
+__global__ void bp_refit(DevBodies B, DevColliders C, const float4* hull_bounds, int init);
+__global__ void bp_bounds(DevColliders C, float4* partial);
+__global__ void bp_morton(DevColliders C, const float4* partial, int nparts, uint64_t* keys, int* vals);
+__global__ void rs_hist(const uint64_t* keys, int n, int shift, int* hist, int nblocks);
+__global__ void rs_scatter(const uint64_t* kin, const int* vin, uint64_t* kout, int* vout, int n, int shift,
+                           const int* offs, int nblocks);
+__global__ void bp_karras(const uint64_t* k, int n, int2* child, int* parent, int2* range);
+__global__ void bp_fit(DevColliders C, const int* sorted, int n, const int2* child, const int* parent, float4* nmin,
+                       float4* nmax, int* flags);
+__global__ void bp_world_roots(const uint64_t* k, int n, const int2* range, int* root);
+__global__ void bp_traverse(DevColliders C, const int* sorted, int n, const int2* child, const float4* nmin,
+                            const float4* nmax, const int* root, const uint64_t* nc, int nnc, int* count,
+                            const int* offset, uint64_t* out_keys, int capacity, int pass, int* status);
+__global__ void bp_merge(DevColliders C, const uint64_t* keys, const int* nkeys_ptr, DevPairs O, int nold, DevPairs N);
+
+// ---------------------------------------------------------------------------------------------
+// exclusive scan (out has n+1 entries; out[n] = total)
+// ---------------------------------------------------------------------------------------------
+constexpr int kScanThreads = 256;
+constexpr int kScanItems = 8;
+constexpr int kScanTile = kScanThreads * kScanItems;
+
+__global__ __launch_bounds__(kScanThreads) void scan_tiles(const int* in, int n, int* out, int* sums) {
+    __shared__ int sm[17];
+    const int base = blockIdx.x * kScanTile + threadIdx.x * kScanItems;
+    int v[kScanItems];
+    int s = 0;
+    for (int k = 0; k < kScanItems; k++) {
+        v[k] = (base + k < n) ? in[base + k] : 0;
+        s += v[k];
+    }
+    int total;
+    int run = block_excl_scan(s, sm, &total);
+    for (int k = 0; k < kScanItems; k++) {
+        if (base + k < n) out[base + k] = run;
+        run += v[k];
+    }
+    if (threadIdx.x == 0) sums[blockIdx.x] = total;
+}
+
+__global__ __launch_bounds__(1024) void scan_sums(int* sums, int nb, int* out_total) {
+    __shared__ int sm[17];
+    int carry = 0;
+    for (int base = 0; base < nb; base += 1024) {
+        const int i = base + threadIdx.x;
+        const int v = i < nb ? sums[i] : 0;
+        int total;
+        const int e = block_excl_scan(v, sm, &total);
+        if (i < nb) sums[i] = e + carry;
+        carry += total;
+    }
+    if (threadIdx.x == 0) *out_total = carry;
+}
+
+__global__ __launch_bounds__(kScanThreads) void scan_add(int* out, int n, const int* sums) {
+    const int base = blockIdx.x * kScanTile + threadIdx.x * kScanItems;
+    const int add = sums[blockIdx.x];
+    for (int k = 0; k < kScanItems; k++)
+        if (base + k < n) out[base + k] += add;
+}
+
+__global__ void fill_int(int* p, int n, int v) { RP3DG_GRID_STRIDE(i, n) p[i] = v; }
+
+}  // namespace rp3dg
+
+using namespace rp3dg;
+
+#define HIPCHECK(x)                                                                              \
+    do {                                                                                         \
+        hipError_t e__ = (x);                                                                    \
+        if (e__ != hipSuccess) {                                                                 \
+            set_error(std::string("HIP error ") + hipGetErrorString(e__) + " at " #x);           \
+            return RP3DG_EHIP;                                                                   \
+        }                                                                                        \
+    } while (0)
+
+static int grid_for(int n, int threads = 256, int max_blocks = 4096) {
+    int g = (n + threads - 1) / threads;
+    if (g < 1) g = 1;
+    if (g > max_blocks) g = max_blocks;
+    return g;
+}
+
+template <typename T>
+static T* dalloc(rp3dg_batch* b, size_t n) {
+    void* p = nullptr;
+    if (n == 0) n = 1;
+    if (hipMalloc(&p, n * sizeof(T)) != hipSuccess) return nullptr;
+    b->allocs.push_back(p);
+    b->bytes += n * sizeof(T);
+    return (T*)p;
+}
+
+void rp3dg_batch::timer_begin(const char* name) {
+    if (!profiling) return;
+    hipEvent_t a, e;
+    hipEventCreate(&a);
+    hipEventCreate(&e);
+    hipEventRecord(a, stream);
+    pending.push_back({name, a, e});
+}
+void rp3dg_batch::timer_end() {
+    if (!profiling || pending.empty()) return;
+    hipEventRecord(pending.back().e, stream);
+}
+void rp3dg_batch::timer_flush() {
+    if (pending.empty()) return;
+    hipStreamSynchronize(stream);
+    for (auto& p : pending) {
+        float ms = 0.f;
+        hipEventElapsedTime(&ms, p.a, p.e);
+        KernelStat& s = stats[p.name];
+        s.total_ms += ms;
+        s.launches += 1;
+        hipEventDestroy(p.a);
+        hipEventDestroy(p.e);
+    }
+    pending.clear();
+}
+
+int rp3dg_batch::scan(const int* in, int n, int* out, hipStream_t s) {
+    const int tiles = std::max(1, (n + kScanTile - 1) / kScanTile);
+    if (tiles > scan_sums_cap) return RP3DG_EOVERFLOW;
+    scan_tiles<<<tiles, kScanThreads, 0, s>>>(in, n, out, scan_sums_buf);
+    scan_sums<<<1, 1024, 0, s>>>(scan_sums_buf, tiles, out + n);
+    scan_add<<<tiles, kScanThreads, 0, s>>>(out, n, scan_sums_buf);
+    return RP3DG_OK;
+}
+
+// Stable LSD radix sort of (keys, vals) over the low `bits` bits.  Result in (*keys, *vals).
+int rp3dg_batch::sort(uint64_t** keys, int** vals, uint64_t* ktmp, int* vtmp, int n, int bits, hipStream_t s) {
+    if (n <= 1) return RP3DG_OK;
+    const int nblocks = (n + 4095) / 4096;
+    if (nblocks * 256 > sort_hist_cap) return RP3DG_EOVERFLOW;
+    uint64_t* ki = *keys;
+    int* vi = *vals;
+    uint64_t* ko = ktmp;
+    int* vo = vtmp;
+    for (int shift = 0; shift < bits; shift += 8) {
+        rs_hist<<<nblocks, 256, 0, s>>>(ki, n, shift, sort_hist, nblocks);
+        int r = scan(sort_hist, nblocks * 256, sort_offs, s);
+        if (r) return r;
+        rs_scatter<<<nblocks, 256, 0, s>>>(ki, vi, ko, vo, n, shift, sort_offs, nblocks);
+        std::swap(ki, ko);
+        std::swap(vi, vo);
+    }
+    if (ki != *keys) {
+        hipMemcpyAsync(*keys, ki, sizeof(uint64_t) * n, hipMemcpyDeviceToDevice, s);
+        hipMemcpyAsync(*vals, vi, sizeof(int) * n, hipMemcpyDeviceToDevice, s);
+    }
+    return RP3DG_OK;
+}
+
+static void alloc_pairs(rp3dg_batch* b, DevPairs& P, int cap) {
+    P.key = dalloc<uint64_t>(b, cap);
+    P.col = dalloc<int2>(b, cap);
+    P.algo = dalloc<int>(b, cap);
+    P.lf = dalloc<uint32_t>(b, cap);
+    P.lfaxis = dalloc<float4>(b, cap);
+    P.ncp = dalloc<int>(b, cap);
+    P.fv1 = dalloc<float4>(b, cap);
+    P.fv2 = dalloc<float4>(b, cap);
+    P.fimp = dalloc<float4>(b, cap);
+    P.pn = dalloc<float4>(b, 4 * (size_t)cap);
+    P.pl1 = dalloc<float4>(b, 4 * (size_t)cap);
+    P.pl2 = dalloc<float4>(b, 4 * (size_t)cap);
+}
+
+// Box half-edge structure (PhysicsCommon.cpp:73-109 face lists), numbered by the reference rule.
+static bool box_topology(HalfEdgeMesh& m) {
+    m.faces = {{0, 1, 2, 3}, {1, 5, 6, 2}, {4, 7, 6, 5}, {4, 0, 3, 7}, {4, 5, 1, 0}, {2, 6, 7, 3}};
+    std::string err;
+    return m.build(8, &err);
+}
+
+// Position of every collider of a world in the iteration order of BroadPhaseSystem::mMovedShapes
+// (Set<int> with identity hash, Set.h:356-400): ascending bucket (broadphase id & (H-1)), and in a
+// bucket the most recently inserted first; colliders enter in creation order.  Broadphase ids of
+// sequentially created colliders are 0, 1, 3, 5, ... (DynamicAABBTree.cpp:93-131: leaf and
+// internal nodes share the node pool).  H is the hash size reached when all colliders were added.
+static void moved_ranks(int n, std::vector<int>& rank) {
+    uint64_t H = 16;
+    auto alloc_entries = [](uint64_t h) -> uint64_t {
+        uint64_t e = (uint64_t)(h * 0.75);
+        return (uint64_t)(std::ceil(e / 16.0f) * 16);
+    };
+    while (alloc_entries(H) < (uint64_t)n) H *= 2;
+    std::vector<std::pair<uint64_t, int>> keyed(n);
+    for (int k = 0; k < n; k++) {
+        const uint64_t bp = k == 0 ? 0 : (uint64_t)(2 * k - 1);
+        keyed[k] = {((bp & (H - 1)) << 32) | (uint64_t)(0x7fffffff - k), k};
+    }
+    std::sort(keyed.begin(), keyed.end());
+    rank.assign(n, 0);
+    for (int r = 0; r < n; r++) rank[keyed[r].second] = r;
+}
+
+int rp3dg_batch::upload(rp3dg_world* const* ws, int nw) {
+    worlds.assign(ws, ws + nw);
+    // host SoA
+    std::vector<float4> x, q, p, v, w, force, torque, ii, com, llock, alock;
+    std::vector<int> flags, bworld;
+    std::vector<int> cbody, ctype, cworld, crank;
+    std::vector<float4> shape, lpos, lq;
+    std::vector<float2> mat;
+    std::vector<uint32_t> filter;
+    std::vector<HullHeader> hh;
+    std::vector<float4> hverts, hnormals, hbounds;
+    std::vector<int4> hedges;
+    std::vector<int> hfedge, hfverts, hfvidx;
+    std::vector<uint64_t> nocol;
+    body_base.clear();
+    collider_base.clear();
+    for (int wi = 0; wi < nw; wi++) {
+        const HostWorld& W = *ws[wi];
+        if (W.cfg.is_sleeping_enabled) {
+            set_error("sleeping is not supported on the device path: set is_sleeping_enabled = 0");
+            return RP3DG_EUNSUPPORTED;
+        }
+        const int b0 = (int)x.size();
+        const int c0 = (int)cbody.size();
+        const int h0 = (int)hh.size();
+        body_base.push_back(b0);
+        collider_base.push_back(c0);
+        for (const HostBody& b : W.bodies) {
+            x.push_back(make_float4(b.com_world.x, b.com_world.y, b.com_world.z, 0.f));
+            q.push_back(make_float4(b.transform.q.x, b.transform.q.y, b.transform.q.z, b.transform.q.w));
+            p.push_back(make_float4(b.transform.p.x, b.transform.p.y, b.transform.p.z, 0.f));
+            v.push_back(make_float4(b.v.x, b.v.y, b.v.z, 0.f));
+            w.push_back(make_float4(b.w.x, b.w.y, b.w.z, 0.f));
+            force.push_back(make_float4(b.force.x, b.force.y, b.force.z, 0.f));
+            torque.push_back(make_float4(b.torque.x, b.torque.y, b.torque.z, 0.f));
+            ii.push_back(make_float4(b.inv_inertia.x, b.inv_inertia.y, b.inv_inertia.z, b.inv_mass));
+            com.push_back(make_float4(b.com_local.x, b.com_local.y, b.com_local.z, b.mass));
+            llock.push_back(make_float4(b.llock.x, b.llock.y, b.llock.z, b.ldamp));
+            alock.push_back(make_float4(b.alock.x, b.alock.y, b.alock.z, b.adamp));
+            int f = 0;
+            if (b.type == RP3DG_DYNAMIC) f |= BF_DYNAMIC;
+            if (b.type == RP3DG_KINEMATIC) f |= BF_KINEMATIC;
+            if (b.type != RP3DG_STATIC) f |= BF_ENABLED;
+            if (b.gravity) f |= BF_GRAVITY;
+            flags.push_back(f);
+            bworld.push_back(wi);
+        }
+        std::vector<int> rank;
+        moved_ranks((int)W.colliders.size(), rank);
+        for (size_t ci = 0; ci < W.colliders.size(); ci++) {
+            const HostCollider& c = W.colliders[ci];
+            cbody.push_back(b0 + c.body);
+            ctype.push_back(c.type);
+            cworld.push_back(wi);
+            crank.push_back(rank[ci]);
+            float4 s = make_float4(c.params[0], c.params[1], c.params[2], c.params[3]);
+            if (c.type == SHAPE_CAPSULE) s = make_float4(c.params[0], c.params[1] * 0.5f, 0.f, 0.f);
+            if (c.type == SHAPE_HULL) s = make_float4((float)(h0 + (int)c.params[0]), 0.f, 0.f, 0.f);
+            shape.push_back(s);
+            lpos.push_back(make_float4(c.local.p.x, c.local.p.y, c.local.p.z, 0.f));
+            lq.push_back(make_float4(c.local.q.x, c.local.q.y, c.local.q.z, c.local.q.w));
+            mat.push_back(make_float2(c.friction_sqrt, c.bounciness));
+            filter.push_back((uint32_t)c.category | ((uint32_t)c.mask << 16));
+        }
+        for (const HostHull& H : W.hulls) {
+            HullHeader h;
+            h.vbase = (int)hverts.size();
+            h.nv = (int)H.vertices.size();
+            h.fbase = (int)hnormals.size();
+            h.nf = (int)H.face_normals.size();
+            h.ebase = (int)hedges.size();
+            h.ne = (int)H.mesh.edges.size();
+            h.fvbase = (int)hfvidx.size();
+            h.nfv = 0;
+            h.cx = H.centroid.x; h.cy = H.centroid.y; h.cz = H.centroid.z; h.pad = 0.f;
+            for (const V3& vv : H.vertices) hverts.push_back(make_float4(vv.x, vv.y, vv.z, 0.f));
+            for (size_t f = 0; f < H.face_normals.size(); f++) {
+                const V3& nn = H.face_normals[f];
+                hnormals.push_back(make_float4(nn.x, nn.y, nn.z, 0.f));
+                hfedge.push_back(H.mesh.face_edge[f]);
+                hfverts.push_back((int)hfvidx.size());
+                hfverts.push_back((int)H.mesh.faces[f].size());
+                for (int vi : H.mesh.faces[f]) hfvidx.push_back(vi);
+                h.nfv += (int)H.mesh.faces[f].size();
+            }
+            for (const HalfEdgeMesh::Edge& e : H.mesh.edges) hedges.push_back(make_int4(e.vertex, e.twin, e.face, e.next));
+            hbounds.push_back(make_float4(H.local_min.x, H.local_min.y, H.local_min.z, 0.f));
+            hbounds.push_back(make_float4(H.local_max.x, H.local_max.y, H.local_max.z, 0.f));
+            hh.push_back(h);
+        }
+        for (const HostJoint& j : W.joints) {
+            if (!j.collide_connected) {
+                const int a = b0 + j.body1, c = b0 + j.body2;
+                nocol.push_back(((uint64_t)(uint32_t)std::min(a, c) << 32) | (uint32_t)std::max(a, c));
+            }
+        }
+        const rp3dg_world_settings& cfg = W.cfg;
+        if (wi == 0) {
+            settings.gravity = v3(cfg.gravity[0], cfg.gravity[1], cfg.gravity[2]);
+            settings.persistent_threshold = cfg.persistent_contact_distance_threshold;
+            settings.restitution_threshold = cfg.restitution_velocity_threshold;
+            settings.cos_similar = cfg.cos_angle_similar_contact_manifold;
+            settings.vel_iters = (int)cfg.default_velocity_solver_iterations;
+            settings.pos_iters = (int)cfg.default_position_solver_iterations;
+            settings.gravity_enabled = cfg.is_gravity_enabled;
+        } else if (memcmp(&cfg, &ws[0]->cfg, sizeof(cfg)) != 0) {
+            set_error("all worlds of a batch must share the same WorldSettings");
+            return RP3DG_EINVAL;
+        }
+    }
+    std::sort(nocol.begin(), nocol.end());
+    nocol.erase(std::unique(nocol.begin(), nocol.end()), nocol.end());
+    nb = (int)x.size();
+    nc = (int)cbody.size();
+    nworlds = nw;
+    world_bits = 0;
+    while ((1 << world_bits) < nw) world_bits++;
+
+    HIPCHECK(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+    B.n = nb;
+    B.x = dalloc<float4>(this, nb); B.q = dalloc<float4>(this, nb); B.p = dalloc<float4>(this, nb);
+    B.v = dalloc<float4>(this, nb); B.w = dalloc<float4>(this, nb);
+    B.cv = dalloc<float4>(this, nb); B.cw = dalloc<float4>(this, nb);
+    B.sv = dalloc<float4>(this, nb); B.sw = dalloc<float4>(this, nb);
+    B.cx = dalloc<float4>(this, nb); B.cq = dalloc<float4>(this, nb);
+    B.force = dalloc<float4>(this, nb); B.torque = dalloc<float4>(this, nb);
+    B.ii = dalloc<float4>(this, nb);
+    B.iw0 = dalloc<float4>(this, nb); B.iw1 = dalloc<float4>(this, nb); B.iw2 = dalloc<float4>(this, nb);
+    B.com = dalloc<float4>(this, nb);
+    B.llock = dalloc<float4>(this, nb); B.alock = dalloc<float4>(this, nb);
+    B.flags = dalloc<int>(this, nb); B.world = dalloc<int>(this, nb);
+    C.n = nc;
+    C.body = dalloc<int>(this, nc); C.type = dalloc<int>(this, nc);
+    C.shape = dalloc<float4>(this, nc);
+    C.lpos = dalloc<float4>(this, nc); C.lq = dalloc<float4>(this, nc);
+    C.wpos = dalloc<float4>(this, nc); C.wq = dalloc<float4>(this, nc);
+    C.amin = dalloc<float4>(this, nc); C.amax = dalloc<float4>(this, nc);
+    C.mat = dalloc<float2>(this, nc);
+    C.filter = dalloc<uint32_t>(this, nc);
+    C.moved = dalloc<int>(this, nc); C.rank = dalloc<int>(this, nc); C.world = dalloc<int>(this, nc);
+    if (!B.world || !C.world) { set_error("device allocation failed"); return RP3DG_EHIP; }
+    auto up = [&](void* d, const void* h, size_t bytes) { return hipMemcpyAsync(d, h, bytes, hipMemcpyHostToDevice, stream); };
+    HIPCHECK(up(B.x, x.data(), nb * 16)); HIPCHECK(up(B.q, q.data(), nb * 16)); HIPCHECK(up(B.p, p.data(), nb * 16));
+    HIPCHECK(up(B.v, v.data(), nb * 16)); HIPCHECK(up(B.w, w.data(), nb * 16));
+    HIPCHECK(up(B.force, force.data(), nb * 16)); HIPCHECK(up(B.torque, torque.data(), nb * 16));
+    HIPCHECK(up(B.ii, ii.data(), nb * 16)); HIPCHECK(up(B.com, com.data(), nb * 16));
+    HIPCHECK(up(B.llock, llock.data(), nb * 16)); HIPCHECK(up(B.alock, alock.data(), nb * 16));
+    HIPCHECK(up(B.flags, flags.data(), nb * 4)); HIPCHECK(up(B.world, bworld.data(), nb * 4));
+    for (float4* a : {B.cv, B.cw, B.sv, B.sw, B.iw0, B.iw1, B.iw2}) HIPCHECK(hipMemsetAsync(a, 0, nb * 16, stream));
+    HIPCHECK(up(B.cx, x.data(), nb * 16)); HIPCHECK(up(B.cq, q.data(), nb * 16));
+    HIPCHECK(up(C.body, cbody.data(), nc * 4)); HIPCHECK(up(C.type, ctype.data(), nc * 4));
+    HIPCHECK(up(C.shape, shape.data(), nc * 16)); HIPCHECK(up(C.lpos, lpos.data(), nc * 16));
+    HIPCHECK(up(C.lq, lq.data(), nc * 16)); HIPCHECK(up(C.mat, mat.data(), nc * 8));
+    HIPCHECK(up(C.filter, filter.data(), nc * 4)); HIPCHECK(up(C.rank, crank.data(), nc * 4));
+    HIPCHECK(up(C.world, cworld.data(), nc * 4));
+    // hulls (at least one dummy record so pointers are valid)
+    if (hh.empty()) { hh.push_back(HullHeader{}); hbounds.resize(2); }
+    d_hulls = dalloc<HullHeader>(this, hh.size());
+    d_hverts = dalloc<float4>(this, hverts.size());
+    d_hnormals = dalloc<float4>(this, hnormals.size());
+    d_hedges = dalloc<int4>(this, hedges.size());
+    d_hfedge = dalloc<int>(this, hfedge.size());
+    d_hfverts = dalloc<int>(this, hfverts.size());
+    d_hfvidx = dalloc<int>(this, hfvidx.size());
+    d_hbounds = dalloc<float4>(this, hbounds.size());
+    HIPCHECK(up(d_hulls, hh.data(), hh.size() * sizeof(HullHeader)));
+    if (!hverts.empty()) HIPCHECK(up(d_hverts, hverts.data(), hverts.size() * 16));
+    if (!hnormals.empty()) HIPCHECK(up(d_hnormals, hnormals.data(), hnormals.size() * 16));
+    if (!hedges.empty()) HIPCHECK(up(d_hedges, hedges.data(), hedges.size() * 16));
+    if (!hfedge.empty()) HIPCHECK(up(d_hfedge, hfedge.data(), hfedge.size() * 4));
+    if (!hfverts.empty()) HIPCHECK(up(d_hfverts, hfverts.data(), hfverts.size() * 4));
+    if (!hfvidx.empty()) HIPCHECK(up(d_hfvidx, hfvidx.data(), hfvidx.size() * 4));
+    HIPCHECK(up(d_hbounds, hbounds.data(), hbounds.size() * 16));
+    nnocol = (int)nocol.size();
+    d_nocol = dalloc<uint64_t>(this, std::max<size_t>(1, nocol.size()));
+    if (nnocol) HIPCHECK(up(d_nocol, nocol.data(), nocol.size() * 8));
+
+    // box topology
+    HalfEdgeMesh box;
+    if (!box_topology(box)) { set_error("box topology"); return RP3DG_EINVAL; }
+    std::vector<int> e4, fe(6), fv;
+    for (auto& e : box.edges) { e4.push_back(e.vertex); e4.push_back(e.twin); e4.push_back(e.face); e4.push_back(e.next); }
+    for (int f = 0; f < 6; f++) fe[f] = box.face_edge[f];
+    for (auto& f : box.faces) for (int vv : f) fv.push_back(vv);
+    upload_box_topology(e4.data(), fe.data(), fv.data());
+
+    // broadphase / pair / solver arenas
+    pair_cap = std::max(4096, 16 * nc);
+    alloc_pairs(this, Pa, pair_cap);
+    alloc_pairs(this, Pb, pair_cap);
+    const int nn = std::max(1, 2 * nc);
+    bp_keys = dalloc<uint64_t>(this, nc); bp_vals = dalloc<int>(this, nc);
+    bp_ktmp = dalloc<uint64_t>(this, nc); bp_vtmp = dalloc<int>(this, nc);
+    bp_child = dalloc<int2>(this, nn); bp_parent = dalloc<int>(this, nn); bp_range = dalloc<int2>(this, nn);
+    bp_nmin = dalloc<float4>(this, nn); bp_nmax = dalloc<float4>(this, nn); bp_flags = dalloc<int>(this, nn);
+    bp_root = dalloc<int>(this, (size_t)1 << world_bits);
+    bp_partial = dalloc<float4>(this, 2 * 256);
+    bp_count = dalloc<int>(this, nc + 1); bp_offs = dalloc<int>(this, nc + 1);
+    bp_out = dalloc<uint64_t>(this, pair_cap);
+    np_lists = dalloc<int>(this, (size_t)ALGO_COUNT * pair_cap);
+    np_counts = dalloc<int>(this, ALGO_COUNT);
+    m_flag = dalloc<int>(this, pair_cap); m_offs = dalloc<int>(this, pair_cap + 1);
+    m_list = dalloc<int>(this, pair_cap);
+    adj_cnt = dalloc<int>(this, nb + 1); adj_offs = dalloc<int>(this, nb + 1); adj_cursor = dalloc<int>(this, nb);
+    adj = dalloc<int>(this, 2 * (size_t)pair_cap); mb = dalloc<int2>(this, pair_cap);
+    color = dalloc<int>(this, pair_cap);
+    ckeys = dalloc<uint64_t>(this, pair_cap); cvals = dalloc<int>(this, pair_cap);
+    cktmp = dalloc<uint64_t>(this, pair_cap); cvtmp = dalloc<int>(this, pair_cap);
+    cstart = dalloc<int>(this, kMaxColors + 1);
+    S.pair = dalloc<int>(this, pair_cap); S.bodies = dalloc<int2>(this, pair_cap); S.npts = dalloc<int>(this, pair_cap);
+    for (float4** a : {&S.n, &S.r1f, &S.r2f, &S.t1, &S.t2, &S.r1ct1, &S.r1ct2, &S.r2ct1, &S.r2ct2, &S.imp, &S.oldt1, &S.oldt2})
+        *a = dalloc<float4>(this, pair_cap);
+    for (float4** a : {&S.pn, &S.r1, &S.r2, &S.i1n, &S.i2n, &S.lam}) *a = dalloc<float4>(this, 4 * (size_t)pair_cap);
+    scan_sums_cap = 1 << 16;
+    scan_sums_buf = dalloc<int>(this, scan_sums_cap);
+    sort_hist_cap = 256 * (std::max(pair_cap, nc) / 4096 + 2);
+    sort_hist = dalloc<int>(this, sort_hist_cap);
+    sort_offs = dalloc<int>(this, sort_hist_cap + 1);
+    HIPCHECK(hipMalloc(&counters, sizeof(DevCounters)));
+    allocs.push_back(counters);
+    HIPCHECK(hipHostMalloc(&hcounters, sizeof(DevCounters), hipHostMallocDefault));
+    HIPCHECK(hipMemsetAsync(counters, 0, sizeof(DevCounters), stream));
+    if (!sort_offs || !S.lam) { set_error("device allocation failed (pair arenas)"); return RP3DG_EHIP; }
+    nb_pairs = 0;
+    // collider creation: world transforms, initial fat AABBs, all colliders in the moved set
+    bp_refit<<<grid_for(nc), 256, 0, stream>>>(B, C, d_hbounds, 1);
+    HIPCHECK(hipGetLastError());
+    HIPCHECK(hipStreamSynchronize(stream));
+    return RP3DG_OK;
+}
+
+#define LAUNCH_T(name, grid, block, kern, ...)               \
+    do {                                                     \
+        timer_begin(name);                                   \
+        kern<<<(grid), (block), 0, stream>>>(__VA_ARGS__);   \
+        timer_end();                                         \
+    } while (0)
+
+int rp3dg_batch::step(float dt) {
+    if (nb == 0) return RP3DG_OK;
+    DevPairs& O = cur_is_a ? Pa : Pb;
+    DevPairs& N = cur_is_a ? Pb : Pa;
+    HIPCHECK(hipMemsetAsync(counters, 0, sizeof(DevCounters), stream));
+    int* dstatus = &counters->status;
+
+    // ---------------- broadphase (CollisionDetectionSystem::computeBroadPhase, CD.cpp:103-121)
+    const int nparts = std::min(256, grid_for(nc));
+    LAUNCH_T("bp_bounds", nparts, 256, bp_bounds, C, bp_partial);
+    LAUNCH_T("bp_morton", grid_for(nc), 256, bp_morton, C, bp_partial, nparts, bp_keys, bp_vals);
+    uint64_t* k = bp_keys;
+    int* vls = bp_vals;
+    timer_begin("bp_sort");
+    int r = sort(&k, &vls, bp_ktmp, bp_vtmp, nc, 30 + world_bits, stream);
+    timer_end();
+    if (r) { set_error("radix sort overflow"); return r; }
+    if (nc > 1) {
+        LAUNCH_T("bp_karras", grid_for(nc - 1), 256, bp_karras, bp_keys, nc, bp_child, bp_parent, bp_range);
+        HIPCHECK(hipMemsetAsync(bp_flags, 0, sizeof(int) * nc, stream));
+    }
+    LAUNCH_T("bp_fit", grid_for(nc), 256, bp_fit, C, bp_vals, nc, bp_child, bp_parent, bp_nmin, bp_nmax, bp_flags);
+    LAUNCH_T("bp_world_roots", grid_for(2 * nc - 1), 256, bp_world_roots, bp_keys, nc, bp_range, bp_root);
+    LAUNCH_T("bp_traverse", grid_for(nc), 256, bp_traverse, C, bp_vals, nc, bp_child, bp_nmin, bp_nmax, bp_root,
+             d_nocol, nnocol, bp_count, bp_offs, bp_out, pair_cap, 0, dstatus);
+    r = scan(bp_count, nc, bp_offs, stream);
+    if (r) { set_error("scan overflow"); return r; }
+    int total = 0;
+    HIPCHECK(hipMemcpyAsync(&hcounters->nb_overlaps, bp_offs + nc, sizeof(int), hipMemcpyDeviceToHost, stream));
+    HIPCHECK(hipStreamSynchronize(stream));
+    total = hcounters->nb_overlaps;
+    if (total > pair_cap) {
+        set_error("overlapping pair arena overflow (" + std::to_string(total) + " > " + std::to_string(pair_cap) + ")");
+        return RP3DG_EOVERFLOW;
+    }
+    LAUNCH_T("bp_traverse", grid_for(nc), 256, bp_traverse, C, bp_vals, nc, bp_child, bp_nmin, bp_nmax, bp_root,
+             d_nocol, nnocol, bp_count, bp_offs, bp_out, pair_cap, 1, dstatus);
+    LAUNCH_T("bp_merge", grid_for(std::max(total, 1)), 256, bp_merge, C, bp_out, bp_offs + nc, O, nb_pairs, N);
+    nb_pairs = total;
+    cur_is_a = !cur_is_a;
+    HIPCHECK(hipMemcpyAsync(&counters->nb_pairs, bp_offs + nc, sizeof(int), hipMemcpyDeviceToDevice, stream));
+
+    // ---------------- narrowphase (CD.cpp:618-718)
+    NpArgs A;
+    A.B = B; A.C = C; A.P = N;
+    A.nb_pairs = &counters->nb_pairs;
+    A.hulls = d_hulls; A.hull_verts = d_hverts; A.hull_normals = d_hnormals; A.hull_edges = d_hedges;
+    A.hull_face_edge = d_hfedge; A.hull_face_verts = d_hfverts; A.hull_face_vidx = d_hfvidx;
+    A.persistent_threshold = settings.persistent_threshold;
+    A.cos_similar = settings.cos_similar;
+    const int pg = grid_for(std::max(total, 1));
+    LAUNCH_T("np_classify", pg, 256, np_classify, A, np_lists, pair_cap, counters->nb_tests);
+    int* L = np_lists;
+    int* T = counters->nb_tests;
+    LAUNCH_T("np_sphere_sphere", pg, 256, np_sphere_sphere, A, L + ALGO_SPHERE_SPHERE * pair_cap, T + ALGO_SPHERE_SPHERE);
+    LAUNCH_T("np_sphere_capsule", pg, 256, np_sphere_capsule, A, L + ALGO_SPHERE_CAPSULE * pair_cap, T + ALGO_SPHERE_CAPSULE);
+    LAUNCH_T("np_capsule_capsule", pg, 256, np_capsule_capsule, A, L + ALGO_CAPSULE_CAPSULE * pair_cap, T + ALGO_CAPSULE_CAPSULE);
+    LAUNCH_T("np_sphere_poly", pg, 256, np_sphere_poly, A, L + ALGO_SPHERE_POLY * pair_cap, T + ALGO_SPHERE_POLY);
+    LAUNCH_T("np_capsule_poly", pg, 256, np_capsule_poly, A, L + ALGO_CAPSULE_POLY * pair_cap, T + ALGO_CAPSULE_POLY);
+    LAUNCH_T("np_poly_poly", pg, 128, np_poly_poly, A, L + ALGO_POLY_POLY * pair_cap, T + ALGO_POLY_POLY);
+
+    // ---------------- manifold list + colouring (replaces createIslands, PhysicsWorld.cpp:695-866)
+    LAUNCH_T("sv_flag", pg, 256, sv_flag_manifolds, N.ncp, &counters->nb_pairs, m_flag);
+    r = scan(m_flag, std::max(total, 1), m_offs, stream);
+    if (r) { set_error("scan overflow"); return r; }
+    LAUNCH_T("sv_compact", pg, 256, sv_compact_manifolds, m_flag, m_offs, &counters->nb_pairs, N.ncp, m_list,
+             &counters->nb_manifolds, &counters->nb_points);
+    HIPCHECK(hipMemsetAsync(adj_cnt, 0, sizeof(int) * (nb + 1), stream));
+    HIPCHECK(hipMemsetAsync(adj_cursor, 0, sizeof(int) * nb, stream));
+    HIPCHECK(hipMemcpyAsync(&hcounters->nb_manifolds, &counters->nb_manifolds, sizeof(int), hipMemcpyDeviceToHost, stream));
+    HIPCHECK(hipStreamSynchronize(stream));
+    const int nm = hcounters->nb_manifolds;
+    const int mg = grid_for(std::max(nm, 1));
+    int ncol = 0;
+    if (nm > 0) {
+        LAUNCH_T("sv_adj_count", mg, 256, sv_adj_count, B, C, N, m_list, &counters->nb_manifolds, adj_cnt);
+        r = scan(adj_cnt, nb, adj_offs, stream);
+        if (r) { set_error("scan overflow"); return r; }
+        LAUNCH_T("sv_adj_fill", mg, 256, sv_adj_fill, B, C, N, m_list, &counters->nb_manifolds, adj_offs, adj_cursor, adj, mb);
+        HIPCHECK(hipMemsetAsync(color, 0xff, sizeof(int) * nm, stream));
+        int rounds = 0;
+        while (true) {
+            for (int k2 = 0; k2 < 4; k2++) {
+                HIPCHECK(hipMemsetAsync(&counters->nb_uncolored, 0, sizeof(int), stream));
+                LAUNCH_T("sv_color_round", mg, 256, sv_color_round, mb, &counters->nb_manifolds, adj_offs, adj, color,
+                         &counters->nb_uncolored, dstatus);
+                rounds++;
+            }
+            HIPCHECK(hipMemcpyAsync(&hcounters->nb_uncolored, &counters->nb_uncolored, sizeof(int), hipMemcpyDeviceToHost, stream));
+            HIPCHECK(hipStreamSynchronize(stream));
+            if (hcounters->nb_uncolored == 0) break;
+            if (rounds > 4096) { set_error("colouring did not converge"); return RP3DG_EOVERFLOW; }
+        }
+        color_rounds = rounds;
+        LAUNCH_T("sv_color_keys", mg, 256, sv_color_keys, color, &counters->nb_manifolds, ckeys, cvals);
+        uint64_t* ck = ckeys;
+        int* cv = cvals;
+        r = sort(&ck, &cv, cktmp, cvtmp, nm, 8, stream);
+        if (r) { set_error("sort overflow"); return r; }
+        LAUNCH_T("sv_color_ranges", mg, 256, sv_color_ranges, ckeys, &counters->nb_manifolds, cstart, &counters->nb_colors);
+        HIPCHECK(hipMemcpyAsync(&hcounters->nb_colors, &counters->nb_colors, sizeof(int), hipMemcpyDeviceToHost, stream));
+        HIPCHECK(hipStreamSynchronize(stream));
+        ncol = hcounters->nb_colors;
+    }
+    last_colors = ncol;
+
+    // ---------------- inverse world inertia + velocity integration (PhysicsWorld.cpp:266-272)
+    const int bg = grid_for(nb);
+    LAUNCH_T("dyn_integrate_velocities", bg, 256, dyn_integrate_velocities, B, settings, dt);
+
+    // ---------------- contact solver (PhysicsWorld.cpp:321-345)
+    SvArgs SA;
+    SA.B = B; SA.C = C; SA.P = N; SA.S = S;
+    SA.nm = &counters->nb_manifolds;
+    SA.mlist = m_list;
+    SA.order = cvals;
+    SA.cstart = cstart;
+    SA.ncolors = &counters->nb_colors;
+    SA.restitution_threshold = settings.restitution_threshold;
+    if (nm > 0) {
+        LAUNCH_T("sv_prepare", mg, 256, sv_prepare, SA);
+        for (int c = 0; c < ncol; c++) LAUNCH_T("sv_warmstart", mg, 256, sv_warmstart, SA, c);
+        const float beta_dt = 0.2f / dt;
+        for (int it = 0; it < settings.vel_iters; it++)
+            for (int c = 0; c < ncol; c++) LAUNCH_T("sv_solve_color", mg, 256, sv_solve_color, SA, c, beta_dt);
+        LAUNCH_T("sv_store", mg, 256, sv_store, SA);
+    }
+
+    // ---------------- position integration, state update, broadphase refit (PhysicsWorld.cpp:275-288)
+    LAUNCH_T("dyn_integrate_positions", bg, 256, dyn_integrate_positions, B, dt);
+    LAUNCH_T("dyn_update_state", bg, 256, dyn_update_state, B);
+    LAUNCH_T("bp_refit", grid_for(nc), 256, bp_refit, B, C, d_hbounds, 0);
+    HIPCHECK(hipGetLastError());
+    HIPCHECK(hipMemcpyAsync(hcounters, counters, sizeof(DevCounters), hipMemcpyDeviceToHost, stream));
+    HIPCHECK(hipStreamSynchronize(stream));
+    timer_flush();
+    if (hcounters->status) {
+        set_error("device status " + std::to_string(hcounters->status));
+        return RP3DG_EOVERFLOW;
+    }
+    steps_done++;
+    return RP3DG_OK;
+}
+
+rp3dg_batch::~rp3dg_batch() {
+    for (void* p : allocs) hipFree(p);
+    if (hcounters) hipHostFree(hcounters);
+    if (stream) hipStreamDestroy(stream);
+}
+
+extern "C" {
+
+rp3dg_batch* rp3dg_batch_create(rp3dg_world* const* worlds, int32_t n) {
+    if (!worlds || n <= 0) { set_error("no worlds"); return nullptr; }
+    rp3dg_batch* b = new rp3dg_batch();
+    const int r = b->upload(worlds, n);
+    if (r) { delete b; return nullptr; }
+    for (int i = 0; i < n; i++) worlds[i]->uploaded = true;
+    return b;
+}
+
+void rp3dg_batch_destroy(rp3dg_batch* b) { delete b; }
+
+int rp3dg_batch_step(rp3dg_batch* b, float dt, int32_t nsteps) {
+    if (!b) return RP3DG_EINVAL;
+    if (!(dt > 0.f)) { set_error("time step must be > 0"); return RP3DG_EINVAL; }
+    for (int i = 0; i < nsteps; i++) {
+        const int r = b->step(dt);
+        if (r) return r;
+    }
+    return RP3DG_OK;
+}
+
+int rp3dg_batch_synchronize(rp3dg_batch* b) {
+    if (!b) return RP3DG_EINVAL;
+    HIPCHECK(hipStreamSynchronize(b->stream));
+    return RP3DG_OK;
+}
+
+int rp3dg_world_update(rp3dg_world* w, float dt) {
+    if (!w) return RP3DG_EINVAL;
+    if (!w->own) {
+        if (w->uploaded) { set_error("world belongs to a batch; step the batch"); return RP3DG_ESTATE; }
+        rp3dg_world* ws[1] = {w};
+        w->own = rp3dg_batch_create(ws, 1);
+        if (!w->own) return RP3DG_EHIP;
+    }
+    return rp3dg_batch_step(w->own, dt, 1);
+}
+
+void rp3dg_world_destroy(rp3dg_world* w) {
+    if (!w) return;
+    if (w->own) rp3dg_batch_destroy(w->own);
+    delete w;
+}
+
+int rp3dg_batch_get_counts(rp3dg_batch* b, rp3dg_counts* o) {
+    if (!b || !o) return RP3DG_EINVAL;
+    o->nb_bodies = b->nb;
+    o->nb_colliders = b->nc;
+    o->nb_joints = 0;
+    o->nb_worlds = b->nworlds;
+    o->nb_pairs = b->nb_pairs;
+    o->nb_manifolds = b->hcounters->nb_manifolds;
+    o->nb_contact_points = b->hcounters->nb_points;
+    o->nb_colors = b->last_colors;
+    o->status = b->hcounters->status;
+    return RP3DG_OK;
+}
+
+int rp3dg_batch_get_bodies(rp3dg_batch* b, float* pos, float* ori, float* lin, float* ang) {
+    if (!b) return RP3DG_EINVAL;
+    std::vector<float4> t(b->nb);
+    auto grab = [&](const float4* d, float* out, int comps) -> int {
+        if (!out) return RP3DG_OK;
+        HIPCHECK(hipMemcpy(t.data(), d, sizeof(float4) * b->nb, hipMemcpyDeviceToHost));
+        for (int i = 0; i < b->nb; i++) {
+            out[comps * i + 0] = t[i].x;
+            out[comps * i + 1] = t[i].y;
+            out[comps * i + 2] = t[i].z;
+            if (comps == 4) out[comps * i + 3] = t[i].w;
+        }
+        return RP3DG_OK;
+    };
+    HIPCHECK(hipStreamSynchronize(b->stream));
+    int r;
+    if ((r = grab(b->B.p, pos, 3))) return r;
+    if ((r = grab(b->B.q, ori, 4))) return r;
+    if ((r = grab(b->B.v, lin, 3))) return r;
+    if ((r = grab(b->B.w, ang, 3))) return r;
+    return RP3DG_OK;
+}
+
+int rp3dg_batch_set_bodies(rp3dg_batch* b, const float* pos, const float* ori, const float* lin, const float* ang) {
+    (void)b; (void)pos; (void)ori; (void)lin; (void)ang;
+    set_error("rp3dg_batch_set_bodies: not implemented");
+    return RP3DG_EUNSUPPORTED;
+}
+
+int rp3dg_batch_get_pairs(rp3dg_batch* b, int32_t* c1, int32_t* c2, int64_t cap) {
+    if (!b) return RP3DG_EINVAL;
+    const DevPairs& P = b->cur_is_a ? b->Pa : b->Pb;
+    const int n = b->nb_pairs;
+    if (cap < n) { set_error("capacity too small"); return RP3DG_EINVAL; }
+    std::vector<int2> col(std::max(n, 1));
+    HIPCHECK(hipStreamSynchronize(b->stream));
+    if (n) HIPCHECK(hipMemcpy(col.data(), P.col, sizeof(int2) * n, hipMemcpyDeviceToHost));
+    for (int i = 0; i < n; i++) { c1[i] = col[i].x; c2[i] = col[i].y; }
+    return n;
+}
+
+int rp3dg_batch_get_contacts(rp3dg_batch* b, rp3dg_contact_view* o, int64_t cap) {
+    if (!b || !o) return RP3DG_EINVAL;
+    const int nm = b->hcounters->nb_manifolds;
+    if (cap < nm) { set_error("capacity too small"); return RP3DG_EINVAL; }
+    if (nm == 0) return 0;
+    const DevPairs& P = b->cur_is_a ? b->Pa : b->Pb;
+    HIPCHECK(hipStreamSynchronize(b->stream));
+    std::vector<int> order(nm), mlist(nm), pairs(nm), colors(nm);
+    std::vector<uint64_t> ck(nm);
+    HIPCHECK(hipMemcpy(pairs.data(), b->S.pair, sizeof(int) * nm, hipMemcpyDeviceToHost));
+    HIPCHECK(hipMemcpy(ck.data(), b->ckeys, sizeof(uint64_t) * nm, hipMemcpyDeviceToHost));
+    const int np = b->nb_pairs;
+    std::vector<int2> col(np);
+    std::vector<int> ncp(np);
+    std::vector<float4> pn(4 * (size_t)np), pl1(4 * (size_t)np), pl2(4 * (size_t)np), fimp(np);
+    HIPCHECK(hipMemcpy(col.data(), P.col, sizeof(int2) * np, hipMemcpyDeviceToHost));
+    HIPCHECK(hipMemcpy(ncp.data(), P.ncp, sizeof(int) * np, hipMemcpyDeviceToHost));
+    HIPCHECK(hipMemcpy(pn.data(), P.pn, sizeof(float4) * 4 * np, hipMemcpyDeviceToHost));
+    HIPCHECK(hipMemcpy(pl1.data(), P.pl1, sizeof(float4) * 4 * np, hipMemcpyDeviceToHost));
+    HIPCHECK(hipMemcpy(pl2.data(), P.pl2, sizeof(float4) * 4 * np, hipMemcpyDeviceToHost));
+    HIPCHECK(hipMemcpy(fimp.data(), P.fimp, sizeof(float4) * np, hipMemcpyDeviceToHost));
+    for (int m = 0; m < nm; m++) {
+        const int k = pairs[m];
+        if (o->colliders) { o->colliders[2 * m] = col[k].x; o->colliders[2 * m + 1] = col[k].y; }
+        if (o->nb_points) o->nb_points[m] = ncp[k];
+        if (o->color) o->color[m] = (int)ck[m];
+        for (int p = 0; p < 4; p++) {
+            const float4 a = pn[4 * k + p], c = pl1[4 * k + p], d = pl2[4 * k + p];
+            if (o->normal_depth) { float* q = o->normal_depth + 16 * m + 4 * p; q[0] = a.x; q[1] = a.y; q[2] = a.z; q[3] = a.w; }
+            if (o->local1) { float* q = o->local1 + 12 * m + 3 * p; q[0] = c.x; q[1] = c.y; q[2] = c.z; }
+            if (o->local2) { float* q = o->local2 + 12 * m + 3 * p; q[0] = d.x; q[1] = d.y; q[2] = d.z; }
+            if (o->impulse) o->impulse[4 * m + p] = c.w;
+        }
+        if (o->friction) { o->friction[3 * m] = fimp[k].x; o->friction[3 * m + 1] = fimp[k].y; o->friction[3 * m + 2] = fimp[k].z; }
+    }
+    return nm;
+}
+
+int rp3dg_batch_get_joint_order(rp3dg_batch* b, int32_t* order, int64_t cap) {
+    (void)order; (void)cap;
+    if (!b) return RP3DG_EINVAL;
+    return 0;
+}
+
+int rp3dg_batch_profile(rp3dg_batch* b, int32_t enable) {
+    if (!b) return RP3DG_EINVAL;
+    b->profiling = enable != 0;
+    b->stats.clear();
+    return RP3DG_OK;
+}
+
+int rp3dg_batch_get_kernel_times(rp3dg_batch* b, const char* name, double* mean_ms, int64_t* launches) {
+    if (!b || !name) return RP3DG_EINVAL;
+    auto it = b->stats.find(name);
+    if (it == b->stats.end()) { if (mean_ms) *mean_ms = 0; if (launches) *launches = 0; return RP3DG_OK; }
+    if (mean_ms) *mean_ms = it->second.launches ? it->second.total_ms / it->second.launches : 0.0;
+    if (launches) *launches = it->second.launches;
+    return RP3DG_OK;
+}
+
+int rp3dg_batch_step_graph(rp3dg_batch* b, float dt, int32_t n) { return rp3dg_batch_step(b, dt, n); }
+
+int rp3dg_batch_set_residual_mode(rp3dg_batch* b, float thr, int32_t maxit) {
+    (void)b; (void)thr; (void)maxit;
+    set_error("residual mode not implemented yet");
+    return RP3DG_EUNSUPPORTED;
+}
+int rp3dg_batch_get_iteration_stats(rp3dg_batch* b, double* mean_it, double* max_res) {
+    (void)b; if (mean_it) *mean_it = 0; if (max_res) *max_res = 0;
+    return RP3DG_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,483 @@
+// Broadphase of the MI355X step path.
+//
+// The reference keeps an incremental DynamicAABBTree of *fat* AABBs and only
+// re-inserts a collider when its tight AABB leaves its fat AABB
+// (DynamicAABBTree.cpp:186-214).  Because a fat AABB changes only on such a
+// move, the reference pair set after computeBroadPhase() is exactly
+//   { collider pairs whose current fat AABBs overlap (closed test, AABB.h:179) }
+// filtered by different-body / category masks / no-collision body pairs
+// (CollisionDetectionSystem.cpp:270-343).  We therefore keep the persistent
+// fat AABB per collider (same FP32 refit as BroadPhaseSystem.cpp:155-185) and
+// rebuild a linear BVH over them every step:
+//   bp_bounds -> bp_morton -> radix sort (LDS, stable, 8-bit digits)
+//   -> bp_karras (Karras 2012 radix tree) -> bp_fit (bottom-up AABBs)
+//   -> bp_traverse (count) -> scan -> bp_traverse (write, per-collider sorted
+//      segments) -> bp_merge (carry orientation, temporal-coherence record and
+//      contact cache of surviving pairs; orient new pairs like the reference).
+#include <hip/hip_runtime.h>
+
+#include "kcommon.h"
+#include "rp3d_internal.h"
+
+namespace rp3dg {
+
+// ---------------------------------------------------------------------------------------------
+// Collider world transforms and fat AABB refit (DynamicsSystem.cpp:104-109, BroadPhaseSystem.cpp
+// :155-185, CollisionShape.cpp:54-100, SphereShape.cpp:49-58, DynamicAABBTree.cpp:146-214).
+// ---------------------------------------------------------------------------------------------
+
+__device__ inline void local_bounds(int type, float4 s, const float4* hull_bounds, V3& lo, V3& hi) {
+    if (type == SHAPE_CAPSULE) {  // CapsuleShape.cpp:90-94
+        lo = v3(-s.x, -s.y - s.x, -s.x);
+        hi = v3(s.x, s.y + s.x, s.x);
+    } else if (type == SHAPE_BOX) {  // BoxShape.cpp:149-151
+        lo = v3(-s.x, -s.y, -s.z);
+        hi = v3(s.x, s.y, s.z);
+    } else {  // ConvexMeshShape.cpp:208-212 (scale 1)
+        const int h = (int)s.x;
+        lo = xyz(hull_bounds[2 * h]);
+        hi = xyz(hull_bounds[2 * h + 1]);
+    }
+}
+
+__device__ inline void transformed_aabb(int type, float4 s, const float4* hull_bounds, const Tf& t, V3& mn, V3& mx) {
+    if (type == SHAPE_SPHERE) {  // SphereShape::computeTransformedAABB
+        const V3 e = v3(s.x, s.x, s.x);
+        mn = t.p - e;
+        mx = t.p + e;
+        return;
+    }
+    V3 lo, hi;
+    local_bounds(type, s, hull_bounds, lo, hi);
+    const M3 m = qmat(t.q);
+    for (int i = 0; i < 3; i++) {
+        float rmin = comp(t.p, i), rmax = comp(t.p, i);
+        for (int j = 0; j < 3; j++) {
+            const float e = mel(m, i, j) * comp(lo, j);
+            const float f = mel(m, i, j) * comp(hi, j);
+            if (e < f) { rmin += e; rmax += f; }
+            else { rmin += f; rmax += e; }
+        }
+        setcomp(mn, i, rmin);
+        setcomp(mx, i, rmax);
+    }
+}
+
+__device__ inline void fatten(V3 mn, V3 mx, V3& fmn, V3& fmx) {
+    // DynamicAABBTree.cpp:153 / :203: gap = extent * 0.08 * 0.5
+    const V3 ext = mx - mn;
+    const V3 gap = (ext * 0.08f) * 0.5f;
+    fmn = mn - gap;
+    fmx = mx + gap;
+}
+
+// init = 1: collider creation (RigidBody::addCollider -> BroadPhaseSystem::addCollider), every collider
+// gets a fat AABB and enters the moved set.  init = 0: end-of-step refit of enabled colliders.
+__global__ void bp_refit(DevBodies B, DevColliders C, const float4* hull_bounds, int init) {
+    RP3DG_GRID_STRIDE(i, C.n) {
+        const int b = C.body[i];
+        const int bf = B.flags[b];
+        int moved = 0;
+        if (init || (bf & BF_ENABLED)) {
+            const Tf bt = tf(ld3(B.p, b), ldq(B.q, b));
+            const Tf lt = tf(ld3(C.lpos, i), ldq(C.lq, i));
+            const Tf wt = tmul(bt, lt);
+            C.wpos[i] = f4(wt.p, 0.f);
+            C.wq[i] = f4q(wt.q);
+            V3 mn, mx;
+            transformed_aabb(C.type[i], C.shape[i], hull_bounds, wt, mn, mx);
+            bool inside = false;
+            if (!init) {
+                const V3 fmn = ld3(C.amin, i), fmx = ld3(C.amax, i);
+                // AABB::contains(const AABB&) (AABB.h:246-256)
+                inside = fmn.x <= mn.x && fmn.y <= mn.y && fmn.z <= mn.z && fmx.x >= mx.x && fmx.y >= mx.y &&
+                         fmx.z >= mx.z;
+            }
+            if (!inside) {
+                V3 fmn, fmx;
+                fatten(mn, mx, fmn, fmx);
+                C.amin[i] = f4(fmn, 0.f);
+                C.amax[i] = f4(fmx, 0.f);
+                moved = 1;
+            }
+        }
+        C.moved[i] = moved;
+    }
+}
+
+// ---------------------------------------------------------------------------------------------
+// Morton keys over fat-AABB centres; the world id sits above the 30 Morton bits so that every
+// world occupies a contiguous key range and owns one subtree of the radix tree.
+// ---------------------------------------------------------------------------------------------
+
+__global__ void bp_bounds(DevColliders C, float4* partial) {
+    __shared__ float smn[3][256], smx[3][256];
+    V3 mn = v3(kDecimalLargest, kDecimalLargest, kDecimalLargest);
+    V3 mx = v3(kDecimalSmallest, kDecimalSmallest, kDecimalSmallest);
+    RP3DG_GRID_STRIDE(i, C.n) {
+        const V3 a = ld3(C.amin, i), b = ld3(C.amax, i);
+        const V3 c = v3((a.x + b.x) * 0.5f, (a.y + b.y) * 0.5f, (a.z + b.z) * 0.5f);
+        mn = vmin(mn, c);
+        mx = vmax(mx, c);
+    }
+    const int t = threadIdx.x;
+    smn[0][t] = mn.x; smn[1][t] = mn.y; smn[2][t] = mn.z;
+    smx[0][t] = mx.x; smx[1][t] = mx.y; smx[2][t] = mx.z;
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+        if (t < s) {
+            for (int k = 0; k < 3; k++) {
+                smn[k][t] = fminf(smn[k][t], smn[k][t + s]);
+                smx[k][t] = fmaxf(smx[k][t], smx[k][t + s]);
+            }
+        }
+        __syncthreads();
+    }
+    if (t == 0) {
+        partial[2 * blockIdx.x] = make_float4(smn[0][0], smn[1][0], smn[2][0], 0.f);
+        partial[2 * blockIdx.x + 1] = make_float4(smx[0][0], smx[1][0], smx[2][0], 0.f);
+    }
+}
+
+__device__ inline uint32_t expand10(uint32_t v) {
+    v &= 0x3ffu;
+    v = (v | (v << 16)) & 0x030000FFu;
+    v = (v | (v << 8)) & 0x0300F00Fu;
+    v = (v | (v << 4)) & 0x030C30C3u;
+    v = (v | (v << 2)) & 0x09249249u;
+    return v;
+}
+
+__global__ void bp_morton(DevColliders C, const float4* partial, int nparts, uint64_t* keys, int* vals) {
+    __shared__ float4 bmn, bmx;
+    if (threadIdx.x == 0) {
+        float4 mn = partial[0], mx = partial[1];
+        for (int p = 1; p < nparts; p++) {
+            const float4 a = partial[2 * p], b = partial[2 * p + 1];
+            mn.x = fminf(mn.x, a.x); mn.y = fminf(mn.y, a.y); mn.z = fminf(mn.z, a.z);
+            mx.x = fmaxf(mx.x, b.x); mx.y = fmaxf(mx.y, b.y); mx.z = fmaxf(mx.z, b.z);
+        }
+        bmn = mn;
+        bmx = mx;
+    }
+    __syncthreads();
+    const float sx = bmx.x > bmn.x ? 1023.0f / (bmx.x - bmn.x) : 0.f;
+    const float sy = bmx.y > bmn.y ? 1023.0f / (bmx.y - bmn.y) : 0.f;
+    const float sz = bmx.z > bmn.z ? 1023.0f / (bmx.z - bmn.z) : 0.f;
+    RP3DG_GRID_STRIDE(i, C.n) {
+        const V3 a = ld3(C.amin, i), b = ld3(C.amax, i);
+        const float cx = (a.x + b.x) * 0.5f, cy = (a.y + b.y) * 0.5f, cz = (a.z + b.z) * 0.5f;
+        const uint32_t qx = (uint32_t)fminf(fmaxf((cx - bmn.x) * sx, 0.f), 1023.f);
+        const uint32_t qy = (uint32_t)fminf(fmaxf((cy - bmn.y) * sy, 0.f), 1023.f);
+        const uint32_t qz = (uint32_t)fminf(fmaxf((cz - bmn.z) * sz, 0.f), 1023.f);
+        const uint32_t m = (expand10(qx) << 2) | (expand10(qy) << 1) | expand10(qz);
+        keys[i] = ((uint64_t)(uint32_t)C.world[i] << 30) | m;
+        vals[i] = i;
+    }
+}
+
+// ---------------------------------------------------------------------------------------------
+// Stable LSD radix sort, 8-bit digits.  Tiles of kSortTile keys per block; inside a tile keys are
+// ranked by wave-level multisplit (8 ballots give the lanes sharing a digit) and cross-wave
+// prefix counts in LDS, so equal digits keep their input order.
+// ---------------------------------------------------------------------------------------------
+
+constexpr int kSortThreads = 256;
+constexpr int kSortItems = 16;
+constexpr int kSortTile = kSortThreads * kSortItems;
+
+__global__ __launch_bounds__(kSortThreads) void rs_hist(const uint64_t* keys, int n, int shift, int* hist, int nblocks) {
+    __shared__ int h[256];
+    h[threadIdx.x] = 0;
+    __syncthreads();
+    const int base = blockIdx.x * kSortTile;
+    for (int k = 0; k < kSortItems; k++) {
+        const int i = base + k * kSortThreads + threadIdx.x;
+        if (i < n) atomicAdd(&h[(keys[i] >> shift) & 0xff], 1);
+    }
+    __syncthreads();
+    hist[threadIdx.x * nblocks + blockIdx.x] = h[threadIdx.x];
+}
+
+__device__ __forceinline__ uint64_t lanemask_lt() {
+    const int lane = lane_id();
+    return lane == 0 ? 0ull : (~0ull >> (64 - lane));
+}
+
+__global__ __launch_bounds__(kSortThreads) void rs_scatter(const uint64_t* kin, const int* vin, uint64_t* kout,
+                                                           int* vout, int n, int shift, const int* offs, int nblocks) {
+    __shared__ int base[256];
+    __shared__ int wcnt[kSortThreads / kWave][256];
+    const int t = threadIdx.x;
+    base[t] = offs[t * nblocks + blockIdx.x];
+    const int wid = t / kWave;
+    const int blk = blockIdx.x * kSortTile;
+    for (int k = 0; k < kSortItems; k++) {
+        for (int w = 0; w < kSortThreads / kWave; w++) wcnt[w][t] = 0;
+        __syncthreads();
+        const int i = blk + k * kSortThreads + t;
+        const bool valid = i < n;
+        uint64_t key = valid ? kin[i] : 0;
+        const int d = valid ? (int)((key >> shift) & 0xff) : 0;
+        uint64_t peers = __ballot(valid);
+#pragma unroll
+        for (int b = 0; b < 8; b++) {
+            const bool bit = (d >> b) & 1;
+            const uint64_t bal = __ballot(bit);
+            peers &= bit ? bal : ~bal;
+        }
+        const int rank = __popcll(peers & lanemask_lt());
+        if (valid && rank == 0) wcnt[wid][d] = __popcll(peers);
+        __syncthreads();
+        {  // exclusive prefix over waves for digit t, advance base
+            int run = 0;
+            for (int w = 0; w < kSortThreads / kWave; w++) {
+                const int c = wcnt[w][t];
+                wcnt[w][t] = run;
+                run += c;
+            }
+            wcnt[0][t] += 0;
+            __syncthreads();
+            if (valid) {
+                const int pos = base[d] + wcnt[wid][d] + rank;
+                kout[pos] = key;
+                vout[pos] = vin[i];
+            }
+            __syncthreads();
+            base[t] += run;
+        }
+        __syncthreads();
+    }
+}
+
+// ---------------------------------------------------------------------------------------------
+// Karras (2012) radix tree: internal nodes [0, n-1), leaf of sorted position s is node n-1+s.
+// ---------------------------------------------------------------------------------------------
+
+__device__ __forceinline__ int delta(const uint64_t* k, int n, int i, int j) {
+    if (j < 0 || j >= n) return -1;
+    const uint64_t a = k[i], b = k[j];
+    if (a == b) return 64 + __clz((uint32_t)(i ^ j));
+    return __clzll(a ^ b);
+}
+
+__global__ void bp_karras(const uint64_t* k, int n, int2* child, int* parent, int2* range) {
+    RP3DG_GRID_STRIDE(i, n - 1) {
+        const int d = (delta(k, n, i, i + 1) - delta(k, n, i, i - 1)) >= 0 ? 1 : -1;
+        const int dmin = delta(k, n, i, i - d);
+        int lmax = 2;
+        while (delta(k, n, i, i + lmax * d) > dmin) lmax <<= 1;
+        int l = 0;
+        for (int t = lmax >> 1; t >= 1; t >>= 1)
+            if (delta(k, n, i, i + (l + t) * d) > dmin) l += t;
+        const int j = i + l * d;
+        const int dnode = delta(k, n, i, j);
+        int s = 0;
+        int t = l;
+        do {
+            t = (t + 1) >> 1;
+            if (delta(k, n, i, i + (s + t) * d) > dnode) s += t;
+        } while (t > 1);
+        const int g = i + s * d + (d < 0 ? d : 0);
+        const int lo = min(i, j), hi = max(i, j);
+        const int left = (lo == g) ? (n - 1 + g) : g;
+        const int right = (hi == g + 1) ? (n - 1 + g + 1) : (g + 1);
+        child[i] = make_int2(left, right);
+        parent[left] = i;
+        parent[right] = i;
+        range[i] = make_int2(lo, hi);
+    }
+}
+
+// Leaves: copy the fat AABBs in sorted order.  Internal nodes: union, bottom-up, the second child
+// to arrive at a node continues upwards (flags reset by the host before the launch).
+__global__ void bp_fit(DevColliders C, const int* sorted, int n, const int2* child, const int* parent, float4* nmin,
+                       float4* nmax, int* flags) {
+    RP3DG_GRID_STRIDE(s, n) {
+        const int c = sorted[s];
+        int node = n - 1 + s;
+        float4 mn = C.amin[c], mx = C.amax[c];
+        nmin[node] = mn;
+        nmax[node] = mx;
+        if (n == 1) continue;
+        int p = parent[node];
+        for (int depth = 0; depth < n; depth++) {   // a path to the root has fewer than n internal nodes
+            __threadfence();  // release this subtree's AABB before signalling the parent
+            if (atomicAdd(&flags[p], 1) == 0) break;
+            __threadfence();  // acquire the sibling's AABB (written on another CU / XCD)
+            const int2 ch = child[p];
+            const int other = (ch.x == node) ? ch.y : ch.x;
+            const float4 omn = nmin[other];
+            const float4 omx = nmax[other];
+            mn = make_float4(fminf(mn.x, omn.x), fminf(mn.y, omn.y), fminf(mn.z, omn.z), 0.f);
+            mx = make_float4(fmaxf(mx.x, omx.x), fmaxf(mx.y, omx.y), fmaxf(mx.z, omx.z), 0.f);
+            nmin[p] = mn;
+            nmax[p] = mx;
+            if (p == 0) break;
+            node = p;
+            p = parent[p];
+        }
+    }
+}
+
+// Root node of every world: the node whose leaf range is exactly the world's key range.
+__global__ void bp_world_roots(const uint64_t* k, int n, const int2* range, int* root) {
+    RP3DG_GRID_STRIDE(i, 2 * n - 1) {
+        int lo, hi;
+        if (i < n - 1) { const int2 r = range[i]; lo = r.x; hi = r.y; }
+        else { lo = hi = i - (n - 1); }
+        const uint32_t w = (uint32_t)(k[lo] >> 30);
+        if ((uint32_t)(k[hi] >> 30) != w) continue;
+        if (lo > 0 && (uint32_t)(k[lo - 1] >> 30) == w) continue;
+        if (hi < n - 1 && (uint32_t)(k[hi + 1] >> 30) == w) continue;
+        root[w] = i;
+    }
+}
+
+__device__ __forceinline__ bool aabb_overlap(float4 amn, float4 amx, float4 bmn, float4 bmx) {
+    // AABB::testCollision (AABB.h:179-187), closed intervals
+    if (amx.x < bmn.x || bmx.x < amn.x) return false;
+    if (amx.y < bmn.y || bmx.y < amn.y) return false;
+    if (amx.z < bmn.z || bmx.z < amn.z) return false;
+    return true;
+}
+
+__device__ __forceinline__ bool no_collision_pair(const uint64_t* nc, int nnc, uint64_t bk) {
+    int lo = 0, hi = nnc - 1;
+    while (lo <= hi) {
+        const int mid = (lo + hi) >> 1;
+        const uint64_t v = nc[mid];
+        if (v == bk) return true;
+        if (v < bk) lo = mid + 1;
+        else hi = mid - 1;
+    }
+    return false;
+}
+
+constexpr int kTravStack = 64;
+
+// Pair filter of CollisionDetectionSystem::updateOverlappingPairs (:284-327).
+__device__ __forceinline__ bool pair_allowed(const DevColliders& C, int a, int b, const uint64_t* nc, int nnc) {
+    const int ba = C.body[a], bb = C.body[b];
+    if (ba == bb) return false;
+    if (nnc > 0) {
+        const uint64_t bk = ((uint64_t)(uint32_t)min(ba, bb) << 32) | (uint32_t)max(ba, bb);
+        if (no_collision_pair(nc, nnc, bk)) return false;
+    }
+    const uint32_t fa = C.filter[a], fb = C.filter[b];
+    const uint32_t cata = fa & 0xffffu, maska = fa >> 16, catb = fb & 0xffffu, maskb = fb >> 16;
+    return (maska & catb) != 0 && (cata & maskb) != 0;
+}
+
+// Pass 0 counts, pass 1 writes each collider's partner list (partners with a larger collider index)
+// sorted ascending into its segment.
+__global__ void bp_traverse(DevColliders C, const int* sorted, int n, const int2* child, const float4* nmin,
+                            const float4* nmax, const int* root, const uint64_t* nc, int nnc, int* count,
+                            const int* offset, uint64_t* out_keys, int capacity, int pass, int* status) {
+    RP3DG_GRID_STRIDE(s, n) {
+        const int c = sorted[s];
+        const float4 amn = C.amin[c], amx = C.amax[c];
+        int stack[kTravStack];
+        int sp = 0;
+        stack[sp++] = root[C.world[c]];
+        int cnt = 0;
+        const int base = pass ? offset[c] : 0;
+        while (sp > 0) {
+            const int node = stack[--sp];
+            if (!aabb_overlap(amn, amx, nmin[node], nmax[node])) continue;
+            if (node >= n - 1) {
+                const int o = sorted[node - (n - 1)];
+                if (o <= c) continue;
+                if (!pair_allowed(C, c, o, nc, nnc)) continue;
+                if (pass) {
+                    const uint64_t key = ((uint64_t)(uint32_t)c << 32) | (uint32_t)o;
+                    int pos = base + cnt;
+                    if (pos < capacity) {
+                        // insertion into the sorted segment
+                        while (pos > base && out_keys[pos - 1] > key) {
+                            out_keys[pos] = out_keys[pos - 1];
+                            pos--;
+                        }
+                        out_keys[pos] = key;
+                    }
+                }
+                cnt++;
+            } else {
+                if (sp + 2 > kTravStack) { atomicOr(status, STATUS_STACK_OVERFLOW); break; }
+                const int2 ch = child[node];
+                stack[sp++] = ch.x;
+                stack[sp++] = ch.y;
+            }
+        }
+        if (!pass) count[c] = cnt;
+    }
+}
+
+// ---------------------------------------------------------------------------------------------
+// Pair table update.  Surviving pairs keep collider order, LastFrameCollisionInfo and the contact
+// cache.  A new pair is oriented as OverlappingPairs::addPair receives it: collider1 is the moved
+// shape whose broadphase query reported the pair first, i.e. the moved one, or the one that comes
+// first in the moved-shape set iteration order (rank) when both moved this step.
+// ---------------------------------------------------------------------------------------------
+
+// CollisionDispatch::selectAlgorithm (CollisionDispatch.cpp:80-115) on CollisionShapeType
+__device__ __forceinline__ int shape_kind(int t) { return t == SHAPE_SPHERE ? 0 : (t == SHAPE_CAPSULE ? 1 : 2); }
+__device__ __forceinline__ int select_algo(int t1, int t2) {
+    int a = shape_kind(t1), b = shape_kind(t2);
+    if (a > b) { const int tmp = a; a = b; b = tmp; }
+    if (a == 0 && b == 0) return ALGO_SPHERE_SPHERE;
+    if (a == 0 && b == 1) return ALGO_SPHERE_CAPSULE;
+    if (a == 1 && b == 1) return ALGO_CAPSULE_CAPSULE;
+    if (a == 0 && b == 2) return ALGO_SPHERE_POLY;
+    if (a == 1 && b == 2) return ALGO_CAPSULE_POLY;
+    return ALGO_POLY_POLY;
+}
+
+__global__ void bp_merge(DevColliders C, const uint64_t* keys, const int* nkeys_ptr, DevPairs O, int nold, DevPairs N) {
+    const int nkeys = *nkeys_ptr;
+    RP3DG_GRID_STRIDE(k, nkeys) {
+        const uint64_t key = keys[k];
+        N.key[k] = key;
+        int lo = 0, hi = nold - 1, found = -1;
+        while (lo <= hi) {
+            const int mid = (lo + hi) >> 1;
+            const uint64_t v = O.key[mid];
+            if (v == key) { found = mid; break; }
+            if (v < key) lo = mid + 1;
+            else hi = mid - 1;
+        }
+        if (found >= 0) {
+            N.col[k] = O.col[found];
+            N.algo[k] = O.algo[found];
+            N.lf[k] = O.lf[found];
+            N.lfaxis[k] = O.lfaxis[found];
+            N.ncp[k] = O.ncp[found];
+            N.fv1[k] = O.fv1[found];
+            N.fv2[k] = O.fv2[found];
+            N.fimp[k] = O.fimp[found];
+            for (int p = 0; p < kMaxManifoldPoints; p++) {
+                N.pn[4 * k + p] = O.pn[4 * found + p];
+                N.pl1[4 * k + p] = O.pl1[4 * found + p];
+                N.pl2[4 * k + p] = O.pl2[4 * found + p];
+            }
+        } else {
+            const int a = (int)(key >> 32), b = (int)(key & 0xffffffffu);
+            const int ma = C.moved[a], mb = C.moved[b];
+            bool a_first;
+            if (ma && !mb) a_first = true;
+            else if (mb && !ma) a_first = false;
+            else a_first = C.rank[a] < C.rank[b];
+            const int c1 = a_first ? a : b, c2 = a_first ? b : a;
+            N.col[k] = make_int2(c1, c2);
+            N.algo[k] = select_algo(C.type[c1], C.type[c2]);
+            N.lf[k] = 0u;
+            N.lfaxis[k] = make_float4(0.f, 1.f, 0.f, 0.f);   // LastFrameCollisionInfo() default axis
+            N.ncp[k] = 0;
+            N.fimp[k] = make_float4(0.f, 0.f, 0.f, 0.f);
+            N.fv1[k] = make_float4(0.f, 0.f, 0.f, 0.f);
+            N.fv2[k] = make_float4(0.f, 0.f, 0.f, 0.f);
+        }
+    }
+}
+
+}  // namespace rp3dg

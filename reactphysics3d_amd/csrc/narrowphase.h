@@ -1,0 +1,33 @@
+// Kernel argument block of the narrowphase kernels.
+#pragma once
+
+#include "rp3d_internal.h"
+
+namespace rp3dg {
+
+struct NpArgs {
+    DevBodies B;
+    DevColliders C;
+    DevPairs P;
+    const int* nb_pairs;
+    const HullHeader* hulls;
+    const float4* hull_verts;
+    const float4* hull_normals;
+    const int4* hull_edges;
+    const int* hull_face_edge;
+    const int* hull_face_verts;
+    const int* hull_face_vidx;
+    float persistent_threshold;
+    float cos_similar;
+};
+
+__global__ void np_classify(NpArgs A, int* lists, int list_stride, int* counts);
+__global__ void np_sphere_sphere(NpArgs A, const int* list, const int* count);
+__global__ void np_sphere_capsule(NpArgs A, const int* list, const int* count);
+__global__ void np_capsule_capsule(NpArgs A, const int* list, const int* count);
+__global__ void np_sphere_poly(NpArgs A, const int* list, const int* count);
+__global__ void np_capsule_poly(NpArgs A, const int* list, const int* count);
+__global__ void np_poly_poly(NpArgs A, const int* list, const int* count);
+void upload_box_topology(const int* edges4, const int* face_edge, const int* face_verts);
+
+}  // namespace rp3dg

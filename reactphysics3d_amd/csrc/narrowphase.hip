@@ -1,0 +1,791 @@
+// Narrowphase of the MI355X step path: one kernel per shape-pair type over a
+// compacted list of pair slots (CollisionDetectionSystem::computeMiddlePhase
+// buckets, :346-413), each thread running one test and then the contact
+// epilogue for its pair:
+//   raw contacts (<= 16, NarrowPhaseInfoBatch.h:170-185)
+//   -> potential manifold (convex pair: exactly one, CD.cpp:1247-1290)
+//   -> reduceContactPoints (CD.cpp:1483-1662) + removeDuplicated (:1673-1705)
+//   -> warm-start transfer from the pair's cached contacts (CD.cpp:1069-1160)
+//   -> cache write (the next step reads it back).
+#include <hip/hip_runtime.h>
+
+#include "kcommon.h"
+#include "narrowphase.h"
+#include "rp3d_internal.h"
+
+namespace rp3dg {
+
+__constant__ int4 c_box_edges[24];   // vertex, twin, face, next (HalfEdgeStructure of the box, PhysicsCommon.cpp:73-109)
+__constant__ int c_box_face_edge[6];
+__constant__ int c_box_face_verts[24];
+
+// ------------------------------------------------------------------------------------------------
+// Polyhedron view: BoxShape (implicit) or ConvexMeshShape (hull arena).
+// ------------------------------------------------------------------------------------------------
+struct Poly {
+    int hull;          // -1 for a box
+    V3 he;             // box half-extents
+    HullHeader h;
+    const float4* hv;  // hull vertices
+    const float4* hn;  // hull face normals
+    const int4* hedges;
+    const int* hfedge;
+    const int* hfverts;   // [2*f] = start, [2*f+1] = count into hfvidx
+    const int* hfvidx;
+
+    __device__ int nfaces() const { return hull < 0 ? 6 : h.nf; }
+    __device__ int nhalfedges() const { return hull < 0 ? 24 : h.ne; }
+    // BoxShape.h:186-205
+    __device__ V3 vertex(int i) const {
+        if (hull >= 0) return xyz(hv[h.vbase + i]);
+        switch (i) {
+            case 0: return v3(-he.x, -he.y, he.z);
+            case 1: return v3(he.x, -he.y, he.z);
+            case 2: return v3(he.x, he.y, he.z);
+            case 3: return v3(-he.x, he.y, he.z);
+            case 4: return v3(-he.x, -he.y, -he.z);
+            case 5: return v3(he.x, -he.y, -he.z);
+            case 6: return v3(he.x, he.y, -he.z);
+            default: return v3(-he.x, he.y, -he.z);
+        }
+    }
+    // BoxShape.h:208-221
+    __device__ V3 face_normal(int f) const {
+        if (hull >= 0) return xyz(hn[h.fbase + f]);
+        switch (f) {
+            case 0: return v3(0, 0, 1);
+            case 1: return v3(1, 0, 0);
+            case 2: return v3(0, 0, -1);
+            case 3: return v3(-1, 0, 0);
+            case 4: return v3(0, -1, 0);
+            default: return v3(0, 1, 0);
+        }
+    }
+    __device__ int4 edge(int e) const { return hull < 0 ? c_box_edges[e] : hedges[h.ebase + e]; }
+    __device__ int face_edge(int f) const { return hull < 0 ? c_box_face_edge[f] : hfedge[h.fbase + f]; }
+    __device__ int face_nverts(int f) const { return hull < 0 ? 4 : hfverts[2 * (h.fbase + f) + 1]; }
+    __device__ int face_vertex(int f, int k) const {
+        return hull < 0 ? c_box_face_verts[4 * f + k] : hfvidx[hfverts[2 * (h.fbase + f)] + k];
+    }
+    __device__ V3 centroid() const { return hull < 0 ? v3zero() : v3(h.cx, h.cy, h.cz); }
+    // BoxShape.h:167-172, ConvexMeshShape.cpp:80-102
+    __device__ V3 support(V3 d) const {
+        if (hull < 0)
+            return v3(d.x < 0.0f ? -he.x : he.x, d.y < 0.0f ? -he.y : he.y, d.z < 0.0f ? -he.z : he.z);
+        float best = kDecimalSmallest;
+        int bi = 0;
+        for (int i = 0; i < h.nv; i++) {
+            const float dp = dot(d, xyz(hv[h.vbase + i]));
+            if (dp > best) { bi = i; best = dp; }
+        }
+        return xyz(hv[h.vbase + bi]);
+    }
+    // ConvexPolyhedronShape.h:105-123
+    __device__ int most_anti_parallel_face(V3 dir) const {
+        float mn = kDecimalLargest;
+        int best = 0;
+        const int nf = nfaces();
+        for (int i = 0; i < nf; i++) {
+            const float dp = dot(face_normal(i), dir);
+            if (dp < mn) { mn = dp; best = i; }
+        }
+        return best;
+    }
+};
+
+__device__ inline Poly make_poly(const NpArgs& A, int collider) {
+    Poly P;
+    const float4 s = A.C.shape[collider];
+    if (A.C.type[collider] == SHAPE_BOX) {
+        P.hull = -1;
+        P.he = v3(s.x, s.y, s.z);
+    } else {
+        P.hull = (int)s.x;
+        P.h = A.hulls[P.hull];
+        P.hv = A.hull_verts;
+        P.hn = A.hull_normals;
+        P.hedges = A.hull_edges;
+        P.hfedge = A.hull_face_edge;
+        P.hfverts = A.hull_face_verts;
+        P.hfvidx = A.hull_face_vidx;
+        P.he = v3zero();
+    }
+    return P;
+}
+
+// ------------------------------------------------------------------------------------------------
+// Raw contact buffer of one narrowphase test
+// ------------------------------------------------------------------------------------------------
+struct Raw {
+    int n;
+    V3 nrm[kMaxRawContacts];
+    float depth[kMaxRawContacts];
+    V3 l1[kMaxRawContacts];
+    V3 l2[kMaxRawContacts];
+    __device__ void add(V3 normal, float d, V3 a, V3 b) {  // NarrowPhaseInfoBatch::addContactPoint
+        if (n < kMaxRawContacts) {
+            nrm[n] = normal; depth[n] = d; l1[n] = a; l2[n] = b;
+            n++;
+        }
+    }
+};
+
+// ------------------------------------------------------------------------------------------------
+// SAT, polyhedron vs polyhedron (SATAlgorithm.cpp:485-1191)
+// ------------------------------------------------------------------------------------------------
+constexpr float kSatRel = 1.002f;    // SATAlgorithm.cpp:43
+constexpr float kSatAbs = 0.0005f;   // SATAlgorithm.cpp:44
+constexpr int kClipMax = 48;
+
+// :1095-1116
+__device__ inline float sat_single_face(const Poly& P1, const Poly& P2, const Tf& t12, int f) {
+    const V3 n = P1.face_normal(f);
+    const V3 n2 = qrot(t12.q, n);
+    const V3 sp = P2.support(-n2);
+    const V3 fv = tapply(t12, P1.vertex(P1.face_vertex(f, 0)));
+    return dot(fv - sp, n2);
+}
+
+// :1119-1143
+__device__ inline float sat_faces(const Poly& P1, const Poly& P2, const Tf& t12, int& minFace) {
+    float mn = kDecimalLargest;
+    const int nf = P1.nfaces();
+    for (int f = 0; f < nf; f++) {
+        const float pen = sat_single_face(P1, P2, t12, f);
+        if (pen <= 0.0f) { minFace = f; return pen; }
+        if (pen < mn) { mn = pen; minFace = f; }
+    }
+    return mn;
+}
+
+// :1146-1191
+__device__ inline bool sat_minkowski_face(const Poly& P1, int4 e1, const Poly& P2, int4 e2, const Tf& t12) {
+    const V3 a = qrot(t12.q, P1.face_normal(e1.z));
+    const V3 b = qrot(t12.q, P1.face_normal(P1.edge(e1.y).z));
+    const V3 c = P2.face_normal(e2.z);
+    const V3 d = P2.face_normal(P2.edge(e2.y).z);
+    const V3 ev1 = P1.vertex(e1.x);
+    const V3 ev2 = P1.vertex(P1.edge(e1.y).x);
+    const V3 bxa = qrot(t12.q, ev1 - ev2);
+    const V3 fv1 = P2.vertex(e2.x);
+    const V3 fv2 = P2.vertex(P2.edge(e2.y).x);
+    const V3 dxc = fv1 - fv2;
+    const V3 nc = -c, nd = -d;
+    const float cba = dot(nc, bxa);
+    const float dba = dot(nd, bxa);
+    const float adc = dot(a, dxc);
+    const float bdc = dot(b, dxc);
+    return cba * dba < 0.0f && adc * bdc < 0.0f && cba * bdc > 0.0f;
+}
+
+// :1040-1083 (shape 1 is never a triangle here)
+__device__ inline float sat_edge_distance(V3 e1a, V3 e2a, V3 c1, V3 e1d, V3 e2d, V3& axis_out) {
+    if (parallel_vectors(e1d, e2d)) return kDecimalLargest;
+    V3 axis = unit(cross(e1d, e2d));
+    const float dp = dot(axis, c1 - e1a);
+    if (dp > 0.0f) axis = -axis;
+    axis_out = axis;
+    return -dot(axis, e2a - e1a);
+}
+
+// mathematics_functions.h:332-392
+__device__ inline int clip_polygon(const V3* in, int nin, V3 pp, V3 pn, V3* out) {
+    int nout = 0;
+    int s = nin - 1;
+    const float pnd = dot(pn, pp);
+    float sdn = dot(in[s] - pp, pn);
+    for (int e = 0; e < nin; e++) {
+        const V3 vs = in[s], ve = in[e];
+        const float edn = dot(ve - pp, pn);
+        if (edn >= 0.0f) {
+            if (sdn < 0.0f) {
+                const float t = plane_segment_intersection(vs, ve, pnd, pn);
+                if (t >= 0.0f && t <= 1.0f) { if (nout < kClipMax) out[nout++] = vs + t * (ve - vs); }
+                else { if (nout < kClipMax) out[nout++] = ve; }
+            }
+            if (nout < kClipMax) out[nout++] = ve;
+        } else {
+            if (sdn >= 0.0f) {
+                const float t = plane_segment_intersection(vs, ve, -pnd, -pn);
+                if (t >= 0.0f && t <= 1.0f) { if (nout < kClipMax) out[nout++] = vs + t * (ve - vs); }
+                else { if (nout < kClipMax) out[nout++] = vs; }
+            }
+        }
+        s = e;
+        sdn = edn;
+    }
+    return nout;
+}
+
+// :893-1036
+__device__ bool sat_face_contacts(bool refIs1, const Poly& P1, const Poly& P2, const Tf& t12, const Tf& t21, int minFace,
+                                  const Tf& T1, const Tf& T2, Raw& raw) {
+    const Tf& refToInc = refIs1 ? t12 : t21;
+    const Tf& incToRef = refIs1 ? t21 : t12;
+    const Poly& R = refIs1 ? P1 : P2;
+    const Poly& I = refIs1 ? P2 : P1;
+    const V3 axisRef = R.face_normal(minFace);
+    const V3 axisInc = qrot(refToInc.q, axisRef);
+    const V3 normalWorld = refIs1 ? qrot(T1.q, axisRef) : -(qrot(T2.q, axisRef));
+    const int incFace = I.most_anti_parallel_face(axisInc);
+    V3 bufA[kClipMax], bufB[kClipMax];
+    int na = 0, nb = 0;
+    const int niv = I.face_nverts(incFace);
+    for (int i = 0; i < niv && na < kClipMax; i++) bufA[na++] = tapply(incToRef, I.vertex(I.face_vertex(incFace, i)));
+    const int firstEdge = R.face_edge(minFace);
+    bool v1Input = false;
+    int nOut = 0;
+    int curIdx = firstEdge;
+    int4 cur = R.edge(firstEdge);
+    V3 ev1 = R.vertex(cur.x);
+    int guard = R.nhalfedges();   // a face has fewer edges than the mesh has half-edges
+    do {
+        if (--guard < 0) break;
+        v1Input = !v1Input;
+        const int4 nxt = R.edge(cur.w);
+        const V3 ev2 = R.vertex(nxt.x);
+        const V3 planeNormal = cross(axisRef, ev2 - ev1);
+        if (v1Input) nb = clip_polygon(bufA, na, ev1, planeNormal, bufB);
+        else na = clip_polygon(bufB, nb, ev1, planeNormal, bufA);
+        curIdx = cur.w;
+        cur = nxt;
+        ev1 = ev2;
+        if (v1Input) { na = 0; nOut = nb; }
+        else { nb = 0; nOut = na; }
+    } while (curIdx != firstEdge && nOut > 0);
+    const V3* clipped = v1Input ? bufB : bufA;
+    const int nclip = v1Input ? nb : na;
+    const V3 refV = R.vertex(R.edge(firstEdge).x);
+    bool found = false;
+    for (int i = 0; i < nclip; i++) {
+        const float pen = dot(refV - clipped[i], axisRef);
+        if (pen > 0.0f) {
+            found = true;
+            const V3 cInc = tapply(refToInc, clipped[i]);
+            const V3 cRef = project_point_onto_plane(clipped[i], axisRef, refV);
+            raw.add(normalWorld, pen, refIs1 ? cRef : cInc, refIs1 ? cInc : cRef);
+        }
+    }
+    return found;
+}
+
+__device__ inline uint32_t lf_set_face(uint32_t lf, bool face1, int face) {
+    lf &= ~(LF_SAT_FACE1 | LF_SAT_FACE2 | 0xff00u);
+    lf |= face1 ? LF_SAT_FACE1 : LF_SAT_FACE2;
+    lf |= ((uint32_t)face & 0xffu) << 8;
+    return lf;
+}
+__device__ inline uint32_t lf_set_edges(uint32_t lf, int e1, int e2) {
+    lf &= ~(LF_SAT_FACE1 | LF_SAT_FACE2 | 0xffff0000u);
+    lf |= ((uint32_t)e1 & 0xffu) << 16;
+    lf |= ((uint32_t)e2 & 0xffu) << 24;
+    return lf;
+}
+
+// Returns isColliding; updates the SAT part of the temporal-coherence word.
+__device__ bool sat_poly_poly(const Poly& P1, const Poly& P2, const Tf& T1, const Tf& T2, uint32_t& lf, Raw& raw) {
+    const Tf t12 = tmul(tinv(T2), T1);
+    const Tf t21 = tinv(t12);
+    float minPen = kDecimalLargest;
+    int minFace = 0;
+    bool isMinFace = false, isMinFace1 = false;
+    int minE1 = 0, minE2 = 0;
+    V3 s1a = v3zero(), s1b = v3zero(), s2a = v3zero(), s2b = v3zero(), minAxis = v3zero();
+    const bool wasColliding = (lf & LF_WAS_COLLIDING) != 0;
+
+    if ((lf & LF_VALID) && (lf & LF_WAS_SAT)) {
+        if (lf & LF_SAT_FACE1) {
+            const int f = (int)lf_face(lf);
+            const float pen = sat_single_face(P1, P2, t12, f);
+            if (!wasColliding && pen <= 0.0f) return false;
+            if (wasColliding && pen > 0.0f) {
+                if (sat_face_contacts(true, P1, P2, t12, t21, f, T1, T2, raw)) {
+                    lf = lf_set_face(lf, true, f);
+                    return true;
+                }
+            }
+        } else if (lf & LF_SAT_FACE2) {
+            const int f = (int)lf_face(lf);
+            const float pen = sat_single_face(P2, P1, t21, f);
+            if (!wasColliding && pen <= 0.0f) return false;
+            if (wasColliding && pen > 0.0f) {
+                if (sat_face_contacts(false, P1, P2, t12, t21, f, T1, T2, raw)) {
+                    lf = lf_set_face(lf, false, f);
+                    return true;
+                }
+            }
+        } else {
+            const int4 e1 = P1.edge((int)lf_edge1(lf));
+            const int4 e2 = P2.edge((int)lf_edge2(lf));
+            const V3 e1a = tapply(t12, P1.vertex(e1.x));
+            const V3 e1b = tapply(t12, P1.vertex(P1.edge(e1.w).x));
+            const V3 e1d = e1b - e1a;
+            const V3 e2a = P2.vertex(e2.x);
+            const V3 e2b = P2.vertex(P2.edge(e2.w).x);
+            const V3 e2d = e2b - e2a;
+            if (sat_minkowski_face(P1, e1, P2, e2, t12)) {
+                V3 axis;
+                const V3 c1 = tapply(t12, P1.centroid());
+                const float pen = sat_edge_distance(e1a, e2a, c1, e1d, e2d, axis);
+                if (!wasColliding && pen <= 0.0f) return false;
+                if (wasColliding && pen > 0.0f && pen < kDecimalLargest) {
+                    V3 cp1, cp2;
+                    closest_points_between_segments(e1a, e1b, e2a, e2b, cp1, cp2);
+                    const V3 vec1 = cp1 - e2a;
+                    const V3 vec2 = cp2 - e1a;
+                    const float l1 = len2(e1d);
+                    const float l2 = len2(e2d);
+                    const float t1 = dot(vec1, e2d) / l2;
+                    const float t2 = dot(vec2, e1d) / l1;
+                    if (t1 >= 0.0f && t1 <= 1.0f && t2 >= 0.0f && t2 <= 1.0f) {
+                        const V3 p1local = tapply(t21, cp1);
+                        const V3 nw = qrot(T2.q, axis);
+                        raw.add(nw, pen, p1local, cp2);
+                        return true;
+                    }
+                }
+            }
+        }
+    }
+
+    minPen = kDecimalLargest;
+    isMinFace = false;
+    int face1 = 0, face2 = 0;
+    const float pen1 = sat_faces(P1, P2, t12, face1);
+    if (pen1 <= 0.0f) { lf = lf_set_face(lf, true, face1); return false; }
+    const float pen2 = sat_faces(P2, P1, t21, face2);
+    if (pen2 <= 0.0f) { lf = lf_set_face(lf, false, face2); return false; }
+    if (pen1 < pen2 * kSatRel + kSatAbs) {
+        isMinFace = true; minPen = fmin_s(pen1, pen2); minFace = face1; isMinFace1 = true;
+    } else {
+        isMinFace = true; minPen = fmin_s(pen1, pen2); minFace = face2; isMinFace1 = false;
+    }
+    const V3 c1 = tapply(t12, P1.centroid());
+    const int ne1 = P1.nhalfedges(), ne2 = P2.nhalfedges();
+    for (int i = 0; i < ne1; i += 2) {
+        const int4 e1 = P1.edge(i);
+        const V3 e1a = tapply(t12, P1.vertex(e1.x));
+        const V3 e1b = tapply(t12, P1.vertex(P1.edge(e1.w).x));
+        const V3 e1d = e1b - e1a;
+        for (int j = 0; j < ne2; j += 2) {
+            const int4 e2 = P2.edge(j);
+            const V3 e2a = P2.vertex(e2.x);
+            const V3 e2b = P2.vertex(P2.edge(e2.w).x);
+            const V3 e2d = e2b - e2a;
+            if (sat_minkowski_face(P1, e1, P2, e2, t12)) {
+                V3 axis;
+                const float pen = sat_edge_distance(e1a, e2a, c1, e1d, e2d, axis);
+                if (pen <= 0.0f) {
+                    lf = lf_set_edges(lf, i, j);
+                    return false;
+                }
+                if ((isMinFace && pen * kSatRel + kSatAbs < minPen) || (!isMinFace && pen < minPen)) {
+                    minPen = pen; isMinFace1 = false; isMinFace = false;
+                    minE1 = i; minE2 = j;
+                    s1a = e1a; s1b = e1b; s2a = e2a; s2b = e2b;
+                    minAxis = axis;
+                }
+            }
+        }
+    }
+    if (isMinFace) {
+        const bool found = sat_face_contacts(isMinFace1, P1, P2, t12, t21, minFace, T1, T2, raw);
+        lf = lf_set_face(lf, isMinFace1, minFace);
+        return found;
+    }
+    V3 cp1, cp2;
+    closest_points_between_segments(s1a, s1b, s2a, s2b, cp1, cp2);
+    const V3 p1local = tapply(t21, cp1);
+    const V3 nw = qrot(T2.q, minAxis);
+    raw.add(nw, minPen, p1local, cp2);
+    lf = lf_set_edges(lf, minE1, minE2);
+    return true;
+}
+
+// ------------------------------------------------------------------------------------------------
+// Sphere / capsule closed forms
+// ------------------------------------------------------------------------------------------------
+
+// SphereVsSphereAlgorithm.cpp:34-113
+__device__ bool sphere_sphere(float r1, float r2, const Tf& T1, const Tf& T2, Raw& raw) {
+    const V3 d = T2.p - T1.p;
+    const float d2 = len2(d);
+    const float sum = r1 + r2;
+    const float sum2 = sum * sum;
+    if (!(d2 < sum2)) return false;
+    const float pen = sum - sqrtf(d2);
+    if (!(pen > 0.0f)) return false;
+    const Tf i1 = tinv(T1), i2 = tinv(T2);
+    V3 p1, p2, n;
+    if (d2 > kMachineEps) {
+        const V3 c2in1 = tapply(i1, T2.p);
+        const V3 c1in2 = tapply(i2, T1.p);
+        p1 = r1 * unit(c2in1);
+        p2 = r2 * unit(c1in2);
+        n = unit(d);
+    } else {
+        n = v3(0, 1, 0);
+        p1 = r1 * qrot(i1.q, n);
+        p2 = r2 * qrot(i2.q, n);
+    }
+    raw.add(n, pen, p1, p2);
+    return true;
+}
+
+// SphereVsCapsuleAlgorithm.cpp:38-152
+__device__ bool sphere_capsule(bool sphereIs1, float rs, float rc, float capHeight, const Tf& Ts, const Tf& Tc, Raw& raw) {
+    const Tf worldToCap = tinv(Tc);
+    const Tf s2c = tmul(worldToCap, Ts);
+    const V3 sc = s2c.p;
+    const float hh = capHeight * 0.5f;
+    const V3 sa = v3(0, -hh, 0), sb = v3(0, hh, 0);
+    const V3 cps = closest_point_on_segment(sa, sb, sc);
+    V3 sc2seg = cps - sc;
+    const float dist2 = len2(sc2seg);
+    const float sumR = rs + rc;
+    if (!(dist2 < sumR * sumR)) return false;
+    float pen;
+    V3 nw, cpS, cpC;
+    if (dist2 > kMachineEps) {
+        const float dist = sqrtf(dist2);
+        sc2seg = sc2seg / dist;
+        cpS = tapply(tinv(s2c), sc + sc2seg * rs);
+        cpC = cps - sc2seg * rc;
+        nw = qrot(Tc.q, sc2seg);
+        pen = sumR - dist;
+        if (!sphereIs1) nw = -nw;
+    } else {
+        const V3 seg = unit(sb - sa);
+        const float cosA1 = fabsf(seg.x);
+        const float cosA2 = fabsf(seg.y);
+        pen = sumR;
+        const V3 ncs = cosA1 < cosA2 ? cross(seg, v3(1, 0, 0)) : cross(seg, v3(0, 1, 0));
+        nw = qrot(Tc.q, ncs);
+        cpS = tapply(tinv(s2c), sc + ncs * rs);
+        cpC = sc - ncs * rc;
+    }
+    if (pen <= 0.0f) return false;
+    raw.add(nw, pen, sphereIs1 ? cpS : cpC, sphereIs1 ? cpC : cpS);
+    return true;
+}
+
+// mathematics_functions.h:216-224
+__device__ inline float point_to_line_distance(V3 a, V3 b, V3 p) {
+    const float dab = len(b - a);
+    if (dab < kMachineEps) return len(p - a);
+    return len(cross(p - a, p - b)) / dab;
+}
+
+// CapsuleVsCapsuleAlgorithm.cpp:37-243
+__device__ bool capsule_capsule(float r1, float h1, float r2, float h2, const Tf& T1, const Tf& T2, Raw& raw) {
+    const Tf c1to2 = tmul(tinv(T2), T1);
+    const float hh1 = h1 * 0.5f;
+    V3 a1 = tapply(c1to2, v3(0, -hh1, 0));
+    V3 b1 = tapply(c1to2, v3(0, hh1, 0));
+    const float hh2 = h2 * 0.5f;
+    const V3 a2 = v3(0, -hh2, 0), b2 = v3(0, hh2, 0);
+    const V3 seg1 = b1 - a1;
+    const V3 seg2 = b2 - a2;
+    const float sumR = r1 + r2;
+    const bool par = parallel_vectors(seg1, seg2);
+    if (par) {
+        const float perp = point_to_line_distance(a1, b1, a2);
+        if (perp >= sumR) return false;
+        const float d1 = dot(seg1, a1);
+        const float d2 = -dot(seg1, b1);
+        float t1 = plane_segment_intersection(b2, a2, d1, seg1);
+        float t2 = plane_segment_intersection(a2, b2, d2, -seg1);
+        if (t1 > 0.0f && t2 > 0.0f) {
+            if (t1 > 1.0f) t1 = 1.0f;
+            const V3 clipA = b2 - t1 * seg2;
+            if (t2 > 1.0f) t2 = 1.0f;
+            const V3 clipB = a2 + t2 * seg2;
+            const V3 s1n = unit(seg1);
+            const V3 pOn1 = a1 + dot(s1n, a2 - a1) * s1n;
+            V3 n2, s1to2;
+            if (perp > kMachineEps) {
+                s1to2 = a2 - pOn1;
+                n2 = unit(s1to2);
+            } else {
+                const V3 s2n = unit(seg2);
+                const float cA1 = fabsf(s2n.x), cA2 = fabsf(s2n.y);
+                s1to2 = v3zero();
+                n2 = cA1 < cA2 ? cross(s2n, v3(1, 0, 0)) : cross(s2n, v3(0, 1, 0));
+            }
+            const Tf c2to1 = tinv(c1to2);
+            const V3 pa1 = tapply(c2to1, clipA - s1to2 + n2 * r1);
+            const V3 pb1 = tapply(c2to1, clipB - s1to2 + n2 * r1);
+            const V3 pa2 = clipA - n2 * r2;
+            const V3 pb2 = clipB - n2 * r2;
+            const float pen = sumR - perp;
+            const V3 nw = qrot(T2.q, n2);
+            raw.add(nw, pen, pa1, pa2);
+            raw.add(nw, pen, pb1, pb2);
+            return true;
+        }
+    }
+    V3 c1, c2;
+    closest_points_between_segments(a1, b1, a2, b2, c1, c2);
+    V3 d = c2 - c1;
+    const float dd = len2(d);
+    if (!(dd < sumR * sumR)) return false;
+    if (dd > kMachineEps) {
+        const float dist = sqrtf(dd);
+        d = d / dist;
+        const V3 p1 = tapply(tinv(c1to2), c1 + d * r1);
+        const V3 p2 = c2 - d * r2;
+        const V3 nw = qrot(T2.q, d);
+        const float pen = fmax_s(sumR - dist, kMachineEps);
+        raw.add(nw, pen, p1, p2);
+    } else if (par) {
+        const float sq = len2(a1 - c2);
+        const V3 ext = sq > kMachineEps ? a1 : b1;
+        const V3 n2 = normalized(c2 - ext);
+        const V3 p1 = tapply(tinv(c1to2), c1 + n2 * r1);
+        const V3 p2 = c2 - n2 * r2;
+        raw.add(qrot(T2.q, n2), sumR, p1, p2);
+    } else {
+        const V3 n2 = normalized(cross(seg1, seg2));
+        const V3 p1 = tapply(tinv(c1to2), c1 + n2 * r1);
+        const V3 p2 = c2 - n2 * r2;
+        raw.add(qrot(T2.q, n2), sumR, p1, p2);
+    }
+    return true;
+}
+
+// ------------------------------------------------------------------------------------------------
+// Contact epilogue: manifold reduction, duplicate removal, warm-start transfer, cache write.
+// ------------------------------------------------------------------------------------------------
+
+// CollisionDetectionSystem::reduceContactPoints (:1483-1662): picks 4 of n > 4 raw points.
+__device__ void reduce_points(const Raw& raw, const Tf& T1, int* keep) {
+    int cand[kMaxRawContacts];
+    int nc = raw.n;
+    for (int i = 0; i < nc; i++) cand[i] = i;
+    const Tf w2s = tinv(T1);
+    const V3 nS1 = qrot(w2s.q, raw.nrm[cand[0]]);
+    const V3 dir = v3(1, 1, 1);
+    float maxDot = kDecimalSmallest;
+    int idx = 0;
+    for (int i = 0; i < nc; i++) {
+        const float dp = dot(dir, raw.l1[cand[i]]);
+        if (dp > maxDot) { maxDot = dp; idx = i; }
+    }
+    keep[0] = cand[idx];
+    cand[idx] = cand[nc - 1]; nc--;
+    float maxDist = 0.0f;
+    idx = 0;
+    for (int i = 0; i < nc; i++) {
+        const float d = len2(raw.l1[keep[0]] - raw.l1[cand[i]]);
+        if (d >= maxDist) { maxDist = d; idx = i; }
+    }
+    keep[1] = cand[idx];
+    cand[idx] = cand[nc - 1]; nc--;
+    int maxI = 0, minI = 0;
+    float minArea = 0.0f, maxArea = 0.0f;
+    for (int i = 0; i < nc; i++) {
+        const V3 nf = raw.l1[keep[0]] - raw.l1[cand[i]];
+        const V3 ns = raw.l1[keep[1]] - raw.l1[cand[i]];
+        const float area = dot(cross(nf, ns), nS1);
+        if (area >= maxArea) { maxArea = area; maxI = i; }
+        if (area <= minArea) { minArea = area; minI = i; }
+    }
+    bool prevPositive;
+    if (maxArea > (-minArea)) {
+        prevPositive = true;
+        keep[2] = cand[maxI];
+        cand[maxI] = cand[nc - 1]; nc--;
+    } else {
+        prevPositive = false;
+        keep[2] = cand[minI];
+        cand[minI] = cand[nc - 1]; nc--;
+    }
+    float largest = 0.0f;
+    idx = 0;
+    for (int i = 0; i < nc; i++) {
+        for (int j = 0; j < 3; j++) {
+            const int va = j, vb = j < 2 ? j + 1 : 0;
+            const V3 nf = raw.l1[keep[va]] - raw.l1[cand[i]];
+            const V3 ns = raw.l1[keep[vb]] - raw.l1[cand[i]];
+            const float area = dot(cross(nf, ns), nS1);
+            if (prevPositive && area <= largest) { largest = area; idx = i; }
+            else if (!prevPositive && area >= largest) { largest = area; idx = i; }
+        }
+    }
+    keep[3] = cand[idx];
+}
+
+__device__ void contact_epilogue(const NpArgs& A, int slot, bool colliding, const Raw& raw, int collider1) {
+    DevPairs P = A.P;
+    const int oldN = P.ncp[slot];
+    if (!colliding || raw.n == 0) {
+        P.ncp[slot] = 0;
+        return;
+    }
+    int idx[kMaxRawContacts];
+    int n = raw.n;
+    for (int i = 0; i < n; i++) idx[i] = i;
+    if (n > kMaxManifoldPoints) {
+        const Tf T1 = tf(ld3(A.C.wpos, collider1), ldq(A.C.wq, collider1));
+        int keep[4];
+        reduce_points(raw, T1, keep);
+        for (int i = 0; i < 4; i++) idx[i] = keep[i];
+        n = 4;
+    }
+    // removeDuplicatedContactPointsInManifold (:1673-1705)
+    const float thr2 = 0.01f * 0.01f;
+    for (int i = 0; i < n; i++) {
+        for (int j = i + 1; j < n; j++) {
+            const float d2 = len2(raw.l1[idx[j]] - raw.l1[idx[i]]);
+            if (d2 < thr2) {
+                idx[j] = idx[n - 1];
+                n--;
+                j--;
+            }
+        }
+    }
+    // warm start transfer (initContactsWithPreviousOnes, :1069-1160)
+    float4 fv1 = make_float4(0.f, 0.f, 0.f, 0.f), fv2 = fv1, fimp = fv1;
+    float4 oldl1[kMaxManifoldPoints];
+    float4 oldl2w[kMaxManifoldPoints];
+    if (oldN > 0) {
+        const float4 oldn0 = P.pn[4 * slot];
+        if (dot(v3(oldn0.x, oldn0.y, oldn0.z), raw.nrm[idx[0]]) >= A.cos_similar) {
+            fv1 = P.fv1[slot];
+            fv2 = P.fv2[slot];
+            fimp = P.fimp[slot];
+        }
+        for (int k = 0; k < oldN; k++) {
+            oldl1[k] = P.pl1[4 * slot + k];
+            oldl2w[k] = P.pl2[4 * slot + k];
+        }
+    }
+    const float pt2 = A.persistent_threshold * A.persistent_threshold;
+    for (int k = 0; k < n; k++) {
+        const int r = idx[k];
+        float imp = 0.0f, resting = 0.0f;
+        for (int o = 0; o < oldN; o++) {
+            const float d2 = len2(raw.l1[r] - xyz(oldl1[o]));
+            if (d2 <= pt2) {
+                imp = oldl1[o].w;
+                resting = oldl2w[o].w;
+                break;
+            }
+        }
+        P.pn[4 * slot + k] = f4(raw.nrm[r], raw.depth[r]);
+        P.pl1[4 * slot + k] = f4(raw.l1[r], imp);
+        P.pl2[4 * slot + k] = f4(raw.l2[r], resting);
+    }
+    P.fv1[slot] = fv1;
+    P.fv2[slot] = fv2;
+    P.fimp[slot] = fimp;
+    P.ncp[slot] = n;
+}
+
+// processPotentialContacts (:1210-1219): the test updates wasColliding / isValid of every tested pair.
+__device__ inline uint32_t lf_after_test(uint32_t lf, bool colliding) {
+    lf |= LF_VALID;
+    if (colliding) lf |= LF_WAS_COLLIDING;
+    else lf &= ~LF_WAS_COLLIDING;
+    return lf;
+}
+
+// ------------------------------------------------------------------------------------------------
+// Kernels
+// ------------------------------------------------------------------------------------------------
+
+// Middle phase: bucket the tested pairs by algorithm; a pair is tested only if at least one of its
+// bodies is enabled (CD.cpp:391-400).  Untested pairs have no contact pair this step.
+__global__ void np_classify(NpArgs A, int* lists, int list_stride, int* counts) {
+    const int np = *A.nb_pairs;
+    RP3DG_GRID_STRIDE(k, np) {
+        const int2 c = A.P.col[k];
+        const int f1 = A.B.flags[A.C.body[c.x]], f2 = A.B.flags[A.C.body[c.y]];
+        if ((f1 & BF_ENABLED) || (f2 & BF_ENABLED)) {
+            const int a = A.P.algo[k];
+            const int pos = atomicAdd(&counts[a], 1);
+            lists[a * list_stride + pos] = k;
+        } else {
+            A.P.ncp[k] = 0;
+        }
+    }
+}
+
+__global__ void np_sphere_sphere(NpArgs A, const int* list, const int* count) {
+    RP3DG_GRID_STRIDE(t, *count) {
+        const int k = list[t];
+        const int2 c = A.P.col[k];
+        const Tf T1 = tf(ld3(A.C.wpos, c.x), ldq(A.C.wq, c.x));
+        const Tf T2 = tf(ld3(A.C.wpos, c.y), ldq(A.C.wq, c.y));
+        Raw raw; raw.n = 0;
+        const bool hit = sphere_sphere(A.C.shape[c.x].x, A.C.shape[c.y].x, T1, T2, raw);
+        A.P.lf[k] = lf_after_test(A.P.lf[k], hit);
+        contact_epilogue(A, k, hit, raw, c.x);
+    }
+}
+
+__global__ void np_sphere_capsule(NpArgs A, const int* list, const int* count) {
+    RP3DG_GRID_STRIDE(t, *count) {
+        const int k = list[t];
+        const int2 c = A.P.col[k];
+        const bool s1 = A.C.type[c.x] == SHAPE_SPHERE;
+        const int cs = s1 ? c.x : c.y, cc = s1 ? c.y : c.x;
+        const Tf Ts = tf(ld3(A.C.wpos, cs), ldq(A.C.wq, cs));
+        const Tf Tc = tf(ld3(A.C.wpos, cc), ldq(A.C.wq, cc));
+        const float4 cap = A.C.shape[cc];
+        Raw raw; raw.n = 0;
+        const bool hit = sphere_capsule(s1, A.C.shape[cs].x, cap.x, cap.y + cap.y, Ts, Tc, raw);
+        A.P.lf[k] = lf_after_test(A.P.lf[k], hit);
+        contact_epilogue(A, k, hit, raw, c.x);
+    }
+}
+
+__global__ void np_capsule_capsule(NpArgs A, const int* list, const int* count) {
+    RP3DG_GRID_STRIDE(t, *count) {
+        const int k = list[t];
+        const int2 c = A.P.col[k];
+        const Tf T1 = tf(ld3(A.C.wpos, c.x), ldq(A.C.wq, c.x));
+        const Tf T2 = tf(ld3(A.C.wpos, c.y), ldq(A.C.wq, c.y));
+        const float4 s1 = A.C.shape[c.x], s2 = A.C.shape[c.y];
+        Raw raw; raw.n = 0;
+        const bool hit = capsule_capsule(s1.x, s1.y + s1.y, s2.x, s2.y + s2.y, T1, T2, raw);
+        A.P.lf[k] = lf_after_test(A.P.lf[k], hit);
+        contact_epilogue(A, k, hit, raw, c.x);
+    }
+}
+
+__global__ void np_poly_poly(NpArgs A, const int* list, const int* count) {
+    RP3DG_GRID_STRIDE(t, *count) {
+        const int k = list[t];
+        const int2 c = A.P.col[k];
+        const Tf T1 = tf(ld3(A.C.wpos, c.x), ldq(A.C.wq, c.x));
+        const Tf T2 = tf(ld3(A.C.wpos, c.y), ldq(A.C.wq, c.y));
+        const Poly P1 = make_poly(A, c.x);
+        const Poly P2 = make_poly(A, c.y);
+        Raw raw; raw.n = 0;
+        uint32_t lf = A.P.lf[k];
+        const bool hit = sat_poly_poly(P1, P2, T1, T2, lf, raw);
+        // ConvexPolyhedronVsConvexPolyhedronAlgorithm.cpp:55-62
+        lf |= LF_WAS_SAT;
+        lf &= ~LF_WAS_GJK;
+        A.P.lf[k] = lf_after_test(lf, hit);
+        contact_epilogue(A, k, hit, raw, c.x);
+    }
+}
+
+void upload_box_topology(const int* edges4, const int* face_edge, const int* face_verts) {
+    hipMemcpyToSymbol(HIP_SYMBOL(c_box_edges), edges4, sizeof(int4) * 24);
+    hipMemcpyToSymbol(HIP_SYMBOL(c_box_face_edge), face_edge, sizeof(int) * 6);
+    hipMemcpyToSymbol(HIP_SYMBOL(c_box_face_verts), face_verts, sizeof(int) * 24);
+}
+
+}  // namespace rp3dg
+
+namespace rp3dg {
+__global__ void np_sphere_poly(NpArgs A, const int* list, const int* count) {
+    RP3DG_GRID_STRIDE(t, *count) { A.P.ncp[list[t]] = 0; }
+}
+__global__ void np_capsule_poly(NpArgs A, const int* list, const int* count) {
+    RP3DG_GRID_STRIDE(t, *count) { A.P.ncp[list[t]] = 0; }
+}
+}  // namespace rp3dg

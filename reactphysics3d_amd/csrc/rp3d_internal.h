@@ -1,0 +1,162 @@
+// Internal host/device layout of the MI355X step path.
+//
+// HBM layout (all SoA, one entry per body / collider / pair / manifold of the
+// whole batch; worlds are concatenated and every record carries its world id):
+//   bodies     : float4 streams (x = centre of mass, q, v, w, ...) so each
+//                per-body kernel reads/writes 16-byte aligned vectors.
+//   colliders  : shape, local/world transform, fat AABB, material, filter.
+//   pairs      : persistent overlapping-pair table sorted by (lo, hi) collider
+//                index, double buffered across steps.  Each pair carries the
+//                SAT/GJK temporal-coherence record and its contact cache (one
+//                manifold of <= 4 points, the only case for convex pairs,
+//                CollisionDetectionSystem.cpp:1247-1290).
+//   manifolds  : per-step compacted, colour-sorted list of colliding pairs
+//                with the solver rows (ContactSolverSystem.h:133-260).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include <string>
+#include <vector>
+
+#include "rp3d_math.h"
+
+namespace rp3dg {
+
+enum ShapeType : int { SHAPE_SPHERE = 0, SHAPE_CAPSULE = 1, SHAPE_BOX = 2, SHAPE_HULL = 3 };
+
+// NarrowPhaseAlgorithmType (include/reactphysics3d/collision/narrowphase/CollisionDispatch.h)
+enum Algo : int {
+    ALGO_NONE = 0,
+    ALGO_SPHERE_SPHERE = 1,
+    ALGO_SPHERE_CAPSULE = 2,
+    ALGO_CAPSULE_CAPSULE = 3,
+    ALGO_SPHERE_POLY = 4,
+    ALGO_CAPSULE_POLY = 5,
+    ALGO_POLY_POLY = 6,
+    ALGO_COUNT = 7
+};
+
+// Body flags
+constexpr int BF_DYNAMIC = 1;   // BodyType::DYNAMIC
+constexpr int BF_ENABLED = 2;   // in the enabled partition (not static, not sleeping)
+constexpr int BF_GRAVITY = 4;   // gravity enabled
+constexpr int BF_KINEMATIC = 8;
+
+// LastFrameCollisionInfo (include/reactphysics3d/engine/OverlappingPairs.h:55-93) packed in a word:
+constexpr uint32_t LF_VALID = 1u << 0;
+constexpr uint32_t LF_WAS_COLLIDING = 1u << 1;
+constexpr uint32_t LF_WAS_GJK = 1u << 2;
+constexpr uint32_t LF_WAS_SAT = 1u << 3;
+constexpr uint32_t LF_SAT_FACE1 = 1u << 4;
+constexpr uint32_t LF_SAT_FACE2 = 1u << 5;
+// bits 8..15 satMinAxisFaceIndex, 16..23 satMinEdge1Index, 24..31 satMinEdge2Index
+__host__ __device__ inline uint32_t lf_face(uint32_t w) { return (w >> 8) & 0xffu; }
+__host__ __device__ inline uint32_t lf_edge1(uint32_t w) { return (w >> 16) & 0xffu; }
+__host__ __device__ inline uint32_t lf_edge2(uint32_t w) { return (w >> 24) & 0xffu; }
+
+constexpr int kMaxRawContacts = 16;   // NB_MAX_CONTACT_POINTS_IN_NARROWPHASE_INFO (configuration.h:134)
+constexpr int kMaxManifoldPoints = 4; // MAX_CONTACT_POINTS_IN_MANIFOLD
+
+// Convex hull arena (ConvexMeshShape + HalfEdgeStructure), one record per hull.
+struct HullHeader {
+    int32_t vbase, nv;     // vertices (float4 xyz) [vbase, vbase+nv)
+    int32_t fbase, nf;     // faces: normal (float4) + first half-edge + vertex list
+    int32_t ebase, ne;     // half-edges (int4: vertex, twin, face, next)
+    int32_t fvbase, nfv;   // concatenated face vertex indices
+    float cx, cy, cz, pad; // centroid
+};
+
+struct DevBodies {
+    float4 *x, *q, *p;          // centre of mass (world), orientation, body origin
+    float4 *v, *w;              // velocities (RigidBodyComponents::mLinear/AngularVelocities)
+    float4 *cv, *cw;            // constrained velocities
+    float4 *sv, *sw;            // split velocities
+    float4 *cx, *cq;            // constrained position / orientation
+    float4 *force, *torque;     // external force / torque
+    float4 *ii;                 // inverse local inertia xyz, w = inverse mass
+    float4 *iw0, *iw1, *iw2;    // world inverse inertia rows
+    float4 *com;                // local centre of mass xyz, w = mass
+    float4 *llock, *alock;      // lock factors xyz; llock.w = linear damping, alock.w = angular damping
+    int *flags;
+    int *world;
+    int n;
+};
+
+struct DevColliders {
+    int *body;
+    int *type;
+    float4 *shape;             // sphere r | capsule r, halfHeight | box hx,hy,hz | hull: x = hull index
+    float4 *lpos, *lq;         // local-to-body transform
+    float4 *wpos, *wq;         // local-to-world transform
+    float4 *amin, *amax;       // fat AABB (broadphase tree node AABB)
+    float2 *mat;               // sqrt(friction), bounciness
+    uint32_t *filter;          // category | mask << 16
+    int *moved;                // fat AABB recomputed by the last refit (BroadPhaseSystem moved set)
+    int *rank;                 // position of the collider in the moved-set iteration order
+    int *world;
+    int n;
+};
+
+struct DevPairs {
+    uint64_t *key;             // (lo << 32) | hi, collider indices
+    int2 *col;                 // collider1, collider2 as the reference orients them
+    int *algo;
+    uint32_t *lf;              // LastFrameCollisionInfo flags + SAT indices
+    float4 *lfaxis;            // gjkSeparatingAxis
+    int *ncp;                  // contact points in the cache (0 = no contact pair)
+    float4 *fv1, *fv2;         // manifold friction vectors
+    float4 *fimp;              // friction1, friction2, twist impulses
+    float4 *pn;                // [4*slot+k] normal xyz, depth
+    float4 *pl1;               // [4*slot+k] local point on shape 1, w = accumulated normal impulse
+    float4 *pl2;               // [4*slot+k] local point on shape 2, w = resting flag (0/1)
+};
+
+// Solver rows of one contact manifold (ContactManifoldSolver + ContactPointSolver), colour order.
+struct DevSolver {
+    int *pair;                 // pair slot of each manifold in solve order
+    int2 *bodies;              // rigid body indices
+    int *npts;
+    float4 *n;                 // manifold normal xyz, w = friction coefficient
+    float4 *r1f, *r2f;         // r1Friction, r2Friction; w: inverse masses body1 / body2
+    float4 *t1, *t2;           // friction vectors; w: inverse friction masses 1 / 2
+    float4 *r1ct1, *r1ct2, *r2ct1, *r2ct2;   // r x t; r1ct1.w = inverse twist mass
+    float4 *imp;               // friction1, friction2, twist impulses, w = sum of penetration impulses (scratch)
+    float4 *oldt1, *oldt2;     // previous friction vectors
+    // per point [4*m + k]
+    float4 *pn;                // normal xyz, penetration depth
+    float4 *r1, *r2;           // w: inverse penetration mass | restitution bias
+    float4 *i1n, *i2n;         // I1 (r1 x n), I2 (r2 x n); w: resting flag | unused
+    float4 *lam;               // penetration impulse, split impulse
+};
+
+struct DevJoints;  // joint solver rows (joints.hip)
+
+struct Settings {
+    V3 gravity;
+    float persistent_threshold;
+    float restitution_threshold;
+    float cos_similar;
+    int vel_iters, pos_iters;
+    int gravity_enabled;
+};
+
+// Counters written by kernels (one device block, read back after each step).
+struct DevCounters {
+    int nb_pairs;              // pairs in the current table
+    int nb_overlaps;
+    int nb_tests[ALGO_COUNT];
+    int nb_manifolds;
+    int nb_points;
+    int nb_colors;
+    int status;
+    int color_rounds;
+    int nb_uncolored;
+    int pad[6];
+};
+
+constexpr int STATUS_PAIR_OVERFLOW = 1;
+constexpr int STATUS_COLOR_OVERFLOW = 2;
+constexpr int STATUS_STACK_OVERFLOW = 4;
+
+}  // namespace rp3dg

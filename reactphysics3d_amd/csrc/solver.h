@@ -1,0 +1,41 @@
+// Kernel argument block and launch declarations of the solver / integration kernels.
+#pragma once
+
+#include "rp3d_internal.h"
+
+namespace rp3dg {
+
+constexpr int kMaxColors = 64;
+
+struct SvArgs {
+    DevBodies B;
+    DevColliders C;
+    DevPairs P;
+    DevSolver S;
+    const int* nm;        // number of manifolds
+    const int* mlist;     // manifold -> pair slot (pair order)
+    const int* order;     // colour-sorted position -> manifold
+    const int* cstart;    // colour ranges [cstart[c], cstart[c+1])
+    const int* ncolors;
+    float restitution_threshold;
+};
+
+__global__ void dyn_integrate_velocities(DevBodies B, Settings S, float dt);
+__global__ void dyn_integrate_positions(DevBodies B, float dt);
+__global__ void dyn_update_state(DevBodies B);
+__global__ void sv_flag_manifolds(const int* ncp, const int* nb_pairs, int* flag);
+__global__ void sv_compact_manifolds(const int* flag, const int* offs, const int* nb_pairs, const int* ncp, int* list,
+                                     int* counters_manifolds, int* counters_points);
+__global__ void sv_adj_count(DevBodies B, DevColliders C, DevPairs P, const int* list, const int* nm_ptr, int* cnt);
+__global__ void sv_adj_fill(DevBodies B, DevColliders C, DevPairs P, const int* list, const int* nm_ptr, const int* offs,
+                            int* cursor, int* adj, int2* mb);
+__global__ void sv_color_round(const int2* mb, const int* nm_ptr, const int* offs, const int* adj, int* color,
+                               int* remaining, int* status);
+__global__ void sv_color_keys(const int* color, const int* nm_ptr, uint64_t* keys, int* vals);
+__global__ void sv_color_ranges(const uint64_t* keys, const int* nm_ptr, int* start, int* ncolors);
+__global__ void sv_prepare(SvArgs A);
+__global__ void sv_warmstart(SvArgs A, int color);
+__global__ void sv_solve_color(SvArgs A, int color, float beta_dt);
+__global__ void sv_store(SvArgs A);
+
+}  // namespace rp3dg

@@ -1,0 +1,86 @@
+// Host-side world model behind the C ABI (the part of PhysicsWorld / RigidBody /
+// Collider that lives on the CPU: object creation, materials, mass properties,
+// hull topology).  Nothing here steps the simulation; rp3dg_world_update()
+// uploads the world to the device and the HIP kernels do the step.
+#pragma once
+
+#include <map>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "../../include/rp3d_mi355x.h"
+#include "rp3d_internal.h"
+
+namespace rp3dg {
+
+struct HalfEdgeMesh {
+    // HalfEdgeStructure (include/reactphysics3d/collision/HalfEdgeStructure.h:46-72)
+    struct Edge { int vertex, twin, face, next; };
+    std::vector<std::vector<int>> faces;   // vertex indices per face
+    std::vector<int> face_edge;            // Face::edgeIndex
+    std::vector<int> vertex_edge;          // Vertex::edgeIndex
+    std::vector<Edge> edges;
+    bool build(int nb_vertices, std::string* err);
+};
+
+struct HostHull {
+    std::vector<V3> vertices;
+    std::vector<V3> face_normals;
+    HalfEdgeMesh mesh;
+    V3 centroid;
+    V3 local_min, local_max;
+    float volume;
+};
+
+struct HostBody {
+    Tf transform;               // body origin + orientation (TransformComponents)
+    V3 com_world, com_local;
+    V3 v, w;
+    V3 force, torque;
+    V3 llock{1, 1, 1}, alock{1, 1, 1};
+    float ldamp = 0.f, adamp = 0.f;
+    float mass = 1.f, inv_mass = 1.f;         // RigidBodyComponents.cpp:225-226
+    V3 inertia{1, 1, 1}, inv_inertia{1, 1, 1};
+    int type = 2;                              // DYNAMIC
+    bool gravity = true;
+    std::vector<int> colliders;
+};
+
+struct HostCollider {
+    int body;
+    int type;
+    float params[4];
+    Tf local;
+    float friction_sqrt, bounciness, density;
+    uint16_t category = 0x0001, mask = 0xFFFF;
+};
+
+struct HostJoint {
+    int type;
+    int body1, body2;
+    V3 anchor_world, axis_world;
+    bool collide_connected;
+    int flags;
+    float limits[4];
+};
+
+struct HostWorld {
+    rp3dg_world_settings cfg;
+    std::vector<HostBody> bodies;
+    std::vector<HostCollider> colliders;
+    std::vector<HostHull> hulls;
+    std::vector<HostJoint> joints;
+    rp3dg_batch* own = nullptr;   // device batch of one, created on first update()
+    bool uploaded = false;
+};
+
+// Shape formulas (SphereShape.h, CapsuleShape.{h,cpp}, BoxShape.{h,cpp}, ConvexMeshShape.cpp)
+float shape_volume(const HostWorld& w, const HostCollider& c);
+V3 shape_local_inertia(const HostWorld& w, const HostCollider& c, float mass);
+
+void set_error(const std::string& msg);
+
+}  // namespace rp3dg
+
+struct rp3dg_world : rp3dg::HostWorld {};

@@ -1,0 +1,268 @@
+"""ctypes binding of the C ABI (include/rp3d_mi355x.h).
+
+The HIP library must be built (``python -m reactphysics3d_amd.build``); loading
+fails loudly otherwise -- there is no CPU fallback for the step path.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+from . import scene as S
+
+LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "_lib", "librp3d_mi355x.so")
+
+_lib = None
+
+
+class WorldSettings(C.Structure):
+    _fields_ = [
+        ("gravity", C.c_float * 3),
+        ("persistent_contact_distance_threshold", C.c_float),
+        ("default_friction_coefficient", C.c_float),
+        ("default_bounciness", C.c_float),
+        ("restitution_velocity_threshold", C.c_float),
+        ("cos_angle_similar_contact_manifold", C.c_float),
+        ("default_velocity_solver_iterations", C.c_uint32),
+        ("default_position_solver_iterations", C.c_uint32),
+        ("is_sleeping_enabled", C.c_int32),
+        ("is_gravity_enabled", C.c_int32),
+    ]
+
+
+class Counts(C.Structure):
+    _fields_ = [(n, C.c_int64) for n in ("nb_bodies", "nb_colliders", "nb_joints", "nb_worlds", "nb_pairs",
+                                         "nb_manifolds", "nb_contact_points", "nb_colors", "status")]
+
+
+class ContactView(C.Structure):
+    _fields_ = [("colliders", C.POINTER(C.c_int32)), ("nb_points", C.POINTER(C.c_int32)),
+                ("color", C.POINTER(C.c_int32)), ("normal_depth", C.POINTER(C.c_float)),
+                ("local1", C.POINTER(C.c_float)), ("local2", C.POINTER(C.c_float)),
+                ("impulse", C.POINTER(C.c_float)), ("friction", C.POINTER(C.c_float))]
+
+
+# Every symbol the header declares (the not-GPU test checks that each is exported).
+EXPORTED = [
+    "rp3dg_world_settings_default", "rp3dg_world_create", "rp3dg_world_destroy", "rp3dg_world_set_iterations",
+    "rp3dg_world_set_gravity", "rp3dg_body_create", "rp3dg_body_set_type", "rp3dg_body_set_velocity",
+    "rp3dg_body_set_damping", "rp3dg_body_enable_gravity", "rp3dg_body_apply_force_torque",
+    "rp3dg_body_set_lock_factors", "rp3dg_body_update_mass_properties", "rp3dg_body_set_mass_properties",
+    "rp3dg_collider_add", "rp3dg_collider_set_material", "rp3dg_collider_set_filter", "rp3dg_hull_create",
+    "rp3dg_joint_create", "rp3dg_world_update", "rp3dg_batch_create", "rp3dg_batch_destroy", "rp3dg_batch_step",
+    "rp3dg_batch_step_graph", "rp3dg_batch_synchronize", "rp3dg_batch_get_counts", "rp3dg_batch_get_bodies",
+    "rp3dg_batch_set_bodies", "rp3dg_batch_get_pairs", "rp3dg_batch_get_contacts", "rp3dg_batch_get_joint_order",
+    "rp3dg_batch_profile", "rp3dg_batch_get_kernel_times", "rp3dg_batch_set_residual_mode",
+    "rp3dg_batch_get_iteration_stats", "rp3dg_last_error", "rp3dg_version",
+]
+
+
+def _fp(a):
+    return a.ctypes.data_as(C.POINTER(C.c_float))
+
+
+def _ip(a):
+    return a.ctypes.data_as(C.POINTER(C.c_int32))
+
+
+def load():
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError(f"HIP step library not built: {LIB_PATH} missing "
+                           "(run `python -m reactphysics3d_amd.build`)")
+    L = C.CDLL(LIB_PATH)
+    vp, f, i32, i64, u32 = C.c_void_p, C.c_float, C.c_int32, C.c_int64, C.c_uint32
+    fptr, iptr = C.POINTER(C.c_float), C.POINTER(C.c_int32)
+    sig = {
+        "rp3dg_world_settings_default": (None, [C.POINTER(WorldSettings)]),
+        "rp3dg_world_create": (vp, [C.POINTER(WorldSettings)]),
+        "rp3dg_world_destroy": (None, [vp]),
+        "rp3dg_world_set_iterations": (C.c_int, [vp, u32, u32]),
+        "rp3dg_world_set_gravity": (C.c_int, [vp, fptr]),
+        "rp3dg_body_create": (i32, [vp, fptr, fptr]),
+        "rp3dg_body_set_type": (C.c_int, [vp, i32, i32]),
+        "rp3dg_body_set_velocity": (C.c_int, [vp, i32, fptr, fptr]),
+        "rp3dg_body_set_damping": (C.c_int, [vp, i32, f, f]),
+        "rp3dg_body_enable_gravity": (C.c_int, [vp, i32, i32]),
+        "rp3dg_body_apply_force_torque": (C.c_int, [vp, i32, fptr, fptr]),
+        "rp3dg_body_set_lock_factors": (C.c_int, [vp, i32, fptr, fptr]),
+        "rp3dg_body_update_mass_properties": (C.c_int, [vp, i32]),
+        "rp3dg_body_set_mass_properties": (C.c_int, [vp, i32, f, fptr, fptr]),
+        "rp3dg_collider_add": (i32, [vp, i32, i32, fptr, fptr, fptr]),
+        "rp3dg_collider_set_material": (C.c_int, [vp, i32, f, f, f]),
+        "rp3dg_collider_set_filter": (C.c_int, [vp, i32, C.c_uint16, C.c_uint16]),
+        "rp3dg_hull_create": (i32, [vp, fptr, i32, iptr, iptr, i32]),
+        "rp3dg_joint_create": (i32, [vp, i32, i32, i32, fptr, fptr, i32, i32, fptr]),
+        "rp3dg_world_update": (C.c_int, [vp, f]),
+        "rp3dg_batch_create": (vp, [C.POINTER(vp), i32]),
+        "rp3dg_batch_destroy": (None, [vp]),
+        "rp3dg_batch_step": (C.c_int, [vp, f, i32]),
+        "rp3dg_batch_step_graph": (C.c_int, [vp, f, i32]),
+        "rp3dg_batch_synchronize": (C.c_int, [vp]),
+        "rp3dg_batch_get_counts": (C.c_int, [vp, C.POINTER(Counts)]),
+        "rp3dg_batch_get_bodies": (C.c_int, [vp, fptr, fptr, fptr, fptr]),
+        "rp3dg_batch_set_bodies": (C.c_int, [vp, fptr, fptr, fptr, fptr]),
+        "rp3dg_batch_get_pairs": (C.c_int, [vp, iptr, iptr, i64]),
+        "rp3dg_batch_get_contacts": (C.c_int, [vp, C.POINTER(ContactView), i64]),
+        "rp3dg_batch_get_joint_order": (C.c_int, [vp, iptr, i64]),
+        "rp3dg_batch_profile": (C.c_int, [vp, i32]),
+        "rp3dg_batch_get_kernel_times": (C.c_int, [vp, C.c_char_p, C.POINTER(C.c_double), C.POINTER(i64)]),
+        "rp3dg_batch_set_residual_mode": (C.c_int, [vp, f, i32]),
+        "rp3dg_batch_get_iteration_stats": (C.c_int, [vp, C.POINTER(C.c_double), C.POINTER(C.c_double)]),
+        "rp3dg_last_error": (C.c_char_p, []),
+        "rp3dg_version": (C.c_char_p, []),
+    }
+    for name, (res, args) in sig.items():
+        fn = getattr(L, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = L
+    return L
+
+
+def last_error() -> str:
+    return load().rp3dg_last_error().decode()
+
+
+def _check(rc, what):
+    if rc < 0:
+        raise RuntimeError(f"{what} failed ({rc}): {last_error()}")
+    return rc
+
+
+def _f3(v):
+    return (C.c_float * 3)(*[float(x) for x in v])
+
+
+def _f4(v):
+    return (C.c_float * 4)(*[float(x) for x in v])
+
+
+def world_from_scene(sc: S.Scene):
+    """Creates a host world through the C ABI with the canonical creation sequence."""
+    L = load()
+    st = WorldSettings()
+    L.rp3dg_world_settings_default(C.byref(st))
+    s = sc.settings
+    for k in range(3):
+        st.gravity[k] = float(s["gravity"][k])
+    st.persistent_contact_distance_threshold = float(s["persistent"])
+    st.default_friction_coefficient = float(s["default_friction"])
+    st.default_bounciness = float(s["default_bounciness"])
+    st.restitution_velocity_threshold = float(s["restitution_threshold"])
+    st.cos_angle_similar_contact_manifold = float(s["cos_similar"])
+    st.default_velocity_solver_iterations = int(s["vel_iters"])
+    st.default_position_solver_iterations = int(s["pos_iters"])
+    st.is_sleeping_enabled = int(s["sleeping"])
+    w = L.rp3dg_world_create(C.byref(st))
+    ci = 0
+    cols = sc.colliders
+    for bi, (bt, pos, q, lin, ang, grav, ld, ad) in enumerate(sc.bodies):
+        b = _check(L.rp3dg_body_create(w, _f3(pos), _f4(q)), "body_create")
+        _check(L.rp3dg_body_set_type(w, b, bt), "body_set_type")
+        while ci < len(cols) and cols[ci][0] == bi:
+            (_, shape, params, lpos, lq, fr, bo, de, cat, mask) = cols[ci]
+            c = _check(L.rp3dg_collider_add(w, b, shape, _f4(params), _f3(lpos), _f4(lq)), "collider_add")
+            _check(L.rp3dg_collider_set_material(w, c, fr, bo, de), "set_material")
+            if cat != 0x0001 or mask != 0xFFFF:
+                _check(L.rp3dg_collider_set_filter(w, c, cat, mask), "set_filter")
+            ci += 1
+        if bt == S.DYNAMIC:
+            _check(L.rp3dg_body_update_mass_properties(w, b), "update_mass_properties")
+        if bt != S.STATIC:
+            _check(L.rp3dg_body_set_velocity(w, b, _f3(lin), _f3(ang)), "set_velocity")
+        if not grav:
+            _check(L.rp3dg_body_enable_gravity(w, b, 0), "enable_gravity")
+        if ld != 0.0 or ad != 0.0:
+            _check(L.rp3dg_body_set_damping(w, b, ld, ad), "set_damping")
+    for (jt, b1, b2, anchor, axis, collide, flags, limits) in sc.joints:
+        _check(L.rp3dg_joint_create(w, jt, b1, b2, _f3(anchor), _f3(axis), collide, flags, _f4(limits)),
+               "joint_create")
+    return w
+
+
+class Batch:
+    """One or many worlds stepped together on the GPU."""
+
+    def __init__(self, scenes):
+        L = load()
+        if isinstance(scenes, S.Scene):
+            scenes = [scenes]
+        self.scenes = list(scenes)
+        self.worlds = [world_from_scene(sc) for sc in self.scenes]
+        arr = (C.c_void_p * len(self.worlds))(*self.worlds)
+        self.h = L.rp3dg_batch_create(arr, len(self.worlds))
+        if not self.h:
+            raise RuntimeError("rp3dg_batch_create failed: " + last_error())
+        self.nb = sum(sc.nb_bodies for sc in self.scenes)
+
+    def close(self):
+        L = load()
+        if self.h:
+            L.rp3dg_batch_destroy(self.h)
+            self.h = None
+        for w in self.worlds:
+            L.rp3dg_world_destroy(w)
+        self.worlds = []
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def step(self, dt, n=1):
+        _check(load().rp3dg_batch_step(self.h, float(dt), int(n)), "batch_step")
+
+    def step_graph(self, dt, n=1):
+        _check(load().rp3dg_batch_step_graph(self.h, float(dt), int(n)), "batch_step_graph")
+
+    def sync(self):
+        _check(load().rp3dg_batch_synchronize(self.h), "synchronize")
+
+    def counts(self) -> dict:
+        c = Counts()
+        _check(load().rp3dg_batch_get_counts(self.h, C.byref(c)), "get_counts")
+        return {n: getattr(c, n) for n, _ in Counts._fields_}
+
+    def bodies(self):
+        n = self.nb
+        p = np.zeros((n, 3), np.float32)
+        q = np.zeros((n, 4), np.float32)
+        v = np.zeros((n, 3), np.float32)
+        w = np.zeros((n, 3), np.float32)
+        _check(load().rp3dg_batch_get_bodies(self.h, _fp(p), _fp(q), _fp(v), _fp(w)), "get_bodies")
+        return p, q, v, w
+
+    def pairs(self):
+        cap = max(1, self.counts()["nb_pairs"])
+        c1 = np.zeros(cap, np.int32)
+        c2 = np.zeros(cap, np.int32)
+        n = _check(load().rp3dg_batch_get_pairs(self.h, _ip(c1), _ip(c2), cap), "get_pairs")
+        return c1[:n], c2[:n]
+
+    def contacts(self):
+        nm = max(1, self.counts()["nb_manifolds"])
+        out = {
+            "colliders": np.zeros((nm, 2), np.int32), "nb_points": np.zeros(nm, np.int32),
+            "color": np.zeros(nm, np.int32), "normal_depth": np.zeros((nm, 4, 4), np.float32),
+            "local1": np.zeros((nm, 4, 3), np.float32), "local2": np.zeros((nm, 4, 3), np.float32),
+            "impulse": np.zeros((nm, 4), np.float32), "friction": np.zeros((nm, 3), np.float32),
+        }
+        v = ContactView(_ip(out["colliders"]), _ip(out["nb_points"]), _ip(out["color"]), _fp(out["normal_depth"]),
+                        _fp(out["local1"]), _fp(out["local2"]), _fp(out["impulse"]), _fp(out["friction"]))
+        n = _check(load().rp3dg_batch_get_contacts(self.h, C.byref(v), nm), "get_contacts")
+        return {k: a[:n] for k, a in out.items()}
+
+    def profile(self, enable=True):
+        _check(load().rp3dg_batch_profile(self.h, int(enable)), "profile")
+
+    def kernel_time(self, name):
+        ms = C.c_double()
+        n = C.c_int64()
+        _check(load().rp3dg_batch_get_kernel_times(self.h, name.encode(), C.byref(ms), C.byref(n)), "kernel_times")
+        return ms.value, n.value

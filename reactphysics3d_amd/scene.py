@@ -1,0 +1,207 @@
+"""Seeded scene descriptions for the step path.
+
+A Scene is a flat description (structured numpy arrays) of one PhysicsWorld:
+settings, bodies, colliders (grouped by body, in creation order) and joints.
+The same description is loaded into the MI355X library through its C ABI
+(``reactphysics3d_amd.lib.DeviceWorld``) and, in tests, into the reference
+harness, always with the same creation sequence (body, setType, colliders,
+materials, mass properties, velocities), so both start from identical state.
+
+Random numbers come from a project-owned splitmix64 stream (no
+``std::*_distribution`` / numpy Generator behaviour is involved): floats are
+the top 24 bits of each 64-bit output, quaternions use Shoemake's method and
+normals Box-Muller.
+"""
+from __future__ import annotations
+
+import math
+from dataclasses import dataclass, field
+
+import numpy as np
+
+STATIC, KINEMATIC, DYNAMIC = 0, 1, 2
+SPHERE, CAPSULE, BOX, HULL = 0, 1, 2, 3
+BALLSOCKET, SLIDER, HINGE, FIXED = 0, 1, 2, 3
+
+SETTINGS_DTYPE = np.dtype([
+    ("gravity", "<f4", 3), ("vel_iters", "<i4"), ("pos_iters", "<i4"),
+    ("persistent", "<f4"), ("restitution_threshold", "<f4"), ("cos_similar", "<f4"),
+    ("default_friction", "<f4"), ("default_bounciness", "<f4"), ("sleeping", "<i4"),
+])
+BODY_DTYPE = np.dtype([
+    ("type", "<i4"), ("pos", "<f4", 3), ("quat", "<f4", 4), ("lin", "<f4", 3), ("ang", "<f4", 3),
+    ("gravity", "<i4"), ("ldamp", "<f4"), ("adamp", "<f4"),
+])
+COLLIDER_DTYPE = np.dtype([
+    ("body", "<i4"), ("shape", "<i4"), ("params", "<f4", 4), ("lpos", "<f4", 3), ("lquat", "<f4", 4),
+    ("friction", "<f4"), ("bounciness", "<f4"), ("density", "<f4"), ("category", "<i4"), ("mask", "<i4"),
+])
+JOINT_DTYPE = np.dtype([
+    ("type", "<i4"), ("body1", "<i4"), ("body2", "<i4"), ("anchor", "<f4", 3), ("axis", "<f4", 3),
+    ("collide", "<i4"), ("flags", "<i4"), ("limits", "<f4", 4),
+])
+
+
+class SplitMix64:
+    """splitmix64 (Steele, Lea, Flood 2014); floats from the top 24 bits."""
+
+    MASK = (1 << 64) - 1
+
+    def __init__(self, seed: int):
+        self.state = seed & self.MASK
+
+    def next_u64(self) -> int:
+        self.state = (self.state + 0x9E3779B97F4A7C15) & self.MASK
+        z = self.state
+        z = ((z ^ (z >> 30)) * 0xBF58476D1CE4E5B9) & self.MASK
+        z = ((z ^ (z >> 27)) * 0x94D049BB133111EB) & self.MASK
+        return z ^ (z >> 31)
+
+    def uniform01(self) -> float:
+        return (self.next_u64() >> 40) * (1.0 / 16777216.0)
+
+    def uniform(self, a: float, b: float) -> float:
+        return a + (b - a) * self.uniform01()
+
+    def normal(self) -> float:
+        u1 = max(self.uniform01(), 1.0 / 16777216.0)
+        u2 = self.uniform01()
+        return math.sqrt(-2.0 * math.log(u1)) * math.cos(2.0 * math.pi * u2)
+
+    def quaternion(self):
+        """Uniform random rotation (Shoemake 1992), returned as (x, y, z, w)."""
+        u1, u2, u3 = self.uniform01(), self.uniform01(), self.uniform01()
+        a, b = math.sqrt(1.0 - u1), math.sqrt(u1)
+        q = (a * math.sin(2 * math.pi * u2), a * math.cos(2 * math.pi * u2),
+             b * math.sin(2 * math.pi * u3), b * math.cos(2 * math.pi * u3))
+        n = math.sqrt(sum(c * c for c in q))
+        return tuple(c / n for c in q)
+
+
+@dataclass
+class Scene:
+    name: str
+    settings: np.ndarray
+    bodies: list = field(default_factory=list)
+    colliders: list = field(default_factory=list)
+    joints: list = field(default_factory=list)
+
+    # -- builders -------------------------------------------------------------------------------
+    def add_body(self, btype, pos, quat=(0.0, 0.0, 0.0, 1.0), lin=(0.0, 0.0, 0.0), ang=(0.0, 0.0, 0.0),
+                 gravity=True, ldamp=0.0, adamp=0.0):
+        self.bodies.append((btype, tuple(pos), tuple(quat), tuple(lin), tuple(ang), int(gravity), ldamp, adamp))
+        return len(self.bodies) - 1
+
+    def add_collider(self, body, shape, params, lpos=(0.0, 0.0, 0.0), lquat=(0.0, 0.0, 0.0, 1.0),
+                     friction=None, bounciness=None, density=1.0, category=0x0001, mask=0xFFFF):
+        if self.colliders and self.colliders[-1][0] > body:
+            raise ValueError("colliders must be added in body order")
+        p = list(params) + [0.0] * (4 - len(params))
+        f = float(self.settings["default_friction"]) if friction is None else friction
+        b = float(self.settings["default_bounciness"]) if bounciness is None else bounciness
+        self.colliders.append((body, shape, tuple(p), tuple(lpos), tuple(lquat), f, b, density, category, mask))
+        return len(self.colliders) - 1
+
+    def add_joint(self, jtype, body1, body2, anchor, axis=(0.0, 0.0, 1.0), collide=False, flags=0,
+                  limits=(0.0, 0.0, 0.0, 0.0)):
+        self.joints.append((jtype, body1, body2, tuple(anchor), tuple(axis), int(collide), flags, tuple(limits)))
+        return len(self.joints) - 1
+
+    # -- packed views ---------------------------------------------------------------------------
+    def body_array(self) -> np.ndarray:
+        return np.array(self.bodies, dtype=BODY_DTYPE)
+
+    def collider_array(self) -> np.ndarray:
+        return np.array(self.colliders, dtype=COLLIDER_DTYPE)
+
+    def joint_array(self) -> np.ndarray:
+        return np.array(self.joints, dtype=JOINT_DTYPE)
+
+    @property
+    def nb_bodies(self) -> int:
+        return len(self.bodies)
+
+    @property
+    def nb_dynamic(self) -> int:
+        return sum(1 for b in self.bodies if b[0] == DYNAMIC)
+
+
+def make_settings(gravity=(0.0, -9.81, 0.0), vel_iters=6, pos_iters=3, friction=0.3, bounciness=0.5,
+                  persistent=0.03, restitution_threshold=0.5, cos_similar=0.95, sleeping=0):
+    s = np.zeros((), dtype=SETTINGS_DTYPE)
+    s["gravity"] = gravity
+    s["vel_iters"], s["pos_iters"] = vel_iters, pos_iters
+    s["persistent"], s["restitution_threshold"], s["cos_similar"] = persistent, restitution_threshold, cos_similar
+    s["default_friction"], s["default_bounciness"], s["sleeping"] = friction, bounciness, sleeping
+    return s
+
+
+# ---------------------------------------------------------------------------------------------
+# Workloads
+# ---------------------------------------------------------------------------------------------
+
+def box_pile(n_side=10, levels=10, spacing=1.5, jitter=0.2, seed=1, friction=0.5, bounciness=0.0):
+    """1 m unit boxes (1 kg) on a grid dropped onto a static 100 x 100 x 1 m ground (gravity -z).
+    n_side = levels = 10 is the 1000-box pile; smaller grids are parity cases."""
+    rng = SplitMix64(seed)
+    sc = Scene(f"box_pile_{n_side * n_side * levels}",
+               make_settings(gravity=(0.0, 0.0, -9.81), vel_iters=10, pos_iters=5, friction=friction,
+                             bounciness=bounciness))
+    g = sc.add_body(STATIC, (0.0, 0.0, -0.5))
+    sc.add_collider(g, BOX, (50.0, 50.0, 0.5), friction=friction, bounciness=bounciness)
+    half = (n_side - 1) / 2.0
+    for k in range(levels):
+        for j in range(n_side):
+            for i in range(n_side):
+                pos = ((i - half) * spacing + rng.uniform(-jitter, jitter),
+                       (j - half) * spacing + rng.uniform(-jitter, jitter),
+                       1.0 + spacing * k + rng.uniform(-jitter, jitter))
+                b = sc.add_body(DYNAMIC, pos, rng.quaternion())
+                sc.add_collider(b, BOX, (0.5, 0.5, 0.5), friction=friction, bounciness=bounciness, density=1.0)
+    return sc
+
+
+def mixed_container(n_bodies=65536, extent=(200.0, 200.0, 100.0), seed=2, vel_iters=6, pos_iters=3):
+    """Spheres r~U[0.25,1], boxes with half-extents ~U[0.25,1], capsules r~U[0.2,0.5] h~U[0.5,2]
+    (one third each, interleaved), density 1000, uniform positions in the volume, random
+    orientations, v ~ N(0,1) per axis, inside a static floor + 4 walls of thickness 1 m."""
+    rng = SplitMix64(seed)
+    sc = Scene(f"mixed_container_{n_bodies}",
+               make_settings(gravity=(0.0, 0.0, -9.81), vel_iters=vel_iters, pos_iters=pos_iters))
+    ex, ey, ez = extent
+    hx, hy = ex / 2.0, ey / 2.0
+    walls = [((0.0, 0.0, -0.5), (hx + 1.0, hy + 1.0, 0.5)),
+             ((hx + 0.5, 0.0, ez / 2.0), (0.5, hy + 1.0, ez / 2.0)),
+             ((-hx - 0.5, 0.0, ez / 2.0), (0.5, hy + 1.0, ez / 2.0)),
+             ((0.0, hy + 0.5, ez / 2.0), (hx, 0.5, ez / 2.0)),
+             ((0.0, -hy - 0.5, ez / 2.0), (hx, 0.5, ez / 2.0))]
+    for pos, he in walls:
+        w = sc.add_body(STATIC, pos)
+        sc.add_collider(w, BOX, he)
+    for i in range(n_bodies):
+        pos = (rng.uniform(-hx, hx), rng.uniform(-hy, hy), rng.uniform(0.0, ez))
+        q = rng.quaternion()
+        kind = i % 3
+        if kind == 0:
+            params = (rng.uniform(0.25, 1.0),)
+            shape = SPHERE
+        elif kind == 1:
+            params = (rng.uniform(0.25, 1.0), rng.uniform(0.25, 1.0), rng.uniform(0.25, 1.0))
+            shape = BOX
+        else:
+            params = (rng.uniform(0.2, 0.5), rng.uniform(0.5, 2.0))
+            shape = CAPSULE
+        lin = (rng.normal(), rng.normal(), rng.normal())
+        b = sc.add_body(DYNAMIC, pos, q, lin=lin)
+        sc.add_collider(b, shape, params, density=1000.0)
+    return sc
+
+
+def two_body(shape1, params1, shape2, params2, pos1, pos2, q1=(0, 0, 0, 1), q2=(0, 0, 0, 1), static2=False):
+    """Two bodies, no gravity: the unit case of each narrowphase pair type."""
+    sc = Scene("two_body", make_settings(gravity=(0.0, 0.0, 0.0)))
+    a = sc.add_body(DYNAMIC, pos1, q1)
+    sc.add_collider(a, shape1, params1)
+    b = sc.add_body(STATIC if static2 else DYNAMIC, pos2, q2)
+    sc.add_collider(b, shape2, params2)
+    return sc
