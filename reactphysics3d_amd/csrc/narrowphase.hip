@@ -782,10 +782,545 @@ void upload_box_topology(const int* edges4, const int* face_edge, const int* fac
 }  // namespace rp3dg
 
 namespace rp3dg {
+
+// ------------------------------------------------------------------------------------------------
+// GJK on the core shapes with margins (GJKAlgorithm.cpp:50-238, VoronoiSimplex.cpp)
+// ------------------------------------------------------------------------------------------------
+
+struct Conv {
+    int type;        // SHAPE_SPHERE / SHAPE_CAPSULE / polyhedron
+    float margin;    // sphere/capsule radius, 0 for polyhedra (ConvexShape.h:62)
+    float hh;        // capsule half height
+    Poly P;
+    // SphereShape.h:149-153, CapsuleShape.h:191-206, BoxShape.h:167-172, ConvexMeshShape.cpp:80-102
+    __device__ V3 support(V3 d) const {
+        if (type == SHAPE_SPHERE) return v3(0.0f, 0.0f, 0.0f);
+        if (type == SHAPE_CAPSULE) {
+            const float top = hh * d.y;
+            const float bottom = -hh * d.y;
+            return top > bottom ? v3(0.0f, hh, 0.0f) : v3(0.0f, -hh, 0.0f);
+        }
+        return P.support(d);
+    }
+};
+
+__device__ inline Conv make_conv(const NpArgs& A, int collider) {
+    Conv c;
+    c.type = A.C.type[collider];
+    const float4 s = A.C.shape[collider];
+    c.hh = 0.0f;
+    c.margin = 0.0f;
+    c.P.hull = -1;
+    c.P.he = v3zero();
+    if (c.type == SHAPE_SPHERE) c.margin = s.x;
+    else if (c.type == SHAPE_CAPSULE) { c.margin = s.x; c.hh = s.y; }
+    else c.P = make_poly(A, collider);
+    return c;
+}
+
+constexpr float kSimplexEps = 0.0001f;   // VoronoiSimplex.cpp:33
+
+struct Simplex {
+    int n;
+    V3 pts[4], sa[4], sb[4];
+    V3 closest, csa, csb;
+    float bary[4];
+    bool recompute, valid;
+
+    __device__ void init() { n = 0; recompute = false; valid = false; }
+    __device__ bool full() const { return n == 4; }
+    __device__ void add(V3 p, V3 a, V3 b) { pts[n] = p; sa[n] = a; sb[n] = b; n++; recompute = true; }
+    __device__ void remove(int i) { n--; pts[i] = pts[n]; sa[i] = sa[n]; sb[i] = sb[n]; }
+    __device__ void reduce(int bits) {
+        if (n >= 4 && (bits & 8) == 0) remove(3);
+        if (n >= 3 && (bits & 4) == 0) remove(2);
+        if (n >= 2 && (bits & 2) == 0) remove(1);
+        if (n >= 1 && (bits & 1) == 0) remove(0);
+    }
+    __device__ bool contains(V3 p) const {
+        for (int i = 0; i < n; i++)
+            if (len2(pts[i] - p) <= kSimplexEps) return true;
+        return false;
+    }
+    __device__ bool affinely_dependent() const {
+        switch (n) {
+            case 2: return len2(pts[1] - pts[0]) <= kSimplexEps;
+            case 3: return len2(cross(pts[1] - pts[0], pts[2] - pts[0])) <= kSimplexEps;
+            case 4: return fabsf(dot(pts[1] - pts[0], cross(pts[2] - pts[0], pts[3] - pts[0]))) <= kSimplexEps;
+        }
+        return false;
+    }
+    __device__ float max_len2() const {
+        float m = 0.0f;
+        for (int i = 0; i < n; i++) {
+            const float l = len2(pts[i]);
+            if (l > m) m = l;
+        }
+        return m;
+    }
+    __device__ void set_bary(float a, float b, float c, float d) { bary[0] = a; bary[1] = b; bary[2] = c; bary[3] = d; }
+    __device__ bool bary_valid() const { return bary[0] >= 0.0f && bary[1] >= 0.0f && bary[2] >= 0.0f && bary[3] >= 0.0f; }
+
+    // VoronoiSimplex.cpp:296-331
+    __device__ static void closest_on_segment(V3 a, V3 b, int& bits, float& t) {
+        const V3 ap = -a;
+        const V3 ab = b - a;
+        const float apab = dot(ap, ab);
+        if (apab > 0.0f) {
+            const float l2 = len2(ab);
+            if (apab < l2) { t = apab / l2; bits = 3; }
+            else { t = 1.0f; bits = 2; }
+        } else { t = 0.0f; bits = 1; }
+    }
+    // VoronoiSimplex.cpp:339-427 (Ericson RTCD 5.1.5)
+    __device__ static void closest_on_triangle(V3 a, V3 b, V3 c, int& bits, V3& bc) {
+        const V3 ab = b - a, ac = c - a, ap = -a;
+        const float d1 = dot(ab, ap), d2 = dot(ac, ap);
+        if (d1 <= 0.0f && d2 <= 0.0f) { bc = v3(1.0f, 0, 0); bits = 1; return; }
+        const V3 bp = -b;
+        const float d3 = dot(ab, bp), d4 = dot(ac, bp);
+        if (d3 >= 0.0f && d4 <= d3) { bc = v3(0.0f, 1.0f, 0); bits = 2; return; }
+        const float vc = d1 * d4 - d3 * d2;
+        if (vc <= 0.0f && d1 >= 0.0f && d3 <= 0.0f) {
+            const float v = d1 / (d1 - d3);
+            bc = v3(1.0f - v, v, 0); bits = 3; return;
+        }
+        const V3 cp = -c;
+        const float d5 = dot(ab, cp), d6 = dot(ac, cp);
+        if (d6 >= 0.0f && d5 <= d6) { bc = v3(0.0f, 0.0f, 1.0f); bits = 4; return; }
+        const float vb = d5 * d2 - d1 * d6;
+        if (vb <= 0.0f && d2 >= 0.0f && d6 <= 0.0f) {
+            const float w = d2 / (d2 - d6);
+            bc = v3(1.0f - w, 0, w); bits = 5; return;
+        }
+        const float va = d3 * d6 - d5 * d4;
+        if (va <= 0.0f && (d4 - d3) >= 0.0f && (d5 - d6) >= 0.0f) {
+            const float w = (d4 - d3) / ((d4 - d3) + (d5 - d6));
+            bc = v3(0.0f, 1.0f - w, w); bits = 6; return;
+        }
+        const float denom = 1.0f / (va + vb + vc);
+        const float v = vb * denom;
+        const float w = vc * denom;
+        bc = v3(1 - v - w, v, w);
+        bits = 7;
+    }
+    // VoronoiSimplex.cpp:601-617
+    __device__ static int outside_plane(V3 a, V3 b, V3 c, V3 d) {
+        const V3 nrm = cross(b - a, c - a);
+        const float sp = dot(-a, nrm);
+        const float sd = dot(d - a, nrm);
+        if (sd * sd < kSimplexEps * kSimplexEps) return -1;
+        return sp * sd < 0.0f;
+    }
+    __device__ static int map_tri(int tb, int f, int s, int t) {
+        return (((1 & tb) != 0) << f) | (((2 & tb) != 0) << s) | (((4 & tb) != 0) << t);
+    }
+    // VoronoiSimplex.cpp:434-598
+    __device__ static bool closest_on_tetra(V3 a, V3 b, V3 c, V3 d, int& bits, float& ab0, float& ab1, float& cd0,
+                                            float& cd1, bool& degenerate) {
+        degenerate = false;
+        bits = 15;
+        ab0 = ab1 = cd0 = cd1 = 0.0f;
+        const int oABC = outside_plane(a, b, c, d);
+        const int oACD = outside_plane(a, c, d, b);
+        const int oADB = outside_plane(a, d, b, c);
+        const int oBDC = outside_plane(b, d, c, a);
+        if (oABC < 0 || oACD < 0 || oADB < 0 || oBDC < 0) { degenerate = true; return false; }
+        if (oABC == 0 && oACD == 0 && oADB == 0 && oBDC == 0) return true;
+        float best = kDecimalLargest;
+        int tb;
+        V3 bc;
+        if (oABC) {
+            closest_on_triangle(a, b, c, tb, bc);
+            const V3 cp = bc.x * a + bc.y * b + bc.z * c;
+            const float sq = len2(cp);
+            if (sq < best) { best = sq; ab0 = bc.x; ab1 = bc.y; cd0 = bc.z; cd1 = 0.0f; bits = tb; }
+        }
+        if (oACD) {
+            closest_on_triangle(a, c, d, tb, bc);
+            const V3 cp = bc.x * a + bc.y * c + bc.z * d;
+            const float sq = len2(cp);
+            if (sq < best) { best = sq; ab0 = bc.x; ab1 = 0.0f; cd0 = bc.y; cd1 = bc.z; bits = map_tri(tb, 0, 2, 3); }
+        }
+        if (oADB) {
+            closest_on_triangle(a, d, b, tb, bc);
+            const V3 cp = bc.x * a + bc.y * d + bc.z * b;
+            const float sq = len2(cp);
+            if (sq < best) { best = sq; ab0 = bc.x; ab1 = bc.z; cd0 = 0.0f; cd1 = bc.y; bits = map_tri(tb, 0, 3, 1); }
+        }
+        if (oBDC) {
+            closest_on_triangle(b, d, c, tb, bc);
+            const V3 cp = bc.x * b + bc.y * d + bc.z * c;
+            const float sq = len2(cp);
+            if (sq < best) { ab0 = 0.0f; ab1 = bc.x; cd0 = bc.z; cd1 = bc.y; bits = map_tri(tb, 1, 3, 2); }
+        }
+        return true;
+    }
+    // VoronoiSimplex.cpp:165-289
+    __device__ bool recompute_closest() {
+        if (recompute) {
+            recompute = false;
+            switch (n) {
+                case 0: valid = false; break;
+                case 1:
+                    closest = pts[0]; csa = sa[0]; csb = sb[0];
+                    set_bary(1, 0, 0, 0);
+                    valid = bary_valid();
+                    break;
+                case 2: {
+                    int bits = 0;
+                    float t;
+                    closest_on_segment(pts[0], pts[1], bits, t);
+                    csa = sa[0] + t * (sa[1] - sa[0]);
+                    csb = sb[0] + t * (sb[1] - sb[0]);
+                    closest = csa - csb;
+                    set_bary(1.0f - t, t, 0, 0);
+                    valid = bary_valid();
+                    reduce(bits);
+                } break;
+                case 3: {
+                    int bits = 0;
+                    V3 bc;
+                    closest_on_triangle(pts[0], pts[1], pts[2], bits, bc);
+                    csa = bc.x * sa[0] + bc.y * sa[1] + bc.z * sa[2];
+                    csb = bc.x * sb[0] + bc.y * sb[1] + bc.z * sb[2];
+                    closest = csa - csb;
+                    set_bary(bc.x, bc.y, bc.z, 0.0f);
+                    valid = bary_valid();
+                    reduce(bits);
+                } break;
+                case 4: {
+                    int bits = 0;
+                    float ab0, ab1, cd0, cd1;
+                    bool degenerate;
+                    const bool outside = closest_on_tetra(pts[0], pts[1], pts[2], pts[3], bits, ab0, ab1, cd0, cd1, degenerate);
+                    if (outside) {
+                        csa = ab0 * sa[0] + ab1 * sa[1] + cd0 * sa[2] + cd1 * sa[3];
+                        csb = ab0 * sb[0] + ab1 * sb[1] + cd0 * sb[2] + cd1 * sb[3];
+                        closest = csa - csb;
+                        set_bary(ab0, ab1, cd0, cd1);
+                        reduce(bits);
+                    } else {
+                        if (degenerate) valid = false;
+                        else {
+                            set_bary(0.0f, 0.0f, 0.0f, 0.0f);
+                            csa = csb = closest = v3zero();
+                            valid = true;
+                        }
+                        break;
+                    }
+                    valid = bary_valid();
+                } break;
+            }
+        }
+        return valid;
+    }
+};
+
+enum GjkResult { GJK_SEPARATED = 0, GJK_COLLIDE_IN_MARGIN = 1, GJK_INTERPENETRATE = 2 };
+
+// GJKAlgorithm::testCollision for one pair; lf_axis is LastFrameCollisionInfo::gjkSeparatingAxis.
+__device__ int gjk(const Conv& S1, const Conv& S2, const Tf& T1, const Tf& T2, uint32_t lf, float4& lf_axis, Raw& raw) {
+    const float relErrSq = 1.0e-3f * 1.0e-3f;   // REL_ERROR_SQUARE (GJKAlgorithm.h:46)
+    const Tf t1inv = tinv(T1);
+    const Tf b2to1 = tmul(t1inv, T2);
+    const Q4 rotTo2 = qmul(qconj(T2.q), T1.q);
+    const float margin = S1.margin + S2.margin;
+    const float marginSq = margin * margin;
+    Simplex sx;
+    sx.init();
+    V3 v;
+    if ((lf & LF_VALID) && (lf & LF_WAS_GJK)) v = xyz(lf_axis);
+    else v = v3(0.0f, 1.0f, 0.0f);
+    float distSq = kDecimalLargest;
+    bool contact = false;
+    V3 suppA, suppB, w;
+    int iters = 0;
+    do {
+        if (++iters > 64) { contact = true; break; }   // safety bound; the reference terminates far earlier
+        suppA = S1.support(-v);
+        suppB = tapply(b2to1, S2.support(qrot(rotTo2, v)));
+        w = suppA - suppB;
+        const float vw = dot(v, w);
+        if (vw > 0.0f && vw * vw > distSq * marginSq) {
+            lf_axis = f4(v, 0.f);
+            return GJK_SEPARATED;
+        }
+        if (sx.contains(w) || distSq - vw <= distSq * relErrSq) { contact = true; break; }
+        sx.add(w, suppA, suppB);
+        if (sx.affinely_dependent()) { contact = true; break; }
+        const bool ok = sx.recompute_closest();
+        v = sx.closest;
+        if (!ok) { contact = true; break; }
+        const float prev = distSq;
+        distSq = len2(v);
+        if (prev - distSq <= kMachineEps * prev) {
+            v = sx.closest;
+            distSq = len2(v);
+            contact = true;
+            break;
+        }
+    } while (!sx.full() && distSq > kMachineEps * sx.max_len2());
+    if (contact && distSq > kMachineEps) {
+        V3 pA = sx.csa, pB = sx.csb;
+        const float dist = sqrtf(distSq);
+        pA = pA - (S1.margin / dist) * v;
+        pB = tapply(tinv(b2to1), pB + (S2.margin / dist) * v);
+        const V3 normal = qrot(T1.q, -unit(v));
+        const float pen = margin - dist;
+        if (pen <= 0.0f) return GJK_SEPARATED;
+        if (len2(normal) < kMachineEps) return GJK_SEPARATED;
+        raw.add(normal, pen, pA, pB);
+        return GJK_COLLIDE_IN_MARGIN;
+    }
+    return GJK_INTERPENETRATE;
+}
+
+// SAT sphere vs polyhedron (SATAlgorithm.cpp:58-144)
+__device__ bool sat_sphere_poly(bool sphere1, float r, const Poly& P, const Tf& Ts, const Tf& Tp, Raw& raw) {
+    const Tf s2p = tmul(tinv(Tp), Ts);
+    const V3 c = s2p.p;
+    float minPen = kDecimalLargest;
+    int minFace = 0;
+    const int nf = P.nfaces();
+    for (int f = 0; f < nf; f++) {
+        const V3 fn = P.face_normal(f);
+        const float pen = dot(P.vertex(P.face_vertex(f, 0)) - c, fn) + r;
+        if (pen <= 0.0f) return false;
+        if (pen < minPen) { minPen = pen; minFace = f; }
+    }
+    const V3 mfn = P.face_normal(minFace);
+    const V3 mfnw = qrot(Tp.q, mfn);
+    const V3 cpS = qrot(tinv(Ts).q, (-mfnw) * r);
+    const V3 cpP = c + mfn * (minPen - r);
+    const V3 nw = sphere1 ? -mfnw : mfnw;
+    raw.add(nw, minPen, sphere1 ? cpS : cpP, sphere1 ? cpP : cpS);
+    return true;
+}
+
+// ConvexShape::getLocalSupportPointWithMargin (ConvexShape.cpp:40-56) for a capsule
+__device__ inline V3 capsule_support_margin(float r, float hh, V3 d) {
+    const float top = hh * d.y, bottom = -hh * d.y;
+    V3 s = top > bottom ? v3(0.0f, hh, 0.0f) : v3(0.0f, -hh, 0.0f);
+    V3 u = v3(0.0f, -1.0f, 0.0f);
+    if (len2(d) > kMachineEps * kMachineEps) u = unit(d);
+    s += u * r;
+    return s;
+}
+
+// clipSegmentWithPlanes (mathematics_functions.h:227-289); planes given by the reference face's edges.
+__device__ int clip_segment_face(const Poly& P, int face, V3 n, V3 a, V3 b, V3* out, V3& plane0) {
+    V3 in0 = a, in1 = b;
+    int nin = 2;
+    const int first = P.face_edge(face);
+    int e = first;
+    bool firstPlane = true;
+    int guard = P.nhalfedges();
+    do {
+        if (--guard < 0) break;
+        const int4 edge = P.edge(e);
+        const int4 twin = P.edge(edge.y);
+        const V3 ev1 = P.vertex(edge.x);
+        const V3 ev2 = P.vertex(twin.x);
+        const V3 pn = cross(n, ev2 - ev1);
+        const V3 pp = P.vertex(edge.x);
+        if (firstPlane) { plane0 = pp; firstPlane = false; }
+        if (nin == 0) return 0;
+        V3 o[2];
+        int no = 0;
+        const float d1 = dot(in0 - pp, pn);
+        const float d2 = dot(in1 - pp, pn);
+        if (d2 >= 0.0f) {
+            if (d1 < 0.0f) {
+                const float t = plane_segment_intersection(in0, in1, dot(pn, pp), pn);
+                if (t >= 0.0f && t <= 1.0f) o[no++] = in0 + t * (in1 - in0);
+                else o[no++] = in1;
+            } else {
+                o[no++] = in0;
+            }
+            o[no++] = in1;
+        } else {
+            if (d1 >= 0.0f) {
+                o[no++] = in0;
+                const float t = plane_segment_intersection(in0, in1, -dot(pn, pp), -pn);
+                if (t >= 0.0f && t <= 1.0f) o[no++] = in0 + t * (in1 - in0);
+            }
+        }
+        nin = no;
+        if (no > 0) in0 = o[0];
+        if (no > 1) in1 = o[1];
+        e = edge.w;
+    } while (e != first);
+    if (nin > 0) out[0] = in0;
+    if (nin > 1) out[1] = in1;
+    return nin;
+}
+
+// SATAlgorithm::computeCapsulePolyhedronFaceContactPoints (:381-470).  Returns contactFound; on the first
+// contact point the raw buffer is reset (NarrowPhaseInfoBatch::resetContactPoints).
+__device__ bool capsule_face_contacts(int face, float r, const Poly& P, float pen, const Tf& p2c, V3 normalWorld,
+                                      V3 sepAxisCap, V3 segA, V3 segB, bool capsule1, Raw& raw) {
+    const V3 fn = P.face_normal(face);
+    V3 clip[2];
+    V3 plane0 = v3zero();
+    const int nclip = clip_segment_face(P, face, fn, segA, segB, clip, plane0);
+    const V3 delta = fn * (pen - r);
+    bool found = false;
+    for (int i = 0; i < nclip; i++) {
+        const float cpd = dot(plane0 - clip[i], fn);
+        if (cpd > pen - r - 0.001f) {
+            if (!found) raw.n = 0;
+            found = true;
+            const V3 cpP = clip[i] + delta;
+            const V3 cpC = tapply(p2c, clip[i]) - sepAxisCap * r;
+            raw.add(normalWorld, pen, capsule1 ? cpC : cpP, capsule1 ? cpP : cpC);
+        }
+    }
+    return found;
+}
+
+// SATAlgorithm::testCollisionCapsuleVsConvexPolyhedron (:165-319)
+__device__ bool sat_capsule_poly(bool capsule1, float r, float hh, const Poly& P, const Tf& Tc, const Tf& Tp, Raw& raw) {
+    const Tf p2c = tmul(tinv(Tc), Tp);
+    const float height = hh + hh;
+    const V3 segA = v3(0.0f, -height * 0.5f, 0.0f);
+    const V3 segB = v3(0.0f, height * 0.5f, 0.0f);
+    const V3 segAxis = segB - segA;
+    float minPen = kDecimalLargest;
+    int minFace = 0;
+    bool isFace = false;
+    V3 sepAxis = v3zero(), sev1 = v3zero(), sev2 = v3zero();
+    const int nf = P.nfaces();
+    for (int f = 0; f < nf; f++) {
+        const V3 fn = P.face_normal(f);
+        const V3 fnc = qrot(p2c.q, fn);
+        const V3 sp = capsule_support_margin(r, hh, -fnc);
+        const V3 pf = tapply(p2c, P.vertex(P.face_vertex(f, 0)));
+        const float pen = dot(pf - sp, fnc);
+        if (pen <= 0.0f) return false;
+        if (pen < minPen) { minPen = pen; minFace = f; isFace = true; sepAxis = fnc; }
+    }
+    const int ne = P.nhalfedges();
+    for (int e = 0; e < ne; e += 2) {
+        const int4 edge = P.edge(e);
+        const V3 ev1 = P.vertex(edge.x);
+        const V3 ev2 = P.vertex(P.edge(edge.w).x);
+        const V3 edc = qrot(p2c.q, ev2 - ev1);
+        const int4 twin = P.edge(edge.y);
+        const V3 n1 = qrot(p2c.q, P.face_normal(edge.z));
+        const V3 n2 = qrot(p2c.q, P.face_normal(twin.z));
+        if (dot(segAxis, n1) * dot(segAxis, n2) < 0.0f) {   // isMinkowskiFaceCapsuleVsEdge (:474-482)
+            // computeEdgeVsCapsuleInnerSegmentPenetrationDepth (:323-352)
+            float pen = kDecimalLargest;
+            V3 axis = cross(segAxis, edc);
+            if (len2(axis) >= 0.00001f) {
+                const V3 pc = tapply(p2c, P.centroid());
+                const V3 pe = tapply(p2c, ev1);
+                if (dot(axis, pe - pc) < 0) axis = -axis;
+                axis = normalized(axis);
+                const V3 sp = capsule_support_margin(r, hh, -axis);
+                pen = dot(pe - sp, axis);
+            }
+            if (pen <= 0.0f) return false;
+            if (pen < minPen) { minPen = pen; isFace = false; sepAxis = axis; sev1 = ev1; sev2 = ev2; }
+        }
+    }
+    const Tf c2p = tinv(p2c);
+    const V3 segAP = tapply(c2p, segA);
+    const V3 segBP = tapply(c2p, segB);
+    V3 nw = qrot(Tc.q, sepAxis);
+    if (capsule1) nw = -nw;
+    if (isFace) return capsule_face_contacts(minFace, r, P, minPen, p2c, nw, sepAxis, segAP, segBP, capsule1, raw);
+    V3 cpPoly, cpSeg;
+    closest_points_between_segments(segAP, segBP, sev1, sev2, cpSeg, cpPoly);
+    const V3 cpCap = tapply(p2c, cpSeg) - sepAxis * r;
+    raw.add(nw, minPen, capsule1 ? cpCap : cpPoly, capsule1 ? cpPoly : cpCap);
+    return true;
+}
+
+// SphereVsConvexPolyhedronAlgorithm.cpp:38-103
 __global__ void np_sphere_poly(NpArgs A, const int* list, const int* count) {
-    RP3DG_GRID_STRIDE(t, *count) { A.P.ncp[list[t]] = 0; }
+    RP3DG_GRID_STRIDE(t, *count) {
+        const int k = list[t];
+        const int2 c = A.P.col[k];
+        const Tf T1 = tf(ld3(A.C.wpos, c.x), ldq(A.C.wq, c.x));
+        const Tf T2 = tf(ld3(A.C.wpos, c.y), ldq(A.C.wq, c.y));
+        const Conv S1 = make_conv(A, c.x), S2 = make_conv(A, c.y);
+        uint32_t lf = A.P.lf[k];
+        float4 axis = A.P.lfaxis[k];
+        Raw raw; raw.n = 0;
+        const int g = gjk(S1, S2, T1, T2, lf, axis, raw);
+        lf |= LF_WAS_GJK;
+        lf &= ~LF_WAS_SAT;
+        bool hit = false;
+        if (g == GJK_COLLIDE_IN_MARGIN) hit = true;
+        else if (g == GJK_INTERPENETRATE) {
+            const bool s1 = S1.type == SHAPE_SPHERE;
+            const Conv& Ss = s1 ? S1 : S2;
+            const Conv& Sp = s1 ? S2 : S1;
+            hit = sat_sphere_poly(s1, Ss.margin, Sp.P, s1 ? T1 : T2, s1 ? T2 : T1, raw);
+            lf &= ~LF_WAS_GJK;
+            lf |= LF_WAS_SAT;
+        }
+        A.P.lfaxis[k] = axis;
+        A.P.lf[k] = lf_after_test(lf, hit);
+        contact_epilogue(A, k, hit, raw, c.x);
+    }
 }
+
+// CapsuleVsConvexPolyhedronAlgorithm.cpp:42-177
 __global__ void np_capsule_poly(NpArgs A, const int* list, const int* count) {
-    RP3DG_GRID_STRIDE(t, *count) { A.P.ncp[list[t]] = 0; }
+    RP3DG_GRID_STRIDE(t, *count) {
+        const int k = list[t];
+        const int2 c = A.P.col[k];
+        const Tf T1 = tf(ld3(A.C.wpos, c.x), ldq(A.C.wq, c.x));
+        const Tf T2 = tf(ld3(A.C.wpos, c.y), ldq(A.C.wq, c.y));
+        const Conv S1 = make_conv(A, c.x), S2 = make_conv(A, c.y);
+        uint32_t lf = A.P.lf[k];
+        float4 axis = A.P.lfaxis[k];
+        Raw raw; raw.n = 0;
+        const int g = gjk(S1, S2, T1, T2, lf, axis, raw);
+        lf |= LF_WAS_GJK;
+        lf &= ~LF_WAS_SAT;
+        bool hit = false;
+        const bool cap1 = S1.type == SHAPE_CAPSULE;
+        const Conv& Sc = cap1 ? S1 : S2;
+        const Conv& Sp = cap1 ? S2 : S1;
+        const Tf& Tc = cap1 ? T1 : T2;
+        const Tf& Tp = cap1 ? T2 : T1;
+        if (g == GJK_COLLIDE_IN_MARGIN) {
+            const V3 cn = raw.nrm[0];
+            const float cpen = raw.depth[0];
+            const float height = Sc.hh + Sc.hh;
+            const int nf = Sp.P.nfaces();
+            for (int f = 0; f < nf; f++) {
+                const V3 fn = Sp.P.face_normal(f);
+                V3 fnw = qrot(Tp.q, fn);
+                const V3 sa = v3(0.0f, -height * 0.5f, 0.0f);
+                const V3 sb = v3(0.0f, height * 0.5f, 0.0f);
+                const V3 sdir = normalized(qrot(Tc.q, sb - sa));
+                const bool inDir = dot(fnw, cn) > 0.0f;
+                const bool inContactDir = (cap1 && !inDir) || (!cap1 && inDir);
+                if (inContactDir && fabsf(dot(fnw, sdir)) < 0.001f && parallel_vectors(fnw, cn)) {
+                    const Tf p2c = tmul(tinv(Tc), Tp);
+                    const Tf c2p = tinv(p2c);
+                    const V3 saP = tapply(c2p, sa);
+                    const V3 sbP = tapply(c2p, sb);
+                    const V3 sepCap = qrot(p2c.q, fn);
+                    if (cap1) fnw = -fnw;
+                    if (capsule_face_contacts(f, Sc.margin, Sp.P, cpen, p2c, fnw, sepCap, saP, sbP, cap1, raw)) break;
+                }
+            }
+            hit = true;
+        } else if (g == GJK_INTERPENETRATE) {
+            hit = sat_capsule_poly(cap1, Sc.margin, Sc.hh, Sp.P, Tc, Tp, raw);
+            lf &= ~LF_WAS_GJK;
+            lf |= LF_WAS_SAT;
+        }
+        A.P.lfaxis[k] = axis;
+        A.P.lf[k] = lf_after_test(lf, hit);
+        contact_epilogue(A, k, hit, raw, c.x);
+    }
 }
+
 }  // namespace rp3dg
