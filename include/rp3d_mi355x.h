@@ -147,6 +147,9 @@ typedef struct rp3dg_counts {
     int64_t nb_contact_points; /* contact points of the last step */
     int64_t nb_colors;         /* solver colours used by the last step */
     int64_t status;            /* device status word (0 = OK) */
+    int64_t nb_tests[7];       /* narrowphase tests of the last step per algorithm (index 1..6:
+                                  sphere-sphere, sphere-capsule, capsule-capsule, sphere-poly,
+                                  capsule-poly, poly-poly) */
 } rp3dg_counts;
 int rp3dg_batch_get_counts(rp3dg_batch* batch, rp3dg_counts* out);
 

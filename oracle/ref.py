@@ -48,6 +48,26 @@ def load(f64=False):
     return L
 
 
+def cpu_bench(sc, threads, warmup, steps, dt=1.0 / 60.0):
+    """Times the reference PhysicsWorld::update() on `threads` copies of the scene, one per thread.
+    Returns (seconds, contact points summed over the timed steps of all copies)."""
+    L = load()
+    L.rp3dref_bench.restype = C.c_int32
+    L.rp3dref_bench.argtypes = [C.c_void_p, C.c_int32, C.c_void_p, C.c_int32, C.c_void_p, C.c_int32, C.c_void_p,
+                                C.c_int32, C.c_int32, C.c_int32, C.c_float, C.POINTER(C.c_double)]
+    from reactphysics3d_amd.scene import JOINT_DTYPE
+    s = np.array(sc.settings)
+    b = sc.body_array()
+    c = sc.collider_array()
+    j = sc.joint_array() if sc.joints else np.zeros(1, dtype=JOINT_DTYPE)
+    out = (C.c_double * 2)()
+    rc = L.rp3dref_bench(s.ctypes.data, len(b), b.ctypes.data, len(c), c.ctypes.data, len(sc.joints), j.ctypes.data,
+                         threads, warmup, steps, float(dt), out)
+    if rc != 0:
+        raise RuntimeError("rp3dref_bench failed")
+    return out[0], out[1]
+
+
 def _fp(a):
     return a.ctypes.data_as(C.POINTER(C.c_float))
 

@@ -16,7 +16,10 @@
 #include <reactphysics3d/reactphysics3d.h>
 
 #include <algorithm>
+#include <atomic>
+#include <chrono>
 #include <cstring>
+#include <thread>
 #include <map>
 #include <unordered_map>
 #include <vector>
@@ -371,6 +374,45 @@ int rp3dref_get_contacts(void* h, int32_t* cols, int32_t* npts, float* nd, float
         }
     }
     return n;
+}
+
+// CPU baseline: `threads` identical reference worlds, one per thread, each stepped `warmup` untimed
+// then `steps` timed steps of PhysicsWorld::update(dt).  Worlds are created serially (the static
+// PhysicsWorld::mNbWorlds counter, src/engine/PhysicsWorld.cpp:46,108, is not atomic).  Wall time is
+// measured from a start barrier to the last join.  out[0] = seconds, out[1] = contact points summed
+// over the timed steps of all worlds.
+int rp3dref_bench(const RefSettings* s, int nb, const RefBody* bodies, int nc, const RefCollider* cols, int nj,
+                  const RefJoint* joints, int threads, int warmup, int steps, float dt, double* out) {
+    std::vector<RefWorld*> ws(threads, nullptr);
+    for (int t = 0; t < threads; t++) {
+        ws[t] = (RefWorld*)rp3dref_create(s, nb, bodies, nc, cols, nj, joints, nullptr, nullptr);
+        if (!ws[t]) return -1;
+        for (int k = 0; k < warmup; k++) ws[t]->world->update(dt);
+    }
+    std::atomic<int> ready(0);
+    std::atomic<bool> go(false);
+    std::vector<double> contacts(threads, 0.0);
+    std::vector<std::thread> pool;
+    for (int t = 0; t < threads; t++) {
+        pool.emplace_back([&, t]() {
+            ready.fetch_add(1);
+            while (!go.load()) std::this_thread::yield();
+            for (int k = 0; k < steps; k++) {
+                ws[t]->world->update(dt);
+                contacts[t] += (double)ws[t]->world->mCollisionDetection.mCurrentContactPoints->size();
+            }
+        });
+    }
+    while (ready.load() < threads) std::this_thread::yield();
+    const auto t0 = std::chrono::steady_clock::now();
+    go.store(true);
+    for (auto& th : pool) th.join();
+    const auto t1 = std::chrono::steady_clock::now();
+    out[0] = std::chrono::duration<double>(t1 - t0).count();
+    out[1] = 0.0;
+    for (double c : contacts) out[1] += c;
+    for (RefWorld* w : ws) rp3dref_destroy(w);
+    return 0;
 }
 
 }  // extern "C"

@@ -67,6 +67,7 @@ struct rp3dg_batch {
 
     bool profiling = false;
     std::vector<Pending> pending;
+    std::vector<hipEvent_t> event_pool;
     std::map<std::string, KernelStat> stats;
 
     ~rp3dg_batch();

@@ -115,9 +115,13 @@ static T* dalloc(rp3dg_batch* b, size_t n) {
 
 void rp3dg_batch::timer_begin(const char* name) {
     if (!profiling) return;
-    hipEvent_t a, e;
-    hipEventCreate(&a);
-    hipEventCreate(&e);
+    const size_t k = pending.size();
+    while (event_pool.size() < 2 * (k + 1)) {
+        hipEvent_t ev;
+        hipEventCreate(&ev);
+        event_pool.push_back(ev);
+    }
+    hipEvent_t a = event_pool[2 * k], e = event_pool[2 * k + 1];
     hipEventRecord(a, stream);
     pending.push_back({name, a, e});
 }
@@ -134,8 +138,6 @@ void rp3dg_batch::timer_flush() {
         KernelStat& s = stats[p.name];
         s.total_ms += ms;
         s.launches += 1;
-        hipEventDestroy(p.a);
-        hipEventDestroy(p.e);
     }
     pending.clear();
 }
@@ -445,6 +447,47 @@ int rp3dg_batch::upload(rp3dg_world* const* ws, int nw) {
     bp_refit<<<grid_for(nc), 256, 0, stream>>>(B, C, d_hbounds, 1);
     HIPCHECK(hipGetLastError());
     HIPCHECK(hipStreamSynchronize(stream));
+
+    // Static colliders never move, so their mutual overlaps are fixed at creation: list them once
+    // (same closed AABB test and pair filter as the device traversal).
+    std::vector<int> is_static(nc, 0), ss_offs(nc + 1, 0), ss_list;
+    for (int i = 0; i < nc; i++) is_static[i] = (flags[cbody[i]] & (BF_DYNAMIC | BF_KINEMATIC)) == 0;
+    {
+        std::vector<float4> amn(nc), amx(nc);
+        HIPCHECK(hipMemcpy(amn.data(), C.amin, sizeof(float4) * nc, hipMemcpyDeviceToHost));
+        HIPCHECK(hipMemcpy(amx.data(), C.amax, sizeof(float4) * nc, hipMemcpyDeviceToHost));
+        std::vector<std::vector<int>> per_world(nw);
+        for (int i = 0; i < nc; i++)
+            if (is_static[i]) per_world[cworld[i]].push_back(i);
+        std::vector<std::vector<int>> partners(nc);
+        for (const auto& sl : per_world) {
+            for (size_t a = 0; a < sl.size(); a++) {
+                for (size_t b2 = a + 1; b2 < sl.size(); b2++) {
+                    const int i = sl[a], j = sl[b2];
+                    const float4 a0 = amn[i], a1 = amx[i], b0 = amn[j], b1 = amx[j];
+                    if (a1.x < b0.x || b1.x < a0.x || a1.y < b0.y || b1.y < a0.y || a1.z < b0.z || b1.z < a0.z) continue;
+                    if (cbody[i] == cbody[j]) continue;
+                    const uint32_t fi = filter[i], fj = filter[j];
+                    if (((fi >> 16) & (fj & 0xffffu)) == 0 || ((fi & 0xffffu) & (fj >> 16)) == 0) continue;
+                    const uint64_t bk = ((uint64_t)(uint32_t)std::min(cbody[i], cbody[j]) << 32) |
+                                        (uint32_t)std::max(cbody[i], cbody[j]);
+                    if (std::binary_search(nocol.begin(), nocol.end(), bk)) continue;
+                    partners[std::min(i, j)].push_back(std::max(i, j));
+                }
+            }
+        }
+        for (int i = 0; i < nc; i++) {
+            std::sort(partners[i].begin(), partners[i].end());
+            ss_offs[i + 1] = ss_offs[i] + (int)partners[i].size();
+            ss_list.insert(ss_list.end(), partners[i].begin(), partners[i].end());
+        }
+    }
+    C.is_static = dalloc<int>(this, nc);
+    C.ss_offs = dalloc<int>(this, nc + 1);
+    C.ss_list = dalloc<int>(this, std::max<size_t>(1, ss_list.size()));
+    HIPCHECK(hipMemcpy(C.is_static, is_static.data(), sizeof(int) * nc, hipMemcpyHostToDevice));
+    HIPCHECK(hipMemcpy(C.ss_offs, ss_offs.data(), sizeof(int) * (nc + 1), hipMemcpyHostToDevice));
+    if (!ss_list.empty()) HIPCHECK(hipMemcpy(C.ss_list, ss_list.data(), sizeof(int) * ss_list.size(), hipMemcpyHostToDevice));
     return RP3DG_OK;
 }
 
@@ -600,6 +643,7 @@ int rp3dg_batch::step(float dt) {
 }
 
 rp3dg_batch::~rp3dg_batch() {
+    for (hipEvent_t ev : event_pool) hipEventDestroy(ev);
     for (void* p : allocs) hipFree(p);
     if (hcounters) hipHostFree(hcounters);
     if (stream) hipStreamDestroy(stream);
@@ -662,6 +706,7 @@ int rp3dg_batch_get_counts(rp3dg_batch* b, rp3dg_counts* o) {
     o->nb_contact_points = b->hcounters->nb_points;
     o->nb_colors = b->last_colors;
     o->status = b->hcounters->status;
+    for (int a = 0; a < 7; a++) o->nb_tests[a] = b->hcounters->nb_tests[a];
     return RP3DG_OK;
 }
 

@@ -369,25 +369,38 @@ __device__ __forceinline__ bool pair_allowed(const DevColliders& C, int a, int b
     return (maska & catb) != 0 && (cata & maskb) != 0;
 }
 
-// Pass 0 counts, pass 1 writes each collider's partner list (partners with a larger collider index)
-// sorted ascending into its segment.
+// Each overlapping pair is emitted by exactly one of its colliders (the pair key's high word):
+//   moving vs moving -> the lower collider index; moving vs static -> the moving one;
+//   static vs static -> the lower index, from the constant list built at creation (statics never
+//   traverse: a floor's AABB overlaps most of the tree and would serialise one thread over it).
+// Pass 0 counts, pass 1 writes each emitter's partner list sorted ascending into its segment, so the
+// pair table is sorted by key.
 __global__ void bp_traverse(DevColliders C, const int* sorted, int n, const int2* child, const float4* nmin,
                             const float4* nmax, const int* root, const uint64_t* nc, int nnc, int* count,
                             const int* offset, uint64_t* out_keys, int capacity, int pass, int* status) {
     RP3DG_GRID_STRIDE(s, n) {
         const int c = sorted[s];
+        const int base = pass ? offset[c] : 0;
+        if (C.is_static[c]) {
+            const int b = C.ss_offs[c], e = C.ss_offs[c + 1];
+            if (pass)
+                for (int k = b; k < e; k++)
+                    if (base + k - b < capacity) out_keys[base + k - b] = ((uint64_t)(uint32_t)c << 32) | (uint32_t)C.ss_list[k];
+            if (!pass) count[c] = e - b;
+            continue;
+        }
         const float4 amn = C.amin[c], amx = C.amax[c];
         int stack[kTravStack];
         int sp = 0;
         stack[sp++] = root[C.world[c]];
         int cnt = 0;
-        const int base = pass ? offset[c] : 0;
         while (sp > 0) {
             const int node = stack[--sp];
             if (!aabb_overlap(amn, amx, nmin[node], nmax[node])) continue;
             if (node >= n - 1) {
                 const int o = sorted[node - (n - 1)];
-                if (o <= c) continue;
+                if (o == c) continue;
+                if (!C.is_static[o] && o < c) continue;
                 if (!pair_allowed(C, c, o, nc, nnc)) continue;
                 if (pass) {
                     const uint64_t key = ((uint64_t)(uint32_t)c << 32) | (uint32_t)o;

@@ -696,17 +696,30 @@ __device__ inline uint32_t lf_after_test(uint32_t lf, bool colliding) {
 
 // Middle phase: bucket the tested pairs by algorithm; a pair is tested only if at least one of its
 // bodies is enabled (CD.cpp:391-400).  Untested pairs have no contact pair this step.
+// List slots are reserved with one atomic per wave and algorithm (ballot + popcount ranks); the list
+// order is irrelevant because every test writes only its own pair slot.
 __global__ void np_classify(NpArgs A, int* lists, int list_stride, int* counts) {
     const int np = *A.nb_pairs;
-    RP3DG_GRID_STRIDE(k, np) {
-        const int2 c = A.P.col[k];
-        const int f1 = A.B.flags[A.C.body[c.x]], f2 = A.B.flags[A.C.body[c.y]];
-        if ((f1 & BF_ENABLED) || (f2 & BF_ENABLED)) {
-            const int a = A.P.algo[k];
-            const int pos = atomicAdd(&counts[a], 1);
-            lists[a * list_stride + pos] = k;
-        } else {
-            A.P.ncp[k] = 0;
+    const int stride = gridDim.x * blockDim.x;
+    const int lane = lane_id();
+    const uint64_t lt = lane == 0 ? 0ull : (~0ull >> (64 - lane));
+    for (int base = blockIdx.x * blockDim.x; base < np; base += stride) {
+        const int k = base + threadIdx.x;
+        int a = 0;
+        if (k < np) {
+            const int2 c = A.P.col[k];
+            const int f1 = A.B.flags[A.C.body[c.x]], f2 = A.B.flags[A.C.body[c.y]];
+            if ((f1 & BF_ENABLED) || (f2 & BF_ENABLED)) a = A.P.algo[k];
+            else A.P.ncp[k] = 0;
+        }
+        for (int t = 1; t < ALGO_COUNT; t++) {
+            const uint64_t m = __ballot(a == t);
+            if (m == 0) continue;
+            const int leader = __ffsll((unsigned long long)m) - 1;
+            int pos = 0;
+            if (lane == leader) pos = atomicAdd(&counts[t], __popcll(m));
+            pos = __shfl(pos, leader, kWave);
+            if (a == t) lists[t * list_stride + pos + __popcll(m & lt)] = k;
         }
     }
 }

@@ -95,6 +95,9 @@ struct DevColliders {
     int *moved;                // fat AABB recomputed by the last refit (BroadPhaseSystem moved set)
     int *rank;                 // position of the collider in the moved-set iteration order
     int *world;
+    int *is_static;            // collider of a static body (never moves, never traverses the BVH)
+    int *ss_offs;              // static-static overlapping partners (CSR, constant after creation)
+    int *ss_list;
     int n;
 };
 

@@ -34,7 +34,8 @@ class WorldSettings(C.Structure):
 
 class Counts(C.Structure):
     _fields_ = [(n, C.c_int64) for n in ("nb_bodies", "nb_colliders", "nb_joints", "nb_worlds", "nb_pairs",
-                                         "nb_manifolds", "nb_contact_points", "nb_colors", "status")]
+                                         "nb_manifolds", "nb_contact_points", "nb_colors", "status")] + \
+               [("nb_tests", C.c_int64 * 7)]
 
 
 class ContactView(C.Structure):
@@ -227,7 +228,9 @@ class Batch:
     def counts(self) -> dict:
         c = Counts()
         _check(load().rp3dg_batch_get_counts(self.h, C.byref(c)), "get_counts")
-        return {n: getattr(c, n) for n, _ in Counts._fields_}
+        out = {n: getattr(c, n) for n, _ in Counts._fields_ if n != "nb_tests"}
+        out["nb_tests"] = list(c.nb_tests)
+        return out
 
     def bodies(self):
         n = self.nb
