@@ -112,6 +112,12 @@ class RefWorld:
     def step(self, dt):
         self.L.rp3dref_step(self.h, float(dt))
 
+    def set_joint_order(self, order):
+        o = np.ascontiguousarray(order, np.int32)
+        self.L.rp3dref_set_joint_order.argtypes = [C.c_void_p, C.POINTER(C.c_int32), C.c_int32]
+        if self.L.rp3dref_set_joint_order(self.h, _ip(o), len(o)) != 0:
+            raise RuntimeError("set_joint_order: unsupported joint kind")
+
     def step_ordered(self, dt, order_c1, order_c2):
         o1 = np.ascontiguousarray(order_c1, np.int32)
         o2 = np.ascontiguousarray(order_c2, np.int32)

@@ -276,6 +276,35 @@ int rp3dref_step_ordered(void* h, float dt, const int32_t* oc1, const int32_t* o
     return 0;
 }
 
+// Reorders the joint components of every joint kind so that the reference solves them in the given
+// order (global joint creation indices, as the device solves them).  Components::swapComponents keeps
+// each component's data (and warm-start impulses) with its entity.
+static void reorder(Components& comps, const std::vector<Entity>& want) {
+    for (uint32 i = 0; i < (uint32)want.size(); i++) {
+        const uint32 cur = comps.getEntityIndex(want[i]);
+        if (cur != i) comps.swapComponents(i, cur);
+    }
+}
+
+int rp3dref_set_joint_order(void* h, const int32_t* order, int n) {
+    RefWorld* R = (RefWorld*)h;
+    PhysicsWorld& W = *R->world;
+    std::vector<Entity> bs, fx, hg;
+    for (int i = 0; i < n; i++) {
+        Joint* j = R->joints[order[i]];
+        switch (j->getType()) {
+            case JointType::BALLSOCKETJOINT: bs.push_back(j->getEntity()); break;
+            case JointType::FIXEDJOINT: fx.push_back(j->getEntity()); break;
+            case JointType::HINGEJOINT: hg.push_back(j->getEntity()); break;
+            default: return -1;
+        }
+    }
+    reorder(W.mBallAndSocketJointsComponents, bs);
+    reorder(W.mFixedJointsComponents, fx);
+    reorder(W.mHingeJointsComponents, hg);
+    return 0;
+}
+
 // Bodies: origin position, orientation (x,y,z,w), centre-of-mass linear velocity, angular velocity.
 int rp3dref_get_bodies(void* h, float* pos, float* q, float* v, float* w) {
     RefWorld* R = (RefWorld*)h;

@@ -7,6 +7,7 @@
 #include <string>
 #include <vector>
 
+#include "joints.h"
 #include "rp3d_internal.h"
 
 struct rp3dg_world;
@@ -26,6 +27,10 @@ struct rp3dg_batch {
     rp3dg::DevColliders C{};
     rp3dg::DevPairs Pa{}, Pb{};
     rp3dg::DevSolver S{};
+    rp3dg::DevJoints J{};
+    int* jt_ranges = nullptr;          // [2*slot] colour ranges of the joint rows, slots in solve order
+    int jt_slots = 0;
+    std::vector<int> joint_order;      // global joint index of each joint row
     bool cur_is_a = true;
     int pair_cap = 0;
     int nb_pairs = 0;
@@ -72,6 +77,7 @@ struct rp3dg_batch {
 
     ~rp3dg_batch();
     int upload(rp3dg_world* const* worlds, int n);
+    int upload_joints(rp3dg_world* const* worlds, int n);
     int step(float dt);
     int scan(const int* in, int n, int* out, hipStream_t s);
     int sort(uint64_t** keys, int** vals, uint64_t* ktmp, int* vtmp, int n, int bits, hipStream_t s);

@@ -219,6 +219,115 @@ static void moved_ranks(int n, std::vector<int>& rank) {
     for (int r = 0; r < n; r++) rank[keyed[r].second] = r;
 }
 
+// Joint rows.  Constants follow the joint constructors with world-space anchors
+// (BallAndSocketJoint.cpp:38-53, FixedJoint.cpp:36-56, HingeJoint.cpp:36-80); the rows are sorted by
+// (kind in solve order, colour, creation index) with a greedy colouring over all bodies.
+int rp3dg_batch::upload_joints(rp3dg_world* const* ws, int nw) {
+    struct Row { int kind, color, order, b1, b2; float4 la1, la2, ax1, ax2, qinit, lim; int flags; };
+    std::vector<Row> rows;
+    int jbase = 0;
+    for (int wi = 0; wi < nw; wi++) {
+        const HostWorld& W = *ws[wi];
+        for (size_t ji = 0; ji < W.joints.size(); ji++) {
+            const HostJoint& hj = W.joints[ji];
+            Row r{};
+            if (hj.type == RP3DG_JOINT_BALLSOCKET) r.kind = JT_BALLSOCKET;
+            else if (hj.type == RP3DG_JOINT_FIXED) r.kind = JT_FIXED;
+            else if (hj.type == RP3DG_JOINT_HINGE) r.kind = JT_HINGE;
+            else { set_error("slider joints are not on the device path"); return RP3DG_EUNSUPPORTED; }
+            r.order = jbase + (int)ji;
+            r.b1 = body_base[wi] + hj.body1;
+            r.b2 = body_base[wi] + hj.body2;
+            const Tf& t1 = W.bodies[hj.body1].transform;
+            const Tf& t2 = W.bodies[hj.body2].transform;
+            const V3 a1 = tapply(tinv(t1), hj.anchor_world);
+            const V3 a2 = tapply(tinv(t2), hj.anchor_world);
+            r.la1 = make_float4(a1.x, a1.y, a1.z, 0.f);
+            r.la2 = make_float4(a2.x, a2.y, a2.z, 0.f);
+            Q4 qi = q4(0, 0, 0, 1);
+            if (r.kind == JT_FIXED) {
+                qi = qmul(qconj(t2.q), t1.q);
+            } else if (r.kind == JT_HINGE) {
+                const V3 h1 = normalized(qrot(qconj(t1.q), hj.axis_world));
+                const V3 h2 = normalized(qrot(qconj(t2.q), hj.axis_world));
+                r.ax1 = make_float4(h1.x, h1.y, h1.z, 0.f);
+                r.ax2 = make_float4(h2.x, h2.y, h2.z, 0.f);
+                Q4 d = qmul(t2.q, qconj(t1.q));
+                const float l = qlen(d);
+                d = q4(d.x / l, d.y / l, d.z / l, d.w / l);
+                qi = q4(-d.x, -d.y, -d.z, d.w);
+            }
+            r.qinit = make_float4(qi.x, qi.y, qi.z, qi.w);
+            r.flags = ((hj.flags & 1) ? JF_LIMIT : 0) | ((hj.flags & 2) ? JF_MOTOR : 0);
+            // HingeJointInfo defaults when the limit is off (HingeJoint.h:97-105): [-1, 1]
+            r.lim = make_float4((hj.flags & 1) ? hj.limits[0] : -1.0f, (hj.flags & 1) ? hj.limits[1] : 1.0f,
+                                hj.limits[2], hj.limits[3]);
+            rows.push_back(r);
+        }
+        jbase += (int)W.joints.size();
+    }
+    // greedy colouring per kind in creation order
+    std::vector<uint64_t> used(nb, 0);
+    for (int kind = 0; kind < 3; kind++) {
+        std::fill(used.begin(), used.end(), 0ull);
+        for (Row& r : rows) {
+            if (r.kind != kind) continue;
+            const uint64_t u = used[r.b1] | used[r.b2];
+            int c = 0;
+            while (c < 63 && (u >> c) & 1ull) c++;
+            r.color = c;
+            used[r.b1] |= 1ull << c;
+            used[r.b2] |= 1ull << c;
+        }
+    }
+    std::stable_sort(rows.begin(), rows.end(), [](const Row& a, const Row& b) {
+        if (a.kind != b.kind) return a.kind < b.kind;
+        if (a.color != b.color) return a.color < b.color;
+        return a.order < b.order;
+    });
+    const int n = (int)rows.size();
+    std::vector<int> ranges;
+    for (int i = 0; i < n; i++) {
+        if (i == 0 || rows[i].kind != rows[i - 1].kind || rows[i].color != rows[i - 1].color) ranges.push_back(i);
+        if (i == n - 1 || rows[i + 1].kind != rows[i].kind || rows[i + 1].color != rows[i].color) ranges.push_back(i + 1);
+    }
+    jt_slots = (int)ranges.size() / 2;
+    joint_order.resize(n);
+    J.n = n;
+    const size_t m = std::max(1, n);
+    J.type = dalloc<int>(this, m); J.bodies = dalloc<int2>(this, m); J.flags = dalloc<int>(this, m);
+    for (float4** a : {&J.la1, &J.la2, &J.ax1, &J.ax2, &J.qinit, &J.lim, &J.impT, &J.impR, &J.impL, &J.i1r0, &J.i1r1,
+                       &J.i1r2, &J.i2r0, &J.i2r1, &J.i2r2, &J.r1, &J.r2, &J.kt0, &J.kt1, &J.kt2, &J.kr0, &J.kr1,
+                       &J.kr2, &J.a1, &J.b2xa1, &J.c2xa1})
+        *a = dalloc<float4>(this, m);
+    J.imlm = dalloc<float>(this, m);
+    jt_ranges = dalloc<int>(this, std::max<size_t>(2, ranges.size()));
+    if (!J.imlm || !jt_ranges) { set_error("device allocation failed (joints)"); return RP3DG_EHIP; }
+    if (n == 0) return RP3DG_OK;
+    std::vector<int> type(n), flags(n);
+    std::vector<int2> bodies(n);
+    std::vector<float4> la1(n), la2(n), ax1(n), ax2(n), qinit(n), lim(n), zero(n, make_float4(0.f, 0.f, 0.f, 0.f));
+    for (int i = 0; i < n; i++) {
+        const Row& r = rows[i];
+        type[i] = r.kind; flags[i] = r.flags; bodies[i] = make_int2(r.b1, r.b2);
+        la1[i] = r.la1; la2[i] = r.la2; ax1[i] = r.ax1; ax2[i] = r.ax2; qinit[i] = r.qinit; lim[i] = r.lim;
+        joint_order[i] = r.order;
+    }
+    HIPCHECK(hipMemcpy(J.type, type.data(), 4 * n, hipMemcpyHostToDevice));
+    HIPCHECK(hipMemcpy(J.flags, flags.data(), 4 * n, hipMemcpyHostToDevice));
+    HIPCHECK(hipMemcpy(J.bodies, bodies.data(), 8 * n, hipMemcpyHostToDevice));
+    HIPCHECK(hipMemcpy(J.la1, la1.data(), 16 * n, hipMemcpyHostToDevice));
+    HIPCHECK(hipMemcpy(J.la2, la2.data(), 16 * n, hipMemcpyHostToDevice));
+    HIPCHECK(hipMemcpy(J.ax1, ax1.data(), 16 * n, hipMemcpyHostToDevice));
+    HIPCHECK(hipMemcpy(J.ax2, ax2.data(), 16 * n, hipMemcpyHostToDevice));
+    HIPCHECK(hipMemcpy(J.qinit, qinit.data(), 16 * n, hipMemcpyHostToDevice));
+    HIPCHECK(hipMemcpy(J.lim, lim.data(), 16 * n, hipMemcpyHostToDevice));
+    for (float4* a : {J.impT, J.impR, J.impL, J.a1, J.b2xa1, J.c2xa1}) HIPCHECK(hipMemcpy(a, zero.data(), 16 * n, hipMemcpyHostToDevice));
+    HIPCHECK(hipMemset(J.imlm, 0, sizeof(float) * n));
+    HIPCHECK(hipMemcpy(jt_ranges, ranges.data(), sizeof(int) * ranges.size(), hipMemcpyHostToDevice));
+    return RP3DG_OK;
+}
+
 int rp3dg_batch::upload(rp3dg_world* const* ws, int nw) {
     worlds.assign(ws, ws + nw);
     // host SoA
@@ -482,6 +591,10 @@ int rp3dg_batch::upload(rp3dg_world* const* ws, int nw) {
             ss_list.insert(ss_list.end(), partners[i].begin(), partners[i].end());
         }
     }
+    {
+        const int r = upload_joints(ws, nw);
+        if (r) return r;
+    }
     C.is_static = dalloc<int>(this, nc);
     C.ss_offs = dalloc<int>(this, nc + 1);
     C.ss_list = dalloc<int>(this, std::max<size_t>(1, ss_list.size()));
@@ -617,17 +730,31 @@ int rp3dg_batch::step(float dt) {
     SA.cstart = cstart;
     SA.ncolors = &counters->nb_colors;
     SA.restitution_threshold = settings.restitution_threshold;
+    const int jg = grid_for(std::max(J.n, 1));
     if (nm > 0) {
         LAUNCH_T("sv_prepare", mg, 256, sv_prepare, SA);
         for (int c = 0; c < ncol; c++) LAUNCH_T("sv_warmstart", mg, 256, sv_warmstart, SA, c);
-        const float beta_dt = 0.2f / dt;
-        for (int it = 0; it < settings.vel_iters; it++)
-            for (int c = 0; c < ncol; c++) LAUNCH_T("sv_solve_color", mg, 256, sv_solve_color, SA, c, beta_dt);
-        LAUNCH_T("sv_store", mg, 256, sv_store, SA);
     }
+    // ConstraintSolverSystem::initialize: initBeforeSolve + warm start of every joint kind
+    if (J.n > 0) {
+        LAUNCH_T("jt_init", jg, 256, jt_init, B, J);
+        for (int s = 0; s < jt_slots; s++) LAUNCH_T("jt_warmstart", jg, 256, jt_warmstart, B, J, jt_ranges, s);
+    }
+    const float beta_dt = 0.2f / dt;
+    for (int it = 0; it < settings.vel_iters; it++) {
+        for (int s = 0; s < jt_slots; s++)
+            LAUNCH_T("jt_solve_velocity", jg, 256, jt_solve_velocity, B, J, jt_ranges, s, dt);
+        if (nm > 0)
+            for (int c = 0; c < ncol; c++) LAUNCH_T("sv_solve_color", mg, 256, sv_solve_color, SA, c, beta_dt);
+    }
+    if (nm > 0) LAUNCH_T("sv_store", mg, 256, sv_store, SA);
 
-    // ---------------- position integration, state update, broadphase refit (PhysicsWorld.cpp:275-288)
+    // ---------------- position integration, joint position correction, state update, refit
+    // (PhysicsWorld.cpp:275-288, :348-360)
     LAUNCH_T("dyn_integrate_positions", bg, 256, dyn_integrate_positions, B, dt);
+    for (int it = 0; it < settings.pos_iters; it++)
+        for (int s = 0; s < jt_slots; s++)
+            LAUNCH_T("jt_solve_position", jg, 256, jt_solve_position, B, J, jt_ranges, s);
     LAUNCH_T("dyn_update_state", bg, 256, dyn_update_state, B);
     LAUNCH_T("bp_refit", grid_for(nc), 256, bp_refit, B, C, d_hbounds, 0);
     HIPCHECK(hipGetLastError());
@@ -699,7 +826,7 @@ int rp3dg_batch_get_counts(rp3dg_batch* b, rp3dg_counts* o) {
     if (!b || !o) return RP3DG_EINVAL;
     o->nb_bodies = b->nb;
     o->nb_colliders = b->nc;
-    o->nb_joints = 0;
+    o->nb_joints = b->J.n;
     o->nb_worlds = b->nworlds;
     o->nb_pairs = b->nb_pairs;
     o->nb_manifolds = b->hcounters->nb_manifolds;
@@ -790,9 +917,11 @@ int rp3dg_batch_get_contacts(rp3dg_batch* b, rp3dg_contact_view* o, int64_t cap)
 }
 
 int rp3dg_batch_get_joint_order(rp3dg_batch* b, int32_t* order, int64_t cap) {
-    (void)order; (void)cap;
     if (!b) return RP3DG_EINVAL;
-    return 0;
+    const int n = (int)b->joint_order.size();
+    if (cap < n || (n > 0 && !order)) { set_error("capacity too small"); return RP3DG_EINVAL; }
+    for (int i = 0; i < n; i++) order[i] = b->joint_order[i];
+    return n;
 }
 
 int rp3dg_batch_profile(rp3dg_batch* b, int32_t enable) {

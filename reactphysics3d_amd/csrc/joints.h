@@ -1,0 +1,43 @@
+// Joint rows and launch declarations (joints.hip).
+#pragma once
+
+#include "rp3d_internal.h"
+
+namespace rp3dg {
+
+// joint kinds in solve order (ConstraintSolverSystem.cpp:94-113); values are internal
+constexpr int JT_BALLSOCKET = 0;
+constexpr int JT_FIXED = 1;
+constexpr int JT_HINGE = 2;
+
+constexpr int JF_LIMIT = 1;
+constexpr int JF_MOTOR = 2;
+constexpr int JF_LOWER_VIOLATED = 4;
+constexpr int JF_UPPER_VIOLATED = 8;
+
+// One entry per joint, sorted by (kind, colour, creation index).
+struct DevJoints {
+    int n;
+    int* type;
+    int2* bodies;
+    float4 *la1, *la2;            // local anchor points (body space)
+    float4 *ax1, *ax2;            // hinge axis in body spaces
+    float4* qinit;                // initial orientation difference inverse (fixed, hinge)
+    int* flags;                   // JF_* (violation bits persist between steps)
+    float4* lim;                  // lower, upper, motor speed, max motor torque
+    float4 *impT, *impR, *impL;   // accumulated impulses: translation, rotation, (lower, upper, motor)
+    // per-step rows
+    float4 *i1r0, *i1r1, *i1r2, *i2r0, *i2r1, *i2r2;
+    float4 *r1, *r2;
+    float4 *kt0, *kt1, *kt2;      // inverse translation mass matrix
+    float4 *kr0, *kr1, *kr2;      // inverse rotation mass matrix (fixed: 3x3, hinge: 2x2 in kr0)
+    float4 *a1, *b2xa1, *c2xa1;   // hinge axes (ball-socket keeps the cone axis in a1)
+    float* imlm;                  // inverse mass of the hinge limit / motor row
+};
+
+__global__ void jt_init(DevBodies B, DevJoints J);
+__global__ void jt_warmstart(DevBodies B, DevJoints J, const int* range, int slot);
+__global__ void jt_solve_velocity(DevBodies B, DevJoints J, const int* range, int slot, float dt);
+__global__ void jt_solve_position(DevBodies B, DevJoints J, const int* range, int slot);
+
+}  // namespace rp3dg

@@ -261,6 +261,12 @@ class Batch:
         n = _check(load().rp3dg_batch_get_contacts(self.h, C.byref(v), nm), "get_contacts")
         return {k: a[:n] for k, a in out.items()}
 
+    def joint_order(self):
+        n = max(1, self.counts()["nb_joints"])
+        o = np.zeros(n, np.int32)
+        k = _check(load().rp3dg_batch_get_joint_order(self.h, _ip(o), n), "get_joint_order")
+        return o[:k]
+
     def profile(self, enable=True):
         _check(load().rp3dg_batch_profile(self.h, int(enable)), "profile")
 

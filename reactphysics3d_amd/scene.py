@@ -197,6 +197,58 @@ def mixed_container(n_bodies=65536, extent=(200.0, 200.0, 100.0), seed=2, vel_it
     return sc
 
 
+def sphere_chains(n_chains=16, length=16, radius=0.3, spacing=3.0, seed=3, settings=None, sc=None):
+    """Chains of `length` spheres linked by ball-and-socket joints at their touching points; the first
+    sphere of each chain hangs from a static anchor body (no collider).  Chains start horizontal in a
+    seeded direction so that they swing down under gravity -z."""
+    rng = SplitMix64(seed)
+    if sc is None:
+        sc = Scene(f"sphere_chains_{n_chains}x{length}",
+                   settings if settings is not None else make_settings(gravity=(0.0, 0.0, -9.81)))
+    side = int(math.ceil(math.sqrt(n_chains)))
+    for c in range(n_chains):
+        ax = (c % side - side / 2.0) * spacing * length * radius
+        ay = (c // side - side / 2.0) * spacing * length * radius
+        az = 2.0 * length * radius + 5.0
+        ang = 2.0 * math.pi * rng.uniform01()
+        d = (math.cos(ang), math.sin(ang), 0.0)
+        anchor = sc.add_body(STATIC, (ax, ay, az))
+        prev = anchor
+        for k in range(length):
+            p = (ax + d[0] * radius * (2 * k + 1), ay + d[1] * radius * (2 * k + 1), az)
+            b = sc.add_body(DYNAMIC, p, rng.quaternion())
+            sc.add_collider(b, SPHERE, (radius,))
+            joint_at = (ax + d[0] * radius * 2 * k, ay + d[1] * radius * 2 * k, az)
+            sc.add_joint(BALLSOCKET, prev, b, joint_at)
+            prev = b
+    return sc
+
+
+def jointed_tower(height=6, seed=4, kinds=(HINGE, FIXED), friction=0.5, gravity=(0.0, 0.0, -9.81), sc=None,
+                  origin=(0.0, 0.0), with_ground=True):
+    """A tower of 1 m boxes on a static ground; consecutive boxes are linked by hinge / fixed joints
+    (cycling through `kinds`) anchored at their shared face, hinge axes seeded in the horizontal plane."""
+    rng = SplitMix64(seed)
+    if sc is None:
+        sc = Scene(f"jointed_tower_{height}", make_settings(gravity=gravity, vel_iters=10, pos_iters=5,
+                                                             friction=friction, bounciness=0.0))
+    if with_ground:
+        g = sc.add_body(STATIC, (origin[0], origin[1], -0.5))
+        sc.add_collider(g, BOX, (20.0, 20.0, 0.5), friction=friction, bounciness=0.0)
+    prev = None
+    for k in range(height):
+        pos = (origin[0] + rng.uniform(-0.05, 0.05), origin[1] + rng.uniform(-0.05, 0.05), 0.5 + 1.0 * k)
+        b = sc.add_body(DYNAMIC, pos)
+        sc.add_collider(b, BOX, (0.5, 0.5, 0.5), friction=friction, bounciness=0.0)
+        if prev is not None:
+            kind = kinds[(k - 1) % len(kinds)]
+            anchor = (pos[0], pos[1], pos[2] - 0.5)
+            a = 2.0 * math.pi * rng.uniform01()
+            sc.add_joint(kind, prev, b, anchor, axis=(math.cos(a), math.sin(a), 0.0))
+        prev = b
+    return sc
+
+
 def two_body(shape1, params1, shape2, params2, pos1, pos2, q1=(0, 0, 0, 1), q2=(0, 0, 0, 1), static2=False):
     """Two bodies, no gravity: the unit case of each narrowphase pair type."""
     sc = Scene("two_body", make_settings(gravity=(0.0, 0.0, 0.0)))
