@@ -130,11 +130,28 @@ int32_t rp3dg_joint_create(rp3dg_world* world, int32_t type, int32_t body1, int3
 /* Steps one world on the GPU (uploads it on first use, as a batch of one). */
 int rp3dg_world_update(rp3dg_world* world, float time_step);
 
-/* Many independent worlds stepped together: one launch sequence per step for all of them. */
+/* Many independent worlds stepped together: one launch sequence per step for all of them.  The
+ * reference has one PhysicsWorld per update() call (src/engine/PhysicsWorld.cpp:239); a batch holds
+ * many, which must share one WorldSettings. */
 rp3dg_batch* rp3dg_batch_create(rp3dg_world* const* worlds, int32_t nb_worlds);
+
+/* Solver layout of a batch. */
+#define RP3DG_SOLVER_AUTO 0   /* FUSED when every world has at most 4096 bodies, else PER_COLOR */
+#define RP3DG_SOLVER_PER_COLOR 1  /* one launch per colour over all worlds (large worlds) */
+#define RP3DG_SOLVER_FUSED 2  /* one workgroup per world runs the whole solver phase (many small worlds) */
+
+typedef struct rp3dg_batch_options {
+    int32_t solver;           /* RP3DG_SOLVER_* */
+    int32_t fused_block;      /* threads per world in the fused solver (64, 128, 192 or 256; 0 = auto) */
+    int32_t pairs_per_collider; /* pair-table capacity per collider (0 = 16) */
+    int32_t reserved[5];
+} rp3dg_batch_options;
+/* As rp3dg_batch_create, with options (NULL = defaults). */
+rp3dg_batch* rp3dg_batch_create_ex(rp3dg_world* const* worlds, int32_t nb_worlds, const rp3dg_batch_options* options);
 void rp3dg_batch_destroy(rp3dg_batch* batch);
 int rp3dg_batch_step(rp3dg_batch* batch, float time_step, int32_t nb_steps);
-/* Capture one step into a hipGraph and replay it nb_steps times (no allocation, no host sync inside). */
+/* Captures one step into a hipGraph (once per time step value) and replays it nb_steps times: no
+ * allocation and no host synchronisation inside the step.  Requires the fused solver layout. */
 int rp3dg_batch_step_graph(rp3dg_batch* batch, float time_step, int32_t nb_steps);
 int rp3dg_batch_synchronize(rp3dg_batch* batch);
 
@@ -184,9 +201,16 @@ int rp3dg_batch_get_joint_order(rp3dg_batch* batch, int32_t* order, int64_t capa
 int rp3dg_batch_profile(rp3dg_batch* batch, int32_t enable);
 int rp3dg_batch_get_kernel_times(rp3dg_batch* batch, const char* kernel, double* mean_ms, int64_t* launches);
 
-/* Solver convergence mode (iterate to a velocity residual instead of fixed counts). */
+/* Solver convergence mode (no counterpart in the reference, which runs a fixed iteration count,
+ * PhysicsWorld.cpp:334): every world iterates until the velocity residual
+ *   max over its dynamic bodies of max(|v_k - v_{k-1}|_inf, |w_k - w_{k-1}|_inf)
+ * between successive sweeps is below residual_threshold, at most max_iterations sweeps.
+ * residual_threshold <= 0 restores the fixed WorldSettings iteration count.  Fused layout only. */
 int rp3dg_batch_set_residual_mode(rp3dg_batch* batch, float residual_threshold, int32_t max_iterations);
-int rp3dg_batch_get_iteration_stats(rp3dg_batch* batch, double* mean_iterations, double* max_residual);
+/* Mean and max over worlds of the velocity iterations run by the last step, and the largest final
+ * residual (0 in fixed-iteration mode). */
+int rp3dg_batch_get_iteration_stats(rp3dg_batch* batch, double* mean_iterations, double* max_iterations,
+                                    double* max_residual);
 
 const char* rp3dg_last_error(void);
 const char* rp3dg_version(void);

@@ -5,13 +5,14 @@ Run as ``python -m reactphysics3d_amd.build`` or through ``__graft_entry__.build
 import os
 import subprocess
 import sys
+from concurrent.futures import ThreadPoolExecutor
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIB_DIR = os.path.join(HERE, "_lib")
 LIB = os.path.join(LIB_DIR, "librp3d_mi355x.so")
 
-SOURCES = ["world.cpp", "broadphase.hip", "narrowphase.hip", "solver.hip", "joints.hip", "batch.hip"]
+SOURCES = ["world.cpp", "broadphase.hip", "narrowphase.hip", "solver.hip", "joints.hip", "fused.hip", "batch.hip"]
 
 # -ffp-contract=off + IEEE div/sqrt keep the FP32 operation sequence of the x86-64 reference build;
 # packed FP32 instructions are disabled (wrong results have been seen from some of their forms).
@@ -20,23 +21,29 @@ HIP_FLAGS = [
     "-ffp-contract=off", "-fno-fast-math", "-fhip-fp32-correctly-rounded-divide-sqrt",
     "-fno-gpu-flush-denormals-to-zero",
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
-    "-Wno-unused-result",
+    "-Wno-unused-result", "-Wno-unused-value",
 ]
 
 
-def build(verbose=False):
+def build(verbose=False, jobs=None):
     os.makedirs(LIB_DIR, exist_ok=True)
-    objs = []
+    cmds, objs = [], []
     for src in SOURCES:
         path = os.path.join(CSRC, src)
         obj = os.path.join(LIB_DIR, src + ".o")
         cmd = ["hipcc", *HIP_FLAGS, "-c", path, "-o", obj]
         if src.endswith(".cpp"):
             cmd = ["hipcc", "-x", "hip", *HIP_FLAGS, "-c", path, "-o", obj]
+        cmds.append(cmd)
+        objs.append(obj)
+
+    def run(cmd):
         if verbose:
             print(" ".join(cmd), flush=True)
         subprocess.run(cmd, check=True)
-        objs.append(obj)
+
+    with ThreadPoolExecutor(max_workers=jobs or min(8, len(cmds))) as ex:
+        list(ex.map(run, cmds))
     tmp = LIB + ".tmp"
     cmd = ["hipcc", "--offload-arch=gfx950", "-shared", "-fPIC", *objs, "-o", tmp]
     subprocess.run(cmd, check=True)

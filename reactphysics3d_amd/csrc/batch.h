@@ -7,6 +7,7 @@
 #include <string>
 #include <vector>
 
+#include "../../include/rp3d_mi355x.h"
 #include "joints.h"
 #include "rp3d_internal.h"
 
@@ -29,14 +30,38 @@ struct rp3dg_batch {
     rp3dg::DevSolver S{};
     rp3dg::DevJoints J{};
     int* jt_ranges = nullptr;          // [2*slot] colour ranges of the joint rows, slots in solve order
+    int2* jt_wr = nullptr;             // fused solver: [w * jt_slots + slot] joint rows of world w
     int jt_slots = 0;
     std::vector<int> joint_order;      // global joint index of each joint row
     bool cur_is_a = true;
     int pair_cap = 0;
-    int nb_pairs = 0;
-    int last_colors = 0;
-    int color_rounds = 0;
+    int* pair_count = nullptr;         // device: sizes of the two pair tables (a, b)
     int64_t steps_done = 0;
+
+    // solver layout: fused = one workgroup per world for the whole solver phase, (world, colour)-major
+    // rows; otherwise one launch per colour over colour-major rows.
+    bool fused = false;
+    int fused_block = 64;
+    int nseg = 0;                      // colour segments: kMaxColors (grid) or nworlds * kMaxColors (fused)
+    int* seg_cnt = nullptr;
+    int* seg_start = nullptr;
+    int *d_wbody = nullptr, *d_wcoll = nullptr, *d_block_world = nullptr;
+    float4 *res_pv = nullptr, *res_pw = nullptr;
+    int* world_iters = nullptr;
+    float* world_res = nullptr;
+    float res_thr = 0.0f;              // residual mode threshold (0: fixed iterations)
+    int res_max_iters = 0;
+    int max_world_bodies = 0;
+
+    // launch hints from the last completed step (grids are grid-stride, any size is correct)
+    int hint_pairs = 0, hint_manifolds = 0, hint_rounds = 8;
+    rp3dg::DevCounters last{};         // counters of the last completed step
+
+    // captured step graphs, one per pair-table parity
+    hipGraphExec_t graph_exec[2] = {nullptr, nullptr};
+    hipGraph_t graph[2] = {nullptr, nullptr};
+    float graph_dt = 0.0f;
+    int graph_rounds = 0;
 
     // hull arena
     rp3dg::HullHeader* d_hulls = nullptr;
@@ -60,7 +85,8 @@ struct rp3dg_batch {
     int2* mb = nullptr;
     int* color = nullptr;
     uint64_t *ckeys = nullptr, *cktmp = nullptr;
-    int *cvals = nullptr, *cvtmp = nullptr, *cstart = nullptr;
+    int *cvals = nullptr, *cvtmp = nullptr;
+    uint64_t* sorted_keys = nullptr;   // segment key of each solver row after the colour sort
     // scan / sort
     int* scan_sums_buf = nullptr;
     int scan_sums_cap = 0;
@@ -76,11 +102,15 @@ struct rp3dg_batch {
     std::map<std::string, KernelStat> stats;
 
     ~rp3dg_batch();
-    int upload(rp3dg_world* const* worlds, int n);
+    int upload(rp3dg_world* const* worlds, int n, const rp3dg_batch_options* opt);
     int upload_joints(rp3dg_world* const* worlds, int n);
+    int enqueue_step(float dt, bool capturing);
+    int finish_steps(int n);
     int step(float dt);
-    int scan(const int* in, int n, int* out, hipStream_t s);
-    int sort(uint64_t** keys, int** vals, uint64_t* ktmp, int* vtmp, int n, int bits, hipStream_t s);
+    int step_graph(float dt, int n);
+    void drop_graphs();
+    int scan(const int* in, int n, const int* nptr, int* out, hipStream_t s);
+    int sort(uint64_t** keys, int** vals, uint64_t* ktmp, int* vtmp, int n, const int* nptr, int bits, hipStream_t s);
     void timer_begin(const char* name);
     void timer_end();
     void timer_flush();

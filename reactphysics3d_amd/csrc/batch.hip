@@ -12,6 +12,7 @@
 
 #include "../../include/rp3d_mi355x.h"
 #include "batch.h"
+#include "fused.h"
 #include "kcommon.h"
 #include "narrowphase.h"
 #include "solver.h"
@@ -23,9 +24,9 @@ namespace rp3dg {
 __global__ void bp_refit(DevBodies B, DevColliders C, const float4* hull_bounds, int init);
 __global__ void bp_bounds(DevColliders C, float4* partial);
 __global__ void bp_morton(DevColliders C, const float4* partial, int nparts, uint64_t* keys, int* vals);
-__global__ void rs_hist(const uint64_t* keys, int n, int shift, int* hist, int nblocks);
-__global__ void rs_scatter(const uint64_t* kin, const int* vin, uint64_t* kout, int* vout, int n, int shift,
-                           const int* offs, int nblocks);
+__global__ void rs_hist(const uint64_t* keys, int n, const int* nptr, int shift, int* hist, int nblocks);
+__global__ void rs_scatter(const uint64_t* kin, const int* vin, uint64_t* kout, int* vout, int n, const int* nptr,
+                           int shift, const int* offs, int nblocks);
 __global__ void bp_karras(const uint64_t* k, int n, int2* child, int* parent, int2* range);
 __global__ void bp_fit(DevColliders C, const int* sorted, int n, const int2* child, const int* parent, float4* nmin,
                        float4* nmax, int* flags);
@@ -33,16 +34,19 @@ __global__ void bp_world_roots(const uint64_t* k, int n, const int2* range, int*
 __global__ void bp_traverse(DevColliders C, const int* sorted, int n, const int2* child, const float4* nmin,
                             const float4* nmax, const int* root, const uint64_t* nc, int nnc, int* count,
                             const int* offset, uint64_t* out_keys, int capacity, int pass, int* status);
-__global__ void bp_merge(DevColliders C, const uint64_t* keys, const int* nkeys_ptr, DevPairs O, int nold, DevPairs N);
+__global__ void bp_merge(DevColliders C, const uint64_t* keys, const int* nkeys_ptr, int capacity, DevPairs O,
+                         const int* nold_ptr, DevPairs N, int* nnew_ptr, int* nb_pairs_ptr);
 
 // ---------------------------------------------------------------------------------------------
-// exclusive scan (out has n+1 entries; out[n] = total)
+// exclusive scan (out has n+1 entries; out[n] = total).  With nptr the element count is
+// min(n, *nptr), read on the device, and n is only the capacity the grid was sized for.
 // ---------------------------------------------------------------------------------------------
 constexpr int kScanThreads = 256;
 constexpr int kScanItems = 8;
 constexpr int kScanTile = kScanThreads * kScanItems;
 
-__global__ __launch_bounds__(kScanThreads) void scan_tiles(const int* in, int n, int* out, int* sums) {
+__global__ __launch_bounds__(kScanThreads) void scan_tiles(const int* in, int n, const int* nptr, int* out, int* sums) {
+    if (nptr) n = min(n, *nptr);
     __shared__ int sm[17];
     const int base = blockIdx.x * kScanTile + threadIdx.x * kScanItems;
     int v[kScanItems];
@@ -60,7 +64,8 @@ __global__ __launch_bounds__(kScanThreads) void scan_tiles(const int* in, int n,
     if (threadIdx.x == 0) sums[blockIdx.x] = total;
 }
 
-__global__ __launch_bounds__(1024) void scan_sums(int* sums, int nb, int* out_total) {
+__global__ __launch_bounds__(1024) void scan_sums(int* sums, int nb, int* out, int n, const int* nptr) {
+    if (nptr) n = min(n, *nptr);
     __shared__ int sm[17];
     int carry = 0;
     for (int base = 0; base < nb; base += 1024) {
@@ -71,17 +76,24 @@ __global__ __launch_bounds__(1024) void scan_sums(int* sums, int nb, int* out_to
         if (i < nb) sums[i] = e + carry;
         carry += total;
     }
-    if (threadIdx.x == 0) *out_total = carry;
+    if (threadIdx.x == 0) out[n] = carry;
 }
 
-__global__ __launch_bounds__(kScanThreads) void scan_add(int* out, int n, const int* sums) {
+__global__ __launch_bounds__(kScanThreads) void scan_add(int* out, int n, const int* nptr, const int* sums) {
+    if (nptr) n = min(n, *nptr);
     const int base = blockIdx.x * kScanTile + threadIdx.x * kScanItems;
     const int add = sums[blockIdx.x];
     for (int k = 0; k < kScanItems; k++)
         if (base + k < n) out[base + k] += add;
 }
 
+// Clears device memory inside the step (a kernel rather than a memset node, so the step can be
+// captured into a graph).
 __global__ void fill_int(int* p, int n, int v) { RP3DG_GRID_STRIDE(i, n) p[i] = v; }
+__global__ void fill_int_n(int* p, int n, const int* nptr, int v) {
+    n = min(n, *nptr);
+    RP3DG_GRID_STRIDE(i, n) p[i] = v;
+}
 
 }  // namespace rp3dg
 
@@ -142,17 +154,19 @@ void rp3dg_batch::timer_flush() {
     pending.clear();
 }
 
-int rp3dg_batch::scan(const int* in, int n, int* out, hipStream_t s) {
+int rp3dg_batch::scan(const int* in, int n, const int* nptr, int* out, hipStream_t s) {
     const int tiles = std::max(1, (n + kScanTile - 1) / kScanTile);
     if (tiles > scan_sums_cap) return RP3DG_EOVERFLOW;
-    scan_tiles<<<tiles, kScanThreads, 0, s>>>(in, n, out, scan_sums_buf);
-    scan_sums<<<1, 1024, 0, s>>>(scan_sums_buf, tiles, out + n);
-    scan_add<<<tiles, kScanThreads, 0, s>>>(out, n, scan_sums_buf);
+    scan_tiles<<<tiles, kScanThreads, 0, s>>>(in, n, nptr, out, scan_sums_buf);
+    scan_sums<<<1, 1024, 0, s>>>(scan_sums_buf, tiles, out, n, nptr);
+    scan_add<<<tiles, kScanThreads, 0, s>>>(out, n, nptr, scan_sums_buf);
     return RP3DG_OK;
 }
 
-// Stable LSD radix sort of (keys, vals) over the low `bits` bits.  Result in (*keys, *vals).
-int rp3dg_batch::sort(uint64_t** keys, int** vals, uint64_t* ktmp, int* vtmp, int n, int bits, hipStream_t s) {
+// Stable LSD radix sort of (keys, vals) over the low `bits` bits; n keys, or min(n, *nptr) when the
+// count lives on the device.  The result is in (*keys, *vals), which may be the tmp buffers.
+int rp3dg_batch::sort(uint64_t** keys, int** vals, uint64_t* ktmp, int* vtmp, int n, const int* nptr, int bits,
+                      hipStream_t s) {
     if (n <= 1) return RP3DG_OK;
     const int nblocks = (n + 4095) / 4096;
     if (nblocks * 256 > sort_hist_cap) return RP3DG_EOVERFLOW;
@@ -161,17 +175,15 @@ int rp3dg_batch::sort(uint64_t** keys, int** vals, uint64_t* ktmp, int* vtmp, in
     uint64_t* ko = ktmp;
     int* vo = vtmp;
     for (int shift = 0; shift < bits; shift += 8) {
-        rs_hist<<<nblocks, 256, 0, s>>>(ki, n, shift, sort_hist, nblocks);
-        int r = scan(sort_hist, nblocks * 256, sort_offs, s);
+        rs_hist<<<nblocks, 256, 0, s>>>(ki, n, nptr, shift, sort_hist, nblocks);
+        int r = scan(sort_hist, nblocks * 256, nullptr, sort_offs, s);
         if (r) return r;
-        rs_scatter<<<nblocks, 256, 0, s>>>(ki, vi, ko, vo, n, shift, sort_offs, nblocks);
+        rs_scatter<<<nblocks, 256, 0, s>>>(ki, vi, ko, vo, n, nptr, shift, sort_offs, nblocks);
         std::swap(ki, ko);
         std::swap(vi, vo);
     }
-    if (ki != *keys) {
-        hipMemcpyAsync(*keys, ki, sizeof(uint64_t) * n, hipMemcpyDeviceToDevice, s);
-        hipMemcpyAsync(*vals, vi, sizeof(int) * n, hipMemcpyDeviceToDevice, s);
-    }
+    *keys = ki;
+    *vals = vi;
     return RP3DG_OK;
 }
 
@@ -223,7 +235,7 @@ static void moved_ranks(int n, std::vector<int>& rank) {
 // (BallAndSocketJoint.cpp:38-53, FixedJoint.cpp:36-56, HingeJoint.cpp:36-80); the rows are sorted by
 // (kind in solve order, colour, creation index) with a greedy colouring over all bodies.
 int rp3dg_batch::upload_joints(rp3dg_world* const* ws, int nw) {
-    struct Row { int kind, color, order, b1, b2; float4 la1, la2, ax1, ax2, qinit, lim; int flags; };
+    struct Row { int kind, color, order, b1, b2, world, slot; float4 la1, la2, ax1, ax2, qinit, lim; int flags; };
     std::vector<Row> rows;
     int jbase = 0;
     for (int wi = 0; wi < nw; wi++) {
@@ -236,6 +248,7 @@ int rp3dg_batch::upload_joints(rp3dg_world* const* ws, int nw) {
             else if (hj.type == RP3DG_JOINT_HINGE) r.kind = JT_HINGE;
             else { set_error("slider joints are not on the device path"); return RP3DG_EUNSUPPORTED; }
             r.order = jbase + (int)ji;
+            r.world = wi;
             r.b1 = body_base[wi] + hj.body1;
             r.b2 = body_base[wi] + hj.body2;
             const Tf& t1 = W.bodies[hj.body1].transform;
@@ -280,18 +293,36 @@ int rp3dg_batch::upload_joints(rp3dg_world* const* ws, int nw) {
             used[r.b2] |= 1ull << c;
         }
     }
-    std::stable_sort(rows.begin(), rows.end(), [](const Row& a, const Row& b) {
-        if (a.kind != b.kind) return a.kind < b.kind;
-        if (a.color != b.color) return a.color < b.color;
+    // slots = the (kind, colour) classes present, in solve order
+    std::vector<int> slot_of(3 * 64, -1);
+    for (const Row& r : rows) slot_of[r.kind * 64 + r.color] = 0;
+    jt_slots = 0;
+    for (int k = 0; k < 3 * 64; k++)
+        if (slot_of[k] >= 0) slot_of[k] = jt_slots++;
+    for (Row& r : rows) r.slot = slot_of[r.kind * 64 + r.color];
+    const bool wmajor = fused;
+    std::stable_sort(rows.begin(), rows.end(), [wmajor](const Row& a, const Row& b) {
+        if (wmajor && a.world != b.world) return a.world < b.world;
+        if (a.slot != b.slot) return a.slot < b.slot;
         return a.order < b.order;
     });
     const int n = (int)rows.size();
-    std::vector<int> ranges;
-    for (int i = 0; i < n; i++) {
-        if (i == 0 || rows[i].kind != rows[i - 1].kind || rows[i].color != rows[i - 1].color) ranges.push_back(i);
-        if (i == n - 1 || rows[i + 1].kind != rows[i].kind || rows[i + 1].color != rows[i].color) ranges.push_back(i + 1);
+    std::vector<int> ranges;   // per-colour kernels: [2*slot] = [first, end) over all worlds
+    std::vector<int2> wr((size_t)nw * std::max(jt_slots, 1), make_int2(0, 0));   // fused: per world and slot
+    if (!wmajor) {
+        for (int i = 0; i < n; i++) {
+            if (i == 0 || rows[i].slot != rows[i - 1].slot) ranges.push_back(i);
+            if (i == n - 1 || rows[i + 1].slot != rows[i].slot) ranges.push_back(i + 1);
+        }
+    } else {
+        std::vector<int> cnt((size_t)nw * std::max(jt_slots, 1), 0);
+        for (const Row& r : rows) cnt[(size_t)r.world * jt_slots + r.slot]++;
+        int run = 0;
+        for (size_t k = 0; k < cnt.size(); k++) {
+            wr[k] = make_int2(run, run + cnt[k]);
+            run += cnt[k];
+        }
     }
-    jt_slots = (int)ranges.size() / 2;
     joint_order.resize(n);
     J.n = n;
     const size_t m = std::max(1, n);
@@ -302,7 +333,8 @@ int rp3dg_batch::upload_joints(rp3dg_world* const* ws, int nw) {
         *a = dalloc<float4>(this, m);
     J.imlm = dalloc<float>(this, m);
     jt_ranges = dalloc<int>(this, std::max<size_t>(2, ranges.size()));
-    if (!J.imlm || !jt_ranges) { set_error("device allocation failed (joints)"); return RP3DG_EHIP; }
+    jt_wr = dalloc<int2>(this, wr.size());
+    if (!J.imlm || !jt_ranges || !jt_wr) { set_error("device allocation failed (joints)"); return RP3DG_EHIP; }
     if (n == 0) return RP3DG_OK;
     std::vector<int> type(n), flags(n);
     std::vector<int2> bodies(n);
@@ -324,12 +356,21 @@ int rp3dg_batch::upload_joints(rp3dg_world* const* ws, int nw) {
     HIPCHECK(hipMemcpy(J.lim, lim.data(), 16 * n, hipMemcpyHostToDevice));
     for (float4* a : {J.impT, J.impR, J.impL, J.a1, J.b2xa1, J.c2xa1}) HIPCHECK(hipMemcpy(a, zero.data(), 16 * n, hipMemcpyHostToDevice));
     HIPCHECK(hipMemset(J.imlm, 0, sizeof(float) * n));
-    HIPCHECK(hipMemcpy(jt_ranges, ranges.data(), sizeof(int) * ranges.size(), hipMemcpyHostToDevice));
+    if (!ranges.empty()) HIPCHECK(hipMemcpy(jt_ranges, ranges.data(), sizeof(int) * ranges.size(), hipMemcpyHostToDevice));
+    HIPCHECK(hipMemcpy(jt_wr, wr.data(), sizeof(int2) * wr.size(), hipMemcpyHostToDevice));
     return RP3DG_OK;
 }
 
-int rp3dg_batch::upload(rp3dg_world* const* ws, int nw) {
+int rp3dg_batch::upload(rp3dg_world* const* ws, int nw, const rp3dg_batch_options* opt) {
     worlds.assign(ws, ws + nw);
+    rp3dg_batch_options o{};
+    if (opt) o = *opt;
+    if (o.solver < RP3DG_SOLVER_AUTO || o.solver > RP3DG_SOLVER_FUSED) { set_error("invalid solver layout"); return RP3DG_EINVAL; }
+    if (o.fused_block && (o.fused_block < 64 || o.fused_block > kFusedMaxBlock || o.fused_block % 64)) {
+        set_error("fused_block must be a multiple of 64 in [64, 256]");
+        return RP3DG_EINVAL;
+    }
+    if (o.pairs_per_collider < 0) { set_error("pairs_per_collider must be >= 0"); return RP3DG_EINVAL; }
     // host SoA
     std::vector<float4> x, q, p, v, w, force, torque, ii, com, llock, alock;
     std::vector<int> flags, bworld;
@@ -445,6 +486,13 @@ int rp3dg_batch::upload(rp3dg_world* const* ws, int nw) {
     nworlds = nw;
     world_bits = 0;
     while ((1 << world_bits) < nw) world_bits++;
+    body_base.push_back(nb);
+    collider_base.push_back(nc);
+    max_world_bodies = 0;
+    for (int wi = 0; wi < nw; wi++) max_world_bodies = std::max(max_world_bodies, body_base[wi + 1] - body_base[wi]);
+    fused = o.solver == RP3DG_SOLVER_FUSED || (o.solver == RP3DG_SOLVER_AUTO && max_world_bodies <= 4096);
+    if (o.fused_block) fused_block = o.fused_block;
+    else fused_block = max_world_bodies <= 128 ? 64 : (max_world_bodies <= 1024 ? 128 : 256);
 
     HIPCHECK(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
     B.n = nb;
@@ -515,7 +563,7 @@ int rp3dg_batch::upload(rp3dg_world* const* ws, int nw) {
     upload_box_topology(e4.data(), fe.data(), fv.data());
 
     // broadphase / pair / solver arenas
-    pair_cap = std::max(4096, 16 * nc);
+    pair_cap = std::max(4096, (o.pairs_per_collider ? o.pairs_per_collider : 16) * nc);
     alloc_pairs(this, Pa, pair_cap);
     alloc_pairs(this, Pb, pair_cap);
     const int nn = std::max(1, 2 * nc);
@@ -536,7 +584,32 @@ int rp3dg_batch::upload(rp3dg_world* const* ws, int nw) {
     color = dalloc<int>(this, pair_cap);
     ckeys = dalloc<uint64_t>(this, pair_cap); cvals = dalloc<int>(this, pair_cap);
     cktmp = dalloc<uint64_t>(this, pair_cap); cvtmp = dalloc<int>(this, pair_cap);
-    cstart = dalloc<int>(this, kMaxColors + 1);
+    nseg = fused ? nw * kMaxColors : kMaxColors;
+    seg_cnt = dalloc<int>(this, nseg + 1);
+    seg_start = dalloc<int>(this, nseg + 1);
+    pair_count = dalloc<int>(this, 2);
+    d_wbody = dalloc<int>(this, nw + 1);
+    d_wcoll = dalloc<int>(this, nw + 1);
+    d_block_world = dalloc<int>(this, nw);
+    res_pv = dalloc<float4>(this, nb);
+    res_pw = dalloc<float4>(this, nb);
+    world_iters = dalloc<int>(this, nw);
+    world_res = dalloc<float>(this, nw);
+    if (!world_res) { set_error("device allocation failed (world tables)"); return RP3DG_EHIP; }
+    {
+        std::vector<int> order(nw);
+        for (int wi = 0; wi < nw; wi++) order[wi] = wi;
+        // largest worlds first, so the longest workgroups start first
+        std::stable_sort(order.begin(), order.end(), [&](int a, int c) {
+            return body_base[a + 1] - body_base[a] > body_base[c + 1] - body_base[c];
+        });
+        HIPCHECK(up(d_wbody, body_base.data(), sizeof(int) * (nw + 1)));
+        HIPCHECK(up(d_wcoll, collider_base.data(), sizeof(int) * (nw + 1)));
+        HIPCHECK(hipMemcpy(d_block_world, order.data(), sizeof(int) * nw, hipMemcpyHostToDevice));
+        HIPCHECK(hipMemsetAsync(pair_count, 0, sizeof(int) * 2, stream));
+        HIPCHECK(hipMemsetAsync(world_iters, 0, sizeof(int) * nw, stream));
+        HIPCHECK(hipMemsetAsync(world_res, 0, sizeof(float) * nw, stream));
+    }
     S.pair = dalloc<int>(this, pair_cap); S.bodies = dalloc<int2>(this, pair_cap); S.npts = dalloc<int>(this, pair_cap);
     for (float4** a : {&S.n, &S.r1f, &S.r2f, &S.t1, &S.t2, &S.r1ct1, &S.r1ct2, &S.r2ct1, &S.r2ct2, &S.imp, &S.oldt1, &S.oldt2})
         *a = dalloc<float4>(this, pair_cap);
@@ -544,6 +617,10 @@ int rp3dg_batch::upload(rp3dg_world* const* ws, int nw) {
     scan_sums_cap = 1 << 16;
     scan_sums_buf = dalloc<int>(this, scan_sums_cap);
     sort_hist_cap = 256 * (std::max(pair_cap, nc) / 4096 + 2);
+    if ((sort_hist_cap + kScanTile - 1) / kScanTile > scan_sums_cap || (pair_cap + kScanTile - 1) / kScanTile > scan_sums_cap) {
+        set_error("batch too large for the scan workspace");
+        return RP3DG_EOVERFLOW;
+    }
     sort_hist = dalloc<int>(this, sort_hist_cap);
     sort_offs = dalloc<int>(this, sort_hist_cap + 1);
     HIPCHECK(hipMalloc(&counters, sizeof(DevCounters)));
@@ -551,7 +628,6 @@ int rp3dg_batch::upload(rp3dg_world* const* ws, int nw) {
     HIPCHECK(hipHostMalloc(&hcounters, sizeof(DevCounters), hipHostMallocDefault));
     HIPCHECK(hipMemsetAsync(counters, 0, sizeof(DevCounters), stream));
     if (!sort_offs || !S.lam) { set_error("device allocation failed (pair arenas)"); return RP3DG_EHIP; }
-    nb_pairs = 0;
     // collider creation: world transforms, initial fat AABBs, all colliders in the moved set
     bp_refit<<<grid_for(nc), 256, 0, stream>>>(B, C, d_hbounds, 1);
     HIPCHECK(hipGetLastError());
@@ -611,12 +687,18 @@ int rp3dg_batch::upload(rp3dg_world* const* ws, int nw) {
         timer_end();                                         \
     } while (0)
 
-int rp3dg_batch::step(float dt) {
-    if (nb == 0) return RP3DG_OK;
+// Enqueues one full step on the batch's stream with no host synchronisation (the per-colour solver
+// layout reads the colour count back once, so only the fused layout can be captured).  Every
+// data-dependent size lives in device memory; kernels loop grid-stride over it, so the grids below
+// are performance hints only.
+int rp3dg_batch::enqueue_step(float dt, bool capturing) {
     DevPairs& O = cur_is_a ? Pa : Pb;
     DevPairs& N = cur_is_a ? Pb : Pa;
-    HIPCHECK(hipMemsetAsync(counters, 0, sizeof(DevCounters), stream));
+    int* old_count = pair_count + (cur_is_a ? 0 : 1);
+    int* new_count = pair_count + (cur_is_a ? 1 : 0);
     int* dstatus = &counters->status;
+    int r;
+    fill_int<<<1, 256, 0, stream>>>((int*)counters, (int)(sizeof(DevCounters) / sizeof(int)), 0);
 
     // ---------------- broadphase (CollisionDetectionSystem::computeBroadPhase, CD.cpp:103-121)
     const int nparts = std::min(256, grid_for(nc));
@@ -625,33 +707,26 @@ int rp3dg_batch::step(float dt) {
     uint64_t* k = bp_keys;
     int* vls = bp_vals;
     timer_begin("bp_sort");
-    int r = sort(&k, &vls, bp_ktmp, bp_vtmp, nc, 30 + world_bits, stream);
+    r = sort(&k, &vls, bp_ktmp, bp_vtmp, nc, nullptr, 30 + world_bits, stream);
     timer_end();
     if (r) { set_error("radix sort overflow"); return r; }
     if (nc > 1) {
-        LAUNCH_T("bp_karras", grid_for(nc - 1), 256, bp_karras, bp_keys, nc, bp_child, bp_parent, bp_range);
-        HIPCHECK(hipMemsetAsync(bp_flags, 0, sizeof(int) * nc, stream));
+        LAUNCH_T("bp_karras", grid_for(nc - 1), 256, bp_karras, k, nc, bp_child, bp_parent, bp_range);
+        fill_int<<<grid_for(nc), 256, 0, stream>>>(bp_flags, nc, 0);
     }
-    LAUNCH_T("bp_fit", grid_for(nc), 256, bp_fit, C, bp_vals, nc, bp_child, bp_parent, bp_nmin, bp_nmax, bp_flags);
-    LAUNCH_T("bp_world_roots", grid_for(2 * nc - 1), 256, bp_world_roots, bp_keys, nc, bp_range, bp_root);
-    LAUNCH_T("bp_traverse", grid_for(nc), 256, bp_traverse, C, bp_vals, nc, bp_child, bp_nmin, bp_nmax, bp_root,
+    LAUNCH_T("bp_fit", grid_for(nc), 256, bp_fit, C, vls, nc, bp_child, bp_parent, bp_nmin, bp_nmax, bp_flags);
+    LAUNCH_T("bp_world_roots", grid_for(2 * nc - 1), 256, bp_world_roots, k, nc, bp_range, bp_root);
+    LAUNCH_T("bp_traverse", grid_for(nc), 256, bp_traverse, C, vls, nc, bp_child, bp_nmin, bp_nmax, bp_root,
              d_nocol, nnocol, bp_count, bp_offs, bp_out, pair_cap, 0, dstatus);
-    r = scan(bp_count, nc, bp_offs, stream);
+    r = scan(bp_count, nc, nullptr, bp_offs, stream);
     if (r) { set_error("scan overflow"); return r; }
-    int total = 0;
-    HIPCHECK(hipMemcpyAsync(&hcounters->nb_overlaps, bp_offs + nc, sizeof(int), hipMemcpyDeviceToHost, stream));
-    HIPCHECK(hipStreamSynchronize(stream));
-    total = hcounters->nb_overlaps;
-    if (total > pair_cap) {
-        set_error("overlapping pair arena overflow (" + std::to_string(total) + " > " + std::to_string(pair_cap) + ")");
-        return RP3DG_EOVERFLOW;
-    }
-    LAUNCH_T("bp_traverse", grid_for(nc), 256, bp_traverse, C, bp_vals, nc, bp_child, bp_nmin, bp_nmax, bp_root,
+    LAUNCH_T("bp_traverse", grid_for(nc), 256, bp_traverse, C, vls, nc, bp_child, bp_nmin, bp_nmax, bp_root,
              d_nocol, nnocol, bp_count, bp_offs, bp_out, pair_cap, 1, dstatus);
-    LAUNCH_T("bp_merge", grid_for(std::max(total, 1)), 256, bp_merge, C, bp_out, bp_offs + nc, O, nb_pairs, N);
-    nb_pairs = total;
+    const int hp = capturing ? pair_cap : std::min(pair_cap, hint_pairs + hint_pairs / 4 + 1024);
+    const int pg = grid_for(hp);
+    LAUNCH_T("bp_merge", pg, 256, bp_merge, C, bp_out, bp_offs + nc, pair_cap, O, old_count, N, new_count,
+             &counters->nb_pairs);
     cur_is_a = !cur_is_a;
-    HIPCHECK(hipMemcpyAsync(&counters->nb_pairs, bp_offs + nc, sizeof(int), hipMemcpyDeviceToDevice, stream));
 
     // ---------------- narrowphase (CD.cpp:618-718)
     NpArgs A;
@@ -661,115 +736,201 @@ int rp3dg_batch::step(float dt) {
     A.hull_face_edge = d_hfedge; A.hull_face_verts = d_hfverts; A.hull_face_vidx = d_hfvidx;
     A.persistent_threshold = settings.persistent_threshold;
     A.cos_similar = settings.cos_similar;
-    const int pg = grid_for(std::max(total, 1));
     LAUNCH_T("np_classify", pg, 256, np_classify, A, np_lists, pair_cap, counters->nb_tests);
     int* L = np_lists;
     int* T = counters->nb_tests;
-    LAUNCH_T("np_sphere_sphere", pg, 256, np_sphere_sphere, A, L + ALGO_SPHERE_SPHERE * pair_cap, T + ALGO_SPHERE_SPHERE);
-    LAUNCH_T("np_sphere_capsule", pg, 256, np_sphere_capsule, A, L + ALGO_SPHERE_CAPSULE * pair_cap, T + ALGO_SPHERE_CAPSULE);
-    LAUNCH_T("np_capsule_capsule", pg, 256, np_capsule_capsule, A, L + ALGO_CAPSULE_CAPSULE * pair_cap, T + ALGO_CAPSULE_CAPSULE);
-    LAUNCH_T("np_sphere_poly", pg, 256, np_sphere_poly, A, L + ALGO_SPHERE_POLY * pair_cap, T + ALGO_SPHERE_POLY);
-    LAUNCH_T("np_capsule_poly", pg, 256, np_capsule_poly, A, L + ALGO_CAPSULE_POLY * pair_cap, T + ALGO_CAPSULE_POLY);
-    LAUNCH_T("np_poly_poly", pg, 128, np_poly_poly, A, L + ALGO_POLY_POLY * pair_cap, T + ALGO_POLY_POLY);
+    auto tg = [&](int algo, int block) {
+        const int h = capturing ? pair_cap : std::min(pair_cap, last.nb_tests[algo] + last.nb_tests[algo] / 4 + 512);
+        return grid_for(h, block);
+    };
+    LAUNCH_T("np_sphere_sphere", tg(ALGO_SPHERE_SPHERE, 256), 256, np_sphere_sphere, A, L + ALGO_SPHERE_SPHERE * pair_cap, T + ALGO_SPHERE_SPHERE);
+    LAUNCH_T("np_sphere_capsule", tg(ALGO_SPHERE_CAPSULE, 256), 256, np_sphere_capsule, A, L + ALGO_SPHERE_CAPSULE * pair_cap, T + ALGO_SPHERE_CAPSULE);
+    LAUNCH_T("np_capsule_capsule", tg(ALGO_CAPSULE_CAPSULE, 256), 256, np_capsule_capsule, A, L + ALGO_CAPSULE_CAPSULE * pair_cap, T + ALGO_CAPSULE_CAPSULE);
+    LAUNCH_T("np_sphere_poly", tg(ALGO_SPHERE_POLY, 256), 256, np_sphere_poly, A, L + ALGO_SPHERE_POLY * pair_cap, T + ALGO_SPHERE_POLY);
+    LAUNCH_T("np_capsule_poly", tg(ALGO_CAPSULE_POLY, 256), 256, np_capsule_poly, A, L + ALGO_CAPSULE_POLY * pair_cap, T + ALGO_CAPSULE_POLY);
+    LAUNCH_T("np_poly_poly", tg(ALGO_POLY_POLY, 128), 128, np_poly_poly, A, L + ALGO_POLY_POLY * pair_cap, T + ALGO_POLY_POLY);
 
     // ---------------- manifold list + colouring (replaces createIslands, PhysicsWorld.cpp:695-866)
+    int* nm_ptr = &counters->nb_manifolds;
     LAUNCH_T("sv_flag", pg, 256, sv_flag_manifolds, N.ncp, &counters->nb_pairs, m_flag);
-    r = scan(m_flag, std::max(total, 1), m_offs, stream);
+    r = scan(m_flag, pair_cap, &counters->nb_pairs, m_offs, stream);
     if (r) { set_error("scan overflow"); return r; }
-    LAUNCH_T("sv_compact", pg, 256, sv_compact_manifolds, m_flag, m_offs, &counters->nb_pairs, N.ncp, m_list,
-             &counters->nb_manifolds, &counters->nb_points);
-    HIPCHECK(hipMemsetAsync(adj_cnt, 0, sizeof(int) * (nb + 1), stream));
-    HIPCHECK(hipMemsetAsync(adj_cursor, 0, sizeof(int) * nb, stream));
-    HIPCHECK(hipMemcpyAsync(&hcounters->nb_manifolds, &counters->nb_manifolds, sizeof(int), hipMemcpyDeviceToHost, stream));
-    HIPCHECK(hipStreamSynchronize(stream));
-    const int nm = hcounters->nb_manifolds;
-    const int mg = grid_for(std::max(nm, 1));
-    int ncol = 0;
-    if (nm > 0) {
-        LAUNCH_T("sv_adj_count", mg, 256, sv_adj_count, B, C, N, m_list, &counters->nb_manifolds, adj_cnt);
-        r = scan(adj_cnt, nb, adj_offs, stream);
-        if (r) { set_error("scan overflow"); return r; }
-        LAUNCH_T("sv_adj_fill", mg, 256, sv_adj_fill, B, C, N, m_list, &counters->nb_manifolds, adj_offs, adj_cursor, adj, mb);
-        HIPCHECK(hipMemsetAsync(color, 0xff, sizeof(int) * nm, stream));
-        int rounds = 0;
-        while (true) {
-            for (int k2 = 0; k2 < 4; k2++) {
-                HIPCHECK(hipMemsetAsync(&counters->nb_uncolored, 0, sizeof(int), stream));
-                LAUNCH_T("sv_color_round", mg, 256, sv_color_round, mb, &counters->nb_manifolds, adj_offs, adj, color,
-                         &counters->nb_uncolored, dstatus);
-                rounds++;
-            }
-            HIPCHECK(hipMemcpyAsync(&hcounters->nb_uncolored, &counters->nb_uncolored, sizeof(int), hipMemcpyDeviceToHost, stream));
-            HIPCHECK(hipStreamSynchronize(stream));
-            if (hcounters->nb_uncolored == 0) break;
-            if (rounds > 4096) { set_error("colouring did not converge"); return RP3DG_EOVERFLOW; }
-        }
-        color_rounds = rounds;
-        LAUNCH_T("sv_color_keys", mg, 256, sv_color_keys, color, &counters->nb_manifolds, ckeys, cvals);
-        uint64_t* ck = ckeys;
-        int* cv = cvals;
-        r = sort(&ck, &cv, cktmp, cvtmp, nm, 8, stream);
-        if (r) { set_error("sort overflow"); return r; }
-        LAUNCH_T("sv_color_ranges", mg, 256, sv_color_ranges, ckeys, &counters->nb_manifolds, cstart, &counters->nb_colors);
-        HIPCHECK(hipMemcpyAsync(&hcounters->nb_colors, &counters->nb_colors, sizeof(int), hipMemcpyDeviceToHost, stream));
-        HIPCHECK(hipStreamSynchronize(stream));
-        ncol = hcounters->nb_colors;
-    }
-    last_colors = ncol;
+    LAUNCH_T("sv_compact", pg, 256, sv_compact_manifolds, m_flag, m_offs, &counters->nb_pairs, N.ncp, m_list, nm_ptr,
+             &counters->nb_points);
+    const int hm = capturing ? pair_cap : std::min(pair_cap, hint_manifolds + hint_manifolds / 4 + 1024);
+    const int mg = grid_for(hm);
+    fill_int<<<grid_for(nb + 1), 256, 0, stream>>>(adj_cnt, nb + 1, 0);
+    fill_int<<<grid_for(nb), 256, 0, stream>>>(adj_cursor, nb, 0);
+    LAUNCH_T("sv_adj_count", mg, 256, sv_adj_count, B, C, N, m_list, nm_ptr, adj_cnt);
+    r = scan(adj_cnt, nb, nullptr, adj_offs, stream);
+    if (r) { set_error("scan overflow"); return r; }
+    LAUNCH_T("sv_adj_fill", mg, 256, sv_adj_fill, B, C, N, m_list, nm_ptr, adj_offs, adj_cursor, adj, mb);
+    fill_int_n<<<mg, 256, 0, stream>>>(color, pair_cap, nm_ptr, -1);
+    const int rounds = capturing ? graph_rounds : std::min(kColorRoundsMax, std::max(4, hint_rounds));
+    for (int q = 0; q < rounds; q++)
+        LAUNCH_T("sv_color_round", mg, 256, sv_color_round, mb, nm_ptr, adj_offs, adj, color,
+                 q ? &counters->color_left[q - 1] : nullptr, &counters->color_left[q], dstatus);
+    LAUNCH_T("sv_color_finish", 1, 1024, sv_color_finish, mb, nm_ptr, adj_offs, adj, color,
+             rounds ? &counters->color_left[rounds - 1] : nullptr, &counters->finish_rounds, dstatus);
+    fill_int<<<grid_for(nseg + 1), 256, 0, stream>>>(seg_cnt, nseg + 1, 0);
+    LAUNCH_T("sv_color_keys", mg, 256, sv_color_keys, color, nm_ptr, C, N, m_list, fused ? 1 : 0, ckeys, cvals, seg_cnt,
+             &counters->nb_colors);
+    uint64_t* ck = ckeys;
+    int* cv = cvals;
+    int sbits = 0;
+    while ((1 << sbits) < nseg) sbits++;
+    timer_begin("sv_color_sort");
+    r = sort(&ck, &cv, cktmp, cvtmp, pair_cap, nm_ptr, sbits, stream);
+    timer_end();
+    if (r) { set_error("sort overflow"); return r; }
+    sorted_keys = ck;
+    r = scan(seg_cnt, nseg, nullptr, seg_start, stream);
+    if (r) { set_error("scan overflow"); return r; }
 
-    // ---------------- inverse world inertia + velocity integration (PhysicsWorld.cpp:266-272)
-    const int bg = grid_for(nb);
-    LAUNCH_T("dyn_integrate_velocities", bg, 256, dyn_integrate_velocities, B, settings, dt);
-
-    // ---------------- contact solver (PhysicsWorld.cpp:321-345)
     SvArgs SA;
     SA.B = B; SA.C = C; SA.P = N; SA.S = S;
-    SA.nm = &counters->nb_manifolds;
+    SA.nm = nm_ptr;
     SA.mlist = m_list;
-    SA.order = cvals;
-    SA.cstart = cstart;
+    SA.order = cv;
+    SA.cstart = seg_start;
     SA.ncolors = &counters->nb_colors;
     SA.restitution_threshold = settings.restitution_threshold;
-    const int jg = grid_for(std::max(J.n, 1));
-    if (nm > 0) {
-        LAUNCH_T("sv_prepare", mg, 256, sv_prepare, SA);
-        for (int c = 0; c < ncol; c++) LAUNCH_T("sv_warmstart", mg, 256, sv_warmstart, SA, c);
+    const float beta_dt = 0.2f / dt;
+
+    if (fused) {
+        // ---------------- one workgroup per world: integration, contact + joint solver, positions, refit
+        FusedArgs F;
+        F.A = SA;
+        F.J = J;
+        F.settings = settings;
+        F.seg_start = seg_start;
+        F.jt_wr = jt_wr;
+        F.jt_slots = jt_slots;
+        F.wbody = d_wbody;
+        F.wcoll = d_wcoll;
+        F.block_world = d_block_world;
+        F.hull_bounds = d_hbounds;
+        F.dt = dt;
+        F.beta_dt = beta_dt;
+        F.res_thr = res_thr;
+        F.max_iters = res_thr > 0.0f ? res_max_iters : settings.vel_iters;
+        F.pv = res_pv;
+        F.pw = res_pw;
+        F.world_iters = world_iters;
+        F.world_res = world_res;
+        LAUNCH_T("sv_world_solve", nworlds, fused_block, sv_world_solve, F);
+        LAUNCH_T("world_position", nworlds, fused_block, world_position, F);
+        return RP3DG_OK;
     }
+
+    // ---------------- per-colour layout: the colour count decides the launches
+    if (capturing) { set_error("graph capture needs the fused solver layout"); return RP3DG_EUNSUPPORTED; }
+    HIPCHECK(hipMemcpyAsync(&hcounters->nb_colors, &counters->nb_colors, sizeof(int), hipMemcpyDeviceToHost, stream));
+    HIPCHECK(hipStreamSynchronize(stream));
+    const int ncol = hcounters->nb_colors;
+    const int bg = grid_for(nb);
+    // inverse world inertia + velocity integration (PhysicsWorld.cpp:266-272)
+    LAUNCH_T("dyn_integrate_velocities", bg, 256, dyn_integrate_velocities, B, settings, dt);
+    // contact solver (PhysicsWorld.cpp:321-345)
+    const int jg = grid_for(std::max(J.n, 1));
+    LAUNCH_T("sv_prepare", mg, 256, sv_prepare, SA);
+    for (int c = 0; c < ncol; c++) LAUNCH_T("sv_warmstart", mg, 256, sv_warmstart, SA, c);
     // ConstraintSolverSystem::initialize: initBeforeSolve + warm start of every joint kind
     if (J.n > 0) {
         LAUNCH_T("jt_init", jg, 256, jt_init, B, J);
         for (int s = 0; s < jt_slots; s++) LAUNCH_T("jt_warmstart", jg, 256, jt_warmstart, B, J, jt_ranges, s);
     }
-    const float beta_dt = 0.2f / dt;
     for (int it = 0; it < settings.vel_iters; it++) {
         for (int s = 0; s < jt_slots; s++)
             LAUNCH_T("jt_solve_velocity", jg, 256, jt_solve_velocity, B, J, jt_ranges, s, dt);
-        if (nm > 0)
-            for (int c = 0; c < ncol; c++) LAUNCH_T("sv_solve_color", mg, 256, sv_solve_color, SA, c, beta_dt);
+        for (int c = 0; c < ncol; c++) LAUNCH_T("sv_solve_color", mg, 256, sv_solve_color, SA, c, beta_dt);
     }
-    if (nm > 0) LAUNCH_T("sv_store", mg, 256, sv_store, SA);
-
-    // ---------------- position integration, joint position correction, state update, refit
-    // (PhysicsWorld.cpp:275-288, :348-360)
+    LAUNCH_T("sv_store", mg, 256, sv_store, SA);
+    // position integration, joint position correction, state update, refit (PhysicsWorld.cpp:275-288, :348-360)
     LAUNCH_T("dyn_integrate_positions", bg, 256, dyn_integrate_positions, B, dt);
     for (int it = 0; it < settings.pos_iters; it++)
         for (int s = 0; s < jt_slots; s++)
             LAUNCH_T("jt_solve_position", jg, 256, jt_solve_position, B, J, jt_ranges, s);
     LAUNCH_T("dyn_update_state", bg, 256, dyn_update_state, B);
     LAUNCH_T("bp_refit", grid_for(nc), 256, bp_refit, B, C, d_hbounds, 0);
+    return RP3DG_OK;
+}
+
+// Reads the counters of the last enqueued step back and checks the device status word.
+int rp3dg_batch::finish_steps(int n) {
     HIPCHECK(hipGetLastError());
     HIPCHECK(hipMemcpyAsync(hcounters, counters, sizeof(DevCounters), hipMemcpyDeviceToHost, stream));
     HIPCHECK(hipStreamSynchronize(stream));
     timer_flush();
-    if (hcounters->status) {
-        set_error("device status " + std::to_string(hcounters->status));
+    last = *hcounters;
+    steps_done += n;
+    if (last.status) {
+        std::string what;
+        if (last.status & STATUS_PAIR_OVERFLOW) what += " pair table overflow (raise pairs_per_collider);";
+        if (last.status & STATUS_COLOR_OVERFLOW) what += " more than 64 solver colours;";
+        if (last.status & STATUS_STACK_OVERFLOW) what += " broadphase traversal stack overflow;";
+        if (last.status & STATUS_COLOR_STALL) what += " colouring did not finish;";
+        set_error("device status " + std::to_string(last.status) + ":" + what);
         return RP3DG_EOVERFLOW;
     }
-    steps_done++;
+    hint_pairs = last.nb_pairs;
+    hint_manifolds = last.nb_manifolds;
+    int used = 0;
+    while (used < kColorRoundsMax && last.color_left[used] != 0) used++;
+    hint_rounds = used + 2;
     return RP3DG_OK;
 }
 
+int rp3dg_batch::step(float dt) {
+    if (nb == 0) return RP3DG_OK;
+    const int r = enqueue_step(dt, false);
+    if (r) return r;
+    return finish_steps(1);
+}
+
+void rp3dg_batch::drop_graphs() {
+    for (int p = 0; p < 2; p++) {
+        if (graph_exec[p]) hipGraphExecDestroy(graph_exec[p]);
+        if (graph[p]) hipGraphDestroy(graph[p]);
+        graph_exec[p] = nullptr;
+        graph[p] = nullptr;
+    }
+}
+
+// The pair tables alternate between steps, so two graphs are captured (one per parity) and replayed
+// alternately.  Capture happens once per time-step value; the colouring round count is frozen at
+// capture (the finisher completes any remainder).
+int rp3dg_batch::step_graph(float dt, int n) {
+    if (nb == 0 || n <= 0) return RP3DG_OK;
+    if (!fused) { set_error("graph capture needs the fused solver layout"); return RP3DG_EUNSUPPORTED; }
+    if (profiling) { set_error("disable kernel profiling before graph replay"); return RP3DG_ESTATE; }
+    if (graph_exec[0] && graph_dt != dt) drop_graphs();
+    if (!graph_exec[0]) {
+        graph_rounds = std::min(kColorRoundsMax, std::max(8, hint_rounds + 4));
+        const bool start = cur_is_a;
+        for (int p = 0; p < 2; p++) {
+            HIPCHECK(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
+            const int r = enqueue_step(dt, true);
+            hipGraph_t g = nullptr;
+            const hipError_t e = hipStreamEndCapture(stream, &g);
+            if (r) { if (g) hipGraphDestroy(g); cur_is_a = start; drop_graphs(); return r; }
+            HIPCHECK(e);
+            const int parity = start == (p == 0) ? 0 : 1;   // parity 0: a -> b
+            graph[parity] = g;
+            HIPCHECK(hipGraphInstantiate(&graph_exec[parity], g, nullptr, nullptr, 0));
+        }
+        cur_is_a = start;   // capture does not run anything; the flips above are undone
+        graph_dt = dt;
+    }
+    for (int i = 0; i < n; i++) {
+        HIPCHECK(hipGraphLaunch(graph_exec[cur_is_a ? 0 : 1], stream));
+        cur_is_a = !cur_is_a;
+    }
+    return finish_steps(n);
+}
+
 rp3dg_batch::~rp3dg_batch() {
+    drop_graphs();
     for (hipEvent_t ev : event_pool) hipEventDestroy(ev);
     for (void* p : allocs) hipFree(p);
     if (hcounters) hipHostFree(hcounters);
@@ -778,14 +939,20 @@ rp3dg_batch::~rp3dg_batch() {
 
 extern "C" {
 
-rp3dg_batch* rp3dg_batch_create(rp3dg_world* const* worlds, int32_t n) {
+rp3dg_batch* rp3dg_batch_create_ex(rp3dg_world* const* worlds, int32_t n, const rp3dg_batch_options* opt) {
     if (!worlds || n <= 0) { set_error("no worlds"); return nullptr; }
+    for (int i = 0; i < n; i++) {
+        if (!worlds[i]) { set_error("null world"); return nullptr; }
+        if (worlds[i]->uploaded) { set_error("a world belongs to a batch already"); return nullptr; }
+    }
     rp3dg_batch* b = new rp3dg_batch();
-    const int r = b->upload(worlds, n);
+    const int r = b->upload(worlds, n, opt);
     if (r) { delete b; return nullptr; }
     for (int i = 0; i < n; i++) worlds[i]->uploaded = true;
     return b;
 }
+
+rp3dg_batch* rp3dg_batch_create(rp3dg_world* const* worlds, int32_t n) { return rp3dg_batch_create_ex(worlds, n, nullptr); }
 
 void rp3dg_batch_destroy(rp3dg_batch* b) { delete b; }
 
@@ -828,12 +995,12 @@ int rp3dg_batch_get_counts(rp3dg_batch* b, rp3dg_counts* o) {
     o->nb_colliders = b->nc;
     o->nb_joints = b->J.n;
     o->nb_worlds = b->nworlds;
-    o->nb_pairs = b->nb_pairs;
-    o->nb_manifolds = b->hcounters->nb_manifolds;
-    o->nb_contact_points = b->hcounters->nb_points;
-    o->nb_colors = b->last_colors;
-    o->status = b->hcounters->status;
-    for (int a = 0; a < 7; a++) o->nb_tests[a] = b->hcounters->nb_tests[a];
+    o->nb_pairs = b->last.nb_pairs;
+    o->nb_manifolds = b->last.nb_manifolds;
+    o->nb_contact_points = b->last.nb_points;
+    o->nb_colors = b->last.nb_colors;
+    o->status = b->last.status;
+    for (int a = 0; a < 7; a++) o->nb_tests[a] = b->last.nb_tests[a];
     return RP3DG_OK;
 }
 
@@ -869,7 +1036,7 @@ int rp3dg_batch_set_bodies(rp3dg_batch* b, const float* pos, const float* ori, c
 int rp3dg_batch_get_pairs(rp3dg_batch* b, int32_t* c1, int32_t* c2, int64_t cap) {
     if (!b) return RP3DG_EINVAL;
     const DevPairs& P = b->cur_is_a ? b->Pa : b->Pb;
-    const int n = b->nb_pairs;
+    const int n = b->last.nb_pairs;
     if (cap < n) { set_error("capacity too small"); return RP3DG_EINVAL; }
     std::vector<int2> col(std::max(n, 1));
     HIPCHECK(hipStreamSynchronize(b->stream));
@@ -880,7 +1047,7 @@ int rp3dg_batch_get_pairs(rp3dg_batch* b, int32_t* c1, int32_t* c2, int64_t cap)
 
 int rp3dg_batch_get_contacts(rp3dg_batch* b, rp3dg_contact_view* o, int64_t cap) {
     if (!b || !o) return RP3DG_EINVAL;
-    const int nm = b->hcounters->nb_manifolds;
+    const int nm = b->last.nb_manifolds;
     if (cap < nm) { set_error("capacity too small"); return RP3DG_EINVAL; }
     if (nm == 0) return 0;
     const DevPairs& P = b->cur_is_a ? b->Pa : b->Pb;
@@ -888,8 +1055,8 @@ int rp3dg_batch_get_contacts(rp3dg_batch* b, rp3dg_contact_view* o, int64_t cap)
     std::vector<int> order(nm), mlist(nm), pairs(nm), colors(nm);
     std::vector<uint64_t> ck(nm);
     HIPCHECK(hipMemcpy(pairs.data(), b->S.pair, sizeof(int) * nm, hipMemcpyDeviceToHost));
-    HIPCHECK(hipMemcpy(ck.data(), b->ckeys, sizeof(uint64_t) * nm, hipMemcpyDeviceToHost));
-    const int np = b->nb_pairs;
+    HIPCHECK(hipMemcpy(ck.data(), b->sorted_keys, sizeof(uint64_t) * nm, hipMemcpyDeviceToHost));
+    const int np = b->last.nb_pairs;
     std::vector<int2> col(np);
     std::vector<int> ncp(np);
     std::vector<float4> pn(4 * (size_t)np), pl1(4 * (size_t)np), pl2(4 * (size_t)np), fimp(np);
@@ -903,7 +1070,7 @@ int rp3dg_batch_get_contacts(rp3dg_batch* b, rp3dg_contact_view* o, int64_t cap)
         const int k = pairs[m];
         if (o->colliders) { o->colliders[2 * m] = col[k].x; o->colliders[2 * m + 1] = col[k].y; }
         if (o->nb_points) o->nb_points[m] = ncp[k];
-        if (o->color) o->color[m] = (int)ck[m];
+        if (o->color) o->color[m] = (int)(ck[m] % kMaxColors);
         for (int p = 0; p < 4; p++) {
             const float4 a = pn[4 * k + p], c = pl1[4 * k + p], d = pl2[4 * k + p];
             if (o->normal_depth) { float* q = o->normal_depth + 16 * m + 4 * p; q[0] = a.x; q[1] = a.y; q[2] = a.z; q[3] = a.w; }
@@ -940,15 +1107,48 @@ int rp3dg_batch_get_kernel_times(rp3dg_batch* b, const char* name, double* mean_
     return RP3DG_OK;
 }
 
-int rp3dg_batch_step_graph(rp3dg_batch* b, float dt, int32_t n) { return rp3dg_batch_step(b, dt, n); }
+int rp3dg_batch_step_graph(rp3dg_batch* b, float dt, int32_t n) {
+    if (!b) return RP3DG_EINVAL;
+    if (!(dt > 0.f)) { set_error("time step must be > 0"); return RP3DG_EINVAL; }
+    return b->step_graph(dt, n);
+}
 
 int rp3dg_batch_set_residual_mode(rp3dg_batch* b, float thr, int32_t maxit) {
-    (void)b; (void)thr; (void)maxit;
-    set_error("residual mode not implemented yet");
-    return RP3DG_EUNSUPPORTED;
+    if (!b) return RP3DG_EINVAL;
+    if (thr > 0.0f) {
+        if (!b->fused) { set_error("residual mode needs the fused solver layout"); return RP3DG_EUNSUPPORTED; }
+        if (maxit < 1 || maxit > 10000) { set_error("max_iterations must be in [1, 10000]"); return RP3DG_EINVAL; }
+        b->res_thr = thr;
+        b->res_max_iters = maxit;
+    } else {
+        b->res_thr = 0.0f;
+        b->res_max_iters = 0;
+    }
+    b->drop_graphs();
+    return RP3DG_OK;
 }
-int rp3dg_batch_get_iteration_stats(rp3dg_batch* b, double* mean_it, double* max_res) {
-    (void)b; if (mean_it) *mean_it = 0; if (max_res) *max_res = 0;
+
+int rp3dg_batch_get_iteration_stats(rp3dg_batch* b, double* mean_it, double* max_it, double* max_res) {
+    if (!b) return RP3DG_EINVAL;
+    double sum = 0.0, mx = 0.0, mr = 0.0;
+    if (b->fused) {
+        std::vector<int> it(b->nworlds);
+        std::vector<float> res(b->nworlds);
+        HIPCHECK(hipStreamSynchronize(b->stream));
+        HIPCHECK(hipMemcpy(it.data(), b->world_iters, sizeof(int) * b->nworlds, hipMemcpyDeviceToHost));
+        HIPCHECK(hipMemcpy(res.data(), b->world_res, sizeof(float) * b->nworlds, hipMemcpyDeviceToHost));
+        for (int w = 0; w < b->nworlds; w++) {
+            sum += it[w];
+            mx = std::max(mx, (double)it[w]);
+            mr = std::max(mr, (double)res[w]);
+        }
+        sum /= std::max(1, b->nworlds);
+    } else {
+        sum = mx = b->settings.vel_iters;
+    }
+    if (mean_it) *mean_it = sum;
+    if (max_it) *max_it = mx;
+    if (max_res) *max_res = mr;
     return RP3DG_OK;
 }
 

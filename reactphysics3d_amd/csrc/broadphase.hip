@@ -18,91 +18,12 @@
 
 #include "kcommon.h"
 #include "rp3d_internal.h"
+#include "refit_dev.h"
 
 namespace rp3dg {
 
-// ---------------------------------------------------------------------------------------------
-// Collider world transforms and fat AABB refit (DynamicsSystem.cpp:104-109, BroadPhaseSystem.cpp
-// :155-185, CollisionShape.cpp:54-100, SphereShape.cpp:49-58, DynamicAABBTree.cpp:146-214).
-// ---------------------------------------------------------------------------------------------
-
-__device__ inline void local_bounds(int type, float4 s, const float4* hull_bounds, V3& lo, V3& hi) {
-    if (type == SHAPE_CAPSULE) {  // CapsuleShape.cpp:90-94
-        lo = v3(-s.x, -s.y - s.x, -s.x);
-        hi = v3(s.x, s.y + s.x, s.x);
-    } else if (type == SHAPE_BOX) {  // BoxShape.cpp:149-151
-        lo = v3(-s.x, -s.y, -s.z);
-        hi = v3(s.x, s.y, s.z);
-    } else {  // ConvexMeshShape.cpp:208-212 (scale 1)
-        const int h = (int)s.x;
-        lo = xyz(hull_bounds[2 * h]);
-        hi = xyz(hull_bounds[2 * h + 1]);
-    }
-}
-
-__device__ inline void transformed_aabb(int type, float4 s, const float4* hull_bounds, const Tf& t, V3& mn, V3& mx) {
-    if (type == SHAPE_SPHERE) {  // SphereShape::computeTransformedAABB
-        const V3 e = v3(s.x, s.x, s.x);
-        mn = t.p - e;
-        mx = t.p + e;
-        return;
-    }
-    V3 lo, hi;
-    local_bounds(type, s, hull_bounds, lo, hi);
-    const M3 m = qmat(t.q);
-    for (int i = 0; i < 3; i++) {
-        float rmin = comp(t.p, i), rmax = comp(t.p, i);
-        for (int j = 0; j < 3; j++) {
-            const float e = mel(m, i, j) * comp(lo, j);
-            const float f = mel(m, i, j) * comp(hi, j);
-            if (e < f) { rmin += e; rmax += f; }
-            else { rmin += f; rmax += e; }
-        }
-        setcomp(mn, i, rmin);
-        setcomp(mx, i, rmax);
-    }
-}
-
-__device__ inline void fatten(V3 mn, V3 mx, V3& fmn, V3& fmx) {
-    // DynamicAABBTree.cpp:153 / :203: gap = extent * 0.08 * 0.5
-    const V3 ext = mx - mn;
-    const V3 gap = (ext * 0.08f) * 0.5f;
-    fmn = mn - gap;
-    fmx = mx + gap;
-}
-
-// init = 1: collider creation (RigidBody::addCollider -> BroadPhaseSystem::addCollider), every collider
-// gets a fat AABB and enters the moved set.  init = 0: end-of-step refit of enabled colliders.
 __global__ void bp_refit(DevBodies B, DevColliders C, const float4* hull_bounds, int init) {
-    RP3DG_GRID_STRIDE(i, C.n) {
-        const int b = C.body[i];
-        const int bf = B.flags[b];
-        int moved = 0;
-        if (init || (bf & BF_ENABLED)) {
-            const Tf bt = tf(ld3(B.p, b), ldq(B.q, b));
-            const Tf lt = tf(ld3(C.lpos, i), ldq(C.lq, i));
-            const Tf wt = tmul(bt, lt);
-            C.wpos[i] = f4(wt.p, 0.f);
-            C.wq[i] = f4q(wt.q);
-            V3 mn, mx;
-            transformed_aabb(C.type[i], C.shape[i], hull_bounds, wt, mn, mx);
-            bool inside = false;
-            if (!init) {
-                const V3 fmn = ld3(C.amin, i), fmx = ld3(C.amax, i);
-                // AABB::contains(const AABB&) (AABB.h:246-256)
-                inside = fmn.x <= mn.x && fmn.y <= mn.y && fmn.z <= mn.z && fmx.x >= mx.x && fmx.y >= mx.y &&
-                         fmx.z >= mx.z;
-            }
-            if (!inside) {
-                V3 fmn, fmx;
-                fatten(mn, mx, fmn, fmx);
-                C.amin[i] = f4(fmn, 0.f);
-                C.amax[i] = f4(fmx, 0.f);
-                moved = 1;
-            }
-        }
-        C.moved[i] = moved;
-    }
+    RP3DG_GRID_STRIDE(i, C.n) refit_one(B, C, hull_bounds, init, i);
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -186,7 +107,10 @@ constexpr int kSortThreads = 256;
 constexpr int kSortItems = 16;
 constexpr int kSortTile = kSortThreads * kSortItems;
 
-__global__ __launch_bounds__(kSortThreads) void rs_hist(const uint64_t* keys, int n, int shift, int* hist, int nblocks) {
+// n: key count, or its upper bound when nptr (the count in device memory) is given.
+__global__ __launch_bounds__(kSortThreads) void rs_hist(const uint64_t* keys, int n, const int* nptr, int shift, int* hist,
+                                                        int nblocks) {
+    if (nptr) n = min(n, *nptr);
     __shared__ int h[256];
     h[threadIdx.x] = 0;
     __syncthreads();
@@ -205,7 +129,9 @@ __device__ __forceinline__ uint64_t lanemask_lt() {
 }
 
 __global__ __launch_bounds__(kSortThreads) void rs_scatter(const uint64_t* kin, const int* vin, uint64_t* kout,
-                                                           int* vout, int n, int shift, const int* offs, int nblocks) {
+                                                           int* vout, int n, const int* nptr, int shift, const int* offs,
+                                                           int nblocks) {
+    if (nptr) n = min(n, *nptr);
     __shared__ int base[256];
     __shared__ int wcnt[kSortThreads / kWave][256];
     const int t = threadIdx.x;
@@ -383,9 +309,11 @@ __global__ void bp_traverse(DevColliders C, const int* sorted, int n, const int2
         const int base = pass ? offset[c] : 0;
         if (C.is_static[c]) {
             const int b = C.ss_offs[c], e = C.ss_offs[c + 1];
-            if (pass)
+            if (pass) {
                 for (int k = b; k < e; k++)
                     if (base + k - b < capacity) out_keys[base + k - b] = ((uint64_t)(uint32_t)c << 32) | (uint32_t)C.ss_list[k];
+                if (base + e - b > capacity) atomicOr(status, STATUS_PAIR_OVERFLOW);
+            }
             if (!pass) count[c] = e - b;
             continue;
         }
@@ -412,6 +340,8 @@ __global__ void bp_traverse(DevColliders C, const int* sorted, int n, const int2
                             pos--;
                         }
                         out_keys[pos] = key;
+                    } else {
+                        atomicOr(status, STATUS_PAIR_OVERFLOW);
                     }
                 }
                 cnt++;
@@ -446,8 +376,16 @@ __device__ __forceinline__ int select_algo(int t1, int t2) {
     return ALGO_POLY_POLY;
 }
 
-__global__ void bp_merge(DevColliders C, const uint64_t* keys, const int* nkeys_ptr, DevPairs O, int nold, DevPairs N) {
-    const int nkeys = *nkeys_ptr;
+// nkeys_ptr: overlapping pairs found (clamped to the capacity; an overflow is flagged by the traversal).
+// nold_ptr: size of the previous table O.  nnew_ptr / nb_pairs_ptr receive the size of the new table N.
+__global__ void bp_merge(DevColliders C, const uint64_t* keys, const int* nkeys_ptr, int capacity, DevPairs O,
+                         const int* nold_ptr, DevPairs N, int* nnew_ptr, int* nb_pairs_ptr) {
+    const int nkeys = min(*nkeys_ptr, capacity);
+    const int nold = *nold_ptr;
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        *nnew_ptr = nkeys;
+        *nb_pairs_ptr = nkeys;
+    }
     RP3DG_GRID_STRIDE(k, nkeys) {
         const uint64_t key = keys[k];
         N.key[k] = key;

@@ -1,0 +1,34 @@
+// Argument block of the fused per-world kernels (fused.hip).
+#pragma once
+
+#include "joints.h"
+#include "rp3d_internal.h"
+#include "solver.h"
+
+namespace rp3dg {
+
+struct FusedArgs {
+    SvArgs A;                  // rows in (world, colour)-major order; A.cstart unused
+    DevJoints J;               // rows sorted by (world, kind, colour, creation index)
+    Settings settings;
+    const int* seg_start;      // [w * kMaxColors + c]: first row of colour c of world w; [nw * kMaxColors] = total
+    const int2* jt_wr;         // [w * jt_slots + s]: joint rows of slot s (kind, colour) of world w
+    int jt_slots;
+    const int* wbody;          // [nw + 1] body ranges of the worlds
+    const int* wcoll;          // [nw + 1] collider ranges of the worlds
+    const int* block_world;    // workgroup -> world (largest worlds first)
+    const float4* hull_bounds;
+    float dt, beta_dt;
+    int max_iters;             // velocity iterations (fixed), or the cap in residual mode
+    float res_thr;             // > 0: iterate to this velocity residual
+    float4 *pv, *pw;           // velocities after the previous sweep (residual mode)
+    int* world_iters;          // iterations each world ran in the last step
+    float* world_res;          // its last residual (residual mode)
+};
+
+constexpr int kFusedMaxBlock = 256;   // threads per world at most (register budget of the fused kernels)
+
+__global__ void sv_world_solve(FusedArgs F);
+__global__ void world_position(FusedArgs F);
+
+}  // namespace rp3dg

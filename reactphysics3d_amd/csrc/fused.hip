@@ -1,0 +1,147 @@
+// Fused per-world solver for batches of many small worlds.
+//
+// One workgroup owns one world for the whole solver phase of a step: velocity
+// integration of its bodies, contact rows, joint rows, warm start, every
+// velocity iteration (colours and joint slots separated by workgroup
+// barriers), impulse store; a second kernel does position integration, the
+// joint position iterations, the state update and the fat-AABB refit.  Worlds
+// share nothing, so no grid-wide synchronisation and no per-colour launches are
+// needed; every world runs its sweeps in the same order as the per-colour
+// kernels (joint slots, then contact colours), so results are identical.
+//
+// In residual mode a world iterates until the velocity residual
+//   r_k = max over its dynamic bodies of max(|dv|_inf, |dw|_inf) between sweeps
+// drops below the threshold (or the iteration cap is reached); this mode has
+// no counterpart in the reference, which always runs a fixed iteration count
+// (PhysicsWorld.cpp:334).
+#include <hip/hip_runtime.h>
+
+#include "fused.h"
+#include "joints_dev.h"
+#include "kcommon.h"
+#include "refit_dev.h"
+#include "solver_dev.h"
+
+namespace rp3dg {
+
+__device__ __forceinline__ float block_maxf(float v, float* red) {
+    v = wave_maxf(v);
+    const int wid = threadIdx.x / kWave;
+    const int nwv = blockDim.x / kWave;
+    if (lane_id() == 0) red[wid] = v;
+    __syncthreads();
+    float r = red[0];
+    for (int k = 1; k < nwv; k++) r = red[k] > r ? red[k] : r;
+    __syncthreads();
+    return r;
+}
+
+__global__ __launch_bounds__(kFusedMaxBlock) void sv_world_solve(FusedArgs F) {
+    __shared__ float red[16];
+    const SvArgs& A = F.A;
+    const DevBodies& B = A.B;
+    const int w = F.block_world[blockIdx.x];
+    const int tid = threadIdx.x, nt = blockDim.x;
+    const int b0 = F.wbody[w], b1 = F.wbody[w + 1];
+    const int* seg = F.seg_start + (size_t)w * kMaxColors;
+    const int m0 = seg[0], m1 = seg[kMaxColors];
+    const int2* jr = F.jt_wr + (size_t)w * F.jt_slots;
+    const int j0 = F.jt_slots ? jr[0].x : 0, j1 = F.jt_slots ? jr[F.jt_slots - 1].y : 0;
+
+    // PhysicsWorld.cpp:266-272: inverse world inertia tensors and velocity integration
+    for (int i = b0 + tid; i < b1; i += nt) dyn_integrate_velocities_one(B, F.settings, F.dt, i);
+    __syncthreads();
+    // ContactSolverSystem::init (rows); ConstraintSolverSystem::initialize (initBeforeSolve)
+    for (int i = m0 + tid; i < m1; i += nt) sv_prepare_one(A, i);
+    for (int j = j0 + tid; j < j1; j += nt) jt_init_one(B, F.J, j);
+    __syncthreads();
+    // contact warm start, colour by colour (CS.cpp:326-476)
+    for (int c = 0; c < kMaxColors; c++) {
+        const int lo = seg[c], hi = seg[c + 1];
+        if (lo == m1) break;
+        for (int i = lo + tid; i < hi; i += nt) sv_warmstart_one(A, i);
+        __syncthreads();
+    }
+    // joint warm start, slot by slot
+    for (int s = 0; s < F.jt_slots; s++) {
+        const int2 r = jr[s];
+        if (r.x == r.y) continue;
+        for (int j = r.x + tid; j < r.y; j += nt) jt_warmstart_one(B, F.J, j);
+        __syncthreads();
+    }
+    const bool residual = F.res_thr > 0.0f;
+    if (residual) {
+        for (int i = b0 + tid; i < b1; i += nt) {
+            F.pv[i] = B.cv[i];
+            F.pw[i] = B.cw[i];
+        }
+        __syncthreads();
+    }
+    int it = 0;
+    float res = 0.0f;
+    while (it < F.max_iters) {
+        for (int s = 0; s < F.jt_slots; s++) {
+            const int2 r = jr[s];
+            if (r.x == r.y) continue;
+            for (int j = r.x + tid; j < r.y; j += nt) jt_solve_velocity_one(B, F.J, j, F.dt);
+            __syncthreads();
+        }
+        for (int c = 0; c < kMaxColors; c++) {
+            const int lo = seg[c], hi = seg[c + 1];
+            if (lo == m1) break;
+            for (int i = lo + tid; i < hi; i += nt) solve_manifold(A, i, F.beta_dt);
+            __syncthreads();
+        }
+        it++;
+        if (residual) {
+            float r = 0.0f;
+            for (int i = b0 + tid; i < b1; i += nt) {
+                if (!dyn_body(B, i)) continue;
+                const float4 v = B.cv[i], wv = B.cw[i], pv = F.pv[i], pw = F.pw[i];
+                float d = fabsf(v.x - pv.x);
+                d = fmaxf(d, fabsf(v.y - pv.y));
+                d = fmaxf(d, fabsf(v.z - pv.z));
+                d = fmaxf(d, fabsf(wv.x - pw.x));
+                d = fmaxf(d, fabsf(wv.y - pw.y));
+                d = fmaxf(d, fabsf(wv.z - pw.z));
+                r = fmaxf(r, d);
+                F.pv[i] = v;
+                F.pw[i] = wv;
+            }
+            res = block_maxf(r, red);
+            if (res < F.res_thr) break;
+        }
+    }
+    // storeImpulses (CS.cpp:763-785)
+    for (int i = m0 + tid; i < m1; i += nt) sv_store_one(A, i);
+    if (tid == 0) {
+        F.world_iters[w] = it;
+        F.world_res[w] = res;
+    }
+}
+
+__global__ __launch_bounds__(kFusedMaxBlock) void world_position(FusedArgs F) {
+    const DevBodies& B = F.A.B;
+    const int w = F.block_world[blockIdx.x];
+    const int tid = threadIdx.x, nt = blockDim.x;
+    const int b0 = F.wbody[w], b1 = F.wbody[w + 1];
+    const int c0 = F.wcoll[w], c1 = F.wcoll[w + 1];
+    const int2* jr = F.jt_wr + (size_t)w * F.jt_slots;
+    // PhysicsWorld.cpp:275-288: integrate positions, joint position correction, update body state
+    for (int i = b0 + tid; i < b1; i += nt) dyn_integrate_positions_one(B, F.dt, i);
+    __syncthreads();
+    for (int it = 0; it < F.settings.pos_iters; it++) {
+        for (int s = 0; s < F.jt_slots; s++) {
+            const int2 r = jr[s];
+            if (r.x == r.y) continue;
+            for (int j = r.x + tid; j < r.y; j += nt) jt_solve_position_one(B, F.J, j);
+            __syncthreads();
+        }
+    }
+    for (int i = b0 + tid; i < b1; i += nt) dyn_update_state_one(B, i);
+    __syncthreads();
+    // collider transforms and fat AABBs for the next broadphase
+    for (int i = c0 + tid; i < c1; i += nt) refit_one(B, F.A.C, F.hull_bounds, 0, i);
+}
+
+}  // namespace rp3dg

@@ -1,0 +1,522 @@
+// Joint rows of the MI355X step path, one joint per call: ball-and-socket, fixed and hinge joints
+// (src/systems/Solve{BallAndSocket,Fixed,Hinge}JointSystem.cpp) with the default
+// non-linear Gauss-Seidel position correction.
+//
+// Joints never change during a run, so the host colours every joint type once
+// (no two joints of a colour share a body, static bodies included: the position
+// solver rewrites and renormalises the constrained orientation of both bodies,
+// static ones too).  Joints of one colour of one type are solved in parallel;
+// types follow the reference's order BallAndSocket -> Fixed -> Hinge
+// (ConstraintSolverSystem.cpp:94-113).  Used by the per-colour kernels (joints.hip)
+// and by the fused per-world solver (fused.hip).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "joints.h"
+#include "kcommon.h"
+
+namespace rp3dg {
+
+// ---- small matrix helpers (Matrix3x3.h / Matrix2x2.h) ----
+__device__ inline M3 m3(float a, float b, float c, float d, float e, float f, float g, float h, float i) {
+    M3 m;
+    m.r[0] = v3(a, b, c); m.r[1] = v3(d, e, f); m.r[2] = v3(g, h, i);
+    return m;
+}
+__device__ inline M3 madd(const M3& a, const M3& b) {
+    M3 r;
+    for (int i = 0; i < 3; i++) r.r[i] = a.r[i] + b.r[i];
+    return r;
+}
+__device__ inline M3 skew(V3 v) { return m3(0, -v.z, v.y, v.z, 0, -v.x, -v.y, v.x, 0); }   // Matrix3x3.h:251-253
+__device__ inline float mdet(const M3& m) {   // Matrix3x3.h:211-217
+    return (m.r[0].x * (m.r[1].y * m.r[2].z - m.r[2].y * m.r[1].z) -
+            m.r[0].y * (m.r[1].x * m.r[2].z - m.r[2].x * m.r[1].z) +
+            m.r[0].z * (m.r[1].x * m.r[2].y - m.r[2].x * m.r[1].y));
+}
+__device__ inline M3 minv(const M3& m, float det) {   // Matrix3x3.cpp:33-52
+    const float id = 1.0f / det;
+    M3 t = m3((m.r[1].y * m.r[2].z - m.r[2].y * m.r[1].z), -(m.r[0].y * m.r[2].z - m.r[2].y * m.r[0].z),
+              (m.r[0].y * m.r[1].z - m.r[0].z * m.r[1].y), -(m.r[1].x * m.r[2].z - m.r[2].x * m.r[1].z),
+              (m.r[0].x * m.r[2].z - m.r[2].x * m.r[0].z), -(m.r[0].x * m.r[1].z - m.r[1].x * m.r[0].z),
+              (m.r[1].x * m.r[2].y - m.r[2].x * m.r[1].y), -(m.r[0].x * m.r[2].y - m.r[2].x * m.r[0].y),
+              (m.r[0].x * m.r[1].y - m.r[0].y * m.r[1].x));
+    for (int i = 0; i < 3; i++) t.r[i] = v3(t.r[i].x * id, t.r[i].y * id, t.r[i].z * id);
+    return t;
+}
+__device__ inline M3 mzero() { return m3(0, 0, 0, 0, 0, 0, 0, 0, 0); }
+
+__device__ inline M3 ldm(const float4* r0, const float4* r1, const float4* r2, int i) {
+    M3 m;
+    m.r[0] = ld3(r0, i); m.r[1] = ld3(r1, i); m.r[2] = ld3(r2, i);
+    return m;
+}
+__device__ inline void stm(float4* r0, float4* r1, float4* r2, int i, const M3& m) {
+    r0[i] = f4(m.r[0], 0.f); r1[i] = f4(m.r[1], 0.f); r2[i] = f4(m.r[2], 0.f);
+}
+
+// K = (m1+m2) I + U1 I1 U1^T + U2 I2 U2^T, inverted when invertible and a body is dynamic
+__device__ inline M3 translation_inverse_mass(float imsum, V3 r1, V3 r2, const M3& I1, const M3& I2, bool anyDyn, bool& ok) {
+    const M3 U1 = skew(r1), U2 = skew(r2);
+    const M3 K = madd(madd(m3(imsum, 0, 0, 0, imsum, 0, 0, 0, imsum), mmul(mmul(U1, I1), mtranspose(U1))),
+                      mmul(mmul(U2, I2), mtranspose(U2)));
+    const float det = mdet(K);
+    ok = fabsf(det) > kMachineEps;
+    if (ok && anyDyn) return minv(K, det);
+    return mzero();
+}
+
+__device__ inline V3 lock_mul(V3 lock, V3 v) { return cmul(lock, v); }
+
+// q += (0, w) q * 0.5 ; q.normalize()
+__device__ inline Q4 nudge(Q4 q, V3 w) {
+    const Q4 d = qscale(qmul(q4(w.x, w.y, w.z, 0.0f), q), 0.5f);
+    Q4 r = qadd(q, d);
+    const float l = qlen(r);
+    return q4(r.x / l, r.y / l, r.z / l, r.w / l);
+}
+
+// ---- hinge angle (SolveHingeJointSystem.cpp:733-799) ----
+__device__ inline float normalized_angle(float a) {
+    const float twopi = 2.0f * kPi;
+    a = fmodf(a, twopi);
+    if (a < -kPi) return a + twopi;
+    if (a > kPi) return a - twopi;
+    return a;
+}
+__device__ inline float angle_near_limits(float a, float lo, float hi) {
+    const float twopi = 2.0f * kPi;
+    if (hi <= lo) return a;
+    if (a > hi) {
+        const float du = fabsf(normalized_angle(a - hi));
+        const float dl = fabsf(normalized_angle(a - lo));
+        return (du > dl) ? (a - twopi) : a;
+    }
+    if (a < lo) {
+        const float du = fabsf(normalized_angle(hi - a));
+        const float dl = fabsf(normalized_angle(lo - a));
+        return (du > dl) ? a : (a + twopi);
+    }
+    return a;
+}
+__device__ inline float hinge_angle(Q4 q1, Q4 q2, Q4 qinit, V3 a1, float lo, float hi) {
+    Q4 diff = qmul(q2, qconj(q1));
+    float l = qlen(diff);
+    diff = q4(diff.x / l, diff.y / l, diff.z / l, diff.w / l);
+    Q4 rel = qmul(diff, qinit);
+    l = qlen(rel);
+    rel = q4(rel.x / l, rel.y / l, rel.z / l, rel.w / l);
+    const float c = rel.w;
+    const V3 vv = v3(rel.x, rel.y, rel.z);
+    const float s = len(vv);
+    const float d = dot(vv, a1);
+    float ang = d >= 0.0f ? 2.0f * atan2f(s, c) : 2.0f * atan2f(s, -c);
+    ang = normalized_angle(ang);
+    return angle_near_limits(ang, lo, hi);
+}
+
+// ------------------------------------------------------------------------------------------------
+// initBeforeSolve (all joints, one launch; no body state is written)
+// ------------------------------------------------------------------------------------------------
+__device__ __forceinline__ void jt_init_one(const DevBodies& B, const DevJoints& J, int j) {
+    {
+        const int2 bb = J.bodies[j];
+        const int b1 = bb.x, b2 = bb.y;
+        const int t = J.type[j];
+        const M3 I1 = ldm(B.iw0, B.iw1, B.iw2, b1), I2 = ldm(B.iw0, B.iw1, B.iw2, b2);
+        stm(J.i1r0, J.i1r1, J.i1r2, j, I1);
+        stm(J.i2r0, J.i2r1, J.i2r2, j, I2);
+        const Q4 q1 = ldq(B.q, b1), q2 = ldq(B.q, b2);
+        const V3 r1 = qrot(q1, ld3(J.la1, j) - ld3(B.com, b1));
+        const V3 r2 = qrot(q2, ld3(J.la2, j) - ld3(B.com, b2));
+        J.r1[j] = f4(r1, 0.f);
+        J.r2[j] = f4(r2, 0.f);
+        const float im1 = B.ii[b1].w, im2 = B.ii[b2].w;
+        const bool anyDyn = ((B.flags[b1] | B.flags[b2]) & BF_DYNAMIC) != 0;
+        bool ok;
+        const M3 Kt = translation_inverse_mass(im1 + im2, r1, r2, I1, I2, anyDyn, ok);
+        stm(J.kt0, J.kt1, J.kt2, j, Kt);
+        if (t == JT_BALLSOCKET) {
+            // cone-limit axis a x b with a = unit(r1), b = -unit(r2) (SolveBallAndSocketJointSystem.cpp:113-115)
+            J.a1[j] = f4(cross(unit(r1), -unit(r2)), 0.f);
+        } else if (t == JT_FIXED) {
+            // SolveFixedJointSystem.cpp:120-129: keeps I1 + I2 when it cannot be inverted
+            M3 Kr = madd(I1, I2);
+            const float det = mdet(Kr);
+            if (fabsf(det) > kMachineEps && anyDyn) Kr = minv(Kr, det);
+            stm(J.kr0, J.kr1, J.kr2, j, Kr);
+        } else if (t == JT_HINGE) {
+            V3 a1 = normalized(qrot(q1, ld3(J.ax1, j)));
+            const V3 a2 = normalized(qrot(q2, ld3(J.ax2, j)));
+            const V3 b2v = one_unit_orthogonal(a2);
+            const V3 c2v = cross(a2, b2v);
+            const V3 b2xa1 = cross(b2v, a1), c2xa1 = cross(c2v, a1);
+            J.a1[j] = f4(a1, 0.f);
+            J.b2xa1[j] = f4(b2xa1, 0.f);
+            J.c2xa1[j] = f4(c2xa1, 0.f);
+            const V3 i1b = mvmul(I1, b2xa1), i1c = mvmul(I1, c2xa1), i2b = mvmul(I2, b2xa1), i2c = mvmul(I2, c2xa1);
+            const float e11 = dot(b2xa1, i1b) + dot(b2xa1, i2b);
+            const float e12 = dot(b2xa1, i1c) + dot(b2xa1, i2c);
+            const float e21 = dot(c2xa1, i1b) + dot(c2xa1, i2b);
+            const float e22 = dot(c2xa1, i1c) + dot(c2xa1, i2c);
+            float4 kr = make_float4(0.f, 0.f, 0.f, 0.f);
+            const float det = e11 * e22 - e21 * e12;
+            if (fabsf(det) > kMachineEps && anyDyn) {
+                const float id = 1.0f / det;
+                kr = make_float4(e22 * id, -e12 * id, -e21 * id, e11 * id);
+            }
+            J.kr0[j] = kr;
+            // limits / motor (SolveHingeJointSystem.cpp:159-212)
+            int fl = J.flags[j];
+            const float4 lim = J.lim[j];
+            float4 il = J.impL[j];
+            if ((fl & JF_LIMIT) || (fl & JF_MOTOR)) {
+                const float ang = hinge_angle(q1, q2, ldq(J.qinit, j), a1, lim.x, lim.y);
+                const float le = ang - lim.x, ue = lim.y - ang;
+                const bool oldL = (fl & JF_LOWER_VIOLATED) != 0, oldU = (fl & JF_UPPER_VIOLATED) != 0;
+                const bool vl = le <= 0, vu = ue <= 0;
+                fl = vl ? (fl | JF_LOWER_VIOLATED) : (fl & ~JF_LOWER_VIOLATED);
+                fl = vu ? (fl | JF_UPPER_VIOLATED) : (fl & ~JF_UPPER_VIOLATED);
+                if (!vl || vl != oldL) il.x = 0.0f;
+                if (!vu || vu != oldU) il.y = 0.0f;
+                if ((fl & JF_MOTOR) || ((fl & JF_LIMIT) && (vl || vu))) {
+                    float m = dot(a1, mvmul(I1, a1)) + dot(a1, mvmul(I2, a1));
+                    m = m > 0.0f ? 1.0f / m : 0.0f;
+                    J.imlm[j] = m;
+                }
+                J.flags[j] = fl;
+                J.impL[j] = il;
+            }
+        }
+    }
+}
+
+// ------------------------------------------------------------------------------------------------
+// warm start (per colour)
+// ------------------------------------------------------------------------------------------------
+__device__ __forceinline__ void jt_warmstart_one(const DevBodies& B, const DevJoints& J, int j) {
+    {
+        const int2 bb = J.bodies[j];
+        const int b1 = bb.x, b2 = bb.y;
+        const int type = J.type[j];
+        V3 v1 = ld3(B.cv, b1), w1 = ld3(B.cw, b1), v2 = ld3(B.cv, b2), w2 = ld3(B.cw, b2);
+        const float im1 = B.ii[b1].w, im2 = B.ii[b2].w;
+        const V3 ll1 = ld3(B.llock, b1), al1 = ld3(B.alock, b1), ll2 = ld3(B.llock, b2), al2 = ld3(B.alock, b2);
+        const M3 I1 = ldm(J.i1r0, J.i1r1, J.i1r2, j), I2 = ldm(J.i2r0, J.i2r1, J.i2r2, j);
+        const V3 r1 = ld3(J.r1, j), r2 = ld3(J.r2, j);
+        const V3 it = ld3(J.impT, j);
+        if (type == JT_BALLSOCKET) {   // SolveBallAndSocketJointSystem.cpp:196-241
+            // cone limit disabled: its accumulated impulse stays 0, but 0 * (a x b) is still added
+            // (keeps the sign of zero components identical to the reference)
+            const V3 cone = 0.0f * ld3(J.a1, j);
+            const V3 li1 = -it;
+            V3 ai1 = cross(it, r1);
+            ai1 += cone;
+            v1 += lock_mul((im1 * ll1), li1);
+            w1 += cmul(al1, mvmul(I1, ai1));
+            V3 ai2 = -cross(it, r2);
+            ai2 += -cone;
+            v2 += lock_mul((im2 * ll2), it);
+            w2 += cmul(al2, mvmul(I2, ai2));
+        } else if (type == JT_FIXED) {   // SolveFixedJointSystem.cpp:142-186
+            const V3 ir = ld3(J.impR, j);
+            const V3 li1 = -it;
+            V3 ai1 = cross(it, r1);
+            ai1 += -ir;
+            v1 += lock_mul((im1 * ll1), li1);
+            w1 += cmul(al1, mvmul(I1, ai1));
+            V3 ai2 = -cross(it, r2);
+            ai2 += ir;
+            v2 += lock_mul((im2 * ll2), it);
+            w2 += cmul(al2, mvmul(I2, ai2));
+        } else {   // hinge, SolveHingeJointSystem.cpp:216-270
+            const float4 ir = J.impR[j];
+            const float4 il = J.impL[j];
+            const V3 a1 = ld3(J.a1, j), bxa = ld3(J.b2xa1, j), cxa = ld3(J.c2xa1, j);
+            const V3 rot = (-bxa) * ir.x - cxa * ir.y;
+            const V3 limI = (il.y - il.x) * a1;
+            const V3 motI = (-il.z) * a1;
+            const V3 li1 = -it;
+            V3 ai1 = cross(it, r1);
+            ai1 += rot;
+            ai1 += limI;
+            ai1 += motI;
+            v1 += lock_mul((im1 * ll1), li1);
+            w1 += cmul(al1, mvmul(I1, ai1));
+            V3 ai2 = -cross(it, r2);
+            ai2 += -rot;
+            ai2 += -limI;
+            ai2 += -motI;
+            v2 += lock_mul((im2 * ll2), it);
+            w2 += cmul(al2, mvmul(I2, ai2));
+        }
+        B.cv[b1] = f4(v1, 0.f); B.cw[b1] = f4(w1, 0.f);
+        B.cv[b2] = f4(v2, 0.f); B.cw[b2] = f4(w2, 0.f);
+    }
+}
+
+// ------------------------------------------------------------------------------------------------
+// velocity constraints (per colour)
+// ------------------------------------------------------------------------------------------------
+__device__ __forceinline__ void jt_solve_velocity_one(const DevBodies& B, const DevJoints& J, int j, float dt) {
+    {
+        const int2 bb = J.bodies[j];
+        const int b1 = bb.x, b2 = bb.y;
+        const int type = J.type[j];
+        V3 v1 = ld3(B.cv, b1), w1 = ld3(B.cw, b1), v2 = ld3(B.cv, b2), w2 = ld3(B.cw, b2);
+        const float im1 = B.ii[b1].w, im2 = B.ii[b2].w;
+        const V3 ll1 = ld3(B.llock, b1), al1 = ld3(B.alock, b1), ll2 = ld3(B.llock, b2), al2 = ld3(B.alock, b2);
+        const M3 I1 = ldm(J.i1r0, J.i1r1, J.i1r2, j), I2 = ldm(J.i2r0, J.i2r1, J.i2r2, j);
+        const V3 r1 = ld3(J.r1, j), r2 = ld3(J.r2, j);
+        const M3 Kt = ldm(J.kt0, J.kt1, J.kt2, j);
+        if (type == JT_BALLSOCKET) {   // SolveBallAndSocketJointSystem.cpp:245-308
+            const V3 Jv = v2 + cross(w2, r2) - v1 - cross(w1, r1);
+            const V3 dl = mvmul(Kt, -Jv - v3(0.0f, 0.0f, 0.0f));
+            J.impT[j] = f4(ld3(J.impT, j) + dl, 0.f);
+            const V3 li1 = -dl;
+            const V3 ai1 = cross(dl, r1);
+            v1 += lock_mul((im1 * ll1), li1);
+            w1 += cmul(al1, mvmul(I1, ai1));
+            const V3 ai2 = -cross(dl, r2);
+            v2 += lock_mul((im2 * ll2), dl);
+            w2 += cmul(al2, mvmul(I2, ai2));
+        } else if (type == JT_FIXED) {   // SolveFixedJointSystem.cpp:189-262
+            const V3 Jvt = v2 + cross(w2, r2) - v1 - cross(w1, r1);
+            const V3 dl = mvmul(Kt, -Jvt - v3(0.0f, 0.0f, 0.0f));
+            J.impT[j] = f4(ld3(J.impT, j) + dl, 0.f);
+            const V3 li1 = -dl;
+            V3 ai1 = cross(dl, r1);
+            v1 += lock_mul((im1 * ll1), li1);
+            w1 += cmul(al1, mvmul(I1, ai1));
+            const V3 ai2 = -cross(dl, r2);
+            v2 += lock_mul((im2 * ll2), dl);
+            w2 += cmul(al2, mvmul(I2, ai2));
+            const V3 Jvr = w2 - w1;
+            const M3 Kr = ldm(J.kr0, J.kr1, J.kr2, j);
+            const V3 dl2 = mvmul(Kr, -Jvr - v3(0.0f, 0.0f, 0.0f));
+            J.impR[j] = f4(ld3(J.impR, j) + dl2, 0.f);
+            ai1 = -dl2;
+            w1 += cmul(al1, mvmul(I1, ai1));
+            w2 += cmul(al2, mvmul(I2, dl2));
+        } else {   // hinge, SolveHingeJointSystem.cpp:273-440
+            const V3 a1 = ld3(J.a1, j);
+            const int fl = J.flags[j];
+            float4 il = J.impL[j];
+            const float imlm = J.imlm[j];
+            if (fl & JF_LIMIT) {
+                if (fl & JF_LOWER_VIOLATED) {
+                    const float JvL = dot(w2 - w1, a1);
+                    float d = imlm * (-JvL - 0.0f);
+                    const float lt = il.x;
+                    il.x = fmax_s(il.x + d, 0.0f);
+                    d = il.x - lt;
+                    const V3 ai1 = (-d) * a1;
+                    w1 += cmul(al1, mvmul(I1, ai1));
+                    const V3 ai2 = d * a1;
+                    w2 += cmul(al2, mvmul(I2, ai2));
+                }
+                if (fl & JF_UPPER_VIOLATED) {
+                    const float JvU = -dot(w2 - w1, a1);
+                    float d = imlm * (-JvU - 0.0f);
+                    const float lt = il.y;
+                    il.y = fmax_s(il.y + d, 0.0f);
+                    d = il.y - lt;
+                    const V3 ai1 = d * a1;
+                    w1 += cmul(al1, mvmul(I1, ai1));
+                    const V3 ai2 = (-d) * a1;
+                    w2 += cmul(al2, mvmul(I2, ai2));
+                }
+            }
+            if (fl & JF_MOTOR) {
+                const float4 lim = J.lim[j];
+                const float JvM = dot(a1, w1 - w2);
+                const float maxI = lim.w * dt;
+                float d = imlm * (-JvM - lim.z);
+                const float lt = il.z;
+                il.z = clampf(il.z + d, -maxI, maxI);
+                d = il.z - lt;
+                const V3 ai1 = (-d) * a1;
+                w1 += cmul(al1, mvmul(I1, ai1));
+                const V3 ai2 = d * a1;
+                w2 += cmul(al2, mvmul(I2, ai2));
+            }
+            J.impL[j] = il;
+            const V3 bxa = ld3(J.b2xa1, j), cxa = ld3(J.c2xa1, j);
+            const float jr0 = -dot(bxa, w1) + dot(bxa, w2);
+            const float jr1 = -dot(cxa, w1) + dot(cxa, w2);
+            const float4 kr = J.kr0[j];
+            const float x0 = -jr0 - 0.0f, x1 = -jr1 - 0.0f;
+            const float d0 = kr.x * x0 + kr.y * x1;
+            const float d1 = kr.z * x0 + kr.w * x1;
+            float4 ir = J.impR[j];
+            ir.x += d0;
+            ir.y += d1;
+            J.impR[j] = ir;
+            V3 ai1 = (-bxa) * d0 - cxa * d1;
+            w1 += cmul(al1, mvmul(I1, ai1));
+            V3 ai2 = bxa * d0 + cxa * d1;
+            w2 += cmul(al2, mvmul(I2, ai2));
+            const V3 Jvt = v2 + cross(w2, r2) - v1 - cross(w1, r1);
+            const V3 dl = mvmul(Kt, -Jvt - v3(0.0f, 0.0f, 0.0f));
+            J.impT[j] = f4(ld3(J.impT, j) + dl, 0.f);
+            const V3 li1 = -dl;
+            ai1 = cross(dl, r1);
+            v1 += lock_mul((im1 * ll1), li1);
+            w1 += cmul(al1, mvmul(I1, ai1));
+            ai2 = -cross(dl, r2);
+            v2 += lock_mul((im2 * ll2), dl);
+            w2 += cmul(al2, mvmul(I2, ai2));
+        }
+        B.cv[b1] = f4(v1, 0.f); B.cw[b1] = f4(w1, 0.f);
+        B.cv[b2] = f4(v2, 0.f); B.cw[b2] = f4(w2, 0.f);
+    }
+}
+
+// ------------------------------------------------------------------------------------------------
+// non-linear Gauss-Seidel position correction (per colour)
+// ------------------------------------------------------------------------------------------------
+__device__ __forceinline__ void jt_solve_position_one(const DevBodies& B, const DevJoints& J, int j) {
+    {
+        const int2 bb = J.bodies[j];
+        const int b1 = bb.x, b2 = bb.y;
+        const int type = J.type[j];
+        Q4 q1 = ldq(B.cq, b1), q2 = ldq(B.cq, b2);
+        V3 x1 = ld3(B.cx, b1), x2 = ld3(B.cx, b2);
+        const float4 ii1 = B.ii[b1], ii2 = B.ii[b2];
+        const float im1 = ii1.w, im2 = ii2.w;
+        const V3 ll1 = ld3(B.llock, b1), al1 = ld3(B.alock, b1), ll2 = ld3(B.llock, b2), al2 = ld3(B.alock, b2);
+        const bool anyDyn = ((B.flags[b1] | B.flags[b2]) & BF_DYNAMIC) != 0;
+        M3 I1 = world_inv_inertia(qmat(q1), v3(ii1.x, ii1.y, ii1.z));
+        M3 I2 = world_inv_inertia(qmat(q2), v3(ii2.x, ii2.y, ii2.z));
+        const V3 r1 = qrot(q1, ld3(J.la1, j) - ld3(B.com, b1));
+        const V3 r2 = qrot(q2, ld3(J.la2, j) - ld3(B.com, b2));
+        if (type == JT_BALLSOCKET) {   // SolveBallAndSocketJointSystem.cpp:312-445
+            bool ok;
+            const M3 Kt = translation_inverse_mass(im1 + im2, r1, r2, I1, I2, anyDyn, ok);
+            if (ok) {
+                const V3 err = x2 + r2 - x1 - r1;
+                const V3 lam = mvmul(Kt, -err);
+                const V3 li1 = -lam;
+                const V3 ai1 = cross(lam, r1);
+                const V3 dv1 = lock_mul((im1 * ll1), li1);
+                const V3 dw1 = cmul(al1, mvmul(I1, ai1));
+                x1 += dv1;
+                q1 = nudge(q1, dw1);
+                const V3 ai2 = -cross(lam, r2);
+                const V3 dv2 = lock_mul((im2 * ll2), lam);
+                const V3 dw2 = cmul(al2, mvmul(I2, ai2));
+                x2 += dv2;
+                q2 = nudge(q2, dw2);
+            }
+        } else if (type == JT_FIXED) {   // SolveFixedJointSystem.cpp:266-448
+            bool ok;
+            const M3 Kt = translation_inverse_mass(im1 + im2, r1, r2, I1, I2, anyDyn, ok);
+            if (ok) {
+                const V3 err = x2 + r2 - x1 - r1;
+                const V3 lam = mvmul(Kt, -err);
+                const V3 li1 = -lam;
+                const V3 ai1 = cross(lam, r1);
+                const V3 dv1 = lock_mul((im1 * ll1), li1);
+                const V3 dw1 = cmul(al1, mvmul(I1, ai1));
+                x1 += dv1;
+                q1 = nudge(q1, dw1);
+                const V3 ai2 = -cross(lam, r2);
+                const V3 dv2 = lock_mul((im2 * ll2), lam);
+                const V3 dw2 = cmul(al2, mvmul(I2, ai2));
+                x2 += dv2;
+                q2 = nudge(q2, dw2);
+            }
+            M3 Kr = madd(I1, I2);
+            const float det = mdet(Kr);
+            if (fabsf(det) > kMachineEps) {
+                if (anyDyn) Kr = minv(Kr, det);
+                const Q4 qe = qmul(qmul(q2, ldq(J.qinit, j)), qconj(q1));
+                const V3 er = 2.0f * v3(qe.x, qe.y, qe.z);
+                const V3 lr = mvmul(Kr, -er);
+                const V3 ai1 = -lr;
+                const V3 dw1 = cmul(al1, mvmul(I1, ai1));
+                q1 = nudge(q1, dw1);
+                const V3 dw2 = cmul(al2, mvmul(I2, lr));
+                q2 = nudge(q2, dw2);
+            }
+        } else {   // hinge, SolveHingeJointSystem.cpp:444-730
+            V3 a1 = normalized(qrot(q1, ld3(J.ax1, j)));
+            const V3 a2 = normalized(qrot(q2, ld3(J.ax2, j)));
+            const V3 b2v = one_unit_orthogonal(a2);
+            const V3 c2v = cross(a2, b2v);
+            const V3 bxa = cross(b2v, a1), cxa = cross(c2v, a1);
+            J.a1[j] = f4(a1, 0.f);
+            J.b2xa1[j] = f4(bxa, 0.f);
+            J.c2xa1[j] = f4(cxa, 0.f);
+            int fl = J.flags[j];
+            const float4 lim = J.lim[j];
+            if (fl & (JF_LIMIT | JF_MOTOR)) {
+                const float ang = hinge_angle(q1, q2, ldq(J.qinit, j), a1, lim.x, lim.y);
+                const float le = ang - lim.x, ue = lim.y - ang;
+                fl = le <= 0 ? (fl | JF_LOWER_VIOLATED) : (fl & ~JF_LOWER_VIOLATED);
+                fl = ue <= 0 ? (fl | JF_UPPER_VIOLATED) : (fl & ~JF_UPPER_VIOLATED);
+                J.flags[j] = fl;
+                if (fl & JF_LIMIT) {
+                    const float imlm_old = J.imlm[j];
+                    if (fl & (JF_LOWER_VIOLATED | JF_UPPER_VIOLATED)) {
+                        const float m = dot(a1, mvmul(I1, a1)) + dot(a1, mvmul(I2, a1));
+                        J.imlm[j] = imlm_old > 0.0f ? 1.0f / m : 0.0f;   // reference tests the old value
+                    }
+                    if (fl & JF_LOWER_VIOLATED) {
+                        const float l = imlm_old * (-le);
+                        q1 = nudge(q1, cmul(al1, mvmul(I1, (-l) * a1)));
+                        q2 = nudge(q2, cmul(al2, mvmul(I2, l * a1)));
+                    }
+                    if (fl & JF_UPPER_VIOLATED) {
+                        const float l = imlm_old * (-ue);
+                        q1 = nudge(q1, cmul(al1, mvmul(I1, l * a1)));
+                        q2 = nudge(q2, cmul(al2, mvmul(I2, (-l) * a1)));
+                    }
+                }
+            }
+            const V3 i1b = mvmul(I1, bxa), i1c = mvmul(I1, cxa), i2b = mvmul(I2, bxa), i2c = mvmul(I2, cxa);
+            const float e11 = dot(bxa, i1b) + dot(bxa, i2b);
+            const float e12 = dot(bxa, i1c) + dot(bxa, i2c);
+            const float e21 = dot(cxa, i1b) + dot(cxa, i2b);
+            const float e22 = dot(cxa, i1c) + dot(cxa, i2c);
+            const float det = e11 * e22 - e21 * e12;
+            if (fabsf(det) > kMachineEps) {
+                float k00 = 0, k01 = 0, k10 = 0, k11 = 0;
+                if (anyDyn) {
+                    const float id = 1.0f / det;
+                    k00 = e22 * id; k01 = -e12 * id; k10 = -e21 * id; k11 = e11 * id;
+                }
+                const float er0 = dot(a1, b2v), er1 = dot(a1, c2v);
+                const float x0 = -er0, x1v = -er1;
+                const float l0 = k00 * x0 + k01 * x1v;
+                const float l1 = k10 * x0 + k11 * x1v;
+                const V3 ai1 = (-bxa) * l0 - cxa * l1;
+                q1 = nudge(q1, cmul(al1, mvmul(I1, ai1)));
+                const V3 ai2 = bxa * l0 + cxa * l1;
+                q2 = nudge(q2, cmul(al2, mvmul(I2, ai2)));
+            }
+            bool ok;
+            const M3 Kt = translation_inverse_mass(im1 + im2, r1, r2, I1, I2, anyDyn, ok);
+            if (ok) {
+                const V3 err = x2 + r2 - x1 - r1;
+                const V3 lam = mvmul(Kt, -err);
+                const V3 li1 = -lam;
+                const V3 ai1 = cross(lam, r1);
+                const V3 dv1 = lock_mul((im1 * ll1), li1);
+                const V3 dw1 = cmul(al1, mvmul(I1, ai1));
+                x1 += dv1;
+                q1 = nudge(q1, dw1);
+                const V3 ai2 = -cross(lam, r2);
+                const V3 dv2 = lock_mul((im2 * ll2), lam);
+                const V3 dw2 = cmul(al2, mvmul(I2, ai2));
+                x2 += dv2;
+                q2 = nudge(q2, dw2);
+            }
+        }
+        B.cx[b1] = f4(x1, 0.f); B.cq[b1] = f4q(q1);
+        B.cx[b2] = f4(x2, 0.f); B.cq[b2] = f4q(q2);
+    }
+}
+
+}  // namespace rp3dg

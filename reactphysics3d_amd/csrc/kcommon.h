@@ -29,6 +29,28 @@ __device__ __forceinline__ int wave_incl_scan(int v) {
     return v;
 }
 
+__device__ __forceinline__ int wave_sum(int v) {
+#pragma unroll
+    for (int off = kWave / 2; off > 0; off >>= 1) v += __shfl_xor(v, off, kWave);
+    return v;
+}
+__device__ __forceinline__ int wave_max(int v) {
+#pragma unroll
+    for (int off = kWave / 2; off > 0; off >>= 1) {
+        const int o = __shfl_xor(v, off, kWave);
+        v = o > v ? o : v;
+    }
+    return v;
+}
+__device__ __forceinline__ float wave_maxf(float v) {
+#pragma unroll
+    for (int off = kWave / 2; off > 0; off >>= 1) {
+        const float o = __shfl_xor(v, off, kWave);
+        v = o > v ? o : v;
+    }
+    return v;
+}
+
 // Block-wide exclusive scan (blockDim.x multiple of 64, <= 1024); returns the exclusive
 // prefix of v and writes the block total to *total.  smem must hold 17 ints.
 __device__ __forceinline__ int block_excl_scan(int v, int* smem, int* total) {

@@ -145,21 +145,26 @@ struct Settings {
 };
 
 // Counters written by kernels (one device block, read back after each step).
+constexpr int kColorRoundsMax = 48;   // parallel colouring rounds launched at most per step
+
+// Per-step counters, cleared by a fill kernel at the start of every step and read back after it.
 struct DevCounters {
     int nb_pairs;              // pairs in the current table
-    int nb_overlaps;
+    int nb_overlaps;           // overlapping pairs found by the traversal (may exceed the capacity)
     int nb_tests[ALGO_COUNT];
     int nb_manifolds;
     int nb_points;
     int nb_colors;
     int status;
-    int color_rounds;
-    int nb_uncolored;
-    int pad[6];
+    int finish_rounds;         // rounds taken by the single-workgroup colouring finisher
+    int nb_new_points;         // contact points without a cached impulse
+    int color_left[kColorRoundsMax];   // manifolds still uncoloured after each parallel round
+    int pad[4];
 };
 
 constexpr int STATUS_PAIR_OVERFLOW = 1;
 constexpr int STATUS_COLOR_OVERFLOW = 2;
 constexpr int STATUS_STACK_OVERFLOW = 4;
+constexpr int STATUS_COLOR_STALL = 8;
 
 }  // namespace rp3dg

@@ -15,7 +15,7 @@ struct SvArgs {
     const int* nm;        // number of manifolds
     const int* mlist;     // manifold -> pair slot (pair order)
     const int* order;     // colour-sorted position -> manifold
-    const int* cstart;    // colour ranges [cstart[c], cstart[c+1])
+    const int* cstart;    // colour ranges [cstart[c], cstart[c+1]) (colour-major layout)
     const int* ncolors;
     float restitution_threshold;
 };
@@ -30,9 +30,11 @@ __global__ void sv_adj_count(DevBodies B, DevColliders C, DevPairs P, const int*
 __global__ void sv_adj_fill(DevBodies B, DevColliders C, DevPairs P, const int* list, const int* nm_ptr, const int* offs,
                             int* cursor, int* adj, int2* mb);
 __global__ void sv_color_round(const int2* mb, const int* nm_ptr, const int* offs, const int* adj, int* color,
-                               int* remaining, int* status);
-__global__ void sv_color_keys(const int* color, const int* nm_ptr, uint64_t* keys, int* vals);
-__global__ void sv_color_ranges(const uint64_t* keys, const int* nm_ptr, int* start, int* ncolors);
+                               const int* left_prev, int* left, int* status);
+__global__ void sv_color_finish(const int2* mb, const int* nm_ptr, const int* offs, const int* adj, int* color,
+                                const int* left_prev, int* rounds, int* status);
+__global__ void sv_color_keys(const int* color, const int* nm_ptr, DevColliders C, DevPairs P, const int* mlist,
+                              int world_major, uint64_t* keys, int* vals, int* seg_cnt, int* ncolors);
 __global__ void sv_prepare(SvArgs A);
 __global__ void sv_warmstart(SvArgs A, int color);
 __global__ void sv_solve_color(SvArgs A, int color, float beta_dt);

@@ -1,93 +1,60 @@
-// Contact solver and integration of the MI355X step path.
+// Manifold list, graph colouring and the per-colour solver kernels.
 //
-// The reference solves contact manifolds sequentially in island order
-// (ContactSolverSystem.cpp:479-759).  Here manifolds are graph-coloured so
-// that no two manifolds of a colour share a dynamic body; one launch per
-// colour then solves all of its manifolds in parallel, which is exactly the
-// sequential Gauss-Seidel sweep in colour order (static and kinematic bodies
-// have zero inverse mass and are never written, so they do not constrain the
-// colouring).  Colouring is Jones-Plassmann with hashed priorities: a
-// manifold takes the smallest colour unused by its higher-priority
-// neighbours once they are all coloured, which equals sequential greedy
-// colouring in priority order and is therefore deterministic.
+// Colouring is Jones-Plassmann with hashed priorities: a manifold takes the
+// smallest colour unused by its higher-priority neighbours once they are all
+// coloured, which equals sequential greedy colouring in priority order and is
+// therefore deterministic.  A fixed number of parallel rounds is launched (each
+// returns at once when nothing is left) and a single-workgroup finisher
+// completes whatever remains, so the step needs no host round trip.
 #include <hip/hip_runtime.h>
 
 #include "kcommon.h"
 #include "rp3d_internal.h"
 #include "solver.h"
+#include "solver_dev.h"
 
 namespace rp3dg {
 
 // ------------------------------------------------------------------------------------------------
-// Integration (DynamicsSystem.cpp, PhysicsWorld.cpp:310-318)
+// Integration and per-colour rows (bodies in solver_dev.h)
 // ------------------------------------------------------------------------------------------------
 
-// updateBodiesInverseWorldInertiaTensors + integrateRigidBodiesVelocities (DynamicsSystem.cpp:117-183)
 __global__ void dyn_integrate_velocities(DevBodies B, Settings S, float dt) {
-    RP3DG_GRID_STRIDE(i, B.n) {
-        const int f = B.flags[i];
-        if (!(f & BF_ENABLED)) continue;
-        const Q4 q = ldq(B.q, i);
-        const float4 ii = B.ii[i];
-        const M3 Iw = world_inv_inertia(qmat(q), v3(ii.x, ii.y, ii.z));
-        B.iw0[i] = f4(Iw.r[0], 0.f);
-        B.iw1[i] = f4(Iw.r[1], 0.f);
-        B.iw2[i] = f4(Iw.r[2], 0.f);
-        B.sv[i] = make_float4(0.f, 0.f, 0.f, 0.f);
-        B.sw[i] = make_float4(0.f, 0.f, 0.f, 0.f);
-        const float4 ll4 = B.llock[i], al4 = B.alock[i];
-        const V3 ll = xyz(ll4), al = xyz(al4);
-        const float im = ii.w;
-        const V3 v = ld3(B.v, i), w = ld3(B.w, i);
-        V3 cv = v + cmul((dt * im) * ll, ld3(B.force, i));
-        V3 cw = w + cmul(dt * al, mvmul(Iw, ld3(B.torque, i)));
-        if (S.gravity_enabled && (f & BF_GRAVITY)) {
-            const float mass = B.com[i].w;
-            cv = cv + cmul(((dt * im) * ll) * mass, S.gravity);
-        }
-        const float ld = 1.0f / (1.0f + ll4.w * dt);
-        const float ad = 1.0f / (1.0f + al4.w * dt);
-        cv = cv * ld;
-        cw = cw * ad;
-        B.cv[i] = f4(cv, 0.f);
-        B.cw[i] = f4(cw, 0.f);
-    }
+    RP3DG_GRID_STRIDE(i, B.n) dyn_integrate_velocities_one(B, S, dt, i);
 }
 
-// integrateRigidBodiesPositions (DynamicsSystem.cpp:44-71), split impulses active
 __global__ void dyn_integrate_positions(DevBodies B, float dt) {
-    RP3DG_GRID_STRIDE(i, B.n) {
-        if (!(B.flags[i] & BF_ENABLED)) continue;
-        const V3 nv = ld3(B.cv, i) + 1.0f * ld3(B.sv, i);
-        const V3 nw = ld3(B.cw, i) + 1.0f * ld3(B.sw, i);
-        const V3 x = ld3(B.x, i);
-        const Q4 q = ldq(B.q, i);
-        const V3 cx = x + nv * dt;
-        const Q4 dq = qscale(qscale(qmul(q4(nw.x, nw.y, nw.z, 0.0f), q), 0.5f), dt);
-        B.cx[i] = f4(cx, 0.f);
-        B.cq[i] = f4q(qadd(q, dq));
-    }
+    RP3DG_GRID_STRIDE(i, B.n) dyn_integrate_positions_one(B, dt, i);
 }
 
-// updateBodiesState (DynamicsSystem.cpp:74-101) and resetBodiesForceAndTorque (:186-195)
 __global__ void dyn_update_state(DevBodies B) {
-    RP3DG_GRID_STRIDE(i, B.n) {
-        if (B.flags[i] & BF_ENABLED) {
-            B.v[i] = B.cv[i];
-            B.w[i] = B.cw[i];
-            const V3 x = ld3(B.cx, i);
-            const Q4 q = qunit(ldq(B.cq, i));
-            B.x[i] = f4(x, 0.f);
-            B.q[i] = f4q(q);
-            B.p[i] = f4(x - qrot(q, ld3(B.com, i)), 0.f);
-        }
-        B.force[i] = make_float4(0.f, 0.f, 0.f, 0.f);
-        B.torque[i] = make_float4(0.f, 0.f, 0.f, 0.f);
-    }
+    RP3DG_GRID_STRIDE(i, B.n) dyn_update_state_one(B, i);
+}
+
+__global__ void sv_prepare(SvArgs A) {
+    const int nm = *A.nm;
+    RP3DG_GRID_STRIDE(i, nm) sv_prepare_one(A, i);
+}
+
+__global__ void sv_warmstart(SvArgs A, int color) {
+    if (color >= *A.ncolors) return;
+    const int lo = A.cstart[color], hi = A.cstart[color + 1];
+    RP3DG_GRID_STRIDE(t, hi - lo) sv_warmstart_one(A, lo + t);
+}
+
+__global__ void sv_solve_color(SvArgs A, int color, float beta_dt) {
+    if (color >= *A.ncolors) return;
+    const int lo = A.cstart[color], hi = A.cstart[color + 1];
+    RP3DG_GRID_STRIDE(t, hi - lo) solve_manifold(A, lo + t, beta_dt);
+}
+
+__global__ void sv_store(SvArgs A) {
+    const int nm = *A.nm;
+    RP3DG_GRID_STRIDE(i, nm) sv_store_one(A, i);
 }
 
 // ------------------------------------------------------------------------------------------------
-// Manifold list and colouring
+// Manifold list: pairs with at least one contact point, in pair order
 // ------------------------------------------------------------------------------------------------
 
 __global__ void sv_flag_manifolds(const int* ncp, const int* nb_pairs, int* flag) {
@@ -98,18 +65,22 @@ __global__ void sv_flag_manifolds(const int* ncp, const int* nb_pairs, int* flag
 __global__ void sv_compact_manifolds(const int* flag, const int* offs, const int* nb_pairs, const int* ncp, int* list,
                                      int* counters_manifolds, int* counters_points) {
     const int np = *nb_pairs;
+    if (blockIdx.x == 0 && threadIdx.x == 0) *counters_manifolds = offs[np];
+    int pts = 0;
     RP3DG_GRID_STRIDE(k, np) {
         if (flag[k]) {
             list[offs[k]] = k;
-            atomicAdd(counters_points, ncp[k]);
+            pts += ncp[k];
         }
-        if (k == np - 1) *counters_manifolds = offs[k] + flag[k];
     }
+    pts = wave_sum(pts);
+    if (lane_id() == 0 && pts) atomicAdd(counters_points, pts);
 }
 
-__device__ __forceinline__ bool dyn_body(const DevBodies& B, int b) { return (B.flags[b] & BF_DYNAMIC) != 0; }
-
+// ------------------------------------------------------------------------------------------------
 // body -> manifold adjacency (counts, then fill after a scan)
+// ------------------------------------------------------------------------------------------------
+
 __global__ void sv_adj_count(DevBodies B, DevColliders C, DevPairs P, const int* list, const int* nm_ptr, int* cnt) {
     const int nm = *nm_ptr;
     RP3DG_GRID_STRIDE(m, nm) {
@@ -134,6 +105,10 @@ __global__ void sv_adj_fill(DevBodies B, DevColliders C, DevPairs P, const int* 
     }
 }
 
+// ------------------------------------------------------------------------------------------------
+// Jones-Plassmann colouring
+// ------------------------------------------------------------------------------------------------
+
 __device__ __forceinline__ uint32_t prio(uint32_t m) {
     uint32_t x = m * 0x9E3779B1u;
     x ^= x >> 16;
@@ -146,400 +121,104 @@ __device__ __forceinline__ bool higher(uint32_t a, uint32_t b) {  // priority of
     return pa > pb || (pa == pb && a > b);
 }
 
-// One Jones-Plassmann round.  color[m] = -1 while uncoloured.
+// Tries to colour manifold m (color[m] == -1 while uncoloured).  Returns false when a
+// higher-priority neighbour is still uncoloured.  A neighbour coloured concurrently is read either
+// as -1 or as its final colour, so the result does not depend on timing.
+__device__ __forceinline__ bool jp_try(int m, const int2* mb, const int* offs, const int* adj, int* color, int* status) {
+    const int2 bb = mb[m];
+    uint64_t used = 0;
+    for (int s = 0; s < 2; s++) {
+        const int b = s == 0 ? bb.x : bb.y;
+        if (b < 0) continue;
+        const int e = offs[b + 1];
+        for (int k = offs[b]; k < e; k++) {
+            const int o = adj[k];
+            if (o == m) continue;
+            if (higher((uint32_t)o, (uint32_t)m)) {
+                const int co = __atomic_load_n(&color[o], __ATOMIC_RELAXED);
+                if (co < 0) return false;
+                if (co < 64) used |= 1ull << co;
+            }
+        }
+    }
+    const uint64_t freem = ~used;
+    int c = freem ? __ffsll((unsigned long long)freem) - 1 : 64;
+    if (c >= kMaxColors) { atomicOr(status, STATUS_COLOR_OVERFLOW); c = kMaxColors - 1; }
+    __atomic_store_n(&color[m], c, __ATOMIC_RELAXED);
+    return true;
+}
+
+// One parallel round; left_prev = manifolds left after the previous round (nullptr for the first).
 __global__ void sv_color_round(const int2* mb, const int* nm_ptr, const int* offs, const int* adj, int* color,
-                               int* remaining, int* status) {
+                               const int* left_prev, int* left, int* status) {
+    if (left_prev && *left_prev == 0) return;
     const int nm = *nm_ptr;
-    int left = 0;
+    int n_left = 0;
     RP3DG_GRID_STRIDE(m, nm) {
         if (color[m] >= 0) continue;
-        const int2 bb = mb[m];
-        bool ready = true;
-        uint64_t used = 0;
-        for (int s = 0; s < 2 && ready; s++) {
-            const int b = s == 0 ? bb.x : bb.y;
-            if (b < 0) continue;
-            const int e = offs[b + 1];
-            for (int k = offs[b]; k < e; k++) {
-                const int o = adj[k];
-                if (o == m) continue;
-                if (higher((uint32_t)o, (uint32_t)m)) {
-                    const int co = __atomic_load_n(&color[o], __ATOMIC_RELAXED);
-                    if (co < 0) { ready = false; break; }
-                    if (co < 64) used |= 1ull << co;
-                }
-            }
-        }
-        if (!ready) { left++; continue; }
-        const uint64_t freem = ~used;
-        int c = freem ? __ffsll((unsigned long long)freem) - 1 : 64;
-        if (c >= kMaxColors) { atomicOr(status, STATUS_COLOR_OVERFLOW); c = kMaxColors - 1; }
-        __atomic_store_n(&color[m], c, __ATOMIC_RELAXED);
+        if (!jp_try(m, mb, offs, adj, color, status)) n_left++;
     }
-    if (left) atomicAdd(remaining, left);
+    n_left = wave_sum(n_left);
+    if (lane_id() == 0 && n_left) atomicAdd(left, n_left);
 }
 
-__global__ void sv_color_keys(const int* color, const int* nm_ptr, uint64_t* keys, int* vals) {
+// Single workgroup: rounds until every manifold is coloured.  Each round colours at least the
+// highest-priority uncoloured manifold, so nm + 1 rounds always suffice.
+__global__ __launch_bounds__(1024) void sv_color_finish(const int2* mb, const int* nm_ptr, const int* offs,
+                                                        const int* adj, int* color, const int* left_prev, int* rounds,
+                                                        int* status) {
+    if (left_prev && *left_prev == 0) return;
+    __shared__ int left;
     const int nm = *nm_ptr;
+    for (int r = 0; r <= nm; r++) {
+        if (threadIdx.x == 0) left = 0;
+        __syncthreads();
+        int mine = 0;
+        for (int m = threadIdx.x; m < nm; m += blockDim.x) {
+            if (color[m] >= 0) continue;
+            if (!jp_try(m, mb, offs, adj, color, status)) mine++;
+        }
+        if (mine) atomicAdd(&left, mine);
+        __syncthreads();
+        const int l = left;
+        __syncthreads();
+        if (l == 0) {
+            if (threadIdx.x == 0) *rounds = r + 1;
+            return;
+        }
+    }
+    if (threadIdx.x == 0) atomicOr(status, STATUS_COLOR_STALL);
+}
+
+// Segment key of every manifold: its colour, or (world, colour) when the solver runs one workgroup per
+// world.  Counts per segment and the number of colours are accumulated for the segment table.
+__global__ void sv_color_keys(const int* color, const int* nm_ptr, DevColliders C, DevPairs P, const int* mlist,
+                              int world_major, uint64_t* keys, int* vals, int* seg_cnt, int* ncolors) {
+    __shared__ int hist[kMaxColors];
+    for (int i = threadIdx.x; i < kMaxColors; i += blockDim.x) hist[i] = 0;
+    __syncthreads();
+    const int nm = *nm_ptr;
+    int cmax = -1;
     RP3DG_GRID_STRIDE(m, nm) {
-        keys[m] = (uint64_t)(uint32_t)color[m];
-        vals[m] = m;
-    }
-}
-
-// colour ranges from the sorted keys
-__global__ void sv_color_ranges(const uint64_t* keys, const int* nm_ptr, int* start, int* ncolors) {
-    const int nm = *nm_ptr;
-    RP3DG_GRID_STRIDE(i, nm) {
-        const int c = (int)keys[i];
-        if (i == 0 || (int)keys[i - 1] != c) start[c] = i;
-        if (i == nm - 1) { start[c + 1] = nm; *ncolors = c + 1; }
-    }
-}
-
-// ------------------------------------------------------------------------------------------------
-// Solver rows (ContactSolverSystem::initializeForIsland, CS.cpp:115-320) in colour order
-// ------------------------------------------------------------------------------------------------
-
-__device__ __forceinline__ M3 body_iw(const DevBodies& B, int b) {
-    M3 m;
-    m.r[0] = ld3(B.iw0, b); m.r[1] = ld3(B.iw1, b); m.r[2] = ld3(B.iw2, b);
-    return m;
-}
-
-// computeFrictionVectors (CS.cpp:789-818)
-__device__ inline void friction_vectors(V3 dv, V3 n, V3& t1, V3& t2) {
-    const float dvn = dot(dv, n);
-    const V3 nv = dvn * n;
-    const V3 tv = v3(dv.x - nv.x, dv.y - nv.y, dv.z - nv.z);
-    const float lt = len(tv);
-    if (lt > kMachineEps) t1 = tv / lt;
-    else t1 = one_unit_orthogonal(n);
-    t2 = cross(n, t1);
-}
-
-__device__ __forceinline__ V3 rel_vel(V3 v1, V3 w1, V3 v2, V3 w2, V3 r1, V3 r2) {
-    return v3(v2.x + w2.y * r2.z - w2.z * r2.y - v1.x - w1.y * r1.z + w1.z * r1.y,
-              v2.y + w2.z * r2.x - w2.x * r2.z - v1.y - w1.z * r1.x + w1.x * r1.z,
-              v2.z + w2.x * r2.y - w2.y * r2.x - v1.z - w1.x * r1.y + w1.y * r1.x);
-}
-
-__global__ void sv_prepare(SvArgs A) {
-    const int nm = *A.nm;
-    RP3DG_GRID_STRIDE(i, nm) {
-        const int m = A.order[i];
-        const int k = A.mlist[m];
-        const DevBodies& B = A.B;
-        const DevColliders& C = A.C;
-        const DevPairs& P = A.P;
-        DevSolver& S = A.S;
-        const int2 c = P.col[k];
-        const int b1 = C.body[c.x], b2 = C.body[c.y];
-        const V3 x1 = ld3(B.x, b1), x2 = ld3(B.x, b2);
-        const M3 I1 = body_iw(B, b1), I2 = body_iw(B, b2);
-        const float im1 = B.ii[b1].w, im2 = B.ii[b2].w;
-        const float2 m1 = C.mat[c.x], m2 = C.mat[c.y];
-        const float mu = m1.x * m2.x;
-        const float rest = (m1.y > m2.y) ? m1.y : m2.y;
-        const V3 v1 = ld3(B.v, b1), w1 = ld3(B.w, b1), v2 = ld3(B.v, b2), w2 = ld3(B.w, b2);
-        const Tf T1 = tf(ld3(C.wpos, c.x), ldq(C.wq, c.x));
-        const Tf T2 = tf(ld3(C.wpos, c.y), ldq(C.wq, c.y));
-        const int np = P.ncp[k];
-        V3 nsum = v3zero(), fp1 = v3zero(), fp2 = v3zero();
-        for (int p = 0; p < np; p++) {
-            const float4 pn = P.pn[4 * k + p];
-            const float4 l1 = P.pl1[4 * k + p];
-            const float4 l2 = P.pl2[4 * k + p];
-            const V3 n = xyz(pn);
-            const V3 p1 = tapply(T1, xyz(l1));
-            const V3 p2 = tapply(T2, xyz(l2));
-            const V3 r1 = v3(p1.x - x1.x, p1.y - x1.y, p1.z - x1.z);
-            const V3 r2 = v3(p2.x - x2.x, p2.y - x2.y, p2.z - x2.z);
-            fp1.x += p1.x; fp1.y += p1.y; fp1.z += p1.z;
-            fp2.x += p2.x; fp2.y += p2.y; fp2.z += p2.z;
-            const V3 dv = rel_vel(v1, w1, v2, w2, r1, r2);
-            const V3 i1n = mvmul(I1, cross(r1, n));
-            const V3 i2n = mvmul(I2, cross(r2, n));
-            const float mp = im1 + im2 + dot(cross(i1n, r1), n) + dot(cross(i2n, r2), n);
-            const float imp = mp > 0.0f ? 1.0f / mp : 0.0f;
-            float rb = 0.0f;
-            const float dvn = dv.x * n.x + dv.y * n.y + dv.z * n.z;
-            if (dvn < -A.restitution_threshold) rb = rest * dvn;
-            nsum.x += n.x; nsum.y += n.y; nsum.z += n.z;
-            S.pn[4 * i + p] = pn;
-            S.r1[4 * i + p] = f4(r1, imp);
-            S.r2[4 * i + p] = f4(r2, rb);
-            S.i1n[4 * i + p] = f4(i1n, l2.w);   // w: resting flag from the warm-start transfer
-            S.i2n[4 * i + p] = f4(i2n, 0.f);
-            S.lam[4 * i + p] = make_float4(l1.w, 0.0f, 0.f, 0.f);
-        }
-        const float fn = (float)np;
-        fp1 = v3(fp1.x / fn, fp1.y / fn, fp1.z / fn);
-        fp2 = v3(fp2.x / fn, fp2.y / fn, fp2.z / fn);
-        const V3 r1f = v3(fp1.x - x1.x, fp1.y - x1.y, fp1.z - x1.z);
-        const V3 r2f = v3(fp2.x - x2.x, fp2.y - x2.y, fp2.z - x2.z);
-        const V3 n = normalized(nsum);
-        const V3 dvf = rel_vel(v1, w1, v2, w2, r1f, r2f);
-        V3 t1, t2;
-        friction_vectors(dvf, n, t1, t2);
-        const V3 r1ct1 = cross(r1f, t1), r1ct2 = cross(r1f, t2);
-        const V3 r2ct1 = cross(r2f, t1), r2ct2 = cross(r2f, t2);
-        const float f1m = im1 + im2 + dot(cross(mvmul(I1, r1ct1), r1f), t1) + dot(cross(mvmul(I2, r2ct1), r2f), t1);
-        const float f2m = im1 + im2 + dot(cross(mvmul(I1, r1ct2), r1f), t2) + dot(cross(mvmul(I2, r2ct2), r2f), t2);
-        const float twm = dot(n, mvmul(I1, n)) + dot(n, mvmul(I2, n));
-        S.pair[i] = k;
-        S.bodies[i] = make_int2(b1, b2);
-        S.npts[i] = np;
-        S.n[i] = f4(n, mu);
-        S.r1f[i] = f4(r1f, im1);
-        S.r2f[i] = f4(r2f, im2);
-        S.t1[i] = f4(t1, f1m > 0.0f ? 1.0f / f1m : 0.0f);
-        S.t2[i] = f4(t2, f2m > 0.0f ? 1.0f / f2m : 0.0f);
-        S.r1ct1[i] = f4(r1ct1, twm > 0.0f ? 1.0f / twm : 0.0f);
-        S.r1ct2[i] = f4(r1ct2, 0.f);
-        S.r2ct1[i] = f4(r2ct1, 0.f);
-        S.r2ct2[i] = f4(r2ct2, 0.f);
-        const float4 fi = P.fimp[k];
-        S.imp[i] = make_float4(fi.x, fi.y, fi.z, 0.f);
-        S.oldt1[i] = P.fv1[k];
-        S.oldt2[i] = P.fv2[k];
-    }
-}
-
-// ------------------------------------------------------------------------------------------------
-// Per-colour sweeps
-// ------------------------------------------------------------------------------------------------
-
-struct BodyVel {
-    V3 v, w;
-};
-
-__device__ __forceinline__ void load_body(const DevBodies& B, int b, const float4* lin, const float4* ang, V3& v, V3& w) {
-    v = ld3(lin, b);
-    w = ld3(ang, b);
-}
-
-// ContactSolverSystem::warmStart (CS.cpp:326-476) for the manifolds [lo, hi) of one colour
-__global__ void sv_warmstart(SvArgs A, int color) {
-    const int lo = A.cstart[color], hi = A.cstart[color + 1];
-    const int cnt = A.ncolors[0] > color ? hi - lo : 0;
-    RP3DG_GRID_STRIDE(t, cnt) {
-        const int i = lo + t;
-        const DevBodies& B = A.B;
-        DevSolver& S = A.S;
-        const int2 bb = S.bodies[i];
-        const int b1 = bb.x, b2 = bb.y;
-        const bool d1 = dyn_body(B, b1), d2 = dyn_body(B, b2);
-        V3 v1 = ld3(B.cv, b1), w1 = ld3(B.cw, b1), v2 = ld3(B.cv, b2), w2 = ld3(B.cw, b2);
-        const float im1 = S.r1f[i].w, im2 = S.r2f[i].w;
-        const V3 ll1 = ld3(B.llock, b1), al1 = ld3(B.alock, b1), ll2 = ld3(B.llock, b2), al2 = ld3(B.alock, b2);
-        const int np = S.npts[i];
-        bool any = false;
-        for (int p = 0; p < np; p++) {
-            const int q = 4 * i + p;
-            const float4 i1n4 = S.i1n[q];
-            if (i1n4.w != 0.0f) {
-                any = true;
-                const float lam = S.lam[q].x;
-                const V3 n = xyz(S.pn[q]);
-                const V3 i1n = xyz(i1n4), i2n = ld3(S.i2n, q);
-                const V3 ip = v3(n.x * lam, n.y * lam, n.z * lam);
-                v1.x -= im1 * ip.x * ll1.x; v1.y -= im1 * ip.y * ll1.y; v1.z -= im1 * ip.z * ll1.z;
-                w1.x -= i1n.x * al1.x * lam; w1.y -= i1n.y * al1.y * lam; w1.z -= i1n.z * al1.z * lam;
-                v2.x += im2 * ip.x * ll2.x; v2.y += im2 * ip.y * ll2.y; v2.z += im2 * ip.z * ll2.z;
-                w2.x += i2n.x * al2.x * lam; w2.y += i2n.y * al2.y * lam; w2.z += i2n.z * al2.z * lam;
-            } else {
-                S.lam[q].x = 0.0f;
-            }
-        }
-        float4 imp = S.imp[i];
-        if (any) {
-            const M3 I1 = body_iw(B, b1), I2 = body_iw(B, b2);
-            const V3 ot1 = ld3(S.oldt1, i), ot2 = ld3(S.oldt2, i);
-            const V3 t1 = ld3(S.t1, i), t2 = ld3(S.t2, i);
-            const V3 oi = v3(imp.x * ot1.x + imp.y * ot2.x, imp.x * ot1.y + imp.y * ot2.y, imp.x * ot1.z + imp.y * ot2.z);
-            imp.x = dot(oi, t1);
-            imp.y = dot(oi, t2);
-            const V3 r1ct1 = ld3(S.r1ct1, i), r2ct1 = ld3(S.r2ct1, i), r1ct2 = ld3(S.r1ct2, i), r2ct2 = ld3(S.r2ct2, i);
-            V3 a1 = v3(-r1ct1.x * imp.x, -r1ct1.y * imp.x, -r1ct1.z * imp.x);
-            V3 l2 = v3(t1.x * imp.x, t1.y * imp.x, t1.z * imp.x);
-            V3 a2 = v3(r2ct1.x * imp.x, r2ct1.y * imp.x, r2ct1.z * imp.x);
-            v1 -= cmul(im1 * l2, ll1);
-            w1 += cmul(al1, mvmul(I1, a1));
-            v2 += cmul(im2 * l2, ll2);
-            w2 += cmul(al2, mvmul(I2, a2));
-            a1 = v3(-r1ct2.x * imp.y, -r1ct2.y * imp.y, -r1ct2.z * imp.y);
-            l2 = v3(t2.x * imp.y, t2.y * imp.y, t2.z * imp.y);
-            a2 = v3(r2ct2.x * imp.y, r2ct2.y * imp.y, r2ct2.z * imp.y);
-            v1.x -= im1 * l2.x * ll1.x; v1.y -= im1 * l2.y * ll1.y; v1.z -= im1 * l2.z * ll1.z;
-            w1 += cmul(al1, mvmul(I1, a1));
-            v2.x += im2 * l2.x * ll2.x; v2.y += im2 * l2.y * ll2.y; v2.z += im2 * l2.z * ll2.z;
-            w2 += cmul(al2, mvmul(I2, a2));
-            const V3 n = ld3(S.n, i);
-            a1 = v3(-n.x * imp.z, -n.y * imp.z, -n.z * imp.z);
-            a2 = v3(n.x * imp.z, n.y * imp.z, n.z * imp.z);
-            w1 += cmul(al1, mvmul(I1, a1));
-            w2 += cmul(al2, mvmul(I2, a2));
-            // the reference applies the twist impulse a second time (CS.cpp:457-466)
-            w1 -= cmul(al1, mvmul(I1, a2));
-            w2 += cmul(al2, mvmul(I2, a2));
+        const int c = color[m];
+        uint64_t seg = (uint64_t)c;
+        if (world_major) {
+            const int w = C.world[P.col[mlist[m]].x];
+            seg = (uint64_t)w * kMaxColors + c;
+            atomicAdd(&seg_cnt[seg], 1);
         } else {
-            imp.x = imp.y = imp.z = 0.0f;
+            atomicAdd(&hist[c], 1);
         }
-        S.imp[i] = imp;
-        if (d1) { B.cv[b1] = f4(v1, 0.f); B.cw[b1] = f4(w1, 0.f); }
-        if (d2) { B.cv[b2] = f4(v2, 0.f); B.cw[b2] = f4(w2, 0.f); }
+        keys[m] = seg;
+        vals[m] = m;
+        cmax = c > cmax ? c : cmax;
     }
-}
-
-// ContactSolverSystem::solve (CS.cpp:479-759), split impulses active, for one colour
-__device__ __forceinline__ void solve_manifold(const SvArgs& A, int i, float beta_dt) {
-    const DevBodies& B = A.B;
-    const DevSolver& S = A.S;
-    const int2 bb = S.bodies[i];
-    const int b1 = bb.x, b2 = bb.y;
-    const bool d1 = dyn_body(B, b1), d2 = dyn_body(B, b2);
-    V3 v1 = ld3(B.cv, b1), w1 = ld3(B.cw, b1), v2 = ld3(B.cv, b2), w2 = ld3(B.cw, b2);
-    V3 sv1 = ld3(B.sv, b1), sw1 = ld3(B.sw, b1), sv2 = ld3(B.sv, b2), sw2 = ld3(B.sw, b2);
-    const float im1 = S.r1f[i].w, im2 = S.r2f[i].w;
-    const V3 ll1 = ld3(B.llock, b1), al1 = ld3(B.alock, b1), ll2 = ld3(B.llock, b2), al2 = ld3(B.alock, b2);
-    const int np = S.npts[i];
-    float sumPen = 0.0f;
-    for (int p = 0; p < np; p++) {
-        const int q = 4 * i + p;
-        const float4 pn = S.pn[q];
-        const V3 n = xyz(pn);
-        const float4 r14 = S.r1[q], r24 = S.r2[q];
-        const V3 r1 = xyz(r14), r2 = xyz(r24);
-        const V3 i1n = ld3(S.i1n, q), i2n = ld3(S.i2n, q);
-        const float ipm = r14.w, rb = r24.w;
-        float4 lam = S.lam[q];
-        const V3 dv = rel_vel(v1, w1, v2, w2, r1, r2);
-        const float Jv = dv.x * n.x + dv.y * n.y + dv.z * n.z;
-        float biasPen = 0.0f;
-        if (pn.w > 0.01f) biasPen = -(beta_dt) * fmax_s(0.0f, pn.w - 0.01f);
-        float dl = -(Jv + rb) * ipm;
-        float lt = lam.x;
-        lam.x = fmax_s(lam.x + dl, 0.0f);
-        dl = lam.x - lt;
-        const V3 li = v3(n.x * dl, n.y * dl, n.z * dl);
-        v1.x -= im1 * li.x * ll1.x; v1.y -= im1 * li.y * ll1.y; v1.z -= im1 * li.z * ll1.z;
-        w1.x -= i1n.x * al1.x * dl; w1.y -= i1n.y * al1.y * dl; w1.z -= i1n.z * al1.z * dl;
-        v2.x += im2 * li.x * ll2.x; v2.y += im2 * li.y * ll2.y; v2.z += im2 * li.z * ll2.z;
-        w2.x += i2n.x * al2.x * dl; w2.y += i2n.y * al2.y * dl; w2.z += i2n.z * al2.z * dl;
-        sumPen += lam.x;
-        // split impulse row
-        const V3 dvs = rel_vel(sv1, sw1, sv2, sw2, r1, r2);
-        const float Jvs = dvs.x * n.x + dvs.y * n.y + dvs.z * n.z;
-        float dls = -(Jvs + biasPen) * ipm;
-        const float lts = lam.y;
-        lam.y = fmax_s(lam.y + dls, 0.0f);
-        dls = lam.y - lts;
-        const V3 ls = v3(n.x * dls, n.y * dls, n.z * dls);
-        sv1.x -= im1 * ls.x * ll1.x; sv1.y -= im1 * ls.y * ll1.y; sv1.z -= im1 * ls.z * ll1.z;
-        sw1.x -= i1n.x * al1.x * dls; sw1.y -= i1n.y * al1.y * dls; sw1.z -= i1n.z * al1.z * dls;
-        sv2.x += im2 * ls.x * ll2.x; sv2.y += im2 * ls.y * ll2.y; sv2.z += im2 * ls.z * ll2.z;
-        sw2.x += i2n.x * al2.x * dls; sw2.y += i2n.y * al2.y * dls; sw2.z += i2n.z * al2.z * dls;
-        S.lam[q] = lam;
-    }
-    const M3 I1 = body_iw(B, b1), I2 = body_iw(B, b2);
-    const V3 r1f = ld3(S.r1f, i), r2f = ld3(S.r2f, i);
-    const float4 t14 = S.t1[i], t24 = S.t2[i], n4 = S.n[i];
-    const V3 t1 = xyz(t14), t2 = xyz(t24), n = xyz(n4);
-    const float mu = n4.w;
-    float4 imp = S.imp[i];
-    // friction 1
-    {
-        const V3 dv = rel_vel(v1, w1, v2, w2, r1f, r2f);
-        const float Jv = dv.x * t1.x + dv.y * t1.y + dv.z * t1.z;
-        float dl = -Jv * t14.w;
-        const float lim = mu * sumPen;
-        const float lt = imp.x;
-        imp.x = fmax_s(-lim, fmin_s(imp.x + dl, lim));
-        dl = imp.x - lt;
-        const V3 r1ct1 = ld3(S.r1ct1, i), r2ct1 = ld3(S.r2ct1, i);
-        const V3 a1 = v3(-r1ct1.x * dl, -r1ct1.y * dl, -r1ct1.z * dl);
-        const V3 l2 = v3(t1.x * dl, t1.y * dl, t1.z * dl);
-        const V3 a2 = v3(r2ct1.x * dl, r2ct1.y * dl, r2ct1.z * dl);
-        v1.x -= im1 * l2.x * ll1.x; v1.y -= im1 * l2.y * ll1.y; v1.z -= im1 * l2.z * ll1.z;
-        const V3 aw1 = cmul(al1, mvmul(I1, a1));
-        w1.x += aw1.x; w1.y += aw1.y; w1.z += aw1.z;
-        v2.x += im2 * l2.x * ll2.x; v2.y += im2 * l2.y * ll2.y; v2.z += im2 * l2.z * ll2.z;
-        const V3 aw2 = cmul(al2, mvmul(I2, a2));
-        w2.x += aw2.x; w2.y += aw2.y; w2.z += aw2.z;
-    }
-    // friction 2
-    {
-        const V3 dv = rel_vel(v1, w1, v2, w2, r1f, r2f);
-        const float Jv = dv.x * t2.x + dv.y * t2.y + dv.z * t2.z;
-        float dl = -Jv * t24.w;
-        const float lim = mu * sumPen;
-        const float lt = imp.y;
-        imp.y = fmax_s(-lim, fmin_s(imp.y + dl, lim));
-        dl = imp.y - lt;
-        const V3 r1ct2 = ld3(S.r1ct2, i), r2ct2 = ld3(S.r2ct2, i);
-        const V3 a1 = v3(-r1ct2.x * dl, -r1ct2.y * dl, -r1ct2.z * dl);
-        const V3 l2 = v3(t2.x * dl, t2.y * dl, t2.z * dl);
-        const V3 a2 = v3(r2ct2.x * dl, r2ct2.y * dl, r2ct2.z * dl);
-        v1.x -= im1 * l2.x * ll1.x; v1.y -= im1 * l2.y * ll1.y; v1.z -= im1 * l2.z * ll1.z;
-        const V3 aw1 = cmul(al1, mvmul(I1, a1));
-        w1.x += aw1.x; w1.y += aw1.y; w1.z += aw1.z;
-        v2.x += im2 * l2.x * ll2.x; v2.y += im2 * l2.y * ll2.y; v2.z += im2 * l2.z * ll2.z;
-        const V3 aw2 = cmul(al2, mvmul(I2, a2));
-        w2.x += aw2.x; w2.y += aw2.y; w2.z += aw2.z;
-    }
-    // twist
-    {
-        const V3 dv = w2 - w1;
-        const float Jv = dv.x * n.x + dv.y * n.y + dv.z * n.z;
-        float dl = -Jv * S.r1ct1[i].w;
-        const float lim = mu * sumPen;
-        const float lt = imp.z;
-        imp.z = fmax_s(-lim, fmin_s(imp.z + dl, lim));
-        dl = imp.z - lt;
-        const V3 a2 = v3(n.x * dl, n.y * dl, n.z * dl);
-        const V3 aw1 = cmul(al1, mvmul(I1, a2));
-        w1.x -= aw1.x; w1.y -= aw1.y; w1.z -= aw1.z;
-        const V3 aw2 = cmul(al2, mvmul(I2, a2));
-        w2.x += aw2.x; w2.y += aw2.y; w2.z += aw2.z;
-    }
-    imp.w = sumPen;
-    S.imp[i] = imp;
-    if (d1) {
-        B.cv[b1] = f4(v1, 0.f); B.cw[b1] = f4(w1, 0.f);
-        B.sv[b1] = f4(sv1, 0.f); B.sw[b1] = f4(sw1, 0.f);
-    }
-    if (d2) {
-        B.cv[b2] = f4(v2, 0.f); B.cw[b2] = f4(w2, 0.f);
-        B.sv[b2] = f4(sv2, 0.f); B.sw[b2] = f4(sw2, 0.f);
-    }
-}
-
-__global__ void sv_solve_color(SvArgs A, int color, float beta_dt) {
-    const int lo = A.cstart[color], hi = A.cstart[color + 1];
-    const int cnt = A.ncolors[0] > color ? hi - lo : 0;
-    RP3DG_GRID_STRIDE(t, cnt) solve_manifold(A, lo + t, beta_dt);
-}
-
-// ContactSolverSystem::storeImpulses (CS.cpp:763-785); initializeForIsland marks every point resting (:161)
-__global__ void sv_store(SvArgs A) {
-    const int nm = *A.nm;
-    RP3DG_GRID_STRIDE(i, nm) {
-        const int k = A.S.pair[i];
-        const int np = A.S.npts[i];
-        for (int p = 0; p < np; p++) {
-            float4 l1 = A.P.pl1[4 * k + p];
-            l1.w = A.S.lam[4 * i + p].x;
-            A.P.pl1[4 * k + p] = l1;
-            float4 l2 = A.P.pl2[4 * k + p];
-            l2.w = 1.0f;
-            A.P.pl2[4 * k + p] = l2;
-        }
-        const float4 imp = A.S.imp[i];
-        A.P.fimp[k] = make_float4(imp.x, imp.y, imp.z, 0.f);
-        A.P.fv1[k] = f4(ld3(A.S.t1, i), 0.f);
-        A.P.fv2[k] = f4(ld3(A.S.t2, i), 0.f);
-    }
+    cmax = wave_max(cmax);
+    if (lane_id() == 0 && cmax >= 0) atomicMax(ncolors, cmax + 1);
+    __syncthreads();
+    if (!world_major)
+        for (int i = threadIdx.x; i < kMaxColors; i += blockDim.x)
+            if (hist[i]) atomicAdd(&seg_cnt[i], hist[i]);
 }
 
 }  // namespace rp3dg

@@ -38,6 +38,14 @@ class Counts(C.Structure):
                [("nb_tests", C.c_int64 * 7)]
 
 
+class BatchOptions(C.Structure):
+    _fields_ = [("solver", C.c_int32), ("fused_block", C.c_int32), ("pairs_per_collider", C.c_int32),
+                ("reserved", C.c_int32 * 5)]
+
+
+SOLVER_MODES = {"auto": 0, "per_color": 1, "fused": 2}
+
+
 class ContactView(C.Structure):
     _fields_ = [("colliders", C.POINTER(C.c_int32)), ("nb_points", C.POINTER(C.c_int32)),
                 ("color", C.POINTER(C.c_int32)), ("normal_depth", C.POINTER(C.c_float)),
@@ -52,7 +60,8 @@ EXPORTED = [
     "rp3dg_body_set_damping", "rp3dg_body_enable_gravity", "rp3dg_body_apply_force_torque",
     "rp3dg_body_set_lock_factors", "rp3dg_body_update_mass_properties", "rp3dg_body_set_mass_properties",
     "rp3dg_collider_add", "rp3dg_collider_set_material", "rp3dg_collider_set_filter", "rp3dg_hull_create",
-    "rp3dg_joint_create", "rp3dg_world_update", "rp3dg_batch_create", "rp3dg_batch_destroy", "rp3dg_batch_step",
+    "rp3dg_joint_create", "rp3dg_world_update", "rp3dg_batch_create", "rp3dg_batch_create_ex",
+    "rp3dg_batch_destroy", "rp3dg_batch_step",
     "rp3dg_batch_step_graph", "rp3dg_batch_synchronize", "rp3dg_batch_get_counts", "rp3dg_batch_get_bodies",
     "rp3dg_batch_set_bodies", "rp3dg_batch_get_pairs", "rp3dg_batch_get_contacts", "rp3dg_batch_get_joint_order",
     "rp3dg_batch_profile", "rp3dg_batch_get_kernel_times", "rp3dg_batch_set_residual_mode",
@@ -100,6 +109,7 @@ def load():
         "rp3dg_joint_create": (i32, [vp, i32, i32, i32, fptr, fptr, i32, i32, fptr]),
         "rp3dg_world_update": (C.c_int, [vp, f]),
         "rp3dg_batch_create": (vp, [C.POINTER(vp), i32]),
+        "rp3dg_batch_create_ex": (vp, [C.POINTER(vp), i32, C.POINTER(BatchOptions)]),
         "rp3dg_batch_destroy": (None, [vp]),
         "rp3dg_batch_step": (C.c_int, [vp, f, i32]),
         "rp3dg_batch_step_graph": (C.c_int, [vp, f, i32]),
@@ -113,7 +123,8 @@ def load():
         "rp3dg_batch_profile": (C.c_int, [vp, i32]),
         "rp3dg_batch_get_kernel_times": (C.c_int, [vp, C.c_char_p, C.POINTER(C.c_double), C.POINTER(i64)]),
         "rp3dg_batch_set_residual_mode": (C.c_int, [vp, f, i32]),
-        "rp3dg_batch_get_iteration_stats": (C.c_int, [vp, C.POINTER(C.c_double), C.POINTER(C.c_double)]),
+        "rp3dg_batch_get_iteration_stats": (C.c_int, [vp, C.POINTER(C.c_double), C.POINTER(C.c_double),
+                                                      C.POINTER(C.c_double)]),
         "rp3dg_last_error": (C.c_char_p, []),
         "rp3dg_version": (C.c_char_p, []),
     }
@@ -189,17 +200,24 @@ def world_from_scene(sc: S.Scene):
 class Batch:
     """One or many worlds stepped together on the GPU."""
 
-    def __init__(self, scenes):
+    def __init__(self, scenes, solver="auto", fused_block=0, pairs_per_collider=0):
         L = load()
         if isinstance(scenes, S.Scene):
             scenes = [scenes]
         self.scenes = list(scenes)
         self.worlds = [world_from_scene(sc) for sc in self.scenes]
         arr = (C.c_void_p * len(self.worlds))(*self.worlds)
-        self.h = L.rp3dg_batch_create(arr, len(self.worlds))
+        opt = BatchOptions()
+        opt.solver = SOLVER_MODES[solver]
+        opt.fused_block = int(fused_block)
+        opt.pairs_per_collider = int(pairs_per_collider)
+        self.h = L.rp3dg_batch_create_ex(arr, len(self.worlds), C.byref(opt))
         if not self.h:
             raise RuntimeError("rp3dg_batch_create failed: " + last_error())
         self.nb = sum(sc.nb_bodies for sc in self.scenes)
+        self.body_base = np.cumsum([0] + [sc.nb_bodies for sc in self.scenes])
+        self.collider_base = np.cumsum([0] + [len(sc.colliders) for sc in self.scenes])
+        self.joint_base = np.cumsum([0] + [len(sc.joints) for sc in self.scenes])
 
     def close(self):
         L = load()
@@ -269,6 +287,14 @@ class Batch:
 
     def profile(self, enable=True):
         _check(load().rp3dg_batch_profile(self.h, int(enable)), "profile")
+
+    def set_residual_mode(self, threshold, max_iterations=64):
+        _check(load().rp3dg_batch_set_residual_mode(self.h, float(threshold), int(max_iterations)), "residual_mode")
+
+    def iteration_stats(self):
+        a, b, c = C.c_double(), C.c_double(), C.c_double()
+        _check(load().rp3dg_batch_get_iteration_stats(self.h, C.byref(a), C.byref(b), C.byref(c)), "iteration_stats")
+        return {"mean_iterations": a.value, "max_iterations": b.value, "max_residual": c.value}
 
     def kernel_time(self, name):
         ms = C.c_double()

@@ -1,0 +1,105 @@
+"""Summarise rocprofv3 results (rocpd SQLite databases) as short markdown.
+
+    python -m reactphysics3d_amd.profile_summary --trace DB [--fetch DB --write DB] [--copy-bytes N]
+                                                 [--command "..."] [--top 25]
+
+--trace  : database of a `rocprofv3 --kernel-trace --stats` run; prints per-kernel calls, mean and
+           total duration and share of GPU time.
+--fetch / --write : databases of `rocprofv3 --pmc FETCH_SIZE` and `--pmc WRITE_SIZE` runs of the same
+           command; prints the mean KiB per dispatch for every kernel.
+--copy-bytes : size in bytes of the plain elementwise copy that the profiled command ran
+           (`bench.py --copy-probe`): its measured FETCH/WRITE KiB divided by the known bytes gives
+           the scale factor that turns the counters into HBM bytes.
+"""
+from __future__ import annotations
+
+import argparse
+import re
+import sqlite3
+import sys
+from collections import defaultdict
+
+
+def short_name(name: str) -> str:
+    """`rp3dg::np_poly_poly(rp3dg::NpArgs, ...)` -> `np_poly_poly`; torch kernels keep a short tag."""
+    base = name.split("(")[0]
+    base = re.sub(r"<.*", "", base)
+    return base.split("::")[-1][:60]
+
+
+def kernel_trace(db: str):
+    c = sqlite3.connect(db)
+    rows = c.execute("select name, total_calls, total_duration, average, percentage from top_kernels").fetchall()
+    agg = defaultdict(lambda: [0, 0.0])
+    for name, calls, total, _avg, _pct in rows:
+        k = short_name(name)
+        agg[k][0] += int(calls)
+        agg[k][1] += float(total)
+    return agg
+
+
+def counter_means(db: str, counter: str):
+    c = sqlite3.connect(db)
+    rows = c.execute("select kernel_name, value from counters_collection where counter_name = ?", (counter,)).fetchall()
+    agg = defaultdict(list)
+    for name, v in rows:
+        agg[short_name(name)].append(float(v))
+    return {k: (sum(v) / len(v), len(v)) for k, v in agg.items()}
+
+
+def copy_kernel(means):
+    for k in means:
+        if "elementwise" in k or "vectorized" in k:
+            return k
+    return None
+
+
+def main(argv=None):
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--trace")
+    ap.add_argument("--fetch")
+    ap.add_argument("--write")
+    ap.add_argument("--copy-bytes", type=float, default=0.0)
+    ap.add_argument("--command", default="")
+    ap.add_argument("--top", type=int, default=25)
+    a = ap.parse_args(argv)
+    out = []
+    if a.command:
+        out += ["Command:", "", "```", a.command, "```", ""]
+    if a.trace:
+        agg = kernel_trace(a.trace)
+        tot = sum(v[1] for v in agg.values()) or 1.0
+        out += ["| kernel | calls | mean us | total ms | share |", "|---|---:|---:|---:|---:|"]
+        for k, (n, t) in sorted(agg.items(), key=lambda kv: -kv[1][1])[: a.top]:
+            out.append(f"| {k} | {n} | {t / max(n, 1) / 1e3:.1f} | {t / 1e6:.3f} | {100 * t / tot:.1f}% |")
+        out.append("")
+    if a.fetch or a.write:
+        f = counter_means(a.fetch, "FETCH_SIZE") if a.fetch else {}
+        w = counter_means(a.write, "WRITE_SIZE") if a.write else {}
+        scale_f = scale_w = None
+        ck = copy_kernel(f) or copy_kernel(w)
+        if ck and a.copy_bytes > 0:
+            if ck in f:
+                scale_f = a.copy_bytes / (f[ck][0] * 1024.0)
+            if ck in w:
+                scale_w = a.copy_bytes / (w[ck][0] * 1024.0)
+            out.append(f"Copy calibration: kernel `{ck}` moving {a.copy_bytes / 2**20:.0f} MiB each way; "
+                       f"bytes per counted KiB: fetch {1024 * scale_f if scale_f else float('nan'):.1f}, "
+                       f"write {1024 * scale_w if scale_w else float('nan'):.1f}")
+            out.append("")
+        out += ["| kernel | dispatches | FETCH_SIZE KiB | WRITE_SIZE KiB | HBM MB/dispatch (scaled) |",
+                "|---|---:|---:|---:|---:|"]
+        keys = sorted(set(f) | set(w), key=lambda k: -(f.get(k, (0, 0))[0] + w.get(k, (0, 0))[0]))
+        for k in keys[: a.top]:
+            fk = f.get(k, (float("nan"), 0))
+            wk = w.get(k, (float("nan"), 0))
+            hbm = ""
+            if scale_f and scale_w:
+                hbm = f"{(fk[0] * 1024 * scale_f + wk[0] * 1024 * scale_w) / 1e6:.3f}"
+            out.append(f"| {k} | {max(fk[1], wk[1])} | {fk[0]:.1f} | {wk[0]:.1f} | {hbm} |")
+        out.append("")
+    sys.stdout.write("\n".join(out) + "\n")
+
+
+if __name__ == "__main__":
+    main()

@@ -249,6 +249,75 @@ def jointed_tower(height=6, seed=4, kinds=(HINGE, FIXED), friction=0.5, gravity=
     return sc
 
 
+def batched_world(world_id, seed=4, min_bodies=8, max_bodies=512, max_joints=32):
+    """One world of the many-world batch: n = round(exp(U[ln min, ln max])) unit boxes (1 kg, friction
+    0.5, restitution 0) on a static ground, either a pile (seeded grid, jitter 0.2 m, random
+    orientations, 1.5 m spacing) or stacks of height U{4..20}; U{0..max_joints} hinge / fixed joints
+    between neighbouring boxes.  Stream seed = seed + world_id."""
+    rng = SplitMix64(seed + world_id)
+    lo, hi = math.log(min_bodies), math.log(max_bodies)
+    n = int(round(math.exp(lo + (hi - lo) * rng.uniform01())))
+    n = max(min_bodies, min(max_bodies, n))
+    stack = rng.uniform01() < 0.5
+    sc = Scene(f"batched_{world_id}", make_settings(gravity=(0.0, 0.0, -9.81), vel_iters=10, pos_iters=5,
+                                                    friction=0.5, bounciness=0.0))
+    g = sc.add_body(STATIC, (0.0, 0.0, -0.5))
+    sc.add_collider(g, BOX, (50.0, 50.0, 0.5), friction=0.5, bounciness=0.0)
+    cand = []   # neighbouring body pairs a joint may link, with the anchor between them
+    if stack:
+        h = 4 + min(16, int(17 * rng.uniform01()))
+        cols = (n + h - 1) // h
+        side = int(math.ceil(math.sqrt(cols)))
+        k = 0
+        for c in range(cols):
+            cx, cy = (c % side - (side - 1) / 2.0) * 2.0, (c // side - (side - 1) / 2.0) * 2.0
+            prev = None
+            for lvl in range(h):
+                if k == n:
+                    break
+                pos = (cx + rng.uniform(-0.05, 0.05), cy + rng.uniform(-0.05, 0.05), 0.5 + 1.01 * lvl)
+                b = sc.add_body(DYNAMIC, pos)
+                sc.add_collider(b, BOX, (0.5, 0.5, 0.5), friction=0.5, bounciness=0.0, density=1.0)
+                if prev is not None:
+                    cand.append((prev, b, (pos[0], pos[1], pos[2] - 0.505)))
+                prev = b
+                k += 1
+    else:
+        levels = 1 + min(7, int(8 * rng.uniform01()))
+        side = int(math.ceil(math.sqrt(n / levels)))
+        half = (side - 1) / 2.0
+        k = 0
+        grid = {}
+        for lvl in range(levels + 1):
+            for j in range(side):
+                for i in range(side):
+                    if k == n:
+                        break
+                    pos = ((i - half) * 1.5 + rng.uniform(-0.2, 0.2), (j - half) * 1.5 + rng.uniform(-0.2, 0.2),
+                           1.0 + 1.5 * lvl + rng.uniform(-0.2, 0.2))
+                    b = sc.add_body(DYNAMIC, pos, rng.quaternion())
+                    sc.add_collider(b, BOX, (0.5, 0.5, 0.5), friction=0.5, bounciness=0.0, density=1.0)
+                    grid[(i, j, lvl)] = (b, pos)
+                    if (i - 1, j, lvl) in grid:
+                        a, pa = grid[(i - 1, j, lvl)]
+                        cand.append((a, b, tuple((pa[t] + pos[t]) * 0.5 for t in range(3))))
+                    k += 1
+    nj = min(len(cand), int((max_joints + 1) * rng.uniform01()))
+    for t in range(nj):   # partial Fisher-Yates over the candidates
+        r = t + min(len(cand) - 1 - t, int((len(cand) - t) * rng.uniform01()))
+        cand[t], cand[r] = cand[r], cand[t]
+        a, b, anchor = cand[t]
+        kind = HINGE if rng.uniform01() < 0.5 else FIXED
+        ang = 2.0 * math.pi * rng.uniform01()
+        sc.add_joint(kind, a, b, anchor, axis=(math.cos(ang), math.sin(ang), 0.0))
+    return sc
+
+
+def batched_worlds(n_worlds=4096, seed=4, **kw):
+    """The many-world batch: world w is batched_world(w, seed)."""
+    return [batched_world(w, seed, **kw) for w in range(n_worlds)]
+
+
 def two_body(shape1, params1, shape2, params2, pos1, pos2, q1=(0, 0, 0, 1), q2=(0, 0, 0, 1), static2=False):
     """Two bodies, no gravity: the unit case of each narrowphase pair type."""
     sc = Scene("two_body", make_settings(gravity=(0.0, 0.0, 0.0)))

@@ -1,0 +1,103 @@
+"""Many uneven worlds in one batch (box piles and stacks with hinge / fixed joints): every world is
+checked against its own reference PhysicsWorld fed that world's share of the device's contact and
+joint order, for both solver layouts; graph replay is checked against eager stepping."""
+import numpy as np
+import pytest
+
+from reactphysics3d_amd import scene as S
+from tests.parity import compare_contacts, pair_set
+
+pytestmark = pytest.mark.gpu
+
+DT = 1.0 / 60.0
+
+
+def _world_views(dev, ct, order):
+    """Splits global contacts / joint order into per-world, world-local pieces (order preserved)."""
+    out = []
+    for w in range(len(dev.scenes)):
+        c0, c1 = dev.collider_base[w], dev.collider_base[w + 1]
+        sel = (ct["colliders"][:, 0] >= c0) & (ct["colliders"][:, 0] < c1)
+        local = {k: a[sel] for k, a in ct.items()}
+        local["colliders"] = local["colliders"] - c0
+        j0, j1 = dev.joint_base[w], dev.joint_base[w + 1]
+        jo = order[(order >= j0) & (order < j1)] - j0
+        out.append((local, jo))
+    return out
+
+
+def _run(scenes, steps, solver, tol=1e-4):
+    from oracle.ref import RefWorld
+    from reactphysics3d_amd.lib import Batch
+    dev = Batch(scenes, solver=solver)
+    refs = [RefWorld(sc) for sc in scenes]
+    order = dev.joint_order()
+    for (_, jo), ref in zip(_world_views(dev, {"colliders": np.zeros((0, 2), np.int32)}, order), refs):
+        ref.set_joint_order(jo)
+    worst = 0.0
+    for k in range(steps):
+        dev.step(DT)
+        ct = dev.contacts()
+        views = _world_views(dev, ct, order)
+        p1, p2 = dev.pairs()
+        pos, q, v, w = dev.bodies()
+        for wi, (ref, (local, _)) in enumerate(zip(refs, views)):
+            ref.step_ordered(DT, local["colliders"][:, 0], local["colliders"][:, 1])
+            c0, c1 = dev.collider_base[wi], dev.collider_base[wi + 1]
+            sel = (p1 >= c0) & (p1 < c1)
+            r1, r2 = ref.pairs()
+            assert pair_set(p1[sel] - c0, p2[sel] - c0) == pair_set(r1, r2), f"step {k} world {wi}: pair sets differ"
+            errs = compare_contacts(local, ref.contacts())
+            assert not errs, f"step {k} world {wi}: " + "; ".join(errs[:5])
+            b0, b1 = dev.body_base[wi], dev.body_base[wi + 1]
+            dev_state = np.concatenate([a[b0:b1].ravel() for a in (pos, q, v, w)])
+            ref_state = np.concatenate([a.ravel() for a in ref.bodies()])
+            worst = max(worst, float(np.max(np.abs(dev_state - ref_state))))
+            assert worst <= tol, f"step {k} world {wi}: body state differs by {worst}"
+    return dev, worst
+
+
+@pytest.mark.parametrize("solver", ["fused", "per_color"])
+def test_uneven_worlds_match_reference(solver):
+    scenes = [S.batched_world(w, seed=100, min_bodies=8, max_bodies=48, max_joints=8) for w in range(12)]
+    dev, worst = _run(scenes, 60, solver)
+    c = dev.counts()
+    print(solver, "worst", worst, c)
+    assert c["nb_worlds"] == 12 and c["nb_manifolds"] > 0 and c["nb_joints"] > 0
+
+
+def test_graph_replay_matches_eager():
+    from reactphysics3d_amd.lib import Batch
+    scenes = [S.batched_world(w, seed=7, min_bodies=8, max_bodies=96, max_joints=16) for w in range(24)]
+    eager = Batch(scenes, solver="fused")
+    graph = Batch(scenes, solver="fused")
+    for _ in range(5):
+        eager.step(DT)
+        graph.step(DT)
+    for rep in range(6):
+        eager.step(DT, 2)
+        graph.step_graph(DT, 2)
+        for a, b in zip(eager.bodies(), graph.bodies()):
+            assert np.array_equal(a, b), f"replay {rep}: graph state differs from eager"
+        assert eager.counts() == graph.counts()
+        # eager work between replays on the graph batch itself
+        graph.step(DT)
+        eager.step(DT)
+
+
+def test_residual_mode_converges():
+    from reactphysics3d_amd.lib import Batch
+    scenes = [S.batched_world(w, seed=11, min_bodies=16, max_bodies=128, max_joints=8) for w in range(16)]
+    dev = Batch(scenes, solver="fused")
+    dev.step(DT, 30)
+    dev.set_residual_mode(1e-4, 200)
+    dev.step(DT)
+    st = dev.iteration_stats()
+    print(st)
+    assert 1 <= st["mean_iterations"] <= st["max_iterations"] <= 200
+    # every world either reached the threshold or ran the cap
+    if st["max_iterations"] < 200:
+        assert st["max_residual"] < 1e-4
+    dev.set_residual_mode(0.0)
+    dev.step(DT)
+    assert dev.iteration_stats()["max_iterations"] == 10

@@ -9,10 +9,10 @@ from tests.parity import compare_contacts, pair_set
 pytestmark = pytest.mark.gpu
 
 
-def _run(sc, steps, ref_available):
+def _run(sc, steps, ref_available, solver="auto"):
     from oracle.ref import RefWorld
     from reactphysics3d_amd.lib import Batch
-    dev = Batch(sc)
+    dev = Batch(sc, solver=solver)
     ref = RefWorld(sc)
     dt = 1.0 / 60.0
     worst = dict(pos=0.0, vel=0.0)
@@ -33,7 +33,8 @@ def _run(sc, steps, ref_available):
     return dev, ref, worst
 
 
-def test_box_pile_small(ref_available):
+@pytest.mark.parametrize("solver", ["fused", "per_color"])
+def test_box_pile_small(ref_available, solver):
     sc = S.box_pile(4, 3, seed=11)
-    dev, ref, worst = _run(sc, 120, ref_available)
+    dev, ref, worst = _run(sc, 120, ref_available, solver=solver)
     print("worst", worst, dev.counts())

@@ -9,10 +9,10 @@ from tests.parity import compare_contacts, pair_set
 pytestmark = pytest.mark.gpu
 
 
-def _run(sc, steps, tol=1e-4):
+def _run(sc, steps, tol=1e-4, solver="auto"):
     from oracle.ref import RefWorld
     from reactphysics3d_amd.lib import Batch
-    dev = Batch(sc)
+    dev = Batch(sc, solver=solver)
     ref = RefWorld(sc)
     ref.set_joint_order(dev.joint_order())
     dt = 1.0 / 60.0
@@ -33,13 +33,15 @@ def _run(sc, steps, tol=1e-4):
     return dev, worst
 
 
-def test_ball_socket_chains():
+@pytest.mark.parametrize("solver", ["fused", "per_color"])
+def test_ball_socket_chains(solver):
     sc = S.sphere_chains(n_chains=9, length=8, seed=31)
-    dev, worst = _run(sc, 120)
+    dev, worst = _run(sc, 120, solver=solver)
     print("worst", worst, dev.counts())
 
 
-def test_hinge_fixed_tower():
+@pytest.mark.parametrize("solver", ["fused", "per_color"])
+def test_hinge_fixed_tower(solver):
     sc = S.jointed_tower(height=8, seed=41)
-    dev, worst = _run(sc, 120)
+    dev, worst = _run(sc, 120, solver=solver)
     print("worst", worst, dev.counts())

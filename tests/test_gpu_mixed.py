@@ -9,10 +9,10 @@ from tests.parity import compare_contacts, pair_set
 pytestmark = pytest.mark.gpu
 
 
-def _run(sc, steps, tol_state=1e-4):
+def _run(sc, steps, tol_state=1e-4, solver="auto"):
     from oracle.ref import RefWorld
     from reactphysics3d_amd.lib import Batch
-    dev = Batch(sc)
+    dev = Batch(sc, solver=solver)
     ref = RefWorld(sc)
     dt = 1.0 / 60.0
     worst = 0.0
@@ -32,8 +32,8 @@ def _run(sc, steps, tol_state=1e-4):
     return dev, worst
 
 
-@pytest.mark.parametrize("seed", [2, 5])
-def test_mixed_container_small(seed):
+@pytest.mark.parametrize("seed,solver", [(2, "fused"), (5, "per_color")])
+def test_mixed_container_small(seed, solver):
     sc = S.mixed_container(600, extent=(24.0, 24.0, 12.0), seed=seed)
-    dev, worst = _run(sc, 90)
+    dev, worst = _run(sc, 90, solver=solver)
     print("worst", worst, dev.counts())
