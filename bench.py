@@ -1,14 +1,20 @@
 #!/usr/bin/env python3
 """Benchmark of the MI355X PhysicsWorld::update() step path.
 
-Default workload (one MI355X): 65,536 mixed dynamic bodies (1/3 spheres r~U[0.25,1],
-1/3 boxes half-extents ~U[0.25,1], 1/3 capsules r~U[0.2,0.5] h~U[0.5,2], density
-1000, uniform in a 200 x 200 x 100 m volume, random orientations, v ~ N(0,1))
-inside a static floor + 4 walls, dt = 1/60 s, FP32.  Inputs are resident in HBM
-before the timed region.  A step is one full PhysicsWorld::update(): broadphase,
-narrowphase, contact creation, coloured solver, integration, refit.
+Default workload (one MI355X): 4096 independent worlds in one batch, body counts
+log-uniform in [8, 512] (about 500k bodies), each a seeded box pile or box stack
+(height 4-20) with 0-32 hinge / fixed joints on a static ground, dt = 1/60 s,
+FP32, stepped by a captured hipGraph, every world iterating its velocity solver
+to the residual 1e-4 (iteration cap 100).  Inputs are resident in HBM before the
+timed region.  A step is one full PhysicsWorld::update() of every world:
+broadphase, narrowphase, contact creation, coloured contact + joint solver,
+integration, refit.
 
-Prints one JSON line (see the repository's bench contract).
+Other workloads: --workload mixed (65,536 spheres/boxes/capsules in a walled
+container, 6/3 iterations) and --workload pile (1000 boxes, 10/5 iterations).
+
+Prints one JSON line (see the repository's bench contract); the per-kernel
+breakdown goes to stderr.
 """
 import argparse
 import json
@@ -20,15 +26,18 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0   # MI355X HBM3E, vendor figure
+DT = 1.0 / 60.0
+RESIDUAL = 1e-4
+RESIDUAL_CAP = 100
 
 
 def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-# Algorithmic (compulsory) HBM bytes per launch of the kernels that can dominate a step: every datum
-# a launch must read or write, counted once.  `c` = counts of the step, `nlaunch` = launches per step.
-def bytes_per_launch(name, c):
+# Algorithmic (compulsory) HBM bytes per launch: every datum a launch must read or write, once.
+# c = counts of a step, nj = joints, ndyn = dynamic bodies.
+def bytes_per_launch(name, c, ndyn, nj):
     nt = c["nb_tests"]
     m, p = c["nb_manifolds"], c["nb_contact_points"]
     ncol = max(1, c["nb_colors"])
@@ -36,22 +45,24 @@ def bytes_per_launch(name, c):
     np_types = {"np_sphere_sphere": 1, "np_sphere_capsule": 2, "np_capsule_capsule": 3, "np_sphere_poly": 4,
                 "np_capsule_poly": 5, "np_poly_poly": 6}
     if name in np_types:
-        # 176 B / test (pair record + 2 collider transforms/shapes + cache) + 40 B / raw contact kept
         tests = nt[np_types[name]]
         return 176 * tests + 40 * p * (tests / max(1, sum(nt)))
-    if name == "sv_solve_color":
-        # one sweep over all colours: 204 B of rows per manifold + 112 B per point +
-        # 2 x 212 B of body state per manifold; a launch is one colour
-        return (204 * m + 112 * p + 424 * m) / ncol
-    if name == "sv_warmstart":
-        return (204 * m + 112 * p + 424 * m) / ncol
+    if name == "sv_world_solve":
+        # whole velocity solver of a step, one launch: 48 B / contact point + 88 B / manifold +
+        # 140 B / dynamic body + 200 B / joint
+        return 48 * p + 88 * m + 140 * ndyn + 200 * nj
+    if name == "world_position":
+        # position integration + commit (192 B / dynamic body) and fat-AABB refit (48 B / collider)
+        return 192 * ndyn + 48 * ncl
+    if name in ("sv_solve_color", "sv_warmstart"):
+        return (48 * p + 88 * m + 140 * 2 * m) / ncol
     if name == "sv_prepare":
         return 420 * m + 160 * p
     if name == "bp_traverse":
         return 48 * ncl + 4 * ncl + 8 * npairs / 2
     if name == "bp_merge":
         return npairs * (8 + 8 + 240 * 2)
-    if name in ("dyn_integrate_velocities",):
+    if name == "dyn_integrate_velocities":
         return nb * (16 * 9 + 16 * 5)
     if name == "dyn_integrate_positions":
         return nb * (16 * 6 + 16 * 2)
@@ -62,101 +73,229 @@ def bytes_per_launch(name, c):
     return None
 
 
+BYTE_MODEL = {
+    "sv_world_solve": "48 B/contact point + 88 B/manifold + 140 B/dynamic body + 200 B/joint per launch (one step)",
+    "world_position": "192 B/dynamic body + 48 B/collider per launch (one step)",
+    "np_poly_poly": "176 B/test + 40 B/contact point",
+}
+
 KERNELS = ["bp_bounds", "bp_morton", "bp_sort", "bp_karras", "bp_fit", "bp_world_roots", "bp_traverse", "bp_merge",
            "np_classify", "np_sphere_sphere", "np_sphere_capsule", "np_capsule_capsule", "np_sphere_poly",
            "np_capsule_poly", "np_poly_poly", "sv_flag", "sv_compact", "sv_adj_count", "sv_adj_fill",
-           "sv_color_round", "sv_color_keys", "sv_color_ranges", "dyn_integrate_velocities", "sv_prepare",
-           "sv_warmstart", "sv_solve_color", "sv_store", "dyn_integrate_positions", "dyn_update_state", "bp_refit"]
+           "sv_color_round", "sv_color_finish", "sv_color_keys", "sv_color_sort", "sv_world_solve", "world_position",
+           "dyn_integrate_velocities", "sv_prepare", "sv_warmstart", "sv_solve_color", "sv_store", "jt_init",
+           "jt_warmstart", "jt_solve_velocity", "jt_solve_position", "dyn_integrate_positions", "dyn_update_state",
+           "bp_refit"]
+
+
+def build_workload(args):
+    from reactphysics3d_amd import scene as S
+    if args.workload == "batched":
+        scenes = S.batched_worlds(args.worlds, seed=4)
+        name = f"batched_{args.worlds}_worlds_8_512_bodies"
+    elif args.workload == "mixed":
+        scenes = [S.mixed_container(args.bodies, seed=2)]
+        name = f"mixed_container_{args.bodies}"
+    elif args.workload == "pile":
+        scenes = [S.box_pile(10, 10, seed=1)]
+        name = "box_pile_1000"
+    else:
+        raise SystemExit(f"unknown workload {args.workload}")
+    return name, scenes
+
+
+def breakdown(dev, steps, tag):
+    out = {}
+    for k in KERNELS:
+        ms, n = dev.kernel_time(k)
+        if n:
+            out[k] = (ms, n)
+    total = sum(ms * n for ms, n in out.values())
+    for k, (ms, n) in sorted(out.items(), key=lambda kv: -kv[1][0] * kv[1][1]):
+        log(f"[bench] {tag} {k:26s} {ms * 1e3:9.1f} us x {n / steps:6.1f}/step  {100 * ms * n / max(total, 1e-12):5.1f}%")
+    return out, total
+
+
+def cpu_baseline(scenes, args, ndyn_total):
+    """Reference PhysicsWorld::update() on the host cores: a bounded sample of the worlds, one world per
+    thread at a time (worlds are created serially), fixed WorldSettings iteration counts."""
+    from concurrent.futures import ThreadPoolExecutor
+
+    from oracle import ref
+    if not ref.available():
+        log("[bench] reference harness not built; no CPU baseline")
+        return None
+    threads = args.cpu_threads or min(16, os.cpu_count() or 1)
+    stride = max(1, len(scenes) // args.cpu_worlds) if len(scenes) > 1 else 1
+    sample = scenes[::stride][: args.cpu_worlds] if len(scenes) > 1 else scenes
+    if len(sample) == 1 and len(scenes) == 1:
+        secs, cts = ref.cpu_bench(sample[0], threads, 0, args.cpu_steps, DT)
+        nd = sample[0].nb_dynamic
+        return {"value": round(nd * args.cpu_steps * threads / secs, 1), "unit": "body-steps/s", "cores": threads,
+                "kind": "reference", "contacts_per_s": round(cts / secs, 1),
+                "sample": f"{threads} threads x 1 copy each of the {sample[0].nb_bodies}-body world, first "
+                          f"{args.cpu_steps} steps, ReactPhysics3D 0.10.2 g++ -O3 FP32, fixed iterations"}
+    worlds = [ref.RefWorld(sc) for sc in sample]
+    nd = sum(sc.nb_dynamic for sc in sample)
+
+    def run(w, n):
+        for _ in range(n):
+            w.step(DT)
+
+    with ThreadPoolExecutor(max_workers=threads) as ex:   # same untimed warmup as the device
+        list(ex.map(lambda w: run(w, args.warmup), worlds))
+    t0 = time.perf_counter()
+    with ThreadPoolExecutor(max_workers=threads) as ex:
+        list(ex.map(lambda w: run(w, args.cpu_steps), worlds))
+    secs = time.perf_counter() - t0
+    for w in worlds:
+        w.close()
+    return {"value": round(nd * args.cpu_steps / secs, 1), "unit": "body-steps/s", "cores": threads,
+            "kind": "reference",
+            "sample": f"{len(sample)} of the {len(scenes)} worlds (every {stride}th, {nd} dynamic bodies), "
+                      f"{args.cpu_steps} steps each after {args.warmup} untimed, one world per thread at a time on "
+                      f"{threads} threads, "
+                      f"ReactPhysics3D 0.10.2 g++ -O3 FP32, fixed 10/5 iterations"}
 
 
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=200)
-    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--steps", type=int, default=100)
+    ap.add_argument("--warmup", type=int, default=30)
+    ap.add_argument("--workload", default="batched", choices=["batched", "mixed", "pile"])
+    ap.add_argument("--worlds", type=int, default=4096)
     ap.add_argument("--bodies", type=int, default=65536)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU reference baseline")
-    ap.add_argument("--cpu-steps", type=int, default=5)
+    ap.add_argument("--cpu-steps", type=int, default=10)
+    ap.add_argument("--cpu-worlds", type=int, default=512)
     ap.add_argument("--cpu-threads", type=int, default=0)
-    ap.add_argument("--no-profile", action="store_true")
+    ap.add_argument("--no-profile", action="store_true", help="no HIP event timing at all")
+    ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
+                    help="run a plain elementwise copy of MIB MiB first (counter calibration under rocprofv3)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (this build has no multi-GPU path); got --gpus {args.gpus}",
               file=sys.stderr)
         sys.exit(2)
 
-    from reactphysics3d_amd import scene as S
+    if args.copy_probe:
+        import torch
+        n = args.copy_probe * (1 << 20) // 4
+        src = torch.ones(n, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            torch.mul(src, 1.0, out=dst)
+        torch.cuda.synchronize()
+        log(f"[bench] copy probe: 3 x {args.copy_probe} MiB elementwise copies")
+        del src, dst
+
     from reactphysics3d_amd.lib import Batch
 
-    dt = 1.0 / 60.0
     t0 = time.time()
-    sc = S.mixed_container(args.bodies, seed=2)
-    log(f"[bench] scene {sc.name}: {sc.nb_bodies} bodies built in {time.time() - t0:.1f}s")
+    name, scenes = build_workload(args)
+    ndyn = sum(sc.nb_dynamic for sc in scenes)
+    nj = sum(len(sc.joints) for sc in scenes)
+    log(f"[bench] workload {name}: {len(scenes)} worlds, {sum(sc.nb_bodies for sc in scenes)} bodies, "
+        f"{ndyn} dynamic, {nj} joints, built in {time.time() - t0:.1f}s")
     t0 = time.time()
-    dev = Batch(sc)
+    dev = Batch(scenes)
+    fused = args.workload == "batched" or len(scenes) > 1
     log(f"[bench] uploaded in {time.time() - t0:.1f}s")
     for _ in range(args.warmup):
-        dev.step(dt)
+        dev.step(DT)
     dev.sync()
     log(f"[bench] warmup {args.warmup} steps done; counts {dev.counts()}")
 
+    residual = args.workload == "batched"
+    if residual:
+        dev.set_residual_mode(RESIDUAL, RESIDUAL_CAP)
+    graph = residual or fused
+
+    # short eager pass with every kernel timed: per-kernel breakdown, dominant kernel
+    dominant = None
     if not args.no_profile:
         dev.profile(True)
-    contacts = 0
+        nprof = 5
+        for _ in range(nprof):
+            dev.step(DT)
+        items, _ = breakdown(dev, nprof, "eager")
+        ranked = sorted(items.items(), key=lambda kv: -kv[1][0] * kv[1][1])
+        for k, _ in ranked:
+            if bytes_per_launch(k, dev.counts(), ndyn, nj) is not None:
+                dominant = k
+                break
+        dev.profile(False)
+        if not graph:
+            # eager timed region: the dominant kernel is timed with two HIP events per step
+            dev.profile(True, only=dominant)
+
+    # ---------------- timed region
+    c0 = dev.counts()
     dev.sync()
     t0 = time.perf_counter()
-    for _ in range(args.steps):
-        dev.step(dt)
-        contacts += dev.counts()["nb_contact_points"]
+    if graph:
+        dev.step_graph(DT, args.steps)
+    else:
+        dev.step(DT, args.steps)
     dev.sync()
-    t1 = time.perf_counter()
-    wall = t1 - t0
-    counts = dev.counts()
-    ndyn = sc.nb_dynamic
+    wall = time.perf_counter() - t0
+    c1 = dev.counts()
+    st = dev.iteration_stats()
     value = ndyn * args.steps / wall
-    log(f"[bench] timed {args.steps} steps: {wall * 1e3 / args.steps:.3f} ms/step, {value:.4g} body-steps/s")
+    contacts = c1["total_contact_points"] - c0["total_contact_points"]
+    log(f"[bench] timed {args.steps} steps ({'graph' if graph else 'eager'}): {wall * 1e3 / args.steps:.3f} ms/step, "
+        f"{value:.4g} body-steps/s, {contacts / wall:.4g} contacts/s, iterations {st}")
 
     roofline = None
-    breakdown = {}
-    if not args.no_profile:
-        for k in KERNELS:
-            ms, n = dev.kernel_time(k)
-            if n:
-                breakdown[k] = (ms, n)
-        total = sum(ms * n for ms, n in breakdown.values())
-        for k, (ms, n) in sorted(breakdown.items(), key=lambda kv: -kv[1][0] * kv[1][1]):
-            log(f"[bench]   {k:28s} {ms * 1e3:9.1f} us x {n / args.steps:6.1f}/step  "
-                f"{100 * ms * n / max(total, 1e-12):5.1f}%")
-        ranked = sorted(breakdown.items(), key=lambda kv: -kv[1][0] * kv[1][1])
-        for name, (ms, n) in ranked:
-            b = bytes_per_launch(name, counts)
-            if b is None:
-                continue
-            achieved = b / (ms * 1e-3) / 1e9
-            roofline = {"kernel": name, "bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
-                        "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": None,
-                        "bytes_per_launch": int(b), "mean_launch_us": round(ms * 1e3, 2),
-                        "share_of_step": round(ms * n / max(total, 1e-12), 3)}
-            break
+    timing_note = "HIP events around the kernel on the batch's stream, every step of the timed region"
+    if dominant and graph:
+        # graph replays are not event-timed: time the dominant kernel over an eager pass of the same
+        # number of steps right after the timed region (two HIP events per step)
+        dev.profile(True, only=dominant)
+        dev.step(DT, args.steps)
+        dev.sync()
+        timing_note = ("HIP events around the kernel on the batch's stream, eager pass of the same number of "
+                       "steps directly after the graph-timed region")
+    if dominant:
+        ms, n = dev.kernel_time(dominant)
+        b = bytes_per_launch(dominant, c1, ndyn, nj)
+        achieved = b / (ms * 1e-3) / 1e9
+        traffic = None
+        tpath = os.path.join(ROOT, "profiles", f"traffic_{name}.json")
+        if os.path.exists(tpath):
+            with open(tpath) as f:
+                traffic = json.load(f).get(dominant)
+        roofline = {"kernel": dominant, "bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
+                    "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": traffic,
+                    "bytes_per_launch": int(b), "byte_model": BYTE_MODEL.get(dominant, ""),
+                    "mean_launch_us": round(ms * 1e3, 2), "launches": n, "timing": timing_note,
+                    "share_of_step": round(ms / (wall * 1e3 / args.steps), 3)}
+        dev.profile(False)
+
+    fixed = None
+    if residual:
+        # same worlds with the fixed WorldSettings iteration count (the reference's own mode)
+        dev.set_residual_mode(0.0)
+        k = max(10, args.steps // 4)
+        dev.sync()
+        t0 = time.perf_counter()
+        dev.step_graph(DT, k)
+        dev.sync()
+        w2 = time.perf_counter() - t0
+        fixed = {"body_steps_per_s": round(ndyn * k / w2, 1), "ms_per_step": round(w2 * 1e3 / k, 4),
+                 "velocity_iterations": 10, "steps": k}
+        log(f"[bench] fixed 10 iterations: {w2 * 1e3 / k:.3f} ms/step, {ndyn * k / w2:.4g} body-steps/s")
 
     cpu = None
     if not args.no_cpu:
         try:
-            from oracle import ref
-            if ref.available():
-                threads = args.cpu_threads or min(16, os.cpu_count() or 1)
-                secs, cts = ref.cpu_bench(sc, threads, 0, args.cpu_steps, dt)
-                cpu = {"value": round(ndyn * args.cpu_steps * threads / secs, 1), "unit": "body-steps/s",
-                       "cores": threads, "kind": "reference",
-                       "contacts_per_s": round(cts / secs, 1),
-                       "sample": f"{threads} threads x 1 copy each of the same {sc.nb_bodies}-body world, "
-                                 f"first {args.cpu_steps} steps, ReactPhysics3D 0.10.2 g++ -O3 FP32"}
-                log(f"[bench] cpu reference: {cpu}")
-            else:
-                log("[bench] reference harness not built; no CPU baseline")
+            cpu = cpu_baseline(scenes, args, ndyn)
+            log(f"[bench] cpu reference: {cpu}")
         except Exception as e:  # the baseline is reported, never required
             log(f"[bench] cpu baseline failed: {e}")
 
+    s0 = scenes[0].settings
     out = {
         "metric": "body-steps/s & contacts/s at fixed residual; learned vs plain warm start",
         "value": round(value, 1),
@@ -171,11 +310,17 @@ def main():
         "dtype": "f32",
         "data": "synthetic",
         "contacts_per_s": round(contacts / wall, 1),
-        "config": {"workload": f"mixed_container_{args.bodies}", "bodies": sc.nb_bodies, "dynamic_bodies": ndyn,
-                   "dt": dt, "velocity_iterations": int(sc.settings["vel_iters"]),
-                   "position_iterations": int(sc.settings["pos_iters"]), "worlds": 1,
-                   "pairs": counts["nb_pairs"], "manifolds": counts["nb_manifolds"],
-                   "contact_points": counts["nb_contact_points"], "colors": counts["nb_colors"]},
+        "config": {"workload": name, "worlds": len(scenes), "bodies": sum(sc.nb_bodies for sc in scenes),
+                   "dynamic_bodies": ndyn, "joints": nj, "dt": DT,
+                   "velocity_solver": (f"residual {RESIDUAL} (cap {RESIDUAL_CAP})" if residual
+                                       else f"{int(s0['vel_iters'])} iterations"),
+                   "position_iterations": int(s0["pos_iters"]), "warm_start": "plain (cached impulses)",
+                   "step": "hipGraph replay" if graph else "eager",
+                   "mean_velocity_iterations": round(st["mean_iterations"], 2),
+                   "max_velocity_iterations": st["max_iterations"],
+                   "pairs": c1["nb_pairs"], "manifolds": c1["nb_manifolds"],
+                   "contact_points": c1["nb_contact_points"], "colors": c1["nb_colors"]},
+        "fixed_iterations": fixed,
         "roofline": roofline,
         "cpu_baseline": cpu,
     }

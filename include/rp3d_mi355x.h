@@ -167,6 +167,8 @@ typedef struct rp3dg_counts {
     int64_t nb_tests[7];       /* narrowphase tests of the last step per algorithm (index 1..6:
                                   sphere-sphere, sphere-capsule, capsule-capsule, sphere-poly,
                                   capsule-poly, poly-poly) */
+    int64_t total_contact_points; /* contact points summed over every step since creation */
+    int64_t steps;             /* steps run since creation */
 } rp3dg_counts;
 int rp3dg_batch_get_counts(rp3dg_batch* batch, rp3dg_counts* out);
 
@@ -196,9 +198,12 @@ int rp3dg_batch_get_contacts(rp3dg_batch* batch, rp3dg_contact_view* out, int64_
 /* Joint solve order (joint indices in the order the device solved them this step, per type block). */
 int rp3dg_batch_get_joint_order(rp3dg_batch* batch, int32_t* order, int64_t capacity);
 
-/* Per-kernel timing with HIP events on the batch's stream (for the roofline figures).
- * rp3dg_batch_profile(batch, 1) enables; get returns the mean launch time in ms of each named kernel. */
-int rp3dg_batch_profile(rp3dg_batch* batch, int32_t enable);
+/* Per-kernel timing with HIP events on the batch's stream (for the roofline figures), eager steps only
+ * (rp3dg_batch_step_graph refuses to run while timing is on).  rp3dg_batch_profile(batch, 1, NULL)
+ * times every kernel, (batch, 1, "name") only that kernel (two events per step), (batch, 0, NULL)
+ * disables and clears.
+ * get returns the mean launch time in ms of each named kernel. */
+int rp3dg_batch_profile(rp3dg_batch* batch, int32_t enable, const char* only_kernel);
 int rp3dg_batch_get_kernel_times(rp3dg_batch* batch, const char* kernel, double* mean_ms, int64_t* launches);
 
 /* Solver convergence mode (no counterpart in the reference, which runs a fixed iteration count,
@@ -211,6 +216,45 @@ int rp3dg_batch_set_residual_mode(rp3dg_batch* batch, float residual_threshold, 
  * residual (0 in fixed-iteration mode). */
 int rp3dg_batch_get_iteration_stats(rp3dg_batch* batch, double* mean_iterations, double* max_iterations,
                                     double* max_residual);
+
+/* ---- learned warm start (no counterpart in the reference) ----
+ * A 24 -> 128 -> 128 -> 3 SiLU network, BF16 on the matrix cores with FP32 accumulation, predicts the
+ * normal impulse of every contact point that has no cached impulse (the reference starts those from 0,
+ * ContactSolverSystem.cpp:344-376) and the two friction impulses of a manifold with no resting point
+ * (the reference zeroes them, :468-474).  Features per point: normal (3), r1 (3), r2 (3), relative
+ * velocity along t1, t2, n (3), inverse masses (2), diagonals of the world inverse inertia tensors
+ * (3 + 3), depth, friction coefficient, inverse normal mass, 1 / points in the manifold.  Targets:
+ * converged normal impulse, manifold friction impulses / points.  Parameters (20,099 floats, fp32
+ * master): W1[24][128], b1[128], W2[128][128], b2[128], W3[128][3], b3[3]. */
+typedef struct rp3dg_mlp rp3dg_mlp;
+#define RP3DG_WARM_START_PLAIN 0    /* cached impulses, zero for new points (the reference) */
+#define RP3DG_WARM_START_LEARNED 1  /* cached impulses, network prediction for new points */
+
+/* Glorot-uniform weights from a splitmix64 stream seeded with seed, zero biases, identity input
+ * normalisation and unit output scale. */
+rp3dg_mlp* rp3dg_mlp_create(uint64_t seed);
+void rp3dg_mlp_destroy(rp3dg_mlp* mlp);
+int rp3dg_mlp_set_params(rp3dg_mlp* mlp, const float* params, int64_t count);
+int rp3dg_mlp_get_params(rp3dg_mlp* mlp, float* params, int64_t capacity);
+/* x_normalised = (x - mu) * inv_sigma; impulse = network output * out_scale. */
+int rp3dg_mlp_set_normalization(rp3dg_mlp* mlp, const float mu[24], const float inv_sigma[24],
+                                const float out_scale[3]);
+/* Host arrays: features [n][24] -> impulses [n][3]. */
+int rp3dg_mlp_forward(rp3dg_mlp* mlp, const float* features, int64_t n, float* impulses);
+/* Gradient of loss = 1/(2n) sum_s |y_s - target_s / out_scale|^2 over n <= 65536 samples. */
+int rp3dg_mlp_gradient(rp3dg_mlp* mlp, const float* features, const float* targets, int64_t n, float* gradient,
+                       float* loss);
+/* Adam (beta 0.9 / 0.999) over shuffled minibatches, on the device. */
+int rp3dg_mlp_train(rp3dg_mlp* mlp, const float* features, const float* targets, int64_t n, int32_t epochs,
+                    int32_t minibatch, float learning_rate, uint64_t seed, float* last_loss);
+
+/* Warm start of a batch (fused layout); the network must outlive its use by the batch. */
+int rp3dg_batch_set_warm_start(rp3dg_batch* batch, int32_t mode, rp3dg_mlp* mlp);
+/* Logs (features, converged impulses) of every point without a cached impulse, up to capacity samples
+ * (0 disables and frees the log). */
+int rp3dg_batch_set_logging(rp3dg_batch* batch, int64_t capacity);
+/* Copies up to capacity logged samples; returns how many were copied. */
+int64_t rp3dg_batch_get_log(rp3dg_batch* batch, float* features, float* targets, int64_t capacity);
 
 const char* rp3dg_last_error(void);
 const char* rp3dg_version(void);

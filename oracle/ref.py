@@ -86,8 +86,11 @@ class RefWorld:
         self._b = sc.body_array()
         self._c = sc.collider_array()
         self._j = sc.joint_array() if sc.joints else np.zeros(1, dtype=self._jdtype())
+        hv, hi = (sc.hull_arrays() if getattr(sc, "hulls", None) else (None, None))
+        self._hv, self._hi = hv, hi
         self.h = self.L.rp3dref_create(self._s.ctypes.data, len(self._b), self._b.ctypes.data, len(self._c),
-                                       self._c.ctypes.data, len(sc.joints), self._j.ctypes.data, None, None)
+                                       self._c.ctypes.data, len(sc.joints), self._j.ctypes.data,
+                                       _fp(hv) if hv is not None else None, _ip(hi) if hi is not None else None)
         if not self.h:
             raise RuntimeError("rp3dref_create failed (unsupported shape?)")
         self.nb = len(self._b)

@@ -89,8 +89,6 @@ int rp3dref_precision_bytes() { return (int)sizeof(decimal); }
 
 void* rp3dref_create(const RefSettings* s, int nb, const RefBody* bodies, int nc, const RefCollider* cols, int nj,
                      const RefJoint* joints, const float* hull_vertices, const int32_t* hull_offsets) {
-    (void)hull_vertices;
-    (void)hull_offsets;
     RefWorld* R = new RefWorld();
     PhysicsWorld::WorldSettings ws;
     ws.gravity = Vector3(s->gravity[0], s->gravity[1], s->gravity[2]);
@@ -103,6 +101,36 @@ void* rp3dref_create(const RefSettings* s, int nb, const RefBody* bodies, int nc
     ws.defaultVelocitySolverNbIterations = (uint16)s->vel_iters;
     ws.defaultPositionSolverNbIterations = (uint16)s->pos_iters;
     R->world = R->common.createPhysicsWorld(ws);
+    // convex meshes from explicit polygons (PhysicsCommon::createConvexMesh(PolygonVertexArray)):
+    // hull_offsets = [nb hulls, then per hull: nv, nf, vertex count of each face, face vertex indices]
+    std::vector<ConvexMesh*> meshes;
+    if (hull_offsets && hull_vertices) {
+        const int nh = hull_offsets[0];
+        int k = 1;
+        size_t voff = 0;
+        for (int h = 0; h < nh; h++) {
+            const int nv = hull_offsets[k++];
+            const int nf = hull_offsets[k++];
+            std::vector<PolygonVertexArray::PolygonFace> faces(nf);
+            uint32 base = 0;
+            for (int f = 0; f < nf; f++) {
+                faces[f].nbVertices = (uint32)hull_offsets[k + f];
+                faces[f].indexBase = base;
+                base += faces[f].nbVertices;
+            }
+            k += nf;
+            const int32_t* idx = hull_offsets + k;
+            k += (int)base;
+            PolygonVertexArray pva((uint32)nv, hull_vertices + 3 * voff, 3 * sizeof(float), idx, sizeof(int32_t),
+                                   (uint32)nf, faces.data(), PolygonVertexArray::VertexDataType::VERTEX_FLOAT_TYPE,
+                                   PolygonVertexArray::IndexDataType::INDEX_INTEGER_TYPE);
+            std::vector<Message> msgs;
+            ConvexMesh* mesh = R->common.createConvexMesh(pva, msgs);
+            if (!mesh) { delete R; return nullptr; }
+            meshes.push_back(mesh);
+            voff += (size_t)nv;
+        }
+    }
     // canonical creation order (identical to the device-side scene loader):
     // body, setType, colliders (+material, filter), mass properties, velocities
     int ci = 0;
@@ -118,6 +146,8 @@ void* rp3dref_create(const RefSettings* s, int nb, const RefBody* bodies, int nc
             if (C.shape == 0) shape = R->common.createSphereShape(C.params[0]);
             else if (C.shape == 1) shape = R->common.createCapsuleShape(C.params[0], C.params[1]);
             else if (C.shape == 2) shape = R->common.createBoxShape(Vector3(C.params[0], C.params[1], C.params[2]));
+            else if (C.shape == 3 && (size_t)C.params[0] < meshes.size())
+                shape = R->common.createConvexMeshShape(meshes[(size_t)C.params[0]]);
             else { delete R; return nullptr; }
             Collider* col = body->addCollider(shape, make_tf(C.lpos, C.lquat));
             col->getMaterial().setFrictionCoefficient(C.friction);

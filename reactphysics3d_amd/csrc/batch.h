@@ -36,6 +36,7 @@ struct rp3dg_batch {
     bool cur_is_a = true;
     int pair_cap = 0;
     int* pair_count = nullptr;         // device: sizes of the two pair tables (a, b)
+    unsigned long long* total_points = nullptr;   // device: contact points summed over all steps
     int64_t steps_done = 0;
 
     // solver layout: fused = one workgroup per world for the whole solver phase, (world, colour)-major
@@ -49,6 +50,15 @@ struct rp3dg_batch {
     float4 *res_pv = nullptr, *res_pw = nullptr;
     int* world_iters = nullptr;
     float* world_res = nullptr;
+    // learned warm start and (feature, converged impulse) sample log
+    rp3dg_mlp* mlp = nullptr;
+    int warm_mode = 0;                 // RP3DG_WARM_START_*
+    float* feat_buf = nullptr;
+    int* row_slot = nullptr;
+    int feat_cap = 0;
+    float *log_feats = nullptr, *log_targets = nullptr;
+    unsigned long long* log_count = nullptr;
+    long long log_cap = 0;
     float res_thr = 0.0f;              // residual mode threshold (0: fixed iterations)
     int res_max_iters = 0;
     int max_world_bodies = 0;
@@ -97,6 +107,9 @@ struct rp3dg_batch {
     rp3dg::DevCounters* hcounters = nullptr;
 
     bool profiling = false;
+    std::string profile_only;          // time only this kernel (empty: all)
+    bool timer_open = false;
+    int capture_parity = -1;           // >= 0 while a graph is being captured
     std::vector<Pending> pending;
     std::vector<hipEvent_t> event_pool;
     std::map<std::string, KernelStat> stats;

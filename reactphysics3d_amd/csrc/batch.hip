@@ -13,6 +13,7 @@
 #include "../../include/rp3d_mi355x.h"
 #include "batch.h"
 #include "fused.h"
+#include "mlp.h"
 #include "kcommon.h"
 #include "narrowphase.h"
 #include "solver.h"
@@ -125,32 +126,39 @@ static T* dalloc(rp3dg_batch* b, size_t n) {
     return (T*)p;
 }
 
+// Kernel timing with HIP events on the batch's stream (eager steps; graphs are not timed).
 void rp3dg_batch::timer_begin(const char* name) {
+    timer_open = false;
     if (!profiling) return;
+    if (!profile_only.empty() && profile_only != name) return;
+    if (capture_parity >= 0) return;   // graphs are not timed
     const size_t k = pending.size();
     while (event_pool.size() < 2 * (k + 1)) {
         hipEvent_t ev;
-        hipEventCreate(&ev);
+        if (hipEventCreate(&ev) != hipSuccess) return;
         event_pool.push_back(ev);
     }
     hipEvent_t a = event_pool[2 * k], e = event_pool[2 * k + 1];
-    hipEventRecord(a, stream);
     pending.push_back({name, a, e});
+    hipEventRecord(a, stream);
+    timer_open = true;
 }
 void rp3dg_batch::timer_end() {
-    if (!profiling || pending.empty()) return;
+    if (!timer_open) return;
+    timer_open = false;
     hipEventRecord(pending.back().e, stream);
+}
+static void accumulate(std::map<std::string, rp3dg_batch::KernelStat>& stats, const rp3dg_batch::Pending& p) {
+    float ms = 0.f;
+    if (hipEventElapsedTime(&ms, p.a, p.e) != hipSuccess) return;
+    rp3dg_batch::KernelStat& s = stats[p.name];
+    s.total_ms += ms;
+    s.launches += 1;
 }
 void rp3dg_batch::timer_flush() {
     if (pending.empty()) return;
     hipStreamSynchronize(stream);
-    for (auto& p : pending) {
-        float ms = 0.f;
-        hipEventElapsedTime(&ms, p.a, p.e);
-        KernelStat& s = stats[p.name];
-        s.total_ms += ms;
-        s.launches += 1;
-    }
+    for (auto& p : pending) accumulate(stats, p);
     pending.clear();
 }
 
@@ -588,6 +596,7 @@ int rp3dg_batch::upload(rp3dg_world* const* ws, int nw, const rp3dg_batch_option
     seg_cnt = dalloc<int>(this, nseg + 1);
     seg_start = dalloc<int>(this, nseg + 1);
     pair_count = dalloc<int>(this, 2);
+    total_points = dalloc<unsigned long long>(this, 1);
     d_wbody = dalloc<int>(this, nw + 1);
     d_wcoll = dalloc<int>(this, nw + 1);
     d_block_world = dalloc<int>(this, nw);
@@ -607,13 +616,14 @@ int rp3dg_batch::upload(rp3dg_world* const* ws, int nw, const rp3dg_batch_option
         HIPCHECK(up(d_wcoll, collider_base.data(), sizeof(int) * (nw + 1)));
         HIPCHECK(hipMemcpy(d_block_world, order.data(), sizeof(int) * nw, hipMemcpyHostToDevice));
         HIPCHECK(hipMemsetAsync(pair_count, 0, sizeof(int) * 2, stream));
+        HIPCHECK(hipMemsetAsync(total_points, 0, sizeof(unsigned long long), stream));
         HIPCHECK(hipMemsetAsync(world_iters, 0, sizeof(int) * nw, stream));
         HIPCHECK(hipMemsetAsync(world_res, 0, sizeof(float) * nw, stream));
     }
     S.pair = dalloc<int>(this, pair_cap); S.bodies = dalloc<int2>(this, pair_cap); S.npts = dalloc<int>(this, pair_cap);
     for (float4** a : {&S.n, &S.r1f, &S.r2f, &S.t1, &S.t2, &S.r1ct1, &S.r1ct2, &S.r2ct1, &S.r2ct2, &S.imp, &S.oldt1, &S.oldt2})
         *a = dalloc<float4>(this, pair_cap);
-    for (float4** a : {&S.pn, &S.r1, &S.r2, &S.i1n, &S.i2n, &S.lam}) *a = dalloc<float4>(this, 4 * (size_t)pair_cap);
+    for (float4** a : {&S.pn, &S.r1, &S.r2, &S.i1n, &S.i2n, &S.lam, &S.pred}) *a = dalloc<float4>(this, 4 * (size_t)pair_cap);
     scan_sums_cap = 1 << 16;
     scan_sums_buf = dalloc<int>(this, scan_sums_cap);
     sort_hist_cap = 256 * (std::max(pair_cap, nc) / 4096 + 2);
@@ -756,7 +766,7 @@ int rp3dg_batch::enqueue_step(float dt, bool capturing) {
     r = scan(m_flag, pair_cap, &counters->nb_pairs, m_offs, stream);
     if (r) { set_error("scan overflow"); return r; }
     LAUNCH_T("sv_compact", pg, 256, sv_compact_manifolds, m_flag, m_offs, &counters->nb_pairs, N.ncp, m_list, nm_ptr,
-             &counters->nb_points);
+             &counters->nb_points, total_points);
     const int hm = capturing ? pair_cap : std::min(pair_cap, hint_manifolds + hint_manifolds / 4 + 1024);
     const int mg = grid_for(hm);
     fill_int<<<grid_for(nb + 1), 256, 0, stream>>>(adj_cnt, nb + 1, 0);
@@ -795,6 +805,7 @@ int rp3dg_batch::enqueue_step(float dt, bool capturing) {
     SA.cstart = seg_start;
     SA.ncolors = &counters->nb_colors;
     SA.restitution_threshold = settings.restitution_threshold;
+    SA.learned = (fused && warm_mode == RP3DG_WARM_START_LEARNED && mlp) ? 1 : 0;
     const float beta_dt = 0.2f / dt;
 
     if (fused) {
@@ -818,7 +829,32 @@ int rp3dg_batch::enqueue_step(float dt, bool capturing) {
         F.pw = res_pw;
         F.world_iters = world_iters;
         F.world_res = world_res;
-        LAUNCH_T("sv_world_solve", nworlds, fused_block, sv_world_solve, F);
+        F.nrows = &counters->nb_new_points;
+        F.feat_cap = feat_cap;
+        F.row_slot = row_slot;
+        F.feats = nullptr;
+        F.phase = 2;
+        const bool features = SA.learned || log_cap > 0;
+        if (!features) {
+            LAUNCH_T("sv_world_solve", nworlds, fused_block, sv_world_solve, F);
+        } else {
+            F.feats = feat_buf;
+            F.phase = 0;
+            LAUNCH_T("sv_world_prepare", nworlds, fused_block, sv_world_solve, F);
+            if (SA.learned) {
+                const int rows = capturing ? feat_cap : std::min(feat_cap, last.nb_new_points + last.nb_new_points / 4 + 1024);
+                const int grid = std::max(1, std::min(2048, (rows + 63) / 64));
+                LAUNCH_T("mlp_infer", grid, 256, mlp_infer, mlp_view(mlp), feat_buf, &counters->nb_new_points, feat_cap,
+                         nullptr, row_slot, S.pred);
+            }
+            F.phase = 1;
+            LAUNCH_T("sv_world_solve", nworlds, fused_block, sv_world_solve, F);
+            if (log_cap > 0) {
+                LAUNCH_T("sv_log_samples", grid_for(feat_cap), 256, sv_log_samples, F, log_feats, log_targets, log_count,
+                         log_cap);
+                sv_log_advance<<<1, 1, 0, stream>>>(&counters->nb_new_points, feat_cap, log_count, log_cap);
+            }
+        }
         LAUNCH_T("world_position", nworlds, fused_block, world_position, F);
         return RP3DG_OK;
     }
@@ -903,19 +939,26 @@ void rp3dg_batch::drop_graphs() {
 int rp3dg_batch::step_graph(float dt, int n) {
     if (nb == 0 || n <= 0) return RP3DG_OK;
     if (!fused) { set_error("graph capture needs the fused solver layout"); return RP3DG_EUNSUPPORTED; }
-    if (profiling) { set_error("disable kernel profiling before graph replay"); return RP3DG_ESTATE; }
+    if (profiling) { set_error("kernel timing is eager-only: disable it before graph replay"); return RP3DG_ESTATE; }
     if (graph_exec[0] && graph_dt != dt) drop_graphs();
     if (!graph_exec[0]) {
         graph_rounds = std::min(kColorRoundsMax, std::max(8, hint_rounds + 4));
         const bool start = cur_is_a;
         for (int p = 0; p < 2; p++) {
+            const int parity = start == (p == 0) ? 0 : 1;   // parity 0: a -> b
             HIPCHECK(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
+            capture_parity = parity;
             const int r = enqueue_step(dt, true);
+            capture_parity = -1;
             hipGraph_t g = nullptr;
             const hipError_t e = hipStreamEndCapture(stream, &g);
-            if (r) { if (g) hipGraphDestroy(g); cur_is_a = start; drop_graphs(); return r; }
-            HIPCHECK(e);
-            const int parity = start == (p == 0) ? 0 : 1;   // parity 0: a -> b
+            if (r || e != hipSuccess) {
+                if (g) hipGraphDestroy(g);
+                cur_is_a = start;
+                drop_graphs();
+                if (r) return r;
+                HIPCHECK(e);
+            }
             graph[parity] = g;
             HIPCHECK(hipGraphInstantiate(&graph_exec[parity], g, nullptr, nullptr, 0));
         }
@@ -931,6 +974,8 @@ int rp3dg_batch::step_graph(float dt, int n) {
 
 rp3dg_batch::~rp3dg_batch() {
     drop_graphs();
+    if (log_feats) hipFree(log_feats);
+    if (log_targets) hipFree(log_targets);
     for (hipEvent_t ev : event_pool) hipEventDestroy(ev);
     for (void* p : allocs) hipFree(p);
     if (hcounters) hipHostFree(hcounters);
@@ -1001,6 +1046,11 @@ int rp3dg_batch_get_counts(rp3dg_batch* b, rp3dg_counts* o) {
     o->nb_colors = b->last.nb_colors;
     o->status = b->last.status;
     for (int a = 0; a < 7; a++) o->nb_tests[a] = b->last.nb_tests[a];
+    unsigned long long tp = 0;
+    HIPCHECK(hipMemcpyAsync(&tp, b->total_points, sizeof(tp), hipMemcpyDeviceToHost, b->stream));
+    HIPCHECK(hipStreamSynchronize(b->stream));
+    o->total_contact_points = (int64_t)tp;
+    o->steps = b->steps_done;
     return RP3DG_OK;
 }
 
@@ -1091,9 +1141,10 @@ int rp3dg_batch_get_joint_order(rp3dg_batch* b, int32_t* order, int64_t cap) {
     return n;
 }
 
-int rp3dg_batch_profile(rp3dg_batch* b, int32_t enable) {
+int rp3dg_batch_profile(rp3dg_batch* b, int32_t enable, const char* only_kernel) {
     if (!b) return RP3DG_EINVAL;
     b->profiling = enable != 0;
+    b->profile_only = (enable && only_kernel) ? only_kernel : "";
     b->stats.clear();
     return RP3DG_OK;
 }
@@ -1126,6 +1177,66 @@ int rp3dg_batch_set_residual_mode(rp3dg_batch* b, float thr, int32_t maxit) {
     }
     b->drop_graphs();
     return RP3DG_OK;
+}
+
+static int ensure_features(rp3dg_batch* b) {
+    if (b->feat_buf) return RP3DG_OK;
+    b->feat_cap = (int)std::min<int64_t>(4 * (int64_t)b->pair_cap, std::max<int64_t>(1 << 16, 4 * (int64_t)b->nc));
+    b->feat_buf = dalloc<float>(b, (size_t)b->feat_cap * kFeatures);
+    b->row_slot = dalloc<int>(b, b->feat_cap);
+    if (!b->feat_buf || !b->row_slot) { set_error("device allocation failed (feature rows)"); return RP3DG_EHIP; }
+    return RP3DG_OK;
+}
+
+int rp3dg_batch_set_warm_start(rp3dg_batch* b, int32_t mode, rp3dg_mlp* mlp) {
+    if (!b) return RP3DG_EINVAL;
+    if (mode == RP3DG_WARM_START_LEARNED) {
+        if (!mlp) { set_error("learned warm start needs a network"); return RP3DG_EINVAL; }
+        if (!b->fused) { set_error("learned warm start needs the fused solver layout"); return RP3DG_EUNSUPPORTED; }
+        const int r = ensure_features(b);
+        if (r) return r;
+    } else if (mode != RP3DG_WARM_START_PLAIN) {
+        set_error("unknown warm start mode");
+        return RP3DG_EINVAL;
+    }
+    b->warm_mode = mode;
+    b->mlp = mode == RP3DG_WARM_START_LEARNED ? mlp : nullptr;
+    b->drop_graphs();
+    return RP3DG_OK;
+}
+
+int rp3dg_batch_set_logging(rp3dg_batch* b, int64_t capacity) {
+    if (!b || capacity < 0) return RP3DG_EINVAL;
+    if (b->log_feats) { hipFree(b->log_feats); b->log_feats = nullptr; }
+    if (b->log_targets) { hipFree(b->log_targets); b->log_targets = nullptr; }
+    b->log_cap = 0;
+    b->drop_graphs();
+    if (capacity == 0) return RP3DG_OK;
+    if (!b->fused) { set_error("sample logging needs the fused solver layout"); return RP3DG_EUNSUPPORTED; }
+    int r = ensure_features(b);
+    if (r) return r;
+    if (!b->log_count) {
+        b->log_count = dalloc<unsigned long long>(b, 1);
+        if (!b->log_count) { set_error("device allocation failed (log)"); return RP3DG_EHIP; }
+    }
+    HIPCHECK(hipMalloc(&b->log_feats, sizeof(float) * kFeatures * capacity));
+    HIPCHECK(hipMalloc(&b->log_targets, sizeof(float) * 3 * capacity));
+    HIPCHECK(hipMemsetAsync(b->log_count, 0, sizeof(unsigned long long), b->stream));
+    HIPCHECK(hipStreamSynchronize(b->stream));
+    b->log_cap = capacity;
+    return RP3DG_OK;
+}
+
+int64_t rp3dg_batch_get_log(rp3dg_batch* b, float* feats, float* targets, int64_t cap) {
+    if (!b || cap < 0) return RP3DG_EINVAL;
+    if (!b->log_cap) return 0;
+    unsigned long long n = 0;
+    HIPCHECK(hipMemcpyAsync(&n, b->log_count, sizeof(n), hipMemcpyDeviceToHost, b->stream));
+    HIPCHECK(hipStreamSynchronize(b->stream));
+    const int64_t k = std::min<int64_t>((int64_t)n, cap);
+    if (k > 0 && feats) HIPCHECK(hipMemcpy(feats, b->log_feats, sizeof(float) * kFeatures * k, hipMemcpyDeviceToHost));
+    if (k > 0 && targets) HIPCHECK(hipMemcpy(targets, b->log_targets, sizeof(float) * 3 * k, hipMemcpyDeviceToHost));
+    return k;
 }
 
 int rp3dg_batch_get_iteration_stats(rp3dg_batch* b, double* mean_it, double* max_it, double* max_res) {

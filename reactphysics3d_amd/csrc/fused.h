@@ -24,11 +24,23 @@ struct FusedArgs {
     float4 *pv, *pw;           // velocities after the previous sweep (residual mode)
     int* world_iters;          // iterations each world ran in the last step
     float* world_res;          // its last residual (residual mode)
+    // learned warm start / sample logging
+    int phase;                 // 0: integration + rows (+ features), 1: warm start .. store, 2: both
+    float* feats;              // [feat_cap][24] features of the points without a cached impulse
+    int* row_slot;             // solver point slot (4 * row + point) of each feature row
+    int* nrows;                // feature rows written this step
+    int feat_cap;
 };
+
+constexpr int kFeatures = 24;
 
 constexpr int kFusedMaxBlock = 256;   // threads per world at most (register budget of the fused kernels)
 
 __global__ void sv_world_solve(FusedArgs F);
 __global__ void world_position(FusedArgs F);
+// Appends (features, converged impulses) of this step's feature rows to the sample log at *logged.
+__global__ void sv_log_samples(FusedArgs F, float* log_feats, float* log_targets, const unsigned long long* logged,
+                               long long log_cap);
+__global__ void sv_log_advance(const int* nrows, int feat_cap, unsigned long long* logged, long long log_cap);
 
 }  // namespace rp3dg

@@ -48,13 +48,48 @@ __global__ __launch_bounds__(kFusedMaxBlock) void sv_world_solve(FusedArgs F) {
     const int2* jr = F.jt_wr + (size_t)w * F.jt_slots;
     const int j0 = F.jt_slots ? jr[0].x : 0, j1 = F.jt_slots ? jr[F.jt_slots - 1].y : 0;
 
-    // PhysicsWorld.cpp:266-272: inverse world inertia tensors and velocity integration
-    for (int i = b0 + tid; i < b1; i += nt) dyn_integrate_velocities_one(B, F.settings, F.dt, i);
-    __syncthreads();
-    // ContactSolverSystem::init (rows); ConstraintSolverSystem::initialize (initBeforeSolve)
-    for (int i = m0 + tid; i < m1; i += nt) sv_prepare_one(A, i);
-    for (int j = j0 + tid; j < j1; j += nt) jt_init_one(B, F.J, j);
-    __syncthreads();
+    if (F.phase != 1) {
+        // PhysicsWorld.cpp:266-272: inverse world inertia tensors and velocity integration
+        for (int i = b0 + tid; i < b1; i += nt) dyn_integrate_velocities_one(B, F.settings, F.dt, i);
+        __syncthreads();
+        // ContactSolverSystem::init (rows); ConstraintSolverSystem::initialize (initBeforeSolve)
+        for (int i = m0 + tid; i < m1; i += nt) sv_prepare_one(A, i);
+        for (int j = j0 + tid; j < j1; j += nt) jt_init_one(B, F.J, j);
+        if (F.feats) {
+            // feature rows of the points without a cached impulse; same manifold -> thread mapping as
+            // above, so every thread reads only rows it wrote itself
+            const int lane = lane_id();
+            for (int base = m0 + (tid & ~(kWave - 1)); base < m1; base += nt) {
+                const int i = base + lane;
+                int k = 0;
+                if (i < m1) {
+                    const int np = A.S.npts[i];
+                    for (int p = 0; p < np; p++) k += A.S.i1n[4 * i + p].w == 0.0f ? 1 : 0;
+                }
+                const int incl = wave_incl_scan(k);
+                int start = 0;
+                if (lane == kWave - 1 && incl > 0) start = atomicAdd(F.nrows, incl);
+                start = __shfl(start, kWave - 1, kWave) + incl - k;
+                if (i < m1) {
+                    const int np = A.S.npts[i];
+                    for (int p = 0; p < np; p++) {
+                        const int q = 4 * i + p;
+                        if (A.S.i1n[q].w != 0.0f) continue;
+                        A.S.pred[q] = make_float4(0.f, 0.f, 0.f, 0.f);
+                        if (start < F.feat_cap) {
+                            float f[kFeatures];
+                            sv_point_features(A, i, p, f);
+                            for (int e = 0; e < kFeatures; e++) F.feats[(size_t)start * kFeatures + e] = f[e];
+                            F.row_slot[start] = q;
+                        }
+                        start++;
+                    }
+                }
+            }
+        }
+        __syncthreads();
+        if (F.phase == 0) return;
+    }
     // contact warm start, colour by colour (CS.cpp:326-476)
     for (int c = 0; c < kMaxColors; c++) {
         const int lo = seg[c], hi = seg[c + 1];
@@ -142,6 +177,29 @@ __global__ __launch_bounds__(kFusedMaxBlock) void world_position(FusedArgs F) {
     __syncthreads();
     // collider transforms and fat AABBs for the next broadphase
     for (int i = c0 + tid; i < c1; i += nt) refit_one(B, F.A.C, F.hull_bounds, 0, i);
+}
+
+__global__ void sv_log_samples(FusedArgs F, float* log_feats, float* log_targets, const unsigned long long* logged,
+                               long long log_cap) {
+    const int n = min(*F.nrows, F.feat_cap);
+    const long long base = (long long)*logged;
+    RP3DG_GRID_STRIDE(r, n) {
+        const long long dst = base + r;
+        if (dst >= log_cap) continue;
+        const int q = F.row_slot[r], i = q >> 2;
+        const float4 imp = F.A.S.imp[i];
+        const float inv_np = 1.0f / (float)F.A.S.npts[i];
+        for (int e = 0; e < kFeatures; e++) log_feats[dst * kFeatures + e] = F.feats[(size_t)r * kFeatures + e];
+        log_targets[dst * 3 + 0] = F.A.S.lam[q].x;
+        log_targets[dst * 3 + 1] = imp.x * inv_np;
+        log_targets[dst * 3 + 2] = imp.y * inv_np;
+    }
+}
+
+__global__ void sv_log_advance(const int* nrows, int feat_cap, unsigned long long* logged, long long log_cap) {
+    const long long n = min(*nrows, feat_cap);
+    const long long v = (long long)*logged + n;
+    *logged = (unsigned long long)(v < log_cap ? v : log_cap);
 }
 
 }  // namespace rp3dg

@@ -42,6 +42,11 @@ __device__ __forceinline__ int wave_max(int v) {
     }
     return v;
 }
+__device__ __forceinline__ float wave_sumf(float v) {
+#pragma unroll
+    for (int off = kWave / 2; off > 0; off >>= 1) v += __shfl_xor(v, off, kWave);
+    return v;
+}
 __device__ __forceinline__ float wave_maxf(float v) {
 #pragma unroll
     for (int off = kWave / 2; off > 0; off >>= 1) {

@@ -131,6 +131,7 @@ struct DevSolver {
     float4 *r1, *r2;           // w: inverse penetration mass | restitution bias
     float4 *i1n, *i2n;         // I1 (r1 x n), I2 (r2 x n); w: resting flag | unused
     float4 *lam;               // penetration impulse, split impulse
+    float4 *pred;              // learned warm start: predicted normal impulse, friction 1, friction 2
 };
 
 struct DevJoints;  // joint solver rows (joints.hip)

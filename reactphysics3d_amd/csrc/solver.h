@@ -18,6 +18,7 @@ struct SvArgs {
     const int* cstart;    // colour ranges [cstart[c], cstart[c+1]) (colour-major layout)
     const int* ncolors;
     float restitution_threshold;
+    int learned;          // learned warm start for points without a cached impulse
 };
 
 __global__ void dyn_integrate_velocities(DevBodies B, Settings S, float dt);
@@ -25,7 +26,8 @@ __global__ void dyn_integrate_positions(DevBodies B, float dt);
 __global__ void dyn_update_state(DevBodies B);
 __global__ void sv_flag_manifolds(const int* ncp, const int* nb_pairs, int* flag);
 __global__ void sv_compact_manifolds(const int* flag, const int* offs, const int* nb_pairs, const int* ncp, int* list,
-                                     int* counters_manifolds, int* counters_points);
+                                     int* counters_manifolds, int* counters_points,
+                                     unsigned long long* total_points);
 __global__ void sv_adj_count(DevBodies B, DevColliders C, DevPairs P, const int* list, const int* nm_ptr, int* cnt);
 __global__ void sv_adj_fill(DevBodies B, DevColliders C, DevPairs P, const int* list, const int* nm_ptr, const int* offs,
                             int* cursor, int* adj, int2* mb);

@@ -63,7 +63,8 @@ __global__ void sv_flag_manifolds(const int* ncp, const int* nb_pairs, int* flag
 }
 
 __global__ void sv_compact_manifolds(const int* flag, const int* offs, const int* nb_pairs, const int* ncp, int* list,
-                                     int* counters_manifolds, int* counters_points) {
+                                     int* counters_manifolds, int* counters_points,
+                                     unsigned long long* total_points) {
     const int np = *nb_pairs;
     if (blockIdx.x == 0 && threadIdx.x == 0) *counters_manifolds = offs[np];
     int pts = 0;
@@ -74,7 +75,10 @@ __global__ void sv_compact_manifolds(const int* flag, const int* offs, const int
         }
     }
     pts = wave_sum(pts);
-    if (lane_id() == 0 && pts) atomicAdd(counters_points, pts);
+    if (lane_id() == 0 && pts) {
+        atomicAdd(counters_points, pts);
+        atomicAdd(total_points, (unsigned long long)pts);
+    }
 }
 
 // ------------------------------------------------------------------------------------------------

@@ -223,12 +223,22 @@ __device__ __forceinline__ void sv_warmstart_one(const SvArgs& A, int i) {
         const V3 ll1 = ld3(B.llock, b1), al1 = ld3(B.alock, b1), ll2 = ld3(B.llock, b2), al2 = ld3(B.alock, b2);
         const int np = S.npts[i];
         bool any = false;
+        float plx = 0.0f, ply = 0.0f;   // learned friction impulses summed over the new points
         for (int p = 0; p < np; p++) {
             const int q = 4 * i + p;
             const float4 i1n4 = S.i1n[q];
-            if (i1n4.w != 0.0f) {
-                any = true;
-                const float lam = S.lam[q].x;
+            if (i1n4.w != 0.0f || A.learned) {
+                float lam;
+                if (i1n4.w != 0.0f) {
+                    any = true;
+                    lam = S.lam[q].x;
+                } else {   // learned warm start of a point with no cached impulse
+                    const float4 pr = S.pred[q];
+                    lam = fmax_s(pr.x, 0.0f);
+                    S.lam[q].x = lam;
+                    plx += pr.y;
+                    ply += pr.z;
+                }
                 const V3 n = xyz(S.pn[q]);
                 const V3 i1n = xyz(i1n4), i2n = ld3(S.i2n, q);
                 const V3 ip = v3(n.x * lam, n.y * lam, n.z * lam);
@@ -241,13 +251,21 @@ __device__ __forceinline__ void sv_warmstart_one(const SvArgs& A, int i) {
             }
         }
         float4 imp = S.imp[i];
-        if (any) {
+        // learned friction only for a manifold with no resting point (the reference then starts from zero)
+        const bool learned_friction = !any && A.learned && np > 0;
+        if (any || learned_friction) {
             const M3 I1 = body_iw(B, b1), I2 = body_iw(B, b2);
-            const V3 ot1 = ld3(S.oldt1, i), ot2 = ld3(S.oldt2, i);
             const V3 t1 = ld3(S.t1, i), t2 = ld3(S.t2, i);
-            const V3 oi = v3(imp.x * ot1.x + imp.y * ot2.x, imp.x * ot1.y + imp.y * ot2.y, imp.x * ot1.z + imp.y * ot2.z);
-            imp.x = dot(oi, t1);
-            imp.y = dot(oi, t2);
+            if (any) {
+                const V3 ot1 = ld3(S.oldt1, i), ot2 = ld3(S.oldt2, i);
+                const V3 oi = v3(imp.x * ot1.x + imp.y * ot2.x, imp.x * ot1.y + imp.y * ot2.y, imp.x * ot1.z + imp.y * ot2.z);
+                imp.x = dot(oi, t1);
+                imp.y = dot(oi, t2);
+            } else {
+                imp.x = plx;
+                imp.y = ply;
+                imp.z = 0.0f;
+            }
             const V3 r1ct1 = ld3(S.r1ct1, i), r2ct1 = ld3(S.r2ct1, i), r1ct2 = ld3(S.r1ct2, i), r2ct2 = ld3(S.r2ct2, i);
             V3 a1 = v3(-r1ct1.x * imp.x, -r1ct1.y * imp.x, -r1ct1.z * imp.x);
             V3 l2 = v3(t1.x * imp.x, t1.y * imp.x, t1.z * imp.x);
@@ -278,6 +296,32 @@ __device__ __forceinline__ void sv_warmstart_one(const SvArgs& A, int i) {
         if (d1) { B.cv[b1] = f4(v1, 0.f); B.cw[b1] = f4(w1, 0.f); }
         if (d2) { B.cv[b2] = f4(v2, 0.f); B.cw[b2] = f4(w2, 0.f); }
     }
+}
+
+// Learned warm start: the 24 features of contact point p of solver row i (after sv_prepare_one and
+// the velocity integration): normal, r1, r2, relative velocity in the (t1, t2, n) frame, inverse
+// masses, diagonals of the world inverse inertia tensors, depth, friction coefficient, inverse
+// normal mass, 1 / points in the manifold.
+__device__ __forceinline__ void sv_point_features(const SvArgs& A, int i, int p, float* f) {
+    const DevBodies& B = A.B;
+    const DevSolver& S = A.S;
+    const int q = 4 * i + p;
+    const int2 bb = S.bodies[i];
+    const float4 pn = S.pn[q], r14 = S.r1[q], r24 = S.r2[q];
+    const V3 n = xyz(pn), r1 = xyz(r14), r2 = xyz(r24);
+    const V3 dv = rel_vel(ld3(B.cv, bb.x), ld3(B.cw, bb.x), ld3(B.cv, bb.y), ld3(B.cw, bb.y), r1, r2);
+    const V3 t1 = ld3(S.t1, i), t2 = ld3(S.t2, i);
+    f[0] = n.x; f[1] = n.y; f[2] = n.z;
+    f[3] = r1.x; f[4] = r1.y; f[5] = r1.z;
+    f[6] = r2.x; f[7] = r2.y; f[8] = r2.z;
+    f[9] = dot(dv, t1); f[10] = dot(dv, t2); f[11] = dot(dv, n);
+    f[12] = S.r1f[i].w; f[13] = S.r2f[i].w;
+    f[14] = B.iw0[bb.x].x; f[15] = B.iw1[bb.x].y; f[16] = B.iw2[bb.x].z;
+    f[17] = B.iw0[bb.y].x; f[18] = B.iw1[bb.y].y; f[19] = B.iw2[bb.y].z;
+    f[20] = pn.w;
+    f[21] = S.n[i].w;
+    f[22] = r14.w;
+    f[23] = 1.0f / (float)S.npts[i];
 }
 
 // ContactSolverSystem::solve (CS.cpp:479-759), split impulses active, for one colour

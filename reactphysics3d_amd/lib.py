@@ -35,7 +35,7 @@ class WorldSettings(C.Structure):
 class Counts(C.Structure):
     _fields_ = [(n, C.c_int64) for n in ("nb_bodies", "nb_colliders", "nb_joints", "nb_worlds", "nb_pairs",
                                          "nb_manifolds", "nb_contact_points", "nb_colors", "status")] + \
-               [("nb_tests", C.c_int64 * 7)]
+               [("nb_tests", C.c_int64 * 7), ("total_contact_points", C.c_int64), ("steps", C.c_int64)]
 
 
 class BatchOptions(C.Structure):
@@ -65,8 +65,14 @@ EXPORTED = [
     "rp3dg_batch_step_graph", "rp3dg_batch_synchronize", "rp3dg_batch_get_counts", "rp3dg_batch_get_bodies",
     "rp3dg_batch_set_bodies", "rp3dg_batch_get_pairs", "rp3dg_batch_get_contacts", "rp3dg_batch_get_joint_order",
     "rp3dg_batch_profile", "rp3dg_batch_get_kernel_times", "rp3dg_batch_set_residual_mode",
-    "rp3dg_batch_get_iteration_stats", "rp3dg_last_error", "rp3dg_version",
+    "rp3dg_batch_get_iteration_stats", "rp3dg_last_error", "rp3dg_version", "rp3dg_mlp_create", "rp3dg_mlp_destroy",
+    "rp3dg_mlp_set_params", "rp3dg_mlp_get_params", "rp3dg_mlp_set_normalization", "rp3dg_mlp_forward",
+    "rp3dg_mlp_gradient", "rp3dg_mlp_train", "rp3dg_batch_set_warm_start", "rp3dg_batch_set_logging",
+    "rp3dg_batch_get_log",
 ]
+
+MLP_PARAMS = 20099
+N_FEATURES = 24
 
 
 def _fp(a):
@@ -120,11 +126,22 @@ def load():
         "rp3dg_batch_get_pairs": (C.c_int, [vp, iptr, iptr, i64]),
         "rp3dg_batch_get_contacts": (C.c_int, [vp, C.POINTER(ContactView), i64]),
         "rp3dg_batch_get_joint_order": (C.c_int, [vp, iptr, i64]),
-        "rp3dg_batch_profile": (C.c_int, [vp, i32]),
+        "rp3dg_batch_profile": (C.c_int, [vp, i32, C.c_char_p]),
         "rp3dg_batch_get_kernel_times": (C.c_int, [vp, C.c_char_p, C.POINTER(C.c_double), C.POINTER(i64)]),
         "rp3dg_batch_set_residual_mode": (C.c_int, [vp, f, i32]),
         "rp3dg_batch_get_iteration_stats": (C.c_int, [vp, C.POINTER(C.c_double), C.POINTER(C.c_double),
                                                       C.POINTER(C.c_double)]),
+        "rp3dg_mlp_create": (vp, [C.c_uint64]),
+        "rp3dg_mlp_destroy": (None, [vp]),
+        "rp3dg_mlp_set_params": (C.c_int, [vp, fptr, i64]),
+        "rp3dg_mlp_get_params": (C.c_int, [vp, fptr, i64]),
+        "rp3dg_mlp_set_normalization": (C.c_int, [vp, fptr, fptr, fptr]),
+        "rp3dg_mlp_forward": (C.c_int, [vp, fptr, i64, fptr]),
+        "rp3dg_mlp_gradient": (C.c_int, [vp, fptr, fptr, i64, fptr, fptr]),
+        "rp3dg_mlp_train": (C.c_int, [vp, fptr, fptr, i64, i32, i32, f, C.c_uint64, fptr]),
+        "rp3dg_batch_set_warm_start": (C.c_int, [vp, i32, vp]),
+        "rp3dg_batch_set_logging": (C.c_int, [vp, i64]),
+        "rp3dg_batch_get_log": (i64, [vp, fptr, fptr, i64]),
         "rp3dg_last_error": (C.c_char_p, []),
         "rp3dg_version": (C.c_char_p, []),
     }
@@ -171,6 +188,11 @@ def world_from_scene(sc: S.Scene):
     st.default_position_solver_iterations = int(s["pos_iters"])
     st.is_sleeping_enabled = int(s["sleeping"])
     w = L.rp3dg_world_create(C.byref(st))
+    for verts, faces in getattr(sc, "hulls", []):
+        v = np.ascontiguousarray(verts, np.float32)
+        counts = np.array([len(f) for f in faces], np.int32)
+        idx = np.array([i for f in faces for i in f], np.int32)
+        _check(L.rp3dg_hull_create(w, _fp(v), len(v), _ip(counts), _ip(idx), len(faces)), "hull_create")
     ci = 0
     cols = sc.colliders
     for bi, (bt, pos, q, lin, ang, grav, ld, ad) in enumerate(sc.bodies):
@@ -285,8 +307,8 @@ class Batch:
         k = _check(load().rp3dg_batch_get_joint_order(self.h, _ip(o), n), "get_joint_order")
         return o[:k]
 
-    def profile(self, enable=True):
-        _check(load().rp3dg_batch_profile(self.h, int(enable)), "profile")
+    def profile(self, enable=True, only=None):
+        _check(load().rp3dg_batch_profile(self.h, int(enable), only.encode() if only else None), "profile")
 
     def set_residual_mode(self, threshold, max_iterations=64):
         _check(load().rp3dg_batch_set_residual_mode(self.h, float(threshold), int(max_iterations)), "residual_mode")
@@ -296,8 +318,80 @@ class Batch:
         _check(load().rp3dg_batch_get_iteration_stats(self.h, C.byref(a), C.byref(b), C.byref(c)), "iteration_stats")
         return {"mean_iterations": a.value, "max_iterations": b.value, "max_residual": c.value}
 
+    def set_warm_start(self, mode, mlp=None):
+        m = {"plain": 0, "learned": 1}[mode]
+        _check(load().rp3dg_batch_set_warm_start(self.h, m, mlp.h if mlp is not None else None), "set_warm_start")
+        self._mlp = mlp   # keep the network alive while the batch may use it
+
+    def set_logging(self, capacity):
+        _check(load().rp3dg_batch_set_logging(self.h, int(capacity)), "set_logging")
+
+    def get_log(self, capacity):
+        f = np.zeros((capacity, N_FEATURES), np.float32)
+        t = np.zeros((capacity, 3), np.float32)
+        n = _check(load().rp3dg_batch_get_log(self.h, _fp(f), _fp(t), capacity), "get_log")
+        return f[:n], t[:n]
+
     def kernel_time(self, name):
         ms = C.c_double()
         n = C.c_int64()
         _check(load().rp3dg_batch_get_kernel_times(self.h, name.encode(), C.byref(ms), C.byref(n)), "kernel_times")
         return ms.value, n.value
+
+
+class Mlp:
+    """The learned warm-start network (BF16 on the matrix cores); parameters live on the device."""
+
+    def __init__(self, seed=1):
+        self.h = load().rp3dg_mlp_create(int(seed))
+        if not self.h:
+            raise RuntimeError("rp3dg_mlp_create failed: " + last_error())
+
+    def close(self):
+        if self.h:
+            load().rp3dg_mlp_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def params(self):
+        p = np.zeros(MLP_PARAMS, np.float32)
+        _check(load().rp3dg_mlp_get_params(self.h, _fp(p), MLP_PARAMS), "mlp_get_params")
+        return p
+
+    def set_params(self, p):
+        p = np.ascontiguousarray(p, np.float32)
+        _check(load().rp3dg_mlp_set_params(self.h, _fp(p), p.size), "mlp_set_params")
+
+    def set_normalization(self, mu, inv_sigma, out_scale):
+        mu = np.ascontiguousarray(mu, np.float32)
+        inv_sigma = np.ascontiguousarray(inv_sigma, np.float32)
+        out_scale = np.ascontiguousarray(out_scale, np.float32)
+        _check(load().rp3dg_mlp_set_normalization(self.h, _fp(mu), _fp(inv_sigma), _fp(out_scale)), "mlp_norm")
+
+    def forward(self, feats):
+        feats = np.ascontiguousarray(feats, np.float32)
+        out = np.zeros((len(feats), 3), np.float32)
+        _check(load().rp3dg_mlp_forward(self.h, _fp(feats), len(feats), _fp(out)), "mlp_forward")
+        return out
+
+    def gradient(self, feats, targets):
+        feats = np.ascontiguousarray(feats, np.float32)
+        targets = np.ascontiguousarray(targets, np.float32)
+        g = np.zeros(MLP_PARAMS, np.float32)
+        loss = C.c_float()
+        _check(load().rp3dg_mlp_gradient(self.h, _fp(feats), _fp(targets), len(feats), _fp(g), C.byref(loss)),
+               "mlp_gradient")
+        return g, loss.value
+
+    def train(self, feats, targets, epochs=10, minibatch=4096, lr=1e-3, seed=1):
+        feats = np.ascontiguousarray(feats, np.float32)
+        targets = np.ascontiguousarray(targets, np.float32)
+        loss = C.c_float()
+        _check(load().rp3dg_mlp_train(self.h, _fp(feats), _fp(targets), len(feats), int(epochs), int(minibatch),
+                                      float(lr), int(seed), C.byref(loss)), "mlp_train")
+        return loss.value

@@ -28,13 +28,12 @@ def short_name(name: str) -> str:
 
 
 def kernel_trace(db: str):
+    """Per kernel: list of dispatch durations in microseconds (from start/end timestamps in ns)."""
     c = sqlite3.connect(db)
-    rows = c.execute("select name, total_calls, total_duration, average, percentage from top_kernels").fetchall()
-    agg = defaultdict(lambda: [0, 0.0])
-    for name, calls, total, _avg, _pct in rows:
-        k = short_name(name)
-        agg[k][0] += int(calls)
-        agg[k][1] += float(total)
+    rows = c.execute("select name, start, end from kernels").fetchall()
+    agg = defaultdict(list)
+    for name, t0, t1 in rows:
+        agg[short_name(name)].append((t1 - t0) / 1e3)
     return agg
 
 
@@ -68,10 +67,14 @@ def main(argv=None):
         out += ["Command:", "", "```", a.command, "```", ""]
     if a.trace:
         agg = kernel_trace(a.trace)
-        tot = sum(v[1] for v in agg.values()) or 1.0
-        out += ["| kernel | calls | mean us | total ms | share |", "|---|---:|---:|---:|---:|"]
-        for k, (n, t) in sorted(agg.items(), key=lambda kv: -kv[1][1])[: a.top]:
-            out.append(f"| {k} | {n} | {t / max(n, 1) / 1e3:.1f} | {t / 1e6:.3f} | {100 * t / tot:.1f}% |")
+        tot = sum(sum(v) for v in agg.values()) or 1.0
+        out += ["| kernel | calls | mean us | median us | min us | max us | total ms | share |",
+                "|---|---:|---:|---:|---:|---:|---:|---:|"]
+        for k, d in sorted(agg.items(), key=lambda kv: -sum(kv[1]))[: a.top]:
+            d = sorted(d)
+            t = sum(d)
+            out.append(f"| {k} | {len(d)} | {t / len(d):.1f} | {d[len(d) // 2]:.1f} | {d[0]:.1f} | {d[-1]:.1f} | "
+                       f"{t / 1e3:.3f} | {100 * t / tot:.1f}% |")
         out.append("")
     if a.fetch or a.write:
         f = counter_means(a.fetch, "FETCH_SIZE") if a.fetch else {}

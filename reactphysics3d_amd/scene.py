@@ -85,6 +85,7 @@ class Scene:
     bodies: list = field(default_factory=list)
     colliders: list = field(default_factory=list)
     joints: list = field(default_factory=list)
+    hulls: list = field(default_factory=list)   # (vertices [V, 3] float32, faces: list of CCW index lists)
 
     # -- builders -------------------------------------------------------------------------------
     def add_body(self, btype, pos, quat=(0.0, 0.0, 0.0, 1.0), lin=(0.0, 0.0, 0.0), ang=(0.0, 0.0, 0.0),
@@ -106,6 +107,22 @@ class Scene:
                   limits=(0.0, 0.0, 0.0, 0.0)):
         self.joints.append((jtype, body1, body2, tuple(anchor), tuple(axis), int(collide), flags, tuple(limits)))
         return len(self.joints) - 1
+
+    def add_hull(self, vertices, faces):
+        """A convex mesh (PhysicsCommon::createConvexMesh(PolygonVertexArray)); returns its index, the
+        params[0] of a HULL collider."""
+        self.hulls.append((np.ascontiguousarray(vertices, np.float32).reshape(-1, 3), [list(f) for f in faces]))
+        return len(self.hulls) - 1
+
+    def hull_arrays(self):
+        """(vertices float32 [sum V * 3], ints int32 [nb hulls, then per hull nv, nf, face sizes,
+        face indices]) -- the layout the reference harness reads."""
+        verts = [h[0].ravel() for h in self.hulls]
+        ints = [len(self.hulls)]
+        for v, faces in self.hulls:
+            ints += [len(v), len(faces)] + [len(f) for f in faces] + [i for f in faces for i in f]
+        vs = np.concatenate(verts) if verts else np.zeros(3, np.float32)
+        return np.ascontiguousarray(vs, np.float32), np.array(ints, np.int32)
 
     # -- packed views ---------------------------------------------------------------------------
     def body_array(self) -> np.ndarray:
@@ -246,6 +263,63 @@ def jointed_tower(height=6, seed=4, kinds=(HINGE, FIXED), friction=0.5, gravity=
             a = 2.0 * math.pi * rng.uniform01()
             sc.add_joint(kind, prev, b, anchor, axis=(math.cos(a), math.sin(a), 0.0))
         prev = b
+    return sc
+
+
+def convex_hull(points):
+    """Convex hull of points in general position (no four coplanar): every triple whose plane has all
+    other points strictly on one side is a face, oriented counter-clockwise seen from outside.
+    Returns (vertices used, faces as index triples into them).  O(n^4) on the host; meant for the
+    small point sets (tens of points) of the convex-mesh scenes."""
+    P = np.asarray(points, np.float64)
+    n = len(P)
+    faces = []
+    for i in range(n):
+        for j in range(i + 1, n):
+            for k in range(j + 1, n):
+                nrm = np.cross(P[j] - P[i], P[k] - P[i])
+                d = (P - P[i]) @ nrm
+                d[[i, j, k]] = 0.0
+                scale = np.abs(d).max()
+                tol = 1e-12 * max(1.0, scale)
+                if np.all(d <= tol):
+                    faces.append((i, j, k))
+                elif np.all(d >= -tol):
+                    faces.append((i, k, j))
+    used = sorted({v for f in faces for v in f})
+    remap = {v: t for t, v in enumerate(used)}
+    return P[used].astype(np.float32), [[remap[v] for v in f] for f in faces]
+
+
+def ellipsoid_hull(rng, n_points=24, lo=0.3, hi=1.2):
+    """Hull of n points on an ellipsoid with semi-axes ~U[lo, hi] (point = axes * uniform unit vector)."""
+    axes = [rng.uniform(lo, hi) for _ in range(3)]
+    pts = []
+    for _ in range(n_points):
+        z = 2.0 * rng.uniform01() - 1.0
+        a = 2.0 * math.pi * rng.uniform01()
+        r = math.sqrt(max(0.0, 1.0 - z * z))
+        pts.append((axes[0] * r * math.cos(a), axes[1] * r * math.sin(a), axes[2] * z))
+    return convex_hull(pts)
+
+
+def hull_scene(n_hulls=16384, n_chains=256, chain_length=16, extent=(120.0, 120.0, 80.0), seed=3, n_points=24):
+    """Convex-mesh bodies (hulls of 24 seeded ellipsoid points, semi-axes U[0.3, 1.2]) at seeded
+    positions / orientations in a volume, plus chains of 0.3 m spheres on ball-and-socket joints
+    hanging from static anchors, above a static ground box."""
+    rng = SplitMix64(seed)
+    sc = Scene(f"hulls_{n_hulls}_chains_{n_chains}x{chain_length}", make_settings(gravity=(0.0, 0.0, -9.81)))
+    g = sc.add_body(STATIC, (0.0, 0.0, -0.5))
+    sc.add_collider(g, BOX, (extent[0] / 2 + 10.0, extent[1] / 2 + 10.0, 0.5))
+    for _ in range(n_hulls):
+        v, f = ellipsoid_hull(rng, n_points)
+        h = sc.add_hull(v, f)
+        pos = (rng.uniform(-extent[0] / 2, extent[0] / 2), rng.uniform(-extent[1] / 2, extent[1] / 2),
+               rng.uniform(1.0, extent[2]))
+        b = sc.add_body(DYNAMIC, pos, rng.quaternion())
+        sc.add_collider(b, HULL, (float(h),), density=1000.0)
+    if n_chains:
+        sphere_chains(n_chains, chain_length, radius=0.3, spacing=3.0, seed=seed + 1, sc=sc)
     return sc
 
 

@@ -1,0 +1,97 @@
+"""Learned warm-start network on the matrix cores vs the float64 oracle (bf16 rounding emulated at the
+device's rounding points): forward, gradient, training; and the end-to-end effect of the trained
+network on the solver's iterations to the velocity residual."""
+import numpy as np
+import pytest
+
+from oracle import mlp_ref as R
+from reactphysics3d_amd import scene as S
+
+pytestmark = pytest.mark.gpu
+
+DT = 1.0 / 60.0
+
+
+def _net(seed):
+    from reactphysics3d_amd.lib import Mlp
+    rng = np.random.default_rng(seed)
+    p = R.init_params(seed).astype(np.float32)
+    mu = rng.normal(0, 0.1, R.N_IN).astype(np.float32)
+    inv = rng.uniform(0.5, 2.0, R.N_IN).astype(np.float32)
+    scale = np.array([0.3, 0.1, 0.1], np.float32)
+    m = Mlp(seed)
+    m.set_params(p)
+    m.set_normalization(mu, inv, scale)
+    return m, p.astype(np.float64), mu, inv, scale, rng
+
+
+@pytest.mark.parametrize("n", [1, 16, 1000])
+def test_forward_matches_oracle(n):
+    m, p, mu, inv, scale, rng = _net(7)
+    X = rng.normal(0, 1, (n, R.N_IN)).astype(np.float32)
+    y = m.forward(X)
+    y_ref, _ = R.forward(p, X, mu, inv, scale, bf=True)
+    err = np.max(np.abs(y - y_ref)) / max(1e-6, np.max(np.abs(y_ref)))
+    print("forward max rel err", err)
+    assert err < 2e-3
+
+
+@pytest.mark.parametrize("n", [64, 200, 1000])
+def test_gradient_matches_oracle(n):
+    m, p, mu, inv, scale, rng = _net(11)
+    X = rng.normal(0, 1, (n, R.N_IN)).astype(np.float32)
+    T = rng.normal(0, 0.2, (n, R.N_OUT)).astype(np.float32)
+    g, loss = m.gradient(X, T)
+    loss_ref, g_ref = R.loss_and_grad(p, X, T, mu, inv, scale, bf=True)
+    assert abs(loss - loss_ref) <= 1e-4 * abs(loss_ref) + 1e-7
+    for name, a, b in [("W1", R.P_W1, R.P_B1), ("b1", R.P_B1, R.P_W2), ("W2", R.P_W2, R.P_B2),
+                       ("b2", R.P_B2, R.P_W3), ("W3", R.P_W3, R.P_B3), ("b3", R.P_B3, R.N_PARAMS)]:
+        rel = np.linalg.norm(g[a:b] - g_ref[a:b]) / max(1e-12, np.linalg.norm(g_ref[a:b]))
+        print(name, "grad rel err", rel)
+        assert rel < 1e-2, name
+
+
+def test_training_reduces_loss():
+    m, p, mu, inv, scale, rng = _net(13)
+    X = rng.normal(0, 1, (8192, R.N_IN)).astype(np.float32)
+    T = np.stack([0.3 * np.tanh(X[:, 0] + 0.5 * X[:, 1]), 0.1 * np.sin(X[:, 2]), 0.05 * X[:, 3] * X[:, 4]], 1)
+    T = T.astype(np.float32)
+    _, l0 = m.gradient(X[:4096], T[:4096])
+    m.train(X, T, epochs=15, minibatch=512, lr=2e-3, seed=3)
+    _, l1 = m.gradient(X[:4096], T[:4096])
+    print("loss", l0, "->", l1)
+    assert l1 < 0.2 * l0
+
+
+def _mean_iterations(dev, steps):
+    its = []
+    for _ in range(steps):
+        dev.step(DT)
+        its.append(dev.iteration_stats()["mean_iterations"])
+    return float(np.mean(its))
+
+
+def test_learned_warm_start_reduces_iterations():
+    from reactphysics3d_amd.lib import Batch, Mlp
+    # training data: converged impulses of new contacts, plain solver, held-out seeds
+    train = [S.batched_world(w, seed=1000, min_bodies=16, max_bodies=160, max_joints=8) for w in range(96)]
+    log = Batch(train, solver="fused")
+    log.set_residual_mode(1e-4, 200)
+    log.set_logging(500_000)
+    log.step(DT, 90)
+    F, T = log.get_log(500_000)
+    assert len(F) > 5000
+    m = Mlp(5)
+    m.set_normalization(F.mean(0), 1.0 / (F.std(0) + 1e-6), T.std(0) + 1e-6)
+    loss = m.train(F, T, epochs=20, minibatch=1024, lr=1e-3, seed=9)
+    test = [S.batched_world(w, seed=2000, min_bodies=16, max_bodies=160, max_joints=8) for w in range(48)]
+    plain = Batch(test, solver="fused")
+    learned = Batch(test, solver="fused")
+    learned.set_warm_start("learned", m)
+    for d in (plain, learned):
+        d.set_residual_mode(1e-4, 200)
+    it_plain = _mean_iterations(plain, 90)
+    it_learned = _mean_iterations(learned, 90)
+    print(f"samples {len(F)} loss {loss:.4g} iterations plain {it_plain:.2f} learned {it_learned:.2f} "
+          f"({100 * (1 - it_learned / it_plain):.1f}% fewer)")
+    assert it_learned < it_plain
