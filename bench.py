@@ -5,13 +5,19 @@ Default workload (one MI355X): 4096 independent worlds in one batch, body counts
 log-uniform in [8, 512] (about 500k bodies), each a seeded box pile or box stack
 (height 4-20) with 0-32 hinge / fixed joints on a static ground, dt = 1/60 s,
 FP32, stepped by a captured hipGraph, every world iterating its velocity solver
-to the residual 1e-4 (iteration cap 100).  Inputs are resident in HBM before the
-timed region.  A step is one full PhysicsWorld::update() of every world:
-broadphase, narrowphase, contact creation, coloured contact + joint solver,
-integration, refit.
+to the residual 1e-4 (iteration cap 100).  Two copies of the batch are timed from
+the same warmed-up state: plain warm start (cached impulses, zero for new
+contacts, as the reference) and learned warm start (an MLP trained just before,
+on the GPU, on converged impulses logged from the plain solver on 1000 held-out
+worlds).  `value` is the learned form; both are reported.  Inputs are resident
+in HBM before the timed regions.  A step is one full PhysicsWorld::update() of
+every world: broadphase, narrowphase, contact creation, coloured contact + joint
+solver, integration, refit.
 
 Other workloads: --workload mixed (65,536 spheres/boxes/capsules in a walled
-container, 6/3 iterations) and --workload pile (1000 boxes, 10/5 iterations).
+container, 6/3 iterations), --workload pile (1000 boxes, 10/5 iterations) and
+--workload hulls (16,384 convex hulls of 24 ellipsoid points + 256 chains of 16
+spheres on ball-and-socket joints, 6/3 iterations).
 
 Prints one JSON line (see the repository's bench contract); the per-kernel
 breakdown goes to stderr.
@@ -79,7 +85,8 @@ BYTE_MODEL = {
     "np_poly_poly": "176 B/test + 40 B/contact point",
 }
 
-KERNELS = ["bp_bounds", "bp_morton", "bp_sort", "bp_karras", "bp_fit", "bp_world_roots", "bp_traverse", "bp_merge",
+KERNELS = ["bp_bounds", "bp_morton", "bp_sort", "bp_karras", "bp_fit", "bp_world_roots", "bp_traverse", "bp_copy_pairs",
+           "bp_merge",
            "np_classify", "np_sphere_sphere", "np_sphere_capsule", "np_capsule_capsule", "np_sphere_poly",
            "np_capsule_poly", "np_poly_poly", "sv_flag", "sv_compact", "sv_adj_count", "sv_adj_fill",
            "sv_color_round", "sv_color_finish", "sv_color_keys", "sv_color_sort", "sv_world_solve", "world_position",
@@ -99,6 +106,9 @@ def build_workload(args):
     elif args.workload == "pile":
         scenes = [S.box_pile(10, 10, seed=1)]
         name = "box_pile_1000"
+    elif args.workload == "hulls":
+        scenes = [S.hull_scene(16384, 256, 16, seed=3)]
+        name = "hulls_16384_chains_256x16"
     else:
         raise SystemExit(f"unknown workload {args.workload}")
     return name, scenes
@@ -158,12 +168,46 @@ def cpu_baseline(scenes, args, ndyn_total):
                       f"ReactPhysics3D 0.10.2 g++ -O3 FP32, fixed 10/5 iterations"}
 
 
+def train_warm_start(args):
+    """Learned warm start: (features, converged impulses) logged from the plain solver at the residual on
+    held-out worlds (stream seeds 1000-1999), network trained on the GPU by the project's kernels."""
+    from reactphysics3d_amd import scene as S
+    from reactphysics3d_amd.lib import Batch, Mlp
+    t0 = time.time()
+    held = Batch(S.batched_worlds(args.train_worlds, seed=1000))
+    held.set_residual_mode(RESIDUAL, RESIDUAL_CAP)
+    held.set_logging(args.train_samples)
+    steps = 0
+    while steps < args.train_max_steps:
+        held.step(DT, 10)
+        steps += 10
+        if log_count(held) >= args.train_samples:
+            break
+    F, T = held.get_log(args.train_samples)
+    held.close()
+    t1 = time.time()
+    mlp = Mlp(seed=7)
+    mlp.set_normalization(F.mean(0), 1.0 / (F.std(0) + 1e-6), T.std(0) + 1e-6)
+    loss = mlp.train(F, T, epochs=args.train_epochs, minibatch=4096, lr=2e-3, seed=11)
+    t2 = time.time()
+    info = {"samples": int(len(F)), "logged_steps": steps, "held_out_worlds": args.train_worlds,
+            "epochs": args.train_epochs, "minibatch": 4096, "final_loss": round(float(loss), 5),
+            "log_s": round(t1 - t0, 2), "train_s": round(t2 - t1, 2)}
+    log(f"[bench] warm-start network: {info}")
+    return mlp, info
+
+
+def log_count(batch):
+    from reactphysics3d_amd import lib
+    return int(lib.load().rp3dg_batch_get_log(batch.h, None, None, 1 << 62))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=100)
     ap.add_argument("--warmup", type=int, default=30)
-    ap.add_argument("--workload", default="batched", choices=["batched", "mixed", "pile"])
+    ap.add_argument("--workload", default="batched", choices=["batched", "mixed", "pile", "hulls"])
     ap.add_argument("--worlds", type=int, default=4096)
     ap.add_argument("--bodies", type=int, default=65536)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU reference baseline")
@@ -171,6 +215,10 @@ def main():
     ap.add_argument("--cpu-worlds", type=int, default=512)
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--no-profile", action="store_true", help="no HIP event timing at all")
+    ap.add_argument("--train-worlds", type=int, default=1000)
+    ap.add_argument("--train-samples", type=int, default=2_000_000)
+    ap.add_argument("--train-max-steps", type=int, default=200)
+    ap.add_argument("--train-epochs", type=int, default=6)
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="run a plain elementwise copy of MIB MiB first (counter calibration under rocprofv3)")
     args = ap.parse_args()
@@ -199,83 +247,105 @@ def main():
     log(f"[bench] workload {name}: {len(scenes)} worlds, {sum(sc.nb_bodies for sc in scenes)} bodies, "
         f"{ndyn} dynamic, {nj} joints, built in {time.time() - t0:.1f}s")
     t0 = time.time()
+    batched = args.workload == "batched"
     dev = Batch(scenes)
-    fused = args.workload == "batched" or len(scenes) > 1
+    dev_l = Batch(scenes) if batched else None   # same worlds, learned warm start
     log(f"[bench] uploaded in {time.time() - t0:.1f}s")
-    for _ in range(args.warmup):
-        dev.step(DT)
-    dev.sync()
+    for d in (dev, dev_l):
+        if d is None:
+            continue
+        for _ in range(args.warmup):
+            d.step(DT)
+        d.sync()
     log(f"[bench] warmup {args.warmup} steps done; counts {dev.counts()}")
+    graph = batched or len(scenes) > 1
 
-    residual = args.workload == "batched"
-    if residual:
-        dev.set_residual_mode(RESIDUAL, RESIDUAL_CAP)
-    graph = residual or fused
+    training = None
+    if batched:
+        mlp, training = train_warm_start(args)
+        for d in (dev, dev_l):
+            d.set_residual_mode(RESIDUAL, RESIDUAL_CAP)
+        dev_l.set_warm_start("learned", mlp)
 
-    # short eager pass with every kernel timed: per-kernel breakdown, dominant kernel
-    dominant = None
+    def timed(d, tag):
+        c0 = d.counts()
+        its = []
+        d.sync()
+        t0 = time.perf_counter()
+        if graph:
+            chunk = 10
+            for _ in range(args.steps // chunk):
+                d.step_graph(DT, chunk)
+                its.append(d.iteration_stats()["mean_iterations"])
+            if args.steps % chunk:
+                d.step_graph(DT, args.steps % chunk)
+        else:
+            d.step(DT, args.steps)
+        d.sync()
+        wall = time.perf_counter() - t0
+        c1 = d.counts()
+        r = {"body_steps_per_s": round(ndyn * args.steps / wall, 1), "ms_per_step": round(wall * 1e3 / args.steps, 4),
+             "contacts_per_s": round((c1["total_contact_points"] - c0["total_contact_points"]) / wall, 1),
+             "mean_velocity_iterations": round(float(sum(its) / len(its)), 2) if its else None, "wall": wall,
+             "counts": c1}
+        log(f"[bench] timed {tag} {args.steps} steps ({'graph' if graph else 'eager'}): {r['ms_per_step']:.3f} ms/step, "
+            f"{r['body_steps_per_s']:.4g} body-steps/s, {r['contacts_per_s']:.4g} contacts/s, "
+            f"mean iterations {r['mean_velocity_iterations']}")
+        return r
+
+    # ---------------- timed regions
+    plain = timed(dev, "plain")
+    learned = timed(dev_l, "learned") if batched else None
+    head = learned if batched else plain
+    main_dev = dev_l if batched else dev
+    wall, c1 = head["wall"], head["counts"]
+    value = head["body_steps_per_s"]
+
+    # per-kernel breakdown (eager, every kernel timed) and the dominant kernel
+    roofline = None
     if not args.no_profile:
-        dev.profile(True)
+        main_dev.profile(True)
         nprof = 5
-        for _ in range(nprof):
-            dev.step(DT)
-        items, _ = breakdown(dev, nprof, "eager")
-        ranked = sorted(items.items(), key=lambda kv: -kv[1][0] * kv[1][1])
-        for k, _ in ranked:
-            if bytes_per_launch(k, dev.counts(), ndyn, nj) is not None:
+        main_dev.step(DT, nprof)
+        items, _ = breakdown(main_dev, nprof, "eager")
+        main_dev.profile(False)
+        dominant = None
+        for k, _ in sorted(items.items(), key=lambda kv: -kv[1][0] * kv[1][1]):
+            if bytes_per_launch(k, main_dev.counts(), ndyn, nj) is not None:
                 dominant = k
                 break
-        dev.profile(False)
-        if not graph:
-            # eager timed region: the dominant kernel is timed with two HIP events per step
-            dev.profile(True, only=dominant)
-
-    # ---------------- timed region
-    c0 = dev.counts()
-    dev.sync()
-    t0 = time.perf_counter()
-    if graph:
-        dev.step_graph(DT, args.steps)
-    else:
-        dev.step(DT, args.steps)
-    dev.sync()
-    wall = time.perf_counter() - t0
-    c1 = dev.counts()
-    st = dev.iteration_stats()
-    value = ndyn * args.steps / wall
-    contacts = c1["total_contact_points"] - c0["total_contact_points"]
-    log(f"[bench] timed {args.steps} steps ({'graph' if graph else 'eager'}): {wall * 1e3 / args.steps:.3f} ms/step, "
-        f"{value:.4g} body-steps/s, {contacts / wall:.4g} contacts/s, iterations {st}")
-
-    roofline = None
-    timing_note = "HIP events around the kernel on the batch's stream, every step of the timed region"
-    if dominant and graph:
-        # graph replays are not event-timed: time the dominant kernel over an eager pass of the same
-        # number of steps right after the timed region (two HIP events per step)
-        dev.profile(True, only=dominant)
-        dev.step(DT, args.steps)
-        dev.sync()
-        timing_note = ("HIP events around the kernel on the batch's stream, eager pass of the same number of "
-                       "steps directly after the graph-timed region")
-    if dominant:
-        ms, n = dev.kernel_time(dominant)
-        b = bytes_per_launch(dominant, c1, ndyn, nj)
-        achieved = b / (ms * 1e-3) / 1e9
-        traffic = None
-        tpath = os.path.join(ROOT, "profiles", f"traffic_{name}.json")
-        if os.path.exists(tpath):
-            with open(tpath) as f:
-                traffic = json.load(f).get(dominant)
-        roofline = {"kernel": dominant, "bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
-                    "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": traffic,
-                    "bytes_per_launch": int(b), "byte_model": BYTE_MODEL.get(dominant, ""),
-                    "mean_launch_us": round(ms * 1e3, 2), "launches": n, "timing": timing_note,
-                    "share_of_step": round(ms / (wall * 1e3 / args.steps), 3)}
-        dev.profile(False)
+        if dominant:
+            # graph replays are not event-timed: the dominant kernel is timed over an eager pass of the same
+            # number of steps right after the timed regions (two HIP events per step, batch stream)
+            main_dev.profile(True, only=dominant)
+            main_dev.step(DT, args.steps)
+            main_dev.sync()
+            ms, n = main_dev.kernel_time(dominant)
+            main_dev.profile(False)
+            b = bytes_per_launch(dominant, c1, ndyn, nj)
+            achieved = b / (ms * 1e-3) / 1e9
+            traffic = None
+            tpath = os.path.join(ROOT, "profiles", f"traffic_{name}.json")
+            if os.path.exists(tpath):
+                with open(tpath) as f:
+                    traffic = json.load(f).get(dominant)
+            roofline = {"kernel": dominant, "bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
+                        "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": traffic,
+                        "bytes_per_launch": int(b), "byte_model": BYTE_MODEL.get(dominant, ""),
+                        "mean_launch_us": round(ms * 1e3, 2), "launches": n,
+                        "timing": "HIP events around the kernel on the batch's stream, eager pass of the same "
+                                  "number of steps directly after the timed regions",
+                        "share_of_step": round(ms / (wall * 1e3 / args.steps), 3)}
+            if dominant == "sv_world_solve" and head.get("mean_velocity_iterations"):
+                # every sweep re-reads the rows: 80 B / point (normal, r1, r2, impulse r/w) + 124 B / manifold
+                sweep = 80 * c1["nb_contact_points"] + 124 * c1["nb_manifolds"]
+                its = head["mean_velocity_iterations"]
+                roofline["row_bytes_per_sweep"] = int(sweep)
+                roofline["row_traffic_gbs"] = round(sweep * its / (ms * 1e-3) / 1e9, 2)
 
     fixed = None
-    if residual:
-        # same worlds with the fixed WorldSettings iteration count (the reference's own mode)
+    if batched:
+        # the same worlds with the fixed WorldSettings iteration count (the reference's own mode)
         dev.set_residual_mode(0.0)
         k = max(10, args.steps // 4)
         dev.sync()
@@ -284,8 +354,8 @@ def main():
         dev.sync()
         w2 = time.perf_counter() - t0
         fixed = {"body_steps_per_s": round(ndyn * k / w2, 1), "ms_per_step": round(w2 * 1e3 / k, 4),
-                 "velocity_iterations": 10, "steps": k}
-        log(f"[bench] fixed 10 iterations: {w2 * 1e3 / k:.3f} ms/step, {ndyn * k / w2:.4g} body-steps/s")
+                 "velocity_iterations": int(scenes[0].settings["vel_iters"]), "steps": k}
+        log(f"[bench] fixed iterations: {w2 * 1e3 / k:.3f} ms/step, {ndyn * k / w2:.4g} body-steps/s")
 
     cpu = None
     if not args.no_cpu:
@@ -309,17 +379,23 @@ def main():
         "vs_baseline": None,
         "dtype": "f32",
         "data": "synthetic",
-        "contacts_per_s": round(contacts / wall, 1),
+        "contacts_per_s": head["contacts_per_s"],
         "config": {"workload": name, "worlds": len(scenes), "bodies": sum(sc.nb_bodies for sc in scenes),
                    "dynamic_bodies": ndyn, "joints": nj, "dt": DT,
-                   "velocity_solver": (f"residual {RESIDUAL} (cap {RESIDUAL_CAP})" if residual
+                   "velocity_solver": (f"residual {RESIDUAL} (cap {RESIDUAL_CAP})" if batched
                                        else f"{int(s0['vel_iters'])} iterations"),
-                   "position_iterations": int(s0["pos_iters"]), "warm_start": "plain (cached impulses)",
+                   "position_iterations": int(s0["pos_iters"]),
+                   "warm_start": "learned (MLP, BF16 MFMA)" if batched else "plain (cached impulses)",
                    "step": "hipGraph replay" if graph else "eager",
-                   "mean_velocity_iterations": round(st["mean_iterations"], 2),
-                   "max_velocity_iterations": st["max_iterations"],
                    "pairs": c1["nb_pairs"], "manifolds": c1["nb_manifolds"],
                    "contact_points": c1["nb_contact_points"], "colors": c1["nb_colors"]},
+        "learned": ({k: v for k, v in learned.items() if k not in ("wall", "counts")} if learned else None),
+        "plain": {k: v for k, v in plain.items() if k not in ("wall", "counts")},
+        "learned_vs_plain": ({"iterations_ratio": round(learned["mean_velocity_iterations"] /
+                                                        max(1e-9, plain["mean_velocity_iterations"]), 4),
+                              "throughput_ratio": round(learned["body_steps_per_s"] / plain["body_steps_per_s"], 4)}
+                             if learned else None),
+        "training": training,
         "fixed_iterations": fixed,
         "roofline": roofline,
         "cpu_baseline": cpu,

@@ -1,0 +1,372 @@
+// C++ mirror of the ReactPhysics3D 0.10.2 public API for the step path, over the rp3d_mi355x C-ABI.
+//
+// An application written against the reference (include/reactphysics3d/engine/PhysicsCommon.h,
+// PhysicsWorld.h, body/RigidBody.h, collision/Collider.h, collision/shapes/*.h,
+// constraint/*Joint.h) includes this header instead and keeps its code: the same class names, the
+// same factory calls and the same PhysicsWorld::update(dt).  Objects are thin handles (world +
+// index) into the host world behind the C-ABI; getters after update() read a host mirror that the
+// library refreshes from the device once per step.  Only what the step path needs is provided:
+// sphere / box / capsule / convex-mesh colliders, ball-and-socket / hinge / fixed joints, materials,
+// collision filters, velocities, damping, gravity; sleeping is off.
+#pragma once
+
+#include <cmath>
+#include <cstdint>
+#include <memory>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "rp3d_mi355x.h"
+
+namespace reactphysics3d {
+
+using decimal = float;
+using uint32 = uint32_t;
+using uint16 = uint16_t;
+
+inline void check(int status, const char* what) {
+    if (status < 0) throw std::runtime_error(std::string(what) + ": " + rp3dg_last_error());
+}
+
+struct Vector3 {
+    decimal x = 0, y = 0, z = 0;
+    Vector3() = default;
+    Vector3(decimal a, decimal b, decimal c) : x(a), y(b), z(c) {}
+    static Vector3 zero() { return Vector3(0, 0, 0); }
+    Vector3 operator+(const Vector3& o) const { return Vector3(x + o.x, y + o.y, z + o.z); }
+    Vector3 operator-(const Vector3& o) const { return Vector3(x - o.x, y - o.y, z - o.z); }
+    Vector3 operator*(decimal s) const { return Vector3(x * s, y * s, z * s); }
+    decimal length() const { return std::sqrt(x * x + y * y + z * z); }
+};
+
+struct Quaternion {
+    decimal x = 0, y = 0, z = 0, w = 1;
+    Quaternion() = default;
+    Quaternion(decimal a, decimal b, decimal c, decimal d) : x(a), y(b), z(c), w(d) {}
+    static Quaternion identity() { return Quaternion(0, 0, 0, 1); }
+};
+
+class Transform {
+   public:
+    Transform() = default;
+    Transform(const Vector3& p, const Quaternion& q) : mPosition(p), mOrientation(q) {}
+    static Transform identity() { return Transform(); }
+    const Vector3& getPosition() const { return mPosition; }
+    const Quaternion& getOrientation() const { return mOrientation; }
+    void setPosition(const Vector3& p) { mPosition = p; }
+    void setOrientation(const Quaternion& q) { mOrientation = q; }
+
+   private:
+    Vector3 mPosition;
+    Quaternion mOrientation;
+};
+
+enum class BodyType { STATIC, KINEMATIC, DYNAMIC };
+
+// ---- shapes (collision/shapes/*.h): parameter holders, one collider each ----
+class CollisionShape {
+   public:
+    virtual ~CollisionShape() = default;
+    int32_t type() const { return mType; }
+    const float* params() const { return mParams; }
+
+   protected:
+    int32_t mType = RP3DG_SHAPE_SPHERE;
+    float mParams[4] = {0, 0, 0, 0};
+};
+class SphereShape : public CollisionShape {
+   public:
+    explicit SphereShape(decimal radius) { mType = RP3DG_SHAPE_SPHERE; mParams[0] = radius; }
+    decimal getRadius() const { return mParams[0]; }
+};
+class CapsuleShape : public CollisionShape {
+   public:
+    CapsuleShape(decimal radius, decimal height) { mType = RP3DG_SHAPE_CAPSULE; mParams[0] = radius; mParams[1] = height; }
+};
+class BoxShape : public CollisionShape {
+   public:
+    explicit BoxShape(const Vector3& halfExtents) {
+        mType = RP3DG_SHAPE_BOX;
+        mParams[0] = halfExtents.x; mParams[1] = halfExtents.y; mParams[2] = halfExtents.z;
+    }
+};
+
+// PolygonVertexArray (collision/PolygonVertexArray.h) with float vertices and int indices.
+class PolygonVertexArray {
+   public:
+    struct PolygonFace { uint32 nbVertices; uint32 indexBase; };
+    PolygonVertexArray(uint32 nbVertices, const float* vertices, const int32_t* indices, uint32 nbFaces,
+                       const PolygonFace* faces)
+        : mVertices(vertices, vertices + 3 * nbVertices), mNbFaces(nbFaces) {
+        for (uint32 f = 0; f < nbFaces; f++) {
+            mCounts.push_back((int32_t)faces[f].nbVertices);
+            for (uint32 k = 0; k < faces[f].nbVertices; k++) mIndices.push_back(indices[faces[f].indexBase + k]);
+        }
+    }
+    std::vector<float> mVertices;
+    std::vector<int32_t> mCounts, mIndices;
+    uint32 mNbFaces;
+};
+class ConvexMesh {
+   public:
+    explicit ConvexMesh(const PolygonVertexArray& a) : mArray(a) {}
+    PolygonVertexArray mArray;
+};
+class ConvexMeshShape : public CollisionShape {
+   public:
+    explicit ConvexMeshShape(ConvexMesh* mesh) : mMesh(mesh) { mType = RP3DG_SHAPE_CONVEX_MESH; }
+    ConvexMesh* mesh() const { return mMesh; }
+
+   private:
+    ConvexMesh* mMesh;
+};
+
+class PhysicsWorld;
+
+class Material {
+   public:
+    void setFrictionCoefficient(decimal f) { mFriction = f; push(); }
+    void setBounciness(decimal b) { mBounciness = b; push(); }
+    void setMassDensity(decimal d) { mDensity = d; push(); }
+    decimal getFrictionCoefficient() const { return mFriction; }
+    decimal getBounciness() const { return mBounciness; }
+    decimal getMassDensity() const { return mDensity; }
+
+   private:
+    friend class RigidBody;
+    void push() { check(rp3dg_collider_set_material(mWorld, mCollider, mFriction, mBounciness, mDensity), "material"); }
+    rp3dg_world* mWorld = nullptr;
+    int32_t mCollider = -1;
+    decimal mFriction = 0.3f, mBounciness = 0.5f, mDensity = 1.0f;
+};
+
+class Collider {
+   public:
+    Material& getMaterial() { return mMaterial; }
+    void setCollisionCategoryBits(uint16 bits) { mCategory = bits; check(rp3dg_collider_set_filter(mWorld, mIndex, mCategory, mMask), "filter"); }
+    void setCollideWithMaskBits(uint16 bits) { mMask = bits; check(rp3dg_collider_set_filter(mWorld, mIndex, mCategory, mMask), "filter"); }
+    int32_t index() const { return mIndex; }
+
+   private:
+    friend class RigidBody;
+    rp3dg_world* mWorld = nullptr;
+    int32_t mIndex = -1;
+    uint16 mCategory = 0x0001, mMask = 0xFFFF;
+    Material mMaterial;
+};
+
+class RigidBody {
+   public:
+    void setType(BodyType t) {
+        check(rp3dg_body_set_type(mWorld, mIndex, t == BodyType::STATIC ? RP3DG_STATIC
+                                                  : t == BodyType::KINEMATIC ? RP3DG_KINEMATIC : RP3DG_DYNAMIC), "setType");
+    }
+    Collider* addCollider(CollisionShape* shape, const Transform& t) {
+        const float p[3] = {t.getPosition().x, t.getPosition().y, t.getPosition().z};
+        const Quaternion& q = t.getOrientation();
+        const float o[4] = {q.x, q.y, q.z, q.w};
+        float params[4] = {shape->params()[0], shape->params()[1], shape->params()[2], shape->params()[3]};
+        if (shape->type() == RP3DG_SHAPE_CONVEX_MESH) {
+            const PolygonVertexArray& a = static_cast<ConvexMeshShape*>(shape)->mesh()->mArray;
+            const int32_t h = rp3dg_hull_create(mWorld, a.mVertices.data(), (int32_t)(a.mVertices.size() / 3),
+                                                a.mCounts.data(), a.mIndices.data(), (int32_t)a.mNbFaces);
+            check(h, "createConvexMesh");
+            params[0] = (float)h;
+        }
+        const int32_t c = rp3dg_collider_add(mWorld, mIndex, shape->type(), params, p, o);
+        check(c, "addCollider");
+        auto col = std::make_unique<Collider>();
+        col->mWorld = mWorld;
+        col->mIndex = c;
+        col->mMaterial.mWorld = mWorld;
+        col->mMaterial.mCollider = c;
+        col->mMaterial.mFriction = mDefaultFriction;     // the world's default material
+        col->mMaterial.mBounciness = mDefaultBounciness;
+        mColliders.push_back(std::move(col));
+        return mColliders.back().get();
+    }
+    void updateMassPropertiesFromColliders() { check(rp3dg_body_update_mass_properties(mWorld, mIndex), "updateMassProperties"); }
+    void setLinearVelocity(const Vector3& v) { mLin = v; pushVelocity(); }
+    void setAngularVelocity(const Vector3& w) { mAng = w; pushVelocity(); }
+    void enableGravity(bool on) { check(rp3dg_body_enable_gravity(mWorld, mIndex, on ? 1 : 0), "enableGravity"); }
+    void setLinearDamping(decimal d) { mLinDamp = d; check(rp3dg_body_set_damping(mWorld, mIndex, mLinDamp, mAngDamp), "damping"); }
+    void setAngularDamping(decimal d) { mAngDamp = d; check(rp3dg_body_set_damping(mWorld, mIndex, mLinDamp, mAngDamp), "damping"); }
+    Transform getTransform() const {
+        float p[3], q[4];
+        check(rp3dg_world_get_body_state(mWorld, mIndex, p, q, nullptr, nullptr), "getTransform");
+        return Transform(Vector3(p[0], p[1], p[2]), Quaternion(q[0], q[1], q[2], q[3]));
+    }
+    Vector3 getLinearVelocity() const {
+        float v[3];
+        check(rp3dg_world_get_body_state(mWorld, mIndex, nullptr, nullptr, v, nullptr), "getLinearVelocity");
+        return Vector3(v[0], v[1], v[2]);
+    }
+    Vector3 getAngularVelocity() const {
+        float w[3];
+        check(rp3dg_world_get_body_state(mWorld, mIndex, nullptr, nullptr, nullptr, w), "getAngularVelocity");
+        return Vector3(w[0], w[1], w[2]);
+    }
+    int32_t index() const { return mIndex; }
+
+   private:
+    friend class PhysicsWorld;
+    void pushVelocity() {
+        const float v[3] = {mLin.x, mLin.y, mLin.z}, w[3] = {mAng.x, mAng.y, mAng.z};
+        check(rp3dg_body_set_velocity(mWorld, mIndex, v, w), "setVelocity");
+    }
+    rp3dg_world* mWorld = nullptr;
+    int32_t mIndex = -1;
+    decimal mDefaultFriction = decimal(0.3), mDefaultBounciness = decimal(0.5);
+    Vector3 mLin, mAng;
+    decimal mLinDamp = 0, mAngDamp = 0;
+    std::vector<std::unique_ptr<Collider>> mColliders;
+};
+
+// ---- joints (constraint/*Joint.h), world-space anchor constructors ----
+struct JointInfo {
+    RigidBody* body1;
+    RigidBody* body2;
+    Vector3 anchorPointWorldSpace;
+    bool isCollisionEnabled = true;
+};
+struct BallAndSocketJointInfo : JointInfo {
+    BallAndSocketJointInfo(RigidBody* b1, RigidBody* b2, const Vector3& anchor) : JointInfo{b1, b2, anchor} {}
+};
+struct FixedJointInfo : JointInfo {
+    FixedJointInfo(RigidBody* b1, RigidBody* b2, const Vector3& anchor) : JointInfo{b1, b2, anchor} {}
+};
+struct HingeJointInfo : JointInfo {
+    HingeJointInfo(RigidBody* b1, RigidBody* b2, const Vector3& anchor, const Vector3& axis)
+        : JointInfo{b1, b2, anchor}, rotationAxisWorld(axis) {}
+    Vector3 rotationAxisWorld;
+    bool isLimitEnabled = false, isMotorEnabled = false;
+    decimal minAngleLimit = -1, maxAngleLimit = 1, motorSpeed = 0, maxMotorTorque = 0;
+};
+class Joint {
+   public:
+    int32_t index() const { return mIndex; }
+
+   private:
+    friend class PhysicsWorld;
+    int32_t mIndex = -1;
+};
+
+class PhysicsWorld {
+   public:
+    struct WorldSettings {
+        Vector3 gravity{0, decimal(-9.81), 0};
+        decimal persistentContactDistanceThreshold = decimal(0.03);
+        decimal defaultFrictionCoefficient = decimal(0.3);
+        decimal defaultBounciness = decimal(0.5);
+        decimal restitutionVelocityThreshold = decimal(0.5);
+        decimal cosAngleSimilarContactManifold = decimal(0.95);
+        uint16 defaultVelocitySolverNbIterations = 6;
+        uint16 defaultPositionSolverNbIterations = 3;
+        bool isSleepingEnabled = false;   // the device step path runs without sleeping
+        bool isGravityEnabled = true;
+    };
+
+    explicit PhysicsWorld(const WorldSettings& ws) : mSettings(ws) {
+        rp3dg_world_settings s;
+        rp3dg_world_settings_default(&s);
+        s.gravity[0] = ws.gravity.x; s.gravity[1] = ws.gravity.y; s.gravity[2] = ws.gravity.z;
+        s.persistent_contact_distance_threshold = ws.persistentContactDistanceThreshold;
+        s.default_friction_coefficient = ws.defaultFrictionCoefficient;
+        s.default_bounciness = ws.defaultBounciness;
+        s.restitution_velocity_threshold = ws.restitutionVelocityThreshold;
+        s.cos_angle_similar_contact_manifold = ws.cosAngleSimilarContactManifold;
+        s.default_velocity_solver_iterations = ws.defaultVelocitySolverNbIterations;
+        s.default_position_solver_iterations = ws.defaultPositionSolverNbIterations;
+        s.is_sleeping_enabled = ws.isSleepingEnabled ? 1 : 0;
+        s.is_gravity_enabled = ws.isGravityEnabled ? 1 : 0;
+        mWorld = rp3dg_world_create(&s);
+        if (!mWorld) throw std::runtime_error(std::string("createPhysicsWorld: ") + rp3dg_last_error());
+    }
+    ~PhysicsWorld() { rp3dg_world_destroy(mWorld); }
+    PhysicsWorld(const PhysicsWorld&) = delete;
+    PhysicsWorld& operator=(const PhysicsWorld&) = delete;
+
+    RigidBody* createRigidBody(const Transform& t) {
+        const float p[3] = {t.getPosition().x, t.getPosition().y, t.getPosition().z};
+        const Quaternion& q = t.getOrientation();
+        const float o[4] = {q.x, q.y, q.z, q.w};
+        const int32_t b = rp3dg_body_create(mWorld, p, o);
+        check(b, "createRigidBody");
+        auto body = std::make_unique<RigidBody>();
+        body->mWorld = mWorld;
+        body->mIndex = b;
+        body->mDefaultFriction = mSettings.defaultFrictionCoefficient;
+        body->mDefaultBounciness = mSettings.defaultBounciness;
+        mBodies.push_back(std::move(body));
+        return mBodies.back().get();
+    }
+    Joint* createJoint(const BallAndSocketJointInfo& i) { return addJoint(RP3DG_JOINT_BALLSOCKET, i, Vector3(), 0, nullptr); }
+    Joint* createJoint(const FixedJointInfo& i) { return addJoint(RP3DG_JOINT_FIXED, i, Vector3(), 0, nullptr); }
+    Joint* createJoint(const HingeJointInfo& i) {
+        const float lim[4] = {i.minAngleLimit, i.maxAngleLimit, i.motorSpeed, i.maxMotorTorque};
+        return addJoint(RP3DG_JOINT_HINGE, i, i.rotationAxisWorld, (i.isLimitEnabled ? 1 : 0) | (i.isMotorEnabled ? 2 : 0), lim);
+    }
+    void setNbIterationsVelocitySolver(uint16 n) { mVel = n; check(rp3dg_world_set_iterations(mWorld, mVel, mPos), "iterations"); }
+    void setNbIterationsPositionSolver(uint16 n) { mPos = n; check(rp3dg_world_set_iterations(mWorld, mVel, mPos), "iterations"); }
+    void setGravity(const Vector3& g) {
+        const float v[3] = {g.x, g.y, g.z};
+        check(rp3dg_world_set_gravity(mWorld, v), "setGravity");
+    }
+    // PhysicsWorld::update (src/engine/PhysicsWorld.cpp:239-307), on the GPU
+    void update(decimal timeStep) { check(rp3dg_world_update(mWorld, timeStep), "update"); }
+    rp3dg_world* handle() const { return mWorld; }
+
+   private:
+    Joint* addJoint(int32_t type, const JointInfo& i, const Vector3& axis, int32_t flags, const float* lim) {
+        const float a[3] = {i.anchorPointWorldSpace.x, i.anchorPointWorldSpace.y, i.anchorPointWorldSpace.z};
+        const float ax[3] = {axis.x, axis.y, axis.z};
+        const float zero[4] = {0, 0, 0, 0};
+        const int32_t j = rp3dg_joint_create(mWorld, type, i.body1->index(), i.body2->index(), a, ax,
+                                             i.isCollisionEnabled ? 1 : 0, flags, lim ? lim : zero);
+        check(j, "createJoint");
+        auto joint = std::make_unique<Joint>();
+        joint->mIndex = j;
+        mJoints.push_back(std::move(joint));
+        return mJoints.back().get();
+    }
+    rp3dg_world* mWorld = nullptr;
+    WorldSettings mSettings;
+    uint16 mVel = 6, mPos = 3;
+    std::vector<std::unique_ptr<RigidBody>> mBodies;
+    std::vector<std::unique_ptr<Joint>> mJoints;
+};
+
+// PhysicsCommon (engine/PhysicsCommon.h:176-257): owns worlds and shapes.
+class PhysicsCommon {
+   public:
+    PhysicsWorld* createPhysicsWorld(const PhysicsWorld::WorldSettings& s = PhysicsWorld::WorldSettings()) {
+        mWorlds.push_back(std::make_unique<PhysicsWorld>(s));
+        return mWorlds.back().get();
+    }
+    void destroyPhysicsWorld(PhysicsWorld* w) {
+        for (auto it = mWorlds.begin(); it != mWorlds.end(); ++it)
+            if (it->get() == w) { mWorlds.erase(it); return; }
+    }
+    SphereShape* createSphereShape(decimal r) { return keep(std::make_unique<SphereShape>(r)); }
+    BoxShape* createBoxShape(const Vector3& he) { return keep(std::make_unique<BoxShape>(he)); }
+    CapsuleShape* createCapsuleShape(decimal r, decimal h) { return keep(std::make_unique<CapsuleShape>(r, h)); }
+    ConvexMesh* createConvexMesh(const PolygonVertexArray& a) {
+        mMeshes.push_back(std::make_unique<ConvexMesh>(a));
+        return mMeshes.back().get();
+    }
+    ConvexMeshShape* createConvexMeshShape(ConvexMesh* m) { return keep(std::make_unique<ConvexMeshShape>(m)); }
+
+   private:
+    template <typename T>
+    T* keep(std::unique_ptr<T> p) {
+        T* r = p.get();
+        mShapes.push_back(std::move(p));
+        return r;
+    }
+    std::vector<std::unique_ptr<PhysicsWorld>> mWorlds;
+    std::vector<std::unique_ptr<CollisionShape>> mShapes;
+    std::vector<std::unique_ptr<ConvexMesh>> mMeshes;
+};
+
+}  // namespace reactphysics3d

@@ -129,6 +129,11 @@ int32_t rp3dg_joint_create(rp3dg_world* world, int32_t type, int32_t body1, int3
 
 /* Steps one world on the GPU (uploads it on first use, as a batch of one). */
 int rp3dg_world_update(rp3dg_world* world, float time_step);
+/* RigidBody::getTransform / getLinearVelocity / getAngularVelocity (src/body/RigidBody.cpp) of a world
+ * stepped with rp3dg_world_update: a host mirror refreshed from the device once after each update.
+ * Before the first update it returns the creation state.  Any output pointer may be NULL. */
+int rp3dg_world_get_body_state(rp3dg_world* world, int32_t body, float position[3], float orientation[4],
+                               float linear_velocity[3], float angular_velocity[3]);
 
 /* Many independent worlds stepped together: one launch sequence per step for all of them.  The
  * reference has one PhysicsWorld per update() call (src/engine/PhysicsWorld.cpp:239); a batch holds
@@ -206,6 +211,12 @@ int rp3dg_batch_get_joint_order(rp3dg_batch* batch, int32_t* order, int64_t capa
 int rp3dg_batch_profile(rp3dg_batch* batch, int32_t enable, const char* only_kernel);
 int rp3dg_batch_get_kernel_times(rp3dg_batch* batch, const char* kernel, double* mean_ms, int64_t* launches);
 
+/* Autotuner: times each candidate launch shape of the solver (fused layout: threads per world 64, 128,
+ * 192, 256; per-colour layout: threads per block 64..512) over steps_per_trial real steps of the
+ * batch (the steps advance the simulation) and keeps the fastest; returns the choice and its time. */
+int rp3dg_batch_autotune(rp3dg_batch* batch, float time_step, int32_t steps_per_trial, int32_t* chosen,
+                         double* best_ms_per_step);
+
 /* Solver convergence mode (no counterpart in the reference, which runs a fixed iteration count,
  * PhysicsWorld.cpp:334): every world iterates until the velocity residual
  *   max over its dynamic bodies of max(|v_k - v_{k-1}|_inf, |w_k - w_{k-1}|_inf)
@@ -222,8 +233,10 @@ int rp3dg_batch_get_iteration_stats(rp3dg_batch* batch, double* mean_iterations,
  * normal impulse of every contact point that has no cached impulse (the reference starts those from 0,
  * ContactSolverSystem.cpp:344-376) and the two friction impulses of a manifold with no resting point
  * (the reference zeroes them, :468-474).  Features per point: normal (3), r1 (3), r2 (3), relative
- * velocity along t1, t2, n (3), inverse masses (2), diagonals of the world inverse inertia tensors
- * (3 + 3), depth, friction coefficient, inverse normal mass, 1 / points in the manifold.  Targets:
+ * velocity along t1, t2, n (3), inverse masses (2), world inverse inertia xx of body 1, the
+ * manifold's cached normal impulse sum and resting point count, world inverse inertia xx of body 2,
+ * manifold counts of both bodies, depth, friction coefficient, inverse normal mass, 1 / points in
+ * the manifold.  Targets:
  * converged normal impulse, manifold friction impulses / points.  Parameters (20,099 floats, fp32
  * master): W1[24][128], b1[128], W2[128][128], b2[128], W3[128][3], b3[3]. */
 typedef struct rp3dg_mlp rp3dg_mlp;

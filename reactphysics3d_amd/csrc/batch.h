@@ -43,6 +43,9 @@ struct rp3dg_batch {
     // rows; otherwise one launch per colour over colour-major rows.
     bool fused = false;
     int fused_block = 64;
+    int color_block = 256;             // per-colour layout: threads per block of the colour kernels
+    std::vector<std::pair<int, double>> tune_table;   // autotuner trials: (shape, ms per step)
+    int lds_bodies = 0;                // fused solver: velocities of worlds up to this size staged in LDS
     int nseg = 0;                      // colour segments: kMaxColors (grid) or nworlds * kMaxColors (fused)
     int* seg_cnt = nullptr;
     int* seg_start = nullptr;
@@ -83,9 +86,11 @@ struct rp3dg_batch {
 
     // broadphase scratch
     uint64_t *bp_keys = nullptr, *bp_ktmp = nullptr, *bp_out = nullptr;
+    uint64_t* bp_slab = nullptr;       // first kTravSlab partners of every collider (first traversal pass)
     int *bp_vals = nullptr, *bp_vtmp = nullptr;
     int2 *bp_child = nullptr, *bp_range = nullptr;
-    int *bp_parent = nullptr, *bp_flags = nullptr, *bp_root = nullptr, *bp_count = nullptr, *bp_offs = nullptr;
+    int *bp_parent = nullptr, *bp_root = nullptr, *bp_count = nullptr, *bp_offs = nullptr;
+    float4 *fit_min = nullptr, *fit_max = nullptr;   // unions of aligned 64 / 4096 / 262144-leaf blocks
     float4 *bp_nmin = nullptr, *bp_nmax = nullptr, *bp_partial = nullptr;
     // narrowphase
     int *np_lists = nullptr, *np_counts = nullptr;

@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <chrono>
 #include <cstdio>
 #include <cstring>
 #include <map>
@@ -29,12 +30,16 @@ __global__ void rs_hist(const uint64_t* keys, int n, const int* nptr, int shift,
 __global__ void rs_scatter(const uint64_t* kin, const int* vin, uint64_t* kout, int* vout, int n, const int* nptr,
                            int shift, const int* offs, int nblocks);
 __global__ void bp_karras(const uint64_t* k, int n, int2* child, int* parent, int2* range);
-__global__ void bp_fit(DevColliders C, const int* sorted, int n, const int2* child, const int* parent, float4* nmin,
-                       float4* nmax, int* flags);
+__global__ void bp_leaves(DevColliders C, const int* sorted, int n, float4* nmin, float4* nmax, float4* bmin,
+                          float4* bmax);
+__global__ void bp_blocks(const float4* imin, const float4* imax, int nin, float4* omin, float4* omax);
+__global__ void bp_internal(int n, const int2* range, float4* nmin, float4* nmax, const float4* b1min,
+                            const float4* b1max, const float4* b2min, const float4* b2max, const float4* b3min,
+                            const float4* b3max);
 __global__ void bp_world_roots(const uint64_t* k, int n, const int2* range, int* root);
 __global__ void bp_traverse(DevColliders C, const int* sorted, int n, const int2* child, const float4* nmin,
                             const float4* nmax, const int* root, const uint64_t* nc, int nnc, int* count,
-                            const int* offset, uint64_t* out_keys, int capacity, int pass, int* status);
+                            const int* offset, uint64_t* out_keys, uint64_t* slab, int capacity, int pass, int* status);
 __global__ void bp_merge(DevColliders C, const uint64_t* keys, const int* nkeys_ptr, int capacity, DevPairs O,
                          const int* nold_ptr, DevPairs N, int* nnew_ptr, int* nb_pairs_ptr);
 
@@ -499,6 +504,9 @@ int rp3dg_batch::upload(rp3dg_world* const* ws, int nw, const rp3dg_batch_option
     max_world_bodies = 0;
     for (int wi = 0; wi < nw; wi++) max_world_bodies = std::max(max_world_bodies, body_base[wi + 1] - body_base[wi]);
     fused = o.solver == RP3DG_SOLVER_FUSED || (o.solver == RP3DG_SOLVER_AUTO && max_world_bodies <= 4096);
+    // solver velocities and inverse inertia of worlds up to 512 bodies live in LDS during the sweeps
+    // (112 B per body, < 64 KiB per block)
+    lds_bodies = (fused && max_world_bodies <= 512) ? ((max_world_bodies + 63) / 64) * 64 : 0;
     if (o.fused_block) fused_block = o.fused_block;
     else fused_block = max_world_bodies <= 128 ? 64 : (max_world_bodies <= 1024 ? 128 : 256);
 
@@ -578,11 +586,17 @@ int rp3dg_batch::upload(rp3dg_world* const* ws, int nw, const rp3dg_batch_option
     bp_keys = dalloc<uint64_t>(this, nc); bp_vals = dalloc<int>(this, nc);
     bp_ktmp = dalloc<uint64_t>(this, nc); bp_vtmp = dalloc<int>(this, nc);
     bp_child = dalloc<int2>(this, nn); bp_parent = dalloc<int>(this, nn); bp_range = dalloc<int2>(this, nn);
-    bp_nmin = dalloc<float4>(this, nn); bp_nmax = dalloc<float4>(this, nn); bp_flags = dalloc<int>(this, nn);
+    bp_nmin = dalloc<float4>(this, nn); bp_nmax = dalloc<float4>(this, nn);
+    {
+        const int nb1 = (nc + 63) / 64, nb2 = (nb1 + 63) / 64, nb3 = (nb2 + 63) / 64;
+        fit_min = dalloc<float4>(this, nb1 + nb2 + nb3);
+        fit_max = dalloc<float4>(this, nb1 + nb2 + nb3);
+    }
     bp_root = dalloc<int>(this, (size_t)1 << world_bits);
     bp_partial = dalloc<float4>(this, 2 * 256);
     bp_count = dalloc<int>(this, nc + 1); bp_offs = dalloc<int>(this, nc + 1);
     bp_out = dalloc<uint64_t>(this, pair_cap);
+    bp_slab = dalloc<uint64_t>(this, (size_t)nc * kTravSlab);
     np_lists = dalloc<int>(this, (size_t)ALGO_COUNT * pair_cap);
     np_counts = dalloc<int>(this, ALGO_COUNT);
     m_flag = dalloc<int>(this, pair_cap); m_offs = dalloc<int>(this, pair_cap + 1);
@@ -621,9 +635,9 @@ int rp3dg_batch::upload(rp3dg_world* const* ws, int nw, const rp3dg_batch_option
         HIPCHECK(hipMemsetAsync(world_res, 0, sizeof(float) * nw, stream));
     }
     S.pair = dalloc<int>(this, pair_cap); S.bodies = dalloc<int2>(this, pair_cap); S.npts = dalloc<int>(this, pair_cap);
-    for (float4** a : {&S.n, &S.r1f, &S.r2f, &S.t1, &S.t2, &S.r1ct1, &S.r1ct2, &S.r2ct1, &S.r2ct2, &S.imp, &S.oldt1, &S.oldt2})
+    for (float4** a : {&S.n, &S.r1f, &S.r2f, &S.t1, &S.t2, &S.imp, &S.oldt1, &S.oldt2})
         *a = dalloc<float4>(this, pair_cap);
-    for (float4** a : {&S.pn, &S.r1, &S.r2, &S.i1n, &S.i2n, &S.lam, &S.pred}) *a = dalloc<float4>(this, 4 * (size_t)pair_cap);
+    for (float4** a : {&S.pn, &S.r1, &S.r2, &S.lam, &S.pred}) *a = dalloc<float4>(this, 4 * (size_t)pair_cap);
     scan_sums_cap = 1 << 16;
     scan_sums_buf = dalloc<int>(this, scan_sums_cap);
     sort_hist_cap = 256 * (std::max(pair_cap, nc) / 4096 + 2);
@@ -696,6 +710,12 @@ int rp3dg_batch::upload(rp3dg_world* const* ws, int nw, const rp3dg_batch_option
         kern<<<(grid), (block), 0, stream>>>(__VA_ARGS__);   \
         timer_end();                                         \
     } while (0)
+#define LAUNCH_TS(name, grid, block, shmem, kern, ...)           \
+    do {                                                         \
+        timer_begin(name);                                       \
+        kern<<<(grid), (block), (shmem), stream>>>(__VA_ARGS__); \
+        timer_end();                                             \
+    } while (0)
 
 // Enqueues one full step on the batch's stream with no host synchronisation (the per-colour solver
 // layout reads the colour count back once, so only the fused layout can be captured).  Every
@@ -720,18 +740,28 @@ int rp3dg_batch::enqueue_step(float dt, bool capturing) {
     r = sort(&k, &vls, bp_ktmp, bp_vtmp, nc, nullptr, 30 + world_bits, stream);
     timer_end();
     if (r) { set_error("radix sort overflow"); return r; }
-    if (nc > 1) {
-        LAUNCH_T("bp_karras", grid_for(nc - 1), 256, bp_karras, k, nc, bp_child, bp_parent, bp_range);
-        fill_int<<<grid_for(nc), 256, 0, stream>>>(bp_flags, nc, 0);
+    if (nc > 1) LAUNCH_T("bp_karras", grid_for(nc - 1), 256, bp_karras, k, nc, bp_child, bp_parent, bp_range);
+    {
+        const int nb1 = (nc + 63) / 64, nb2 = (nb1 + 63) / 64, nb3 = (nb2 + 63) / 64;
+        timer_begin("bp_fit");
+        bp_leaves<<<grid_for(nc), 256, 0, stream>>>(C, vls, nc, bp_nmin, bp_nmax, fit_min, fit_max);
+        bp_blocks<<<grid_for(nb1), 256, 0, stream>>>(fit_min, fit_max, nb1, fit_min + nb1, fit_max + nb1);
+        bp_blocks<<<grid_for(nb2), 256, 0, stream>>>(fit_min + nb1, fit_max + nb1, nb2, fit_min + nb1 + nb2,
+                                                      fit_max + nb1 + nb2);
+        if (nc > 1)
+            bp_internal<<<grid_for(nc - 1), 256, 0, stream>>>(nc, bp_range, bp_nmin, bp_nmax, fit_min, fit_max,
+                                                              fit_min + nb1, fit_max + nb1, fit_min + nb1 + nb2,
+                                                              fit_max + nb1 + nb2);
+        timer_end();
+        (void)nb3;
     }
-    LAUNCH_T("bp_fit", grid_for(nc), 256, bp_fit, C, vls, nc, bp_child, bp_parent, bp_nmin, bp_nmax, bp_flags);
     LAUNCH_T("bp_world_roots", grid_for(2 * nc - 1), 256, bp_world_roots, k, nc, bp_range, bp_root);
-    LAUNCH_T("bp_traverse", grid_for(nc), 256, bp_traverse, C, vls, nc, bp_child, bp_nmin, bp_nmax, bp_root,
-             d_nocol, nnocol, bp_count, bp_offs, bp_out, pair_cap, 0, dstatus);
+    LAUNCH_T("bp_traverse", grid_for(nc, kTravBlock, 1 << 20), kTravBlock, bp_traverse, C, vls, nc, bp_child, bp_nmin,
+             bp_nmax, bp_root, d_nocol, nnocol, bp_count, bp_offs, bp_out, bp_slab, pair_cap, 0, dstatus);
     r = scan(bp_count, nc, nullptr, bp_offs, stream);
     if (r) { set_error("scan overflow"); return r; }
-    LAUNCH_T("bp_traverse", grid_for(nc), 256, bp_traverse, C, vls, nc, bp_child, bp_nmin, bp_nmax, bp_root,
-             d_nocol, nnocol, bp_count, bp_offs, bp_out, pair_cap, 1, dstatus);
+    LAUNCH_T("bp_copy_pairs", grid_for(nc, kTravBlock, 1 << 20), kTravBlock, bp_traverse, C, vls, nc, bp_child, bp_nmin,
+             bp_nmax, bp_root, d_nocol, nnocol, bp_count, bp_offs, bp_out, bp_slab, pair_cap, 1, dstatus);
     const int hp = capturing ? pair_cap : std::min(pair_cap, hint_pairs + hint_pairs / 4 + 1024);
     const int pg = grid_for(hp);
     LAUNCH_T("bp_merge", pg, 256, bp_merge, C, bp_out, bp_offs + nc, pair_cap, O, old_count, N, new_count,
@@ -806,6 +836,7 @@ int rp3dg_batch::enqueue_step(float dt, bool capturing) {
     SA.ncolors = &counters->nb_colors;
     SA.restitution_threshold = settings.restitution_threshold;
     SA.learned = (fused && warm_mode == RP3DG_WARM_START_LEARNED && mlp) ? 1 : 0;
+    SA.adj_offs = adj_offs;
     const float beta_dt = 0.2f / dt;
 
     if (fused) {
@@ -832,15 +863,17 @@ int rp3dg_batch::enqueue_step(float dt, bool capturing) {
         F.nrows = &counters->nb_new_points;
         F.feat_cap = feat_cap;
         F.row_slot = row_slot;
+        F.lds_bodies = lds_bodies;
+        const size_t lds = (size_t)lds_bodies * 7 * sizeof(float4);
         F.feats = nullptr;
         F.phase = 2;
         const bool features = SA.learned || log_cap > 0;
         if (!features) {
-            LAUNCH_T("sv_world_solve", nworlds, fused_block, sv_world_solve, F);
+            LAUNCH_TS("sv_world_solve", nworlds, fused_block, lds, sv_world_solve, F);
         } else {
             F.feats = feat_buf;
             F.phase = 0;
-            LAUNCH_T("sv_world_prepare", nworlds, fused_block, sv_world_solve, F);
+            LAUNCH_TS("sv_world_prepare", nworlds, fused_block, 0, sv_world_solve, F);
             if (SA.learned) {
                 const int rows = capturing ? feat_cap : std::min(feat_cap, last.nb_new_points + last.nb_new_points / 4 + 1024);
                 const int grid = std::max(1, std::min(2048, (rows + 63) / 64));
@@ -848,7 +881,7 @@ int rp3dg_batch::enqueue_step(float dt, bool capturing) {
                          nullptr, row_slot, S.pred);
             }
             F.phase = 1;
-            LAUNCH_T("sv_world_solve", nworlds, fused_block, sv_world_solve, F);
+            LAUNCH_TS("sv_world_solve", nworlds, fused_block, lds, sv_world_solve, F);
             if (log_cap > 0) {
                 LAUNCH_T("sv_log_samples", grid_for(feat_cap), 256, sv_log_samples, F, log_feats, log_targets, log_count,
                          log_cap);
@@ -870,7 +903,8 @@ int rp3dg_batch::enqueue_step(float dt, bool capturing) {
     // contact solver (PhysicsWorld.cpp:321-345)
     const int jg = grid_for(std::max(J.n, 1));
     LAUNCH_T("sv_prepare", mg, 256, sv_prepare, SA);
-    for (int c = 0; c < ncol; c++) LAUNCH_T("sv_warmstart", mg, 256, sv_warmstart, SA, c);
+    const int cg = grid_for(hm / std::max(1, ncol) + 1, color_block);
+    for (int c = 0; c < ncol; c++) LAUNCH_T("sv_warmstart", cg, color_block, sv_warmstart, SA, c);
     // ConstraintSolverSystem::initialize: initBeforeSolve + warm start of every joint kind
     if (J.n > 0) {
         LAUNCH_T("jt_init", jg, 256, jt_init, B, J);
@@ -879,7 +913,7 @@ int rp3dg_batch::enqueue_step(float dt, bool capturing) {
     for (int it = 0; it < settings.vel_iters; it++) {
         for (int s = 0; s < jt_slots; s++)
             LAUNCH_T("jt_solve_velocity", jg, 256, jt_solve_velocity, B, J, jt_ranges, s, dt);
-        for (int c = 0; c < ncol; c++) LAUNCH_T("sv_solve_color", mg, 256, sv_solve_color, SA, c, beta_dt);
+        for (int c = 0; c < ncol; c++) LAUNCH_T("sv_solve_color", cg, color_block, sv_solve_color, SA, c, beta_dt);
     }
     LAUNCH_T("sv_store", mg, 256, sv_store, SA);
     // position integration, joint position correction, state update, refit (PhysicsWorld.cpp:275-288, :348-360)
@@ -1025,7 +1059,37 @@ int rp3dg_world_update(rp3dg_world* w, float dt) {
         w->own = rp3dg_batch_create(ws, 1);
         if (!w->own) return RP3DG_EHIP;
     }
-    return rp3dg_batch_step(w->own, dt, 1);
+    const int r = rp3dg_batch_step(w->own, dt, 1);
+    w->mirror_dirty = true;
+    return r;
+}
+
+int rp3dg_world_get_body_state(rp3dg_world* w, int32_t body, float* pos, float* ori, float* lin, float* ang) {
+    if (!w || body < 0 || body >= (int32_t)w->bodies.size()) { set_error("invalid body index"); return RP3DG_EINVAL; }
+    if (!w->own) {   // not stepped yet: the creation state
+        const HostBody& b = w->bodies[body];
+        if (pos) { pos[0] = b.transform.p.x; pos[1] = b.transform.p.y; pos[2] = b.transform.p.z; }
+        if (ori) { ori[0] = b.transform.q.x; ori[1] = b.transform.q.y; ori[2] = b.transform.q.z; ori[3] = b.transform.q.w; }
+        if (lin) { lin[0] = b.v.x; lin[1] = b.v.y; lin[2] = b.v.z; }
+        if (ang) { ang[0] = b.w.x; ang[1] = b.w.y; ang[2] = b.w.z; }
+        return RP3DG_OK;
+    }
+    if (w->mirror_dirty || w->mirror_pos.empty()) {
+        const size_t n = w->bodies.size();
+        w->mirror_pos.resize(3 * n); w->mirror_q.resize(4 * n); w->mirror_v.resize(3 * n); w->mirror_w.resize(3 * n);
+        const int r = rp3dg_batch_get_bodies(w->own, w->mirror_pos.data(), w->mirror_q.data(), w->mirror_v.data(),
+                                             w->mirror_w.data());
+        if (r) return r;
+        w->mirror_dirty = false;
+    }
+    for (int k = 0; k < 3; k++) {
+        if (pos) pos[k] = w->mirror_pos[3 * body + k];
+        if (lin) lin[k] = w->mirror_v[3 * body + k];
+        if (ang) ang[k] = w->mirror_w[3 * body + k];
+    }
+    for (int k = 0; k < 4; k++)
+        if (ori) ori[k] = w->mirror_q[4 * body + k];
+    return RP3DG_OK;
 }
 
 void rp3dg_world_destroy(rp3dg_world* w) {
@@ -1237,6 +1301,42 @@ int64_t rp3dg_batch_get_log(rp3dg_batch* b, float* feats, float* targets, int64_
     if (k > 0 && feats) HIPCHECK(hipMemcpy(feats, b->log_feats, sizeof(float) * kFeatures * k, hipMemcpyDeviceToHost));
     if (k > 0 && targets) HIPCHECK(hipMemcpy(targets, b->log_targets, sizeof(float) * 3 * k, hipMemcpyDeviceToHost));
     return k;
+}
+
+// Autotuner: times every candidate launch shape of the solver over a few real steps of this batch
+// (the steps are not undone) and keeps the fastest.  Fused layout: threads per world (64..256);
+// per-colour layout: threads per block of the colour kernels.
+int rp3dg_batch_autotune(rp3dg_batch* b, float dt, int32_t steps_per_trial, int32_t* chosen, double* best_ms) {
+    if (!b || !(dt > 0.f) || steps_per_trial < 1 || steps_per_trial > 1000) return RP3DG_EINVAL;
+    std::vector<int> cand = b->fused ? std::vector<int>{64, 128, 192, 256} : std::vector<int>{64, 128, 256, 512};
+    const bool prof = b->profiling;
+    b->profiling = false;
+    b->drop_graphs();
+    int best = -1;
+    double best_t = 1e300;
+    for (int c : cand) {
+        if (b->fused) b->fused_block = c;
+        else b->color_block = c;
+        int r = b->step(dt);   // one untimed step with the new shape
+        if (r) { b->profiling = prof; return r; }
+        HIPCHECK(hipStreamSynchronize(b->stream));
+        const auto t0 = std::chrono::steady_clock::now();
+        for (int s = 0; s < steps_per_trial; s++) {
+            r = b->step(dt);
+            if (r) { b->profiling = prof; return r; }
+        }
+        HIPCHECK(hipStreamSynchronize(b->stream));
+        const double ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count() /
+                          steps_per_trial;
+        b->tune_table.push_back({c, ms});
+        if (ms < best_t) { best_t = ms; best = c; }
+    }
+    if (b->fused) b->fused_block = best;
+    else b->color_block = best;
+    b->profiling = prof;
+    if (chosen) *chosen = best;
+    if (best_ms) *best_ms = best_t;
+    return RP3DG_OK;
 }
 
 int rp3dg_batch_get_iteration_stats(rp3dg_batch* b, double* mean_it, double* max_it, double* max_res) {

@@ -10,7 +10,8 @@
 // fat AABB per collider (same FP32 refit as BroadPhaseSystem.cpp:155-185) and
 // rebuild a linear BVH over them every step:
 //   bp_bounds -> bp_morton -> radix sort (LDS, stable, 8-bit digits)
-//   -> bp_karras (Karras 2012 radix tree) -> bp_fit (bottom-up AABBs)
+//   -> bp_karras (Karras 2012 radix tree) -> node boxes as unions over leaf ranges
+//      (bp_leaves, bp_blocks, bp_internal; no inter-thread synchronisation)
 //   -> bp_traverse (count) -> scan -> bp_traverse (write, per-collider sorted
 //      segments) -> bp_merge (carry orientation, temporal-coherence record and
 //      contact cache of surviving pairs; orient new pairs like the reference).
@@ -215,34 +216,92 @@ __global__ void bp_karras(const uint64_t* k, int n, int2* child, int* parent, in
     }
 }
 
-// Leaves: copy the fat AABBs in sorted order.  Internal nodes: union, bottom-up, the second child
-// to arrive at a node continues upwards (flags reset by the host before the launch).
-__global__ void bp_fit(DevColliders C, const int* sorted, int n, const int2* child, const int* parent, float4* nmin,
-                       float4* nmax, int* flags) {
-    RP3DG_GRID_STRIDE(s, n) {
-        const int c = sorted[s];
-        int node = n - 1 + s;
-        float4 mn = C.amin[c], mx = C.amax[c];
-        nmin[node] = mn;
-        nmax[node] = mx;
-        if (n == 1) continue;
-        int p = parent[node];
-        for (int depth = 0; depth < n; depth++) {   // a path to the root has fewer than n internal nodes
-            __threadfence();  // release this subtree's AABB before signalling the parent
-            if (atomicAdd(&flags[p], 1) == 0) break;
-            __threadfence();  // acquire the sibling's AABB (written on another CU / XCD)
-            const int2 ch = child[p];
-            const int other = (ch.x == node) ? ch.y : ch.x;
-            const float4 omn = nmin[other];
-            const float4 omx = nmax[other];
-            mn = make_float4(fminf(mn.x, omn.x), fminf(mn.y, omn.y), fminf(mn.z, omn.z), 0.f);
-            mx = make_float4(fmaxf(mx.x, omx.x), fmaxf(mx.y, omx.y), fmaxf(mx.z, omx.z), 0.f);
-            nmin[p] = mn;
-            nmax[p] = mx;
-            if (p == 0) break;
-            node = p;
-            p = parent[p];
+// Synchronisation-free fit.  An internal node's box is the union of the fat AABBs of its leaf range
+// [lo, hi] (range[i] from bp_karras).  Min / max are exact, so any union order gives the same bits
+// as a bottom-up fit.  Unions over aligned blocks of 64, 4096 and 262144 sorted leaves are
+// precomputed, so a range costs at most ~64 reads per level.
+constexpr int kFitBlock = 64;
+
+__device__ __forceinline__ void box_union(float4& mn, float4& mx, float4 bmn, float4 bmx) {
+    mn = make_float4(fminf(mn.x, bmn.x), fminf(mn.y, bmn.y), fminf(mn.z, bmn.z), 0.f);
+    mx = make_float4(fmaxf(mx.x, bmx.x), fmaxf(mx.y, bmx.y), fmaxf(mx.z, bmx.z), 0.f);
+}
+
+__device__ __forceinline__ void wave_box_union(float4& mn, float4& mx) {
+#pragma unroll
+    for (int off = kWave / 2; off > 0; off >>= 1) {
+        float4 omn, omx;
+        omn.x = __shfl_xor(mn.x, off, kWave); omn.y = __shfl_xor(mn.y, off, kWave); omn.z = __shfl_xor(mn.z, off, kWave);
+        omx.x = __shfl_xor(mx.x, off, kWave); omx.y = __shfl_xor(mx.y, off, kWave); omx.z = __shfl_xor(mx.z, off, kWave);
+        box_union(mn, mx, omn, omx);
+    }
+}
+
+// Leaves (node n-1+s = sorted collider s) and the unions of aligned blocks of 64 leaves.
+__global__ void bp_leaves(DevColliders C, const int* sorted, int n, float4* nmin, float4* nmax, float4* bmin,
+                          float4* bmax) {
+    const int nb = (n + kWave - 1) / kWave;
+    for (int base = (blockIdx.x * blockDim.x + threadIdx.x) & ~(kWave - 1); base < nb * kWave;
+         base += gridDim.x * blockDim.x) {
+        const int s = base + lane_id();
+        float4 mn = make_float4(INFINITY, INFINITY, INFINITY, 0.f), mx = make_float4(-INFINITY, -INFINITY, -INFINITY, 0.f);
+        if (s < n) {
+            const int c = sorted[s];
+            mn = C.amin[c];
+            mx = C.amax[c];
+            nmin[n - 1 + s] = mn;
+            nmax[n - 1 + s] = mx;
         }
+        wave_box_union(mn, mx);
+        if (lane_id() == 0) {
+            bmin[base / kWave] = mn;
+            bmax[base / kWave] = mx;
+        }
+    }
+}
+
+// Unions of aligned groups of 64 entries of the level below.
+__global__ void bp_blocks(const float4* imin, const float4* imax, int nin, float4* omin, float4* omax) {
+    const int nout = (nin + kWave - 1) / kWave;
+    for (int base = (blockIdx.x * blockDim.x + threadIdx.x) & ~(kWave - 1); base < nout * kWave;
+         base += gridDim.x * blockDim.x) {
+        const int s = base + lane_id();
+        float4 mn = make_float4(INFINITY, INFINITY, INFINITY, 0.f), mx = make_float4(-INFINITY, -INFINITY, -INFINITY, 0.f);
+        if (s < nin) { mn = imin[s]; mx = imax[s]; }
+        wave_box_union(mn, mx);
+        if (lane_id() == 0) {
+            omin[base / kWave] = mn;
+            omax[base / kWave] = mx;
+        }
+    }
+}
+
+// Internal nodes: union over the leaf range with the largest aligned blocks that fit.
+__global__ void bp_internal(int n, const int2* range, float4* nmin, float4* nmax, const float4* b1min,
+                            const float4* b1max, const float4* b2min, const float4* b2max, const float4* b3min,
+                            const float4* b3max) {
+    RP3DG_GRID_STRIDE(i, n - 1) {
+        const int2 r = range[i];
+        int s = r.x;
+        const int e = r.y + 1;   // exclusive
+        float4 mn = make_float4(INFINITY, INFINITY, INFINITY, 0.f), mx = make_float4(-INFINITY, -INFINITY, -INFINITY, 0.f);
+        while (s < e) {
+            if ((s & (262144 - 1)) == 0 && s + 262144 <= e) {
+                box_union(mn, mx, b3min[s >> 18], b3max[s >> 18]);
+                s += 262144;
+            } else if ((s & (4096 - 1)) == 0 && s + 4096 <= e) {
+                box_union(mn, mx, b2min[s >> 12], b2max[s >> 12]);
+                s += 4096;
+            } else if ((s & (kFitBlock - 1)) == 0 && s + kFitBlock <= e) {
+                box_union(mn, mx, b1min[s >> 6], b1max[s >> 6]);
+                s += kFitBlock;
+            } else {
+                box_union(mn, mx, nmin[n - 1 + s], nmax[n - 1 + s]);
+                s += 1;
+            }
+        }
+        nmin[i] = mn;
+        nmax[i] = mx;
     }
 }
 
@@ -280,7 +339,7 @@ __device__ __forceinline__ bool no_collision_pair(const uint64_t* nc, int nnc, u
     return false;
 }
 
-constexpr int kTravStack = 64;
+constexpr int kTravStack = 64;   // traversal stack depth (LDS, per thread)
 
 // Pair filter of CollisionDetectionSystem::updateOverlappingPairs (:284-327).
 __device__ __forceinline__ bool pair_allowed(const DevColliders& C, int a, int b, const uint64_t* nc, int nnc) {
@@ -301,9 +360,21 @@ __device__ __forceinline__ bool pair_allowed(const DevColliders& C, int a, int b
 //   traverse: a floor's AABB overlaps most of the tree and would serialise one thread over it).
 // Pass 0 counts, pass 1 writes each emitter's partner list sorted ascending into its segment, so the
 // pair table is sorted by key.
-__global__ void bp_traverse(DevColliders C, const int* sorted, int n, const int2* child, const float4* nmin,
-                            const float4* nmax, const int* root, const uint64_t* nc, int nnc, int* count,
-                            const int* offset, uint64_t* out_keys, int capacity, int pass, int* status) {
+// Pass 0 traverses once, counts every collider's partners and keeps the first kTravSlab of them,
+// sorted, in a per-collider slab.  Pass 1 (after the scan of the counts) copies the slabs into the
+// emitters' segments of the pair-key table and traverses again only for colliders with more
+// partners than the slab holds.  The traversal stack lives in LDS (one column per thread).
+__global__ __launch_bounds__(kTravBlock) void bp_traverse(DevColliders C, const int* sorted, int n, const int2* child,
+                                                          const float4* nmin, const float4* nmax, const int* root,
+                                                          const uint64_t* nc, int nnc, int* count, const int* offset,
+                                                          uint64_t* out_keys, uint64_t* slab, int capacity, int pass,
+                                                          int* status) {
+    __shared__ int lstack[kTravStack][kTravBlock];
+    const int t = threadIdx.x;
+    if (blockDim.x > kTravBlock) {   // the LDS stack has one column per thread of a kTravBlock block
+        if (t == 0) atomicOr(status, STATUS_STACK_OVERFLOW);
+        return;
+    }
     RP3DG_GRID_STRIDE(s, n) {
         const int c = sorted[s];
         const int base = pass ? offset[c] : 0;
@@ -317,21 +388,29 @@ __global__ void bp_traverse(DevColliders C, const int* sorted, int n, const int2
             if (!pass) count[c] = e - b;
             continue;
         }
+        uint64_t* sl = slab + (size_t)c * kTravSlab;
+        if (pass) {
+            const int total = count[c];
+            if (total <= kTravSlab) {
+                if (base + total > capacity) { atomicOr(status, STATUS_PAIR_OVERFLOW); continue; }
+                for (int k = 0; k < total; k++) out_keys[base + k] = sl[k];
+                continue;
+            }
+        }
         const float4 amn = C.amin[c], amx = C.amax[c];
-        int stack[kTravStack];
         int sp = 0;
-        stack[sp++] = root[C.world[c]];
+        lstack[sp++][t] = root[C.world[c]];
         int cnt = 0;
         while (sp > 0) {
-            const int node = stack[--sp];
+            const int node = lstack[--sp][t];
             if (!aabb_overlap(amn, amx, nmin[node], nmax[node])) continue;
             if (node >= n - 1) {
                 const int o = sorted[node - (n - 1)];
                 if (o == c) continue;
                 if (!C.is_static[o] && o < c) continue;
                 if (!pair_allowed(C, c, o, nc, nnc)) continue;
+                const uint64_t key = ((uint64_t)(uint32_t)c << 32) | (uint32_t)o;
                 if (pass) {
-                    const uint64_t key = ((uint64_t)(uint32_t)c << 32) | (uint32_t)o;
                     int pos = base + cnt;
                     if (pos < capacity) {
                         // insertion into the sorted segment
@@ -343,13 +422,20 @@ __global__ void bp_traverse(DevColliders C, const int* sorted, int n, const int2
                     } else {
                         atomicOr(status, STATUS_PAIR_OVERFLOW);
                     }
+                } else if (cnt < kTravSlab) {
+                    int pos = cnt;
+                    while (pos > 0 && sl[pos - 1] > key) {
+                        sl[pos] = sl[pos - 1];
+                        pos--;
+                    }
+                    sl[pos] = key;
                 }
                 cnt++;
             } else {
                 if (sp + 2 > kTravStack) { atomicOr(status, STATUS_STACK_OVERFLOW); break; }
                 const int2 ch = child[node];
-                stack[sp++] = ch.x;
-                stack[sp++] = ch.y;
+                lstack[sp++][t] = ch.x;
+                lstack[sp++][t] = ch.y;
             }
         }
         if (!pass) count[c] = cnt;

@@ -30,6 +30,8 @@ struct FusedArgs {
     int* row_slot;             // solver point slot (4 * row + point) of each feature row
     int* nrows;                // feature rows written this step
     int feat_cap;
+    int lds_bodies;            // worlds with at most this many bodies keep cv, cw, sv, sw and the world
+                               // inverse inertia rows in LDS (112 * lds_bodies bytes); 0 = global memory
 };
 
 constexpr int kFeatures = 24;

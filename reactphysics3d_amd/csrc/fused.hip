@@ -38,8 +38,9 @@ __device__ __forceinline__ float block_maxf(float v, float* red) {
 
 __global__ __launch_bounds__(kFusedMaxBlock) void sv_world_solve(FusedArgs F) {
     __shared__ float red[16];
-    const SvArgs& A = F.A;
-    const DevBodies& B = A.B;
+    extern __shared__ float4 s_vel[];   // [7][lds_bodies]: cv, cw, sv, sw, iw0, iw1, iw2 of the world's bodies
+    const SvArgs& AG = F.A;
+    const DevBodies& B = AG.B;
     const int w = F.block_world[blockIdx.x];
     const int tid = threadIdx.x, nt = blockDim.x;
     const int b0 = F.wbody[w], b1 = F.wbody[w + 1];
@@ -53,6 +54,7 @@ __global__ __launch_bounds__(kFusedMaxBlock) void sv_world_solve(FusedArgs F) {
         for (int i = b0 + tid; i < b1; i += nt) dyn_integrate_velocities_one(B, F.settings, F.dt, i);
         __syncthreads();
         // ContactSolverSystem::init (rows); ConstraintSolverSystem::initialize (initBeforeSolve)
+        const SvArgs& A = AG;
         for (int i = m0 + tid; i < m1; i += nt) sv_prepare_one(A, i);
         for (int j = j0 + tid; j < j1; j += nt) jt_init_one(B, F.J, j);
         if (F.feats) {
@@ -64,7 +66,7 @@ __global__ __launch_bounds__(kFusedMaxBlock) void sv_world_solve(FusedArgs F) {
                 int k = 0;
                 if (i < m1) {
                     const int np = A.S.npts[i];
-                    for (int p = 0; p < np; p++) k += A.S.i1n[4 * i + p].w == 0.0f ? 1 : 0;
+                    for (int p = 0; p < np; p++) k += A.S.lam[4 * i + p].z == 0.0f ? 1 : 0;
                 }
                 const int incl = wave_incl_scan(k);
                 int start = 0;
@@ -74,7 +76,7 @@ __global__ __launch_bounds__(kFusedMaxBlock) void sv_world_solve(FusedArgs F) {
                     const int np = A.S.npts[i];
                     for (int p = 0; p < np; p++) {
                         const int q = 4 * i + p;
-                        if (A.S.i1n[q].w != 0.0f) continue;
+                        if (A.S.lam[q].z != 0.0f) continue;
                         A.S.pred[q] = make_float4(0.f, 0.f, 0.f, 0.f);
                         if (start < F.feat_cap) {
                             float f[kFeatures];
@@ -90,6 +92,30 @@ __global__ __launch_bounds__(kFusedMaxBlock) void sv_world_solve(FusedArgs F) {
         __syncthreads();
         if (F.phase == 0) return;
     }
+    // stage the world's solver velocities and world inverse inertia rows in LDS for the sweeps (the
+    // rows and the rest of the body data stay in HBM)
+    const int nbw = b1 - b0;
+    const bool staged = nbw <= F.lds_bodies;
+    SvArgs A = AG;
+    if (staged) {
+        const int L = F.lds_bodies;
+        float4 *lcv = s_vel, *lcw = s_vel + L, *lsv = s_vel + 2 * L, *lsw = s_vel + 3 * L;
+        float4 *li0 = s_vel + 4 * L, *li1 = s_vel + 5 * L, *li2 = s_vel + 6 * L;
+        for (int i = tid; i < nbw; i += nt) {
+            lcv[i] = B.cv[b0 + i];
+            lcw[i] = B.cw[b0 + i];
+            lsv[i] = B.sv[b0 + i];
+            lsw[i] = B.sw[b0 + i];
+            li0[i] = B.iw0[b0 + i];
+            li1[i] = B.iw1[b0 + i];
+            li2[i] = B.iw2[b0 + i];
+        }
+        A.B.cv = lcv; A.B.cw = lcw; A.B.sv = lsv; A.B.sw = lsw;
+        A.B.iw0 = li0; A.B.iw1 = li1; A.B.iw2 = li2;
+        A.B.vbase = b0;
+        __syncthreads();
+    }
+    const DevBodies& BS = A.B;
     // contact warm start, colour by colour (CS.cpp:326-476)
     for (int c = 0; c < kMaxColors; c++) {
         const int lo = seg[c], hi = seg[c + 1];
@@ -101,14 +127,14 @@ __global__ __launch_bounds__(kFusedMaxBlock) void sv_world_solve(FusedArgs F) {
     for (int s = 0; s < F.jt_slots; s++) {
         const int2 r = jr[s];
         if (r.x == r.y) continue;
-        for (int j = r.x + tid; j < r.y; j += nt) jt_warmstart_one(B, F.J, j);
+        for (int j = r.x + tid; j < r.y; j += nt) jt_warmstart_one(BS, F.J, j);
         __syncthreads();
     }
     const bool residual = F.res_thr > 0.0f;
     if (residual) {
         for (int i = b0 + tid; i < b1; i += nt) {
-            F.pv[i] = B.cv[i];
-            F.pw[i] = B.cw[i];
+            F.pv[i] = BS.cv[i - BS.vbase];
+            F.pw[i] = BS.cw[i - BS.vbase];
         }
         __syncthreads();
     }
@@ -118,7 +144,7 @@ __global__ __launch_bounds__(kFusedMaxBlock) void sv_world_solve(FusedArgs F) {
         for (int s = 0; s < F.jt_slots; s++) {
             const int2 r = jr[s];
             if (r.x == r.y) continue;
-            for (int j = r.x + tid; j < r.y; j += nt) jt_solve_velocity_one(B, F.J, j, F.dt);
+            for (int j = r.x + tid; j < r.y; j += nt) jt_solve_velocity_one(BS, F.J, j, F.dt);
             __syncthreads();
         }
         for (int c = 0; c < kMaxColors; c++) {
@@ -132,7 +158,7 @@ __global__ __launch_bounds__(kFusedMaxBlock) void sv_world_solve(FusedArgs F) {
             float r = 0.0f;
             for (int i = b0 + tid; i < b1; i += nt) {
                 if (!dyn_body(B, i)) continue;
-                const float4 v = B.cv[i], wv = B.cw[i], pv = F.pv[i], pw = F.pw[i];
+                const float4 v = BS.cv[i - BS.vbase], wv = BS.cw[i - BS.vbase], pv = F.pv[i], pw = F.pw[i];
                 float d = fabsf(v.x - pv.x);
                 d = fmaxf(d, fabsf(v.y - pv.y));
                 d = fmaxf(d, fabsf(v.z - pv.z));
@@ -149,6 +175,15 @@ __global__ __launch_bounds__(kFusedMaxBlock) void sv_world_solve(FusedArgs F) {
     }
     // storeImpulses (CS.cpp:763-785)
     for (int i = m0 + tid; i < m1; i += nt) sv_store_one(A, i);
+    if (staged) {
+        __syncthreads();
+        for (int i = tid; i < nbw; i += nt) {
+            B.cv[b0 + i] = BS.cv[i];
+            B.cw[b0 + i] = BS.cw[i];
+            B.sv[b0 + i] = BS.sv[i];
+            B.sw[b0 + i] = BS.sw[i];
+        }
+    }
     if (tid == 0) {
         F.world_iters[w] = it;
         F.world_res[w] = res;

@@ -200,7 +200,8 @@ __device__ __forceinline__ void jt_warmstart_one(const DevBodies& B, const DevJo
         const int2 bb = J.bodies[j];
         const int b1 = bb.x, b2 = bb.y;
         const int type = J.type[j];
-        V3 v1 = ld3(B.cv, b1), w1 = ld3(B.cw, b1), v2 = ld3(B.cv, b2), w2 = ld3(B.cw, b2);
+        const int u1 = b1 - B.vbase, u2 = b2 - B.vbase;
+        V3 v1 = ld3(B.cv, u1), w1 = ld3(B.cw, u1), v2 = ld3(B.cv, u2), w2 = ld3(B.cw, u2);
         const float im1 = B.ii[b1].w, im2 = B.ii[b2].w;
         const V3 ll1 = ld3(B.llock, b1), al1 = ld3(B.alock, b1), ll2 = ld3(B.llock, b2), al2 = ld3(B.alock, b2);
         const M3 I1 = ldm(J.i1r0, J.i1r1, J.i1r2, j), I2 = ldm(J.i2r0, J.i2r1, J.i2r2, j);
@@ -251,8 +252,8 @@ __device__ __forceinline__ void jt_warmstart_one(const DevBodies& B, const DevJo
             v2 += lock_mul((im2 * ll2), it);
             w2 += cmul(al2, mvmul(I2, ai2));
         }
-        B.cv[b1] = f4(v1, 0.f); B.cw[b1] = f4(w1, 0.f);
-        B.cv[b2] = f4(v2, 0.f); B.cw[b2] = f4(w2, 0.f);
+        B.cv[u1] = f4(v1, 0.f); B.cw[u1] = f4(w1, 0.f);
+        B.cv[u2] = f4(v2, 0.f); B.cw[u2] = f4(w2, 0.f);
     }
 }
 
@@ -264,7 +265,8 @@ __device__ __forceinline__ void jt_solve_velocity_one(const DevBodies& B, const 
         const int2 bb = J.bodies[j];
         const int b1 = bb.x, b2 = bb.y;
         const int type = J.type[j];
-        V3 v1 = ld3(B.cv, b1), w1 = ld3(B.cw, b1), v2 = ld3(B.cv, b2), w2 = ld3(B.cw, b2);
+        const int u1 = b1 - B.vbase, u2 = b2 - B.vbase;
+        V3 v1 = ld3(B.cv, u1), w1 = ld3(B.cw, u1), v2 = ld3(B.cv, u2), w2 = ld3(B.cw, u2);
         const float im1 = B.ii[b1].w, im2 = B.ii[b2].w;
         const V3 ll1 = ld3(B.llock, b1), al1 = ld3(B.alock, b1), ll2 = ld3(B.llock, b2), al2 = ld3(B.alock, b2);
         const M3 I1 = ldm(J.i1r0, J.i1r1, J.i1r2, j), I2 = ldm(J.i2r0, J.i2r1, J.i2r2, j);
@@ -368,8 +370,8 @@ __device__ __forceinline__ void jt_solve_velocity_one(const DevBodies& B, const 
             v2 += lock_mul((im2 * ll2), dl);
             w2 += cmul(al2, mvmul(I2, ai2));
         }
-        B.cv[b1] = f4(v1, 0.f); B.cw[b1] = f4(w1, 0.f);
-        B.cv[b2] = f4(v2, 0.f); B.cw[b2] = f4(w2, 0.f);
+        B.cv[u1] = f4(v1, 0.f); B.cw[u1] = f4(w1, 0.f);
+        B.cv[u2] = f4(v2, 0.f); B.cw[u2] = f4(w2, 0.f);
     }
 }
 

@@ -55,6 +55,8 @@ __host__ __device__ inline uint32_t lf_face(uint32_t w) { return (w >> 8) & 0xff
 __host__ __device__ inline uint32_t lf_edge1(uint32_t w) { return (w >> 16) & 0xffu; }
 __host__ __device__ inline uint32_t lf_edge2(uint32_t w) { return (w >> 24) & 0xffu; }
 
+constexpr int kTravBlock = 128;       // threads per block of the broadphase traversal (LDS stack columns)
+constexpr int kTravSlab = 8;          // partners kept per collider by the first traversal pass
 constexpr int kMaxRawContacts = 16;   // NB_MAX_CONTACT_POINTS_IN_NARROWPHASE_INFO (configuration.h:134)
 constexpr int kMaxManifoldPoints = 4; // MAX_CONTACT_POINTS_IN_MANIFOLD
 
@@ -70,8 +72,8 @@ struct HullHeader {
 struct DevBodies {
     float4 *x, *q, *p;          // centre of mass (world), orientation, body origin
     float4 *v, *w;              // velocities (RigidBodyComponents::mLinear/AngularVelocities)
-    float4 *cv, *cw;            // constrained velocities
-    float4 *sv, *sw;            // split velocities
+    float4 *cv, *cw;            // constrained velocities, indexed [body - vbase]
+    float4 *sv, *sw;            // split velocities, indexed [body - vbase]
     float4 *cx, *cq;            // constrained position / orientation
     float4 *force, *torque;     // external force / torque
     float4 *ii;                 // inverse local inertia xyz, w = inverse mass
@@ -81,6 +83,7 @@ struct DevBodies {
     int *flags;
     int *world;
     int n;
+    int vbase;                  // 0, or the first body of a world whose velocities are staged in LDS
 };
 
 struct DevColliders {
@@ -123,14 +126,12 @@ struct DevSolver {
     float4 *n;                 // manifold normal xyz, w = friction coefficient
     float4 *r1f, *r2f;         // r1Friction, r2Friction; w: inverse masses body1 / body2
     float4 *t1, *t2;           // friction vectors; w: inverse friction masses 1 / 2
-    float4 *r1ct1, *r1ct2, *r2ct1, *r2ct2;   // r x t; r1ct1.w = inverse twist mass
-    float4 *imp;               // friction1, friction2, twist impulses, w = sum of penetration impulses (scratch)
+    float4 *imp;               // friction1, friction2, twist impulses; w: inverse twist mass
     float4 *oldt1, *oldt2;     // previous friction vectors
     // per point [4*m + k]
     float4 *pn;                // normal xyz, penetration depth
     float4 *r1, *r2;           // w: inverse penetration mass | restitution bias
-    float4 *i1n, *i2n;         // I1 (r1 x n), I2 (r2 x n); w: resting flag | unused
-    float4 *lam;               // penetration impulse, split impulse
+    float4 *lam;               // penetration impulse, split impulse, resting flag (warm-start transfer)
     float4 *pred;              // learned warm start: predicted normal impulse, friction 1, friction 2
 };
 

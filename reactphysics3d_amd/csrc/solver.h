@@ -19,6 +19,7 @@ struct SvArgs {
     const int* ncolors;
     float restitution_threshold;
     int learned;          // learned warm start for points without a cached impulse
+    const int* adj_offs;  // body -> manifold adjacency offsets (manifold count of a dynamic body)
 };
 
 __global__ void dyn_integrate_velocities(DevBodies B, Settings S, float dt);

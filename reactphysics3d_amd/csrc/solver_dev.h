@@ -93,9 +93,11 @@ __device__ __forceinline__ void dyn_update_state_one(const DevBodies& B, int i) 
 // Solver rows (ContactSolverSystem::initializeForIsland, CS.cpp:115-320) in colour order
 // ------------------------------------------------------------------------------------------------
 
+// world inverse inertia rows, indexed [body - vbase] like the velocities (staged in LDS per world)
 __device__ __forceinline__ M3 body_iw(const DevBodies& B, int b) {
+    const int u = b - B.vbase;
     M3 m;
-    m.r[0] = ld3(B.iw0, b); m.r[1] = ld3(B.iw1, b); m.r[2] = ld3(B.iw2, b);
+    m.r[0] = ld3(B.iw0, u); m.r[1] = ld3(B.iw1, u); m.r[2] = ld3(B.iw2, u);
     return m;
 }
 
@@ -160,9 +162,8 @@ __device__ __forceinline__ void sv_prepare_one(const SvArgs& A, int i) {
             S.pn[4 * i + p] = pn;
             S.r1[4 * i + p] = f4(r1, imp);
             S.r2[4 * i + p] = f4(r2, rb);
-            S.i1n[4 * i + p] = f4(i1n, l2.w);   // w: resting flag from the warm-start transfer
-            S.i2n[4 * i + p] = f4(i2n, 0.f);
-            S.lam[4 * i + p] = make_float4(l1.w, 0.0f, 0.f, 0.f);
+            // lam: cached normal impulse, split impulse, resting flag from the warm-start transfer
+            S.lam[4 * i + p] = make_float4(l1.w, 0.0f, l2.w, 0.f);
         }
         const float fn = (float)np;
         fp1 = v3(fp1.x / fn, fp1.y / fn, fp1.z / fn);
@@ -186,12 +187,10 @@ __device__ __forceinline__ void sv_prepare_one(const SvArgs& A, int i) {
         S.r2f[i] = f4(r2f, im2);
         S.t1[i] = f4(t1, f1m > 0.0f ? 1.0f / f1m : 0.0f);
         S.t2[i] = f4(t2, f2m > 0.0f ? 1.0f / f2m : 0.0f);
-        S.r1ct1[i] = f4(r1ct1, twm > 0.0f ? 1.0f / twm : 0.0f);
-        S.r1ct2[i] = f4(r1ct2, 0.f);
-        S.r2ct1[i] = f4(r2ct1, 0.f);
-        S.r2ct2[i] = f4(r2ct2, 0.f);
+        // r x t and I (r x n) are recomputed by the sweeps (same operations, same bits) instead of
+        // being stored: every iteration re-reads the rows, so smaller rows mean less traffic
         const float4 fi = P.fimp[k];
-        S.imp[i] = make_float4(fi.x, fi.y, fi.z, 0.f);
+        S.imp[i] = make_float4(fi.x, fi.y, fi.z, twm > 0.0f ? 1.0f / twm : 0.0f);   // w: inverse twist mass
         S.oldt1[i] = P.fv1[k];
         S.oldt2[i] = P.fv2[k];
     }
@@ -218,20 +217,22 @@ __device__ __forceinline__ void sv_warmstart_one(const SvArgs& A, int i) {
         const int2 bb = S.bodies[i];
         const int b1 = bb.x, b2 = bb.y;
         const bool d1 = dyn_body(B, b1), d2 = dyn_body(B, b2);
-        V3 v1 = ld3(B.cv, b1), w1 = ld3(B.cw, b1), v2 = ld3(B.cv, b2), w2 = ld3(B.cw, b2);
+        const int u1 = b1 - B.vbase, u2 = b2 - B.vbase;
+        V3 v1 = ld3(B.cv, u1), w1 = ld3(B.cw, u1), v2 = ld3(B.cv, u2), w2 = ld3(B.cw, u2);
         const float im1 = S.r1f[i].w, im2 = S.r2f[i].w;
         const V3 ll1 = ld3(B.llock, b1), al1 = ld3(B.alock, b1), ll2 = ld3(B.llock, b2), al2 = ld3(B.alock, b2);
         const int np = S.npts[i];
+        const M3 I1 = body_iw(B, b1), I2 = body_iw(B, b2);
         bool any = false;
         float plx = 0.0f, ply = 0.0f;   // learned friction impulses summed over the new points
         for (int p = 0; p < np; p++) {
             const int q = 4 * i + p;
-            const float4 i1n4 = S.i1n[q];
-            if (i1n4.w != 0.0f || A.learned) {
+            const float4 lam4 = S.lam[q];
+            if (lam4.z != 0.0f || A.learned) {
                 float lam;
-                if (i1n4.w != 0.0f) {
+                if (lam4.z != 0.0f) {
                     any = true;
-                    lam = S.lam[q].x;
+                    lam = lam4.x;
                 } else {   // learned warm start of a point with no cached impulse
                     const float4 pr = S.pred[q];
                     lam = fmax_s(pr.x, 0.0f);
@@ -240,7 +241,7 @@ __device__ __forceinline__ void sv_warmstart_one(const SvArgs& A, int i) {
                     ply += pr.z;
                 }
                 const V3 n = xyz(S.pn[q]);
-                const V3 i1n = xyz(i1n4), i2n = ld3(S.i2n, q);
+                const V3 i1n = mvmul(I1, cross(ld3(S.r1, q), n)), i2n = mvmul(I2, cross(ld3(S.r2, q), n));
                 const V3 ip = v3(n.x * lam, n.y * lam, n.z * lam);
                 v1.x -= im1 * ip.x * ll1.x; v1.y -= im1 * ip.y * ll1.y; v1.z -= im1 * ip.z * ll1.z;
                 w1.x -= i1n.x * al1.x * lam; w1.y -= i1n.y * al1.y * lam; w1.z -= i1n.z * al1.z * lam;
@@ -254,7 +255,6 @@ __device__ __forceinline__ void sv_warmstart_one(const SvArgs& A, int i) {
         // learned friction only for a manifold with no resting point (the reference then starts from zero)
         const bool learned_friction = !any && A.learned && np > 0;
         if (any || learned_friction) {
-            const M3 I1 = body_iw(B, b1), I2 = body_iw(B, b2);
             const V3 t1 = ld3(S.t1, i), t2 = ld3(S.t2, i);
             if (any) {
                 const V3 ot1 = ld3(S.oldt1, i), ot2 = ld3(S.oldt2, i);
@@ -266,7 +266,8 @@ __device__ __forceinline__ void sv_warmstart_one(const SvArgs& A, int i) {
                 imp.y = ply;
                 imp.z = 0.0f;
             }
-            const V3 r1ct1 = ld3(S.r1ct1, i), r2ct1 = ld3(S.r2ct1, i), r1ct2 = ld3(S.r1ct2, i), r2ct2 = ld3(S.r2ct2, i);
+            const V3 r1f = ld3(S.r1f, i), r2f = ld3(S.r2f, i);
+            const V3 r1ct1 = cross(r1f, t1), r2ct1 = cross(r2f, t1), r1ct2 = cross(r1f, t2), r2ct2 = cross(r2f, t2);
             V3 a1 = v3(-r1ct1.x * imp.x, -r1ct1.y * imp.x, -r1ct1.z * imp.x);
             V3 l2 = v3(t1.x * imp.x, t1.y * imp.x, t1.z * imp.x);
             V3 a2 = v3(r2ct1.x * imp.x, r2ct1.y * imp.x, r2ct1.z * imp.x);
@@ -293,15 +294,16 @@ __device__ __forceinline__ void sv_warmstart_one(const SvArgs& A, int i) {
             imp.x = imp.y = imp.z = 0.0f;
         }
         S.imp[i] = imp;
-        if (d1) { B.cv[b1] = f4(v1, 0.f); B.cw[b1] = f4(w1, 0.f); }
-        if (d2) { B.cv[b2] = f4(v2, 0.f); B.cw[b2] = f4(w2, 0.f); }
+        if (d1) { B.cv[u1] = f4(v1, 0.f); B.cw[u1] = f4(w1, 0.f); }
+        if (d2) { B.cv[u2] = f4(v2, 0.f); B.cw[u2] = f4(w2, 0.f); }
     }
 }
 
 // Learned warm start: the 24 features of contact point p of solver row i (after sv_prepare_one and
 // the velocity integration): normal, r1, r2, relative velocity in the (t1, t2, n) frame, inverse
-// masses, diagonals of the world inverse inertia tensors, depth, friction coefficient, inverse
-// normal mass, 1 / points in the manifold.
+// masses, world inverse inertia (xx of each body), the manifold's cached normal impulses (sum over
+// its resting points, their count), the manifold counts of both bodies, depth, friction
+// coefficient, inverse normal mass, 1 / points in the manifold.
 __device__ __forceinline__ void sv_point_features(const SvArgs& A, int i, int p, float* f) {
     const DevBodies& B = A.B;
     const DevSolver& S = A.S;
@@ -309,15 +311,27 @@ __device__ __forceinline__ void sv_point_features(const SvArgs& A, int i, int p,
     const int2 bb = S.bodies[i];
     const float4 pn = S.pn[q], r14 = S.r1[q], r24 = S.r2[q];
     const V3 n = xyz(pn), r1 = xyz(r14), r2 = xyz(r24);
-    const V3 dv = rel_vel(ld3(B.cv, bb.x), ld3(B.cw, bb.x), ld3(B.cv, bb.y), ld3(B.cw, bb.y), r1, r2);
+    const int u1 = bb.x - B.vbase, u2 = bb.y - B.vbase;
+    const V3 dv = rel_vel(ld3(B.cv, u1), ld3(B.cw, u1), ld3(B.cv, u2), ld3(B.cw, u2), r1, r2);
     const V3 t1 = ld3(S.t1, i), t2 = ld3(S.t2, i);
     f[0] = n.x; f[1] = n.y; f[2] = n.z;
     f[3] = r1.x; f[4] = r1.y; f[5] = r1.z;
     f[6] = r2.x; f[7] = r2.y; f[8] = r2.z;
     f[9] = dot(dv, t1); f[10] = dot(dv, t2); f[11] = dot(dv, n);
     f[12] = S.r1f[i].w; f[13] = S.r2f[i].w;
-    f[14] = B.iw0[bb.x].x; f[15] = B.iw1[bb.x].y; f[16] = B.iw2[bb.x].z;
-    f[17] = B.iw0[bb.y].x; f[18] = B.iw1[bb.y].y; f[19] = B.iw2[bb.y].z;
+    float cached = 0.0f, resting = 0.0f;
+    for (int k = 0; k < S.npts[i]; k++) {
+        const float4 l = S.lam[4 * i + k];
+        if (l.z != 0.0f) {
+            cached += l.x;
+            resting += 1.0f;
+        }
+    }
+    const int* ao = A.adj_offs;
+    f[14] = B.iw0[bb.x].x; f[15] = cached; f[16] = resting;
+    f[17] = B.iw0[bb.y].x;
+    f[18] = dyn_body(B, bb.x) ? (float)(ao[bb.x + 1] - ao[bb.x]) : 0.0f;
+    f[19] = dyn_body(B, bb.y) ? (float)(ao[bb.y + 1] - ao[bb.y]) : 0.0f;
     f[20] = pn.w;
     f[21] = S.n[i].w;
     f[22] = r14.w;
@@ -331,11 +345,13 @@ __device__ __forceinline__ void solve_manifold(const SvArgs& A, int i, float bet
     const int2 bb = S.bodies[i];
     const int b1 = bb.x, b2 = bb.y;
     const bool d1 = dyn_body(B, b1), d2 = dyn_body(B, b2);
-    V3 v1 = ld3(B.cv, b1), w1 = ld3(B.cw, b1), v2 = ld3(B.cv, b2), w2 = ld3(B.cw, b2);
-    V3 sv1 = ld3(B.sv, b1), sw1 = ld3(B.sw, b1), sv2 = ld3(B.sv, b2), sw2 = ld3(B.sw, b2);
+    const int u1 = b1 - B.vbase, u2 = b2 - B.vbase;
+    V3 v1 = ld3(B.cv, u1), w1 = ld3(B.cw, u1), v2 = ld3(B.cv, u2), w2 = ld3(B.cw, u2);
+    V3 sv1 = ld3(B.sv, u1), sw1 = ld3(B.sw, u1), sv2 = ld3(B.sv, u2), sw2 = ld3(B.sw, u2);
     const float im1 = S.r1f[i].w, im2 = S.r2f[i].w;
     const V3 ll1 = ld3(B.llock, b1), al1 = ld3(B.alock, b1), ll2 = ld3(B.llock, b2), al2 = ld3(B.alock, b2);
     const int np = S.npts[i];
+    const M3 I1 = body_iw(B, b1), I2 = body_iw(B, b2);
     float sumPen = 0.0f;
     for (int p = 0; p < np; p++) {
         const int q = 4 * i + p;
@@ -343,7 +359,7 @@ __device__ __forceinline__ void solve_manifold(const SvArgs& A, int i, float bet
         const V3 n = xyz(pn);
         const float4 r14 = S.r1[q], r24 = S.r2[q];
         const V3 r1 = xyz(r14), r2 = xyz(r24);
-        const V3 i1n = ld3(S.i1n, q), i2n = ld3(S.i2n, q);
+        const V3 i1n = mvmul(I1, cross(r1, n)), i2n = mvmul(I2, cross(r2, n));
         const float ipm = r14.w, rb = r24.w;
         float4 lam = S.lam[q];
         const V3 dv = rel_vel(v1, w1, v2, w2, r1, r2);
@@ -374,7 +390,6 @@ __device__ __forceinline__ void solve_manifold(const SvArgs& A, int i, float bet
         sw2.x += i2n.x * al2.x * dls; sw2.y += i2n.y * al2.y * dls; sw2.z += i2n.z * al2.z * dls;
         S.lam[q] = lam;
     }
-    const M3 I1 = body_iw(B, b1), I2 = body_iw(B, b2);
     const V3 r1f = ld3(S.r1f, i), r2f = ld3(S.r2f, i);
     const float4 t14 = S.t1[i], t24 = S.t2[i], n4 = S.n[i];
     const V3 t1 = xyz(t14), t2 = xyz(t24), n = xyz(n4);
@@ -389,7 +404,7 @@ __device__ __forceinline__ void solve_manifold(const SvArgs& A, int i, float bet
         const float lt = imp.x;
         imp.x = fmax_s(-lim, fmin_s(imp.x + dl, lim));
         dl = imp.x - lt;
-        const V3 r1ct1 = ld3(S.r1ct1, i), r2ct1 = ld3(S.r2ct1, i);
+        const V3 r1ct1 = cross(r1f, t1), r2ct1 = cross(r2f, t1);
         const V3 a1 = v3(-r1ct1.x * dl, -r1ct1.y * dl, -r1ct1.z * dl);
         const V3 l2 = v3(t1.x * dl, t1.y * dl, t1.z * dl);
         const V3 a2 = v3(r2ct1.x * dl, r2ct1.y * dl, r2ct1.z * dl);
@@ -409,7 +424,7 @@ __device__ __forceinline__ void solve_manifold(const SvArgs& A, int i, float bet
         const float lt = imp.y;
         imp.y = fmax_s(-lim, fmin_s(imp.y + dl, lim));
         dl = imp.y - lt;
-        const V3 r1ct2 = ld3(S.r1ct2, i), r2ct2 = ld3(S.r2ct2, i);
+        const V3 r1ct2 = cross(r1f, t2), r2ct2 = cross(r2f, t2);
         const V3 a1 = v3(-r1ct2.x * dl, -r1ct2.y * dl, -r1ct2.z * dl);
         const V3 l2 = v3(t2.x * dl, t2.y * dl, t2.z * dl);
         const V3 a2 = v3(r2ct2.x * dl, r2ct2.y * dl, r2ct2.z * dl);
@@ -424,7 +439,7 @@ __device__ __forceinline__ void solve_manifold(const SvArgs& A, int i, float bet
     {
         const V3 dv = w2 - w1;
         const float Jv = dv.x * n.x + dv.y * n.y + dv.z * n.z;
-        float dl = -Jv * S.r1ct1[i].w;
+        float dl = -Jv * imp.w;   // inverse twist mass
         const float lim = mu * sumPen;
         const float lt = imp.z;
         imp.z = fmax_s(-lim, fmin_s(imp.z + dl, lim));
@@ -435,15 +450,14 @@ __device__ __forceinline__ void solve_manifold(const SvArgs& A, int i, float bet
         const V3 aw2 = cmul(al2, mvmul(I2, a2));
         w2.x += aw2.x; w2.y += aw2.y; w2.z += aw2.z;
     }
-    imp.w = sumPen;
     S.imp[i] = imp;
     if (d1) {
-        B.cv[b1] = f4(v1, 0.f); B.cw[b1] = f4(w1, 0.f);
-        B.sv[b1] = f4(sv1, 0.f); B.sw[b1] = f4(sw1, 0.f);
+        B.cv[u1] = f4(v1, 0.f); B.cw[u1] = f4(w1, 0.f);
+        B.sv[u1] = f4(sv1, 0.f); B.sw[u1] = f4(sw1, 0.f);
     }
     if (d2) {
-        B.cv[b2] = f4(v2, 0.f); B.cw[b2] = f4(w2, 0.f);
-        B.sv[b2] = f4(sv2, 0.f); B.sw[b2] = f4(sw2, 0.f);
+        B.cv[u2] = f4(v2, 0.f); B.cw[u2] = f4(w2, 0.f);
+        B.sv[u2] = f4(sv2, 0.f); B.sw[u2] = f4(sw2, 0.f);
     }
 }
 

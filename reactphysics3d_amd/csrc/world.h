@@ -73,6 +73,9 @@ struct HostWorld {
     std::vector<HostJoint> joints;
     rp3dg_batch* own = nullptr;   // device batch of one, created on first update()
     bool uploaded = false;
+    // host mirror of the device body state, refreshed lazily after update()
+    std::vector<float> mirror_pos, mirror_q, mirror_v, mirror_w;
+    bool mirror_dirty = false;
 };
 
 // Shape formulas (SphereShape.h, CapsuleShape.{h,cpp}, BoxShape.{h,cpp}, ConvexMeshShape.cpp)

@@ -68,7 +68,7 @@ EXPORTED = [
     "rp3dg_batch_get_iteration_stats", "rp3dg_last_error", "rp3dg_version", "rp3dg_mlp_create", "rp3dg_mlp_destroy",
     "rp3dg_mlp_set_params", "rp3dg_mlp_get_params", "rp3dg_mlp_set_normalization", "rp3dg_mlp_forward",
     "rp3dg_mlp_gradient", "rp3dg_mlp_train", "rp3dg_batch_set_warm_start", "rp3dg_batch_set_logging",
-    "rp3dg_batch_get_log",
+    "rp3dg_batch_get_log", "rp3dg_batch_autotune", "rp3dg_world_get_body_state",
 ]
 
 MLP_PARAMS = 20099
@@ -142,6 +142,8 @@ def load():
         "rp3dg_batch_set_warm_start": (C.c_int, [vp, i32, vp]),
         "rp3dg_batch_set_logging": (C.c_int, [vp, i64]),
         "rp3dg_batch_get_log": (i64, [vp, fptr, fptr, i64]),
+        "rp3dg_batch_autotune": (C.c_int, [vp, f, i32, iptr, C.POINTER(C.c_double)]),
+        "rp3dg_world_get_body_state": (C.c_int, [vp, i32, fptr, fptr, fptr, fptr]),
         "rp3dg_last_error": (C.c_char_p, []),
         "rp3dg_version": (C.c_char_p, []),
     }
@@ -317,6 +319,13 @@ class Batch:
         a, b, c = C.c_double(), C.c_double(), C.c_double()
         _check(load().rp3dg_batch_get_iteration_stats(self.h, C.byref(a), C.byref(b), C.byref(c)), "iteration_stats")
         return {"mean_iterations": a.value, "max_iterations": b.value, "max_residual": c.value}
+
+    def autotune(self, dt, steps_per_trial=3):
+        c = C.c_int32()
+        ms = C.c_double()
+        _check(load().rp3dg_batch_autotune(self.h, float(dt), int(steps_per_trial), C.byref(c), C.byref(ms)),
+               "autotune")
+        return c.value, ms.value
 
     def set_warm_start(self, mode, mlp=None):
         m = {"plain": 0, "learned": 1}[mode]

@@ -10,6 +10,8 @@
 --copy-bytes : size in bytes of the plain elementwise copy that the profiled command ran
            (`bench.py --copy-probe`): its measured FETCH/WRITE KiB divided by the known bytes gives
            the scale factor that turns the counters into HBM bytes.
+--traffic-json : also write the scaled HBM bytes per dispatch of every kernel as JSON (bench.py
+           reads profiles/traffic_<workload>.json for the roofline's "traffic").
 """
 from __future__ import annotations
 
@@ -61,6 +63,7 @@ def main(argv=None):
     ap.add_argument("--copy-bytes", type=float, default=0.0)
     ap.add_argument("--command", default="")
     ap.add_argument("--top", type=int, default=25)
+    ap.add_argument("--traffic-json", help="write {kernel: scaled HBM bytes per dispatch} to this file")
     a = ap.parse_args(argv)
     out = []
     if a.command:
@@ -101,6 +104,11 @@ def main(argv=None):
                 hbm = f"{(fk[0] * 1024 * scale_f + wk[0] * 1024 * scale_w) / 1e6:.3f}"
             out.append(f"| {k} | {max(fk[1], wk[1])} | {fk[0]:.1f} | {wk[0]:.1f} | {hbm} |")
         out.append("")
+        if a.traffic_json and scale_f and scale_w:
+            import json
+            tj = {k: int(f[k][0] * 1024 * scale_f + w[k][0] * 1024 * scale_w) for k in set(f) & set(w)}
+            with open(a.traffic_json, "w") as fh:
+                json.dump(tj, fh, indent=1, sort_keys=True)
     sys.stdout.write("\n".join(out) + "\n")
 
 

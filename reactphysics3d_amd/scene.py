@@ -16,6 +16,8 @@ from __future__ import annotations
 
 import math
 from dataclasses import dataclass, field
+from functools import lru_cache
+from itertools import combinations
 
 import numpy as np
 
@@ -266,6 +268,11 @@ def jointed_tower(height=6, seed=4, kinds=(HINGE, FIXED), friction=0.5, gravity=
     return sc
 
 
+@lru_cache(maxsize=8)
+def _triples(n):
+    return np.array(list(combinations(range(n), 3)), np.int64)
+
+
 def convex_hull(points):
     """Convex hull of points in general position (no four coplanar): every triple whose plane has all
     other points strictly on one side is a face, oriented counter-clockwise seen from outside.
@@ -273,19 +280,18 @@ def convex_hull(points):
     small point sets (tens of points) of the convex-mesh scenes."""
     P = np.asarray(points, np.float64)
     n = len(P)
-    faces = []
-    for i in range(n):
-        for j in range(i + 1, n):
-            for k in range(j + 1, n):
-                nrm = np.cross(P[j] - P[i], P[k] - P[i])
-                d = (P - P[i]) @ nrm
-                d[[i, j, k]] = 0.0
-                scale = np.abs(d).max()
-                tol = 1e-12 * max(1.0, scale)
-                if np.all(d <= tol):
-                    faces.append((i, j, k))
-                elif np.all(d >= -tol):
-                    faces.append((i, k, j))
+    tri = _triples(n)
+    nrm = np.cross(P[tri[:, 1]] - P[tri[:, 0]], P[tri[:, 2]] - P[tri[:, 0]])
+    d = np.einsum("tpc,tc->tp", P[None, :, :] - P[tri[:, 0]][:, None, :], nrm)
+    d[np.arange(len(tri))[:, None], tri] = 0.0
+    tol = 1e-12 * np.maximum(1.0, np.abs(d).max(1))
+    below = np.all(d <= tol[:, None], 1)
+    above = np.all(d >= -tol[:, None], 1)
+    sel = below | above
+    f = tri[sel].copy()
+    flip = ~below[sel]                      # all other points above the plane: reverse the winding
+    f[flip] = f[flip][:, [0, 2, 1]]
+    faces = [tuple(int(v) for v in row) for row in f]
     used = sorted({v for f in faces for v in f})
     remap = {v: t for t, v in enumerate(used)}
     return P[used].astype(np.float32), [[remap[v] for v in f] for f in faces]
