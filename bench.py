@@ -215,6 +215,7 @@ def main():
     ap.add_argument("--cpu-worlds", type=int, default=512)
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--no-profile", action="store_true", help="no HIP event timing at all")
+    ap.add_argument("--no-tune", action="store_true", help="skip the solver launch-shape autotuner")
     ap.add_argument("--train-worlds", type=int, default=1000)
     ap.add_argument("--train-samples", type=int, default=2_000_000)
     ap.add_argument("--train-max-steps", type=int, default=200)
@@ -259,6 +260,16 @@ def main():
         d.sync()
     log(f"[bench] warmup {args.warmup} steps done; counts {dev.counts()}")
     graph = batched or len(scenes) > 1
+    tuned = None
+    if not args.no_tune:
+        # autotuner: solver launch shape timed over real steps (both copies take the same steps)
+        for d in (dev, dev_l):
+            if d is not None:
+                choice, ms = d.autotune(DT, 2)
+        meaning = ("world-class workgroup = clamp(bodies bound / choice, 64, 256)" if graph
+                   else "threads per block of the colour kernels")
+        tuned = {"choice": choice, "meaning": meaning, "ms_per_step": round(ms, 3)}
+        log(f"[bench] autotuned solver launch shape: {tuned}")
 
     training = None
     if batched:
@@ -269,7 +280,7 @@ def main():
 
     def timed(d, tag):
         c0 = d.counts()
-        its = []
+        its, newfrac = [], []
         d.sync()
         t0 = time.perf_counter()
         if graph:
@@ -277,6 +288,8 @@ def main():
             for _ in range(args.steps // chunk):
                 d.step_graph(DT, chunk)
                 its.append(d.iteration_stats()["mean_iterations"])
+                cc = d.counts()
+                newfrac.append(cc["nb_new_points"] / max(1, cc["nb_contact_points"]))
             if args.steps % chunk:
                 d.step_graph(DT, args.steps % chunk)
         else:
@@ -286,7 +299,8 @@ def main():
         c1 = d.counts()
         r = {"body_steps_per_s": round(ndyn * args.steps / wall, 1), "ms_per_step": round(wall * 1e3 / args.steps, 4),
              "contacts_per_s": round((c1["total_contact_points"] - c0["total_contact_points"]) / wall, 1),
-             "mean_velocity_iterations": round(float(sum(its) / len(its)), 2) if its else None, "wall": wall,
+             "mean_velocity_iterations": round(float(sum(its) / len(its)), 2) if its else None,
+             "new_point_fraction": round(float(sum(newfrac) / len(newfrac)), 4) if newfrac else None, "wall": wall,
              "counts": c1}
         log(f"[bench] timed {tag} {args.steps} steps ({'graph' if graph else 'eager'}): {r['ms_per_step']:.3f} ms/step, "
             f"{r['body_steps_per_s']:.4g} body-steps/s, {r['contacts_per_s']:.4g} contacts/s, "
@@ -396,6 +410,7 @@ def main():
                               "throughput_ratio": round(learned["body_steps_per_s"] / plain["body_steps_per_s"], 4)}
                              if learned else None),
         "training": training,
+        "autotune": tuned,
         "fixed_iterations": fixed,
         "roofline": roofline,
         "cpu_baseline": cpu,

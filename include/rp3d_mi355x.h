@@ -147,7 +147,8 @@ rp3dg_batch* rp3dg_batch_create(rp3dg_world* const* worlds, int32_t nb_worlds);
 
 typedef struct rp3dg_batch_options {
     int32_t solver;           /* RP3DG_SOLVER_* */
-    int32_t fused_block;      /* threads per world in the fused solver (64, 128, 192 or 256; 0 = auto) */
+    int32_t fused_block;      /* threads per world in the fused solver (64, 128, 192 or 256; 0 = per world size
+                                 class: bodies bound / 2, clamped to [64, 256], autotunable) */
     int32_t pairs_per_collider; /* pair-table capacity per collider (0 = 16) */
     int32_t reserved[5];
 } rp3dg_batch_options;
@@ -174,6 +175,8 @@ typedef struct rp3dg_counts {
                                   capsule-poly, poly-poly) */
     int64_t total_contact_points; /* contact points summed over every step since creation */
     int64_t steps;             /* steps run since creation */
+    int64_t nb_new_points;     /* contact points of the last step without a cached impulse (counted when the
+                                  learned warm start or sample logging is on, else 0) */
 } rp3dg_counts;
 int rp3dg_batch_get_counts(rp3dg_batch* batch, rp3dg_counts* out);
 
@@ -211,9 +214,11 @@ int rp3dg_batch_get_joint_order(rp3dg_batch* batch, int32_t* order, int64_t capa
 int rp3dg_batch_profile(rp3dg_batch* batch, int32_t enable, const char* only_kernel);
 int rp3dg_batch_get_kernel_times(rp3dg_batch* batch, const char* kernel, double* mean_ms, int64_t* launches);
 
-/* Autotuner: times each candidate launch shape of the solver (fused layout: threads per world 64, 128,
- * 192, 256; per-colour layout: threads per block 64..512) over steps_per_trial real steps of the
- * batch (the steps advance the simulation) and keeps the fastest; returns the choice and its time. */
+/* Autotuner: times each candidate launch shape of the solver over steps_per_trial real steps of the
+ * batch (the steps advance the simulation) and keeps the fastest; returns the choice and its time.
+ * Fused layout: worlds run in size classes (bodies in (L/2, L], L = 64 .. 4096), one concurrent launch
+ * each; the candidates are the divisor d of the class workgroup clamp(L / d, 64, 256), d = 1, 2, 4, 8.
+ * Per-colour layout: threads per block of the colour kernels, 64..512. */
 int rp3dg_batch_autotune(rp3dg_batch* batch, float time_step, int32_t steps_per_trial, int32_t* chosen,
                          double* best_ms_per_step);
 

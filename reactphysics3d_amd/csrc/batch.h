@@ -8,6 +8,7 @@
 #include <vector>
 
 #include "../../include/rp3d_mi355x.h"
+#include "fused.h"
 #include "joints.h"
 #include "rp3d_internal.h"
 
@@ -42,10 +43,17 @@ struct rp3dg_batch {
     // solver layout: fused = one workgroup per world for the whole solver phase, (world, colour)-major
     // rows; otherwise one launch per colour over colour-major rows.
     bool fused = false;
-    int fused_block = 64;
+    int fused_block = 64;              // threads per world of the position kernel (and of the single-class solve)
     int color_block = 256;             // per-colour layout: threads per block of the colour kernels
+    // fused layout: worlds grouped into size classes (bodies in (L/2, L], L = 64, 128, ... 4096), largest
+    // first; each class is one launch with its own workgroup size and LDS, the classes run concurrently
+    struct WorldClass { int first, count, max_bodies, block, lds_bodies; };
+    std::vector<WorldClass> wclasses;
+    int block_div = 2;                 // class workgroup = clamp(L / block_div, 64, 256) (autotuned)
+    std::vector<hipStream_t> side_streams;
+    std::vector<hipEvent_t> fork_events;
     std::vector<std::pair<int, double>> tune_table;   // autotuner trials: (shape, ms per step)
-    int lds_bodies = 0;                // fused solver: velocities of worlds up to this size staged in LDS
+    int user_block = 0;                // threads per world requested by the caller (0 = per class, autotuned)
     int nseg = 0;                      // colour segments: kMaxColors (grid) or nworlds * kMaxColors (fused)
     int* seg_cnt = nullptr;
     int* seg_start = nullptr;
@@ -121,6 +129,8 @@ struct rp3dg_batch {
 
     ~rp3dg_batch();
     int upload(rp3dg_world* const* worlds, int n, const rp3dg_batch_options* opt);
+    void set_class_blocks();
+    int launch_world_classes(const char* name, const rp3dg::FusedArgs& F, bool lds);
     int upload_joints(rp3dg_world* const* worlds, int n);
     int enqueue_step(float dt, bool capturing);
     int finish_steps(int n);

@@ -96,6 +96,18 @@ __global__ __launch_bounds__(kScanThreads) void scan_add(int* out, int n, const 
 // Clears device memory inside the step (a kernel rather than a memset node, so the step can be
 // captured into a graph).
 __global__ void fill_int(int* p, int n, int v) { RP3DG_GRID_STRIDE(i, n) p[i] = v; }
+// centre of mass (and its constrained copy) from the body origin and orientation:
+// x = p + q * com_local (RigidBody::setTransform, RigidBody.cpp)
+__global__ void set_body_com(DevBodies B) {
+    RP3DG_GRID_STRIDE(i, B.n) {
+        const Q4 q = ldq(B.q, i);
+        const V3 x = ld3(B.p, i) + qrot(q, ld3(B.com, i));
+        B.x[i] = f4(x, 0.f);
+        B.cx[i] = f4(x, 0.f);
+        B.cq[i] = f4q(q);
+    }
+}
+
 __global__ void fill_int_n(int* p, int n, const int* nptr, int v) {
     n = min(n, *nptr);
     RP3DG_GRID_STRIDE(i, n) p[i] = v;
@@ -504,9 +516,9 @@ int rp3dg_batch::upload(rp3dg_world* const* ws, int nw, const rp3dg_batch_option
     max_world_bodies = 0;
     for (int wi = 0; wi < nw; wi++) max_world_bodies = std::max(max_world_bodies, body_base[wi + 1] - body_base[wi]);
     fused = o.solver == RP3DG_SOLVER_FUSED || (o.solver == RP3DG_SOLVER_AUTO && max_world_bodies <= 4096);
-    // solver velocities and inverse inertia of worlds up to 512 bodies live in LDS during the sweeps
-    // (112 B per body, < 64 KiB per block)
-    lds_bodies = (fused && max_world_bodies <= 512) ? ((max_world_bodies + 63) / 64) * 64 : 0;
+    // (solver velocities and inverse inertia of worlds up to 512 bodies live in LDS during the sweeps,
+    // 112 B per body: see the world classes below)
+    user_block = o.fused_block;
     if (o.fused_block) fused_block = o.fused_block;
     else fused_block = max_world_bodies <= 128 ? 64 : (max_world_bodies <= 1024 ? 128 : 256);
 
@@ -629,6 +641,31 @@ int rp3dg_batch::upload(rp3dg_world* const* ws, int nw, const rp3dg_batch_option
         HIPCHECK(up(d_wbody, body_base.data(), sizeof(int) * (nw + 1)));
         HIPCHECK(up(d_wcoll, collider_base.data(), sizeof(int) * (nw + 1)));
         HIPCHECK(hipMemcpy(d_block_world, order.data(), sizeof(int) * nw, hipMemcpyHostToDevice));
+        // size classes over the size-sorted world order (largest class first)
+        wclasses.clear();
+        if (fused) {
+            for (int i = 0; i < nw;) {
+                const int sz = body_base[order[i] + 1] - body_base[order[i]];
+                int L = 64;
+                while (L < sz) L *= 2;
+                int j = i;
+                while (j < nw && body_base[order[j] + 1] - body_base[order[j]] > L / 2) j++;
+                if (L == 64) j = nw;   // everything up to 64 bodies is one class
+                wclasses.push_back({i, j - i, L, 64, L <= 512 ? L : 0});
+                i = j;
+            }
+            set_class_blocks();
+            for (size_t c = 1; c < wclasses.size(); c++) {
+                hipStream_t s;
+                HIPCHECK(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+                side_streams.push_back(s);
+            }
+            for (size_t c = 0; c < 2 * wclasses.size(); c++) {
+                hipEvent_t e;
+                HIPCHECK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+                fork_events.push_back(e);
+            }
+        }
         HIPCHECK(hipMemsetAsync(pair_count, 0, sizeof(int) * 2, stream));
         HIPCHECK(hipMemsetAsync(total_points, 0, sizeof(unsigned long long), stream));
         HIPCHECK(hipMemsetAsync(world_iters, 0, sizeof(int) * nw, stream));
@@ -863,13 +900,13 @@ int rp3dg_batch::enqueue_step(float dt, bool capturing) {
         F.nrows = &counters->nb_new_points;
         F.feat_cap = feat_cap;
         F.row_slot = row_slot;
-        F.lds_bodies = lds_bodies;
-        const size_t lds = (size_t)lds_bodies * 7 * sizeof(float4);
+        F.lds_bodies = 0;
         F.feats = nullptr;
         F.phase = 2;
         const bool features = SA.learned || log_cap > 0;
         if (!features) {
-            LAUNCH_TS("sv_world_solve", nworlds, fused_block, lds, sv_world_solve, F);
+            r = launch_world_classes("sv_world_solve", F, true);
+            if (r) return r;
         } else {
             F.feats = feat_buf;
             F.phase = 0;
@@ -881,7 +918,8 @@ int rp3dg_batch::enqueue_step(float dt, bool capturing) {
                          nullptr, row_slot, S.pred);
             }
             F.phase = 1;
-            LAUNCH_TS("sv_world_solve", nworlds, fused_block, lds, sv_world_solve, F);
+            r = launch_world_classes("sv_world_solve", F, true);
+            if (r) return r;
             if (log_cap > 0) {
                 LAUNCH_T("sv_log_samples", grid_for(feat_cap), 256, sv_log_samples, F, log_feats, log_targets, log_count,
                          log_cap);
@@ -923,6 +961,41 @@ int rp3dg_batch::enqueue_step(float dt, bool capturing) {
             LAUNCH_T("jt_solve_position", jg, 256, jt_solve_position, B, J, jt_ranges, s);
     LAUNCH_T("dyn_update_state", bg, 256, dyn_update_state, B);
     LAUNCH_T("bp_refit", grid_for(nc), 256, bp_refit, B, C, d_hbounds, 0);
+    return RP3DG_OK;
+}
+
+// Workgroup size of every world class: clamp(L / block_div, 64, 256).
+void rp3dg_batch::set_class_blocks() {
+    for (WorldClass& c : wclasses)
+        c.block = user_block ? user_block : std::max(64, std::min(kFusedMaxBlock, c.max_bodies / block_div));
+}
+
+// One launch per world class of the fused solver, each on its own stream between a fork and a join
+// event (captured as parallel graph branches), so small and large worlds share the CUs.
+int rp3dg_batch::launch_world_classes(const char* name, const FusedArgs& F0, bool lds) {
+    timer_begin(name);
+    const int nclass = (int)wclasses.size();
+    if (nclass > 1) {
+        HIPCHECK(hipEventRecord(fork_events[0], stream));
+        for (int c = 1; c < nclass; c++) HIPCHECK(hipStreamWaitEvent(side_streams[c - 1], fork_events[0], 0));
+    }
+    for (int c = 0; c < nclass; c++) {
+        const WorldClass& wc = wclasses[c];
+        if (wc.count == 0) continue;
+        FusedArgs F = F0;
+        F.block_world = d_block_world + wc.first;
+        F.lds_bodies = lds ? wc.lds_bodies : 0;
+        const size_t shmem = (size_t)F.lds_bodies * 7 * sizeof(float4);
+        hipStream_t s = c == 0 ? stream : side_streams[c - 1];
+        sv_world_solve<<<wc.count, wc.block, shmem, s>>>(F);
+    }
+    if (nclass > 1) {
+        for (int c = 1; c < nclass; c++) {
+            HIPCHECK(hipEventRecord(fork_events[c], side_streams[c - 1]));
+            HIPCHECK(hipStreamWaitEvent(stream, fork_events[c], 0));
+        }
+    }
+    timer_end();
     return RP3DG_OK;
 }
 
@@ -1008,6 +1081,8 @@ int rp3dg_batch::step_graph(float dt, int n) {
 
 rp3dg_batch::~rp3dg_batch() {
     drop_graphs();
+    for (hipStream_t s : side_streams) hipStreamDestroy(s);
+    for (hipEvent_t e : fork_events) hipEventDestroy(e);
     if (log_feats) hipFree(log_feats);
     if (log_targets) hipFree(log_targets);
     for (hipEvent_t ev : event_pool) hipEventDestroy(ev);
@@ -1115,6 +1190,7 @@ int rp3dg_batch_get_counts(rp3dg_batch* b, rp3dg_counts* o) {
     HIPCHECK(hipStreamSynchronize(b->stream));
     o->total_contact_points = (int64_t)tp;
     o->steps = b->steps_done;
+    o->nb_new_points = b->last.nb_new_points;
     return RP3DG_OK;
 }
 
@@ -1141,10 +1217,34 @@ int rp3dg_batch_get_bodies(rp3dg_batch* b, float* pos, float* ori, float* lin, f
     return RP3DG_OK;
 }
 
+// RigidBody::setTransform / setLinearVelocity / setAngularVelocity for every body of the batch
+// (src/body/RigidBody.cpp): the new origin/orientation, the centre of mass that follows from them,
+// and the collider transforms and fat AABBs (refit, as the next step would).  NULL leaves a part as is.
 int rp3dg_batch_set_bodies(rp3dg_batch* b, const float* pos, const float* ori, const float* lin, const float* ang) {
-    (void)b; (void)pos; (void)ori; (void)lin; (void)ang;
-    set_error("rp3dg_batch_set_bodies: not implemented");
-    return RP3DG_EUNSUPPORTED;
+    if (!b) return RP3DG_EINVAL;
+    HIPCHECK(hipStreamSynchronize(b->stream));
+    const int n = b->nb;
+    std::vector<float4> t(n);
+    auto put = [&](float4* d, const float* in, int comps) -> int {
+        if (!in) return RP3DG_OK;
+        HIPCHECK(hipMemcpy(t.data(), d, sizeof(float4) * n, hipMemcpyDeviceToHost));
+        for (int i = 0; i < n; i++) {
+            t[i].x = in[comps * i]; t[i].y = in[comps * i + 1]; t[i].z = in[comps * i + 2];
+            if (comps == 4) t[i].w = in[comps * i + 3];
+        }
+        HIPCHECK(hipMemcpy(d, t.data(), sizeof(float4) * n, hipMemcpyHostToDevice));
+        return RP3DG_OK;
+    };
+    int r;
+    if ((r = put(b->B.p, pos, 3)) || (r = put(b->B.q, ori, 4)) || (r = put(b->B.v, lin, 3)) || (r = put(b->B.w, ang, 3)))
+        return r;
+    if (pos || ori) {
+        set_body_com<<<grid_for(n), 256, 0, b->stream>>>(b->B);
+        bp_refit<<<grid_for(b->nc), 256, 0, b->stream>>>(b->B, b->C, b->d_hbounds, 0);
+        HIPCHECK(hipGetLastError());
+        HIPCHECK(hipStreamSynchronize(b->stream));
+    }
+    return RP3DG_OK;
 }
 
 int rp3dg_batch_get_pairs(rp3dg_batch* b, int32_t* c1, int32_t* c2, int64_t cap) {
@@ -1304,18 +1404,19 @@ int64_t rp3dg_batch_get_log(rp3dg_batch* b, float* feats, float* targets, int64_
 }
 
 // Autotuner: times every candidate launch shape of the solver over a few real steps of this batch
-// (the steps are not undone) and keeps the fastest.  Fused layout: threads per world (64..256);
-// per-colour layout: threads per block of the colour kernels.
+// (the steps are not undone) and keeps the fastest.  Fused layout: workgroup size of each world
+// class = clamp(bodies bound / d, 64, 256) for d = 1, 2, 4, 8 (the choice returned is d); per-colour
+// layout: threads per block of the colour kernels.
 int rp3dg_batch_autotune(rp3dg_batch* b, float dt, int32_t steps_per_trial, int32_t* chosen, double* best_ms) {
     if (!b || !(dt > 0.f) || steps_per_trial < 1 || steps_per_trial > 1000) return RP3DG_EINVAL;
-    std::vector<int> cand = b->fused ? std::vector<int>{64, 128, 192, 256} : std::vector<int>{64, 128, 256, 512};
+    std::vector<int> cand = b->fused ? std::vector<int>{1, 2, 4, 8} : std::vector<int>{64, 128, 256, 512};
     const bool prof = b->profiling;
     b->profiling = false;
     b->drop_graphs();
     int best = -1;
     double best_t = 1e300;
     for (int c : cand) {
-        if (b->fused) b->fused_block = c;
+        if (b->fused) { b->block_div = c; b->set_class_blocks(); }
         else b->color_block = c;
         int r = b->step(dt);   // one untimed step with the new shape
         if (r) { b->profiling = prof; return r; }
@@ -1331,7 +1432,7 @@ int rp3dg_batch_autotune(rp3dg_batch* b, float dt, int32_t steps_per_trial, int3
         b->tune_table.push_back({c, ms});
         if (ms < best_t) { best_t = ms; best = c; }
     }
-    if (b->fused) b->fused_block = best;
+    if (b->fused) { b->block_div = best; b->set_class_blocks(); }
     else b->color_block = best;
     b->profiling = prof;
     if (chosen) *chosen = best;

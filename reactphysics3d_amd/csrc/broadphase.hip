@@ -339,7 +339,7 @@ __device__ __forceinline__ bool no_collision_pair(const uint64_t* nc, int nnc, u
     return false;
 }
 
-constexpr int kTravStack = 64;   // traversal stack depth (LDS, per thread)
+constexpr int kTravStack = 64;   // traversal stack depth (per thread)
 
 // Pair filter of CollisionDetectionSystem::updateOverlappingPairs (:284-327).
 __device__ __forceinline__ bool pair_allowed(const DevColliders& C, int a, int b, const uint64_t* nc, int nnc) {
@@ -360,23 +360,18 @@ __device__ __forceinline__ bool pair_allowed(const DevColliders& C, int a, int b
 //   traverse: a floor's AABB overlaps most of the tree and would serialise one thread over it).
 // Pass 0 counts, pass 1 writes each emitter's partner list sorted ascending into its segment, so the
 // pair table is sorted by key.
-// Pass 0 traverses once, counts every collider's partners and keeps the first kTravSlab of them,
-// sorted, in a per-collider slab.  Pass 1 (after the scan of the counts) copies the slabs into the
-// emitters' segments of the pair-key table and traverses again only for colliders with more
-// partners than the slab holds.  The traversal stack lives in LDS (one column per thread).
+// Pass 0 traverses once (colliders in Morton order, so neighbouring lanes walk similar paths),
+// counts every collider's partners and keeps the first kTravSlab of them, sorted, in a
+// per-collider slab.  Pass 1 (after the scan of the counts) runs over colliders in index order,
+// copies the slabs into the emitters' segments of the pair-key table (coalesced), and traverses
+// again only for colliders with more partners than the slab holds.
 __global__ __launch_bounds__(kTravBlock) void bp_traverse(DevColliders C, const int* sorted, int n, const int2* child,
                                                           const float4* nmin, const float4* nmax, const int* root,
                                                           const uint64_t* nc, int nnc, int* count, const int* offset,
                                                           uint64_t* out_keys, uint64_t* slab, int capacity, int pass,
                                                           int* status) {
-    __shared__ int lstack[kTravStack][kTravBlock];
-    const int t = threadIdx.x;
-    if (blockDim.x > kTravBlock) {   // the LDS stack has one column per thread of a kTravBlock block
-        if (t == 0) atomicOr(status, STATUS_STACK_OVERFLOW);
-        return;
-    }
     RP3DG_GRID_STRIDE(s, n) {
-        const int c = sorted[s];
+        const int c = pass ? s : sorted[s];
         const int base = pass ? offset[c] : 0;
         if (C.is_static[c]) {
             const int b = C.ss_offs[c], e = C.ss_offs[c + 1];
@@ -398,11 +393,12 @@ __global__ __launch_bounds__(kTravBlock) void bp_traverse(DevColliders C, const 
             }
         }
         const float4 amn = C.amin[c], amx = C.amax[c];
+        int stack[kTravStack];
         int sp = 0;
-        lstack[sp++][t] = root[C.world[c]];
+        stack[sp++] = root[C.world[c]];
         int cnt = 0;
         while (sp > 0) {
-            const int node = lstack[--sp][t];
+            const int node = stack[--sp];
             if (!aabb_overlap(amn, amx, nmin[node], nmax[node])) continue;
             if (node >= n - 1) {
                 const int o = sorted[node - (n - 1)];
@@ -434,8 +430,8 @@ __global__ __launch_bounds__(kTravBlock) void bp_traverse(DevColliders C, const 
             } else {
                 if (sp + 2 > kTravStack) { atomicOr(status, STATUS_STACK_OVERFLOW); break; }
                 const int2 ch = child[node];
-                lstack[sp++][t] = ch.x;
-                lstack[sp++][t] = ch.y;
+                stack[sp++] = ch.x;
+                stack[sp++] = ch.y;
             }
         }
         if (!pass) count[c] = cnt;

@@ -218,7 +218,7 @@ __device__ inline int clip_polygon(const V3* in, int nin, V3 pp, V3 pn, V3* out)
 }
 
 // :893-1036
-__device__ bool sat_face_contacts(bool refIs1, const Poly& P1, const Poly& P2, const Tf& t12, const Tf& t21, int minFace,
+__device__ __noinline__ bool sat_face_contacts(bool refIs1, const Poly& P1, const Poly& P2, const Tf& t12, const Tf& t21, int minFace,
                                   const Tf& T1, const Tf& T2, Raw& raw) {
     const Tf& refToInc = refIs1 ? t12 : t21;
     const Tf& incToRef = refIs1 ? t21 : t12;
@@ -767,7 +767,7 @@ __global__ void np_capsule_capsule(NpArgs A, const int* list, const int* count) 
     }
 }
 
-__global__ void np_poly_poly(NpArgs A, const int* list, const int* count) {
+__global__ __launch_bounds__(128) void np_poly_poly(NpArgs A, const int* list, const int* count) {
     RP3DG_GRID_STRIDE(t, *count) {
         const int k = list[t];
         const int2 c = A.P.col[k];

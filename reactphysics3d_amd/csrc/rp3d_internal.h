@@ -55,7 +55,7 @@ __host__ __device__ inline uint32_t lf_face(uint32_t w) { return (w >> 8) & 0xff
 __host__ __device__ inline uint32_t lf_edge1(uint32_t w) { return (w >> 16) & 0xffu; }
 __host__ __device__ inline uint32_t lf_edge2(uint32_t w) { return (w >> 24) & 0xffu; }
 
-constexpr int kTravBlock = 128;       // threads per block of the broadphase traversal (LDS stack columns)
+constexpr int kTravBlock = 128;       // threads per block of the broadphase traversal
 constexpr int kTravSlab = 8;          // partners kept per collider by the first traversal pass
 constexpr int kMaxRawContacts = 16;   // NB_MAX_CONTACT_POINTS_IN_NARROWPHASE_INFO (configuration.h:134)
 constexpr int kMaxManifoldPoints = 4; // MAX_CONTACT_POINTS_IN_MANIFOLD

@@ -33,7 +33,7 @@ int main(int argc, char** argv) {
         b->getMaterial().setMassDensity(1);
         box->updateMassPropertiesFromColliders();
 
-        RigidBody* ball = world->createRigidBody(Transform(Vector3(2, 0, 3), Quaternion::identity()));
+        RigidBody* ball = world->createRigidBody(Transform(Vector3(decimal(3.5), 0, 3), Quaternion::identity()));
         Collider* s = ball->addCollider(common.createSphereShape(decimal(0.4)), Transform::identity());
         s->getMaterial().setFrictionCoefficient(decimal(0.5));
         s->getMaterial().setBounciness(0);

@@ -35,7 +35,8 @@ class WorldSettings(C.Structure):
 class Counts(C.Structure):
     _fields_ = [(n, C.c_int64) for n in ("nb_bodies", "nb_colliders", "nb_joints", "nb_worlds", "nb_pairs",
                                          "nb_manifolds", "nb_contact_points", "nb_colors", "status")] + \
-               [("nb_tests", C.c_int64 * 7), ("total_contact_points", C.c_int64), ("steps", C.c_int64)]
+               [("nb_tests", C.c_int64 * 7), ("total_contact_points", C.c_int64), ("steps", C.c_int64),
+                ("nb_new_points", C.c_int64)]
 
 
 class BatchOptions(C.Structure):
@@ -282,6 +283,12 @@ class Batch:
         w = np.zeros((n, 3), np.float32)
         _check(load().rp3dg_batch_get_bodies(self.h, _fp(p), _fp(q), _fp(v), _fp(w)), "get_bodies")
         return p, q, v, w
+
+    def set_bodies(self, pos=None, quat=None, lin=None, ang=None):
+        """RigidBody::setTransform / setLinearVelocity / setAngularVelocity for every body (None = keep)."""
+        arrs = [None if a is None else np.ascontiguousarray(a, np.float32) for a in (pos, quat, lin, ang)]
+        ptrs = [None if a is None else _fp(a) for a in arrs]
+        _check(load().rp3dg_batch_set_bodies(self.h, *ptrs), "set_bodies")
 
     def pairs(self):
         cap = max(1, self.counts()["nb_pairs"])
