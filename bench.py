@@ -50,6 +50,10 @@ def bytes_per_launch(name, c, ndyn, nj):
     nb, ncl, npairs = c["nb_bodies"], c["nb_colliders"], c["nb_pairs"]
     np_types = {"np_sphere_sphere": 1, "np_sphere_capsule": 2, "np_capsule_capsule": 3, "np_sphere_poly": 4,
                 "np_capsule_poly": 5, "np_poly_poly": 6}
+    if name in ("np_box_box", "np_poly_poly"):
+        # box-box pairs run in np_box_box, pairs with a convex mesh in np_poly_poly
+        tests = c["nb_box_tests"] if name == "np_box_box" else nt[6] - c["nb_box_tests"]
+        return 176 * tests + 40 * p * (tests / max(1, sum(nt)))
     if name in np_types:
         tests = nt[np_types[name]]
         return 176 * tests + 40 * p * (tests / max(1, sum(nt)))
@@ -83,13 +87,15 @@ BYTE_MODEL = {
     "sv_world_solve": "48 B/contact point + 88 B/manifold + 140 B/dynamic body + 200 B/joint per launch (one step)",
     "world_position": "192 B/dynamic body + 48 B/collider per launch (one step)",
     "np_poly_poly": "176 B/test + 40 B/contact point",
+    "np_box_box": "176 B/test + 40 B/contact point",
 }
 
 KERNELS = ["bp_bounds", "bp_morton", "bp_sort", "bp_karras", "bp_fit", "bp_world_roots", "bp_traverse", "bp_copy_pairs",
            "bp_merge",
            "np_classify", "np_sphere_sphere", "np_sphere_capsule", "np_capsule_capsule", "np_sphere_poly",
-           "np_capsule_poly", "np_poly_poly", "sv_flag", "sv_compact", "sv_adj_count", "sv_adj_fill",
-           "sv_color_round", "sv_color_finish", "sv_color_keys", "sv_color_sort", "sv_world_solve", "world_position",
+           "np_capsule_poly", "np_box_box", "np_poly_poly", "sv_flag", "sv_compact", "sv_adj_count", "sv_adj_fill",
+           "sv_color_round", "sv_color_finish", "sv_color_keys", "sv_color_sort", "sv_world_ranges", "sv_group_color",
+           "sv_world_prepare", "mlp_infer", "sv_world_solve", "sv_log_samples", "world_position",
            "dyn_integrate_velocities", "sv_prepare", "sv_warmstart", "sv_solve_color", "sv_store", "jt_init",
            "jt_warmstart", "jt_solve_velocity", "jt_solve_position", "dyn_integrate_positions", "dyn_update_state",
            "bp_refit"]
@@ -216,6 +222,9 @@ def main():
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--no-profile", action="store_true", help="no HIP event timing at all")
     ap.add_argument("--no-tune", action="store_true", help="skip the solver launch-shape autotuner")
+    ap.add_argument("--group-bodies", type=int, default=0,
+                    help="fused solver: pack consecutive worlds into workgroups of up to this many bodies "
+                         "(0 = library default, 1 = one world per workgroup)")
     ap.add_argument("--train-worlds", type=int, default=1000)
     ap.add_argument("--train-samples", type=int, default=2_000_000)
     ap.add_argument("--train-max-steps", type=int, default=200)
@@ -249,8 +258,8 @@ def main():
         f"{ndyn} dynamic, {nj} joints, built in {time.time() - t0:.1f}s")
     t0 = time.time()
     batched = args.workload == "batched"
-    dev = Batch(scenes)
-    dev_l = Batch(scenes) if batched else None   # same worlds, learned warm start
+    dev = Batch(scenes, group_bodies=args.group_bodies)
+    dev_l = Batch(scenes, group_bodies=args.group_bodies) if batched else None   # same worlds, learned warm start
     log(f"[bench] uploaded in {time.time() - t0:.1f}s")
     for d in (dev, dev_l):
         if d is None:
@@ -260,23 +269,27 @@ def main():
         d.sync()
     log(f"[bench] warmup {args.warmup} steps done; counts {dev.counts()}")
     graph = batched or len(scenes) > 1
-    tuned = None
-    if not args.no_tune:
-        # autotuner: solver launch shape timed over real steps (both copies take the same steps)
-        for d in (dev, dev_l):
-            if d is not None:
-                choice, ms = d.autotune(DT, 2)
-        meaning = ("world-class workgroup = clamp(bodies bound / choice, 64, 256)" if graph
-                   else "threads per block of the colour kernels")
-        tuned = {"choice": choice, "meaning": meaning, "ms_per_step": round(ms, 3)}
-        log(f"[bench] autotuned solver launch shape: {tuned}")
-
     training = None
     if batched:
         mlp, training = train_warm_start(args)
         for d in (dev, dev_l):
             d.set_residual_mode(RESIDUAL, RESIDUAL_CAP)
         dev_l.set_warm_start("learned", mlp)
+
+    tuned = None
+    if not args.no_tune:
+        # autotuner: solver launch shape timed over real steps in the timed configuration (residual
+        # mode, warm start); each copy keeps its own choice
+        tuned = {}
+        for tag, d in (("plain", dev), ("learned", dev_l)):
+            if d is None:
+                continue
+            choice, ms = d.autotune(DT, 2)
+            tuned[tag] = {"choice": choice, "ms_per_step": round(ms, 3),
+                          "trials": [[c, round(t, 3)] for c, t in d.tune_table()]}
+        tuned["meaning"] = ("world-class workgroup = clamp(bodies bound / choice, 64, 256)" if graph
+                            else "threads per block of the colour kernels")
+        log(f"[bench] autotuned solver launch shape: {tuned}")
 
     def timed(d, tag):
         c0 = d.counts()
@@ -401,6 +414,7 @@ def main():
                    "position_iterations": int(s0["pos_iters"]),
                    "warm_start": "learned (MLP, BF16 MFMA)" if batched else "plain (cached impulses)",
                    "step": "hipGraph replay" if graph else "eager",
+                   "group_bodies": args.group_bodies or 256,
                    "pairs": c1["nb_pairs"], "manifolds": c1["nb_manifolds"],
                    "contact_points": c1["nb_contact_points"], "colors": c1["nb_colors"]},
         "learned": ({k: v for k, v in learned.items() if k not in ("wall", "counts")} if learned else None),

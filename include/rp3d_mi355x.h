@@ -143,14 +143,17 @@ rp3dg_batch* rp3dg_batch_create(rp3dg_world* const* worlds, int32_t nb_worlds);
 /* Solver layout of a batch. */
 #define RP3DG_SOLVER_AUTO 0   /* FUSED when every world has at most 4096 bodies, else PER_COLOR */
 #define RP3DG_SOLVER_PER_COLOR 1  /* one launch per colour over all worlds (large worlds) */
-#define RP3DG_SOLVER_FUSED 2  /* one workgroup per world runs the whole solver phase (many small worlds) */
+#define RP3DG_SOLVER_FUSED 2  /* one workgroup per group of small worlds runs the whole solver phase (worlds up to 4096 bodies) */
 
 typedef struct rp3dg_batch_options {
     int32_t solver;           /* RP3DG_SOLVER_* */
     int32_t fused_block;      /* threads per world in the fused solver (64, 128, 192 or 256; 0 = per world size
                                  class: bodies bound / 2, clamped to [64, 256], autotunable) */
     int32_t pairs_per_collider; /* pair-table capacity per collider (0 = 16) */
-    int32_t reserved[5];
+    int32_t group_bodies;     /* fused solver: consecutive worlds are packed into one workgroup up to this many
+                                 bodies together (at most 64 worlds, 512 bodies; 0 = 256, 1 = one world per
+                                 workgroup); results do not depend on it */
+    int32_t reserved[4];
 } rp3dg_batch_options;
 /* As rp3dg_batch_create, with options (NULL = defaults). */
 rp3dg_batch* rp3dg_batch_create_ex(rp3dg_world* const* worlds, int32_t nb_worlds, const rp3dg_batch_options* options);
@@ -177,6 +180,7 @@ typedef struct rp3dg_counts {
     int64_t steps;             /* steps run since creation */
     int64_t nb_new_points;     /* contact points of the last step without a cached impulse (counted when the
                                   learned warm start or sample logging is on, else 0) */
+    int64_t nb_box_tests;      /* box-box subset of nb_tests[6] */
 } rp3dg_counts;
 int rp3dg_batch_get_counts(rp3dg_batch* batch, rp3dg_counts* out);
 
@@ -217,10 +221,13 @@ int rp3dg_batch_get_kernel_times(rp3dg_batch* batch, const char* kernel, double*
 /* Autotuner: times each candidate launch shape of the solver over steps_per_trial real steps of the
  * batch (the steps advance the simulation) and keeps the fastest; returns the choice and its time.
  * Fused layout: worlds run in size classes (bodies in (L/2, L], L = 64 .. 4096), one concurrent launch
- * each; the candidates are the divisor d of the class workgroup clamp(L / d, 64, 256), d = 1, 2, 4, 8.
+ * each; the candidates are the divisor d of the class workgroup clamp(L / d, 64, 256), d = 1, 2, 4, 8, 16.
  * Per-colour layout: threads per block of the colour kernels, 64..512. */
 int rp3dg_batch_autotune(rp3dg_batch* batch, float time_step, int32_t steps_per_trial, int32_t* chosen,
                          double* best_ms_per_step);
+/* The trials of the last rp3dg_batch_autotune call: up to capacity (choice, ms per step) pairs;
+ * returns the number of trials. */
+int32_t rp3dg_batch_get_tune_table(rp3dg_batch* batch, int32_t* choices, double* ms_per_step, int32_t capacity);
 
 /* Solver convergence mode (no counterpart in the reference, which runs a fixed iteration count,
  * PhysicsWorld.cpp:334): every world iterates until the velocity residual

@@ -58,6 +58,14 @@ struct rp3dg_batch {
     int* seg_cnt = nullptr;
     int* seg_start = nullptr;
     int *d_wbody = nullptr, *d_wcoll = nullptr, *d_block_world = nullptr;
+    // world groups of the fused solver: consecutive worlds sharing one workgroup (d_block_world holds
+    // the group of each workgroup, largest groups first)
+    int ngroups = 0, group_bodies = 256;
+    std::vector<int> group_world;        // [ngroups + 1] first world of each group
+    std::vector<int> world_group;        // [nworlds]
+    int *d_gworld = nullptr, *d_world_group = nullptr;
+    int* d_wmstart = nullptr;            // [nworlds + 1] first manifold of each world (fused colouring)
+    int max_group_bodies = 0;
     float4 *res_pv = nullptr, *res_pw = nullptr;
     int* world_iters = nullptr;
     float* world_res = nullptr;

@@ -260,7 +260,7 @@ static void moved_ranks(int n, std::vector<int>& rank) {
 // (BallAndSocketJoint.cpp:38-53, FixedJoint.cpp:36-56, HingeJoint.cpp:36-80); the rows are sorted by
 // (kind in solve order, colour, creation index) with a greedy colouring over all bodies.
 int rp3dg_batch::upload_joints(rp3dg_world* const* ws, int nw) {
-    struct Row { int kind, color, order, b1, b2, world, slot; float4 la1, la2, ax1, ax2, qinit, lim; int flags; };
+    struct Row { int kind, color, order, b1, b2, group, slot; float4 la1, la2, ax1, ax2, qinit, lim; int flags; };
     std::vector<Row> rows;
     int jbase = 0;
     for (int wi = 0; wi < nw; wi++) {
@@ -273,7 +273,7 @@ int rp3dg_batch::upload_joints(rp3dg_world* const* ws, int nw) {
             else if (hj.type == RP3DG_JOINT_HINGE) r.kind = JT_HINGE;
             else { set_error("slider joints are not on the device path"); return RP3DG_EUNSUPPORTED; }
             r.order = jbase + (int)ji;
-            r.world = wi;
+            r.group = world_group[wi];   // fused layout: rows are ordered by world group
             r.b1 = body_base[wi] + hj.body1;
             r.b2 = body_base[wi] + hj.body2;
             const Tf& t1 = W.bodies[hj.body1].transform;
@@ -327,21 +327,21 @@ int rp3dg_batch::upload_joints(rp3dg_world* const* ws, int nw) {
     for (Row& r : rows) r.slot = slot_of[r.kind * 64 + r.color];
     const bool wmajor = fused;
     std::stable_sort(rows.begin(), rows.end(), [wmajor](const Row& a, const Row& b) {
-        if (wmajor && a.world != b.world) return a.world < b.world;
+        if (wmajor && a.group != b.group) return a.group < b.group;
         if (a.slot != b.slot) return a.slot < b.slot;
         return a.order < b.order;
     });
     const int n = (int)rows.size();
     std::vector<int> ranges;   // per-colour kernels: [2*slot] = [first, end) over all worlds
-    std::vector<int2> wr((size_t)nw * std::max(jt_slots, 1), make_int2(0, 0));   // fused: per world and slot
+    std::vector<int2> wr((size_t)ngroups * std::max(jt_slots, 1), make_int2(0, 0));   // fused: per group and slot
     if (!wmajor) {
         for (int i = 0; i < n; i++) {
             if (i == 0 || rows[i].slot != rows[i - 1].slot) ranges.push_back(i);
             if (i == n - 1 || rows[i + 1].slot != rows[i].slot) ranges.push_back(i + 1);
         }
     } else {
-        std::vector<int> cnt((size_t)nw * std::max(jt_slots, 1), 0);
-        for (const Row& r : rows) cnt[(size_t)r.world * jt_slots + r.slot]++;
+        std::vector<int> cnt((size_t)ngroups * std::max(jt_slots, 1), 0);
+        for (const Row& r : rows) cnt[(size_t)r.group * jt_slots + r.slot]++;
         int run = 0;
         for (size_t k = 0; k < cnt.size(); k++) {
             wr[k] = make_int2(run, run + cnt[k]);
@@ -396,6 +396,7 @@ int rp3dg_batch::upload(rp3dg_world* const* ws, int nw, const rp3dg_batch_option
         return RP3DG_EINVAL;
     }
     if (o.pairs_per_collider < 0) { set_error("pairs_per_collider must be >= 0"); return RP3DG_EINVAL; }
+    if (o.group_bodies < 0) { set_error("group_bodies must be >= 0"); return RP3DG_EINVAL; }
     // host SoA
     std::vector<float4> x, q, p, v, w, force, torque, ii, com, llock, alock;
     std::vector<int> flags, bworld;
@@ -438,6 +439,9 @@ int rp3dg_batch::upload(rp3dg_world* const* ws, int nw, const rp3dg_batch_option
             if (b.type == RP3DG_KINEMATIC) f |= BF_KINEMATIC;
             if (b.type != RP3DG_STATIC) f |= BF_ENABLED;
             if (b.gravity) f |= BF_GRAVITY;
+            if (b.llock.x == 1.f && b.llock.y == 1.f && b.llock.z == 1.f && b.alock.x == 1.f && b.alock.y == 1.f &&
+                b.alock.z == 1.f)
+                f |= BF_UNIT_LOCKS;
             flags.push_back(f);
             bworld.push_back(wi);
         }
@@ -516,8 +520,32 @@ int rp3dg_batch::upload(rp3dg_world* const* ws, int nw, const rp3dg_batch_option
     max_world_bodies = 0;
     for (int wi = 0; wi < nw; wi++) max_world_bodies = std::max(max_world_bodies, body_base[wi + 1] - body_base[wi]);
     fused = o.solver == RP3DG_SOLVER_FUSED || (o.solver == RP3DG_SOLVER_AUTO && max_world_bodies <= 4096);
+    if (fused && max_world_bodies > 4096) {
+        set_error("the fused solver layout takes worlds of up to 4096 bodies (use the per-colour layout)");
+        return RP3DG_EINVAL;
+    }
     // (solver velocities and inverse inertia of worlds up to 512 bodies live in LDS during the sweeps,
     // 112 B per body: see the world classes below)
+    // world groups: consecutive worlds share a workgroup up to group_bodies bodies (64 worlds), so the
+    // colour phases of small worlds fill the wavefronts; a larger world is a group of its own
+    group_bodies = std::min(o.group_bodies ? o.group_bodies : 256, 512);
+    group_world.clear();
+    world_group.assign(nw, 0);
+    for (int wi = 0; wi < nw;) {
+        int wj = wi + 1, sum = body_base[wi + 1] - body_base[wi];
+        while (fused && wj < nw && wj - wi < 64 && sum + body_base[wj + 1] - body_base[wj] <= group_bodies) {
+            sum += body_base[wj + 1] - body_base[wj];
+            wj++;
+        }
+        for (int k = wi; k < wj; k++) world_group[k] = (int)group_world.size();
+        group_world.push_back(wi);
+        wi = wj;
+    }
+    group_world.push_back(nw);
+    ngroups = (int)group_world.size() - 1;
+    max_group_bodies = 0;
+    for (int g = 0; g < ngroups; g++)
+        max_group_bodies = std::max(max_group_bodies, body_base[group_world[g + 1]] - body_base[group_world[g]]);
     user_block = o.fused_block;
     if (o.fused_block) fused_block = o.fused_block;
     else fused_block = max_world_bodies <= 128 ? 64 : (max_world_bodies <= 1024 ? 128 : 256);
@@ -618,39 +646,44 @@ int rp3dg_batch::upload(rp3dg_world* const* ws, int nw, const rp3dg_batch_option
     color = dalloc<int>(this, pair_cap);
     ckeys = dalloc<uint64_t>(this, pair_cap); cvals = dalloc<int>(this, pair_cap);
     cktmp = dalloc<uint64_t>(this, pair_cap); cvtmp = dalloc<int>(this, pair_cap);
-    nseg = fused ? nw * kMaxColors : kMaxColors;
+    nseg = fused ? ngroups * kMaxColors : kMaxColors;
     seg_cnt = dalloc<int>(this, nseg + 1);
     seg_start = dalloc<int>(this, nseg + 1);
     pair_count = dalloc<int>(this, 2);
     total_points = dalloc<unsigned long long>(this, 1);
     d_wbody = dalloc<int>(this, nw + 1);
     d_wcoll = dalloc<int>(this, nw + 1);
-    d_block_world = dalloc<int>(this, nw);
+    d_block_world = dalloc<int>(this, ngroups);
+    d_gworld = dalloc<int>(this, ngroups + 1);
+    d_world_group = dalloc<int>(this, nw);
+    d_wmstart = dalloc<int>(this, nw + 1);
     res_pv = dalloc<float4>(this, nb);
     res_pw = dalloc<float4>(this, nb);
     world_iters = dalloc<int>(this, nw);
     world_res = dalloc<float>(this, nw);
     if (!world_res) { set_error("device allocation failed (world tables)"); return RP3DG_EHIP; }
     {
-        std::vector<int> order(nw);
-        for (int wi = 0; wi < nw; wi++) order[wi] = wi;
-        // largest worlds first, so the longest workgroups start first
-        std::stable_sort(order.begin(), order.end(), [&](int a, int c) {
-            return body_base[a + 1] - body_base[a] > body_base[c + 1] - body_base[c];
-        });
+        const int ng = ngroups;
+        auto gsize = [&](int g) { return body_base[group_world[g + 1]] - body_base[group_world[g]]; };
+        std::vector<int> order(ng);
+        for (int g = 0; g < ng; g++) order[g] = g;
+        // largest groups first, so the longest workgroups start first
+        std::stable_sort(order.begin(), order.end(), [&](int a, int c) { return gsize(a) > gsize(c); });
         HIPCHECK(up(d_wbody, body_base.data(), sizeof(int) * (nw + 1)));
         HIPCHECK(up(d_wcoll, collider_base.data(), sizeof(int) * (nw + 1)));
-        HIPCHECK(hipMemcpy(d_block_world, order.data(), sizeof(int) * nw, hipMemcpyHostToDevice));
-        // size classes over the size-sorted world order (largest class first)
+        HIPCHECK(hipMemcpy(d_block_world, order.data(), sizeof(int) * ng, hipMemcpyHostToDevice));
+        HIPCHECK(hipMemcpy(d_gworld, group_world.data(), sizeof(int) * (ng + 1), hipMemcpyHostToDevice));
+        HIPCHECK(hipMemcpy(d_world_group, world_group.data(), sizeof(int) * nw, hipMemcpyHostToDevice));
+        // size classes over the size-sorted group order (largest class first)
         wclasses.clear();
         if (fused) {
-            for (int i = 0; i < nw;) {
-                const int sz = body_base[order[i] + 1] - body_base[order[i]];
+            for (int i = 0; i < ng;) {
+                const int sz = gsize(order[i]);
                 int L = 64;
                 while (L < sz) L *= 2;
                 int j = i;
-                while (j < nw && body_base[order[j] + 1] - body_base[order[j]] > L / 2) j++;
-                if (L == 64) j = nw;   // everything up to 64 bodies is one class
+                while (j < ng && gsize(order[j]) > L / 2) j++;
+                if (L == 64) j = ng;   // everything up to 64 bodies is one class
                 wclasses.push_back({i, j - i, L, 64, L <= 512 ? L : 0});
                 i = j;
             }
@@ -813,19 +846,23 @@ int rp3dg_batch::enqueue_step(float dt, bool capturing) {
     A.hull_face_edge = d_hfedge; A.hull_face_verts = d_hfverts; A.hull_face_vidx = d_hfvidx;
     A.persistent_threshold = settings.persistent_threshold;
     A.cos_similar = settings.cos_similar;
-    LAUNCH_T("np_classify", pg, 256, np_classify, A, np_lists, pair_cap, counters->nb_tests);
+    LAUNCH_T("np_classify", pg, 256, np_classify, A, np_lists, pair_cap, counters->nb_tests, &counters->nb_box_tests,
+             &counters->nb_hull_tests);
     int* L = np_lists;
     int* T = counters->nb_tests;
-    auto tg = [&](int algo, int block) {
-        const int h = capturing ? pair_cap : std::min(pair_cap, last.nb_tests[algo] + last.nb_tests[algo] / 4 + 512);
+    auto hint = [&](int n, int block) {
+        const int h = capturing ? pair_cap : std::min(pair_cap, n + n / 4 + 512);
         return grid_for(h, block);
     };
+    auto tg = [&](int algo, int block) { return hint(last.nb_tests[algo], block); };
     LAUNCH_T("np_sphere_sphere", tg(ALGO_SPHERE_SPHERE, 256), 256, np_sphere_sphere, A, L + ALGO_SPHERE_SPHERE * pair_cap, T + ALGO_SPHERE_SPHERE);
     LAUNCH_T("np_sphere_capsule", tg(ALGO_SPHERE_CAPSULE, 256), 256, np_sphere_capsule, A, L + ALGO_SPHERE_CAPSULE * pair_cap, T + ALGO_SPHERE_CAPSULE);
     LAUNCH_T("np_capsule_capsule", tg(ALGO_CAPSULE_CAPSULE, 256), 256, np_capsule_capsule, A, L + ALGO_CAPSULE_CAPSULE * pair_cap, T + ALGO_CAPSULE_CAPSULE);
     LAUNCH_T("np_sphere_poly", tg(ALGO_SPHERE_POLY, 256), 256, np_sphere_poly, A, L + ALGO_SPHERE_POLY * pair_cap, T + ALGO_SPHERE_POLY);
     LAUNCH_T("np_capsule_poly", tg(ALGO_CAPSULE_POLY, 256), 256, np_capsule_poly, A, L + ALGO_CAPSULE_POLY * pair_cap, T + ALGO_CAPSULE_POLY);
-    LAUNCH_T("np_poly_poly", tg(ALGO_POLY_POLY, 128), 128, np_poly_poly, A, L + ALGO_POLY_POLY * pair_cap, T + ALGO_POLY_POLY);
+    LAUNCH_T("np_box_box", hint(last.nb_box_tests, 128), 128, np_box_box, A, L, &counters->nb_box_tests);
+    LAUNCH_T("np_poly_poly", hint(last.nb_hull_tests, 128), 128, np_poly_poly, A, L + ALGO_POLY_POLY * pair_cap,
+             &counters->nb_hull_tests);
 
     // ---------------- manifold list + colouring (replaces createIslands, PhysicsWorld.cpp:695-866)
     int* nm_ptr = &counters->nb_manifolds;
@@ -836,33 +873,46 @@ int rp3dg_batch::enqueue_step(float dt, bool capturing) {
              &counters->nb_points, total_points);
     const int hm = capturing ? pair_cap : std::min(pair_cap, hint_manifolds + hint_manifolds / 4 + 1024);
     const int mg = grid_for(hm);
-    fill_int<<<grid_for(nb + 1), 256, 0, stream>>>(adj_cnt, nb + 1, 0);
-    fill_int<<<grid_for(nb), 256, 0, stream>>>(adj_cursor, nb, 0);
-    LAUNCH_T("sv_adj_count", mg, 256, sv_adj_count, B, C, N, m_list, nm_ptr, adj_cnt);
-    r = scan(adj_cnt, nb, nullptr, adj_offs, stream);
-    if (r) { set_error("scan overflow"); return r; }
-    LAUNCH_T("sv_adj_fill", mg, 256, sv_adj_fill, B, C, N, m_list, nm_ptr, adj_offs, adj_cursor, adj, mb);
-    fill_int_n<<<mg, 256, 0, stream>>>(color, pair_cap, nm_ptr, -1);
-    const int rounds = capturing ? graph_rounds : std::min(kColorRoundsMax, std::max(4, hint_rounds));
-    for (int q = 0; q < rounds; q++)
-        LAUNCH_T("sv_color_round", mg, 256, sv_color_round, mb, nm_ptr, adj_offs, adj, color,
-                 q ? &counters->color_left[q - 1] : nullptr, &counters->color_left[q], dstatus);
-    LAUNCH_T("sv_color_finish", 1, 1024, sv_color_finish, mb, nm_ptr, adj_offs, adj, color,
-             rounds ? &counters->color_left[rounds - 1] : nullptr, &counters->finish_rounds, dstatus);
-    fill_int<<<grid_for(nseg + 1), 256, 0, stream>>>(seg_cnt, nseg + 1, 0);
-    LAUNCH_T("sv_color_keys", mg, 256, sv_color_keys, color, nm_ptr, C, N, m_list, fused ? 1 : 0, ckeys, cvals, seg_cnt,
-             &counters->nb_colors);
+    const bool learned = fused && warm_mode == RP3DG_WARM_START_LEARNED && mlp;
+    const bool features = learned || log_cap > 0;
+    // manifold counts per body (the learned warm start's features read them; Jones-Plassmann needs them)
+    if (!fused || features) {
+        fill_int<<<grid_for(nb + 1), 256, 0, stream>>>(adj_cnt, nb + 1, 0);
+        LAUNCH_T("sv_adj_count", mg, 256, sv_adj_count, B, C, N, m_list, nm_ptr, adj_cnt);
+        r = scan(adj_cnt, nb, nullptr, adj_offs, stream);
+        if (r) { set_error("scan overflow"); return r; }
+    }
     uint64_t* ck = ckeys;
     int* cv = cvals;
-    int sbits = 0;
-    while ((1 << sbits) < nseg) sbits++;
-    timer_begin("sv_color_sort");
-    r = sort(&ck, &cv, cktmp, cvtmp, pair_cap, nm_ptr, sbits, stream);
-    timer_end();
-    if (r) { set_error("sort overflow"); return r; }
+    if (fused) {
+        // greedy colouring per world group in LDS, rows placed colour by colour within each group
+        LAUNCH_T("sv_world_ranges", mg, 256, sv_world_ranges, B, C, N, m_list, nm_ptr, nworlds, mb, d_wmstart);
+        LAUNCH_TS("sv_group_color", ngroups, kWave, sizeof(unsigned long long) * max_group_bodies, sv_group_color, mb,
+                  d_wmstart, d_gworld, d_wbody, ngroups, color, cvals, ckeys, seg_start, &counters->nb_colors, dstatus);
+    } else {
+        // Jones-Plassmann over the whole batch, then a radix sort of the rows by colour
+        fill_int<<<grid_for(nb), 256, 0, stream>>>(adj_cursor, nb, 0);
+        LAUNCH_T("sv_adj_fill", mg, 256, sv_adj_fill, B, C, N, m_list, nm_ptr, adj_offs, adj_cursor, adj, mb);
+        fill_int_n<<<mg, 256, 0, stream>>>(color, pair_cap, nm_ptr, -1);
+        const int rounds = capturing ? graph_rounds : std::min(kColorRoundsMax, std::max(4, hint_rounds));
+        for (int q = 0; q < rounds; q++)
+            LAUNCH_T("sv_color_round", mg, 256, sv_color_round, mb, nm_ptr, adj_offs, adj, color,
+                     q ? &counters->color_left[q - 1] : nullptr, &counters->color_left[q], dstatus);
+        LAUNCH_T("sv_color_finish", 1, 1024, sv_color_finish, mb, nm_ptr, adj_offs, adj, color,
+                 rounds ? &counters->color_left[rounds - 1] : nullptr, &counters->finish_rounds, dstatus);
+        fill_int<<<grid_for(nseg + 1), 256, 0, stream>>>(seg_cnt, nseg + 1, 0);
+        LAUNCH_T("sv_color_keys", mg, 256, sv_color_keys, color, nm_ptr, C, N, m_list, nullptr, ckeys, cvals, seg_cnt,
+                 &counters->nb_colors);
+        int sbits = 0;
+        while ((1 << sbits) < nseg) sbits++;
+        timer_begin("sv_color_sort");
+        r = sort(&ck, &cv, cktmp, cvtmp, pair_cap, nm_ptr, sbits, stream);
+        timer_end();
+        if (r) { set_error("sort overflow"); return r; }
+        r = scan(seg_cnt, nseg, nullptr, seg_start, stream);
+        if (r) { set_error("scan overflow"); return r; }
+    }
     sorted_keys = ck;
-    r = scan(seg_cnt, nseg, nullptr, seg_start, stream);
-    if (r) { set_error("scan overflow"); return r; }
 
     SvArgs SA;
     SA.B = B; SA.C = C; SA.P = N; SA.S = S;
@@ -872,7 +922,7 @@ int rp3dg_batch::enqueue_step(float dt, bool capturing) {
     SA.cstart = seg_start;
     SA.ncolors = &counters->nb_colors;
     SA.restitution_threshold = settings.restitution_threshold;
-    SA.learned = (fused && warm_mode == RP3DG_WARM_START_LEARNED && mlp) ? 1 : 0;
+    SA.learned = learned ? 1 : 0;
     SA.adj_offs = adj_offs;
     const float beta_dt = 0.2f / dt;
 
@@ -888,6 +938,7 @@ int rp3dg_batch::enqueue_step(float dt, bool capturing) {
         F.wbody = d_wbody;
         F.wcoll = d_wcoll;
         F.block_world = d_block_world;
+        F.gworld = d_gworld;
         F.hull_bounds = d_hbounds;
         F.dt = dt;
         F.beta_dt = beta_dt;
@@ -903,14 +954,13 @@ int rp3dg_batch::enqueue_step(float dt, bool capturing) {
         F.lds_bodies = 0;
         F.feats = nullptr;
         F.phase = 2;
-        const bool features = SA.learned || log_cap > 0;
         if (!features) {
             r = launch_world_classes("sv_world_solve", F, true);
             if (r) return r;
         } else {
             F.feats = feat_buf;
             F.phase = 0;
-            LAUNCH_TS("sv_world_prepare", nworlds, fused_block, 0, sv_world_solve, F);
+            LAUNCH_TS("sv_world_prepare", ngroups, fused_block, 0, sv_world_solve, F);
             if (SA.learned) {
                 const int rows = capturing ? feat_cap : std::min(feat_cap, last.nb_new_points + last.nb_new_points / 4 + 1024);
                 const int grid = std::max(1, std::min(2048, (rows + 63) / 64));
@@ -926,7 +976,7 @@ int rp3dg_batch::enqueue_step(float dt, bool capturing) {
                 sv_log_advance<<<1, 1, 0, stream>>>(&counters->nb_new_points, feat_cap, log_count, log_cap);
             }
         }
-        LAUNCH_T("world_position", nworlds, fused_block, world_position, F);
+        LAUNCH_T("world_position", ngroups, fused_block, world_position, F);
         return RP3DG_OK;
     }
 
@@ -985,7 +1035,7 @@ int rp3dg_batch::launch_world_classes(const char* name, const FusedArgs& F0, boo
         FusedArgs F = F0;
         F.block_world = d_block_world + wc.first;
         F.lds_bodies = lds ? wc.lds_bodies : 0;
-        const size_t shmem = (size_t)F.lds_bodies * 7 * sizeof(float4);
+        const size_t shmem = (size_t)F.lds_bodies * (7 * sizeof(float4) + sizeof(int));
         hipStream_t s = c == 0 ? stream : side_streams[c - 1];
         sv_world_solve<<<wc.count, wc.block, shmem, s>>>(F);
     }
@@ -1191,6 +1241,7 @@ int rp3dg_batch_get_counts(rp3dg_batch* b, rp3dg_counts* o) {
     o->total_contact_points = (int64_t)tp;
     o->steps = b->steps_done;
     o->nb_new_points = b->last.nb_new_points;
+    o->nb_box_tests = b->last.nb_box_tests;
     return RP3DG_OK;
 }
 
@@ -1405,14 +1456,15 @@ int64_t rp3dg_batch_get_log(rp3dg_batch* b, float* feats, float* targets, int64_
 
 // Autotuner: times every candidate launch shape of the solver over a few real steps of this batch
 // (the steps are not undone) and keeps the fastest.  Fused layout: workgroup size of each world
-// class = clamp(bodies bound / d, 64, 256) for d = 1, 2, 4, 8 (the choice returned is d); per-colour
+// class = clamp(bodies bound / d, 64, 256) for d = 1, 2, 4, 8, 16 (the choice returned is d); per-colour
 // layout: threads per block of the colour kernels.
 int rp3dg_batch_autotune(rp3dg_batch* b, float dt, int32_t steps_per_trial, int32_t* chosen, double* best_ms) {
     if (!b || !(dt > 0.f) || steps_per_trial < 1 || steps_per_trial > 1000) return RP3DG_EINVAL;
-    std::vector<int> cand = b->fused ? std::vector<int>{1, 2, 4, 8} : std::vector<int>{64, 128, 256, 512};
+    std::vector<int> cand = b->fused ? std::vector<int>{1, 2, 4, 8, 16} : std::vector<int>{64, 128, 256, 512};
     const bool prof = b->profiling;
     b->profiling = false;
     b->drop_graphs();
+    b->tune_table.clear();
     int best = -1;
     double best_t = 1e300;
     for (int c : cand) {
@@ -1438,6 +1490,16 @@ int rp3dg_batch_autotune(rp3dg_batch* b, float dt, int32_t steps_per_trial, int3
     if (chosen) *chosen = best;
     if (best_ms) *best_ms = best_t;
     return RP3DG_OK;
+}
+
+int32_t rp3dg_batch_get_tune_table(rp3dg_batch* b, int32_t* choices, double* ms, int32_t cap) {
+    if (!b || cap < 0) return RP3DG_EINVAL;
+    const int32_t n = (int32_t)b->tune_table.size();
+    for (int32_t i = 0; i < n && i < cap; i++) {
+        if (choices) choices[i] = b->tune_table[i].first;
+        if (ms) ms[i] = b->tune_table[i].second;
+    }
+    return n;
 }
 
 int rp3dg_batch_get_iteration_stats(rp3dg_batch* b, double* mean_it, double* max_it, double* max_res) {

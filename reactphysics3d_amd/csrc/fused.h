@@ -8,15 +8,16 @@
 namespace rp3dg {
 
 struct FusedArgs {
-    SvArgs A;                  // rows in (world, colour)-major order; A.cstart unused
-    DevJoints J;               // rows sorted by (world, kind, colour, creation index)
+    SvArgs A;                  // rows in (world group, colour)-major order; A.cstart unused
+    DevJoints J;               // rows sorted by (world group, kind, colour, creation index)
     Settings settings;
-    const int* seg_start;      // [w * kMaxColors + c]: first row of colour c of world w; [nw * kMaxColors] = total
-    const int2* jt_wr;         // [w * jt_slots + s]: joint rows of slot s (kind, colour) of world w
+    const int* seg_start;      // [g * kMaxColors + c]: first row of colour c of group g; [ng * kMaxColors] = total
+    const int2* jt_wr;         // [g * jt_slots + s]: joint rows of slot s (kind, colour) of group g
     int jt_slots;
     const int* wbody;          // [nw + 1] body ranges of the worlds
     const int* wcoll;          // [nw + 1] collider ranges of the worlds
-    const int* block_world;    // workgroup -> world (largest worlds first)
+    const int* gworld;         // [ng + 1] world ranges of the groups (consecutive worlds)
+    const int* block_world;    // workgroup -> group (largest groups first)
     const float4* hull_bounds;
     float dt, beta_dt;
     int max_iters;             // velocity iterations (fixed), or the cap in residual mode
@@ -30,13 +31,15 @@ struct FusedArgs {
     int* row_slot;             // solver point slot (4 * row + point) of each feature row
     int* nrows;                // feature rows written this step
     int feat_cap;
-    int lds_bodies;            // worlds with at most this many bodies keep cv, cw, sv, sw and the world
-                               // inverse inertia rows in LDS (112 * lds_bodies bytes); 0 = global memory
+    int lds_bodies;            // worlds with at most this many bodies keep cv, cw, sv, sw, the world inverse
+                               // inertia rows and the flags in LDS (116 * lds_bodies bytes); 0 = HBM
 };
 
 constexpr int kFeatures = 24;
 
-constexpr int kFusedMaxBlock = 256;   // threads per world at most (register budget of the fused kernels)
+constexpr int kFusedMaxBlock = 256;   // threads per group at most (register budget of the fused kernels)
+constexpr int kGroupMaxWorlds = 64;   // worlds per group at most (one bit each in the convergence mask)
+constexpr int kGroupShift = 16;       // staged body flags carry the body's world index in its group here
 
 __global__ void sv_world_solve(FusedArgs F);
 __global__ void world_position(FusedArgs F);

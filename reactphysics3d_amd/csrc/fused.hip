@@ -1,18 +1,22 @@
 // Fused per-world solver for batches of many small worlds.
 //
-// One workgroup owns one world for the whole solver phase of a step: velocity
-// integration of its bodies, contact rows, joint rows, warm start, every
-// velocity iteration (colours and joint slots separated by workgroup
-// barriers), impulse store; a second kernel does position integration, the
-// joint position iterations, the state update and the fat-AABB refit.  Worlds
-// share nothing, so no grid-wide synchronisation and no per-colour launches are
-// needed; every world runs its sweeps in the same order as the per-colour
-// kernels (joint slots, then contact colours), so results are identical.
+// One workgroup owns one group of consecutive worlds for the whole solver phase
+// of a step: velocity integration of its bodies, contact rows, joint rows, warm
+// start, every velocity iteration (colours and joint slots separated by
+// workgroup barriers), impulse store; a second kernel does position
+// integration, the joint position iterations, the state update and the
+// fat-AABB refit.  Groups share nothing, so no grid-wide synchronisation and no
+// per-colour launches are needed.  Colour c of a group is colour c of each of
+// its worlds side by side (worlds share no body), so every world runs its
+// sweeps in the same order as the per-colour kernels (joint slots, then contact
+// colours) and results are identical; packing worlds fills the wavefronts of
+// the colour phases, which a single small world leaves mostly idle.
 //
 // In residual mode a world iterates until the velocity residual
 //   r_k = max over its dynamic bodies of max(|dv|_inf, |dw|_inf) between sweeps
-// drops below the threshold (or the iteration cap is reached); this mode has
-// no counterpart in the reference, which always runs a fixed iteration count
+// drops below the threshold (or the iteration cap is reached); a converged
+// world of a group skips the remaining sweeps of the others.  This mode has no
+// counterpart in the reference, which always runs a fixed iteration count
 // (PhysicsWorld.cpp:334).
 #include <hip/hip_runtime.h>
 
@@ -38,15 +42,18 @@ __device__ __forceinline__ float block_maxf(float v, float* red) {
 
 __global__ __launch_bounds__(kFusedMaxBlock) void sv_world_solve(FusedArgs F) {
     __shared__ float red[16];
-    extern __shared__ float4 s_vel[];   // [7][lds_bodies]: cv, cw, sv, sw, iw0, iw1, iw2 of the world's bodies
+    __shared__ unsigned int s_wres[kGroupMaxWorlds];   // residual of each world of the group (float bits)
+    __shared__ unsigned long long s_done;               // converged worlds of the group
+    extern __shared__ float4 s_vel[];   // [7][lds_bodies] cv, cw, sv, sw, iw0, iw1, iw2; then [lds_bodies] flags
     const SvArgs& AG = F.A;
     const DevBodies& B = AG.B;
-    const int w = F.block_world[blockIdx.x];
+    const int g = F.block_world[blockIdx.x];
     const int tid = threadIdx.x, nt = blockDim.x;
-    const int b0 = F.wbody[w], b1 = F.wbody[w + 1];
-    const int* seg = F.seg_start + (size_t)w * kMaxColors;
+    const int w0 = F.gworld[g], ngw = F.gworld[g + 1] - w0;
+    const int b0 = F.wbody[w0], b1 = F.wbody[w0 + ngw];
+    const int* seg = F.seg_start + (size_t)g * kMaxColors;
     const int m0 = seg[0], m1 = seg[kMaxColors];
-    const int2* jr = F.jt_wr + (size_t)w * F.jt_slots;
+    const int2* jr = F.jt_wr + (size_t)g * F.jt_slots;
     const int j0 = F.jt_slots ? jr[0].x : 0, j1 = F.jt_slots ? jr[F.jt_slots - 1].y : 0;
 
     if (F.phase != 1) {
@@ -101,7 +108,9 @@ __global__ __launch_bounds__(kFusedMaxBlock) void sv_world_solve(FusedArgs F) {
         const int L = F.lds_bodies;
         float4 *lcv = s_vel, *lcw = s_vel + L, *lsv = s_vel + 2 * L, *lsw = s_vel + 3 * L;
         float4 *li0 = s_vel + 4 * L, *li1 = s_vel + 5 * L, *li2 = s_vel + 6 * L;
+        int* lfl = reinterpret_cast<int*>(s_vel + 7 * L);
         for (int i = tid; i < nbw; i += nt) {
+            lfl[i] = B.flags[b0 + i] | ((B.world[b0 + i] - w0) << kGroupShift);
             lcv[i] = B.cv[b0 + i];
             lcw[i] = B.cw[b0 + i];
             lsv[i] = B.sv[b0 + i];
@@ -112,6 +121,7 @@ __global__ __launch_bounds__(kFusedMaxBlock) void sv_world_solve(FusedArgs F) {
         }
         A.B.cv = lcv; A.B.cw = lcw; A.B.sv = lsv; A.B.sw = lsw;
         A.B.iw0 = li0; A.B.iw1 = li1; A.B.iw2 = li2;
+        A.B.flags = lfl;
         A.B.vbase = b0;
         __syncthreads();
     }
@@ -131,11 +141,23 @@ __global__ __launch_bounds__(kFusedMaxBlock) void sv_world_solve(FusedArgs F) {
         __syncthreads();
     }
     const bool residual = F.res_thr > 0.0f;
+    // residual mode with several worlds: each world stops on its own residual (mask of converged worlds)
+    const bool masked = residual && ngw > 1;
+    const unsigned long long all = ngw >= 64 ? ~0ull : (1ull << ngw) - 1ull;
+    unsigned long long done = 0;
+    // world of body b within the group
+    auto wk = [&](int b) { return staged ? (BS.flags[b - BS.vbase] >> kGroupShift) & 63 : B.world[b] - w0; };
     if (residual) {
         for (int i = b0 + tid; i < b1; i += nt) {
             F.pv[i] = BS.cv[i - BS.vbase];
             F.pw[i] = BS.cw[i - BS.vbase];
         }
+        if (masked)
+            for (int k = tid; k < ngw; k += nt) {
+                s_wres[k] = 0u;
+                F.world_iters[w0 + k] = 0;
+                F.world_res[w0 + k] = 0.0f;
+            }
         __syncthreads();
     }
     int it = 0;
@@ -144,13 +166,19 @@ __global__ __launch_bounds__(kFusedMaxBlock) void sv_world_solve(FusedArgs F) {
         for (int s = 0; s < F.jt_slots; s++) {
             const int2 r = jr[s];
             if (r.x == r.y) continue;
-            for (int j = r.x + tid; j < r.y; j += nt) jt_solve_velocity_one(BS, F.J, j, F.dt);
+            for (int j = r.x + tid; j < r.y; j += nt) {
+                if (masked && ((done >> wk(F.J.bodies[j].x)) & 1ull)) continue;
+                jt_solve_velocity_one(BS, F.J, j, F.dt);
+            }
             __syncthreads();
         }
         for (int c = 0; c < kMaxColors; c++) {
             const int lo = seg[c], hi = seg[c + 1];
             if (lo == m1) break;
-            for (int i = lo + tid; i < hi; i += nt) solve_manifold(A, i, F.beta_dt);
+            for (int i = lo + tid; i < hi; i += nt) {
+                if (masked && ((done >> wk(A.S.bodies[i].x)) & 1ull)) continue;
+                solve_manifold(A, i, F.beta_dt);
+            }
             __syncthreads();
         }
         it++;
@@ -158,6 +186,11 @@ __global__ __launch_bounds__(kFusedMaxBlock) void sv_world_solve(FusedArgs F) {
             float r = 0.0f;
             for (int i = b0 + tid; i < b1; i += nt) {
                 if (!dyn_body(B, i)) continue;
+                int k = 0;
+                if (masked) {
+                    k = wk(i);
+                    if ((done >> k) & 1ull) continue;
+                }
                 const float4 v = BS.cv[i - BS.vbase], wv = BS.cw[i - BS.vbase], pv = F.pv[i], pw = F.pw[i];
                 float d = fabsf(v.x - pv.x);
                 d = fmaxf(d, fabsf(v.y - pv.y));
@@ -165,12 +198,32 @@ __global__ __launch_bounds__(kFusedMaxBlock) void sv_world_solve(FusedArgs F) {
                 d = fmaxf(d, fabsf(wv.x - pw.x));
                 d = fmaxf(d, fabsf(wv.y - pw.y));
                 d = fmaxf(d, fabsf(wv.z - pw.z));
-                r = fmaxf(r, d);
+                if (masked) atomicMax(&s_wres[k], __float_as_uint(d));   // d >= 0: bit order = value order
+                else r = fmaxf(r, d);
                 F.pv[i] = v;
                 F.pw[i] = wv;
             }
-            res = block_maxf(r, red);
-            if (res < F.res_thr) break;
+            if (!masked) {
+                res = block_maxf(r, red);
+                if (res < F.res_thr) break;
+            } else {
+                __syncthreads();
+                if (tid < kWave) {
+                    bool fin = false;
+                    if (tid < ngw && !((done >> tid) & 1ull)) {
+                        const float rk = __uint_as_float(s_wres[tid]);
+                        s_wres[tid] = 0u;
+                        fin = rk < F.res_thr;
+                        F.world_iters[w0 + tid] = it;
+                        F.world_res[w0 + tid] = rk;
+                    }
+                    const unsigned long long bal = __ballot(fin);
+                    if (tid == 0) s_done = done | bal;
+                }
+                __syncthreads();
+                done = s_done;
+                if (done == all) break;
+            }
         }
     }
     // storeImpulses (CS.cpp:763-785)
@@ -184,19 +237,21 @@ __global__ __launch_bounds__(kFusedMaxBlock) void sv_world_solve(FusedArgs F) {
             B.sw[b0 + i] = BS.sw[i];
         }
     }
-    if (tid == 0) {
-        F.world_iters[w] = it;
-        F.world_res[w] = res;
-    }
+    if (!masked)
+        for (int k = tid; k < ngw; k += nt) {
+            F.world_iters[w0 + k] = it;
+            F.world_res[w0 + k] = res;
+        }
 }
 
 __global__ __launch_bounds__(kFusedMaxBlock) void world_position(FusedArgs F) {
     const DevBodies& B = F.A.B;
-    const int w = F.block_world[blockIdx.x];
+    const int g = F.block_world[blockIdx.x];
     const int tid = threadIdx.x, nt = blockDim.x;
-    const int b0 = F.wbody[w], b1 = F.wbody[w + 1];
-    const int c0 = F.wcoll[w], c1 = F.wcoll[w + 1];
-    const int2* jr = F.jt_wr + (size_t)w * F.jt_slots;
+    const int w0 = F.gworld[g], w1 = F.gworld[g + 1];
+    const int b0 = F.wbody[w0], b1 = F.wbody[w1];
+    const int c0 = F.wcoll[w0], c1 = F.wcoll[w1];
+    const int2* jr = F.jt_wr + (size_t)g * F.jt_slots;
     // PhysicsWorld.cpp:275-288: integrate positions, joint position correction, update body state
     for (int i = b0 + tid; i < b1; i += nt) dyn_integrate_positions_one(B, F.dt, i);
     __syncthreads();

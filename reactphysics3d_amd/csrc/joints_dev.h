@@ -15,6 +15,7 @@
 
 #include "joints.h"
 #include "kcommon.h"
+#include "solver_dev.h"
 
 namespace rp3dg {
 
@@ -133,7 +134,7 @@ __device__ __forceinline__ void jt_init_one(const DevBodies& B, const DevJoints&
         J.r1[j] = f4(r1, 0.f);
         J.r2[j] = f4(r2, 0.f);
         const float im1 = B.ii[b1].w, im2 = B.ii[b2].w;
-        const bool anyDyn = ((B.flags[b1] | B.flags[b2]) & BF_DYNAMIC) != 0;
+        const bool anyDyn = ((body_flags(B, b1) | body_flags(B, b2)) & BF_DYNAMIC) != 0;
         bool ok;
         const M3 Kt = translation_inverse_mass(im1 + im2, r1, r2, I1, I2, anyDyn, ok);
         stm(J.kt0, J.kt1, J.kt2, j, Kt);
@@ -203,7 +204,9 @@ __device__ __forceinline__ void jt_warmstart_one(const DevBodies& B, const DevJo
         const int u1 = b1 - B.vbase, u2 = b2 - B.vbase;
         V3 v1 = ld3(B.cv, u1), w1 = ld3(B.cw, u1), v2 = ld3(B.cv, u2), w2 = ld3(B.cw, u2);
         const float im1 = B.ii[b1].w, im2 = B.ii[b2].w;
-        const V3 ll1 = ld3(B.llock, b1), al1 = ld3(B.alock, b1), ll2 = ld3(B.llock, b2), al2 = ld3(B.alock, b2);
+        V3 ll1, al1, ll2, al2;
+        body_locks(B, b1, ll1, al1);
+        body_locks(B, b2, ll2, al2);
         const M3 I1 = ldm(J.i1r0, J.i1r1, J.i1r2, j), I2 = ldm(J.i2r0, J.i2r1, J.i2r2, j);
         const V3 r1 = ld3(J.r1, j), r2 = ld3(J.r2, j);
         const V3 it = ld3(J.impT, j);
@@ -268,7 +271,9 @@ __device__ __forceinline__ void jt_solve_velocity_one(const DevBodies& B, const 
         const int u1 = b1 - B.vbase, u2 = b2 - B.vbase;
         V3 v1 = ld3(B.cv, u1), w1 = ld3(B.cw, u1), v2 = ld3(B.cv, u2), w2 = ld3(B.cw, u2);
         const float im1 = B.ii[b1].w, im2 = B.ii[b2].w;
-        const V3 ll1 = ld3(B.llock, b1), al1 = ld3(B.alock, b1), ll2 = ld3(B.llock, b2), al2 = ld3(B.alock, b2);
+        V3 ll1, al1, ll2, al2;
+        body_locks(B, b1, ll1, al1);
+        body_locks(B, b2, ll2, al2);
         const M3 I1 = ldm(J.i1r0, J.i1r1, J.i1r2, j), I2 = ldm(J.i2r0, J.i2r1, J.i2r2, j);
         const V3 r1 = ld3(J.r1, j), r2 = ld3(J.r2, j);
         const M3 Kt = ldm(J.kt0, J.kt1, J.kt2, j);
@@ -387,8 +392,10 @@ __device__ __forceinline__ void jt_solve_position_one(const DevBodies& B, const 
         V3 x1 = ld3(B.cx, b1), x2 = ld3(B.cx, b2);
         const float4 ii1 = B.ii[b1], ii2 = B.ii[b2];
         const float im1 = ii1.w, im2 = ii2.w;
-        const V3 ll1 = ld3(B.llock, b1), al1 = ld3(B.alock, b1), ll2 = ld3(B.llock, b2), al2 = ld3(B.alock, b2);
-        const bool anyDyn = ((B.flags[b1] | B.flags[b2]) & BF_DYNAMIC) != 0;
+        V3 ll1, al1, ll2, al2;
+        body_locks(B, b1, ll1, al1);
+        body_locks(B, b2, ll2, al2);
+        const bool anyDyn = ((body_flags(B, b1) | body_flags(B, b2)) & BF_DYNAMIC) != 0;
         M3 I1 = world_inv_inertia(qmat(q1), v3(ii1.x, ii1.y, ii1.z));
         M3 I2 = world_inv_inertia(qmat(q2), v3(ii2.x, ii2.y, ii2.z));
         const V3 r1 = qrot(q1, ld3(J.la1, j) - ld3(B.com, b1));

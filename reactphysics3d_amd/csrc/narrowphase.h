@@ -21,13 +21,14 @@ struct NpArgs {
     float cos_similar;
 };
 
-__global__ void np_classify(NpArgs A, int* lists, int list_stride, int* counts);
+__global__ void np_classify(NpArgs A, int* lists, int list_stride, int* counts, int* box_count, int* hull_count);
 __global__ void np_sphere_sphere(NpArgs A, const int* list, const int* count);
 __global__ void np_sphere_capsule(NpArgs A, const int* list, const int* count);
 __global__ void np_capsule_capsule(NpArgs A, const int* list, const int* count);
 __global__ void np_sphere_poly(NpArgs A, const int* list, const int* count);
 __global__ void np_capsule_poly(NpArgs A, const int* list, const int* count);
 __global__ void np_poly_poly(NpArgs A, const int* list, const int* count);
+__global__ void np_box_box(NpArgs A, const int* list, const int* count);
 void upload_box_topology(const int* edges4, const int* face_edge, const int* face_verts);
 
 }  // namespace rp3dg

@@ -116,19 +116,26 @@ __device__ inline Poly make_poly(const NpArgs& A, int collider) {
 // ------------------------------------------------------------------------------------------------
 // Raw contact buffer of one narrowphase test
 // ------------------------------------------------------------------------------------------------
-struct Raw {
+// N = kMaxRawContacts in general; box-box tests produce at most 8 points (a quad clipped by four
+// planes), so their kernel uses a buffer of 8 and the reference's cap of 16 is never reached.
+template <int N>
+struct RawBuf {
+    static constexpr int kN = N;
     int n;
-    V3 nrm[kMaxRawContacts];
-    float depth[kMaxRawContacts];
-    V3 l1[kMaxRawContacts];
-    V3 l2[kMaxRawContacts];
+    V3 nrm[N];
+    float depth[N];
+    V3 l1[N];
+    V3 l2[N];
     __device__ void add(V3 normal, float d, V3 a, V3 b) {  // NarrowPhaseInfoBatch::addContactPoint
-        if (n < kMaxRawContacts) {
+        if (n < N) {
             nrm[n] = normal; depth[n] = d; l1[n] = a; l2[n] = b;
             n++;
         }
     }
 };
+using Raw = RawBuf<kMaxRawContacts>;
+constexpr int kBoxRaw = 8;      // raw contacts of a box-box test at most
+constexpr int kBoxClip = 8;     // vertices of a box face clipped by the four side planes at most
 
 // ------------------------------------------------------------------------------------------------
 // SAT, polyhedron vs polyhedron (SATAlgorithm.cpp:485-1191)
@@ -189,6 +196,7 @@ __device__ inline float sat_edge_distance(V3 e1a, V3 e2a, V3 c1, V3 e1d, V3 e2d,
 }
 
 // mathematics_functions.h:332-392
+template <int CM>
 __device__ inline int clip_polygon(const V3* in, int nin, V3 pp, V3 pn, V3* out) {
     int nout = 0;
     int s = nin - 1;
@@ -218,8 +226,9 @@ __device__ inline int clip_polygon(const V3* in, int nin, V3 pp, V3 pn, V3* out)
 }
 
 // :893-1036
-__device__ __noinline__ bool sat_face_contacts(bool refIs1, const Poly& P1, const Poly& P2, const Tf& t12, const Tf& t21, int minFace,
-                                  const Tf& T1, const Tf& T2, Raw& raw) {
+template <int CM, class RB>
+__device__ __forceinline__ bool sat_face_contacts_t(bool refIs1, const Poly& P1, const Poly& P2, const Tf& t12,
+                                                    const Tf& t21, int minFace, const Tf& T1, const Tf& T2, RB& raw) {
     const Tf& refToInc = refIs1 ? t12 : t21;
     const Tf& incToRef = refIs1 ? t21 : t12;
     const Poly& R = refIs1 ? P1 : P2;
@@ -228,10 +237,10 @@ __device__ __noinline__ bool sat_face_contacts(bool refIs1, const Poly& P1, cons
     const V3 axisInc = qrot(refToInc.q, axisRef);
     const V3 normalWorld = refIs1 ? qrot(T1.q, axisRef) : -(qrot(T2.q, axisRef));
     const int incFace = I.most_anti_parallel_face(axisInc);
-    V3 bufA[kClipMax], bufB[kClipMax];
+    V3 bufA[CM], bufB[CM];
     int na = 0, nb = 0;
     const int niv = I.face_nverts(incFace);
-    for (int i = 0; i < niv && na < kClipMax; i++) bufA[na++] = tapply(incToRef, I.vertex(I.face_vertex(incFace, i)));
+    for (int i = 0; i < niv && na < CM; i++) bufA[na++] = tapply(incToRef, I.vertex(I.face_vertex(incFace, i)));
     const int firstEdge = R.face_edge(minFace);
     bool v1Input = false;
     int nOut = 0;
@@ -245,8 +254,8 @@ __device__ __noinline__ bool sat_face_contacts(bool refIs1, const Poly& P1, cons
         const int4 nxt = R.edge(cur.w);
         const V3 ev2 = R.vertex(nxt.x);
         const V3 planeNormal = cross(axisRef, ev2 - ev1);
-        if (v1Input) nb = clip_polygon(bufA, na, ev1, planeNormal, bufB);
-        else na = clip_polygon(bufB, nb, ev1, planeNormal, bufA);
+        if (v1Input) nb = clip_polygon<CM>(bufA, na, ev1, planeNormal, bufB);
+        else na = clip_polygon<CM>(bufB, nb, ev1, planeNormal, bufA);
         curIdx = cur.w;
         cur = nxt;
         ev1 = ev2;
@@ -269,6 +278,20 @@ __device__ __noinline__ bool sat_face_contacts(bool refIs1, const Poly& P1, cons
     return found;
 }
 
+// General polyhedra (hulls): out of line, large clip buffers
+__device__ __noinline__ bool sat_face_contacts(bool refIs1, const Poly& P1, const Poly& P2, const Tf& t12,
+                                               const Tf& t21, int minFace, const Tf& T1, const Tf& T2, Raw& raw) {
+    return sat_face_contacts_t<kClipMax>(refIs1, P1, P2, t12, t21, minFace, T1, T2, raw);
+}
+
+// Box-box tests inline the face clipping with buffers sized for a box; hull tests call it out of line.
+template <bool BOX, class R>
+__device__ __forceinline__ bool face_contacts(bool refIs1, const Poly& P1, const Poly& P2, const Tf& t12, const Tf& t21,
+                                              int minFace, const Tf& T1, const Tf& T2, R& raw) {
+    if constexpr (BOX) return sat_face_contacts_t<kBoxClip>(refIs1, P1, P2, t12, t21, minFace, T1, T2, raw);
+    else return sat_face_contacts(refIs1, P1, P2, t12, t21, minFace, T1, T2, raw);
+}
+
 __device__ inline uint32_t lf_set_face(uint32_t lf, bool face1, int face) {
     lf &= ~(LF_SAT_FACE1 | LF_SAT_FACE2 | 0xff00u);
     lf |= face1 ? LF_SAT_FACE1 : LF_SAT_FACE2;
@@ -283,7 +306,9 @@ __device__ inline uint32_t lf_set_edges(uint32_t lf, int e1, int e2) {
 }
 
 // Returns isColliding; updates the SAT part of the temporal-coherence word.
-__device__ bool sat_poly_poly(const Poly& P1, const Poly& P2, const Tf& T1, const Tf& T2, uint32_t& lf, Raw& raw) {
+template <bool BOX, class R>
+__device__ __forceinline__ bool sat_poly_poly(const Poly& P1, const Poly& P2, const Tf& T1, const Tf& T2, uint32_t& lf,
+                                              R& raw) {
     const Tf t12 = tmul(tinv(T2), T1);
     const Tf t21 = tinv(t12);
     float minPen = kDecimalLargest;
@@ -299,7 +324,7 @@ __device__ bool sat_poly_poly(const Poly& P1, const Poly& P2, const Tf& T1, cons
             const float pen = sat_single_face(P1, P2, t12, f);
             if (!wasColliding && pen <= 0.0f) return false;
             if (wasColliding && pen > 0.0f) {
-                if (sat_face_contacts(true, P1, P2, t12, t21, f, T1, T2, raw)) {
+                if (face_contacts<BOX>(true, P1, P2, t12, t21, f, T1, T2, raw)) {
                     lf = lf_set_face(lf, true, f);
                     return true;
                 }
@@ -309,7 +334,7 @@ __device__ bool sat_poly_poly(const Poly& P1, const Poly& P2, const Tf& T1, cons
             const float pen = sat_single_face(P2, P1, t21, f);
             if (!wasColliding && pen <= 0.0f) return false;
             if (wasColliding && pen > 0.0f) {
-                if (sat_face_contacts(false, P1, P2, t12, t21, f, T1, T2, raw)) {
+                if (face_contacts<BOX>(false, P1, P2, t12, t21, f, T1, T2, raw)) {
                     lf = lf_set_face(lf, false, f);
                     return true;
                 }
@@ -389,7 +414,7 @@ __device__ bool sat_poly_poly(const Poly& P1, const Poly& P2, const Tf& T1, cons
         }
     }
     if (isMinFace) {
-        const bool found = sat_face_contacts(isMinFace1, P1, P2, t12, t21, minFace, T1, T2, raw);
+        const bool found = face_contacts<BOX>(isMinFace1, P1, P2, t12, t21, minFace, T1, T2, raw);
         lf = lf_set_face(lf, isMinFace1, minFace);
         return found;
     }
@@ -558,8 +583,9 @@ __device__ bool capsule_capsule(float r1, float h1, float r2, float h2, const Tf
 // ------------------------------------------------------------------------------------------------
 
 // CollisionDetectionSystem::reduceContactPoints (:1483-1662): picks 4 of n > 4 raw points.
-__device__ void reduce_points(const Raw& raw, const Tf& T1, int* keep) {
-    int cand[kMaxRawContacts];
+template <class R>
+__device__ void reduce_points(const R& raw, const Tf& T1, int* keep) {
+    int cand[R::kN];
     int nc = raw.n;
     for (int i = 0; i < nc; i++) cand[i] = i;
     const Tf w2s = tinv(T1);
@@ -615,14 +641,15 @@ __device__ void reduce_points(const Raw& raw, const Tf& T1, int* keep) {
     keep[3] = cand[idx];
 }
 
-__device__ void contact_epilogue(const NpArgs& A, int slot, bool colliding, const Raw& raw, int collider1) {
+template <class R>
+__device__ void contact_epilogue(const NpArgs& A, int slot, bool colliding, const R& raw, int collider1) {
     DevPairs P = A.P;
     const int oldN = P.ncp[slot];
     if (!colliding || raw.n == 0) {
         P.ncp[slot] = 0;
         return;
     }
-    int idx[kMaxRawContacts];
+    int idx[R::kN];
     int n = raw.n;
     for (int i = 0; i < n; i++) idx[i] = i;
     if (n > kMaxManifoldPoints) {
@@ -698,7 +725,10 @@ __device__ inline uint32_t lf_after_test(uint32_t lf, bool colliding) {
 // bodies is enabled (CD.cpp:391-400).  Untested pairs have no contact pair this step.
 // List slots are reserved with one atomic per wave and algorithm (ballot + popcount ranks); the list
 // order is irrelevant because every test writes only its own pair slot.
-__global__ void np_classify(NpArgs A, int* lists, int list_stride, int* counts) {
+// Polyhedron pairs are split further: box-box pairs go to list slot 0 (ALGO_NONE is never listed)
+// counted in box_count, the others stay in the ALGO_POLY_POLY slot counted in hull_count;
+// counts[ALGO_POLY_POLY] keeps the total.
+__global__ void np_classify(NpArgs A, int* lists, int list_stride, int* counts, int* box_count, int* hull_count) {
     const int np = *A.nb_pairs;
     const int stride = gridDim.x * blockDim.x;
     const int lane = lane_id();
@@ -706,20 +736,39 @@ __global__ void np_classify(NpArgs A, int* lists, int list_stride, int* counts) 
     for (int base = blockIdx.x * blockDim.x; base < np; base += stride) {
         const int k = base + threadIdx.x;
         int a = 0;
+        bool boxes = false;
         if (k < np) {
             const int2 c = A.P.col[k];
             const int f1 = A.B.flags[A.C.body[c.x]], f2 = A.B.flags[A.C.body[c.y]];
             if ((f1 & BF_ENABLED) || (f2 & BF_ENABLED)) a = A.P.algo[k];
             else A.P.ncp[k] = 0;
+            if (a == ALGO_POLY_POLY) boxes = A.C.type[c.x] == SHAPE_BOX && A.C.type[c.y] == SHAPE_BOX;
         }
         for (int t = 1; t < ALGO_COUNT; t++) {
             const uint64_t m = __ballot(a == t);
             if (m == 0) continue;
             const int leader = __ffsll((unsigned long long)m) - 1;
-            int pos = 0;
-            if (lane == leader) pos = atomicAdd(&counts[t], __popcll(m));
-            pos = __shfl(pos, leader, kWave);
-            if (a == t) lists[t * list_stride + pos + __popcll(m & lt)] = k;
+            if (t != ALGO_POLY_POLY) {
+                int pos = 0;
+                if (lane == leader) pos = atomicAdd(&counts[t], __popcll(m));
+                pos = __shfl(pos, leader, kWave);
+                if (a == t) lists[t * list_stride + pos + __popcll(m & lt)] = k;
+                continue;
+            }
+            if (lane == leader) atomicAdd(&counts[t], __popcll(m));
+            const uint64_t mb = __ballot(a == t && boxes);
+            const uint64_t mh = m & ~mb;
+            int pb = 0, ph = 0;
+            if (lane == leader) {
+                if (mb) pb = atomicAdd(box_count, __popcll(mb));
+                if (mh) ph = atomicAdd(hull_count, __popcll(mh));
+            }
+            pb = __shfl(pb, leader, kWave);
+            ph = __shfl(ph, leader, kWave);
+            if (a == t) {
+                if (boxes) lists[pb + __popcll(mb & lt)] = k;
+                else lists[t * list_stride + ph + __popcll(mh & lt)] = k;
+            }
         }
     }
 }
@@ -777,7 +826,35 @@ __global__ __launch_bounds__(128) void np_poly_poly(NpArgs A, const int* list, c
         const Poly P2 = make_poly(A, c.y);
         Raw raw; raw.n = 0;
         uint32_t lf = A.P.lf[k];
-        const bool hit = sat_poly_poly(P1, P2, T1, T2, lf, raw);
+        const bool hit = sat_poly_poly<false>(P1, P2, T1, T2, lf, raw);
+        // ConvexPolyhedronVsConvexPolyhedronAlgorithm.cpp:55-62
+        lf |= LF_WAS_SAT;
+        lf &= ~LF_WAS_GJK;
+        A.P.lf[k] = lf_after_test(lf, hit);
+        contact_epilogue(A, k, hit, raw, c.x);
+    }
+}
+
+// Box-box pairs (the same SAT test with the box topology known at compile time and box-sized buffers)
+__device__ __forceinline__ Poly box_poly(const NpArgs& A, int c) {
+    Poly P{};
+    const float4 s = A.C.shape[c];
+    P.hull = -1;
+    P.he = v3(s.x, s.y, s.z);
+    return P;
+}
+
+__global__ __launch_bounds__(128) void np_box_box(NpArgs A, const int* list, const int* count) {
+    RP3DG_GRID_STRIDE(t, *count) {
+        const int k = list[t];
+        const int2 c = A.P.col[k];
+        const Tf T1 = tf(ld3(A.C.wpos, c.x), ldq(A.C.wq, c.x));
+        const Tf T2 = tf(ld3(A.C.wpos, c.y), ldq(A.C.wq, c.y));
+        const Poly P1 = box_poly(A, c.x);
+        const Poly P2 = box_poly(A, c.y);
+        RawBuf<kBoxRaw> raw; raw.n = 0;
+        uint32_t lf = A.P.lf[k];
+        const bool hit = sat_poly_poly<true>(P1, P2, T1, T2, lf, raw);
         // ConvexPolyhedronVsConvexPolyhedronAlgorithm.cpp:55-62
         lf |= LF_WAS_SAT;
         lf &= ~LF_WAS_GJK;

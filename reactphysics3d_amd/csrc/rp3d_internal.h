@@ -42,6 +42,7 @@ constexpr int BF_DYNAMIC = 1;   // BodyType::DYNAMIC
 constexpr int BF_ENABLED = 2;   // in the enabled partition (not static, not sleeping)
 constexpr int BF_GRAVITY = 4;   // gravity enabled
 constexpr int BF_KINEMATIC = 8;
+constexpr int BF_UNIT_LOCKS = 16;   // linear and angular lock factors are (1, 1, 1): the solver skips them
 
 // LastFrameCollisionInfo (include/reactphysics3d/engine/OverlappingPairs.h:55-93) packed in a word:
 constexpr uint32_t LF_VALID = 1u << 0;
@@ -80,7 +81,7 @@ struct DevBodies {
     float4 *iw0, *iw1, *iw2;    // world inverse inertia rows
     float4 *com;                // local centre of mass xyz, w = mass
     float4 *llock, *alock;      // lock factors xyz; llock.w = linear damping, alock.w = angular damping
-    int *flags;
+    int *flags;                 // indexed [body - vbase] like the velocities
     int *world;
     int n;
     int vbase;                  // 0, or the first body of a world whose velocities are staged in LDS
@@ -161,7 +162,9 @@ struct DevCounters {
     int finish_rounds;         // rounds taken by the single-workgroup colouring finisher
     int nb_new_points;         // contact points without a cached impulse
     int color_left[kColorRoundsMax];   // manifolds still uncoloured after each parallel round
-    int pad[4];
+    int nb_box_tests;          // box-box subset of nb_tests[ALGO_POLY_POLY]
+    int nb_hull_tests;         // the rest of nb_tests[ALGO_POLY_POLY] (a convex mesh involved)
+    int pad[2];
 };
 
 constexpr int STATUS_PAIR_OVERFLOW = 1;

@@ -36,8 +36,12 @@ __global__ void sv_color_round(const int2* mb, const int* nm_ptr, const int* off
                                const int* left_prev, int* left, int* status);
 __global__ void sv_color_finish(const int2* mb, const int* nm_ptr, const int* offs, const int* adj, int* color,
                                 const int* left_prev, int* rounds, int* status);
+__global__ void sv_world_ranges(DevBodies B, DevColliders C, DevPairs P, const int* list, const int* nm_ptr, int nw,
+                                int2* mb, int* wmstart);
+__global__ void sv_group_color(const int2* mb, const int* wmstart, const int* gworld, const int* wbody, int ngroups,
+                               int* color, int* order, uint64_t* keys, int* seg_start, int* ncolors, int* status);
 __global__ void sv_color_keys(const int* color, const int* nm_ptr, DevColliders C, DevPairs P, const int* mlist,
-                              int world_major, uint64_t* keys, int* vals, int* seg_cnt, int* ncolors);
+                              const int* world_group, uint64_t* keys, int* vals, int* seg_cnt, int* ncolors);
 __global__ void sv_prepare(SvArgs A);
 __global__ void sv_warmstart(SvArgs A, int color);
 __global__ void sv_solve_color(SvArgs A, int color, float beta_dt);

@@ -194,10 +194,121 @@ __global__ __launch_bounds__(1024) void sv_color_finish(const int2* mb, const in
     if (threadIdx.x == 0) atomicOr(status, STATUS_COLOR_STALL);
 }
 
-// Segment key of every manifold: its colour, or (world, colour) when the solver runs one workgroup per
-// world.  Counts per segment and the number of colours are accumulated for the segment table.
+// ------------------------------------------------------------------------------------------------
+// Colouring per world group (fused layout)
+// ------------------------------------------------------------------------------------------------
+
+// The manifold list is in pair-table order (sorted by the lower collider index) and colliders are
+// numbered world by world, so the manifolds of a world form one contiguous range.  One pass writes the
+// dynamic bodies of every manifold (-1 for static or kinematic) and the first manifold of every world
+// (wmstart[nw] = number of manifolds).
+__global__ void sv_world_ranges(DevBodies B, DevColliders C, DevPairs P, const int* list, const int* nm_ptr, int nw,
+                                int2* mb, int* wmstart) {
+    const int nm = *nm_ptr;
+    if (nm == 0) {
+        RP3DG_GRID_STRIDE(w, nw + 1) wmstart[w] = 0;
+        return;
+    }
+    RP3DG_GRID_STRIDE(m, nm) {
+        const int2 c = P.col[list[m]];
+        const int b1 = C.body[c.x], b2 = C.body[c.y];
+        mb[m] = make_int2(dyn_body(B, b1) ? b1 : -1, dyn_body(B, b2) ? b2 : -1);
+        const int w = C.world[c.x];
+        const int wp = m ? C.world[P.col[list[m - 1]].x] : -1;
+        for (int k = wp + 1; k <= w; k++) wmstart[k] = m;
+        if (m == nm - 1)
+            for (int k = w + 1; k <= nw; k++) wmstart[k] = nm;
+    }
+}
+
+// One 64-lane workgroup per world group.  Lane k colours the manifolds of world k of the group
+// greedily in manifold order (smallest colour unused at either dynamic body; worlds share no body, so
+// this equals greedy colouring of the whole group in manifold order), then the group's manifolds are
+// placed colour by colour, in manifold order within a colour: order[row] = manifold, keys[row] =
+// group * kMaxColors + colour, seg_start[group * kMaxColors + c] = first row of colour c.
+// Dynamic LDS: one 64-bit colour mask per body of the largest group.
+__global__ __launch_bounds__(kWave) void sv_group_color(const int2* mb, const int* wmstart, const int* gworld,
+                                                        const int* wbody, int ngroups, int* color, int* order,
+                                                        uint64_t* keys, int* seg_start, int* ncolors, int* status) {
+    extern __shared__ unsigned long long s_mask[];
+    __shared__ int s_hist[kMaxColors];
+    const int g = blockIdx.x;
+    const int lane = threadIdx.x;
+    const int w0 = gworld[g], w1 = gworld[g + 1];
+    const int b0 = wbody[w0], nbg = wbody[w1] - b0;
+    const int ma = wmstart[w0], mz = wmstart[w1];
+    for (int i = lane; i < nbg; i += kWave) s_mask[i] = 0ull;
+    s_hist[lane] = 0;
+    __syncthreads();
+    // greedy colouring, one world per lane; the body pairs are fetched eight manifolds ahead
+    int cmax = -1;
+    if (lane < w1 - w0) {
+        const int s = wmstart[w0 + lane], e = wmstart[w0 + lane + 1];
+        for (int i = s; i < e; i += 8) {
+            int2 pre[8];
+#pragma unroll
+            for (int u = 0; u < 8; u++) pre[u] = i + u < e ? mb[i + u] : make_int2(-1, -1);
+#pragma unroll
+            for (int u = 0; u < 8; u++) {
+                if (i + u >= e) break;
+                const int2 bb = pre[u];
+                unsigned long long used = 0ull;
+                if (bb.x >= 0) used |= s_mask[bb.x - b0];
+                if (bb.y >= 0) used |= s_mask[bb.y - b0];
+                int c = ~used ? __ffsll(~used) - 1 : kMaxColors;
+                if (c >= kMaxColors) {
+                    atomicOr(status, STATUS_COLOR_OVERFLOW);
+                    c = kMaxColors - 1;
+                }
+                const unsigned long long bit = 1ull << c;
+                if (bb.x >= 0) s_mask[bb.x - b0] |= bit;
+                if (bb.y >= 0) s_mask[bb.y - b0] |= bit;
+                color[i + u] = c;
+                cmax = c > cmax ? c : cmax;
+            }
+        }
+    }
+    cmax = wave_max(cmax);
+    if (lane == 0 && cmax >= 0) atomicMax(ncolors, cmax + 1);
+    __syncthreads();   // the colours written above are read back by other lanes of this workgroup
+    for (int i = ma + lane; i < mz; i += kWave) atomicAdd(&s_hist[color[i]], 1);
+    __syncthreads();
+    // segment starts: exclusive prefix of the colour counts
+    const int cnt = s_hist[lane];
+    const int base = ma + wave_incl_scan(cnt) - cnt;
+    seg_start[(size_t)g * kMaxColors + lane] = base;
+    if (g == ngroups - 1 && lane == 0) seg_start[(size_t)ngroups * kMaxColors] = wmstart[gworld[ngroups]];
+    __syncthreads();
+    s_hist[lane] = base;   // running cursor of each colour
+    __syncthreads();
+    // stable placement, 64 manifolds at a time: lanes of equal colour take consecutive rows
+    const unsigned long long below = (1ull << lane) - 1ull;
+    for (int i0 = ma; i0 < mz; i0 += kWave) {
+        const int i = i0 + lane;
+        const int c = i < mz ? color[i] : -1;
+        unsigned long long pending = __ballot(c >= 0);
+        while (pending) {
+            const int leader = __ffsll(pending) - 1;
+            const int cl = __shfl(c, leader, kWave);
+            const unsigned long long same = __ballot(c == cl);
+            if (c == cl) {
+                const int row = s_hist[cl] + __popcll(same & below);
+                order[row] = i;
+                keys[row] = (uint64_t)g * kMaxColors + cl;
+            }
+            pending &= ~same;
+            __syncthreads();
+            if (lane == leader) s_hist[cl] += __popcll(same);
+            __syncthreads();
+        }
+    }
+}
+
+// Segment key of every manifold: its colour, or (world group, colour) when the solver runs one workgroup
+// per group of consecutive worlds (world_group != null).  Counts per segment and the number of colours
+// are accumulated for the segment table.
 __global__ void sv_color_keys(const int* color, const int* nm_ptr, DevColliders C, DevPairs P, const int* mlist,
-                              int world_major, uint64_t* keys, int* vals, int* seg_cnt, int* ncolors) {
+                              const int* world_group, uint64_t* keys, int* vals, int* seg_cnt, int* ncolors) {
     __shared__ int hist[kMaxColors];
     for (int i = threadIdx.x; i < kMaxColors; i += blockDim.x) hist[i] = 0;
     __syncthreads();
@@ -206,9 +317,9 @@ __global__ void sv_color_keys(const int* color, const int* nm_ptr, DevColliders 
     RP3DG_GRID_STRIDE(m, nm) {
         const int c = color[m];
         uint64_t seg = (uint64_t)c;
-        if (world_major) {
-            const int w = C.world[P.col[mlist[m]].x];
-            seg = (uint64_t)w * kMaxColors + c;
+        if (world_group) {
+            const int g = world_group[C.world[P.col[mlist[m]].x]];
+            seg = (uint64_t)g * kMaxColors + c;
             atomicAdd(&seg_cnt[seg], 1);
         } else {
             atomicAdd(&hist[c], 1);
@@ -220,7 +331,7 @@ __global__ void sv_color_keys(const int* color, const int* nm_ptr, DevColliders 
     cmax = wave_max(cmax);
     if (lane_id() == 0 && cmax >= 0) atomicMax(ncolors, cmax + 1);
     __syncthreads();
-    if (!world_major)
+    if (!world_group)
         for (int i = threadIdx.x; i < kMaxColors; i += blockDim.x)
             if (hist[i]) atomicAdd(&seg_cnt[i], hist[i]);
 }

@@ -18,7 +18,18 @@
 
 namespace rp3dg {
 
-__device__ __forceinline__ bool dyn_body(const DevBodies& B, int b) { return (B.flags[b] & BF_DYNAMIC) != 0; }
+__device__ __forceinline__ int body_flags(const DevBodies& B, int b) { return B.flags[b - B.vbase]; }
+__device__ __forceinline__ bool dyn_body(const DevBodies& B, int b) { return (body_flags(B, b) & BF_DYNAMIC) != 0; }
+// lock factors of body b (absolute index; multiplying by 1 is exact, so unit factors are not loaded)
+__device__ __forceinline__ void body_locks(const DevBodies& B, int b, V3& ll, V3& al) {
+    if (body_flags(B, b) & BF_UNIT_LOCKS) {
+        ll = v3(1.0f, 1.0f, 1.0f);
+        al = v3(1.0f, 1.0f, 1.0f);
+    } else {
+        ll = ld3(B.llock, b);
+        al = ld3(B.alock, b);
+    }
+}
 
 // ------------------------------------------------------------------------------------------------
 // Integration (DynamicsSystem.cpp, PhysicsWorld.cpp:310-318)
@@ -220,7 +231,9 @@ __device__ __forceinline__ void sv_warmstart_one(const SvArgs& A, int i) {
         const int u1 = b1 - B.vbase, u2 = b2 - B.vbase;
         V3 v1 = ld3(B.cv, u1), w1 = ld3(B.cw, u1), v2 = ld3(B.cv, u2), w2 = ld3(B.cw, u2);
         const float im1 = S.r1f[i].w, im2 = S.r2f[i].w;
-        const V3 ll1 = ld3(B.llock, b1), al1 = ld3(B.alock, b1), ll2 = ld3(B.llock, b2), al2 = ld3(B.alock, b2);
+        V3 ll1, al1, ll2, al2;
+        body_locks(B, b1, ll1, al1);
+        body_locks(B, b2, ll2, al2);
         const int np = S.npts[i];
         const M3 I1 = body_iw(B, b1), I2 = body_iw(B, b2);
         bool any = false;
@@ -349,7 +362,9 @@ __device__ __forceinline__ void solve_manifold(const SvArgs& A, int i, float bet
     V3 v1 = ld3(B.cv, u1), w1 = ld3(B.cw, u1), v2 = ld3(B.cv, u2), w2 = ld3(B.cw, u2);
     V3 sv1 = ld3(B.sv, u1), sw1 = ld3(B.sw, u1), sv2 = ld3(B.sv, u2), sw2 = ld3(B.sw, u2);
     const float im1 = S.r1f[i].w, im2 = S.r2f[i].w;
-    const V3 ll1 = ld3(B.llock, b1), al1 = ld3(B.alock, b1), ll2 = ld3(B.llock, b2), al2 = ld3(B.alock, b2);
+    V3 ll1, al1, ll2, al2;
+    body_locks(B, b1, ll1, al1);
+    body_locks(B, b2, ll2, al2);
     const int np = S.npts[i];
     const M3 I1 = body_iw(B, b1), I2 = body_iw(B, b2);
     float sumPen = 0.0f;

@@ -36,12 +36,12 @@ class Counts(C.Structure):
     _fields_ = [(n, C.c_int64) for n in ("nb_bodies", "nb_colliders", "nb_joints", "nb_worlds", "nb_pairs",
                                          "nb_manifolds", "nb_contact_points", "nb_colors", "status")] + \
                [("nb_tests", C.c_int64 * 7), ("total_contact_points", C.c_int64), ("steps", C.c_int64),
-                ("nb_new_points", C.c_int64)]
+                ("nb_new_points", C.c_int64), ("nb_box_tests", C.c_int64)]
 
 
 class BatchOptions(C.Structure):
     _fields_ = [("solver", C.c_int32), ("fused_block", C.c_int32), ("pairs_per_collider", C.c_int32),
-                ("reserved", C.c_int32 * 5)]
+                ("group_bodies", C.c_int32), ("reserved", C.c_int32 * 4)]
 
 
 SOLVER_MODES = {"auto": 0, "per_color": 1, "fused": 2}
@@ -69,7 +69,7 @@ EXPORTED = [
     "rp3dg_batch_get_iteration_stats", "rp3dg_last_error", "rp3dg_version", "rp3dg_mlp_create", "rp3dg_mlp_destroy",
     "rp3dg_mlp_set_params", "rp3dg_mlp_get_params", "rp3dg_mlp_set_normalization", "rp3dg_mlp_forward",
     "rp3dg_mlp_gradient", "rp3dg_mlp_train", "rp3dg_batch_set_warm_start", "rp3dg_batch_set_logging",
-    "rp3dg_batch_get_log", "rp3dg_batch_autotune", "rp3dg_world_get_body_state",
+    "rp3dg_batch_get_log", "rp3dg_batch_autotune", "rp3dg_batch_get_tune_table", "rp3dg_world_get_body_state",
 ]
 
 MLP_PARAMS = 20099
@@ -144,6 +144,7 @@ def load():
         "rp3dg_batch_set_logging": (C.c_int, [vp, i64]),
         "rp3dg_batch_get_log": (i64, [vp, fptr, fptr, i64]),
         "rp3dg_batch_autotune": (C.c_int, [vp, f, i32, iptr, C.POINTER(C.c_double)]),
+        "rp3dg_batch_get_tune_table": (i32, [vp, iptr, C.POINTER(C.c_double), i32]),
         "rp3dg_world_get_body_state": (C.c_int, [vp, i32, fptr, fptr, fptr, fptr]),
         "rp3dg_last_error": (C.c_char_p, []),
         "rp3dg_version": (C.c_char_p, []),
@@ -225,7 +226,7 @@ def world_from_scene(sc: S.Scene):
 class Batch:
     """One or many worlds stepped together on the GPU."""
 
-    def __init__(self, scenes, solver="auto", fused_block=0, pairs_per_collider=0):
+    def __init__(self, scenes, solver="auto", fused_block=0, pairs_per_collider=0, group_bodies=0):
         L = load()
         if isinstance(scenes, S.Scene):
             scenes = [scenes]
@@ -236,6 +237,7 @@ class Batch:
         opt.solver = SOLVER_MODES[solver]
         opt.fused_block = int(fused_block)
         opt.pairs_per_collider = int(pairs_per_collider)
+        opt.group_bodies = int(group_bodies)
         self.h = L.rp3dg_batch_create_ex(arr, len(self.worlds), C.byref(opt))
         if not self.h:
             raise RuntimeError("rp3dg_batch_create failed: " + last_error())
@@ -333,6 +335,14 @@ class Batch:
         _check(load().rp3dg_batch_autotune(self.h, float(dt), int(steps_per_trial), C.byref(c), C.byref(ms)),
                "autotune")
         return c.value, ms.value
+
+    def tune_table(self):
+        """(choice, ms per step) of every trial of the last autotune call."""
+        ch = (C.c_int32 * 16)()
+        ms = (C.c_double * 16)()
+        n = load().rp3dg_batch_get_tune_table(self.h, ch, ms, 16)
+        _check(min(n, 0), "get_tune_table")
+        return [(ch[i], ms[i]) for i in range(min(n, 16))]
 
     def set_warm_start(self, mode, mlp=None):
         m = {"plain": 0, "learned": 1}[mode]

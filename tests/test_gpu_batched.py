@@ -26,10 +26,10 @@ def _world_views(dev, ct, order):
     return out
 
 
-def _run(scenes, steps, solver, tol=1e-4):
+def _run(scenes, steps, solver, tol=1e-4, group_bodies=0):
     from oracle.ref import RefWorld
     from reactphysics3d_amd.lib import Batch
-    dev = Batch(scenes, solver=solver)
+    dev = Batch(scenes, solver=solver, group_bodies=group_bodies)
     refs = [RefWorld(sc) for sc in scenes]
     order = dev.joint_order()
     for (_, jo), ref in zip(_world_views(dev, {"colliders": np.zeros((0, 2), np.int32)}, order), refs):
@@ -57,10 +57,11 @@ def _run(scenes, steps, solver, tol=1e-4):
     return dev, worst
 
 
-@pytest.mark.parametrize("solver", ["fused", "per_color"])
-def test_uneven_worlds_match_reference(solver):
+@pytest.mark.parametrize("solver,group_bodies", [("fused", 0), ("fused", 1), ("per_color", 0)])
+def test_uneven_worlds_match_reference(solver, group_bodies):
+    # fused, 0: worlds packed into groups of up to 256 bodies; fused, 1: one world per workgroup
     scenes = [S.batched_world(w, seed=100, min_bodies=8, max_bodies=48, max_joints=8) for w in range(12)]
-    dev, worst = _run(scenes, 60, solver)
+    dev, worst = _run(scenes, 60, solver, group_bodies=group_bodies)
     c = dev.counts()
     print(solver, "worst", worst, c)
     assert c["nb_worlds"] == 12 and c["nb_manifolds"] > 0 and c["nb_joints"] > 0
@@ -101,3 +102,31 @@ def test_residual_mode_converges():
     dev.set_residual_mode(0.0)
     dev.step(DT)
     assert dev.iteration_stats()["max_iterations"] == 10
+
+
+@pytest.mark.parametrize("residual", [0.0, 1e-4])
+def test_world_groups_do_not_change_results(residual):
+    """Packing worlds into one workgroup changes nothing: per-world results, iteration counts and
+    residuals are identical to one world per workgroup, in fixed and residual mode (a converged world
+    of a group skips the sweeps its neighbours still run)."""
+    from reactphysics3d_amd.lib import Batch
+    scenes = [S.batched_world(w, seed=21, min_bodies=8, max_bodies=160, max_joints=8) for w in range(40)]
+    runs = []
+    for gb in (1, 128, 512):
+        dev = Batch(scenes, solver="fused", group_bodies=gb)
+        if residual > 0:
+            dev.set_residual_mode(residual, 150)
+        stats = []
+        for _ in range(4):
+            dev.step(DT, 5)
+            stats.append(dev.iteration_stats())
+        runs.append((dev.bodies(), stats, dev.counts()))
+    base = runs[0]
+    for bodies, stats, counts in runs[1:]:
+        for a, b in zip(base[0], bodies):
+            assert np.array_equal(a, b), "grouped worlds differ from single-world workgroups"
+        assert stats == base[1]
+        assert counts == base[2]
+    if residual > 0:
+        assert base[1][-1]["max_iterations"] < 150 or base[1][-1]["max_residual"] >= residual
+        assert base[1][-1]["mean_iterations"] < base[1][-1]["max_iterations"]
