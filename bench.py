@@ -91,6 +91,7 @@ BYTE_MODEL = {
 }
 
 KERNELS = ["bp_bounds", "bp_morton", "bp_sort", "bp_karras", "bp_fit", "bp_world_roots", "bp_traverse", "bp_copy_pairs",
+           "bp_walk_again", "bp_world_scan",
            "bp_merge",
            "np_classify", "np_sphere_sphere", "np_sphere_capsule", "np_capsule_capsule", "np_sphere_poly",
            "np_capsule_poly", "np_box_box", "np_poly_poly", "sv_flag", "sv_compact", "sv_adj_count", "sv_adj_fill",
@@ -297,7 +298,7 @@ def main():
         d.sync()
         t0 = time.perf_counter()
         if graph:
-            chunk = 10
+            chunk = max(1, min(10, args.steps))
             for _ in range(args.steps // chunk):
                 d.step_graph(DT, chunk)
                 its.append(d.iteration_stats()["mean_iterations"])

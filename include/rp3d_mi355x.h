@@ -143,6 +143,10 @@ rp3dg_batch* rp3dg_batch_create(rp3dg_world* const* worlds, int32_t nb_worlds);
 /* Solver layout of a batch. */
 #define RP3DG_SOLVER_AUTO 0   /* FUSED when every world has at most 4096 bodies, else PER_COLOR */
 #define RP3DG_SOLVER_PER_COLOR 1  /* one launch per colour over all worlds (large worlds) */
+#define RP3DG_BROADPHASE_AUTO 0        /* world scan when it applies, else the tree */
+#define RP3DG_BROADPHASE_TREE 1        /* linear BVH over all colliders, rebuilt every step */
+#define RP3DG_BROADPHASE_WORLD_SCAN 2  /* fused layout, worlds of <= 1024 colliders: every collider tests its
+                                          world's boxes in LDS (no tree) */
 #define RP3DG_SOLVER_FUSED 2  /* one workgroup per group of small worlds runs the whole solver phase (worlds up to 4096 bodies) */
 
 typedef struct rp3dg_batch_options {
@@ -153,7 +157,8 @@ typedef struct rp3dg_batch_options {
     int32_t group_bodies;     /* fused solver: consecutive worlds are packed into one workgroup up to this many
                                  bodies together (at most 64 worlds, 512 bodies; 0 = 256, 1 = one world per
                                  workgroup); results do not depend on it */
-    int32_t reserved[4];
+    int32_t broadphase;       /* RP3DG_BROADPHASE_*; the pair set does not depend on it */
+    int32_t reserved[3];
 } rp3dg_batch_options;
 /* As rp3dg_batch_create, with options (NULL = defaults). */
 rp3dg_batch* rp3dg_batch_create_ex(rp3dg_world* const* worlds, int32_t nb_worlds, const rp3dg_batch_options* options);

@@ -66,6 +66,8 @@ struct rp3dg_batch {
     int *d_gworld = nullptr, *d_world_group = nullptr;
     int* d_wmstart = nullptr;            // [nworlds + 1] first manifold of each world (fused colouring)
     int max_group_bodies = 0;
+    bool world_scan = false;             // broadphase by per-world box scans in LDS instead of the tree
+    int max_group_colliders = 0;
     float4 *res_pv = nullptr, *res_pw = nullptr;
     int* world_iters = nullptr;
     float* world_res = nullptr;
@@ -97,12 +99,13 @@ struct rp3dg_batch {
     float4 *d_hverts = nullptr, *d_hnormals = nullptr, *d_hbounds = nullptr;
     int4* d_hedges = nullptr;
     int *d_hfedge = nullptr, *d_hfverts = nullptr, *d_hfvidx = nullptr;
-    uint64_t* d_nocol = nullptr;
-    int nnocol = 0;
+    rp3dg::NoColLists nocol_lists{};          // per-body partners that must not collide
+    int *d_nocol_offs = nullptr, *d_nocol_list = nullptr;
 
     // broadphase scratch
     uint64_t *bp_keys = nullptr, *bp_ktmp = nullptr, *bp_out = nullptr;
     uint64_t* bp_slab = nullptr;       // first kTravSlab partners of every collider (first traversal pass)
+    int* bp_walk = nullptr;            // colliders walked again after the scan (more partners than the slab)
     int *bp_vals = nullptr, *bp_vtmp = nullptr;
     int2 *bp_child = nullptr, *bp_range = nullptr;
     int *bp_parent = nullptr, *bp_root = nullptr, *bp_count = nullptr, *bp_offs = nullptr;

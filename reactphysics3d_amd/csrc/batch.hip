@@ -38,8 +38,13 @@ __global__ void bp_internal(int n, const int2* range, float4* nmin, float4* nmax
                             const float4* b3max);
 __global__ void bp_world_roots(const uint64_t* k, int n, const int2* range, int* root);
 __global__ void bp_traverse(DevColliders C, const int* sorted, int n, const int2* child, const float4* nmin,
-                            const float4* nmax, const int* root, const uint64_t* nc, int nnc, int* count,
-                            const int* offset, uint64_t* out_keys, uint64_t* slab, int capacity, int pass, int* status);
+                            const float4* nmax, const int* root, NoColLists nc, int* count,
+                            const int* offset, uint64_t* out_keys, uint64_t* slab, int capacity, int pass,
+                            int* walk_list, int* walk_count, int* status);
+__global__ void bp_copy_slabs(DevColliders C, int n, const int* count, const int* offset, const uint64_t* slab,
+                              uint64_t* out_keys, int capacity, int* status);
+__global__ void bp_world_scan(DevColliders C, const int* gworld, const int* wcoll, const int* block_group, NoColLists L,
+                              int* count, const int* offset, uint64_t* out_keys, int capacity, int pass, int* status);
 __global__ void bp_merge(DevColliders C, const uint64_t* keys, const int* nkeys_ptr, int capacity, DevPairs O,
                          const int* nold_ptr, DevPairs N, int* nnew_ptr, int* nb_pairs_ptr);
 
@@ -397,6 +402,10 @@ int rp3dg_batch::upload(rp3dg_world* const* ws, int nw, const rp3dg_batch_option
     }
     if (o.pairs_per_collider < 0) { set_error("pairs_per_collider must be >= 0"); return RP3DG_EINVAL; }
     if (o.group_bodies < 0) { set_error("group_bodies must be >= 0"); return RP3DG_EINVAL; }
+    if (o.broadphase < RP3DG_BROADPHASE_AUTO || o.broadphase > RP3DG_BROADPHASE_WORLD_SCAN) {
+        set_error("invalid broadphase");
+        return RP3DG_EINVAL;
+    }
     // host SoA
     std::vector<float4> x, q, p, v, w, force, torque, ii, com, llock, alock;
     std::vector<int> flags, bworld;
@@ -544,8 +553,24 @@ int rp3dg_batch::upload(rp3dg_world* const* ws, int nw, const rp3dg_batch_option
     group_world.push_back(nw);
     ngroups = (int)group_world.size() - 1;
     max_group_bodies = 0;
-    for (int g = 0; g < ngroups; g++)
+    max_group_colliders = 0;
+    for (int g = 0; g < ngroups; g++) {
         max_group_bodies = std::max(max_group_bodies, body_base[group_world[g + 1]] - body_base[group_world[g]]);
+        max_group_colliders =
+            std::max(max_group_colliders, collider_base[group_world[g + 1]] - collider_base[group_world[g]]);
+    }
+    {
+        int max_world_colliders = 0;
+        for (int wi = 0; wi < nw; wi++)
+            max_world_colliders = std::max(max_world_colliders, collider_base[wi + 1] - collider_base[wi]);
+        // 32 B of LDS per collider of a group
+        const bool fits = fused && max_world_colliders <= 1024 && max_group_colliders <= 2048;
+        if (o.broadphase == RP3DG_BROADPHASE_WORLD_SCAN && !fits) {
+            set_error("the world-scan broadphase needs the fused layout and worlds of at most 1024 colliders");
+            return RP3DG_EINVAL;
+        }
+        world_scan = fits && o.broadphase != RP3DG_BROADPHASE_TREE;
+    }
     user_block = o.fused_block;
     if (o.fused_block) fused_block = o.fused_block;
     else fused_block = max_world_bodies <= 128 ? 64 : (max_world_bodies <= 1024 ? 128 : 256);
@@ -605,9 +630,28 @@ int rp3dg_batch::upload(rp3dg_world* const* ws, int nw, const rp3dg_batch_option
     if (!hfverts.empty()) HIPCHECK(up(d_hfverts, hfverts.data(), hfverts.size() * 4));
     if (!hfvidx.empty()) HIPCHECK(up(d_hfvidx, hfvidx.data(), hfvidx.size() * 4));
     HIPCHECK(up(d_hbounds, hbounds.data(), hbounds.size() * 16));
-    nnocol = (int)nocol.size();
-    d_nocol = dalloc<uint64_t>(this, std::max<size_t>(1, nocol.size()));
-    if (nnocol) HIPCHECK(up(d_nocol, nocol.data(), nocol.size() * 8));
+    {
+        // per-body lists of the body pairs that must not collide, both directions
+        std::vector<int> offs(nb + 1, 0), list(2 * nocol.size());
+        for (uint64_t k : nocol) {
+            offs[(int)(k >> 32) + 1]++;
+            offs[(int)(uint32_t)k + 1]++;
+        }
+        for (int i = 0; i < nb; i++) offs[i + 1] += offs[i];
+        std::vector<int> cur(offs.begin(), offs.end() - 1);
+        for (uint64_t k : nocol) {
+            const int a = (int)(k >> 32), c = (int)(uint32_t)k;
+            list[cur[a]++] = c;
+            list[cur[c]++] = a;
+        }
+        d_nocol_offs = dalloc<int>(this, nb + 1);
+        d_nocol_list = dalloc<int>(this, std::max<size_t>(1, list.size()));
+        if (!d_nocol_list) { set_error("device allocation failed"); return RP3DG_EHIP; }
+        HIPCHECK(up(d_nocol_offs, offs.data(), offs.size() * 4));
+        if (!list.empty()) HIPCHECK(up(d_nocol_list, list.data(), list.size() * 4));
+        nocol_lists.offs = d_nocol_offs;
+        nocol_lists.list = d_nocol_list;
+    }
 
     // box topology
     HalfEdgeMesh box;
@@ -637,6 +681,7 @@ int rp3dg_batch::upload(rp3dg_world* const* ws, int nw, const rp3dg_batch_option
     bp_count = dalloc<int>(this, nc + 1); bp_offs = dalloc<int>(this, nc + 1);
     bp_out = dalloc<uint64_t>(this, pair_cap);
     bp_slab = dalloc<uint64_t>(this, (size_t)nc * kTravSlab);
+    bp_walk = dalloc<int>(this, nc);
     np_lists = dalloc<int>(this, (size_t)ALGO_COUNT * pair_cap);
     np_counts = dalloc<int>(this, ALGO_COUNT);
     m_flag = dalloc<int>(this, pair_cap); m_offs = dalloc<int>(this, pair_cap + 1);
@@ -801,37 +846,53 @@ int rp3dg_batch::enqueue_step(float dt, bool capturing) {
     fill_int<<<1, 256, 0, stream>>>((int*)counters, (int)(sizeof(DevCounters) / sizeof(int)), 0);
 
     // ---------------- broadphase (CollisionDetectionSystem::computeBroadPhase, CD.cpp:103-121)
-    const int nparts = std::min(256, grid_for(nc));
-    LAUNCH_T("bp_bounds", nparts, 256, bp_bounds, C, bp_partial);
-    LAUNCH_T("bp_morton", grid_for(nc), 256, bp_morton, C, bp_partial, nparts, bp_keys, bp_vals);
     uint64_t* k = bp_keys;
     int* vls = bp_vals;
-    timer_begin("bp_sort");
-    r = sort(&k, &vls, bp_ktmp, bp_vtmp, nc, nullptr, 30 + world_bits, stream);
-    timer_end();
-    if (r) { set_error("radix sort overflow"); return r; }
-    if (nc > 1) LAUNCH_T("bp_karras", grid_for(nc - 1), 256, bp_karras, k, nc, bp_child, bp_parent, bp_range);
-    {
-        const int nb1 = (nc + 63) / 64, nb2 = (nb1 + 63) / 64, nb3 = (nb2 + 63) / 64;
-        timer_begin("bp_fit");
-        bp_leaves<<<grid_for(nc), 256, 0, stream>>>(C, vls, nc, bp_nmin, bp_nmax, fit_min, fit_max);
-        bp_blocks<<<grid_for(nb1), 256, 0, stream>>>(fit_min, fit_max, nb1, fit_min + nb1, fit_max + nb1);
-        bp_blocks<<<grid_for(nb2), 256, 0, stream>>>(fit_min + nb1, fit_max + nb1, nb2, fit_min + nb1 + nb2,
-                                                      fit_max + nb1 + nb2);
-        if (nc > 1)
-            bp_internal<<<grid_for(nc - 1), 256, 0, stream>>>(nc, bp_range, bp_nmin, bp_nmax, fit_min, fit_max,
-                                                              fit_min + nb1, fit_max + nb1, fit_min + nb1 + nb2,
-                                                              fit_max + nb1 + nb2);
+    const size_t scan_lds = 2 * sizeof(float4) * max_group_colliders;
+    if (world_scan) {
+        // small worlds: every dynamic collider tests the boxes of its world in LDS
+        LAUNCH_TS("bp_world_scan", ngroups, 256, scan_lds, bp_world_scan, C, d_gworld, d_wcoll, d_block_world,
+                  nocol_lists, bp_count, bp_offs, bp_out, pair_cap, 0, dstatus);
+    } else {
+        const int nparts = std::min(256, grid_for(nc));
+        LAUNCH_T("bp_bounds", nparts, 256, bp_bounds, C, bp_partial);
+        LAUNCH_T("bp_morton", grid_for(nc), 256, bp_morton, C, bp_partial, nparts, bp_keys, bp_vals);
+        timer_begin("bp_sort");
+        r = sort(&k, &vls, bp_ktmp, bp_vtmp, nc, nullptr, 30 + world_bits, stream);
         timer_end();
-        (void)nb3;
+        if (r) { set_error("radix sort overflow"); return r; }
+        if (nc > 1) LAUNCH_T("bp_karras", grid_for(nc - 1), 256, bp_karras, k, nc, bp_child, bp_parent, bp_range);
+        {
+            const int nb1 = (nc + 63) / 64, nb2 = (nb1 + 63) / 64;
+            timer_begin("bp_fit");
+            bp_leaves<<<grid_for(nc), 256, 0, stream>>>(C, vls, nc, bp_nmin, bp_nmax, fit_min, fit_max);
+            bp_blocks<<<grid_for(nb1), 256, 0, stream>>>(fit_min, fit_max, nb1, fit_min + nb1, fit_max + nb1);
+            bp_blocks<<<grid_for(nb2), 256, 0, stream>>>(fit_min + nb1, fit_max + nb1, nb2, fit_min + nb1 + nb2,
+                                                          fit_max + nb1 + nb2);
+            if (nc > 1)
+                bp_internal<<<grid_for(nc - 1), 256, 0, stream>>>(nc, bp_range, bp_nmin, bp_nmax, fit_min, fit_max,
+                                                                  fit_min + nb1, fit_max + nb1, fit_min + nb1 + nb2,
+                                                                  fit_max + nb1 + nb2);
+            timer_end();
+        }
+        LAUNCH_T("bp_world_roots", grid_for(2 * nc - 1), 256, bp_world_roots, k, nc, bp_range, bp_root);
+        LAUNCH_T("bp_traverse", grid_for(nc, kTravBlock, 1 << 20), kTravBlock, bp_traverse, C, vls, nc, bp_child,
+                 bp_nmin, bp_nmax, bp_root, nocol_lists, bp_count, bp_offs, bp_out, bp_slab, pair_cap, 0, bp_walk,
+                 &counters->nb_walk_again, dstatus);
     }
-    LAUNCH_T("bp_world_roots", grid_for(2 * nc - 1), 256, bp_world_roots, k, nc, bp_range, bp_root);
-    LAUNCH_T("bp_traverse", grid_for(nc, kTravBlock, 1 << 20), kTravBlock, bp_traverse, C, vls, nc, bp_child, bp_nmin,
-             bp_nmax, bp_root, d_nocol, nnocol, bp_count, bp_offs, bp_out, bp_slab, pair_cap, 0, dstatus);
     r = scan(bp_count, nc, nullptr, bp_offs, stream);
     if (r) { set_error("scan overflow"); return r; }
-    LAUNCH_T("bp_copy_pairs", grid_for(nc, kTravBlock, 1 << 20), kTravBlock, bp_traverse, C, vls, nc, bp_child, bp_nmin,
-             bp_nmax, bp_root, d_nocol, nnocol, bp_count, bp_offs, bp_out, bp_slab, pair_cap, 1, dstatus);
+    if (world_scan) {
+        LAUNCH_TS("bp_world_scan", ngroups, 256, scan_lds, bp_world_scan, C, d_gworld, d_wcoll, d_block_world,
+                  nocol_lists, bp_count, bp_offs, bp_out, pair_cap, 1, dstatus);
+    } else {
+        LAUNCH_T("bp_copy_pairs", grid_for(nc * kTravSlab, 256, 1 << 20), 256, bp_copy_slabs, C, nc, bp_count, bp_offs,
+                 bp_slab, bp_out, pair_cap, dstatus);
+        const int hw = capturing ? nc : std::min(nc, last.nb_walk_again + last.nb_walk_again / 4 + 256);
+        LAUNCH_T("bp_walk_again", grid_for(hw, kTravBlock, 1 << 20), kTravBlock, bp_traverse, C, vls, nc, bp_child,
+                 bp_nmin, bp_nmax, bp_root, nocol_lists, bp_count, bp_offs, bp_out, bp_slab, pair_cap, 1, bp_walk,
+                 &counters->nb_walk_again, dstatus);
+    }
     const int hp = capturing ? pair_cap : std::min(pair_cap, hint_pairs + hint_pairs / 4 + 1024);
     const int pg = grid_for(hp);
     LAUNCH_T("bp_merge", pg, 256, bp_merge, C, bp_out, bp_offs + nc, pair_cap, O, old_count, N, new_count,

@@ -12,9 +12,11 @@
 //   bp_bounds -> bp_morton -> radix sort (LDS, stable, 8-bit digits)
 //   -> bp_karras (Karras 2012 radix tree) -> node boxes as unions over leaf ranges
 //      (bp_leaves, bp_blocks, bp_internal; no inter-thread synchronisation)
-//   -> bp_traverse (count) -> scan -> bp_traverse (write, per-collider sorted
-//      segments) -> bp_merge (carry orientation, temporal-coherence record and
-//      contact cache of surviving pairs; orient new pairs like the reference).
+//   -> bp_traverse (count, first 8 partners sorted into a slab) -> scan
+//   -> bp_copy_slabs (slab -> per-collider sorted segment) + bp_traverse again
+//      for the few colliders with more partners
+//   -> bp_merge (carry orientation, temporal-coherence record and contact cache
+//      of surviving pairs; orient new pairs like the reference).
 #include <hip/hip_runtime.h>
 
 #include "kcommon.h"
@@ -327,32 +329,29 @@ __device__ __forceinline__ bool aabb_overlap(float4 amn, float4 amx, float4 bmn,
     return true;
 }
 
-__device__ __forceinline__ bool no_collision_pair(const uint64_t* nc, int nnc, uint64_t bk) {
-    int lo = 0, hi = nnc - 1;
-    while (lo <= hi) {
-        const int mid = (lo + hi) >> 1;
-        const uint64_t v = nc[mid];
-        if (v == bk) return true;
-        if (v < bk) lo = mid + 1;
-        else hi = mid - 1;
-    }
+// Body pairs that must not collide (joints with collideConnected = false): per-body partner lists
+__device__ __forceinline__ bool no_collision(const NoColLists& L, int ba, int bb) {
+    const int e = L.offs[ba + 1];
+    for (int k = L.offs[ba]; k < e; k++)
+        if (L.list[k] == bb) return true;
     return false;
 }
 
-constexpr int kTravStack = 64;   // traversal stack depth (per thread)
-
-// Pair filter of CollisionDetectionSystem::updateOverlappingPairs (:284-327).
-__device__ __forceinline__ bool pair_allowed(const DevColliders& C, int a, int b, const uint64_t* nc, int nnc) {
-    const int ba = C.body[a], bb = C.body[b];
-    if (ba == bb) return false;
-    if (nnc > 0) {
-        const uint64_t bk = ((uint64_t)(uint32_t)min(ba, bb) << 32) | (uint32_t)max(ba, bb);
-        if (no_collision_pair(nc, nnc, bk)) return false;
-    }
-    const uint32_t fa = C.filter[a], fb = C.filter[b];
+__device__ __forceinline__ bool filters_pass(uint32_t fa, uint32_t fb) {
     const uint32_t cata = fa & 0xffffu, maska = fa >> 16, catb = fb & 0xffffu, maskb = fb >> 16;
     return (maska & catb) != 0 && (cata & maskb) != 0;
 }
+
+// Pair filter of CollisionDetectionSystem::updateOverlappingPairs (CD.cpp:284-327): different bodies, no
+// joint excluding the pair, collision category / mask
+__device__ __forceinline__ bool pair_allowed(const DevColliders& C, int a, int b, const NoColLists& L) {
+    const int ba = C.body[a], bb = C.body[b];
+    if (ba == bb) return false;
+    if (no_collision(L, ba, bb)) return false;
+    return filters_pass(C.filter[a], C.filter[b]);
+}
+
+constexpr int kTravStack = 64;   // traversal stack depth (per thread)
 
 // Each overlapping pair is emitted by exactly one of its colliders (the pair key's high word):
 //   moving vs moving -> the lower collider index; moving vs static -> the moving one;
@@ -367,31 +366,18 @@ __device__ __forceinline__ bool pair_allowed(const DevColliders& C, int a, int b
 // again only for colliders with more partners than the slab holds.
 __global__ __launch_bounds__(kTravBlock) void bp_traverse(DevColliders C, const int* sorted, int n, const int2* child,
                                                           const float4* nmin, const float4* nmax, const int* root,
-                                                          const uint64_t* nc, int nnc, int* count, const int* offset,
+                                                          NoColLists nc, int* count, const int* offset,
                                                           uint64_t* out_keys, uint64_t* slab, int capacity, int pass,
-                                                          int* status) {
-    RP3DG_GRID_STRIDE(s, n) {
-        const int c = pass ? s : sorted[s];
+                                                          int* walk_list, int* walk_count, int* status) {
+    const int nw = pass ? *walk_count : n;
+    RP3DG_GRID_STRIDE(s, nw) {
+        const int c = pass ? walk_list[s] : sorted[s];
         const int base = pass ? offset[c] : 0;
-        if (C.is_static[c]) {
-            const int b = C.ss_offs[c], e = C.ss_offs[c + 1];
-            if (pass) {
-                for (int k = b; k < e; k++)
-                    if (base + k - b < capacity) out_keys[base + k - b] = ((uint64_t)(uint32_t)c << 32) | (uint32_t)C.ss_list[k];
-                if (base + e - b > capacity) atomicOr(status, STATUS_PAIR_OVERFLOW);
-            }
-            if (!pass) count[c] = e - b;
+        if (!pass && C.is_static[c]) {
+            count[c] = C.ss_offs[c + 1] - C.ss_offs[c];
             continue;
         }
         uint64_t* sl = slab + (size_t)c * kTravSlab;
-        if (pass) {
-            const int total = count[c];
-            if (total <= kTravSlab) {
-                if (base + total > capacity) { atomicOr(status, STATUS_PAIR_OVERFLOW); continue; }
-                for (int k = 0; k < total; k++) out_keys[base + k] = sl[k];
-                continue;
-            }
-        }
         const float4 amn = C.amin[c], amx = C.amax[c];
         int stack[kTravStack];
         int sp = 0;
@@ -404,7 +390,7 @@ __global__ __launch_bounds__(kTravBlock) void bp_traverse(DevColliders C, const 
                 const int o = sorted[node - (n - 1)];
                 if (o == c) continue;
                 if (!C.is_static[o] && o < c) continue;
-                if (!pair_allowed(C, c, o, nc, nnc)) continue;
+                if (!pair_allowed(C, c, o, nc)) continue;
                 const uint64_t key = ((uint64_t)(uint32_t)c << 32) | (uint32_t)o;
                 if (pass) {
                     int pos = base + cnt;
@@ -433,6 +419,105 @@ __global__ __launch_bounds__(kTravBlock) void bp_traverse(DevColliders C, const 
                 stack[sp++] = ch.x;
                 stack[sp++] = ch.y;
             }
+        }
+        if (!pass) {
+            count[c] = cnt;
+            if (cnt > kTravSlab) walk_list[atomicAdd(walk_count, 1)] = c;   // walked again after the scan
+        }
+    }
+}
+
+// Second traversal pass for the colliders with at most kTravSlab partners: their sorted partners are
+// in the slab; eight lanes per collider copy them to the collider's segment of the key array.  Static
+// colliders copy their static-static partner list.
+__global__ void bp_copy_slabs(DevColliders C, int n, const int* count, const int* offset, const uint64_t* slab,
+                              uint64_t* out_keys, int capacity, int* status) {
+    RP3DG_GRID_STRIDE(t, n * kTravSlab) {
+        const int c = t / kTravSlab, k = t % kTravSlab;
+        const int base = offset[c];
+        if (C.is_static[c]) {
+            if (k != 0) continue;
+            const int b = C.ss_offs[c], e = C.ss_offs[c + 1];
+            for (int q = b; q < e; q++)
+                if (base + q - b < capacity) out_keys[base + q - b] = ((uint64_t)(uint32_t)c << 32) | (uint32_t)C.ss_list[q];
+            if (base + e - b > capacity) atomicOr(status, STATUS_PAIR_OVERFLOW);
+            continue;
+        }
+        const int total = count[c];
+        if (total > kTravSlab || k >= total) continue;
+        if (base + total > capacity) {
+            if (k == 0) atomicOr(status, STATUS_PAIR_OVERFLOW);
+            continue;
+        }
+        out_keys[base + k] = slab[(size_t)c * kTravSlab + k];
+    }
+}
+
+// ---------------------------------------------------------------------------------------------
+// Batches of small worlds: no tree.  One workgroup per world group stages the fat AABBs of its
+// colliders in LDS and every dynamic collider tests the boxes of all colliders of its own world in
+// index order, so its partners come out sorted; the scan runs twice (count, then write after the
+// prefix sum of the counts).  The pair set is the tree query's (the same
+// AABB::testCollision on the same boxes, the same emitter rule and filters); the cost is
+// sum over worlds of (colliders)^2 box tests, cheaper than building and walking a tree per world up to
+// about a thousand colliders.  LDS per collider: min (w = body | static << 31), max (w = filter bits).
+// ---------------------------------------------------------------------------------------------
+// pass 0 writes the partner count of every collider, pass 1 (after the scan) its sorted keys.
+__global__ __launch_bounds__(256) void bp_world_scan(DevColliders C, const int* gworld, const int* wcoll,
+                                                     const int* block_group, NoColLists L, int* count,
+                                                     const int* offset, uint64_t* out_keys, int capacity, int pass,
+                                                     int* status) {
+    extern __shared__ float4 s_box[];   // [group colliders] min boxes, then [group colliders] max boxes
+    const int g = block_group[blockIdx.x];
+    const int gc0 = wcoll[gworld[g]], gc1 = wcoll[gworld[g + 1]];
+    const int ncg = gc1 - gc0;
+    float4* smn = s_box;
+    float4* smx = s_box + ncg;
+    for (int i = threadIdx.x; i < ncg; i += blockDim.x) {
+        const int c = gc0 + i;
+        float4 mn = C.amin[c], mx = C.amax[c];
+        mn.w = __int_as_float(C.body[c] | (C.is_static[c] ? (int)0x80000000u : 0));
+        mx.w = __uint_as_float(C.filter[c]);
+        smn[i] = mn;
+        smx[i] = mx;
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < ncg; i += blockDim.x) {
+        const int c = gc0 + i;
+        const float4 amn = smn[i], amx = smx[i];
+        const int bc = __float_as_int(amn.w);
+        const int base = pass ? offset[c] : 0;
+        if (bc < 0) {   // static colliders do not query (their static-static pairs are listed at upload)
+            const int b = C.ss_offs[c], e = C.ss_offs[c + 1];
+            if (!pass) {
+                count[c] = e - b;
+            } else {
+                for (int q = b; q < e; q++)
+                    if (base + q - b < capacity) out_keys[base + q - b] = ((uint64_t)(uint32_t)c << 32) | (uint32_t)C.ss_list[q];
+                if (base + e - b > capacity) atomicOr(status, STATUS_PAIR_OVERFLOW);
+            }
+            continue;
+        }
+        const int w = C.world[c];
+        const int j0 = wcoll[w] - gc0, j1 = wcoll[w + 1] - gc0;
+        const uint32_t fc = __float_as_uint(amx.w);
+        const int n0 = L.offs[bc], n1 = L.offs[bc + 1];   // bodies this collider's body must not collide with
+        int cnt = 0;
+        for (int j = j0; j < j1; j++) {
+            const float4 bmn = smn[j], bmx = smx[j];
+            const int bo = __float_as_int(bmn.w);
+            if (j == i || (bo >= 0 && j < i)) continue;
+            if (!aabb_overlap(amn, amx, bmn, bmx)) continue;
+            const int bb = bo & 0x7fffffff;
+            if (bb == bc) continue;
+            bool excluded = false;
+            for (int k = n0; k < n1; k++) excluded |= L.list[k] == bb;
+            if (excluded || !filters_pass(fc, __float_as_uint(bmx.w))) continue;
+            if (pass) {
+                if (base + cnt < capacity) out_keys[base + cnt] = ((uint64_t)(uint32_t)c << 32) | (uint32_t)(gc0 + j);
+                else atomicOr(status, STATUS_PAIR_OVERFLOW);
+            }
+            cnt++;
         }
         if (!pass) count[c] = cnt;
     }

@@ -40,91 +40,14 @@ __device__ __forceinline__ float block_maxf(float v, float* red) {
     return r;
 }
 
-__global__ __launch_bounds__(kFusedMaxBlock) void sv_world_solve(FusedArgs F) {
-    __shared__ float red[16];
-    __shared__ unsigned int s_wres[kGroupMaxWorlds];   // residual of each world of the group (float bits)
-    __shared__ unsigned long long s_done;               // converged worlds of the group
-    extern __shared__ float4 s_vel[];   // [7][lds_bodies] cv, cw, sv, sw, iw0, iw1, iw2; then [lds_bodies] flags
-    const SvArgs& AG = F.A;
-    const DevBodies& B = AG.B;
-    const int g = F.block_world[blockIdx.x];
+// Warm start, velocity iterations and impulse store of one group.  Instantiated twice: with the body
+// state in LDS (A.B's velocity, inertia and flag arrays point into the workgroup's LDS, so every access
+// compiles to an LDS instruction) and with it in HBM (groups larger than the LDS budget).
+__device__ __forceinline__ void group_sweeps(const FusedArgs& F, const SvArgs& A, const DevBodies& B, bool staged,
+                                             int w0, int ngw, int b0, int b1, int m1, const int* seg, const int2* jr,
+                                             float* red, unsigned int* s_wres, unsigned long long* s_done) {
     const int tid = threadIdx.x, nt = blockDim.x;
-    const int w0 = F.gworld[g], ngw = F.gworld[g + 1] - w0;
-    const int b0 = F.wbody[w0], b1 = F.wbody[w0 + ngw];
-    const int* seg = F.seg_start + (size_t)g * kMaxColors;
-    const int m0 = seg[0], m1 = seg[kMaxColors];
-    const int2* jr = F.jt_wr + (size_t)g * F.jt_slots;
-    const int j0 = F.jt_slots ? jr[0].x : 0, j1 = F.jt_slots ? jr[F.jt_slots - 1].y : 0;
-
-    if (F.phase != 1) {
-        // PhysicsWorld.cpp:266-272: inverse world inertia tensors and velocity integration
-        for (int i = b0 + tid; i < b1; i += nt) dyn_integrate_velocities_one(B, F.settings, F.dt, i);
-        __syncthreads();
-        // ContactSolverSystem::init (rows); ConstraintSolverSystem::initialize (initBeforeSolve)
-        const SvArgs& A = AG;
-        for (int i = m0 + tid; i < m1; i += nt) sv_prepare_one(A, i);
-        for (int j = j0 + tid; j < j1; j += nt) jt_init_one(B, F.J, j);
-        if (F.feats) {
-            // feature rows of the points without a cached impulse; same manifold -> thread mapping as
-            // above, so every thread reads only rows it wrote itself
-            const int lane = lane_id();
-            for (int base = m0 + (tid & ~(kWave - 1)); base < m1; base += nt) {
-                const int i = base + lane;
-                int k = 0;
-                if (i < m1) {
-                    const int np = A.S.npts[i];
-                    for (int p = 0; p < np; p++) k += A.S.lam[4 * i + p].z == 0.0f ? 1 : 0;
-                }
-                const int incl = wave_incl_scan(k);
-                int start = 0;
-                if (lane == kWave - 1 && incl > 0) start = atomicAdd(F.nrows, incl);
-                start = __shfl(start, kWave - 1, kWave) + incl - k;
-                if (i < m1) {
-                    const int np = A.S.npts[i];
-                    for (int p = 0; p < np; p++) {
-                        const int q = 4 * i + p;
-                        if (A.S.lam[q].z != 0.0f) continue;
-                        A.S.pred[q] = make_float4(0.f, 0.f, 0.f, 0.f);
-                        if (start < F.feat_cap) {
-                            float f[kFeatures];
-                            sv_point_features(A, i, p, f);
-                            for (int e = 0; e < kFeatures; e++) F.feats[(size_t)start * kFeatures + e] = f[e];
-                            F.row_slot[start] = q;
-                        }
-                        start++;
-                    }
-                }
-            }
-        }
-        __syncthreads();
-        if (F.phase == 0) return;
-    }
-    // stage the world's solver velocities and world inverse inertia rows in LDS for the sweeps (the
-    // rows and the rest of the body data stay in HBM)
     const int nbw = b1 - b0;
-    const bool staged = nbw <= F.lds_bodies;
-    SvArgs A = AG;
-    if (staged) {
-        const int L = F.lds_bodies;
-        float4 *lcv = s_vel, *lcw = s_vel + L, *lsv = s_vel + 2 * L, *lsw = s_vel + 3 * L;
-        float4 *li0 = s_vel + 4 * L, *li1 = s_vel + 5 * L, *li2 = s_vel + 6 * L;
-        int* lfl = reinterpret_cast<int*>(s_vel + 7 * L);
-        for (int i = tid; i < nbw; i += nt) {
-            lfl[i] = B.flags[b0 + i] | ((B.world[b0 + i] - w0) << kGroupShift);
-            lcv[i] = B.cv[b0 + i];
-            lcw[i] = B.cw[b0 + i];
-            lsv[i] = B.sv[b0 + i];
-            lsw[i] = B.sw[b0 + i];
-            li0[i] = B.iw0[b0 + i];
-            li1[i] = B.iw1[b0 + i];
-            li2[i] = B.iw2[b0 + i];
-        }
-        A.B.cv = lcv; A.B.cw = lcw; A.B.sv = lsv; A.B.sw = lsw;
-        A.B.iw0 = li0; A.B.iw1 = li1; A.B.iw2 = li2;
-        A.B.flags = lfl;
-        A.B.vbase = b0;
-        __syncthreads();
-    }
     const DevBodies& BS = A.B;
     // contact warm start, colour by colour (CS.cpp:326-476)
     for (int c = 0; c < kMaxColors; c++) {
@@ -218,16 +141,16 @@ __global__ __launch_bounds__(kFusedMaxBlock) void sv_world_solve(FusedArgs F) {
                         F.world_res[w0 + tid] = rk;
                     }
                     const unsigned long long bal = __ballot(fin);
-                    if (tid == 0) s_done = done | bal;
+                    if (tid == 0) *s_done = done | bal;
                 }
                 __syncthreads();
-                done = s_done;
+                done = *s_done;
                 if (done == all) break;
             }
         }
     }
     // storeImpulses (CS.cpp:763-785)
-    for (int i = m0 + tid; i < m1; i += nt) sv_store_one(A, i);
+    for (int i = seg[0] + tid; i < m1; i += nt) sv_store_one(A, i);
     if (staged) {
         __syncthreads();
         for (int i = tid; i < nbw; i += nt) {
@@ -242,6 +165,95 @@ __global__ __launch_bounds__(kFusedMaxBlock) void sv_world_solve(FusedArgs F) {
             F.world_iters[w0 + k] = it;
             F.world_res[w0 + k] = res;
         }
+}
+
+__global__ __launch_bounds__(kFusedMaxBlock) void sv_world_solve(FusedArgs F) {
+    __shared__ float red[16];
+    __shared__ unsigned int s_wres[kGroupMaxWorlds];   // residual of each world of the group (float bits)
+    __shared__ unsigned long long s_done;               // converged worlds of the group
+    extern __shared__ float4 s_vel[];   // [7][lds_bodies] cv, cw, sv, sw, iw0, iw1, iw2; then [lds_bodies] flags
+    const SvArgs& AG = F.A;
+    const DevBodies& B = AG.B;
+    const int g = F.block_world[blockIdx.x];
+    const int tid = threadIdx.x, nt = blockDim.x;
+    const int w0 = F.gworld[g], ngw = F.gworld[g + 1] - w0;
+    const int b0 = F.wbody[w0], b1 = F.wbody[w0 + ngw];
+    const int* seg = F.seg_start + (size_t)g * kMaxColors;
+    const int m0 = seg[0], m1 = seg[kMaxColors];
+    const int2* jr = F.jt_wr + (size_t)g * F.jt_slots;
+    const int j0 = F.jt_slots ? jr[0].x : 0, j1 = F.jt_slots ? jr[F.jt_slots - 1].y : 0;
+
+    if (F.phase != 1) {
+        // PhysicsWorld.cpp:266-272: inverse world inertia tensors and velocity integration
+        for (int i = b0 + tid; i < b1; i += nt) dyn_integrate_velocities_one(B, F.settings, F.dt, i);
+        __syncthreads();
+        // ContactSolverSystem::init (rows); ConstraintSolverSystem::initialize (initBeforeSolve)
+        const SvArgs& A = AG;
+        for (int i = m0 + tid; i < m1; i += nt) sv_prepare_one(A, i);
+        for (int j = j0 + tid; j < j1; j += nt) jt_init_one(B, F.J, j);
+        if (F.feats) {
+            // feature rows of the points without a cached impulse; same manifold -> thread mapping as
+            // above, so every thread reads only rows it wrote itself
+            const int lane = lane_id();
+            for (int base = m0 + (tid & ~(kWave - 1)); base < m1; base += nt) {
+                const int i = base + lane;
+                int k = 0;
+                if (i < m1) {
+                    const int np = A.S.npts[i];
+                    for (int p = 0; p < np; p++) k += A.S.lam[4 * i + p].z == 0.0f ? 1 : 0;
+                }
+                const int incl = wave_incl_scan(k);
+                int start = 0;
+                if (lane == kWave - 1 && incl > 0) start = atomicAdd(F.nrows, incl);
+                start = __shfl(start, kWave - 1, kWave) + incl - k;
+                if (i < m1) {
+                    const int np = A.S.npts[i];
+                    for (int p = 0; p < np; p++) {
+                        const int q = 4 * i + p;
+                        if (A.S.lam[q].z != 0.0f) continue;
+                        A.S.pred[q] = make_float4(0.f, 0.f, 0.f, 0.f);
+                        if (start < F.feat_cap) {
+                            float f[kFeatures];
+                            sv_point_features(A, i, p, f);
+                            for (int e = 0; e < kFeatures; e++) F.feats[(size_t)start * kFeatures + e] = f[e];
+                            F.row_slot[start] = q;
+                        }
+                        start++;
+                    }
+                }
+            }
+        }
+        __syncthreads();
+        if (F.phase == 0) return;
+    }
+    // stage the group's solver velocities, world inverse inertia rows and flags in LDS for the sweeps
+    // (the rows and the rest of the body data stay in HBM)
+    const int nbw = b1 - b0;
+    if (nbw <= F.lds_bodies) {
+        const int L = F.lds_bodies;
+        float4 *lcv = s_vel, *lcw = s_vel + L, *lsv = s_vel + 2 * L, *lsw = s_vel + 3 * L;
+        float4 *li0 = s_vel + 4 * L, *li1 = s_vel + 5 * L, *li2 = s_vel + 6 * L;
+        int* lfl = reinterpret_cast<int*>(s_vel + 7 * L);
+        for (int i = tid; i < nbw; i += nt) {
+            lfl[i] = B.flags[b0 + i] | ((B.world[b0 + i] - w0) << kGroupShift);
+            lcv[i] = B.cv[b0 + i];
+            lcw[i] = B.cw[b0 + i];
+            lsv[i] = B.sv[b0 + i];
+            lsw[i] = B.sw[b0 + i];
+            li0[i] = B.iw0[b0 + i];
+            li1[i] = B.iw1[b0 + i];
+            li2[i] = B.iw2[b0 + i];
+        }
+        SvArgs A = AG;
+        A.B.cv = lcv; A.B.cw = lcw; A.B.sv = lsv; A.B.sw = lsw;
+        A.B.iw0 = li0; A.B.iw1 = li1; A.B.iw2 = li2;
+        A.B.flags = lfl;
+        A.B.vbase = b0;
+        __syncthreads();
+        group_sweeps(F, A, B, true, w0, ngw, b0, b1, m1, seg, jr, red, s_wres, &s_done);
+    } else {
+        group_sweeps(F, AG, B, false, w0, ngw, b0, b1, m1, seg, jr, red, s_wres, &s_done);
+    }
 }
 
 __global__ __launch_bounds__(kFusedMaxBlock) void world_position(FusedArgs F) {

@@ -87,6 +87,13 @@ struct DevBodies {
     int vbase;                  // 0, or the first body of a world whose velocities are staged in LDS
 };
 
+// Body pairs excluded from collision (joints with collideConnected = false), per body: partners of body
+// b are list[offs[b] .. offs[b + 1])
+struct NoColLists {
+    const int* offs;
+    const int* list;
+};
+
 struct DevColliders {
     int *body;
     int *type;
@@ -164,7 +171,8 @@ struct DevCounters {
     int color_left[kColorRoundsMax];   // manifolds still uncoloured after each parallel round
     int nb_box_tests;          // box-box subset of nb_tests[ALGO_POLY_POLY]
     int nb_hull_tests;         // the rest of nb_tests[ALGO_POLY_POLY] (a convex mesh involved)
-    int pad[2];
+    int nb_walk_again;         // colliders with more partners than the traversal slab holds
+    int pad[1];
 };
 
 constexpr int STATUS_PAIR_OVERFLOW = 1;

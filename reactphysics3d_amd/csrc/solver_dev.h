@@ -31,6 +31,18 @@ __device__ __forceinline__ void body_locks(const DevBodies& B, int b, V3& ll, V3
     }
 }
 
+// x * f, or x when the lock factor is known to be 1
+template <bool LK>
+__device__ __forceinline__ float lk(float x, float f) {
+    if constexpr (LK) return x * f;
+    else return x;
+}
+template <bool LK>
+__device__ __forceinline__ V3 lkv(V3 f, V3 x) {
+    if constexpr (LK) return cmul(f, x);
+    else return x;
+}
+
 // ------------------------------------------------------------------------------------------------
 // Integration (DynamicsSystem.cpp, PhysicsWorld.cpp:310-318)
 // ------------------------------------------------------------------------------------------------
@@ -352,7 +364,10 @@ __device__ __forceinline__ void sv_point_features(const SvArgs& A, int i, int p,
 }
 
 // ContactSolverSystem::solve (CS.cpp:479-759), split impulses active, for one colour
-__device__ __forceinline__ void solve_manifold(const SvArgs& A, int i, float beta_dt) {
+// LK = false: both bodies have unit lock factors, and every multiplication by a factor (x * 1 = x
+// exactly) is left out; LK = true multiplies as the reference does.
+template <bool LK>
+__device__ __forceinline__ void solve_manifold_t(const SvArgs& A, int i, float beta_dt) {
     const DevBodies& B = A.B;
     const DevSolver& S = A.S;
     const int2 bb = S.bodies[i];
@@ -362,9 +377,11 @@ __device__ __forceinline__ void solve_manifold(const SvArgs& A, int i, float bet
     V3 v1 = ld3(B.cv, u1), w1 = ld3(B.cw, u1), v2 = ld3(B.cv, u2), w2 = ld3(B.cw, u2);
     V3 sv1 = ld3(B.sv, u1), sw1 = ld3(B.sw, u1), sv2 = ld3(B.sv, u2), sw2 = ld3(B.sw, u2);
     const float im1 = S.r1f[i].w, im2 = S.r2f[i].w;
-    V3 ll1, al1, ll2, al2;
-    body_locks(B, b1, ll1, al1);
-    body_locks(B, b2, ll2, al2);
+    V3 ll1 = v3(1.0f, 1.0f, 1.0f), al1 = ll1, ll2 = ll1, al2 = ll1;
+    if constexpr (LK) {
+        body_locks(B, b1, ll1, al1);
+        body_locks(B, b2, ll2, al2);
+    }
     const int np = S.npts[i];
     const M3 I1 = body_iw(B, b1), I2 = body_iw(B, b2);
     float sumPen = 0.0f;
@@ -386,10 +403,10 @@ __device__ __forceinline__ void solve_manifold(const SvArgs& A, int i, float bet
         lam.x = fmax_s(lam.x + dl, 0.0f);
         dl = lam.x - lt;
         const V3 li = v3(n.x * dl, n.y * dl, n.z * dl);
-        v1.x -= im1 * li.x * ll1.x; v1.y -= im1 * li.y * ll1.y; v1.z -= im1 * li.z * ll1.z;
-        w1.x -= i1n.x * al1.x * dl; w1.y -= i1n.y * al1.y * dl; w1.z -= i1n.z * al1.z * dl;
-        v2.x += im2 * li.x * ll2.x; v2.y += im2 * li.y * ll2.y; v2.z += im2 * li.z * ll2.z;
-        w2.x += i2n.x * al2.x * dl; w2.y += i2n.y * al2.y * dl; w2.z += i2n.z * al2.z * dl;
+        v1.x -= lk<LK>(im1 * li.x, ll1.x); v1.y -= lk<LK>(im1 * li.y, ll1.y); v1.z -= lk<LK>(im1 * li.z, ll1.z);
+        w1.x -= lk<LK>(i1n.x, al1.x) * dl; w1.y -= lk<LK>(i1n.y, al1.y) * dl; w1.z -= lk<LK>(i1n.z, al1.z) * dl;
+        v2.x += lk<LK>(im2 * li.x, ll2.x); v2.y += lk<LK>(im2 * li.y, ll2.y); v2.z += lk<LK>(im2 * li.z, ll2.z);
+        w2.x += lk<LK>(i2n.x, al2.x) * dl; w2.y += lk<LK>(i2n.y, al2.y) * dl; w2.z += lk<LK>(i2n.z, al2.z) * dl;
         sumPen += lam.x;
         // split impulse row
         const V3 dvs = rel_vel(sv1, sw1, sv2, sw2, r1, r2);
@@ -399,10 +416,10 @@ __device__ __forceinline__ void solve_manifold(const SvArgs& A, int i, float bet
         lam.y = fmax_s(lam.y + dls, 0.0f);
         dls = lam.y - lts;
         const V3 ls = v3(n.x * dls, n.y * dls, n.z * dls);
-        sv1.x -= im1 * ls.x * ll1.x; sv1.y -= im1 * ls.y * ll1.y; sv1.z -= im1 * ls.z * ll1.z;
-        sw1.x -= i1n.x * al1.x * dls; sw1.y -= i1n.y * al1.y * dls; sw1.z -= i1n.z * al1.z * dls;
-        sv2.x += im2 * ls.x * ll2.x; sv2.y += im2 * ls.y * ll2.y; sv2.z += im2 * ls.z * ll2.z;
-        sw2.x += i2n.x * al2.x * dls; sw2.y += i2n.y * al2.y * dls; sw2.z += i2n.z * al2.z * dls;
+        sv1.x -= lk<LK>(im1 * ls.x, ll1.x); sv1.y -= lk<LK>(im1 * ls.y, ll1.y); sv1.z -= lk<LK>(im1 * ls.z, ll1.z);
+        sw1.x -= lk<LK>(i1n.x, al1.x) * dls; sw1.y -= lk<LK>(i1n.y, al1.y) * dls; sw1.z -= lk<LK>(i1n.z, al1.z) * dls;
+        sv2.x += lk<LK>(im2 * ls.x, ll2.x); sv2.y += lk<LK>(im2 * ls.y, ll2.y); sv2.z += lk<LK>(im2 * ls.z, ll2.z);
+        sw2.x += lk<LK>(i2n.x, al2.x) * dls; sw2.y += lk<LK>(i2n.y, al2.y) * dls; sw2.z += lk<LK>(i2n.z, al2.z) * dls;
         S.lam[q] = lam;
     }
     const V3 r1f = ld3(S.r1f, i), r2f = ld3(S.r2f, i);
@@ -423,11 +440,11 @@ __device__ __forceinline__ void solve_manifold(const SvArgs& A, int i, float bet
         const V3 a1 = v3(-r1ct1.x * dl, -r1ct1.y * dl, -r1ct1.z * dl);
         const V3 l2 = v3(t1.x * dl, t1.y * dl, t1.z * dl);
         const V3 a2 = v3(r2ct1.x * dl, r2ct1.y * dl, r2ct1.z * dl);
-        v1.x -= im1 * l2.x * ll1.x; v1.y -= im1 * l2.y * ll1.y; v1.z -= im1 * l2.z * ll1.z;
-        const V3 aw1 = cmul(al1, mvmul(I1, a1));
+        v1.x -= lk<LK>(im1 * l2.x, ll1.x); v1.y -= lk<LK>(im1 * l2.y, ll1.y); v1.z -= lk<LK>(im1 * l2.z, ll1.z);
+        const V3 aw1 = lkv<LK>(al1, mvmul(I1, a1));
         w1.x += aw1.x; w1.y += aw1.y; w1.z += aw1.z;
-        v2.x += im2 * l2.x * ll2.x; v2.y += im2 * l2.y * ll2.y; v2.z += im2 * l2.z * ll2.z;
-        const V3 aw2 = cmul(al2, mvmul(I2, a2));
+        v2.x += lk<LK>(im2 * l2.x, ll2.x); v2.y += lk<LK>(im2 * l2.y, ll2.y); v2.z += lk<LK>(im2 * l2.z, ll2.z);
+        const V3 aw2 = lkv<LK>(al2, mvmul(I2, a2));
         w2.x += aw2.x; w2.y += aw2.y; w2.z += aw2.z;
     }
     // friction 2
@@ -443,11 +460,11 @@ __device__ __forceinline__ void solve_manifold(const SvArgs& A, int i, float bet
         const V3 a1 = v3(-r1ct2.x * dl, -r1ct2.y * dl, -r1ct2.z * dl);
         const V3 l2 = v3(t2.x * dl, t2.y * dl, t2.z * dl);
         const V3 a2 = v3(r2ct2.x * dl, r2ct2.y * dl, r2ct2.z * dl);
-        v1.x -= im1 * l2.x * ll1.x; v1.y -= im1 * l2.y * ll1.y; v1.z -= im1 * l2.z * ll1.z;
-        const V3 aw1 = cmul(al1, mvmul(I1, a1));
+        v1.x -= lk<LK>(im1 * l2.x, ll1.x); v1.y -= lk<LK>(im1 * l2.y, ll1.y); v1.z -= lk<LK>(im1 * l2.z, ll1.z);
+        const V3 aw1 = lkv<LK>(al1, mvmul(I1, a1));
         w1.x += aw1.x; w1.y += aw1.y; w1.z += aw1.z;
-        v2.x += im2 * l2.x * ll2.x; v2.y += im2 * l2.y * ll2.y; v2.z += im2 * l2.z * ll2.z;
-        const V3 aw2 = cmul(al2, mvmul(I2, a2));
+        v2.x += lk<LK>(im2 * l2.x, ll2.x); v2.y += lk<LK>(im2 * l2.y, ll2.y); v2.z += lk<LK>(im2 * l2.z, ll2.z);
+        const V3 aw2 = lkv<LK>(al2, mvmul(I2, a2));
         w2.x += aw2.x; w2.y += aw2.y; w2.z += aw2.z;
     }
     // twist
@@ -460,9 +477,9 @@ __device__ __forceinline__ void solve_manifold(const SvArgs& A, int i, float bet
         imp.z = fmax_s(-lim, fmin_s(imp.z + dl, lim));
         dl = imp.z - lt;
         const V3 a2 = v3(n.x * dl, n.y * dl, n.z * dl);
-        const V3 aw1 = cmul(al1, mvmul(I1, a2));
+        const V3 aw1 = lkv<LK>(al1, mvmul(I1, a2));
         w1.x -= aw1.x; w1.y -= aw1.y; w1.z -= aw1.z;
-        const V3 aw2 = cmul(al2, mvmul(I2, a2));
+        const V3 aw2 = lkv<LK>(al2, mvmul(I2, a2));
         w2.x += aw2.x; w2.y += aw2.y; w2.z += aw2.z;
     }
     S.imp[i] = imp;
@@ -474,6 +491,12 @@ __device__ __forceinline__ void solve_manifold(const SvArgs& A, int i, float bet
         B.cv[u2] = f4(v2, 0.f); B.cw[u2] = f4(w2, 0.f);
         B.sv[u2] = f4(sv2, 0.f); B.sw[u2] = f4(sw2, 0.f);
     }
+}
+
+__device__ __forceinline__ void solve_manifold(const SvArgs& A, int i, float beta_dt) {
+    const int2 bb = A.S.bodies[i];
+    if (body_flags(A.B, bb.x) & body_flags(A.B, bb.y) & BF_UNIT_LOCKS) solve_manifold_t<false>(A, i, beta_dt);
+    else solve_manifold_t<true>(A, i, beta_dt);
 }
 
 // ContactSolverSystem::storeImpulses (CS.cpp:763-785); initializeForIsland marks every point resting (:161)

@@ -41,7 +41,7 @@ class Counts(C.Structure):
 
 class BatchOptions(C.Structure):
     _fields_ = [("solver", C.c_int32), ("fused_block", C.c_int32), ("pairs_per_collider", C.c_int32),
-                ("group_bodies", C.c_int32), ("reserved", C.c_int32 * 4)]
+                ("group_bodies", C.c_int32), ("broadphase", C.c_int32), ("reserved", C.c_int32 * 3)]
 
 
 SOLVER_MODES = {"auto": 0, "per_color": 1, "fused": 2}
@@ -226,7 +226,7 @@ def world_from_scene(sc: S.Scene):
 class Batch:
     """One or many worlds stepped together on the GPU."""
 
-    def __init__(self, scenes, solver="auto", fused_block=0, pairs_per_collider=0, group_bodies=0):
+    def __init__(self, scenes, solver="auto", fused_block=0, pairs_per_collider=0, group_bodies=0, broadphase="auto"):
         L = load()
         if isinstance(scenes, S.Scene):
             scenes = [scenes]
@@ -238,6 +238,7 @@ class Batch:
         opt.fused_block = int(fused_block)
         opt.pairs_per_collider = int(pairs_per_collider)
         opt.group_bodies = int(group_bodies)
+        opt.broadphase = {"auto": 0, "tree": 1, "world_scan": 2}[broadphase]
         self.h = L.rp3dg_batch_create_ex(arr, len(self.worlds), C.byref(opt))
         if not self.h:
             raise RuntimeError("rp3dg_batch_create failed: " + last_error())

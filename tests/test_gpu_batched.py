@@ -130,3 +130,24 @@ def test_world_groups_do_not_change_results(residual):
     if residual > 0:
         assert base[1][-1]["max_iterations"] < 150 or base[1][-1]["max_residual"] >= residual
         assert base[1][-1]["mean_iterations"] < base[1][-1]["max_iterations"]
+
+
+@pytest.mark.parametrize("kind", ["batched", "mixed"])
+def test_world_scan_broadphase_matches_tree(kind):
+    """The per-world box scan finds the pair set of the tree query: identical pair tables, contacts and
+    body states after every step (dense piles make some colliders exceed the eight-partner slab)."""
+    from reactphysics3d_amd.lib import Batch
+    if kind == "batched":
+        scenes = [S.batched_world(w, seed=31, min_bodies=8, max_bodies=300, max_joints=8) for w in range(24)]
+    else:
+        scenes = [S.mixed_container(400, extent=(8.0, 8.0, 10.0), seed=5)]
+    tree = Batch(scenes, solver="fused", broadphase="tree")
+    scan = Batch(scenes, solver="fused", broadphase="world_scan")
+    for step in range(40):
+        tree.step(DT)
+        scan.step(DT)
+        pt, ps = tree.pairs(), scan.pairs()
+        assert all(np.array_equal(a, b) for a, b in zip(pt, ps)), f"step {step}: pair tables differ"
+        for a, b in zip(tree.bodies(), scan.bodies()):
+            assert np.array_equal(a, b), f"step {step}: body states differ"
+    assert tree.counts() == scan.counts()
