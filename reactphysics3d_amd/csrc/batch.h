@@ -108,7 +108,8 @@ struct rp3dg_batch {
     int* bp_walk = nullptr;            // colliders walked again after the scan (more partners than the slab)
     int *bp_vals = nullptr, *bp_vtmp = nullptr;
     int2 *bp_child = nullptr, *bp_range = nullptr;
-    int *bp_parent = nullptr, *bp_root = nullptr, *bp_count = nullptr, *bp_offs = nullptr;
+    int *bp_parent = nullptr, *bp_root = nullptr, *bp_count = nullptr;
+    int* bp_offs_ab[2] = {nullptr, nullptr};   // per-collider segment starts of pair table A / B
     float4 *fit_min = nullptr, *fit_max = nullptr;   // unions of aligned 64 / 4096 / 262144-leaf blocks
     float4 *bp_nmin = nullptr, *bp_nmax = nullptr, *bp_partial = nullptr;
     // narrowphase

@@ -46,7 +46,7 @@ __global__ void bp_copy_slabs(DevColliders C, int n, const int* count, const int
 __global__ void bp_world_scan(DevColliders C, const int* gworld, const int* wcoll, const int* block_group, NoColLists L,
                               int* count, const int* offset, uint64_t* out_keys, int capacity, int pass, int* status);
 __global__ void bp_merge(DevColliders C, const uint64_t* keys, const int* nkeys_ptr, int capacity, DevPairs O,
-                         const int* nold_ptr, DevPairs N, int* nnew_ptr, int* nb_pairs_ptr);
+                         const int* nold_ptr, const int* old_offs, DevPairs N, int* nnew_ptr, int* nb_pairs_ptr);
 
 // ---------------------------------------------------------------------------------------------
 // exclusive scan (out has n+1 entries; out[n] = total).  With nptr the element count is
@@ -678,11 +678,15 @@ int rp3dg_batch::upload(rp3dg_world* const* ws, int nw, const rp3dg_batch_option
     }
     bp_root = dalloc<int>(this, (size_t)1 << world_bits);
     bp_partial = dalloc<float4>(this, 2 * 256);
-    bp_count = dalloc<int>(this, nc + 1); bp_offs = dalloc<int>(this, nc + 1);
+    bp_count = dalloc<int>(this, nc + 1);
+    for (int t = 0; t < 2; t++) {
+        bp_offs_ab[t] = dalloc<int>(this, nc + 1);
+        if (bp_offs_ab[t]) HIPCHECK(hipMemset(bp_offs_ab[t], 0, sizeof(int) * (nc + 1)));
+    }
     bp_out = dalloc<uint64_t>(this, pair_cap);
     bp_slab = dalloc<uint64_t>(this, (size_t)nc * kTravSlab);
     bp_walk = dalloc<int>(this, nc);
-    np_lists = dalloc<int>(this, (size_t)ALGO_COUNT * pair_cap);
+    np_lists = dalloc<int>(this, (size_t)kNpListSlots * pair_cap);
     np_counts = dalloc<int>(this, ALGO_COUNT);
     m_flag = dalloc<int>(this, pair_cap); m_offs = dalloc<int>(this, pair_cap + 1);
     m_list = dalloc<int>(this, pair_cap);
@@ -729,7 +733,7 @@ int rp3dg_batch::upload(rp3dg_world* const* ws, int nw, const rp3dg_batch_option
                 int j = i;
                 while (j < ng && gsize(order[j]) > L / 2) j++;
                 if (L == 64) j = ng;   // everything up to 64 bodies is one class
-                wclasses.push_back({i, j - i, L, 64, L <= 512 ? L : 0});
+                wclasses.push_back({i, j - i, L, 64, L <= kFusedMaxLds ? L : 0});   // LDS: 64, 128, 256 or 512
                 i = j;
             }
             set_class_blocks();
@@ -841,6 +845,8 @@ int rp3dg_batch::enqueue_step(float dt, bool capturing) {
     DevPairs& N = cur_is_a ? Pb : Pa;
     int* old_count = pair_count + (cur_is_a ? 0 : 1);
     int* new_count = pair_count + (cur_is_a ? 1 : 0);
+    int* bp_offs = bp_offs_ab[cur_is_a ? 1 : 0];          // segments of the table written this step
+    const int* old_offs = bp_offs_ab[cur_is_a ? 0 : 1];   // segments of the previous table
     int* dstatus = &counters->status;
     int r;
     fill_int<<<1, 256, 0, stream>>>((int*)counters, (int)(sizeof(DevCounters) / sizeof(int)), 0);
@@ -895,7 +901,7 @@ int rp3dg_batch::enqueue_step(float dt, bool capturing) {
     }
     const int hp = capturing ? pair_cap : std::min(pair_cap, hint_pairs + hint_pairs / 4 + 1024);
     const int pg = grid_for(hp);
-    LAUNCH_T("bp_merge", pg, 256, bp_merge, C, bp_out, bp_offs + nc, pair_cap, O, old_count, N, new_count,
+    LAUNCH_T("bp_merge", pg, 256, bp_merge, C, bp_out, bp_offs + nc, pair_cap, O, old_count, old_offs, N, new_count,
              &counters->nb_pairs);
     cur_is_a = !cur_is_a;
 
@@ -907,8 +913,8 @@ int rp3dg_batch::enqueue_step(float dt, bool capturing) {
     A.hull_face_edge = d_hfedge; A.hull_face_verts = d_hfverts; A.hull_face_vidx = d_hfvidx;
     A.persistent_threshold = settings.persistent_threshold;
     A.cos_similar = settings.cos_similar;
-    LAUNCH_T("np_classify", pg, 256, np_classify, A, np_lists, pair_cap, counters->nb_tests, &counters->nb_box_tests,
-             &counters->nb_hull_tests);
+    LAUNCH_T("np_classify", std::min(pg, 1024), 256, np_classify, A, np_lists, pair_cap, counters->nb_tests,
+             &counters->nb_box_tests, &counters->nb_box_sep_tests, &counters->nb_hull_tests);
     int* L = np_lists;
     int* T = counters->nb_tests;
     auto hint = [&](int n, int block) {
@@ -921,7 +927,10 @@ int rp3dg_batch::enqueue_step(float dt, bool capturing) {
     LAUNCH_T("np_capsule_capsule", tg(ALGO_CAPSULE_CAPSULE, 256), 256, np_capsule_capsule, A, L + ALGO_CAPSULE_CAPSULE * pair_cap, T + ALGO_CAPSULE_CAPSULE);
     LAUNCH_T("np_sphere_poly", tg(ALGO_SPHERE_POLY, 256), 256, np_sphere_poly, A, L + ALGO_SPHERE_POLY * pair_cap, T + ALGO_SPHERE_POLY);
     LAUNCH_T("np_capsule_poly", tg(ALGO_CAPSULE_POLY, 256), 256, np_capsule_poly, A, L + ALGO_CAPSULE_POLY * pair_cap, T + ALGO_CAPSULE_POLY);
-    LAUNCH_T("np_box_box", hint(last.nb_box_tests, 128), 128, np_box_box, A, L, &counters->nb_box_tests);
+    LAUNCH_T("np_box_box", hint(last.nb_box_tests, 128), 128, np_box_box, A, L + np_list_slot(ALGO_COUNT) * pair_cap,
+             &counters->nb_box_tests);
+    LAUNCH_T("np_box_box", hint(last.nb_box_sep_tests, 128), 128, np_box_box, A, L + np_list_slot(ALGO_COUNT + 1) * pair_cap,
+             &counters->nb_box_sep_tests);
     LAUNCH_T("np_poly_poly", hint(last.nb_hull_tests, 128), 128, np_poly_poly, A, L + ALGO_POLY_POLY * pair_cap,
              &counters->nb_hull_tests);
 
@@ -930,7 +939,7 @@ int rp3dg_batch::enqueue_step(float dt, bool capturing) {
     LAUNCH_T("sv_flag", pg, 256, sv_flag_manifolds, N.ncp, &counters->nb_pairs, m_flag);
     r = scan(m_flag, pair_cap, &counters->nb_pairs, m_offs, stream);
     if (r) { set_error("scan overflow"); return r; }
-    LAUNCH_T("sv_compact", pg, 256, sv_compact_manifolds, m_flag, m_offs, &counters->nb_pairs, N.ncp, m_list, nm_ptr,
+    LAUNCH_T("sv_compact", std::min(pg, 1024), 256, sv_compact_manifolds, m_flag, m_offs, &counters->nb_pairs, N.ncp, m_list, nm_ptr,
              &counters->nb_points, total_points);
     const int hm = capturing ? pair_cap : std::min(pair_cap, hint_manifolds + hint_manifolds / 4 + 1024);
     const int mg = grid_for(hm);
@@ -1302,7 +1311,7 @@ int rp3dg_batch_get_counts(rp3dg_batch* b, rp3dg_counts* o) {
     o->total_contact_points = (int64_t)tp;
     o->steps = b->steps_done;
     o->nb_new_points = b->last.nb_new_points;
-    o->nb_box_tests = b->last.nb_box_tests;
+    o->nb_box_tests = b->last.nb_box_tests + b->last.nb_box_sep_tests;
     return RP3DG_OK;
 }
 

@@ -545,8 +545,10 @@ __device__ __forceinline__ int select_algo(int t1, int t2) {
 
 // nkeys_ptr: overlapping pairs found (clamped to the capacity; an overflow is flagged by the traversal).
 // nold_ptr: size of the previous table O.  nnew_ptr / nb_pairs_ptr receive the size of the new table N.
+// old_offs: first entry of every emitter's segment in the previous table (its keys are sorted and
+// grouped by emitter, key >> 32), so a key is searched only within its emitter's few old entries.
 __global__ void bp_merge(DevColliders C, const uint64_t* keys, const int* nkeys_ptr, int capacity, DevPairs O,
-                         const int* nold_ptr, DevPairs N, int* nnew_ptr, int* nb_pairs_ptr) {
+                         const int* nold_ptr, const int* old_offs, DevPairs N, int* nnew_ptr, int* nb_pairs_ptr) {
     const int nkeys = min(*nkeys_ptr, capacity);
     const int nold = *nold_ptr;
     if (blockIdx.x == 0 && threadIdx.x == 0) {
@@ -556,7 +558,8 @@ __global__ void bp_merge(DevColliders C, const uint64_t* keys, const int* nkeys_
     RP3DG_GRID_STRIDE(k, nkeys) {
         const uint64_t key = keys[k];
         N.key[k] = key;
-        int lo = 0, hi = nold - 1, found = -1;
+        const int e = (int)(key >> 32);
+        int lo = min(old_offs[e], nold), hi = min(old_offs[e + 1], nold) - 1, found = -1;
         while (lo <= hi) {
             const int mid = (lo + hi) >> 1;
             const uint64_t v = O.key[mid];

@@ -38,6 +38,7 @@ struct FusedArgs {
 constexpr int kFeatures = 24;
 
 constexpr int kFusedMaxBlock = 256;   // threads per group at most (register budget of the fused kernels)
+constexpr int kFusedMaxLds = 512;     // bodies per group staged in LDS at most (the classes use 64 .. 512)
 constexpr int kGroupMaxWorlds = 64;   // worlds per group at most (one bit each in the convergence mask)
 constexpr int kGroupShift = 16;       // staged body flags carry the body's world index in its group here
 

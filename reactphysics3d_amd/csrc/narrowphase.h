@@ -21,7 +21,20 @@ struct NpArgs {
     float cos_similar;
 };
 
-__global__ void np_classify(NpArgs A, int* lists, int list_stride, int* counts, int* box_count, int* hull_count);
+// Narrowphase test lists: bins 1..5 closed forms, ALGO_POLY_POLY pairs with a convex mesh,
+// ALGO_COUNT box-box pairs colliding last step, ALGO_COUNT + 1 box-box pairs separated last step.
+// List slot of each bin (kNpListSlots lists of the pair capacity each are allocated).
+constexpr int kNpBins = ALGO_COUNT + 2;
+constexpr int kNpListSlots = ALGO_COUNT + 1;
+__host__ __device__ constexpr int np_list_slot(int bin) {
+    return bin == ALGO_COUNT ? 0 : (bin == ALGO_COUNT + 1 ? ALGO_COUNT : bin);
+}
+static_assert(np_list_slot(ALGO_COUNT + 1) < kNpListSlots && np_list_slot(ALGO_COUNT) < kNpListSlots &&
+                  np_list_slot(ALGO_POLY_POLY) < kNpListSlots,
+              "every narrowphase bin needs an allocated list slot");
+
+__global__ void np_classify(NpArgs A, int* lists, int list_stride, int* counts, int* box_count, int* box_sep_count,
+                            int* hull_count);
 __global__ void np_sphere_sphere(NpArgs A, const int* list, const int* count);
 __global__ void np_sphere_capsule(NpArgs A, const int* list, const int* count);
 __global__ void np_capsule_capsule(NpArgs A, const int* list, const int* count);

@@ -132,9 +132,34 @@ struct RawBuf {
             n++;
         }
     }
+    __device__ V3 normal(int r) const { return nrm[r]; }
 };
 using Raw = RawBuf<kMaxRawContacts>;
-constexpr int kBoxRaw = 8;      // raw contacts of a box-box test at most
+
+// Box-box tests: every contact point of one test carries the same normal (the reference face normal
+// of the face contacts, or the single edge-edge contact), so it is kept once.
+struct RawBox {
+    static constexpr int kN = 8;
+    int n;
+    V3 nrm0;
+    float depth[kN];
+    V3 l1[kN];
+    V3 l2[kN];
+    __device__ void add(V3 normal, float d, V3 a, V3 b) {
+        if (n < kN) {
+            nrm0 = normal; depth[n] = d; l1[n] = a; l2[n] = b;
+            n++;
+        }
+    }
+    __device__ V3 normal(int) const { return nrm0; }
+};
+
+// Up to 16 indices below 16, four bits each, in one register (dynamic indexing without scratch)
+struct IdxList {
+    uint64_t v = 0;
+    __device__ int get(int i) const { return (int)((v >> (4 * i)) & 15u); }
+    __device__ void set(int i, int x) { v = (v & ~(15ull << (4 * i))) | ((uint64_t)x << (4 * i)); }
+};
 constexpr int kBoxClip = 8;     // vertices of a box face clipped by the four side planes at most
 
 // ------------------------------------------------------------------------------------------------
@@ -585,33 +610,33 @@ __device__ bool capsule_capsule(float r1, float h1, float r2, float h2, const Tf
 // CollisionDetectionSystem::reduceContactPoints (:1483-1662): picks 4 of n > 4 raw points.
 template <class R>
 __device__ void reduce_points(const R& raw, const Tf& T1, int* keep) {
-    int cand[R::kN];
+    IdxList cand;
     int nc = raw.n;
-    for (int i = 0; i < nc; i++) cand[i] = i;
+    for (int i = 0; i < nc; i++) cand.set(i, i);
     const Tf w2s = tinv(T1);
-    const V3 nS1 = qrot(w2s.q, raw.nrm[cand[0]]);
+    const V3 nS1 = qrot(w2s.q, raw.normal(cand.get(0)));
     const V3 dir = v3(1, 1, 1);
     float maxDot = kDecimalSmallest;
     int idx = 0;
     for (int i = 0; i < nc; i++) {
-        const float dp = dot(dir, raw.l1[cand[i]]);
+        const float dp = dot(dir, raw.l1[cand.get(i)]);
         if (dp > maxDot) { maxDot = dp; idx = i; }
     }
-    keep[0] = cand[idx];
-    cand[idx] = cand[nc - 1]; nc--;
+    keep[0] = cand.get(idx);
+    cand.set(idx, cand.get(nc - 1)); nc--;
     float maxDist = 0.0f;
     idx = 0;
     for (int i = 0; i < nc; i++) {
-        const float d = len2(raw.l1[keep[0]] - raw.l1[cand[i]]);
+        const float d = len2(raw.l1[keep[0]] - raw.l1[cand.get(i)]);
         if (d >= maxDist) { maxDist = d; idx = i; }
     }
-    keep[1] = cand[idx];
-    cand[idx] = cand[nc - 1]; nc--;
+    keep[1] = cand.get(idx);
+    cand.set(idx, cand.get(nc - 1)); nc--;
     int maxI = 0, minI = 0;
     float minArea = 0.0f, maxArea = 0.0f;
     for (int i = 0; i < nc; i++) {
-        const V3 nf = raw.l1[keep[0]] - raw.l1[cand[i]];
-        const V3 ns = raw.l1[keep[1]] - raw.l1[cand[i]];
+        const V3 nf = raw.l1[keep[0]] - raw.l1[cand.get(i)];
+        const V3 ns = raw.l1[keep[1]] - raw.l1[cand.get(i)];
         const float area = dot(cross(nf, ns), nS1);
         if (area >= maxArea) { maxArea = area; maxI = i; }
         if (area <= minArea) { minArea = area; minI = i; }
@@ -619,26 +644,27 @@ __device__ void reduce_points(const R& raw, const Tf& T1, int* keep) {
     bool prevPositive;
     if (maxArea > (-minArea)) {
         prevPositive = true;
-        keep[2] = cand[maxI];
-        cand[maxI] = cand[nc - 1]; nc--;
+        keep[2] = cand.get(maxI);
+        cand.set(maxI, cand.get(nc - 1)); nc--;
     } else {
         prevPositive = false;
-        keep[2] = cand[minI];
-        cand[minI] = cand[nc - 1]; nc--;
+        keep[2] = cand.get(minI);
+        cand.set(minI, cand.get(nc - 1)); nc--;
     }
     float largest = 0.0f;
     idx = 0;
     for (int i = 0; i < nc; i++) {
+#pragma unroll
         for (int j = 0; j < 3; j++) {
             const int va = j, vb = j < 2 ? j + 1 : 0;
-            const V3 nf = raw.l1[keep[va]] - raw.l1[cand[i]];
-            const V3 ns = raw.l1[keep[vb]] - raw.l1[cand[i]];
+            const V3 nf = raw.l1[keep[va]] - raw.l1[cand.get(i)];
+            const V3 ns = raw.l1[keep[vb]] - raw.l1[cand.get(i)];
             const float area = dot(cross(nf, ns), nS1);
             if (prevPositive && area <= largest) { largest = area; idx = i; }
             else if (!prevPositive && area >= largest) { largest = area; idx = i; }
         }
     }
-    keep[3] = cand[idx];
+    keep[3] = cand.get(idx);
 }
 
 template <class R>
@@ -649,57 +675,69 @@ __device__ void contact_epilogue(const NpArgs& A, int slot, bool colliding, cons
         P.ncp[slot] = 0;
         return;
     }
-    int idx[R::kN];
+    IdxList idx;
     int n = raw.n;
-    for (int i = 0; i < n; i++) idx[i] = i;
+    for (int i = 0; i < n; i++) idx.set(i, i);
     if (n > kMaxManifoldPoints) {
         const Tf T1 = tf(ld3(A.C.wpos, collider1), ldq(A.C.wq, collider1));
         int keep[4];
         reduce_points(raw, T1, keep);
-        for (int i = 0; i < 4; i++) idx[i] = keep[i];
+        for (int i = 0; i < 4; i++) idx.set(i, keep[i]);
         n = 4;
     }
     // removeDuplicatedContactPointsInManifold (:1673-1705)
     const float thr2 = 0.01f * 0.01f;
     for (int i = 0; i < n; i++) {
         for (int j = i + 1; j < n; j++) {
-            const float d2 = len2(raw.l1[idx[j]] - raw.l1[idx[i]]);
+            const float d2 = len2(raw.l1[idx.get(j)] - raw.l1[idx.get(i)]);
             if (d2 < thr2) {
-                idx[j] = idx[n - 1];
+                idx.set(j, idx.get(n - 1));
                 n--;
                 j--;
             }
         }
     }
-    // warm start transfer (initContactsWithPreviousOnes, :1069-1160)
+    // warm start transfer (initContactsWithPreviousOnes, :1069-1160); the old points are held in
+    // registers (fixed-trip loops over the four cache slots)
     float4 fv1 = make_float4(0.f, 0.f, 0.f, 0.f), fv2 = fv1, fimp = fv1;
     float4 oldl1[kMaxManifoldPoints];
-    float4 oldl2w[kMaxManifoldPoints];
+    float oldrest[kMaxManifoldPoints];
+#pragma unroll
+    for (int k = 0; k < kMaxManifoldPoints; k++) {
+        oldl1[k] = make_float4(0.f, 0.f, 0.f, 0.f);
+        oldrest[k] = 0.0f;
+    }
     if (oldN > 0) {
         const float4 oldn0 = P.pn[4 * slot];
-        if (dot(v3(oldn0.x, oldn0.y, oldn0.z), raw.nrm[idx[0]]) >= A.cos_similar) {
+        if (dot(v3(oldn0.x, oldn0.y, oldn0.z), raw.normal(idx.get(0))) >= A.cos_similar) {
             fv1 = P.fv1[slot];
             fv2 = P.fv2[slot];
             fimp = P.fimp[slot];
         }
-        for (int k = 0; k < oldN; k++) {
-            oldl1[k] = P.pl1[4 * slot + k];
-            oldl2w[k] = P.pl2[4 * slot + k];
+#pragma unroll
+        for (int k = 0; k < kMaxManifoldPoints; k++) {
+            if (k < oldN) {
+                oldl1[k] = P.pl1[4 * slot + k];
+                oldrest[k] = P.pl2[4 * slot + k].w;
+            }
         }
     }
     const float pt2 = A.persistent_threshold * A.persistent_threshold;
     for (int k = 0; k < n; k++) {
-        const int r = idx[k];
+        const int r = idx.get(k);
         float imp = 0.0f, resting = 0.0f;
-        for (int o = 0; o < oldN; o++) {
+        bool matched = false;
+#pragma unroll
+        for (int o = 0; o < kMaxManifoldPoints; o++) {
+            if (matched || o >= oldN) continue;
             const float d2 = len2(raw.l1[r] - xyz(oldl1[o]));
             if (d2 <= pt2) {
                 imp = oldl1[o].w;
-                resting = oldl2w[o].w;
-                break;
+                resting = oldrest[o];
+                matched = true;
             }
         }
-        P.pn[4 * slot + k] = f4(raw.nrm[r], raw.depth[r]);
+        P.pn[4 * slot + k] = f4(raw.normal(r), raw.depth[r]);
         P.pl1[4 * slot + k] = f4(raw.l1[r], imp);
         P.pl2[4 * slot + k] = f4(raw.l2[r], resting);
     }
@@ -722,53 +760,75 @@ __device__ inline uint32_t lf_after_test(uint32_t lf, bool colliding) {
 // ------------------------------------------------------------------------------------------------
 
 // Middle phase: bucket the tested pairs by algorithm; a pair is tested only if at least one of its
-// bodies is enabled (CD.cpp:391-400).  Untested pairs have no contact pair this step.
-// List slots are reserved with one atomic per wave and algorithm (ballot + popcount ranks); the list
-// order is irrelevant because every test writes only its own pair slot.
-// Polyhedron pairs are split further: box-box pairs go to list slot 0 (ALGO_NONE is never listed)
-// counted in box_count, the others stay in the ALGO_POLY_POLY slot counted in hull_count;
-// counts[ALGO_POLY_POLY] keeps the total.
-__global__ void np_classify(NpArgs A, int* lists, int list_stride, int* counts, int* box_count, int* hull_count) {
+// bodies is enabled (CD.cpp:391-400).  Untested pairs have no contact pair this step.  The list order
+// is irrelevant because every test writes only its own pair slot.
+// Polyhedron pairs are split further (np_list_slot): box-box pairs that collided in the last step go
+// to list slot 0 (ALGO_NONE is never listed) counted in box_count, the other box-box pairs to slot
+// ALGO_COUNT counted in box_sep_count (a wave of either kind follows one path through the test), pairs
+// with a mesh stay in the ALGO_POLY_POLY slot counted in hull_count; counts[ALGO_POLY_POLY] keeps the
+// total.  Each workgroup counts its pairs per bin first and reserves one contiguous range per bin with
+// a single global atomic (same-address atomics from every wave serialise in the L2), then lists its
+// pairs into those ranges.
+__device__ __forceinline__ int np_bin(const NpArgs& A, int k, bool count_pass) {
+    const int2 c = A.P.col[k];
+    const int f1 = A.B.flags[A.C.body[c.x]], f2 = A.B.flags[A.C.body[c.y]];
+    if (!((f1 & BF_ENABLED) || (f2 & BF_ENABLED))) {
+        if (count_pass) A.P.ncp[k] = 0;
+        return 0;
+    }
+    const int a = A.P.algo[k];
+    if (a == ALGO_POLY_POLY && A.C.type[c.x] == SHAPE_BOX && A.C.type[c.y] == SHAPE_BOX)   // box-box
+        return (A.P.lf[k] & LF_WAS_COLLIDING) ? ALGO_COUNT : ALGO_COUNT + 1;
+    return a;
+}
+
+__global__ __launch_bounds__(256) void np_classify(NpArgs A, int* lists, int list_stride, int* counts, int* box_count,
+                                                   int* box_sep_count, int* hull_count) {
+    constexpr int kBins = kNpBins;
+    __shared__ int s_cnt[kBins];
+    __shared__ int s_base[kBins];
+    __shared__ int s_cur[kBins];
     const int np = *A.nb_pairs;
     const int stride = gridDim.x * blockDim.x;
     const int lane = lane_id();
     const uint64_t lt = lane == 0 ? 0ull : (~0ull >> (64 - lane));
+    if (threadIdx.x < kBins) {
+        s_cnt[threadIdx.x] = 0;
+        s_cur[threadIdx.x] = 0;
+    }
+    __syncthreads();
     for (int base = blockIdx.x * blockDim.x; base < np; base += stride) {
         const int k = base + threadIdx.x;
-        int a = 0;
-        bool boxes = false;
-        if (k < np) {
-            const int2 c = A.P.col[k];
-            const int f1 = A.B.flags[A.C.body[c.x]], f2 = A.B.flags[A.C.body[c.y]];
-            if ((f1 & BF_ENABLED) || (f2 & BF_ENABLED)) a = A.P.algo[k];
-            else A.P.ncp[k] = 0;
-            if (a == ALGO_POLY_POLY) boxes = A.C.type[c.x] == SHAPE_BOX && A.C.type[c.y] == SHAPE_BOX;
+        const int b = k < np ? np_bin(A, k, true) : 0;
+        for (int t = 1; t < kBins; t++) {
+            const uint64_t m = __ballot(b == t);
+            if (m && lane == __ffsll((unsigned long long)m) - 1) atomicAdd(&s_cnt[t], __popcll(m));
         }
-        for (int t = 1; t < ALGO_COUNT; t++) {
-            const uint64_t m = __ballot(a == t);
+    }
+    __syncthreads();
+    if (threadIdx.x < kBins && threadIdx.x > 0) {
+        const int t = threadIdx.x, n = s_cnt[t];
+        int* ctr = t == ALGO_COUNT ? box_count
+                   : t == ALGO_COUNT + 1 ? box_sep_count
+                   : t == ALGO_POLY_POLY ? hull_count : &counts[t];
+        s_base[t] = n ? atomicAdd(ctr, n) : 0;
+        if (t == ALGO_POLY_POLY) {
+            const int polys = n + s_cnt[ALGO_COUNT] + s_cnt[ALGO_COUNT + 1];
+            if (polys) atomicAdd(&counts[ALGO_POLY_POLY], polys);
+        }
+    }
+    __syncthreads();
+    for (int base = blockIdx.x * blockDim.x; base < np; base += stride) {
+        const int k = base + threadIdx.x;
+        const int b = k < np ? np_bin(A, k, false) : 0;
+        for (int t = 1; t < kBins; t++) {
+            const uint64_t m = __ballot(b == t);
             if (m == 0) continue;
             const int leader = __ffsll((unsigned long long)m) - 1;
-            if (t != ALGO_POLY_POLY) {
-                int pos = 0;
-                if (lane == leader) pos = atomicAdd(&counts[t], __popcll(m));
-                pos = __shfl(pos, leader, kWave);
-                if (a == t) lists[t * list_stride + pos + __popcll(m & lt)] = k;
-                continue;
-            }
-            if (lane == leader) atomicAdd(&counts[t], __popcll(m));
-            const uint64_t mb = __ballot(a == t && boxes);
-            const uint64_t mh = m & ~mb;
-            int pb = 0, ph = 0;
-            if (lane == leader) {
-                if (mb) pb = atomicAdd(box_count, __popcll(mb));
-                if (mh) ph = atomicAdd(hull_count, __popcll(mh));
-            }
-            pb = __shfl(pb, leader, kWave);
-            ph = __shfl(ph, leader, kWave);
-            if (a == t) {
-                if (boxes) lists[pb + __popcll(mb & lt)] = k;
-                else lists[t * list_stride + ph + __popcll(mh & lt)] = k;
-            }
+            int pos = 0;
+            if (lane == leader) pos = atomicAdd(&s_cur[t], __popcll(m));
+            pos = __shfl(pos, leader, kWave) + s_base[t] + __popcll(m & lt);
+            if (b == t) lists[np_list_slot(t) * list_stride + pos] = k;
         }
     }
 }
@@ -852,7 +912,7 @@ __global__ __launch_bounds__(128) void np_box_box(NpArgs A, const int* list, con
         const Tf T2 = tf(ld3(A.C.wpos, c.y), ldq(A.C.wq, c.y));
         const Poly P1 = box_poly(A, c.x);
         const Poly P2 = box_poly(A, c.y);
-        RawBuf<kBoxRaw> raw; raw.n = 0;
+        RawBox raw; raw.n = 0;
         uint32_t lf = A.P.lf[k];
         const bool hit = sat_poly_poly<true>(P1, P2, T1, T2, lf, raw);
         // ConvexPolyhedronVsConvexPolyhedronAlgorithm.cpp:55-62

@@ -169,10 +169,10 @@ struct DevCounters {
     int finish_rounds;         // rounds taken by the single-workgroup colouring finisher
     int nb_new_points;         // contact points without a cached impulse
     int color_left[kColorRoundsMax];   // manifolds still uncoloured after each parallel round
-    int nb_box_tests;          // box-box subset of nb_tests[ALGO_POLY_POLY]
+    int nb_box_tests;          // box-box subset of nb_tests[ALGO_POLY_POLY] that collided last step
+    int nb_box_sep_tests;      // box-box pairs that did not (cheap temporal-coherence rejects, listed apart)
     int nb_hull_tests;         // the rest of nb_tests[ALGO_POLY_POLY] (a convex mesh involved)
     int nb_walk_again;         // colliders with more partners than the traversal slab holds
-    int pad[1];
 };
 
 constexpr int STATUS_PAIR_OVERFLOW = 1;

@@ -74,10 +74,16 @@ __global__ void sv_compact_manifolds(const int* flag, const int* offs, const int
             pts += ncp[k];
         }
     }
+    // one atomic per workgroup: same-address atomics from every wave would serialise in the L2
+    __shared__ int s_pts;
+    if (threadIdx.x == 0) s_pts = 0;
+    __syncthreads();
     pts = wave_sum(pts);
-    if (lane_id() == 0 && pts) {
-        atomicAdd(counters_points, pts);
-        atomicAdd(total_points, (unsigned long long)pts);
+    if (lane_id() == 0 && pts) atomicAdd(&s_pts, pts);
+    __syncthreads();
+    if (threadIdx.x == 0 && s_pts) {
+        atomicAdd(counters_points, s_pts);
+        atomicAdd(total_points, (unsigned long long)s_pts);
     }
 }
 

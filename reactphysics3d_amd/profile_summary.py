@@ -12,6 +12,8 @@
            the scale factor that turns the counters into HBM bytes.
 --traffic-json : also write the scaled HBM bytes per dispatch of every kernel as JSON (bench.py
            reads profiles/traffic_<workload>.json for the roofline's "traffic").
+--min-us KERNEL=US : count only the dispatches of KERNEL that ran at least US microseconds (repeatable;
+           separates, e.g., residual-mode solver launches from fixed-iteration ones in one run).
 """
 from __future__ import annotations
 
@@ -29,22 +31,29 @@ def short_name(name: str) -> str:
     return base.split("::")[-1][:60]
 
 
-def kernel_trace(db: str):
+def kernel_trace(db: str, min_us=None):
     """Per kernel: list of dispatch durations in microseconds (from start/end timestamps in ns)."""
     c = sqlite3.connect(db)
     rows = c.execute("select name, start, end from kernels").fetchall()
     agg = defaultdict(list)
     for name, t0, t1 in rows:
-        agg[short_name(name)].append((t1 - t0) / 1e3)
+        k, us = short_name(name), (t1 - t0) / 1e3
+        if min_us and us < min_us.get(k, 0.0):
+            continue
+        agg[k].append(us)
     return agg
 
 
-def counter_means(db: str, counter: str):
+def counter_means(db: str, counter: str, min_us=None):
     c = sqlite3.connect(db)
-    rows = c.execute("select kernel_name, value from counters_collection where counter_name = ?", (counter,)).fetchall()
+    rows = c.execute("select kernel_name, value, start, end from counters_collection where counter_name = ?",
+                     (counter,)).fetchall()
     agg = defaultdict(list)
-    for name, v in rows:
-        agg[short_name(name)].append(float(v))
+    for name, v, t0, t1 in rows:
+        k = short_name(name)
+        if min_us and (t1 - t0) / 1e3 < min_us.get(k, 0.0):
+            continue
+        agg[k].append(float(v))
     return {k: (sum(v) / len(v), len(v)) for k, v in agg.items()}
 
 
@@ -64,12 +73,18 @@ def main(argv=None):
     ap.add_argument("--command", default="")
     ap.add_argument("--top", type=int, default=25)
     ap.add_argument("--traffic-json", help="write {kernel: scaled HBM bytes per dispatch} to this file")
+    ap.add_argument("--min-us", action="append", default=[], metavar="KERNEL=US")
     a = ap.parse_args(argv)
+    min_us = {kv.split("=")[0]: float(kv.split("=")[1]) for kv in a.min_us}
+    if min_us:
+        out_filter = ", ".join(f"{k} >= {v:g} us" for k, v in min_us.items())
     out = []
     if a.command:
         out += ["Command:", "", "```", a.command, "```", ""]
+    if min_us:
+        out += [f"Dispatches counted: {out_filter} (other kernels: all).", ""]
     if a.trace:
-        agg = kernel_trace(a.trace)
+        agg = kernel_trace(a.trace, min_us)
         tot = sum(sum(v) for v in agg.values()) or 1.0
         out += ["| kernel | calls | mean us | median us | min us | max us | total ms | share |",
                 "|---|---:|---:|---:|---:|---:|---:|---:|"]
@@ -80,8 +95,8 @@ def main(argv=None):
                        f"{t / 1e3:.3f} | {100 * t / tot:.1f}% |")
         out.append("")
     if a.fetch or a.write:
-        f = counter_means(a.fetch, "FETCH_SIZE") if a.fetch else {}
-        w = counter_means(a.write, "WRITE_SIZE") if a.write else {}
+        f = counter_means(a.fetch, "FETCH_SIZE", min_us) if a.fetch else {}
+        w = counter_means(a.write, "WRITE_SIZE", min_us) if a.write else {}
         scale_f = scale_w = None
         ck = copy_kernel(f) or copy_kernel(w)
         if ck and a.copy_bytes > 0:
