@@ -224,6 +224,7 @@ static void alloc_pairs(rp3dg_batch* b, DevPairs& P, int cap) {
     P.lf = dalloc<uint32_t>(b, cap);
     P.lfaxis = dalloc<float4>(b, cap);
     P.ncp = dalloc<int>(b, cap);
+    P.prev = dalloc<int>(b, cap);
     P.fv1 = dalloc<float4>(b, cap);
     P.fv2 = dalloc<float4>(b, cap);
     P.fimp = dalloc<float4>(b, cap);
@@ -907,7 +908,7 @@ int rp3dg_batch::enqueue_step(float dt, bool capturing) {
 
     // ---------------- narrowphase (CD.cpp:618-718)
     NpArgs A;
-    A.B = B; A.C = C; A.P = N;
+    A.B = B; A.C = C; A.P = N; A.O = O;
     A.nb_pairs = &counters->nb_pairs;
     A.hulls = d_hulls; A.hull_verts = d_hverts; A.hull_normals = d_hnormals; A.hull_edges = d_hedges;
     A.hull_face_edge = d_hfedge; A.hull_face_verts = d_hfverts; A.hull_face_vidx = d_hfvidx;

@@ -567,20 +567,13 @@ __global__ void bp_merge(DevColliders C, const uint64_t* keys, const int* nkeys_
             if (v < key) lo = mid + 1;
             else hi = mid - 1;
         }
+        N.prev[k] = found;
         if (found >= 0) {
+            // the contact cache stays in O; the narrowphase reads it there through N.prev
             N.col[k] = O.col[found];
             N.algo[k] = O.algo[found];
             N.lf[k] = O.lf[found];
             N.lfaxis[k] = O.lfaxis[found];
-            N.ncp[k] = O.ncp[found];
-            N.fv1[k] = O.fv1[found];
-            N.fv2[k] = O.fv2[found];
-            N.fimp[k] = O.fimp[found];
-            for (int p = 0; p < kMaxManifoldPoints; p++) {
-                N.pn[4 * k + p] = O.pn[4 * found + p];
-                N.pl1[4 * k + p] = O.pl1[4 * found + p];
-                N.pl2[4 * k + p] = O.pl2[4 * found + p];
-            }
         } else {
             const int a = (int)(key >> 32), b = (int)(key & 0xffffffffu);
             const int ma = C.moved[a], mb = C.moved[b];

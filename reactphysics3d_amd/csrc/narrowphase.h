@@ -8,7 +8,8 @@ namespace rp3dg {
 struct NpArgs {
     DevBodies B;
     DevColliders C;
-    DevPairs P;
+    DevPairs P;                // pair table of this step
+    DevPairs O;                // previous table: contact caches of surviving pairs (P.prev)
     const int* nb_pairs;
     const HullHeader* hulls;
     const float4* hull_verts;

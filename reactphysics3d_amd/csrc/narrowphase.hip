@@ -670,11 +670,13 @@ __device__ void reduce_points(const R& raw, const Tf& T1, int* keep) {
 template <class R>
 __device__ void contact_epilogue(const NpArgs& A, int slot, bool colliding, const R& raw, int collider1) {
     DevPairs P = A.P;
-    const int oldN = P.ncp[slot];
+    const DevPairs& O = A.O;
+    const int prev = P.prev[slot];   // this pair's entry in the previous table, -1 when new
     if (!colliding || raw.n == 0) {
         P.ncp[slot] = 0;
         return;
     }
+    const int oldN = prev >= 0 ? O.ncp[prev] : 0;
     IdxList idx;
     int n = raw.n;
     for (int i = 0; i < n; i++) idx.set(i, i);
@@ -708,17 +710,17 @@ __device__ void contact_epilogue(const NpArgs& A, int slot, bool colliding, cons
         oldrest[k] = 0.0f;
     }
     if (oldN > 0) {
-        const float4 oldn0 = P.pn[4 * slot];
+        const float4 oldn0 = O.pn[4 * prev];
         if (dot(v3(oldn0.x, oldn0.y, oldn0.z), raw.normal(idx.get(0))) >= A.cos_similar) {
-            fv1 = P.fv1[slot];
-            fv2 = P.fv2[slot];
-            fimp = P.fimp[slot];
+            fv1 = O.fv1[prev];
+            fv2 = O.fv2[prev];
+            fimp = O.fimp[prev];
         }
 #pragma unroll
         for (int k = 0; k < kMaxManifoldPoints; k++) {
             if (k < oldN) {
-                oldl1[k] = P.pl1[4 * slot + k];
-                oldrest[k] = P.pl2[4 * slot + k].w;
+                oldl1[k] = O.pl1[4 * prev + k];
+                oldrest[k] = O.pl2[4 * prev + k].w;
             }
         }
     }

@@ -119,6 +119,8 @@ struct DevPairs {
     uint32_t *lf;              // LastFrameCollisionInfo flags + SAT indices
     float4 *lfaxis;            // gjkSeparatingAxis
     int *ncp;                  // contact points in the cache (0 = no contact pair)
+    int *prev;                 // entry of the same pair in the previous table (-1: new pair); its contact
+                               // cache (ncp, points, friction) is read there, not copied
     float4 *fv1, *fv2;         // manifold friction vectors
     float4 *fimp;              // friction1, friction2, twist impulses
     float4 *pn;                // [4*slot+k] normal xyz, depth
