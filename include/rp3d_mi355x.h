@@ -120,7 +120,11 @@ int32_t rp3dg_hull_create(rp3dg_world* world, const float* vertices, int32_t nb_
 /* PhysicsWorld::createJoint (src/engine/PhysicsWorld.cpp:483-600) with anchor / axis given in world space
  * (the JointInfo "world space" constructors, e.g. BallAndSocketJointInfo(body1, body2, anchor)).
  * axis is used by hinge and slider; limits[4] = {min, max, motor speed, max motor force/torque}
- * with flags bit0 limit enabled, bit1 motor enabled.  Returns the joint index. */
+ * with flags bit0 limit enabled, bit1 motor enabled (HingeJointInfo / SliderJointInfo fields).
+ * Ball-and-socket: flags bit0 enables the cone limit with half angle limits[0] in [0, pi]
+ * (BallAndSocketJoint::enableConeLimit / setConeLimitHalfAngle, BallAndSocketJoint.cpp:71-96).
+ * Slider limits must satisfy min <= 0 <= max and the max motor force must be >= 0
+ * (SliderJoint.cpp:41-43); RP3DG_EINVAL otherwise.  Returns the joint index. */
 int32_t rp3dg_joint_create(rp3dg_world* world, int32_t type, int32_t body1, int32_t body2,
                            const float anchor_world[3], const float axis_world[3], int32_t collide_connected,
                            int32_t flags, const float limits[4]);

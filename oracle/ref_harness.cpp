@@ -179,6 +179,11 @@ void* rp3dref_create(const RefSettings* s, int nb, const RefBody* bodies, int nc
             BallAndSocketJointInfo info(b1, b2, anchor);
             info.isCollisionEnabled = J.collide != 0;
             joint = R->world->createJoint(info);
+            if (J.flags & 1) {   // cone limit with half angle limits[0]
+                BallAndSocketJoint* bs = static_cast<BallAndSocketJoint*>(joint);
+                bs->enableConeLimit(true);
+                bs->setConeLimitHalfAngle(J.limits[0]);
+            }
         } else if (J.type == 1) {
             SliderJointInfo info(b1, b2, anchor, axis);
             info.isCollisionEnabled = J.collide != 0;
@@ -319,19 +324,21 @@ static void reorder(Components& comps, const std::vector<Entity>& want) {
 int rp3dref_set_joint_order(void* h, const int32_t* order, int n) {
     RefWorld* R = (RefWorld*)h;
     PhysicsWorld& W = *R->world;
-    std::vector<Entity> bs, fx, hg;
+    std::vector<Entity> bs, fx, hg, sl;
     for (int i = 0; i < n; i++) {
         Joint* j = R->joints[order[i]];
         switch (j->getType()) {
             case JointType::BALLSOCKETJOINT: bs.push_back(j->getEntity()); break;
             case JointType::FIXEDJOINT: fx.push_back(j->getEntity()); break;
             case JointType::HINGEJOINT: hg.push_back(j->getEntity()); break;
+            case JointType::SLIDERJOINT: sl.push_back(j->getEntity()); break;
             default: return -1;
         }
     }
     reorder(W.mBallAndSocketJointsComponents, bs);
     reorder(W.mFixedJointsComponents, fx);
     reorder(W.mHingeJointsComponents, hg);
+    reorder(W.mSliderJointsComponents, sl);
     return 0;
 }
 

@@ -263,8 +263,8 @@ static void moved_ranks(int n, std::vector<int>& rank) {
 }
 
 // Joint rows.  Constants follow the joint constructors with world-space anchors
-// (BallAndSocketJoint.cpp:38-53, FixedJoint.cpp:36-56, HingeJoint.cpp:36-80); the rows are sorted by
-// (kind in solve order, colour, creation index) with a greedy colouring over all bodies.
+// (BallAndSocketJoint.cpp:38-53, FixedJoint.cpp:36-56, HingeJoint.cpp:36-80, SliderJoint.cpp:38-91); the
+// rows are sorted by (kind in solve order, colour, creation index) with a greedy colouring over all bodies.
 int rp3dg_batch::upload_joints(rp3dg_world* const* ws, int nw) {
     struct Row { int kind, color, order, b1, b2, group, slot; float4 la1, la2, ax1, ax2, qinit, lim; int flags; };
     std::vector<Row> rows;
@@ -277,7 +277,7 @@ int rp3dg_batch::upload_joints(rp3dg_world* const* ws, int nw) {
             if (hj.type == RP3DG_JOINT_BALLSOCKET) r.kind = JT_BALLSOCKET;
             else if (hj.type == RP3DG_JOINT_FIXED) r.kind = JT_FIXED;
             else if (hj.type == RP3DG_JOINT_HINGE) r.kind = JT_HINGE;
-            else { set_error("slider joints are not on the device path"); return RP3DG_EUNSUPPORTED; }
+            else r.kind = JT_SLIDER;
             r.order = jbase + (int)ji;
             r.group = world_group[wi];   // fused layout: rows are ordered by world group
             r.b1 = body_base[wi] + hj.body1;
@@ -291,6 +291,10 @@ int rp3dg_batch::upload_joints(rp3dg_world* const* ws, int nw) {
             Q4 qi = q4(0, 0, 0, 1);
             if (r.kind == JT_FIXED) {
                 qi = qmul(qconj(t2.q), t1.q);
+            } else if (r.kind == JT_SLIDER) {
+                const V3 h1 = normalized(qrot(qconj(t1.q), hj.axis_world));
+                r.ax1 = make_float4(h1.x, h1.y, h1.z, 0.f);
+                qi = qmul(qconj(t2.q), t1.q);
             } else if (r.kind == JT_HINGE) {
                 const V3 h1 = normalized(qrot(qconj(t1.q), hj.axis_world));
                 const V3 h2 = normalized(qrot(qconj(t2.q), hj.axis_world));
@@ -303,16 +307,23 @@ int rp3dg_batch::upload_joints(rp3dg_world* const* ws, int nw) {
             }
             r.qinit = make_float4(qi.x, qi.y, qi.z, qi.w);
             r.flags = ((hj.flags & 1) ? JF_LIMIT : 0) | ((hj.flags & 2) ? JF_MOTOR : 0);
-            // HingeJointInfo defaults when the limit is off (HingeJoint.h:97-105): [-1, 1]
-            r.lim = make_float4((hj.flags & 1) ? hj.limits[0] : -1.0f, (hj.flags & 1) ? hj.limits[1] : 1.0f,
-                                hj.limits[2], hj.limits[3]);
+            // HingeJointInfo / SliderJointInfo defaults when the limit is off (HingeJoint.h:97-105,
+            // SliderJoint.h:93-101): [-1, 1]; ball-and-socket: cone half angle, pi when the cone limit is
+            // off (PhysicsWorld.cpp:500)
+            if (r.kind == JT_BALLSOCKET)
+                r.lim = make_float4((hj.flags & 1) ? hj.limits[0] : kPi, 0.f, 0.f, 0.f);
+            else
+                r.lim = make_float4((hj.flags & 1) ? hj.limits[0] : -1.0f, (hj.flags & 1) ? hj.limits[1] : 1.0f,
+                                    hj.limits[2], hj.limits[3]);
+            if (r.kind == JT_BALLSOCKET) r.flags &= JF_LIMIT;   // cone limit, no motor
+            else if (r.kind == JT_FIXED) r.flags = 0;
             rows.push_back(r);
         }
         jbase += (int)W.joints.size();
     }
     // greedy colouring per kind in creation order
     std::vector<uint64_t> used(nb, 0);
-    for (int kind = 0; kind < 3; kind++) {
+    for (int kind = 0; kind < JT_KINDS; kind++) {
         std::fill(used.begin(), used.end(), 0ull);
         for (Row& r : rows) {
             if (r.kind != kind) continue;
@@ -325,10 +336,10 @@ int rp3dg_batch::upload_joints(rp3dg_world* const* ws, int nw) {
         }
     }
     // slots = the (kind, colour) classes present, in solve order
-    std::vector<int> slot_of(3 * 64, -1);
+    std::vector<int> slot_of(JT_KINDS * 64, -1);
     for (const Row& r : rows) slot_of[r.kind * 64 + r.color] = 0;
     jt_slots = 0;
-    for (int k = 0; k < 3 * 64; k++)
+    for (int k = 0; k < JT_KINDS * 64; k++)
         if (slot_of[k] >= 0) slot_of[k] = jt_slots++;
     for (Row& r : rows) r.slot = slot_of[r.kind * 64 + r.color];
     const bool wmajor = fused;
@@ -360,12 +371,13 @@ int rp3dg_batch::upload_joints(rp3dg_world* const* ws, int nw) {
     J.type = dalloc<int>(this, m); J.bodies = dalloc<int2>(this, m); J.flags = dalloc<int>(this, m);
     for (float4** a : {&J.la1, &J.la2, &J.ax1, &J.ax2, &J.qinit, &J.lim, &J.impT, &J.impR, &J.impL, &J.i1r0, &J.i1r1,
                        &J.i1r2, &J.i2r0, &J.i2r1, &J.i2r2, &J.r1, &J.r2, &J.kt0, &J.kt1, &J.kt2, &J.kr0, &J.kr1,
-                       &J.kr2, &J.a1, &J.b2xa1, &J.c2xa1})
+                       &J.kr2, &J.a1, &J.b2xa1, &J.c2xa1, &J.un1, &J.un2, &J.ua, &J.rn1, &J.rn2, &J.ra})
         *a = dalloc<float4>(this, m);
     J.imlm = dalloc<float>(this, m);
+    J.imm = dalloc<float>(this, m);
     jt_ranges = dalloc<int>(this, std::max<size_t>(2, ranges.size()));
     jt_wr = dalloc<int2>(this, wr.size());
-    if (!J.imlm || !jt_ranges || !jt_wr) { set_error("device allocation failed (joints)"); return RP3DG_EHIP; }
+    if (!J.imlm || !J.imm || !jt_ranges || !jt_wr) { set_error("device allocation failed (joints)"); return RP3DG_EHIP; }
     if (n == 0) return RP3DG_OK;
     std::vector<int> type(n), flags(n);
     std::vector<int2> bodies(n);
@@ -385,8 +397,10 @@ int rp3dg_batch::upload_joints(rp3dg_world* const* ws, int nw) {
     HIPCHECK(hipMemcpy(J.ax2, ax2.data(), 16 * n, hipMemcpyHostToDevice));
     HIPCHECK(hipMemcpy(J.qinit, qinit.data(), 16 * n, hipMemcpyHostToDevice));
     HIPCHECK(hipMemcpy(J.lim, lim.data(), 16 * n, hipMemcpyHostToDevice));
-    for (float4* a : {J.impT, J.impR, J.impL, J.a1, J.b2xa1, J.c2xa1}) HIPCHECK(hipMemcpy(a, zero.data(), 16 * n, hipMemcpyHostToDevice));
+    for (float4* a : {J.impT, J.impR, J.impL, J.a1, J.b2xa1, J.c2xa1, J.ra})
+        HIPCHECK(hipMemcpy(a, zero.data(), 16 * n, hipMemcpyHostToDevice));
     HIPCHECK(hipMemset(J.imlm, 0, sizeof(float) * n));
+    HIPCHECK(hipMemset(J.imm, 0, sizeof(float) * n));
     if (!ranges.empty()) HIPCHECK(hipMemcpy(jt_ranges, ranges.data(), sizeof(int) * ranges.size(), hipMemcpyHostToDevice));
     HIPCHECK(hipMemcpy(jt_wr, wr.data(), sizeof(int2) * wr.size(), hipMemcpyHostToDevice));
     return RP3DG_OK;

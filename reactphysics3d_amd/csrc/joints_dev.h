@@ -1,12 +1,12 @@
-// Joint rows of the MI355X step path, one joint per call: ball-and-socket, fixed and hinge joints
-// (src/systems/Solve{BallAndSocket,Fixed,Hinge}JointSystem.cpp) with the default
-// non-linear Gauss-Seidel position correction.
+// Joint rows of the MI355X step path, one joint per call: ball-and-socket (with its cone limit), fixed,
+// hinge and slider joints (src/systems/Solve{BallAndSocket,Fixed,Hinge,Slider}JointSystem.cpp) with the
+// default non-linear Gauss-Seidel position correction.
 //
 // Joints never change during a run, so the host colours every joint type once
 // (no two joints of a colour share a body, static bodies included: the position
 // solver rewrites and renormalises the constrained orientation of both bodies,
 // static ones too).  Joints of one colour of one type are solved in parallel;
-// types follow the reference's order BallAndSocket -> Fixed -> Hinge
+// types follow the reference's order BallAndSocket -> Fixed -> Hinge -> Slider
 // (ConstraintSolverSystem.cpp:94-113).  Used by the per-colour kernels (joints.hip)
 // and by the fused per-world solver (fused.hip).
 #pragma once
@@ -112,7 +112,7 @@ __device__ inline float hinge_angle(Q4 q1, Q4 q2, Q4 qinit, V3 a1, float lo, flo
     const V3 vv = v3(rel.x, rel.y, rel.z);
     const float s = len(vv);
     const float d = dot(vv, a1);
-    float ang = d >= 0.0f ? 2.0f * atan2f(s, c) : 2.0f * atan2f(s, -c);
+    float ang = d >= 0.0f ? 2.0f * ref_atan2f(s, c) : 2.0f * ref_atan2f(s, -c);
     ang = normalized_angle(ang);
     return angle_near_limits(ang, lo, hi);
 }
@@ -140,7 +140,26 @@ __device__ __forceinline__ void jt_init_one(const DevBodies& B, const DevJoints&
         stm(J.kt0, J.kt1, J.kt2, j, Kt);
         if (t == JT_BALLSOCKET) {
             // cone-limit axis a x b with a = unit(r1), b = -unit(r2) (SolveBallAndSocketJointSystem.cpp:113-115)
-            J.a1[j] = f4(cross(unit(r1), -unit(r2)), 0.f);
+            const V3 r1u = unit(r1), r2u = -unit(r2);
+            const V3 acb = cross(r1u, r2u);
+            J.a1[j] = f4(acb, 0.f);
+            int fl = J.flags[j];
+            if (fl & JF_LIMIT) {
+                // SolveBallAndSocketJointSystem.cpp:117-146.  A disabled limit keeps the half angle pi
+                // (PhysicsWorld.cpp:500), which acos never exceeds: never violated, impulse stays 0.
+                const float err = J.lim[j].x - ref_acosf(dot(r1u, r2u));
+                const bool old = (fl & JF_LOWER_VIOLATED) != 0;
+                const bool vio = err < 0.0f;
+                fl = vio ? (fl | JF_LOWER_VIOLATED) : (fl & ~JF_LOWER_VIOLATED);
+                if (!vio || vio != old) {
+                    float4 il = J.impL[j];
+                    il.x = 0.0f;
+                    J.impL[j] = il;
+                }
+                float m = dot(acb, mvmul(I1, acb)) + dot(acb, mvmul(I2, acb));
+                J.imlm[j] = m > 0.0f ? 1.0f / m : 0.0f;
+                J.flags[j] = fl;
+            }
         } else if (t == JT_FIXED) {
             // SolveFixedJointSystem.cpp:120-129: keeps I1 + I2 when it cannot be inverted
             M3 Kr = madd(I1, I2);
@@ -148,7 +167,7 @@ __device__ __forceinline__ void jt_init_one(const DevBodies& B, const DevJoints&
             if (fabsf(det) > kMachineEps && anyDyn) Kr = minv(Kr, det);
             stm(J.kr0, J.kr1, J.kr2, j, Kr);
         } else if (t == JT_HINGE) {
-            V3 a1 = normalized(qrot(q1, ld3(J.ax1, j)));
+            const V3 a1 = normalized(qrot(q1, ld3(J.ax1, j)));
             const V3 a2 = normalized(qrot(q2, ld3(J.ax2, j)));
             const V3 b2v = one_unit_orthogonal(a2);
             const V3 c2v = cross(a2, b2v);
@@ -189,6 +208,61 @@ __device__ __forceinline__ void jt_init_one(const DevBodies& B, const DevJoints&
                 J.flags[j] = fl;
                 J.impL[j] = il;
             }
+        } else {   // slider, SolveSliderJointSystem.cpp:52-237
+            const V3 ax = normalized(qrot(q1, ld3(J.ax1, j)));
+            const V3 n1 = one_unit_orthogonal(ax);
+            const V3 n2 = cross(ax, n1);
+            const V3 u = ld3(B.x, b2) + r2 - ld3(B.x, b1) - r1;
+            const V3 rpu = r1 + u;
+            const V3 un1 = cross(rpu, n1), un2 = cross(rpu, n2), ua = cross(rpu, ax);
+            J.a1[j] = f4(ax, 0.f);
+            J.b2xa1[j] = f4(n1, 0.f);
+            J.c2xa1[j] = f4(n2, 0.f);
+            J.un1[j] = f4(un1, 0.f);
+            J.un2[j] = f4(un2, 0.f);
+            J.ua[j] = f4(ua, 0.f);
+            // limit state is tracked whether or not the limit is enabled
+            const float ud = dot(u, ax);
+            const float4 lim = J.lim[j];
+            const float le = ud - lim.x, ue = lim.y - ud;
+            int fl = J.flags[j];
+            float4 il = J.impL[j];
+            const bool oldL = (fl & JF_LOWER_VIOLATED) != 0, oldU = (fl & JF_UPPER_VIOLATED) != 0;
+            const bool vl = le <= 0, vu = ue <= 0;
+            fl = vl ? (fl | JF_LOWER_VIOLATED) : (fl & ~JF_LOWER_VIOLATED);
+            fl = vu ? (fl | JF_UPPER_VIOLATED) : (fl & ~JF_UPPER_VIOLATED);
+            if (!vl || vl != oldL) il.x = 0.0f;
+            if (!vu || vu != oldU) il.y = 0.0f;
+            J.flags[j] = fl;
+            J.impL[j] = il;
+            const float ims = im1 + im2;
+            if ((fl & JF_LIMIT) && (vl || vu)) {
+                // reads r2 x axis of the previous step (recomputed only below)
+                const V3 ra_old = ld3(J.ra, j);
+                const float m = ims + dot(ua, mvmul(I1, ua)) + dot(ra_old, mvmul(I2, ra_old));
+                J.imlm[j] = m > 0.0f ? 1.0f / m : 0.0f;
+            }
+            const V3 rn1 = cross(r2, n1), rn2 = cross(r2, n2);
+            J.rn1[j] = f4(rn1, 0.f);
+            J.rn2[j] = f4(rn2, 0.f);
+            J.ra[j] = f4(cross(r2, ax), 0.f);
+            const V3 i1n1 = mvmul(I1, un1), i1n2 = mvmul(I1, un2), i2n1 = mvmul(I2, rn1), i2n2 = mvmul(I2, rn2);
+            const float e11 = ims + dot(un1, i1n1) + dot(rn1, i2n1);
+            const float e12 = dot(un1, i1n2) + dot(rn1, i2n2);
+            const float e21 = dot(un2, i1n1) + dot(rn2, i2n1);
+            const float e22 = ims + dot(un2, i1n2) + dot(rn2, i2n2);
+            float4 kt = make_float4(0.f, 0.f, 0.f, 0.f);
+            const float det = e11 * e22 - e21 * e12;
+            if (fabsf(det) > kMachineEps && anyDyn) {
+                const float id = 1.0f / det;
+                kt = make_float4(e22 * id, -e12 * id, -e21 * id, e11 * id);
+            }
+            J.kt0[j] = kt;
+            M3 Kr = madd(I1, I2);
+            const float detr = mdet(Kr);
+            if (fabsf(detr) > kMachineEps && anyDyn) Kr = minv(Kr, detr);
+            stm(J.kr0, J.kr1, J.kr2, j, Kr);
+            if (fl & JF_MOTOR) J.imm[j] = ims > 0.0f ? 1.0f / ims : 0.0f;
         }
     }
 }
@@ -211,9 +285,9 @@ __device__ __forceinline__ void jt_warmstart_one(const DevBodies& B, const DevJo
         const V3 r1 = ld3(J.r1, j), r2 = ld3(J.r2, j);
         const V3 it = ld3(J.impT, j);
         if (type == JT_BALLSOCKET) {   // SolveBallAndSocketJointSystem.cpp:196-241
-            // cone limit disabled: its accumulated impulse stays 0, but 0 * (a x b) is still added
-            // (keeps the sign of zero components identical to the reference)
-            const V3 cone = 0.0f * ld3(J.a1, j);
+            // the cone impulse stays 0 while the limit is disabled; 0 * (a x b) is still added (keeps the
+            // sign of zero components identical to the reference)
+            const V3 cone = J.impL[j].x * ld3(J.a1, j);
             const V3 li1 = -it;
             V3 ai1 = cross(it, r1);
             ai1 += cone;
@@ -234,7 +308,7 @@ __device__ __forceinline__ void jt_warmstart_one(const DevBodies& B, const DevJo
             ai2 += ir;
             v2 += lock_mul((im2 * ll2), it);
             w2 += cmul(al2, mvmul(I2, ai2));
-        } else {   // hinge, SolveHingeJointSystem.cpp:216-270
+        } else if (type == JT_HINGE) {   // SolveHingeJointSystem.cpp:216-270
             const float4 ir = J.impR[j];
             const float4 il = J.impL[j];
             const V3 a1 = ld3(J.a1, j), bxa = ld3(J.b2xa1, j), cxa = ld3(J.c2xa1, j);
@@ -253,6 +327,30 @@ __device__ __forceinline__ void jt_warmstart_one(const DevBodies& B, const DevJo
             ai2 += -limI;
             ai2 += -motI;
             v2 += lock_mul((im2 * ll2), it);
+            w2 += cmul(al2, mvmul(I2, ai2));
+        } else {   // slider, SolveSliderJointSystem.cpp:240-314
+            const float4 ir = J.impR[j];
+            const float4 il = J.impL[j];
+            const V3 ax = ld3(J.a1, j), n1 = ld3(J.b2xa1, j), n2 = ld3(J.c2xa1, j);
+            const V3 un1 = ld3(J.un1, j), un2 = ld3(J.un2, j);
+            const float lims = il.y - il.x;
+            const V3 linLim = lims * ax;
+            const V3 motI = il.z * ax;
+            V3 li1 = (-n1) * it.x - n2 * it.y;
+            V3 ai1 = (-un1) * it.x - un2 * it.y;
+            ai1 += -v3(ir.x, ir.y, ir.z);
+            li1 += linLim;
+            ai1 += lims * ld3(J.ua, j);
+            li1 += motI;
+            v1 += lock_mul((im1 * ll1), li1);
+            w1 += cmul(al1, mvmul(I1, ai1));
+            V3 li2 = n1 * it.x + n2 * it.y;
+            V3 ai2 = ld3(J.rn1, j) * it.x + ld3(J.rn2, j) * it.y;
+            ai2 += v3(ir.x, ir.y, ir.z);
+            li2 += -linLim;
+            ai2 += (-lims) * ld3(J.ra, j);
+            li2 += -motI;
+            v2 += lock_mul((im2 * ll2), li2);
             w2 += cmul(al2, mvmul(I2, ai2));
         }
         B.cv[u1] = f4(v1, 0.f); B.cw[u1] = f4(w1, 0.f);
@@ -278,6 +376,18 @@ __device__ __forceinline__ void jt_solve_velocity_one(const DevBodies& B, const 
         const V3 r1 = ld3(J.r1, j), r2 = ld3(J.r2, j);
         const M3 Kt = ldm(J.kt0, J.kt1, J.kt2, j);
         if (type == JT_BALLSOCKET) {   // SolveBallAndSocketJointSystem.cpp:245-308
+            if ((J.flags[j] & (JF_LIMIT | JF_LOWER_VIOLATED)) == (JF_LIMIT | JF_LOWER_VIOLATED)) {
+                const V3 acb = ld3(J.a1, j);
+                const float JvC = dot(acb, w1 - w2);
+                float d = J.imlm[j] * (-JvC - 0.0f);
+                float4 il = J.impL[j];
+                const float lt = il.x;
+                il.x = fmax_s(il.x + d, 0.0f);
+                d = il.x - lt;
+                J.impL[j] = il;
+                w1 += cmul(al1, mvmul(I1, d * acb));
+                w2 += cmul(al2, mvmul(I2, (-d) * acb));
+            }
             const V3 Jv = v2 + cross(w2, r2) - v1 - cross(w1, r1);
             const V3 dl = mvmul(Kt, -Jv - v3(0.0f, 0.0f, 0.0f));
             J.impT[j] = f4(ld3(J.impT, j) + dl, 0.f);
@@ -306,7 +416,7 @@ __device__ __forceinline__ void jt_solve_velocity_one(const DevBodies& B, const 
             ai1 = -dl2;
             w1 += cmul(al1, mvmul(I1, ai1));
             w2 += cmul(al2, mvmul(I2, dl2));
-        } else {   // hinge, SolveHingeJointSystem.cpp:273-440
+        } else if (type == JT_HINGE) {   // SolveHingeJointSystem.cpp:273-440
             const V3 a1 = ld3(J.a1, j);
             const int fl = J.flags[j];
             float4 il = J.impL[j];
@@ -374,6 +484,74 @@ __device__ __forceinline__ void jt_solve_velocity_one(const DevBodies& B, const 
             ai2 = -cross(dl, r2);
             v2 += lock_mul((im2 * ll2), dl);
             w2 += cmul(al2, mvmul(I2, ai2));
+        } else {   // slider, SolveSliderJointSystem.cpp:317-510
+            const V3 ax = ld3(J.a1, j), ua = ld3(J.ua, j), ra = ld3(J.ra, j);
+            const int fl = J.flags[j];
+            float4 il = J.impL[j];
+            if (fl & JF_LIMIT) {
+                const float imlm = J.imlm[j];
+                if (fl & JF_LOWER_VIOLATED) {
+                    const float JvL = dot(ax, v2) + dot(ra, w2) - dot(ax, v1) - dot(ua, w1);
+                    float d = imlm * (-JvL - 0.0f);
+                    const float lt = il.x;
+                    il.x = fmax_s(il.x + d, 0.0f);
+                    d = il.x - lt;
+                    v1 += lock_mul((im1 * ll1), (-d) * ax);
+                    w1 += cmul(al1, mvmul(I1, (-d) * ua));
+                    v2 += lock_mul((im2 * ll2), d * ax);
+                    w2 += cmul(al2, mvmul(I2, d * ra));
+                }
+                if (fl & JF_UPPER_VIOLATED) {
+                    const float JvU = dot(ax, v1) + dot(ua, w1) - dot(ax, v2) - dot(ra, w2);
+                    float d = imlm * (-JvU - 0.0f);
+                    const float lt = il.y;
+                    il.y = fmax_s(il.y + d, 0.0f);
+                    d = il.y - lt;
+                    v1 += lock_mul((im1 * ll1), d * ax);
+                    w1 += cmul(al1, mvmul(I1, d * ua));
+                    v2 += lock_mul((im2 * ll2), (-d) * ax);
+                    w2 += cmul(al2, mvmul(I2, (-d) * ra));
+                }
+            }
+            if (fl & JF_MOTOR) {
+                const float4 lim = J.lim[j];
+                const float JvM = dot(ax, v1) - dot(ax, v2);
+                const float maxI = lim.w * dt;
+                float d = J.imm[j] * (-JvM - lim.z);
+                const float lt = il.z;
+                il.z = clampf(il.z + d, -maxI, maxI);
+                d = il.z - lt;
+                v1 += lock_mul((im1 * ll1), d * ax);
+                v2 += lock_mul((im2 * ll2), (-d) * ax);
+            }
+            J.impL[j] = il;
+            // 3 rotation rows
+            const V3 Jvr = w2 - w1;
+            const M3 Kr = ldm(J.kr0, J.kr1, J.kr2, j);
+            const V3 dr = mvmul(Kr, -Jvr - v3(0.0f, 0.0f, 0.0f));
+            J.impR[j] = f4(ld3(J.impR, j) + dr, 0.f);
+            w1 += cmul(al1, mvmul(I1, -dr));
+            w2 += cmul(al2, mvmul(I2, dr));
+            // 2 translation rows
+            const V3 n1 = ld3(J.b2xa1, j), n2 = ld3(J.c2xa1, j);
+            const V3 un1 = ld3(J.un1, j), un2 = ld3(J.un2, j), rn1 = ld3(J.rn1, j), rn2 = ld3(J.rn2, j);
+            const float el1 = -dot(n1, v1) - dot(w1, un1) + dot(n1, v2) + dot(w2, rn1);
+            const float el2 = -dot(n2, v1) - dot(w1, un2) + dot(n2, v2) + dot(w2, rn2);
+            const float x0 = -el1 - 0.0f, x1 = -el2 - 0.0f;
+            const float4 kt = J.kt0[j];
+            const float d0 = kt.x * x0 + kt.y * x1;
+            const float d1 = kt.z * x0 + kt.w * x1;
+            float4 it = J.impT[j];
+            it.x += d0;
+            it.y += d1;
+            J.impT[j] = it;
+            const V3 li1 = (-n1) * d0 - n2 * d1;
+            const V3 ai1 = (-un1) * d0 - un2 * d1;
+            v1 += lock_mul((im1 * ll1), li1);
+            w1 += cmul(al1, mvmul(I1, ai1));
+            const V3 ai2 = rn1 * d0 + rn2 * d1;
+            v2 += lock_mul((im2 * ll2), -li1);
+            w2 += cmul(al2, mvmul(I2, ai2));
         }
         B.cv[u1] = f4(v1, 0.f); B.cw[u1] = f4(w1, 0.f);
         B.cv[u2] = f4(v2, 0.f); B.cw[u2] = f4(w2, 0.f);
@@ -401,6 +579,20 @@ __device__ __forceinline__ void jt_solve_position_one(const DevBodies& B, const 
         const V3 r1 = qrot(q1, ld3(J.la1, j) - ld3(B.com, b1));
         const V3 r2 = qrot(q2, ld3(J.la2, j) - ld3(B.com, b2));
         if (type == JT_BALLSOCKET) {   // SolveBallAndSocketJointSystem.cpp:312-445
+            int fl = J.flags[j];
+            if (fl & JF_LIMIT) {
+                const V3 r1u = unit(r1), r2u = -unit(r2);
+                const V3 acb = cross(r1u, r2u);
+                const float err = J.lim[j].x - ref_acosf(dot(r1u, r2u));
+                fl = err < 0.0f ? (fl | JF_LOWER_VIOLATED) : (fl & ~JF_LOWER_VIOLATED);
+                J.flags[j] = fl;
+                if (fl & JF_LOWER_VIOLATED) {
+                    const float m = dot(acb, mvmul(I1, acb)) + dot(acb, mvmul(I2, acb));
+                    const float l = (m > 0.0f ? 1.0f / m : 0.0f) * (-err);
+                    q1 = nudge(q1, cmul(al1, mvmul(I1, l * acb)));
+                    q2 = nudge(q2, cmul(al2, mvmul(I2, (-l) * acb)));
+                }
+            }
             bool ok;
             const M3 Kt = translation_inverse_mass(im1 + im2, r1, r2, I1, I2, anyDyn, ok);
             if (ok) {
@@ -449,7 +641,7 @@ __device__ __forceinline__ void jt_solve_position_one(const DevBodies& B, const 
                 const V3 dw2 = cmul(al2, mvmul(I2, lr));
                 q2 = nudge(q2, dw2);
             }
-        } else {   // hinge, SolveHingeJointSystem.cpp:444-730
+        } else if (type == JT_HINGE) {   // SolveHingeJointSystem.cpp:444-730
             V3 a1 = normalized(qrot(q1, ld3(J.ax1, j)));
             const V3 a2 = normalized(qrot(q2, ld3(J.ax2, j)));
             const V3 b2v = one_unit_orthogonal(a2);
@@ -521,6 +713,82 @@ __device__ __forceinline__ void jt_solve_position_one(const DevBodies& B, const 
                 const V3 dw2 = cmul(al2, mvmul(I2, ai2));
                 x2 += dv2;
                 q2 = nudge(q2, dw2);
+            }
+        } else {   // slider, SolveSliderJointSystem.cpp:514-826
+            const V3 u = x2 + r2 - x1 - r1;
+            const V3 ax = normalized(qrot(q1, ld3(J.ax1, j)));
+            const V3 n1 = one_unit_orthogonal(ax);
+            const V3 n2 = cross(ax, n1);
+            const float4 lim = J.lim[j];
+            const float ud = dot(u, ax);
+            int fl = J.flags[j];
+            fl = (ud - lim.x) <= 0 ? (fl | JF_LOWER_VIOLATED) : (fl & ~JF_LOWER_VIOLATED);
+            fl = (lim.y - ud) <= 0 ? (fl | JF_UPPER_VIOLATED) : (fl & ~JF_UPPER_VIOLATED);
+            J.flags[j] = fl;
+            const V3 rn1 = cross(r2, n1), rn2 = cross(r2, n2), ra = cross(r2, ax);
+            J.ra[j] = f4(ra, 0.f);   // read by the next step's limit inverse mass
+            const V3 rpu = r1 + u;
+            const V3 un1 = cross(rpu, n1), un2 = cross(rpu, n2), ua = cross(rpu, ax);
+            const float ims = im1 + im2;
+            if (fl & JF_LIMIT) {
+                float imlm = J.imlm[j];
+                if (fl & (JF_LOWER_VIOLATED | JF_UPPER_VIOLATED)) {
+                    const float m = ims + dot(ua, mvmul(I1, ua)) + dot(ra, mvmul(I2, ra));
+                    imlm = m > 0.0f ? 1.0f / m : 0.0f;
+                    J.imlm[j] = imlm;
+                }
+                if (fl & JF_LOWER_VIOLATED) {
+                    const float err = dot(x2 + r2 - x1 - r1, ax) - lim.x;
+                    const float l = imlm * (-err);
+                    x1 += lock_mul((im1 * ll1), (-l) * ax);
+                    q1 = nudge(q1, cmul(al1, mvmul(I1, (-l) * ua)));
+                    x2 += lock_mul((im2 * ll2), l * ax);
+                    q2 = nudge(q2, cmul(al2, mvmul(I2, l * ra)));
+                }
+                if (fl & JF_UPPER_VIOLATED) {
+                    const float err = lim.y - dot(x2 + r2 - x1 - r1, ax);
+                    const float l = imlm * (-err);
+                    x1 += lock_mul((im1 * ll1), l * ax);
+                    q1 = nudge(q1, cmul(al1, mvmul(I1, l * ua)));
+                    x2 += lock_mul((im2 * ll2), (-l) * ax);
+                    q2 = nudge(q2, cmul(al2, mvmul(I2, (-l) * ra)));
+                }
+            }
+            // 3 rotation rows
+            M3 Kr = madd(I1, I2);
+            const float detr = mdet(Kr);
+            if (fabsf(detr) > kMachineEps) {
+                if (anyDyn) Kr = minv(Kr, detr);
+                const Q4 qe = qmul(qmul(q2, ldq(J.qinit, j)), qconj(q1));
+                const V3 er = 2.0f * v3(qe.x, qe.y, qe.z);
+                const V3 lr = mvmul(Kr, -er);
+                q1 = nudge(q1, cmul(al1, mvmul(I1, -lr)));
+                q2 = nudge(q2, cmul(al2, mvmul(I2, lr)));
+            }
+            // 2 translation rows (with u and the cross products from before the corrections above)
+            const V3 i1n1 = mvmul(I1, un1), i1n2 = mvmul(I1, un2), i2n1 = mvmul(I2, rn1), i2n2 = mvmul(I2, rn2);
+            const float e11 = ims + dot(un1, i1n1) + dot(rn1, i2n1);
+            const float e12 = dot(un1, i1n2) + dot(rn1, i2n2);
+            const float e21 = dot(un2, i1n1) + dot(rn2, i2n1);
+            const float e22 = ims + dot(un2, i1n2) + dot(rn2, i2n2);
+            const float det = e11 * e22 - e21 * e12;
+            if (fabsf(det) > kMachineEps) {
+                float k00 = 0, k01 = 0, k10 = 0, k11 = 0;
+                if (anyDyn) {
+                    const float id = 1.0f / det;
+                    k00 = e22 * id; k01 = -e12 * id; k10 = -e21 * id; k11 = e11 * id;
+                }
+                const float t0 = -dot(u, n1), t1 = -dot(u, n2);
+                const float l0 = k00 * t0 + k01 * t1;
+                const float l1 = k10 * t0 + k11 * t1;
+                const V3 li1 = (-n1) * l0 - n2 * l1;
+                const V3 ai1 = (-un1) * l0 - un2 * l1;
+                x1 += lock_mul((im1 * ll1), li1);
+                q1 = nudge(q1, cmul(al1, mvmul(I1, ai1)));
+                const V3 li2 = n1 * l0 + n2 * l1;
+                const V3 ai2 = rn1 * l0 + rn2 * l1;
+                x2 += lock_mul((im2 * ll2), li2);
+                q2 = nudge(q2, cmul(al2, mvmul(I2, ai2)));
             }
         }
         B.cx[b1] = f4(x1, 0.f); B.cq[b1] = f4q(q1);

@@ -19,6 +19,125 @@ constexpr float kDecimalLargest = 3.40282346638528859812e+38f;  // configuration
 constexpr float kDecimalSmallest = -3.40282346638528859812e+38f;  // configuration.h:114
 constexpr float kPi = 3.141592653589f;                          // configuration.h:123
 
+// ---- acos / atan2 with the bits of the reference's C library ----
+// The reference calls std::acos / std::atan2 on floats (SolveBallAndSocketJointSystem.h:153,
+// SolveHingeJointSystem.cpp:786-789), i.e. acosf / atan2f of the x86-64 glibc it links, whose float
+// versions follow fdlibm's e_acosf.c, s_atanf.c and e_atan2f.c (Sun Microsystems, 1993).  The GPU's own
+// acosf / atan2f differ from those by 1-3 ulp on about a third of the arguments, which the hinge and
+// cone limits turn into different limit decisions, so the step uses this restatement of the fdlibm
+// method instead: same constants (as the float values the decimal literals round to), same operation
+// order.  Compiled with -ffp-contract=off and IEEE division / sqrt it returns glibc's bits (checked on
+// every float in [-1, 1] for acos and on 6e8 arguments for atan / atan2).
+RD float fbits_as(int32_t i) { return __builtin_bit_cast(float, i); }
+RD int32_t fbits_of(float f) { return __builtin_bit_cast(int32_t, f); }
+
+RD float ref_acosf(float x) {
+    const float one = 1.0f, pi = fbits_as(0x40490fda), pio2_hi = fbits_as(0x3fc90fda), pio2_lo = fbits_as(0x33a22168);
+    const float pS0 = fbits_as(0x3e2aaaab), pS1 = fbits_as((int32_t)0xbea6b090), pS2 = fbits_as(0x3e4e0aa8),
+                pS3 = fbits_as((int32_t)0xbd241146), pS4 = fbits_as(0x3a4f7f04), pS5 = fbits_as(0x3811ef08);
+    const float qS1 = fbits_as((int32_t)0xc019d139), qS2 = fbits_as(0x4001572d), qS3 = fbits_as((int32_t)0xbf303361),
+                qS4 = fbits_as(0x3d9dc62e);
+    const int32_t hx = fbits_of(x), ix = hx & 0x7fffffff;
+    if (ix == 0x3f800000) return hx > 0 ? 0.0f : pi + 2.0f * pio2_lo;   // |x| = 1
+    if (ix > 0x3f800000) return (x - x) / (x - x);                        // |x| > 1: NaN
+    if (ix < 0x3f000000) {                                                 // |x| < 0.5
+        if (ix <= 0x23000000) return pio2_hi + pio2_lo;
+        const float z = x * x;
+        const float p = z * (pS0 + z * (pS1 + z * (pS2 + z * (pS3 + z * (pS4 + z * pS5)))));
+        const float q = one + z * (qS1 + z * (qS2 + z * (qS3 + z * qS4)));
+        const float r = p / q;
+        return pio2_hi - (x - (pio2_lo - x * r));
+    }
+    if (hx < 0) {                                                          // x <= -0.5
+        const float z = (one + x) * 0.5f;
+        const float p = z * (pS0 + z * (pS1 + z * (pS2 + z * (pS3 + z * (pS4 + z * pS5)))));
+        const float q = one + z * (qS1 + z * (qS2 + z * (qS3 + z * qS4)));
+        const float s = sqrtf(z);
+        const float r = p / q;
+        const float w = r * s - pio2_lo;
+        return pi - 2.0f * (s + w);
+    }
+    const float z = (one - x) * 0.5f;                                      // x >= 0.5
+    const float s = sqrtf(z);
+    const float df = fbits_as(fbits_of(s) & (int32_t)0xfffff000);
+    const float c = (z - df * df) / (s + df);
+    const float p = z * (pS0 + z * (pS1 + z * (pS2 + z * (pS3 + z * (pS4 + z * pS5)))));
+    const float q = one + z * (qS1 + z * (qS2 + z * (qS3 + z * qS4)));
+    const float r = p / q;
+    const float w = r * s + c;
+    return 2.0f * (df + w);
+}
+
+RD float ref_atanf(float x) {
+    const float one = 1.0f;
+    const int32_t hx = fbits_of(x), ix = hx & 0x7fffffff;
+    int id;
+    if (ix >= 0x4c000000) {                                                // |x| >= 2^25
+        if (ix > 0x7f800000) return x + x;
+        const float hi = fbits_as(0x3fc90fda), lo = fbits_as(0x33a22168);
+        return hx > 0 ? hi + lo : -hi - lo;
+    }
+    if (ix < 0x3ee00000) {                                                 // |x| < 7/16
+        if (ix < 0x31000000) return x;
+        id = -1;
+    } else {
+        x = fabsf(x);
+        if (ix < 0x3f980000) {
+            if (ix < 0x3f300000) { id = 0; x = (2.0f * x - one) / (2.0f + x); }
+            else { id = 1; x = (x - one) / (x + one); }
+        } else if (ix < 0x401c0000) {
+            id = 2; x = (x - 1.5f) / (one + 1.5f * x);
+        } else {
+            id = 3; x = -1.0f / x;
+        }
+    }
+    // atan(x) = x - x^3 a0 + x^5 a1 - ...: odd and even halves of the series in w = x^4
+    const float a0 = fbits_as(0x3eaaaaab), a1 = fbits_as((int32_t)0xbe4ccccd), a2 = fbits_as(0x3e124925),
+                a3 = fbits_as((int32_t)0xbde38e38), a4 = fbits_as(0x3dba2e6e), a5 = fbits_as((int32_t)0xbd9d8795),
+                a6 = fbits_as(0x3d886b35), a7 = fbits_as((int32_t)0xbd6ef16b), a8 = fbits_as(0x3d4bda59),
+                a9 = fbits_as((int32_t)0xbd15a221), a10 = fbits_as(0x3c8569d7);
+    float z = x * x;
+    const float w = z * z;
+    const float s1 = z * (a0 + w * (a2 + w * (a4 + w * (a6 + w * (a8 + w * a10)))));
+    const float s2 = w * (a1 + w * (a3 + w * (a5 + w * (a7 + w * a9))));
+    if (id < 0) return x - x * (s1 + s2);
+    // atan(0.5), atan(1), atan(1.5), atan(inf) split into high and low parts
+    const float hi = id == 0 ? fbits_as(0x3eed6338) : id == 1 ? fbits_as(0x3f490fda)
+                   : id == 2 ? fbits_as(0x3f7b985e) : fbits_as(0x3fc90fda);
+    const float lo = id == 0 ? fbits_as(0x31ac3769) : id == 1 ? fbits_as(0x33222168)
+                   : id == 2 ? fbits_as(0x33140fb4) : fbits_as(0x33a22168);
+    z = hi - ((x * (s1 + s2) - lo) - x);
+    return hx < 0 ? -z : z;
+}
+
+RD float ref_atan2f(float y, float x) {
+    const float tiny = 1.0e-30f, pi_o_4 = fbits_as(0x3f490fdb), pi_o_2 = fbits_as(0x3fc90fdb),
+                pi = fbits_as(0x40490fdb), pi_lo = fbits_as((int32_t)0xb3bbbd2e);
+    const int32_t hx = fbits_of(x), ix = hx & 0x7fffffff, hy = fbits_of(y), iy = hy & 0x7fffffff;
+    if (ix > 0x7f800000 || iy > 0x7f800000) return x + y;                // NaN
+    if (hx == 0x3f800000) return ref_atanf(y);                            // x = 1
+    const int m = ((hy >> 31) & 1) | ((hx >> 30) & 2);                    // 2 sign(x) + sign(y)
+    if (iy == 0) return m <= 1 ? y : (m == 2 ? pi + tiny : -pi - tiny);
+    if (ix == 0) return hy < 0 ? -pi_o_2 - tiny : pi_o_2 + tiny;
+    if (ix == 0x7f800000) {
+        if (iy == 0x7f800000)
+            return m == 0 ? pi_o_4 + tiny : m == 1 ? -pi_o_4 - tiny : m == 2 ? 3.0f * pi_o_4 + tiny : -3.0f * pi_o_4 - tiny;
+        return m == 0 ? 0.0f : m == 1 ? -0.0f : m == 2 ? pi + tiny : -pi - tiny;
+    }
+    if (iy == 0x7f800000) return hy < 0 ? -pi_o_2 - tiny : pi_o_2 + tiny;
+    const int k = (iy - ix) >> 23;
+    float z;
+    if (k > 60) z = pi_o_2 + 0.5f * pi_lo;                                // |y / x| > 2^60
+    else if (hx < 0 && k < -60) z = 0.0f;
+    else z = ref_atanf(fabsf(y / x));
+    switch (m) {
+        case 0: return z;
+        case 1: return -z;
+        case 2: return pi - (z - pi_lo);
+        default: return (z - pi_lo) - pi;
+    }
+}
+
 struct V3 { float x, y, z; };
 struct Q4 { float x, y, z, w; };
 struct Tf { V3 p; Q4 q; };

@@ -493,6 +493,14 @@ int32_t rp3dg_joint_create(rp3dg_world* w, int32_t type, int32_t b1, int32_t b2,
     if (!mutable_world(w)) return RP3DG_ESTATE;
     if (b1 == b2) { set_error("a joint needs two different bodies"); return RP3DG_EINVAL; }
     if (type < RP3DG_JOINT_BALLSOCKET || type > RP3DG_JOINT_FIXED) { set_error("invalid joint type"); return RP3DG_EINVAL; }
+    if (type == RP3DG_JOINT_SLIDER && limits) {
+        // SliderJoint.cpp:41-43: lower limit <= 0 <= upper limit, max motor force >= 0
+        if ((flags & 1) && !(limits[0] <= 0.0f && limits[1] >= 0.0f)) {
+            set_error("slider limits must satisfy min <= 0 <= max");
+            return RP3DG_EINVAL;
+        }
+        if ((flags & 2) && !(limits[3] >= 0.0f)) { set_error("slider max motor force must be >= 0"); return RP3DG_EINVAL; }
+    }
     HostJoint j;
     j.type = type;
     j.body1 = b1;

@@ -244,9 +244,10 @@ def sphere_chains(n_chains=16, length=16, radius=0.3, spacing=3.0, seed=3, setti
 
 
 def jointed_tower(height=6, seed=4, kinds=(HINGE, FIXED), friction=0.5, gravity=(0.0, 0.0, -9.81), sc=None,
-                  origin=(0.0, 0.0), with_ground=True):
-    """A tower of 1 m boxes on a static ground; consecutive boxes are linked by hinge / fixed joints
-    (cycling through `kinds`) anchored at their shared face, hinge axes seeded in the horizontal plane."""
+                  origin=(0.0, 0.0), with_ground=True, limits=None):
+    """A tower of 1 m boxes on a static ground; consecutive boxes are linked by joints (cycling through
+    `kinds`) anchored at their shared face, hinge / slider axes seeded in the horizontal plane.
+    `limits` maps a joint kind to the (flags, limits[4]) its joints get (none by default)."""
     rng = SplitMix64(seed)
     if sc is None:
         sc = Scene(f"jointed_tower_{height}", make_settings(gravity=gravity, vel_iters=10, pos_iters=5,
@@ -263,8 +264,103 @@ def jointed_tower(height=6, seed=4, kinds=(HINGE, FIXED), friction=0.5, gravity=
             kind = kinds[(k - 1) % len(kinds)]
             anchor = (pos[0], pos[1], pos[2] - 0.5)
             a = 2.0 * math.pi * rng.uniform01()
-            sc.add_joint(kind, prev, b, anchor, axis=(math.cos(a), math.sin(a), 0.0))
+            fl, lim = (limits or {}).get(kind, (0, (0.0, 0.0, 0.0, 0.0)))
+            sc.add_joint(kind, prev, b, anchor, axis=(math.cos(a), math.sin(a), 0.0), flags=fl, limits=lim)
         prev = b
+    return sc
+
+
+def cone_chains(n_chains=6, length=6, radius=0.3, half_angle=(0.2, 0.6), spacing=3.0, seed=7, sc=None):
+    """Sphere chains like sphere_chains whose ball-and-socket joints carry a cone limit with a half angle
+    ~U[half_angle].  The static anchor sits one radius behind the first joint, so that both cone axes
+    (anchor minus centre of mass, for each body) of every joint lie along the chain at rest; spheres
+    start with angular velocities ~N(0, 2) per axis so that the swinging chains bend past their cones."""
+    rng = SplitMix64(seed)
+    if sc is None:
+        sc = Scene(f"cone_chains_{n_chains}x{length}", make_settings(gravity=(0.0, 0.0, -9.81)))
+    side = int(math.ceil(math.sqrt(n_chains)))
+    for c in range(n_chains):
+        ax = (c % side - side / 2.0) * spacing * length * radius
+        ay = (c // side - side / 2.0) * spacing * length * radius
+        az = 2.0 * length * radius + 5.0
+        ang = 2.0 * math.pi * rng.uniform01()
+        d = (math.cos(ang), math.sin(ang), 0.0)
+        prev = sc.add_body(STATIC, (ax - d[0] * radius, ay - d[1] * radius, az))
+        for k in range(length):
+            p = (ax + d[0] * radius * (2 * k + 1), ay + d[1] * radius * (2 * k + 1), az)
+            w = (2.0 * rng.normal(), 2.0 * rng.normal(), 2.0 * rng.normal())
+            b = sc.add_body(DYNAMIC, p, rng.quaternion(), ang=w)
+            sc.add_collider(b, SPHERE, (radius,))
+            joint_at = (ax + d[0] * radius * 2 * k, ay + d[1] * radius * 2 * k, az)
+            cone = rng.uniform(half_angle[0], half_angle[1])
+            sc.add_joint(BALLSOCKET, prev, b, joint_at, flags=1, limits=(cone, 0.0, 0.0, 0.0))
+            prev = b
+    return sc
+
+
+def slider_rigs(n_rigs=8, spacing=4.0, seed=8, sc=None):
+    """Two-stage slider rigs without colliders: a static post, a carriage box (half extents 0.3 m,
+    density 1) 0.8 m below it on a slider whose axis is tilted 20-70 degrees from the vertical, up or
+    down by the seed (limits [-0.4, 0.6] m), and a second box 0.8 m beside the carriage on a slider
+    along a seeded horizontal axis (limits [-0.3, 0.3] m).  Rig r has a motor (speed ~U[-1, 1] m/s,
+    max force 1 N) on the carriage when r % 3 == 1, on the second box when r % 3 == 2.  Gravity
+    -z drives the carriages into one of their limits, the second box swings it around."""
+    rng = SplitMix64(seed)
+    if sc is None:
+        sc = Scene(f"slider_rigs_{n_rigs}", make_settings(gravity=(0.0, 0.0, -9.81), vel_iters=10, pos_iters=5))
+    side = int(math.ceil(math.sqrt(n_rigs)))
+    for r in range(n_rigs):
+        px, py, pz = (r % side) * spacing, (r // side) * spacing, 5.0
+        post = sc.add_body(STATIC, (px, py, pz))
+        tilt = math.radians(rng.uniform(20.0, 70.0))
+        yaw = 2.0 * math.pi * rng.uniform01()
+        sgn = 1.0 if rng.uniform01() < 0.5 else -1.0
+        axis1 = (math.sin(tilt) * math.cos(yaw), math.sin(tilt) * math.sin(yaw), sgn * math.cos(tilt))
+        c = (px, py, pz - 0.8)
+        car = sc.add_body(DYNAMIC, c, rng.quaternion())
+        sc.add_collider(car, BOX, (0.3, 0.3, 0.3))
+        m1 = r % 3 == 1
+        sc.add_joint(SLIDER, post, car, c, axis=axis1, flags=1 | (2 if m1 else 0),
+                     limits=(-0.4, 0.6, rng.uniform(-1.0, 1.0) if m1 else 0.0, 1.0 if m1 else 0.0))
+        yaw2 = 2.0 * math.pi * rng.uniform01()
+        p2 = (c[0] + 0.8 * math.cos(yaw2), c[1] + 0.8 * math.sin(yaw2), c[2])
+        box = sc.add_body(DYNAMIC, p2, rng.quaternion())
+        sc.add_collider(box, BOX, (0.3, 0.3, 0.3))
+        a2 = 2.0 * math.pi * rng.uniform01()
+        m2 = r % 3 == 2
+        sc.add_joint(SLIDER, car, box, p2, axis=(math.cos(a2), math.sin(a2), 0.0), flags=1 | (2 if m2 else 0),
+                     limits=(-0.3, 0.3, rng.uniform(-1.0, 1.0) if m2 else 0.0, 1.0 if m2 else 0.0))
+    return sc
+
+
+def hinge_rigs(n_rigs=8, spacing=5.0, seed=9, sc=None):
+    """Two-paddle hinge rigs without static colliders: a static post and two paddles (boxes 1.0 x 0.2 x
+    0.6 m, density 1) hinged end to end on a horizontal axis (seeded yaw), starting horizontal.  Paddle
+    1 has limits [-0.6, 0.4] rad, paddle 2 limits [-0.5, 0.5] rad; rig r has a motor (speed ~U[-2, 2]
+    rad/s, max torque 0.5 N m) on paddle 1 when r % 3 == 1, on paddle 2 when r % 3 == 2.  Gravity -z
+    swings the paddles into their limits."""
+    rng = SplitMix64(seed)
+    if sc is None:
+        sc = Scene(f"hinge_rigs_{n_rigs}", make_settings(gravity=(0.0, 0.0, -9.81), vel_iters=10, pos_iters=5))
+    side = int(math.ceil(math.sqrt(n_rigs)))
+    for r in range(n_rigs):
+        px, py, pz = (r % side) * spacing, (r // side) * spacing, 5.0
+        post = sc.add_body(STATIC, (px, py, pz))
+        yaw = 2.0 * math.pi * rng.uniform01()
+        d = (math.cos(yaw), math.sin(yaw), 0.0)
+        axis = (-d[1], d[0], 0.0)
+        q = (0.0, 0.0, math.sin(yaw / 2.0), math.cos(yaw / 2.0))
+        prev, lims = post, ((-0.6, 0.4), (-0.5, 0.5))
+        for k in range(2):
+            centre = (px + d[0] * (0.6 + 1.1 * k), py + d[1] * (0.6 + 1.1 * k), pz)
+            pad = sc.add_body(DYNAMIC, centre, q)
+            sc.add_collider(pad, BOX, (0.5, 0.1, 0.3))
+            hinge_at = (px + d[0] * (0.05 + 1.1 * k), py + d[1] * (0.05 + 1.1 * k), pz)
+            motor = r % 3 == k + 1
+            sc.add_joint(HINGE, prev, pad, hinge_at, axis=axis, flags=1 | (2 if motor else 0),
+                         limits=(lims[k][0], lims[k][1], rng.uniform(-2.0, 2.0) if motor else 0.0,
+                                 0.5 if motor else 0.0))
+            prev = pad
     return sc
 
 

@@ -50,3 +50,105 @@ def test_ordered_step_with_reference_order_is_the_reference_step(ref_available):
         b.step_ordered(DT, ct["colliders"][:, 0], ct["colliders"][:, 1])
         for x, y in zip(a.bodies(), b.bodies()):
             assert np.array_equal(x, y), f"step {k}"
+
+
+def _swing_angle_about_y(q):
+    """Rotation angle about +y of a quaternion (x, y, z, w) that only rotates about y."""
+    return 2.0 * np.arctan2(q[1], q[3])
+
+
+def test_slider_motor_then_limit_closed_form(ref_available):
+    """A box on a slider along +x from a static body, no gravity: the motor sets the speed along the
+    axis to the motor speed in one velocity solve, nothing moves off the axis, and the upper limit
+    stops the box at u.axis = max."""
+    from oracle.ref import RefWorld
+    sc = S.Scene("slider_pin", S.make_settings(gravity=(0.0, 0.0, 0.0)))
+    a = sc.add_body(S.STATIC, (0.0, 0.0, 0.0))
+    b = sc.add_body(S.DYNAMIC, (0.0, 0.0, 0.0))
+    sc.add_collider(b, S.BOX, (0.2, 0.2, 0.2))
+    sc.add_joint(S.SLIDER, a, b, (0.0, 0.0, 0.0), axis=(1.0, 0.0, 0.0), flags=3, limits=(-0.5, 0.5, 0.3, 100.0))
+    ref = RefWorld(sc)
+    for k in range(1, 31):
+        ref.step(DT)
+        p, q, v, w = ref.bodies()
+        assert v[1, 0] == pytest.approx(0.3, abs=1e-5)
+        assert np.all(p[1, 1:] == 0.0) and np.all(v[1, 1:] == 0.0) and np.all(w[1] == 0.0)
+        assert p[1, 0] == pytest.approx(0.3 * DT * k, abs=1e-5)
+    for _ in range(150):
+        ref.step(DT)
+    p, q, v, w = ref.bodies()
+    assert p[1, 0] == pytest.approx(0.5, abs=2e-3)
+    assert abs(v[1, 0]) < 1e-3
+
+
+def test_slider_lower_limit_under_gravity(ref_available):
+    """Gravity along -axis pulls a slider box onto its lower limit, where it rests."""
+    from oracle.ref import RefWorld
+    sc = S.Scene("slider_fall", S.make_settings(gravity=(0.0, 0.0, -9.81), vel_iters=10, pos_iters=5))
+    a = sc.add_body(S.STATIC, (0.0, 0.0, 2.0))
+    b = sc.add_body(S.DYNAMIC, (0.0, 0.0, 2.0))
+    sc.add_collider(b, S.BOX, (0.2, 0.2, 0.2))
+    sc.add_joint(S.SLIDER, a, b, (0.0, 0.0, 2.0), axis=(0.0, 0.0, 1.0), flags=1, limits=(-0.7, 0.3, 0.0, 0.0))
+    ref = RefWorld(sc)
+    for _ in range(180):
+        ref.step(DT)
+    p, q, v, w = ref.bodies()
+    assert p[1, 2] == pytest.approx(2.0 - 0.7, abs=2e-3)
+    assert np.all(p[1, :2] == 0.0) and abs(v[1, 2]) < 1e-2
+
+
+def test_ball_socket_cone_limit_stops_the_pendulum(ref_available):
+    """A sphere hangs from a static body by a ball-and-socket joint, starting horizontal (cone axes
+    aligned).  Without a cone it swings down through 90 degrees; a cone limit of half angle 0.3 rad
+    holds the arm (joint -> sphere centre) at 0.3 rad below the horizontal."""
+    from oracle.ref import RefWorld
+
+    def arm_angle(flags):
+        sc = S.Scene("cone_pin", S.make_settings(gravity=(0.0, 0.0, -9.81), vel_iters=10, pos_iters=5))
+        a = sc.add_body(S.STATIC, (0.0, 0.0, 5.0))
+        b = sc.add_body(S.DYNAMIC, (1.0, 0.0, 5.0))
+        sc.add_collider(b, S.SPHERE, (0.2,))
+        sc.add_joint(S.BALLSOCKET, a, b, (0.5, 0.0, 5.0), flags=flags, limits=(0.3, 0.0, 0.0, 0.0))
+        ref = RefWorld(sc)
+        angles = []
+        for _ in range(120):
+            ref.step(DT)
+            p, q, v, w = ref.bodies()
+            arm = p[1] - np.array([0.5, 0.0, 5.0])
+            angles.append(np.arctan2(-arm[2], arm[0]))
+        return np.array(angles)
+
+    free, limited = arm_angle(0), arm_angle(1)
+    assert free.max() > 1.4
+    # the limit acts on the body-fixed arm, so the sag of the point constraint and the overshoot of one
+    # step show up as a few hundredths of a radian
+    assert limited.max() < 0.3 + 0.07
+    assert limited[-1] == pytest.approx(0.3, abs=0.03)
+
+
+def test_hinge_limit_stops_the_paddle(ref_available):
+    """A paddle hinged to a static body on the +y axis, starting horizontal along +x, swings down under
+    gravity (a positive rotation about +y) through 90 degrees without a limit, and stops at the upper
+    limit of 0.4 rad with one.  (The hinge motor row is not pinned here: SolveHingeJointSystem.cpp:393-409
+    measures a1.(w1 - w2) but pushes body 2 along +a1, so it feeds back until its impulse clamps.)"""
+    from oracle.ref import RefWorld
+
+    def run(flags, limits, steps):
+        sc = S.Scene("hinge_pin", S.make_settings(gravity=(0.0, 0.0, -9.81), vel_iters=10, pos_iters=5))
+        a = sc.add_body(S.STATIC, (0.0, 0.0, 5.0))
+        b = sc.add_body(S.DYNAMIC, (0.6, 0.0, 5.0))
+        sc.add_collider(b, S.BOX, (0.5, 0.1, 0.3))
+        sc.add_joint(S.HINGE, a, b, (0.0, 0.0, 5.0), axis=(0.0, 1.0, 0.0), flags=flags, limits=limits)
+        ref = RefWorld(sc)
+        out = []
+        for _ in range(steps):
+            ref.step(DT)
+            p, q, v, w = ref.bodies()
+            out.append((_swing_angle_about_y(q[1]), w[1, 1]))
+        return np.array(out)
+
+    free = run(0, (0.0, 0.0, 0.0, 0.0), 40)
+    assert free[:, 0].max() > 1.4
+    down = run(1, (-0.6, 0.4, 0.0, 0.0), 120)
+    assert down[:, 0].max() < 0.4 + 0.05
+    assert down[-1, 0] == pytest.approx(0.4, abs=0.02)
