@@ -243,13 +243,63 @@ struct HingeJointInfo : JointInfo {
     bool isLimitEnabled = false, isMotorEnabled = false;
     decimal minAngleLimit = -1, maxAngleLimit = 1, motorSpeed = 0, maxMotorTorque = 0;
 };
+struct SliderJointInfo : JointInfo {   // SliderJoint.h:93-101
+    SliderJointInfo(RigidBody* b1, RigidBody* b2, const Vector3& anchor, const Vector3& axis)
+        : JointInfo{b1, b2, anchor}, sliderAxisWorldSpace(axis) {}
+    Vector3 sliderAxisWorldSpace;
+    bool isLimitEnabled = false, isMotorEnabled = false;
+    decimal minTranslationLimit = -1, maxTranslationLimit = 1, motorSpeed = 0, maxMotorForce = 0;
+};
+
+// Joint handles: the limit / motor setters of the reference's joint classes, applied before the first
+// update() (the device uploads joints once).  createJoint returns the derived class as Joint*, which the
+// caller casts as with the reference (static_cast<HingeJoint*>(world->createJoint(info))).
 class Joint {
    public:
+    virtual ~Joint() = default;
     int32_t index() const { return mIndex; }
 
-   private:
+   protected:
     friend class PhysicsWorld;
+    void apply() {
+        const float lim[4] = {mLim[0], mLim[1], mLim[2], mLim[3]};
+        check(rp3dg_joint_set_limits(mWorld, mIndex, (mLimit ? 1 : 0) | (mMotor ? 2 : 0), lim), "joint limits");
+    }
+    rp3dg_world* mWorld = nullptr;
     int32_t mIndex = -1;
+    bool mLimit = false, mMotor = false;
+    decimal mLim[4] = {0, 0, 0, 0};
+};
+class BallAndSocketJoint : public Joint {   // BallAndSocketJoint.cpp:71-105
+   public:
+    BallAndSocketJoint() { mLim[0] = decimal(3.141592653589); }
+    void enableConeLimit(bool on) { mLimit = on; apply(); }
+    bool isConeLimitEnabled() const { return mLimit; }
+    void setConeLimitHalfAngle(decimal a) { mLim[0] = a; apply(); }
+    decimal getConeLimitHalfAngle() const { return mLim[0]; }
+};
+class FixedJoint : public Joint {};
+class HingeJoint : public Joint {   // HingeJoint.cpp:90-195
+   public:
+    void enableLimit(bool on) { mLimit = on; apply(); }
+    void enableMotor(bool on) { mMotor = on; apply(); }
+    void setMinAngleLimit(decimal v) { mLim[0] = v; apply(); }
+    void setMaxAngleLimit(decimal v) { mLim[1] = v; apply(); }
+    void setMotorSpeed(decimal v) { mLim[2] = v; apply(); }
+    void setMaxMotorTorque(decimal v) { mLim[3] = v; apply(); }
+    bool isLimitEnabled() const { return mLimit; }
+    bool isMotorEnabled() const { return mMotor; }
+};
+class SliderJoint : public Joint {   // SliderJoint.cpp:98-235
+   public:
+    void enableLimit(bool on) { mLimit = on; apply(); }
+    void enableMotor(bool on) { mMotor = on; apply(); }
+    void setMinTranslationLimit(decimal v) { mLim[0] = v; apply(); }
+    void setMaxTranslationLimit(decimal v) { mLim[1] = v; apply(); }
+    void setMotorSpeed(decimal v) { mLim[2] = v; apply(); }
+    void setMaxMotorForce(decimal v) { mLim[3] = v; apply(); }
+    bool isLimitEnabled() const { return mLimit; }
+    bool isMotorEnabled() const { return mMotor; }
 };
 
 class PhysicsWorld {
@@ -301,11 +351,21 @@ class PhysicsWorld {
         mBodies.push_back(std::move(body));
         return mBodies.back().get();
     }
-    Joint* createJoint(const BallAndSocketJointInfo& i) { return addJoint(RP3DG_JOINT_BALLSOCKET, i, Vector3(), 0, nullptr); }
-    Joint* createJoint(const FixedJointInfo& i) { return addJoint(RP3DG_JOINT_FIXED, i, Vector3(), 0, nullptr); }
+    Joint* createJoint(const BallAndSocketJointInfo& i) {
+        return addJoint(std::make_unique<BallAndSocketJoint>(), RP3DG_JOINT_BALLSOCKET, i, Vector3(), false, false, nullptr);
+    }
+    Joint* createJoint(const FixedJointInfo& i) {
+        return addJoint(std::make_unique<FixedJoint>(), RP3DG_JOINT_FIXED, i, Vector3(), false, false, nullptr);
+    }
     Joint* createJoint(const HingeJointInfo& i) {
-        const float lim[4] = {i.minAngleLimit, i.maxAngleLimit, i.motorSpeed, i.maxMotorTorque};
-        return addJoint(RP3DG_JOINT_HINGE, i, i.rotationAxisWorld, (i.isLimitEnabled ? 1 : 0) | (i.isMotorEnabled ? 2 : 0), lim);
+        const decimal lim[4] = {i.minAngleLimit, i.maxAngleLimit, i.motorSpeed, i.maxMotorTorque};
+        return addJoint(std::make_unique<HingeJoint>(), RP3DG_JOINT_HINGE, i, i.rotationAxisWorld, i.isLimitEnabled,
+                        i.isMotorEnabled, lim);
+    }
+    Joint* createJoint(const SliderJointInfo& i) {
+        const decimal lim[4] = {i.minTranslationLimit, i.maxTranslationLimit, i.motorSpeed, i.maxMotorForce};
+        return addJoint(std::make_unique<SliderJoint>(), RP3DG_JOINT_SLIDER, i, i.sliderAxisWorldSpace, i.isLimitEnabled,
+                        i.isMotorEnabled, lim);
     }
     void setNbIterationsVelocitySolver(uint16 n) { mVel = n; check(rp3dg_world_set_iterations(mWorld, mVel, mPos), "iterations"); }
     void setNbIterationsPositionSolver(uint16 n) { mPos = n; check(rp3dg_world_set_iterations(mWorld, mVel, mPos), "iterations"); }
@@ -318,14 +378,19 @@ class PhysicsWorld {
     rp3dg_world* handle() const { return mWorld; }
 
    private:
-    Joint* addJoint(int32_t type, const JointInfo& i, const Vector3& axis, int32_t flags, const float* lim) {
+    Joint* addJoint(std::unique_ptr<Joint> joint, int32_t type, const JointInfo& i, const Vector3& axis, bool limit,
+                    bool motor, const decimal* lim) {
         const float a[3] = {i.anchorPointWorldSpace.x, i.anchorPointWorldSpace.y, i.anchorPointWorldSpace.z};
         const float ax[3] = {axis.x, axis.y, axis.z};
-        const float zero[4] = {0, 0, 0, 0};
+        if (lim)
+            for (int k = 0; k < 4; k++) joint->mLim[k] = lim[k];
+        joint->mLimit = limit;
+        joint->mMotor = motor;
+        const float l[4] = {joint->mLim[0], joint->mLim[1], joint->mLim[2], joint->mLim[3]};
         const int32_t j = rp3dg_joint_create(mWorld, type, i.body1->index(), i.body2->index(), a, ax,
-                                             i.isCollisionEnabled ? 1 : 0, flags, lim ? lim : zero);
+                                             i.isCollisionEnabled ? 1 : 0, (limit ? 1 : 0) | (motor ? 2 : 0), l);
         check(j, "createJoint");
-        auto joint = std::make_unique<Joint>();
+        joint->mWorld = mWorld;
         joint->mIndex = j;
         mJoints.push_back(std::move(joint));
         return mJoints.back().get();

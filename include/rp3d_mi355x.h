@@ -128,6 +128,13 @@ int32_t rp3dg_hull_create(rp3dg_world* world, const float* vertices, int32_t nb_
 int32_t rp3dg_joint_create(rp3dg_world* world, int32_t type, int32_t body1, int32_t body2,
                            const float anchor_world[3], const float axis_world[3], int32_t collide_connected,
                            int32_t flags, const float limits[4]);
+/* Replaces the limit / motor state of a joint before the world's first update (flags and limits as in
+ * rp3dg_joint_create; RP3DG_ESTATE afterwards): BallAndSocketJoint::enableConeLimit /
+ * setConeLimitHalfAngle (BallAndSocketJoint.cpp:71-96), HingeJoint::enableLimit / enableMotor /
+ * setMinAngleLimit / setMaxAngleLimit / setMotorSpeed / setMaxMotorTorque (HingeJoint.cpp:90-195),
+ * SliderJoint::enableLimit / enableMotor / setMinTranslationLimit / setMaxTranslationLimit /
+ * setMotorSpeed / setMaxMotorForce (SliderJoint.cpp:98-235). */
+int rp3dg_joint_set_limits(rp3dg_world* world, int32_t joint, int32_t flags, const float limits[4]);
 
 /* ---- stepping (replaces PhysicsWorld::update, src/engine/PhysicsWorld.cpp:239-307) ---- */
 

@@ -49,14 +49,15 @@ def build(verbose=False, jobs=None):
     cmd = ["hipcc", "--offload-arch=gfx950", "-shared", "-fPIC", *objs, "-o", tmp]
     subprocess.run(cmd, check=True)
     os.replace(tmp, LIB)
-    # the C++ API example (host code only: g++ against the C++ mirror header and the library)
-    ex = os.path.join(LIB_DIR, "hello_world")
-    cmd = ["g++", "-std=c++17", "-O2", "-I", os.path.join(os.path.dirname(HERE), "include"),
-           os.path.join(HERE, "examples", "hello_world.cpp"), "-L", LIB_DIR, "-lrp3d_mi355x",
-           "-Wl,-rpath,$ORIGIN", "-o", ex]
-    if verbose:
-        print(" ".join(cmd), flush=True)
-    subprocess.run(cmd, check=True)
+    # the C++ API examples (host code only: g++ against the C++ mirror header and the library)
+    for name in ("hello_world", "joints"):
+        ex = os.path.join(LIB_DIR, name)
+        cmd = ["g++", "-std=c++17", "-O2", "-I", os.path.join(os.path.dirname(HERE), "include"),
+               os.path.join(HERE, "examples", name + ".cpp"), "-L", LIB_DIR, "-lrp3d_mi355x",
+               "-Wl,-rpath,$ORIGIN", "-o", ex]
+        if verbose:
+            print(" ".join(cmd), flush=True)
+        subprocess.run(cmd, check=True)
     return LIB
 
 

@@ -205,6 +205,16 @@ static bool valid_body(HostWorld* w, int32_t b) {
     if (!w || b < 0 || b >= (int32_t)w->bodies.size()) { set_error("invalid body index"); return false; }
     return true;
 }
+// SliderJoint.cpp:41-43: lower limit <= 0 <= upper limit, max motor force >= 0
+static bool valid_limits(int32_t type, int32_t flags, const float* limits) {
+    if (type != RP3DG_JOINT_SLIDER || !limits) return true;
+    if ((flags & 1) && !(limits[0] <= 0.0f && limits[1] >= 0.0f)) {
+        set_error("slider limits must satisfy min <= 0 <= max");
+        return false;
+    }
+    if ((flags & 2) && !(limits[3] >= 0.0f)) { set_error("slider max motor force must be >= 0"); return false; }
+    return true;
+}
 static bool mutable_world(HostWorld* w) {
     if (w->uploaded) { set_error("world already uploaded to the device; edit before the first update()"); return false; }
     return true;
@@ -493,14 +503,7 @@ int32_t rp3dg_joint_create(rp3dg_world* w, int32_t type, int32_t b1, int32_t b2,
     if (!mutable_world(w)) return RP3DG_ESTATE;
     if (b1 == b2) { set_error("a joint needs two different bodies"); return RP3DG_EINVAL; }
     if (type < RP3DG_JOINT_BALLSOCKET || type > RP3DG_JOINT_FIXED) { set_error("invalid joint type"); return RP3DG_EINVAL; }
-    if (type == RP3DG_JOINT_SLIDER && limits) {
-        // SliderJoint.cpp:41-43: lower limit <= 0 <= upper limit, max motor force >= 0
-        if ((flags & 1) && !(limits[0] <= 0.0f && limits[1] >= 0.0f)) {
-            set_error("slider limits must satisfy min <= 0 <= max");
-            return RP3DG_EINVAL;
-        }
-        if ((flags & 2) && !(limits[3] >= 0.0f)) { set_error("slider max motor force must be >= 0"); return RP3DG_EINVAL; }
-    }
+    if (!valid_limits(type, flags, limits)) return RP3DG_EINVAL;
     HostJoint j;
     j.type = type;
     j.body1 = b1;
@@ -512,6 +515,16 @@ int32_t rp3dg_joint_create(rp3dg_world* w, int32_t type, int32_t b1, int32_t b2,
     for (int i = 0; i < 4; i++) j.limits[i] = limits ? limits[i] : 0.0f;
     w->joints.push_back(j);
     return (int32_t)w->joints.size() - 1;
+}
+
+int rp3dg_joint_set_limits(rp3dg_world* w, int32_t joint, int32_t flags, const float limits[4]) {
+    if (!w || joint < 0 || joint >= (int32_t)w->joints.size()) { set_error("invalid joint index"); return RP3DG_EINVAL; }
+    if (!mutable_world(w)) return RP3DG_ESTATE;
+    HostJoint& j = w->joints[joint];
+    if (!valid_limits(j.type, flags, limits)) return RP3DG_EINVAL;
+    j.flags = flags;
+    for (int i = 0; i < 4; i++) j.limits[i] = limits ? limits[i] : 0.0f;
+    return RP3DG_OK;
 }
 
 }  // extern "C"

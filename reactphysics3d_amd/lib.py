@@ -61,7 +61,7 @@ EXPORTED = [
     "rp3dg_body_set_damping", "rp3dg_body_enable_gravity", "rp3dg_body_apply_force_torque",
     "rp3dg_body_set_lock_factors", "rp3dg_body_update_mass_properties", "rp3dg_body_set_mass_properties",
     "rp3dg_collider_add", "rp3dg_collider_set_material", "rp3dg_collider_set_filter", "rp3dg_hull_create",
-    "rp3dg_joint_create", "rp3dg_world_update", "rp3dg_batch_create", "rp3dg_batch_create_ex",
+    "rp3dg_joint_create", "rp3dg_joint_set_limits", "rp3dg_world_update", "rp3dg_batch_create", "rp3dg_batch_create_ex",
     "rp3dg_batch_destroy", "rp3dg_batch_step",
     "rp3dg_batch_step_graph", "rp3dg_batch_synchronize", "rp3dg_batch_get_counts", "rp3dg_batch_get_bodies",
     "rp3dg_batch_set_bodies", "rp3dg_batch_get_pairs", "rp3dg_batch_get_contacts", "rp3dg_batch_get_joint_order",
@@ -114,6 +114,7 @@ def load():
         "rp3dg_collider_set_filter": (C.c_int, [vp, i32, C.c_uint16, C.c_uint16]),
         "rp3dg_hull_create": (i32, [vp, fptr, i32, iptr, iptr, i32]),
         "rp3dg_joint_create": (i32, [vp, i32, i32, i32, fptr, fptr, i32, i32, fptr]),
+        "rp3dg_joint_set_limits": (C.c_int, [vp, i32, i32, fptr]),
         "rp3dg_world_update": (C.c_int, [vp, f]),
         "rp3dg_batch_create": (vp, [C.POINTER(vp), i32]),
         "rp3dg_batch_create_ex": (vp, [C.POINTER(vp), i32, C.POINTER(BatchOptions)]),

@@ -63,6 +63,30 @@ def test_invalid_arguments_are_rejected():
     L.rp3dg_world_destroy(w)
 
 
+def test_joint_limits_are_validated():
+    """SliderJoint.cpp:41-43 asserts min <= 0 <= max and a max motor force >= 0; the C-ABI rejects
+    them, at creation and through rp3dg_joint_set_limits, which only acts before the first update."""
+    L = lib.load()
+    w = L.rp3dg_world_create(None)
+    f3 = (C.c_float * 3)(0, 0, 0)
+    ax = (C.c_float * 3)(1, 0, 0)
+    q = (C.c_float * 4)(0, 0, 0, 1)
+    b1, b2 = L.rp3dg_body_create(w, f3, q), L.rp3dg_body_create(w, f3, q)
+    above = (C.c_float * 4)(0.2, 1.0, 0.0, 0.0)
+    assert L.rp3dg_joint_create(w, S.SLIDER, b1, b2, f3, ax, 0, 1, above) == -1
+    assert "min <= 0 <= max" in lib.last_error()
+    assert L.rp3dg_joint_create(w, S.SLIDER, b1, b2, f3, ax, 0, 0, above) == 0     # limit off: not checked
+    neg_force = (C.c_float * 4)(-1.0, 1.0, 0.5, -2.0)
+    assert L.rp3dg_joint_set_limits(w, 0, 2, neg_force) == -1
+    ok = (C.c_float * 4)(-1.0, 1.0, 0.5, 2.0)
+    assert L.rp3dg_joint_set_limits(w, 0, 3, ok) == 0
+    assert L.rp3dg_joint_set_limits(w, 1, 3, ok) == -1                             # no such joint
+    cone = (C.c_float * 4)(0.4, 0.0, 0.0, 0.0)
+    j = L.rp3dg_joint_create(w, S.BALLSOCKET, b1, b2, f3, f3, 0, 0, None)
+    assert j == 1 and L.rp3dg_joint_set_limits(w, j, 1, cone) == 0
+    L.rp3dg_world_destroy(w)
+
+
 def test_hull_rejects_open_mesh():
     L = lib.load()
     w = L.rp3dg_world_create(None)

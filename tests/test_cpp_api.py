@@ -51,3 +51,55 @@ def test_example_steps_like_the_reference(ref_available):
     err = np.max(np.abs(st - want))
     print("max |device - reference|", err)
     assert err <= 1e-4
+
+
+JOINTS_EXE = os.path.join(ROOT, "reactphysics3d_amd", "_lib", "joints")
+
+
+def _run_joints(steps):
+    if not os.path.exists(JOINTS_EXE):
+        pytest.skip("example not built (python -m reactphysics3d_amd.build)")
+    out = subprocess.run([JOINTS_EXE, str(steps)], capture_output=True, text=True, timeout=120)
+    assert out.returncode == 0, out.stderr
+    rows = [list(map(float, line.split()[2:])) for line in out.stdout.splitlines() if line.startswith("body")]
+    return np.array(rows, np.float32)
+
+
+def _joints_scene():
+    """The world of examples/joints.cpp as a scene description."""
+    sc = S.Scene("joints_example", S.make_settings(gravity=(0.0, 0.0, -9.81), vel_iters=10, pos_iters=5))
+    post = sc.add_body(S.STATIC, (0.0, 0.0, 5.0))
+    car = sc.add_body(S.DYNAMIC, (0.0, 0.0, 4.2))
+    sc.add_collider(car, S.BOX, (0.3, 0.3, 0.3))
+    pivot = sc.add_body(S.STATIC, (4.0, 0.0, 5.0))
+    paddle = sc.add_body(S.DYNAMIC, (4.6, 0.0, 5.0))
+    sc.add_collider(paddle, S.BOX, (0.5, 0.1, 0.3))
+    hook = sc.add_body(S.STATIC, (8.0, 0.0, 5.0))
+    ball = sc.add_body(S.DYNAMIC, (9.0, 0.0, 5.0))
+    sc.add_collider(ball, S.SPHERE, (0.2,))
+    sc.add_joint(S.SLIDER, post, car, (0.0, 0.0, 4.2), axis=(0.6, 0.0, 0.8), flags=3, limits=(-0.4, 0.6, 0.5, 1.0))
+    sc.add_joint(S.HINGE, pivot, paddle, (4.05, 0.0, 5.0), axis=(0.0, 1.0, 0.0), flags=1, limits=(-0.6, 0.4, 0.0, 0.0))
+    sc.add_joint(S.BALLSOCKET, hook, ball, (8.5, 0.0, 5.0), flags=1, limits=(0.3, 0.0, 0.0, 0.0))
+    return sc
+
+
+def test_joints_example_builds_and_creates_world():
+    st = _run_joints(0)
+    assert st.shape == (6, 13)
+    assert np.allclose(st[:, 2], 5.0 - 0.8 * (np.arange(6) == 1))
+
+
+@pytest.mark.gpu
+def test_joints_example_steps_like_the_reference(ref_available):
+    """Limits and motors set through SliderJointInfo fields and through the joint setters after
+    createJoint reach the device: the example matches the reference world built with the same limits."""
+    from oracle.ref import RefWorld
+    st = _run_joints(120)
+    ref = RefWorld(_joints_scene())
+    for _ in range(120):
+        ref.step(1.0 / 60.0)
+    p, q, v, w = ref.bodies()
+    want = np.concatenate([p, q, v, w], 1)
+    err = np.max(np.abs(st - want))
+    print("max |device - reference|", err)
+    assert err <= 1e-4
