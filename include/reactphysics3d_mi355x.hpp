@@ -6,8 +6,8 @@
 // same factory calls and the same PhysicsWorld::update(dt).  Objects are thin handles (world +
 // index) into the host world behind the C-ABI; getters after update() read a host mirror that the
 // library refreshes from the device once per step.  Only what the step path needs is provided:
-// sphere / box / capsule / convex-mesh colliders, ball-and-socket / hinge / fixed joints, materials,
-// collision filters, velocities, damping, gravity; sleeping is off.
+// sphere / box / capsule / convex-mesh colliders, ball-and-socket / hinge / slider / fixed joints with
+// their limits and motors, materials, collision filters, velocities, damping, gravity, sleeping.
 #pragma once
 
 #include <cmath>
@@ -188,6 +188,15 @@ class RigidBody {
     }
     void updateMassPropertiesFromColliders() { check(rp3dg_body_update_mass_properties(mWorld, mIndex), "updateMassProperties"); }
     void setLinearVelocity(const Vector3& v) { mLin = v; pushVelocity(); }
+    // RigidBody::setIsAllowedToSleep / isSleeping (RigidBody.cpp:1006-1170)
+    void setIsAllowedToSleep(bool allowed) {
+        check(rp3dg_body_set_allowed_to_sleep(mWorld, mIndex, allowed ? 1 : 0), "setIsAllowedToSleep");
+    }
+    bool isSleeping() const {
+        const int r = rp3dg_world_get_body_sleeping(mWorld, mIndex);
+        check(r, "isSleeping");
+        return r == 1;
+    }
     void setAngularVelocity(const Vector3& w) { mAng = w; pushVelocity(); }
     void enableGravity(bool on) { check(rp3dg_body_enable_gravity(mWorld, mIndex, on ? 1 : 0), "enableGravity"); }
     void setLinearDamping(decimal d) { mLinDamp = d; check(rp3dg_body_set_damping(mWorld, mIndex, mLinDamp, mAngDamp), "damping"); }
@@ -313,8 +322,11 @@ class PhysicsWorld {
         decimal cosAngleSimilarContactManifold = decimal(0.95);
         uint16 defaultVelocitySolverNbIterations = 6;
         uint16 defaultPositionSolverNbIterations = 3;
-        bool isSleepingEnabled = false;   // the device step path runs without sleeping
+        bool isSleepingEnabled = true;
         bool isGravityEnabled = true;
+        float defaultTimeBeforeSleep = 1.0f;
+        decimal defaultSleepLinearVelocity = decimal(0.02);
+        decimal defaultSleepAngularVelocity = decimal(3.0) * (decimal(3.141592653589) / decimal(180.0));
     };
 
     explicit PhysicsWorld(const WorldSettings& ws) : mSettings(ws) {
@@ -330,6 +342,9 @@ class PhysicsWorld {
         s.default_position_solver_iterations = ws.defaultPositionSolverNbIterations;
         s.is_sleeping_enabled = ws.isSleepingEnabled ? 1 : 0;
         s.is_gravity_enabled = ws.isGravityEnabled ? 1 : 0;
+        s.time_before_sleep = ws.defaultTimeBeforeSleep;
+        s.sleep_linear_velocity = ws.defaultSleepLinearVelocity;
+        s.sleep_angular_velocity = ws.defaultSleepAngularVelocity;
         mWorld = rp3dg_world_create(&s);
         if (!mWorld) throw std::runtime_error(std::string("createPhysicsWorld: ") + rp3dg_last_error());
     }

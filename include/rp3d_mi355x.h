@@ -63,8 +63,11 @@ typedef struct rp3dg_world_settings {
     float cos_angle_similar_contact_manifold;    /* 0.95 */
     uint32_t default_velocity_solver_iterations; /* 6 */
     uint32_t default_position_solver_iterations; /* 3 */
-    int32_t is_sleeping_enabled;                 /* device path requires 0 (sleeping not on the device) */
+    int32_t is_sleeping_enabled;                 /* 1: islands of still bodies fall asleep */
     int32_t is_gravity_enabled;                  /* 1 */
+    float time_before_sleep;                     /* 1.0 s */
+    float sleep_linear_velocity;                 /* 0.02 m/s */
+    float sleep_angular_velocity;                /* 3 * (pi / 180) rad/s */
 } rp3dg_world_settings;
 
 /* ---- world construction (host side, replaces PhysicsCommon / PhysicsWorld factories) ---- */
@@ -108,6 +111,8 @@ int32_t rp3dg_collider_add(rp3dg_world* world, int32_t body, int32_t shape_type,
 /* Collider::getMaterial().set{FrictionCoefficient,Bounciness,MassDensity} (engine/Material.h). */
 int rp3dg_collider_set_material(rp3dg_world* world, int32_t collider, float friction, float bounciness,
                                 float mass_density);
+/* RigidBody::setIsAllowedToSleep (src/body/RigidBody.cpp:1154-1163); bodies are allowed by default. */
+int rp3dg_body_set_allowed_to_sleep(rp3dg_world* world, int32_t body, int32_t allowed);
 /* Collider::setCollisionCategoryBits / setCollideWithMaskBits (src/collision/Collider.cpp). */
 int rp3dg_collider_set_filter(rp3dg_world* world, int32_t collider, uint16_t category, uint16_t mask);
 
@@ -145,6 +150,8 @@ int rp3dg_world_update(rp3dg_world* world, float time_step);
  * Before the first update it returns the creation state.  Any output pointer may be NULL. */
 int rp3dg_world_get_body_state(rp3dg_world* world, int32_t body, float position[3], float orientation[4],
                                float linear_velocity[3], float angular_velocity[3]);
+/* RigidBody::isSleeping (src/body/RigidBody.cpp:1006-1057) after the last update: 1 asleep, 0 awake. */
+int rp3dg_world_get_body_sleeping(rp3dg_world* world, int32_t body);
 
 /* Many independent worlds stepped together: one launch sequence per step for all of them.  The
  * reference has one PhysicsWorld per update() call (src/engine/PhysicsWorld.cpp:239); a batch holds
@@ -223,6 +230,8 @@ typedef struct rp3dg_contact_view {
     float* friction;        /* 3 per manifold: friction1, friction2, twist impulses */
 } rp3dg_contact_view;
 int rp3dg_batch_get_contacts(rp3dg_batch* batch, rp3dg_contact_view* out, int64_t capacity);
+/* RigidBody::isSleeping of every body of the batch (batch order), 0 / 1; capacity >= body count. */
+int rp3dg_batch_get_sleeping(rp3dg_batch* batch, int32_t* sleeping, int64_t capacity);
 /* Joint solve order (joint indices in the order the device solved them this step, per type block). */
 int rp3dg_batch_get_joint_order(rp3dg_batch* batch, int32_t* order, int64_t capacity);
 

@@ -121,6 +121,13 @@ class RefWorld:
         if self.L.rp3dref_set_joint_order(self.h, _ip(o), len(o)) != 0:
             raise RuntimeError("set_joint_order: unsupported joint kind")
 
+    def sleeping(self):
+        """RigidBody::isSleeping of every body, as int32 0 / 1."""
+        out = np.zeros(self.nb, np.int32)
+        self.L.rp3dref_get_sleeping.argtypes = [C.c_void_p, C.POINTER(C.c_int32)]
+        self.L.rp3dref_get_sleeping(self.h, _ip(out))
+        return out
+
     def step_ordered(self, dt, order_c1, order_c2):
         o1 = np.ascontiguousarray(order_c1, np.int32)
         o2 = np.ascontiguousarray(order_c2, np.int32)

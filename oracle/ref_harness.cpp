@@ -76,6 +76,7 @@ struct RefWorld {
     std::vector<Joint*> joints;
     std::unordered_map<uint32_t, int> collider_of_entity;
     std::unordered_map<uint32_t, int> body_of_entity;
+    std::vector<int32_t> joint_order;   // solve order requested by rp3dref_set_joint_order (joint indices)
     float last_dt = 0.f;
 };
 
@@ -225,6 +226,8 @@ static int collider_index(RefWorld* R, Entity e) {
     return it == R->collider_of_entity.end() ? -1 : it->second;
 }
 
+static int apply_joint_order(RefWorld* R);
+
 // ContactSolverSystem::init (src/systems/ContactSolverSystem.cpp:66-105) with the rows of each
 // manifold permuted into the order of (order_c1[i], order_c2[i]) before warmStart().
 static int solver_init_ordered(RefWorld* R, float dt, const int32_t* oc1, const int32_t* oc2, int n) {
@@ -288,6 +291,7 @@ int rp3dref_step_ordered(void* h, float dt, const int32_t* oc1, const int32_t* o
     W.mCollisionDetection.reportContactsAndTriggers();
     W.updateBodiesInverseWorldInertiaTensors();
     W.enableDisableJoints();
+    if (!R->joint_order.empty() && apply_joint_order(R) != 0) return -1;
     W.mDynamicsSystem.integrateRigidBodiesVelocities(dt);
     // solveContactsAndConstraints (PhysicsWorld.cpp:321-345)
     if (solver_init_ordered(R, dt, oc1, oc2, n) != 0) return -1;
@@ -314,19 +318,24 @@ int rp3dref_step_ordered(void* h, float dt, const int32_t* oc1, const int32_t* o
 // Reorders the joint components of every joint kind so that the reference solves them in the given
 // order (global joint creation indices, as the device solves them).  Components::swapComponents keeps
 // each component's data (and warm-start impulses) with its entity.
+// Only the enabled components are reordered (the solvers iterate those; a joint whose two bodies are
+// static or sleeping sits in the disabled part, PhysicsWorld.cpp:363-393).
 static void reorder(Components& comps, const std::vector<Entity>& want) {
-    for (uint32 i = 0; i < (uint32)want.size(); i++) {
-        const uint32 cur = comps.getEntityIndex(want[i]);
+    const uint32 nen = comps.getNbEnabledComponents();
+    uint32 i = 0;
+    for (const Entity& e : want) {
+        const uint32 cur = comps.getEntityIndex(e);
+        if (cur >= nen) continue;
         if (cur != i) comps.swapComponents(i, cur);
+        i++;
     }
 }
 
-int rp3dref_set_joint_order(void* h, const int32_t* order, int n) {
-    RefWorld* R = (RefWorld*)h;
+static int apply_joint_order(RefWorld* R) {
     PhysicsWorld& W = *R->world;
     std::vector<Entity> bs, fx, hg, sl;
-    for (int i = 0; i < n; i++) {
-        Joint* j = R->joints[order[i]];
+    for (int32_t o : R->joint_order) {
+        Joint* j = R->joints[o];
         switch (j->getType()) {
             case JointType::BALLSOCKETJOINT: bs.push_back(j->getEntity()); break;
             case JointType::FIXEDJOINT: fx.push_back(j->getEntity()); break;
@@ -340,6 +349,23 @@ int rp3dref_set_joint_order(void* h, const int32_t* order, int n) {
     reorder(W.mHingeJointsComponents, hg);
     reorder(W.mSliderJointsComponents, sl);
     return 0;
+}
+
+// Sets the joint solve order; with sleeping enabled it is re-applied to the enabled joints after every
+// enableDisableJoints (disabling and enabling a component moves it, Components.cpp:159-194).
+int rp3dref_set_joint_order(void* h, const int32_t* order, int n) {
+    RefWorld* R = (RefWorld*)h;
+    for (int i = 0; i < n; i++)
+        if (order[i] < 0 || order[i] >= (int)R->joints.size()) return -1;
+    R->joint_order.assign(order, order + n);
+    return apply_joint_order(R);
+}
+
+// RigidBody::isSleeping of every body.
+int rp3dref_get_sleeping(void* h, int32_t* out) {
+    RefWorld* R = (RefWorld*)h;
+    for (size_t i = 0; i < R->bodies.size(); i++) out[i] = R->bodies[i]->isSleeping() ? 1 : 0;
+    return (int)R->bodies.size();
 }
 
 // Bodies: origin position, orientation (x,y,z,w), centre-of-mass linear velocity, angular velocity.
@@ -391,16 +417,22 @@ int rp3dref_get_broadphase_ids(void* h, int32_t* ids) {
 }
 
 // Convex overlapping pairs: collider1 / collider2 as the reference oriented them.
+// Overlapping convex pairs, the enabled ones first, then the disabled ones (both bodies static or
+// sleeping, kept aside untested: OverlappingPairs::mDisabledConvexPairs).
 int rp3dref_get_pairs(void* h, int32_t* c1, int32_t* c2, int cap) {
     RefWorld* R = (RefWorld*)h;
-    auto& pairs = R->world->mCollisionDetection.mOverlappingPairs.mConvexPairs;
-    const int n = (int)pairs.size();
-    if (n > cap) return -n;
-    for (int i = 0; i < n; i++) {
-        c1[i] = collider_index(R, pairs[i].collider1);
-        c2[i] = collider_index(R, pairs[i].collider2);
+    auto& OP = R->world->mCollisionDetection.mOverlappingPairs;
+    const int ne = (int)OP.mConvexPairs.size(), nd = (int)OP.mDisabledConvexPairs.size();
+    if (ne + nd > cap) return -(ne + nd);
+    for (int i = 0; i < ne; i++) {
+        c1[i] = collider_index(R, OP.mConvexPairs[i].collider1);
+        c2[i] = collider_index(R, OP.mConvexPairs[i].collider2);
     }
-    return n;
+    for (int i = 0; i < nd; i++) {
+        c1[ne + i] = collider_index(R, OP.mDisabledConvexPairs[i].collider1);
+        c2[ne + i] = collider_index(R, OP.mDisabledConvexPairs[i].collider2);
+    }
+    return ne + nd;
 }
 
 // Contacts of the last step: per manifold (island order) collider1/2, nb points, friction impulses

@@ -9,6 +9,7 @@
 
 #include "../../include/rp3d_mi355x.h"
 #include "fused.h"
+#include "islands.h"
 #include "joints.h"
 #include "rp3d_internal.h"
 
@@ -30,6 +31,7 @@ struct rp3dg_batch {
     rp3dg::DevPairs Pa{}, Pb{};
     rp3dg::DevSolver S{};
     rp3dg::DevJoints J{};
+    rp3dg::DevIslands isl{};           // allocated when the worlds have sleeping enabled
     int* jt_ranges = nullptr;          // [2*slot] colour ranges of the joint rows, slots in solve order
     int2* jt_wr = nullptr;             // fused solver: [w * jt_slots + slot] joint rows of world w
     int jt_slots = 0;
@@ -144,6 +146,7 @@ struct rp3dg_batch {
     void set_class_blocks();
     int launch_world_classes(const char* name, const rp3dg::FusedArgs& F, bool lds);
     int upload_joints(rp3dg_world* const* worlds, int n);
+    void enqueue_sleep(float dt, const rp3dg::DevPairs& N, int pair_grid);
     int enqueue_step(float dt, bool capturing);
     int finish_steps(int n);
     int step(float dt);

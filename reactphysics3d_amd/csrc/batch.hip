@@ -437,10 +437,6 @@ int rp3dg_batch::upload(rp3dg_world* const* ws, int nw, const rp3dg_batch_option
     collider_base.clear();
     for (int wi = 0; wi < nw; wi++) {
         const HostWorld& W = *ws[wi];
-        if (W.cfg.is_sleeping_enabled) {
-            set_error("sleeping is not supported on the device path: set is_sleeping_enabled = 0");
-            return RP3DG_EUNSUPPORTED;
-        }
         const int b0 = (int)x.size();
         const int c0 = (int)cbody.size();
         const int h0 = (int)hh.size();
@@ -463,6 +459,7 @@ int rp3dg_batch::upload(rp3dg_world* const* ws, int nw, const rp3dg_batch_option
             if (b.type == RP3DG_KINEMATIC) f |= BF_KINEMATIC;
             if (b.type != RP3DG_STATIC) f |= BF_ENABLED;
             if (b.gravity) f |= BF_GRAVITY;
+            if (b.allowed_to_sleep) f |= BF_ALLOW_SLEEP;
             if (b.llock.x == 1.f && b.llock.y == 1.f && b.llock.z == 1.f && b.alock.x == 1.f && b.alock.y == 1.f &&
                 b.alock.z == 1.f)
                 f |= BF_UNIT_LOCKS;
@@ -527,6 +524,10 @@ int rp3dg_batch::upload(rp3dg_world* const* ws, int nw, const rp3dg_batch_option
             settings.vel_iters = (int)cfg.default_velocity_solver_iterations;
             settings.pos_iters = (int)cfg.default_position_solver_iterations;
             settings.gravity_enabled = cfg.is_gravity_enabled;
+            settings.sleeping_enabled = cfg.is_sleeping_enabled ? 1 : 0;
+            settings.sleep_lin_sq = cfg.sleep_linear_velocity * cfg.sleep_linear_velocity;
+            settings.sleep_ang_sq = cfg.sleep_angular_velocity * cfg.sleep_angular_velocity;
+            settings.time_before_sleep = cfg.time_before_sleep;
         } else if (memcmp(&cfg, &ws[0]->cfg, sizeof(cfg)) != 0) {
             set_error("all worlds of a batch must share the same WorldSettings");
             return RP3DG_EINVAL;
@@ -835,6 +836,14 @@ int rp3dg_batch::upload(rp3dg_world* const* ws, int nw, const rp3dg_batch_option
     HIPCHECK(hipMemcpy(C.is_static, is_static.data(), sizeof(int) * nc, hipMemcpyHostToDevice));
     HIPCHECK(hipMemcpy(C.ss_offs, ss_offs.data(), sizeof(int) * (nc + 1), hipMemcpyHostToDevice));
     if (!ss_list.empty()) HIPCHECK(hipMemcpy(C.ss_list, ss_list.data(), sizeof(int) * ss_list.size(), hipMemcpyHostToDevice));
+    if (settings.sleeping_enabled) {
+        isl.parent = dalloc<int>(this, nb);
+        isl.minst = dalloc<unsigned int>(this, nb);
+        isl.active = dalloc<int>(this, nb);
+        isl.sleep_time = dalloc<float>(this, nb);
+        if (!isl.sleep_time) { set_error("device allocation failed (islands)"); return RP3DG_EHIP; }
+        HIPCHECK(hipMemset(isl.sleep_time, 0, sizeof(float) * nb));
+    }
     return RP3DG_OK;
 }
 
@@ -958,6 +967,14 @@ int rp3dg_batch::enqueue_step(float dt, bool capturing) {
              &counters->nb_points, total_points);
     const int hm = capturing ? pair_cap : std::min(pair_cap, hint_manifolds + hint_manifolds / 4 + 1024);
     const int mg = grid_for(hm);
+    if (settings.sleeping_enabled) {
+        // ---------------- islands and wake-up (PhysicsWorld::createIslands, PhysicsWorld.cpp:695-866)
+        LAUNCH_T("isl_init", grid_for(nb), 256, isl_init, B, isl);
+        LAUNCH_T("isl_link", mg, 256, isl_link_manifolds, B, C, N, m_list, nm_ptr, isl);
+        if (J.n > 0) LAUNCH_T("isl_link", grid_for(J.n), 256, isl_link_joints, B, J, isl);
+        LAUNCH_T("isl_mark", grid_for(nb), 256, isl_mark, B, isl);
+        LAUNCH_T("isl_wake", grid_for(nb), 256, isl_wake, B, isl);
+    }
     const bool learned = fused && warm_mode == RP3DG_WARM_START_LEARNED && mlp;
     const bool features = learned || log_cap > 0;
     // manifold counts per body (the learned warm start's features read them; Jones-Plassmann needs them)
@@ -1062,6 +1079,7 @@ int rp3dg_batch::enqueue_step(float dt, bool capturing) {
             }
         }
         LAUNCH_T("world_position", ngroups, fused_block, world_position, F);
+        enqueue_sleep(dt, N, pg);
         return RP3DG_OK;
     }
 
@@ -1096,7 +1114,16 @@ int rp3dg_batch::enqueue_step(float dt, bool capturing) {
             LAUNCH_T("jt_solve_position", jg, 256, jt_solve_position, B, J, jt_ranges, s);
     LAUNCH_T("dyn_update_state", bg, 256, dyn_update_state, B);
     LAUNCH_T("bp_refit", grid_for(nc), 256, bp_refit, B, C, d_hbounds, 0);
+    enqueue_sleep(dt, N, pg);
     return RP3DG_OK;
+}
+
+// PhysicsWorld::updateSleepingBodies (PhysicsWorld.cpp:871-926), after the colliders are refit
+void rp3dg_batch::enqueue_sleep(float dt, const DevPairs& N, int pair_grid) {
+    if (!settings.sleeping_enabled) return;
+    LAUNCH_T("isl_sleep", grid_for(nb), 256, isl_sleep_time, B, isl, settings, dt);
+    LAUNCH_T("isl_sleep", grid_for(nb), 256, isl_sleep, B, isl, settings);
+    LAUNCH_T("isl_sleep", pair_grid, 256, isl_reset_disabled_pairs, B, C, N, &counters->nb_pairs);
 }
 
 // Workgroup size of every world class: clamp(L / block_div, 64, 256).
@@ -1287,8 +1314,11 @@ int rp3dg_world_get_body_state(rp3dg_world* w, int32_t body, float* pos, float* 
     if (w->mirror_dirty || w->mirror_pos.empty()) {
         const size_t n = w->bodies.size();
         w->mirror_pos.resize(3 * n); w->mirror_q.resize(4 * n); w->mirror_v.resize(3 * n); w->mirror_w.resize(3 * n);
-        const int r = rp3dg_batch_get_bodies(w->own, w->mirror_pos.data(), w->mirror_q.data(), w->mirror_v.data(),
-                                             w->mirror_w.data());
+        int r = rp3dg_batch_get_bodies(w->own, w->mirror_pos.data(), w->mirror_q.data(), w->mirror_v.data(),
+                                       w->mirror_w.data());
+        if (r) return r;
+        w->mirror_sleeping.resize(n);
+        r = rp3dg_batch_get_sleeping(w->own, w->mirror_sleeping.data(), (int64_t)n);
         if (r) return r;
         w->mirror_dirty = false;
     }
@@ -1300,6 +1330,14 @@ int rp3dg_world_get_body_state(rp3dg_world* w, int32_t body, float* pos, float* 
     for (int k = 0; k < 4; k++)
         if (ori) ori[k] = w->mirror_q[4 * body + k];
     return RP3DG_OK;
+}
+
+int rp3dg_world_get_body_sleeping(rp3dg_world* w, int32_t body) {
+    if (!w || body < 0 || body >= (int32_t)w->bodies.size()) { set_error("invalid body index"); return RP3DG_EINVAL; }
+    if (!w->own) return 0;   // bodies are created awake
+    const int r = rp3dg_world_get_body_state(w, body, nullptr, nullptr, nullptr, nullptr);
+    if (r) return r;
+    return w->mirror_sleeping[body];
 }
 
 void rp3dg_world_destroy(rp3dg_world* w) {
@@ -1327,6 +1365,17 @@ int rp3dg_batch_get_counts(rp3dg_batch* b, rp3dg_counts* o) {
     o->steps = b->steps_done;
     o->nb_new_points = b->last.nb_new_points;
     o->nb_box_tests = b->last.nb_box_tests + b->last.nb_box_sep_tests;
+    return RP3DG_OK;
+}
+
+int rp3dg_batch_get_sleeping(rp3dg_batch* b, int32_t* sleeping, int64_t capacity) {
+    if (!b || !sleeping || capacity < b->nb) { set_error("sleeping: capacity below the body count"); return RP3DG_EINVAL; }
+    std::vector<int> f(b->nb);
+    HIPCHECK(hipStreamSynchronize(b->stream));
+    HIPCHECK(hipMemcpy(f.data(), b->B.flags, sizeof(int) * b->nb, hipMemcpyDeviceToHost));
+    // a non-static body outside the enabled partition is asleep (RigidBody::isSleeping)
+    for (int i = 0; i < b->nb; i++)
+        sleeping[i] = ((f[i] & (BF_DYNAMIC | BF_KINEMATIC)) && !(f[i] & BF_ENABLED)) ? 1 : 0;
     return RP3DG_OK;
 }
 
@@ -1374,6 +1423,24 @@ int rp3dg_batch_set_bodies(rp3dg_batch* b, const float* pos, const float* ori, c
     int r;
     if ((r = put(b->B.p, pos, 3)) || (r = put(b->B.q, ori, 4)) || (r = put(b->B.v, lin, 3)) || (r = put(b->B.w, ang, 3)))
         return r;
+    if (b->settings.sleeping_enabled && (pos || ori || lin || ang)) {
+        // setTransform wakes the body, setLinear/AngularVelocity do when the new velocity is not zero
+        // (RigidBody.cpp); waking = setIsSleeping(false) (RigidBody.cpp:1006-1057)
+        std::vector<int> f(n);
+        std::vector<float> st(n);
+        HIPCHECK(hipMemcpy(f.data(), b->B.flags, sizeof(int) * n, hipMemcpyDeviceToHost));
+        HIPCHECK(hipMemcpy(st.data(), b->isl.sleep_time, sizeof(float) * n, hipMemcpyDeviceToHost));
+        auto nonzero = [](const float* a, int i) { return a && (a[3 * i] != 0.f || a[3 * i + 1] != 0.f || a[3 * i + 2] != 0.f); };
+        for (int i = 0; i < n; i++) {
+            const bool asleep = (f[i] & (BF_DYNAMIC | BF_KINEMATIC)) && !(f[i] & BF_ENABLED);
+            if (asleep && (pos || ori || nonzero(lin, i) || nonzero(ang, i))) {
+                f[i] |= BF_ENABLED | BF_WOKEN;
+                st[i] = 0.0f;
+            }
+        }
+        HIPCHECK(hipMemcpy(b->B.flags, f.data(), sizeof(int) * n, hipMemcpyHostToDevice));
+        HIPCHECK(hipMemcpy(b->isl.sleep_time, st.data(), sizeof(float) * n, hipMemcpyHostToDevice));
+    }
     if (pos || ori) {
         set_body_com<<<grid_for(n), 256, 0, b->stream>>>(b->B);
         bp_refit<<<grid_for(b->nc), 256, 0, b->stream>>>(b->B, b->C, b->d_hbounds, 0);

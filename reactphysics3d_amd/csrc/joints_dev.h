@@ -117,6 +117,12 @@ __device__ inline float hinge_angle(Q4 q1, Q4 q2, Q4 qinit, V3 a1, float lo, flo
     return angle_near_limits(ang, lo, hi);
 }
 
+// PhysicsWorld::enableDisableJoints (PhysicsWorld.cpp:363-393): a joint whose bodies are both disabled
+// (static or sleeping) is not solved and keeps its impulses
+__device__ __forceinline__ bool joint_enabled(const DevBodies& B, int b1, int b2) {
+    return ((body_flags(B, b1) | body_flags(B, b2)) & BF_ENABLED) != 0;
+}
+
 // ------------------------------------------------------------------------------------------------
 // initBeforeSolve (all joints, one launch; no body state is written)
 // ------------------------------------------------------------------------------------------------
@@ -124,6 +130,7 @@ __device__ __forceinline__ void jt_init_one(const DevBodies& B, const DevJoints&
     {
         const int2 bb = J.bodies[j];
         const int b1 = bb.x, b2 = bb.y;
+        if (!joint_enabled(B, b1, b2)) return;
         const int t = J.type[j];
         const M3 I1 = ldm(B.iw0, B.iw1, B.iw2, b1), I2 = ldm(B.iw0, B.iw1, B.iw2, b2);
         stm(J.i1r0, J.i1r1, J.i1r2, j, I1);
@@ -274,6 +281,7 @@ __device__ __forceinline__ void jt_warmstart_one(const DevBodies& B, const DevJo
     {
         const int2 bb = J.bodies[j];
         const int b1 = bb.x, b2 = bb.y;
+        if (!joint_enabled(B, b1, b2)) return;
         const int type = J.type[j];
         const int u1 = b1 - B.vbase, u2 = b2 - B.vbase;
         V3 v1 = ld3(B.cv, u1), w1 = ld3(B.cw, u1), v2 = ld3(B.cv, u2), w2 = ld3(B.cw, u2);
@@ -365,6 +373,7 @@ __device__ __forceinline__ void jt_solve_velocity_one(const DevBodies& B, const 
     {
         const int2 bb = J.bodies[j];
         const int b1 = bb.x, b2 = bb.y;
+        if (!joint_enabled(B, b1, b2)) return;
         const int type = J.type[j];
         const int u1 = b1 - B.vbase, u2 = b2 - B.vbase;
         V3 v1 = ld3(B.cv, u1), w1 = ld3(B.cw, u1), v2 = ld3(B.cv, u2), w2 = ld3(B.cw, u2);
@@ -565,6 +574,7 @@ __device__ __forceinline__ void jt_solve_position_one(const DevBodies& B, const 
     {
         const int2 bb = J.bodies[j];
         const int b1 = bb.x, b2 = bb.y;
+        if (!joint_enabled(B, b1, b2)) return;
         const int type = J.type[j];
         Q4 q1 = ldq(B.cq, b1), q2 = ldq(B.cq, b2);
         V3 x1 = ld3(B.cx, b1), x2 = ld3(B.cx, b2);

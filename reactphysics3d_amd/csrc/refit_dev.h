@@ -89,6 +89,9 @@ __device__ __forceinline__ void refit_one(const DevBodies& B, const DevColliders
                 C.amax[i] = f4(fmx, 0.f);
                 moved = 1;
             }
+            // a body woken this step put its colliders into the moved set without a new fat AABB
+            // (RigidBody::askForBroadPhaseCollisionCheck)
+            if (bf & BF_WOKEN) moved = 1;
         }
         C.moved[i] = moved;
     }

@@ -43,6 +43,8 @@ constexpr int BF_ENABLED = 2;   // in the enabled partition (not static, not sle
 constexpr int BF_GRAVITY = 4;   // gravity enabled
 constexpr int BF_KINEMATIC = 8;
 constexpr int BF_UNIT_LOCKS = 16;   // linear and angular lock factors are (1, 1, 1): the solver skips them
+constexpr int BF_ALLOW_SLEEP = 32;  // RigidBody::isAllowedToSleep
+constexpr int BF_WOKEN = 64;        // woken up this step: its colliders enter the moved set (RigidBody.cpp:1045-1051)
 
 // LastFrameCollisionInfo (include/reactphysics3d/engine/OverlappingPairs.h:55-93) packed in a word:
 constexpr uint32_t LF_VALID = 1u << 0;
@@ -154,6 +156,9 @@ struct Settings {
     float cos_similar;
     int vel_iters, pos_iters;
     int gravity_enabled;
+    int sleeping_enabled;
+    float sleep_lin_sq, sleep_ang_sq;   // squared sleep velocities (PhysicsWorld.cpp:875-876)
+    float time_before_sleep;
 };
 
 // Counters written by kernels (one device block, read back after each step).

@@ -241,6 +241,9 @@ void rp3dg_world_settings_default(rp3dg_world_settings* s) {
     s->default_position_solver_iterations = 3;
     s->is_sleeping_enabled = 1;
     s->is_gravity_enabled = 1;
+    s->time_before_sleep = 1.0f;
+    s->sleep_linear_velocity = 0.02f;
+    s->sleep_angular_velocity = 3.0f * (3.141592653589f / 180.0f);
 }
 
 rp3dg_world* rp3dg_world_create(const rp3dg_world_settings* s) {
@@ -315,6 +318,13 @@ int rp3dg_body_set_damping(rp3dg_world* w, int32_t bi, float l, float a) {
     if (l < 0.f || a < 0.f) { set_error("damping must be >= 0"); return RP3DG_EINVAL; }
     w->bodies[bi].ldamp = l;
     w->bodies[bi].adamp = a;
+    return RP3DG_OK;
+}
+
+int rp3dg_body_set_allowed_to_sleep(rp3dg_world* w, int32_t bi, int32_t allowed) {
+    if (!valid_body(w, bi)) return RP3DG_EINVAL;
+    if (!mutable_world(w)) return RP3DG_ESTATE;
+    w->bodies[bi].allowed_to_sleep = allowed != 0;
     return RP3DG_OK;
 }
 

@@ -44,6 +44,7 @@ struct HostBody {
     V3 inertia{1, 1, 1}, inv_inertia{1, 1, 1};
     int type = 2;                              // DYNAMIC
     bool gravity = true;
+    bool allowed_to_sleep = true;
     std::vector<int> colliders;
 };
 
@@ -75,6 +76,7 @@ struct HostWorld {
     bool uploaded = false;
     // host mirror of the device body state, refreshed lazily after update()
     std::vector<float> mirror_pos, mirror_q, mirror_v, mirror_w;
+    std::vector<int32_t> mirror_sleeping;
     bool mirror_dirty = false;
 };
 

@@ -29,6 +29,9 @@ class WorldSettings(C.Structure):
         ("default_position_solver_iterations", C.c_uint32),
         ("is_sleeping_enabled", C.c_int32),
         ("is_gravity_enabled", C.c_int32),
+        ("time_before_sleep", C.c_float),
+        ("sleep_linear_velocity", C.c_float),
+        ("sleep_angular_velocity", C.c_float),
     ]
 
 
@@ -61,7 +64,9 @@ EXPORTED = [
     "rp3dg_body_set_damping", "rp3dg_body_enable_gravity", "rp3dg_body_apply_force_torque",
     "rp3dg_body_set_lock_factors", "rp3dg_body_update_mass_properties", "rp3dg_body_set_mass_properties",
     "rp3dg_collider_add", "rp3dg_collider_set_material", "rp3dg_collider_set_filter", "rp3dg_hull_create",
-    "rp3dg_joint_create", "rp3dg_joint_set_limits", "rp3dg_world_update", "rp3dg_batch_create", "rp3dg_batch_create_ex",
+    "rp3dg_joint_create", "rp3dg_joint_set_limits", "rp3dg_body_set_allowed_to_sleep", "rp3dg_batch_get_sleeping",
+    "rp3dg_world_get_body_sleeping",
+    "rp3dg_world_update", "rp3dg_batch_create", "rp3dg_batch_create_ex",
     "rp3dg_batch_destroy", "rp3dg_batch_step",
     "rp3dg_batch_step_graph", "rp3dg_batch_synchronize", "rp3dg_batch_get_counts", "rp3dg_batch_get_bodies",
     "rp3dg_batch_set_bodies", "rp3dg_batch_get_pairs", "rp3dg_batch_get_contacts", "rp3dg_batch_get_joint_order",
@@ -115,6 +120,9 @@ def load():
         "rp3dg_hull_create": (i32, [vp, fptr, i32, iptr, iptr, i32]),
         "rp3dg_joint_create": (i32, [vp, i32, i32, i32, fptr, fptr, i32, i32, fptr]),
         "rp3dg_joint_set_limits": (C.c_int, [vp, i32, i32, fptr]),
+        "rp3dg_body_set_allowed_to_sleep": (C.c_int, [vp, i32, i32]),
+        "rp3dg_batch_get_sleeping": (C.c_int, [vp, iptr, C.c_int64]),
+        "rp3dg_world_get_body_sleeping": (C.c_int, [vp, i32]),
         "rp3dg_world_update": (C.c_int, [vp, f]),
         "rp3dg_batch_create": (vp, [C.POINTER(vp), i32]),
         "rp3dg_batch_create_ex": (vp, [C.POINTER(vp), i32, C.POINTER(BatchOptions)]),
@@ -287,6 +295,12 @@ class Batch:
         w = np.zeros((n, 3), np.float32)
         _check(load().rp3dg_batch_get_bodies(self.h, _fp(p), _fp(q), _fp(v), _fp(w)), "get_bodies")
         return p, q, v, w
+
+    def sleeping(self):
+        """RigidBody::isSleeping of every body, int32 0 / 1."""
+        out = np.zeros(self.nb, np.int32)
+        _check(load().rp3dg_batch_get_sleeping(self.h, _ip(out), self.nb), "get_sleeping")
+        return out
 
     def set_bodies(self, pos=None, quat=None, lin=None, ang=None):
         """RigidBody::setTransform / setLinearVelocity / setAngularVelocity for every body (None = keep)."""

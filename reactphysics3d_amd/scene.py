@@ -425,6 +425,42 @@ def hull_scene(n_hulls=16384, n_chains=256, chain_length=16, extent=(120.0, 120.
     return sc
 
 
+def sleeping_pile(n_side=3, levels=2, seed=5, ball_height=None, ball_radius=0.4):
+    """box_pile with sleeping enabled (WorldSettings defaults: 1 s still below 0.02 m/s and 3 deg/s);
+    with ball_height, a dense sphere (density 5) dropped from that height near the pile's centre
+    lands on the pile after it fell asleep and wakes the island it hits."""
+    sc = box_pile(n_side, levels, seed=seed)
+    sc.name = f"sleeping_pile_{n_side * n_side * levels}" + ("_ball" if ball_height else "")
+    sc.settings["sleeping"] = 1
+    if ball_height:
+        b = sc.add_body(DYNAMIC, (0.3, 0.2, ball_height))
+        sc.add_collider(b, SPHERE, (ball_radius,), density=5.0)
+    return sc
+
+
+def sleeping_chain(length=6, radius=0.3, hit_distance=5.0, hit_speed=3.0, seed=6):
+    """A vertical chain of spheres hanging at rest from a static anchor by ball-and-socket joints, with
+    sleeping enabled: it falls asleep after the time before sleep.  A sphere without gravity flies at
+    hit_speed towards the lowest sphere from hit_distance away, hits it after the chain slept, and
+    wakes the whole chain through its joints.  Friction 0.5 and no bounce, as in box_pile, so that both
+    can share a batch."""
+    rng = SplitMix64(seed)
+    sc = Scene(f"sleeping_chain_{length}", make_settings(gravity=(0.0, 0.0, -9.81), vel_iters=10, pos_iters=5,
+                                                         friction=0.5, bounciness=0.0, sleeping=1))
+    top = 2.0 * radius * length + 2.0
+    prev = sc.add_body(STATIC, (0.0, 0.0, top + radius))
+    for k in range(length):
+        z = top - radius * (2 * k + 1)
+        b = sc.add_body(DYNAMIC, (0.0, 0.0, z), rng.quaternion())
+        sc.add_collider(b, SPHERE, (radius,))
+        sc.add_joint(BALLSOCKET, prev, b, (0.0, 0.0, z + radius))
+        prev = b
+    z_low = top - radius * (2 * length - 1)
+    ball = sc.add_body(DYNAMIC, (hit_distance, 0.05, z_low), lin=(-hit_speed, 0.0, 0.0), gravity=False)
+    sc.add_collider(ball, SPHERE, (radius,), density=2.0)
+    return sc
+
+
 def batched_world(world_id, seed=4, min_bodies=8, max_bodies=512, max_joints=32):
     """One world of the many-world batch: n = round(exp(U[ln min, ln max])) unit boxes (1 kg, friction
     0.5, restitution 0) on a static ground, either a pile (seeded grid, jitter 0.2 m, random

@@ -22,7 +22,8 @@ def _run(steps):
 
 
 def _scene():
-    sc = S.Scene("hello_world", S.make_settings(gravity=(0.0, 0.0, -9.81), vel_iters=10, pos_iters=5))
+    # WorldSettings defaults: sleeping on
+    sc = S.Scene("hello_world", S.make_settings(gravity=(0.0, 0.0, -9.81), vel_iters=10, pos_iters=5, sleeping=1))
     g = sc.add_body(S.STATIC, (0.0, 0.0, -0.5))
     sc.add_collider(g, S.BOX, (10.0, 10.0, 0.5), friction=0.5, bounciness=0.0)
     b = sc.add_body(S.DYNAMIC, (0.0, 0.0, 2.0), (0.0998334, 0.0, 0.0, 0.9950042))
@@ -67,7 +68,7 @@ def _run_joints(steps):
 
 def _joints_scene():
     """The world of examples/joints.cpp as a scene description."""
-    sc = S.Scene("joints_example", S.make_settings(gravity=(0.0, 0.0, -9.81), vel_iters=10, pos_iters=5))
+    sc = S.Scene("joints_example", S.make_settings(gravity=(0.0, 0.0, -9.81), vel_iters=10, pos_iters=5, sleeping=1))
     post = sc.add_body(S.STATIC, (0.0, 0.0, 5.0))
     car = sc.add_body(S.DYNAMIC, (0.0, 0.0, 4.2))
     sc.add_collider(car, S.BOX, (0.3, 0.3, 0.3))
@@ -92,12 +93,14 @@ def test_joints_example_builds_and_creates_world():
 @pytest.mark.gpu
 def test_joints_example_steps_like_the_reference(ref_available):
     """Limits and motors set through SliderJointInfo fields and through the joint setters after
-    createJoint reach the device: the example matches the reference world built with the same limits."""
+    createJoint reach the device: the example matches the reference world built with the same limits
+    (over 240 steps, long enough for the pendulums at rest on their limits to fall asleep)."""
     from oracle.ref import RefWorld
-    st = _run_joints(120)
+    st = _run_joints(240)
     ref = RefWorld(_joints_scene())
-    for _ in range(120):
+    for _ in range(240):
         ref.step(1.0 / 60.0)
+    print("asleep in the reference:", ref.sleeping())
     p, q, v, w = ref.bodies()
     want = np.concatenate([p, q, v, w], 1)
     err = np.max(np.abs(st - want))

@@ -152,3 +152,57 @@ def test_hinge_limit_stops_the_paddle(ref_available):
     down = run(1, (-0.6, 0.4, 0.0, 0.0), 120)
     assert down[:, 0].max() < 0.4 + 0.05
     assert down[-1, 0] == pytest.approx(0.4, abs=0.02)
+
+
+def _first_step_asleep(dt=DT, before_sleep=1.0):
+    """First step k whose float32 running sum of dt reaches the time before sleep (PhysicsWorld.cpp:871-926
+    adds the step to each still body's sleep time and compares the island minimum)."""
+    t, dt32 = np.float32(0.0), np.float32(dt)
+    for k in range(1, 100000):
+        t = np.float32(t + dt32)
+        if t >= np.float32(before_sleep):
+            return k
+    raise AssertionError
+
+
+def test_still_bodies_fall_asleep_after_the_time_before_sleep(ref_available):
+    """Zero gravity: a body at rest and one drifting at 0.01 m/s (below the 0.02 m/s sleep velocity)
+    fall asleep at the first step whose accumulated sleep time reaches 1 s, and the drifting one stops;
+    a body at 0.03 m/s never sleeps."""
+    from oracle.ref import RefWorld
+    sc = S.Scene("sleep_pin", S.make_settings(gravity=(0.0, 0.0, 0.0), sleeping=1))
+    for i, v in enumerate((0.0, 0.01, 0.03)):
+        b = sc.add_body(S.DYNAMIC, (3.0 * i, 0.0, 0.0), lin=(v, 0.0, 0.0))
+        sc.add_collider(b, S.SPHERE, (0.5,))
+    ref = RefWorld(sc)
+    k_sleep = _first_step_asleep()
+    for k in range(1, k_sleep + 20):
+        ref.step(DT)
+        asleep = list(ref.sleeping())
+        assert asleep == [int(k >= k_sleep), int(k >= k_sleep), 0], f"step {k}"
+    p, q, v, w = ref.bodies()
+    assert v[0, 0] == 0.0 and v[1, 0] == 0.0 and v[2, 0] == pytest.approx(0.03)
+    assert p[1, 0] == pytest.approx(3.0 + 0.01 * DT * k_sleep, abs=1e-6)   # it still moves in the step it falls asleep
+
+
+def test_a_joint_keeps_its_island_awake(ref_available):
+    """A body at rest joined by a ball-and-socket joint (anchored at the other body's centre) to a body
+    spinning at 1 rad/s shares its island, so the island's minimum sleep time stays 0 and neither sleeps;
+    the same body without the joint falls asleep on time."""
+    from oracle.ref import RefWorld
+
+    def run(with_joint):
+        sc = S.Scene("island_pin", S.make_settings(gravity=(0.0, 0.0, 0.0), sleeping=1))
+        a = sc.add_body(S.DYNAMIC, (0.0, 0.0, 0.0))
+        sc.add_collider(a, S.SPHERE, (0.5,))
+        b = sc.add_body(S.DYNAMIC, (2.0, 0.0, 0.0), ang=(0.0, 0.0, 1.0))
+        sc.add_collider(b, S.SPHERE, (0.5,))
+        if with_joint:
+            sc.add_joint(S.BALLSOCKET, a, b, (2.0, 0.0, 0.0))
+        ref = RefWorld(sc)
+        for _ in range(_first_step_asleep() + 5):
+            ref.step(DT)
+        return list(ref.sleeping())
+
+    assert run(True) == [0, 0]
+    assert run(False) == [1, 0]
