@@ -230,6 +230,22 @@ typedef struct rp3dg_contact_view {
     float* friction;        /* 3 per manifold: friction1, friction2, twist impulses */
 } rp3dg_contact_view;
 int rp3dg_batch_get_contacts(rp3dg_batch* batch, rp3dg_contact_view* out, int64_t capacity);
+/* Contact events, what PhysicsWorld::setEventListener + EventListener::onContact deliver
+ * (CollisionDetectionSystem::reportContacts, CollisionDetectionSystem.cpp:1708-1741): once enabled,
+ * every step records one event per colliding pair, RP3DG_CONTACT_START or RP3DG_CONTACT_STAY (by
+ * whether the pair collided in the previous step), and RP3DG_CONTACT_EXIT for every pair that stopped
+ * colliding or that the broadphase removed while colliding (CollisionCallback.cpp:69-76).  Enabling or
+ * disabling drops captured step graphs. */
+#define RP3DG_CONTACT_START 0
+#define RP3DG_CONTACT_STAY 1
+#define RP3DG_CONTACT_EXIT 2
+int rp3dg_batch_set_contact_events(rp3dg_batch* batch, int32_t enable);
+/* The events of the last step (of the last step of a graph replay), in no particular order: colliders
+ * (batch indices, oriented as the pair), event type and contact points (0 for an exit).  Returns the
+ * count (only the count when every output pointer is NULL), or a negative status (RP3DG_ESTATE when
+ * events are off, RP3DG_EINVAL if capacity is short). */
+int64_t rp3dg_batch_get_contact_events(rp3dg_batch* batch, int32_t* collider1, int32_t* collider2, int32_t* type,
+                                       int32_t* nb_points, int64_t capacity);
 /* RigidBody::isSleeping of every body of the batch (batch order), 0 / 1; capacity >= body count. */
 int rp3dg_batch_get_sleeping(rp3dg_batch* batch, int32_t* sleeping, int64_t capacity);
 /* Joint solve order (joint indices in the order the device solved them this step, per type block). */

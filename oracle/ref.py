@@ -121,6 +121,22 @@ class RefWorld:
         if self.L.rp3dref_set_joint_order(self.h, _ip(o), len(o)) != 0:
             raise RuntimeError("set_joint_order: unsupported joint kind")
 
+    def enable_events(self):
+        """PhysicsWorld::setEventListener with a recorder of the contact events of every step."""
+        self.L.rp3dref_enable_events.argtypes = [C.c_void_p]
+        self.L.rp3dref_enable_events(self.h)
+
+    def events(self):
+        """Contact events of the last step, rows (collider1, collider2, type, contact points) sorted by
+        (collider1, collider2); type 0 ContactStart, 1 ContactStay, 2 ContactExit."""
+        cap = 64 * max(1, self.nc)
+        out = np.zeros((cap, 4), np.int32)
+        self.L.rp3dref_get_events.argtypes = [C.c_void_p, C.POINTER(C.c_int32), C.c_int32]
+        n = self.L.rp3dref_get_events(self.h, _ip(out), cap)
+        assert n >= 0
+        ev = out[:n]
+        return ev[np.lexsort((ev[:, 1], ev[:, 0]))] if n else ev
+
     def sleeping(self):
         """RigidBody::isSleeping of every body, as int32 0 / 1."""
         out = np.zeros(self.nb, np.int32)

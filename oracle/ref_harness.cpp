@@ -68,6 +68,16 @@ struct RefJoint {
 
 }  // extern "C"
 
+struct RefWorld;
+
+// EventListener that records every contact event of a step as (collider1, collider2, event type,
+// contact points).
+struct EventRecorder : public EventListener {
+    RefWorld* R = nullptr;
+    std::vector<int32_t> ev;
+    void onContact(const CollisionCallback::CallbackData& data) override;
+};
+
 struct RefWorld {
     PhysicsCommon common;
     PhysicsWorld* world = nullptr;
@@ -77,6 +87,7 @@ struct RefWorld {
     std::unordered_map<uint32_t, int> collider_of_entity;
     std::unordered_map<uint32_t, int> body_of_entity;
     std::vector<int32_t> joint_order;   // solve order requested by rp3dref_set_joint_order (joint indices)
+    EventRecorder* events = nullptr;    // set by rp3dref_enable_events
     float last_dt = 0.f;
 };
 
@@ -211,11 +222,33 @@ void rp3dref_destroy(void* h) {
     RefWorld* R = (RefWorld*)h;
     if (!R) return;
     R->common.destroyPhysicsWorld(R->world);
+    delete R->events;
     delete R;
+}
+
+// PhysicsWorld::setEventListener with a recorder; rp3dref_get_events returns the last step's events.
+int rp3dref_enable_events(void* h) {
+    RefWorld* R = (RefWorld*)h;
+    if (!R->events) {
+        R->events = new EventRecorder();
+        R->events->R = R;
+        R->world->setEventListener(R->events);
+    }
+    return 0;
+}
+
+int rp3dref_get_events(void* h, int32_t* out, int cap) {
+    RefWorld* R = (RefWorld*)h;
+    if (!R->events) return -1;
+    const int n = (int)R->events->ev.size() / 4;
+    if (n > cap) return -n;
+    std::memcpy(out, R->events->ev.data(), sizeof(int32_t) * 4 * n);
+    return n;
 }
 
 int rp3dref_step(void* h, float dt) {
     RefWorld* R = (RefWorld*)h;
+    if (R->events) R->events->ev.clear();
     R->world->update(dt);
     R->last_dt = dt;
     return 0;
@@ -224,6 +257,16 @@ int rp3dref_step(void* h, float dt) {
 static int collider_index(RefWorld* R, Entity e) {
     auto it = R->collider_of_entity.find(e.id);
     return it == R->collider_of_entity.end() ? -1 : it->second;
+}
+
+void EventRecorder::onContact(const CollisionCallback::CallbackData& data) {
+    for (uint32 i = 0; i < data.getNbContactPairs(); i++) {
+        const CollisionCallback::ContactPair p = data.getContactPair(i);
+        ev.push_back(collider_index(R, p.getCollider1()->getEntity()));
+        ev.push_back(collider_index(R, p.getCollider2()->getEntity()));
+        ev.push_back((int32_t)p.getEventType());
+        ev.push_back((int32_t)p.getNbContactPoints());
+    }
 }
 
 static int apply_joint_order(RefWorld* R);
@@ -285,6 +328,7 @@ static int solver_init_ordered(RefWorld* R, float dt, const int32_t* oc1, const 
 int rp3dref_step_ordered(void* h, float dt, const int32_t* oc1, const int32_t* oc2, int n) {
     RefWorld* R = (RefWorld*)h;
     PhysicsWorld& W = *R->world;
+    if (R->events) R->events->ev.clear();
     W.mCollisionDetection.computeCollisionDetection();
     W.createIslands();
     W.mCollisionDetection.createContacts();

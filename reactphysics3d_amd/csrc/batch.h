@@ -8,6 +8,7 @@
 #include <vector>
 
 #include "../../include/rp3d_mi355x.h"
+#include "events.h"
 #include "fused.h"
 #include "islands.h"
 #include "joints.h"
@@ -32,6 +33,9 @@ struct rp3dg_batch {
     rp3dg::DevSolver S{};
     rp3dg::DevJoints J{};
     rp3dg::DevIslands isl{};           // allocated when the worlds have sleeping enabled
+    int4* ev_buf = nullptr;            // contact events of the last step (allocated when first enabled)
+    int ev_cap = 0;
+    bool events_on = false;
     int* jt_ranges = nullptr;          // [2*slot] colour ranges of the joint rows, slots in solve order
     int2* jt_wr = nullptr;             // fused solver: [w * jt_slots + slot] joint rows of world w
     int jt_slots = 0;

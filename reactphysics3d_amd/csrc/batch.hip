@@ -231,6 +231,8 @@ static void alloc_pairs(rp3dg_batch* b, DevPairs& P, int cap) {
     P.pn = dalloc<float4>(b, 4 * (size_t)cap);
     P.pl1 = dalloc<float4>(b, 4 * (size_t)cap);
     P.pl2 = dalloc<float4>(b, 4 * (size_t)cap);
+    P.evf = dalloc<int>(b, cap);
+    P.kept = dalloc<int>(b, cap);
 }
 
 // Box half-edge structure (PhysicsCommon.cpp:73-109 face lists), numbered by the reference rule.
@@ -925,8 +927,11 @@ int rp3dg_batch::enqueue_step(float dt, bool capturing) {
     }
     const int hp = capturing ? pair_cap : std::min(pair_cap, hint_pairs + hint_pairs / 4 + 1024);
     const int pg = grid_for(hp);
+    DevEvents E{ev_buf, ev_cap, &counters->nb_events, events_on ? 1 : 0};
+    if (events_on) fill_int_n<<<pg, 256, 0, stream>>>(O.kept, pair_cap, old_count, 0);
     LAUNCH_T("bp_merge", pg, 256, bp_merge, C, bp_out, bp_offs + nc, pair_cap, O, old_count, old_offs, N, new_count,
              &counters->nb_pairs);
+    if (events_on) LAUNCH_T("ev_removed", pg, 256, ev_removed, O, old_count, E);
     cur_is_a = !cur_is_a;
 
     // ---------------- narrowphase (CD.cpp:618-718)
@@ -958,6 +963,9 @@ int rp3dg_batch::enqueue_step(float dt, bool capturing) {
     LAUNCH_T("np_poly_poly", hint(last.nb_hull_tests, 128), 128, np_poly_poly, A, L + ALGO_POLY_POLY * pair_cap,
              &counters->nb_hull_tests);
 
+    // ---------------- contact flags and events (CD.cpp:975-1011, :1708-1741)
+    LAUNCH_T("ev_update", pg, 256, ev_update, B, C, N, &counters->nb_pairs, E, settings.sleeping_enabled ? 0 : 1);
+
     // ---------------- manifold list + colouring (replaces createIslands, PhysicsWorld.cpp:695-866)
     int* nm_ptr = &counters->nb_manifolds;
     LAUNCH_T("sv_flag", pg, 256, sv_flag_manifolds, N.ncp, &counters->nb_pairs, m_flag);
@@ -974,6 +982,7 @@ int rp3dg_batch::enqueue_step(float dt, bool capturing) {
         if (J.n > 0) LAUNCH_T("isl_link", grid_for(J.n), 256, isl_link_joints, B, J, isl);
         LAUNCH_T("isl_mark", grid_for(nb), 256, isl_mark, B, isl);
         LAUNCH_T("isl_wake", grid_for(nb), 256, isl_wake, B, isl);
+        LAUNCH_T("ev_shift", pg, 256, ev_shift, B, C, N, &counters->nb_pairs);
     }
     const bool learned = fused && warm_mode == RP3DG_WARM_START_LEARNED && mlp;
     const bool features = learned || log_cap > 0;
@@ -1366,6 +1375,38 @@ int rp3dg_batch_get_counts(rp3dg_batch* b, rp3dg_counts* o) {
     o->nb_new_points = b->last.nb_new_points;
     o->nb_box_tests = b->last.nb_box_tests + b->last.nb_box_sep_tests;
     return RP3DG_OK;
+}
+
+int rp3dg_batch_set_contact_events(rp3dg_batch* b, int32_t enable) {
+    if (!b) return RP3DG_EINVAL;
+    if (enable && !b->ev_buf) {
+        b->ev_cap = 2 * b->pair_cap;   // <= one event per pair of the new table + one per removed pair
+        b->ev_buf = dalloc<int4>(b, (size_t)b->ev_cap);
+        if (!b->ev_buf) { set_error("device allocation failed (events)"); return RP3DG_EHIP; }
+    }
+    if ((enable != 0) != b->events_on) b->drop_graphs();
+    b->events_on = enable != 0;
+    return RP3DG_OK;
+}
+
+int64_t rp3dg_batch_get_contact_events(rp3dg_batch* b, int32_t* c1, int32_t* c2, int32_t* type, int32_t* nb_points,
+                                       int64_t capacity) {
+    if (!b) return RP3DG_EINVAL;
+    if (!b->events_on) { set_error("contact events are off: rp3dg_batch_set_contact_events(batch, 1)"); return RP3DG_ESTATE; }
+    const int n = b->last.nb_events;
+    if (n > b->ev_cap) { set_error("contact event buffer overflow"); return RP3DG_EOVERFLOW; }
+    if (!c1 && !c2 && !type && !nb_points) return n;   // count query
+    if (capacity < n) { set_error("capacity below the event count"); return RP3DG_EINVAL; }
+    std::vector<int4> e(std::max(n, 1));
+    HIPCHECK(hipStreamSynchronize(b->stream));
+    if (n) HIPCHECK(hipMemcpy(e.data(), b->ev_buf, sizeof(int4) * n, hipMemcpyDeviceToHost));
+    for (int i = 0; i < n; i++) {
+        if (c1) c1[i] = e[i].x;
+        if (c2) c2[i] = e[i].y;
+        if (type) type[i] = e[i].z;
+        if (nb_points) nb_points[i] = e[i].w;
+    }
+    return n;
 }
 
 int rp3dg_batch_get_sleeping(rp3dg_batch* b, int32_t* sleeping, int64_t capacity) {

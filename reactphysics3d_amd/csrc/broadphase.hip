@@ -574,7 +574,10 @@ __global__ void bp_merge(DevColliders C, const uint64_t* keys, const int* nkeys_
             N.algo[k] = O.algo[found];
             N.lf[k] = O.lf[found];
             N.lfaxis[k] = O.lfaxis[found];
+            N.evf[k] = O.evf[found];
+            O.kept[found] = 1;
         } else {
+            N.evf[k] = 0;
             const int a = (int)(key >> 32), b = (int)(key & 0xffffffffu);
             const int ma = C.moved[a], mb = C.moved[b];
             bool a_first;

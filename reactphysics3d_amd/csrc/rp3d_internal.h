@@ -128,7 +128,16 @@ struct DevPairs {
     float4 *pn;                // [4*slot+k] normal xyz, depth
     float4 *pl1;               // [4*slot+k] local point on shape 1, w = accumulated normal impulse
     float4 *pl2;               // [4*slot+k] local point on shape 2, w = resting flag (0/1)
+    int *evf;                  // EV_CUR / EV_PREV: OverlappingPair::collidingInCurrentFrame / InPreviousFrame
+    int *kept;                 // set by the merge on every entry of the previous table that survives
 };
+
+constexpr int EV_CUR = 1;
+constexpr int EV_PREV = 2;
+// contact events (CollisionCallback::ContactPair::EventType, CollisionCallback.h)
+constexpr int EV_START = 0;
+constexpr int EV_STAY = 1;
+constexpr int EV_EXIT = 2;
 
 // Solver rows of one contact manifold (ContactManifoldSolver + ContactPointSolver), colour order.
 struct DevSolver {
@@ -180,6 +189,7 @@ struct DevCounters {
     int nb_box_sep_tests;      // box-box pairs that did not (cheap temporal-coherence rejects, listed apart)
     int nb_hull_tests;         // the rest of nb_tests[ALGO_POLY_POLY] (a convex mesh involved)
     int nb_walk_again;         // colliders with more partners than the traversal slab holds
+    int nb_events;             // contact events of the step (may exceed the event buffer)
 };
 
 constexpr int STATUS_PAIR_OVERFLOW = 1;

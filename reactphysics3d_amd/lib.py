@@ -65,7 +65,7 @@ EXPORTED = [
     "rp3dg_body_set_lock_factors", "rp3dg_body_update_mass_properties", "rp3dg_body_set_mass_properties",
     "rp3dg_collider_add", "rp3dg_collider_set_material", "rp3dg_collider_set_filter", "rp3dg_hull_create",
     "rp3dg_joint_create", "rp3dg_joint_set_limits", "rp3dg_body_set_allowed_to_sleep", "rp3dg_batch_get_sleeping",
-    "rp3dg_world_get_body_sleeping",
+    "rp3dg_world_get_body_sleeping", "rp3dg_batch_set_contact_events", "rp3dg_batch_get_contact_events",
     "rp3dg_world_update", "rp3dg_batch_create", "rp3dg_batch_create_ex",
     "rp3dg_batch_destroy", "rp3dg_batch_step",
     "rp3dg_batch_step_graph", "rp3dg_batch_synchronize", "rp3dg_batch_get_counts", "rp3dg_batch_get_bodies",
@@ -123,6 +123,8 @@ def load():
         "rp3dg_body_set_allowed_to_sleep": (C.c_int, [vp, i32, i32]),
         "rp3dg_batch_get_sleeping": (C.c_int, [vp, iptr, C.c_int64]),
         "rp3dg_world_get_body_sleeping": (C.c_int, [vp, i32]),
+        "rp3dg_batch_set_contact_events": (C.c_int, [vp, i32]),
+        "rp3dg_batch_get_contact_events": (C.c_int64, [vp, iptr, iptr, iptr, iptr, C.c_int64]),
         "rp3dg_world_update": (C.c_int, [vp, f]),
         "rp3dg_batch_create": (vp, [C.POINTER(vp), i32]),
         "rp3dg_batch_create_ex": (vp, [C.POINTER(vp), i32, C.POINTER(BatchOptions)]),
@@ -295,6 +297,24 @@ class Batch:
         w = np.zeros((n, 3), np.float32)
         _check(load().rp3dg_batch_get_bodies(self.h, _fp(p), _fp(q), _fp(v), _fp(w)), "get_bodies")
         return p, q, v, w
+
+    def set_contact_events(self, enable=True):
+        """PhysicsWorld::setEventListener: record the contact events of every step from now on."""
+        _check(load().rp3dg_batch_set_contact_events(self.h, int(enable)), "set_contact_events")
+
+    def contact_events(self):
+        """Events of the last step as an int32 array of rows (collider1, collider2, type, contact points),
+        type 0 ContactStart, 1 ContactStay, 2 ContactExit; sorted by (collider1, collider2)."""
+        L = load()
+        cap = L.rp3dg_batch_get_contact_events(self.h, None, None, None, None, 0)
+        if cap < 0:
+            raise RuntimeError(f"get_contact_events: {last_error()}")
+        out = np.zeros((4, max(1, cap)), np.int32)
+        n = L.rp3dg_batch_get_contact_events(self.h, _ip(out[0]), _ip(out[1]), _ip(out[2]), _ip(out[3]), cap)
+        if n < 0:
+            raise RuntimeError(f"get_contact_events: {last_error()}")
+        ev = out[:, :n].T.copy()
+        return ev[np.lexsort((ev[:, 1], ev[:, 0]))] if n else ev
 
     def sleeping(self):
         """RigidBody::isSleeping of every body, int32 0 / 1."""

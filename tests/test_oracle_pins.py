@@ -206,3 +206,33 @@ def test_a_joint_keeps_its_island_awake(ref_available):
 
     assert run(True) == [0, 0]
     assert run(False) == [1, 0]
+
+
+def test_contact_events_follow_the_contacts(ref_available):
+    """A bouncy sphere dropped on the ground: the recorder reports ContactStart on the first step the
+    pair has a contact manifold, ContactStay while it keeps one, ContactExit on the first step without
+    one, and nothing while the ball is in the air."""
+    from oracle.ref import RefWorld
+    sc = S.Scene("events_pin", S.make_settings(gravity=(0.0, 0.0, -9.81), bounciness=1.0))
+    g = sc.add_body(S.STATIC, (0.0, 0.0, -0.5))
+    sc.add_collider(g, S.BOX, (5.0, 5.0, 0.5), bounciness=1.0)
+    b = sc.add_body(S.DYNAMIC, (0.0, 0.0, 1.0))
+    sc.add_collider(b, S.SPHERE, (0.25,), bounciness=1.0)
+    ref = RefWorld(sc)
+    ref.enable_events()
+    touching_before, kinds = False, []
+    for _ in range(240):
+        ref.step(DT)
+        touching = len(ref.contacts()["colliders"]) > 0
+        ev = ref.events()
+        if touching:
+            assert len(ev) == 1 and tuple(ev[0, :2]) in ((0, 1), (1, 0))
+            assert ev[0, 2] == (1 if touching_before else 0) and ev[0, 3] >= 1
+        elif touching_before:
+            assert len(ev) == 1 and ev[0, 2] == 2 and ev[0, 3] == 0
+        else:
+            assert len(ev) == 0
+        if len(ev):
+            kinds.append(int(ev[0, 2]))
+        touching_before = touching
+    assert kinds.count(0) >= 2 and kinds.count(2) >= 2   # it bounced at least twice
