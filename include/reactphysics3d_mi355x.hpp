@@ -12,7 +12,9 @@
 
 #include <cmath>
 #include <cstdint>
+#include <map>
 #include <memory>
+#include <utility>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -23,6 +25,7 @@ namespace reactphysics3d {
 
 using decimal = float;
 using uint32 = uint32_t;
+using uint64 = uint64_t;
 using uint16 = uint16_t;
 
 inline void check(int status, const char* what) {
@@ -311,6 +314,49 @@ class SliderJoint : public Joint {   // SliderJoint.cpp:98-235
     bool isMotorEnabled() const { return mMotor; }
 };
 
+// ---- contact events (collision/CollisionCallback.h, engine/EventListener.h) ----
+// The callback data is built after update() from the device's event list and contact manifolds; it
+// is valid during onContact only, as with the reference.
+class CollisionCallback {
+   public:
+    class ContactPoint {
+       public:
+        const Vector3& getWorldNormal() const { return mNormal; }
+        decimal getPenetrationDepth() const { return mDepth; }
+        const Vector3& getLocalPointOnCollider1() const { return mLocal1; }
+        const Vector3& getLocalPointOnCollider2() const { return mLocal2; }
+        Vector3 mNormal, mLocal1, mLocal2;
+        decimal mDepth = 0;
+    };
+    class ContactPair {
+       public:
+        enum class EventType { ContactStart, ContactStay, ContactExit };
+        Collider* getCollider1() const { return mCollider1; }
+        Collider* getCollider2() const { return mCollider2; }
+        RigidBody* getBody1() const { return mBody1; }
+        RigidBody* getBody2() const { return mBody2; }
+        uint32 getNbContactPoints() const { return (uint32)mPoints.size(); }
+        ContactPoint getContactPoint(uint32 i) const { return mPoints[i]; }
+        EventType getEventType() const { return mType; }
+        Collider *mCollider1 = nullptr, *mCollider2 = nullptr;
+        RigidBody *mBody1 = nullptr, *mBody2 = nullptr;
+        EventType mType = EventType::ContactStart;
+        std::vector<ContactPoint> mPoints;
+    };
+    class CallbackData {
+       public:
+        uint32 getNbContactPairs() const { return (uint32)mPairs.size(); }
+        ContactPair getContactPair(uint64 i) const { return mPairs[i]; }
+        std::vector<ContactPair> mPairs;
+    };
+    virtual ~CollisionCallback() = default;
+    virtual void onContact(const CallbackData& data) = 0;
+};
+class EventListener : public CollisionCallback {
+   public:
+    void onContact(const CollisionCallback::CallbackData&) override {}
+};
+
 class PhysicsWorld {
    public:
     struct WorldSettings {
@@ -388,11 +434,68 @@ class PhysicsWorld {
         const float v[3] = {g.x, g.y, g.z};
         check(rp3dg_world_set_gravity(mWorld, v), "setGravity");
     }
-    // PhysicsWorld::update (src/engine/PhysicsWorld.cpp:239-307), on the GPU
-    void update(decimal timeStep) { check(rp3dg_world_update(mWorld, timeStep), "update"); }
+    // PhysicsWorld::update (src/engine/PhysicsWorld.cpp:239-307), on the GPU; the contact events of the
+    // step go to the event listener (CollisionDetectionSystem::reportContactsAndTriggers)
+    void update(decimal timeStep) {
+        check(rp3dg_world_update(mWorld, timeStep), "update");
+        if (mListener) report();
+    }
+    // PhysicsWorld::setEventListener (src/engine/PhysicsWorld.h)
+    void setEventListener(EventListener* listener) {
+        mListener = listener;
+        check(rp3dg_world_set_contact_events(mWorld, listener ? 1 : 0), "setEventListener");
+    }
     rp3dg_world* handle() const { return mWorld; }
 
    private:
+    // events of the last step with their contact points (one manifold per convex pair), delivered
+    // when there is at least one (CollisionDetectionSystem.cpp:1733-1741)
+    void report() {
+        rp3dg_batch* b = rp3dg_world_get_batch(mWorld);
+        const int64_t n = rp3dg_batch_get_contact_events(b, nullptr, nullptr, nullptr, nullptr, 0);
+        check((int)n, "contact events");
+        if (n == 0) return;
+        std::vector<int32_t> c1(n), c2(n), type(n), np(n);
+        check((int)rp3dg_batch_get_contact_events(b, c1.data(), c2.data(), type.data(), np.data(), n), "contact events");
+        rp3dg_counts counts;
+        check(rp3dg_batch_get_counts(b, &counts), "counts");
+        const int64_t nm = counts.nb_manifolds;
+        std::vector<int32_t> mc(2 * nm + 2), mn(nm + 1), color(nm + 1);
+        std::vector<float> nd(16 * nm + 16), l1(12 * nm + 12), l2(12 * nm + 12), imp(4 * nm + 4), fr(3 * nm + 3);
+        rp3dg_contact_view v{mc.data(), mn.data(), color.data(), nd.data(), l1.data(), l2.data(), imp.data(), fr.data()};
+        check(rp3dg_batch_get_contacts(b, &v, nm), "contacts");
+        std::map<std::pair<int32_t, int32_t>, int64_t> manifold;
+        for (int64_t m = 0; m < nm; m++) manifold[{mc[2 * m], mc[2 * m + 1]}] = m;
+        std::map<int32_t, std::pair<Collider*, RigidBody*>> owner;
+        for (auto& body : mBodies)
+            for (auto& col : body->mColliders) owner[col->index()] = {col.get(), body.get()};
+        CollisionCallback::CallbackData data;
+        for (int64_t e = 0; e < n; e++) {
+            CollisionCallback::ContactPair p;
+            p.mCollider1 = owner[c1[e]].first;
+            p.mBody1 = owner[c1[e]].second;
+            p.mCollider2 = owner[c2[e]].first;
+            p.mBody2 = owner[c2[e]].second;
+            p.mType = type[e] == RP3DG_CONTACT_START ? CollisionCallback::ContactPair::EventType::ContactStart
+                      : type[e] == RP3DG_CONTACT_STAY ? CollisionCallback::ContactPair::EventType::ContactStay
+                                                      : CollisionCallback::ContactPair::EventType::ContactExit;
+            auto it = manifold.find({c1[e], c2[e]});
+            if (type[e] != RP3DG_CONTACT_EXIT && it != manifold.end()) {
+                const int64_t m = it->second;
+                for (int k = 0; k < mn[m]; k++) {
+                    CollisionCallback::ContactPoint cp;
+                    const float* d = &nd[16 * m + 4 * k];
+                    cp.mNormal = Vector3(d[0], d[1], d[2]);
+                    cp.mDepth = d[3];
+                    cp.mLocal1 = Vector3(l1[12 * m + 3 * k], l1[12 * m + 3 * k + 1], l1[12 * m + 3 * k + 2]);
+                    cp.mLocal2 = Vector3(l2[12 * m + 3 * k], l2[12 * m + 3 * k + 1], l2[12 * m + 3 * k + 2]);
+                    p.mPoints.push_back(cp);
+                }
+            }
+            data.mPairs.push_back(std::move(p));
+        }
+        mListener->onContact(data);
+    }
     Joint* addJoint(std::unique_ptr<Joint> joint, int32_t type, const JointInfo& i, const Vector3& axis, bool limit,
                     bool motor, const decimal* lim) {
         const float a[3] = {i.anchorPointWorldSpace.x, i.anchorPointWorldSpace.y, i.anchorPointWorldSpace.z};
@@ -415,6 +518,7 @@ class PhysicsWorld {
     uint16 mVel = 6, mPos = 3;
     std::vector<std::unique_ptr<RigidBody>> mBodies;
     std::vector<std::unique_ptr<Joint>> mJoints;
+    EventListener* mListener = nullptr;
 };
 
 // PhysicsCommon (engine/PhysicsCommon.h:176-257): owns worlds and shapes.

@@ -246,6 +246,11 @@ int rp3dg_batch_set_contact_events(rp3dg_batch* batch, int32_t enable);
  * events are off, RP3DG_EINVAL if capacity is short). */
 int64_t rp3dg_batch_get_contact_events(rp3dg_batch* batch, int32_t* collider1, int32_t* collider2, int32_t* type,
                                        int32_t* nb_points, int64_t capacity);
+/* PhysicsWorld::setEventListener for a world stepped with rp3dg_world_update: events are recorded from
+ * the next update on; read them (and the contact points) through rp3dg_world_get_batch. */
+int rp3dg_world_set_contact_events(rp3dg_world* world, int32_t enable);
+/* The batch of one that rp3dg_world_update steps (NULL before the first update). */
+rp3dg_batch* rp3dg_world_get_batch(rp3dg_world* world);
 /* RigidBody::isSleeping of every body of the batch (batch order), 0 / 1; capacity >= body count. */
 int rp3dg_batch_get_sleeping(rp3dg_batch* batch, int32_t* sleeping, int64_t capacity);
 /* Joint solve order (joint indices in the order the device solved them this step, per type block). */

@@ -1304,6 +1304,10 @@ int rp3dg_world_update(rp3dg_world* w, float dt) {
         rp3dg_world* ws[1] = {w};
         w->own = rp3dg_batch_create(ws, 1);
         if (!w->own) return RP3DG_EHIP;
+        if (w->events) {
+            const int r = rp3dg_batch_set_contact_events(w->own, 1);
+            if (r) return r;
+        }
     }
     const int r = rp3dg_batch_step(w->own, dt, 1);
     w->mirror_dirty = true;
@@ -1340,6 +1344,14 @@ int rp3dg_world_get_body_state(rp3dg_world* w, int32_t body, float* pos, float* 
         if (ori) ori[k] = w->mirror_q[4 * body + k];
     return RP3DG_OK;
 }
+
+int rp3dg_world_set_contact_events(rp3dg_world* w, int32_t enable) {
+    if (!w) return RP3DG_EINVAL;
+    w->events = enable != 0;
+    return w->own ? rp3dg_batch_set_contact_events(w->own, enable) : RP3DG_OK;
+}
+
+rp3dg_batch* rp3dg_world_get_batch(rp3dg_world* w) { return w ? w->own : nullptr; }
 
 int rp3dg_world_get_body_sleeping(rp3dg_world* w, int32_t body) {
     if (!w || body < 0 || body >= (int32_t)w->bodies.size()) { set_error("invalid body index"); return RP3DG_EINVAL; }

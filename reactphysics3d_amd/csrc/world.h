@@ -73,6 +73,7 @@ struct HostWorld {
     std::vector<HostHull> hulls;
     std::vector<HostJoint> joints;
     rp3dg_batch* own = nullptr;   // device batch of one, created on first update()
+    bool events = false;          // record contact events in the own batch
     bool uploaded = false;
     // host mirror of the device body state, refreshed lazily after update()
     std::vector<float> mirror_pos, mirror_q, mirror_v, mirror_w;

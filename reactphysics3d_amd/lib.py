@@ -66,6 +66,7 @@ EXPORTED = [
     "rp3dg_collider_add", "rp3dg_collider_set_material", "rp3dg_collider_set_filter", "rp3dg_hull_create",
     "rp3dg_joint_create", "rp3dg_joint_set_limits", "rp3dg_body_set_allowed_to_sleep", "rp3dg_batch_get_sleeping",
     "rp3dg_world_get_body_sleeping", "rp3dg_batch_set_contact_events", "rp3dg_batch_get_contact_events",
+    "rp3dg_world_set_contact_events", "rp3dg_world_get_batch",
     "rp3dg_world_update", "rp3dg_batch_create", "rp3dg_batch_create_ex",
     "rp3dg_batch_destroy", "rp3dg_batch_step",
     "rp3dg_batch_step_graph", "rp3dg_batch_synchronize", "rp3dg_batch_get_counts", "rp3dg_batch_get_bodies",
@@ -124,6 +125,8 @@ def load():
         "rp3dg_batch_get_sleeping": (C.c_int, [vp, iptr, C.c_int64]),
         "rp3dg_world_get_body_sleeping": (C.c_int, [vp, i32]),
         "rp3dg_batch_set_contact_events": (C.c_int, [vp, i32]),
+        "rp3dg_world_set_contact_events": (C.c_int, [vp, i32]),
+        "rp3dg_world_get_batch": (vp, [vp]),
         "rp3dg_batch_get_contact_events": (C.c_int64, [vp, iptr, iptr, iptr, iptr, C.c_int64]),
         "rp3dg_world_update": (C.c_int, [vp, f]),
         "rp3dg_batch_create": (vp, [C.POINTER(vp), i32]),

@@ -106,3 +106,41 @@ def test_joints_example_steps_like_the_reference(ref_available):
     err = np.max(np.abs(st - want))
     print("max |device - reference|", err)
     assert err <= 1e-4
+
+
+CONTACTS_EXE = os.path.join(ROOT, "reactphysics3d_amd", "_lib", "contacts")
+
+
+def _contacts_scene():
+    """The world of examples/contacts.cpp (WorldSettings defaults, gravity -z).  The boxes never touch
+    each other, so no two manifolds share a dynamic body and the reference's own solve order gives
+    the device's results."""
+    sc = S.Scene("contacts_example", S.make_settings(gravity=(0.0, 0.0, -9.81), sleeping=1))
+    g = sc.add_body(S.STATIC, (0.0, 0.0, -0.5))
+    sc.add_collider(g, S.BOX, (10.0, 10.0, 0.5))
+    for p in ((0.0, 0.0, 1.0), (3.0, 0.0, 2.5), (-3.0, 0.5, 4.0)):
+        b = sc.add_body(S.DYNAMIC, p)
+        sc.add_collider(b, S.BOX, (0.5, 0.5, 0.5))
+    return sc
+
+
+@pytest.mark.gpu
+def test_event_listener_sees_the_reference_events(ref_available):
+    """An EventListener passed to PhysicsWorld::setEventListener of the C++ mirror receives, every
+    step, the reference's counts of ContactStart / ContactStay / ContactExit pairs and contact points."""
+    from oracle.ref import RefWorld
+    if not os.path.exists(CONTACTS_EXE):
+        pytest.skip("example not built (python -m reactphysics3d_amd.build)")
+    out = subprocess.run([CONTACTS_EXE, "150"], capture_output=True, text=True, timeout=120)
+    assert out.returncode == 0, out.stderr
+    rows = np.array([list(map(int, line.split()[2:])) for line in out.stdout.splitlines() if line.startswith("step")])
+    ref = RefWorld(_contacts_scene())
+    ref.enable_events()
+    want = []
+    for _ in range(150):
+        ref.step(1.0 / 60.0)
+        ev = ref.events()
+        want.append([int((ev[:, 2] == t).sum()) for t in range(3)] + [int(ev[:, 3].sum())])
+    want = np.array(want)
+    assert np.array_equal(rows, want)
+    assert want[:, 0].sum() >= 4 and want[:, 1].sum() > 100 and want[:, 2].sum() >= 1
