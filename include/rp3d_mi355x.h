@@ -251,6 +251,15 @@ int64_t rp3dg_batch_get_contact_events(rp3dg_batch* batch, int32_t* collider1, i
 int rp3dg_world_set_contact_events(rp3dg_world* world, int32_t enable);
 /* The batch of one that rp3dg_world_update steps (NULL before the first update). */
 rp3dg_batch* rp3dg_world_get_batch(rp3dg_world* world);
+/* PhysicsWorld::raycast (PhysicsWorld.h:486-490, CollisionDetectionSystem.cpp:1186-1195)
+ * with a RaycastCallback that keeps the closest hit (notifyRaycastHit returns the hit fraction), for
+ * n rays at once: ray i runs from point1 to point2 (xyz each) in world world[i] (NULL: world 0) with
+ * max fraction max_fraction[i] (NULL: 1) against the colliders whose category bits meet
+ * category_mask, in the state after the last step.  Out: collider[i] (batch index, -1 for no hit),
+ * hit point and unit normal in world space (xyz each) and hit fraction; NULL outputs are skipped. */
+int rp3dg_batch_raycast(rp3dg_batch* batch, int64_t n, const int32_t* world, const float* point1, const float* point2,
+                        const float* max_fraction, uint16_t category_mask, int32_t* collider, float* point,
+                        float* normal, float* fraction);
 /* RigidBody::isSleeping of every body of the batch (batch order), 0 / 1; capacity >= body count. */
 int rp3dg_batch_get_sleeping(rp3dg_batch* batch, int32_t* sleeping, int64_t capacity);
 /* Joint solve order (joint indices in the order the device solved them this step, per type block). */

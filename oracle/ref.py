@@ -137,6 +137,23 @@ class RefWorld:
         ev = out[:n]
         return ev[np.lexsort((ev[:, 1], ev[:, 0]))] if n else ev
 
+    def raycast(self, p1, p2, max_fraction=None, mask=0xFFFF):
+        """PhysicsWorld::raycast with a closest-hit callback for each ray (p1, p2: [n, 3]).  Returns
+        (collider [n], point [n, 3], normal [n, 3], fraction [n]); collider -1 for no hit."""
+        p1 = np.ascontiguousarray(p1, np.float32).reshape(-1, 3)
+        p2 = np.ascontiguousarray(p2, np.float32).reshape(-1, 3)
+        n = len(p1)
+        mf = None if max_fraction is None else np.ascontiguousarray(max_fraction, np.float32)
+        col = np.zeros(n, np.int32)
+        pt = np.zeros((n, 3), np.float32)
+        nr = np.zeros((n, 3), np.float32)
+        fr = np.zeros(n, np.float32)
+        fp = C.POINTER(C.c_float)
+        self.L.rp3dref_raycast.argtypes = [C.c_void_p, C.c_int32, fp, fp, fp, C.c_int32, C.POINTER(C.c_int32), fp, fp, fp]
+        self.L.rp3dref_raycast(self.h, n, _fp(p1), _fp(p2), _fp(mf) if mf is not None else None, int(mask), _ip(col),
+                               _fp(pt), _fp(nr), _fp(fr))
+        return col, pt, nr, fr
+
     def sleeping(self):
         """RigidBody::isSleeping of every body, as int32 0 / 1."""
         out = np.zeros(self.nb, np.int32)

@@ -405,6 +405,40 @@ int rp3dref_set_joint_order(void* h, const int32_t* order, int n) {
     return apply_joint_order(R);
 }
 
+// PhysicsWorld::raycast of n rays with a callback that keeps the closest hit (returns the hit fraction,
+// which clips the ray; the last reported hit is the closest).  collider[i] = -1 when nothing is hit.
+struct ClosestHit : public RaycastCallback {
+    RefWorld* R = nullptr;
+    int collider = -1;
+    RaycastInfo info;
+    decimal notifyRaycastHit(const RaycastInfo& i) override {
+        info.worldPoint = i.worldPoint;
+        info.worldNormal = i.worldNormal;
+        info.hitFraction = i.hitFraction;
+        collider = collider_index(R, i.collider->getEntity());
+        return i.hitFraction;
+    }
+};
+
+int rp3dref_raycast(void* h, int n, const float* p1, const float* p2, const float* maxf, int mask, int32_t* collider,
+                    float* point, float* normal, float* fraction) {
+    RefWorld* R = (RefWorld*)h;
+    for (int i = 0; i < n; i++) {
+        const Ray ray(Vector3(p1[3 * i], p1[3 * i + 1], p1[3 * i + 2]), Vector3(p2[3 * i], p2[3 * i + 1], p2[3 * i + 2]),
+                      maxf ? maxf[i] : decimal(1.0));
+        ClosestHit cb;
+        cb.R = R;
+        R->world->raycast(ray, &cb, (unsigned short)mask);
+        collider[i] = cb.collider;
+        for (int k = 0; k < 3; k++) {
+            point[3 * i + k] = cb.collider >= 0 ? (float)cb.info.worldPoint[k] : 0.f;
+            normal[3 * i + k] = cb.collider >= 0 ? (float)cb.info.worldNormal[k] : 0.f;
+        }
+        fraction[i] = cb.collider >= 0 ? (float)cb.info.hitFraction : 0.f;
+    }
+    return 0;
+}
+
 // RigidBody::isSleeping of every body.
 int rp3dref_get_sleeping(void* h, int32_t* out) {
     RefWorld* R = (RefWorld*)h;

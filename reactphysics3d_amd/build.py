@@ -13,7 +13,7 @@ LIB_DIR = os.path.join(HERE, "_lib")
 LIB = os.path.join(LIB_DIR, "librp3d_mi355x.so")
 
 SOURCES = ["world.cpp", "broadphase.hip", "narrowphase.hip", "solver.hip", "joints.hip", "fused.hip", "mlp.hip",
-           "islands.hip", "events.hip",
+           "islands.hip", "events.hip", "raycast.hip",
            "batch.hip"]
 
 # -ffp-contract=off + IEEE div/sqrt keep the FP32 operation sequence of the x86-64 reference build;

@@ -33,6 +33,8 @@ struct rp3dg_batch {
     rp3dg::DevSolver S{};
     rp3dg::DevJoints J{};
     rp3dg::DevIslands isl{};           // allocated when the worlds have sleeping enabled
+    void* ray_buf = nullptr;           // raycast staging (grown on demand, freed with the batch)
+    size_t ray_cap = 0;
     int4* ev_buf = nullptr;            // contact events of the last step (allocated when first enabled)
     int ev_cap = 0;
     bool events_on = false;

@@ -13,6 +13,7 @@
 
 #include "../../include/rp3d_mi355x.h"
 #include "batch.h"
+#include "raycast.h"
 #include "fused.h"
 #include "mlp.h"
 #include "kcommon.h"
@@ -1256,6 +1257,7 @@ rp3dg_batch::~rp3dg_batch() {
     for (hipEvent_t e : fork_events) hipEventDestroy(e);
     if (log_feats) hipFree(log_feats);
     if (log_targets) hipFree(log_targets);
+    if (ray_buf) hipFree(ray_buf);
     for (hipEvent_t ev : event_pool) hipEventDestroy(ev);
     for (void* p : allocs) hipFree(p);
     if (hcounters) hipHostFree(hcounters);
@@ -1419,6 +1421,57 @@ int64_t rp3dg_batch_get_contact_events(rp3dg_batch* b, int32_t* c1, int32_t* c2,
         if (nb_points) nb_points[i] = e[i].w;
     }
     return n;
+}
+
+int rp3dg_batch_raycast(rp3dg_batch* b, int64_t n, const int32_t* world, const float* point1, const float* point2,
+                        const float* max_fraction, uint16_t category_mask, int32_t* collider, float* point,
+                        float* normal, float* fraction) {
+    if (!b || n < 0 || !point1 || !point2) return RP3DG_EINVAL;
+    if (n == 0) return RP3DG_OK;
+    if (n > (1 << 26)) { set_error("at most 2^26 rays per call"); return RP3DG_EINVAL; }
+    std::vector<int> wv((size_t)n);
+    std::vector<float4> a((size_t)n), c((size_t)n);
+    for (int64_t i = 0; i < n; i++) {
+        const int w = world ? world[i] : 0;
+        if (w < 0 || w >= b->nworlds) { set_error("raycast: invalid world index"); return RP3DG_EINVAL; }
+        const float mf = max_fraction ? max_fraction[i] : 1.0f;
+        wv[i] = w;
+        a[i] = make_float4(point1[3 * i], point1[3 * i + 1], point1[3 * i + 2], 0.f);
+        c[i] = make_float4(point2[3 * i], point2[3 * i + 1], point2[3 * i + 2], mf);
+    }
+    // staging: p1, p2, hit point, hit normal (float4 each), world, hit collider (int each)
+    const size_t need = (size_t)n * (4 * sizeof(float4) + 2 * sizeof(int));
+    if (need > b->ray_cap) {
+        if (b->ray_buf) HIPCHECK(hipFree(b->ray_buf));
+        b->ray_buf = nullptr;
+        b->ray_cap = 0;
+        HIPCHECK(hipMalloc(&b->ray_buf, need));
+        b->ray_cap = need;
+    }
+    float4* dp1 = (float4*)b->ray_buf;
+    float4* dp2 = dp1 + n;
+    float4* dhp = dp2 + n;
+    float4* dhn = dhp + n;
+    int* dw = (int*)(dhn + n);
+    int* dhc = dw + n;
+    HIPCHECK(hipMemcpyAsync(dp1, a.data(), sizeof(float4) * n, hipMemcpyHostToDevice, b->stream));
+    HIPCHECK(hipMemcpyAsync(dp2, c.data(), sizeof(float4) * n, hipMemcpyHostToDevice, b->stream));
+    HIPCHECK(hipMemcpyAsync(dw, wv.data(), sizeof(int) * n, hipMemcpyHostToDevice, b->stream));
+    RayArgs A{b->C, b->d_hulls, b->d_hverts, b->d_hnormals, b->d_hfverts, b->d_hfvidx, b->d_wcoll};
+    rc_closest<<<grid_for((int)n), 256, 0, b->stream>>>(A, dw, dp1, dp2, (int)n, category_mask, dhc, dhp, dhn);
+    HIPCHECK(hipGetLastError());
+    std::vector<int> hc((size_t)n);
+    HIPCHECK(hipMemcpyAsync(hc.data(), dhc, sizeof(int) * n, hipMemcpyDeviceToHost, b->stream));
+    HIPCHECK(hipMemcpyAsync(a.data(), dhp, sizeof(float4) * n, hipMemcpyDeviceToHost, b->stream));
+    HIPCHECK(hipMemcpyAsync(c.data(), dhn, sizeof(float4) * n, hipMemcpyDeviceToHost, b->stream));
+    HIPCHECK(hipStreamSynchronize(b->stream));
+    for (int64_t i = 0; i < n; i++) {
+        if (collider) collider[i] = hc[i];
+        if (point) { point[3 * i] = a[i].x; point[3 * i + 1] = a[i].y; point[3 * i + 2] = a[i].z; }
+        if (normal) { normal[3 * i] = c[i].x; normal[3 * i + 1] = c[i].y; normal[3 * i + 2] = c[i].z; }
+        if (fraction) fraction[i] = a[i].w;
+    }
+    return RP3DG_OK;
 }
 
 int rp3dg_batch_get_sleeping(rp3dg_batch* b, int32_t* sleeping, int64_t capacity) {

@@ -66,7 +66,7 @@ EXPORTED = [
     "rp3dg_collider_add", "rp3dg_collider_set_material", "rp3dg_collider_set_filter", "rp3dg_hull_create",
     "rp3dg_joint_create", "rp3dg_joint_set_limits", "rp3dg_body_set_allowed_to_sleep", "rp3dg_batch_get_sleeping",
     "rp3dg_world_get_body_sleeping", "rp3dg_batch_set_contact_events", "rp3dg_batch_get_contact_events",
-    "rp3dg_world_set_contact_events", "rp3dg_world_get_batch",
+    "rp3dg_world_set_contact_events", "rp3dg_world_get_batch", "rp3dg_batch_raycast",
     "rp3dg_world_update", "rp3dg_batch_create", "rp3dg_batch_create_ex",
     "rp3dg_batch_destroy", "rp3dg_batch_step",
     "rp3dg_batch_step_graph", "rp3dg_batch_synchronize", "rp3dg_batch_get_counts", "rp3dg_batch_get_bodies",
@@ -127,6 +127,7 @@ def load():
         "rp3dg_batch_set_contact_events": (C.c_int, [vp, i32]),
         "rp3dg_world_set_contact_events": (C.c_int, [vp, i32]),
         "rp3dg_world_get_batch": (vp, [vp]),
+        "rp3dg_batch_raycast": (C.c_int, [vp, C.c_int64, iptr, fptr, fptr, fptr, C.c_uint16, iptr, fptr, fptr, fptr]),
         "rp3dg_batch_get_contact_events": (C.c_int64, [vp, iptr, iptr, iptr, iptr, C.c_int64]),
         "rp3dg_world_update": (C.c_int, [vp, f]),
         "rp3dg_batch_create": (vp, [C.POINTER(vp), i32]),
@@ -318,6 +319,24 @@ class Batch:
             raise RuntimeError(f"get_contact_events: {last_error()}")
         ev = out[:, :n].T.copy()
         return ev[np.lexsort((ev[:, 1], ev[:, 0]))] if n else ev
+
+    def raycast(self, p1, p2, world=None, max_fraction=None, mask=0xFFFF):
+        """PhysicsWorld::raycast with a closest-hit callback for n rays (p1, p2: [n, 3]; world: [n] world
+        index, default 0).  Returns (collider [n], point [n, 3], normal [n, 3], fraction [n]) with batch
+        collider indices, -1 for no hit."""
+        p1 = np.ascontiguousarray(p1, np.float32).reshape(-1, 3)
+        p2 = np.ascontiguousarray(p2, np.float32).reshape(-1, 3)
+        n = len(p1)
+        w = None if world is None else np.ascontiguousarray(world, np.int32)
+        mf = None if max_fraction is None else np.ascontiguousarray(max_fraction, np.float32)
+        col = np.zeros(n, np.int32)
+        pt = np.zeros((n, 3), np.float32)
+        nr = np.zeros((n, 3), np.float32)
+        fr = np.zeros(n, np.float32)
+        _check(load().rp3dg_batch_raycast(self.h, n, _ip(w) if w is not None else None, _fp(p1), _fp(p2),
+                                          _fp(mf) if mf is not None else None, int(mask), _ip(col), _fp(pt), _fp(nr),
+                                          _fp(fr)), "raycast")
+        return col, pt, nr, fr
 
     def sleeping(self):
         """RigidBody::isSleeping of every body, int32 0 / 1."""

@@ -1,0 +1,90 @@
+"""Batched raycasts (rp3dg_batch_raycast) against PhysicsWorld::raycast with a closest-hit callback:
+random rays through a settled box pile, a mixed sphere / box / capsule container, convex-mesh hulls,
+and rays spread over a batch of worlds, after the worlds have been stepped on both sides.  Hit
+collider, fraction, point and normal must agree bit for bit."""
+import numpy as np
+import pytest
+
+from reactphysics3d_amd import scene as S
+
+pytestmark = pytest.mark.gpu
+
+DT = 1.0 / 60.0
+
+
+def _rays(rng, n, lo, hi):
+    p1 = rng.uniform(lo, hi, (n, 3)).astype(np.float32)
+    p2 = rng.uniform(lo, hi, (n, 3)).astype(np.float32)
+    return p1, p2
+
+
+def _check(dev_out, ref_out, what):
+    dc, dp, dn, df = dev_out
+    rc, rp, rn, rf = ref_out
+    assert np.array_equal(dc, rc), f"{what}: hit colliders differ at {np.nonzero(dc != rc)[0][:10]}"
+    hit = rc >= 0
+    for name, a, b in (("fraction", df[hit], rf[hit]), ("point", dp[hit], rp[hit]), ("normal", dn[hit], rn[hit])):
+        assert np.array_equal(a, b), f"{what}: {name} differs by {np.max(np.abs(a - b))}"
+    return int(hit.sum())
+
+
+def _stepped(sc, steps):
+    from oracle.ref import RefWorld
+    from reactphysics3d_amd.lib import Batch
+    dev = Batch([sc])
+    ref = RefWorld(sc)
+    ref.set_joint_order(dev.joint_order())
+    for _ in range(steps):
+        dev.step(DT)
+        ct = dev.contacts()
+        ref.step_ordered(DT, ct["colliders"][:, 0], ct["colliders"][:, 1])
+    return dev, ref
+
+
+@pytest.mark.parametrize("name", ["pile", "mixed", "hulls"])
+def test_raycasts_match_reference(name):
+    sc, lo, hi = {"pile": (S.box_pile(4, 3, seed=11), (-6.0, -6.0, -1.0), (6.0, 6.0, 8.0)),
+                  "mixed": (S.mixed_container(400, extent=(16.0, 16.0, 10.0), seed=2), (-9.0, -9.0, -1.0), (9.0, 9.0, 11.0)),
+                  "hulls": (S.hull_scene(n_hulls=80, n_chains=2, chain_length=4, extent=(10.0, 10.0, 8.0), seed=5),
+                            (-6.0, -6.0, -1.0), (6.0, 6.0, 9.0))}[name]
+    dev, ref = _stepped(sc, 40)
+    rng = np.random.default_rng(7)
+    p1, p2 = _rays(rng, 4000, lo, hi)
+    mf = rng.uniform(0.2, 1.0, 4000).astype(np.float32)
+    hits = _check(dev.raycast(p1, p2, max_fraction=mf), ref.raycast(p1, p2, mf), name)
+    print(name, "hits", hits, "of 4000")
+    assert hits > 200
+    # category filter: only colliders whose category meets the mask are hit
+    hits_masked = _check(dev.raycast(p1, p2, max_fraction=mf, mask=0x0002), ref.raycast(p1, p2, mf, mask=0x0002), name)
+    assert hits_masked == 0
+
+
+def test_raycasts_over_a_batch_of_worlds():
+    from oracle.ref import RefWorld
+    from reactphysics3d_amd.lib import Batch
+    scenes = [S.batched_world(w, seed=100, min_bodies=8, max_bodies=48, max_joints=8) for w in range(6)]
+    dev = Batch(scenes, solver="fused")
+    refs = [RefWorld(sc) for sc in scenes]
+    order = dev.joint_order()
+    for w, ref in enumerate(refs):
+        j0, j1 = dev.joint_base[w], dev.joint_base[w + 1]
+        ref.set_joint_order(order[(order >= j0) & (order < j1)] - j0)
+    for _ in range(30):
+        dev.step(DT)
+        ct = dev.contacts()
+        for w, ref in enumerate(refs):
+            c0, c1 = dev.collider_base[w], dev.collider_base[w + 1]
+            sel = (ct["colliders"][:, 0] >= c0) & (ct["colliders"][:, 0] < c1)
+            ref.step_ordered(DT, ct["colliders"][sel, 0] - c0, ct["colliders"][sel, 1] - c0)
+    rng = np.random.default_rng(3)
+    n = 3000
+    world = rng.integers(0, len(scenes), n).astype(np.int32)
+    p1, p2 = _rays(rng, n, (-8.0, -8.0, -1.0), (8.0, 8.0, 12.0))
+    dc, dp, dn, df = dev.raycast(p1, p2, world=world)
+    total = 0
+    for w, ref in enumerate(refs):
+        sel = world == w
+        mine = (np.where(dc[sel] >= 0, dc[sel] - dev.collider_base[w], -1), dp[sel], dn[sel], df[sel])
+        total += _check(mine, ref.raycast(p1[sel], p2[sel]), f"world {w}")
+    print("hits", total, "of", n)
+    assert total > 300

@@ -236,3 +236,49 @@ def test_contact_events_follow_the_contacts(ref_available):
             kinds.append(int(ev[0, 2]))
         touching_before = touching
     assert kinds.count(0) >= 2 and kinds.count(2) >= 2   # it bounced at least twice
+
+
+def _cube_hull():
+    """A 2 m cube as a convex mesh with six quad faces, counter-clockwise seen from outside."""
+    verts = np.array([[2 * x - 1, 2 * y - 1, 2 * z - 1] for x in (0, 1) for y in (0, 1) for z in (0, 1)], np.float32)
+    faces = [[0, 1, 3, 2], [4, 6, 7, 5], [0, 4, 5, 1], [2, 3, 7, 6], [0, 2, 6, 4], [1, 5, 7, 3]]
+    for f in faces:
+        a, b, c = verts[f[0]], verts[f[1]], verts[f[2]]
+        if np.dot(np.cross(b - a, c - a), verts[f].mean(0)) < 0:
+            f.reverse()
+    return verts, faces
+
+
+def test_raycast_closed_forms(ref_available):
+    """PhysicsWorld::raycast with a closest-hit callback on shapes whose hits are known in closed form:
+    a unit sphere, a box's top face, a capsule's side and end cap, a cube-shaped convex mesh; the max
+    fraction clips, the closest of two shapes wins, and the category mask filters."""
+    from oracle.ref import RefWorld
+    sc = S.Scene("rays", S.make_settings(gravity=(0.0, 0.0, 0.0)))
+    sph = sc.add_body(S.STATIC, (0.0, 0.0, 0.0))
+    sc.add_collider(sph, S.SPHERE, (1.0,))
+    box = sc.add_body(S.STATIC, (10.0, 0.0, 0.0))
+    sc.add_collider(box, S.BOX, (1.0, 2.0, 3.0))
+    cap = sc.add_body(S.STATIC, (20.0, 0.0, 0.0))
+    sc.add_collider(cap, S.CAPSULE, (0.5, 2.0))
+    masked = sc.add_body(S.STATIC, (30.0, 0.0, 0.0))
+    sc.add_collider(masked, S.SPHERE, (1.0,), category=2)
+    h = sc.add_hull(*_cube_hull())
+    hb = sc.add_body(S.STATIC, (40.0, 0.0, 0.0))
+    sc.add_collider(hb, S.HULL, (float(h),))
+    ref = RefWorld(sc)
+    p1 = np.array([[-5, 0, 0], [10, 0, 10], [25, 0, 0], [20, 5, 0], [-5, 0, 0], [-5, 0, 0], [30, 0, 5], [40, 0, 5]],
+                  np.float32)
+    p2 = np.array([[5, 0, 0], [10, 0, -10], [15, 0, 0], [20, -5, 0], [5, 0, 0], [13, 0, 0], [30, 0, -5], [40, 0, -5]],
+                  np.float32)
+    mf = np.array([1, 1, 1, 1, 0.3, 1, 1, 1], np.float32)
+    col, pt, nr, fr = ref.raycast(p1, p2, mf)
+    assert list(col) == [0, 1, 2, 2, -1, 0, 3, 4]
+    assert fr[0] == pytest.approx(0.4) and np.allclose(pt[0], [-1, 0, 0]) and np.allclose(nr[0], [-1, 0, 0])
+    assert fr[1] == pytest.approx(0.35) and np.allclose(pt[1], [10, 0, 3]) and np.allclose(nr[1], [0, 0, 1])
+    assert fr[2] == pytest.approx(0.45) and np.allclose(nr[2], [1, 0, 0], atol=1e-6)
+    assert fr[3] == pytest.approx(0.35) and np.allclose(pt[3], [20, 1.5, 0]) and np.allclose(nr[3], [0, 1, 0], atol=1e-6)
+    assert fr[5] == pytest.approx(4.0 / 18.0)
+    assert fr[7] == pytest.approx(0.4) and np.allclose(pt[7], [40, 0, 1]) and np.allclose(nr[7], [0, 0, 1])
+    col, *_ = ref.raycast(p1[6:7], p2[6:7], mask=1)   # category 2 is not in the mask
+    assert col[0] == -1
