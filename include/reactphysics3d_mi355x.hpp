@@ -357,6 +357,24 @@ class EventListener : public CollisionCallback {
     void onContact(const CollisionCallback::CallbackData&) override {}
 };
 
+// ---- raycasts (mathematics/Ray.h, collision/RaycastInfo.h) ----
+struct Ray {
+    Ray(const Vector3& p1, const Vector3& p2, decimal maxFrac = decimal(1.0)) : point1(p1), point2(p2), maxFraction(maxFrac) {}
+    Vector3 point1, point2;
+    decimal maxFraction;
+};
+struct RaycastInfo {
+    Vector3 worldPoint, worldNormal;
+    decimal hitFraction = 0;
+    RigidBody* body = nullptr;
+    Collider* collider = nullptr;
+};
+class RaycastCallback {
+   public:
+    virtual ~RaycastCallback() = default;
+    virtual decimal notifyRaycastHit(const RaycastInfo& info) = 0;
+};
+
 class PhysicsWorld {
    public:
     struct WorldSettings {
@@ -439,6 +457,27 @@ class PhysicsWorld {
     void update(decimal timeStep) {
         check(rp3dg_world_update(mWorld, timeStep), "update");
         if (mListener) report();
+    }
+    // PhysicsWorld::raycast (PhysicsWorld.h:486-490) for the closest hit: the device finds the hit a
+    // callback returning info.hitFraction would keep, and the callback is called once with it (a
+    // callback that gathers every hit along the ray is not served).  Valid after the first update().
+    void raycast(const Ray& ray, RaycastCallback* callback, unsigned short mask = 0xFFFF) const {
+        rp3dg_batch* b = rp3dg_world_get_batch(mWorld);
+        if (!b) throw std::runtime_error("raycast: step the world once first");
+        const float p1[3] = {ray.point1.x, ray.point1.y, ray.point1.z}, p2[3] = {ray.point2.x, ray.point2.y, ray.point2.z};
+        const float mf = ray.maxFraction;
+        int32_t c = -1;
+        float pt[3], nr[3], fr = 0.f;
+        check(rp3dg_batch_raycast(b, 1, nullptr, p1, p2, &mf, mask, &c, pt, nr, &fr), "raycast");
+        if (c < 0) return;
+        RaycastInfo info;
+        info.worldPoint = Vector3(pt[0], pt[1], pt[2]);
+        info.worldNormal = Vector3(nr[0], nr[1], nr[2]);
+        info.hitFraction = fr;
+        for (const auto& body : mBodies)
+            for (const auto& col : body->mColliders)
+                if (col->index() == c) { info.collider = col.get(); info.body = body.get(); }
+        callback->notifyRaycastHit(info);
     }
     // PhysicsWorld::setEventListener (src/engine/PhysicsWorld.h)
     void setEventListener(EventListener* listener) {

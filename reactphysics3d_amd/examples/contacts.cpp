@@ -42,6 +42,14 @@ int main(int argc, char** argv) {
             b->addCollider(common.createBoxShape(Vector3(decimal(0.5), decimal(0.5), decimal(0.5))), Transform::identity());
             b->updateMassPropertiesFromColliders();
         }
+        // PhysicsWorld::raycast with a closest-hit callback, before and after the boxes fell
+        struct Closest : public RaycastCallback {
+            decimal fraction = -1;
+            decimal notifyRaycastHit(const RaycastInfo& info) override {
+                fraction = info.hitFraction;
+                return info.hitFraction;
+            }
+        };
         Listener listener;
         world->setEventListener(&listener);
         const decimal dt = decimal(1.0) / decimal(60.0);
@@ -50,6 +58,9 @@ int main(int argc, char** argv) {
             world->update(dt);
             std::printf("step %d %d %d %d %d\n", i, listener.start, listener.stay, listener.exit_, listener.points);
         }
+        Closest hit;
+        world->raycast(Ray(Vector3(0, 0, 10), Vector3(0, 0, -10)), &hit);
+        std::printf("ray %.9g\n", hit.fraction);
     } catch (const std::exception& e) {
         std::fprintf(stderr, "error: %s\n", e.what());
         return 1;

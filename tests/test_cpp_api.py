@@ -143,4 +143,8 @@ def test_event_listener_sees_the_reference_events(ref_available):
         want.append([int((ev[:, 2] == t).sum()) for t in range(3)] + [int(ev[:, 3].sum())])
     want = np.array(want)
     assert np.array_equal(rows, want)
+    # PhysicsWorld::raycast through the mirror: the closest hit straight down onto the first box
+    ray = [float(line.split()[1]) for line in out.stdout.splitlines() if line.startswith("ray")]
+    col, pt, nr, fr = ref.raycast(np.array([[0, 0, 10]], np.float32), np.array([[0, 0, -10]], np.float32))
+    assert col[0] == 1 and len(ray) == 1 and np.float32(ray[0]) == fr[0]
     assert want[:, 0].sum() >= 4 and want[:, 1].sum() > 100 and want[:, 2].sum() >= 1
