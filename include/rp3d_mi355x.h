@@ -251,6 +251,10 @@ int64_t rp3dg_batch_get_contact_events(rp3dg_batch* batch, int32_t* collider1, i
 int rp3dg_world_set_contact_events(rp3dg_world* world, int32_t enable);
 /* The batch of one that rp3dg_world_update steps (NULL before the first update). */
 rp3dg_batch* rp3dg_world_get_batch(rp3dg_world* world);
+/* RigidBody::applyWorldForceAtCenterOfMass / applyWorldTorque (src/body/RigidBody.cpp:298-311, 938-951)
+ * on every dynamic body of the batch whose force / torque (xyz per body, NULL = none) is not zero,
+ * before the next step (the step clears the accumulated forces); a sleeping body is woken. */
+int rp3dg_batch_apply_forces(rp3dg_batch* batch, const float* force, const float* torque);
 /* PhysicsWorld::raycast (PhysicsWorld.h:486-490, CollisionDetectionSystem.cpp:1186-1195)
  * with a RaycastCallback that keeps the closest hit (notifyRaycastHit returns the hit fraction), for
  * n rays at once: ray i runs from point1 to point2 (xyz each) in world world[i] (NULL: world 0) with

@@ -154,6 +154,14 @@ class RefWorld:
                                _fp(pt), _fp(nr), _fp(fr))
         return col, pt, nr, fr
 
+    def apply_forces(self, force=None, torque=None):
+        """RigidBody::applyWorldForceAtCenterOfMass / applyWorldTorque on every body ([nb, 3] each)."""
+        f = None if force is None else np.ascontiguousarray(force, np.float32)
+        t = None if torque is None else np.ascontiguousarray(torque, np.float32)
+        fp = C.POINTER(C.c_float)
+        self.L.rp3dref_apply_forces.argtypes = [C.c_void_p, fp, fp]
+        self.L.rp3dref_apply_forces(self.h, _fp(f) if f is not None else None, _fp(t) if t is not None else None)
+
     def sleeping(self):
         """RigidBody::isSleeping of every body, as int32 0 / 1."""
         out = np.zeros(self.nb, np.int32)

@@ -44,6 +44,7 @@ struct RefBody {
     float ang[3];
     int32_t gravity;
     float ldamp, adamp;
+    float llock[3], alock[3];
 };
 
 struct RefCollider {
@@ -179,6 +180,10 @@ void* rp3dref_create(const RefSettings* s, int nb, const RefBody* bodies, int nc
         if (!B.gravity) body->enableGravity(false);
         if (B.ldamp != 0.f) body->setLinearDamping(B.ldamp);
         if (B.adamp != 0.f) body->setAngularDamping(B.adamp);
+        if (B.llock[0] != 1.f || B.llock[1] != 1.f || B.llock[2] != 1.f)
+            body->setLinearLockAxisFactor(Vector3(B.llock[0], B.llock[1], B.llock[2]));
+        if (B.alock[0] != 1.f || B.alock[1] != 1.f || B.alock[2] != 1.f)
+            body->setAngularLockAxisFactor(Vector3(B.alock[0], B.alock[1], B.alock[2]));
     }
     for (int j = 0; j < nj; j++) {
         const RefJoint& J = joints[j];
@@ -435,6 +440,18 @@ int rp3dref_raycast(void* h, int n, const float* p1, const float* p2, const floa
             normal[3 * i + k] = cb.collider >= 0 ? (float)cb.info.worldNormal[k] : 0.f;
         }
         fraction[i] = cb.collider >= 0 ? (float)cb.info.hitFraction : 0.f;
+    }
+    return 0;
+}
+
+// RigidBody::applyWorldForceAtCenterOfMass / applyWorldTorque on every body whose force / torque (xyz
+// each, NULL = none) is not zero.
+int rp3dref_apply_forces(void* h, const float* f, const float* t) {
+    RefWorld* R = (RefWorld*)h;
+    auto nonzero = [](const float* a, size_t i) { return a && (a[3 * i] != 0.f || a[3 * i + 1] != 0.f || a[3 * i + 2] != 0.f); };
+    for (size_t i = 0; i < R->bodies.size(); i++) {
+        if (nonzero(f, i)) R->bodies[i]->applyWorldForceAtCenterOfMass(Vector3(f[3 * i], f[3 * i + 1], f[3 * i + 2]));
+        if (nonzero(t, i)) R->bodies[i]->applyWorldTorque(Vector3(t[3 * i], t[3 * i + 1], t[3 * i + 2]));
     }
     return 0;
 }

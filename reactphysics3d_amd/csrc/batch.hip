@@ -1423,6 +1423,40 @@ int64_t rp3dg_batch_get_contact_events(rp3dg_batch* b, int32_t* c1, int32_t* c2,
     return n;
 }
 
+int rp3dg_batch_apply_forces(rp3dg_batch* b, const float* force, const float* torque) {
+    if (!b) return RP3DG_EINVAL;
+    if (!force && !torque) return RP3DG_OK;
+    const int n = b->nb;
+    std::vector<float4> f(n), t(n);
+    std::vector<int> fl(n);
+    std::vector<float> st;
+    HIPCHECK(hipStreamSynchronize(b->stream));
+    HIPCHECK(hipMemcpy(f.data(), b->B.force, sizeof(float4) * n, hipMemcpyDeviceToHost));
+    HIPCHECK(hipMemcpy(t.data(), b->B.torque, sizeof(float4) * n, hipMemcpyDeviceToHost));
+    HIPCHECK(hipMemcpy(fl.data(), b->B.flags, sizeof(int) * n, hipMemcpyDeviceToHost));
+    if (b->settings.sleeping_enabled) {
+        st.resize(n);
+        HIPCHECK(hipMemcpy(st.data(), b->isl.sleep_time, sizeof(float) * n, hipMemcpyDeviceToHost));
+    }
+    auto nonzero = [](const float* a, int i) { return a && (a[3 * i] != 0.f || a[3 * i + 1] != 0.f || a[3 * i + 2] != 0.f); };
+    for (int i = 0; i < n; i++) {
+        if (!(fl[i] & BF_DYNAMIC)) continue;   // only dynamic bodies take forces (RigidBody.cpp:298-311)
+        const bool nf = nonzero(force, i), nt = nonzero(torque, i);
+        if (!nf && !nt) continue;
+        if (!(fl[i] & BF_ENABLED)) {   // asleep: setIsSleeping(false)
+            fl[i] |= BF_ENABLED | BF_WOKEN;
+            st[i] = 0.0f;
+        }
+        if (nf) { f[i].x += force[3 * i]; f[i].y += force[3 * i + 1]; f[i].z += force[3 * i + 2]; }
+        if (nt) { t[i].x += torque[3 * i]; t[i].y += torque[3 * i + 1]; t[i].z += torque[3 * i + 2]; }
+    }
+    HIPCHECK(hipMemcpy(b->B.force, f.data(), sizeof(float4) * n, hipMemcpyHostToDevice));
+    HIPCHECK(hipMemcpy(b->B.torque, t.data(), sizeof(float4) * n, hipMemcpyHostToDevice));
+    HIPCHECK(hipMemcpy(b->B.flags, fl.data(), sizeof(int) * n, hipMemcpyHostToDevice));
+    if (!st.empty()) HIPCHECK(hipMemcpy(b->isl.sleep_time, st.data(), sizeof(float) * n, hipMemcpyHostToDevice));
+    return RP3DG_OK;
+}
+
 int rp3dg_batch_raycast(rp3dg_batch* b, int64_t n, const int32_t* world, const float* point1, const float* point2,
                         const float* max_fraction, uint16_t category_mask, int32_t* collider, float* point,
                         float* normal, float* fraction) {

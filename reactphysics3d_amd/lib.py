@@ -66,7 +66,7 @@ EXPORTED = [
     "rp3dg_collider_add", "rp3dg_collider_set_material", "rp3dg_collider_set_filter", "rp3dg_hull_create",
     "rp3dg_joint_create", "rp3dg_joint_set_limits", "rp3dg_body_set_allowed_to_sleep", "rp3dg_batch_get_sleeping",
     "rp3dg_world_get_body_sleeping", "rp3dg_batch_set_contact_events", "rp3dg_batch_get_contact_events",
-    "rp3dg_world_set_contact_events", "rp3dg_world_get_batch", "rp3dg_batch_raycast",
+    "rp3dg_world_set_contact_events", "rp3dg_world_get_batch", "rp3dg_batch_raycast", "rp3dg_batch_apply_forces",
     "rp3dg_world_update", "rp3dg_batch_create", "rp3dg_batch_create_ex",
     "rp3dg_batch_destroy", "rp3dg_batch_step",
     "rp3dg_batch_step_graph", "rp3dg_batch_synchronize", "rp3dg_batch_get_counts", "rp3dg_batch_get_bodies",
@@ -128,6 +128,7 @@ def load():
         "rp3dg_world_set_contact_events": (C.c_int, [vp, i32]),
         "rp3dg_world_get_batch": (vp, [vp]),
         "rp3dg_batch_raycast": (C.c_int, [vp, C.c_int64, iptr, fptr, fptr, fptr, C.c_uint16, iptr, fptr, fptr, fptr]),
+        "rp3dg_batch_apply_forces": (C.c_int, [vp, fptr, fptr]),
         "rp3dg_batch_get_contact_events": (C.c_int64, [vp, iptr, iptr, iptr, iptr, C.c_int64]),
         "rp3dg_world_update": (C.c_int, [vp, f]),
         "rp3dg_batch_create": (vp, [C.POINTER(vp), i32]),
@@ -214,7 +215,7 @@ def world_from_scene(sc: S.Scene):
         _check(L.rp3dg_hull_create(w, _fp(v), len(v), _ip(counts), _ip(idx), len(faces)), "hull_create")
     ci = 0
     cols = sc.colliders
-    for bi, (bt, pos, q, lin, ang, grav, ld, ad) in enumerate(sc.bodies):
+    for bi, (bt, pos, q, lin, ang, grav, ld, ad, llock, alock) in enumerate(sc.bodies):
         b = _check(L.rp3dg_body_create(w, _f3(pos), _f4(q)), "body_create")
         _check(L.rp3dg_body_set_type(w, b, bt), "body_set_type")
         while ci < len(cols) and cols[ci][0] == bi:
@@ -232,6 +233,8 @@ def world_from_scene(sc: S.Scene):
             _check(L.rp3dg_body_enable_gravity(w, b, 0), "enable_gravity")
         if ld != 0.0 or ad != 0.0:
             _check(L.rp3dg_body_set_damping(w, b, ld, ad), "set_damping")
+        if tuple(llock) != (1.0, 1.0, 1.0) or tuple(alock) != (1.0, 1.0, 1.0):
+            _check(L.rp3dg_body_set_lock_factors(w, b, _f3(llock), _f3(alock)), "set_lock_factors")
     for (jt, b1, b2, anchor, axis, collide, flags, limits) in sc.joints:
         _check(L.rp3dg_joint_create(w, jt, b1, b2, _f3(anchor), _f3(axis), collide, flags, _f4(limits)),
                "joint_create")
@@ -337,6 +340,14 @@ class Batch:
                                           _fp(mf) if mf is not None else None, int(mask), _ip(col), _fp(pt), _fp(nr),
                                           _fp(fr)), "raycast")
         return col, pt, nr, fr
+
+    def apply_forces(self, force=None, torque=None):
+        """RigidBody::applyWorldForceAtCenterOfMass / applyWorldTorque on every body ([nb, 3] each, None =
+        none) before the next step; the step clears them."""
+        f = None if force is None else np.ascontiguousarray(force, np.float32)
+        t = None if torque is None else np.ascontiguousarray(torque, np.float32)
+        _check(load().rp3dg_batch_apply_forces(self.h, _fp(f) if f is not None else None,
+                                               _fp(t) if t is not None else None), "apply_forces")
 
     def sleeping(self):
         """RigidBody::isSleeping of every body, int32 0 / 1."""

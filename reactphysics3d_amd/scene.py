@@ -32,7 +32,7 @@ SETTINGS_DTYPE = np.dtype([
 ])
 BODY_DTYPE = np.dtype([
     ("type", "<i4"), ("pos", "<f4", 3), ("quat", "<f4", 4), ("lin", "<f4", 3), ("ang", "<f4", 3),
-    ("gravity", "<i4"), ("ldamp", "<f4"), ("adamp", "<f4"),
+    ("gravity", "<i4"), ("ldamp", "<f4"), ("adamp", "<f4"), ("llock", "<f4", 3), ("alock", "<f4", 3),
 ])
 COLLIDER_DTYPE = np.dtype([
     ("body", "<i4"), ("shape", "<i4"), ("params", "<f4", 4), ("lpos", "<f4", 3), ("lquat", "<f4", 4),
@@ -91,8 +91,10 @@ class Scene:
 
     # -- builders -------------------------------------------------------------------------------
     def add_body(self, btype, pos, quat=(0.0, 0.0, 0.0, 1.0), lin=(0.0, 0.0, 0.0), ang=(0.0, 0.0, 0.0),
-                 gravity=True, ldamp=0.0, adamp=0.0):
-        self.bodies.append((btype, tuple(pos), tuple(quat), tuple(lin), tuple(ang), int(gravity), ldamp, adamp))
+                 gravity=True, ldamp=0.0, adamp=0.0, llock=(1.0, 1.0, 1.0), alock=(1.0, 1.0, 1.0)):
+        """llock / alock: RigidBody::setLinearLockAxisFactor / setAngularLockAxisFactor."""
+        self.bodies.append((btype, tuple(pos), tuple(quat), tuple(lin), tuple(ang), int(gravity), ldamp, adamp,
+                            tuple(llock), tuple(alock)))
         return len(self.bodies) - 1
 
     def add_collider(self, body, shape, params, lpos=(0.0, 0.0, 0.0), lquat=(0.0, 0.0, 0.0, 1.0),
@@ -458,6 +460,34 @@ def sleeping_chain(length=6, radius=0.3, hit_distance=5.0, hit_speed=3.0, seed=6
     z_low = top - radius * (2 * length - 1)
     ball = sc.add_body(DYNAMIC, (hit_distance, 0.05, z_low), lin=(-hit_speed, 0.0, 0.0), gravity=False)
     sc.add_collider(ball, SPHERE, (radius,), density=2.0)
+    return sc
+
+
+def body_features(seed=10):
+    """Bodies beyond plain dynamic ones: a kinematic platform (translating at 0.5 m/s and turning at
+    0.3 rad/s, 0.25 m above the ground) carrying two boxes, spheres with linear damping 0.5 and angular
+    damping 0.8, a box whose motion is locked to the x-z plane and to rotation about y (lock factors
+    (1, 0, 1) and (0, 1, 0)), and a capsule without gravity drifting and spinning into the pile."""
+    rng = SplitMix64(seed)
+    sc = Scene("body_features", make_settings(gravity=(0.0, 0.0, -9.81), vel_iters=10, pos_iters=5, friction=0.5,
+                                              bounciness=0.2))
+    g = sc.add_body(STATIC, (0.0, 0.0, -0.5))
+    sc.add_collider(g, BOX, (20.0, 20.0, 0.5))
+    plat = sc.add_body(KINEMATIC, (0.0, 0.0, 0.5), lin=(0.5, 0.0, 0.0), ang=(0.0, 0.0, 0.3))
+    sc.add_collider(plat, BOX, (2.0, 2.0, 0.25))
+    for k in range(2):
+        b = sc.add_body(DYNAMIC, (-0.8 + 1.6 * k, 0.3 * k, 1.06), rng.quaternion() if k else (0.0, 0.0, 0.0, 1.0))
+        sc.add_collider(b, BOX, (0.3, 0.3, 0.3))
+    for k in range(3):
+        b = sc.add_body(DYNAMIC, (4.0 + 0.9 * k, rng.uniform(-0.3, 0.3), 1.0 + 1.2 * k),
+                        ang=(rng.normal(), rng.normal(), 4.0 * rng.normal()), ldamp=0.5, adamp=0.8)
+        sc.add_collider(b, SPHERE, (0.4,))
+    lk = sc.add_body(DYNAMIC, (-4.0, 0.0, 2.0), rng.quaternion(), lin=(1.0, 1.0, 0.0), ang=(2.0, 1.0, 3.0),
+                     llock=(1.0, 0.0, 1.0), alock=(0.0, 1.0, 0.0))
+    sc.add_collider(lk, BOX, (0.4, 0.3, 0.5))
+    cap = sc.add_body(DYNAMIC, (8.0, 0.0, 0.9), rng.quaternion(), lin=(-1.5, 0.0, 0.0), ang=(0.0, 2.0, 0.0),
+                      gravity=False)
+    sc.add_collider(cap, CAPSULE, (0.3, 1.2))
     return sc
 
 
