@@ -161,7 +161,7 @@ rp3dg_batch* rp3dg_batch_create(rp3dg_world* const* worlds, int32_t nb_worlds);
 /* Solver layout of a batch. */
 #define RP3DG_SOLVER_AUTO 0   /* FUSED when every world has at most 4096 bodies, else PER_COLOR */
 #define RP3DG_SOLVER_PER_COLOR 1  /* one launch per colour over all worlds (large worlds) */
-#define RP3DG_BROADPHASE_AUTO 0        /* world scan when it applies, else the tree */
+#define RP3DG_BROADPHASE_AUTO 0        /* world scan when it applies and pays (>= 64 worlds, or worlds of <= 256 colliders), else the tree */
 #define RP3DG_BROADPHASE_TREE 1        /* linear BVH over all colliders, rebuilt every step */
 #define RP3DG_BROADPHASE_WORLD_SCAN 2  /* fused layout, worlds of <= 1024 colliders: every collider tests its
                                           world's boxes in LDS (no tree) */

@@ -588,7 +588,10 @@ int rp3dg_batch::upload(rp3dg_world* const* ws, int nw, const rp3dg_batch_option
             set_error("the world-scan broadphase needs the fused layout and worlds of at most 1024 colliders");
             return RP3DG_EINVAL;
         }
-        world_scan = fits && o.broadphase != RP3DG_BROADPHASE_TREE;
+        // automatic choice: one workgroup scans a whole world (its colliders squared box tests), which
+        // pays for itself with many worlds sharing the GPU or small worlds, not for a few large ones
+        const bool worth = max_world_colliders <= 256 || nw >= 64;
+        world_scan = fits && (o.broadphase == RP3DG_BROADPHASE_WORLD_SCAN || (o.broadphase == RP3DG_BROADPHASE_AUTO && worth));
     }
     user_block = o.fused_block;
     if (o.fused_block) fused_block = o.fused_block;
