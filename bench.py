@@ -270,6 +270,15 @@ def main():
         d.sync()
     log(f"[bench] warmup {args.warmup} steps done; counts {dev.counts()}")
     graph = batched or len(scenes) > 1
+    if not graph:
+        # a single world in the fused layout replays captured steps as well (the per-colour layout of
+        # large worlds reads its colour count back every step and runs eagerly)
+        try:
+            dev.step_graph(DT, 1)
+            dev.sync()
+            graph = True
+        except RuntimeError:
+            pass
     training = None
     if batched:
         mlp, training = train_warm_start(args)
