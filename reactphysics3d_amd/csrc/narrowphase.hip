@@ -233,15 +233,15 @@ __device__ inline int clip_polygon(const V3* in, int nin, V3 pp, V3 pn, V3* out)
         if (edn >= 0.0f) {
             if (sdn < 0.0f) {
                 const float t = plane_segment_intersection(vs, ve, pnd, pn);
-                if (t >= 0.0f && t <= 1.0f) { if (nout < kClipMax) out[nout++] = vs + t * (ve - vs); }
-                else { if (nout < kClipMax) out[nout++] = ve; }
+                if (t >= 0.0f && t <= 1.0f) { if (nout < CM) out[nout++] = vs + t * (ve - vs); }
+                else { if (nout < CM) out[nout++] = ve; }
             }
-            if (nout < kClipMax) out[nout++] = ve;
+            if (nout < CM) out[nout++] = ve;
         } else {
             if (sdn >= 0.0f) {
                 const float t = plane_segment_intersection(vs, ve, -pnd, -pn);
-                if (t >= 0.0f && t <= 1.0f) { if (nout < kClipMax) out[nout++] = vs + t * (ve - vs); }
-                else { if (nout < kClipMax) out[nout++] = vs; }
+                if (t >= 0.0f && t <= 1.0f) { if (nout < CM) out[nout++] = vs + t * (ve - vs); }
+                else { if (nout < CM) out[nout++] = vs; }
             }
         }
         s = e;
