@@ -960,9 +960,9 @@ int rp3dg_batch::enqueue_step(float dt, bool capturing) {
     LAUNCH_T("np_capsule_capsule", tg(ALGO_CAPSULE_CAPSULE, 256), 256, np_capsule_capsule, A, L + ALGO_CAPSULE_CAPSULE * pair_cap, T + ALGO_CAPSULE_CAPSULE);
     LAUNCH_T("np_sphere_poly", tg(ALGO_SPHERE_POLY, 256), 256, np_sphere_poly, A, L + ALGO_SPHERE_POLY * pair_cap, T + ALGO_SPHERE_POLY);
     LAUNCH_T("np_capsule_poly", tg(ALGO_CAPSULE_POLY, 256), 256, np_capsule_poly, A, L + ALGO_CAPSULE_POLY * pair_cap, T + ALGO_CAPSULE_POLY);
-    LAUNCH_T("np_box_box", hint(last.nb_box_tests, 128), 128, np_box_box, A, L + np_list_slot(ALGO_COUNT) * pair_cap,
+    LAUNCH_T("np_box_box", hint(last.nb_box_tests, kNpBoxThreads), kNpBoxThreads, np_box_box, A, L + np_list_slot(ALGO_COUNT) * pair_cap,
              &counters->nb_box_tests);
-    LAUNCH_T("np_box_box", hint(last.nb_box_sep_tests, 128), 128, np_box_box, A, L + np_list_slot(ALGO_COUNT + 1) * pair_cap,
+    LAUNCH_T("np_box_box", hint(last.nb_box_sep_tests, kNpBoxThreads), kNpBoxThreads, np_box_box, A, L + np_list_slot(ALGO_COUNT + 1) * pair_cap,
              &counters->nb_box_sep_tests);
     LAUNCH_T("np_poly_poly", hint(last.nb_hull_tests, 128), 128, np_poly_poly, A, L + ALGO_POLY_POLY * pair_cap,
              &counters->nb_hull_tests);

@@ -42,6 +42,9 @@ __global__ void np_capsule_capsule(NpArgs A, const int* list, const int* count);
 __global__ void np_sphere_poly(NpArgs A, const int* list, const int* count);
 __global__ void np_capsule_poly(NpArgs A, const int* list, const int* count);
 __global__ void np_poly_poly(NpArgs A, const int* list, const int* count);
+// np_box_box runs in workgroups of exactly this many threads (its clip buffers are sliced from LDS by
+// threadIdx.x)
+constexpr int kNpBoxThreads = 128;
 __global__ void np_box_box(NpArgs A, const int* list, const int* count);
 void upload_box_topology(const int* edges4, const int* face_edge, const int* face_verts);
 

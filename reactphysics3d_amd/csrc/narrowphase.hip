@@ -9,6 +9,8 @@
 //   -> cache write (the next step reads it back).
 #include <hip/hip_runtime.h>
 
+#include <type_traits>
+
 #include "kcommon.h"
 #include "narrowphase.h"
 #include "rp3d_internal.h"
@@ -133,26 +135,10 @@ struct RawBuf {
         }
     }
     __device__ V3 normal(int r) const { return nrm[r]; }
+    __device__ V3 p1(int r) const { return l1[r]; }
+    __device__ void point(int r, V3& a, V3& b, float& d) const { a = l1[r]; b = l2[r]; d = depth[r]; }
 };
 using Raw = RawBuf<kMaxRawContacts>;
-
-// Box-box tests: every contact point of one test carries the same normal (the reference face normal
-// of the face contacts, or the single edge-edge contact), so it is kept once.
-struct RawBox {
-    static constexpr int kN = 8;
-    int n;
-    V3 nrm0;
-    float depth[kN];
-    V3 l1[kN];
-    V3 l2[kN];
-    __device__ void add(V3 normal, float d, V3 a, V3 b) {
-        if (n < kN) {
-            nrm0 = normal; depth[n] = d; l1[n] = a; l2[n] = b;
-            n++;
-        }
-    }
-    __device__ V3 normal(int) const { return nrm0; }
-};
 
 // Up to 16 indices below 16, four bits each, in one register (dynamic indexing without scratch)
 struct IdxList {
@@ -160,6 +146,73 @@ struct IdxList {
     __device__ int get(int i) const { return (int)((v >> (4 * i)) & 15u); }
     __device__ void set(int i, int x) { v = (v & ~(15ull << (4 * i))) | ((uint64_t)x << (4 * i)); }
 };
+
+// Clip buffers of one thread.  General polyhedra keep them in private arrays; the box-box kernel keeps
+// them in a strided slice of its workgroup's LDS, so that it needs no scratch memory.
+template <int CM>
+struct PrivClip {
+    V3 v[CM];
+    __device__ V3 get(int i) const { return v[i]; }
+    __device__ void put(int i, V3 x) { v[i] = x; }
+};
+struct LdsClip {
+    float* p;   // component c of vertex i at p[(3 * i + c) * kNpBoxThreads]
+    __device__ V3 get(int i) const {
+        return v3(p[(3 * i) * kNpBoxThreads], p[(3 * i + 1) * kNpBoxThreads], p[(3 * i + 2) * kNpBoxThreads]);
+    }
+    __device__ void put(int i, V3 x) {
+        p[(3 * i) * kNpBoxThreads] = x.x;
+        p[(3 * i + 1) * kNpBoxThreads] = x.y;
+        p[(3 * i + 2) * kNpBoxThreads] = x.z;
+    }
+};
+
+// Raw contacts of a box-box test, held without private arrays.  Every contact point of one test carries
+// the same normal (the reference face normal of the face contacts, or the single edge-edge contact).
+// Face contacts are kept as indices of clipped vertices in the LDS clip buffer, and each read recomputes
+// the point pair and depth from the vertex with the operations that the general buffer stores
+// (SATAlgorithm.cpp:1010-1025), so the values are identical; the one edge-edge contact is kept as is.
+struct RawBoxView {
+    static constexpr int kN = 8;
+    int n;
+    V3 nrm0;
+    IdxList ci;       // clip-buffer index of each face contact
+    LdsClip clip;     // the buffer that holds the clipped incident face
+    bool face;        // contacts are face contacts (else the one edge-edge contact below)
+    bool refIs1;
+    Tf refToInc;
+    V3 axisRef, refV;
+    V3 e1, e2;
+    float ed;
+    __device__ void add(V3 normal, float d, V3 a, V3 b) {   // edge-edge contact
+        if (n < kN) {
+            nrm0 = normal; face = false; e1 = a; e2 = b; ed = d;
+            n++;
+        }
+    }
+    __device__ void add_clipped(V3 normal, int i) {
+        if (n < kN) {
+            nrm0 = normal; face = true; ci.set(n, i);
+            n++;
+        }
+    }
+    __device__ V3 normal(int) const { return nrm0; }
+    __device__ void point(int r, V3& a, V3& b, float& d) const {
+        if (!face) { a = e1; b = e2; d = ed; return; }
+        const V3 c = clip.get(ci.get(r));
+        d = dot(refV - c, axisRef);
+        const V3 cInc = tapply(refToInc, c);
+        const V3 cRef = project_point_onto_plane(c, axisRef, refV);
+        a = refIs1 ? cRef : cInc;
+        b = refIs1 ? cInc : cRef;
+    }
+    __device__ V3 p1(int r) const {
+        if (!face) return e1;
+        const V3 c = clip.get(ci.get(r));
+        return refIs1 ? project_point_onto_plane(c, axisRef, refV) : tapply(refToInc, c);
+    }
+};
+
 constexpr int kBoxClip = 8;     // vertices of a box face clipped by the four side planes at most
 
 // ------------------------------------------------------------------------------------------------
@@ -221,27 +274,27 @@ __device__ inline float sat_edge_distance(V3 e1a, V3 e2a, V3 c1, V3 e1d, V3 e2d,
 }
 
 // mathematics_functions.h:332-392
-template <int CM>
-__device__ inline int clip_polygon(const V3* in, int nin, V3 pp, V3 pn, V3* out) {
+template <int CM, class CB>
+__device__ inline int clip_polygon(const CB& in, int nin, V3 pp, V3 pn, CB& out) {
     int nout = 0;
     int s = nin - 1;
     const float pnd = dot(pn, pp);
-    float sdn = dot(in[s] - pp, pn);
+    float sdn = dot(in.get(s) - pp, pn);
     for (int e = 0; e < nin; e++) {
-        const V3 vs = in[s], ve = in[e];
+        const V3 vs = in.get(s), ve = in.get(e);
         const float edn = dot(ve - pp, pn);
         if (edn >= 0.0f) {
             if (sdn < 0.0f) {
                 const float t = plane_segment_intersection(vs, ve, pnd, pn);
-                if (t >= 0.0f && t <= 1.0f) { if (nout < CM) out[nout++] = vs + t * (ve - vs); }
-                else { if (nout < CM) out[nout++] = ve; }
+                if (t >= 0.0f && t <= 1.0f) { if (nout < CM) out.put(nout++, vs + t * (ve - vs)); }
+                else { if (nout < CM) out.put(nout++, ve); }
             }
-            if (nout < CM) out[nout++] = ve;
+            if (nout < CM) out.put(nout++, ve);
         } else {
             if (sdn >= 0.0f) {
                 const float t = plane_segment_intersection(vs, ve, -pnd, -pn);
-                if (t >= 0.0f && t <= 1.0f) { if (nout < CM) out[nout++] = vs + t * (ve - vs); }
-                else { if (nout < CM) out[nout++] = vs; }
+                if (t >= 0.0f && t <= 1.0f) { if (nout < CM) out.put(nout++, vs + t * (ve - vs)); }
+                else { if (nout < CM) out.put(nout++, vs); }
             }
         }
         s = e;
@@ -251,9 +304,10 @@ __device__ inline int clip_polygon(const V3* in, int nin, V3 pp, V3 pn, V3* out)
 }
 
 // :893-1036
-template <int CM, class RB>
+template <int CM, class RB, class CB>
 __device__ __forceinline__ bool sat_face_contacts_t(bool refIs1, const Poly& P1, const Poly& P2, const Tf& t12,
-                                                    const Tf& t21, int minFace, const Tf& T1, const Tf& T2, RB& raw) {
+                                                    const Tf& t21, int minFace, const Tf& T1, const Tf& T2, RB& raw,
+                                                    CB& bufA, CB& bufB) {
     const Tf& refToInc = refIs1 ? t12 : t21;
     const Tf& incToRef = refIs1 ? t21 : t12;
     const Poly& R = refIs1 ? P1 : P2;
@@ -262,10 +316,9 @@ __device__ __forceinline__ bool sat_face_contacts_t(bool refIs1, const Poly& P1,
     const V3 axisInc = qrot(refToInc.q, axisRef);
     const V3 normalWorld = refIs1 ? qrot(T1.q, axisRef) : -(qrot(T2.q, axisRef));
     const int incFace = I.most_anti_parallel_face(axisInc);
-    V3 bufA[CM], bufB[CM];
     int na = 0, nb = 0;
     const int niv = I.face_nverts(incFace);
-    for (int i = 0; i < niv && na < CM; i++) bufA[na++] = tapply(incToRef, I.vertex(I.face_vertex(incFace, i)));
+    for (int i = 0; i < niv && na < CM; i++) bufA.put(na++, tapply(incToRef, I.vertex(I.face_vertex(incFace, i))));
     const int firstEdge = R.face_edge(minFace);
     bool v1Input = false;
     int nOut = 0;
@@ -279,25 +332,40 @@ __device__ __forceinline__ bool sat_face_contacts_t(bool refIs1, const Poly& P1,
         const int4 nxt = R.edge(cur.w);
         const V3 ev2 = R.vertex(nxt.x);
         const V3 planeNormal = cross(axisRef, ev2 - ev1);
-        if (v1Input) nb = clip_polygon<CM>(bufA, na, ev1, planeNormal, bufB);
-        else na = clip_polygon<CM>(bufB, nb, ev1, planeNormal, bufA);
+        // one inlined copy of the clipping, alternating its input and output buffers
+        const int nn = clip_polygon<CM>(v1Input ? bufA : bufB, v1Input ? na : nb, ev1, planeNormal,
+                                        v1Input ? bufB : bufA);
+        if (v1Input) nb = nn;
+        else na = nn;
         curIdx = cur.w;
         cur = nxt;
         ev1 = ev2;
         if (v1Input) { na = 0; nOut = nb; }
         else { nb = 0; nOut = na; }
     } while (curIdx != firstEdge && nOut > 0);
-    const V3* clipped = v1Input ? bufB : bufA;
+    const CB& clipped = v1Input ? bufB : bufA;
     const int nclip = v1Input ? nb : na;
     const V3 refV = R.vertex(R.edge(firstEdge).x);
     bool found = false;
+    if constexpr (std::is_same<RB, RawBoxView>::value) {
+        raw.clip = clipped;
+        raw.refIs1 = refIs1;
+        raw.refToInc = refToInc;
+        raw.axisRef = axisRef;
+        raw.refV = refV;
+    }
     for (int i = 0; i < nclip; i++) {
-        const float pen = dot(refV - clipped[i], axisRef);
+        const V3 ci = clipped.get(i);
+        const float pen = dot(refV - ci, axisRef);
         if (pen > 0.0f) {
             found = true;
-            const V3 cInc = tapply(refToInc, clipped[i]);
-            const V3 cRef = project_point_onto_plane(clipped[i], axisRef, refV);
-            raw.add(normalWorld, pen, refIs1 ? cRef : cInc, refIs1 ? cInc : cRef);
+            if constexpr (std::is_same<RB, RawBoxView>::value) {
+                raw.add_clipped(normalWorld, i);
+            } else {
+                const V3 cInc = tapply(refToInc, ci);
+                const V3 cRef = project_point_onto_plane(ci, axisRef, refV);
+                raw.add(normalWorld, pen, refIs1 ? cRef : cInc, refIs1 ? cInc : cRef);
+            }
         }
     }
     return found;
@@ -306,15 +374,24 @@ __device__ __forceinline__ bool sat_face_contacts_t(bool refIs1, const Poly& P1,
 // General polyhedra (hulls): out of line, large clip buffers
 __device__ __noinline__ bool sat_face_contacts(bool refIs1, const Poly& P1, const Poly& P2, const Tf& t12,
                                                const Tf& t21, int minFace, const Tf& T1, const Tf& T2, Raw& raw) {
-    return sat_face_contacts_t<kClipMax>(refIs1, P1, P2, t12, t21, minFace, T1, T2, raw);
+    PrivClip<kClipMax> bufA, bufB;
+    return sat_face_contacts_t<kClipMax>(refIs1, P1, P2, t12, t21, minFace, T1, T2, raw, bufA, bufB);
 }
 
 // Box-box tests inline the face clipping with buffers sized for a box; hull tests call it out of line.
 template <bool BOX, class R>
 __device__ __forceinline__ bool face_contacts(bool refIs1, const Poly& P1, const Poly& P2, const Tf& t12, const Tf& t21,
                                               int minFace, const Tf& T1, const Tf& T2, R& raw) {
-    if constexpr (BOX) return sat_face_contacts_t<kBoxClip>(refIs1, P1, P2, t12, t21, minFace, T1, T2, raw);
-    else return sat_face_contacts(refIs1, P1, P2, t12, t21, minFace, T1, T2, raw);
+    if constexpr (BOX) {
+        // one slice per thread of the np_box_box workgroup (kNpBoxThreads threads), shared by every
+        // inlined call site of the test
+        __shared__ float s_box_clip[2 * 3 * kBoxClip * kNpBoxThreads];
+        LdsClip bufA{s_box_clip + threadIdx.x};
+        LdsClip bufB{s_box_clip + 3 * kBoxClip * kNpBoxThreads + threadIdx.x};
+        return sat_face_contacts_t<kBoxClip>(refIs1, P1, P2, t12, t21, minFace, T1, T2, raw, bufA, bufB);
+    } else {
+        return sat_face_contacts(refIs1, P1, P2, t12, t21, minFace, T1, T2, raw);
+    }
 }
 
 __device__ inline uint32_t lf_set_face(uint32_t lf, bool face1, int face) {
@@ -342,28 +419,23 @@ __device__ __forceinline__ bool sat_poly_poly(const Poly& P1, const Poly& P2, co
     int minE1 = 0, minE2 = 0;
     V3 s1a = v3zero(), s1b = v3zero(), s2a = v3zero(), s2b = v3zero(), minAxis = v3zero();
     const bool wasColliding = (lf & LF_WAS_COLLIDING) != 0;
+    // The face clipping has one call site (its code is large, and a single copy keeps the kernel's
+    // instructions cached): the cached face of the last step is clipped first, and when that finds no
+    // contact the full test below runs and may come back with its own minimum face.
+    int clipFace = -1;
+    bool clipRef1 = false, clipFinal = false;
 
     if ((lf & LF_VALID) && (lf & LF_WAS_SAT)) {
         if (lf & LF_SAT_FACE1) {
             const int f = (int)lf_face(lf);
             const float pen = sat_single_face(P1, P2, t12, f);
             if (!wasColliding && pen <= 0.0f) return false;
-            if (wasColliding && pen > 0.0f) {
-                if (face_contacts<BOX>(true, P1, P2, t12, t21, f, T1, T2, raw)) {
-                    lf = lf_set_face(lf, true, f);
-                    return true;
-                }
-            }
+            if (wasColliding && pen > 0.0f) { clipFace = f; clipRef1 = true; }
         } else if (lf & LF_SAT_FACE2) {
             const int f = (int)lf_face(lf);
             const float pen = sat_single_face(P2, P1, t21, f);
             if (!wasColliding && pen <= 0.0f) return false;
-            if (wasColliding && pen > 0.0f) {
-                if (face_contacts<BOX>(false, P1, P2, t12, t21, f, T1, T2, raw)) {
-                    lf = lf_set_face(lf, false, f);
-                    return true;
-                }
-            }
+            if (wasColliding && pen > 0.0f) { clipFace = f; clipRef1 = false; }
         } else {
             const int4 e1 = P1.edge((int)lf_edge1(lf));
             const int4 e2 = P2.edge((int)lf_edge2(lf));
@@ -397,7 +469,16 @@ __device__ __forceinline__ bool sat_poly_poly(const Poly& P1, const Poly& P2, co
             }
         }
     }
-
+    if (clipFace < 0) goto full_test;
+clip_face:
+    {
+        const bool found = face_contacts<BOX>(clipRef1, P1, P2, t12, t21, clipFace, T1, T2, raw);
+        if (found || clipFinal) {
+            lf = lf_set_face(lf, clipRef1, clipFace);
+            return found;
+        }
+    }
+full_test:
     minPen = kDecimalLargest;
     isMinFace = false;
     int face1 = 0, face2 = 0;
@@ -439,9 +520,10 @@ __device__ __forceinline__ bool sat_poly_poly(const Poly& P1, const Poly& P2, co
         }
     }
     if (isMinFace) {
-        const bool found = face_contacts<BOX>(isMinFace1, P1, P2, t12, t21, minFace, T1, T2, raw);
-        lf = lf_set_face(lf, isMinFace1, minFace);
-        return found;
+        clipFace = minFace;
+        clipRef1 = isMinFace1;
+        clipFinal = true;
+        goto clip_face;
     }
     V3 cp1, cp2;
     closest_points_between_segments(s1a, s1b, s2a, s2b, cp1, cp2);
@@ -619,7 +701,7 @@ __device__ void reduce_points(const R& raw, const Tf& T1, int* keep) {
     float maxDot = kDecimalSmallest;
     int idx = 0;
     for (int i = 0; i < nc; i++) {
-        const float dp = dot(dir, raw.l1[cand.get(i)]);
+        const float dp = dot(dir, raw.p1(cand.get(i)));
         if (dp > maxDot) { maxDot = dp; idx = i; }
     }
     keep[0] = cand.get(idx);
@@ -627,7 +709,7 @@ __device__ void reduce_points(const R& raw, const Tf& T1, int* keep) {
     float maxDist = 0.0f;
     idx = 0;
     for (int i = 0; i < nc; i++) {
-        const float d = len2(raw.l1[keep[0]] - raw.l1[cand.get(i)]);
+        const float d = len2(raw.p1(keep[0]) - raw.p1(cand.get(i)));
         if (d >= maxDist) { maxDist = d; idx = i; }
     }
     keep[1] = cand.get(idx);
@@ -635,8 +717,8 @@ __device__ void reduce_points(const R& raw, const Tf& T1, int* keep) {
     int maxI = 0, minI = 0;
     float minArea = 0.0f, maxArea = 0.0f;
     for (int i = 0; i < nc; i++) {
-        const V3 nf = raw.l1[keep[0]] - raw.l1[cand.get(i)];
-        const V3 ns = raw.l1[keep[1]] - raw.l1[cand.get(i)];
+        const V3 nf = raw.p1(keep[0]) - raw.p1(cand.get(i));
+        const V3 ns = raw.p1(keep[1]) - raw.p1(cand.get(i));
         const float area = dot(cross(nf, ns), nS1);
         if (area >= maxArea) { maxArea = area; maxI = i; }
         if (area <= minArea) { minArea = area; minI = i; }
@@ -657,8 +739,8 @@ __device__ void reduce_points(const R& raw, const Tf& T1, int* keep) {
 #pragma unroll
         for (int j = 0; j < 3; j++) {
             const int va = j, vb = j < 2 ? j + 1 : 0;
-            const V3 nf = raw.l1[keep[va]] - raw.l1[cand.get(i)];
-            const V3 ns = raw.l1[keep[vb]] - raw.l1[cand.get(i)];
+            const V3 nf = raw.p1(keep[va]) - raw.p1(cand.get(i));
+            const V3 ns = raw.p1(keep[vb]) - raw.p1(cand.get(i));
             const float area = dot(cross(nf, ns), nS1);
             if (prevPositive && area <= largest) { largest = area; idx = i; }
             else if (!prevPositive && area >= largest) { largest = area; idx = i; }
@@ -691,7 +773,7 @@ __device__ void contact_epilogue(const NpArgs& A, int slot, bool colliding, cons
     const float thr2 = 0.01f * 0.01f;
     for (int i = 0; i < n; i++) {
         for (int j = i + 1; j < n; j++) {
-            const float d2 = len2(raw.l1[idx.get(j)] - raw.l1[idx.get(i)]);
+            const float d2 = len2(raw.p1(idx.get(j)) - raw.p1(idx.get(i)));
             if (d2 < thr2) {
                 idx.set(j, idx.get(n - 1));
                 n--;
@@ -727,21 +809,24 @@ __device__ void contact_epilogue(const NpArgs& A, int slot, bool colliding, cons
     const float pt2 = A.persistent_threshold * A.persistent_threshold;
     for (int k = 0; k < n; k++) {
         const int r = idx.get(k);
+        V3 pl1, pl2;
+        float depth;
+        raw.point(r, pl1, pl2, depth);
         float imp = 0.0f, resting = 0.0f;
         bool matched = false;
 #pragma unroll
         for (int o = 0; o < kMaxManifoldPoints; o++) {
             if (matched || o >= oldN) continue;
-            const float d2 = len2(raw.l1[r] - xyz(oldl1[o]));
+            const float d2 = len2(pl1 - xyz(oldl1[o]));
             if (d2 <= pt2) {
                 imp = oldl1[o].w;
                 resting = oldrest[o];
                 matched = true;
             }
         }
-        P.pn[4 * slot + k] = f4(raw.normal(r), raw.depth[r]);
-        P.pl1[4 * slot + k] = f4(raw.l1[r], imp);
-        P.pl2[4 * slot + k] = f4(raw.l2[r], resting);
+        P.pn[4 * slot + k] = f4(raw.normal(r), depth);
+        P.pl1[4 * slot + k] = f4(pl1, imp);
+        P.pl2[4 * slot + k] = f4(pl2, resting);
     }
     P.fv1[slot] = fv1;
     P.fv2[slot] = fv2;
@@ -906,7 +991,7 @@ __device__ __forceinline__ Poly box_poly(const NpArgs& A, int c) {
     return P;
 }
 
-__global__ __launch_bounds__(128) void np_box_box(NpArgs A, const int* list, const int* count) {
+__global__ __launch_bounds__(kNpBoxThreads) void np_box_box(NpArgs A, const int* list, const int* count) {
     RP3DG_GRID_STRIDE(t, *count) {
         const int k = list[t];
         const int2 c = A.P.col[k];
@@ -914,7 +999,7 @@ __global__ __launch_bounds__(128) void np_box_box(NpArgs A, const int* list, con
         const Tf T2 = tf(ld3(A.C.wpos, c.y), ldq(A.C.wq, c.y));
         const Poly P1 = box_poly(A, c.x);
         const Poly P2 = box_poly(A, c.y);
-        RawBox raw; raw.n = 0;
+        RawBoxView raw; raw.n = 0;
         uint32_t lf = A.P.lf[k];
         const bool hit = sat_poly_poly<true>(P1, P2, T1, T2, lf, raw);
         // ConvexPolyhedronVsConvexPolyhedronAlgorithm.cpp:55-62
