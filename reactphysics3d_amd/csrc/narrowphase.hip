@@ -9,6 +9,7 @@
 //   -> cache write (the next step reads it back).
 #include <hip/hip_runtime.h>
 
+#include <climits>
 #include <type_traits>
 
 #include "kcommon.h"
@@ -37,31 +38,17 @@ struct Poly {
 
     __device__ int nfaces() const { return hull < 0 ? 6 : h.nf; }
     __device__ int nhalfedges() const { return hull < 0 ? 24 : h.ne; }
-    // BoxShape.h:186-205
+    // BoxShape.h:186-205 (vertices 0-3 on +z, 4-7 on -z; +x for 1, 2, 5, 6; +y for 2, 3, 6, 7), as
+    // selects so that lanes holding different indices do not branch
     __device__ V3 vertex(int i) const {
         if (hull >= 0) return xyz(hv[h.vbase + i]);
-        switch (i) {
-            case 0: return v3(-he.x, -he.y, he.z);
-            case 1: return v3(he.x, -he.y, he.z);
-            case 2: return v3(he.x, he.y, he.z);
-            case 3: return v3(-he.x, he.y, he.z);
-            case 4: return v3(-he.x, -he.y, -he.z);
-            case 5: return v3(he.x, -he.y, -he.z);
-            case 6: return v3(he.x, he.y, -he.z);
-            default: return v3(-he.x, he.y, -he.z);
-        }
+        return v3(((i + 1) & 2) ? he.x : -he.x, (i & 2) ? he.y : -he.y, i < 4 ? he.z : -he.z);
     }
-    // BoxShape.h:208-221
+    // BoxShape.h:208-221 (faces +z, +x, -z, -x, -y, +y)
     __device__ V3 face_normal(int f) const {
         if (hull >= 0) return xyz(hn[h.fbase + f]);
-        switch (f) {
-            case 0: return v3(0, 0, 1);
-            case 1: return v3(1, 0, 0);
-            case 2: return v3(0, 0, -1);
-            case 3: return v3(-1, 0, 0);
-            case 4: return v3(0, -1, 0);
-            default: return v3(0, 1, 0);
-        }
+        return v3(f == 1 ? 1.0f : f == 3 ? -1.0f : 0.0f, f == 5 ? 1.0f : f == 4 ? -1.0f : 0.0f,
+                  f == 0 ? 1.0f : f == 2 ? -1.0f : 0.0f);
     }
     __device__ int4 edge(int e) const { return hull < 0 ? c_box_edges[e] : hedges[h.ebase + e]; }
     __device__ int face_edge(int f) const { return hull < 0 ? c_box_face_edge[f] : hfedge[h.fbase + f]; }
@@ -991,22 +978,224 @@ __device__ __forceinline__ Poly box_poly(const NpArgs& A, int c) {
     return P;
 }
 
+// Box-box test in three stages so that the costly search over the 144 edge pairs can be shared by a
+// wavefront.  Stage 1 (one lane per pair): the cached-axis shortcut and both face searches of
+// sat_poly_poly.  Returns 0 / 1 when the test is decided (no contact / contact), 2 when the edge pairs
+// must be searched; minPen0, minFace and isMinFace1 then hold the face result (SATAlgorithm.cpp:485-640).
+__device__ __forceinline__ int box_sat_begin(const Poly& P1, const Poly& P2, const Tf& T1, const Tf& T2, const Tf& t12,
+                                             const Tf& t21, uint32_t& lf, RawBoxView& raw, float& minPen0,
+                                             int& minFace, bool& isMinFace1) {
+    const bool wasColliding = (lf & LF_WAS_COLLIDING) != 0;
+    if ((lf & LF_VALID) && (lf & LF_WAS_SAT)) {
+        if (lf & (LF_SAT_FACE1 | LF_SAT_FACE2)) {
+            const bool ref1 = (lf & LF_SAT_FACE1) != 0;
+            const int f = (int)lf_face(lf);
+            const float pen = ref1 ? sat_single_face(P1, P2, t12, f) : sat_single_face(P2, P1, t21, f);
+            if (!wasColliding && pen <= 0.0f) return 0;
+            if (wasColliding && pen > 0.0f) {
+                if (face_contacts<true>(ref1, P1, P2, t12, t21, f, T1, T2, raw)) {
+                    lf = lf_set_face(lf, ref1, f);
+                    return 1;
+                }
+            }
+        } else {
+            const int4 e1 = P1.edge((int)lf_edge1(lf));
+            const int4 e2 = P2.edge((int)lf_edge2(lf));
+            const V3 e1a = tapply(t12, P1.vertex(e1.x));
+            const V3 e1b = tapply(t12, P1.vertex(P1.edge(e1.w).x));
+            const V3 e1d = e1b - e1a;
+            const V3 e2a = P2.vertex(e2.x);
+            const V3 e2b = P2.vertex(P2.edge(e2.w).x);
+            const V3 e2d = e2b - e2a;
+            if (sat_minkowski_face(P1, e1, P2, e2, t12)) {
+                V3 axis;
+                const V3 c1 = tapply(t12, P1.centroid());
+                const float pen = sat_edge_distance(e1a, e2a, c1, e1d, e2d, axis);
+                if (!wasColliding && pen <= 0.0f) return 0;
+                if (wasColliding && pen > 0.0f && pen < kDecimalLargest) {
+                    V3 cp1, cp2;
+                    closest_points_between_segments(e1a, e1b, e2a, e2b, cp1, cp2);
+                    const V3 vec1 = cp1 - e2a;
+                    const V3 vec2 = cp2 - e1a;
+                    const float l1 = len2(e1d);
+                    const float l2 = len2(e2d);
+                    const float t1 = dot(vec1, e2d) / l2;
+                    const float t2 = dot(vec2, e1d) / l1;
+                    if (t1 >= 0.0f && t1 <= 1.0f && t2 >= 0.0f && t2 <= 1.0f) {
+                        raw.add(qrot(T2.q, axis), pen, tapply(t21, cp1), cp2);
+                        return 1;
+                    }
+                }
+            }
+        }
+    }
+    int face1 = 0, face2 = 0;
+    const float pen1 = sat_faces(P1, P2, t12, face1);
+    if (pen1 <= 0.0f) { lf = lf_set_face(lf, true, face1); return 0; }
+    const float pen2 = sat_faces(P2, P1, t21, face2);
+    if (pen2 <= 0.0f) { lf = lf_set_face(lf, false, face2); return 0; }
+    minPen0 = fmin_s(pen1, pen2);
+    isMinFace1 = pen1 < pen2 * kSatRel + kSatAbs;
+    minFace = isMinFace1 ? face1 : face2;
+    return 2;
+}
+
+// Edge pair k of two boxes in the order of the reference's double loop over half-edges i, j = 0, 2, ..,
+// 22 (k = (i / 2) * 12 + j / 2); false when the edges do not build a face of the Minkowski difference.
+constexpr int kBoxEdgePairs = 144;
+__device__ __forceinline__ bool box_edge_pair(const Poly& P1, const Poly& P2, const Tf& t12, int k, float& pen,
+                                              V3* axis_out = nullptr) {
+    const int i = 2 * (k / 12), j = 2 * (k % 12);
+    const int4 e1 = P1.edge(i);
+    const int4 e2 = P2.edge(j);
+    if (!sat_minkowski_face(P1, e1, P2, e2, t12)) return false;
+    const V3 e1a = tapply(t12, P1.vertex(e1.x));
+    const V3 e1b = tapply(t12, P1.vertex(P1.edge(e1.w).x));
+    const V3 e2a = P2.vertex(e2.x);
+    const V3 e2b = P2.vertex(P2.edge(e2.w).x);
+    V3 axis = v3zero();
+    pen = sat_edge_distance(e1a, e2a, tapply(t12, P1.centroid()), e1b - e1a, e2b - e2a, axis);
+    if (axis_out) *axis_out = axis;
+    return true;
+}
+
+// The reference walks the edge pairs in order and returns at the first separating pair; otherwise an
+// edge pair replaces the face result when pen * kSatRel + kSatAbs < the face penetration, and later
+// pairs replace it when strictly smaller (SATAlgorithm.cpp:587-631).  Since rounding is monotonic, the
+// pair it ends with is the first pair of smallest penetration, provided that this one passes the face
+// test; so the walk reduces to (first separating k, first k of the smallest positive penetration),
+// which lanes can compute over disjoint subsets of k and combine.
+struct EdgeSearch {
+    int sep;     // first separating pair, INT_MAX when none
+    int k;       // first pair of the smallest penetration among the others, INT_MAX when none
+    float pen;   // its penetration (+inf when none)
+    __device__ void init() { sep = INT_MAX; k = INT_MAX; pen = __builtin_huge_valf(); }
+    __device__ void visit(int kk, bool valid, float p) {   // in increasing kk within one lane
+        if (!valid) return;
+        if (p <= 0.0f) { if (kk < sep) sep = kk; return; }
+        if (p < pen) { pen = p; k = kk; }
+    }
+    __device__ void combine(const EdgeSearch& o) {
+        if (o.sep < sep) sep = o.sep;
+        if (o.pen < pen || (o.pen == pen && o.k < k)) { pen = o.pen; k = o.k; }
+    }
+};
+
+// Stage 3: the decision of sat_poly_poly after the edge search, then the contacts.
+__device__ __forceinline__ bool box_sat_finish(const Poly& P1, const Poly& P2, const Tf& T1, const Tf& T2, const Tf& t12,
+                                               const Tf& t21, uint32_t& lf, RawBoxView& raw, float minPen0,
+                                               int minFace, bool isMinFace1, const EdgeSearch& es) {
+    if (es.sep != INT_MAX) {
+        lf = lf_set_edges(lf, 2 * (es.sep / 12), 2 * (es.sep % 12));
+        return false;
+    }
+    if (es.k != INT_MAX && es.pen * kSatRel + kSatAbs < minPen0) {
+        const int i = 2 * (es.k / 12), j = 2 * (es.k % 12);
+        float pen;
+        V3 axis;
+        box_edge_pair(P1, P2, t12, es.k, pen, &axis);
+        const int4 e1 = P1.edge(i);
+        const int4 e2 = P2.edge(j);
+        const V3 s1a = tapply(t12, P1.vertex(e1.x));
+        const V3 s1b = tapply(t12, P1.vertex(P1.edge(e1.w).x));
+        const V3 s2a = P2.vertex(e2.x);
+        const V3 s2b = P2.vertex(P2.edge(e2.w).x);
+        V3 cp1, cp2;
+        closest_points_between_segments(s1a, s1b, s2a, s2b, cp1, cp2);
+        raw.add(qrot(T2.q, axis), es.pen, tapply(t21, cp1), cp2);
+        lf = lf_set_edges(lf, i, j);
+        return true;
+    }
+    const bool found = face_contacts<true>(isMinFace1, P1, P2, t12, t21, minFace, T1, T2, raw);
+    lf = lf_set_face(lf, isMinFace1, minFace);
+    return found;
+}
+
+// Box-box pairs (the SAT test with the box topology known at compile time and box-sized buffers).  The
+// edge search of a pair that needs one is shared by the whole wavefront (each lane takes every 64th
+// pair and the results are combined across lanes) while at most kBoxCoopMax lanes need it; beyond that
+// every such lane walks its own 144 pairs.  Both give the same result.
+constexpr int kBoxCoopMax = 32;
 __global__ __launch_bounds__(kNpBoxThreads) void np_box_box(NpArgs A, const int* list, const int* count) {
-    RP3DG_GRID_STRIDE(t, *count) {
-        const int k = list[t];
-        const int2 c = A.P.col[k];
-        const Tf T1 = tf(ld3(A.C.wpos, c.x), ldq(A.C.wq, c.x));
-        const Tf T2 = tf(ld3(A.C.wpos, c.y), ldq(A.C.wq, c.y));
-        const Poly P1 = box_poly(A, c.x);
-        const Poly P2 = box_poly(A, c.y);
-        RawBoxView raw; raw.n = 0;
-        uint32_t lf = A.P.lf[k];
-        const bool hit = sat_poly_poly<true>(P1, P2, T1, T2, lf, raw);
-        // ConvexPolyhedronVsConvexPolyhedronAlgorithm.cpp:55-62
-        lf |= LF_WAS_SAT;
-        lf &= ~LF_WAS_GJK;
-        A.P.lf[k] = lf_after_test(lf, hit);
-        contact_epilogue(A, k, hit, raw, c.x);
+    const int n = *count;
+    const int lane = lane_id();
+    for (int base = blockIdx.x * blockDim.x; base < n; base += gridDim.x * blockDim.x) {   // uniform in the block
+        const int t = base + threadIdx.x;
+        const bool act = t < n;
+        int k = 0;
+        int2 c = make_int2(0, 0);
+        Poly P1{}, P2{};
+        Tf T1{}, T2{}, t12{}, t21{};
+        RawBoxView raw;
+        raw.n = 0;
+        uint32_t lf = 0;
+        int stage = 0;
+        float minPen0 = 0.0f;
+        int minFace = 0;
+        bool isMinFace1 = false;
+        if (act) {
+            k = list[t];
+            c = A.P.col[k];
+            T1 = tf(ld3(A.C.wpos, c.x), ldq(A.C.wq, c.x));
+            T2 = tf(ld3(A.C.wpos, c.y), ldq(A.C.wq, c.y));
+            P1 = box_poly(A, c.x);
+            P2 = box_poly(A, c.y);
+            lf = A.P.lf[k];
+            t12 = tmul(tinv(T2), T1);
+            t21 = tinv(t12);
+            stage = box_sat_begin(P1, P2, T1, T2, t12, t21, lf, raw, minPen0, minFace, isMinFace1);
+        }
+        EdgeSearch es;
+        es.init();
+        uint64_t m = __ballot(stage == 2);
+        if (__popcll(m) > kBoxCoopMax) {
+            if (stage == 2) {
+                for (int kk = 0; kk < kBoxEdgePairs; kk++) {
+                    float p = 0.0f;
+                    const bool v = box_edge_pair(P1, P2, t12, kk, p);
+                    es.visit(kk, v, p);
+                    if (es.sep != INT_MAX) break;
+                }
+            }
+        } else {
+            while (m) {   // wave-uniform
+                const int L = __ffsll((unsigned long long)m) - 1;
+                m &= m - 1;
+                Tf tb;
+                tb.p = v3(__shfl(t12.p.x, L), __shfl(t12.p.y, L), __shfl(t12.p.z, L));
+                tb.q.x = __shfl(t12.q.x, L); tb.q.y = __shfl(t12.q.y, L);
+                tb.q.z = __shfl(t12.q.z, L); tb.q.w = __shfl(t12.q.w, L);
+                Poly B1{}, B2{};
+                B1.hull = -1;
+                B2.hull = -1;
+                B1.he = v3(__shfl(P1.he.x, L), __shfl(P1.he.y, L), __shfl(P1.he.z, L));
+                B2.he = v3(__shfl(P2.he.x, L), __shfl(P2.he.y, L), __shfl(P2.he.z, L));
+                EdgeSearch w;
+                w.init();
+                for (int kk = lane; kk < kBoxEdgePairs; kk += kWave) {
+                    float p = 0.0f;
+                    const bool v = box_edge_pair(B1, B2, tb, kk, p);
+                    w.visit(kk, v, p);
+                }
+                for (int off = kWave / 2; off > 0; off >>= 1) {
+                    EdgeSearch o;
+                    o.sep = __shfl_xor(w.sep, off);
+                    o.k = __shfl_xor(w.k, off);
+                    o.pen = __shfl_xor(w.pen, off);
+                    w.combine(o);
+                }
+                if (lane == L) es = w;
+            }
+        }
+        if (act) {
+            bool hit = stage == 1;
+            if (stage == 2) hit = box_sat_finish(P1, P2, T1, T2, t12, t21, lf, raw, minPen0, minFace, isMinFace1, es);
+            // ConvexPolyhedronVsConvexPolyhedronAlgorithm.cpp:55-62
+            lf |= LF_WAS_SAT;
+            lf &= ~LF_WAS_GJK;
+            A.P.lf[k] = lf_after_test(lf, hit);
+            contact_epilogue(A, k, hit, raw, c.x);
+        }
     }
 }
 
