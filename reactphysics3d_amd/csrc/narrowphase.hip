@@ -395,131 +395,6 @@ __device__ inline uint32_t lf_set_edges(uint32_t lf, int e1, int e2) {
 }
 
 // Returns isColliding; updates the SAT part of the temporal-coherence word.
-template <bool BOX, class R>
-__device__ __forceinline__ bool sat_poly_poly(const Poly& P1, const Poly& P2, const Tf& T1, const Tf& T2, uint32_t& lf,
-                                              R& raw) {
-    const Tf t12 = tmul(tinv(T2), T1);
-    const Tf t21 = tinv(t12);
-    float minPen = kDecimalLargest;
-    int minFace = 0;
-    bool isMinFace = false, isMinFace1 = false;
-    int minE1 = 0, minE2 = 0;
-    V3 s1a = v3zero(), s1b = v3zero(), s2a = v3zero(), s2b = v3zero(), minAxis = v3zero();
-    const bool wasColliding = (lf & LF_WAS_COLLIDING) != 0;
-    // The face clipping has one call site (its code is large, and a single copy keeps the kernel's
-    // instructions cached): the cached face of the last step is clipped first, and when that finds no
-    // contact the full test below runs and may come back with its own minimum face.
-    int clipFace = -1;
-    bool clipRef1 = false, clipFinal = false;
-
-    if ((lf & LF_VALID) && (lf & LF_WAS_SAT)) {
-        if (lf & LF_SAT_FACE1) {
-            const int f = (int)lf_face(lf);
-            const float pen = sat_single_face(P1, P2, t12, f);
-            if (!wasColliding && pen <= 0.0f) return false;
-            if (wasColliding && pen > 0.0f) { clipFace = f; clipRef1 = true; }
-        } else if (lf & LF_SAT_FACE2) {
-            const int f = (int)lf_face(lf);
-            const float pen = sat_single_face(P2, P1, t21, f);
-            if (!wasColliding && pen <= 0.0f) return false;
-            if (wasColliding && pen > 0.0f) { clipFace = f; clipRef1 = false; }
-        } else {
-            const int4 e1 = P1.edge((int)lf_edge1(lf));
-            const int4 e2 = P2.edge((int)lf_edge2(lf));
-            const V3 e1a = tapply(t12, P1.vertex(e1.x));
-            const V3 e1b = tapply(t12, P1.vertex(P1.edge(e1.w).x));
-            const V3 e1d = e1b - e1a;
-            const V3 e2a = P2.vertex(e2.x);
-            const V3 e2b = P2.vertex(P2.edge(e2.w).x);
-            const V3 e2d = e2b - e2a;
-            if (sat_minkowski_face(P1, e1, P2, e2, t12)) {
-                V3 axis;
-                const V3 c1 = tapply(t12, P1.centroid());
-                const float pen = sat_edge_distance(e1a, e2a, c1, e1d, e2d, axis);
-                if (!wasColliding && pen <= 0.0f) return false;
-                if (wasColliding && pen > 0.0f && pen < kDecimalLargest) {
-                    V3 cp1, cp2;
-                    closest_points_between_segments(e1a, e1b, e2a, e2b, cp1, cp2);
-                    const V3 vec1 = cp1 - e2a;
-                    const V3 vec2 = cp2 - e1a;
-                    const float l1 = len2(e1d);
-                    const float l2 = len2(e2d);
-                    const float t1 = dot(vec1, e2d) / l2;
-                    const float t2 = dot(vec2, e1d) / l1;
-                    if (t1 >= 0.0f && t1 <= 1.0f && t2 >= 0.0f && t2 <= 1.0f) {
-                        const V3 p1local = tapply(t21, cp1);
-                        const V3 nw = qrot(T2.q, axis);
-                        raw.add(nw, pen, p1local, cp2);
-                        return true;
-                    }
-                }
-            }
-        }
-    }
-    if (clipFace < 0) goto full_test;
-clip_face:
-    {
-        const bool found = face_contacts<BOX>(clipRef1, P1, P2, t12, t21, clipFace, T1, T2, raw);
-        if (found || clipFinal) {
-            lf = lf_set_face(lf, clipRef1, clipFace);
-            return found;
-        }
-    }
-full_test:
-    minPen = kDecimalLargest;
-    isMinFace = false;
-    int face1 = 0, face2 = 0;
-    const float pen1 = sat_faces(P1, P2, t12, face1);
-    if (pen1 <= 0.0f) { lf = lf_set_face(lf, true, face1); return false; }
-    const float pen2 = sat_faces(P2, P1, t21, face2);
-    if (pen2 <= 0.0f) { lf = lf_set_face(lf, false, face2); return false; }
-    if (pen1 < pen2 * kSatRel + kSatAbs) {
-        isMinFace = true; minPen = fmin_s(pen1, pen2); minFace = face1; isMinFace1 = true;
-    } else {
-        isMinFace = true; minPen = fmin_s(pen1, pen2); minFace = face2; isMinFace1 = false;
-    }
-    const V3 c1 = tapply(t12, P1.centroid());
-    const int ne1 = P1.nhalfedges(), ne2 = P2.nhalfedges();
-    for (int i = 0; i < ne1; i += 2) {
-        const int4 e1 = P1.edge(i);
-        const V3 e1a = tapply(t12, P1.vertex(e1.x));
-        const V3 e1b = tapply(t12, P1.vertex(P1.edge(e1.w).x));
-        const V3 e1d = e1b - e1a;
-        for (int j = 0; j < ne2; j += 2) {
-            const int4 e2 = P2.edge(j);
-            const V3 e2a = P2.vertex(e2.x);
-            const V3 e2b = P2.vertex(P2.edge(e2.w).x);
-            const V3 e2d = e2b - e2a;
-            if (sat_minkowski_face(P1, e1, P2, e2, t12)) {
-                V3 axis;
-                const float pen = sat_edge_distance(e1a, e2a, c1, e1d, e2d, axis);
-                if (pen <= 0.0f) {
-                    lf = lf_set_edges(lf, i, j);
-                    return false;
-                }
-                if ((isMinFace && pen * kSatRel + kSatAbs < minPen) || (!isMinFace && pen < minPen)) {
-                    minPen = pen; isMinFace1 = false; isMinFace = false;
-                    minE1 = i; minE2 = j;
-                    s1a = e1a; s1b = e1b; s2a = e2a; s2b = e2b;
-                    minAxis = axis;
-                }
-            }
-        }
-    }
-    if (isMinFace) {
-        clipFace = minFace;
-        clipRef1 = isMinFace1;
-        clipFinal = true;
-        goto clip_face;
-    }
-    V3 cp1, cp2;
-    closest_points_between_segments(s1a, s1b, s2a, s2b, cp1, cp2);
-    const V3 p1local = tapply(t21, cp1);
-    const V3 nw = qrot(T2.q, minAxis);
-    raw.add(nw, minPen, p1local, cp2);
-    lf = lf_set_edges(lf, minE1, minE2);
-    return true;
-}
 
 // ------------------------------------------------------------------------------------------------
 // Sphere / capsule closed forms
@@ -950,25 +825,6 @@ __global__ void np_capsule_capsule(NpArgs A, const int* list, const int* count) 
     }
 }
 
-__global__ __launch_bounds__(128) void np_poly_poly(NpArgs A, const int* list, const int* count) {
-    RP3DG_GRID_STRIDE(t, *count) {
-        const int k = list[t];
-        const int2 c = A.P.col[k];
-        const Tf T1 = tf(ld3(A.C.wpos, c.x), ldq(A.C.wq, c.x));
-        const Tf T2 = tf(ld3(A.C.wpos, c.y), ldq(A.C.wq, c.y));
-        const Poly P1 = make_poly(A, c.x);
-        const Poly P2 = make_poly(A, c.y);
-        Raw raw; raw.n = 0;
-        uint32_t lf = A.P.lf[k];
-        const bool hit = sat_poly_poly<false>(P1, P2, T1, T2, lf, raw);
-        // ConvexPolyhedronVsConvexPolyhedronAlgorithm.cpp:55-62
-        lf |= LF_WAS_SAT;
-        lf &= ~LF_WAS_GJK;
-        A.P.lf[k] = lf_after_test(lf, hit);
-        contact_epilogue(A, k, hit, raw, c.x);
-    }
-}
-
 // Box-box pairs (the same SAT test with the box topology known at compile time and box-sized buffers)
 __device__ __forceinline__ Poly box_poly(const NpArgs& A, int c) {
     Poly P{};
@@ -978,13 +834,14 @@ __device__ __forceinline__ Poly box_poly(const NpArgs& A, int c) {
     return P;
 }
 
-// Box-box test in three stages so that the costly search over the 144 edge pairs can be shared by a
+// The polyhedron test in three stages so that the costly search over the edge pairs can be shared by a
 // wavefront.  Stage 1 (one lane per pair): the cached-axis shortcut and both face searches of
-// sat_poly_poly.  Returns 0 / 1 when the test is decided (no contact / contact), 2 when the edge pairs
-// must be searched; minPen0, minFace and isMinFace1 then hold the face result (SATAlgorithm.cpp:485-640).
-__device__ __forceinline__ int box_sat_begin(const Poly& P1, const Poly& P2, const Tf& T1, const Tf& T2, const Tf& t12,
-                                             const Tf& t21, uint32_t& lf, RawBoxView& raw, float& minPen0,
-                                             int& minFace, bool& isMinFace1) {
+// testCollisionConvexPolyhedronVsConvexPolyhedron.  Returns 0 / 1 when the test is decided (no contact / contact), 2 when the edge pairs
+// must be searched; minPen0, minFace and isMinFace1 then hold the face result (SATAlgorithm.cpp:485-747).
+template <bool BOX, class R>
+__device__ __forceinline__ int sat_begin(const Poly& P1, const Poly& P2, const Tf& T1, const Tf& T2, const Tf& t12,
+                                         const Tf& t21, uint32_t& lf, R& raw, float& minPen0, int& minFace,
+                                         bool& isMinFace1) {
     const bool wasColliding = (lf & LF_WAS_COLLIDING) != 0;
     if ((lf & LF_VALID) && (lf & LF_WAS_SAT)) {
         if (lf & (LF_SAT_FACE1 | LF_SAT_FACE2)) {
@@ -993,7 +850,7 @@ __device__ __forceinline__ int box_sat_begin(const Poly& P1, const Poly& P2, con
             const float pen = ref1 ? sat_single_face(P1, P2, t12, f) : sat_single_face(P2, P1, t21, f);
             if (!wasColliding && pen <= 0.0f) return 0;
             if (wasColliding && pen > 0.0f) {
-                if (face_contacts<true>(ref1, P1, P2, t12, t21, f, T1, T2, raw)) {
+                if (face_contacts<BOX>(ref1, P1, P2, t12, t21, f, T1, T2, raw)) {
                     lf = lf_set_face(lf, ref1, f);
                     return 1;
                 }
@@ -1040,12 +897,16 @@ __device__ __forceinline__ int box_sat_begin(const Poly& P1, const Poly& P2, con
     return 2;
 }
 
-// Edge pair k of two boxes in the order of the reference's double loop over half-edges i, j = 0, 2, ..,
-// 22 (k = (i / 2) * 12 + j / 2); false when the edges do not build a face of the Minkowski difference.
-constexpr int kBoxEdgePairs = 144;
-__device__ __forceinline__ bool box_edge_pair(const Poly& P1, const Poly& P2, const Tf& t12, int k, float& pen,
+// Edge pair k in the order of the reference's double loop over the half-edges i = 0, 2, .. of shape 1
+// and j = 0, 2, .. of shape 2 (k = (i / 2) * h2 + j / 2, h2 = half-edges of shape 2 / 2); false when the
+// edges do not build a face of the Minkowski difference.
+__device__ __forceinline__ int sat_edge_pairs(const Poly& P1, const Poly& P2, int& h2) {
+    h2 = (P2.nhalfedges() + 1) / 2;
+    return ((P1.nhalfedges() + 1) / 2) * h2;
+}
+__device__ __forceinline__ bool sat_edge_pair(const Poly& P1, const Poly& P2, const Tf& t12, int k, int h2, float& pen,
                                               V3* axis_out = nullptr) {
-    const int i = 2 * (k / 12), j = 2 * (k % 12);
+    const int i = 2 * (k / h2), j = 2 * (k % h2);
     const int4 e1 = P1.edge(i);
     const int4 e2 = P2.edge(j);
     if (!sat_minkowski_face(P1, e1, P2, e2, t12)) return false;
@@ -1061,7 +922,7 @@ __device__ __forceinline__ bool box_edge_pair(const Poly& P1, const Poly& P2, co
 
 // The reference walks the edge pairs in order and returns at the first separating pair; otherwise an
 // edge pair replaces the face result when pen * kSatRel + kSatAbs < the face penetration, and later
-// pairs replace it when strictly smaller (SATAlgorithm.cpp:587-631).  Since rounding is monotonic, the
+// pairs replace it when strictly smaller (SATAlgorithm.cpp:749-815).  Since rounding is monotonic, the
 // pair it ends with is the first pair of smallest penetration, provided that this one passes the face
 // test; so the walk reduces to (first separating k, first k of the smallest positive penetration),
 // which lanes can compute over disjoint subsets of k and combine.
@@ -1081,19 +942,22 @@ struct EdgeSearch {
     }
 };
 
-// Stage 3: the decision of sat_poly_poly after the edge search, then the contacts.
-__device__ __forceinline__ bool box_sat_finish(const Poly& P1, const Poly& P2, const Tf& T1, const Tf& T2, const Tf& t12,
-                                               const Tf& t21, uint32_t& lf, RawBoxView& raw, float minPen0,
-                                               int minFace, bool isMinFace1, const EdgeSearch& es) {
+// Stage 3: the choice between the face and the edge result (SATAlgorithm.cpp:828-880), then the contacts.
+template <bool BOX, class R>
+__device__ __forceinline__ bool sat_finish(const Poly& P1, const Poly& P2, const Tf& T1, const Tf& T2, const Tf& t12,
+                                           const Tf& t21, uint32_t& lf, R& raw, float minPen0, int minFace,
+                                           bool isMinFace1, const EdgeSearch& es) {
+    int h2;
+    sat_edge_pairs(P1, P2, h2);
     if (es.sep != INT_MAX) {
-        lf = lf_set_edges(lf, 2 * (es.sep / 12), 2 * (es.sep % 12));
+        lf = lf_set_edges(lf, 2 * (es.sep / h2), 2 * (es.sep % h2));
         return false;
     }
     if (es.k != INT_MAX && es.pen * kSatRel + kSatAbs < minPen0) {
-        const int i = 2 * (es.k / 12), j = 2 * (es.k % 12);
+        const int i = 2 * (es.k / h2), j = 2 * (es.k % h2);
         float pen;
         V3 axis;
-        box_edge_pair(P1, P2, t12, es.k, pen, &axis);
+        sat_edge_pair(P1, P2, t12, es.k, h2, pen, &axis);
         const int4 e1 = P1.edge(i);
         const int4 e2 = P2.edge(j);
         const V3 s1a = tapply(t12, P1.vertex(e1.x));
@@ -1106,18 +970,17 @@ __device__ __forceinline__ bool box_sat_finish(const Poly& P1, const Poly& P2, c
         lf = lf_set_edges(lf, i, j);
         return true;
     }
-    const bool found = face_contacts<true>(isMinFace1, P1, P2, t12, t21, minFace, T1, T2, raw);
+    const bool found = face_contacts<BOX>(isMinFace1, P1, P2, t12, t21, minFace, T1, T2, raw);
     lf = lf_set_face(lf, isMinFace1, minFace);
     return found;
 }
 
-// Box-box pairs (the SAT test with the box topology known at compile time and box-sized buffers).  The
-// edge search of a pair that needs one is shared by the whole wavefront (each lane takes every 64th
-// pair and the results are combined across lanes) while at most kBoxCoopMax lanes need it; beyond that
-// every such lane walks its own 144 pairs.  Both give the same result.
-constexpr int kBoxCoopMax = 32;
-__global__ __launch_bounds__(kNpBoxThreads) void np_box_box(NpArgs A, const int* list, const int* count) {
-    const int n = *count;
+// Polyhedron pairs, one lane per pair.  The edge search of a pair that needs one is shared by the whole
+// wavefront (each lane takes every 64th edge pair and the results are combined across lanes) while at
+// most coop_max lanes need it; beyond that every such lane walks its own pairs.  Both give the same
+// result.
+template <bool BOX, class R>
+__device__ __forceinline__ void sat_pair_list(const NpArgs& A, const int* list, int n, int coop_max) {
     const int lane = lane_id();
     for (int base = blockIdx.x * blockDim.x; base < n; base += gridDim.x * blockDim.x) {   // uniform in the block
         const int t = base + threadIdx.x;
@@ -1126,7 +989,7 @@ __global__ __launch_bounds__(kNpBoxThreads) void np_box_box(NpArgs A, const int*
         int2 c = make_int2(0, 0);
         Poly P1{}, P2{};
         Tf T1{}, T2{}, t12{}, t21{};
-        RawBoxView raw;
+        R raw;
         raw.n = 0;
         uint32_t lf = 0;
         int stage = 0;
@@ -1138,21 +1001,23 @@ __global__ __launch_bounds__(kNpBoxThreads) void np_box_box(NpArgs A, const int*
             c = A.P.col[k];
             T1 = tf(ld3(A.C.wpos, c.x), ldq(A.C.wq, c.x));
             T2 = tf(ld3(A.C.wpos, c.y), ldq(A.C.wq, c.y));
-            P1 = box_poly(A, c.x);
-            P2 = box_poly(A, c.y);
+            P1 = BOX ? box_poly(A, c.x) : make_poly(A, c.x);
+            P2 = BOX ? box_poly(A, c.y) : make_poly(A, c.y);
             lf = A.P.lf[k];
             t12 = tmul(tinv(T2), T1);
             t21 = tinv(t12);
-            stage = box_sat_begin(P1, P2, T1, T2, t12, t21, lf, raw, minPen0, minFace, isMinFace1);
+            stage = sat_begin<BOX>(P1, P2, T1, T2, t12, t21, lf, raw, minPen0, minFace, isMinFace1);
         }
         EdgeSearch es;
         es.init();
         uint64_t m = __ballot(stage == 2);
-        if (__popcll(m) > kBoxCoopMax) {
+        if (__popcll(m) > coop_max) {
             if (stage == 2) {
-                for (int kk = 0; kk < kBoxEdgePairs; kk++) {
+                int h2;
+                const int np = sat_edge_pairs(P1, P2, h2);
+                for (int kk = 0; kk < np; kk++) {
                     float p = 0.0f;
-                    const bool v = box_edge_pair(P1, P2, t12, kk, p);
+                    const bool v = sat_edge_pair(P1, P2, t12, kk, h2, p);
                     es.visit(kk, v, p);
                     if (es.sep != INT_MAX) break;
                 }
@@ -1165,16 +1030,16 @@ __global__ __launch_bounds__(kNpBoxThreads) void np_box_box(NpArgs A, const int*
                 tb.p = v3(__shfl(t12.p.x, L), __shfl(t12.p.y, L), __shfl(t12.p.z, L));
                 tb.q.x = __shfl(t12.q.x, L); tb.q.y = __shfl(t12.q.y, L);
                 tb.q.z = __shfl(t12.q.z, L); tb.q.w = __shfl(t12.q.w, L);
-                Poly B1{}, B2{};
-                B1.hull = -1;
-                B2.hull = -1;
-                B1.he = v3(__shfl(P1.he.x, L), __shfl(P1.he.y, L), __shfl(P1.he.z, L));
-                B2.he = v3(__shfl(P2.he.x, L), __shfl(P2.he.y, L), __shfl(P2.he.z, L));
+                const int cx = __shfl(c.x, L), cy = __shfl(c.y, L);
+                const Poly B1 = BOX ? box_poly(A, cx) : make_poly(A, cx);
+                const Poly B2 = BOX ? box_poly(A, cy) : make_poly(A, cy);
+                int h2;
+                const int np = sat_edge_pairs(B1, B2, h2);
                 EdgeSearch w;
                 w.init();
-                for (int kk = lane; kk < kBoxEdgePairs; kk += kWave) {
+                for (int kk = lane; kk < np; kk += kWave) {
                     float p = 0.0f;
-                    const bool v = box_edge_pair(B1, B2, tb, kk, p);
+                    const bool v = sat_edge_pair(B1, B2, tb, kk, h2, p);
                     w.visit(kk, v, p);
                 }
                 for (int off = kWave / 2; off > 0; off >>= 1) {
@@ -1189,7 +1054,8 @@ __global__ __launch_bounds__(kNpBoxThreads) void np_box_box(NpArgs A, const int*
         }
         if (act) {
             bool hit = stage == 1;
-            if (stage == 2) hit = box_sat_finish(P1, P2, T1, T2, t12, t21, lf, raw, minPen0, minFace, isMinFace1, es);
+            if (stage == 2)
+                hit = sat_finish<BOX>(P1, P2, T1, T2, t12, t21, lf, raw, minPen0, minFace, isMinFace1, es);
             // ConvexPolyhedronVsConvexPolyhedronAlgorithm.cpp:55-62
             lf |= LF_WAS_SAT;
             lf &= ~LF_WAS_GJK;
@@ -1197,6 +1063,18 @@ __global__ __launch_bounds__(kNpBoxThreads) void np_box_box(NpArgs A, const int*
             contact_epilogue(A, k, hit, raw, c.x);
         }
     }
+}
+
+// Pairs with a convex mesh (the general buffers); their edge searches run over hundreds of pairs, so
+// they are always shared.
+__global__ __launch_bounds__(128) void np_poly_poly(NpArgs A, const int* list, const int* count) {
+    sat_pair_list<false, Raw>(A, list, *count, kWave);
+}
+
+// Box-box pairs: the box topology is known at compile time and the buffers are box-sized.  A lane's own
+// walk over the 144 edge pairs costs about as much as 32 shared searches, hence the limit.
+__global__ __launch_bounds__(kNpBoxThreads) void np_box_box(NpArgs A, const int* list, const int* count) {
+    sat_pair_list<true, RawBoxView>(A, list, *count, 32);
 }
 
 void upload_box_topology(const int* edges4, const int* face_edge, const int* face_verts) {
