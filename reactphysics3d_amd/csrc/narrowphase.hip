@@ -838,7 +838,8 @@ __device__ __forceinline__ Poly box_poly(const NpArgs& A, int c) {
 // wavefront.  Stage 1 (one lane per pair): the cached-axis shortcut and both face searches of
 // testCollisionConvexPolyhedronVsConvexPolyhedron.  Returns 0 / 1 when the test is decided (no contact / contact), 2 when the edge pairs
 // must be searched; minPen0, minFace and isMinFace1 then hold the face result (SATAlgorithm.cpp:485-747).
-template <bool BOX, class R>
+// With FACES false it returns 3 instead of running the face searches (the caller shares them).
+template <bool BOX, bool FACES, class R>
 __device__ __forceinline__ int sat_begin(const Poly& P1, const Poly& P2, const Tf& T1, const Tf& T2, const Tf& t12,
                                          const Tf& t21, uint32_t& lf, R& raw, float& minPen0, int& minFace,
                                          bool& isMinFace1) {
@@ -886,6 +887,7 @@ __device__ __forceinline__ int sat_begin(const Poly& P1, const Poly& P2, const T
             }
         }
     }
+    if constexpr (!FACES) return 3;
     int face1 = 0, face2 = 0;
     const float pen1 = sat_faces(P1, P2, t12, face1);
     if (pen1 <= 0.0f) { lf = lf_set_face(lf, true, face1); return 0; }
@@ -930,7 +932,7 @@ struct EdgeSearch {
     int sep;     // first separating pair, INT_MAX when none
     int k;       // first pair of the smallest penetration among the others, INT_MAX when none
     float pen;   // its penetration (+inf when none)
-    __device__ void init() { sep = INT_MAX; k = INT_MAX; pen = __builtin_huge_valf(); }
+    __device__ void init(float start = __builtin_huge_valf()) { sep = INT_MAX; k = INT_MAX; pen = start; }
     __device__ void visit(int kk, bool valid, float p) {   // in increasing kk within one lane
         if (!valid) return;
         if (p <= 0.0f) { if (kk < sep) sep = kk; return; }
@@ -975,6 +977,50 @@ __device__ __forceinline__ bool sat_finish(const Poly& P1, const Poly& P2, const
     return found;
 }
 
+// One search shared by the wavefront for every lane in m (wave-uniform): the lane's colliders (ca, cb)
+// and transform t are broadcast, each lane evaluates every 64th candidate kk < count(B1, B2) with
+// eval(B1, B2, tb, kk, pen) -> valid, and the combined result lands in es of the lane it belongs to.
+template <bool BOX, class Count, class Eval>
+__device__ __forceinline__ void shared_search(const NpArgs& A, uint64_t m, int lane, const Tf& t, int ca, int cb,
+                                              float start, EdgeSearch& es, Count count, Eval eval) {
+    while (m) {
+        const int L = __ffsll((unsigned long long)m) - 1;
+        m &= m - 1;
+        Tf tb;
+        tb.p = v3(__shfl(t.p.x, L), __shfl(t.p.y, L), __shfl(t.p.z, L));
+        tb.q.x = __shfl(t.q.x, L); tb.q.y = __shfl(t.q.y, L);
+        tb.q.z = __shfl(t.q.z, L); tb.q.w = __shfl(t.q.w, L);
+        const int cx = __shfl(ca, L), cy = __shfl(cb, L);
+        const Poly B1 = BOX ? box_poly(A, cx) : make_poly(A, cx);
+        const Poly B2 = BOX ? box_poly(A, cy) : make_poly(A, cy);
+        const int np = count(B1, B2);
+        EdgeSearch w;
+        w.init(start);
+        for (int kk = lane; kk < np; kk += kWave) {
+            float p = 0.0f;
+            const bool v = eval(B1, B2, tb, kk, p);
+            w.visit(kk, v, p);
+        }
+        for (int off = kWave / 2; off > 0; off >>= 1) {
+            EdgeSearch o;
+            o.sep = __shfl_xor(w.sep, off);
+            o.k = __shfl_xor(w.k, off);
+            o.pen = __shfl_xor(w.pen, off);
+            w.combine(o);
+        }
+        if (lane == L) es = w;
+    }
+}
+
+// sat_faces (SATAlgorithm.cpp:1119-1143) as a shared search: the first face with pen <= 0, else the
+// first face of the smallest penetration below kDecimalLargest (face 0 and kDecimalLargest when none).
+__device__ __forceinline__ float face_search_result(const EdgeSearch& es, int& face) {
+    if (es.sep != INT_MAX) { face = es.sep; return 0.0f; }
+    if (es.k == INT_MAX) { face = 0; return kDecimalLargest; }
+    face = es.k;
+    return es.pen;
+}
+
 // Polyhedron pairs, one lane per pair.  The edge search of a pair that needs one is shared by the whole
 // wavefront (each lane takes every 64th edge pair and the results are combined across lanes) while at
 // most coop_max lanes need it; beyond that every such lane walks its own pairs.  Both give the same
@@ -1006,7 +1052,42 @@ __device__ __forceinline__ void sat_pair_list(const NpArgs& A, const int* list, 
             lf = A.P.lf[k];
             t12 = tmul(tinv(T2), T1);
             t21 = tinv(t12);
-            stage = sat_begin<BOX>(P1, P2, T1, T2, t12, t21, lf, raw, minPen0, minFace, isMinFace1);
+            stage = sat_begin<BOX, BOX>(P1, P2, T1, T2, t12, t21, lf, raw, minPen0, minFace, isMinFace1);
+        }
+        if constexpr (!BOX) {
+            // mesh pairs share the two face searches as well (each face's support point walks every
+            // vertex of the other mesh)
+            auto nfaces = [](const Poly& B1, const Poly&) { return B1.nfaces(); };
+            auto face_pen = [](const Poly& B1, const Poly& B2, const Tf& tb, int f, float& p) {
+                p = sat_single_face(B1, B2, tb, f);
+                return true;
+            };
+            EdgeSearch fs;
+            fs.init(kDecimalLargest);
+            shared_search<BOX>(A, __ballot(stage == 3), lane, t12, c.x, c.y, kDecimalLargest, fs, nfaces, face_pen);
+            int face1 = 0, face2 = 0;
+            float pen1 = 0.0f;
+            if (stage == 3) {
+                pen1 = face_search_result(fs, face1);
+                if (fs.sep != INT_MAX) {
+                    lf = lf_set_face(lf, true, face1);
+                    stage = 0;
+                }
+            }
+            fs.init(kDecimalLargest);
+            shared_search<BOX>(A, __ballot(stage == 3), lane, t21, c.y, c.x, kDecimalLargest, fs, nfaces, face_pen);
+            if (stage == 3) {
+                const float pen2 = face_search_result(fs, face2);
+                if (fs.sep != INT_MAX) {
+                    lf = lf_set_face(lf, false, face2);
+                    stage = 0;
+                } else {
+                    minPen0 = fmin_s(pen1, pen2);
+                    isMinFace1 = pen1 < pen2 * kSatRel + kSatAbs;
+                    minFace = isMinFace1 ? face1 : face2;
+                    stage = 2;
+                }
+            }
         }
         EdgeSearch es;
         es.init();
@@ -1023,34 +1104,16 @@ __device__ __forceinline__ void sat_pair_list(const NpArgs& A, const int* list, 
                 }
             }
         } else {
-            while (m) {   // wave-uniform
-                const int L = __ffsll((unsigned long long)m) - 1;
-                m &= m - 1;
-                Tf tb;
-                tb.p = v3(__shfl(t12.p.x, L), __shfl(t12.p.y, L), __shfl(t12.p.z, L));
-                tb.q.x = __shfl(t12.q.x, L); tb.q.y = __shfl(t12.q.y, L);
-                tb.q.z = __shfl(t12.q.z, L); tb.q.w = __shfl(t12.q.w, L);
-                const int cx = __shfl(c.x, L), cy = __shfl(c.y, L);
-                const Poly B1 = BOX ? box_poly(A, cx) : make_poly(A, cx);
-                const Poly B2 = BOX ? box_poly(A, cy) : make_poly(A, cy);
+            auto npairs = [](const Poly& B1, const Poly& B2) {
                 int h2;
-                const int np = sat_edge_pairs(B1, B2, h2);
-                EdgeSearch w;
-                w.init();
-                for (int kk = lane; kk < np; kk += kWave) {
-                    float p = 0.0f;
-                    const bool v = sat_edge_pair(B1, B2, tb, kk, h2, p);
-                    w.visit(kk, v, p);
-                }
-                for (int off = kWave / 2; off > 0; off >>= 1) {
-                    EdgeSearch o;
-                    o.sep = __shfl_xor(w.sep, off);
-                    o.k = __shfl_xor(w.k, off);
-                    o.pen = __shfl_xor(w.pen, off);
-                    w.combine(o);
-                }
-                if (lane == L) es = w;
-            }
+                return sat_edge_pairs(B1, B2, h2);
+            };
+            auto edge_pen = [](const Poly& B1, const Poly& B2, const Tf& tb, int kk, float& p) {
+                int h2;
+                sat_edge_pairs(B1, B2, h2);
+                return sat_edge_pair(B1, B2, tb, kk, h2, p);
+            };
+            shared_search<BOX>(A, m, lane, t12, c.x, c.y, __builtin_huge_valf(), es, npairs, edge_pen);
         }
         if (act) {
             bool hit = stage == 1;
