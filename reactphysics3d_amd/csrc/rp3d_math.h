@@ -28,6 +28,19 @@ constexpr float kPi = 3.141592653589f;                          // configuration
 // method instead: same constants (as the float values the decimal literals round to), same operation
 // order.  Compiled with -ffp-contract=off and IEEE division / sqrt it returns glibc's bits (checked on
 // every float in [-1, 1] for acos and on 6e8 arguments for atan / atan2).
+//
+// ref_acosf, ref_atanf and ref_atan2f restate fdlibm's float routines, which carry this notice:
+//
+// ====================================================
+// Copyright (C) 1993 by Sun Microsystems, Inc. All rights reserved.
+//
+// Developed at SunPro, a Sun Microsystems, Inc. business.
+// Permission to use, copy, modify, and distribute this
+// software is freely granted, provided that this notice
+// is preserved.
+// ====================================================
+//
+// (float conversion of e_acosf.c, s_atanf.c and e_atan2f.c by Ian Lance Taylor, Cygnus Support.)
 RD float fbits_as(int32_t i) { return __builtin_bit_cast(float, i); }
 RD int32_t fbits_of(float f) { return __builtin_bit_cast(int32_t, f); }
 

@@ -1,5 +1,7 @@
-// The reference's helloworld (helloworld/Main.cpp) written against the C++ mirror of its API:
-// a box and a sphere dropped on a static ground, stepped with PhysicsWorld::update on the GPU.
+// A helloworld-style program written against the C++ mirror of the reference's API (the reference's
+// own helloworld/Main.cpp steps one body with no collider; this one gives its bodies colliders so that
+// the step has contacts): a box and a sphere dropped on a static ground, stepped with
+// PhysicsWorld::update on the GPU.
 // Usage: hello_world [steps]   (steps = 0 builds the world and prints the creation state only)
 #include <cstdio>
 #include <cstdlib>
