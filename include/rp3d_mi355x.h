@@ -2,7 +2,7 @@
  * rp3d_mi355x — C ABI of the MI355X-native PhysicsWorld::update() step path.
  *
  * The ABI is the drop-in boundary: the C++ facade in
- * include/reactphysics3d_amd/reactphysics3d.h (namespace reactphysics3d,
+ * include/reactphysics3d_mi355x.hpp (namespace reactphysics3d,
  * PhysicsCommon / PhysicsWorld / RigidBody / Collider / shapes / joints) and
  * the Python tests and bench call only these functions.  Plain pointers,
  * sizes and POD structs cross it; no C++ or torch types.
@@ -299,6 +299,10 @@ int rp3dg_batch_set_residual_mode(rp3dg_batch* batch, float residual_threshold, 
 int rp3dg_batch_get_iteration_stats(rp3dg_batch* batch, double* mean_iterations, double* max_iterations,
                                     double* max_residual);
 
+/* Velocity iterations each world ran in the last step and its last residual (0 in fixed-iteration
+ * mode), world order; capacity >= number of worlds.  Returns the number of worlds. */
+int rp3dg_batch_get_world_iterations(rp3dg_batch* batch, int32_t* iterations, float* residuals, int64_t capacity);
+
 /* ---- learned warm start (no counterpart in the reference) ----
  * A 24 -> 128 -> 128 -> 3 SiLU network, BF16 on the matrix cores with FP32 accumulation, predicts the
  * normal impulse of every contact point that has no cached impulse (the reference starts those from 0,
@@ -339,6 +343,13 @@ int rp3dg_batch_set_warm_start(rp3dg_batch* batch, int32_t mode, rp3dg_mlp* mlp)
 int rp3dg_batch_set_logging(rp3dg_batch* batch, int64_t capacity);
 /* Copies up to capacity logged samples; returns how many were copied. */
 int64_t rp3dg_batch_get_log(rp3dg_batch* batch, float* features, float* targets, int64_t capacity);
+
+/* The learned warm start predictions of the last step, one record of 12 floats per manifold in solver
+ * order (the order of rp3dg_batch_get_contacts): 4 point slots x {normal impulse, friction 1, friction 2}
+ * as the network predicted them for the points that had no cached impulse (other slots hold stale
+ * values of earlier steps; all are 0 when the warm start is plain).  Returns the manifold count (only
+ * the count when pred is NULL). */
+int64_t rp3dg_batch_get_warm_start_predictions(rp3dg_batch* batch, float* pred, int64_t capacity);
 
 const char* rp3dg_last_error(void);
 const char* rp3dg_version(void);

@@ -54,14 +54,17 @@ def cpu_bench(sc, threads, warmup, steps, dt=1.0 / 60.0):
     L = load()
     L.rp3dref_bench.restype = C.c_int32
     L.rp3dref_bench.argtypes = [C.c_void_p, C.c_int32, C.c_void_p, C.c_int32, C.c_void_p, C.c_int32, C.c_void_p,
-                                C.c_int32, C.c_int32, C.c_int32, C.c_float, C.POINTER(C.c_double)]
+                                C.POINTER(C.c_float), C.POINTER(C.c_int32), C.c_int32, C.c_int32, C.c_int32, C.c_float,
+                                C.POINTER(C.c_double)]
     from reactphysics3d_amd.scene import JOINT_DTYPE
     s = np.array(sc.settings)
     b = sc.body_array()
     c = sc.collider_array()
     j = sc.joint_array() if sc.joints else np.zeros(1, dtype=JOINT_DTYPE)
+    hv, hi = (sc.hull_arrays() if getattr(sc, "hulls", None) else (None, None))
     out = (C.c_double * 2)()
     rc = L.rp3dref_bench(s.ctypes.data, len(b), b.ctypes.data, len(c), c.ctypes.data, len(sc.joints), j.ctypes.data,
+                         _fp(hv) if hv is not None else None, _ip(hi) if hi is not None else None,
                          threads, warmup, steps, float(dt), out)
     if rc != 0:
         raise RuntimeError("rp3dref_bench failed")
@@ -176,6 +179,30 @@ class RefWorld:
         if rc != 0:
             raise RuntimeError("coloured-order step: a reference manifold is missing from the given order")
 
+    def step_ordered_mode(self, dt, order_c1, order_c2, residual=0.0, max_iterations=0, pred=None):
+        """Ordered step with the residual stopping rule (residual > 0, at most max_iterations sweeps)
+        and / or the learned warm start (pred: [manifolds, 12] in the given order).  Returns
+        (sweeps run, last residual)."""
+        o1 = np.ascontiguousarray(order_c1, np.int32)
+        o2 = np.ascontiguousarray(order_c2, np.int32)
+        pr = None if pred is None else np.ascontiguousarray(np.asarray(pred, np.float32).reshape(-1, 12))
+        it = C.c_int32()
+        res = C.c_float()
+        fp, ip = C.POINTER(C.c_float), C.POINTER(C.c_int32)
+        self.L.rp3dref_step_ordered_mode.argtypes = [C.c_void_p, C.c_float, ip, ip, C.c_int32, C.c_float, C.c_int32,
+                                                     fp, ip, fp]
+        rc = self.L.rp3dref_step_ordered_mode(self.h, float(dt), _ip(o1), _ip(o2), len(o1), float(residual),
+                                              int(max_iterations), _fp(pr) if pr is not None else None,
+                                              C.byref(it), C.byref(res))
+        if rc != 0:
+            raise RuntimeError("coloured-order step: a reference manifold is missing from the given order")
+        return it.value, res.value
+
+    def set_iterations(self, velocity, position):
+        """PhysicsWorld::setNbIterationsVelocitySolver / setNbIterationsPositionSolver."""
+        self.L.rp3dref_set_iterations.argtypes = [C.c_void_p, C.c_int32, C.c_int32]
+        self.L.rp3dref_set_iterations(self.h, int(velocity), int(position))
+
     def bodies(self):
         p = np.zeros((self.nb, 3), np.float32)
         q = np.zeros((self.nb, 4), np.float32)
@@ -221,5 +248,96 @@ class RefWorld:
         n = self.L.rp3dref_get_contacts(self.h, _ip(out["colliders"]), _ip(out["nb_points"]),
                                         _fp(out["normal_depth"]), _fp(out["local1"]), _fp(out["local2"]),
                                         _fp(out["impulse"]), _fp(out["friction"]), cap)
+        assert n >= 0
+        return {k: a[:n] for k, a in out.items()}
+
+
+class RefBatch:
+    """Reference PhysicsWorlds, one per scene, stepped together in the coloured order on host threads
+    (rp3dref_batch_step_ordered) and read back concatenated in batch numbering."""
+
+    def __init__(self, scenes, threads=None):
+        self.worlds = [RefWorld(sc) for sc in scenes]
+        self.L = self.worlds[0].L
+        self.nw = len(self.worlds)
+        self.h = (C.c_void_p * self.nw)(*[w.h for w in self.worlds])
+        nb = np.array([w.nb for w in self.worlds], np.int64)
+        nc = np.array([w.nc for w in self.worlds], np.int64)
+        self.body_base = np.concatenate([[0], np.cumsum(nb)])
+        self.collider_base = np.concatenate([[0], np.cumsum(nc)]).astype(np.int32)
+        self.threads = int(threads or min(16, os.cpu_count() or 1))
+        L = self.L
+        vpp = C.POINTER(C.c_void_p)
+        fp, ip = C.POINTER(C.c_float), C.POINTER(C.c_int32)
+        lp = C.POINTER(C.c_int64)
+        L.rp3dref_batch_step_ordered.restype = C.c_int32
+        L.rp3dref_batch_step_ordered.argtypes = [vpp, C.c_int32, C.c_float, ip, ip, lp, C.c_float, C.c_int32, fp,
+                                                 ip, fp, C.c_int32]
+        L.rp3dref_batch_get_bodies.argtypes = [vpp, C.c_int32, fp, fp, fp, fp]
+        L.rp3dref_batch_get_pairs.restype = C.c_int64
+        L.rp3dref_batch_get_pairs.argtypes = [vpp, C.c_int32, ip, ip, ip, C.c_int64]
+        L.rp3dref_batch_get_contacts.restype = C.c_int64
+        L.rp3dref_batch_get_contacts.argtypes = [vpp, C.c_int32, ip, ip, ip, fp, fp, fp, fp, fp, C.c_int64]
+
+    def close(self):
+        for w in self.worlds:
+            w.close()
+
+    def set_joint_orders(self, orders):
+        for w, o in zip(self.worlds, orders):
+            w.set_joint_order(o)
+
+    def step_ordered(self, dt, c1, c2, world_of_manifold, residual=0.0, max_iterations=0, pred=None):
+        """One ordered step of every world.  c1 / c2: batch-global collider indices of the device's
+        manifolds in solve order (each world's manifolds contiguous or not: the order within a world is
+        kept); world_of_manifold: world of each.  pred: [n, 12] learned warm start predictions of those
+        manifolds, or None.  Returns (iterations [nw], last residual [nw])."""
+        wm = np.asarray(world_of_manifold, np.int64)
+        order = np.argsort(wm, kind="stable")
+        cb = self.collider_base.astype(np.int64)
+        o1 = np.ascontiguousarray(np.asarray(c1, np.int64)[order] - cb[wm[order]], np.int32)
+        o2 = np.ascontiguousarray(np.asarray(c2, np.int64)[order] - cb[wm[order]], np.int32)
+        moff = np.ascontiguousarray(np.concatenate([[0], np.cumsum(np.bincount(wm, minlength=self.nw))]), np.int64)
+        pr = None
+        if pred is not None:
+            pr = np.ascontiguousarray(np.asarray(pred, np.float32).reshape(-1, 12)[order])
+        iters = np.zeros(self.nw, np.int32)
+        res = np.zeros(self.nw, np.float32)
+        rc = self.L.rp3dref_batch_step_ordered(self.h, self.nw, float(dt), _ip(o1), _ip(o2),
+                                               moff.ctypes.data_as(C.POINTER(C.c_int64)), float(residual),
+                                               int(max_iterations), _fp(pr) if pr is not None else None, _ip(iters),
+                                               _fp(res), self.threads)
+        if rc != 0:
+            raise RuntimeError(f"coloured-order step: world {-rc - 1} has a manifold missing from the given order")
+        return iters, res
+
+    def bodies(self):
+        n = int(self.body_base[-1])
+        p = np.zeros((n, 3), np.float32)
+        q = np.zeros((n, 4), np.float32)
+        v = np.zeros((n, 3), np.float32)
+        w = np.zeros((n, 3), np.float32)
+        self.L.rp3dref_batch_get_bodies(self.h, self.nw, _fp(p), _fp(q), _fp(v), _fp(w))
+        return p, q, v, w
+
+    def pairs(self):
+        cap = 64 * int(self.collider_base[-1])
+        c1 = np.zeros(cap, np.int32)
+        c2 = np.zeros(cap, np.int32)
+        n = self.L.rp3dref_batch_get_pairs(self.h, self.nw, _ip(self.collider_base), _ip(c1), _ip(c2), cap)
+        assert n >= 0
+        return c1[:n], c2[:n]
+
+    def contacts(self):
+        cap = 16 * int(self.collider_base[-1])
+        out = {
+            "colliders": np.zeros((cap, 2), np.int32), "nb_points": np.zeros(cap, np.int32),
+            "normal_depth": np.zeros((cap, 4, 4), np.float32), "local1": np.zeros((cap, 4, 3), np.float32),
+            "local2": np.zeros((cap, 4, 3), np.float32), "impulse": np.zeros((cap, 4), np.float32),
+            "friction": np.zeros((cap, 3), np.float32),
+        }
+        n = self.L.rp3dref_batch_get_contacts(self.h, self.nw, _ip(self.collider_base), _ip(out["colliders"]),
+                                              _ip(out["nb_points"]), _fp(out["normal_depth"]), _fp(out["local1"]),
+                                              _fp(out["local2"]), _fp(out["impulse"]), _fp(out["friction"]), cap)
         assert n >= 0
         return {k: a[:n] for k, a in out.items()}

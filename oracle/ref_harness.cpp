@@ -18,6 +18,7 @@
 #include <algorithm>
 #include <atomic>
 #include <chrono>
+#include <cmath>
 #include <cstring>
 #include <thread>
 #include <map>
@@ -231,6 +232,14 @@ void rp3dref_destroy(void* h) {
     delete R;
 }
 
+// PhysicsWorld::setNbIterationsVelocitySolver / setNbIterationsPositionSolver (PhysicsWorld.cpp:663,957).
+int rp3dref_set_iterations(void* h, int vel, int pos) {
+    RefWorld* R = (RefWorld*)h;
+    R->world->setNbIterationsVelocitySolver((uint16)vel);
+    R->world->setNbIterationsPositionSolver((uint16)pos);
+    return 0;
+}
+
 // PhysicsWorld::setEventListener with a recorder; rp3dref_get_events returns the last step's events.
 int rp3dref_enable_events(void* h) {
     RefWorld* R = (RefWorld*)h;
@@ -278,7 +287,7 @@ static int apply_joint_order(RefWorld* R);
 
 // ContactSolverSystem::init (src/systems/ContactSolverSystem.cpp:66-105) with the rows of each
 // manifold permuted into the order of (order_c1[i], order_c2[i]) before warmStart().
-static int solver_init_ordered(RefWorld* R, float dt, const int32_t* oc1, const int32_t* oc2, int n) {
+static int solver_init_ordered(RefWorld* R, float dt, const int32_t* oc1, const int32_t* oc2, int n, bool warm_start) {
     PhysicsWorld& W = *R->world;
     ContactSolverSystem& CS = W.mContactSolverSystem;
     CollisionDetectionSystem& CD = W.mCollisionDetection;
@@ -325,13 +334,127 @@ static int solver_init_ordered(RefWorld* R, float dt, const int32_t* oc1, const 
         CS.mContactConstraints[c] = ms[src];
         for (int k = 0; k < ms[src].nbContacts; k++) CS.mContactPoints[wp++] = ps[first[src] + k];
     }
-    CS.warmStart();
+    if (warm_start) CS.warmStart();
     return 0;
 }
 
+// Learned warm start, the project's extension of ContactSolverSystem::warmStart
+// (src/systems/ContactSolverSystem.cpp:326-476): a point without a resting (cached) impulse starts from
+// max(pred_n, 0) instead of 0 and is applied like a resting point; a manifold with no resting point
+// starts its two friction impulses from the sums of its points' predicted friction impulses (twist 0)
+// and applies them like a projected cached impulse.  Everything else, and every operation order, is the
+// reference's.  pred: 12 floats per solver manifold (4 point slots x {normal, friction 1, friction 2}).
+static void warm_start_learned(ContactSolverSystem& CS, const float* pred) {
+    RigidBodyComponents& RB = CS.mRigidBodyComponents;
+    uint32 cpi = 0;
+    for (uint32 c = 0; c < CS.mNbContactManifolds; c++) {
+        ContactSolverSystem::ContactManifoldSolver& M = CS.mContactConstraints[c];
+        const uint32 b1 = M.rigidBodyComponentIndexBody1, b2 = M.rigidBodyComponentIndexBody2;
+        bool resting = false;
+        decimal plx = 0, ply = 0;
+        for (short int i = 0; i < M.nbContacts; i++) {
+            ContactSolverSystem::ContactPointSolver& P = CS.mContactPoints[cpi];
+            if (!P.isRestingContact) {
+                const float* pr = pred + 12 * c + 3 * i;
+                P.penetrationImpulse = pr[0] > 0.0f ? pr[0] : 0.0f;
+                plx += pr[1];
+                ply += pr[2];
+            } else {
+                resting = true;
+            }
+            const decimal lam = P.penetrationImpulse;
+            const Vector3 ip(P.normal.x * lam, P.normal.y * lam, P.normal.z * lam);
+            RB.mConstrainedLinearVelocities[b1].x -= M.massInverseBody1 * ip.x * M.linearLockAxisFactorBody1.x;
+            RB.mConstrainedLinearVelocities[b1].y -= M.massInverseBody1 * ip.y * M.linearLockAxisFactorBody1.y;
+            RB.mConstrainedLinearVelocities[b1].z -= M.massInverseBody1 * ip.z * M.linearLockAxisFactorBody1.z;
+            RB.mConstrainedAngularVelocities[b1].x -= P.i1TimesR1CrossN.x * M.angularLockAxisFactorBody1.x * lam;
+            RB.mConstrainedAngularVelocities[b1].y -= P.i1TimesR1CrossN.y * M.angularLockAxisFactorBody1.y * lam;
+            RB.mConstrainedAngularVelocities[b1].z -= P.i1TimesR1CrossN.z * M.angularLockAxisFactorBody1.z * lam;
+            RB.mConstrainedLinearVelocities[b2].x += M.massInverseBody2 * ip.x * M.linearLockAxisFactorBody2.x;
+            RB.mConstrainedLinearVelocities[b2].y += M.massInverseBody2 * ip.y * M.linearLockAxisFactorBody2.y;
+            RB.mConstrainedLinearVelocities[b2].z += M.massInverseBody2 * ip.z * M.linearLockAxisFactorBody2.z;
+            RB.mConstrainedAngularVelocities[b2].x += P.i2TimesR2CrossN.x * M.angularLockAxisFactorBody2.x * lam;
+            RB.mConstrainedAngularVelocities[b2].y += P.i2TimesR2CrossN.y * M.angularLockAxisFactorBody2.y * lam;
+            RB.mConstrainedAngularVelocities[b2].z += P.i2TimesR2CrossN.z * M.angularLockAxisFactorBody2.z * lam;
+            cpi++;
+        }
+        if (resting) {
+            const Vector3 old(M.friction1Impulse * M.oldFrictionVector1.x + M.friction2Impulse * M.oldFrictionVector2.x,
+                              M.friction1Impulse * M.oldFrictionVector1.y + M.friction2Impulse * M.oldFrictionVector2.y,
+                              M.friction1Impulse * M.oldFrictionVector1.z + M.friction2Impulse * M.oldFrictionVector2.z);
+            M.friction1Impulse = old.dot(M.frictionVector1);
+            M.friction2Impulse = old.dot(M.frictionVector2);
+        } else if (M.nbContacts > 0) {
+            M.friction1Impulse = plx;
+            M.friction2Impulse = ply;
+            M.frictionTwistImpulse = 0;
+        } else {
+            M.friction1Impulse = M.friction2Impulse = M.frictionTwistImpulse = 0;
+            continue;
+        }
+        const decimal f1 = M.friction1Impulse, f2 = M.friction2Impulse, tw = M.frictionTwistImpulse;
+        Vector3 a1(-M.r1CrossT1.x * f1, -M.r1CrossT1.y * f1, -M.r1CrossT1.z * f1);
+        Vector3 l2(M.frictionVector1.x * f1, M.frictionVector1.y * f1, M.frictionVector1.z * f1);
+        Vector3 a2(M.r2CrossT1.x * f1, M.r2CrossT1.y * f1, M.r2CrossT1.z * f1);
+        RB.mConstrainedLinearVelocities[b1] -= M.massInverseBody1 * l2 * M.linearLockAxisFactorBody1;
+        RB.mConstrainedAngularVelocities[b1] += M.angularLockAxisFactorBody1 * (M.inverseInertiaTensorBody1 * a1);
+        RB.mConstrainedLinearVelocities[b2] += M.massInverseBody2 * l2 * M.linearLockAxisFactorBody2;
+        RB.mConstrainedAngularVelocities[b2] += M.angularLockAxisFactorBody2 * (M.inverseInertiaTensorBody2 * a2);
+        a1 = Vector3(-M.r1CrossT2.x * f2, -M.r1CrossT2.y * f2, -M.r1CrossT2.z * f2);
+        l2 = Vector3(M.frictionVector2.x * f2, M.frictionVector2.y * f2, M.frictionVector2.z * f2);
+        a2 = Vector3(M.r2CrossT2.x * f2, M.r2CrossT2.y * f2, M.r2CrossT2.z * f2);
+        RB.mConstrainedLinearVelocities[b1].x -= M.massInverseBody1 * l2.x * M.linearLockAxisFactorBody1.x;
+        RB.mConstrainedLinearVelocities[b1].y -= M.massInverseBody1 * l2.y * M.linearLockAxisFactorBody1.y;
+        RB.mConstrainedLinearVelocities[b1].z -= M.massInverseBody1 * l2.z * M.linearLockAxisFactorBody1.z;
+        RB.mConstrainedAngularVelocities[b1] += M.angularLockAxisFactorBody1 * (M.inverseInertiaTensorBody1 * a1);
+        RB.mConstrainedLinearVelocities[b2].x += M.massInverseBody2 * l2.x * M.linearLockAxisFactorBody2.x;
+        RB.mConstrainedLinearVelocities[b2].y += M.massInverseBody2 * l2.y * M.linearLockAxisFactorBody2.y;
+        RB.mConstrainedLinearVelocities[b2].z += M.massInverseBody2 * l2.z * M.linearLockAxisFactorBody2.z;
+        RB.mConstrainedAngularVelocities[b2] += M.angularLockAxisFactorBody2 * (M.inverseInertiaTensorBody2 * a2);
+        a1 = Vector3(-M.normal.x * tw, -M.normal.y * tw, -M.normal.z * tw);
+        a2 = Vector3(M.normal.x * tw, M.normal.y * tw, M.normal.z * tw);
+        RB.mConstrainedAngularVelocities[b1] += M.angularLockAxisFactorBody1 * (M.inverseInertiaTensorBody1 * a1);
+        RB.mConstrainedAngularVelocities[b2] += M.angularLockAxisFactorBody2 * (M.inverseInertiaTensorBody2 * a2);
+        // the twist impulse is applied a second time, as in the reference (ContactSolverSystem.cpp:457-466)
+        RB.mConstrainedAngularVelocities[b1] -= M.angularLockAxisFactorBody1 * (M.inverseInertiaTensorBody1 * a2);
+        RB.mConstrainedAngularVelocities[b2] += M.angularLockAxisFactorBody2 * (M.inverseInertiaTensorBody2 * a2);
+    }
+}
+
+// How the velocity solver of one ordered step runs.
+struct SolveMode {
+    float res_thr = 0.0f;        // > 0: iterate until the velocity residual is below it (else fixed count)
+    int max_iters = 0;           // iteration cap in residual mode
+    const float* pred = nullptr; // learned warm start predictions (12 per manifold, in the given order)
+    int iters = 0;               // out: velocity iterations run
+    float res = 0.0f;            // out: last residual (residual mode)
+};
+
+// Velocity residual between two sweeps: max over the enabled dynamic bodies of
+// max(|v_k - v_{k-1}|_inf, |w_k - w_{k-1}|_inf).  The reference has no such rule (it runs
+// mNbVelocitySolverIterations, PhysicsWorld.cpp:334); this is the stopping rule of the residual mode.
+static float sweep_residual(RigidBodyComponents& RB, std::vector<Vector3>& pv, std::vector<Vector3>& pw) {
+    const uint32 n = RB.getNbEnabledComponents();
+    float r = 0.0f;
+    for (uint32 i = 0; i < n; i++) {
+        const Vector3 v = RB.mConstrainedLinearVelocities[i], w = RB.mConstrainedAngularVelocities[i];
+        if (RB.mBodyTypes[i] == BodyType::DYNAMIC) {
+            float d = std::fabs((float)(v.x - pv[i].x));
+            d = std::fmax(d, std::fabs((float)(v.y - pv[i].y)));
+            d = std::fmax(d, std::fabs((float)(v.z - pv[i].z)));
+            d = std::fmax(d, std::fabs((float)(w.x - pw[i].x)));
+            d = std::fmax(d, std::fabs((float)(w.y - pw[i].y)));
+            d = std::fmax(d, std::fabs((float)(w.z - pw[i].z)));
+            r = std::fmax(r, d);
+        }
+        pv[i] = v;
+        pw[i] = w;
+    }
+    return r;
+}
+
 // PhysicsWorld::update (src/engine/PhysicsWorld.cpp:239-307), phase by phase, with the ordered solver init.
-int rp3dref_step_ordered(void* h, float dt, const int32_t* oc1, const int32_t* oc2, int n) {
-    RefWorld* R = (RefWorld*)h;
+static int step_ordered_mode(RefWorld* R, float dt, const int32_t* oc1, const int32_t* oc2, int n, SolveMode* mode) {
     PhysicsWorld& W = *R->world;
     if (R->events) R->events->ev.clear();
     W.mCollisionDetection.computeCollisionDetection();
@@ -343,11 +466,32 @@ int rp3dref_step_ordered(void* h, float dt, const int32_t* oc1, const int32_t* o
     if (!R->joint_order.empty() && apply_joint_order(R) != 0) return -1;
     W.mDynamicsSystem.integrateRigidBodiesVelocities(dt);
     // solveContactsAndConstraints (PhysicsWorld.cpp:321-345)
-    if (solver_init_ordered(R, dt, oc1, oc2, n) != 0) return -1;
+    const float* pred = mode ? mode->pred : nullptr;
+    if (solver_init_ordered(R, dt, oc1, oc2, n, pred == nullptr) != 0) return -1;
+    if (pred) warm_start_learned(W.mContactSolverSystem, pred);
     W.mConstraintSolverSystem.initialize(dt);
-    for (uint32 i = 0; i < W.mNbVelocitySolverIterations; i++) {
-        W.mConstraintSolverSystem.solveVelocityConstraints();
-        W.mContactSolverSystem.solve();
+    if (mode && mode->res_thr > 0.0f) {
+        RigidBodyComponents& RB = W.mRigidBodyComponents;
+        const uint32 nb = RB.getNbEnabledComponents();
+        std::vector<Vector3> pv(nb), pw(nb);
+        for (uint32 i = 0; i < nb; i++) { pv[i] = RB.mConstrainedLinearVelocities[i]; pw[i] = RB.mConstrainedAngularVelocities[i]; }
+        int it = 0;
+        float res = 0.0f;
+        while (it < mode->max_iters) {
+            W.mConstraintSolverSystem.solveVelocityConstraints();
+            W.mContactSolverSystem.solve();
+            it++;
+            res = sweep_residual(RB, pv, pw);
+            if (res < mode->res_thr) break;
+        }
+        mode->iters = it;
+        mode->res = res;
+    } else {
+        for (uint32 i = 0; i < W.mNbVelocitySolverIterations; i++) {
+            W.mConstraintSolverSystem.solveVelocityConstraints();
+            W.mContactSolverSystem.solve();
+        }
+        if (mode) { mode->iters = (int)W.mNbVelocitySolverIterations; mode->res = 0.0f; }
     }
     W.mContactSolverSystem.storeImpulses();
     W.mContactSolverSystem.reset();
@@ -361,6 +505,64 @@ int rp3dref_step_ordered(void* h, float dt, const int32_t* oc1, const int32_t* o
     W.mProcessContactPairsOrderIslands.clear(true);
     W.mMemoryManager.resetFrameAllocator();
     R->last_dt = dt;
+    return 0;
+}
+
+int rp3dref_step_ordered(void* h, float dt, const int32_t* oc1, const int32_t* oc2, int n) {
+    return step_ordered_mode((RefWorld*)h, dt, oc1, oc2, n, nullptr);
+}
+
+// rp3dref_step_ordered with the residual stopping rule (res_thr > 0, at most max_iters sweeps) and / or
+// the learned warm start (pred != NULL); iters / res (may be NULL) receive the sweeps run and the last
+// residual.
+int rp3dref_step_ordered_mode(void* h, float dt, const int32_t* oc1, const int32_t* oc2, int n, float res_thr,
+                              int max_iters, const float* pred, int32_t* iters, float* res) {
+    SolveMode m;
+    m.res_thr = res_thr;
+    m.max_iters = max_iters;
+    m.pred = pred;
+    const int rc = step_ordered_mode((RefWorld*)h, dt, oc1, oc2, n, &m);
+    if (iters) *iters = m.iters;
+    if (res) *res = m.res;
+    return rc;
+}
+
+}  // extern "C"
+
+// Runs fn(w) for w in [0, n) on `threads` host threads (worlds are independent).
+template <class F>
+static void parallel_worlds(int n, int threads, F fn) {
+    threads = std::max(1, std::min(threads, n));
+    std::atomic<int> next(0);
+    std::vector<std::thread> pool;
+    for (int t = 0; t < threads; t++)
+        pool.emplace_back([&]() {
+            for (int w = next.fetch_add(1); w < n; w = next.fetch_add(1)) fn(w);
+        });
+    for (auto& th : pool) th.join();
+}
+
+extern "C" {
+
+// One ordered step of nw reference worlds on `threads` threads.  The manifold order of world w is
+// (oc1, oc2)[moff[w] .. moff[w + 1]) in world-local collider indices; pred (NULL: plain warm start)
+// holds 12 floats per manifold at the same offsets.  iters / res receive each world's sweeps and last
+// residual.  Returns 0, or -(1 + w) for the first world whose manifolds are missing from its order.
+int rp3dref_batch_step_ordered(void** hs, int nw, float dt, const int32_t* oc1, const int32_t* oc2,
+                               const int64_t* moff, float res_thr, int max_iters, const float* pred, int32_t* iters,
+                               float* res, int threads) {
+    std::vector<int> rc(nw, 0);
+    parallel_worlds(nw, threads, [&](int w) {
+        SolveMode m;
+        m.res_thr = res_thr;
+        m.max_iters = max_iters;
+        m.pred = pred ? pred + 12 * moff[w] : nullptr;
+        rc[w] = step_ordered_mode((RefWorld*)hs[w], dt, oc1 + moff[w], oc2 + moff[w], (int)(moff[w + 1] - moff[w]), &m);
+        if (iters) iters[w] = m.iters;
+        if (res) res[w] = m.res;
+    });
+    for (int w = 0; w < nw; w++)
+        if (rc[w] != 0) return -(1 + w);
     return 0;
 }
 
@@ -569,17 +771,68 @@ int rp3dref_get_contacts(void* h, int32_t* cols, int32_t* npts, float* nd, float
     return n;
 }
 
+// Batch readback over nw worlds, concatenated in world order; collider indices are offset by
+// collider_base[w] (the batch's global numbering).  Each returns the total count, or -(needed) when cap
+// is too small.
+int rp3dref_batch_get_bodies(void** hs, int nw, float* pos, float* q, float* v, float* w) {
+    int64_t off = 0;
+    for (int k = 0; k < nw; k++) {
+        RefWorld* R = (RefWorld*)hs[k];
+        rp3dref_get_bodies(R, pos ? pos + 3 * off : nullptr, q ? q + 4 * off : nullptr, v ? v + 3 * off : nullptr,
+                           w ? w + 3 * off : nullptr);
+        off += (int64_t)R->bodies.size();
+    }
+    return (int)off;
+}
+
+int64_t rp3dref_batch_get_pairs(void** hs, int nw, const int32_t* collider_base, int32_t* c1, int32_t* c2, int64_t cap) {
+    int64_t tot = 0;
+    for (int k = 0; k < nw; k++) {
+        auto& OP = ((RefWorld*)hs[k])->world->mCollisionDetection.mOverlappingPairs;
+        tot += (int64_t)OP.mConvexPairs.size() + (int64_t)OP.mDisabledConvexPairs.size();
+    }
+    if (tot > cap) return -tot;
+    int64_t o = 0;
+    for (int k = 0; k < nw; k++) {
+        RefWorld* R = (RefWorld*)hs[k];
+        const int n = rp3dref_get_pairs(R, c1 + o, c2 + o, (int)(cap - o));
+        for (int i = 0; i < n; i++) { c1[o + i] += collider_base[k]; c2[o + i] += collider_base[k]; }
+        o += n;
+    }
+    return o;
+}
+
+int64_t rp3dref_batch_get_contacts(void** hs, int nw, const int32_t* collider_base, int32_t* cols, int32_t* npts,
+                                   float* nd, float* l1, float* l2, float* imp, float* fric, int64_t cap) {
+    int64_t tot = 0;
+    for (int k = 0; k < nw; k++) tot += (int64_t)((RefWorld*)hs[k])->world->mCollisionDetection.mCurrentContactManifolds->size();
+    if (tot > cap) return -tot;
+    int64_t o = 0;
+    for (int k = 0; k < nw; k++) {
+        RefWorld* R = (RefWorld*)hs[k];
+        const int n = rp3dref_get_contacts(R, cols + 2 * o, npts + o, nd + 16 * o, l1 + 12 * o, l2 + 12 * o, imp + 4 * o,
+                                           fric + 3 * o, (int)(cap - o));
+        for (int i = 0; i < 2 * n; i++) cols[2 * o + i] += collider_base[k];
+        o += n;
+    }
+    return o;
+}
+
 // CPU baseline: `threads` identical reference worlds, one per thread, each stepped `warmup` untimed
 // then `steps` timed steps of PhysicsWorld::update(dt).  Worlds are created serially (the static
 // PhysicsWorld::mNbWorlds counter, src/engine/PhysicsWorld.cpp:46,108, is not atomic).  Wall time is
 // measured from a start barrier to the last join.  out[0] = seconds, out[1] = contact points summed
 // over the timed steps of all worlds.
 int rp3dref_bench(const RefSettings* s, int nb, const RefBody* bodies, int nc, const RefCollider* cols, int nj,
-                  const RefJoint* joints, int threads, int warmup, int steps, float dt, double* out) {
+                  const RefJoint* joints, const float* hull_vertices, const int32_t* hull_offsets, int threads,
+                  int warmup, int steps, float dt, double* out) {
     std::vector<RefWorld*> ws(threads, nullptr);
     for (int t = 0; t < threads; t++) {
-        ws[t] = (RefWorld*)rp3dref_create(s, nb, bodies, nc, cols, nj, joints, nullptr, nullptr);
-        if (!ws[t]) return -1;
+        ws[t] = (RefWorld*)rp3dref_create(s, nb, bodies, nc, cols, nj, joints, hull_vertices, hull_offsets);
+        if (!ws[t]) {
+            for (RefWorld* w : ws) rp3dref_destroy(w);
+            return -1;
+        }
         for (int k = 0; k < warmup; k++) ws[t]->world->update(dt);
     }
     std::atomic<int> ready(0);

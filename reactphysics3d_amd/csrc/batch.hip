@@ -1643,6 +1643,47 @@ int rp3dg_batch_get_contacts(rp3dg_batch* b, rp3dg_contact_view* o, int64_t cap)
     return nm;
 }
 
+int rp3dg_batch_get_world_iterations(rp3dg_batch* b, int32_t* iterations, float* residuals, int64_t cap) {
+    if (!b) return RP3DG_EINVAL;
+    if (cap < b->nworlds) { set_error("capacity too small"); return RP3DG_EINVAL; }
+    HIPCHECK(hipStreamSynchronize(b->stream));
+    if (!b->fused) {
+        for (int w = 0; w < b->nworlds; w++) {
+            if (iterations) iterations[w] = b->settings.vel_iters;
+            if (residuals) residuals[w] = 0.0f;
+        }
+        return b->nworlds;
+    }
+    if (iterations) HIPCHECK(hipMemcpy(iterations, b->world_iters, sizeof(int) * b->nworlds, hipMemcpyDeviceToHost));
+    if (residuals) HIPCHECK(hipMemcpy(residuals, b->world_res, sizeof(float) * b->nworlds, hipMemcpyDeviceToHost));
+    return b->nworlds;
+}
+
+int64_t rp3dg_batch_get_warm_start_predictions(rp3dg_batch* b, float* pred, int64_t cap) {
+    if (!b) return RP3DG_EINVAL;
+    if (!b->fused) { set_error("the learned warm start runs in the fused solver layout only"); return RP3DG_ESTATE; }
+    const int nm = b->last.nb_manifolds;
+    if (!pred) return nm;
+    if (cap < nm) { set_error("capacity too small"); return RP3DG_EINVAL; }
+    if (nm == 0) return 0;
+    HIPCHECK(hipStreamSynchronize(b->stream));
+    std::vector<float4> pr(4 * (size_t)nm);
+    std::vector<float4> lam(4 * (size_t)nm);
+    std::vector<int> npts(nm);
+    HIPCHECK(hipMemcpy(pr.data(), b->S.pred, sizeof(float4) * 4 * nm, hipMemcpyDeviceToHost));
+    HIPCHECK(hipMemcpy(npts.data(), b->S.npts, sizeof(int) * nm, hipMemcpyDeviceToHost));
+    for (int m = 0; m < nm; m++)
+        for (int p = 0; p < 4; p++) {
+            const bool live = b->warm_mode == RP3DG_WARM_START_LEARNED && p < npts[m];
+            const float4 v = pr[4 * (size_t)m + p];
+            float* o = pred + 12 * (size_t)m + 3 * p;
+            o[0] = live ? v.x : 0.0f;
+            o[1] = live ? v.y : 0.0f;
+            o[2] = live ? v.z : 0.0f;
+        }
+    return nm;
+}
+
 int rp3dg_batch_get_joint_order(rp3dg_batch* b, int32_t* order, int64_t cap) {
     if (!b) return RP3DG_EINVAL;
     const int n = (int)b->joint_order.size();

@@ -76,6 +76,7 @@ EXPORTED = [
     "rp3dg_mlp_set_params", "rp3dg_mlp_get_params", "rp3dg_mlp_set_normalization", "rp3dg_mlp_forward",
     "rp3dg_mlp_gradient", "rp3dg_mlp_train", "rp3dg_batch_set_warm_start", "rp3dg_batch_set_logging",
     "rp3dg_batch_get_log", "rp3dg_batch_autotune", "rp3dg_batch_get_tune_table", "rp3dg_world_get_body_state",
+    "rp3dg_batch_get_world_iterations", "rp3dg_batch_get_warm_start_predictions",
 ]
 
 MLP_PARAMS = 20099
@@ -148,6 +149,8 @@ def load():
         "rp3dg_batch_set_residual_mode": (C.c_int, [vp, f, i32]),
         "rp3dg_batch_get_iteration_stats": (C.c_int, [vp, C.POINTER(C.c_double), C.POINTER(C.c_double),
                                                       C.POINTER(C.c_double)]),
+        "rp3dg_batch_get_world_iterations": (C.c_int, [vp, iptr, fptr, i64]),
+        "rp3dg_batch_get_warm_start_predictions": (i64, [vp, fptr, i64]),
         "rp3dg_mlp_create": (vp, [C.c_uint64]),
         "rp3dg_mlp_destroy": (None, [vp]),
         "rp3dg_mlp_set_params": (C.c_int, [vp, fptr, i64]),
@@ -397,6 +400,25 @@ class Batch:
         a, b, c = C.c_double(), C.c_double(), C.c_double()
         _check(load().rp3dg_batch_get_iteration_stats(self.h, C.byref(a), C.byref(b), C.byref(c)), "iteration_stats")
         return {"mean_iterations": a.value, "max_iterations": b.value, "max_residual": c.value}
+
+    def world_iterations(self):
+        """(velocity iterations [nworlds], last residual [nworlds]) of the last step."""
+        nw = len(self.scenes)
+        it = np.zeros(nw, np.int32)
+        res = np.zeros(nw, np.float32)
+        rc = load().rp3dg_batch_get_world_iterations(self.h, _ip(it), _fp(res), nw)
+        _check(rc if rc < 0 else 0, "world_iterations")
+        return it, res
+
+    def warm_start_predictions(self):
+        """Learned warm start predictions of the last step, [manifolds, 4, 3] in solver order."""
+        L = load()
+        n = L.rp3dg_batch_get_warm_start_predictions(self.h, None, 0)
+        _check(n if n < 0 else 0, "warm_start_predictions")
+        out = np.zeros((max(n, 1), 4, 3), np.float32)
+        n = L.rp3dg_batch_get_warm_start_predictions(self.h, _fp(out), len(out))
+        _check(n if n < 0 else 0, "warm_start_predictions")
+        return out[:n]
 
     def autotune(self, dt, steps_per_trial=3):
         c = C.c_int32()

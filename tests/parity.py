@@ -41,3 +41,42 @@ def compare_contacts(dev, ref, tol=1e-5):
             if e > tol:
                 errs.append(f"pair {k}: {f} differs by {e:.3e}")
     return errs
+
+
+def _pair_keys(c1, c2):
+    return (np.asarray(c1, np.int64) << 32) | np.asarray(c2, np.int64)
+
+
+def same_pair_sets(c1, c2, r1, r2):
+    """Oriented pair sets equal (vectorised pair_set comparison)."""
+    a = np.sort(_pair_keys(c1, c2))
+    b = np.sort(_pair_keys(r1, r2))
+    return a.shape == b.shape and bool(np.array_equal(a, b))
+
+
+def compare_contacts_batch(dev, ref, tol=1e-5):
+    """compare_contacts over many manifolds at once (batch numbering on both sides)."""
+    kd = _pair_keys(dev["colliders"][:, 0], dev["colliders"][:, 1])
+    kr = _pair_keys(ref["colliders"][:, 0], ref["colliders"][:, 1])
+    od, orf = np.argsort(kd, kind="stable"), np.argsort(kr, kind="stable")
+    if kd.shape != kr.shape or not np.array_equal(kd[od], kr[orf]):
+        sd, sr = set(kd.tolist()), set(kr.tolist())
+        return [f"manifold pair sets differ: {len(sd - sr)} device-only, {len(sr - sd)} reference-only "
+                f"(e.g. {[(k >> 32, k & 0xFFFFFFFF) for k in sorted(sd ^ sr)[:4]]})"]
+    nd_, nr_ = dev["nb_points"][od], ref["nb_points"][orf]
+    errs = []
+    bad = np.nonzero(nd_ != nr_)[0]
+    if len(bad):
+        errs.append(f"{len(bad)} manifolds with different point counts, e.g. pair "
+                    f"{tuple(dev['colliders'][od[bad[0]]])}: {nd_[bad[0]]} vs {nr_[bad[0]]}")
+        return errs
+    live = np.arange(4)[None, :] < nd_[:, None]
+    for f in ("normal_depth", "local1", "local2"):
+        a = np.asarray(dev[f][od], np.float64)
+        b = np.asarray(ref[f][orf], np.float64)
+        e = np.abs(a - b) / np.maximum(np.abs(b), 1.0)
+        e = np.where(live[:, :, None], e, 0.0)
+        if e.size and float(e.max()) > tol:
+            i = int(np.argmax(e.max(axis=(1, 2))))
+            errs.append(f"{f} differs by {float(e.max()):.3e} (pair {tuple(dev['colliders'][od[i]])})")
+    return errs

@@ -282,3 +282,135 @@ def test_raycast_closed_forms(ref_available):
     assert fr[7] == pytest.approx(0.4) and np.allclose(pt[7], [40, 0, 1]) and np.allclose(nr[7], [0, 0, 1])
     col, *_ = ref.raycast(p1[6:7], p2[6:7], mask=1)   # category 2 is not in the mask
     assert col[0] == -1
+
+
+def _copies(sc, n, prefix_steps):
+    from oracle.ref import RefWorld
+    ws = [RefWorld(sc) for _ in range(n)]
+    for w in ws:
+        for _ in range(prefix_steps):
+            w.step(DT)
+    return ws
+
+
+def test_residual_rule_reduces_to_fixed_iterations(ref_available):
+    """With a threshold no sweep reaches, the residual rule runs exactly the cap and equals the reference's
+    own step with that many velocity iterations; with a threshold every sweep meets, it runs one sweep
+    and equals the reference's step with one iteration."""
+    sc = S.box_pile(3, 2, seed=23)
+    for thr, cap, fixed in ((1e-30, 17, 17), (1e30, 40, 1)):
+        a, b = _copies(sc, 2, 35)
+        a.set_iterations(fixed, 5)
+        for k in range(10):
+            a.step(DT)
+            ct = a.contacts()
+            it, _ = b.step_ordered_mode(DT, ct["colliders"][:, 0], ct["colliders"][:, 1], residual=thr,
+                                        max_iterations=cap)
+            assert it == fixed
+            for x, y in zip(a.bodies(), b.bodies()):
+                assert np.array_equal(x, y), f"threshold {thr} step {k}"
+
+
+def test_residual_rule_stops_at_first_sweep_below_threshold(ref_available):
+    """The sweep count of the residual rule is the first i whose residual r_i = max over dynamic bodies of
+    the velocity change between sweeps i - 1 and i is below the threshold.  The final body velocities of
+    the reference's own step with i and with i - 1 velocity iterations (same prior state) are v_i and
+    v_{i-1}, so r_i follows from the reference alone."""
+    sc = S.box_pile(3, 2, seed=29)
+    cap = 40
+    prefix = 38   # the pile has landed: resting and new contacts
+    ws = _copies(sc, cap + 2, prefix)
+    vels = []
+    for i, w in enumerate(ws[:cap + 1]):
+        w.set_iterations(i, 5)
+        w.step(DT)
+        _, _, v, om = w.bodies()
+        vels.append(np.concatenate([v, om], axis=1))
+    dyn = sc.body_array()["type"] == S.DYNAMIC
+    r = [float(np.max(np.abs(vels[i][dyn] - vels[i - 1][dyn]))) for i in range(1, cap + 1)]
+    ct = ws[cap].contacts()
+    for thr in (1e-2, 1e-3, 1e-4, 1e-5):
+        expect = next((i + 1 for i, x in enumerate(r) if x < thr), cap)
+        w = _copies(sc, 1, prefix)[0]
+        it, res = w.step_ordered_mode(DT, ct["colliders"][:, 0], ct["colliders"][:, 1], residual=thr,
+                                      max_iterations=cap)
+        assert it == expect, f"threshold {thr}: {it} sweeps, first below is {expect} (residuals {r[:12]})"
+        assert res == np.float32(r[it - 1])
+        for x, y in zip(w.bodies(), ws[it].bodies()):
+            assert np.array_equal(x, y)
+
+
+def test_learned_warm_start_with_zero_predictions_is_the_reference(ref_available):
+    """Zero predictions start new points and new manifolds from zero, which is the reference's own warm
+    start: the learned path of the harness reproduces the reference's step bit for bit."""
+    sc = S.box_pile(3, 2, seed=31)
+    a, b = _copies(sc, 2, 0)
+    for k in range(60):
+        a.step(DT)
+        ct = a.contacts()
+        b.step_ordered_mode(DT, ct["colliders"][:, 0], ct["colliders"][:, 1],
+                            pred=np.zeros((len(ct["nb_points"]), 12), np.float32))
+        for x, y in zip(a.bodies(), b.bodies()):
+            assert np.array_equal(x, y), f"step {k}"
+
+
+def test_learned_warm_start_injection_closed_form(ref_available):
+    """A sphere just touching a static ground, no gravity, no velocity iterations: the step's velocity is
+    the warm start alone.  A new point predicted at lambda_n pushes the sphere off the ground at
+    lambda_n / m (negative predictions clamp to 0), and a new manifold's predicted friction impulses
+    (f1, f2) along the orthonormal t1, t2 move it sideways at |(f1, f2)| / m."""
+    from oracle.ref import RefWorld
+    sc = S.Scene("inject", S.make_settings(gravity=(0.0, 0.0, 0.0), vel_iters=0, pos_iters=0))
+    g = sc.add_body(S.STATIC, (0.0, 0.0, -0.5))
+    sc.add_collider(g, S.BOX, (5.0, 5.0, 0.5))
+    s = sc.add_body(S.DYNAMIC, (0.0, 0.0, 0.49))
+    sc.add_collider(s, S.SPHERE, (0.5,), density=2.0)
+    m = 2.0 * 4.0 / 3.0 * np.pi * 0.125
+    probe = RefWorld(sc)
+    probe.step(DT)
+    ct = probe.contacts()
+    assert len(ct["nb_points"]) == 1 and ct["nb_points"][0] == 1
+    for lam, f1, f2 in ((0.3, 0.0, 0.0), (-0.2, 0.0, 0.0), (0.5, 0.12, -0.05)):
+        w = RefWorld(sc)
+        pred = np.zeros((1, 12), np.float32)
+        pred[0, :3] = (lam, f1, f2)
+        w.step_ordered_mode(DT, ct["colliders"][:, 0], ct["colliders"][:, 1], pred=pred)
+        _, _, v, _ = w.bodies()
+        assert v[1, 2] == pytest.approx(max(lam, 0.0) / m, rel=1e-5)
+        assert np.hypot(v[1, 0], v[1, 1]) == pytest.approx(np.hypot(f1, f2) / m, rel=1e-5, abs=1e-7)
+        _, _, _, _ = w.bodies()
+        c = w.contacts()
+        assert c["impulse"][0, 0] == pytest.approx(max(lam, 0.0), rel=1e-6)
+        assert c["friction"][0, :2] == pytest.approx([f1, f2], rel=1e-6, abs=1e-7)
+
+
+def test_batch_harness_steps_each_world_as_its_own_reference(ref_available):
+    """The threaded batch harness (worlds concatenated in batch numbering, manifold orders interleaved
+    across worlds) steps every world exactly as that world's own reference step."""
+    from oracle.ref import RefBatch, RefWorld
+    scenes = [S.batched_world(w, seed=40, min_bodies=8, max_bodies=40, max_joints=4) for w in range(6)]
+    own = [RefWorld(sc) for sc in scenes]
+    rb = RefBatch(scenes, threads=3)
+    rng = np.random.default_rng(0)
+    for k in range(45):
+        c1, c2, wm = [], [], []
+        for w, r in enumerate(own):
+            r.step(DT)
+            ct = r.contacts()
+            c1.append(ct["colliders"][:, 0] + rb.collider_base[w])
+            c2.append(ct["colliders"][:, 1] + rb.collider_base[w])
+            wm.append(np.full(len(ct["nb_points"]), w))
+        c1, c2, wm = np.concatenate(c1), np.concatenate(c2), np.concatenate(wm)
+        # a random interleaving of the worlds that keeps each world's own order
+        slots = rng.permutation(wm)
+        sel = np.empty(len(wm), np.int64)
+        for w in range(len(scenes)):
+            sel[slots == w] = np.nonzero(wm == w)[0]
+        it, _ = rb.step_ordered(DT, c1[sel], c2[sel], wm[sel])
+        assert np.all(it == 10)
+        mine = rb.bodies()
+        theirs = [np.concatenate([r.bodies()[i] for r in own]) for i in range(4)]
+        for x, y in zip(mine, theirs):
+            assert np.array_equal(x, y), f"step {k}"
+        p1, p2 = rb.pairs()
+        assert len(p1) == sum(len(r.pairs()[0]) for r in own)
