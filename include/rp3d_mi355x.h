@@ -122,6 +122,23 @@ int rp3dg_collider_set_filter(rp3dg_world* world, int32_t collider, uint16_t cat
 int32_t rp3dg_hull_create(rp3dg_world* world, const float* vertices, int32_t nb_vertices,
                           const int32_t* face_counts, const int32_t* face_indices, int32_t nb_faces);
 
+/* PhysicsCommon::createConvexMesh(const VertexArray&, std::vector<Message>&) (src/engine/PhysicsCommon.cpp:638-660):
+ * the convex hull of nb_vertices points, computed as the reference's QuickHull computes it
+ * (src/utils/quickhull/QuickHull.cpp:44-131: same faces, face order and vertex numbering), then built as
+ * rp3dg_hull_create builds a polygon mesh.  Vertex i is 3 floats (is_double = 0, VERTEX_FLOAT_TYPE) or
+ * 3 doubles (VERTEX_DOUBLE_TYPE) at vertices + i * stride bytes (VertexArray.h).  Returns the hull
+ * index, or RP3DG_EINVAL with the reference's message in rp3dg_last_error() (fewer than 4 distinct
+ * points, points too close to each other). */
+int32_t rp3dg_hull_create_from_points(rp3dg_world* world, const void* vertices, int32_t nb_vertices, int32_t stride,
+                                      int32_t is_double);
+/* QuickHull::computeConvexHull alone (src/utils/quickhull/QuickHull.cpp:44-131): the polygons that
+ * createConvexMesh(VertexArray) builds its mesh from.  Writes up to cap_vertices vertices (xyz),
+ * cap_faces face vertex counts and cap_indices face vertex indices; *nb_out_vertices / *nb_out_faces
+ * receive the counts even when a capacity is short (then RP3DG_EINVAL). */
+int rp3dg_quickhull(const void* vertices, int32_t nb_vertices, int32_t stride, int32_t is_double, float* out_vertices,
+                    int32_t cap_vertices, int32_t* face_counts, int32_t cap_faces, int32_t* face_indices,
+                    int32_t cap_indices, int32_t* nb_out_vertices, int32_t* nb_out_faces);
+
 /* PhysicsWorld::createJoint (src/engine/PhysicsWorld.cpp:483-600) with anchor / axis given in world space
  * (the JointInfo "world space" constructors, e.g. BallAndSocketJointInfo(body1, body2, anchor)).
  * axis is used by hinge and slider; limits[4] = {min, max, motor speed, max motor force/torque}

@@ -341,3 +341,23 @@ class RefBatch:
                                               _fp(out["local2"]), _fp(out["impulse"]), _fp(out["friction"]), cap)
         assert n >= 0
         return {k: a[:n] for k, a in out.items()}
+
+
+def quickhull(points):
+    """The reference's QuickHull::computeConvexHull of a point cloud ([n, 3] float32): (vertices [nv, 3],
+    face vertex counts [nf], concatenated face indices), or None when the reference refuses the points."""
+    L = load()
+    pts = np.ascontiguousarray(points, np.float32).reshape(-1, 3)
+    n = len(pts)
+    cap_v, cap_f, cap_i = max(4, n), max(4, 2 * n), max(12, 6 * n)
+    v = np.zeros((cap_v, 3), np.float32)
+    c = np.zeros(cap_f, np.int32)
+    i = np.zeros(cap_i, np.int32)
+    nv, nf = C.c_int32(), C.c_int32()
+    fp, ip = C.POINTER(C.c_float), C.POINTER(C.c_int32)
+    L.rp3dref_quickhull.argtypes = [fp, C.c_int32, fp, C.c_int32, ip, C.c_int32, ip, C.c_int32, ip, ip]
+    rc = L.rp3dref_quickhull(_fp(pts), n, _fp(v), cap_v, _ip(c), cap_f, _ip(i), cap_i, C.byref(nv), C.byref(nf))
+    if rc == -1:
+        return None
+    assert rc == 0
+    return v[:nv.value].copy(), c[:nf.value].copy(), i[:int(c[:nf.value].sum())].copy()

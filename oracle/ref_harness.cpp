@@ -116,7 +116,8 @@ void* rp3dref_create(const RefSettings* s, int nb, const RefBody* bodies, int nc
     ws.defaultPositionSolverNbIterations = (uint16)s->pos_iters;
     R->world = R->common.createPhysicsWorld(ws);
     // convex meshes from explicit polygons (PhysicsCommon::createConvexMesh(PolygonVertexArray)):
-    // hull_offsets = [nb hulls, then per hull: nv, nf, vertex count of each face, face vertex indices]
+    // hull_offsets = [nb hulls, then per hull: nv, nf, vertex count of each face, face vertex indices];
+    // nf = -1: the hull's nv vertices are a point cloud for createConvexMesh(VertexArray)
     std::vector<ConvexMesh*> meshes;
     if (hull_offsets && hull_vertices) {
         const int nh = hull_offsets[0];
@@ -125,6 +126,15 @@ void* rp3dref_create(const RefSettings* s, int nb, const RefBody* bodies, int nc
         for (int h = 0; h < nh; h++) {
             const int nv = hull_offsets[k++];
             const int nf = hull_offsets[k++];
+            if (nf < 0) {   // a point cloud: PhysicsCommon::createConvexMesh(VertexArray) (QuickHull)
+                VertexArray va(hull_vertices + 3 * voff, 3 * sizeof(float), (uint32)nv, VertexArray::DataType::VERTEX_FLOAT_TYPE);
+                std::vector<Message> msgs;
+                ConvexMesh* mesh = R->common.createConvexMesh(va, msgs);
+                if (!mesh) { delete R; return nullptr; }
+                meshes.push_back(mesh);
+                voff += (size_t)nv;
+                continue;
+            }
             std::vector<PolygonVertexArray::PolygonFace> faces(nf);
             uint32 base = 0;
             for (int f = 0; f < nf; f++) {
@@ -816,6 +826,35 @@ int64_t rp3dref_batch_get_contacts(void** hs, int nw, const int32_t* collider_ba
         o += n;
     }
     return o;
+}
+
+// PhysicsCommon::createConvexMesh(const VertexArray&, messages) (src/engine/PhysicsCommon.cpp:638-660, QuickHull
+// then ConvexMesh::init): the mesh's vertices and its faces' vertex lists.  points: n x 3 floats.  Returns 0
+// and the counts, -1 when the reference refuses the points, -2 when a capacity is short.
+int rp3dref_quickhull(const float* points, int n, float* verts, int cap_v, int32_t* counts, int cap_f, int32_t* idx,
+                      int cap_i, int32_t* nv, int32_t* nf) {
+    PhysicsCommon common;
+    VertexArray va(points, 3 * sizeof(float), (uint32)n, VertexArray::DataType::VERTEX_FLOAT_TYPE);
+    std::vector<Message> msgs;
+    ConvexMesh* mesh = common.createConvexMesh(va, msgs);
+    if (!mesh) return -1;
+    const HalfEdgeStructure& H = mesh->getHalfEdgeStructure();
+    *nv = (int32_t)mesh->getNbVertices();
+    *nf = (int32_t)mesh->getNbFaces();
+    int ni = 0;
+    for (uint32 f = 0; f < mesh->getNbFaces(); f++) ni += (int)H.getFace(f).faceVertices.size();
+    if (*nv > cap_v || *nf > cap_f || ni > cap_i) return -2;
+    for (uint32 i = 0; i < mesh->getNbVertices(); i++) {
+        const Vector3& p = mesh->getVertex(i);
+        verts[3 * i] = (float)p.x; verts[3 * i + 1] = (float)p.y; verts[3 * i + 2] = (float)p.z;
+    }
+    int o = 0;
+    for (uint32 f = 0; f < mesh->getNbFaces(); f++) {
+        const auto& fv = H.getFace(f).faceVertices;
+        counts[f] = (int32_t)fv.size();
+        for (uint32 k = 0; k < fv.size(); k++) idx[o++] = (int32_t)fv[k];
+    }
+    return 0;
 }
 
 // CPU baseline: `threads` identical reference worlds, one per thread, each stepped `warmup` untimed

@@ -12,7 +12,7 @@ CSRC = os.path.join(HERE, "csrc")
 LIB_DIR = os.path.join(HERE, "_lib")
 LIB = os.path.join(LIB_DIR, "librp3d_mi355x.so")
 
-SOURCES = ["world.cpp", "broadphase.hip", "narrowphase.hip", "solver.hip", "joints.hip", "fused.hip", "mlp.hip",
+SOURCES = ["world.cpp", "quickhull.cpp", "broadphase.hip", "narrowphase.hip", "solver.hip", "joints.hip", "fused.hip", "mlp.hip",
            "islands.hip", "events.hip", "raycast.hip",
            "batch.hip"]
 
@@ -34,7 +34,7 @@ def build(verbose=False, jobs=None):
         path = os.path.join(CSRC, src)
         obj = os.path.join(LIB_DIR, src + ".o")
         cmd = ["hipcc", *HIP_FLAGS, "-c", path, "-o", obj]
-        if src.endswith(".cpp"):
+        if src.endswith(".cpp"):  # host code, same FP flags
             cmd = ["hipcc", "-x", "hip", *HIP_FLAGS, "-c", path, "-o", obj]
         cmds.append(cmd)
         objs.append(obj)

@@ -7,6 +7,7 @@
 #include <utility>
 
 #include "../../include/rp3d_mi355x.h"
+#include "quickhull.h"
 
 namespace rp3dg {
 
@@ -442,6 +443,63 @@ int rp3dg_collider_set_filter(rp3dg_world* w, int32_t ci, uint16_t cat, uint16_t
 
 // ConvexMesh::init from a PolygonVertexArray (src/collision/ConvexMesh.cpp:55-264): centroid,
 // bounds, half-edge structure, Newell face normals, volume.
+// VertexArray (src/collision/VertexArray.cpp) read as QuickHull::extractPoints does: 3 floats or 3 doubles
+// per vertex at `stride` bytes, doubles converted to float.
+static bool read_points(const void* data, int32_t n, int32_t stride, int32_t is_double, std::vector<V3>& out) {
+    const int32_t need = is_double ? 3 * (int32_t)sizeof(double) : 3 * (int32_t)sizeof(float);
+    if (!data || n < 0 || stride < need) { set_error("invalid vertex array (null data, negative count or stride too small)"); return false; }
+    const unsigned char* base = static_cast<const unsigned char*>(data);
+    out.resize(n);
+    for (int32_t i = 0; i < n; i++) {
+        const unsigned char* p = base + (size_t)i * (size_t)stride;
+        if (is_double) {
+            double d[3];
+            std::memcpy(d, p, sizeof d);
+            out[i] = v3((float)d[0], (float)d[1], (float)d[2]);
+        } else {
+            float f[3];
+            std::memcpy(f, p, sizeof f);
+            out[i] = v3(f[0], f[1], f[2]);
+        }
+    }
+    return true;
+}
+
+int rp3dg_quickhull(const void* vertices, int32_t nb_vertices, int32_t stride, int32_t is_double, float* out_vertices,
+                    int32_t cap_vertices, int32_t* face_counts, int32_t cap_faces, int32_t* face_indices,
+                    int32_t cap_indices, int32_t* nb_out_vertices, int32_t* nb_out_faces) {
+    std::vector<V3> pts;
+    if (!read_points(vertices, nb_vertices, stride, is_double, pts)) return RP3DG_EINVAL;
+    std::vector<float> v;
+    std::vector<int32_t> c, idx;
+    std::string err;
+    if (!quickhull_polygons(pts, v, c, idx, &err)) { set_error(err); return RP3DG_EINVAL; }
+    const int32_t nv = (int32_t)(v.size() / 3), nf = (int32_t)c.size(), ni = (int32_t)idx.size();
+    if (nb_out_vertices) *nb_out_vertices = nv;
+    if (nb_out_faces) *nb_out_faces = nf;
+    if (nv > cap_vertices || nf > cap_faces || ni > cap_indices || !out_vertices || !face_counts || !face_indices) {
+        set_error("quickhull: output capacity too small (" + std::to_string(nv) + " vertices, " + std::to_string(nf) +
+                  " faces, " + std::to_string(ni) + " indices)");
+        return RP3DG_EINVAL;
+    }
+    std::memcpy(out_vertices, v.data(), sizeof(float) * v.size());
+    std::memcpy(face_counts, c.data(), sizeof(int32_t) * c.size());
+    std::memcpy(face_indices, idx.data(), sizeof(int32_t) * idx.size());
+    return RP3DG_OK;
+}
+
+int32_t rp3dg_hull_create_from_points(rp3dg_world* w, const void* vertices, int32_t nb_vertices, int32_t stride,
+                                      int32_t is_double) {
+    if (!w) { set_error("null world"); return RP3DG_EINVAL; }
+    std::vector<V3> pts;
+    if (!read_points(vertices, nb_vertices, stride, is_double, pts)) return RP3DG_EINVAL;
+    std::vector<float> v;
+    std::vector<int32_t> c, idx;
+    std::string err;
+    if (!quickhull_polygons(pts, v, c, idx, &err)) { set_error(err); return RP3DG_EINVAL; }
+    return rp3dg_hull_create(w, v.data(), (int32_t)(v.size() / 3), c.data(), idx.data(), (int32_t)c.size());
+}
+
 int32_t rp3dg_hull_create(rp3dg_world* w, const float* verts, int32_t nv, const int32_t* counts,
                           const int32_t* idx, int32_t nf) {
     if (!w || !verts || !counts || !idx || nv < 4 || nf < 4) { set_error("invalid hull arguments"); return RP3DG_EINVAL; }

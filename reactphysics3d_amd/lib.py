@@ -77,6 +77,7 @@ EXPORTED = [
     "rp3dg_mlp_gradient", "rp3dg_mlp_train", "rp3dg_batch_set_warm_start", "rp3dg_batch_set_logging",
     "rp3dg_batch_get_log", "rp3dg_batch_autotune", "rp3dg_batch_get_tune_table", "rp3dg_world_get_body_state",
     "rp3dg_batch_get_world_iterations", "rp3dg_batch_get_warm_start_predictions",
+    "rp3dg_hull_create_from_points", "rp3dg_quickhull",
 ]
 
 MLP_PARAMS = 20099
@@ -151,6 +152,8 @@ def load():
                                                       C.POINTER(C.c_double)]),
         "rp3dg_batch_get_world_iterations": (C.c_int, [vp, iptr, fptr, i64]),
         "rp3dg_batch_get_warm_start_predictions": (i64, [vp, fptr, i64]),
+        "rp3dg_hull_create_from_points": (i32, [vp, vp, i32, i32, i32]),
+        "rp3dg_quickhull": (C.c_int, [vp, i32, i32, i32, fptr, i32, iptr, i32, iptr, i32, iptr, iptr]),
         "rp3dg_mlp_create": (vp, [C.c_uint64]),
         "rp3dg_mlp_destroy": (None, [vp]),
         "rp3dg_mlp_set_params": (C.c_int, [vp, fptr, i64]),
@@ -211,7 +214,12 @@ def world_from_scene(sc: S.Scene):
     st.default_position_solver_iterations = int(s["pos_iters"])
     st.is_sleeping_enabled = int(s["sleeping"])
     w = L.rp3dg_world_create(C.byref(st))
-    for verts, faces in getattr(sc, "hulls", []):
+    points = getattr(sc, "hull_points", None) or [None] * len(getattr(sc, "hulls", []))
+    for (verts, faces), pts in zip(getattr(sc, "hulls", []), points):
+        if pts is not None:   # PhysicsCommon::createConvexMesh(VertexArray)
+            p = np.ascontiguousarray(pts, np.float32)
+            _check(L.rp3dg_hull_create_from_points(w, p.ctypes.data, len(p), 12, 0), "hull_create_from_points")
+            continue
         v = np.ascontiguousarray(verts, np.float32)
         counts = np.array([len(f) for f in faces], np.int32)
         idx = np.array([i for f in faces for i in f], np.int32)
@@ -242,6 +250,22 @@ def world_from_scene(sc: S.Scene):
         _check(L.rp3dg_joint_create(w, jt, b1, b2, _f3(anchor), _f3(axis), collide, flags, _f4(limits)),
                "joint_create")
     return w
+
+
+def quickhull(points):
+    """QuickHull polygons of a point cloud ([n, 3]) as PhysicsCommon::createConvexMesh(VertexArray) computes
+    them: (vertices [nv, 3] float32, face vertex counts [nf], concatenated face indices)."""
+    pts = np.ascontiguousarray(points, np.float32).reshape(-1, 3)
+    n = len(pts)
+    cap_v, cap_f, cap_i = max(4, n), max(4, 2 * n), max(12, 6 * n)
+    v = np.zeros((cap_v, 3), np.float32)
+    c = np.zeros(cap_f, np.int32)
+    i = np.zeros(cap_i, np.int32)
+    nv, nf = C.c_int32(), C.c_int32()
+    rc = load().rp3dg_quickhull(pts.ctypes.data, n, 12, 0, _fp(v), cap_v, _ip(c), cap_f, _ip(i), cap_i, C.byref(nv),
+                                C.byref(nf))
+    _check(rc, "quickhull")
+    return v[:nv.value].copy(), c[:nf.value].copy(), i[:int(c[:nf.value].sum())].copy()
 
 
 class Batch:

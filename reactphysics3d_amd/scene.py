@@ -16,8 +16,6 @@ from __future__ import annotations
 
 import math
 from dataclasses import dataclass, field
-from functools import lru_cache
-from itertools import combinations
 
 import numpy as np
 
@@ -88,6 +86,7 @@ class Scene:
     colliders: list = field(default_factory=list)
     joints: list = field(default_factory=list)
     hulls: list = field(default_factory=list)   # (vertices [V, 3] float32, faces: list of CCW index lists)
+    hull_points: list = field(default_factory=list)   # per hull: the point cloud it was built from, or None
 
     # -- builders -------------------------------------------------------------------------------
     def add_body(self, btype, pos, quat=(0.0, 0.0, 0.0, 1.0), lin=(0.0, 0.0, 0.0), ang=(0.0, 0.0, 0.0),
@@ -116,14 +115,29 @@ class Scene:
         """A convex mesh (PhysicsCommon::createConvexMesh(PolygonVertexArray)); returns its index, the
         params[0] of a HULL collider."""
         self.hulls.append((np.ascontiguousarray(vertices, np.float32).reshape(-1, 3), [list(f) for f in faces]))
+        self.hull_points.append(None)
+        return len(self.hulls) - 1
+
+    def add_hull_from_points(self, points):
+        """A convex mesh from a point cloud (PhysicsCommon::createConvexMesh(VertexArray): QuickHull); the
+        polygons are the ones the library's QuickHull computes, the reference's own."""
+        pts = np.ascontiguousarray(points, np.float32).reshape(-1, 3)
+        v, f = convex_hull(pts)
+        self.hulls.append((v, f))
+        self.hull_points.append(pts)
         return len(self.hulls) - 1
 
     def hull_arrays(self):
         """(vertices float32 [sum V * 3], ints int32 [nb hulls, then per hull nv, nf, face sizes,
         face indices]) -- the layout the reference harness reads."""
-        verts = [h[0].ravel() for h in self.hulls]
+        verts = []
         ints = [len(self.hulls)]
-        for v, faces in self.hulls:
+        for (v, faces), pts in zip(self.hulls, self.hull_points):
+            if pts is not None:   # point cloud: the reference runs its own QuickHull (nf = -1)
+                verts.append(pts.ravel())
+                ints += [len(pts), -1]
+                continue
+            verts.append(v.ravel())
             ints += [len(v), len(faces)] + [len(f) for f in faces] + [i for f in faces for i in f]
         vs = np.concatenate(verts) if verts else np.zeros(3, np.float32)
         return np.ascontiguousarray(vs, np.float32), np.array(ints, np.int32)
@@ -366,37 +380,21 @@ def hinge_rigs(n_rigs=8, spacing=5.0, seed=9, sc=None):
     return sc
 
 
-@lru_cache(maxsize=8)
-def _triples(n):
-    return np.array(list(combinations(range(n), 3)), np.int64)
-
-
 def convex_hull(points):
-    """Convex hull of points in general position (no four coplanar): every triple whose plane has all
-    other points strictly on one side is a face, oriented counter-clockwise seen from outside.
-    Returns (vertices used, faces as index triples into them).  O(n^4) on the host; meant for the
-    small point sets (tens of points) of the convex-mesh scenes."""
-    P = np.asarray(points, np.float64)
-    n = len(P)
-    tri = _triples(n)
-    nrm = np.cross(P[tri[:, 1]] - P[tri[:, 0]], P[tri[:, 2]] - P[tri[:, 0]])
-    d = np.einsum("tpc,tc->tp", P[None, :, :] - P[tri[:, 0]][:, None, :], nrm)
-    d[np.arange(len(tri))[:, None], tri] = 0.0
-    tol = 1e-12 * np.maximum(1.0, np.abs(d).max(1))
-    below = np.all(d <= tol[:, None], 1)
-    above = np.all(d >= -tol[:, None], 1)
-    sel = below | above
-    f = tri[sel].copy()
-    flip = ~below[sel]                      # all other points above the plane: reverse the winding
-    f[flip] = f[flip][:, [0, 2, 1]]
-    faces = [tuple(int(v) for v in row) for row in f]
-    used = sorted({v for f in faces for v in f})
-    remap = {v: t for t, v in enumerate(used)}
-    return P[used].astype(np.float32), [[remap[v] for v in f] for f in faces]
+    """Convex hull of a point cloud as PhysicsCommon::createConvexMesh(VertexArray) builds it (QuickHull, in
+    the library's host code): (hull vertices [V, 3] float32 numbered in order of first use, faces as CCW
+    index lists in the reference's face order)."""
+    from reactphysics3d_amd import lib
+    v, counts, idx = lib.quickhull(points)
+    faces, o = [], 0
+    for c in counts:
+        faces.append([int(i) for i in idx[o:o + c]])
+        o += c
+    return v, faces
 
 
-def ellipsoid_hull(rng, n_points=24, lo=0.3, hi=1.2):
-    """Hull of n points on an ellipsoid with semi-axes ~U[lo, hi] (point = axes * uniform unit vector)."""
+def ellipsoid_points(rng, n_points=24, lo=0.3, hi=1.2):
+    """n points on an ellipsoid with semi-axes ~U[lo, hi] (point = axes * uniform unit vector)."""
     axes = [rng.uniform(lo, hi) for _ in range(3)]
     pts = []
     for _ in range(n_points):
@@ -404,7 +402,12 @@ def ellipsoid_hull(rng, n_points=24, lo=0.3, hi=1.2):
         a = 2.0 * math.pi * rng.uniform01()
         r = math.sqrt(max(0.0, 1.0 - z * z))
         pts.append((axes[0] * r * math.cos(a), axes[1] * r * math.sin(a), axes[2] * z))
-    return convex_hull(pts)
+    return np.array(pts, np.float32)
+
+
+def ellipsoid_hull(rng, n_points=24, lo=0.3, hi=1.2):
+    """QuickHull of n points on an ellipsoid with semi-axes ~U[lo, hi]: (vertices, faces)."""
+    return convex_hull(ellipsoid_points(rng, n_points, lo, hi))
 
 
 def hull_scene(n_hulls=16384, n_chains=256, chain_length=16, extent=(120.0, 120.0, 80.0), seed=3, n_points=24):
@@ -416,8 +419,7 @@ def hull_scene(n_hulls=16384, n_chains=256, chain_length=16, extent=(120.0, 120.
     g = sc.add_body(STATIC, (0.0, 0.0, -0.5))
     sc.add_collider(g, BOX, (extent[0] / 2 + 10.0, extent[1] / 2 + 10.0, 0.5))
     for _ in range(n_hulls):
-        v, f = ellipsoid_hull(rng, n_points)
-        h = sc.add_hull(v, f)
+        h = sc.add_hull_from_points(ellipsoid_points(rng, n_points))
         pos = (rng.uniform(-extent[0] / 2, extent[0] / 2), rng.uniform(-extent[1] / 2, extent[1] / 2),
                rng.uniform(1.0, extent[2]))
         b = sc.add_body(DYNAMIC, pos, rng.quaternion())
