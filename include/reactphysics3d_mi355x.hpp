@@ -149,6 +149,17 @@ class Collider {
     Material& getMaterial() { return mMaterial; }
     void setCollisionCategoryBits(uint16 bits) { mCategory = bits; check(rp3dg_collider_set_filter(mWorld, mIndex, mCategory, mMask), "filter"); }
     void setCollideWithMaskBits(uint16 bits) { mMask = bits; check(rp3dg_collider_set_filter(mWorld, mIndex, mCategory, mMask), "filter"); }
+    // Collider.cpp:240-300
+    bool getIsTrigger() const { return (rp3dg_collider_get_kind(mWorld, mIndex) & RP3DG_COLLIDER_TRIGGER) != 0; }
+    void setIsTrigger(bool on) const { check(rp3dg_collider_set_is_trigger(mWorld, mIndex, on ? 1 : 0), "setIsTrigger"); }
+    bool getIsSimulationCollider() const { return (rp3dg_collider_get_kind(mWorld, mIndex) & RP3DG_COLLIDER_SIMULATION) != 0; }
+    void setIsSimulationCollider(bool on) const {
+        check(rp3dg_collider_set_is_simulation_collider(mWorld, mIndex, on ? 1 : 0), "setIsSimulationCollider");
+    }
+    bool getIsWorldQueryCollider() const { return (rp3dg_collider_get_kind(mWorld, mIndex) & RP3DG_COLLIDER_WORLD_QUERY) != 0; }
+    void setIsWorldQueryCollider(bool on) const {
+        check(rp3dg_collider_set_is_world_query_collider(mWorld, mIndex, on ? 1 : 0), "setIsWorldQueryCollider");
+    }
     int32_t index() const { return mIndex; }
 
    private:
@@ -352,9 +363,35 @@ class CollisionCallback {
     virtual ~CollisionCallback() = default;
     virtual void onContact(const CallbackData& data) = 0;
 };
+// Overlaps of trigger colliders (collision/OverlapCallback.h), delivered to EventListener::onTrigger.
+class OverlapCallback {
+   public:
+    class OverlapPair {
+       public:
+        enum class EventType { OverlapStart, OverlapStay, OverlapExit };
+        Collider* getCollider1() const { return mCollider1; }
+        Collider* getCollider2() const { return mCollider2; }
+        RigidBody* getBody1() const { return mBody1; }
+        RigidBody* getBody2() const { return mBody2; }
+        EventType getEventType() const { return mType; }
+        Collider *mCollider1 = nullptr, *mCollider2 = nullptr;
+        RigidBody *mBody1 = nullptr, *mBody2 = nullptr;
+        EventType mType = EventType::OverlapStart;
+    };
+    class CallbackData {
+       public:
+        uint32 getNbOverlappingPairs() const { return (uint32)mPairs.size(); }
+        OverlapPair getOverlappingPair(uint32 i) const { return mPairs[i]; }
+        std::vector<OverlapPair> mPairs;
+    };
+    virtual ~OverlapCallback() = default;
+    virtual void onOverlap(CallbackData& data) = 0;
+};
+// engine/EventListener.h:45-72: contacts of simulation colliders to onContact, trigger overlaps to onTrigger.
 class EventListener : public CollisionCallback {
    public:
     void onContact(const CollisionCallback::CallbackData&) override {}
+    virtual void onTrigger(const OverlapCallback::CallbackData&) {}
 };
 
 // ---- raycasts (mathematics/Ray.h, collision/RaycastInfo.h) ----
@@ -509,7 +546,20 @@ class PhysicsWorld {
         for (auto& body : mBodies)
             for (auto& col : body->mColliders) owner[col->index()] = {col.get(), body.get()};
         CollisionCallback::CallbackData data;
+        OverlapCallback::CallbackData overlaps;
         for (int64_t e = 0; e < n; e++) {
+            if (type[e] >= RP3DG_TRIGGER_START) {   // onTrigger (CollisionDetectionSystem.cpp:1745-1757)
+                OverlapCallback::OverlapPair o;
+                o.mCollider1 = owner[c1[e]].first;
+                o.mBody1 = owner[c1[e]].second;
+                o.mCollider2 = owner[c2[e]].first;
+                o.mBody2 = owner[c2[e]].second;
+                o.mType = type[e] == RP3DG_TRIGGER_START ? OverlapCallback::OverlapPair::EventType::OverlapStart
+                          : type[e] == RP3DG_TRIGGER_STAY ? OverlapCallback::OverlapPair::EventType::OverlapStay
+                                                          : OverlapCallback::OverlapPair::EventType::OverlapExit;
+                overlaps.mPairs.push_back(o);
+                continue;
+            }
             CollisionCallback::ContactPair p;
             p.mCollider1 = owner[c1[e]].first;
             p.mBody1 = owner[c1[e]].second;
@@ -533,7 +583,9 @@ class PhysicsWorld {
             }
             data.mPairs.push_back(std::move(p));
         }
+        // both callbacks run whenever the step has a colliding or a lost pair (CollisionDetectionSystem.cpp:1708-1757)
         mListener->onContact(data);
+        mListener->onTrigger(overlaps);
     }
     Joint* addJoint(std::unique_ptr<Joint> joint, int32_t type, const JointInfo& i, const Vector3& axis, bool limit,
                     bool motor, const decimal* lim) {

@@ -116,6 +116,23 @@ int rp3dg_body_set_allowed_to_sleep(rp3dg_world* world, int32_t body, int32_t al
 /* Collider::setCollisionCategoryBits / setCollideWithMaskBits (src/collision/Collider.cpp). */
 int rp3dg_collider_set_filter(rp3dg_world* world, int32_t collider, uint16_t category, uint16_t mask);
 
+/* Collider kinds (Collider::getIsTrigger / getIsSimulationCollider / getIsWorldQueryCollider,
+ * src/collision/Collider.cpp:240-300).  A new collider is a simulation and world-query collider. */
+#define RP3DG_COLLIDER_TRIGGER 1
+#define RP3DG_COLLIDER_SIMULATION 2
+#define RP3DG_COLLIDER_WORLD_QUERY 4
+/* Collider::setIsTrigger (src/collision/Collider.cpp:250-257): a trigger reports overlaps (onTrigger) and
+ * takes no part in the simulation; setting it clears the simulation flag. */
+int rp3dg_collider_set_is_trigger(rp3dg_world* world, int32_t collider, int32_t is_trigger);
+/* Collider::setIsSimulationCollider (src/collision/Collider.cpp:272-287); setting it clears the trigger flag.
+ * A collider that is neither is never tested in the step. */
+int rp3dg_collider_set_is_simulation_collider(rp3dg_world* world, int32_t collider, int32_t is_simulation);
+/* Collider::setIsWorldQueryCollider (src/collision/Collider.cpp:298-300): whether testOverlap /
+ * testCollision queries see the collider. */
+int rp3dg_collider_set_is_world_query_collider(rp3dg_world* world, int32_t collider, int32_t is_world_query);
+/* The RP3DG_COLLIDER_* bits of a collider (or a negative code). */
+int rp3dg_collider_get_kind(rp3dg_world* world, int32_t collider);
+
 /* PhysicsCommon::createConvexMesh(PolygonVertexArray) (src/engine/PhysicsCommon.cpp:600-630):
  * vertices xyz[nv], faces given as counts[nf] + concatenated vertex indices (CCW seen from outside).
  * Returns the hull index used as params[0] of RP3DG_SHAPE_CONVEX_MESH. */
@@ -256,6 +273,12 @@ int rp3dg_batch_get_contacts(rp3dg_batch* batch, rp3dg_contact_view* out, int64_
 #define RP3DG_CONTACT_START 0
 #define RP3DG_CONTACT_STAY 1
 #define RP3DG_CONTACT_EXIT 2
+/* Pairs with a trigger collider report to EventListener::onTrigger instead
+ * (CollisionDetectionSystem::reportTriggers, CollisionDetectionSystem.cpp:1745-1757): OverlapStart /
+ * OverlapStay / OverlapExit (OverlapCallback::OverlapPair::EventType), without contact points. */
+#define RP3DG_TRIGGER_START 3
+#define RP3DG_TRIGGER_STAY 4
+#define RP3DG_TRIGGER_EXIT 5
 int rp3dg_batch_set_contact_events(rp3dg_batch* batch, int32_t enable);
 /* The events of the last step (of the last step of a graph replay), in no particular order: colliders
  * (batch indices, oriented as the pair), event type and contact points (0 for an exit).  Returns the

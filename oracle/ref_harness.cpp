@@ -56,6 +56,7 @@ struct RefCollider {
     float lquat[4];
     float friction, bounciness, density;
     int32_t category, mask;
+    int32_t kind;     // bit0 trigger, bit1 simulation collider, bit2 world-query collider
 };
 
 struct RefJoint {
@@ -78,6 +79,7 @@ struct EventRecorder : public EventListener {
     RefWorld* R = nullptr;
     std::vector<int32_t> ev;
     void onContact(const CollisionCallback::CallbackData& data) override;
+    void onTrigger(const OverlapCallback::CallbackData& data) override;   // recorded as type 3 + OverlapPair::EventType
 };
 
 struct RefWorld {
@@ -179,6 +181,9 @@ void* rp3dref_create(const RefSettings* s, int nb, const RefBody* bodies, int nc
             col->getMaterial().setMassDensity(C.density);
             if (C.category != 0x0001) col->setCollisionCategoryBits((unsigned short)C.category);
             if (C.mask != 0xFFFF) col->setCollideWithMaskBits((unsigned short)C.mask);
+            if (C.kind & 1) col->setIsTrigger(true);
+            else if (!(C.kind & 2)) col->setIsSimulationCollider(false);
+            if (!(C.kind & 4)) col->setIsWorldQueryCollider(false);
             R->colliders.push_back(col);
             R->collider_of_entity[col->getEntity().id] = ci;
             ci++;
@@ -293,6 +298,16 @@ void EventRecorder::onContact(const CollisionCallback::CallbackData& data) {
     }
 }
 
+void EventRecorder::onTrigger(const OverlapCallback::CallbackData& data) {
+    for (uint32 i = 0; i < data.getNbOverlappingPairs(); i++) {
+        const OverlapCallback::OverlapPair p = data.getOverlappingPair(i);
+        ev.push_back(collider_index(R, p.getCollider1()->getEntity()));
+        ev.push_back(collider_index(R, p.getCollider2()->getEntity()));
+        ev.push_back(3 + (int32_t)p.getEventType());
+        ev.push_back(0);
+    }
+}
+
 static int apply_joint_order(RefWorld* R);
 
 // ContactSolverSystem::init (src/systems/ContactSolverSystem.cpp:66-105) with the rows of each
@@ -328,13 +343,18 @@ static int solver_init_ordered(RefWorld* R, float dt, const int32_t* oc1, const 
     uint32 acc = 0;
     for (uint32 c = 0; c < m; c++) { first[c] = acc; acc += CS.mContactConstraints[c].nbContacts; }
     std::vector<std::pair<int, uint32>> keyed(m);
+    bool missing = false;
     for (uint32 c = 0; c < m; c++) {
         const ContactManifold* em = CS.mContactConstraints[c].externalContactManifold;
         const int a = collider_index(R, em->colliderEntity1), b = collider_index(R, em->colliderEntity2);
         auto it = pos.find({a, b});
-        if (it == pos.end()) return -1;
-        keyed[c] = {it->second, c};
+        if (it == pos.end()) missing = true;
+        keyed[c] = {it == pos.end() ? 0 : it->second, c};
     }
+    // a manifold missing from the given order: keep the reference's own order so that the step still
+    // completes (the world stays consistent and can be destroyed) and report the mismatch
+    if (missing)
+        for (uint32 c = 0; c < m; c++) keyed[c] = {(int)c, c};
     std::sort(keyed.begin(), keyed.end());
     std::vector<ContactSolverSystem::ContactManifoldSolver> ms(CS.mContactConstraints, CS.mContactConstraints + m);
     std::vector<ContactSolverSystem::ContactPointSolver> ps(CS.mContactPoints, CS.mContactPoints + CS.mNbContactPoints);
@@ -345,7 +365,7 @@ static int solver_init_ordered(RefWorld* R, float dt, const int32_t* oc1, const 
         for (int k = 0; k < ms[src].nbContacts; k++) CS.mContactPoints[wp++] = ps[first[src] + k];
     }
     if (warm_start) CS.warmStart();
-    return 0;
+    return missing ? 1 : 0;
 }
 
 // Learned warm start, the project's extension of ContactSolverSystem::warmStart
@@ -477,7 +497,7 @@ static int step_ordered_mode(RefWorld* R, float dt, const int32_t* oc1, const in
     W.mDynamicsSystem.integrateRigidBodiesVelocities(dt);
     // solveContactsAndConstraints (PhysicsWorld.cpp:321-345)
     const float* pred = mode ? mode->pred : nullptr;
-    if (solver_init_ordered(R, dt, oc1, oc2, n, pred == nullptr) != 0) return -1;
+    const int order_rc = solver_init_ordered(R, dt, oc1, oc2, n, pred == nullptr);
     if (pred) warm_start_learned(W.mContactSolverSystem, pred);
     W.mConstraintSolverSystem.initialize(dt);
     if (mode && mode->res_thr > 0.0f) {
@@ -515,7 +535,7 @@ static int step_ordered_mode(RefWorld* R, float dt, const int32_t* oc1, const in
     W.mProcessContactPairsOrderIslands.clear(true);
     W.mMemoryManager.resetFrameAllocator();
     R->last_dt = dt;
-    return 0;
+    return order_rc != 0 ? -1 : 0;
 }
 
 int rp3dref_step_ordered(void* h, float dt, const int32_t* oc1, const int32_t* oc2, int n) {

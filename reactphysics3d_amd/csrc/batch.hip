@@ -431,6 +431,7 @@ int rp3dg_batch::upload(rp3dg_world* const* ws, int nw, const rp3dg_batch_option
     std::vector<float4> shape, lpos, lq;
     std::vector<float2> mat;
     std::vector<uint32_t> filter;
+    std::vector<int> ckind;
     std::vector<HullHeader> hh;
     std::vector<float4> hverts, hnormals, hbounds;
     std::vector<int4> hedges;
@@ -485,6 +486,7 @@ int rp3dg_batch::upload(rp3dg_world* const* ws, int nw, const rp3dg_batch_option
             lq.push_back(make_float4(c.local.q.x, c.local.q.y, c.local.q.z, c.local.q.w));
             mat.push_back(make_float2(c.friction_sqrt, c.bounciness));
             filter.push_back((uint32_t)c.category | ((uint32_t)c.mask << 16));
+            ckind.push_back(c.kind);
         }
         for (const HostHull& H : W.hulls) {
             HullHeader h;
@@ -618,6 +620,7 @@ int rp3dg_batch::upload(rp3dg_world* const* ws, int nw, const rp3dg_batch_option
     C.amin = dalloc<float4>(this, nc); C.amax = dalloc<float4>(this, nc);
     C.mat = dalloc<float2>(this, nc);
     C.filter = dalloc<uint32_t>(this, nc);
+    C.kind = dalloc<int>(this, nc);
     C.moved = dalloc<int>(this, nc); C.rank = dalloc<int>(this, nc); C.world = dalloc<int>(this, nc);
     if (!B.world || !C.world) { set_error("device allocation failed"); return RP3DG_EHIP; }
     auto up = [&](void* d, const void* h, size_t bytes) { return hipMemcpyAsync(d, h, bytes, hipMemcpyHostToDevice, stream); };
@@ -633,6 +636,7 @@ int rp3dg_batch::upload(rp3dg_world* const* ws, int nw, const rp3dg_batch_option
     HIPCHECK(up(C.shape, shape.data(), nc * 16)); HIPCHECK(up(C.lpos, lpos.data(), nc * 16));
     HIPCHECK(up(C.lq, lq.data(), nc * 16)); HIPCHECK(up(C.mat, mat.data(), nc * 8));
     HIPCHECK(up(C.filter, filter.data(), nc * 4)); HIPCHECK(up(C.rank, crank.data(), nc * 4));
+    HIPCHECK(up(C.kind, ckind.data(), nc * 4));
     HIPCHECK(up(C.world, cworld.data(), nc * 4));
     // hulls (at least one dummy record so pointers are valid)
     if (hh.empty()) { hh.push_back(HullHeader{}); hbounds.resize(2); }
@@ -935,7 +939,7 @@ int rp3dg_batch::enqueue_step(float dt, bool capturing) {
     if (events_on) fill_int_n<<<pg, 256, 0, stream>>>(O.kept, pair_cap, old_count, 0);
     LAUNCH_T("bp_merge", pg, 256, bp_merge, C, bp_out, bp_offs + nc, pair_cap, O, old_count, old_offs, N, new_count,
              &counters->nb_pairs);
-    if (events_on) LAUNCH_T("ev_removed", pg, 256, ev_removed, O, old_count, E);
+    if (events_on) LAUNCH_T("ev_removed", pg, 256, ev_removed, C, O, old_count, E);
     cur_is_a = !cur_is_a;
 
     // ---------------- narrowphase (CD.cpp:618-718)

@@ -6,7 +6,8 @@
 
 namespace rp3dg {
 
-// One event: collider1, collider2 (as the pair orients them), EV_START / EV_STAY / EV_EXIT, contact points.
+// One event: collider1, collider2 (as the pair orients them), EV_START / EV_STAY / EV_EXIT (contact pairs) or
+// EV_TRIGGER_START / _STAY / _EXIT (pairs with a trigger collider), contact points.
 struct DevEvents {
     int4* ev;
     int cap;
@@ -14,7 +15,7 @@ struct DevEvents {
     int record;   // 0: only the per-pair colliding flags are kept up to date
 };
 
-__global__ void ev_removed(DevPairs O, const int* nold, DevEvents E);
+__global__ void ev_removed(DevColliders C, DevPairs O, const int* nold, DevEvents E);
 __global__ void ev_update(DevBodies B, DevColliders C, DevPairs N, const int* nb_pairs, DevEvents E, int shift);
 __global__ void ev_shift(DevBodies B, DevColliders C, DevPairs N, const int* nb_pairs);
 

@@ -2,6 +2,8 @@
 // OverlappingPairs, CD.cpp:174-234, :380-441, :975-1011, :1224-1303, :1708-1741;
 // OverlappingPairs.cpp:465-482).
 //
+// Pairs with a trigger collider report the same Start / Stay / Exit sequence to onTrigger
+// (CollisionDetectionSystem::reportTriggers, CD.cpp:1745-1757) as EV_TRIGGER_* events, without points.
 // Every pair keeps the reference's two flags: collidingInCurrentFrame (EV_CUR, set by this step's
 // narrowphase on the pairs it tests) and collidingInPreviousFrame (EV_PREV, copied from EV_CUR by
 // updateCollidingInPreviousFrame on the enabled pairs after the islands are built).  Events of a
@@ -31,13 +33,19 @@ __device__ __forceinline__ void ev_push(const DevEvents& E, bool want, int4 e) {
     }
 }
 
-// pairs of the previous table that the broadphase removed this step (CD.cpp:189-202)
-__global__ void ev_removed(DevPairs O, const int* nold, DevEvents E) {
+__device__ __forceinline__ bool is_trigger_pair(const DevColliders& C, int2 c) {
+    return ((C.kind[c.x] | C.kind[c.y]) & CK_TRIGGER) != 0;
+}
+
+// pairs of the previous table that the broadphase removed this step (CD.cpp:189-202); a trigger pair
+// reports OverlapExit to onTrigger instead (OverlapCallback.cpp:79-95)
+__global__ void ev_removed(DevColliders C, DevPairs O, const int* nold, DevEvents E) {
     const int n = *nold;
     RP3DG_GRID_STRIDE(k, n) {
         const bool lost = !O.kept[k] && (O.evf[k] & EV_PREV);
         const int2 c = O.col[k];
-        ev_push(E, lost, make_int4(c.x, c.y, EV_EXIT, 0));
+        const int type = lost && is_trigger_pair(C, c) ? EV_TRIGGER_EXIT : EV_EXIT;
+        ev_push(E, lost, make_int4(c.x, c.y, type, 0));
     }
 }
 
@@ -48,16 +56,24 @@ __global__ void ev_update(DevBodies B, DevColliders C, DevPairs N, const int* nb
     const int n = *nb_pairs;
     RP3DG_GRID_STRIDE(k, n) {
         const int2 c = N.col[k];
+        // the middle phase clears collidingInCurrentFrame of every pair with an enabled body (CD.cpp:389) and
+        // tests those whose colliders are both simulation colliders or triggers
         const bool tested = ((B.flags[C.body[c.x]] | B.flags[C.body[c.y]]) & BF_ENABLED) != 0;
+        const int k1 = C.kind[c.x], k2 = C.kind[c.y];
+        const bool active = (k1 & (CK_SIMULATION | CK_TRIGGER)) && (k2 & (CK_SIMULATION | CK_TRIGGER));
+        const bool trig = ((k1 | k2) & CK_TRIGGER) != 0;
         const int np = N.ncp[k];
         const int f = N.evf[k];
-        const bool cur = tested ? np > 0 : (f & EV_CUR) != 0;
+        // a trigger pair collides without contact points: its test result is the wasColliding flag
+        const bool hit = trig ? (N.lf[k] & LF_WAS_COLLIDING) != 0 : np > 0;
+        const bool cur = tested ? (active && hit) : (f & EV_CUR) != 0;
         const bool prev = (f & EV_PREV) != 0;
         if (tested) N.evf[k] = shift ? (cur ? EV_CUR | EV_PREV : 0) : ((prev ? EV_PREV : 0) | (cur ? EV_CUR : 0));
         if (E.record) {
             const bool emit = tested && (cur || prev);
-            const int type = cur ? (prev ? EV_STAY : EV_START) : EV_EXIT;
-            ev_push(E, emit, make_int4(c.x, c.y, type, cur ? np : 0));
+            int type = cur ? (prev ? EV_STAY : EV_START) : EV_EXIT;
+            if (trig) type += EV_TRIGGER_START;   // onTrigger: OverlapStart / OverlapStay / OverlapExit
+            ev_push(E, emit, make_int4(c.x, c.y, type, cur && !trig ? np : 0));
         }
     }
 }

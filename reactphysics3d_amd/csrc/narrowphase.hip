@@ -611,12 +611,19 @@ __device__ void reduce_points(const R& raw, const Tf& T1, int* keep) {
     keep[3] = cand.get(idx);
 }
 
+// A pair with a trigger collider is tested with reportContacts = false (CD.cpp:402): it can collide, but it
+// has no contact points, no manifold and no solver rows.
+__device__ __forceinline__ bool trigger_pair(const NpArgs& A, int slot) {
+    const int2 c = A.P.col[slot];
+    return ((A.C.kind[c.x] | A.C.kind[c.y]) & CK_TRIGGER) != 0;
+}
+
 template <class R>
 __device__ void contact_epilogue(const NpArgs& A, int slot, bool colliding, const R& raw, int collider1) {
     DevPairs P = A.P;
     const DevPairs& O = A.O;
     const int prev = P.prev[slot];   // this pair's entry in the previous table, -1 when new
-    if (!colliding || raw.n == 0) {
+    if (!colliding || raw.n == 0 || trigger_pair(A, slot)) {
         P.ncp[slot] = 0;
         return;
     }
@@ -721,7 +728,10 @@ __device__ inline uint32_t lf_after_test(uint32_t lf, bool colliding) {
 __device__ __forceinline__ int np_bin(const NpArgs& A, int k, bool count_pass) {
     const int2 c = A.P.col[k];
     const int f1 = A.B.flags[A.C.body[c.x]], f2 = A.B.flags[A.C.body[c.y]];
-    if (!((f1 & BF_ENABLED) || (f2 & BF_ENABLED))) {
+    // tested only if a body is enabled and both colliders are simulation colliders or triggers (CD.cpp:386-400)
+    const int k1 = A.C.kind[c.x], k2 = A.C.kind[c.y];
+    if (!((f1 & BF_ENABLED) || (f2 & BF_ENABLED)) || !(k1 & (CK_SIMULATION | CK_TRIGGER)) ||
+        !(k2 & (CK_SIMULATION | CK_TRIGGER))) {
         if (count_pass) A.P.ncp[k] = 0;
         return 0;
     }
@@ -1547,7 +1557,8 @@ __device__ bool capsule_face_contacts(int face, float r, const Poly& P, float pe
 }
 
 // SATAlgorithm::testCollisionCapsuleVsConvexPolyhedron (:165-319)
-__device__ bool sat_capsule_poly(bool capsule1, float r, float hh, const Poly& P, const Tf& Tc, const Tf& Tp, Raw& raw) {
+__device__ bool sat_capsule_poly(bool capsule1, float r, float hh, const Poly& P, const Tf& Tc, const Tf& Tp, Raw& raw,
+                                 bool report) {
     const Tf p2c = tmul(tinv(Tc), Tp);
     const float height = hh + hh;
     const V3 segA = v3(0.0f, -height * 0.5f, 0.0f);
@@ -1597,7 +1608,9 @@ __device__ bool sat_capsule_poly(bool capsule1, float r, float hh, const Poly& P
     const V3 segBP = tapply(c2p, segB);
     V3 nw = qrot(Tc.q, sepAxis);
     if (capsule1) nw = -nw;
-    if (isFace) return capsule_face_contacts(minFace, r, P, minPen, p2c, nw, sepAxis, segAP, segBP, capsule1, raw);
+    // without contact reporting the face case collides whatever the clipping finds (SAT.cpp:278-287)
+    if (isFace) return report ? capsule_face_contacts(minFace, r, P, minPen, p2c, nw, sepAxis, segAP, segBP, capsule1, raw)
+                              : true;
     V3 cpPoly, cpSeg;
     closest_points_between_segments(segAP, segBP, sev1, sev2, cpSeg, cpPoly);
     const V3 cpCap = tapply(p2c, cpSeg) - sepAxis * r;
@@ -1655,7 +1668,10 @@ __global__ void np_capsule_poly(NpArgs A, const int* list, const int* count) {
         const Conv& Sp = cap1 ? S2 : S1;
         const Tf& Tc = cap1 ? T1 : T2;
         const Tf& Tp = cap1 ? T2 : T1;
-        if (g == GJK_COLLIDE_IN_MARGIN) {
+        const bool report = !trigger_pair(A, k);
+        if (g == GJK_COLLIDE_IN_MARGIN && !report) {
+            hit = true;   // the face-contact search only adds points (CapsuleVsConvexPolyhedronAlgorithm.cpp:83)
+        } else if (g == GJK_COLLIDE_IN_MARGIN) {
             const V3 cn = raw.nrm[0];
             const float cpen = raw.depth[0];
             const float height = Sc.hh + Sc.hh;
@@ -1680,7 +1696,7 @@ __global__ void np_capsule_poly(NpArgs A, const int* list, const int* count) {
             }
             hit = true;
         } else if (g == GJK_INTERPENETRATE) {
-            hit = sat_capsule_poly(cap1, Sc.margin, Sc.hh, Sp.P, Tc, Tp, raw);
+            hit = sat_capsule_poly(cap1, Sc.margin, Sc.hh, Sp.P, Tc, Tp, raw, report);
             lf &= ~LF_WAS_GJK;
             lf |= LF_WAS_SAT;
         }

@@ -96,6 +96,12 @@ struct NoColLists {
     const int* list;
 };
 
+// Collider kinds (Collider::getIsTrigger / getIsSimulationCollider / getIsWorldQueryCollider, Collider.cpp:240-300);
+// the values are the RP3DG_COLLIDER_* bits of the C ABI.
+constexpr int CK_TRIGGER = 1;
+constexpr int CK_SIMULATION = 2;
+constexpr int CK_WORLD_QUERY = 4;
+
 struct DevColliders {
     int *body;
     int *type;
@@ -105,6 +111,7 @@ struct DevColliders {
     float4 *amin, *amax;       // fat AABB (broadphase tree node AABB)
     float2 *mat;               // sqrt(friction), bounciness
     uint32_t *filter;          // category | mask << 16
+    int *kind;                 // CK_* bits: trigger, simulation collider, world-query collider
     int *moved;                // fat AABB recomputed by the last refit (BroadPhaseSystem moved set)
     int *rank;                 // position of the collider in the moved-set iteration order
     int *world;
@@ -138,6 +145,10 @@ constexpr int EV_PREV = 2;
 constexpr int EV_START = 0;
 constexpr int EV_STAY = 1;
 constexpr int EV_EXIT = 2;
+// trigger events (OverlapCallback::OverlapPair::EventType + 3)
+constexpr int EV_TRIGGER_START = 3;
+constexpr int EV_TRIGGER_STAY = 4;
+constexpr int EV_TRIGGER_EXIT = 5;
 
 // Solver rows of one contact manifold (ContactManifoldSolver + ContactPointSolver), colour order.
 struct DevSolver {

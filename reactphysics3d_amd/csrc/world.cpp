@@ -441,8 +441,42 @@ int rp3dg_collider_set_filter(rp3dg_world* w, int32_t ci, uint16_t cat, uint16_t
     return RP3DG_OK;
 }
 
-// ConvexMesh::init from a PolygonVertexArray (src/collision/ConvexMesh.cpp:55-264): centroid,
-// bounds, half-edge structure, Newell face normals, volume.
+// Collider::setIsTrigger (src/collision/Collider.cpp:250-257): a trigger stops being a simulation collider.
+int rp3dg_collider_set_is_trigger(rp3dg_world* w, int32_t ci, int32_t is_trigger) {
+    if (!w || ci < 0 || ci >= (int32_t)w->colliders.size()) { set_error("invalid collider"); return RP3DG_EINVAL; }
+    if (!mutable_world(w)) return RP3DG_ESTATE;
+    int& k = w->colliders[ci].kind;
+    k = is_trigger ? (k | CK_TRIGGER) : (k & ~CK_TRIGGER);
+    if (is_trigger) k &= ~CK_SIMULATION;
+    return RP3DG_OK;
+}
+
+// Collider::setIsSimulationCollider (src/collision/Collider.cpp:272-287): a simulation collider stops being a
+// trigger.
+int rp3dg_collider_set_is_simulation_collider(rp3dg_world* w, int32_t ci, int32_t is_simulation) {
+    if (!w || ci < 0 || ci >= (int32_t)w->colliders.size()) { set_error("invalid collider"); return RP3DG_EINVAL; }
+    if (!mutable_world(w)) return RP3DG_ESTATE;
+    int& k = w->colliders[ci].kind;
+    k = is_simulation ? (k | CK_SIMULATION) : (k & ~CK_SIMULATION);
+    if (is_simulation) k &= ~CK_TRIGGER;
+    return RP3DG_OK;
+}
+
+// Collider::setIsWorldQueryCollider (src/collision/Collider.cpp:298-300).
+int rp3dg_collider_set_is_world_query_collider(rp3dg_world* w, int32_t ci, int32_t is_query) {
+    if (!w || ci < 0 || ci >= (int32_t)w->colliders.size()) { set_error("invalid collider"); return RP3DG_EINVAL; }
+    if (!mutable_world(w)) return RP3DG_ESTATE;
+    int& k = w->colliders[ci].kind;
+    k = is_query ? (k | CK_WORLD_QUERY) : (k & ~CK_WORLD_QUERY);
+    return RP3DG_OK;
+}
+
+// Collider::getIsTrigger / getIsSimulationCollider / getIsWorldQueryCollider as RP3DG_COLLIDER_* bits.
+int rp3dg_collider_get_kind(rp3dg_world* w, int32_t ci) {
+    if (!w || ci < 0 || ci >= (int32_t)w->colliders.size()) { set_error("invalid collider"); return RP3DG_EINVAL; }
+    return w->colliders[ci].kind;
+}
+
 // VertexArray (src/collision/VertexArray.cpp) read as QuickHull::extractPoints does: 3 floats or 3 doubles
 // per vertex at `stride` bytes, doubles converted to float.
 static bool read_points(const void* data, int32_t n, int32_t stride, int32_t is_double, std::vector<V3>& out) {
@@ -500,6 +534,8 @@ int32_t rp3dg_hull_create_from_points(rp3dg_world* w, const void* vertices, int3
     return rp3dg_hull_create(w, v.data(), (int32_t)(v.size() / 3), c.data(), idx.data(), (int32_t)c.size());
 }
 
+// ConvexMesh::init from a PolygonVertexArray (src/collision/ConvexMesh.cpp:55-264): centroid,
+// bounds, half-edge structure, Newell face normals, volume.
 int32_t rp3dg_hull_create(rp3dg_world* w, const float* verts, int32_t nv, const int32_t* counts,
                           const int32_t* idx, int32_t nf) {
     if (!w || !verts || !counts || !idx || nv < 4 || nf < 4) { set_error("invalid hull arguments"); return RP3DG_EINVAL; }

@@ -55,6 +55,7 @@ struct HostCollider {
     Tf local;
     float friction_sqrt, bounciness, density;
     uint16_t category = 0x0001, mask = 0xFFFF;
+    int kind = CK_SIMULATION | CK_WORLD_QUERY;   // ColliderComponents defaults (ColliderComponents.h)
 };
 
 struct HostJoint {

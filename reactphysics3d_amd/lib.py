@@ -77,7 +77,9 @@ EXPORTED = [
     "rp3dg_mlp_gradient", "rp3dg_mlp_train", "rp3dg_batch_set_warm_start", "rp3dg_batch_set_logging",
     "rp3dg_batch_get_log", "rp3dg_batch_autotune", "rp3dg_batch_get_tune_table", "rp3dg_world_get_body_state",
     "rp3dg_batch_get_world_iterations", "rp3dg_batch_get_warm_start_predictions",
-    "rp3dg_hull_create_from_points", "rp3dg_quickhull",
+    "rp3dg_hull_create_from_points", "rp3dg_quickhull", "rp3dg_collider_set_is_trigger",
+    "rp3dg_collider_set_is_simulation_collider", "rp3dg_collider_set_is_world_query_collider",
+    "rp3dg_collider_get_kind",
 ]
 
 MLP_PARAMS = 20099
@@ -153,6 +155,10 @@ def load():
         "rp3dg_batch_get_world_iterations": (C.c_int, [vp, iptr, fptr, i64]),
         "rp3dg_batch_get_warm_start_predictions": (i64, [vp, fptr, i64]),
         "rp3dg_hull_create_from_points": (i32, [vp, vp, i32, i32, i32]),
+        "rp3dg_collider_set_is_trigger": (C.c_int, [vp, i32, i32]),
+        "rp3dg_collider_set_is_simulation_collider": (C.c_int, [vp, i32, i32]),
+        "rp3dg_collider_set_is_world_query_collider": (C.c_int, [vp, i32, i32]),
+        "rp3dg_collider_get_kind": (C.c_int, [vp, i32]),
         "rp3dg_quickhull": (C.c_int, [vp, i32, i32, i32, fptr, i32, iptr, i32, iptr, i32, iptr, iptr]),
         "rp3dg_mlp_create": (vp, [C.c_uint64]),
         "rp3dg_mlp_destroy": (None, [vp]),
@@ -230,11 +236,17 @@ def world_from_scene(sc: S.Scene):
         b = _check(L.rp3dg_body_create(w, _f3(pos), _f4(q)), "body_create")
         _check(L.rp3dg_body_set_type(w, b, bt), "body_set_type")
         while ci < len(cols) and cols[ci][0] == bi:
-            (_, shape, params, lpos, lq, fr, bo, de, cat, mask) = cols[ci]
+            (_, shape, params, lpos, lq, fr, bo, de, cat, mask, kind) = cols[ci]
             c = _check(L.rp3dg_collider_add(w, b, shape, _f4(params), _f3(lpos), _f4(lq)), "collider_add")
             _check(L.rp3dg_collider_set_material(w, c, fr, bo, de), "set_material")
             if cat != 0x0001 or mask != 0xFFFF:
                 _check(L.rp3dg_collider_set_filter(w, c, cat, mask), "set_filter")
+            if kind & S.COLLIDER_TRIGGER:
+                _check(L.rp3dg_collider_set_is_trigger(w, c, 1), "set_is_trigger")
+            elif not kind & S.COLLIDER_SIMULATION:
+                _check(L.rp3dg_collider_set_is_simulation_collider(w, c, 0), "set_is_simulation_collider")
+            if not kind & S.COLLIDER_WORLD_QUERY:
+                _check(L.rp3dg_collider_set_is_world_query_collider(w, c, 0), "set_is_world_query_collider")
             ci += 1
         if bt == S.DYNAMIC:
             _check(L.rp3dg_body_update_mass_properties(w, b), "update_mass_properties")

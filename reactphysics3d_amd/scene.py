@@ -32,9 +32,12 @@ BODY_DTYPE = np.dtype([
     ("type", "<i4"), ("pos", "<f4", 3), ("quat", "<f4", 4), ("lin", "<f4", 3), ("ang", "<f4", 3),
     ("gravity", "<i4"), ("ldamp", "<f4"), ("adamp", "<f4"), ("llock", "<f4", 3), ("alock", "<f4", 3),
 ])
+# Collider kind bits (RP3DG_COLLIDER_*): Collider::getIsTrigger / getIsSimulationCollider / getIsWorldQueryCollider
+COLLIDER_TRIGGER, COLLIDER_SIMULATION, COLLIDER_WORLD_QUERY = 1, 2, 4
 COLLIDER_DTYPE = np.dtype([
     ("body", "<i4"), ("shape", "<i4"), ("params", "<f4", 4), ("lpos", "<f4", 3), ("lquat", "<f4", 4),
     ("friction", "<f4"), ("bounciness", "<f4"), ("density", "<f4"), ("category", "<i4"), ("mask", "<i4"),
+    ("kind", "<i4"),
 ])
 JOINT_DTYPE = np.dtype([
     ("type", "<i4"), ("body1", "<i4"), ("body2", "<i4"), ("anchor", "<f4", 3), ("axis", "<f4", 3),
@@ -97,13 +100,20 @@ class Scene:
         return len(self.bodies) - 1
 
     def add_collider(self, body, shape, params, lpos=(0.0, 0.0, 0.0), lquat=(0.0, 0.0, 0.0, 1.0),
-                     friction=None, bounciness=None, density=1.0, category=0x0001, mask=0xFFFF):
+                     friction=None, bounciness=None, density=1.0, category=0x0001, mask=0xFFFF, trigger=False,
+                     simulation=None, world_query=True):
+        """trigger / simulation / world_query: Collider::setIsTrigger / setIsSimulationCollider /
+        setIsWorldQueryCollider (a trigger is not a simulation collider unless simulation=True is asked,
+        which the reference turns back into a plain simulation collider)."""
         if self.colliders and self.colliders[-1][0] > body:
             raise ValueError("colliders must be added in body order")
         p = list(params) + [0.0] * (4 - len(params))
         f = float(self.settings["default_friction"]) if friction is None else friction
         b = float(self.settings["default_bounciness"]) if bounciness is None else bounciness
-        self.colliders.append((body, shape, tuple(p), tuple(lpos), tuple(lquat), f, b, density, category, mask))
+        sim = (not trigger) if simulation is None else bool(simulation)
+        kind = (COLLIDER_TRIGGER if trigger and not sim else 0) | (COLLIDER_SIMULATION if sim else 0) | \
+            (COLLIDER_WORLD_QUERY if world_query else 0)
+        self.colliders.append((body, shape, tuple(p), tuple(lpos), tuple(lquat), f, b, density, category, mask, kind))
         return len(self.colliders) - 1
 
     def add_joint(self, jtype, body1, body2, anchor, axis=(0.0, 0.0, 1.0), collide=False, flags=0,
@@ -229,6 +239,49 @@ def mixed_container(n_bodies=65536, extent=(200.0, 200.0, 100.0), seed=2, vel_it
         lin = (rng.normal(), rng.normal(), rng.normal())
         b = sc.add_body(DYNAMIC, pos, q, lin=lin)
         sc.add_collider(b, shape, params, density=1000.0)
+    return sc
+
+
+def trigger_scene(n_bodies=60, seed=12, extent=(10.0, 10.0, 12.0)):
+    """Trigger colliders (Collider::setIsTrigger) among simulated bodies: static trigger zones (a box, a
+    sphere, a capsule and a convex mesh) above a static ground, dynamic spheres / boxes / capsules /
+    hulls falling through them, bodies whose only collider is a trigger (they fall through everything,
+    reporting overlaps), compound bodies with a simulation box and a trigger sphere, and one collider that
+    is neither a trigger nor a simulation collider (never tested)."""
+    rng = SplitMix64(seed)
+    sc = Scene(f"triggers_{n_bodies}", make_settings(gravity=(0.0, 0.0, -9.81)))
+    ex, ey, ez = extent
+    g = sc.add_body(STATIC, (0.0, 0.0, -0.5))
+    sc.add_collider(g, BOX, (ex, ey, 0.5))
+    zones = sc.add_body(STATIC, (0.0, 0.0, 0.0))
+    sc.add_collider(zones, BOX, (ex / 2, ey / 2, 0.6), lpos=(0.0, 0.0, 3.0), trigger=True)
+    sc.add_collider(zones, SPHERE, (2.0,), lpos=(2.0, 2.0, 6.5), trigger=True)
+    sc.add_collider(zones, CAPSULE, (0.8, 6.0), lpos=(-2.0, -1.0, 5.0), lquat=(0.7071068, 0.0, 0.0, 0.7071068),
+                    trigger=True)
+    h = sc.add_hull_from_points(ellipsoid_points(rng, 24, 1.0, 2.0))
+    sc.add_collider(zones, HULL, (float(h),), lpos=(-2.5, 2.5, 8.0), trigger=True)
+    hull_small = sc.add_hull_from_points(ellipsoid_points(rng, 24, 0.3, 0.6))
+    for i in range(n_bodies):
+        pos = (rng.uniform(-ex / 2, ex / 2), rng.uniform(-ey / 2, ey / 2), rng.uniform(2.0, ez))
+        b = sc.add_body(DYNAMIC, pos, rng.quaternion(), lin=(0.3 * rng.normal(), 0.3 * rng.normal(), 0.0))
+        kind = i % 6
+        if kind == 0:
+            sc.add_collider(b, SPHERE, (rng.uniform(0.25, 0.6),))
+        elif kind == 1:
+            sc.add_collider(b, BOX, (rng.uniform(0.25, 0.6), rng.uniform(0.25, 0.6), rng.uniform(0.25, 0.6)))
+        elif kind == 2:
+            sc.add_collider(b, CAPSULE, (rng.uniform(0.2, 0.4), rng.uniform(0.5, 1.2)))
+        elif kind == 3:
+            sc.add_collider(b, HULL, (float(hull_small),), density=500.0)
+        elif kind == 4:   # only a trigger: falls through everything
+            shape = (SPHERE, CAPSULE, BOX)[i % 3]
+            params = {SPHERE: (0.4,), CAPSULE: (0.3, 0.8), BOX: (0.3, 0.3, 0.3)}[shape]
+            sc.add_collider(b, shape, params, trigger=True)
+        else:             # compound: a simulation box and a trigger sphere around it
+            sc.add_collider(b, BOX, (0.35, 0.35, 0.35))
+            sc.add_collider(b, SPHERE, (0.7,), trigger=True)
+            if i % 12 == 11:   # and a third collider that is neither (never tested)
+                sc.add_collider(b, SPHERE, (0.9,), lpos=(0.0, 0.0, 0.2), simulation=False)
     return sc
 
 

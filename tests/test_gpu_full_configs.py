@@ -4,7 +4,10 @@
   all 600 steps: it lands, piles up and settles; the ground's partner list overflows the traversal slab,
   so the overflow re-walk runs every step;
 * the 65,536-body container (spheres, boxes and capsules, N(0, 1) velocities, 5 static walls), 50 steps:
-  the per-colour solver layout, the tree broadphase over 65k colliders and every convex pair type.
+  the per-colour solver layout, the tree broadphase over 65k colliders and every convex pair type;
+* 16,384 convex hulls (QuickHull of 24 ellipsoid points each, built from the point clouds on both sides)
+  and 256 chains of 16 spheres on ball-and-socket joints hanging from static anchors, 50 steps:
+  polyhedron SAT, GJK, sphere-polyhedron and the joint solver.
 
 Each step: oriented pair sets equal, per-manifold point counts equal, contacts within 1e-5 relative,
 body states within 1e-4."""
@@ -27,6 +30,8 @@ def _run(sc, steps, report_every=100):
     t0 = time.time()
     dev = Batch(sc)
     ref = RefWorld(sc)
+    if sc.joints:
+        ref.set_joint_order(dev.joint_order())
     worst, peak = 0.0, {}
     for k in range(steps):
         dev.step(DT)
@@ -66,3 +71,12 @@ def test_mixed_container_65536_for_50_steps():
     c = dev.counts()
     print("mixed", worst, peak, c)
     assert sum(c["nb_tests"]) > 0 and c["nb_tests"][6] > 0
+
+
+def test_hulls_16384_and_chains_256x16_for_50_steps():
+    sc = S.hull_scene(16384, 256, 16, seed=3)
+    assert len(sc.hulls) == 16384 and len(sc.joints) == 256 * 16
+    dev, worst, peak = _run(sc, 50, report_every=10)
+    c = dev.counts()
+    print("hulls", worst, peak, c)
+    assert c["nb_tests"][6] > 0 and c["nb_joints"] == 256 * 16

@@ -414,3 +414,23 @@ def test_batch_harness_steps_each_world_as_its_own_reference(ref_available):
             assert np.array_equal(x, y), f"step {k}"
         p1, p2 = rb.pairs()
         assert len(p1) == sum(len(r.pairs()[0]) for r in own)
+
+
+def test_ordered_step_with_a_missing_manifold_fails_cleanly(ref_available):
+    """An order that lacks one of the reference's manifolds is reported, and the step still completes in
+    the reference's own order, so the world can be stepped on and destroyed (a step abandoned halfway
+    leaves frame-allocated arrays that the world's destructor cannot free)."""
+    from oracle.ref import RefWorld
+    sc = S.box_pile(3, 2, seed=21)
+    a, b = RefWorld(sc), RefWorld(sc)
+    for k in range(40):
+        a.step(DT)
+        ct = a.contacts()
+        if len(ct["nb_points"]) > 1:
+            with pytest.raises(RuntimeError, match="missing"):
+                b.step_ordered(DT, ct["colliders"][1:, 0], ct["colliders"][1:, 1])
+            break
+        b.step_ordered(DT, ct["colliders"][:, 0], ct["colliders"][:, 1])
+    for _ in range(5):
+        b.step(DT)
+    b.close()
