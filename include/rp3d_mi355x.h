@@ -372,6 +372,18 @@ int rp3dg_mlp_forward(rp3dg_mlp* mlp, const float* features, int64_t n, float* i
 /* Gradient of loss = 1/(2n) sum_s |y_s - target_s / out_scale|^2 over n <= 65536 samples. */
 int rp3dg_mlp_gradient(rp3dg_mlp* mlp, const float* features, const float* targets, int64_t n, float* gradient,
                        float* loss);
+/* No counterpart in the reference.  rp3dg_mlp_gradient for callers that cannot allocate device memory
+ * (code under HIP graph capture, tight training or serving loops, harnesses that forbid hipMalloc):
+ * features [n][24], targets [n][3], gradient [20099] and loss [1] are DEVICE pointers, and the
+ * per-block partial gradients go to a fixed device array of RP3DG_MLP_NOALLOC_MAX_SAMPLES / 64 partials
+ * (5.1 MB on each GPU the library runs on, held until the process ends) instead of library allocations.
+ * n must be in [1, RP3DG_MLP_NOALLOC_MAX_SAMPLES]; RP3DG_EINVAL with a message otherwise.  Same
+ * kernels and summation order as rp3dg_mlp_gradient, so the gradient and loss are identical.  Runs on
+ * the network's stream and returns when the outputs are written; the inputs must be ready on the device.
+ * Calls that overlap on one GPU (any networks) share the array and race. */
+#define RP3DG_MLP_NOALLOC_MAX_SAMPLES 4096
+int rp3dg_mlp_gradient_noalloc(rp3dg_mlp* mlp, const float* features, const float* targets, int64_t n,
+                               float* gradient, float* loss);
 /* Adam (beta 0.9 / 0.999) over shuffled minibatches, on the device. */
 int rp3dg_mlp_train(rp3dg_mlp* mlp, const float* features, const float* targets, int64_t n, int32_t epochs,
                     int32_t minibatch, float learning_rate, uint64_t seed, float* last_loss);

@@ -7,6 +7,8 @@
 // contraction index; those go through LDS.
 #include <hip/hip_runtime.h>
 
+#include <mutex>
+
 #include <cmath>
 #include <cstring>
 #include <string>
@@ -327,6 +329,7 @@ struct rp3dg_mlp {
     uint16_t *w1t = nullptr, *w2t = nullptr, *w2n = nullptr, *w3t = nullptr, *w3n = nullptr;
     float *partial = nullptr, *loss_partial = nullptr;
     int nparts_cap = 0;
+    float* noalloc_scratch = nullptr;  // device address of the fixed scratch array (device of creation)
     int64_t adam_t = 0;
     hipStream_t stream = nullptr;
     std::vector<void*> allocs;
@@ -374,6 +377,49 @@ static int mlp_refresh_copies(rp3dg_mlp* m) {
 }
 
 constexpr int kMaxParts = 1024;   // samples per gradient pass <= 65,536
+
+// Allocation-free gradient (rp3dg_mlp_gradient_noalloc): the partial gradients of its blocks and their
+// losses live in this fixed array instead of memory the library allocates.  Each GPU holds its own copy
+// until the process ends; host code uses the address hipGetSymbolAddress returns for the current device
+// (the array's name is not a device address on the host).  Every element a call reads is written by the
+// same call first, so nothing needs clearing; calls that overlap on one GPU share it and race.
+constexpr int kNoAllocParts = RP3DG_MLP_NOALLOC_MAX_SAMPLES / kChunk;
+static_assert(kNoAllocParts * kChunk == RP3DG_MLP_NOALLOC_MAX_SAMPLES, "sample limit must be whole chunks");
+static __device__ __align__(256) float scratch[(size_t)kNoAllocParts * kMlpParams + kNoAllocParts];
+
+constexpr int kMaxDevices = 64;
+static std::mutex g_scratch_mu;
+static float* g_scratch_addr[kMaxDevices] = {};
+
+// Device address of `scratch` on the current device, looked up once per device.
+static float* noalloc_scratch() {
+    int dev = -1;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= kMaxDevices) {
+        set_error("no current HIP device for the allocation-free gradient");
+        return nullptr;
+    }
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_addr[dev]) {
+        void* ptr = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !ptr) {
+            set_error(std::string("hipGetSymbolAddress(scratch) failed: ") + hipGetErrorString(e));
+            return nullptr;
+        }
+        g_scratch_addr[dev] = (float*)ptr;
+    }
+    return g_scratch_addr[dev];
+}
+
+// Mean loss from the per-block losses, summed in block order in double precision (as the host does for
+// rp3dg_mlp_gradient).
+__global__ void mlp_loss_mean(const float* loss_partial, int nparts, int n, float* loss) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        double s = 0.0;
+        for (int b = 0; b < nparts; b++) s += loss_partial[b];
+        loss[0] = (float)(s / n);
+    }
+}
 
 static int mlp_ensure_parts(rp3dg_mlp* m, int nparts) {
     if (nparts <= m->nparts_cap) return RP3DG_OK;
@@ -430,6 +476,8 @@ rp3dg_mlp* rp3dg_mlp_create(uint64_t seed) {
     m->w3t = mlp_alloc<uint16_t>(m, kMlpOutPad * kMlpHid);
     m->w3n = mlp_alloc<uint16_t>(m, kMlpHid * kMlpOutPad);
     if (!m->w3n || mlp_ensure_parts(m, 16)) { set_error("device allocation failed (mlp)"); delete m; return nullptr; }
+    m->noalloc_scratch = noalloc_scratch();
+    if (!m->noalloc_scratch) { delete m; return nullptr; }
     // Glorot-uniform weights from a splitmix64 stream, zero biases, identity normalisation
     std::vector<float> P(kMlpParams, 0.0f);
     uint64_t st = seed;
@@ -534,6 +582,26 @@ int rp3dg_mlp_gradient(rp3dg_mlp* m, const float* feats, const float* targets, i
     hipFree(df);
     hipFree(dt);
     return r;
+}
+
+int rp3dg_mlp_gradient_noalloc(rp3dg_mlp* m, const float* feats, const float* targets, int64_t n, float* grad,
+                               float* loss) {
+    if (!m || !feats || !targets || !grad || !loss) { set_error("mlp gradient (no alloc): null argument"); return RP3DG_EINVAL; }
+    if (n <= 0 || n > RP3DG_MLP_NOALLOC_MAX_SAMPLES) {
+        set_error("mlp gradient (no alloc): sample count must be in [1, " +
+                  std::to_string(RP3DG_MLP_NOALLOC_MAX_SAMPLES) + "], got " + std::to_string((long long)n));
+        return RP3DG_EINVAL;
+    }
+    float* partial = m->noalloc_scratch;
+    float* loss_partial = partial + (size_t)kNoAllocParts * kMlpParams;
+    const int nparts = (int)((n + kChunk - 1) / kChunk);
+    mlp_grad<<<nparts, 256, 0, m->stream>>>(m->view(), feats, targets, nullptr, (int)n, partial, loss_partial);
+    mlp_reduce_adam<<<(kMlpParams + 255) / 256, 256, 0, m->stream>>>(m->params, partial, nparts, grad, nullptr, nullptr,
+                                                                      0.0f, 0.9f, 0.999f, 1e-8f, 1.0f, 1.0f);
+    mlp_loss_mean<<<1, 64, 0, m->stream>>>(loss_partial, nparts, (int)n, loss);
+    MLPCHECK(hipGetLastError());
+    MLPCHECK(hipStreamSynchronize(m->stream));
+    return RP3DG_OK;
 }
 
 int rp3dg_mlp_train(rp3dg_mlp* m, const float* feats, const float* targets, int64_t n, int32_t epochs,

@@ -76,7 +76,7 @@ EXPORTED = [
     "rp3dg_mlp_set_params", "rp3dg_mlp_get_params", "rp3dg_mlp_set_normalization", "rp3dg_mlp_forward",
     "rp3dg_mlp_gradient", "rp3dg_mlp_train", "rp3dg_batch_set_warm_start", "rp3dg_batch_set_logging",
     "rp3dg_batch_get_log", "rp3dg_batch_autotune", "rp3dg_batch_get_tune_table", "rp3dg_world_get_body_state",
-    "rp3dg_batch_get_world_iterations", "rp3dg_batch_get_warm_start_predictions",
+    "rp3dg_batch_get_world_iterations", "rp3dg_batch_get_warm_start_predictions", "rp3dg_mlp_gradient_noalloc",
     "rp3dg_hull_create_from_points", "rp3dg_quickhull", "rp3dg_collider_set_is_trigger",
     "rp3dg_collider_set_is_simulation_collider", "rp3dg_collider_set_is_world_query_collider",
     "rp3dg_collider_get_kind",
@@ -154,6 +154,7 @@ def load():
                                                       C.POINTER(C.c_double)]),
         "rp3dg_batch_get_world_iterations": (C.c_int, [vp, iptr, fptr, i64]),
         "rp3dg_batch_get_warm_start_predictions": (i64, [vp, fptr, i64]),
+        "rp3dg_mlp_gradient_noalloc": (C.c_int, [vp, vp, vp, i64, vp, vp]),
         "rp3dg_hull_create_from_points": (i32, [vp, vp, i32, i32, i32]),
         "rp3dg_collider_set_is_trigger": (C.c_int, [vp, i32, i32]),
         "rp3dg_collider_set_is_simulation_collider": (C.c_int, [vp, i32, i32]),
