@@ -324,6 +324,10 @@ int rp3dg_batch_get_kernel_times(rp3dg_batch* batch, const char* kernel, double*
  * Per-colour layout: threads per block of the colour kernels, 64..512. */
 int rp3dg_batch_autotune(rp3dg_batch* batch, float time_step, int32_t steps_per_trial, int32_t* chosen,
                          double* best_ms_per_step);
+/* Sets the solver launch shape to an autotuner choice (fused layout: the class workgroup divisor 1, 2, 4, 8
+ * or 16; per-colour layout: threads per block, a multiple of 64 in 64..512), e.g. to time two batches
+ * with the same shape.  Drops captured step graphs. */
+int rp3dg_batch_set_solver_shape(rp3dg_batch* batch, int32_t choice);
 /* The trials of the last rp3dg_batch_autotune call: up to capacity (choice, ms per step) pairs;
  * returns the number of trials. */
 int32_t rp3dg_batch_get_tune_table(rp3dg_batch* batch, int32_t* choices, double* ms_per_step, int32_t capacity);

@@ -91,6 +91,8 @@ struct rp3dg_batch {
     float res_thr = 0.0f;              // residual mode threshold (0: fixed iterations)
     int res_max_iters = 0;
     int max_world_bodies = 0;
+    int max_lds_per_block = 65536;     // hipDeviceAttributeMaxSharedMemoryPerBlock of the batch's device
+    int max_lds_per_cu = 65536;        // hipDeviceAttributeMaxSharedMemoryPerMultiprocessor
 
     // launch hints from the last completed step (grids are grid-stride, any size is correct)
     int hint_pairs = 0, hint_manifolds = 0, hint_rounds = 8;

@@ -445,8 +445,10 @@ __global__ void bp_copy_slabs(DevColliders C, int n, const int* count, const int
         }
         const int total = count[c];
         if (total > kTravSlab || k >= total) continue;
-        if (base + total > capacity) {
-            if (k == 0) atomicOr(status, STATUS_PAIR_OVERFLOW);
+        // past the capacity the table is truncated (and the step reports the overflow), but every entry
+        // below it is written, since the merge reads all of them
+        if (base + k >= capacity) {
+            atomicOr(status, STATUS_PAIR_OVERFLOW);
             continue;
         }
         out_keys[base + k] = slab[(size_t)c * kTravSlab + k];

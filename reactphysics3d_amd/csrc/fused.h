@@ -32,8 +32,15 @@ struct FusedArgs {
     int* nrows;                // feature rows written this step
     int feat_cap;
     int lds_bodies;            // worlds with at most this many bodies keep cv, cw, sv, sw, the world inverse
-                               // inertia rows and the flags in LDS (116 * lds_bodies bytes); 0 = HBM
+                               // inertia rows and the flags in LDS (100 * lds_bodies bytes); 0 = HBM
+    int lds_residual;          // residual mode: the previous sweep's velocities in LDS too (32 B per body
+                               // more) instead of pv / pw
 };
+
+// LDS bytes of a fused solver workgroup staging `bodies` bodies (with the residual's previous velocities)
+__host__ __device__ constexpr size_t fused_lds_bytes(int bodies, bool residual) {
+    return (size_t)bodies * (6 * sizeof(float4) + sizeof(int)) + (residual ? (size_t)bodies * 2 * sizeof(float4) : 0);
+}
 
 constexpr int kFeatures = 24;
 

@@ -43,9 +43,11 @@ __device__ __forceinline__ float block_maxf(float v, float* red) {
 // Warm start, velocity iterations and impulse store of one group.  Instantiated twice: with the body
 // state in LDS (A.B's velocity, inertia and flag arrays point into the workgroup's LDS, so every access
 // compiles to an LDS instruction) and with it in HBM (groups larger than the LDS budget).
+// PV / PW: the velocities after the previous sweep (residual mode), indexed by body - b0, in LDS or HBM.
 __device__ __forceinline__ void group_sweeps(const FusedArgs& F, const SvArgs& A, const DevBodies& B, bool staged,
                                              int w0, int ngw, int b0, int b1, int m1, const int* seg, const int2* jr,
-                                             float* red, unsigned int* s_wres, unsigned long long* s_done) {
+                                             float* red, unsigned int* s_wres, unsigned long long* s_done,
+                                             float4* PV, float4* PW) {
     const int tid = threadIdx.x, nt = blockDim.x;
     const int nbw = b1 - b0;
     const DevBodies& BS = A.B;
@@ -72,8 +74,8 @@ __device__ __forceinline__ void group_sweeps(const FusedArgs& F, const SvArgs& A
     auto wk = [&](int b) { return staged ? (BS.flags[b - BS.vbase] >> kGroupShift) & 63 : B.world[b] - w0; };
     if (residual) {
         for (int i = b0 + tid; i < b1; i += nt) {
-            F.pv[i] = BS.cv[i - BS.vbase];
-            F.pw[i] = BS.cw[i - BS.vbase];
+            PV[i - b0] = BS.cv[i - BS.vbase];
+            PW[i - b0] = BS.cw[i - BS.vbase];
         }
         if (masked)
             for (int k = tid; k < ngw; k += nt) {
@@ -114,7 +116,7 @@ __device__ __forceinline__ void group_sweeps(const FusedArgs& F, const SvArgs& A
                     k = wk(i);
                     if ((done >> k) & 1ull) continue;
                 }
-                const float4 v = BS.cv[i - BS.vbase], wv = BS.cw[i - BS.vbase], pv = F.pv[i], pw = F.pw[i];
+                const float4 v = BS.cv[i - BS.vbase], wv = BS.cw[i - BS.vbase], pv = PV[i - b0], pw = PW[i - b0];
                 float d = fabsf(v.x - pv.x);
                 d = fmaxf(d, fabsf(v.y - pv.y));
                 d = fmaxf(d, fabsf(v.z - pv.z));
@@ -123,8 +125,8 @@ __device__ __forceinline__ void group_sweeps(const FusedArgs& F, const SvArgs& A
                 d = fmaxf(d, fabsf(wv.z - pw.z));
                 if (masked) atomicMax(&s_wres[k], __float_as_uint(d));   // d >= 0: bit order = value order
                 else r = fmaxf(r, d);
-                F.pv[i] = v;
-                F.pw[i] = wv;
+                PV[i - b0] = v;
+                PW[i - b0] = wv;
             }
             if (!masked) {
                 res = block_maxf(r, red);
@@ -171,7 +173,8 @@ __global__ __launch_bounds__(kFusedMaxBlock) void sv_world_solve(FusedArgs F) {
     __shared__ float red[16];
     __shared__ unsigned int s_wres[kGroupMaxWorlds];   // residual of each world of the group (float bits)
     __shared__ unsigned long long s_done;               // converged worlds of the group
-    extern __shared__ float4 s_vel[];   // [7][lds_bodies] cv, cw, sv, sw, iw0, iw1, iw2; then [lds_bodies] flags
+    extern __shared__ float4 s_vel[];   // [6][lds_bodies] cv, cw, sv, sw (w: third inertia row), iw0, iw1;
+                                        // [lds_bodies] flags; with lds_residual [2][lds_bodies] previous cv, cw
     const SvArgs& AG = F.A;
     const DevBodies& B = AG.B;
     const int g = F.block_world[blockIdx.x];
@@ -227,32 +230,41 @@ __global__ __launch_bounds__(kFusedMaxBlock) void sv_world_solve(FusedArgs F) {
         if (F.phase == 0) return;
     }
     // stage the group's solver velocities, world inverse inertia rows and flags in LDS for the sweeps
-    // (the rows and the rest of the body data stay in HBM)
+    // (the rows and the rest of the body data stay in HBM); the third inertia row rides in the w lanes of
+    // cw, sv and sw: 100 bytes a body, so six 256-body groups fit the 160 KB of a CU (five at 116 bytes)
     const int nbw = b1 - b0;
     if (nbw <= F.lds_bodies) {
         const int L = F.lds_bodies;
         float4 *lcv = s_vel, *lcw = s_vel + L, *lsv = s_vel + 2 * L, *lsw = s_vel + 3 * L;
-        float4 *li0 = s_vel + 4 * L, *li1 = s_vel + 5 * L, *li2 = s_vel + 6 * L;
-        int* lfl = reinterpret_cast<int*>(s_vel + 7 * L);
+        float4 *li0 = s_vel + 4 * L, *li1 = s_vel + 5 * L;
+        int* lfl = reinterpret_cast<int*>(s_vel + 6 * L);
         for (int i = tid; i < nbw; i += nt) {
             lfl[i] = B.flags[b0 + i] | ((B.world[b0 + i] - w0) << kGroupShift);
+            const float4 r2 = B.iw2[b0 + i];
+            float4 t = B.cw[b0 + i];
+            t.w = r2.x;
+            lcw[i] = t;
+            t = B.sv[b0 + i];
+            t.w = r2.y;
+            lsv[i] = t;
+            t = B.sw[b0 + i];
+            t.w = r2.z;
+            lsw[i] = t;
             lcv[i] = B.cv[b0 + i];
-            lcw[i] = B.cw[b0 + i];
-            lsv[i] = B.sv[b0 + i];
-            lsw[i] = B.sw[b0 + i];
             li0[i] = B.iw0[b0 + i];
             li1[i] = B.iw1[b0 + i];
-            li2[i] = B.iw2[b0 + i];
         }
         SvArgs A = AG;
         A.B.cv = lcv; A.B.cw = lcw; A.B.sv = lsv; A.B.sw = lsw;
-        A.B.iw0 = li0; A.B.iw1 = li1; A.B.iw2 = li2;
+        A.B.iw0 = li0; A.B.iw1 = li1; A.B.iw2 = nullptr;
         A.B.flags = lfl;
         A.B.vbase = b0;
+        float4* lpv = F.lds_residual ? reinterpret_cast<float4*>(lfl + L) : F.pv + b0;
+        float4* lpw = F.lds_residual ? lpv + L : F.pw + b0;
         __syncthreads();
-        group_sweeps(F, A, B, true, w0, ngw, b0, b1, m1, seg, jr, red, s_wres, &s_done);
+        group_sweeps(F, A, B, true, w0, ngw, b0, b1, m1, seg, jr, red, s_wres, &s_done, lpv, lpw);
     } else {
-        group_sweeps(F, AG, B, false, w0, ngw, b0, b1, m1, seg, jr, red, s_wres, &s_done);
+        group_sweeps(F, AG, B, false, w0, ngw, b0, b1, m1, seg, jr, red, s_wres, &s_done, F.pv + b0, F.pw + b0);
     }
 }
 

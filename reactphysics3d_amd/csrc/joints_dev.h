@@ -361,8 +361,8 @@ __device__ __forceinline__ void jt_warmstart_one(const DevBodies& B, const DevJo
             v2 += lock_mul((im2 * ll2), li2);
             w2 += cmul(al2, mvmul(I2, ai2));
         }
-        B.cv[u1] = f4(v1, 0.f); B.cw[u1] = f4(w1, 0.f);
-        B.cv[u2] = f4(v2, 0.f); B.cw[u2] = f4(w2, 0.f);
+        B.cv[u1] = f4(v1, 0.f); st3(B.cw, u1, w1);
+        B.cv[u2] = f4(v2, 0.f); st3(B.cw, u2, w2);
     }
 }
 
@@ -562,8 +562,8 @@ __device__ __forceinline__ void jt_solve_velocity_one(const DevBodies& B, const 
             v2 += lock_mul((im2 * ll2), -li1);
             w2 += cmul(al2, mvmul(I2, ai2));
         }
-        B.cv[u1] = f4(v1, 0.f); B.cw[u1] = f4(w1, 0.f);
-        B.cv[u2] = f4(v2, 0.f); B.cw[u2] = f4(w2, 0.f);
+        B.cv[u1] = f4(v1, 0.f); st3(B.cw, u1, w1);
+        B.cv[u2] = f4(v2, 0.f); st3(B.cw, u2, w2);
     }
 }
 

@@ -13,6 +13,13 @@ constexpr int kWave = 64;
 __device__ __forceinline__ V3 ld3(const float4* a, int i) { float4 t = a[i]; return v3(t.x, t.y, t.z); }
 __device__ __forceinline__ Q4 ldq(const float4* a, int i) { float4 t = a[i]; return q4(t.x, t.y, t.z, t.w); }
 __device__ __forceinline__ float4 f4(V3 v, float w) { return make_float4(v.x, v.y, v.z, w); }
+// stores x, y, z of a float4 slot and leaves its w lane as it is
+__device__ __forceinline__ void st3(float4* a, int i, V3 v) {
+    float4& r = a[i];
+    r.x = v.x;
+    r.y = v.y;
+    r.z = v.z;
+}
 __device__ __forceinline__ float4 f4q(Q4 q) { return make_float4(q.x, q.y, q.z, q.w); }
 __device__ __forceinline__ V3 xyz(float4 t) { return v3(t.x, t.y, t.z); }
 

@@ -117,10 +117,13 @@ __device__ __forceinline__ void dyn_update_state_one(const DevBodies& B, int i) 
 // ------------------------------------------------------------------------------------------------
 
 // world inverse inertia rows, indexed [body - vbase] like the velocities (staged in LDS per world)
+// World inverse inertia of body b.  Staged in LDS (iw2 == nullptr), its third row lives in the w lanes of
+// cw, sv and sw, which the solver rows therefore store with st3 (x, y, z only).
 __device__ __forceinline__ M3 body_iw(const DevBodies& B, int b) {
     const int u = b - B.vbase;
     M3 m;
-    m.r[0] = ld3(B.iw0, u); m.r[1] = ld3(B.iw1, u); m.r[2] = ld3(B.iw2, u);
+    m.r[0] = ld3(B.iw0, u); m.r[1] = ld3(B.iw1, u);
+    m.r[2] = B.iw2 ? ld3(B.iw2, u) : v3(B.cw[u].w, B.sv[u].w, B.sw[u].w);
     return m;
 }
 
@@ -319,8 +322,8 @@ __device__ __forceinline__ void sv_warmstart_one(const SvArgs& A, int i) {
             imp.x = imp.y = imp.z = 0.0f;
         }
         S.imp[i] = imp;
-        if (d1) { B.cv[u1] = f4(v1, 0.f); B.cw[u1] = f4(w1, 0.f); }
-        if (d2) { B.cv[u2] = f4(v2, 0.f); B.cw[u2] = f4(w2, 0.f); }
+        if (d1) { B.cv[u1] = f4(v1, 0.f); st3(B.cw, u1, w1); }
+        if (d2) { B.cv[u2] = f4(v2, 0.f); st3(B.cw, u2, w2); }
     }
 }
 
@@ -484,12 +487,12 @@ __device__ __forceinline__ void solve_manifold_t(const SvArgs& A, int i, float b
     }
     S.imp[i] = imp;
     if (d1) {
-        B.cv[u1] = f4(v1, 0.f); B.cw[u1] = f4(w1, 0.f);
-        B.sv[u1] = f4(sv1, 0.f); B.sw[u1] = f4(sw1, 0.f);
+        B.cv[u1] = f4(v1, 0.f); st3(B.cw, u1, w1);
+        st3(B.sv, u1, sv1); st3(B.sw, u1, sw1);
     }
     if (d2) {
-        B.cv[u2] = f4(v2, 0.f); B.cw[u2] = f4(w2, 0.f);
-        B.sv[u2] = f4(sv2, 0.f); B.sw[u2] = f4(sw2, 0.f);
+        B.cv[u2] = f4(v2, 0.f); st3(B.cw, u2, w2);
+        st3(B.sv, u2, sv2); st3(B.sw, u2, sw2);
     }
 }
 

@@ -79,7 +79,7 @@ EXPORTED = [
     "rp3dg_batch_get_world_iterations", "rp3dg_batch_get_warm_start_predictions", "rp3dg_mlp_gradient_noalloc",
     "rp3dg_hull_create_from_points", "rp3dg_quickhull", "rp3dg_collider_set_is_trigger",
     "rp3dg_collider_set_is_simulation_collider", "rp3dg_collider_set_is_world_query_collider",
-    "rp3dg_collider_get_kind",
+    "rp3dg_collider_get_kind", "rp3dg_batch_set_solver_shape",
 ]
 
 MLP_PARAMS = 20099
@@ -160,6 +160,7 @@ def load():
         "rp3dg_collider_set_is_simulation_collider": (C.c_int, [vp, i32, i32]),
         "rp3dg_collider_set_is_world_query_collider": (C.c_int, [vp, i32, i32]),
         "rp3dg_collider_get_kind": (C.c_int, [vp, i32]),
+        "rp3dg_batch_set_solver_shape": (C.c_int, [vp, i32]),
         "rp3dg_quickhull": (C.c_int, [vp, i32, i32, i32, fptr, i32, iptr, i32, iptr, i32, iptr, iptr]),
         "rp3dg_mlp_create": (vp, [C.c_uint64]),
         "rp3dg_mlp_destroy": (None, [vp]),
@@ -463,6 +464,9 @@ class Batch:
         _check(load().rp3dg_batch_autotune(self.h, float(dt), int(steps_per_trial), C.byref(c), C.byref(ms)),
                "autotune")
         return c.value, ms.value
+
+    def set_solver_shape(self, choice):
+        _check(load().rp3dg_batch_set_solver_shape(self.h, int(choice)), "set_solver_shape")
 
     def tune_table(self):
         """(choice, ms per step) of every trial of the last autotune call."""

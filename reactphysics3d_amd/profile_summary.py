@@ -71,6 +71,7 @@ def main(argv=None):
     ap.add_argument("--write")
     ap.add_argument("--copy-bytes", type=float, default=0.0)
     ap.add_argument("--command", default="")
+    ap.add_argument("--commit", default="", help="commit the measured tree was at (stamped into --traffic-json)")
     ap.add_argument("--top", type=int, default=25)
     ap.add_argument("--traffic-json", help="write {kernel: scaled HBM bytes per dispatch} to this file")
     ap.add_argument("--min-us", action="append", default=[], metavar="KERNEL=US")
@@ -122,6 +123,8 @@ def main(argv=None):
         if a.traffic_json and scale_f and scale_w:
             import json
             tj = {k: int(f[k][0] * 1024 * scale_f + w[k][0] * 1024 * scale_w) for k in set(f) & set(w)}
+            tj["_measured"] = {"commit": a.commit, "command": a.command,
+                               "fetch": a.fetch, "write": a.write, "copy_bytes": a.copy_bytes}
             with open(a.traffic_json, "w") as fh:
                 json.dump(tj, fh, indent=1, sort_keys=True)
     sys.stdout.write("\n".join(out) + "\n")

@@ -27,8 +27,16 @@ def euler_quat(ax, ay, az):
     return q / np.linalg.norm(q)
 
 
-def _shape(spec):
+# the fixture's convex meshes (TestWorldQueries.h:377-434): a box given as 8 vertices and 6 quads
+_CUBE_SIGNS = [(-1, -1, 1), (1, -1, 1), (1, -1, -1), (-1, -1, -1), (-1, 1, 1), (1, 1, 1), (1, 1, -1), (-1, 1, -1)]
+_CUBE_FACES = [[0, 3, 2, 1], [4, 5, 6, 7], [0, 1, 5, 4], [1, 2, 6, 5], [2, 3, 7, 6], [0, 4, 7, 3]]
+
+
+def _shape(sc, spec):
     kind, params = spec
+    if kind == "convex":
+        v = np.array([[sx * params[0], sy * params[1], sz * params[2]] for sx, sy, sz in _CUBE_SIGNS], np.float32)
+        return S.HULL, (float(sc.add_hull(v, _CUBE_FACES)),)
     return {"sphere": S.SPHERE, "capsule": S.CAPSULE, "box": S.BOX}[kind], tuple(params)
 
 
@@ -36,10 +44,12 @@ def scene(case):
     sc = S.Scene(case["name"], S.make_settings(gravity=(0.0, 0.0, 0.0)))
     q1 = tuple(euler_quat(*case["euler1"]).astype(np.float32))
     q2 = tuple(euler_quat(*case["euler2"]).astype(np.float32))
+    s1 = _shape(sc, case["shape1"])
+    s2 = _shape(sc, case["shape2"])
     b1 = sc.add_body(S.DYNAMIC, case["pos1"], q1)
-    sc.add_collider(b1, *_shape(case["shape1"]))
+    sc.add_collider(b1, *s1)
     b2 = sc.add_body(S.STATIC, case["pos2"], q2)
-    sc.add_collider(b2, *_shape(case["shape2"]))
+    sc.add_collider(b2, *s2)
     return sc, q2
 
 

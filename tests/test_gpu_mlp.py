@@ -71,13 +71,11 @@ def _mean_iterations(dev, steps):
     return float(np.mean(its))
 
 
-def test_learned_warm_start_predicts_impulses():
-    """The network trained on logged (features, converged impulse) samples predicts the normal impulse
-    of new contacts on held-out worlds better than the plain warm start's zero, and the learned warm
-    start runs the residual solver to within a few percent of the plain iteration count (the effect
-    on iterations is small: only new contacts are predicted)."""
+@pytest.fixture(scope="module")
+def learned_setup():
+    """A network trained on logged (features, converged impulse) samples of the plain residual solver on
+    held-out seeds, and the held-out worlds it is judged on."""
     from reactphysics3d_amd.lib import Batch, Mlp
-    # training data: converged impulses of new contacts, plain solver, held-out seeds
     train = [S.batched_world(w, seed=1000, min_bodies=16, max_bodies=160, max_joints=8) for w in range(96)]
     log = Batch(train, solver="fused")
     log.set_residual_mode(1e-4, 200)
@@ -92,7 +90,14 @@ def test_learned_warm_start_predicts_impulses():
     m.set_normalization(F.mean(0), 1.0 / (F.std(0) + 1e-6), T.std(0) + 1e-6)
     loss = m.train(F, T, epochs=20, minibatch=1024, lr=1e-3, seed=9)
     test = [S.batched_world(w, seed=2000, min_bodies=16, max_bodies=160, max_joints=8) for w in range(48)]
-    # held-out samples: the normal impulse is predicted better than by zero
+    return m, loss, len(F), test
+
+
+def test_learned_warm_start_predicts_impulses(learned_setup):
+    """On held-out worlds the trained network predicts the converged normal impulse of new contacts better
+    than the plain warm start's zero."""
+    from reactphysics3d_amd.lib import Batch
+    m, loss, nsamples, test = learned_setup
     hold = Batch(test, solver="fused")
     hold.set_residual_mode(1e-4, 200)
     hold.set_logging(200_000)
@@ -101,6 +106,18 @@ def test_learned_warm_start_predicts_impulses():
     P = m.forward(Fh)
     mse_net = float(np.mean((P[:, 0] - Th[:, 0]) ** 2))
     mse_zero = float(np.mean(Th[:, 0] ** 2))
+    print(f"samples {nsamples} loss {loss:.4g} held-out normal-impulse MSE net {mse_net:.4g} zero {mse_zero:.4g}")
+    assert mse_net < 0.8 * mse_zero
+
+
+@pytest.mark.xfail(strict=False, reason="measured: the learned warm start saves about 1 % of the sweeps to the "
+                                        "residual (only new contacts are predicted; tall stacks of cached "
+                                        "contacts set the sweep count), not the 30 % targeted")
+def test_learned_warm_start_reaches_the_residual_in_30_percent_fewer_sweeps(learned_setup):
+    """The target of the learned warm start: the residual 1e-4 in at least 30 % fewer sweeps than the plain
+    warm start on the held-out worlds, from the same initial state, with the same launch shape."""
+    from reactphysics3d_amd.lib import Batch
+    m, _, _, test = learned_setup
     plain = Batch(test, solver="fused")
     learned = Batch(test, solver="fused")
     learned.set_warm_start("learned", m)
@@ -108,7 +125,5 @@ def test_learned_warm_start_predicts_impulses():
         d.set_residual_mode(1e-4, 200)
     it_plain = _mean_iterations(plain, 90)
     it_learned = _mean_iterations(learned, 90)
-    print(f"samples {len(F)} loss {loss:.4g} held-out normal-impulse MSE net {mse_net:.4g} zero {mse_zero:.4g}; "
-          f"iterations plain {it_plain:.2f} learned {it_learned:.2f} ({100 * (1 - it_learned / it_plain):.1f}% fewer)")
-    assert mse_net < 0.8 * mse_zero
-    assert it_learned < 1.03 * it_plain
+    print(f"iterations plain {it_plain:.2f} learned {it_learned:.2f} ({100 * (1 - it_learned / it_plain):.1f}% fewer)")
+    assert it_learned <= 0.7 * it_plain
