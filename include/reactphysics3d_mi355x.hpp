@@ -7,7 +7,9 @@
 // index) into the host world behind the C-ABI; getters after update() read a host mirror that the
 // library refreshes from the device once per step.  Only what the step path needs is provided:
 // sphere / box / capsule / convex-mesh colliders, ball-and-socket / hinge / slider / fixed joints with
-// their limits and motors, materials, collision filters, velocities, damping, gravity, sleeping.
+// their limits and motors, materials, collision filters, triggers, velocities, damping, gravity,
+// sleeping, and the RigidBody calls between steps (setTransform, velocities, forces and torques at the
+// centre of mass or at a point, resets, setIsSleeping).
 #pragma once
 
 #include <cmath>
@@ -201,7 +203,49 @@ class RigidBody {
         return mColliders.back().get();
     }
     void updateMassPropertiesFromColliders() { check(rp3dg_body_update_mass_properties(mWorld, mIndex), "updateMassProperties"); }
-    void setLinearVelocity(const Vector3& v) { mLin = v; pushVelocity(); }
+    // RigidBody::setLinearVelocity / setAngularVelocity (RigidBody.cpp:794-828)
+    void setLinearVelocity(const Vector3& v) {
+        const float a[3] = {v.x, v.y, v.z};
+        check(rp3dg_body_set_velocity(mWorld, mIndex, a, nullptr), "setLinearVelocity");
+    }
+    // RigidBody::setTransform (RigidBody.cpp:837-866)
+    void setTransform(const Transform& t) {
+        const float p[3] = {t.getPosition().x, t.getPosition().y, t.getPosition().z};
+        const Quaternion& q = t.getOrientation();
+        const float o[4] = {q.x, q.y, q.z, q.w};
+        check(rp3dg_body_set_transform(mWorld, mIndex, p, o), "setTransform");
+    }
+    // RigidBody::apply{World,Local}ForceAt{CenterOfMass,WorldPosition,LocalPosition} (RigidBody.cpp:151-311)
+    void applyWorldForceAtCenterOfMass(const Vector3& f) { force(f, nullptr, 0); }
+    void applyLocalForceAtCenterOfMass(const Vector3& f) { force(f, nullptr, RP3DG_FORCE_LOCAL); }
+    void applyWorldForceAtWorldPosition(const Vector3& f, const Vector3& p) { force(f, &p, RP3DG_FORCE_AT_POINT); }
+    void applyLocalForceAtWorldPosition(const Vector3& f, const Vector3& p) {
+        force(f, &p, RP3DG_FORCE_AT_POINT | RP3DG_FORCE_LOCAL);
+    }
+    void applyWorldForceAtLocalPosition(const Vector3& f, const Vector3& p) {
+        force(f, &p, RP3DG_FORCE_AT_POINT | RP3DG_FORCE_POINT_LOCAL);
+    }
+    void applyLocalForceAtLocalPosition(const Vector3& f, const Vector3& p) {
+        force(f, &p, RP3DG_FORCE_AT_POINT | RP3DG_FORCE_POINT_LOCAL | RP3DG_FORCE_LOCAL);
+    }
+    // RigidBody::applyWorldTorque / applyLocalTorque / resetForce / resetTorque (RigidBody.cpp:938-987)
+    void applyWorldTorque(const Vector3& t) { torque(t, 0); }
+    void applyLocalTorque(const Vector3& t) { torque(t, RP3DG_FORCE_LOCAL); }
+    void resetForce() { check(rp3dg_body_reset_force_torque(mWorld, mIndex, 1, 0), "resetForce"); }
+    void resetTorque() { check(rp3dg_body_reset_force_torque(mWorld, mIndex, 0, 1), "resetTorque"); }
+    // RigidBody::getForce / getTorque (RigidBody.cpp:993-1003)
+    Vector3 getForce() const {
+        float f[3];
+        check(rp3dg_body_get_force_torque(mWorld, mIndex, f, nullptr), "getForce");
+        return Vector3(f[0], f[1], f[2]);
+    }
+    Vector3 getTorque() const {
+        float t[3];
+        check(rp3dg_body_get_force_torque(mWorld, mIndex, nullptr, t), "getTorque");
+        return Vector3(t[0], t[1], t[2]);
+    }
+    // RigidBody::setIsSleeping (RigidBody.cpp:1006-1057)
+    void setIsSleeping(bool sleeping) { check(rp3dg_body_set_sleeping(mWorld, mIndex, sleeping ? 1 : 0), "setIsSleeping"); }
     // RigidBody::setIsAllowedToSleep / isSleeping (RigidBody.cpp:1006-1170)
     void setIsAllowedToSleep(bool allowed) {
         check(rp3dg_body_set_allowed_to_sleep(mWorld, mIndex, allowed ? 1 : 0), "setIsAllowedToSleep");
@@ -211,7 +255,10 @@ class RigidBody {
         check(r, "isSleeping");
         return r == 1;
     }
-    void setAngularVelocity(const Vector3& w) { mAng = w; pushVelocity(); }
+    void setAngularVelocity(const Vector3& w) {
+        const float a[3] = {w.x, w.y, w.z};
+        check(rp3dg_body_set_velocity(mWorld, mIndex, nullptr, a), "setAngularVelocity");
+    }
     void enableGravity(bool on) { check(rp3dg_body_enable_gravity(mWorld, mIndex, on ? 1 : 0), "enableGravity"); }
     void setLinearDamping(decimal d) { mLinDamp = d; check(rp3dg_body_set_damping(mWorld, mIndex, mLinDamp, mAngDamp), "damping"); }
     void setAngularDamping(decimal d) { mAngDamp = d; check(rp3dg_body_set_damping(mWorld, mIndex, mLinDamp, mAngDamp), "damping"); }
@@ -234,14 +281,18 @@ class RigidBody {
 
    private:
     friend class PhysicsWorld;
-    void pushVelocity() {
-        const float v[3] = {mLin.x, mLin.y, mLin.z}, w[3] = {mAng.x, mAng.y, mAng.z};
-        check(rp3dg_body_set_velocity(mWorld, mIndex, v, w), "setVelocity");
+    void force(const Vector3& f, const Vector3* p, int32_t flags) {
+        const float a[3] = {f.x, f.y, f.z};
+        const float c[3] = {p ? p->x : 0.f, p ? p->y : 0.f, p ? p->z : 0.f};
+        check(rp3dg_body_apply_force(mWorld, mIndex, a, p ? c : nullptr, flags), "applyForce");
+    }
+    void torque(const Vector3& t, int32_t flags) {
+        const float a[3] = {t.x, t.y, t.z};
+        check(rp3dg_body_apply_torque(mWorld, mIndex, a, flags), "applyTorque");
     }
     rp3dg_world* mWorld = nullptr;
     int32_t mIndex = -1;
     decimal mDefaultFriction = decimal(0.3), mDefaultBounciness = decimal(0.5);
-    Vector3 mLin, mAng;
     decimal mLinDamp = 0, mAngDamp = 0;
     std::vector<std::unique_ptr<Collider>> mColliders;
 };

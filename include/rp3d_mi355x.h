@@ -87,13 +87,31 @@ int rp3dg_world_set_gravity(rp3dg_world* world, const float gravity[3]);
 int32_t rp3dg_body_create(rp3dg_world* world, const float position[3], const float orientation[4]);
 /* RigidBody::setType (src/body/RigidBody.cpp:66-130). */
 int rp3dg_body_set_type(rp3dg_world* world, int32_t body, int32_t type);
-/* RigidBody::setLinearVelocity / setAngularVelocity (src/body/RigidBody.cpp). */
+/* RigidBody::setLinearVelocity / setAngularVelocity (src/body/RigidBody.cpp:794-828); NULL leaves one as
+ * is.  This and the other RigidBody calls below (velocity, transform, forces, torques, sleeping) may be
+ * made between update() calls of a world stepped with rp3dg_world_update: they are queued for its
+ * device batch (see rp3dg_batch_body_set_transform).  For a world in a shared batch they return
+ * RP3DG_ESTATE after the batch exists; use the rp3dg_batch_body_* calls instead. */
 int rp3dg_body_set_velocity(rp3dg_world* world, int32_t body, const float linear[3], const float angular[3]);
+/* RigidBody::setTransform (src/body/RigidBody.cpp:837-866). */
+int rp3dg_body_set_transform(rp3dg_world* world, int32_t body, const float position[3], const float orientation[4]);
+/* RigidBody::apply{World,Local}ForceAt{CenterOfMass,WorldPosition,LocalPosition}
+ * (src/body/RigidBody.cpp:151-239, 298-311), flags RP3DG_FORCE_* as for rp3dg_batch_body_apply_force. */
+int rp3dg_body_apply_force(rp3dg_world* world, int32_t body, const float force[3], const float point[3], int32_t flags);
+/* RigidBody::applyWorldTorque / applyLocalTorque (src/body/RigidBody.cpp:938-967). */
+int rp3dg_body_apply_torque(rp3dg_world* world, int32_t body, const float torque[3], int32_t flags);
+/* RigidBody::resetForce / resetTorque (src/body/RigidBody.cpp:970-987). */
+int rp3dg_body_reset_force_torque(rp3dg_world* world, int32_t body, int32_t force, int32_t torque);
+/* RigidBody::getForce / getTorque (src/body/RigidBody.cpp:993-1003). */
+int rp3dg_body_get_force_torque(rp3dg_world* world, int32_t body, float force[3], float torque[3]);
+/* RigidBody::setIsSleeping (src/body/RigidBody.cpp:1006-1057), after the first update (bodies are
+ * created awake; RP3DG_EUNSUPPORTED to put one to sleep before). */
+int rp3dg_body_set_sleeping(rp3dg_world* world, int32_t body, int32_t sleeping);
 /* RigidBody::setLinearDamping / setAngularDamping. */
 int rp3dg_body_set_damping(rp3dg_world* world, int32_t body, float linear, float angular);
 /* RigidBody::enableGravity. */
 int rp3dg_body_enable_gravity(rp3dg_world* world, int32_t body, int32_t enabled);
-/* RigidBody::applyWorldForceAtCenterOfMass / applyLocalTorque accumulate until the next update(). */
+/* RigidBody::applyWorldForceAtCenterOfMass + applyWorldTorque: accumulate until the next update(). */
 int rp3dg_body_apply_force_torque(rp3dg_world* world, int32_t body, const float force[3], const float torque[3]);
 /* RigidBody::setLinearLockAxisFactor / setAngularLockAxisFactor. */
 int rp3dg_body_set_lock_factors(rp3dg_world* world, int32_t body, const float linear[3], const float angular[3]);
@@ -295,6 +313,45 @@ rp3dg_batch* rp3dg_world_get_batch(rp3dg_world* world);
  * on every dynamic body of the batch whose force / torque (xyz per body, NULL = none) is not zero,
  * before the next step (the step clears the accumulated forces); a sleeping body is woken. */
 int rp3dg_batch_apply_forces(rp3dg_batch* batch, const float* force, const float* torque);
+/* RigidBody calls on one body of a stepped batch (global body index), for applications that drive
+ * bodies between update() calls.  Calls are queued in the order they are made and applied on the
+ * device, in that order, before the next step, graph replay, raycast or read-back, so a sequence of
+ * calls on one body composes as the reference's calls do.  rp3dg_world_* setters of a world stepped
+ * with rp3dg_world_update go through the same queue.
+ * RigidBody::setTransform (src/body/RigidBody.cpp:837-866, Body.cpp:397-407): the centre of mass
+ * follows, the linear velocity gains angular x (new - old centre of mass), the body wakes and its
+ * colliders are refit (a shape that left its fat AABB gets a new one and joins the moved set).  A static
+ * body's transform is fixed once the batch exists (RP3DG_EUNSUPPORTED): its static-static pairs are
+ * listed at creation. */
+int rp3dg_batch_body_set_transform(rp3dg_batch* batch, int32_t body, const float position[3],
+                                   const float orientation[4]);
+/* RigidBody::setLinearVelocity / setAngularVelocity (src/body/RigidBody.cpp:794-828): ignored for a
+ * static body, a non-zero velocity wakes the body; NULL leaves that velocity as is. */
+int rp3dg_batch_body_set_velocity(rp3dg_batch* batch, int32_t body, const float linear[3], const float angular[3]);
+/* RigidBody::apply{World,Local}ForceAt{CenterOfMass,WorldPosition,LocalPosition}
+ * (src/body/RigidBody.cpp:151-239, 298-311): the force (world frame, or body frame with
+ * RP3DG_FORCE_LOCAL) adds to the body's external force; with RP3DG_FORCE_AT_POINT the point (world
+ * frame, or body frame with RP3DG_FORCE_POINT_LOCAL) adds (point - centre of mass) x force to the
+ * torque.  Dynamic bodies only; a sleeping body wakes.  The step clears the sums. */
+#define RP3DG_FORCE_LOCAL 1
+#define RP3DG_FORCE_AT_POINT 2
+#define RP3DG_FORCE_POINT_LOCAL 4
+int rp3dg_batch_body_apply_force(rp3dg_batch* batch, int32_t body, const float force[3], const float point[3],
+                                 int32_t flags);
+/* RigidBody::applyWorldTorque / applyLocalTorque (src/body/RigidBody.cpp:938-967); flags 0 or
+ * RP3DG_FORCE_LOCAL. */
+int rp3dg_batch_body_apply_torque(rp3dg_batch* batch, int32_t body, const float torque[3], int32_t flags);
+/* RigidBody::resetForce / resetTorque (src/body/RigidBody.cpp:970-987), each when its flag is set. */
+int rp3dg_batch_body_reset_force_torque(rp3dg_batch* batch, int32_t body, int32_t force, int32_t torque);
+/* RigidBody::setIsSleeping (src/body/RigidBody.cpp:1006-1057): putting a body to sleep zeroes its
+ * velocities and external force/torque, waking it puts its colliders into the moved set; a static body
+ * stays awake.  Pairs whose two bodies are disabled (asleep or static) once the queued calls have run
+ * restart their LastFrameCollisionInfo, as the reference's disabled-pair move does
+ * (OverlappingPairs.cpp:301-346).  RP3DG_ESTATE to put a body to sleep when sleeping is disabled. */
+int rp3dg_batch_body_set_sleeping(rp3dg_batch* batch, int32_t body, int32_t sleeping);
+/* RigidBody::getForce / getTorque (src/body/RigidBody.cpp:993-1003): the external force and torque
+ * accumulated since the last step, xyz per body (NULL = skip). */
+int rp3dg_batch_get_force_torque(rp3dg_batch* batch, float* force3, float* torque3);
 /* PhysicsWorld::raycast (PhysicsWorld.h:486-490, CollisionDetectionSystem.cpp:1186-1195)
  * with a RaycastCallback that keeps the closest hit (notifyRaycastHit returns the hit fraction), for
  * n rays at once: ray i runs from point1 to point2 (xyz each) in world world[i] (NULL: world 0) with

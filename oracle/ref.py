@@ -165,6 +165,29 @@ class RefWorld:
         self.L.rp3dref_apply_forces.argtypes = [C.c_void_p, fp, fp]
         self.L.rp3dref_apply_forces(self.h, _fp(f) if f is not None else None, _fp(t) if t is not None else None)
 
+
+    def body_ops(self, ops):
+        """RigidBody calls on the reference world, in order: (code, body, flags, a, c) as the device
+        library's Batch.body_ops takes them."""
+        fp = C.POINTER(C.c_float)
+        self.L.rp3dref_body_op.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int, fp, fp]
+        self.L.rp3dref_body_op.restype = C.c_int
+        for code, body, flags, a, c in ops:
+            av = None if a is None else np.ascontiguousarray(a, np.float32)
+            cv = None if c is None else np.ascontiguousarray(c, np.float32)
+            r = self.L.rp3dref_body_op(self.h, int(body), int(code), int(flags),
+                                       _fp(av) if av is not None else None, _fp(cv) if cv is not None else None)
+            if r != 0:
+                raise RuntimeError(f"reference RigidBody call {code} on body {body} failed")
+
+    def force_torque(self):
+        n = self.nb
+        f = np.zeros((n, 3), np.float32)
+        t = np.zeros((n, 3), np.float32)
+        fp = C.POINTER(C.c_float)
+        self.L.rp3dref_get_force_torque.argtypes = [C.c_void_p, fp, fp]
+        self.L.rp3dref_get_force_torque(self.h, _fp(f), _fp(t))
+        return f, t
     def sleeping(self):
         """RigidBody::isSleeping of every body, as int32 0 / 1."""
         out = np.zeros(self.nb, np.int32)

@@ -688,6 +688,57 @@ int rp3dref_apply_forces(void* h, const float* f, const float* t) {
     return 0;
 }
 
+// One RigidBody call on body b, by op code: 1 setTransform(a = position, c = orientation xyzw),
+// 2 setLinearVelocity(a), 3 setAngularVelocity(a), 4 apply*Force* (a = force, c = point; flags 1 = local
+// force, 2 = at a point, 4 = point in the body frame), 5 applyWorldTorque / applyLocalTorque (flags 1),
+// 6 resetForce, 7 resetTorque, 8 setIsSleeping(flags != 0).
+int rp3dref_body_op(void* h, int b, int op, int flags, const float* a, const float* c) {
+    RefWorld* R = (RefWorld*)h;
+    if (b < 0 || b >= (int)R->bodies.size()) return -1;
+    RigidBody* B = R->bodies[b];
+    const Vector3 va = a ? Vector3(a[0], a[1], a[2]) : Vector3::zero();
+    const Vector3 vc = c ? Vector3(c[0], c[1], c[2]) : Vector3::zero();
+    switch (op) {
+        case 1: B->setTransform(make_tf(a, c)); break;
+        case 2: B->setLinearVelocity(va); break;
+        case 3: B->setAngularVelocity(va); break;
+        case 4:
+            if (!(flags & 2)) {
+                if (flags & 1) B->applyLocalForceAtCenterOfMass(va);
+                else B->applyWorldForceAtCenterOfMass(va);
+            } else if (flags & 4) {
+                if (flags & 1) B->applyLocalForceAtLocalPosition(va, vc);
+                else B->applyWorldForceAtLocalPosition(va, vc);
+            } else {
+                if (flags & 1) B->applyLocalForceAtWorldPosition(va, vc);
+                else B->applyWorldForceAtWorldPosition(va, vc);
+            }
+            break;
+        case 5:
+            if (flags & 1) B->applyLocalTorque(va);
+            else B->applyWorldTorque(va);
+            break;
+        case 6: B->resetForce(); break;
+        case 7: B->resetTorque(); break;
+        case 8: B->setIsSleeping(flags != 0); break;
+        default: return -1;
+    }
+    return 0;
+}
+
+// RigidBody::getForce / getTorque of every body (xyz each).
+int rp3dref_get_force_torque(void* h, float* f, float* t) {
+    RefWorld* R = (RefWorld*)h;
+    for (size_t i = 0; i < R->bodies.size(); i++) {
+        const Vector3 fi = R->bodies[i]->getForce(), ti = R->bodies[i]->getTorque();
+        for (int k = 0; k < 3; k++) {
+            if (f) f[3 * i + k] = (float)fi[k];
+            if (t) t[3 * i + k] = (float)ti[k];
+        }
+    }
+    return (int)R->bodies.size();
+}
+
 // RigidBody::isSleeping of every body.
 int rp3dref_get_sleeping(void* h, int32_t* out) {
     RefWorld* R = (RefWorld*)h;

@@ -13,7 +13,7 @@ LIB_DIR = os.path.join(HERE, "_lib")
 LIB = os.path.join(LIB_DIR, "librp3d_mi355x.so")
 
 SOURCES = ["world.cpp", "quickhull.cpp", "broadphase.hip", "narrowphase.hip", "solver.hip", "joints.hip", "fused.hip", "mlp.hip",
-           "islands.hip", "events.hip", "raycast.hip",
+           "islands.hip", "events.hip", "raycast.hip", "api.hip",
            "batch.hip"]
 
 # -ffp-contract=off + IEEE div/sqrt keep the FP32 operation sequence of the x86-64 reference build;
@@ -51,7 +51,7 @@ def build(verbose=False, jobs=None):
     subprocess.run(cmd, check=True)
     os.replace(tmp, LIB)
     # the C++ API examples (host code only: g++ against the C++ mirror header and the library)
-    for name in ("hello_world", "joints", "contacts"):
+    for name in ("hello_world", "joints", "contacts", "driven"):
         ex = os.path.join(LIB_DIR, name)
         cmd = ["g++", "-std=c++17", "-O2", "-I", os.path.join(os.path.dirname(HERE), "include"),
                os.path.join(HERE, "examples", name + ".cpp"), "-L", LIB_DIR, "-lrp3d_mi355x",

@@ -8,6 +8,7 @@
 #include <vector>
 
 #include "../../include/rp3d_mi355x.h"
+#include "api.h"
 #include "events.h"
 #include "fused.h"
 #include "islands.h"
@@ -93,6 +94,16 @@ struct rp3dg_batch {
     int max_world_bodies = 0;
     int max_lds_per_block = 65536;     // hipDeviceAttributeMaxSharedMemoryPerBlock of the batch's device
     int max_lds_per_cu = 65536;        // hipDeviceAttributeMaxSharedMemoryPerMultiprocessor
+
+    // RigidBody calls between steps (setTransform, setLinearVelocity, apply*Force, setIsSleeping, ...),
+    // queued in call order and applied by one device thread before the next step or read-back
+    std::vector<rp3dg::ApiOp> api_ops;
+    rp3dg::ApiOp* d_api_ops = nullptr;
+    size_t api_cap = 0;
+    bool api_sleep_op = false;         // a queued call puts a body to sleep (its pairs restart their records)
+    int *d_coll_offs = nullptr, *d_coll_list = nullptr;   // colliders of each body (CSR)
+    std::vector<int> body_type;        // RP3DG_STATIC / KINEMATIC / DYNAMIC of every body
+    int flush_api();
 
     // launch hints from the last completed step (grids are grid-stride, any size is correct)
     int hint_pairs = 0, hint_manifolds = 0, hint_rounds = 8;

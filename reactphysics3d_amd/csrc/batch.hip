@@ -469,6 +469,7 @@ int rp3dg_batch::upload(rp3dg_world* const* ws, int nw, const rp3dg_batch_option
                 f |= BF_UNIT_LOCKS;
             flags.push_back(f);
             bworld.push_back(wi);
+            body_type.push_back(b.type);
         }
         std::vector<int> rank;
         moved_ranks((int)W.colliders.size(), rank);
@@ -664,6 +665,19 @@ int rp3dg_batch::upload(rp3dg_world* const* ws, int nw, const rp3dg_batch_option
     if (!hfverts.empty()) HIPCHECK(up(d_hfverts, hfverts.data(), hfverts.size() * 4));
     if (!hfvidx.empty()) HIPCHECK(up(d_hfvidx, hfvidx.data(), hfvidx.size() * 4));
     HIPCHECK(up(d_hbounds, hbounds.data(), hbounds.size() * 16));
+    {
+        // colliders of every body (the between-step RigidBody calls refit them)
+        std::vector<int> offs(nb + 1, 0), list(nc);
+        for (int i = 0; i < nc; i++) offs[cbody[i] + 1]++;
+        for (int i = 0; i < nb; i++) offs[i + 1] += offs[i];
+        std::vector<int> cur(offs.begin(), offs.end() - 1);
+        for (int i = 0; i < nc; i++) list[cur[cbody[i]]++] = i;
+        d_coll_offs = dalloc<int>(this, nb + 1);
+        d_coll_list = dalloc<int>(this, std::max(1, nc));
+        if (!d_coll_list) { set_error("device allocation failed (body colliders)"); return RP3DG_EHIP; }
+        HIPCHECK(hipMemcpy(d_coll_offs, offs.data(), sizeof(int) * (nb + 1), hipMemcpyHostToDevice));
+        if (nc) HIPCHECK(hipMemcpy(d_coll_list, list.data(), sizeof(int) * nc, hipMemcpyHostToDevice));
+    }
     {
         // per-body lists of the body pairs that must not collide, both directions
         std::vector<int> offs(nb + 1, 0), list(2 * nocol.size());
@@ -1223,7 +1237,9 @@ int rp3dg_batch::finish_steps(int n) {
 
 int rp3dg_batch::step(float dt) {
     if (nb == 0) return RP3DG_OK;
-    const int r = enqueue_step(dt, false);
+    int r = flush_api();
+    if (r) return r;
+    r = enqueue_step(dt, false);
     if (r) return r;
     return finish_steps(1);
 }
@@ -1244,6 +1260,8 @@ int rp3dg_batch::step_graph(float dt, int n) {
     if (nb == 0 || n <= 0) return RP3DG_OK;
     if (!fused) { set_error("graph capture needs the fused solver layout"); return RP3DG_EUNSUPPORTED; }
     if (profiling) { set_error("kernel timing is eager-only: disable it before graph replay"); return RP3DG_ESTATE; }
+    const int fr = flush_api();
+    if (fr) return fr;
     if (graph_exec[0] && graph_dt != dt) drop_graphs();
     if (!graph_exec[0]) {
         graph_rounds = std::min(kColorRoundsMax, std::max(8, hint_rounds + 4));
@@ -1276,6 +1294,32 @@ int rp3dg_batch::step_graph(float dt, int n) {
     return finish_steps(n);
 }
 
+// Applies the queued RigidBody calls on the device, in call order (api.hip), on the batch's stream.
+int rp3dg_batch::flush_api() {
+    if (api_ops.empty()) return RP3DG_OK;
+    const size_t n = api_ops.size();
+    if (n > api_cap) {
+        if (d_api_ops) HIPCHECK(hipFree(d_api_ops));
+        d_api_ops = nullptr;
+        api_cap = std::max<size_t>(256, 2 * n);
+        HIPCHECK(hipMalloc(&d_api_ops, sizeof(ApiOp) * api_cap));
+    }
+    HIPCHECK(hipMemcpyAsync(d_api_ops, api_ops.data(), sizeof(ApiOp) * n, hipMemcpyHostToDevice, stream));
+    ApiArgs A{B, C, d_hbounds, d_coll_offs, d_coll_list, settings.sleeping_enabled ? isl.sleep_time : nullptr};
+    api_apply_ops<<<1, 1, 0, stream>>>(A, d_api_ops, (int)n);
+    HIPCHECK(hipGetLastError());
+    if (api_sleep_op) {   // pairs of two disabled bodies restart their LastFrameCollisionInfo
+        const DevPairs& P = cur_is_a ? Pa : Pb;
+        isl_reset_disabled_pairs<<<grid_for(std::max(1, last.nb_pairs)), 256, 0, stream>>>(B, C, P, pair_count + (cur_is_a ? 0 : 1));
+        HIPCHECK(hipGetLastError());
+    }
+    // the host copy stays alive until the copy has been consumed
+    HIPCHECK(hipStreamSynchronize(stream));
+    api_ops.clear();
+    api_sleep_op = false;
+    return RP3DG_OK;
+}
+
 rp3dg_batch::~rp3dg_batch() {
     drop_graphs();
     for (hipStream_t s : side_streams) hipStreamDestroy(s);
@@ -1283,6 +1327,7 @@ rp3dg_batch::~rp3dg_batch() {
     if (log_feats) hipFree(log_feats);
     if (log_targets) hipFree(log_targets);
     if (ray_buf) hipFree(ray_buf);
+    if (d_api_ops) hipFree(d_api_ops);
     for (hipEvent_t ev : event_pool) hipEventDestroy(ev);
     for (void* p : allocs) hipFree(p);
     if (hcounters) hipHostFree(hcounters);
@@ -1320,6 +1365,8 @@ int rp3dg_batch_step(rp3dg_batch* b, float dt, int32_t nsteps) {
 
 int rp3dg_batch_synchronize(rp3dg_batch* b) {
     if (!b) return RP3DG_EINVAL;
+    const int r = b->flush_api();
+    if (r) return r;
     HIPCHECK(hipStreamSynchronize(b->stream));
     return RP3DG_OK;
 }
@@ -1451,7 +1498,10 @@ int64_t rp3dg_batch_get_contact_events(rp3dg_batch* b, int32_t* c1, int32_t* c2,
 int rp3dg_batch_apply_forces(rp3dg_batch* b, const float* force, const float* torque) {
     if (!b) return RP3DG_EINVAL;
     if (!force && !torque) return RP3DG_OK;
+    const int fr = b->flush_api();
+    if (fr) return fr;
     const int n = b->nb;
+    bool woke = false;
     std::vector<float4> f(n), t(n);
     std::vector<int> fl(n);
     std::vector<float> st;
@@ -1471,6 +1521,7 @@ int rp3dg_batch_apply_forces(rp3dg_batch* b, const float* force, const float* to
         if (!(fl[i] & BF_ENABLED)) {   // asleep: setIsSleeping(false)
             fl[i] |= BF_ENABLED | BF_WOKEN;
             st[i] = 0.0f;
+            woke = true;
         }
         if (nf) { f[i].x += force[3 * i]; f[i].y += force[3 * i + 1]; f[i].z += force[3 * i + 2]; }
         if (nt) { t[i].x += torque[3 * i]; t[i].y += torque[3 * i + 1]; t[i].z += torque[3 * i + 2]; }
@@ -1479,6 +1530,11 @@ int rp3dg_batch_apply_forces(rp3dg_batch* b, const float* force, const float* to
     HIPCHECK(hipMemcpy(b->B.torque, t.data(), sizeof(float4) * n, hipMemcpyHostToDevice));
     HIPCHECK(hipMemcpy(b->B.flags, fl.data(), sizeof(int) * n, hipMemcpyHostToDevice));
     if (!st.empty()) HIPCHECK(hipMemcpy(b->isl.sleep_time, st.data(), sizeof(float) * n, hipMemcpyHostToDevice));
+    if (woke) {   // askForBroadPhaseCollisionCheck (RigidBody.cpp:1045-1051): the moved set takes their colliders
+        bp_refit<<<grid_for(b->nc), 256, 0, b->stream>>>(b->B, b->C, b->d_hbounds, 2);
+        HIPCHECK(hipGetLastError());
+        HIPCHECK(hipStreamSynchronize(b->stream));
+    }
     return RP3DG_OK;
 }
 
@@ -1488,6 +1544,7 @@ int rp3dg_batch_raycast(rp3dg_batch* b, int64_t n, const int32_t* world, const f
     if (!b || n < 0 || !point1 || !point2) return RP3DG_EINVAL;
     if (n == 0) return RP3DG_OK;
     if (n > (1 << 26)) { set_error("at most 2^26 rays per call"); return RP3DG_EINVAL; }
+    if (const int fr = b->flush_api()) return fr;
     std::vector<int> wv((size_t)n);
     std::vector<float4> a((size_t)n), c((size_t)n);
     for (int64_t i = 0; i < n; i++) {
@@ -1535,6 +1592,7 @@ int rp3dg_batch_raycast(rp3dg_batch* b, int64_t n, const int32_t* world, const f
 
 int rp3dg_batch_get_sleeping(rp3dg_batch* b, int32_t* sleeping, int64_t capacity) {
     if (!b || !sleeping || capacity < b->nb) { set_error("sleeping: capacity below the body count"); return RP3DG_EINVAL; }
+    if (const int fr = b->flush_api()) return fr;
     std::vector<int> f(b->nb);
     HIPCHECK(hipStreamSynchronize(b->stream));
     HIPCHECK(hipMemcpy(f.data(), b->B.flags, sizeof(int) * b->nb, hipMemcpyDeviceToHost));
@@ -1546,6 +1604,7 @@ int rp3dg_batch_get_sleeping(rp3dg_batch* b, int32_t* sleeping, int64_t capacity
 
 int rp3dg_batch_get_bodies(rp3dg_batch* b, float* pos, float* ori, float* lin, float* ang) {
     if (!b) return RP3DG_EINVAL;
+    if (const int fr = b->flush_api()) return fr;
     std::vector<float4> t(b->nb);
     auto grab = [&](const float4* d, float* out, int comps) -> int {
         if (!out) return RP3DG_OK;
@@ -1572,9 +1631,27 @@ int rp3dg_batch_get_bodies(rp3dg_batch* b, float* pos, float* ori, float* lin, f
 // and the collider transforms and fat AABBs (refit, as the next step would).  NULL leaves a part as is.
 int rp3dg_batch_set_bodies(rp3dg_batch* b, const float* pos, const float* ori, const float* lin, const float* ang) {
     if (!b) return RP3DG_EINVAL;
+    int r = b->flush_api();
+    if (r) return r;
     HIPCHECK(hipStreamSynchronize(b->stream));
     const int n = b->nb;
     std::vector<float4> t(n);
+    // a static body keeps its transform: its colliders' static-static pairs are listed once at upload
+    auto check_static = [&](const float4* d, const float* in, int comps) -> int {
+        if (!in) return RP3DG_OK;
+        HIPCHECK(hipMemcpy(t.data(), d, sizeof(float4) * n, hipMemcpyDeviceToHost));
+        for (int i = 0; i < n; i++) {
+            if (b->body_type[i] != RP3DG_STATIC) continue;
+            const float cur[4] = {t[i].x, t[i].y, t[i].z, t[i].w};
+            for (int k = 0; k < comps; k++)
+                if (cur[k] != in[comps * i + k]) {
+                    set_error("set_bodies: body " + std::to_string(i) + " is static; its transform is fixed after the first update");
+                    return RP3DG_EUNSUPPORTED;
+                }
+        }
+        return RP3DG_OK;
+    };
+    if ((r = check_static(b->B.p, pos, 3)) || (r = check_static(b->B.q, ori, 4))) return r;
     auto put = [&](float4* d, const float* in, int comps) -> int {
         if (!in) return RP3DG_OK;
         HIPCHECK(hipMemcpy(t.data(), d, sizeof(float4) * n, hipMemcpyDeviceToHost));
@@ -1585,9 +1662,9 @@ int rp3dg_batch_set_bodies(rp3dg_batch* b, const float* pos, const float* ori, c
         HIPCHECK(hipMemcpy(d, t.data(), sizeof(float4) * n, hipMemcpyHostToDevice));
         return RP3DG_OK;
     };
-    int r;
     if ((r = put(b->B.p, pos, 3)) || (r = put(b->B.q, ori, 4)) || (r = put(b->B.v, lin, 3)) || (r = put(b->B.w, ang, 3)))
         return r;
+    bool woke = false;
     if (b->settings.sleeping_enabled && (pos || ori || lin || ang)) {
         // setTransform wakes the body, setLinear/AngularVelocity do when the new velocity is not zero
         // (RigidBody.cpp); waking = setIsSleeping(false) (RigidBody.cpp:1006-1057)
@@ -1601,22 +1678,102 @@ int rp3dg_batch_set_bodies(rp3dg_batch* b, const float* pos, const float* ori, c
             if (asleep && (pos || ori || nonzero(lin, i) || nonzero(ang, i))) {
                 f[i] |= BF_ENABLED | BF_WOKEN;
                 st[i] = 0.0f;
+                woke = true;
             }
         }
         HIPCHECK(hipMemcpy(b->B.flags, f.data(), sizeof(int) * n, hipMemcpyHostToDevice));
         HIPCHECK(hipMemcpy(b->isl.sleep_time, st.data(), sizeof(float) * n, hipMemcpyHostToDevice));
     }
-    if (pos || ori) {
-        set_body_com<<<grid_for(n), 256, 0, b->stream>>>(b->B);
-        bp_refit<<<grid_for(b->nc), 256, 0, b->stream>>>(b->B, b->C, b->d_hbounds, 0);
+    if (pos || ori) set_body_com<<<grid_for(n), 256, 0, b->stream>>>(b->B);
+    if (pos || ori || woke) {   // moved and woken bodies put their colliders into the moved set
+        bp_refit<<<grid_for(b->nc), 256, 0, b->stream>>>(b->B, b->C, b->d_hbounds, 2);
         HIPCHECK(hipGetLastError());
         HIPCHECK(hipStreamSynchronize(b->stream));
     }
     return RP3DG_OK;
 }
 
+// RigidBody calls on one body between steps: queued in call order, applied on the device (api.hip)
+// before the next step, graph replay, read-back or raycast.
+static int queue_body_op(rp3dg_batch* b, int32_t body, int op, int flags, const float* a, int na, const float* c, int nc) {
+    if (!b) return RP3DG_EINVAL;
+    if (body < 0 || body >= b->nb) { set_error("invalid body index " + std::to_string(body)); return RP3DG_EINVAL; }
+    ApiOp o{};
+    o.op = op;
+    o.body = body;
+    o.flags = flags;
+    float va[4] = {0.f, 0.f, 0.f, 0.f}, vc[4] = {0.f, 0.f, 0.f, 0.f};
+    for (int k = 0; k < na; k++) va[k] = a[k];
+    for (int k = 0; k < nc; k++) vc[k] = c[k];
+    o.a = make_float4(va[0], va[1], va[2], va[3]);
+    o.b = make_float4(vc[0], vc[1], vc[2], vc[3]);
+    b->api_ops.push_back(o);
+    if (op == API_SET_SLEEPING && flags) b->api_sleep_op = true;
+    return RP3DG_OK;
+}
+
+int rp3dg_batch_body_set_transform(rp3dg_batch* b, int32_t body, const float position[3], const float orientation[4]) {
+    if (!b || !position || !orientation) { set_error("null argument"); return RP3DG_EINVAL; }
+    if (body >= 0 && body < b->nb && b->body_type[body] == RP3DG_STATIC) {
+        set_error("body " + std::to_string(body) + " is static; its transform is fixed after the first update");
+        return RP3DG_EUNSUPPORTED;
+    }
+    return queue_body_op(b, body, API_SET_TRANSFORM, 0, position, 3, orientation, 4);
+}
+
+int rp3dg_batch_body_set_velocity(rp3dg_batch* b, int32_t body, const float linear[3], const float angular[3]) {
+    int r = RP3DG_OK;
+    if (linear) r = queue_body_op(b, body, API_SET_LINEAR_VELOCITY, 0, linear, 3, nullptr, 0);
+    if (!r && angular) r = queue_body_op(b, body, API_SET_ANGULAR_VELOCITY, 0, angular, 3, nullptr, 0);
+    return r;
+}
+
+int rp3dg_batch_body_apply_force(rp3dg_batch* b, int32_t body, const float force[3], const float point[3], int32_t flags) {
+    if (!force) { set_error("null force"); return RP3DG_EINVAL; }
+    if (flags & ~(RP3DG_FORCE_LOCAL | RP3DG_FORCE_AT_POINT | RP3DG_FORCE_POINT_LOCAL)) { set_error("unknown force flags"); return RP3DG_EINVAL; }
+    if ((flags & RP3DG_FORCE_AT_POINT) && !point) { set_error("null point"); return RP3DG_EINVAL; }
+    return queue_body_op(b, body, API_APPLY_FORCE, flags, force, 3, (flags & RP3DG_FORCE_AT_POINT) ? point : nullptr,
+                         (flags & RP3DG_FORCE_AT_POINT) ? 3 : 0);
+}
+
+int rp3dg_batch_body_apply_torque(rp3dg_batch* b, int32_t body, const float torque[3], int32_t flags) {
+    if (!torque) { set_error("null torque"); return RP3DG_EINVAL; }
+    if (flags & ~RP3DG_FORCE_LOCAL) { set_error("unknown torque flags"); return RP3DG_EINVAL; }
+    return queue_body_op(b, body, API_APPLY_TORQUE, flags, torque, 3, nullptr, 0);
+}
+
+int rp3dg_batch_body_reset_force_torque(rp3dg_batch* b, int32_t body, int32_t force, int32_t torque) {
+    int r = RP3DG_OK;
+    if (force) r = queue_body_op(b, body, API_RESET_FORCE, 0, nullptr, 0, nullptr, 0);
+    if (!r && torque) r = queue_body_op(b, body, API_RESET_TORQUE, 0, nullptr, 0, nullptr, 0);
+    return r;
+}
+
+int rp3dg_batch_body_set_sleeping(rp3dg_batch* b, int32_t body, int32_t sleeping) {
+    if (b && !b->settings.sleeping_enabled && sleeping) {
+        set_error("sleeping is disabled in the worlds of this batch");
+        return RP3DG_ESTATE;
+    }
+    return queue_body_op(b, body, API_SET_SLEEPING, sleeping ? 1 : 0, nullptr, 0, nullptr, 0);
+}
+
+int rp3dg_batch_get_force_torque(rp3dg_batch* b, float* force, float* torque) {
+    if (!b) return RP3DG_EINVAL;
+    if (const int fr = b->flush_api()) return fr;
+    std::vector<float4> t(b->nb);
+    HIPCHECK(hipStreamSynchronize(b->stream));
+    for (int part = 0; part < 2; part++) {
+        float* out = part ? torque : force;
+        if (!out) continue;
+        HIPCHECK(hipMemcpy(t.data(), part ? b->B.torque : b->B.force, sizeof(float4) * b->nb, hipMemcpyDeviceToHost));
+        for (int i = 0; i < b->nb; i++) { out[3 * i] = t[i].x; out[3 * i + 1] = t[i].y; out[3 * i + 2] = t[i].z; }
+    }
+    return RP3DG_OK;
+}
+
 int rp3dg_batch_get_pairs(rp3dg_batch* b, int32_t* c1, int32_t* c2, int64_t cap) {
     if (!b) return RP3DG_EINVAL;
+    if (const int fr = b->flush_api()) return fr;
     const DevPairs& P = b->cur_is_a ? b->Pa : b->Pb;
     const int n = b->last.nb_pairs;
     if (cap < n) { set_error("capacity too small"); return RP3DG_EINVAL; }

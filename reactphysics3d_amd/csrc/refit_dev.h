@@ -61,13 +61,16 @@ __device__ inline void fatten(V3 mn, V3 mx, V3& fmn, V3& fmx) {
 
 // init = 1: collider creation (RigidBody::addCollider -> BroadPhaseSystem::addCollider), every collider
 // gets a fat AABB and enters the moved set.  init = 0: end-of-step refit of enabled colliders.
+// init = 2: a body moved or woken between two steps (Body::updateBroadPhaseState, Body.cpp:226-241):
+// the colliders of enabled bodies are refit and join the moved set, which keeps what the last step's
+// refit put there.
 __device__ __forceinline__ void refit_one(const DevBodies& B, const DevColliders& C, const float4* hull_bounds, int init,
                                           int i) {
     {
         const int b = C.body[i];
         const int bf = B.flags[b];
         int moved = 0;
-        if (init || (bf & BF_ENABLED)) {
+        if (init == 1 || (bf & BF_ENABLED)) {
             const Tf bt = tf(ld3(B.p, b), ldq(B.q, b));
             const Tf lt = tf(ld3(C.lpos, i), ldq(C.lq, i));
             const Tf wt = tmul(bt, lt);
@@ -76,7 +79,7 @@ __device__ __forceinline__ void refit_one(const DevBodies& B, const DevColliders
             V3 mn, mx;
             transformed_aabb(C.type[i], C.shape[i], hull_bounds, wt, mn, mx);
             bool inside = false;
-            if (!init) {
+            if (init != 1) {
                 const V3 fmn = ld3(C.amin, i), fmx = ld3(C.amax, i);
                 // AABB::contains(const AABB&) (AABB.h:246-256)
                 inside = fmn.x <= mn.x && fmn.y <= mn.y && fmn.z <= mn.z && fmx.x >= mx.x && fmx.y >= mx.y &&
@@ -93,7 +96,11 @@ __device__ __forceinline__ void refit_one(const DevBodies& B, const DevColliders
             // (RigidBody::askForBroadPhaseCollisionCheck)
             if (bf & BF_WOKEN) moved = 1;
         }
-        C.moved[i] = moved;
+        if (init == 2) {
+            if (moved) C.moved[i] = 1;
+        } else {
+            C.moved[i] = moved;
+        }
     }
 }
 

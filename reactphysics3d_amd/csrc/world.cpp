@@ -220,6 +220,13 @@ static bool mutable_world(HostWorld* w) {
     if (w->uploaded) { set_error("world already uploaded to the device; edit before the first update()"); return false; }
     return true;
 }
+// RigidBody calls after the first update go to the device batch that rp3dg_world_update steps; a world
+// that shares a batch with others takes them through rp3dg_batch_body_* (global body indices).
+static rp3dg_batch* stepping_batch(HostWorld* w) {
+    if (!w->own) set_error("world belongs to a shared batch; use rp3dg_batch_body_* with the batch's body index");
+    else w->mirror_dirty = true;
+    return w->own;
+}
 
 }  // namespace rp3dg
 
@@ -305,7 +312,10 @@ int rp3dg_body_set_type(rp3dg_world* w, int32_t bi, int32_t type) {
 
 int rp3dg_body_set_velocity(rp3dg_world* w, int32_t bi, const float lin[3], const float ang[3]) {
     if (!valid_body(w, bi)) return RP3DG_EINVAL;
-    if (!mutable_world(w)) return RP3DG_ESTATE;
+    if (w->uploaded) {
+        rp3dg_batch* b = stepping_batch(w);
+        return b ? rp3dg_batch_body_set_velocity(b, bi, lin, ang) : RP3DG_ESTATE;
+    }
     HostBody& b = w->bodies[bi];
     if (b.type == RP3DG_STATIC) return RP3DG_OK;  // RigidBody::setLinearVelocity ignores static bodies
     if (lin) b.v = v3(lin[0], lin[1], lin[2]);
@@ -338,11 +348,120 @@ int rp3dg_body_enable_gravity(rp3dg_world* w, int32_t bi, int32_t e) {
 
 int rp3dg_body_apply_force_torque(rp3dg_world* w, int32_t bi, const float f[3], const float t[3]) {
     if (!valid_body(w, bi)) return RP3DG_EINVAL;
-    if (!mutable_world(w)) return RP3DG_ESTATE;
+    if (w->uploaded) {
+        rp3dg_batch* b = stepping_batch(w);
+        if (!b) return RP3DG_ESTATE;
+        int r = f ? rp3dg_batch_body_apply_force(b, bi, f, nullptr, 0) : RP3DG_OK;
+        if (!r && t) r = rp3dg_batch_body_apply_torque(b, bi, t, 0);
+        return r;
+    }
     HostBody& b = w->bodies[bi];
     if (b.type != RP3DG_DYNAMIC) return RP3DG_OK;
     if (f) b.force += v3(f[0], f[1], f[2]);
     if (t) b.torque += v3(t[0], t[1], t[2]);
+    return RP3DG_OK;
+}
+
+// RigidBody::apply{World,Local}ForceAt{CenterOfMass,WorldPosition,LocalPosition} (RigidBody.cpp:151-239,
+// 298-311); before the first update the body is awake and only its sums change.
+int rp3dg_body_apply_force(rp3dg_world* w, int32_t bi, const float f[3], const float point[3], int32_t flags) {
+    if (!valid_body(w, bi)) return RP3DG_EINVAL;
+    if (w->uploaded) {
+        rp3dg_batch* b = stepping_batch(w);
+        return b ? rp3dg_batch_body_apply_force(b, bi, f, point, flags) : RP3DG_ESTATE;
+    }
+    if (!f) { set_error("null force"); return RP3DG_EINVAL; }
+    if (flags & ~(RP3DG_FORCE_LOCAL | RP3DG_FORCE_AT_POINT | RP3DG_FORCE_POINT_LOCAL)) { set_error("unknown force flags"); return RP3DG_EINVAL; }
+    if ((flags & RP3DG_FORCE_AT_POINT) && !point) { set_error("null point"); return RP3DG_EINVAL; }
+    HostBody& b = w->bodies[bi];
+    V3 force = v3(f[0], f[1], f[2]);
+    if (flags & RP3DG_FORCE_LOCAL) force = qrot(b.transform.q, force);
+    if (b.type != RP3DG_DYNAMIC) return RP3DG_OK;
+    b.force += force;
+    if (flags & RP3DG_FORCE_AT_POINT) {
+        V3 p = v3(point[0], point[1], point[2]);
+        if (flags & RP3DG_FORCE_POINT_LOCAL) p = tapply(b.transform, p);
+        b.torque += cross(p - b.com_world, force);
+    }
+    return RP3DG_OK;
+}
+
+// RigidBody::applyWorldTorque / applyLocalTorque (RigidBody.cpp:938-967).
+int rp3dg_body_apply_torque(rp3dg_world* w, int32_t bi, const float t[3], int32_t flags) {
+    if (!valid_body(w, bi)) return RP3DG_EINVAL;
+    if (w->uploaded) {
+        rp3dg_batch* b = stepping_batch(w);
+        return b ? rp3dg_batch_body_apply_torque(b, bi, t, flags) : RP3DG_ESTATE;
+    }
+    if (!t) { set_error("null torque"); return RP3DG_EINVAL; }
+    if (flags & ~RP3DG_FORCE_LOCAL) { set_error("unknown torque flags"); return RP3DG_EINVAL; }
+    HostBody& b = w->bodies[bi];
+    V3 torque = v3(t[0], t[1], t[2]);
+    if (flags & RP3DG_FORCE_LOCAL) torque = qrot(b.transform.q, torque);
+    if (b.type == RP3DG_DYNAMIC) b.torque += torque;
+    return RP3DG_OK;
+}
+
+// RigidBody::resetForce / resetTorque (RigidBody.cpp:970-987).
+int rp3dg_body_reset_force_torque(rp3dg_world* w, int32_t bi, int32_t force, int32_t torque) {
+    if (!valid_body(w, bi)) return RP3DG_EINVAL;
+    if (w->uploaded) {
+        rp3dg_batch* b = stepping_batch(w);
+        return b ? rp3dg_batch_body_reset_force_torque(b, bi, force, torque) : RP3DG_ESTATE;
+    }
+    HostBody& b = w->bodies[bi];
+    if (b.type != RP3DG_DYNAMIC) return RP3DG_OK;
+    if (force) b.force = v3zero();
+    if (torque) b.torque = v3zero();
+    return RP3DG_OK;
+}
+
+// RigidBody::setTransform (RigidBody.cpp:837-866): before the first update the centre of mass follows
+// and the linear velocity gains angular x (new - old centre of mass); colliders are placed at upload.
+int rp3dg_body_set_transform(rp3dg_world* w, int32_t bi, const float pos[3], const float q[4]) {
+    if (!valid_body(w, bi)) return RP3DG_EINVAL;
+    if (!pos || !q) { set_error("null argument"); return RP3DG_EINVAL; }
+    if (w->uploaded) {
+        rp3dg_batch* b = stepping_batch(w);
+        return b ? rp3dg_batch_body_set_transform(b, bi, pos, q) : RP3DG_ESTATE;
+    }
+    HostBody& b = w->bodies[bi];
+    const V3 old = b.com_world;
+    b.transform = tf(v3(pos[0], pos[1], pos[2]), q4(q[0], q[1], q[2], q[3]));
+    b.com_world = tapply(b.transform, b.com_local);
+    b.v += cross(b.w, b.com_world - old);
+    return RP3DG_OK;
+}
+
+// RigidBody::setIsSleeping (RigidBody.cpp:1006-1057) after the first update (bodies are created awake).
+int rp3dg_body_set_sleeping(rp3dg_world* w, int32_t bi, int32_t sleeping) {
+    if (!valid_body(w, bi)) return RP3DG_EINVAL;
+    if (!w->uploaded) {
+        if (!sleeping) return RP3DG_OK;
+        set_error("bodies are awake until the first update(); put a body to sleep after it");
+        return RP3DG_EUNSUPPORTED;
+    }
+    rp3dg_batch* b = stepping_batch(w);
+    return b ? rp3dg_batch_body_set_sleeping(b, bi, sleeping) : RP3DG_ESTATE;
+}
+
+int rp3dg_body_get_force_torque(rp3dg_world* w, int32_t bi, float f[3], float t[3]) {
+    if (!valid_body(w, bi)) return RP3DG_EINVAL;
+    if (!w->uploaded) {
+        const HostBody& b = w->bodies[bi];
+        if (f) { f[0] = b.force.x; f[1] = b.force.y; f[2] = b.force.z; }
+        if (t) { t[0] = b.torque.x; t[1] = b.torque.y; t[2] = b.torque.z; }
+        return RP3DG_OK;
+    }
+    if (!w->own) { set_error("world belongs to a shared batch; use rp3dg_batch_get_force_torque"); return RP3DG_ESTATE; }
+    const size_t n = w->bodies.size();
+    std::vector<float> F(3 * n), T(3 * n);
+    const int r = rp3dg_batch_get_force_torque(w->own, F.data(), T.data());
+    if (r) return r;
+    for (int k = 0; k < 3; k++) {
+        if (f) f[k] = F[3 * bi + k];
+        if (t) t[k] = T[3 * bi + k];
+    }
     return RP3DG_OK;
 }
 

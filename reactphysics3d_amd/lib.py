@@ -80,7 +80,16 @@ EXPORTED = [
     "rp3dg_hull_create_from_points", "rp3dg_quickhull", "rp3dg_collider_set_is_trigger",
     "rp3dg_collider_set_is_simulation_collider", "rp3dg_collider_set_is_world_query_collider",
     "rp3dg_collider_get_kind", "rp3dg_batch_set_solver_shape",
+    "rp3dg_batch_body_set_transform", "rp3dg_batch_body_set_velocity", "rp3dg_batch_body_apply_force",
+    "rp3dg_batch_body_apply_torque", "rp3dg_batch_body_reset_force_torque", "rp3dg_batch_body_set_sleeping",
+    "rp3dg_batch_get_force_torque", "rp3dg_body_set_transform", "rp3dg_body_apply_force", "rp3dg_body_apply_torque",
+    "rp3dg_body_reset_force_torque", "rp3dg_body_get_force_torque", "rp3dg_body_set_sleeping",
 ]
+
+# RigidBody call codes of Batch.body_ops (the order of the C-ABI's per-body calls)
+OP_SET_TRANSFORM, OP_SET_LINEAR_VELOCITY, OP_SET_ANGULAR_VELOCITY = 1, 2, 3
+OP_APPLY_FORCE, OP_APPLY_TORQUE, OP_RESET_FORCE, OP_RESET_TORQUE, OP_SET_SLEEPING = 4, 5, 6, 7, 8
+FORCE_LOCAL, FORCE_AT_POINT, FORCE_POINT_LOCAL = 1, 2, 4
 
 MLP_PARAMS = 20099
 N_FEATURES = 24
@@ -161,6 +170,19 @@ def load():
         "rp3dg_collider_set_is_world_query_collider": (C.c_int, [vp, i32, i32]),
         "rp3dg_collider_get_kind": (C.c_int, [vp, i32]),
         "rp3dg_batch_set_solver_shape": (C.c_int, [vp, i32]),
+        "rp3dg_batch_body_set_transform": (C.c_int, [vp, i32, fptr, fptr]),
+        "rp3dg_batch_body_set_velocity": (C.c_int, [vp, i32, fptr, fptr]),
+        "rp3dg_batch_body_apply_force": (C.c_int, [vp, i32, fptr, fptr, i32]),
+        "rp3dg_batch_body_apply_torque": (C.c_int, [vp, i32, fptr, i32]),
+        "rp3dg_batch_body_reset_force_torque": (C.c_int, [vp, i32, i32, i32]),
+        "rp3dg_batch_body_set_sleeping": (C.c_int, [vp, i32, i32]),
+        "rp3dg_batch_get_force_torque": (C.c_int, [vp, fptr, fptr]),
+        "rp3dg_body_set_transform": (C.c_int, [vp, i32, fptr, fptr]),
+        "rp3dg_body_apply_force": (C.c_int, [vp, i32, fptr, fptr, i32]),
+        "rp3dg_body_apply_torque": (C.c_int, [vp, i32, fptr, i32]),
+        "rp3dg_body_reset_force_torque": (C.c_int, [vp, i32, i32, i32]),
+        "rp3dg_body_get_force_torque": (C.c_int, [vp, i32, fptr, fptr]),
+        "rp3dg_body_set_sleeping": (C.c_int, [vp, i32, i32]),
         "rp3dg_quickhull": (C.c_int, [vp, i32, i32, i32, fptr, i32, iptr, i32, iptr, i32, iptr, iptr]),
         "rp3dg_mlp_create": (vp, [C.c_uint64]),
         "rp3dg_mlp_destroy": (None, [vp]),
@@ -401,6 +423,44 @@ class Batch:
         arrs = [None if a is None else np.ascontiguousarray(a, np.float32) for a in (pos, quat, lin, ang)]
         ptrs = [None if a is None else _fp(a) for a in arrs]
         _check(load().rp3dg_batch_set_bodies(self.h, *ptrs), "set_bodies")
+
+    def body_ops(self, ops):
+        """RigidBody calls between steps, in order: each op is (code, body, flags, a, c) with code one of
+        OP_* (a = position / velocity / force / torque, c = orientation xyzw or application point, None
+        where unused; flags FORCE_* for forces and torques, 1 = asleep for OP_SET_SLEEPING).  Queued and
+        applied on the device before the next step or read-back."""
+        L = load()
+
+        def vec(x, n):
+            return None if x is None else _fp(np.ascontiguousarray(x, np.float32).reshape(n))
+
+        for code, body, flags, a, c in ops:
+            body, flags = int(body), int(flags)
+            if code == OP_SET_TRANSFORM:
+                r = L.rp3dg_batch_body_set_transform(self.h, body, vec(a, 3), vec(c, 4))
+            elif code == OP_SET_LINEAR_VELOCITY:
+                r = L.rp3dg_batch_body_set_velocity(self.h, body, vec(a, 3), None)
+            elif code == OP_SET_ANGULAR_VELOCITY:
+                r = L.rp3dg_batch_body_set_velocity(self.h, body, None, vec(a, 3))
+            elif code == OP_APPLY_FORCE:
+                r = L.rp3dg_batch_body_apply_force(self.h, body, vec(a, 3), vec(c, 3), flags)
+            elif code == OP_APPLY_TORQUE:
+                r = L.rp3dg_batch_body_apply_torque(self.h, body, vec(a, 3), flags)
+            elif code in (OP_RESET_FORCE, OP_RESET_TORQUE):
+                r = L.rp3dg_batch_body_reset_force_torque(self.h, body, int(code == OP_RESET_FORCE),
+                                                          int(code == OP_RESET_TORQUE))
+            elif code == OP_SET_SLEEPING:
+                r = L.rp3dg_batch_body_set_sleeping(self.h, body, flags)
+            else:
+                raise ValueError(f"unknown RigidBody call code {code}")
+            _check(r, "body_ops")
+
+    def force_torque(self):
+        """RigidBody::getForce / getTorque of every body, ([nb, 3], [nb, 3])."""
+        f = np.zeros((self.nb, 3), np.float32)
+        t = np.zeros((self.nb, 3), np.float32)
+        _check(load().rp3dg_batch_get_force_torque(self.h, _fp(f), _fp(t)), "get_force_torque")
+        return f, t
 
     def pairs(self):
         cap = max(1, self.counts()["nb_pairs"])

@@ -148,3 +148,90 @@ def test_event_listener_sees_the_reference_events(ref_available):
     col, pt, nr, fr = ref.raycast(np.array([[0, 0, 10]], np.float32), np.array([[0, 0, -10]], np.float32))
     assert col[0] == 1 and len(ray) == 1 and np.float32(ray[0]) == fr[0]
     assert want[:, 0].sum() >= 4 and want[:, 1].sum() > 100 and want[:, 2].sum() >= 1
+
+
+DRIVEN_EXE = os.path.join(ROOT, "reactphysics3d_amd", "_lib", "driven")
+
+
+def _driven_scene():
+    """The world of examples/driven.cpp (WorldSettings defaults apart from gravity -z and 10 / 5
+    iterations; the box and the ball never touch, so no two manifolds share a dynamic body)."""
+    sc = S.Scene("driven_example", S.make_settings(gravity=(0.0, 0.0, -9.81), vel_iters=10, pos_iters=5, sleeping=1))
+    g = sc.add_body(S.STATIC, (0.0, 0.0, -0.5))
+    sc.add_collider(g, S.BOX, (20.0, 20.0, 0.5), friction=0.5, bounciness=0.0)
+    b = sc.add_body(S.DYNAMIC, (0.0, 0.0, 0.6))
+    sc.add_collider(b, S.BOX, (0.5, 0.5, 0.5), friction=0.5, bounciness=0.0)
+    s = sc.add_body(S.DYNAMIC, (6.0, 0.0, 0.5))
+    sc.add_collider(s, S.SPHERE, (0.4,), friction=0.5, bounciness=0.0)
+    return sc
+
+
+def _driven_calls(i):
+    """The calls examples/driven.cpp makes before step i, as (code, body, flags, a, c)."""
+    from reactphysics3d_amd import lib as L
+    f32 = lambda *v: np.array(v, np.float32)
+    ops = []
+    if 20 <= i < 40:
+        ops.append((L.OP_APPLY_FORCE, 1, L.FORCE_AT_POINT | L.FORCE_POINT_LOCAL | L.FORCE_LOCAL, f32(12, 0, 0), f32(0, 0.3, 0.4)))
+    if i == 30:
+        ops.append((L.OP_APPLY_TORQUE, 1, 0, f32(0, 0, 3), None))
+    if i == 35:
+        ops.append((L.OP_APPLY_FORCE, 1, L.FORCE_AT_POINT, f32(0, 4, 0), f32(0, 0, 1)))
+    if i == 38:
+        ops.append((L.OP_RESET_TORQUE, 1, 0, None, None))
+    if i == 25:
+        ops.append((L.OP_SET_LINEAR_VELOCITY, 2, 0, f32(-1, 2, 0), None))
+    if i == 50:
+        ops.append((L.OP_SET_TRANSFORM, 2, 0, f32(4, 4, 2), f32(0, 0.6, 0, 0.8)))
+    if i == 51:
+        ops.append((L.OP_SET_ANGULAR_VELOCITY, 2, 0, f32(0, 5, 0), None))
+    if i == 52:
+        ops.append((L.OP_APPLY_FORCE, 2, L.FORCE_LOCAL, f32(0, 0, 30), None))
+    if i == 53:
+        ops.append((L.OP_APPLY_TORQUE, 2, L.FORCE_LOCAL, f32(1, 0, 0), None))
+    if i == 60:
+        ops.append((L.OP_SET_SLEEPING, 1, 1, None, None))
+    if i == 90:
+        ops.append((L.OP_APPLY_FORCE, 1, 0, f32(0, 0, 20), None))
+    if i == 91:
+        ops.append((L.OP_APPLY_FORCE, 1, L.FORCE_AT_POINT | L.FORCE_LOCAL, f32(5, 0, 0), f32(0, 0, 0)))
+    if i == 92:
+        ops.append((L.OP_RESET_FORCE, 1, 0, None, None))
+    return ops
+
+
+@pytest.mark.gpu
+def test_driven_example_calls_like_the_reference(ref_available):
+    """RigidBody::setTransform / set*Velocity / apply*Force* / apply*Torque / reset* / setIsSleeping made
+    between update() calls through the C++ mirror: the force and torque sums after the calls of a
+    frame, the final states and the sleeping flags match the reference making the same calls."""
+    from oracle.ref import RefWorld
+    if not os.path.exists(DRIVEN_EXE):
+        pytest.skip("example not built (python -m reactphysics3d_amd.build)")
+    steps = 150
+    probes = (20, 35, 38, 53, 91)
+    ref = RefWorld(_driven_scene())
+    want_probe = {}
+    for i in range(steps):
+        ref.body_ops(_driven_calls(i))
+        if i in probes:
+            f, t = ref.force_torque()
+            want_probe[i] = np.concatenate([f[1], t[1]])
+        if i == 61:
+            assert ref.sleeping()[1] == 1
+        ref.step(1.0 / 60.0)
+    p, q, v, w = ref.bodies()
+    want = np.concatenate([p, q, v, w], 1)
+    for k, probe in enumerate(probes):
+        out = subprocess.run([DRIVEN_EXE, str(steps), str(probe)], capture_output=True, text=True, timeout=120)
+        assert out.returncode == 0, out.stderr
+        lines = out.stdout.splitlines()
+        got = np.array([list(map(float, line.split()[2:])) for line in lines if line.startswith("probe")][0], np.float32)
+        assert np.array_equal(got, want_probe[probe]), f"frame {probe}: {got} vs {want_probe[probe]}"
+        if k == 0:
+            st = np.array([list(map(float, line.split()[2:])) for line in lines if line.startswith("body")], np.float32)
+            err = np.max(np.abs(st - want))
+            print("max |device - reference|", err)
+            assert err <= 1e-4
+            sl = [list(map(int, line.split()[1:])) for line in lines if line.startswith("sleeping")][0]
+            assert sl == list(ref.sleeping()[1:])
