@@ -97,25 +97,96 @@ class BoxShape : public CollisionShape {
     }
 };
 
-// PolygonVertexArray (collision/PolygonVertexArray.h) with float vertices and int indices.
+// Message (utils/Message.h): what createConvexMesh reports when it refuses a mesh.
+struct Message {
+    enum class Type { Error = 1, Warning = 2, Information = 4 };
+    std::string text;
+    Type type;
+    Message(std::string t, Type ty = Type::Error) : text(std::move(t)), type(ty) {}
+};
+
+// PolygonVertexArray (collision/PolygonVertexArray.h:45-140): strided float or double vertices,
+// strided int or short indices, faces as (nbVertices, indexBase).  The data is copied at construction.
 class PolygonVertexArray {
    public:
+    enum class VertexDataType { VERTEX_FLOAT_TYPE, VERTEX_DOUBLE_TYPE };
+    enum class IndexDataType { INDEX_INTEGER_TYPE, INDEX_SHORT_TYPE };
     struct PolygonFace { uint32 nbVertices; uint32 indexBase; };
-    PolygonVertexArray(uint32 nbVertices, const float* vertices, const int32_t* indices, uint32 nbFaces,
-                       const PolygonFace* faces)
-        : mVertices(vertices, vertices + 3 * nbVertices), mNbFaces(nbFaces) {
+    PolygonVertexArray() = default;
+    PolygonVertexArray(uint32 nbVertices, const void* verticesStart, uint32 verticesStride, const void* indexesStart,
+                       uint32 indexesStride, uint32 nbFaces, PolygonFace* facesStart, VertexDataType vertexDataType,
+                       IndexDataType indexDataType) {
+        init(nbVertices, verticesStart, verticesStride, indexesStart, indexesStride, nbFaces, facesStart,
+             vertexDataType, indexDataType);
+    }
+    void init(uint32 nbVertices, const void* verticesStart, uint32 verticesStride, const void* indexesStart,
+              uint32 indexesStride, uint32 nbFaces, PolygonFace* facesStart, VertexDataType vertexDataType,
+              IndexDataType indexDataType) {
+        mVertexType = vertexDataType;
+        mIndexType = indexDataType;
+        mVertices.clear();
+        mCounts.clear();
+        mIndices.clear();
+        mFaces.assign(facesStart, facesStart + nbFaces);
+        const unsigned char* v = static_cast<const unsigned char*>(verticesStart);
+        for (uint32 i = 0; i < nbVertices; i++)
+            for (int k = 0; k < 3; k++)
+                mVertices.push_back(vertexDataType == VertexDataType::VERTEX_FLOAT_TYPE
+                                        ? reinterpret_cast<const float*>(v + i * verticesStride)[k]
+                                        : (float)reinterpret_cast<const double*>(v + i * verticesStride)[k]);
+        const unsigned char* ix = static_cast<const unsigned char*>(indexesStart);
         for (uint32 f = 0; f < nbFaces; f++) {
-            mCounts.push_back((int32_t)faces[f].nbVertices);
-            for (uint32 k = 0; k < faces[f].nbVertices; k++) mIndices.push_back(indices[faces[f].indexBase + k]);
+            mCounts.push_back((int32_t)facesStart[f].nbVertices);
+            for (uint32 k = 0; k < facesStart[f].nbVertices; k++) {
+                const unsigned char* at = ix + (facesStart[f].indexBase + k) * indexesStride;
+                mIndices.push_back(indexDataType == IndexDataType::INDEX_INTEGER_TYPE
+                                       ? (int32_t) * reinterpret_cast<const uint32*>(at)
+                                       : (int32_t) * reinterpret_cast<const uint16*>(at));
+            }
         }
     }
-    std::vector<float> mVertices;
-    std::vector<int32_t> mCounts, mIndices;
-    uint32 mNbFaces;
+    VertexDataType getVertexDataType() const { return mVertexType; }
+    IndexDataType getIndexDataType() const { return mIndexType; }
+    uint32 getNbVertices() const { return (uint32)(mVertices.size() / 3); }
+    uint32 getNbFaces() const { return (uint32)mCounts.size(); }
+    Vector3 getVertex(uint32 i) const { return Vector3(mVertices[3 * i], mVertices[3 * i + 1], mVertices[3 * i + 2]); }
+    uint32 getVertexIndexInFace(uint32 face, uint32 k) const { return (uint32)mIndices[mFaces[face].indexBase + k]; }
+    const PolygonFace* getPolygonFace(uint32 face) const { return &mFaces[face]; }
+
+    std::vector<float> mVertices;            // xyz per vertex
+    std::vector<int32_t> mCounts, mIndices;  // per face: vertex count; concatenated face vertex indices
+    std::vector<PolygonFace> mFaces;
+
+   private:
+    VertexDataType mVertexType = VertexDataType::VERTEX_FLOAT_TYPE;
+    IndexDataType mIndexType = IndexDataType::INDEX_INTEGER_TYPE;
 };
+
+// VertexArray (collision/VertexArray.h:40-90): a point cloud for createConvexMesh's QuickHull.
+class VertexArray {
+   public:
+    enum class DataType { VERTEX_FLOAT_TYPE, VERTEX_DOUBLE_TYPE };
+    VertexArray(const void* start, uint32 stride, uint32 nbVertices, DataType dataType)
+        : mStart(start), mStride(stride), mNbVertices(nbVertices), mDataType(dataType) {}
+    DataType getDataType() const { return mDataType; }
+    uint32 getNbVertices() const { return mNbVertices; }
+    uint32 getStride() const { return mStride; }
+    const void* getStart() const { return mStart; }
+
+   private:
+    const void* mStart;
+    uint32 mStride, mNbVertices;
+    DataType mDataType;
+};
+
+// ConvexMesh (collision/ConvexMesh.h): the polygons of a hull; each collider that uses it uploads it
+// into its world.
 class ConvexMesh {
    public:
-    explicit ConvexMesh(const PolygonVertexArray& a) : mArray(a) {}
+    explicit ConvexMesh(PolygonVertexArray a) : mArray(std::move(a)) {}
+    uint32 getNbVertices() const { return mArray.getNbVertices(); }
+    uint32 getNbFaces() const { return mArray.getNbFaces(); }
+    const Vector3 getVertex(uint32 i) const { return mArray.getVertex(i); }
     PolygonVertexArray mArray;
 };
 class ConvexMeshShape : public CollisionShape {
@@ -186,7 +257,7 @@ class RigidBody {
         if (shape->type() == RP3DG_SHAPE_CONVEX_MESH) {
             const PolygonVertexArray& a = static_cast<ConvexMeshShape*>(shape)->mesh()->mArray;
             const int32_t h = rp3dg_hull_create(mWorld, a.mVertices.data(), (int32_t)(a.mVertices.size() / 3),
-                                                a.mCounts.data(), a.mIndices.data(), (int32_t)a.mNbFaces);
+                                                a.mCounts.data(), a.mIndices.data(), (int32_t)a.getNbFaces());
             check(h, "createConvexMesh");
             params[0] = (float)h;
         }
@@ -677,11 +748,53 @@ class PhysicsCommon {
     SphereShape* createSphereShape(decimal r) { return keep(std::make_unique<SphereShape>(r)); }
     BoxShape* createBoxShape(const Vector3& he) { return keep(std::make_unique<BoxShape>(he)); }
     CapsuleShape* createCapsuleShape(decimal r, decimal h) { return keep(std::make_unique<CapsuleShape>(r, h)); }
-    ConvexMesh* createConvexMesh(const PolygonVertexArray& a) {
+    // PhysicsCommon::createConvexMesh(PolygonVertexArray, messages) (PhysicsCommon.cpp:600-630): the mesh
+    // is checked as the library builds it (closed, Euler's formula, face size); nullptr and an error
+    // message when it is refused.
+    ConvexMesh* createConvexMesh(const PolygonVertexArray& a, std::vector<Message>& messages) {
+        rp3dg_world* probe = rp3dg_world_create(nullptr);
+        const int32_t h = rp3dg_hull_create(probe, a.mVertices.data(), (int32_t)a.getNbVertices(), a.mCounts.data(),
+                                            a.mIndices.data(), (int32_t)a.getNbFaces());
+        if (h < 0) messages.emplace_back(std::string("Error when creating a ConvexMesh: ") + rp3dg_last_error());
+        rp3dg_world_destroy(probe);
+        if (h < 0) return nullptr;
         mMeshes.push_back(std::make_unique<ConvexMesh>(a));
         return mMeshes.back().get();
     }
-    ConvexMeshShape* createConvexMeshShape(ConvexMesh* m) { return keep(std::make_unique<ConvexMeshShape>(m)); }
+    // PhysicsCommon::createConvexMesh(VertexArray, messages) (PhysicsCommon.cpp:638-660): the convex hull
+    // of the points, polygon for polygon the reference's QuickHull result.
+    ConvexMesh* createConvexMesh(const VertexArray& va, std::vector<Message>& messages) {
+        const int is_double = va.getDataType() == VertexArray::DataType::VERTEX_DOUBLE_TYPE ? 1 : 0;
+        int32_t nv = 0, nf = 0;
+        rp3dg_quickhull(va.getStart(), (int32_t)va.getNbVertices(), (int32_t)va.getStride(), is_double, nullptr, 0,
+                        nullptr, 0, nullptr, 0, &nv, &nf);
+        if (nv <= 0 || nf <= 0) {
+            messages.emplace_back(std::string("Error during the convex hull construction: ") + rp3dg_last_error());
+            return nullptr;
+        }
+        std::vector<float> v(3 * (size_t)nv);
+        std::vector<int32_t> counts((size_t)nf), idx(3 * (size_t)nv + 6 * (size_t)nf);
+        int32_t nv2 = 0, nf2 = 0;
+        if (rp3dg_quickhull(va.getStart(), (int32_t)va.getNbVertices(), (int32_t)va.getStride(), is_double, v.data(),
+                            nv, counts.data(), nf, idx.data(), (int32_t)idx.size(), &nv2, &nf2) < 0) {
+            messages.emplace_back(std::string("Error during the convex hull construction: ") + rp3dg_last_error());
+            return nullptr;
+        }
+        std::vector<PolygonVertexArray::PolygonFace> faces((size_t)nf);
+        uint32 base = 0;
+        for (int32_t f = 0; f < nf; f++) { faces[f] = {(uint32)counts[f], base}; base += (uint32)counts[f]; }
+        return createConvexMesh(PolygonVertexArray((uint32)nv, v.data(), 3 * sizeof(float), idx.data(), sizeof(int32_t),
+                                                   (uint32)nf, faces.data(),
+                                                   PolygonVertexArray::VertexDataType::VERTEX_FLOAT_TYPE,
+                                                   PolygonVertexArray::IndexDataType::INDEX_INTEGER_TYPE),
+                                messages);
+    }
+    // createConvexMeshShape (PhysicsCommon.h:206): a scaled mesh is not supported (the scaling must be 1).
+    ConvexMeshShape* createConvexMeshShape(ConvexMesh* m, const Vector3& scaling = Vector3(1, 1, 1)) {
+        if (scaling.x != 1 || scaling.y != 1 || scaling.z != 1)
+            throw std::runtime_error("createConvexMeshShape: scaled convex meshes are not supported");
+        return keep(std::make_unique<ConvexMeshShape>(m));
+    }
 
    private:
     template <typename T>

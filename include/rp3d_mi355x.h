@@ -153,7 +153,8 @@ int rp3dg_collider_get_kind(rp3dg_world* world, int32_t collider);
 
 /* PhysicsCommon::createConvexMesh(PolygonVertexArray) (src/engine/PhysicsCommon.cpp:600-630):
  * vertices xyz[nv], faces given as counts[nf] + concatenated vertex indices (CCW seen from outside).
- * Returns the hull index used as params[0] of RP3DG_SHAPE_CONVEX_MESH. */
+ * Returns the hull index used as params[0] of RP3DG_SHAPE_CONVEX_MESH.  Limits: at most 255 half-edges
+ * and 24 vertices per face (RP3DG_EUNSUPPORTED beyond). */
 int32_t rp3dg_hull_create(rp3dg_world* world, const float* vertices, int32_t nb_vertices,
                           const int32_t* face_counts, const int32_t* face_indices, int32_t nb_faces);
 

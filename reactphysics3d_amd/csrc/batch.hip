@@ -972,6 +972,7 @@ int rp3dg_batch::enqueue_step(float dt, bool capturing) {
     A.hull_face_edge = d_hfedge; A.hull_face_verts = d_hfverts; A.hull_face_vidx = d_hfvidx;
     A.persistent_threshold = settings.persistent_threshold;
     A.cos_similar = settings.cos_similar;
+    A.status = dstatus;
     LAUNCH_T("np_classify", std::min(pg, 1024), 256, np_classify, A, np_lists, pair_cap, counters->nb_tests,
              &counters->nb_box_tests, &counters->nb_box_sep_tests, &counters->nb_hull_tests);
     int* L = np_lists;
@@ -1224,6 +1225,7 @@ int rp3dg_batch::finish_steps(int n) {
         if (last.status & STATUS_COLOR_OVERFLOW) what += " more than 64 solver colours;";
         if (last.status & STATUS_STACK_OVERFLOW) what += " broadphase traversal stack overflow;";
         if (last.status & STATUS_COLOR_STALL) what += " colouring did not finish;";
+        if (last.status & STATUS_CLIP_OVERFLOW) what += " face clipping exceeded its buffer;";
         set_error("device status " + std::to_string(last.status) + ":" + what);
         return RP3DG_EOVERFLOW;
     }

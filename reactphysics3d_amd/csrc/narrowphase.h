@@ -20,6 +20,7 @@ struct NpArgs {
     const int* hull_face_vidx;
     float persistent_threshold;
     float cos_similar;
+    int* status;               // device status word (STATUS_*)
 };
 
 // Narrowphase test lists: bins 1..5 closed forms, ALGO_POLY_POLY pairs with a convex mesh,

@@ -111,6 +111,7 @@ template <int N>
 struct RawBuf {
     static constexpr int kN = N;
     int n;
+    bool clip_full = false;   // a clipped polygon had more vertices than the clip buffer holds
     V3 nrm[N];
     float depth[N];
     V3 l1[N];
@@ -162,6 +163,7 @@ struct LdsClip {
 struct RawBoxView {
     static constexpr int kN = 8;
     int n;
+    bool clip_full = false;
     V3 nrm0;
     IdxList ci;       // clip-buffer index of each face contact
     LdsClip clip;     // the buffer that holds the clipped incident face
@@ -207,7 +209,7 @@ constexpr int kBoxClip = 8;     // vertices of a box face clipped by the four si
 // ------------------------------------------------------------------------------------------------
 constexpr float kSatRel = 1.002f;    // SATAlgorithm.cpp:43
 constexpr float kSatAbs = 0.0005f;   // SATAlgorithm.cpp:44
-constexpr int kClipMax = 48;
+constexpr int kClipMax = 2 * kMaxHullFaceVertices;
 
 // :1095-1116
 __device__ inline float sat_single_face(const Poly& P1, const Poly& P2, const Tf& t12, int f) {
@@ -261,8 +263,9 @@ __device__ inline float sat_edge_distance(V3 e1a, V3 e2a, V3 c1, V3 e1d, V3 e2d,
 }
 
 // mathematics_functions.h:332-392
+// A vertex past the buffer's CM is dropped and `full` is set (the caller reports it in the status word).
 template <int CM, class CB>
-__device__ inline int clip_polygon(const CB& in, int nin, V3 pp, V3 pn, CB& out) {
+__device__ inline int clip_polygon(const CB& in, int nin, V3 pp, V3 pn, CB& out, bool& full) {
     int nout = 0;
     int s = nin - 1;
     const float pnd = dot(pn, pp);
@@ -273,15 +276,16 @@ __device__ inline int clip_polygon(const CB& in, int nin, V3 pp, V3 pn, CB& out)
         if (edn >= 0.0f) {
             if (sdn < 0.0f) {
                 const float t = plane_segment_intersection(vs, ve, pnd, pn);
-                if (t >= 0.0f && t <= 1.0f) { if (nout < CM) out.put(nout++, vs + t * (ve - vs)); }
-                else { if (nout < CM) out.put(nout++, ve); }
+                if (t >= 0.0f && t <= 1.0f) { if (nout < CM) out.put(nout++, vs + t * (ve - vs)); else full = true; }
+                else { if (nout < CM) out.put(nout++, ve); else full = true; }
             }
             if (nout < CM) out.put(nout++, ve);
+            else full = true;
         } else {
             if (sdn >= 0.0f) {
                 const float t = plane_segment_intersection(vs, ve, -pnd, -pn);
-                if (t >= 0.0f && t <= 1.0f) { if (nout < CM) out.put(nout++, vs + t * (ve - vs)); }
-                else { if (nout < CM) out.put(nout++, vs); }
+                if (t >= 0.0f && t <= 1.0f) { if (nout < CM) out.put(nout++, vs + t * (ve - vs)); else full = true; }
+                else { if (nout < CM) out.put(nout++, vs); else full = true; }
             }
         }
         s = e;
@@ -305,6 +309,7 @@ __device__ __forceinline__ bool sat_face_contacts_t(bool refIs1, const Poly& P1,
     const int incFace = I.most_anti_parallel_face(axisInc);
     int na = 0, nb = 0;
     const int niv = I.face_nverts(incFace);
+    if (niv > CM) raw.clip_full = true;
     for (int i = 0; i < niv && na < CM; i++) bufA.put(na++, tapply(incToRef, I.vertex(I.face_vertex(incFace, i))));
     const int firstEdge = R.face_edge(minFace);
     bool v1Input = false;
@@ -321,7 +326,7 @@ __device__ __forceinline__ bool sat_face_contacts_t(bool refIs1, const Poly& P1,
         const V3 planeNormal = cross(axisRef, ev2 - ev1);
         // one inlined copy of the clipping, alternating its input and output buffers
         const int nn = clip_polygon<CM>(v1Input ? bufA : bufB, v1Input ? na : nb, ev1, planeNormal,
-                                        v1Input ? bufB : bufA);
+                                        v1Input ? bufB : bufA, raw.clip_full);
         if (v1Input) nb = nn;
         else na = nn;
         curIdx = cur.w;
@@ -623,6 +628,7 @@ __device__ void contact_epilogue(const NpArgs& A, int slot, bool colliding, cons
     DevPairs P = A.P;
     const DevPairs& O = A.O;
     const int prev = P.prev[slot];   // this pair's entry in the previous table, -1 when new
+    if (raw.clip_full) atomicOr(A.status, STATUS_CLIP_OVERFLOW);
     if (!colliding || raw.n == 0 || trigger_pair(A, slot)) {
         P.ncp[slot] = 0;
         return;

@@ -62,6 +62,9 @@ constexpr int kTravBlock = 128;       // threads per block of the broadphase tra
 constexpr int kTravSlab = 8;          // partners kept per collider by the first traversal pass
 constexpr int kMaxRawContacts = 16;   // NB_MAX_CONTACT_POINTS_IN_NARROWPHASE_INFO (configuration.h:134)
 constexpr int kMaxManifoldPoints = 4; // MAX_CONTACT_POINTS_IN_MANIFOLD
+// Hull faces have at most this many vertices: clipping an incident face by the side planes of a
+// reference face adds at most one vertex per plane, so the clip buffers hold 2 * 24 = 48 vertices.
+constexpr int kMaxHullFaceVertices = 24;
 
 // Convex hull arena (ConvexMeshShape + HalfEdgeStructure), one record per hull.
 struct HullHeader {
@@ -207,5 +210,6 @@ constexpr int STATUS_PAIR_OVERFLOW = 1;
 constexpr int STATUS_COLOR_OVERFLOW = 2;
 constexpr int STATUS_STACK_OVERFLOW = 4;
 constexpr int STATUS_COLOR_STALL = 8;
+constexpr int STATUS_CLIP_OVERFLOW = 16;   // a face clip produced more vertices than its buffer holds
 
 }  // namespace rp3dg

@@ -678,6 +678,11 @@ int32_t rp3dg_hull_create(rp3dg_world* w, const float* verts, int32_t nv, const 
     int nedges = 0;
     for (int f = 0; f < nf; f++) {
         if (counts[f] < 3) { set_error("hull face with less than three vertices"); return RP3DG_EINVAL; }
+        if (counts[f] > kMaxHullFaceVertices) {
+            // the face-contact clipping keeps at most 2 * kMaxHullFaceVertices vertices per polygon
+            set_error("hull face with more than " + std::to_string(kMaxHullFaceVertices) + " vertices");
+            return RP3DG_EUNSUPPORTED;
+        }
         std::vector<int> fv(idx + off, idx + off + counts[f]);
         off += counts[f];
         nedges += counts[f];

@@ -54,3 +54,29 @@ def cone_anchor_at_com():
     sc.add_collider(b, S.BOX, (0.3, 0.2, 0.6))
     sc.add_joint(S.BALLSOCKET, a, b, (0.0, 0.0, 4.0), flags=1, limits=(0.4, 0.0, 0.0, 0.0))
     return sc
+
+
+def prism_hull(n, radius=1.0, half_height=0.5):
+    """An n-gon prism as a polygon mesh: bottom ring then top ring of vertices, two n-vertex caps and n
+    quads, counter-clockwise seen from outside."""
+    a = 2.0 * np.pi * np.arange(n) / n
+    ring = np.stack([radius * np.cos(a), radius * np.sin(a)], 1)
+    v = np.concatenate([np.c_[ring, -half_height * np.ones(n)], np.c_[ring, half_height * np.ones(n)]])
+    faces = [list(range(n - 1, -1, -1)), list(range(n, 2 * n))]
+    faces += [[i, (i + 1) % n, n + (i + 1) % n, n + i] for i in range(n)]
+    return v.astype(np.float32), faces
+
+
+def stacked_prisms(n=24, twist=0.07):
+    """A 24-gon prism resting cap to cap on a static one, turned by `twist` radians about the common
+    axis, gravity on: the face contact clips a 24-vertex incident cap by the 24 side planes of the
+    reference cap, the largest polygon the clip buffers hold (48 vertices)."""
+    sc = S.Scene("stacked_prisms", S.make_settings(gravity=(0.0, 0.0, -9.81)))
+    v, f = prism_hull(n)
+    h = sc.add_hull(v, f)
+    base = sc.add_body(S.STATIC, (0.0, 0.0, 0.0))
+    sc.add_collider(base, S.HULL, (float(h),))
+    s, c = np.sin(twist / 2.0), np.cos(twist / 2.0)
+    top = sc.add_body(S.DYNAMIC, (0.02, -0.01, 0.995), (0.0, 0.0, s, c))
+    sc.add_collider(top, S.HULL, (float(h),))
+    return sc

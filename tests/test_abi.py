@@ -100,6 +100,31 @@ def test_hull_rejects_open_mesh():
     L.rp3dg_world_destroy(w)
 
 
+def _prism(n):
+    """An n-gon prism: vertices (bottom ring, top ring), two n-vertex caps and n quads, CCW outside."""
+    import numpy as np
+    a = 2.0 * np.pi * np.arange(n) / n
+    ring = np.stack([np.cos(a), np.sin(a)], 1)
+    verts = np.concatenate([np.c_[ring, -np.ones(n)], np.c_[ring, np.ones(n)]]).astype(np.float32)
+    faces = [list(range(n - 1, -1, -1)), list(range(n, 2 * n))]
+    faces += [[i, (i + 1) % n, n + (i + 1) % n, n + i] for i in range(n)]
+    counts = np.array([len(f) for f in faces], np.int32)
+    idx = np.concatenate([np.array(f, np.int32) for f in faces])
+    return verts, counts, idx
+
+
+def test_hull_face_size_limit():
+    """Faces of up to 24 vertices are accepted (the face clipping holds incident face + reference
+    face sides = 48 vertices); a 25-vertex cap is refused with RP3DG_EUNSUPPORTED."""
+    L = lib.load()
+    w = L.rp3dg_world_create(None)
+    for n, want in ((24, 0), (25, -5)):
+        v, c, i = _prism(n)
+        r = L.rp3dg_hull_create(w, lib._fp(v), len(v), lib._ip(c), lib._ip(i), len(c))
+        assert (r >= 0) if want == 0 else r == want, (n, r, L.rp3dg_last_error())
+    L.rp3dg_world_destroy(w)
+
+
 def test_scene_loader_runs_on_host():
     sc = S.box_pile(2, 2)
     w = lib.world_from_scene(sc)

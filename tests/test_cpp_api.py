@@ -235,3 +235,36 @@ def test_driven_example_calls_like_the_reference(ref_available):
             assert err <= 1e-4
             sl = [list(map(int, line.split()[1:])) for line in lines if line.startswith("sleeping")][0]
             assert sl == list(ref.sleeping()[1:])
+
+
+MESHES_EXE = os.path.join(ROOT, "reactphysics3d_amd", "_lib", "meshes")
+
+
+def test_convex_mesh_signatures_of_the_reference():
+    """PolygonVertexArray with strided double vertices and short indices, VertexArray + QuickHull and
+    createConvexMesh(..., messages) as the reference declares them (PolygonVertexArray.h:102,
+    VertexArray.h:66, PhysicsCommon.h:230-233): the box mesh is accepted, the hull's polygons are the
+    library's QuickHull polygons of the same points, the open mesh is refused with one error message."""
+    from reactphysics3d_amd import lib
+    if not os.path.exists(MESHES_EXE):
+        pytest.skip("example not built (python -m reactphysics3d_amd.build)")
+    out = subprocess.run([MESHES_EXE], capture_output=True, text=True, timeout=120)
+    assert out.returncode == 0, out.stderr
+    lines = out.stdout.splitlines()
+    get = lambda tag: [l.split()[1:] for l in lines if l.startswith(tag + " ")]
+    assert get("box") == [["1", "8", "6"]]
+    corners = [[1.0 if i & 1 else -1.0, 1.0 if i & 2 else -1.0, 1.0 if i & 4 else -1.0] for i in range(8)]
+    extra = [[0, 0, 0], [0.5, 0.25, 1], [-0.25, 1, 0.5], [1, 0, 0], [0.3, -1, 0.1], [0, 0, -1]]
+    v, c, idx = lib.quickhull(np.array(corners + extra, np.float32))
+    assert get("hull") == [["1", str(len(v)), str(len(c))]]
+    want, o = [], 0
+    for f, n in enumerate(c):
+        want.append(v[idx[o:o + n]].ravel())
+        o += n
+    faces = get("face")
+    assert len(faces) == len(want)
+    for f, row in enumerate(faces):
+        assert np.array_equal(np.array(row[1:], np.float32), want[f]), f"face {f}"
+    assert get("open") == [["0", "1"]]
+    assert get("message")[0][0] == "1"   # Message::Type::Error
+    assert get("colliders") == [["0", "1"]]

@@ -1,8 +1,9 @@
 """The narrowphase's degenerate branches and the device's overflow report, against the reference:
 coincident sphere centres, parallel capsules (apart and with overlapping inner segments), a capsule
 lying on a convex-mesh face (GJK margin and SAT face cases), a cone limit anchored at its body's centre of
-mass (bit for bit, with the contact and joint order fed to the oracle), and a pair table too small for
-a settling pile (the step reports RP3DG_EOVERFLOW with the reason instead of writing past the table)."""
+mass, two 24-gon prisms stacked cap to cap (the largest clipped polygon the face clipping holds), bit
+for bit with the contact and joint order fed to the oracle, and a pair table too small for a settling
+pile (the step reports RP3DG_EOVERFLOW with the reason instead of writing past the table)."""
 import numpy as np
 import pytest
 
@@ -22,6 +23,7 @@ DT = 1.0 / 60.0
     ("capsule_on_face_gjk", lambda: E.capsule_on_mesh_face(0.02), 30, 2),
     ("capsule_on_face_sat", lambda: E.capsule_on_mesh_face(0.2), 30, 2),
     ("cone_anchor_at_com", E.cone_anchor_at_com, 120, None),
+    ("stacked_24gon_prisms", E.stacked_prisms, 60, 4),
 ])
 @pytest.mark.parametrize("solver", ["fused", "per_color"])
 def test_degenerate_cases_match_reference(name, make, steps, points, solver):
