@@ -310,6 +310,30 @@ int64_t rp3dg_batch_get_contact_events(rp3dg_batch* batch, int32_t* collider1, i
 int rp3dg_world_set_contact_events(rp3dg_world* world, int32_t enable);
 /* The batch of one that rp3dg_world_update steps (NULL before the first update). */
 rp3dg_batch* rp3dg_world_get_batch(rp3dg_world* world);
+/* World queries (PhysicsWorld::testOverlap / testCollision, PhysicsWorld.h:319-334,
+ * CollisionDetectionSystem.cpp:1775-1896) on the current state.  As in the reference, a query first runs
+ * the broadphase (computeBroadPhase): pairs are created and removed, the moved set empties, and pairs it
+ * removes while colliding report ContactExit at the next step; the pairs it tests update their
+ * LastFrameCollisionInfo (SAT without clipWithPreviousAxisIfStillColliding), the contacts it finds are
+ * reported here and do not replace the last step's contacts.  A query applies to every world of the
+ * batch, as if each world made it (bodies are batch indices and belong to one world); pairs are listed in
+ * no particular order.
+ * testOverlap(body1, body2): 1 when a pair of the two bodies' colliders collides, else 0. */
+int rp3dg_batch_test_overlap_bodies(rp3dg_batch* batch, int32_t body1, int32_t body2);
+/* testOverlap(OverlapCallback&) (body < 0: pairs of world-query colliders) or testOverlap(body,
+ * OverlapCallback&) (pairs of the body): returns the number of overlapping collider pairs; read them with
+ * rp3dg_batch_get_query_results (OverlapPair::getCollider1/2, no points). */
+int64_t rp3dg_batch_test_overlap(rp3dg_batch* batch, int32_t body);
+/* testCollision(CollisionCallback&) (body1 < 0: pairs of world-query colliders), testCollision(body1,
+ * CollisionCallback&) (body2 < 0) or testCollision(body1, body2, CollisionCallback&): returns the number
+ * of colliding pairs; read them with rp3dg_batch_get_query_results.  Pairs with a trigger collider are not
+ * listed (CollisionCallback::CallbackData keeps contact pairs only, CollisionCallback.cpp:60-70). */
+int64_t rp3dg_batch_test_collision(rp3dg_batch* batch, int32_t body1, int32_t body2);
+/* What the CallbackData of the last rp3dg_batch_test_overlap / test_collision holds, one record per pair
+ * in the contact view layout: colliders, nb_points, normal_depth, local1, local2
+ * (ContactPoint::getWorldNormal / getPenetrationDepth / getLocalPointOnCollider1/2); impulse, friction
+ * and color are written 0.  Reading does not run the query again.  Returns the count. */
+int64_t rp3dg_batch_get_query_results(rp3dg_batch* batch, rp3dg_contact_view* out, int64_t capacity);
 /* RigidBody::applyWorldForceAtCenterOfMass / applyWorldTorque (src/body/RigidBody.cpp:298-311, 938-951)
  * on every dynamic body of the batch whose force / torque (xyz per body, NULL = none) is not zero,
  * before the next step (the step clears the accumulated forces); a sleeping body is woken. */
@@ -356,12 +380,21 @@ int rp3dg_batch_get_force_torque(rp3dg_batch* batch, float* force3, float* torqu
 /* PhysicsWorld::raycast (PhysicsWorld.h:486-490, CollisionDetectionSystem.cpp:1186-1195)
  * with a RaycastCallback that keeps the closest hit (notifyRaycastHit returns the hit fraction), for
  * n rays at once: ray i runs from point1 to point2 (xyz each) in world world[i] (NULL: world 0) with
- * max fraction max_fraction[i] (NULL: 1) against the colliders whose category bits meet
+ * max fraction max_fraction[i] (NULL: 1) against the world-query colliders whose category bits meet
  * category_mask, in the state after the last step.  Out: collider[i] (batch index, -1 for no hit),
  * hit point and unit normal in world space (xyz each) and hit fraction; NULL outputs are skipped. */
 int rp3dg_batch_raycast(rp3dg_batch* batch, int64_t n, const int32_t* world, const float* point1, const float* point2,
                         const float* max_fraction, uint16_t category_mask, int32_t* collider, float* point,
                         float* normal, float* fraction);
+/* PhysicsWorld::raycast (PhysicsWorld.h:486-490) with a RaycastCallback whose notifyRaycastHit returns 1
+ * (DynamicAABBTree.cpp:684-747: the ray is never clipped): every world-query collider whose category
+ * passes the mask and whose shape the ray crosses within its max fraction, for n rays at once (arguments
+ * as rp3dg_batch_raycast).  Hits are written sorted by (ray, collider): ray index, collider, world point,
+ * world normal and fraction (NULL = skip).  Returns the hit count, or RP3DG_EINVAL when it exceeds
+ * capacity (capacity 0 with NULL ray and collider returns the count). */
+int64_t rp3dg_batch_raycast_all(rp3dg_batch* batch, int64_t n, const int32_t* world, const float* point1,
+                                const float* point2, const float* max_fraction, uint16_t category_mask, int32_t* ray,
+                                int32_t* collider, float* point, float* normal, float* fraction, int64_t capacity);
 /* RigidBody::isSleeping of every body of the batch (batch order), 0 / 1; capacity >= body count. */
 int rp3dg_batch_get_sleeping(rp3dg_batch* batch, int32_t* sleeping, int64_t capacity);
 /* Joint solve order (joint indices in the order the device solved them this step, per type block). */

@@ -157,6 +157,67 @@ class RefWorld:
                                _fp(pt), _fp(nr), _fp(fr))
         return col, pt, nr, fr
 
+
+    def test_overlap_bodies(self, b1, b2):
+        """PhysicsWorld::testOverlap(body1, body2) -> bool."""
+        self.L.rp3dref_test_overlap_bodies.argtypes = [C.c_void_p, C.c_int, C.c_int]
+        self.L.rp3dref_test_overlap_bodies.restype = C.c_int
+        return bool(self.L.rp3dref_test_overlap_bodies(self.h, int(b1), int(b2)))
+
+    def test_overlap(self, body=-1):
+        """testOverlap(callback) (body < 0) or testOverlap(body, callback): (collider1, collider2) arrays in
+        the reference's report order."""
+        ip = C.POINTER(C.c_int32)
+        self.L.rp3dref_test_overlap.argtypes = [C.c_void_p, C.c_int, ip, ip, C.c_int]
+        self.L.rp3dref_test_overlap.restype = C.c_int
+        cap = 64 * max(1, self.nc)
+        c1 = np.zeros(cap, np.int32)
+        c2 = np.zeros(cap, np.int32)
+        n = self.L.rp3dref_test_overlap(self.h, int(body), _ip(c1), _ip(c2), cap)
+        if n > cap:   # a second call would not see the same state: the query changes it
+            raise RuntimeError(f"test_overlap: {n} pairs, capacity {cap}")
+        return c1[:n], c2[:n]
+
+    def test_collision(self, b1=-1, b2=-1):
+        """testCollision(callback / body1 / body1, body2): contact records as Batch.test_collision."""
+        ip = C.POINTER(C.c_int32)
+        fp = C.POINTER(C.c_float)
+        self.L.rp3dref_test_collision.argtypes = [C.c_void_p, C.c_int, C.c_int, ip, ip, fp, fp, fp, C.c_int]
+        self.L.rp3dref_test_collision.restype = C.c_int
+        cap = 64 * max(1, self.nc)
+        out = {"colliders": np.zeros((cap, 2), np.int32), "nb_points": np.zeros(cap, np.int32),
+               "normal_depth": np.zeros((cap, 4, 4), np.float32), "local1": np.zeros((cap, 4, 3), np.float32),
+               "local2": np.zeros((cap, 4, 3), np.float32)}
+        n = self.L.rp3dref_test_collision(self.h, int(b1), int(b2), _ip(out["colliders"]), _ip(out["nb_points"]),
+                                          _fp(out["normal_depth"]), _fp(out["local1"]), _fp(out["local2"]), cap)
+        if n > cap:
+            raise RuntimeError(f"test_collision: {n} contact pairs, capacity {cap}")
+        return {k: a[:n] for k, a in out.items()}
+
+    def broadphase_only(self):
+        """computeBroadPhase alone (the broadphase every query runs in its world)."""
+        self.L.rp3dref_broadphase_only.argtypes = [C.c_void_p]
+        self.L.rp3dref_broadphase_only(self.h)
+
+    def raycast_all(self, p1, p2, max_fraction=None, mask=0xFFFF):
+        """Every hit (callback returning 1), in the reference's report order: (ray, collider, point,
+        normal, fraction) arrays."""
+        p1 = np.ascontiguousarray(p1, np.float32)
+        p2 = np.ascontiguousarray(p2, np.float32)
+        n = len(p1)
+        mf = None if max_fraction is None else np.ascontiguousarray(max_fraction, np.float32)
+        fp, ip = C.POINTER(C.c_float), C.POINTER(C.c_int32)
+        self.L.rp3dref_raycast_all.argtypes = [C.c_void_p, C.c_int, fp, fp, fp, C.c_int, ip, fp, C.c_int]
+        self.L.rp3dref_raycast_all.restype = C.c_int
+        cap = 0
+        while True:
+            rc = np.zeros((max(cap, 1), 2), np.int32)
+            pnf = np.zeros((max(cap, 1), 7), np.float32)
+            total = self.L.rp3dref_raycast_all(self.h, n, _fp(p1), _fp(p2), _fp(mf) if mf is not None else None, int(mask),
+                                               rc.ctypes.data_as(ip), _fp(pnf), cap)
+            if total <= cap:
+                return rc[:total, 0], rc[:total, 1], pnf[:total, 0:3], pnf[:total, 3:6], pnf[:total, 6]
+            cap = total
     def apply_forces(self, force=None, torque=None):
         """RigidBody::applyWorldForceAtCenterOfMass / applyWorldTorque on every body ([nb, 3] each)."""
         f = None if force is None else np.ascontiguousarray(force, np.float32)

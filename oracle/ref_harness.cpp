@@ -676,6 +676,132 @@ int rp3dref_raycast(void* h, int n, const float* p1, const float* p2, const floa
     return 0;
 }
 
+// PhysicsWorld::raycast of n rays with a callback that keeps every hit (notifyRaycastHit returns 1):
+// the hits in the order the reference reports them, (ray, collider, point, normal, fraction) each.
+// Returns the number of hits (only counted past cap).
+struct AllHits : public RaycastCallback {
+    RefWorld* R = nullptr;
+    int ray = 0;
+    std::vector<int32_t> rc;
+    std::vector<float> pnf;
+    decimal notifyRaycastHit(const RaycastInfo& i) override {
+        rc.push_back(ray);
+        rc.push_back(collider_index(R, i.collider->getEntity()));
+        for (int k = 0; k < 3; k++) pnf.push_back((float)i.worldPoint[k]);
+        for (int k = 0; k < 3; k++) pnf.push_back((float)i.worldNormal[k]);
+        pnf.push_back((float)i.hitFraction);
+        return decimal(1.0);
+    }
+};
+
+int rp3dref_raycast_all(void* h, int n, const float* p1, const float* p2, const float* maxf, int mask,
+                        int32_t* ray_collider, float* point_normal_fraction, int cap) {
+    RefWorld* R = (RefWorld*)h;
+    AllHits cb;
+    cb.R = R;
+    for (int i = 0; i < n; i++) {
+        const Ray ray(Vector3(p1[3 * i], p1[3 * i + 1], p1[3 * i + 2]), Vector3(p2[3 * i], p2[3 * i + 1], p2[3 * i + 2]),
+                      maxf ? maxf[i] : decimal(1.0));
+        cb.ray = i;
+        R->world->raycast(ray, &cb, (unsigned short)mask);
+    }
+    const int total = (int)(cb.rc.size() / 2);
+    for (int k = 0; k < total && k < cap; k++) {
+        ray_collider[2 * k] = cb.rc[2 * k];
+        ray_collider[2 * k + 1] = cb.rc[2 * k + 1];
+        for (int j = 0; j < 7; j++) point_normal_fraction[7 * k + j] = cb.pnf[7 * k + j];
+    }
+    return total;
+}
+
+// World queries.  rp3dref_test_overlap_bodies: PhysicsWorld::testOverlap(body1, body2).
+int rp3dref_test_overlap_bodies(void* h, int b1, int b2) {
+    RefWorld* R = (RefWorld*)h;
+    return R->world->testOverlap(R->bodies[b1], R->bodies[b2]) ? 1 : 0;
+}
+
+struct OverlapRecorder : public OverlapCallback {
+    RefWorld* R = nullptr;
+    std::vector<int32_t> cols;
+    void onOverlap(CallbackData& data) override {
+        for (uint32 i = 0; i < data.getNbOverlappingPairs(); i++) {
+            const OverlapPair p = data.getOverlappingPair(i);
+            cols.push_back(collider_index(R, p.getCollider1()->getEntity()));
+            cols.push_back(collider_index(R, p.getCollider2()->getEntity()));
+        }
+    }
+};
+
+// testOverlap(callback) (body < 0) or testOverlap(body, callback): the reported collider pairs in report
+// order; returns their count (pairs past cap are counted only).
+int rp3dref_test_overlap(void* h, int body, int32_t* c1, int32_t* c2, int cap) {
+    RefWorld* R = (RefWorld*)h;
+    OverlapRecorder cb;
+    cb.R = R;
+    if (body < 0) R->world->testOverlap(cb);
+    else R->world->testOverlap(R->bodies[body], cb);
+    const int n = (int)cb.cols.size() / 2;
+    for (int i = 0; i < n && i < cap; i++) { c1[i] = cb.cols[2 * i]; c2[i] = cb.cols[2 * i + 1]; }
+    return n;
+}
+
+struct CollisionRecorder : public CollisionCallback {
+    RefWorld* R = nullptr;
+    std::vector<int32_t> cols, npts;
+    std::vector<float> nd, l1, l2;
+    void onContact(const CallbackData& data) override {
+        for (uint32 i = 0; i < data.getNbContactPairs(); i++) {
+            const ContactPair p = data.getContactPair(i);
+            cols.push_back(collider_index(R, p.getCollider1()->getEntity()));
+            cols.push_back(collider_index(R, p.getCollider2()->getEntity()));
+            const uint32 n = p.getNbContactPoints();
+            npts.push_back((int32_t)n);
+            for (uint32 k = 0; k < 4; k++) {
+                if (k < n) {
+                    const ContactPoint c = p.getContactPoint(k);
+                    for (int j = 0; j < 3; j++) nd.push_back((float)c.getWorldNormal()[j]);
+                    nd.push_back((float)c.getPenetrationDepth());
+                    for (int j = 0; j < 3; j++) l1.push_back((float)c.getLocalPointOnCollider1()[j]);
+                    for (int j = 0; j < 3; j++) l2.push_back((float)c.getLocalPointOnCollider2()[j]);
+                } else {
+                    for (int j = 0; j < 4; j++) nd.push_back(0.f);
+                    for (int j = 0; j < 3; j++) { l1.push_back(0.f); l2.push_back(0.f); }
+                }
+            }
+        }
+    }
+};
+
+// testCollision(callback) (b1 < 0), testCollision(b1, callback) (b2 < 0) or testCollision(b1, b2,
+// callback): one record per reported contact pair (colliders, point count, 4 x normal + depth, 4 x local
+// point on collider 1 and 2), in report order; returns the count (records past cap are counted only).
+int rp3dref_test_collision(void* h, int b1, int b2, int32_t* cols, int32_t* npts, float* nd, float* l1, float* l2,
+                           int cap) {
+    RefWorld* R = (RefWorld*)h;
+    CollisionRecorder cb;
+    cb.R = R;
+    if (b1 < 0) R->world->testCollision(cb);
+    else if (b2 < 0) R->world->testCollision(R->bodies[b1], cb);
+    else R->world->testCollision(R->bodies[b1], R->bodies[b2], cb);
+    const int n = (int)cb.npts.size();
+    for (int i = 0; i < n && i < cap; i++) {
+        cols[2 * i] = cb.cols[2 * i];
+        cols[2 * i + 1] = cb.cols[2 * i + 1];
+        npts[i] = cb.npts[i];
+        for (int j = 0; j < 16; j++) nd[16 * i + j] = cb.nd[16 * i + j];
+        for (int j = 0; j < 12; j++) { l1[12 * i + j] = cb.l1[12 * i + j]; l2[12 * i + j] = cb.l2[12 * i + j]; }
+    }
+    return n;
+}
+
+// CollisionDetectionSystem::computeBroadPhase alone: what a query about bodies of another world leaves in
+// this one (a query runs the broadphase of its world before filtering pairs by body).
+int rp3dref_broadphase_only(void* h) {
+    RefWorld* R = (RefWorld*)h;
+    R->world->mCollisionDetection.computeBroadPhase();
+    return 0;
+}
+
 // RigidBody::applyWorldForceAtCenterOfMass / applyWorldTorque on every body whose force / torque (xyz
 // each, NULL = none) is not zero.
 int rp3dref_apply_forces(void* h, const float* f, const float* t) {

@@ -13,6 +13,7 @@
 #include "fused.h"
 #include "islands.h"
 #include "joints.h"
+#include "narrowphase.h"
 #include "rp3d_internal.h"
 
 struct rp3dg_world;
@@ -103,7 +104,24 @@ struct rp3dg_batch {
     bool api_sleep_op = false;         // a queued call puts a body to sleep (its pairs restart their records)
     int *d_coll_offs = nullptr, *d_coll_list = nullptr;   // colliders of each body (CSR)
     std::vector<int> body_type;        // RP3DG_STATIC / KINEMATIC / DYNAMIC of every body
+    std::vector<int> coll_kind;        // CK_* bits of every collider
     int flush_api();
+
+    // world queries (testOverlap / testCollision): their contacts go to these arrays, not to the pair
+    // table, whose contact caches keep the last step's; exits of the pairs their broadphase removes wait in
+    // ev_pend for the next step
+    int* q_ncp = nullptr;
+    float4 *q_pn = nullptr, *q_pl1 = nullptr, *q_pl2 = nullptr, *q_fv1 = nullptr, *q_fv2 = nullptr, *q_fimp = nullptr;
+    int4* ev_pend = nullptr;
+    int* ev_pend_count = nullptr;
+    bool ev_pending = false;
+    bool query_points = false;         // the last query was testCollision (its contact points are kept)
+    bool contacts_is_a = true;         // pair table that holds the last step's contacts (read by get_contacts)
+    int contacts_np = 0;               // its pair count
+    int enqueue_broadphase(bool capturing, const rp3dg::DevEvents& E, int pg);
+    rp3dg::NpArgs np_args(const rp3dg::DevPairs& P, const rp3dg::DevPairs& O, int query, int report, int body1, int body2);
+    int enqueue_narrowphase(const rp3dg::NpArgs& A, bool capturing, int pg);
+    int query(int mode, int report, int body1, int body2);
 
     // launch hints from the last completed step (grids are grid-stride, any size is correct)
     int hint_pairs = 0, hint_manifolds = 0, hint_rounds = 8;

@@ -114,6 +114,29 @@ __global__ void set_body_com(DevBodies B) {
     }
 }
 
+// After a world query's broadphase: every surviving pair of the new table takes the contact cache of its
+// entry in the previous table (the last step's contacts, which the next step warm-starts from: the
+// reference's snapshot queries leave mPreviousContactManifolds alone); new pairs have none.
+__global__ void qr_carry(DevPairs N, DevPairs O, const int* n_new) {
+    const int n = *n_new;
+    RP3DG_GRID_STRIDE(k, n) {
+        const int p = N.prev[k];
+        if (p < 0) {
+            N.ncp[k] = 0;
+            continue;
+        }
+        N.ncp[k] = O.ncp[p];
+        for (int j = 0; j < 4; j++) {
+            N.pn[4 * k + j] = O.pn[4 * p + j];
+            N.pl1[4 * k + j] = O.pl1[4 * p + j];
+            N.pl2[4 * k + j] = O.pl2[4 * p + j];
+        }
+        N.fv1[k] = O.fv1[p];
+        N.fv2[k] = O.fv2[p];
+        N.fimp[k] = O.fimp[p];
+    }
+}
+
 __global__ void fill_int_n(int* p, int n, const int* nptr, int v) {
     n = min(n, *nptr);
     RP3DG_GRID_STRIDE(i, n) p[i] = v;
@@ -646,6 +669,7 @@ int rp3dg_batch::upload(rp3dg_world* const* ws, int nw, const rp3dg_batch_option
     HIPCHECK(up(C.lq, lq.data(), nc * 16)); HIPCHECK(up(C.mat, mat.data(), nc * 8));
     HIPCHECK(up(C.filter, filter.data(), nc * 4)); HIPCHECK(up(C.rank, crank.data(), nc * 4));
     HIPCHECK(up(C.kind, ckind.data(), nc * 4));
+    coll_kind = ckind;
     HIPCHECK(up(C.world, cworld.data(), nc * 4));
     // hulls (at least one dummy record so pointers are valid)
     if (hh.empty()) { hh.push_back(HullHeader{}); hbounds.resize(2); }
@@ -896,7 +920,9 @@ int rp3dg_batch::upload(rp3dg_world* const* ws, int nw, const rp3dg_batch_option
 // layout reads the colour count back once, so only the fused layout can be captured).  Every
 // data-dependent size lives in device memory; kernels loop grid-stride over it, so the grids below
 // are performance hints only.
-int rp3dg_batch::enqueue_step(float dt, bool capturing) {
+// CollisionDetectionSystem::computeBroadPhase (CD.cpp:103-121): the pair table of the current state from
+// the previous one, for a step or a world query; E records the pairs it removes while colliding.
+int rp3dg_batch::enqueue_broadphase(bool capturing, const DevEvents& E, int pg) {
     DevPairs& O = cur_is_a ? Pa : Pb;
     DevPairs& N = cur_is_a ? Pb : Pa;
     int* old_count = pair_count + (cur_is_a ? 0 : 1);
@@ -905,9 +931,6 @@ int rp3dg_batch::enqueue_step(float dt, bool capturing) {
     const int* old_offs = bp_offs_ab[cur_is_a ? 0 : 1];   // segments of the previous table
     int* dstatus = &counters->status;
     int r;
-    fill_int<<<1, 256, 0, stream>>>((int*)counters, (int)(sizeof(DevCounters) / sizeof(int)), 0);
-
-    // ---------------- broadphase (CollisionDetectionSystem::computeBroadPhase, CD.cpp:103-121)
     uint64_t* k = bp_keys;
     int* vls = bp_vals;
     const size_t scan_lds = 2 * sizeof(float4) * max_group_colliders;
@@ -955,24 +978,35 @@ int rp3dg_batch::enqueue_step(float dt, bool capturing) {
                  bp_nmin, bp_nmax, bp_root, nocol_lists, bp_count, bp_offs, bp_out, bp_slab, pair_cap, 1, bp_walk,
                  &counters->nb_walk_again, dstatus);
     }
-    const int hp = capturing ? pair_cap : std::min(pair_cap, hint_pairs + hint_pairs / 4 + 1024);
-    const int pg = grid_for(hp);
-    DevEvents E{ev_buf, ev_cap, &counters->nb_events, events_on ? 1 : 0};
     if (events_on) fill_int_n<<<pg, 256, 0, stream>>>(O.kept, pair_cap, old_count, 0);
     LAUNCH_T("bp_merge", pg, 256, bp_merge, C, bp_out, bp_offs + nc, pair_cap, O, old_count, old_offs, N, new_count,
              &counters->nb_pairs);
     if (events_on) LAUNCH_T("ev_removed", pg, 256, ev_removed, C, O, old_count, E);
     cur_is_a = !cur_is_a;
+    return RP3DG_OK;
+}
 
-    // ---------------- narrowphase (CD.cpp:618-718)
+NpArgs rp3dg_batch::np_args(const DevPairs& P, const DevPairs& O, int query, int report, int body1, int body2) {
     NpArgs A;
-    A.B = B; A.C = C; A.P = N; A.O = O;
+    A.B = B; A.C = C; A.P = P; A.O = O;
     A.nb_pairs = &counters->nb_pairs;
     A.hulls = d_hulls; A.hull_verts = d_hverts; A.hull_normals = d_hnormals; A.hull_edges = d_hedges;
     A.hull_face_edge = d_hfedge; A.hull_face_verts = d_hfverts; A.hull_face_vidx = d_hfvidx;
     A.persistent_threshold = settings.persistent_threshold;
     A.cos_similar = settings.cos_similar;
-    A.status = dstatus;
+    A.status = &counters->status;
+    A.query = query;
+    A.qbody1 = body1;
+    A.qbody2 = body2;
+    A.report = report;
+    A.clip_prev = query == NP_STEP ? 1 : 0;           // testNarrowPhaseCollision(..., false) in snapshots
+    A.lf_update = report == NP_REPORT_NONE ? 0 : 1;   // overlap snapshots skip processAllPotentialContacts
+    return A;
+}
+
+// CollisionDetectionSystem::testNarrowPhaseCollision (CD.cpp:618-665): the pairs are binned by algorithm
+// and each bin is one launch.
+int rp3dg_batch::enqueue_narrowphase(const NpArgs& A, bool capturing, int pg) {
     LAUNCH_T("np_classify", std::min(pg, 1024), 256, np_classify, A, np_lists, pair_cap, counters->nb_tests,
              &counters->nb_box_tests, &counters->nb_box_sep_tests, &counters->nb_hull_tests);
     int* L = np_lists;
@@ -993,6 +1027,36 @@ int rp3dg_batch::enqueue_step(float dt, bool capturing) {
              &counters->nb_box_sep_tests);
     LAUNCH_T("np_poly_poly", hint(last.nb_hull_tests, 128), 128, np_poly_poly, A, L + ALGO_POLY_POLY * pair_cap,
              &counters->nb_hull_tests);
+
+    return RP3DG_OK;
+}
+
+int rp3dg_batch::enqueue_step(float dt, bool capturing) {
+    DevPairs& O = cur_is_a ? Pa : Pb;
+    DevPairs& N = cur_is_a ? Pb : Pa;
+    int* old_count = pair_count + (cur_is_a ? 0 : 1);
+    int* new_count = pair_count + (cur_is_a ? 1 : 0);
+    int* bp_offs = bp_offs_ab[cur_is_a ? 1 : 0];          // segments of the table written this step
+    const int* old_offs = bp_offs_ab[cur_is_a ? 0 : 1];   // segments of the previous table
+    int* dstatus = &counters->status;
+    int r;
+    fill_int<<<1, 256, 0, stream>>>((int*)counters, (int)(sizeof(DevCounters) / sizeof(int)), 0);
+
+    const int hp = capturing ? pair_cap : std::min(pair_cap, hint_pairs + hint_pairs / 4 + 1024);
+    const int pg = grid_for(hp);
+    DevEvents E{ev_buf, ev_cap, &counters->nb_events, events_on ? 1 : 0};
+    if (ev_pending) {   // pairs that a world query's broadphase removed while colliding report ContactExit now
+        ev_take_pending<<<1, 256, 0, stream>>>(ev_pend, ev_pend_count, ev_buf, ev_cap, &counters->nb_events);
+        ev_pending = false;
+    }
+    r = enqueue_broadphase(capturing, E, pg);
+    if (r) return r;
+    contacts_is_a = cur_is_a;
+
+    // ---------------- narrowphase (CD.cpp:618-718)
+    NpArgs A = np_args(N, O, NP_STEP, NP_REPORT_NO_TRIGGER, -1, -1);
+    r = enqueue_narrowphase(A, capturing, pg);
+    if (r) return r;
 
     // ---------------- contact flags and events (CD.cpp:975-1011, :1708-1741)
     LAUNCH_T("ev_update", pg, 256, ev_update, B, C, N, &counters->nb_pairs, E, settings.sleeping_enabled ? 0 : 1);
@@ -1230,6 +1294,7 @@ int rp3dg_batch::finish_steps(int n) {
         return RP3DG_EOVERFLOW;
     }
     hint_pairs = last.nb_pairs;
+    contacts_np = last.nb_pairs;
     hint_manifolds = last.nb_manifolds;
     int used = 0;
     while (used < kColorRoundsMax && last.color_left[used] != 0) used++;
@@ -1264,6 +1329,10 @@ int rp3dg_batch::step_graph(float dt, int n) {
     if (profiling) { set_error("kernel timing is eager-only: disable it before graph replay"); return RP3DG_ESTATE; }
     const int fr = flush_api();
     if (fr) return fr;
+    if (ev_pending) {   // the exits a world query recorded open the next step's events: one eager step
+        const int r0 = step(dt);
+        if (r0 || --n == 0) return r0;
+    }
     if (graph_exec[0] && graph_dt != dt) drop_graphs();
     if (!graph_exec[0]) {
         graph_rounds = std::min(kColorRoundsMax, std::max(8, hint_rounds + 4));
@@ -1293,7 +1362,63 @@ int rp3dg_batch::step_graph(float dt, int n) {
         HIPCHECK(hipGraphLaunch(graph_exec[cur_is_a ? 0 : 1], stream));
         cur_is_a = !cur_is_a;
     }
+    contacts_is_a = cur_is_a;
     return finish_steps(n);
+}
+
+// A world query (PhysicsWorld::testOverlap / testCollision, CD.cpp:1775-1896): the broadphase of the
+// current state, as the reference's computeBroadPhase runs it (the pair table changes and the moved set
+// empties for the next step too), then the narrowphase of the pairs the query selects, its contacts kept
+// apart in the q_* arrays: q_ncp[k] > 0 points, -1 colliding without points, 0 not colliding.
+int rp3dg_batch::query(int mode, int report, int body1, int body2) {
+    if (nb == 0) return RP3DG_OK;
+    int r = flush_api();
+    if (r) return r;
+    if (!q_ncp) {
+        q_ncp = dalloc<int>(this, pair_cap);
+        q_pn = dalloc<float4>(this, 4 * (size_t)pair_cap);
+        q_pl1 = dalloc<float4>(this, 4 * (size_t)pair_cap);
+        q_pl2 = dalloc<float4>(this, 4 * (size_t)pair_cap);
+        q_fv1 = dalloc<float4>(this, pair_cap);
+        q_fv2 = dalloc<float4>(this, pair_cap);
+        q_fimp = dalloc<float4>(this, pair_cap);
+        if (!q_fimp) { set_error("device allocation failed (query contacts)"); return RP3DG_EHIP; }
+    }
+    if (events_on && !ev_pend) {
+        ev_pend = dalloc<int4>(this, (size_t)ev_cap);
+        ev_pend_count = dalloc<int>(this, 1);
+        if (!ev_pend_count) { set_error("device allocation failed (query events)"); return RP3DG_EHIP; }
+        HIPCHECK(hipMemset(ev_pend_count, 0, sizeof(int)));
+    }
+    fill_int<<<1, 256, 0, stream>>>((int*)counters, (int)(sizeof(DevCounters) / sizeof(int)), 0);
+    const int pg = grid_for(std::min(pair_cap, hint_pairs + hint_pairs / 4 + 1024));
+    DevEvents E{ev_pend, ev_cap, ev_pend_count, events_on ? 1 : 0};
+    r = enqueue_broadphase(false, E, pg);
+    if (r) return r;
+    if (events_on) ev_pending = true;
+    fill_int<<<grid_for(nc), 256, 0, stream>>>(C.moved, nc, 0);   // BroadPhaseSystem: mMovedShapes.clear()
+    DevPairs& N = cur_is_a ? Pa : Pb;
+    DevPairs& O = cur_is_a ? Pb : Pa;
+    DevPairs Q = N;   // the new table, with the query's contacts written apart
+    Q.ncp = q_ncp; Q.pn = q_pn; Q.pl1 = q_pl1; Q.pl2 = q_pl2; Q.fv1 = q_fv1; Q.fv2 = q_fv2; Q.fimp = q_fimp;
+    r = enqueue_narrowphase(np_args(Q, O, mode, report, body1, body2), false, pg);
+    if (r) return r;
+    qr_carry<<<pg, 256, 0, stream>>>(N, O, &counters->nb_pairs);
+    HIPCHECK(hipGetLastError());
+    DevCounters c{};
+    HIPCHECK(hipMemcpyAsync(&c, counters, sizeof(DevCounters), hipMemcpyDeviceToHost, stream));
+    HIPCHECK(hipStreamSynchronize(stream));
+    if (c.status) {
+        std::string what;
+        if (c.status & STATUS_PAIR_OVERFLOW) what += " pair table overflow (raise pairs_per_collider);";
+        if (c.status & STATUS_STACK_OVERFLOW) what += " broadphase traversal stack overflow;";
+        if (c.status & STATUS_CLIP_OVERFLOW) what += " face clipping exceeded its buffer;";
+        set_error("world query: device status " + std::to_string(c.status) + ":" + what);
+        return RP3DG_EOVERFLOW;
+    }
+    last.nb_pairs = c.nb_pairs;
+    hint_pairs = std::max(hint_pairs, c.nb_pairs);
+    return RP3DG_OK;
 }
 
 // Applies the queued RigidBody calls on the device, in call order (api.hip), on the batch's stream.
@@ -1592,6 +1717,180 @@ int rp3dg_batch_raycast(rp3dg_batch* b, int64_t n, const int32_t* world, const f
     return RP3DG_OK;
 }
 
+// PhysicsWorld::raycast with a RaycastCallback whose notifyRaycastHit returns 1 (every hit, the ray is
+// not clipped): the hits of all rays, sorted by (ray, collider).
+int64_t rp3dg_batch_raycast_all(rp3dg_batch* b, int64_t n, const int32_t* world, const float* point1,
+                                const float* point2, const float* max_fraction, uint16_t category_mask, int32_t* ray,
+                                int32_t* collider, float* point, float* normal, float* fraction, int64_t capacity) {
+    if (!b || n < 0 || !point1 || !point2 || capacity < 0) { set_error("raycast_all: invalid arguments"); return RP3DG_EINVAL; }
+    if (n == 0) return 0;
+    if (n > (1 << 26)) { set_error("at most 2^26 rays per call"); return RP3DG_EINVAL; }
+    if (const int fr = b->flush_api()) return fr;
+    std::vector<int> wv((size_t)n);
+    std::vector<float4> a((size_t)n), c((size_t)n);
+    for (int64_t i = 0; i < n; i++) {
+        const int w = world ? world[i] : 0;
+        if (w < 0 || w >= b->nworlds) { set_error("raycast: invalid world index"); return RP3DG_EINVAL; }
+        wv[i] = w;
+        a[i] = make_float4(point1[3 * i], point1[3 * i + 1], point1[3 * i + 2], 0.f);
+        c[i] = make_float4(point2[3 * i], point2[3 * i + 1], point2[3 * i + 2], max_fraction ? max_fraction[i] : 1.0f);
+    }
+    // staging: rays (2 float4 + world) and up to hcap hits (int2 + 2 float4) + the hit count
+    const int64_t hcap = std::min<int64_t>(std::max<int64_t>(capacity, 1), 1 << 26);
+    const size_t need = (size_t)n * (2 * sizeof(float4) + sizeof(int)) + (size_t)hcap * (sizeof(int2) + 2 * sizeof(float4)) + 64;
+    if (need > b->ray_cap) {
+        if (b->ray_buf) HIPCHECK(hipFree(b->ray_buf));
+        b->ray_buf = nullptr;
+        b->ray_cap = 0;
+        HIPCHECK(hipMalloc(&b->ray_buf, need));
+        b->ray_cap = need;
+    }
+    float4* dp1 = (float4*)b->ray_buf;
+    float4* dp2 = dp1 + n;
+    float4* dhp = dp2 + n;
+    float4* dhn = dhp + hcap;
+    int2* drc = (int2*)(dhn + hcap);
+    int* dw = (int*)(drc + hcap);
+    int* dcount = dw + n;
+    HIPCHECK(hipMemcpyAsync(dp1, a.data(), sizeof(float4) * n, hipMemcpyHostToDevice, b->stream));
+    HIPCHECK(hipMemcpyAsync(dp2, c.data(), sizeof(float4) * n, hipMemcpyHostToDevice, b->stream));
+    HIPCHECK(hipMemcpyAsync(dw, wv.data(), sizeof(int) * n, hipMemcpyHostToDevice, b->stream));
+    fill_int<<<1, 64, 0, b->stream>>>(dcount, 1, 0);
+    RayArgs A{b->C, b->d_hulls, b->d_hverts, b->d_hnormals, b->d_hfverts, b->d_hfvidx, b->d_wcoll};
+    rc_all<<<grid_for((int)n), 256, 0, b->stream>>>(A, dw, dp1, dp2, (int)n, category_mask, drc, dhp, dhn, (int)hcap,
+                                                     dcount);
+    HIPCHECK(hipGetLastError());
+    int total = 0;
+    HIPCHECK(hipMemcpyAsync(&total, dcount, sizeof(int), hipMemcpyDeviceToHost, b->stream));
+    HIPCHECK(hipStreamSynchronize(b->stream));
+    if (total > capacity) {
+        set_error("raycast_all: " + std::to_string(total) + " hits, capacity " + std::to_string(capacity));
+        return capacity == 0 && !ray && !collider ? total : RP3DG_EINVAL;
+    }
+    std::vector<int2> rc((size_t)total);
+    std::vector<float4> hp((size_t)total), hn((size_t)total);
+    if (total) {
+        HIPCHECK(hipMemcpy(rc.data(), drc, sizeof(int2) * total, hipMemcpyDeviceToHost));
+        HIPCHECK(hipMemcpy(hp.data(), dhp, sizeof(float4) * total, hipMemcpyDeviceToHost));
+        HIPCHECK(hipMemcpy(hn.data(), dhn, sizeof(float4) * total, hipMemcpyDeviceToHost));
+    }
+    std::vector<int> order((size_t)total);
+    for (int i = 0; i < total; i++) order[i] = i;
+    std::sort(order.begin(), order.end(), [&](int x, int y) {
+        return rc[x].x != rc[y].x ? rc[x].x < rc[y].x : rc[x].y < rc[y].y;
+    });
+    for (int k = 0; k < total; k++) {
+        const int i = order[k];
+        if (ray) ray[k] = rc[i].x;
+        if (collider) collider[k] = rc[i].y;
+        if (point) { point[3 * k] = hp[i].x; point[3 * k + 1] = hp[i].y; point[3 * k + 2] = hp[i].z; }
+        if (normal) { normal[3 * k] = hn[i].x; normal[3 * k + 1] = hn[i].y; normal[3 * k + 2] = hn[i].z; }
+        if (fraction) fraction[k] = hp[i].w;
+    }
+    return total;
+}
+
+// Results of the last world query: the colliding pairs (q_ncp != 0) of the current table with their
+// contact points.
+static int query_results(rp3dg_batch* b, std::vector<int2>& col, std::vector<int>& ncp, std::vector<int>& hits) {
+    const DevPairs& P = b->cur_is_a ? b->Pa : b->Pb;
+    const int np = b->last.nb_pairs;
+    col.resize(std::max(np, 1));
+    ncp.resize(std::max(np, 1));
+    if (np) {
+        HIPCHECK(hipMemcpy(col.data(), P.col, sizeof(int2) * np, hipMemcpyDeviceToHost));
+        HIPCHECK(hipMemcpy(ncp.data(), b->q_ncp, sizeof(int) * np, hipMemcpyDeviceToHost));
+    }
+    hits.clear();
+    for (int k = 0; k < np; k++) {
+        if (ncp[k] == 0) continue;
+        // CollisionCallback::CallbackData keeps the pairs without a trigger (CollisionCallback.cpp:60-70)
+        if (b->query_points && ((b->coll_kind[col[k].x] | b->coll_kind[col[k].y]) & CK_TRIGGER)) continue;
+        hits.push_back(k);
+    }
+    return RP3DG_OK;
+}
+
+static bool valid_query_body(rp3dg_batch* b, int32_t body, bool allow_none) {
+    if ((allow_none && body < 0) || (body >= 0 && body < b->nb)) return true;
+    set_error("world query: invalid body index " + std::to_string(body));
+    return false;
+}
+
+int rp3dg_batch_test_overlap_bodies(rp3dg_batch* b, int32_t body1, int32_t body2) {
+    if (!b) return RP3DG_EINVAL;
+    if (!valid_query_body(b, body1, false) || !valid_query_body(b, body2, false)) return RP3DG_EINVAL;
+    int r = b->query(NP_QUERY_BODIES, NP_REPORT_NONE, body1, body2);
+    if (r) return r;
+    b->query_points = false;
+    std::vector<int2> col;
+    std::vector<int> ncp, hits;
+    if ((r = query_results(b, col, ncp, hits))) return r;
+    return hits.empty() ? 0 : 1;
+}
+
+int64_t rp3dg_batch_test_overlap(rp3dg_batch* b, int32_t body) {
+    if (!b) return RP3DG_EINVAL;
+    if (!valid_query_body(b, body, true)) return RP3DG_EINVAL;
+    int r = body < 0 ? b->query(NP_QUERY_ALL, NP_REPORT_NONE, -1, -1) : b->query(NP_QUERY_BODIES, NP_REPORT_NONE, body, -1);
+    if (r) return r;
+    b->query_points = false;
+    std::vector<int2> col;
+    std::vector<int> ncp, hits;
+    if ((r = query_results(b, col, ncp, hits))) return r;
+    return (int64_t)hits.size();
+}
+
+int64_t rp3dg_batch_test_collision(rp3dg_batch* b, int32_t body1, int32_t body2) {
+    if (!b) return RP3DG_EINVAL;
+    if (!valid_query_body(b, body1, true) || !valid_query_body(b, body2, true)) return RP3DG_EINVAL;
+    if (body1 < 0 && body2 >= 0) { set_error("test_collision: give body1 to filter by body2"); return RP3DG_EINVAL; }
+    int r = body1 < 0 ? b->query(NP_QUERY_ALL, NP_REPORT_NO_TRIGGER, -1, -1)
+                      : b->query(NP_QUERY_BODIES, NP_REPORT_ALL, body1, body2);
+    if (r) return r;
+    b->query_points = true;
+    std::vector<int2> col;
+    std::vector<int> ncp, hits;
+    if ((r = query_results(b, col, ncp, hits))) return r;
+    return (int64_t)hits.size();
+}
+
+int64_t rp3dg_batch_get_query_results(rp3dg_batch* b, rp3dg_contact_view* o, int64_t capacity) {
+    if (!b || !o) return RP3DG_EINVAL;
+    if (!b->q_ncp) { set_error("no world query has run on this batch"); return RP3DG_ESTATE; }
+    std::vector<int2> col;
+    std::vector<int> ncp, hits;
+    int r = query_results(b, col, ncp, hits);
+    if (r) return r;
+    const int64_t n = (int64_t)hits.size();
+    if (capacity < n) { set_error("query results: capacity below the pair count"); return RP3DG_EINVAL; }
+    const int np = b->last.nb_pairs;
+    std::vector<float4> pn(4 * (size_t)std::max(np, 1)), pl1(pn.size()), pl2(pn.size());
+    if (np && b->query_points) {
+        HIPCHECK(hipMemcpy(pn.data(), b->q_pn, sizeof(float4) * 4 * np, hipMemcpyDeviceToHost));
+        HIPCHECK(hipMemcpy(pl1.data(), b->q_pl1, sizeof(float4) * 4 * np, hipMemcpyDeviceToHost));
+        HIPCHECK(hipMemcpy(pl2.data(), b->q_pl2, sizeof(float4) * 4 * np, hipMemcpyDeviceToHost));
+    }
+    for (int64_t m = 0; m < n; m++) {
+        const int k = hits[m];
+        const int cnt = b->query_points ? std::max(ncp[k], 0) : 0;
+        if (o->colliders) { o->colliders[2 * m] = col[k].x; o->colliders[2 * m + 1] = col[k].y; }
+        if (o->nb_points) o->nb_points[m] = cnt;
+        if (o->color) o->color[m] = 0;
+        for (int p = 0; p < 4; p++) {
+            const bool in = p < cnt;
+            const float4 z = make_float4(0.f, 0.f, 0.f, 0.f);
+            const float4 a = in ? pn[4 * k + p] : z, c = in ? pl1[4 * k + p] : z, d = in ? pl2[4 * k + p] : z;
+            if (o->normal_depth) { float* q = o->normal_depth + 16 * m + 4 * p; q[0] = a.x; q[1] = a.y; q[2] = a.z; q[3] = a.w; }
+            if (o->local1) { float* q = o->local1 + 12 * m + 3 * p; q[0] = c.x; q[1] = c.y; q[2] = c.z; }
+            if (o->local2) { float* q = o->local2 + 12 * m + 3 * p; q[0] = d.x; q[1] = d.y; q[2] = d.z; }
+            if (o->impulse) o->impulse[4 * m + p] = 0.f;
+        }
+        if (o->friction) { o->friction[3 * m] = 0.f; o->friction[3 * m + 1] = 0.f; o->friction[3 * m + 2] = 0.f; }
+    }
+    return n;
+}
+
 int rp3dg_batch_get_sleeping(rp3dg_batch* b, int32_t* sleeping, int64_t capacity) {
     if (!b || !sleeping || capacity < b->nb) { set_error("sleeping: capacity below the body count"); return RP3DG_EINVAL; }
     if (const int fr = b->flush_api()) return fr;
@@ -1791,13 +2090,13 @@ int rp3dg_batch_get_contacts(rp3dg_batch* b, rp3dg_contact_view* o, int64_t cap)
     const int nm = b->last.nb_manifolds;
     if (cap < nm) { set_error("capacity too small"); return RP3DG_EINVAL; }
     if (nm == 0) return 0;
-    const DevPairs& P = b->cur_is_a ? b->Pa : b->Pb;
+    const DevPairs& P = b->contacts_is_a ? b->Pa : b->Pb;   // a world query since the step moved to the other table
     HIPCHECK(hipStreamSynchronize(b->stream));
     std::vector<int> order(nm), mlist(nm), pairs(nm), colors(nm);
     std::vector<uint64_t> ck(nm);
     HIPCHECK(hipMemcpy(pairs.data(), b->S.pair, sizeof(int) * nm, hipMemcpyDeviceToHost));
     HIPCHECK(hipMemcpy(ck.data(), b->sorted_keys, sizeof(uint64_t) * nm, hipMemcpyDeviceToHost));
-    const int np = b->last.nb_pairs;
+    const int np = b->contacts_np;
     std::vector<int2> col(np);
     std::vector<int> ncp(np);
     std::vector<float4> pn(4 * (size_t)np), pl1(4 * (size_t)np), pl2(4 * (size_t)np), fimp(np);

@@ -18,5 +18,7 @@ struct DevEvents {
 __global__ void ev_removed(DevColliders C, DevPairs O, const int* nold, DevEvents E);
 __global__ void ev_update(DevBodies B, DevColliders C, DevPairs N, const int* nb_pairs, DevEvents E, int shift);
 __global__ void ev_shift(DevBodies B, DevColliders C, DevPairs N, const int* nb_pairs);
+// the exits recorded by world queries since the last step open this step's events
+__global__ void ev_take_pending(const int4* pend, int* pend_count, int4* ev, int cap, int* count);
 
 }  // namespace rp3dg

@@ -49,6 +49,19 @@ __global__ void ev_removed(DevColliders C, DevPairs O, const int* nold, DevEvent
     }
 }
 
+// A world query's broadphase removes pairs like a step's does; the reference keeps them in its lost contact
+// pairs until the next step reports them (CD.cpp:158-170, 1723-1725).  Run right after the step clears
+// its counters, in one workgroup.
+__global__ void ev_take_pending(const int4* pend, int* pend_count, int4* ev, int cap, int* count) {
+    const int n = min(*pend_count, cap);
+    for (int i = threadIdx.x; i < n; i += blockDim.x) ev[i] = pend[i];
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        *count = *pend_count;
+        *pend_count = 0;
+    }
+}
+
 // after the narrowphase, before the islands: the tested pairs (a body enabled) get EV_CUR; colliding
 // pairs report Start / Stay, tested pairs that stopped colliding report Exit (CD.cpp:975-996)
 // shift = 1 (no sleeping: the enabled pairs cannot change before the islands) also does ev_shift's work

@@ -21,7 +21,15 @@ struct NpArgs {
     float persistent_threshold;
     float cos_similar;
     int* status;               // device status word (STATUS_*)
+    // world queries (PhysicsWorld::testOverlap / testCollision, CD.cpp:1775-1896) run the same kernels:
+    int query;                 // NP_STEP, NP_QUERY_ALL (both colliders world-query colliders) or NP_QUERY_BODIES
+    int qbody1, qbody2;        // NP_QUERY_BODIES: pairs with body qbody1 (and qbody2 when >= 0)
+    int report;                // NP_REPORT_NO_TRIGGER, NP_REPORT_ALL or NP_REPORT_NONE (reportContacts)
+    int clip_prev;             // SATAlgorithm's clipWithPreviousAxisIfStillColliding (1 in steps, 0 in queries)
+    int lf_update;             // processPotentialContacts updates wasColliding / isValid (0: overlap queries)
 };
+constexpr int NP_STEP = 0, NP_QUERY_ALL = 1, NP_QUERY_BODIES = 2;
+constexpr int NP_REPORT_NO_TRIGGER = 0, NP_REPORT_ALL = 1, NP_REPORT_NONE = 2;
 
 // Narrowphase test lists: bins 1..5 closed forms, ALGO_POLY_POLY pairs with a convex mesh,
 // ALGO_COUNT box-box pairs colliding last step, ALGO_COUNT + 1 box-box pairs separated last step.

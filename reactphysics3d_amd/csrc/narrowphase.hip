@@ -623,14 +623,22 @@ __device__ __forceinline__ bool trigger_pair(const NpArgs& A, int slot) {
     return ((A.C.kind[c.x] | A.C.kind[c.y]) & CK_TRIGGER) != 0;
 }
 
+// NarrowPhaseInfo::reportContacts of a tested pair: steps and testCollision(callback) leave trigger pairs
+// without contacts (CD.cpp:402), testCollision(body[, body]) reports every pair (CD.cpp:462-511),
+// testOverlap reports none.
+__device__ __forceinline__ bool pair_reports(const NpArgs& A, int slot) {
+    return A.report == NP_REPORT_ALL || (A.report == NP_REPORT_NO_TRIGGER && !trigger_pair(A, slot));
+}
+
 template <class R>
 __device__ void contact_epilogue(const NpArgs& A, int slot, bool colliding, const R& raw, int collider1) {
     DevPairs P = A.P;
     const DevPairs& O = A.O;
     const int prev = P.prev[slot];   // this pair's entry in the previous table, -1 when new
     if (raw.clip_full) atomicOr(A.status, STATUS_CLIP_OVERFLOW);
-    if (!colliding || raw.n == 0 || trigger_pair(A, slot)) {
-        P.ncp[slot] = 0;
+    if (!colliding || raw.n == 0 || !pair_reports(A, slot)) {
+        // a query marks a colliding pair without points with -1 (its overlap is reported)
+        P.ncp[slot] = (A.query != NP_STEP && colliding) ? -1 : 0;
         return;
     }
     const int oldN = prev >= 0 ? O.ncp[prev] : 0;
@@ -734,10 +742,21 @@ __device__ inline uint32_t lf_after_test(uint32_t lf, bool colliding) {
 __device__ __forceinline__ int np_bin(const NpArgs& A, int k, bool count_pass) {
     const int2 c = A.P.col[k];
     const int f1 = A.B.flags[A.C.body[c.x]], f2 = A.B.flags[A.C.body[c.y]];
-    // tested only if a body is enabled and both colliders are simulation colliders or triggers (CD.cpp:386-400)
+    // tested only if a body is enabled and both colliders are simulation colliders or triggers (CD.cpp:386-400);
+    // a world query tests the pairs of world-query colliders (:386), a query about bodies the pairs of those
+    // bodies (filterOverlappingPairs, :1898-1940), among the enabled pairs in both cases
     const int k1 = A.C.kind[c.x], k2 = A.C.kind[c.y];
-    if (!((f1 & BF_ENABLED) || (f2 & BF_ENABLED)) || !(k1 & (CK_SIMULATION | CK_TRIGGER)) ||
-        !(k2 & (CK_SIMULATION | CK_TRIGGER))) {
+    bool tested = (f1 & BF_ENABLED) || (f2 & BF_ENABLED);
+    if (A.query == NP_STEP) {
+        tested = tested && (k1 & (CK_SIMULATION | CK_TRIGGER)) && (k2 & (CK_SIMULATION | CK_TRIGGER));
+    } else if (A.query == NP_QUERY_ALL) {
+        tested = tested && (k1 & CK_WORLD_QUERY) && (k2 & CK_WORLD_QUERY);
+    } else {
+        const int b1 = A.C.body[c.x], b2 = A.C.body[c.y];
+        tested = tested && (A.qbody2 < 0 ? (b1 == A.qbody1 || b2 == A.qbody1)
+                                         : ((b1 == A.qbody1 && b2 == A.qbody2) || (b1 == A.qbody2 && b2 == A.qbody1)));
+    }
+    if (!tested) {
         if (count_pass) A.P.ncp[k] = 0;
         return 0;
     }
@@ -806,7 +825,7 @@ __global__ void np_sphere_sphere(NpArgs A, const int* list, const int* count) {
         const Tf T2 = tf(ld3(A.C.wpos, c.y), ldq(A.C.wq, c.y));
         Raw raw; raw.n = 0;
         const bool hit = sphere_sphere(A.C.shape[c.x].x, A.C.shape[c.y].x, T1, T2, raw);
-        A.P.lf[k] = lf_after_test(A.P.lf[k], hit);
+        A.P.lf[k] = A.lf_update ? lf_after_test(A.P.lf[k], hit) : A.P.lf[k];
         contact_epilogue(A, k, hit, raw, c.x);
     }
 }
@@ -822,7 +841,7 @@ __global__ void np_sphere_capsule(NpArgs A, const int* list, const int* count) {
         const float4 cap = A.C.shape[cc];
         Raw raw; raw.n = 0;
         const bool hit = sphere_capsule(s1, A.C.shape[cs].x, cap.x, cap.y + cap.y, Ts, Tc, raw);
-        A.P.lf[k] = lf_after_test(A.P.lf[k], hit);
+        A.P.lf[k] = A.lf_update ? lf_after_test(A.P.lf[k], hit) : A.P.lf[k];
         contact_epilogue(A, k, hit, raw, c.x);
     }
 }
@@ -836,7 +855,7 @@ __global__ void np_capsule_capsule(NpArgs A, const int* list, const int* count) 
         const float4 s1 = A.C.shape[c.x], s2 = A.C.shape[c.y];
         Raw raw; raw.n = 0;
         const bool hit = capsule_capsule(s1.x, s1.y + s1.y, s2.x, s2.y + s2.y, T1, T2, raw);
-        A.P.lf[k] = lf_after_test(A.P.lf[k], hit);
+        A.P.lf[k] = A.lf_update ? lf_after_test(A.P.lf[k], hit) : A.P.lf[k];
         contact_epilogue(A, k, hit, raw, c.x);
     }
 }
@@ -858,15 +877,17 @@ __device__ __forceinline__ Poly box_poly(const NpArgs& A, int c) {
 template <bool BOX, bool FACES, class R>
 __device__ __forceinline__ int sat_begin(const Poly& P1, const Poly& P2, const Tf& T1, const Tf& T2, const Tf& t12,
                                          const Tf& t21, uint32_t& lf, R& raw, float& minPen0, int& minFace,
-                                         bool& isMinFace1) {
+                                         bool& isMinFace1, bool clip_prev) {
     const bool wasColliding = (lf & LF_WAS_COLLIDING) != 0;
+    // clipWithPreviousAxisIfStillColliding (SATAlgorithm.cpp:538, 578, 636): off in world queries
+    const bool clipAgain = wasColliding && clip_prev;
     if ((lf & LF_VALID) && (lf & LF_WAS_SAT)) {
         if (lf & (LF_SAT_FACE1 | LF_SAT_FACE2)) {
             const bool ref1 = (lf & LF_SAT_FACE1) != 0;
             const int f = (int)lf_face(lf);
             const float pen = ref1 ? sat_single_face(P1, P2, t12, f) : sat_single_face(P2, P1, t21, f);
             if (!wasColliding && pen <= 0.0f) return 0;
-            if (wasColliding && pen > 0.0f) {
+            if (clipAgain && pen > 0.0f) {
                 if (face_contacts<BOX>(ref1, P1, P2, t12, t21, f, T1, T2, raw)) {
                     lf = lf_set_face(lf, ref1, f);
                     return 1;
@@ -886,7 +907,7 @@ __device__ __forceinline__ int sat_begin(const Poly& P1, const Poly& P2, const T
                 const V3 c1 = tapply(t12, P1.centroid());
                 const float pen = sat_edge_distance(e1a, e2a, c1, e1d, e2d, axis);
                 if (!wasColliding && pen <= 0.0f) return 0;
-                if (wasColliding && pen > 0.0f && pen < kDecimalLargest) {
+                if (clipAgain && pen > 0.0f && pen < kDecimalLargest) {
                     V3 cp1, cp2;
                     closest_points_between_segments(e1a, e1b, e2a, e2b, cp1, cp2);
                     const V3 vec1 = cp1 - e2a;
@@ -1068,7 +1089,7 @@ __device__ __forceinline__ void sat_pair_list(const NpArgs& A, const int* list, 
             lf = A.P.lf[k];
             t12 = tmul(tinv(T2), T1);
             t21 = tinv(t12);
-            stage = sat_begin<BOX, BOX>(P1, P2, T1, T2, t12, t21, lf, raw, minPen0, minFace, isMinFace1);
+            stage = sat_begin<BOX, BOX>(P1, P2, T1, T2, t12, t21, lf, raw, minPen0, minFace, isMinFace1, A.clip_prev != 0);
         }
         if constexpr (!BOX) {
             // mesh pairs share the two face searches as well (each face's support point walks every
@@ -1138,7 +1159,7 @@ __device__ __forceinline__ void sat_pair_list(const NpArgs& A, const int* list, 
             // ConvexPolyhedronVsConvexPolyhedronAlgorithm.cpp:55-62
             lf |= LF_WAS_SAT;
             lf &= ~LF_WAS_GJK;
-            A.P.lf[k] = lf_after_test(lf, hit);
+            A.P.lf[k] = A.lf_update ? lf_after_test(lf, hit) : lf;
             contact_epilogue(A, k, hit, raw, c.x);
         }
     }
@@ -1649,7 +1670,7 @@ __global__ void np_sphere_poly(NpArgs A, const int* list, const int* count) {
             lf |= LF_WAS_SAT;
         }
         A.P.lfaxis[k] = axis;
-        A.P.lf[k] = lf_after_test(lf, hit);
+        A.P.lf[k] = A.lf_update ? lf_after_test(lf, hit) : lf;
         contact_epilogue(A, k, hit, raw, c.x);
     }
 }
@@ -1674,7 +1695,7 @@ __global__ void np_capsule_poly(NpArgs A, const int* list, const int* count) {
         const Conv& Sp = cap1 ? S2 : S1;
         const Tf& Tc = cap1 ? T1 : T2;
         const Tf& Tp = cap1 ? T2 : T1;
-        const bool report = !trigger_pair(A, k);
+        const bool report = pair_reports(A, k);
         if (g == GJK_COLLIDE_IN_MARGIN && !report) {
             hit = true;   // the face-contact search only adds points (CapsuleVsConvexPolyhedronAlgorithm.cpp:83)
         } else if (g == GJK_COLLIDE_IN_MARGIN) {
@@ -1707,7 +1728,7 @@ __global__ void np_capsule_poly(NpArgs A, const int* list, const int* count) {
             lf |= LF_WAS_SAT;
         }
         A.P.lfaxis[k] = axis;
-        A.P.lf[k] = lf_after_test(lf, hit);
+        A.P.lf[k] = A.lf_update ? lf_after_test(lf, hit) : lf;
         contact_epilogue(A, k, hit, raw, c.x);
     }
 }

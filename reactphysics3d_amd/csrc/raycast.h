@@ -19,4 +19,8 @@ struct RayArgs {
 __global__ void rc_closest(RayArgs A, const int* ray_world, const float4* p1s, const float4* p2s, int n, uint32_t mask,
                            int* hit_collider, float4* hit_point, float4* hit_normal);
 
+// every hit: (ray, collider), point (w = fraction) and normal per hit, at most cap written, *count = all
+__global__ void rc_all(RayArgs A, const int* ray_world, const float4* p1s, const float4* p2s, int n, uint32_t mask,
+                       int2* hit_ray_collider, float4* hit_point, float4* hit_normal, int cap, int* count);
+
 }  // namespace rp3dg

@@ -1,7 +1,10 @@
 // World raycasts (PhysicsWorld::raycast -> CollisionDetectionSystem::raycast -> BroadPhaseSystem::raycast
 // -> DynamicAABBTree::raycast -> Collider::raycast -> <Shape>::raycast), batched: one thread per ray,
 // with the callback that keeps the closest hit (RaycastCallback::notifyRaycastHit returning the hit
-// fraction, which clips the ray).
+// fraction, which clips the ray), or the one that keeps every hit (notifyRaycastHit returning 1: the
+// ray is never clipped, so the hits are the colliders whose shape the ray crosses within its own max
+// fraction, a set that does not depend on the visiting order).  Only world-query colliders whose
+// category passes the mask are tested (BroadPhaseRaycastCallback, BroadPhaseSystem.cpp:224-241).
 //
 // A thread walks the colliders of its ray's world in index order instead of the tree: every collider
 // whose fat AABB the (clipped) ray crosses (AABB::testRayIntersect, AABB.h:285-313) and whose category
@@ -177,7 +180,55 @@ __device__ __forceinline__ bool ray_aabb(V3 mn, V3 mx, V3 o, V3 inv, float maxf)
     return tmax >= fmax_s(tmin, 0.0f);
 }
 
+// One collider against a ray (BroadPhaseRaycastCallback::raycastBroadPhaseShape + Collider::raycast,
+// Collider.cpp:173-193): the ray in the collider's space, the hit back in world space.
+__device__ bool ray_collider(const RayArgs& A, int c, uint32_t mask, V3 p1, V3 p2, V3 inv, float maxf, Hit& hw) {
+    if ((A.C.filter[c] & mask & 0xffffu) == 0u || !(A.C.kind[c] & CK_WORLD_QUERY)) return false;
+    if (!ray_aabb(ld3(A.C.amin, c), ld3(A.C.amax, c), p1, inv, maxf)) return false;
+    const Tf l2w = tf(ld3(A.C.wpos, c), ldq(A.C.wq, c));
+    const Tf w2l = tinv(l2w);
+    const V3 lp1 = tapply(w2l, p1), lp2 = tapply(w2l, p2);
+    const float4 s = A.C.shape[c];
+    Hit h;
+    bool hit;
+    switch (A.C.type[c]) {
+        case SHAPE_SPHERE: hit = ray_sphere(lp1, lp2, v3(0.f, 0.f, 0.f), s.x, maxf, h); break;
+        case SHAPE_BOX: hit = ray_box(lp1, lp2, v3(s.x, s.y, s.z), maxf, h); break;
+        case SHAPE_CAPSULE: hit = ray_capsule(lp1, lp2, s.x, s.y, maxf, h); break;
+        default: hit = ray_hull(A, (int)s.x, lp1, lp2, maxf, h); break;
+    }
+    if (!hit) return false;
+    hw.t = h.t;
+    hw.p = tapply(l2w, h.p);
+    hw.n = normalized(qrot(l2w.q, h.n));
+    return true;
+}
+
 }  // namespace
+
+// Every hit of every ray: hits are appended at an atomically reserved slot (those past cap are counted,
+// not written); *count ends as the number of hits.
+__global__ void rc_all(RayArgs A, const int* ray_world, const float4* p1s, const float4* p2s, int n, uint32_t mask,
+                       int2* hit_ray_collider, float4* hit_point, float4* hit_normal, int cap, int* count) {
+    RP3DG_GRID_STRIDE(i, n) {
+        const float4 a = p1s[i], b = p2s[i];
+        const V3 p1 = v3(a.x, a.y, a.z), p2 = v3(b.x, b.y, b.z);
+        const float maxf = b.w;
+        const V3 dir = p2 - p1;
+        const V3 inv = v3(1.0f / dir.x, 1.0f / dir.y, 1.0f / dir.z);
+        const int w = ray_world[i];
+        for (int c = A.wcoll[w]; c < A.wcoll[w + 1]; c++) {
+            Hit h;
+            if (!ray_collider(A, c, mask, p1, p2, inv, maxf, h)) continue;
+            const int slot = atomicAdd(count, 1);
+            if (slot < cap) {
+                hit_ray_collider[slot] = make_int2(i, c);
+                hit_point[slot] = make_float4(h.p.x, h.p.y, h.p.z, h.t);
+                hit_normal[slot] = make_float4(h.n.x, h.n.y, h.n.z, 0.f);
+            }
+        }
+    }
+}
 
 __global__ void rc_closest(RayArgs A, const int* ray_world, const float4* p1s, const float4* p2s, int n, uint32_t mask,
                            int* hit_collider, float4* hit_point, float4* hit_normal) {
@@ -192,25 +243,9 @@ __global__ void rc_closest(RayArgs A, const int* ray_world, const float4* p1s, c
         int best = -1;
         Hit hb{0.f, v3(0.f, 0.f, 0.f), v3(0.f, 0.f, 0.f)};
         for (int c = c0; c < c1; c++) {
-            if ((A.C.filter[c] & mask & 0xffffu) == 0u) continue;
-            if (!ray_aabb(ld3(A.C.amin, c), ld3(A.C.amax, c), p1, inv, maxf)) continue;
-            // Collider::raycast (Collider.cpp:173-193): the ray in the collider's space, the hit back
-            const Tf l2w = tf(ld3(A.C.wpos, c), ldq(A.C.wq, c));
-            const Tf w2l = tinv(l2w);
-            const V3 lp1 = tapply(w2l, p1), lp2 = tapply(w2l, p2);
-            const float4 s = A.C.shape[c];
             Hit h;
-            bool hit;
-            switch (A.C.type[c]) {
-                case SHAPE_SPHERE: hit = ray_sphere(lp1, lp2, v3(0.f, 0.f, 0.f), s.x, maxf, h); break;
-                case SHAPE_BOX: hit = ray_box(lp1, lp2, v3(s.x, s.y, s.z), maxf, h); break;
-                case SHAPE_CAPSULE: hit = ray_capsule(lp1, lp2, s.x, s.y, maxf, h); break;
-                default: hit = ray_hull(A, (int)s.x, lp1, lp2, maxf, h); break;
-            }
-            if (!hit) continue;
-            hb.t = h.t;
-            hb.p = tapply(l2w, h.p);
-            hb.n = normalized(qrot(l2w.q, h.n));
+            if (!ray_collider(A, c, mask, p1, p2, inv, maxf, h)) continue;
+            hb = h;
             best = c;
             // DynamicAABBTree::raycast (DynamicAABBTree.cpp:705-721) with the closest-hit callback
             if (h.t == 0.0f) break;

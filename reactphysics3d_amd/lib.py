@@ -84,6 +84,8 @@ EXPORTED = [
     "rp3dg_batch_body_apply_torque", "rp3dg_batch_body_reset_force_torque", "rp3dg_batch_body_set_sleeping",
     "rp3dg_batch_get_force_torque", "rp3dg_body_set_transform", "rp3dg_body_apply_force", "rp3dg_body_apply_torque",
     "rp3dg_body_reset_force_torque", "rp3dg_body_get_force_torque", "rp3dg_body_set_sleeping",
+    "rp3dg_batch_raycast_all", "rp3dg_batch_test_overlap_bodies", "rp3dg_batch_test_overlap",
+    "rp3dg_batch_test_collision", "rp3dg_batch_get_query_results",
 ]
 
 # RigidBody call codes of Batch.body_ops (the order of the C-ABI's per-body calls)
@@ -177,6 +179,12 @@ def load():
         "rp3dg_batch_body_reset_force_torque": (C.c_int, [vp, i32, i32, i32]),
         "rp3dg_batch_body_set_sleeping": (C.c_int, [vp, i32, i32]),
         "rp3dg_batch_get_force_torque": (C.c_int, [vp, fptr, fptr]),
+        "rp3dg_batch_test_overlap_bodies": (C.c_int, [vp, i32, i32]),
+        "rp3dg_batch_test_overlap": (C.c_int64, [vp, i32]),
+        "rp3dg_batch_test_collision": (C.c_int64, [vp, i32, i32]),
+        "rp3dg_batch_get_query_results": (C.c_int64, [vp, C.POINTER(ContactView), C.c_int64]),
+        "rp3dg_batch_raycast_all": (C.c_int64, [vp, C.c_int64, iptr, fptr, fptr, fptr, C.c_uint16, iptr, iptr, fptr, fptr,
+                                                fptr, C.c_int64]),
         "rp3dg_body_set_transform": (C.c_int, [vp, i32, fptr, fptr]),
         "rp3dg_body_apply_force": (C.c_int, [vp, i32, fptr, fptr, i32]),
         "rp3dg_body_apply_torque": (C.c_int, [vp, i32, fptr, i32]),
@@ -403,6 +411,64 @@ class Batch:
                                           _fp(mf) if mf is not None else None, int(mask), _ip(col), _fp(pt), _fp(nr),
                                           _fp(fr)), "raycast")
         return col, pt, nr, fr
+
+    def test_overlap_bodies(self, body1, body2):
+        """PhysicsWorld::testOverlap(body1, body2) -> bool (batch body indices)."""
+        r = load().rp3dg_batch_test_overlap_bodies(self.h, int(body1), int(body2))
+        _check(r if r < 0 else 0, "test_overlap_bodies")
+        return bool(r)
+
+    def test_overlap(self, body=-1):
+        """testOverlap(callback) (body < 0) or testOverlap(body, callback): overlapping (collider1,
+        collider2) arrays."""
+        n = load().rp3dg_batch_test_overlap(self.h, int(body))
+        _check(n if n < 0 else 0, "test_overlap")
+        r = self.query_results(n)
+        return r["colliders"][:, 0], r["colliders"][:, 1]
+
+    def test_collision(self, body1=-1, body2=-1):
+        """testCollision(callback / body1 / body1, body2): one record per colliding pair (colliders,
+        nb_points, normal_depth [4, 4], local1 / local2 [4, 3])."""
+        n = load().rp3dg_batch_test_collision(self.h, int(body1), int(body2))
+        _check(n if n < 0 else 0, "test_collision")
+        return self.query_results(n)
+
+    def query_results(self, n):
+        cap = max(1, int(n))
+        out = {
+            "colliders": np.zeros((cap, 2), np.int32), "nb_points": np.zeros(cap, np.int32),
+            "color": np.zeros(cap, np.int32), "normal_depth": np.zeros((cap, 4, 4), np.float32),
+            "local1": np.zeros((cap, 4, 3), np.float32), "local2": np.zeros((cap, 4, 3), np.float32),
+            "impulse": np.zeros((cap, 4), np.float32), "friction": np.zeros((cap, 3), np.float32),
+        }
+        v = ContactView(_ip(out["colliders"]), _ip(out["nb_points"]), _ip(out["color"]), _fp(out["normal_depth"]),
+                        _fp(out["local1"]), _fp(out["local2"]), _fp(out["impulse"]), _fp(out["friction"]))
+        k = load().rp3dg_batch_get_query_results(self.h, C.byref(v), cap)
+        _check(k if k < 0 else 0, "get_query_results")
+        return {key: out[key][:k] for key in ("colliders", "nb_points", "normal_depth", "local1", "local2")}
+
+    def raycast_all(self, p1, p2, world=None, max_fraction=None, mask=0xFFFF):
+        """PhysicsWorld::raycast with a callback keeping every hit: (ray, collider, point, normal, fraction)
+        arrays sorted by (ray, collider)."""
+        L = load()
+        p1 = np.ascontiguousarray(p1, np.float32).reshape(-1, 3)
+        p2 = np.ascontiguousarray(p2, np.float32).reshape(-1, 3)
+        n = len(p1)
+        w = None if world is None else np.ascontiguousarray(world, np.int32)
+        mf = None if max_fraction is None else np.ascontiguousarray(max_fraction, np.float32)
+        args = (self.h, n, _ip(w) if w is not None else None, _fp(p1), _fp(p2), _fp(mf) if mf is not None else None,
+                int(mask))
+        total = L.rp3dg_batch_raycast_all(*args, None, None, None, None, None, 0)
+        _check(total if total < 0 else 0, "raycast_all")
+        cap = max(int(total), 1)
+        ray = np.zeros(cap, np.int32)
+        col = np.zeros(cap, np.int32)
+        pt = np.zeros((cap, 3), np.float32)
+        nr = np.zeros((cap, 3), np.float32)
+        fr = np.zeros(cap, np.float32)
+        k = L.rp3dg_batch_raycast_all(*args, _ip(ray), _ip(col), _fp(pt), _fp(nr), _fp(fr), cap)
+        _check(k if k < 0 else 0, "raycast_all")
+        return ray[:k], col[:k], pt[:k], nr[:k], fr[:k]
 
     def apply_forces(self, force=None, torque=None):
         """RigidBody::applyWorldForceAtCenterOfMass / applyWorldTorque on every body ([nb, 3] each, None =

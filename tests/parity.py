@@ -11,8 +11,7 @@ def contacts_by_pair(ct):
     for m in range(len(ct["nb_points"])):
         key = (int(ct["colliders"][m, 0]), int(ct["colliders"][m, 1]))
         n = int(ct["nb_points"][m])
-        out[key] = dict(n=n, nd=ct["normal_depth"][m, :n], l1=ct["local1"][m, :n], l2=ct["local2"][m, :n],
-                        imp=ct["impulse"][m, :n], fric=ct["friction"][m])
+        out[key] = dict(n=n, nd=ct["normal_depth"][m, :n], l1=ct["local1"][m, :n], l2=ct["local2"][m, :n])
     return out
 
 

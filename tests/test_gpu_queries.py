@@ -1,0 +1,195 @@
+"""World queries on the device against the reference (PhysicsWorld::testOverlap / testCollision,
+CollisionDetectionSystem.cpp:1775-1896): between steps a seeded script asks testOverlap(body1, body2),
+testOverlap(callback), testOverlap(body, callback), testCollision(callback), testCollision(body, callback)
+and testCollision(body1, body2, callback) of the device and of the reference.  The answers agree (pair
+sets; contact points within 1e-5 as the steps' contacts), and so does everything after them: a query runs
+the broadphase (pairs created and removed, the moved set emptied, exits of removed colliding pairs
+reported at the next step) and updates the tested pairs' LastFrameCollisionInfo, so the following steps
+are compared bit for bit (pairs, contacts, bodies, contact events)."""
+import numpy as np
+import pytest
+
+from reactphysics3d_amd import scene as S
+from tests.parity import compare_contacts, pair_set
+
+pytestmark = pytest.mark.gpu
+
+DT = 1.0 / 60.0
+KINDS = ("overlap_bodies", "overlap_all", "overlap_body", "collision_all", "collision_body", "collision_bodies")
+
+
+def _local(ct, c0, c1, key="colliders"):
+    sel = (ct[key][:, 0] >= c0) & (ct[key][:, 0] < c1)
+    out = {k: a[sel] for k, a in ct.items()}
+    out[key] = out[key] - c0
+    return out
+
+
+def _query(kind, dev, refs, rng, scenes):
+    """One query of the given kind on the device and on every reference world (a query about bodies of
+    world w leaves only the broadphase in the other worlds).  Returns the number of colliding pairs."""
+    w = int(rng.integers(len(scenes)))
+    b0 = dev.body_base[w]
+    nbw = scenes[w].nb_bodies
+    types = scenes[w].body_array()["type"]
+    movable = np.nonzero(types != S.STATIC)[0]
+    b1 = int(rng.choice(movable))
+    # a second body: usually one the first touches (from the last step's pairs), else any
+    col_body = scenes[w].collider_array()["body"]
+    p1, p2 = dev.pairs()
+    c0, c1 = dev.collider_base[w], dev.collider_base[w + 1]
+    touching = [int(col_body[b - c0]) for a, b in zip(p1, p2) if c0 <= a < c1 and col_body[a - c0] == b1] + \
+               [int(col_body[a - c0]) for a, b in zip(p1, p2) if c0 <= a < c1 and col_body[b - c0] == b1]
+    b2 = int(rng.choice(touching)) if touching and rng.random() < 0.8 else int(rng.integers(nbw))
+    if b2 == b1:
+        b2 = (b1 + 1) % nbw
+
+    def others():
+        for wi, ref in enumerate(refs):
+            if wi != w:
+                ref.broadphase_only()
+
+    if kind == "overlap_bodies":
+        got = dev.test_overlap_bodies(b0 + b1, b0 + b2)
+        want = refs[w].test_overlap_bodies(b1, b2)
+        others()
+        assert got == want, f"testOverlap(body {b1}, body {b2}) in world {w}: {got} vs {want}"
+        return int(got)
+    if kind in ("overlap_all", "overlap_body"):
+        body = -1 if kind == "overlap_all" else b0 + b1
+        d1, d2 = dev.test_overlap(body)
+        n = 0
+        for wi, ref in enumerate(refs):
+            if kind == "overlap_body" and wi != w:
+                ref.broadphase_only()
+                continue
+            r1, r2 = ref.test_overlap(-1 if kind == "overlap_all" else b1)
+            cw0, cw1 = dev.collider_base[wi], dev.collider_base[wi + 1]
+            sel = (d1 >= cw0) & (d1 < cw1)
+            assert pair_set(d1[sel] - cw0, d2[sel] - cw0) == pair_set(r1, r2), f"{kind} world {wi}: pairs differ"
+            n += len(r1)
+        return n
+    if kind == "collision_all":
+        ct = dev.test_collision()
+        n = 0
+        for wi, ref in enumerate(refs):
+            want = ref.test_collision()
+            mine = _local(ct, dev.collider_base[wi], dev.collider_base[wi + 1])
+            errs = compare_contacts(mine, want)
+            assert not errs, f"testCollision(callback) world {wi}: " + "; ".join(errs[:5])
+            n += len(want["colliders"])
+        return n
+    body2 = -1 if kind == "collision_body" else b2
+    ct = dev.test_collision(b0 + b1, -1 if body2 < 0 else b0 + body2)
+    want = refs[w].test_collision(b1, body2)
+    others()
+    mine = _local(ct, dev.collider_base[w], dev.collider_base[w + 1])
+    errs = compare_contacts(mine, want)
+    assert not errs, f"{kind} ({b1}, {body2}) world {w}: " + "; ".join(errs[:5])
+    return len(want["colliders"])
+
+
+def _run(scenes, steps, seed, query_every=1, solver="auto", kinds=KINDS):
+    from oracle.ref import RefWorld
+    from reactphysics3d_amd.lib import Batch
+    dev = Batch(scenes, solver=solver)
+    dev.set_contact_events(True)
+    refs = [RefWorld(sc) for sc in scenes]
+    for r in refs:
+        r.enable_events()
+    order = dev.joint_order()
+    for w, ref in enumerate(refs):
+        j0, j1 = dev.joint_base[w], dev.joint_base[w + 1]
+        ref.set_joint_order(order[(order >= j0) & (order < j1)] - j0)
+    rng = np.random.default_rng(seed)
+    found = {k: 0 for k in kinds}
+    exits = 0
+    for k in range(steps):
+        if k % query_every == 0:
+            for _ in range(int(rng.integers(1, 3))):
+                kind = kinds[int(rng.integers(len(kinds)))]
+                found[kind] += _query(kind, dev, refs, rng, scenes)
+        dev.step(DT)
+        ct = dev.contacts()
+        p1, p2 = dev.pairs()
+        ev = dev.contact_events()
+        pos, q, v, w = dev.bodies()
+        for wi, ref in enumerate(refs):
+            c0, c1 = dev.collider_base[wi], dev.collider_base[wi + 1]
+            local = _local(ct, c0, c1)
+            ref.step_ordered(DT, local["colliders"][:, 0], local["colliders"][:, 1])
+            psel = (p1 >= c0) & (p1 < c1)
+            r1, r2 = ref.pairs()
+            assert pair_set(p1[psel] - c0, p2[psel] - c0) == pair_set(r1, r2), f"step {k} world {wi}: pair sets differ"
+            errs = compare_contacts(local, ref.contacts())
+            assert not errs, f"step {k} world {wi}: " + "; ".join(errs[:5])
+            mine = ev[(ev[:, 0] >= c0) & (ev[:, 0] < c1)].copy()
+            mine[:, :2] -= c0
+            want = ref.events()
+            assert np.array_equal(mine, want), f"step {k} world {wi}: events\n{mine}\nvs reference\n{want}"
+            exits += int(np.sum(want[:, 2] == 2))
+            b0, b1 = dev.body_base[wi], dev.body_base[wi + 1]
+            dev_state = np.concatenate([a[b0:b1].ravel() for a in (pos, q, v, w)])
+            ref_state = np.concatenate([a.ravel() for a in ref.bodies()])
+            err = float(np.max(np.abs(dev_state - ref_state)))
+            assert err == 0.0, f"step {k} world {wi}: body state differs by {err}"
+    return dev, found, exits
+
+
+def test_queries_between_steps_of_a_bouncing_container():
+    dev, found, exits = _run([S.mixed_container(300, extent=(12.0, 12.0, 8.0), seed=3)], 90, seed=1)
+    print("colliding pairs found per query kind", found, "exits", exits)
+    assert all(found[k] > 0 for k in KINDS), found
+    assert exits > 0
+
+
+def test_queries_on_hulls_over_a_batch_of_worlds():
+    """Hull pairs (SAT without clipping with the previous axis in queries) in two worlds of one batch:
+    a query about bodies of one world leaves only its broadphase in the other."""
+    scenes = [S.hull_scene(n_hulls=60, n_chains=1, chain_length=4, extent=(8.0, 8.0, 6.0), seed=s) for s in (5, 6)]
+    dev, found, exits = _run(scenes, 120, seed=2, query_every=2)
+    print("colliding pairs found per query kind", found, "exits", exits)
+    assert found["collision_all"] > 0 and found["overlap_all"] > 0
+
+
+def test_queries_around_sleep_and_wake():
+    """Queries test the pairs with an enabled body only: a pile that falls asleep and a ball that wakes
+    it, queried every third step."""
+    dev, found, exits = _run([S.sleeping_pile(3, 2, seed=5, ball_height=40.0)], 260, seed=4, query_every=3)
+    print("colliding pairs found per query kind", found, "exits", exits)
+    assert dev.sleeping().sum() >= 0 and found["collision_all"] > 0
+
+
+def test_queries_see_world_query_flags_and_triggers():
+    """testOverlap / testCollision(callback) test world-query colliders only; triggers overlap without
+    points in testCollision(callback) and with points when a body query names them."""
+    sc = S.trigger_scene(60, seed=12)
+    rng = S.SplitMix64(7)
+    for i, c in enumerate(sc.colliders):
+        if i % 7 == 3 and not (c[-1] & S.COLLIDER_TRIGGER):
+            sc.colliders[i] = c[:-1] + (c[-1] & ~S.COLLIDER_WORLD_QUERY,)
+    dev, found, exits = _run([sc], 80, seed=3)
+    print("colliding pairs found per query kind", found)
+    assert found["overlap_all"] > 0 and found["collision_all"] > 0
+
+
+def test_query_before_the_first_step_and_graph_replay_after_queries():
+    """A query on the creation state (every collider still in the moved set), then graph replays whose
+    first step reports the exits the queries recorded: the replayed state equals the eager one."""
+    from reactphysics3d_amd.lib import Batch
+    sc = S.mixed_container(200, extent=(10.0, 10.0, 8.0), seed=4)
+    eager, graph = Batch([sc], solver="fused"), Batch([sc], solver="fused")
+    for b in (eager, graph):
+        b.set_contact_events(True)
+        c = b.test_collision()
+        assert len(c["colliders"]) >= 0
+    for k in range(6):
+        for b in (eager, graph):
+            b.test_overlap()
+        for _ in range(5):
+            eager.step(DT)
+        graph.step_graph(DT, 5)
+        for a, b in zip(eager.bodies(), graph.bodies()):
+            assert np.array_equal(a, b), f"round {k}: graph replay after queries differs from eager steps"
+        e1, e2 = eager.contact_events(), graph.contact_events()
+        assert np.array_equal(e1[np.lexsort(e1.T[::-1])], e2[np.lexsort(e2.T[::-1])])

@@ -88,3 +88,50 @@ def test_raycasts_over_a_batch_of_worlds():
         total += _check(mine, ref.raycast(p1[sel], p2[sel]), f"world {w}")
     print("hits", total, "of", n)
     assert total > 300
+
+
+def _sorted_hits(out):
+    ray, col, pt, nr, fr = out
+    o = np.lexsort((col, ray))
+    return ray[o], col[o], pt[o], nr[o], fr[o]
+
+
+@pytest.mark.parametrize("name", ["pile", "mixed", "hulls"])
+def test_every_hit_raycasts_match_reference(name):
+    """A callback returning 1 keeps every hit: the device's hits, sorted by (ray, collider), are the
+    reference's hits bit for bit (the reference reports them in tree order)."""
+    sc, lo, hi = {"pile": (S.box_pile(4, 3, seed=11), (-6.0, -6.0, -1.0), (6.0, 6.0, 8.0)),
+                  "mixed": (S.mixed_container(400, extent=(16.0, 16.0, 10.0), seed=2), (-9.0, -9.0, -1.0), (9.0, 9.0, 11.0)),
+                  "hulls": (S.hull_scene(n_hulls=80, n_chains=2, chain_length=4, extent=(10.0, 10.0, 8.0), seed=5),
+                            (-6.0, -6.0, -1.0), (6.0, 6.0, 9.0))}[name]
+    dev, ref = _stepped(sc, 40)
+    rng = np.random.default_rng(17)
+    p1, p2 = _rays(rng, 2000, lo, hi)
+    mf = rng.uniform(0.2, 1.0, 2000).astype(np.float32)
+    d = dev.raycast_all(p1, p2, max_fraction=mf)
+    r = _sorted_hits(ref.raycast_all(p1, p2, mf))
+    assert len(d[0]) == len(r[0]), f"{name}: {len(d[0])} hits vs {len(r[0])}"
+    for what, a, b in zip(("ray", "collider", "point", "normal", "fraction"), d, r):
+        assert np.array_equal(a, b), f"{name}: {what} differs"
+    per_ray = np.bincount(d[0], minlength=2000)
+    print(name, "hits", len(d[0]), "rays with 2+ hits", int((per_ray >= 2).sum()))
+    assert (per_ray >= 2).sum() > 50
+
+
+def test_rays_skip_colliders_that_are_not_world_query_colliders():
+    """Collider::setIsWorldQueryCollider(false) hides a collider from raycasts (BroadPhaseSystem.cpp:232)."""
+    from oracle.ref import RefWorld
+    from reactphysics3d_amd.lib import Batch
+    sc = S.Scene("query_flags", S.make_settings(gravity=(0.0, 0.0, 0.0)))
+    for k in range(8):
+        b = sc.add_body(S.STATIC, (2.0 * k, 0.0, 0.0))
+        sc.add_collider(b, S.SPHERE if k % 2 else S.BOX, (0.6, 0.5, 0.5), world_query=(k % 3 != 1))
+    dev, ref = Batch([sc]), RefWorld(sc)
+    p1 = np.array([[-2.0, 0.1, 0.05], [15.0, -0.1, 0.0], [2.0, 3.0, 0.0], [8.0, 0.0, 3.0]], np.float32)
+    p2 = np.array([[16.0, 0.1, 0.05], [-2.0, 0.2, 0.1], [2.0, -3.0, 0.0], [8.0, 0.0, -3.0]], np.float32)
+    _check(dev.raycast(p1, p2), ref.raycast(p1, p2, None), "closest")
+    d = dev.raycast_all(p1, p2)
+    r = _sorted_hits(ref.raycast_all(p1, p2))
+    for a, b in zip(d, r):
+        assert np.array_equal(a, b)
+    assert set(d[1].tolist()) == {0, 2, 3, 5, 6}   # colliders 1, 4 and 7 are not world-query colliders

@@ -434,3 +434,32 @@ def test_ordered_step_with_a_missing_manifold_fails_cleanly(ref_available):
     for _ in range(5):
         b.step(DT)
     b.close()
+
+
+def test_every_hit_raycast_contains_the_closest_hit(ref_available):
+    """The reference with a callback keeping every hit (returning 1) and with the closest-hit callback
+    agree: every ray's smallest all-hits fraction is the closest hit, on the same collider with the same
+    point, and no hit lies beyond the ray's max fraction.  Pins the all-hits harness entry against the
+    closest-hit one."""
+    from oracle.ref import RefWorld
+    sc = S.box_pile(4, 3, seed=11)
+    ref = RefWorld(sc)
+    for _ in range(30):
+        ref.step(1.0 / 60.0)
+    rng = np.random.default_rng(5)
+    p1 = rng.uniform((-6, -6, -1), (6, 6, 8), (1500, 3)).astype(np.float32)
+    p2 = rng.uniform((-6, -6, -1), (6, 6, 8), (1500, 3)).astype(np.float32)
+    mf = rng.uniform(0.2, 1.0, 1500).astype(np.float32)
+    ray, col, pt, nr, fr = ref.raycast_all(p1, p2, mf)
+    cc, cp, cn, cf = ref.raycast(p1, p2, mf)
+    assert np.all(fr <= mf[ray])
+    multi = 0
+    for i in range(1500):
+        sel = np.nonzero(ray == i)[0]
+        if cc[i] < 0:
+            assert len(sel) == 0
+            continue
+        multi += len(sel) >= 2
+        j = sel[np.argmin(fr[sel])]
+        assert fr[j] == cf[i] and col[j] == cc[i] and np.array_equal(pt[j], cp[i])
+    assert multi > 50
