@@ -12,6 +12,7 @@
 // centre of mass or at a point, resets, setIsSleeping).
 #pragma once
 
+#include <algorithm>
 #include <cmath>
 #include <cstdint>
 #include <map>
@@ -617,27 +618,56 @@ class PhysicsWorld {
         check(rp3dg_world_update(mWorld, timeStep), "update");
         if (mListener) report();
     }
-    // PhysicsWorld::raycast (PhysicsWorld.h:486-490) for the closest hit: the device finds the hit a
-    // callback returning info.hitFraction would keep, and the callback is called once with it (a
-    // callback that gathers every hit along the ray is not served).  Valid after the first update().
+    // PhysicsWorld::raycast (PhysicsWorld.h:486-490): the device finds every hit of the ray (world-query
+    // colliders whose category meets the mask), and the callback sees them from the closest on, with
+    // DynamicAABBTree::raycast's use of its return value (DynamicAABBTree.cpp:705-721): 0 stops, a
+    // fraction in (0, max) clips the ray (farther hits are not reported), 1 or -1 go on.  A callback that
+    // keeps the closest hit, or every hit, sees what the reference reports; one that stops at its first
+    // hit gets the closest (the reference gives the first its tree visits).
     void raycast(const Ray& ray, RaycastCallback* callback, unsigned short mask = 0xFFFF) const {
-        rp3dg_batch* b = rp3dg_world_get_batch(mWorld);
-        if (!b) throw std::runtime_error("raycast: step the world once first");
+        rp3dg_batch* b = device();
         const float p1[3] = {ray.point1.x, ray.point1.y, ray.point1.z}, p2[3] = {ray.point2.x, ray.point2.y, ray.point2.z};
         const float mf = ray.maxFraction;
-        int32_t c = -1;
-        float pt[3], nr[3], fr = 0.f;
-        check(rp3dg_batch_raycast(b, 1, nullptr, p1, p2, &mf, mask, &c, pt, nr, &fr), "raycast");
-        if (c < 0) return;
-        RaycastInfo info;
-        info.worldPoint = Vector3(pt[0], pt[1], pt[2]);
-        info.worldNormal = Vector3(nr[0], nr[1], nr[2]);
-        info.hitFraction = fr;
-        for (const auto& body : mBodies)
-            for (const auto& col : body->mColliders)
-                if (col->index() == c) { info.collider = col.get(); info.body = body.get(); }
-        callback->notifyRaycastHit(info);
+        const int64_t n = rp3dg_batch_raycast_all(b, 1, nullptr, p1, p2, &mf, mask, nullptr, nullptr, nullptr, nullptr,
+                                                  nullptr, 0);
+        check((int)std::min<int64_t>(n, 0), "raycast");
+        if (n == 0) return;
+        std::vector<int32_t> ray_index(n), col(n);
+        std::vector<float> pt(3 * n), nr(3 * n), fr(n);
+        check((int)std::min<int64_t>(rp3dg_batch_raycast_all(b, 1, nullptr, p1, p2, &mf, mask, ray_index.data(), col.data(),
+                                                              pt.data(), nr.data(), fr.data(), n), 0), "raycast");
+        std::vector<int64_t> order(n);
+        for (int64_t i = 0; i < n; i++) order[i] = i;
+        std::stable_sort(order.begin(), order.end(), [&](int64_t a, int64_t c) { return fr[a] < fr[c]; });
+        const auto owners = owner_map();
+        decimal maxf = ray.maxFraction;
+        for (int64_t i : order) {
+            if (fr[i] > maxf) break;
+            RaycastInfo info;
+            info.worldPoint = Vector3(pt[3 * i], pt[3 * i + 1], pt[3 * i + 2]);
+            info.worldNormal = Vector3(nr[3 * i], nr[3 * i + 1], nr[3 * i + 2]);
+            info.hitFraction = fr[i];
+            auto it = owners.find(col[i]);
+            if (it != owners.end()) { info.collider = it->second.first; info.body = it->second.second; }
+            const decimal r = callback->notifyRaycastHit(info);
+            if (r == decimal(0)) break;
+            if (r > decimal(0) && r < maxf) maxf = r;
+        }
     }
+    // PhysicsWorld::testOverlap / testCollision (PhysicsWorld.h:319-334, CollisionDetectionSystem.cpp:
+    // 1775-1896), on the device; like the reference's, each runs the broadphase first.
+    bool testOverlap(RigidBody* body1, RigidBody* body2) {
+        const int r = rp3dg_batch_test_overlap_bodies(device(), body1->index(), body2->index());
+        check(r, "testOverlap");
+        return r == 1;
+    }
+    void testOverlap(RigidBody* body, OverlapCallback& callback) { overlap(body ? body->index() : -1, callback); }
+    void testOverlap(OverlapCallback& callback) { overlap(-1, callback); }
+    void testCollision(RigidBody* body1, RigidBody* body2, CollisionCallback& callback) {
+        collision(body1->index(), body2->index(), callback);
+    }
+    void testCollision(RigidBody* body, CollisionCallback& callback) { collision(body->index(), -1, callback); }
+    void testCollision(CollisionCallback& callback) { collision(-1, -1, callback); }
     // PhysicsWorld::setEventListener (src/engine/PhysicsWorld.h)
     void setEventListener(EventListener* listener) {
         mListener = listener;
@@ -646,6 +676,80 @@ class PhysicsWorld {
     rp3dg_world* handle() const { return mWorld; }
 
    private:
+    rp3dg_batch* device() const {
+        rp3dg_batch* b = rp3dg_world_upload(mWorld);
+        if (!b) throw std::runtime_error(std::string("world query: ") + rp3dg_last_error());
+        return b;
+    }
+    std::map<int32_t, std::pair<Collider*, RigidBody*>> owner_map() const {
+        std::map<int32_t, std::pair<Collider*, RigidBody*>> owner;
+        for (auto& body : mBodies)
+            for (auto& col : body->mColliders) owner[col->index()] = {col.get(), body.get()};
+        return owner;
+    }
+    struct QueryRecords {
+        std::vector<int32_t> cols, npts, ev;
+        std::vector<float> nd, l1, l2;
+    };
+    QueryRecords query_records(rp3dg_batch* b, int64_t n) const {
+        QueryRecords q;
+        q.cols.resize(2 * n + 2); q.npts.resize(n + 1); q.ev.resize(n + 1);
+        q.nd.resize(16 * n + 16); q.l1.resize(12 * n + 12); q.l2.resize(12 * n + 12);
+        std::vector<float> imp(4 * n + 4), fr(3 * n + 3);
+        rp3dg_contact_view v{q.cols.data(), q.npts.data(), q.ev.data(), q.nd.data(), q.l1.data(), q.l2.data(), imp.data(),
+                             fr.data()};
+        check((int)std::min<int64_t>(rp3dg_batch_get_query_results(b, &v, n), 0), "query results");
+        return q;
+    }
+    void overlap(int32_t body, OverlapCallback& callback) {
+        rp3dg_batch* b = device();
+        const int64_t n = rp3dg_batch_test_overlap(b, body);
+        check((int)std::min<int64_t>(n, 0), "testOverlap");
+        if (n == 0) return;
+        const QueryRecords q = query_records(b, n);
+        const auto owners = owner_map();
+        OverlapCallback::CallbackData data;
+        for (int64_t m = 0; m < n; m++) {
+            OverlapCallback::OverlapPair o;
+            o.mCollider1 = owners.at(q.cols[2 * m]).first;
+            o.mBody1 = owners.at(q.cols[2 * m]).second;
+            o.mCollider2 = owners.at(q.cols[2 * m + 1]).first;
+            o.mBody2 = owners.at(q.cols[2 * m + 1]).second;
+            o.mType = q.ev[m] ? OverlapCallback::OverlapPair::EventType::OverlapStay
+                              : OverlapCallback::OverlapPair::EventType::OverlapStart;
+            data.mPairs.push_back(o);
+        }
+        callback.onOverlap(data);
+    }
+    void collision(int32_t body1, int32_t body2, CollisionCallback& callback) {
+        rp3dg_batch* b = device();
+        const int64_t n = rp3dg_batch_test_collision(b, body1, body2);
+        check((int)std::min<int64_t>(n, 0), "testCollision");
+        if (n == 0) return;
+        const QueryRecords q = query_records(b, n);
+        const auto owners = owner_map();
+        CollisionCallback::CallbackData data;
+        for (int64_t m = 0; m < n; m++) {
+            CollisionCallback::ContactPair p;
+            p.mCollider1 = owners.at(q.cols[2 * m]).first;
+            p.mBody1 = owners.at(q.cols[2 * m]).second;
+            p.mCollider2 = owners.at(q.cols[2 * m + 1]).first;
+            p.mBody2 = owners.at(q.cols[2 * m + 1]).second;
+            p.mType = q.ev[m] ? CollisionCallback::ContactPair::EventType::ContactStay
+                              : CollisionCallback::ContactPair::EventType::ContactStart;
+            for (int k = 0; k < q.npts[m]; k++) {
+                CollisionCallback::ContactPoint cp;
+                const float* d = &q.nd[16 * m + 4 * k];
+                cp.mNormal = Vector3(d[0], d[1], d[2]);
+                cp.mDepth = d[3];
+                cp.mLocal1 = Vector3(q.l1[12 * m + 3 * k], q.l1[12 * m + 3 * k + 1], q.l1[12 * m + 3 * k + 2]);
+                cp.mLocal2 = Vector3(q.l2[12 * m + 3 * k], q.l2[12 * m + 3 * k + 1], q.l2[12 * m + 3 * k + 2]);
+                p.mPoints.push_back(cp);
+            }
+            data.mPairs.push_back(p);
+        }
+        callback.onContact(data);
+    }
     // events of the last step with their contact points (one manifold per convex pair), delivered
     // when there is at least one (CollisionDetectionSystem.cpp:1733-1741)
     void report() {

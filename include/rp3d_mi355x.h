@@ -310,6 +310,9 @@ int64_t rp3dg_batch_get_contact_events(rp3dg_batch* batch, int32_t* collider1, i
 int rp3dg_world_set_contact_events(rp3dg_world* world, int32_t enable);
 /* The batch of one that rp3dg_world_update steps (NULL before the first update). */
 rp3dg_batch* rp3dg_world_get_batch(rp3dg_world* world);
+/* Creates that batch now, as the first update() would, without stepping (for raycasts and world queries
+ * before the first step); returns it (NULL with rp3dg_last_error() on failure). */
+rp3dg_batch* rp3dg_world_upload(rp3dg_world* world);
 /* World queries (PhysicsWorld::testOverlap / testCollision, PhysicsWorld.h:319-334,
  * CollisionDetectionSystem.cpp:1775-1896) on the current state.  As in the reference, a query first runs
  * the broadphase (computeBroadPhase): pairs are created and removed, the moved set empties, and pairs it
@@ -331,8 +334,11 @@ int64_t rp3dg_batch_test_overlap(rp3dg_batch* batch, int32_t body);
 int64_t rp3dg_batch_test_collision(rp3dg_batch* batch, int32_t body1, int32_t body2);
 /* What the CallbackData of the last rp3dg_batch_test_overlap / test_collision holds, one record per pair
  * in the contact view layout: colliders, nb_points, normal_depth, local1, local2
- * (ContactPoint::getWorldNormal / getPenetrationDepth / getLocalPointOnCollider1/2); impulse, friction
- * and color are written 0.  Reading does not run the query again.  Returns the count. */
+ * (ContactPoint::getWorldNormal / getPenetrationDepth / getLocalPointOnCollider1/2), and in color the
+ * event type: testCollision gives 0 ContactStart or 1 ContactStay by whether the pair collided in the
+ * previous step, testOverlap always 0 OverlapStart (CollisionDetectionSystem.cpp:831); impulse and
+ * friction are written 0.  Reading does not run the query again.  Returns
+ * the count. */
 int64_t rp3dg_batch_get_query_results(rp3dg_batch* batch, rp3dg_contact_view* out, int64_t capacity);
 /* RigidBody::applyWorldForceAtCenterOfMass / applyWorldTorque (src/body/RigidBody.cpp:298-311, 938-951)
  * on every dynamic body of the batch whose force / torque (xyz per body, NULL = none) is not zero,

@@ -165,31 +165,34 @@ class RefWorld:
         return bool(self.L.rp3dref_test_overlap_bodies(self.h, int(b1), int(b2)))
 
     def test_overlap(self, body=-1):
-        """testOverlap(callback) (body < 0) or testOverlap(body, callback): (collider1, collider2) arrays in
-        the reference's report order."""
+        """testOverlap(callback) (body < 0) or testOverlap(body, callback): (collider1, collider2, event
+        type) arrays in the reference's report order."""
         ip = C.POINTER(C.c_int32)
-        self.L.rp3dref_test_overlap.argtypes = [C.c_void_p, C.c_int, ip, ip, C.c_int]
+        self.L.rp3dref_test_overlap.argtypes = [C.c_void_p, C.c_int, ip, ip, ip, C.c_int]
         self.L.rp3dref_test_overlap.restype = C.c_int
         cap = 64 * max(1, self.nc)
         c1 = np.zeros(cap, np.int32)
         c2 = np.zeros(cap, np.int32)
-        n = self.L.rp3dref_test_overlap(self.h, int(body), _ip(c1), _ip(c2), cap)
+        ev = np.zeros(cap, np.int32)
+        n = self.L.rp3dref_test_overlap(self.h, int(body), _ip(c1), _ip(c2), _ip(ev), cap)
         if n > cap:   # a second call would not see the same state: the query changes it
             raise RuntimeError(f"test_overlap: {n} pairs, capacity {cap}")
-        return c1[:n], c2[:n]
+        return c1[:n], c2[:n], ev[:n]
 
     def test_collision(self, b1=-1, b2=-1):
         """testCollision(callback / body1 / body1, body2): contact records as Batch.test_collision."""
         ip = C.POINTER(C.c_int32)
         fp = C.POINTER(C.c_float)
-        self.L.rp3dref_test_collision.argtypes = [C.c_void_p, C.c_int, C.c_int, ip, ip, fp, fp, fp, C.c_int]
+        self.L.rp3dref_test_collision.argtypes = [C.c_void_p, C.c_int, C.c_int, ip, ip, ip, fp, fp, fp, C.c_int]
         self.L.rp3dref_test_collision.restype = C.c_int
         cap = 64 * max(1, self.nc)
         out = {"colliders": np.zeros((cap, 2), np.int32), "nb_points": np.zeros(cap, np.int32),
+               "event": np.zeros(cap, np.int32),
                "normal_depth": np.zeros((cap, 4, 4), np.float32), "local1": np.zeros((cap, 4, 3), np.float32),
                "local2": np.zeros((cap, 4, 3), np.float32)}
         n = self.L.rp3dref_test_collision(self.h, int(b1), int(b2), _ip(out["colliders"]), _ip(out["nb_points"]),
-                                          _fp(out["normal_depth"]), _fp(out["local1"]), _fp(out["local2"]), cap)
+                                          _ip(out["event"]), _fp(out["normal_depth"]), _fp(out["local1"]),
+                                          _fp(out["local2"]), cap)
         if n > cap:
             raise RuntimeError(f"test_collision: {n} contact pairs, capacity {cap}")
         return {k: a[:n] for k, a in out.items()}

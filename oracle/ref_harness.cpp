@@ -722,38 +722,40 @@ int rp3dref_test_overlap_bodies(void* h, int b1, int b2) {
 
 struct OverlapRecorder : public OverlapCallback {
     RefWorld* R = nullptr;
-    std::vector<int32_t> cols;
+    std::vector<int32_t> cols, evs;
     void onOverlap(CallbackData& data) override {
         for (uint32 i = 0; i < data.getNbOverlappingPairs(); i++) {
             const OverlapPair p = data.getOverlappingPair(i);
             cols.push_back(collider_index(R, p.getCollider1()->getEntity()));
             cols.push_back(collider_index(R, p.getCollider2()->getEntity()));
+            evs.push_back((int32_t)p.getEventType());
         }
     }
 };
 
-// testOverlap(callback) (body < 0) or testOverlap(body, callback): the reported collider pairs in report
-// order; returns their count (pairs past cap are counted only).
-int rp3dref_test_overlap(void* h, int body, int32_t* c1, int32_t* c2, int cap) {
+// testOverlap(callback) (body < 0) or testOverlap(body, callback): the reported collider pairs and their
+// OverlapPair::EventType in report order; returns their count (pairs past cap are counted only).
+int rp3dref_test_overlap(void* h, int body, int32_t* c1, int32_t* c2, int32_t* ev, int cap) {
     RefWorld* R = (RefWorld*)h;
     OverlapRecorder cb;
     cb.R = R;
     if (body < 0) R->world->testOverlap(cb);
     else R->world->testOverlap(R->bodies[body], cb);
     const int n = (int)cb.cols.size() / 2;
-    for (int i = 0; i < n && i < cap; i++) { c1[i] = cb.cols[2 * i]; c2[i] = cb.cols[2 * i + 1]; }
+    for (int i = 0; i < n && i < cap; i++) { c1[i] = cb.cols[2 * i]; c2[i] = cb.cols[2 * i + 1]; ev[i] = cb.evs[i]; }
     return n;
 }
 
 struct CollisionRecorder : public CollisionCallback {
     RefWorld* R = nullptr;
-    std::vector<int32_t> cols, npts;
+    std::vector<int32_t> cols, npts, evs;
     std::vector<float> nd, l1, l2;
     void onContact(const CallbackData& data) override {
         for (uint32 i = 0; i < data.getNbContactPairs(); i++) {
             const ContactPair p = data.getContactPair(i);
             cols.push_back(collider_index(R, p.getCollider1()->getEntity()));
             cols.push_back(collider_index(R, p.getCollider2()->getEntity()));
+            evs.push_back((int32_t)p.getEventType());
             const uint32 n = p.getNbContactPoints();
             npts.push_back((int32_t)n);
             for (uint32 k = 0; k < 4; k++) {
@@ -773,10 +775,10 @@ struct CollisionRecorder : public CollisionCallback {
 };
 
 // testCollision(callback) (b1 < 0), testCollision(b1, callback) (b2 < 0) or testCollision(b1, b2,
-// callback): one record per reported contact pair (colliders, point count, 4 x normal + depth, 4 x local
-// point on collider 1 and 2), in report order; returns the count (records past cap are counted only).
-int rp3dref_test_collision(void* h, int b1, int b2, int32_t* cols, int32_t* npts, float* nd, float* l1, float* l2,
-                           int cap) {
+// callback): one record per reported contact pair (colliders, point count, ContactPair::EventType, 4 x
+// normal + depth, 4 x local point on collider 1 and 2), in report order; returns the count (records past cap are counted only).
+int rp3dref_test_collision(void* h, int b1, int b2, int32_t* cols, int32_t* npts, int32_t* ev, float* nd, float* l1,
+                           float* l2, int cap) {
     RefWorld* R = (RefWorld*)h;
     CollisionRecorder cb;
     cb.R = R;
@@ -788,6 +790,7 @@ int rp3dref_test_collision(void* h, int b1, int b2, int32_t* cols, int32_t* npts
         cols[2 * i] = cb.cols[2 * i];
         cols[2 * i + 1] = cb.cols[2 * i + 1];
         npts[i] = cb.npts[i];
+        ev[i] = cb.evs[i];
         for (int j = 0; j < 16; j++) nd[16 * i + j] = cb.nd[16 * i + j];
         for (int j = 0; j < 12; j++) { l1[12 * i + j] = cb.l1[12 * i + j]; l2[12 * i + j] = cb.l2[12 * i + j]; }
     }

@@ -1498,18 +1498,22 @@ int rp3dg_batch_synchronize(rp3dg_batch* b) {
     return RP3DG_OK;
 }
 
-int rp3dg_world_update(rp3dg_world* w, float dt) {
-    if (!w) return RP3DG_EINVAL;
+rp3dg_batch* rp3dg_world_upload(rp3dg_world* w) {
+    if (!w) { set_error("null world"); return nullptr; }
     if (!w->own) {
-        if (w->uploaded) { set_error("world belongs to a batch; step the batch"); return RP3DG_ESTATE; }
+        if (w->uploaded) { set_error("world belongs to a batch; step the batch"); return nullptr; }
         rp3dg_world* ws[1] = {w};
         w->own = rp3dg_batch_create(ws, 1);
-        if (!w->own) return RP3DG_EHIP;
-        if (w->events) {
-            const int r = rp3dg_batch_set_contact_events(w->own, 1);
-            if (r) return r;
-        }
+        if (!w->own) return nullptr;
+        if (w->events && rp3dg_batch_set_contact_events(w->own, 1)) return nullptr;
+        w->mirror_dirty = true;
     }
+    return w->own;
+}
+
+int rp3dg_world_update(rp3dg_world* w, float dt) {
+    if (!w) return RP3DG_EINVAL;
+    if (!w->own && !rp3dg_world_upload(w)) return w->uploaded ? RP3DG_ESTATE : RP3DG_EHIP;
     const int r = rp3dg_batch_step(w->own, dt, 1);
     w->mirror_dirty = true;
     return r;
@@ -1865,6 +1869,8 @@ int64_t rp3dg_batch_get_query_results(rp3dg_batch* b, rp3dg_contact_view* o, int
     const int64_t n = (int64_t)hits.size();
     if (capacity < n) { set_error("query results: capacity below the pair count"); return RP3DG_EINVAL; }
     const int np = b->last.nb_pairs;
+    std::vector<int> evf((size_t)std::max(np, 1));
+    if (np) HIPCHECK(hipMemcpy(evf.data(), (b->cur_is_a ? b->Pa : b->Pb).evf, sizeof(int) * np, hipMemcpyDeviceToHost));
     std::vector<float4> pn(4 * (size_t)std::max(np, 1)), pl1(pn.size()), pl2(pn.size());
     if (np && b->query_points) {
         HIPCHECK(hipMemcpy(pn.data(), b->q_pn, sizeof(float4) * 4 * np, hipMemcpyDeviceToHost));
@@ -1876,7 +1882,9 @@ int64_t rp3dg_batch_get_query_results(rp3dg_batch* b, rp3dg_contact_view* o, int
         const int cnt = b->query_points ? std::max(ncp[k], 0) : 0;
         if (o->colliders) { o->colliders[2 * m] = col[k].x; o->colliders[2 * m + 1] = col[k].y; }
         if (o->nb_points) o->nb_points[m] = cnt;
-        if (o->color) o->color[m] = 0;
+        // ContactPair::collidingInPreviousFrame: the pair's flag in testCollision (CD.cpp:1232), false in
+        // testOverlap's pairs (:831), which are therefore all OverlapStart
+        if (o->color) o->color[m] = (b->query_points && (evf[k] & EV_PREV)) ? 1 : 0;
         for (int p = 0; p < 4; p++) {
             const bool in = p < cnt;
             const float4 z = make_float4(0.f, 0.f, 0.f, 0.f);

@@ -85,7 +85,7 @@ EXPORTED = [
     "rp3dg_batch_get_force_torque", "rp3dg_body_set_transform", "rp3dg_body_apply_force", "rp3dg_body_apply_torque",
     "rp3dg_body_reset_force_torque", "rp3dg_body_get_force_torque", "rp3dg_body_set_sleeping",
     "rp3dg_batch_raycast_all", "rp3dg_batch_test_overlap_bodies", "rp3dg_batch_test_overlap",
-    "rp3dg_batch_test_collision", "rp3dg_batch_get_query_results",
+    "rp3dg_batch_test_collision", "rp3dg_batch_get_query_results", "rp3dg_world_upload",
 ]
 
 # RigidBody call codes of Batch.body_ops (the order of the C-ABI's per-body calls)
@@ -183,6 +183,7 @@ def load():
         "rp3dg_batch_test_overlap": (C.c_int64, [vp, i32]),
         "rp3dg_batch_test_collision": (C.c_int64, [vp, i32, i32]),
         "rp3dg_batch_get_query_results": (C.c_int64, [vp, C.POINTER(ContactView), C.c_int64]),
+        "rp3dg_world_upload": (vp, [vp]),
         "rp3dg_batch_raycast_all": (C.c_int64, [vp, C.c_int64, iptr, fptr, fptr, fptr, C.c_uint16, iptr, iptr, fptr, fptr,
                                                 fptr, C.c_int64]),
         "rp3dg_body_set_transform": (C.c_int, [vp, i32, fptr, fptr]),
@@ -420,11 +421,11 @@ class Batch:
 
     def test_overlap(self, body=-1):
         """testOverlap(callback) (body < 0) or testOverlap(body, callback): overlapping (collider1,
-        collider2) arrays."""
+        collider2, event type: 0 OverlapStart / 1 OverlapStay) arrays."""
         n = load().rp3dg_batch_test_overlap(self.h, int(body))
         _check(n if n < 0 else 0, "test_overlap")
         r = self.query_results(n)
-        return r["colliders"][:, 0], r["colliders"][:, 1]
+        return r["colliders"][:, 0], r["colliders"][:, 1], r["event"]
 
     def test_collision(self, body1=-1, body2=-1):
         """testCollision(callback / body1 / body1, body2): one record per colliding pair (colliders,
@@ -445,7 +446,9 @@ class Batch:
                         _fp(out["local1"]), _fp(out["local2"]), _fp(out["impulse"]), _fp(out["friction"]))
         k = load().rp3dg_batch_get_query_results(self.h, C.byref(v), cap)
         _check(k if k < 0 else 0, "get_query_results")
-        return {key: out[key][:k] for key in ("colliders", "nb_points", "normal_depth", "local1", "local2")}
+        res = {key: out[key][:k] for key in ("colliders", "nb_points", "normal_depth", "local1", "local2")}
+        res["event"] = out["color"][:k]   # 0 Start, 1 Stay
+        return res
 
     def raycast_all(self, p1, p2, world=None, max_fraction=None, mask=0xFFFF):
         """PhysicsWorld::raycast with a callback keeping every hit: (ray, collider, point, normal, fraction)

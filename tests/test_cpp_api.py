@@ -268,3 +268,73 @@ def test_convex_mesh_signatures_of_the_reference():
     assert get("open") == [["0", "1"]]
     assert get("message")[0][0] == "1"   # Message::Type::Error
     assert get("colliders") == [["0", "1"]]
+
+
+QUERIES_EXE = os.path.join(ROOT, "reactphysics3d_amd", "_lib", "queries")
+
+
+def _queries_scene():
+    """The world of examples/queries.cpp (WorldSettings defaults, gravity -z): three boxes apart on a
+    ground, so no two manifolds share a dynamic body."""
+    sc = S.Scene("queries_example", S.make_settings(gravity=(0.0, 0.0, -9.81), sleeping=1))
+    g = sc.add_body(S.STATIC, (0.0, 0.0, -0.5))
+    sc.add_collider(g, S.BOX, (10.0, 10.0, 0.5))
+    for i in range(3):
+        b = sc.add_body(S.DYNAMIC, (3.0 * i - 3.0, 0.0, 1.0 + i))
+        sc.add_collider(b, S.BOX, (0.5, 0.5, 0.5))
+    return sc
+
+
+@pytest.mark.gpu
+def test_world_queries_through_the_cpp_mirror(ref_available):
+    """PhysicsWorld::testOverlap / testCollision / raycast (closest-hit and every-hit callbacks) of the C++
+    mirror between update() calls answer as the reference does, and the steps after them agree."""
+    from oracle.ref import RefWorld
+    if not os.path.exists(QUERIES_EXE):
+        pytest.skip("example not built (python -m reactphysics3d_amd.build)")
+    out = subprocess.run([QUERIES_EXE, "60", "30"], capture_output=True, text=True, timeout=120)
+    assert out.returncode == 0, out.stderr
+    lines = out.stdout.splitlines()
+    rows = lambda tag: [l.split()[1:] for l in lines if l.startswith(tag + " ")]
+    ref = RefWorld(_queries_scene())
+    for _ in range(60):
+        ref.step(1.0 / 60.0)
+    want_touch = [int(ref.test_overlap_bodies(1, 0)), int(ref.test_overlap_bodies(1, 2))]
+    assert [int(x) for x in rows("touching")[0]] == want_touch
+    o1 = ref.test_overlap(2)
+    o2 = ref.test_overlap(-1)
+    want_overlap = sorted({(int(a), int(b), int(e)) for a, b, e in zip(*o1)}) + \
+        sorted({(int(a), int(b), int(e)) for a, b, e in zip(*o2)})
+    got = [tuple(int(x) for x in r) for r in rows("overlap")]
+    n1 = len(o1[0])
+    assert sorted(got[:n1]) + sorted(got[n1:]) == want_overlap
+    c_all = ref.test_collision()
+    c_pair = ref.test_collision(3, 0)
+    got_c = rows("contact")
+    assert len(got_c) == len(c_all["colliders"]) + len(c_pair["colliders"])
+    want_c = {}
+    for ct in (c_all, c_pair):
+        for m in range(len(ct["colliders"])):
+            n = int(ct["nb_points"][m])
+            want_c.setdefault((int(ct["colliders"][m, 0]), int(ct["colliders"][m, 1]), int(ct["event"][m]), n), []).append(
+                ct["normal_depth"][m, :n].ravel())
+    for r in got_c:
+        key = tuple(int(x) for x in r[:4])
+        assert key in want_c, f"unexpected contact pair {key}"
+        vals = np.array(r[4:], np.float32)
+        assert any(np.allclose(vals, w, atol=1e-5) for w in want_c[key])
+    p1 = np.array([[-8.0, 0.0, 0.3]], np.float32)
+    p2 = np.array([[8.0, 0.0, 0.3]], np.float32)
+    cc, _, _, cf = ref.raycast(p1, p2)
+    got_closest = rows("closest")
+    assert len(got_closest) == 1 and int(got_closest[0][0]) == int(cc[0]) and np.float32(got_closest[0][1]) == cf[0]
+    ray, col, _, _, fr = ref.raycast_all(p1, p2)
+    got_hits = sorted((int(r[0]), float(np.float32(r[1]))) for r in rows("hit"))
+    assert got_hits == sorted((int(c), float(f)) for c, f in zip(col, fr))
+    for _ in range(30):
+        ref.step(1.0 / 60.0)
+    p, q, v, w = ref.bodies()
+    st = np.array([list(map(float, r[1:])) for r in rows("body")], np.float32)
+    err = np.max(np.abs(st - np.concatenate([p, q, v, w], 1)))
+    print("max |device - reference|", err)
+    assert err <= 1e-4

@@ -25,6 +25,14 @@ def _local(ct, c0, c1, key="colliders"):
     return out
 
 
+def _compare_query_contacts(mine, want, what):
+    errs = compare_contacts(mine, want)
+    assert not errs, what + ": " + "; ".join(errs[:5])
+    ev_mine = {(int(a), int(b), int(e)) for (a, b), e in zip(mine["colliders"], mine["event"])}
+    ev_want = {(int(a), int(b), int(e)) for (a, b), e in zip(want["colliders"], want["event"])}
+    assert ev_mine == ev_want, what + ": ContactStart / ContactStay differ"
+
+
 def _query(kind, dev, refs, rng, scenes):
     """One query of the given kind on the device and on every reference world (a query about bodies of
     world w leaves only the broadphase in the other worlds).  Returns the number of colliding pairs."""
@@ -57,16 +65,18 @@ def _query(kind, dev, refs, rng, scenes):
         return int(got)
     if kind in ("overlap_all", "overlap_body"):
         body = -1 if kind == "overlap_all" else b0 + b1
-        d1, d2 = dev.test_overlap(body)
+        d1, d2, de = dev.test_overlap(body)
         n = 0
         for wi, ref in enumerate(refs):
             if kind == "overlap_body" and wi != w:
                 ref.broadphase_only()
                 continue
-            r1, r2 = ref.test_overlap(-1 if kind == "overlap_all" else b1)
+            r1, r2, re = ref.test_overlap(-1 if kind == "overlap_all" else b1)
             cw0, cw1 = dev.collider_base[wi], dev.collider_base[wi + 1]
             sel = (d1 >= cw0) & (d1 < cw1)
-            assert pair_set(d1[sel] - cw0, d2[sel] - cw0) == pair_set(r1, r2), f"{kind} world {wi}: pairs differ"
+            mine = {(int(a) - cw0, int(b) - cw0, int(e)) for a, b, e in zip(d1[sel], d2[sel], de[sel])}
+            want = {(int(a), int(b), int(e)) for a, b, e in zip(r1, r2, re)}
+            assert mine == want, f"{kind} world {wi}: pairs or OverlapStart / OverlapStay differ"
             n += len(r1)
         return n
     if kind == "collision_all":
@@ -75,8 +85,7 @@ def _query(kind, dev, refs, rng, scenes):
         for wi, ref in enumerate(refs):
             want = ref.test_collision()
             mine = _local(ct, dev.collider_base[wi], dev.collider_base[wi + 1])
-            errs = compare_contacts(mine, want)
-            assert not errs, f"testCollision(callback) world {wi}: " + "; ".join(errs[:5])
+            _compare_query_contacts(mine, want, f"testCollision(callback) world {wi}")
             n += len(want["colliders"])
         return n
     body2 = -1 if kind == "collision_body" else b2
@@ -84,8 +93,7 @@ def _query(kind, dev, refs, rng, scenes):
     want = refs[w].test_collision(b1, body2)
     others()
     mine = _local(ct, dev.collider_base[w], dev.collider_base[w + 1])
-    errs = compare_contacts(mine, want)
-    assert not errs, f"{kind} ({b1}, {body2}) world {w}: " + "; ".join(errs[:5])
+    _compare_query_contacts(mine, want, f"{kind} ({b1}, {body2}) world {w}")
     return len(want["colliders"])
 
 
