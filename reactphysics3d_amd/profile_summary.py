@@ -14,6 +14,13 @@
            reads profiles/traffic_<workload>.json for the roofline's "traffic").
 --min-us KERNEL=US : count only the dispatches of KERNEL that ran at least US microseconds (repeatable;
            separates, e.g., residual-mode solver launches from fixed-iteration ones in one run).
+--pmc DB : database of any `rocprofv3 --pmc ...` run (repeatable); prints, for every counter it holds,
+           the mean value per dispatch of each kernel.  With --copy-bytes, a counter named in
+           --bytes-read / --bytes-write (repeatable) is also turned into bytes by the value it reads on
+           the copy kernel, which moves copy-bytes each way.
+--kernels K1,K2 : restrict the --pmc tables to these kernels (and the copy kernel).
+--top-dispatches KERNEL=N : count only the N dispatches of KERNEL with the largest value of each
+           counter (repeatable; for counter runs whose records carry no usable dispatch times).
 """
 from __future__ import annotations
 
@@ -44,6 +51,14 @@ def kernel_trace(db: str, min_us=None):
     return agg
 
 
+TOP = {}   # kernel -> number of largest dispatches kept (--top-dispatches)
+
+
+def _keep(k, vals):
+    n = TOP.get(k)
+    return sorted(vals)[-n:] if n else vals
+
+
 def counter_means(db: str, counter: str, min_us=None):
     c = sqlite3.connect(db)
     rows = c.execute("select kernel_name, value, start, end from counters_collection where counter_name = ?",
@@ -54,7 +69,25 @@ def counter_means(db: str, counter: str, min_us=None):
         if min_us and (t1 - t0) / 1e3 < min_us.get(k, 0.0):
             continue
         agg[k].append(float(v))
-    return {k: (sum(v) / len(v), len(v)) for k, v in agg.items()}
+    agg = {k: _keep(k, v) for k, v in agg.items()}
+    return {k: (sum(v) / len(v), len(v)) for k, v in agg.items() if v}
+
+
+def all_counters(db: str, min_us=None):
+    """{counter: {kernel: (mean per dispatch, dispatches)}} of every counter in a --pmc database."""
+    c = sqlite3.connect(db)
+    rows = c.execute("select counter_name, kernel_name, value, start, end from counters_collection").fetchall()
+    agg = defaultdict(lambda: defaultdict(list))
+    for cn, name, v, t0, t1 in rows:
+        k = short_name(name)
+        if min_us and (t1 - t0) / 1e3 < min_us.get(k, 0.0):
+            continue
+        agg[cn][k].append(float(v))
+    out = {}
+    for cn, ks in agg.items():
+        ks = {k: _keep(k, v) for k, v in ks.items()}
+        out[cn] = {k: (sum(v) / len(v), len(v)) for k, v in ks.items() if v}
+    return out
 
 
 def copy_kernel(means):
@@ -75,8 +108,14 @@ def main(argv=None):
     ap.add_argument("--top", type=int, default=25)
     ap.add_argument("--traffic-json", help="write {kernel: scaled HBM bytes per dispatch} to this file")
     ap.add_argument("--min-us", action="append", default=[], metavar="KERNEL=US")
+    ap.add_argument("--pmc", action="append", default=[], metavar="DB")
+    ap.add_argument("--bytes-read", action="append", default=[], metavar="COUNTER")
+    ap.add_argument("--bytes-write", action="append", default=[], metavar="COUNTER")
+    ap.add_argument("--kernels", default="")
+    ap.add_argument("--top-dispatches", action="append", default=[], metavar="KERNEL=N")
     a = ap.parse_args(argv)
     min_us = {kv.split("=")[0]: float(kv.split("=")[1]) for kv in a.min_us}
+    TOP.update({kv.split("=")[0]: int(kv.split("=")[1]) for kv in a.top_dispatches})
     if min_us:
         out_filter = ", ".join(f"{k} >= {v:g} us" for k, v in min_us.items())
     out = []
@@ -84,6 +123,9 @@ def main(argv=None):
         out += ["Command:", "", "```", a.command, "```", ""]
     if min_us:
         out += [f"Dispatches counted: {out_filter} (other kernels: all).", ""]
+    if TOP:
+        out += ["Dispatches counted: " + ", ".join(f"the {n} largest of {k}" for k, n in TOP.items()) +
+                " (other kernels: all).", ""]
     if a.trace:
         agg = kernel_trace(a.trace, min_us)
         tot = sum(sum(v) for v in agg.values()) or 1.0
@@ -127,6 +169,43 @@ def main(argv=None):
                                "fetch": a.fetch, "write": a.write, "copy_bytes": a.copy_bytes}
             with open(a.traffic_json, "w") as fh:
                 json.dump(tj, fh, indent=1, sort_keys=True)
+    for db in a.pmc:
+        counters = all_counters(db, min_us)
+        names = sorted(counters)
+        kernels = set()
+        for cn in names:
+            kernels |= set(counters[cn])
+        ck = copy_kernel({k: 0 for k in kernels})
+        want = [k for k in a.kernels.split(",") if k]
+        if want:
+            kernels = {k for k in kernels if k in want or k == ck}
+        first = counters[names[0]] if names else {}
+        order = sorted(kernels, key=lambda k: -(first.get(k, (0.0, 0))[0]))
+        out += [f"Counters of `{db}` (mean per dispatch):", ""]
+        out += ["| kernel | dispatches | " + " | ".join(names) + " |",
+                "|---|---:|" + "---:|" * len(names)]
+        for k in order[: a.top]:
+            disp = max(counters[cn].get(k, (0.0, 0))[1] for cn in names)
+            vals = []
+            for cn in names:
+                v = counters[cn].get(k)
+                vals.append(f"{v[0]:.6g}" if v else "")
+            out.append(f"| {k} | {disp} | " + " | ".join(vals) + " |")
+        out.append("")
+        scaled = [cn for cn in names if cn in a.bytes_read or cn in a.bytes_write]
+        if scaled and ck and a.copy_bytes > 0:
+            out += [f"Bytes per dispatch, each counter scaled by its value on `{ck}` "
+                    f"({a.copy_bytes / 2**20:.0f} MiB read and written):", ""]
+            out += ["| kernel | " + " | ".join(f"{cn} MB" for cn in scaled) + " |",
+                    "|---|" + "---:|" * len(scaled)]
+            for k in order[: a.top]:
+                vals = []
+                for cn in scaled:
+                    base = counters[cn].get(ck, (0.0, 0))[0]
+                    v = counters[cn].get(k)
+                    vals.append(f"{v[0] / base * a.copy_bytes / 1e6:.3f}" if v and base > 0 else "")
+                out.append(f"| {k} | " + " | ".join(vals) + " |")
+            out.append("")
     sys.stdout.write("\n".join(out) + "\n")
 
 
