@@ -43,3 +43,19 @@ def test_capsule_on_a_mesh_face_two_points(ref_available, depth):
     assert ct["nb_points"].tolist() == [2]
     assert np.allclose(ct["normal_depth"][0, :2, 3], depth, atol=1e-5)
     assert np.allclose(np.abs(ct["normal_depth"][0, :2, 2]), 1.0, atol=1e-6)   # the face normal, +-z
+
+
+def test_stacked_24gon_prisms_face_contact(ref_available):
+    """Two 24-gon prisms cap to cap, the top one turned by 0.07 rad and sunk 0.005 into the base: the
+    face contact clips a 24-vertex cap by 24 side planes (a 48-vertex polygon) and reduces it to 4 points
+    on the rim of the overlap, at depth 0.005 along the common axis."""
+    ct = _contacts(E.stacked_prisms())
+    assert ct["nb_points"].tolist() == [4]
+    nd = ct["normal_depth"][0]
+    assert np.allclose(np.abs(nd[:, 2]), 1.0, atol=1e-6)
+    assert np.allclose(nd[:, 3], 0.005, atol=2e-5)
+    # the base is collider 0 at the origin: its local points are on its top cap (z = 0.5), near the rim
+    local = ct["local1"][0] if int(ct["colliders"][0, 0]) == 0 else ct["local2"][0]
+    r = np.hypot(local[:, 0], local[:, 1])
+    assert np.allclose(local[:, 2], 0.5, atol=1e-5)
+    assert np.all(r <= 1.0 + 1e-5) and np.all(r >= 0.95)
