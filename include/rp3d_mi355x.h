@@ -425,6 +425,11 @@ int rp3dg_batch_autotune(rp3dg_batch* batch, float time_step, int32_t steps_per_
  * or 16; per-colour layout: threads per block, a multiple of 64 in 64..512), e.g. to time two batches
  * with the same shape.  Drops captured step graphs. */
 int rp3dg_batch_set_solver_shape(rp3dg_batch* batch, int32_t choice);
+/* Fused layout, residual mode: each step dispatches the workgroups of a size class longest expected
+ * first (the largest sweep count of a group's worlds in the previous step), so groups that stop early
+ * fill the tail (by_sweeps = 1, the default), or in the static order of group sizes (0).  Results do not
+ * depend on it. */
+int rp3dg_batch_set_group_order(rp3dg_batch* batch, int32_t by_sweeps);
 /* The trials of the last rp3dg_batch_autotune call: up to capacity (choice, ms per step) pairs;
  * returns the number of trials. */
 int32_t rp3dg_batch_get_tune_table(rp3dg_batch* batch, int32_t* choices, double* ms_per_step, int32_t capacity);

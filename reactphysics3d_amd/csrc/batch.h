@@ -72,6 +72,10 @@ struct rp3dg_batch {
     // the group of each workgroup, largest groups first)
     int ngroups = 0, group_bodies = 256;
     std::vector<int> group_world;        // [ngroups + 1] first world of each group
+    int* d_block_static = nullptr;       // the size-sorted group order (d_block_world is re-ranked per step in
+                                         // residual mode: longest expected group first)
+    int2* d_class_ranges = nullptr;      // (first, count) of every world class in that order
+    bool order_by_sweeps = true;
     std::vector<int> world_group;        // [nworlds]
     int *d_gworld = nullptr, *d_world_group = nullptr;
     int* d_wmstart = nullptr;            // [nworlds + 1] first manifold of each world (fused colouring)

@@ -137,6 +137,38 @@ __global__ void qr_carry(DevPairs N, DevPairs O, const int* n_new) {
     }
 }
 
+// Dispatch order of the fused solver's workgroups inside one size class: longest expected group first
+// (list scheduling), the expected length being the largest sweep count of the group's worlds in the
+// previous step, ties kept in the static (size) order.  One workgroup per class; n <= kOrderMax.
+constexpr int kOrderMax = 4096;
+__global__ void sv_order_groups(const int* static_order, const int* gworld, const int* world_iters, int* block_world,
+                                const int2* ranges, int nclass) {
+    __shared__ int s_key[kOrderMax];
+    const int c = blockIdx.x;
+    if (c >= nclass) return;
+    const int first = ranges[c].x, n = ranges[c].y;
+    if (n > kOrderMax) {   // too many to rank here: the static order
+        for (int i = threadIdx.x; i < n; i += blockDim.x) block_world[first + i] = static_order[first + i];
+        return;
+    }
+    for (int i = threadIdx.x; i < n; i += blockDim.x) {
+        const int g = static_order[first + i];
+        int m = 0;
+        for (int w = gworld[g]; w < gworld[g + 1]; w++) m = max(m, world_iters[w]);
+        s_key[i] = m;
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < n; i += blockDim.x) {
+        const int ki = s_key[i];
+        int rank = 0;
+        for (int j = 0; j < n; j++) {
+            const int kj = s_key[j];
+            rank += (kj > ki) || (kj == ki && j < i);
+        }
+        block_world[first + rank] = static_order[first + i];
+    }
+}
+
 __global__ void fill_int_n(int* p, int n, const int* nptr, int v) {
     n = min(n, *nptr);
     RP3DG_GRID_STRIDE(i, n) p[i] = v;
@@ -793,6 +825,9 @@ int rp3dg_batch::upload(rp3dg_world* const* ws, int nw, const rp3dg_batch_option
         HIPCHECK(up(d_wbody, body_base.data(), sizeof(int) * (nw + 1)));
         HIPCHECK(up(d_wcoll, collider_base.data(), sizeof(int) * (nw + 1)));
         HIPCHECK(hipMemcpy(d_block_world, order.data(), sizeof(int) * ng, hipMemcpyHostToDevice));
+        d_block_static = dalloc<int>(this, ng);
+        if (!d_block_static) { set_error("device allocation failed (group order)"); return RP3DG_EHIP; }
+        HIPCHECK(hipMemcpy(d_block_static, order.data(), sizeof(int) * ng, hipMemcpyHostToDevice));
         HIPCHECK(hipMemcpy(d_gworld, group_world.data(), sizeof(int) * (ng + 1), hipMemcpyHostToDevice));
         HIPCHECK(hipMemcpy(d_world_group, world_group.data(), sizeof(int) * nw, hipMemcpyHostToDevice));
         // size classes over the size-sorted group order (largest class first)
@@ -809,6 +844,11 @@ int rp3dg_batch::upload(rp3dg_world* const* ws, int nw, const rp3dg_batch_option
                 i = j;
             }
             set_class_blocks();
+            std::vector<int2> cr;
+            for (const WorldClass& wc : wclasses) cr.push_back(make_int2(wc.first, wc.count));
+            d_class_ranges = dalloc<int2>(this, std::max<size_t>(1, cr.size()));
+            if (!d_class_ranges) { set_error("device allocation failed (class ranges)"); return RP3DG_EHIP; }
+            HIPCHECK(hipMemcpy(d_class_ranges, cr.data(), sizeof(int2) * cr.size(), hipMemcpyHostToDevice));
             for (size_t c = 1; c < wclasses.size(); c++) {
                 hipStream_t s;
                 HIPCHECK(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
@@ -1161,6 +1201,9 @@ int rp3dg_batch::enqueue_step(float dt, bool capturing) {
         F.lds_residual = 0;
         F.feats = nullptr;
         F.phase = 2;
+        if (res_thr > 0.0f && order_by_sweeps && !wclasses.empty())   // residual mode: groups differ in length
+            LAUNCH_TS("sv_order_groups", (int)wclasses.size(), 256, 0, sv_order_groups, d_block_static, d_gworld,
+                      world_iters, d_block_world, d_class_ranges, (int)wclasses.size());
         if (!features) {
             r = launch_world_classes("sv_world_solve", F, true);
             if (r) return r;
@@ -2313,6 +2356,16 @@ int rp3dg_batch_autotune(rp3dg_batch* b, float dt, int32_t steps_per_trial, int3
     b->profiling = prof;
     if (chosen) *chosen = best;
     if (best_ms) *best_ms = best_t;
+    return RP3DG_OK;
+}
+
+int rp3dg_batch_set_group_order(rp3dg_batch* b, int32_t by_sweeps) {
+    if (!b) return RP3DG_EINVAL;
+    if ((by_sweeps != 0) != b->order_by_sweeps) b->drop_graphs();
+    b->order_by_sweeps = by_sweeps != 0;
+    if (!b->order_by_sweeps && b->d_block_static && b->ngroups)   // back to the static order
+        HIPCHECK(hipMemcpyAsync(b->d_block_world, b->d_block_static, sizeof(int) * b->ngroups, hipMemcpyDeviceToDevice,
+                                b->stream));
     return RP3DG_OK;
 }
 

@@ -86,6 +86,7 @@ EXPORTED = [
     "rp3dg_body_reset_force_torque", "rp3dg_body_get_force_torque", "rp3dg_body_set_sleeping",
     "rp3dg_batch_raycast_all", "rp3dg_batch_test_overlap_bodies", "rp3dg_batch_test_overlap",
     "rp3dg_batch_test_collision", "rp3dg_batch_get_query_results", "rp3dg_world_upload",
+    "rp3dg_batch_set_group_order",
 ]
 
 # RigidBody call codes of Batch.body_ops (the order of the C-ABI's per-body calls)
@@ -184,6 +185,7 @@ def load():
         "rp3dg_batch_test_collision": (C.c_int64, [vp, i32, i32]),
         "rp3dg_batch_get_query_results": (C.c_int64, [vp, C.POINTER(ContactView), C.c_int64]),
         "rp3dg_world_upload": (vp, [vp]),
+        "rp3dg_batch_set_group_order": (C.c_int, [vp, i32]),
         "rp3dg_batch_raycast_all": (C.c_int64, [vp, C.c_int64, iptr, fptr, fptr, fptr, C.c_uint16, iptr, iptr, fptr, fptr,
                                                 fptr, C.c_int64]),
         "rp3dg_body_set_transform": (C.c_int, [vp, i32, fptr, fptr]),
@@ -593,6 +595,9 @@ class Batch:
         _check(load().rp3dg_batch_autotune(self.h, float(dt), int(steps_per_trial), C.byref(c), C.byref(ms)),
                "autotune")
         return c.value, ms.value
+
+    def set_group_order(self, by_sweeps=True):
+        _check(load().rp3dg_batch_set_group_order(self.h, int(by_sweeps)), "set_group_order")
 
     def set_solver_shape(self, choice):
         _check(load().rp3dg_batch_set_solver_shape(self.h, int(choice)), "set_solver_shape")

@@ -108,12 +108,14 @@ def test_residual_mode_converges():
 def test_world_groups_do_not_change_results(residual):
     """Packing worlds into one workgroup changes nothing: per-world results, iteration counts and
     residuals are identical to one world per workgroup, in fixed and residual mode (a converged world
-    of a group skips the sweeps its neighbours still run)."""
+    of a group skips the sweeps its neighbours still run), and so does the order the workgroups are
+    dispatched in (by the previous step's sweep counts, or by size)."""
     from reactphysics3d_amd.lib import Batch
     scenes = [S.batched_world(w, seed=21, min_bodies=8, max_bodies=160, max_joints=8) for w in range(40)]
     runs = []
-    for gb in (1, 128, 512):
+    for gb, by_sweeps in ((1, True), (128, True), (512, True), (128, False)):
         dev = Batch(scenes, solver="fused", group_bodies=gb)
+        dev.set_group_order(by_sweeps)   # workgroup dispatch order: longest expected first, or static
         if residual > 0:
             dev.set_residual_mode(residual, 150)
         stats = []
