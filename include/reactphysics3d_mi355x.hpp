@@ -69,6 +69,8 @@ class Transform {
 };
 
 enum class BodyType { STATIC, KINEMATIC, DYNAMIC };
+// configuration.h:101-109
+enum class ContactsPositionCorrectionTechnique { BAUMGARTE_CONTACTS, SPLIT_IMPULSES };
 
 // ---- shapes (collision/shapes/*.h): parameter holders, one collider each ----
 class CollisionShape {
@@ -608,6 +610,13 @@ class PhysicsWorld {
     }
     void setNbIterationsVelocitySolver(uint16 n) { mVel = n; check(rp3dg_world_set_iterations(mWorld, mVel, mPos), "iterations"); }
     void setNbIterationsPositionSolver(uint16 n) { mPos = n; check(rp3dg_world_set_iterations(mWorld, mVel, mPos), "iterations"); }
+    // PhysicsWorld::setContactsPositionCorrectionTechnique (PhysicsWorld.h:599-607)
+    void setContactsPositionCorrectionTechnique(ContactsPositionCorrectionTechnique t) {
+        check(rp3dg_world_set_contacts_position_correction(mWorld, t == ContactsPositionCorrectionTechnique::BAUMGARTE_CONTACTS
+                                                                       ? RP3DG_CONTACTS_BAUMGARTE
+                                                                       : RP3DG_CONTACTS_SPLIT_IMPULSES),
+              "setContactsPositionCorrectionTechnique");
+    }
     void setGravity(const Vector3& g) {
         const float v[3] = {g.x, g.y, g.z};
         check(rp3dg_world_set_gravity(mWorld, v), "setGravity");

@@ -82,6 +82,14 @@ void rp3dg_world_destroy(rp3dg_world* world);
 int rp3dg_world_set_iterations(rp3dg_world* world, uint32_t velocity_iterations, uint32_t position_iterations);
 /* PhysicsWorld::setGravity (src/engine/PhysicsWorld.cpp:969). */
 int rp3dg_world_set_gravity(rp3dg_world* world, const float gravity[3]);
+/* PhysicsWorld::setContactsPositionCorrectionTechnique (PhysicsWorld.h:599-607,
+ * ContactSolverSystem.cpp:486-590, DynamicsSystem.cpp:44-71): RP3DG_CONTACTS_SPLIT_IMPULSES (the default)
+ * or RP3DG_CONTACTS_BAUMGARTE (the penetration bias joins the normal impulse; no split velocities).  Any
+ * time; worlds of one batch share it (rp3dg_batch_set_contacts_position_correction). */
+#define RP3DG_CONTACTS_BAUMGARTE 0
+#define RP3DG_CONTACTS_SPLIT_IMPULSES 1
+int rp3dg_world_set_contacts_position_correction(rp3dg_world* world, int32_t technique);
+int rp3dg_batch_set_contacts_position_correction(rp3dg_batch* batch, int32_t technique);
 
 /* PhysicsWorld::createRigidBody (src/engine/PhysicsWorld.cpp:401-442); returns the body index. */
 int32_t rp3dg_body_create(rp3dg_world* world, const float position[3], const float orientation[4]);

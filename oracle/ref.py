@@ -158,6 +158,11 @@ class RefWorld:
         return col, pt, nr, fr
 
 
+    def set_contacts_baumgarte(self, on=True):
+        """PhysicsWorld::setContactsPositionCorrectionTechnique(BAUMGARTE_CONTACTS or SPLIT_IMPULSES)."""
+        self.L.rp3dref_set_contacts_baumgarte.argtypes = [C.c_void_p, C.c_int]
+        self.L.rp3dref_set_contacts_baumgarte(self.h, int(bool(on)))
+
     def test_overlap_bodies(self, b1, b2):
         """PhysicsWorld::testOverlap(body1, body2) -> bool."""
         self.L.rp3dref_test_overlap_bodies.argtypes = [C.c_void_p, C.c_int, C.c_int]

@@ -714,6 +714,14 @@ int rp3dref_raycast_all(void* h, int n, const float* p1, const float* p2, const 
     return total;
 }
 
+// PhysicsWorld::setContactsPositionCorrectionTechnique: 1 Baumgarte contacts, 0 split impulses.
+int rp3dref_set_contacts_baumgarte(void* h, int baumgarte) {
+    RefWorld* R = (RefWorld*)h;
+    R->world->setContactsPositionCorrectionTechnique(baumgarte ? ContactsPositionCorrectionTechnique::BAUMGARTE_CONTACTS
+                                                               : ContactsPositionCorrectionTechnique::SPLIT_IMPULSES);
+    return 0;
+}
+
 // World queries.  rp3dref_test_overlap_bodies: PhysicsWorld::testOverlap(body1, body2).
 int rp3dref_test_overlap_bodies(void* h, int b1, int b2) {
     RefWorld* R = (RefWorld*)h;

@@ -589,7 +589,8 @@ int rp3dg_batch::upload(rp3dg_world* const* ws, int nw, const rp3dg_batch_option
             settings.sleep_lin_sq = cfg.sleep_linear_velocity * cfg.sleep_linear_velocity;
             settings.sleep_ang_sq = cfg.sleep_angular_velocity * cfg.sleep_angular_velocity;
             settings.time_before_sleep = cfg.time_before_sleep;
-        } else if (memcmp(&cfg, &ws[0]->cfg, sizeof(cfg)) != 0) {
+            settings.split_impulse = W.split_impulse ? 1 : 0;
+        } else if (memcmp(&cfg, &ws[0]->cfg, sizeof(cfg)) != 0 || W.split_impulse != ws[0]->split_impulse) {
             set_error("all worlds of a batch must share the same WorldSettings");
             return RP3DG_EINVAL;
         }
@@ -1170,6 +1171,7 @@ int rp3dg_batch::enqueue_step(float dt, bool capturing) {
     SA.restitution_threshold = settings.restitution_threshold;
     SA.learned = learned ? 1 : 0;
     SA.adj_offs = adj_offs;
+    SA.split = settings.split_impulse;
     const float beta_dt = 0.2f / dt;
 
     if (fused) {
@@ -1256,7 +1258,7 @@ int rp3dg_batch::enqueue_step(float dt, bool capturing) {
     }
     LAUNCH_T("sv_store", mg, 256, sv_store, SA);
     // position integration, joint position correction, state update, refit (PhysicsWorld.cpp:275-288, :348-360)
-    LAUNCH_T("dyn_integrate_positions", bg, 256, dyn_integrate_positions, B, dt);
+    LAUNCH_T("dyn_integrate_positions", bg, 256, dyn_integrate_positions, B, dt, settings.split_impulse ? 1.0f : 0.0f);
     for (int it = 0; it < settings.pos_iters; it++)
         for (int s = 0; s < jt_slots; s++)
             LAUNCH_T("jt_solve_position", jg, 256, jt_solve_position, B, J, jt_ranges, s);
@@ -2356,6 +2358,18 @@ int rp3dg_batch_autotune(rp3dg_batch* b, float dt, int32_t steps_per_trial, int3
     b->profiling = prof;
     if (chosen) *chosen = best;
     if (best_ms) *best_ms = best_t;
+    return RP3DG_OK;
+}
+
+int rp3dg_batch_set_contacts_position_correction(rp3dg_batch* b, int32_t technique) {
+    if (!b) return RP3DG_EINVAL;
+    if (technique != RP3DG_CONTACTS_BAUMGARTE && technique != RP3DG_CONTACTS_SPLIT_IMPULSES) {
+        set_error("unknown contacts position correction technique");
+        return RP3DG_EINVAL;
+    }
+    const int split = technique == RP3DG_CONTACTS_SPLIT_IMPULSES ? 1 : 0;
+    if (split != b->settings.split_impulse) b->drop_graphs();
+    b->settings.split_impulse = split;
     return RP3DG_OK;
 }
 

@@ -277,7 +277,8 @@ __global__ __launch_bounds__(kFusedMaxBlock) void world_position(FusedArgs F) {
     const int c0 = F.wcoll[w0], c1 = F.wcoll[w1];
     const int2* jr = F.jt_wr + (size_t)g * F.jt_slots;
     // PhysicsWorld.cpp:275-288: integrate positions, joint position correction, update body state
-    for (int i = b0 + tid; i < b1; i += nt) dyn_integrate_positions_one(B, F.dt, i);
+    const float split_factor = F.settings.split_impulse ? 1.0f : 0.0f;
+    for (int i = b0 + tid; i < b1; i += nt) dyn_integrate_positions_one(B, F.dt, i, split_factor);
     __syncthreads();
     for (int it = 0; it < F.settings.pos_iters; it++) {
         for (int s = 0; s < F.jt_slots; s++) {

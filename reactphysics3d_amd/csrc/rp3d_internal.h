@@ -182,6 +182,7 @@ struct Settings {
     int sleeping_enabled;
     float sleep_lin_sq, sleep_ang_sq;   // squared sleep velocities (PhysicsWorld.cpp:875-876)
     float time_before_sleep;
+    int split_impulse;                  // contacts: SPLIT_IMPULSES (1, the default) or BAUMGARTE_CONTACTS (0)
 };
 
 // Counters written by kernels (one device block, read back after each step).

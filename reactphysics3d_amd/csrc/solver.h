@@ -20,10 +20,11 @@ struct SvArgs {
     float restitution_threshold;
     int learned;          // learned warm start for points without a cached impulse
     const int* adj_offs;  // body -> manifold adjacency offsets (manifold count of a dynamic body)
+    int split;            // split impulses (1) or Baumgarte contacts (0), PhysicsWorld.h:599-607
 };
 
 __global__ void dyn_integrate_velocities(DevBodies B, Settings S, float dt);
-__global__ void dyn_integrate_positions(DevBodies B, float dt);
+__global__ void dyn_integrate_positions(DevBodies B, float dt, float split_factor);
 __global__ void dyn_update_state(DevBodies B);
 __global__ void sv_flag_manifolds(const int* ncp, const int* nb_pairs, int* flag);
 __global__ void sv_compact_manifolds(const int* flag, const int* offs, const int* nb_pairs, const int* ncp, int* list,

@@ -23,8 +23,8 @@ __global__ void dyn_integrate_velocities(DevBodies B, Settings S, float dt) {
     RP3DG_GRID_STRIDE(i, B.n) dyn_integrate_velocities_one(B, S, dt, i);
 }
 
-__global__ void dyn_integrate_positions(DevBodies B, float dt) {
-    RP3DG_GRID_STRIDE(i, B.n) dyn_integrate_positions_one(B, dt, i);
+__global__ void dyn_integrate_positions(DevBodies B, float dt, float split_factor) {
+    RP3DG_GRID_STRIDE(i, B.n) dyn_integrate_positions_one(B, dt, i, split_factor);
 }
 
 __global__ void dyn_update_state(DevBodies B) {

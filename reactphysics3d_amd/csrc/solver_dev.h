@@ -79,12 +79,13 @@ __device__ __forceinline__ void dyn_integrate_velocities_one(const DevBodies& B,
     }
 }
 
-// integrateRigidBodiesPositions (DynamicsSystem.cpp:44-71), split impulses active
-__device__ __forceinline__ void dyn_integrate_positions_one(const DevBodies& B, float dt, int i) {
+// integrateRigidBodiesPositions (DynamicsSystem.cpp:44-71): the split velocities enter with the factor
+// 1 (split impulses) or 0 (Baumgarte contacts), multiplied as the reference does
+__device__ __forceinline__ void dyn_integrate_positions_one(const DevBodies& B, float dt, int i, float split_factor) {
     {
         if (!(B.flags[i] & BF_ENABLED)) return;
-        const V3 nv = ld3(B.cv, i) + 1.0f * ld3(B.sv, i);
-        const V3 nw = ld3(B.cw, i) + 1.0f * ld3(B.sw, i);
+        const V3 nv = ld3(B.cv, i) + split_factor * ld3(B.sv, i);
+        const V3 nw = ld3(B.cw, i) + split_factor * ld3(B.sw, i);
         const V3 x = ld3(B.x, i);
         const Q4 q = ldq(B.q, i);
         const V3 cx = x + nv * dt;
@@ -366,7 +367,8 @@ __device__ __forceinline__ void sv_point_features(const SvArgs& A, int i, int p,
     f[23] = 1.0f / (float)S.npts[i];
 }
 
-// ContactSolverSystem::solve (CS.cpp:479-759), split impulses active, for one colour
+// ContactSolverSystem::solve (CS.cpp:479-759), for one colour: with split impulses the penetration bias
+// drives the split row; with Baumgarte contacts it joins the restitution bias in the main row
 // LK = false: both bodies have unit lock factors, and every multiplication by a factor (x * 1 = x
 // exactly) is left out; LK = true multiplies as the reference does.
 template <bool LK>
@@ -401,7 +403,7 @@ __device__ __forceinline__ void solve_manifold_t(const SvArgs& A, int i, float b
         const float Jv = dv.x * n.x + dv.y * n.y + dv.z * n.z;
         float biasPen = 0.0f;
         if (pn.w > 0.01f) biasPen = -(beta_dt) * fmax_s(0.0f, pn.w - 0.01f);
-        float dl = -(Jv + rb) * ipm;
+        float dl = A.split ? -(Jv + rb) * ipm : -(Jv + (biasPen + rb)) * ipm;
         float lt = lam.x;
         lam.x = fmax_s(lam.x + dl, 0.0f);
         dl = lam.x - lt;
@@ -411,6 +413,10 @@ __device__ __forceinline__ void solve_manifold_t(const SvArgs& A, int i, float b
         v2.x += lk<LK>(im2 * li.x, ll2.x); v2.y += lk<LK>(im2 * li.y, ll2.y); v2.z += lk<LK>(im2 * li.z, ll2.z);
         w2.x += lk<LK>(i2n.x, al2.x) * dl; w2.y += lk<LK>(i2n.y, al2.y) * dl; w2.z += lk<LK>(i2n.z, al2.z) * dl;
         sumPen += lam.x;
+        if (!A.split) {
+            S.lam[q] = lam;
+            continue;
+        }
         // split impulse row
         const V3 dvs = rel_vel(sv1, sw1, sv2, sw2, r1, r2);
         const float Jvs = dvs.x * n.x + dvs.y * n.y + dvs.z * n.z;

@@ -268,6 +268,21 @@ int rp3dg_world_set_iterations(rp3dg_world* w, uint32_t vi, uint32_t pi) {
     return RP3DG_OK;
 }
 
+// PhysicsWorld::setContactsPositionCorrectionTechnique (PhysicsWorld.h:599-607)
+int rp3dg_world_set_contacts_position_correction(rp3dg_world* w, int32_t technique) {
+    if (!w) return RP3DG_EINVAL;
+    if (technique != RP3DG_CONTACTS_BAUMGARTE && technique != RP3DG_CONTACTS_SPLIT_IMPULSES) {
+        set_error("unknown contacts position correction technique");
+        return RP3DG_EINVAL;
+    }
+    if (w->uploaded && !w->own) {
+        set_error("world belongs to a shared batch; use rp3dg_batch_set_contacts_position_correction");
+        return RP3DG_ESTATE;
+    }
+    w->split_impulse = technique == RP3DG_CONTACTS_SPLIT_IMPULSES;
+    return w->own ? rp3dg_batch_set_contacts_position_correction(w->own, technique) : RP3DG_OK;
+}
+
 int rp3dg_world_set_gravity(rp3dg_world* w, const float g[3]) {
     if (!w || !g) return RP3DG_EINVAL;
     w->cfg.gravity[0] = g[0]; w->cfg.gravity[1] = g[1]; w->cfg.gravity[2] = g[2];

@@ -75,6 +75,7 @@ struct HostWorld {
     std::vector<HostJoint> joints;
     rp3dg_batch* own = nullptr;   // device batch of one, created on first update()
     bool events = false;          // record contact events in the own batch
+    bool split_impulse = true;    // ContactsPositionCorrectionTechnique::SPLIT_IMPULSES (the default)
     bool uploaded = false;
     // host mirror of the device body state, refreshed lazily after update()
     std::vector<float> mirror_pos, mirror_q, mirror_v, mirror_w;

@@ -86,7 +86,8 @@ EXPORTED = [
     "rp3dg_body_reset_force_torque", "rp3dg_body_get_force_torque", "rp3dg_body_set_sleeping",
     "rp3dg_batch_raycast_all", "rp3dg_batch_test_overlap_bodies", "rp3dg_batch_test_overlap",
     "rp3dg_batch_test_collision", "rp3dg_batch_get_query_results", "rp3dg_world_upload",
-    "rp3dg_batch_set_group_order",
+    "rp3dg_batch_set_group_order", "rp3dg_world_set_contacts_position_correction",
+    "rp3dg_batch_set_contacts_position_correction",
 ]
 
 # RigidBody call codes of Batch.body_ops (the order of the C-ABI's per-body calls)
@@ -186,6 +187,8 @@ def load():
         "rp3dg_batch_get_query_results": (C.c_int64, [vp, C.POINTER(ContactView), C.c_int64]),
         "rp3dg_world_upload": (vp, [vp]),
         "rp3dg_batch_set_group_order": (C.c_int, [vp, i32]),
+        "rp3dg_world_set_contacts_position_correction": (C.c_int, [vp, i32]),
+        "rp3dg_batch_set_contacts_position_correction": (C.c_int, [vp, i32]),
         "rp3dg_batch_raycast_all": (C.c_int64, [vp, C.c_int64, iptr, fptr, fptr, fptr, C.c_uint16, iptr, iptr, fptr, fptr,
                                                 fptr, C.c_int64]),
         "rp3dg_body_set_transform": (C.c_int, [vp, i32, fptr, fptr]),
@@ -595,6 +598,10 @@ class Batch:
         _check(load().rp3dg_batch_autotune(self.h, float(dt), int(steps_per_trial), C.byref(c), C.byref(ms)),
                "autotune")
         return c.value, ms.value
+
+    def set_contacts_baumgarte(self, on=True):
+        """PhysicsWorld::setContactsPositionCorrectionTechnique: BAUMGARTE_CONTACTS (on) or SPLIT_IMPULSES."""
+        _check(load().rp3dg_batch_set_contacts_position_correction(self.h, 0 if on else 1), "contacts technique")
 
     def set_group_order(self, by_sweeps=True):
         _check(load().rp3dg_batch_set_group_order(self.h, int(by_sweeps)), "set_group_order")

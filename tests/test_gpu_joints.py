@@ -12,12 +12,14 @@ pytestmark = pytest.mark.gpu
 
 # The contract for body state is 1e-4; the device joint rows keep the reference's operation order and
 # its C library's acos / atan2 bits, so these scenes hold the stronger bit-for-bit equality.
-def _run(sc, steps, tol=0.0, solver="auto"):
+def _run(sc, steps, tol=0.0, solver="auto", setup=None):
     from oracle.ref import RefWorld
     from reactphysics3d_amd.lib import Batch
     dev = Batch(sc, solver=solver)
     ref = RefWorld(sc)
     ref.set_joint_order(dev.joint_order())
+    if setup:
+        setup(dev, ref)
     dt = 1.0 / 60.0
     worst = 0.0
     for k in range(steps):
