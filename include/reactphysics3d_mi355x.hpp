@@ -69,7 +69,8 @@ class Transform {
 };
 
 enum class BodyType { STATIC, KINEMATIC, DYNAMIC };
-// configuration.h:101-109
+// configuration.h:95-109
+enum class JointsPositionCorrectionTechnique { BAUMGARTE_JOINTS, NON_LINEAR_GAUSS_SEIDEL };
 enum class ContactsPositionCorrectionTechnique { BAUMGARTE_CONTACTS, SPLIT_IMPULSES };
 
 // ---- shapes (collision/shapes/*.h): parameter holders, one collider each ----
@@ -377,6 +378,7 @@ struct JointInfo {
     RigidBody* body2;
     Vector3 anchorPointWorldSpace;
     bool isCollisionEnabled = true;
+    JointsPositionCorrectionTechnique positionCorrectionTechnique = JointsPositionCorrectionTechnique::NON_LINEAR_GAUSS_SEIDEL;
 };
 struct BallAndSocketJointInfo : JointInfo {
     BallAndSocketJointInfo(RigidBody* b1, RigidBody* b2, const Vector3& anchor) : JointInfo{b1, b2, anchor} {}
@@ -832,7 +834,11 @@ class PhysicsWorld {
         joint->mMotor = motor;
         const float l[4] = {joint->mLim[0], joint->mLim[1], joint->mLim[2], joint->mLim[3]};
         const int32_t j = rp3dg_joint_create(mWorld, type, i.body1->index(), i.body2->index(), a, ax,
-                                             i.isCollisionEnabled ? 1 : 0, (limit ? 1 : 0) | (motor ? 2 : 0), l);
+                                             i.isCollisionEnabled ? 1 : 0,
+                                             (limit ? 1 : 0) | (motor ? 2 : 0) |
+                                                 (i.positionCorrectionTechnique == JointsPositionCorrectionTechnique::BAUMGARTE_JOINTS
+                                                      ? RP3DG_JOINT_BAUMGARTE : 0),
+                                             l);
         check(j, "createJoint");
         joint->mWorld = mWorld;
         joint->mIndex = j;

@@ -49,6 +49,9 @@ extern "C" {
 #define RP3DG_JOINT_SLIDER 1
 #define RP3DG_JOINT_HINGE 2
 #define RP3DG_JOINT_FIXED 3
+/* rp3dg_joint_create flags bit2: JointInfo::positionCorrectionTechnique = BAUMGARTE_JOINTS
+ * (include/reactphysics3d/constraint/Joint.h:66, configuration.h:101); clear: NON_LINEAR_GAUSS_SEIDEL */
+#define RP3DG_JOINT_BAUMGARTE 4
 
 typedef struct rp3dg_world rp3dg_world;
 typedef struct rp3dg_batch rp3dg_batch;
@@ -186,7 +189,12 @@ int rp3dg_quickhull(const void* vertices, int32_t nb_vertices, int32_t stride, i
 /* PhysicsWorld::createJoint (src/engine/PhysicsWorld.cpp:483-600) with anchor / axis given in world space
  * (the JointInfo "world space" constructors, e.g. BallAndSocketJointInfo(body1, body2, anchor)).
  * axis is used by hinge and slider; limits[4] = {min, max, motor speed, max motor force/torque}
- * with flags bit0 limit enabled, bit1 motor enabled (HingeJointInfo / SliderJointInfo fields).
+ * with flags bit0 limit enabled, bit1 motor enabled (HingeJointInfo / SliderJointInfo fields) and
+ * RP3DG_JOINT_BAUMGARTE (bit2) for JointInfo::positionCorrectionTechnique = BAUMGARTE_JOINTS: the position
+ * error enters the velocity rows as a bias of 0.2 / dt times the error and the joint gets no non-linear
+ * Gauss-Seidel position correction (Solve*JointSystem::initBeforeSolve / solvePositionConstraint).  As in
+ * the reference, the sliders of a world solved after a Baumgarte slider get no position correction either
+ * (SolveSliderJointSystem.cpp:527 returns).
  * Ball-and-socket: flags bit0 enables the cone limit with half angle limits[0] in [0, pi]
  * (BallAndSocketJoint::enableConeLimit / setConeLimitHalfAngle, BallAndSocketJoint.cpp:71-96).
  * Slider limits must satisfy min <= 0 <= max and the max motor force must be >= 0
@@ -194,8 +202,8 @@ int rp3dg_quickhull(const void* vertices, int32_t nb_vertices, int32_t stride, i
 int32_t rp3dg_joint_create(rp3dg_world* world, int32_t type, int32_t body1, int32_t body2,
                            const float anchor_world[3], const float axis_world[3], int32_t collide_connected,
                            int32_t flags, const float limits[4]);
-/* Replaces the limit / motor state of a joint before the world's first update (flags and limits as in
- * rp3dg_joint_create; RP3DG_ESTATE afterwards): BallAndSocketJoint::enableConeLimit /
+/* Replaces the limit / motor state of a joint before the world's first update (flags bits 0-1 and limits
+ * as in rp3dg_joint_create, the position correction technique stays; RP3DG_ESTATE afterwards): BallAndSocketJoint::enableConeLimit /
  * setConeLimitHalfAngle (BallAndSocketJoint.cpp:71-96), HingeJoint::enableLimit / enableMotor /
  * setMinAngleLimit / setMaxAngleLimit / setMotorSpeed / setMaxMotorTorque (HingeJoint.cpp:90-195),
  * SliderJoint::enableLimit / enableMotor / setMinTranslationLimit / setMaxTranslationLimit /

@@ -65,7 +65,7 @@ struct RefJoint {
     float anchor[3];
     float axis[3];
     int32_t collide;
-    int32_t flags;    // bit0 limit, bit1 motor
+    int32_t flags;    // bit0 limit, bit1 motor, bit2 Baumgarte position correction
     float limits[4];  // min, max, motor speed, max motor force/torque
 };
 
@@ -211,6 +211,7 @@ void* rp3dref_create(const RefSettings* s, int nb, const RefBody* bodies, int nc
         if (J.type == 0) {
             BallAndSocketJointInfo info(b1, b2, anchor);
             info.isCollisionEnabled = J.collide != 0;
+            if (J.flags & 4) info.positionCorrectionTechnique = JointsPositionCorrectionTechnique::BAUMGARTE_JOINTS;
             joint = R->world->createJoint(info);
             if (J.flags & 1) {   // cone limit with half angle limits[0]
                 BallAndSocketJoint* bs = static_cast<BallAndSocketJoint*>(joint);
@@ -220,18 +221,21 @@ void* rp3dref_create(const RefSettings* s, int nb, const RefBody* bodies, int nc
         } else if (J.type == 1) {
             SliderJointInfo info(b1, b2, anchor, axis);
             info.isCollisionEnabled = J.collide != 0;
+            if (J.flags & 4) info.positionCorrectionTechnique = JointsPositionCorrectionTechnique::BAUMGARTE_JOINTS;
             if (J.flags & 1) { info.isLimitEnabled = true; info.minTranslationLimit = J.limits[0]; info.maxTranslationLimit = J.limits[1]; }
             if (J.flags & 2) { info.isMotorEnabled = true; info.motorSpeed = J.limits[2]; info.maxMotorForce = J.limits[3]; }
             joint = R->world->createJoint(info);
         } else if (J.type == 2) {
             HingeJointInfo info(b1, b2, anchor, axis);
             info.isCollisionEnabled = J.collide != 0;
+            if (J.flags & 4) info.positionCorrectionTechnique = JointsPositionCorrectionTechnique::BAUMGARTE_JOINTS;
             if (J.flags & 1) { info.isLimitEnabled = true; info.minAngleLimit = J.limits[0]; info.maxAngleLimit = J.limits[1]; }
             if (J.flags & 2) { info.isMotorEnabled = true; info.motorSpeed = J.limits[2]; info.maxMotorTorque = J.limits[3]; }
             joint = R->world->createJoint(info);
         } else {
             FixedJointInfo info(b1, b2, anchor);
             info.isCollisionEnabled = J.collide != 0;
+            if (J.flags & 4) info.positionCorrectionTechnique = JointsPositionCorrectionTechnique::BAUMGARTE_JOINTS;
             joint = R->world->createJoint(info);
         }
         R->joints.push_back(joint);

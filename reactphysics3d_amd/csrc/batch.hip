@@ -324,7 +324,7 @@ static void moved_ranks(int n, std::vector<int>& rank) {
 // (BallAndSocketJoint.cpp:38-53, FixedJoint.cpp:36-56, HingeJoint.cpp:36-80, SliderJoint.cpp:38-91); the
 // rows are sorted by (kind in solve order, colour, creation index) with a greedy colouring over all bodies.
 int rp3dg_batch::upload_joints(rp3dg_world* const* ws, int nw) {
-    struct Row { int kind, color, order, b1, b2, group, slot; float4 la1, la2, ax1, ax2, qinit, lim; int flags; };
+    struct Row { int kind, color, order, b1, b2, group, slot, world; float4 la1, la2, ax1, ax2, qinit, lim; int flags; };
     std::vector<Row> rows;
     int jbase = 0;
     for (int wi = 0; wi < nw; wi++) {
@@ -364,7 +364,9 @@ int rp3dg_batch::upload_joints(rp3dg_world* const* ws, int nw) {
                 qi = q4(-d.x, -d.y, -d.z, d.w);
             }
             r.qinit = make_float4(qi.x, qi.y, qi.z, qi.w);
-            r.flags = ((hj.flags & 1) ? JF_LIMIT : 0) | ((hj.flags & 2) ? JF_MOTOR : 0);
+            r.flags = ((hj.flags & 1) ? JF_LIMIT : 0) | ((hj.flags & 2) ? JF_MOTOR : 0) |
+                      ((hj.flags & RP3DG_JOINT_BAUMGARTE) ? JF_BAUMGARTE : 0);
+            r.world = wi;
             // HingeJointInfo / SliderJointInfo defaults when the limit is off (HingeJoint.h:97-105,
             // SliderJoint.h:93-101): [-1, 1]; ball-and-socket: cone half angle, pi when the cone limit is
             // off (PhysicsWorld.cpp:500)
@@ -373,8 +375,8 @@ int rp3dg_batch::upload_joints(rp3dg_world* const* ws, int nw) {
             else
                 r.lim = make_float4((hj.flags & 1) ? hj.limits[0] : -1.0f, (hj.flags & 1) ? hj.limits[1] : 1.0f,
                                     hj.limits[2], hj.limits[3]);
-            if (r.kind == JT_BALLSOCKET) r.flags &= JF_LIMIT;   // cone limit, no motor
-            else if (r.kind == JT_FIXED) r.flags = 0;
+            if (r.kind == JT_BALLSOCKET) r.flags &= JF_LIMIT | JF_BAUMGARTE;   // cone limit, no motor
+            else if (r.kind == JT_FIXED) r.flags &= JF_BAUMGARTE;
             rows.push_back(r);
         }
         jbase += (int)W.joints.size();
@@ -431,6 +433,23 @@ int rp3dg_batch::upload_joints(rp3dg_world* const* ws, int nw) {
                        &J.i1r2, &J.i2r0, &J.i2r1, &J.i2r2, &J.r1, &J.r2, &J.kt0, &J.kt1, &J.kt2, &J.kr0, &J.kr1,
                        &J.kr2, &J.a1, &J.b2xa1, &J.c2xa1, &J.un1, &J.un2, &J.ua, &J.rn1, &J.rn2, &J.ra})
         *a = dalloc<float4>(this, m);
+    J.bias0 = dalloc<float4>(this, m);
+    J.bias1 = dalloc<float4>(this, m);
+    J.srank = dalloc<int>(this, m);
+    if (!J.bias0 || !J.bias1 || !J.srank) { set_error("device allocation failed (joints)"); return RP3DG_EHIP; }
+    // slider ranks in each world's solve order (rows are sorted by slot, then creation index, within a world)
+    std::vector<int> srank(m, -1), nsl(nw, 0);
+    bool any_bslider = false;
+    for (int i = 0; i < n; i++) {
+        if (rows[i].kind != JT_SLIDER) continue;
+        srank[i] = nsl[rows[i].world]++;
+        any_bslider |= (rows[i].flags & JF_BAUMGARTE) != 0;
+    }
+    J.bfirst = nullptr;
+    if (any_bslider) {
+        J.bfirst = dalloc<int>(this, std::max(nw, 1));
+        if (!J.bfirst) { set_error("device allocation failed (joints)"); return RP3DG_EHIP; }
+    }
     J.imlm = dalloc<float>(this, m);
     J.imm = dalloc<float>(this, m);
     jt_ranges = dalloc<int>(this, std::max<size_t>(2, ranges.size()));
@@ -455,6 +474,8 @@ int rp3dg_batch::upload_joints(rp3dg_world* const* ws, int nw) {
     HIPCHECK(hipMemcpy(J.ax2, ax2.data(), 16 * n, hipMemcpyHostToDevice));
     HIPCHECK(hipMemcpy(J.qinit, qinit.data(), 16 * n, hipMemcpyHostToDevice));
     HIPCHECK(hipMemcpy(J.lim, lim.data(), 16 * n, hipMemcpyHostToDevice));
+    HIPCHECK(hipMemcpy(J.srank, srank.data(), 4 * n, hipMemcpyHostToDevice));
+    for (float4* a : {J.bias0, J.bias1}) HIPCHECK(hipMemcpy(a, zero.data(), 16 * n, hipMemcpyHostToDevice));
     for (float4* a : {J.impT, J.impR, J.impL, J.a1, J.b2xa1, J.c2xa1, J.ra})
         HIPCHECK(hipMemcpy(a, zero.data(), 16 * n, hipMemcpyHostToDevice));
     HIPCHECK(hipMemset(J.imlm, 0, sizeof(float) * n));
@@ -1248,7 +1269,8 @@ int rp3dg_batch::enqueue_step(float dt, bool capturing) {
     for (int c = 0; c < ncol; c++) LAUNCH_T("sv_warmstart", cg, color_block, sv_warmstart, SA, c);
     // ConstraintSolverSystem::initialize: initBeforeSolve + warm start of every joint kind
     if (J.n > 0) {
-        LAUNCH_T("jt_init", jg, 256, jt_init, B, J);
+        if (J.bfirst) LAUNCH_T("fill_int", grid_for(nworlds), 256, fill_int, J.bfirst, nworlds, INT_MAX);
+        LAUNCH_T("jt_init", jg, 256, jt_init, B, J, dt);
         for (int s = 0; s < jt_slots; s++) LAUNCH_T("jt_warmstart", jg, 256, jt_warmstart, B, J, jt_ranges, s);
     }
     for (int it = 0; it < settings.vel_iters; it++) {

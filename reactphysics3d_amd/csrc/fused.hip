@@ -189,11 +189,13 @@ __global__ __launch_bounds__(kFusedMaxBlock) void sv_world_solve(FusedArgs F) {
     if (F.phase != 1) {
         // PhysicsWorld.cpp:266-272: inverse world inertia tensors and velocity integration
         for (int i = b0 + tid; i < b1; i += nt) dyn_integrate_velocities_one(B, F.settings, F.dt, i);
+        if (F.J.bfirst)
+            for (int w = tid; w < ngw; w += nt) F.J.bfirst[w0 + w] = INT_MAX;
         __syncthreads();
         // ContactSolverSystem::init (rows); ConstraintSolverSystem::initialize (initBeforeSolve)
         const SvArgs& A = AG;
         for (int i = m0 + tid; i < m1; i += nt) sv_prepare_one(A, i);
-        for (int j = j0 + tid; j < j1; j += nt) jt_init_one(B, F.J, j);
+        for (int j = j0 + tid; j < j1; j += nt) jt_init_one(B, F.J, j, F.dt);
         if (F.feats) {
             // feature rows of the points without a cached impulse; same manifold -> thread mapping as
             // above, so every thread reads only rows it wrote itself

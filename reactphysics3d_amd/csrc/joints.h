@@ -16,6 +16,9 @@ constexpr int JF_LIMIT = 1;
 constexpr int JF_MOTOR = 2;
 constexpr int JF_LOWER_VIOLATED = 4;
 constexpr int JF_UPPER_VIOLATED = 8;   // ball-and-socket: JF_LOWER_VIOLATED marks the violated cone limit
+// JointsPositionCorrectionTechnique::BAUMGARTE_JOINTS (configuration.h:101): the position error enters the
+// velocity rows as a bias of 0.2 / dt times the error, and the position solver skips the joint
+constexpr int JF_BAUMGARTE = 16;
 
 // One entry per joint, sorted by (kind, colour, creation index).
 struct DevJoints {
@@ -41,9 +44,21 @@ struct DevJoints {
     // slider: (r1 + u) x {n1, n2, axis} and r2 x {n1, n2, axis}; r2 x axis persists between steps
     // (the limit row's inverse mass of the next step reads the value the position solver left)
     float4 *un1, *un2, *ua, *rn1, *rn2, *ra;
+    // Baumgarte biases of the step (JF_BAUMGARTE joints only):
+    //   ball-and-socket: bias0 = (translation bias, cone limit bias)
+    //   fixed:           bias0.xyz = translation bias, bias1.xyz = rotation bias
+    //   hinge:           bias0.xyz = translation bias, bias1 = (rotation bias x, y, lower limit, upper limit)
+    //   slider:          bias0 = (translation bias x, y, lower limit, upper limit), bias1.xyz = rotation bias
+    float4 *bias0, *bias1;
+    // SolveSliderJointSystem::solvePositionConstraint returns (SolveSliderJointSystem.cpp:527) at the first
+    // Baumgarte slider it meets, so a world's sliders from that one on get no position correction:
+    // srank = rank of a slider in its world's solve order, bfirst[world] = smallest srank of an enabled
+    // Baumgarte slider of the step (INT_MAX: none).  bfirst is null when no world has a Baumgarte slider.
+    int* srank;
+    int* bfirst;
 };
 
-__global__ void jt_init(DevBodies B, DevJoints J);
+__global__ void jt_init(DevBodies B, DevJoints J, float dt);
 __global__ void jt_warmstart(DevBodies B, DevJoints J, const int* range, int slot);
 __global__ void jt_solve_velocity(DevBodies B, DevJoints J, const int* range, int slot, float dt);
 __global__ void jt_solve_position(DevBodies B, DevJoints J, const int* range, int slot);

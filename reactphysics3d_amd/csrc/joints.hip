@@ -8,8 +8,8 @@
 
 namespace rp3dg {
 
-__global__ void jt_init(DevBodies B, DevJoints J) {
-    RP3DG_GRID_STRIDE(j, J.n) jt_init_one(B, J, j);
+__global__ void jt_init(DevBodies B, DevJoints J, float dt) {
+    RP3DG_GRID_STRIDE(j, J.n) jt_init_one(B, J, j, dt);
 }
 
 __global__ void jt_warmstart(DevBodies B, DevJoints J, const int* range, int slot) {

@@ -126,7 +126,7 @@ __device__ __forceinline__ bool joint_enabled(const DevBodies& B, int b1, int b2
 // ------------------------------------------------------------------------------------------------
 // initBeforeSolve (all joints, one launch; no body state is written)
 // ------------------------------------------------------------------------------------------------
-__device__ __forceinline__ void jt_init_one(const DevBodies& B, const DevJoints& J, int j) {
+__device__ __forceinline__ void jt_init_one(const DevBodies& B, const DevJoints& J, int j, float dt) {
     {
         const int2 bb = J.bodies[j];
         const int b1 = bb.x, b2 = bb.y;
@@ -145,12 +145,16 @@ __device__ __forceinline__ void jt_init_one(const DevBodies& B, const DevJoints&
         bool ok;
         const M3 Kt = translation_inverse_mass(im1 + im2, r1, r2, I1, I2, anyDyn, ok);
         stm(J.kt0, J.kt1, J.kt2, j, Kt);
+        // Baumgarte biases (BETA = 0.2, biasFactor = BETA / mTimeStep in every Solve*JointSystem::initBeforeSolve)
+        const bool bg = (J.flags[j] & JF_BAUMGARTE) != 0;
+        const float bf = 0.2f / dt;
         if (t == JT_BALLSOCKET) {
             // cone-limit axis a x b with a = unit(r1), b = -unit(r2) (SolveBallAndSocketJointSystem.cpp:113-115)
             const V3 r1u = unit(r1), r2u = -unit(r2);
             const V3 acb = cross(r1u, r2u);
             J.a1[j] = f4(acb, 0.f);
             int fl = J.flags[j];
+            float cone_bias = 0.0f;
             if (fl & JF_LIMIT) {
                 // SolveBallAndSocketJointSystem.cpp:117-146.  A disabled limit keeps the half angle pi
                 // (PhysicsWorld.cpp:500), which acos never exceeds: never violated, impulse stays 0.
@@ -166,13 +170,21 @@ __device__ __forceinline__ void jt_init_one(const DevBodies& B, const DevJoints&
                 float m = dot(acb, mvmul(I1, acb)) + dot(acb, mvmul(I2, acb));
                 J.imlm[j] = m > 0.0f ? 1.0f / m : 0.0f;
                 J.flags[j] = fl;
+                cone_bias = bf * err;   // SolveBallAndSocketJointSystem.cpp:146-149
             }
+            // SolveBallAndSocketJointSystem.cpp:113-117
+            if (bg) J.bias0[j] = f4(bf * (ld3(B.x, b2) + r2 - ld3(B.x, b1) - r1), cone_bias);
         } else if (t == JT_FIXED) {
             // SolveFixedJointSystem.cpp:120-129: keeps I1 + I2 when it cannot be inverted
             M3 Kr = madd(I1, I2);
             const float det = mdet(Kr);
             if (fabsf(det) > kMachineEps && anyDyn) Kr = minv(Kr, det);
             stm(J.kr0, J.kr1, J.kr2, j, Kr);
+            if (bg) {   // SolveFixedJointSystem.cpp:110-132
+                J.bias0[j] = f4(bf * (ld3(B.x, b2) + r2 - ld3(B.x, b1) - r1), 0.f);
+                const Q4 qe = qmul(qmul(q2, ldq(J.qinit, j)), qconj(q1));
+                J.bias1[j] = f4((bf * 2.0f) * v3(qe.x, qe.y, qe.z), 0.f);
+            }
         } else if (t == JT_HINGE) {
             const V3 a1 = normalized(qrot(q1, ld3(J.ax1, j)));
             const V3 a2 = normalized(qrot(q2, ld3(J.ax2, j)));
@@ -198,9 +210,14 @@ __device__ __forceinline__ void jt_init_one(const DevBodies& B, const DevJoints&
             int fl = J.flags[j];
             const float4 lim = J.lim[j];
             float4 il = J.impL[j];
+            // SolveHingeJointSystem.cpp:91-95, 124-128, 198-209 (the limit biases are read only while a
+            // limit is violated, a step in which they are computed)
+            float4 brot = make_float4(bf * dot(a1, b2v), bf * dot(a1, c2v), 0.f, 0.f);
             if ((fl & JF_LIMIT) || (fl & JF_MOTOR)) {
                 const float ang = hinge_angle(q1, q2, ldq(J.qinit, j), a1, lim.x, lim.y);
                 const float le = ang - lim.x, ue = lim.y - ang;
+                brot.z = bf * le;
+                brot.w = bf * ue;
                 const bool oldL = (fl & JF_LOWER_VIOLATED) != 0, oldU = (fl & JF_UPPER_VIOLATED) != 0;
                 const bool vl = le <= 0, vu = ue <= 0;
                 fl = vl ? (fl | JF_LOWER_VIOLATED) : (fl & ~JF_LOWER_VIOLATED);
@@ -214,6 +231,10 @@ __device__ __forceinline__ void jt_init_one(const DevBodies& B, const DevJoints&
                 }
                 J.flags[j] = fl;
                 J.impL[j] = il;
+            }
+            if (bg) {
+                J.bias0[j] = f4(bf * (ld3(B.x, b2) + r2 - ld3(B.x, b1) - r1), 0.f);
+                J.bias1[j] = brot;
             }
         } else {   // slider, SolveSliderJointSystem.cpp:52-237
             const V3 ax = normalized(qrot(q1, ld3(J.ax1, j)));
@@ -242,6 +263,12 @@ __device__ __forceinline__ void jt_init_one(const DevBodies& B, const DevJoints&
             if (!vu || vu != oldU) il.y = 0.0f;
             J.flags[j] = fl;
             J.impL[j] = il;
+            if (bg) {   // SolveSliderJointSystem.cpp:116-122, 142-152, 208-213
+                J.bias0[j] = make_float4(dot(u, n1) * bf, dot(u, n2) * bf, bf * le, bf * ue);
+                const Q4 qe = qmul(qmul(q2, ldq(J.qinit, j)), qconj(q1));
+                J.bias1[j] = f4((bf * 2.0f) * v3(qe.x, qe.y, qe.z), 0.f);
+                if (J.bfirst) atomicMin(J.bfirst + B.world[b1], J.srank[j]);
+            }
             const float ims = im1 + im2;
             if ((fl & JF_LIMIT) && (vl || vu)) {
                 // reads r2 x axis of the previous step (recomputed only below)
@@ -384,11 +411,18 @@ __device__ __forceinline__ void jt_solve_velocity_one(const DevBodies& B, const 
         const M3 I1 = ldm(J.i1r0, J.i1r1, J.i1r2, j), I2 = ldm(J.i2r0, J.i2r1, J.i2r2, j);
         const V3 r1 = ld3(J.r1, j), r2 = ld3(J.r2, j);
         const M3 Kt = ldm(J.kt0, J.kt1, J.kt2, j);
+        // Baumgarte biases; zero under non-linear Gauss-Seidel position correction
+        float4 bias0 = make_float4(0.f, 0.f, 0.f, 0.f), bias1 = bias0;
+        if (J.flags[j] & JF_BAUMGARTE) {
+            bias0 = J.bias0[j];
+            bias1 = J.bias1[j];
+        }
+        const V3 bt = v3(bias0.x, bias0.y, bias0.z);
         if (type == JT_BALLSOCKET) {   // SolveBallAndSocketJointSystem.cpp:245-308
             if ((J.flags[j] & (JF_LIMIT | JF_LOWER_VIOLATED)) == (JF_LIMIT | JF_LOWER_VIOLATED)) {
                 const V3 acb = ld3(J.a1, j);
                 const float JvC = dot(acb, w1 - w2);
-                float d = J.imlm[j] * (-JvC - 0.0f);
+                float d = J.imlm[j] * (-JvC - bias0.w);
                 float4 il = J.impL[j];
                 const float lt = il.x;
                 il.x = fmax_s(il.x + d, 0.0f);
@@ -398,7 +432,7 @@ __device__ __forceinline__ void jt_solve_velocity_one(const DevBodies& B, const 
                 w2 += cmul(al2, mvmul(I2, (-d) * acb));
             }
             const V3 Jv = v2 + cross(w2, r2) - v1 - cross(w1, r1);
-            const V3 dl = mvmul(Kt, -Jv - v3(0.0f, 0.0f, 0.0f));
+            const V3 dl = mvmul(Kt, -Jv - bt);
             J.impT[j] = f4(ld3(J.impT, j) + dl, 0.f);
             const V3 li1 = -dl;
             const V3 ai1 = cross(dl, r1);
@@ -409,7 +443,7 @@ __device__ __forceinline__ void jt_solve_velocity_one(const DevBodies& B, const 
             w2 += cmul(al2, mvmul(I2, ai2));
         } else if (type == JT_FIXED) {   // SolveFixedJointSystem.cpp:189-262
             const V3 Jvt = v2 + cross(w2, r2) - v1 - cross(w1, r1);
-            const V3 dl = mvmul(Kt, -Jvt - v3(0.0f, 0.0f, 0.0f));
+            const V3 dl = mvmul(Kt, -Jvt - bt);
             J.impT[j] = f4(ld3(J.impT, j) + dl, 0.f);
             const V3 li1 = -dl;
             V3 ai1 = cross(dl, r1);
@@ -420,7 +454,7 @@ __device__ __forceinline__ void jt_solve_velocity_one(const DevBodies& B, const 
             w2 += cmul(al2, mvmul(I2, ai2));
             const V3 Jvr = w2 - w1;
             const M3 Kr = ldm(J.kr0, J.kr1, J.kr2, j);
-            const V3 dl2 = mvmul(Kr, -Jvr - v3(0.0f, 0.0f, 0.0f));
+            const V3 dl2 = mvmul(Kr, -Jvr - v3(bias1.x, bias1.y, bias1.z));
             J.impR[j] = f4(ld3(J.impR, j) + dl2, 0.f);
             ai1 = -dl2;
             w1 += cmul(al1, mvmul(I1, ai1));
@@ -433,7 +467,7 @@ __device__ __forceinline__ void jt_solve_velocity_one(const DevBodies& B, const 
             if (fl & JF_LIMIT) {
                 if (fl & JF_LOWER_VIOLATED) {
                     const float JvL = dot(w2 - w1, a1);
-                    float d = imlm * (-JvL - 0.0f);
+                    float d = imlm * (-JvL - bias1.z);
                     const float lt = il.x;
                     il.x = fmax_s(il.x + d, 0.0f);
                     d = il.x - lt;
@@ -444,7 +478,7 @@ __device__ __forceinline__ void jt_solve_velocity_one(const DevBodies& B, const 
                 }
                 if (fl & JF_UPPER_VIOLATED) {
                     const float JvU = -dot(w2 - w1, a1);
-                    float d = imlm * (-JvU - 0.0f);
+                    float d = imlm * (-JvU - bias1.w);
                     const float lt = il.y;
                     il.y = fmax_s(il.y + d, 0.0f);
                     d = il.y - lt;
@@ -472,7 +506,7 @@ __device__ __forceinline__ void jt_solve_velocity_one(const DevBodies& B, const 
             const float jr0 = -dot(bxa, w1) + dot(bxa, w2);
             const float jr1 = -dot(cxa, w1) + dot(cxa, w2);
             const float4 kr = J.kr0[j];
-            const float x0 = -jr0 - 0.0f, x1 = -jr1 - 0.0f;
+            const float x0 = -jr0 - bias1.x, x1 = -jr1 - bias1.y;
             const float d0 = kr.x * x0 + kr.y * x1;
             const float d1 = kr.z * x0 + kr.w * x1;
             float4 ir = J.impR[j];
@@ -484,7 +518,7 @@ __device__ __forceinline__ void jt_solve_velocity_one(const DevBodies& B, const 
             V3 ai2 = bxa * d0 + cxa * d1;
             w2 += cmul(al2, mvmul(I2, ai2));
             const V3 Jvt = v2 + cross(w2, r2) - v1 - cross(w1, r1);
-            const V3 dl = mvmul(Kt, -Jvt - v3(0.0f, 0.0f, 0.0f));
+            const V3 dl = mvmul(Kt, -Jvt - bt);
             J.impT[j] = f4(ld3(J.impT, j) + dl, 0.f);
             const V3 li1 = -dl;
             ai1 = cross(dl, r1);
@@ -501,7 +535,7 @@ __device__ __forceinline__ void jt_solve_velocity_one(const DevBodies& B, const 
                 const float imlm = J.imlm[j];
                 if (fl & JF_LOWER_VIOLATED) {
                     const float JvL = dot(ax, v2) + dot(ra, w2) - dot(ax, v1) - dot(ua, w1);
-                    float d = imlm * (-JvL - 0.0f);
+                    float d = imlm * (-JvL - bias0.z);
                     const float lt = il.x;
                     il.x = fmax_s(il.x + d, 0.0f);
                     d = il.x - lt;
@@ -512,7 +546,7 @@ __device__ __forceinline__ void jt_solve_velocity_one(const DevBodies& B, const 
                 }
                 if (fl & JF_UPPER_VIOLATED) {
                     const float JvU = dot(ax, v1) + dot(ua, w1) - dot(ax, v2) - dot(ra, w2);
-                    float d = imlm * (-JvU - 0.0f);
+                    float d = imlm * (-JvU - bias0.w);
                     const float lt = il.y;
                     il.y = fmax_s(il.y + d, 0.0f);
                     d = il.y - lt;
@@ -537,7 +571,7 @@ __device__ __forceinline__ void jt_solve_velocity_one(const DevBodies& B, const 
             // 3 rotation rows
             const V3 Jvr = w2 - w1;
             const M3 Kr = ldm(J.kr0, J.kr1, J.kr2, j);
-            const V3 dr = mvmul(Kr, -Jvr - v3(0.0f, 0.0f, 0.0f));
+            const V3 dr = mvmul(Kr, -Jvr - v3(bias1.x, bias1.y, bias1.z));
             J.impR[j] = f4(ld3(J.impR, j) + dr, 0.f);
             w1 += cmul(al1, mvmul(I1, -dr));
             w2 += cmul(al2, mvmul(I2, dr));
@@ -546,7 +580,7 @@ __device__ __forceinline__ void jt_solve_velocity_one(const DevBodies& B, const 
             const V3 un1 = ld3(J.un1, j), un2 = ld3(J.un2, j), rn1 = ld3(J.rn1, j), rn2 = ld3(J.rn2, j);
             const float el1 = -dot(n1, v1) - dot(w1, un1) + dot(n1, v2) + dot(w2, rn1);
             const float el2 = -dot(n2, v1) - dot(w1, un2) + dot(n2, v2) + dot(w2, rn2);
-            const float x0 = -el1 - 0.0f, x1 = -el2 - 0.0f;
+            const float x0 = -el1 - bias0.x, x1 = -el2 - bias0.y;
             const float4 kt = J.kt0[j];
             const float d0 = kt.x * x0 + kt.y * x1;
             const float d1 = kt.z * x0 + kt.w * x1;
@@ -576,6 +610,11 @@ __device__ __forceinline__ void jt_solve_position_one(const DevBodies& B, const 
         const int b1 = bb.x, b2 = bb.y;
         if (!joint_enabled(B, b1, b2)) return;
         const int type = J.type[j];
+        // only non-linear Gauss-Seidel joints (SolveBallAndSocketJointSystem.cpp:309,
+        // SolveFixedJointSystem.cpp:299, SolveHingeJointSystem.cpp:479); the slider system stops at its
+        // first Baumgarte joint (SolveSliderJointSystem.cpp:527)
+        if (J.flags[j] & JF_BAUMGARTE) return;
+        if (type == JT_SLIDER && J.bfirst && J.srank[j] >= J.bfirst[B.world[b1]]) return;
         Q4 q1 = ldq(B.cq, b1), q2 = ldq(B.cq, b2);
         V3 x1 = ld3(B.cx, b1), x2 = ld3(B.cx, b2);
         const float4 ii1 = B.ii[b1], ii2 = B.ii[b2];

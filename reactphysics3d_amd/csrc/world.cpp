@@ -765,7 +765,7 @@ int rp3dg_joint_set_limits(rp3dg_world* w, int32_t joint, int32_t flags, const f
     if (!mutable_world(w)) return RP3DG_ESTATE;
     HostJoint& j = w->joints[joint];
     if (!valid_limits(j.type, flags, limits)) return RP3DG_EINVAL;
-    j.flags = flags;
+    j.flags = (flags & 3) | (j.flags & RP3DG_JOINT_BAUMGARTE);
     for (int i = 0; i < 4; i++) j.limits[i] = limits ? limits[i] : 0.0f;
     return RP3DG_OK;
 }

@@ -2,7 +2,8 @@
 // and a motor from SliderJointInfo fields, a hinge whose limits are switched on through the joint after
 // createJoint (HingeJoint::enableLimit / setMinAngleLimit / setMaxAngleLimit), and a ball-and-socket
 // pendulum with a cone limit (BallAndSocketJoint::enableConeLimit / setConeLimitHalfAngle).
-// Usage: joints [steps]   (steps = 0 builds the world and prints the creation state only)
+// Usage: joints [steps] [baumgarte]   (steps = 0 builds the world and prints the creation state only;
+// baumgarte = 1 gives every joint JointsPositionCorrectionTechnique::BAUMGARTE_JOINTS)
 #include <cstdio>
 #include <cstdlib>
 
@@ -19,6 +20,9 @@ static RigidBody* box_body(PhysicsCommon& common, PhysicsWorld* world, const Vec
 
 int main(int argc, char** argv) {
     const int steps = argc > 1 ? std::atoi(argv[1]) : 120;
+    const JointsPositionCorrectionTechnique technique = argc > 2 && std::atoi(argv[2]) != 0
+                                                            ? JointsPositionCorrectionTechnique::BAUMGARTE_JOINTS
+                                                            : JointsPositionCorrectionTechnique::NON_LINEAR_GAUSS_SEIDEL;
     try {
         PhysicsCommon common;
         PhysicsWorld::WorldSettings settings;
@@ -33,6 +37,7 @@ int main(int argc, char** argv) {
         RigidBody* car = box_body(common, world, Vector3(0, 0, decimal(4.2)), Vector3(decimal(0.3), decimal(0.3), decimal(0.3)));
         SliderJointInfo si(post, car, Vector3(0, 0, decimal(4.2)), Vector3(decimal(0.6), 0, decimal(0.8)));
         si.isCollisionEnabled = false;
+        si.positionCorrectionTechnique = technique;
         si.isLimitEnabled = true;
         si.minTranslationLimit = decimal(-0.4);
         si.maxTranslationLimit = decimal(0.6);
@@ -47,6 +52,7 @@ int main(int argc, char** argv) {
         RigidBody* paddle = box_body(common, world, Vector3(decimal(4.6), 0, 5), Vector3(decimal(0.5), decimal(0.1), decimal(0.3)));
         HingeJointInfo hi(pivot, paddle, Vector3(decimal(4.05), 0, 5), Vector3(0, 1, 0));
         hi.isCollisionEnabled = false;
+        hi.positionCorrectionTechnique = technique;
         HingeJoint* hinge = static_cast<HingeJoint*>(world->createJoint(hi));
         hinge->enableLimit(true);
         hinge->setMinAngleLimit(decimal(-0.6));
@@ -60,6 +66,7 @@ int main(int argc, char** argv) {
         ball->updateMassPropertiesFromColliders();
         BallAndSocketJointInfo bi(hook, ball, Vector3(decimal(8.5), 0, 5));
         bi.isCollisionEnabled = false;
+        bi.positionCorrectionTechnique = technique;
         BallAndSocketJoint* cone = static_cast<BallAndSocketJoint*>(world->createJoint(bi));
         cone->enableConeLimit(true);
         cone->setConeLimitHalfAngle(decimal(0.3));

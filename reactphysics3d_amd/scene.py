@@ -22,6 +22,8 @@ import numpy as np
 STATIC, KINEMATIC, DYNAMIC = 0, 1, 2
 SPHERE, CAPSULE, BOX, HULL = 0, 1, 2, 3
 BALLSOCKET, SLIDER, HINGE, FIXED = 0, 1, 2, 3
+# Joint flag bits (rp3dg_joint_create): limit, motor, Baumgarte position correction (RP3DG_JOINT_BAUMGARTE)
+JOINT_LIMIT, JOINT_MOTOR, JOINT_BAUMGARTE = 1, 2, 4
 
 SETTINGS_DTYPE = np.dtype([
     ("gravity", "<f4", 3), ("vel_iters", "<i4"), ("pos_iters", "<i4"),

@@ -57,17 +57,19 @@ def test_example_steps_like_the_reference(ref_available):
 JOINTS_EXE = os.path.join(ROOT, "reactphysics3d_amd", "_lib", "joints")
 
 
-def _run_joints(steps):
+def _run_joints(steps, baumgarte=False):
     if not os.path.exists(JOINTS_EXE):
         pytest.skip("example not built (python -m reactphysics3d_amd.build)")
-    out = subprocess.run([JOINTS_EXE, str(steps)], capture_output=True, text=True, timeout=120)
+    out = subprocess.run([JOINTS_EXE, str(steps), "1" if baumgarte else "0"], capture_output=True, text=True,
+                         timeout=120)
     assert out.returncode == 0, out.stderr
     rows = [list(map(float, line.split()[2:])) for line in out.stdout.splitlines() if line.startswith("body")]
     return np.array(rows, np.float32)
 
 
-def _joints_scene():
+def _joints_scene(baumgarte=False):
     """The world of examples/joints.cpp as a scene description."""
+    bg = S.JOINT_BAUMGARTE if baumgarte else 0
     sc = S.Scene("joints_example", S.make_settings(gravity=(0.0, 0.0, -9.81), vel_iters=10, pos_iters=5, sleeping=1))
     post = sc.add_body(S.STATIC, (0.0, 0.0, 5.0))
     car = sc.add_body(S.DYNAMIC, (0.0, 0.0, 4.2))
@@ -78,9 +80,10 @@ def _joints_scene():
     hook = sc.add_body(S.STATIC, (8.0, 0.0, 5.0))
     ball = sc.add_body(S.DYNAMIC, (9.0, 0.0, 5.0))
     sc.add_collider(ball, S.SPHERE, (0.2,))
-    sc.add_joint(S.SLIDER, post, car, (0.0, 0.0, 4.2), axis=(0.6, 0.0, 0.8), flags=3, limits=(-0.4, 0.6, 0.5, 1.0))
-    sc.add_joint(S.HINGE, pivot, paddle, (4.05, 0.0, 5.0), axis=(0.0, 1.0, 0.0), flags=1, limits=(-0.6, 0.4, 0.0, 0.0))
-    sc.add_joint(S.BALLSOCKET, hook, ball, (8.5, 0.0, 5.0), flags=1, limits=(0.3, 0.0, 0.0, 0.0))
+    sc.add_joint(S.SLIDER, post, car, (0.0, 0.0, 4.2), axis=(0.6, 0.0, 0.8), flags=3 | bg, limits=(-0.4, 0.6, 0.5, 1.0))
+    sc.add_joint(S.HINGE, pivot, paddle, (4.05, 0.0, 5.0), axis=(0.0, 1.0, 0.0), flags=1 | bg,
+                 limits=(-0.6, 0.4, 0.0, 0.0))
+    sc.add_joint(S.BALLSOCKET, hook, ball, (8.5, 0.0, 5.0), flags=1 | bg, limits=(0.3, 0.0, 0.0, 0.0))
     return sc
 
 
@@ -91,13 +94,15 @@ def test_joints_example_builds_and_creates_world():
 
 
 @pytest.mark.gpu
-def test_joints_example_steps_like_the_reference(ref_available):
+@pytest.mark.parametrize("baumgarte", [False, True])
+def test_joints_example_steps_like_the_reference(ref_available, baumgarte):
     """Limits and motors set through SliderJointInfo fields and through the joint setters after
-    createJoint reach the device: the example matches the reference world built with the same limits
-    (over 240 steps, long enough for the pendulums at rest on their limits to fall asleep)."""
+    createJoint reach the device, and so does JointInfo::positionCorrectionTechnique: the example matches
+    the reference world built with the same limits (over 240 steps, long enough for the pendulums at rest
+    on their limits to fall asleep)."""
     from oracle.ref import RefWorld
-    st = _run_joints(240)
-    ref = RefWorld(_joints_scene())
+    st = _run_joints(240, baumgarte)
+    ref = RefWorld(_joints_scene(baumgarte))
     for _ in range(240):
         ref.step(1.0 / 60.0)
     print("asleep in the reference:", ref.sleeping())
