@@ -27,15 +27,40 @@ HIP_FLAGS = [
 ]
 
 
-def build(verbose=False, jobs=None):
-    os.makedirs(LIB_DIR, exist_ok=True)
+# AddressSanitizer build of the host code (C-ABI, world description, QuickHull, batch set-up); the
+# device code is built as usual.  The library needs the ASan runtime preloaded; see asan_run().
+ASAN_DIR = os.path.join(HERE, "_lib_asan")
+ASAN_FLAGS = ["-Xarch_host", "-fsanitize=address", "-Xarch_host", "-fno-omit-frame-pointer"]
+
+
+def asan_runtime():
+    """Path of the compiler's shared ASan runtime (clang's, as hipcc compiles the host code)."""
+    out = subprocess.run(["hipcc", "-print-file-name=libclang_rt.asan-x86_64.so"], capture_output=True, text=True)
+    path = out.stdout.strip()
+    if not os.path.isabs(path) or not os.path.exists(path):
+        raise RuntimeError("clang ASan runtime not found")
+    return path
+
+
+def asan_run(args):
+    """Runs a command (e.g. the CPU test suite) against the ASan build of the library."""
+    env = dict(os.environ, LD_PRELOAD=asan_runtime(), RP3DG_LIBRARY=os.path.join(ASAN_DIR, "librp3d_mi355x.so"),
+               ASAN_OPTIONS="detect_leaks=0:abort_on_error=0:halt_on_error=1")
+    return subprocess.run(args, env=env).returncode
+
+
+def build(verbose=False, jobs=None, asan=False):
+    lib_dir = ASAN_DIR if asan else LIB_DIR
+    lib = os.path.join(lib_dir, "librp3d_mi355x.so")
+    extra = ASAN_FLAGS if asan else []
+    os.makedirs(lib_dir, exist_ok=True)
     cmds, objs = [], []
     for src in SOURCES:
         path = os.path.join(CSRC, src)
-        obj = os.path.join(LIB_DIR, src + ".o")
-        cmd = ["hipcc", *HIP_FLAGS, "-c", path, "-o", obj]
+        obj = os.path.join(lib_dir, src + ".o")
+        cmd = ["hipcc", *HIP_FLAGS, *extra, "-c", path, "-o", obj]
         if src.endswith(".cpp"):  # host code, same FP flags
-            cmd = ["hipcc", "-x", "hip", *HIP_FLAGS, "-c", path, "-o", obj]
+            cmd = ["hipcc", "-x", "hip", *HIP_FLAGS, *extra, "-c", path, "-o", obj]
         cmds.append(cmd)
         objs.append(obj)
 
@@ -46,10 +71,12 @@ def build(verbose=False, jobs=None):
 
     with ThreadPoolExecutor(max_workers=jobs or min(8, len(cmds))) as ex:
         list(ex.map(run, cmds))
-    tmp = LIB + ".tmp"
+    tmp = lib + ".tmp"
     cmd = ["hipcc", "--offload-arch=gfx950", "-shared", "-fPIC", *objs, "-o", tmp]
     subprocess.run(cmd, check=True)
-    os.replace(tmp, LIB)
+    os.replace(tmp, lib)
+    if asan:
+        return lib
     # the C++ API examples (host code only: g++ against the C++ mirror header and the library)
     for name in ("hello_world", "joints", "contacts", "driven", "meshes", "queries"):
         ex = os.path.join(LIB_DIR, name)
@@ -63,5 +90,12 @@ def build(verbose=False, jobs=None):
 
 
 if __name__ == "__main__":
+    # python -m reactphysics3d_amd.build [-v] [--asan [-- command...]]: with --asan, builds the ASan
+    # library and runs the command (default: the CPU test suite) against it
+    if "--asan" in sys.argv:
+        build(verbose="-v" in sys.argv, asan=True)
+        rest = sys.argv[sys.argv.index("--") + 1:] if "--" in sys.argv else \
+            [sys.executable, "-m", "pytest", "tests", "-q", "-m", "not gpu", "-p", "no:cacheprovider"]
+        sys.exit(asan_run(rest))
     build(verbose="-v" in sys.argv)
     print(LIB)

@@ -12,7 +12,9 @@ import numpy as np
 
 from . import scene as S
 
-LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "_lib", "librp3d_mi355x.so")
+# RP3DG_LIBRARY selects another build of the library (the AddressSanitizer build of the host code)
+LIB_PATH = os.environ.get("RP3DG_LIBRARY") or \
+    os.path.join(os.path.dirname(os.path.abspath(__file__)), "_lib", "librp3d_mi355x.so")
 
 _lib = None
 
