@@ -43,6 +43,8 @@ extern "C" {
 #define RP3DG_SHAPE_CAPSULE 1
 #define RP3DG_SHAPE_BOX 2
 #define RP3DG_SHAPE_CONVEX_MESH 3
+#define RP3DG_SHAPE_CONCAVE_MESH 4   /* params = {mesh index, scale x, y, z} (ConcaveMeshShape) */
+#define RP3DG_SHAPE_HEIGHTFIELD 5    /* params = {height field index, scale x, y, z} (HeightFieldShape) */
 
 /* Joint kinds (reactphysics3d::JointType, constraint/Joint.h). */
 #define RP3DG_JOINT_BALLSOCKET 0
@@ -185,6 +187,30 @@ int32_t rp3dg_hull_create_from_points(rp3dg_world* world, const void* vertices, 
 int rp3dg_quickhull(const void* vertices, int32_t nb_vertices, int32_t stride, int32_t is_double, float* out_vertices,
                     int32_t cap_vertices, int32_t* face_counts, int32_t cap_faces, int32_t* face_indices,
                     int32_t cap_indices, int32_t* nb_out_vertices, int32_t* nb_out_faces);
+
+/* PhysicsCommon::createTriangleMesh(TriangleVertexArray, messages) (src/collision/TriangleMesh.cpp:39-249):
+ * nb_vertices xyz vertices, nb_triangles triplets of vertex indices (counter-clockwise seen from the front),
+ * optional xyz vertex normals (NULL: TriangleMesh::computeVerticesNormals, angle-weighted).  Triangles
+ * with an almost zero area or edge are dropped (the reference's warning messages; their number goes to
+ * *nb_skipped when not NULL) and unused vertices removed; the mesh's AABB tree is built as the reference
+ * builds it.  Returns the mesh index used as params[0] of RP3DG_SHAPE_CONCAVE_MESH, or RP3DG_EINVAL (an
+ * index out of range, a too short given normal, no valid triangle). */
+int32_t rp3dg_triangle_mesh_create(rp3dg_world* world, const float* vertices, int32_t nb_vertices,
+                                   const int32_t* indices, int32_t nb_triangles, const float* normals,
+                                   int32_t* nb_skipped);
+/* PhysicsCommon::createHeightField(columns, rows, heights, HEIGHT_FLOAT_TYPE, messages)
+ * (src/collision/HeightField.cpp:43-130): columns x rows samples, height of grid point (x, y) at
+ * heights[y * columns + x]; the field is centred on the origin with y up, one unit between grid
+ * points before the shape's scale.  Returns the index used as params[0] of RP3DG_SHAPE_HEIGHTFIELD. */
+int32_t rp3dg_heightfield_create(rp3dg_world* world, int32_t columns, int32_t rows, const float* heights);
+/* ConcaveShape::computeOverlappingTriangles (ConcaveMeshShape.cpp:148-176, HeightField.cpp:133-180) of a
+ * concave collider for an AABB in the shape's local space: the triangles in the order of the middle
+ * phase, as ids (mesh: triangle index; height field: (j * (columns - 1) + i) * 2 + k), 9 vertex and 9
+ * vertex-normal floats each (scaled local space).  Returns the number of triangles (outputs hold at
+ * most capacity of them; NULL outputs are skipped). */
+int32_t rp3dg_concave_overlapping_triangles(rp3dg_world* world, int32_t collider, const float aabb_min[3],
+                                            const float aabb_max[3], int32_t* ids, float* vertices,
+                                            float* normals, int32_t capacity);
 
 /* PhysicsWorld::createJoint (src/engine/PhysicsWorld.cpp:483-600) with anchor / axis given in world space
  * (the JointInfo "world space" constructors, e.g. BallAndSocketJointInfo(body1, body2, anchor)).

@@ -32,6 +32,8 @@ def load(f64=False):
     L = C.CDLL(path)
     vp, i32, f = C.c_void_p, C.c_int32, C.c_float
     fptr, iptr = C.POINTER(C.c_float), C.POINTER(C.c_int32)
+    L.rp3dref_concave_overlapping_triangles.restype = i32
+    L.rp3dref_concave_overlapping_triangles.argtypes = [vp, i32, fptr, fptr, iptr, fptr, fptr, i32]
     L.rp3dref_create.restype = vp
     L.rp3dref_create.argtypes = [vp, i32, vp, i32, vp, i32, vp, fptr, iptr]
     L.rp3dref_destroy.argtypes = [vp]
@@ -89,7 +91,7 @@ class RefWorld:
         self._b = sc.body_array()
         self._c = sc.collider_array()
         self._j = sc.joint_array() if sc.joints else np.zeros(1, dtype=self._jdtype())
-        hv, hi = (sc.hull_arrays() if getattr(sc, "hulls", None) else (None, None))
+        hv, hi = sc.hull_arrays()
         self._hv, self._hi = hv, hi
         self.h = self.L.rp3dref_create(self._s.ctypes.data, len(self._b), self._b.ctypes.data, len(self._c),
                                        self._c.ctypes.data, len(sc.joints), self._j.ctypes.data,
@@ -320,6 +322,20 @@ class RefWorld:
         ids = np.zeros(self.nc, np.int32)
         self.L.rp3dref_get_broadphase_ids(self.h, _ip(ids))
         return ids
+
+    def concave_overlapping_triangles(self, collider, aabb_min, aabb_max, capacity=4096):
+        """ConcaveShape::computeOverlappingTriangles of a concave collider for an AABB in its local space:
+        (shape ids, vertices [n, 3, 3], vertex normals [n, 3, 3]) in the middle phase's order."""
+        ids = np.zeros(capacity, np.int32)
+        vs = np.zeros((capacity, 3, 3), np.float32)
+        ns = np.zeros((capacity, 3, 3), np.float32)
+        lo = np.ascontiguousarray(aabb_min, np.float32)
+        hi = np.ascontiguousarray(aabb_max, np.float32)
+        n = self.L.rp3dref_concave_overlapping_triangles(self.h, collider, _fp(lo), _fp(hi), _ip(ids), _fp(vs),
+                                                         _fp(ns), capacity)
+        if n < 0 or n > capacity:
+            raise RuntimeError("rp3dref_concave_overlapping_triangles failed")
+        return ids[:n], vs[:n], ns[:n]
 
     def pairs(self):
         cap = 64 * max(1, self.nc)

@@ -91,6 +91,8 @@ struct RefWorld {
     std::unordered_map<uint32_t, int> collider_of_entity;
     std::unordered_map<uint32_t, int> body_of_entity;
     std::vector<int32_t> joint_order;   // solve order requested by rp3dref_set_joint_order (joint indices)
+    std::vector<TriangleMesh*> tri_meshes;   // concave shapes' data (PhysicsCommon::createTriangleMesh)
+    std::vector<HeightField*> heightfields;  // (PhysicsCommon::createHeightField)
     EventRecorder* events = nullptr;    // set by rp3dref_enable_events
     float last_dt = 0.f;
 };
@@ -156,6 +158,46 @@ void* rp3dref_create(const RefSettings* s, int nb, const RefBody* bodies, int nc
             meshes.push_back(mesh);
             voff += (size_t)nv;
         }
+        // concave shapes follow the hulls: [nb triangle meshes, then per mesh nv, nt, has normals, 3 nt
+        // indices] with nv vertices (and nv normals) in the float array, then [nb height fields, then per
+        // field columns, rows] with columns * rows heights in the float array
+        const int nm = hull_offsets[k++];
+        for (int m = 0; m < nm; m++) {
+            const int nv = hull_offsets[k++], nt = hull_offsets[k++], hasn = hull_offsets[k++];
+            const int32_t* idx = hull_offsets + k;
+            k += 3 * nt;
+            const float* vs = hull_vertices + 3 * voff;
+            voff += (size_t)nv;
+            std::vector<Message> msgs;
+            TriangleMesh* tm = nullptr;
+            if (hasn) {
+                const float* ns = hull_vertices + 3 * voff;
+                voff += (size_t)nv;
+                TriangleVertexArray tva((uint32)nv, vs, 3 * sizeof(float), ns, 3 * sizeof(float), (uint32)nt, idx,
+                                        3 * sizeof(int32_t), TriangleVertexArray::VertexDataType::VERTEX_FLOAT_TYPE,
+                                        TriangleVertexArray::NormalDataType::NORMAL_FLOAT_TYPE,
+                                        TriangleVertexArray::IndexDataType::INDEX_INTEGER_TYPE);
+                tm = R->common.createTriangleMesh(tva, msgs);
+            } else {
+                TriangleVertexArray tva((uint32)nv, vs, 3 * sizeof(float), (uint32)nt, idx, 3 * sizeof(int32_t),
+                                        TriangleVertexArray::VertexDataType::VERTEX_FLOAT_TYPE,
+                                        TriangleVertexArray::IndexDataType::INDEX_INTEGER_TYPE);
+                tm = R->common.createTriangleMesh(tva, msgs);
+            }
+            if (!tm) { delete R; return nullptr; }
+            R->tri_meshes.push_back(tm);
+        }
+        const int nhf = hull_offsets[k++];
+        for (int f = 0; f < nhf; f++) {
+            const int cols = hull_offsets[k++], rows = hull_offsets[k++];
+            std::vector<Message> msgs;
+            HeightField* hf = R->common.createHeightField(cols, rows, hull_vertices + 3 * voff,
+                                                          HeightField::HeightDataType::HEIGHT_FLOAT_TYPE, msgs);
+            // heights take cols * rows floats, padded to whole xyz triplets
+            voff += ((size_t)cols * rows + 2) / 3;
+            if (!hf) { delete R; return nullptr; }
+            R->heightfields.push_back(hf);
+        }
     }
     // canonical creation order (identical to the device-side scene loader):
     // body, setType, colliders (+material, filter), mass properties, velocities
@@ -174,6 +216,12 @@ void* rp3dref_create(const RefSettings* s, int nb, const RefBody* bodies, int nc
             else if (C.shape == 2) shape = R->common.createBoxShape(Vector3(C.params[0], C.params[1], C.params[2]));
             else if (C.shape == 3 && (size_t)C.params[0] < meshes.size())
                 shape = R->common.createConvexMeshShape(meshes[(size_t)C.params[0]]);
+            else if (C.shape == 4 && (size_t)C.params[0] < R->tri_meshes.size())
+                shape = R->common.createConcaveMeshShape(R->tri_meshes[(size_t)C.params[0]],
+                                                         Vector3(C.params[1], C.params[2], C.params[3]));
+            else if (C.shape == 5 && (size_t)C.params[0] < R->heightfields.size())
+                shape = R->common.createHeightFieldShape(R->heightfields[(size_t)C.params[0]],
+                                                         Vector3(C.params[1], C.params[2], C.params[3]));
             else { delete R; return nullptr; }
             Collider* col = body->addCollider(shape, make_tf(C.lpos, C.lquat));
             col->getMaterial().setFrictionCoefficient(C.friction);
@@ -634,6 +682,33 @@ static int apply_joint_order(RefWorld* R) {
     reorder(W.mHingeJointsComponents, hg);
     reorder(W.mSliderJointsComponents, sl);
     return 0;
+}
+
+// ConcaveShape::computeOverlappingTriangles of a concave collider's shape for an AABB in its local space:
+// shape ids, vertices and vertex normals of the overlapping triangles in the middle phase's order.
+int rp3dref_concave_overlapping_triangles(void* h, int collider, const float* aabb_min, const float* aabb_max,
+                                          int32_t* ids, float* vertices, float* normals, int capacity) {
+    RefWorld* R = (RefWorld*)h;
+    if (collider < 0 || collider >= (int)R->colliders.size()) return -1;
+    CollisionShape* cs = R->colliders[collider]->getCollisionShape();
+    if (cs->isConvex()) return -1;
+    ConcaveShape* shape = static_cast<ConcaveShape*>(cs);
+    MemoryAllocator& alloc = R->common.mMemoryManager.getHeapAllocator();
+    Array<Vector3> tv(alloc), tn(alloc);
+    Array<uint32> sid(alloc);
+    const AABB aabb(Vector3(aabb_min[0], aabb_min[1], aabb_min[2]), Vector3(aabb_max[0], aabb_max[1], aabb_max[2]));
+    shape->computeOverlappingTriangles(aabb, tv, tn, sid, alloc);
+    const int n = (int)sid.size();
+    for (int i = 0; i < n && i < capacity; i++) {
+        ids[i] = (int32_t)sid[i];
+        for (int k = 0; k < 3; k++) {
+            const Vector3& v = tv[3 * i + k];
+            const Vector3& m = tn[3 * i + k];
+            vertices[9 * i + 3 * k] = v.x; vertices[9 * i + 3 * k + 1] = v.y; vertices[9 * i + 3 * k + 2] = v.z;
+            normals[9 * i + 3 * k] = m.x; normals[9 * i + 3 * k + 1] = m.y; normals[9 * i + 3 * k + 2] = m.z;
+        }
+    }
+    return n;
 }
 
 // Sets the joint solve order; with sleeping enabled it is re-applied to the enabled joints after every

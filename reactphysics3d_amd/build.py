@@ -12,7 +12,7 @@ CSRC = os.path.join(HERE, "csrc")
 LIB_DIR = os.path.join(HERE, "_lib")
 LIB = os.path.join(LIB_DIR, "librp3d_mi355x.so")
 
-SOURCES = ["world.cpp", "quickhull.cpp", "broadphase.hip", "narrowphase.hip", "solver.hip", "joints.hip", "fused.hip", "mlp.hip",
+SOURCES = ["world.cpp", "quickhull.cpp", "concave.cpp", "broadphase.hip", "narrowphase.hip", "solver.hip", "joints.hip", "fused.hip", "mlp.hip",
            "islands.hip", "events.hip", "raycast.hip", "api.hip",
            "batch.hip"]
 

@@ -23,7 +23,8 @@
 
 namespace rp3dg {
 
-enum ShapeType : int { SHAPE_SPHERE = 0, SHAPE_CAPSULE = 1, SHAPE_BOX = 2, SHAPE_HULL = 3 };
+enum ShapeType : int { SHAPE_SPHERE = 0, SHAPE_CAPSULE = 1, SHAPE_BOX = 2, SHAPE_HULL = 3, SHAPE_CONCAVE_MESH = 4, SHAPE_HEIGHTFIELD = 5 };
+__host__ __device__ constexpr bool shape_is_concave(int t) { return t >= SHAPE_CONCAVE_MESH; }
 
 // NarrowPhaseAlgorithmType (include/reactphysics3d/collision/narrowphase/CollisionDispatch.h)
 enum Algo : int {

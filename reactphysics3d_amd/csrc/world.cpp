@@ -101,6 +101,13 @@ float shape_volume(const HostWorld& w, const HostCollider& c) {
             return 8 * c.params[0] * c.params[1] * c.params[2];
         case SHAPE_HULL:
             return w.hulls[(int)c.params[0]].volume;
+        case SHAPE_CONCAVE_MESH:
+        case SHAPE_HEIGHTFIELD: {  // ConcaveShape::getVolume (ConcaveShape.cpp:41-51): volume of the local AABB
+            V3 lo, hi;
+            concave_local_bounds(w, c, lo, hi);
+            const float lx = hi.x - lo.x, ly = hi.y - lo.y, lz = hi.z - lo.z;
+            return lx * ly * lz;
+        }
     }
     return 0.f;
 }
@@ -144,6 +151,9 @@ V3 shape_local_inertia(const HostWorld& w, const HostCollider& c, float mass) {
             const float z2 = realExtent.z * realExtent.z;
             return v3(factor * (y2 + z2), factor * (x2 + z2), factor * (x2 + y2));
         }
+        case SHAPE_CONCAVE_MESH:
+        case SHAPE_HEIGHTFIELD:   // ConcaveShape::getLocalInertiaTensor (ConcaveShape.h:168-175)
+            return v3(mass, mass, mass);
     }
     return v3zero();
 }
@@ -542,8 +552,27 @@ int32_t rp3dg_collider_add(rp3dg_world* w, int32_t bi, int32_t shape, const floa
         case SHAPE_HULL:
             if ((int)params[0] < 0 || (int)params[0] >= (int)w->hulls.size()) { set_error("invalid hull index"); return RP3DG_EINVAL; }
             break;
+        case SHAPE_CONCAVE_MESH:
+        case SHAPE_HEIGHTFIELD: {
+            const int n = shape == SHAPE_CONCAVE_MESH ? (int)w->meshes.size() : (int)w->heightfields.size();
+            if ((int)params[0] < 0 || (int)params[0] >= n || params[0] != (float)(int)params[0]) {
+                set_error(shape == SHAPE_CONCAVE_MESH ? "invalid triangle mesh index" : "invalid height field index");
+                return RP3DG_EINVAL;
+            }
+            if (!(params[1] > 0.f) || !(params[2] > 0.f) || !(params[3] > 0.f)) {
+                set_error("concave shape scale must be > 0");
+                return RP3DG_EINVAL;
+            }
+            // a concave shape collides only with convex ones and is meant for bodies that do not respond to
+            // contacts (the reference's user manual: static bodies)
+            if (w->bodies[bi].type == 2) {
+                set_error("concave shapes (triangle mesh, height field) go on static or kinematic bodies only");
+                return RP3DG_EUNSUPPORTED;
+            }
+            break;
+        }
         default:
-            set_error("unsupported collision shape (concave shapes are not on the device path)");
+            set_error("unsupported collision shape");
             return RP3DG_EUNSUPPORTED;
     }
     c.local = tf(lp ? v3(lp[0], lp[1], lp[2]) : v3zero(), lq ? q4(lq[0], lq[1], lq[2], lq[3]) : q4(0, 0, 0, 1));

@@ -33,6 +33,28 @@ struct HostHull {
     float volume;
 };
 
+// TriangleMesh (src/collision/TriangleMesh.cpp): vertices and triangles after the reference's clean-up
+// (degenerate triangles dropped, unused vertices removed), vertex normals (given or angle-weighted), and
+// the mesh's DynamicAABBTree built by inserting the triangles' AABBs in order (DynamicAABBTree.cpp:146-
+// 348, 420-587), node for node.  Leaves: child0 = -1, data = triangle index.
+struct HostTriMesh {
+    struct Node { V3 mn, mx; int parent, child0, child1, height, data; };
+    std::vector<V3> vertices, normals;
+    std::vector<int> tris;        // 3 vertex indices per triangle
+    std::vector<Node> nodes;
+    int root = -1;
+    int nb_skipped = 0;           // triangles dropped as degenerate (Message::Type::Warning)
+};
+
+// HeightField (src/collision/HeightField.cpp:43-130) with float heights: grid of nb_columns x nb_rows
+// samples, height of (x, y) at heights[y * nb_columns + x], centred on the origin with y up.
+struct HostHeightField {
+    int columns = 0, rows = 0;
+    std::vector<float> heights;
+    float width = 0.f, length = 0.f, min_h = 0.f, max_h = 0.f, origin = 0.f;
+    V3 bmin, bmax;
+};
+
 struct HostBody {
     Tf transform;               // body origin + orientation (TransformComponents)
     V3 com_world, com_local;
@@ -72,6 +94,8 @@ struct HostWorld {
     std::vector<HostBody> bodies;
     std::vector<HostCollider> colliders;
     std::vector<HostHull> hulls;
+    std::vector<HostTriMesh> meshes;
+    std::vector<HostHeightField> heightfields;
     std::vector<HostJoint> joints;
     rp3dg_batch* own = nullptr;   // device batch of one, created on first update()
     bool events = false;          // record contact events in the own batch
@@ -88,6 +112,17 @@ float shape_volume(const HostWorld& w, const HostCollider& c);
 V3 shape_local_inertia(const HostWorld& w, const HostCollider& c, float mass);
 
 void set_error(const std::string& msg);
+
+// Concave shapes (concave.cpp): triangles overlapping an AABB given in the shape's local space, in the
+// order of ConcaveShape::computeOverlappingTriangles (ConcaveMeshShape.cpp:148-176, HeightField.cpp:
+// 133-180): triangle ids (mesh: triangle index; height field: HeightField::computeTriangleShapeId),
+// vertices and vertex normals in the shape's scaled local space.
+void concave_overlapping_triangles(const HostWorld& w, const HostCollider& c, V3 aabb_min, V3 aabb_max,
+                                   std::vector<int>& ids, std::vector<V3>& vertices, std::vector<V3>& normals);
+// ConcaveShape local bounds (ConcaveMeshShape.cpp:246-250, HeightFieldShape.cpp:40-44), scale applied
+void concave_local_bounds(const HostWorld& w, const HostCollider& c, V3& lo, V3& hi);
+// ConcaveMeshShape::computeScaledVerticesNormals (ConcaveMeshShape.cpp:53-70)
+std::vector<V3> concave_scaled_normals(const HostTriMesh& m, V3 scale);
 
 }  // namespace rp3dg
 

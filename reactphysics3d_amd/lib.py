@@ -66,6 +66,7 @@ EXPORTED = [
     "rp3dg_body_set_damping", "rp3dg_body_enable_gravity", "rp3dg_body_apply_force_torque",
     "rp3dg_body_set_lock_factors", "rp3dg_body_update_mass_properties", "rp3dg_body_set_mass_properties",
     "rp3dg_collider_add", "rp3dg_collider_set_material", "rp3dg_collider_set_filter", "rp3dg_hull_create",
+    "rp3dg_triangle_mesh_create", "rp3dg_heightfield_create", "rp3dg_concave_overlapping_triangles",
     "rp3dg_joint_create", "rp3dg_joint_set_limits", "rp3dg_body_set_allowed_to_sleep", "rp3dg_batch_get_sleeping",
     "rp3dg_world_get_body_sleeping", "rp3dg_batch_set_contact_events", "rp3dg_batch_get_contact_events",
     "rp3dg_world_set_contact_events", "rp3dg_world_get_batch", "rp3dg_batch_raycast", "rp3dg_batch_apply_forces",
@@ -138,6 +139,9 @@ def load():
         "rp3dg_collider_set_material": (C.c_int, [vp, i32, f, f, f]),
         "rp3dg_collider_set_filter": (C.c_int, [vp, i32, C.c_uint16, C.c_uint16]),
         "rp3dg_hull_create": (i32, [vp, fptr, i32, iptr, iptr, i32]),
+        "rp3dg_triangle_mesh_create": (i32, [vp, fptr, i32, iptr, i32, fptr, iptr]),
+        "rp3dg_heightfield_create": (i32, [vp, i32, i32, fptr]),
+        "rp3dg_concave_overlapping_triangles": (i32, [vp, i32, fptr, fptr, iptr, fptr, fptr, i32]),
         "rp3dg_joint_create": (i32, [vp, i32, i32, i32, fptr, fptr, i32, i32, fptr]),
         "rp3dg_joint_set_limits": (C.c_int, [vp, i32, i32, fptr]),
         "rp3dg_body_set_allowed_to_sleep": (C.c_int, [vp, i32, i32]),
@@ -270,6 +274,11 @@ def world_from_scene(sc: S.Scene):
         counts = np.array([len(f) for f in faces], np.int32)
         idx = np.array([i for f in faces for i in f], np.int32)
         _check(L.rp3dg_hull_create(w, _fp(v), len(v), _ip(counts), _ip(idx), len(faces)), "hull_create")
+    for v, t, n in getattr(sc, "tri_meshes", []):
+        _check(L.rp3dg_triangle_mesh_create(w, _fp(v), len(v), _ip(t), len(t), None if n is None else _fp(n), None),
+               "triangle_mesh_create")
+    for cols, rows, h in getattr(sc, "heightfields", []):
+        _check(L.rp3dg_heightfield_create(w, cols, rows, _fp(h)), "heightfield_create")
     ci = 0
     cols = sc.colliders
     for bi, (bt, pos, q, lin, ang, grav, ld, ad, llock, alock) in enumerate(sc.bodies):

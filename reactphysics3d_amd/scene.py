@@ -21,6 +21,8 @@ import numpy as np
 
 STATIC, KINEMATIC, DYNAMIC = 0, 1, 2
 SPHERE, CAPSULE, BOX, HULL = 0, 1, 2, 3
+# concave shapes (static or kinematic bodies only); params = (data index, scale x, y, z)
+CONCAVE_MESH, HEIGHTFIELD = 4, 5
 BALLSOCKET, SLIDER, HINGE, FIXED = 0, 1, 2, 3
 # Joint flag bits (rp3dg_joint_create): limit, motor, Baumgarte position correction (RP3DG_JOINT_BAUMGARTE)
 JOINT_LIMIT, JOINT_MOTOR, JOINT_BAUMGARTE = 1, 2, 4
@@ -92,6 +94,8 @@ class Scene:
     joints: list = field(default_factory=list)
     hulls: list = field(default_factory=list)   # (vertices [V, 3] float32, faces: list of CCW index lists)
     hull_points: list = field(default_factory=list)   # per hull: the point cloud it was built from, or None
+    tri_meshes: list = field(default_factory=list)    # (vertices [V, 3] float32, triangles [T, 3] int32, normals or None)
+    heightfields: list = field(default_factory=list)  # (columns, rows, heights [rows, columns] float32)
 
     # -- builders -------------------------------------------------------------------------------
     def add_body(self, btype, pos, quat=(0.0, 0.0, 0.0, 1.0), lin=(0.0, 0.0, 0.0), ang=(0.0, 0.0, 0.0),
@@ -139,6 +143,22 @@ class Scene:
         self.hull_points.append(pts)
         return len(self.hulls) - 1
 
+    def add_triangle_mesh(self, vertices, triangles, normals=None):
+        """A triangle mesh (PhysicsCommon::createTriangleMesh(TriangleVertexArray)); returns its index, the
+        params[0] of a CONCAVE_MESH collider."""
+        v = np.ascontiguousarray(vertices, np.float32).reshape(-1, 3)
+        t = np.ascontiguousarray(triangles, np.int32).reshape(-1, 3)
+        n = None if normals is None else np.ascontiguousarray(normals, np.float32).reshape(-1, 3)
+        self.tri_meshes.append((v, t, n))
+        return len(self.tri_meshes) - 1
+
+    def add_heightfield(self, heights):
+        """A height field of float heights [rows, columns] (PhysicsCommon::createHeightField); returns its
+        index, the params[0] of a HEIGHTFIELD collider."""
+        h = np.ascontiguousarray(heights, np.float32)
+        self.heightfields.append((h.shape[1], h.shape[0], h))
+        return len(self.heightfields) - 1
+
     def hull_arrays(self):
         """(vertices float32 [sum V * 3], ints int32 [nb hulls, then per hull nv, nf, face sizes,
         face indices]) -- the layout the reference harness reads."""
@@ -151,6 +171,18 @@ class Scene:
                 continue
             verts.append(v.ravel())
             ints += [len(v), len(faces)] + [len(f) for f in faces] + [i for f in faces for i in f]
+        # concave shapes: triangle meshes, then height fields (heights padded to whole xyz triplets)
+        ints.append(len(self.tri_meshes))
+        for v, t, n in self.tri_meshes:
+            ints += [len(v), len(t), 0 if n is None else 1] + [int(i) for i in t.ravel()]
+            verts.append(v.ravel())
+            if n is not None:
+                verts.append(n.ravel())
+        ints.append(len(self.heightfields))
+        for cols, rows, h in self.heightfields:
+            ints += [cols, rows]
+            pad = (-h.size) % 3
+            verts.append(np.concatenate([h.ravel(), np.zeros(pad, np.float32)]))
         vs = np.concatenate(verts) if verts else np.zeros(3, np.float32)
         return np.ascontiguousarray(vs, np.float32), np.array(ints, np.int32)
 
@@ -481,6 +513,68 @@ def hull_scene(n_hulls=16384, n_chains=256, chain_length=16, extent=(120.0, 120.
         sc.add_collider(b, HULL, (float(h),), density=1000.0)
     if n_chains:
         sphere_chains(n_chains, chain_length, radius=0.3, spacing=3.0, seed=seed + 1, sc=sc)
+    return sc
+
+
+def terrain_mesh(nx=12, ny=12, cell=1.0, amp=0.35, seed=9):
+    """A bumpy grid terrain as a triangle mesh in the xy plane (z up): (nx + 1) x (ny + 1) vertices
+    with seeded heights ~U[-amp, amp], two counter-clockwise triangles per cell with the diagonal
+    alternating from cell to cell."""
+    rng = SplitMix64(seed)
+    v = np.array([((i - nx / 2) * cell, (j - ny / 2) * cell, rng.uniform(-amp, amp))
+                  for j in range(ny + 1) for i in range(nx + 1)], np.float32)
+    t = []
+    for j in range(ny):
+        for i in range(nx):
+            a, b, c, d = j * (nx + 1) + i, j * (nx + 1) + i + 1, (j + 1) * (nx + 1) + i + 1, (j + 1) * (nx + 1) + i
+            if (i + j) % 2 == 0:
+                t += [(a, b, c), (a, c, d)]
+            else:
+                t += [(a, b, d), (b, c, d)]
+    return v, np.array(t, np.int32)
+
+
+def concave_scene(kind="mesh", n_bodies=40, shapes=(SPHERE, CAPSULE, BOX, HULL), nx=10, scale=(1.0, 1.0, 1.0),
+                  seed=9, height=(1.5, 6.0), normals=False):
+    """Dynamic bodies of the given shapes (cycled) dropped at seeded positions and orientations onto a
+    static concave ground: a bumpy triangle mesh (kind "mesh", z up; normals=True hands the mesh its
+    own vertex normals instead of letting the library compute them) or a height field (kind
+    "heightfield", y up, gravity -y) of nx x nx cells, with the given shape scale."""
+    rng = SplitMix64(seed)
+    up = 2 if kind == "mesh" else 1
+    g = [0.0, 0.0, 0.0]
+    g[up] = -9.81
+    sc = Scene(f"concave_{kind}_{n_bodies}", make_settings(gravity=tuple(g)))
+    ground = sc.add_body(STATIC, (0.0, 0.0, 0.0))
+    if kind == "mesh":
+        v, t = terrain_mesh(nx, nx, seed=seed + 1)
+        n = None
+        if normals:
+            n = np.zeros_like(v)
+            n[:, 2] = 1.0
+            n[:, 0] = 0.05 * np.sin(v[:, 0])
+            n /= np.linalg.norm(n, axis=1, keepdims=True)
+        m = sc.add_triangle_mesh(v, t, n)
+        sc.add_collider(ground, CONCAVE_MESH, (float(m),) + tuple(scale))
+    else:
+        h = np.array([[rng.uniform(-0.35, 0.35) for _ in range(nx + 1)] for _ in range(nx + 1)], np.float32)
+        f = sc.add_heightfield(h)
+        sc.add_collider(ground, HEIGHTFIELD, (float(f),) + tuple(scale))
+    half = 0.5 * nx * min(scale[0], scale[2] if kind != "mesh" else scale[1]) - 1.0
+    for k in range(n_bodies):
+        shape = shapes[k % len(shapes)]
+        pos = [rng.uniform(-half, half), rng.uniform(-half, half), rng.uniform(-half, half)]
+        pos[up] = rng.uniform(height[0], height[1])
+        b = sc.add_body(DYNAMIC, tuple(pos), rng.quaternion())
+        if shape == SPHERE:
+            sc.add_collider(b, SPHERE, (rng.uniform(0.2, 0.5),))
+        elif shape == CAPSULE:
+            sc.add_collider(b, CAPSULE, (rng.uniform(0.15, 0.3), rng.uniform(0.3, 0.8)))
+        elif shape == BOX:
+            sc.add_collider(b, BOX, (rng.uniform(0.2, 0.5), rng.uniform(0.2, 0.5), rng.uniform(0.2, 0.5)))
+        else:
+            hh = sc.add_hull_from_points(ellipsoid_points(rng, 16, 0.25, 0.5))
+            sc.add_collider(b, HULL, (float(hh),))
     return sc
 
 
