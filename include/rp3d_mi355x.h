@@ -323,6 +323,8 @@ typedef struct rp3dg_contact_view {
     float* local2;          /* 12 per manifold */
     float* impulse;         /* 4 per manifold */
     float* friction;        /* 3 per manifold: friction1, friction2, twist impulses */
+    int32_t* manifold;      /* 1 per manifold: its index among its pair's manifolds (0 but for a pair of a
+                             * convex and a concave shape, which has up to 4, ContactPair.h) */
 } rp3dg_contact_view;
 int rp3dg_batch_get_contacts(rp3dg_batch* batch, rp3dg_contact_view* out, int64_t capacity);
 /* Contact events, what PhysicsWorld::setEventListener + EventListener::onContact deliver

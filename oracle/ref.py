@@ -266,9 +266,13 @@ class RefWorld:
         self.L.rp3dref_get_sleeping(self.h, _ip(out))
         return out
 
-    def step_ordered(self, dt, order_c1, order_c2):
+    def step_ordered(self, dt, order_c1, order_c2, order_sub=None):
+        """order_sub: index of each manifold among its pair's (a convex vs concave pair has up to four),
+        carried in bits 28-31 of collider 2."""
         o1 = np.ascontiguousarray(order_c1, np.int32)
         o2 = np.ascontiguousarray(order_c2, np.int32)
+        if order_sub is not None:
+            o2 = np.ascontiguousarray(o2 | (np.asarray(order_sub, np.int32) << 28), np.int32)
         rc = self.L.rp3dref_step_ordered(self.h, float(dt), _ip(o1), _ip(o2), len(o1))
         if rc != 0:
             raise RuntimeError("coloured-order step: a reference manifold is missing from the given order")
@@ -357,7 +361,15 @@ class RefWorld:
                                         _fp(out["normal_depth"]), _fp(out["local1"]), _fp(out["local2"]),
                                         _fp(out["impulse"]), _fp(out["friction"]), cap)
         assert n >= 0
-        return {k: a[:n] for k, a in out.items()}
+        out = {k: a[:n] for k, a in out.items()}
+        # index of each manifold among its pair's (created one after the other)
+        seen = {}
+        sub = np.zeros(n, np.int32)
+        for m, (a, b) in enumerate(out["colliders"]):
+            sub[m] = seen.get((int(a), int(b)), 0)
+            seen[(int(a), int(b))] = sub[m] + 1
+        out["manifold"] = sub
+        return out
 
 
 class RefBatch:

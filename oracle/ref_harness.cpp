@@ -21,6 +21,7 @@
 #include <cmath>
 #include <cstring>
 #include <thread>
+#include <tuple>
 #include <map>
 #include <unordered_map>
 #include <vector>
@@ -387,9 +388,18 @@ static int solver_init_ordered(RefWorld* R, float dt, const int32_t* oc1, const 
     const uint32 nbIslands = W.mIslands.getNbIslands();
     for (uint32 i = 0; i < nbIslands; i++)
         if (W.mIslands.nbContactManifolds[i] > 0) CS.initializeForIsland(i);
-    // permutation
-    std::map<std::pair<int, int>, int> pos;
-    for (int i = 0; i < n; i++) pos[{oc1[i], oc2[i]}] = i;
+    // permutation; a pair of a convex and a concave shape has up to four manifolds, created one after the
+    // other: oc2's bits 28-31 carry the manifold's index among its pair's (its sub-index)
+    std::map<std::tuple<int, int, int>, int> pos;
+    for (int i = 0; i < n; i++) pos[std::make_tuple(oc1[i], oc2[i] & 0x0fffffff, (int)((uint32_t)oc2[i] >> 28))] = i;
+    std::vector<int> msub(nbM, 0);
+    {
+        std::map<std::pair<int, int>, int> seen;
+        for (uint32 c = 0; c < nbM; c++) {
+            const ContactManifold& cm = (*manifolds)[c];
+            msub[c] = seen[{collider_index(R, cm.colliderEntity1), collider_index(R, cm.colliderEntity2)}]++;
+        }
+    }
     const uint32 m = CS.mNbContactManifolds;
     std::vector<uint32> first(m);
     uint32 acc = 0;
@@ -399,7 +409,7 @@ static int solver_init_ordered(RefWorld* R, float dt, const int32_t* oc1, const 
     for (uint32 c = 0; c < m; c++) {
         const ContactManifold* em = CS.mContactConstraints[c].externalContactManifold;
         const int a = collider_index(R, em->colliderEntity1), b = collider_index(R, em->colliderEntity2);
-        auto it = pos.find({a, b});
+        auto it = pos.find(std::make_tuple(a, b, msub[(size_t)(em - &(*manifolds)[0])]));
         if (it == pos.end()) missing = true;
         keyed[c] = {it == pos.end() ? 0 : it->second, c};
     }
@@ -1017,16 +1027,16 @@ int rp3dref_get_pairs(void* h, int32_t* c1, int32_t* c2, int cap) {
     RefWorld* R = (RefWorld*)h;
     auto& OP = R->world->mCollisionDetection.mOverlappingPairs;
     const int ne = (int)OP.mConvexPairs.size(), nd = (int)OP.mDisabledConvexPairs.size();
-    if (ne + nd > cap) return -(ne + nd);
-    for (int i = 0; i < ne; i++) {
-        c1[i] = collider_index(R, OP.mConvexPairs[i].collider1);
-        c2[i] = collider_index(R, OP.mConvexPairs[i].collider2);
-    }
-    for (int i = 0; i < nd; i++) {
-        c1[ne + i] = collider_index(R, OP.mDisabledConvexPairs[i].collider1);
-        c2[ne + i] = collider_index(R, OP.mDisabledConvexPairs[i].collider2);
-    }
-    return ne + nd;
+    const int nce = (int)OP.mConcavePairs.size(), ncd = (int)OP.mDisabledConcavePairs.size();
+    const int n = ne + nd + nce + ncd;
+    if (n > cap) return -n;
+    int o = 0;
+    auto put = [&](Entity a, Entity b) { c1[o] = collider_index(R, a); c2[o] = collider_index(R, b); o++; };
+    for (int i = 0; i < ne; i++) put(OP.mConvexPairs[i].collider1, OP.mConvexPairs[i].collider2);
+    for (int i = 0; i < nd; i++) put(OP.mDisabledConvexPairs[i].collider1, OP.mDisabledConvexPairs[i].collider2);
+    for (int i = 0; i < nce; i++) put(OP.mConcavePairs[i].collider1, OP.mConcavePairs[i].collider2);
+    for (int i = 0; i < ncd; i++) put(OP.mDisabledConcavePairs[i].collider1, OP.mDisabledConcavePairs[i].collider2);
+    return n;
 }
 
 // Contacts of the last step: per manifold (island order) collider1/2, nb points, friction impulses

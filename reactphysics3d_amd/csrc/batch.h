@@ -139,6 +139,19 @@ struct rp3dg_batch {
 
     // hull arena
     rp3dg::HullHeader* d_hulls = nullptr;
+    // concave shapes (ConcaveInst per concave collider, mesh / height field arenas), the triangle tests
+    // of the step and of the last step (alternating with the pair tables: CTa goes with Pa), their
+    // contact points, and the tail entries of the pair tables for concave pairs' extra manifolds
+    bool has_concave = false;
+    rp3dg::ConcaveInst* d_cinst = nullptr;
+    float4 *d_cverts = nullptr, *d_cnorm = nullptr, *d_cnodes = nullptr;
+    int4* d_ctris = nullptr;
+    float* d_cheights = nullptr;
+    rp3dg::ConcaveTests CTa{}, CTb{};
+    float4 *cp_nd = nullptr, *cp_l1 = nullptr, *cp_l2 = nullptr;
+    int cp_cap = 0;
+    int tail_cap = 0;
+    int man_cap = 0;           // contact manifolds of a step: pair_cap + tail_cap
     float4 *d_hverts = nullptr, *d_hnormals = nullptr, *d_hbounds = nullptr;
     int4* d_hedges = nullptr;
     int *d_hfedge = nullptr, *d_hfverts = nullptr, *d_hfvidx = nullptr;

@@ -273,22 +273,42 @@ int rp3dg_batch::sort(uint64_t** keys, int** vals, uint64_t* ktmp, int* vtmp, in
     return RP3DG_OK;
 }
 
-static void alloc_pairs(rp3dg_batch* b, DevPairs& P, int cap) {
+// tail: entries past the pairs for the extra contact manifolds of concave pairs (their col and contact
+// cache arrays only)
+static void alloc_pairs(rp3dg_batch* b, DevPairs& P, int cap, int tail) {
+    const size_t ct = (size_t)cap + tail;
     P.key = dalloc<uint64_t>(b, cap);
-    P.col = dalloc<int2>(b, cap);
+    P.col = dalloc<int2>(b, ct);
     P.algo = dalloc<int>(b, cap);
     P.lf = dalloc<uint32_t>(b, cap);
     P.lfaxis = dalloc<float4>(b, cap);
-    P.ncp = dalloc<int>(b, cap);
+    P.ncp = dalloc<int>(b, ct);
     P.prev = dalloc<int>(b, cap);
-    P.fv1 = dalloc<float4>(b, cap);
-    P.fv2 = dalloc<float4>(b, cap);
-    P.fimp = dalloc<float4>(b, cap);
-    P.pn = dalloc<float4>(b, 4 * (size_t)cap);
-    P.pl1 = dalloc<float4>(b, 4 * (size_t)cap);
-    P.pl2 = dalloc<float4>(b, 4 * (size_t)cap);
+    P.fv1 = dalloc<float4>(b, ct);
+    P.fv2 = dalloc<float4>(b, ct);
+    P.fimp = dalloc<float4>(b, ct);
+    P.pn = dalloc<float4>(b, 4 * ct);
+    P.pl1 = dalloc<float4>(b, 4 * ct);
+    P.pl2 = dalloc<float4>(b, 4 * ct);
     P.evf = dalloc<int>(b, cap);
     P.kept = dalloc<int>(b, cap);
+    P.ct0 = P.ctn = P.xm = P.xs = nullptr;
+    if (b->has_concave) {
+        P.ct0 = dalloc<int>(b, cap); P.ctn = dalloc<int>(b, cap);
+        P.xm = dalloc<int>(b, cap); P.xs = dalloc<int>(b, cap);
+        if (P.xs) {
+            hipMemset(P.ctn, 0, sizeof(int) * cap);
+            hipMemset(P.xm, 0, sizeof(int) * cap);
+        }
+    }
+}
+
+static void alloc_tests(rp3dg_batch* b, ConcaveTests& T, int cap) {
+    T.cap = cap;
+    T.pair = dalloc<int>(b, cap); T.tri = dalloc<int>(b, cap);
+    T.lf = dalloc<uint32_t>(b, cap); T.lfaxis = dalloc<float4>(b, cap);
+    T.hit = dalloc<int>(b, cap); T.pts = dalloc<int2>(b, cap);
+    T.count = nullptr;   // the step's counter (np_args), the counters block is allocated later
 }
 
 // Box half-edge structure (PhysicsCommon.cpp:73-109 face lists), numbered by the reference rule.
@@ -513,6 +533,13 @@ int rp3dg_batch::upload(rp3dg_world* const* ws, int nw, const rp3dg_batch_option
     std::vector<int4> hedges;
     std::vector<int> hfedge, hfverts, hfvidx;
     std::vector<uint64_t> nocol;
+    // concave shapes: one instance per concave collider (its scale), the meshes and height fields of its
+    // world once per batch; the collider's shape.x is fixed up to its bounds slot after the hulls
+    std::vector<ConcaveInst> cinst;
+    std::vector<float4> cverts, cnorm, cnodes, cbounds;
+    std::vector<int4> ctris;
+    std::vector<float> cheights;
+    std::vector<int> cinst_collider;
     body_base.clear();
     collider_base.clear();
     for (int wi = 0; wi < nw; wi++) {
@@ -558,6 +585,49 @@ int rp3dg_batch::upload(rp3dg_world* const* ws, int nw, const rp3dg_batch_option
             float4 s = make_float4(c.params[0], c.params[1], c.params[2], c.params[3]);
             if (c.type == SHAPE_CAPSULE) s = make_float4(c.params[0], c.params[1] * 0.5f, 0.f, 0.f);
             if (c.type == SHAPE_HULL) s = make_float4((float)(h0 + (int)c.params[0]), 0.f, 0.f, 0.f);
+            if (shape_is_concave(c.type)) {
+                if (W.bodies[c.body].type == RP3DG_DYNAMIC) {
+                    set_error("concave shapes (triangle mesh, height field) go on static or kinematic bodies only");
+                    return RP3DG_EUNSUPPORTED;
+                }
+                ConcaveInst I{};
+                I.kind = c.type;
+                I.sx = c.params[1]; I.sy = c.params[2]; I.sz = c.params[3];
+                I.isx = 1.0f / I.sx; I.isy = 1.0f / I.sy; I.isz = 1.0f / I.sz;
+                if (c.type == SHAPE_CONCAVE_MESH) {
+                    const HostTriMesh& M = W.meshes[(int)c.params[0]];
+                    I.vbase = (int)cverts.size();
+                    I.tbase = (int)ctris.size();
+                    I.nbase = (int)cnorm.size();
+                    I.node_base = (int)cnodes.size() / 2;
+                    I.root = M.root;
+                    for (const V3& vv : M.vertices) cverts.push_back(make_float4(vv.x, vv.y, vv.z, 0.f));
+                    for (size_t t3 = 0; t3 + 2 < M.tris.size(); t3 += 3)
+                        ctris.push_back(make_int4(M.tris[t3], M.tris[t3 + 1], M.tris[t3 + 2], 0));
+                    for (const V3& nn : concave_scaled_normals(M, v3(I.sx, I.sy, I.sz)))
+                        cnorm.push_back(make_float4(nn.x, nn.y, nn.z, 0.f));
+                    for (const HostTriMesh::Node& nd : M.nodes) {
+                        auto bits = [](int v) { float f; memcpy(&f, &v, sizeof f); return f; };
+                        cnodes.push_back(make_float4(nd.mn.x, nd.mn.y, nd.mn.z, bits(nd.child0)));
+                        cnodes.push_back(make_float4(nd.mx.x, nd.mx.y, nd.mx.z, bits(nd.child0 < 0 ? nd.data : nd.child1)));
+                    }
+                } else {
+                    const HostHeightField& H = W.heightfields[(int)c.params[0]];
+                    I.columns = H.columns; I.rows = H.rows;
+                    I.hbase = (int)cheights.size();
+                    I.width = H.width; I.length = H.length; I.origin = H.origin;
+                    I.bminx = H.bmin.x; I.bminy = H.bmin.y; I.bminz = H.bmin.z;
+                    I.bmaxx = H.bmax.x; I.bmaxy = H.bmax.y; I.bmaxz = H.bmax.z;
+                    cheights.insert(cheights.end(), H.heights.begin(), H.heights.end());
+                }
+                V3 lo, hi;
+                concave_local_bounds(W, c, lo, hi);
+                cbounds.push_back(make_float4(lo.x, lo.y, lo.z, 0.f));
+                cbounds.push_back(make_float4(hi.x, hi.y, hi.z, 0.f));
+                s = make_float4(0.f, (float)cinst.size(), 0.f, 0.f);
+                cinst_collider.push_back((int)cbody.size() - 1);
+                cinst.push_back(I);
+            }
             shape.push_back(s);
             lpos.push_back(make_float4(c.local.p.x, c.local.p.y, c.local.p.z, 0.f));
             lq.push_back(make_float4(c.local.q.x, c.local.q.y, c.local.q.z, c.local.q.w));
@@ -618,6 +688,10 @@ int rp3dg_batch::upload(rp3dg_world* const* ws, int nw, const rp3dg_batch_option
     }
     std::sort(nocol.begin(), nocol.end());
     nocol.erase(std::unique(nocol.begin(), nocol.end()), nocol.end());
+    // concave local bounds follow the hulls' in the bounds table
+    const int nhull_slots = (int)hh.size();
+    for (size_t i = 0; i < cinst.size(); i++) shape[cinst_collider[i]].x = (float)(nhull_slots + (int)i);
+    has_concave = !cinst.empty();
     nb = (int)x.size();
     nc = (int)cbody.size();
     nworlds = nw;
@@ -725,8 +799,20 @@ int rp3dg_batch::upload(rp3dg_world* const* ws, int nw, const rp3dg_batch_option
     HIPCHECK(up(C.kind, ckind.data(), nc * 4));
     coll_kind = ckind;
     HIPCHECK(up(C.world, cworld.data(), nc * 4));
-    // hulls (at least one dummy record so pointers are valid)
+    // hulls (at least one dummy record so pointers are valid), then the concave shapes' bounds
     if (hh.empty()) { hh.push_back(HullHeader{}); hbounds.resize(2); }
+    if (has_concave) {
+        hbounds.resize(2 * (size_t)nhull_slots);
+        hbounds.insert(hbounds.end(), cbounds.begin(), cbounds.end());
+        auto upv = [&](auto*& d, const auto& v) {
+            using T = typename std::decay<decltype(v)>::type::value_type;
+            d = dalloc<T>(this, std::max<size_t>(1, v.size()));
+            if (d && !v.empty()) return hipMemcpy(d, v.data(), sizeof(T) * v.size(), hipMemcpyHostToDevice);
+            return d ? hipSuccess : hipErrorOutOfMemory;
+        };
+        HIPCHECK(upv(d_cinst, cinst)); HIPCHECK(upv(d_cverts, cverts)); HIPCHECK(upv(d_ctris, ctris));
+        HIPCHECK(upv(d_cnorm, cnorm)); HIPCHECK(upv(d_cnodes, cnodes)); HIPCHECK(upv(d_cheights, cheights));
+    }
     d_hulls = dalloc<HullHeader>(this, hh.size());
     d_hverts = dalloc<float4>(this, hverts.size());
     d_hnormals = dalloc<float4>(this, hnormals.size());
@@ -790,8 +876,28 @@ int rp3dg_batch::upload(rp3dg_world* const* ws, int nw, const rp3dg_batch_option
 
     // broadphase / pair / solver arenas
     pair_cap = std::max(4096, (o.pairs_per_collider ? o.pairs_per_collider : 16) * nc);
-    alloc_pairs(this, Pa, pair_cap);
-    alloc_pairs(this, Pb, pair_cap);
+    tail_cap = has_concave ? pair_cap : 0;
+    man_cap = pair_cap + tail_cap;
+    alloc_pairs(this, Pa, pair_cap, tail_cap);
+    alloc_pairs(this, Pb, pair_cap, tail_cap);
+    if (has_concave) {
+        const int ct_cap = std::max(1 << 16, 16 * nc);
+        alloc_tests(this, CTa, ct_cap);
+        alloc_tests(this, CTb, ct_cap);
+        cp_cap = 2 * ct_cap;
+        cp_nd = dalloc<float4>(this, cp_cap); cp_l1 = dalloc<float4>(this, cp_cap); cp_l2 = dalloc<float4>(this, cp_cap);
+        if (!cp_l2 || !CTb.pts || !Pb.xs) { set_error("device allocation failed (concave shapes)"); return RP3DG_EHIP; }
+        // TriangleShape's half-edge structure (PhysicsCommon.cpp initTriangleShapeHalfEdgeStructure)
+        HalfEdgeMesh tri;
+        tri.faces = {{0, 1, 2}, {0, 2, 1}};
+        std::string err;
+        if (!tri.build(3, &err)) { set_error("triangle topology"); return RP3DG_EINVAL; }
+        std::vector<int> t4, tfe(2), tfv;
+        for (auto& e : tri.edges) { t4.push_back(e.vertex); t4.push_back(e.twin); t4.push_back(e.face); t4.push_back(e.next); }
+        for (int f = 0; f < 2; f++) tfe[f] = tri.face_edge[f];
+        for (auto& f : tri.faces) for (int vv : f) tfv.push_back(vv);
+        upload_triangle_topology(t4.data(), tfe.data(), tfv.data());
+    }
     const int nn = std::max(1, 2 * nc);
     bp_keys = dalloc<uint64_t>(this, nc); bp_vals = dalloc<int>(this, nc);
     bp_ktmp = dalloc<uint64_t>(this, nc); bp_vtmp = dalloc<int>(this, nc);
@@ -815,12 +921,12 @@ int rp3dg_batch::upload(rp3dg_world* const* ws, int nw, const rp3dg_batch_option
     np_lists = dalloc<int>(this, (size_t)kNpListSlots * pair_cap);
     np_counts = dalloc<int>(this, ALGO_COUNT);
     m_flag = dalloc<int>(this, pair_cap); m_offs = dalloc<int>(this, pair_cap + 1);
-    m_list = dalloc<int>(this, pair_cap);
+    m_list = dalloc<int>(this, man_cap);
     adj_cnt = dalloc<int>(this, nb + 1); adj_offs = dalloc<int>(this, nb + 1); adj_cursor = dalloc<int>(this, nb);
-    adj = dalloc<int>(this, 2 * (size_t)pair_cap); mb = dalloc<int2>(this, pair_cap);
-    color = dalloc<int>(this, pair_cap);
-    ckeys = dalloc<uint64_t>(this, pair_cap); cvals = dalloc<int>(this, pair_cap);
-    cktmp = dalloc<uint64_t>(this, pair_cap); cvtmp = dalloc<int>(this, pair_cap);
+    adj = dalloc<int>(this, 2 * (size_t)man_cap); mb = dalloc<int2>(this, man_cap);
+    color = dalloc<int>(this, man_cap);
+    ckeys = dalloc<uint64_t>(this, man_cap); cvals = dalloc<int>(this, man_cap);
+    cktmp = dalloc<uint64_t>(this, man_cap); cvtmp = dalloc<int>(this, man_cap);
     nseg = fused ? ngroups * kMaxColors : kMaxColors;
     seg_cnt = dalloc<int>(this, nseg + 1);
     seg_start = dalloc<int>(this, nseg + 1);
@@ -887,14 +993,14 @@ int rp3dg_batch::upload(rp3dg_world* const* ws, int nw, const rp3dg_batch_option
         HIPCHECK(hipMemsetAsync(world_iters, 0, sizeof(int) * nw, stream));
         HIPCHECK(hipMemsetAsync(world_res, 0, sizeof(float) * nw, stream));
     }
-    S.pair = dalloc<int>(this, pair_cap); S.bodies = dalloc<int2>(this, pair_cap); S.npts = dalloc<int>(this, pair_cap);
+    S.pair = dalloc<int>(this, man_cap); S.bodies = dalloc<int2>(this, man_cap); S.npts = dalloc<int>(this, man_cap);
     for (float4** a : {&S.n, &S.r1f, &S.r2f, &S.t1, &S.t2, &S.imp, &S.oldt1, &S.oldt2})
-        *a = dalloc<float4>(this, pair_cap);
-    for (float4** a : {&S.pn, &S.r1, &S.r2, &S.lam, &S.pred}) *a = dalloc<float4>(this, 4 * (size_t)pair_cap);
+        *a = dalloc<float4>(this, man_cap);
+    for (float4** a : {&S.pn, &S.r1, &S.r2, &S.lam, &S.pred}) *a = dalloc<float4>(this, 4 * (size_t)man_cap);
     scan_sums_cap = 1 << 16;
     scan_sums_buf = dalloc<int>(this, scan_sums_cap);
-    sort_hist_cap = 256 * (std::max(pair_cap, nc) / 4096 + 2);
-    if ((sort_hist_cap + kScanTile - 1) / kScanTile > scan_sums_cap || (pair_cap + kScanTile - 1) / kScanTile > scan_sums_cap) {
+    sort_hist_cap = 256 * (std::max(man_cap, nc) / 4096 + 2);
+    if ((sort_hist_cap + kScanTile - 1) / kScanTile > scan_sums_cap || (man_cap + kScanTile - 1) / kScanTile > scan_sums_cap) {
         set_error("batch too large for the scan workspace");
         return RP3DG_EOVERFLOW;
     }
@@ -929,6 +1035,7 @@ int rp3dg_batch::upload(rp3dg_world* const* ws, int nw, const rp3dg_batch_option
                     const float4 a0 = amn[i], a1 = amx[i], b0 = amn[j], b1 = amx[j];
                     if (a1.x < b0.x || b1.x < a0.x || a1.y < b0.y || b1.y < a0.y || a1.z < b0.z || b1.z < a0.z) continue;
                     if (cbody[i] == cbody[j]) continue;
+                    if (shape_is_concave(ctype[i]) && shape_is_concave(ctype[j])) continue;
                     const uint32_t fi = filter[i], fj = filter[j];
                     if (((fi >> 16) & (fj & 0xffffu)) == 0 || ((fi & 0xffffu) & (fj >> 16)) == 0) continue;
                     const uint64_t bk = ((uint64_t)(uint32_t)std::min(cbody[i], cbody[j]) << 32) |
@@ -1063,6 +1170,20 @@ NpArgs rp3dg_batch::np_args(const DevPairs& P, const DevPairs& O, int query, int
     A.report = report;
     A.clip_prev = query == NP_STEP ? 1 : 0;           // testNarrowPhaseCollision(..., false) in snapshots
     A.lf_update = report == NP_REPORT_NONE ? 0 : 1;   // overlap snapshots skip processAllPotentialContacts
+    A.cinst = d_cinst; A.cverts = d_cverts; A.ctris = d_ctris; A.cnorm = d_cnorm; A.cnodes = d_cnodes;
+    A.cheights = d_cheights;
+    A.hull_bounds = d_hbounds;
+    const bool na = P.key == Pa.key;   // the test lists alternate with the pair tables
+    A.CT = na ? CTa : CTb;
+    A.CTo = na ? CTb : CTa;
+    A.CT.count = &counters->nb_concave_tests;
+    A.CTo.count = nullptr;   // the last step's tests are read through the previous table's ranges
+    A.cp_nd = cp_nd; A.cp_l1 = cp_l1; A.cp_l2 = cp_l2;
+    A.cp_cap = cp_cap;
+    A.cp_count = &counters->nb_concave_points;
+    A.tail_base = pair_cap;
+    A.tail_cap = tail_cap;
+    A.tail_count = &counters->nb_tails;
     return A;
 }
 
@@ -1089,6 +1210,13 @@ int rp3dg_batch::enqueue_narrowphase(const NpArgs& A, bool capturing, int pg) {
              &counters->nb_box_sep_tests);
     LAUNCH_T("np_poly_poly", hint(last.nb_hull_tests, 128), 128, np_poly_poly, A, L + ALGO_POLY_POLY * pair_cap,
              &counters->nb_hull_tests);
+    if (has_concave) {   // convex vs concave pairs: middle phase, triangle tests, manifolds
+        const int* lc = L + np_list_slot(ALGO_CONCAVE) * pair_cap;
+        LAUNCH_T("np_concave_mid", tg(ALGO_CONCAVE, 128), 128, np_concave_mid, A, lc, T + ALGO_CONCAVE);
+        const int ht = capturing ? A.CT.cap : std::min(A.CT.cap, last.nb_concave_tests + last.nb_concave_tests / 4 + 512);
+        LAUNCH_T("np_concave_tri", grid_for(ht, 128), 128, np_concave_tri, A);
+        LAUNCH_T("np_concave_reduce", tg(ALGO_CONCAVE, 64), 64, np_concave_reduce, A, lc, T + ALGO_CONCAVE);
+    }
 
     return RP3DG_OK;
 }
@@ -1125,12 +1253,12 @@ int rp3dg_batch::enqueue_step(float dt, bool capturing) {
 
     // ---------------- manifold list + colouring (replaces createIslands, PhysicsWorld.cpp:695-866)
     int* nm_ptr = &counters->nb_manifolds;
-    LAUNCH_T("sv_flag", pg, 256, sv_flag_manifolds, N.ncp, &counters->nb_pairs, m_flag);
+    LAUNCH_T("sv_flag", pg, 256, sv_flag_manifolds, N.ncp, N.xm, &counters->nb_pairs, m_flag);
     r = scan(m_flag, pair_cap, &counters->nb_pairs, m_offs, stream);
     if (r) { set_error("scan overflow"); return r; }
-    LAUNCH_T("sv_compact", std::min(pg, 1024), 256, sv_compact_manifolds, m_flag, m_offs, &counters->nb_pairs, N.ncp, m_list, nm_ptr,
-             &counters->nb_points, total_points);
-    const int hm = capturing ? pair_cap : std::min(pair_cap, hint_manifolds + hint_manifolds / 4 + 1024);
+    LAUNCH_T("sv_compact", std::min(pg, 1024), 256, sv_compact_manifolds, m_flag, m_offs, &counters->nb_pairs, N.ncp, N.xm,
+             N.xs, m_list, nm_ptr, &counters->nb_points, total_points);
+    const int hm = capturing ? man_cap : std::min(man_cap, hint_manifolds + hint_manifolds / 4 + 1024);
     const int mg = grid_for(hm);
     if (settings.sleeping_enabled) {
         // ---------------- islands and wake-up (PhysicsWorld::createIslands, PhysicsWorld.cpp:695-866)
@@ -1161,7 +1289,7 @@ int rp3dg_batch::enqueue_step(float dt, bool capturing) {
         // Jones-Plassmann over the whole batch, then a radix sort of the rows by colour
         fill_int<<<grid_for(nb), 256, 0, stream>>>(adj_cursor, nb, 0);
         LAUNCH_T("sv_adj_fill", mg, 256, sv_adj_fill, B, C, N, m_list, nm_ptr, adj_offs, adj_cursor, adj, mb);
-        fill_int_n<<<mg, 256, 0, stream>>>(color, pair_cap, nm_ptr, -1);
+        fill_int_n<<<mg, 256, 0, stream>>>(color, man_cap, nm_ptr, -1);
         const int rounds = capturing ? graph_rounds : std::min(kColorRoundsMax, std::max(4, hint_rounds));
         for (int q = 0; q < rounds; q++)
             LAUNCH_T("sv_color_round", mg, 256, sv_color_round, mb, nm_ptr, adj_offs, adj, color,
@@ -1174,7 +1302,7 @@ int rp3dg_batch::enqueue_step(float dt, bool capturing) {
         int sbits = 0;
         while ((1 << sbits) < nseg) sbits++;
         timer_begin("sv_color_sort");
-        r = sort(&ck, &cv, cktmp, cvtmp, pair_cap, nm_ptr, sbits, stream);
+        r = sort(&ck, &cv, cktmp, cvtmp, man_cap, nm_ptr, sbits, stream);
         timer_end();
         if (r) { set_error("sort overflow"); return r; }
         r = scan(seg_cnt, nseg, nullptr, seg_start, stream);
@@ -1439,6 +1567,10 @@ int rp3dg_batch::step_graph(float dt, int n) {
 // apart in the q_* arrays: q_ncp[k] > 0 points, -1 colliding without points, 0 not colliding.
 int rp3dg_batch::query(int mode, int report, int body1, int body2) {
     if (nb == 0) return RP3DG_OK;
+    if (has_concave) {
+        set_error("world queries (testOverlap / testCollision) do not take batches with concave shapes yet");
+        return RP3DG_EUNSUPPORTED;
+    }
     int r = flush_api();
     if (r) return r;
     if (!q_ncp) {
@@ -1742,6 +1874,7 @@ int rp3dg_batch_raycast(rp3dg_batch* b, int64_t n, const int32_t* world, const f
     if (!b || n < 0 || !point1 || !point2) return RP3DG_EINVAL;
     if (n == 0) return RP3DG_OK;
     if (n > (1 << 26)) { set_error("at most 2^26 rays per call"); return RP3DG_EINVAL; }
+    if (b->has_concave) { set_error("raycasts do not take batches with concave shapes yet"); return RP3DG_EUNSUPPORTED; }
     if (const int fr = b->flush_api()) return fr;
     std::vector<int> wv((size_t)n);
     std::vector<float4> a((size_t)n), c((size_t)n);
@@ -1795,6 +1928,7 @@ int64_t rp3dg_batch_raycast_all(rp3dg_batch* b, int64_t n, const int32_t* world,
                                 int32_t* collider, float* point, float* normal, float* fraction, int64_t capacity) {
     if (!b || n < 0 || !point1 || !point2 || capacity < 0) { set_error("raycast_all: invalid arguments"); return RP3DG_EINVAL; }
     if (n == 0) return 0;
+    if (b->has_concave) { set_error("raycasts do not take batches with concave shapes yet"); return RP3DG_EUNSUPPORTED; }
     if (n > (1 << 26)) { set_error("at most 2^26 rays per call"); return RP3DG_EINVAL; }
     if (const int fr = b->flush_api()) return fr;
     std::vector<int> wv((size_t)n);
@@ -2171,18 +2305,38 @@ int rp3dg_batch_get_contacts(rp3dg_batch* b, rp3dg_contact_view* o, int64_t cap)
     std::vector<uint64_t> ck(nm);
     HIPCHECK(hipMemcpy(pairs.data(), b->S.pair, sizeof(int) * nm, hipMemcpyDeviceToHost));
     HIPCHECK(hipMemcpy(ck.data(), b->sorted_keys, sizeof(uint64_t) * nm, hipMemcpyDeviceToHost));
+    // the pair entries, then the tail entries of concave pairs' extra manifolds packed after them
     const int np = b->contacts_np;
-    std::vector<int2> col(np);
-    std::vector<int> ncp(np);
-    std::vector<float4> pn(4 * (size_t)np), pl1(4 * (size_t)np), pl2(4 * (size_t)np), fimp(np);
-    HIPCHECK(hipMemcpy(col.data(), P.col, sizeof(int2) * np, hipMemcpyDeviceToHost));
-    HIPCHECK(hipMemcpy(ncp.data(), P.ncp, sizeof(int) * np, hipMemcpyDeviceToHost));
-    HIPCHECK(hipMemcpy(pn.data(), P.pn, sizeof(float4) * 4 * np, hipMemcpyDeviceToHost));
-    HIPCHECK(hipMemcpy(pl1.data(), P.pl1, sizeof(float4) * 4 * np, hipMemcpyDeviceToHost));
-    HIPCHECK(hipMemcpy(pl2.data(), P.pl2, sizeof(float4) * 4 * np, hipMemcpyDeviceToHost));
-    HIPCHECK(hipMemcpy(fimp.data(), P.fimp, sizeof(float4) * np, hipMemcpyDeviceToHost));
+    const int nt = b->has_concave ? std::min(b->last.nb_tails, b->tail_cap) : 0;
+    const size_t ne = (size_t)np + nt;
+    std::vector<int2> col(ne);
+    std::vector<int> ncp(ne), sub(ne, 0);
+    std::vector<float4> pn(4 * ne), pl1(4 * ne), pl2(4 * ne), fimp(ne);
+    auto fetch = [&](size_t dst, size_t src, size_t n) -> hipError_t {
+        hipError_t e;
+        if (!n) return hipSuccess;
+        if ((e = hipMemcpy(col.data() + dst, P.col + src, sizeof(int2) * n, hipMemcpyDeviceToHost))) return e;
+        if ((e = hipMemcpy(ncp.data() + dst, P.ncp + src, sizeof(int) * n, hipMemcpyDeviceToHost))) return e;
+        if ((e = hipMemcpy(pn.data() + 4 * dst, P.pn + 4 * src, sizeof(float4) * 4 * n, hipMemcpyDeviceToHost))) return e;
+        if ((e = hipMemcpy(pl1.data() + 4 * dst, P.pl1 + 4 * src, sizeof(float4) * 4 * n, hipMemcpyDeviceToHost))) return e;
+        if ((e = hipMemcpy(pl2.data() + 4 * dst, P.pl2 + 4 * src, sizeof(float4) * 4 * n, hipMemcpyDeviceToHost))) return e;
+        return hipMemcpy(fimp.data() + dst, P.fimp + src, sizeof(float4) * n, hipMemcpyDeviceToHost);
+    };
+    HIPCHECK(fetch(0, 0, np));
+    HIPCHECK(fetch(np, b->pair_cap, nt));
+    if (nt) {   // index of each tail manifold among its pair's manifolds
+        std::vector<int> xm(np), xs(np);
+        HIPCHECK(hipMemcpy(xm.data(), P.xm, sizeof(int) * np, hipMemcpyDeviceToHost));
+        HIPCHECK(hipMemcpy(xs.data(), P.xs, sizeof(int) * np, hipMemcpyDeviceToHost));
+        for (int k = 0; k < np; k++)
+            for (int j = 0; j < xm[k]; j++) {
+                const int t = xs[k] + j - b->pair_cap;
+                if (t >= 0 && t < nt) sub[np + t] = j + 1;
+            }
+    }
     for (int m = 0; m < nm; m++) {
-        const int k = pairs[m];
+        const int k = pairs[m] >= b->pair_cap ? np + pairs[m] - b->pair_cap : pairs[m];
+        if (o->manifold) o->manifold[m] = sub[k];
         if (o->colliders) { o->colliders[2 * m] = col[k].x; o->colliders[2 * m + 1] = col[k].y; }
         if (o->nb_points) o->nb_points[m] = ncp[k];
         if (o->color) o->color[m] = (int)(ck[m] % kMaxColors);

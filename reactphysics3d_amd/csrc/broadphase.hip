@@ -342,12 +342,13 @@ __device__ __forceinline__ bool filters_pass(uint32_t fa, uint32_t fb) {
     return (maska & catb) != 0 && (cata & maskb) != 0;
 }
 
-// Pair filter of CollisionDetectionSystem::updateOverlappingPairs (CD.cpp:284-327): different bodies, no
-// joint excluding the pair, collision category / mask
+// Pair filter of CollisionDetectionSystem::updateOverlappingPairs (CD.cpp:284-331): different bodies, no
+// joint excluding the pair, collision category / mask, at least one convex shape (:324-331)
 __device__ __forceinline__ bool pair_allowed(const DevColliders& C, int a, int b, const NoColLists& L) {
     const int ba = C.body[a], bb = C.body[b];
     if (ba == bb) return false;
     if (no_collision(L, ba, bb)) return false;
+    if (shape_is_concave(C.type[a]) && shape_is_concave(C.type[b])) return false;
     return filters_pass(C.filter[a], C.filter[b]);
 }
 
@@ -503,6 +504,7 @@ __global__ __launch_bounds__(256) void bp_world_scan(DevColliders C, const int* 
         const int w = C.world[c];
         const int j0 = wcoll[w] - gc0, j1 = wcoll[w + 1] - gc0;
         const uint32_t fc = __float_as_uint(amx.w);
+        const bool concave = shape_is_concave(C.type[c]);   // concave vs concave pairs are not created
         const int n0 = L.offs[bc], n1 = L.offs[bc + 1];   // bodies this collider's body must not collide with
         int cnt = 0;
         for (int j = j0; j < j1; j++) {
@@ -515,6 +517,7 @@ __global__ __launch_bounds__(256) void bp_world_scan(DevColliders C, const int* 
             bool excluded = false;
             for (int k = n0; k < n1; k++) excluded |= L.list[k] == bb;
             if (excluded || !filters_pass(fc, __float_as_uint(bmx.w))) continue;
+            if (concave && shape_is_concave(C.type[gc0 + j])) continue;
             if (pass) {
                 if (base + cnt < capacity) out_keys[base + cnt] = ((uint64_t)(uint32_t)c << 32) | (uint32_t)(gc0 + j);
                 else atomicOr(status, STATUS_PAIR_OVERFLOW);
@@ -535,6 +538,7 @@ __global__ __launch_bounds__(256) void bp_world_scan(DevColliders C, const int* 
 // CollisionDispatch::selectAlgorithm (CollisionDispatch.cpp:80-115) on CollisionShapeType
 __device__ __forceinline__ int shape_kind(int t) { return t == SHAPE_SPHERE ? 0 : (t == SHAPE_CAPSULE ? 1 : 2); }
 __device__ __forceinline__ int select_algo(int t1, int t2) {
+    if (shape_is_concave(t1) || shape_is_concave(t2)) return ALGO_CONCAVE;   // OverlappingPairs.cpp:427-440
     int a = shape_kind(t1), b = shape_kind(t2);
     if (a > b) { const int tmp = a; a = b; b = tmp; }
     if (a == 0 && b == 0) return ALGO_SPHERE_SPHERE;

@@ -75,7 +75,9 @@ __global__ void ev_update(DevBodies B, DevColliders C, DevPairs N, const int* nb
         const int k1 = C.kind[c.x], k2 = C.kind[c.y];
         const bool active = (k1 & (CK_SIMULATION | CK_TRIGGER)) && (k2 & (CK_SIMULATION | CK_TRIGGER));
         const bool trig = ((k1 | k2) & CK_TRIGGER) != 0;
-        const int np = N.ncp[k];
+        int np = N.ncp[k];
+        if (N.xm && np > 0)   // a concave pair's points: its first manifold's and its tail manifolds'
+            for (int j = 0; j < N.xm[k]; j++) np += N.ncp[N.xs[k] + j];
         const int f = N.evf[k];
         // a trigger pair collides without contact points: its test result is the wasColliding flag
         const bool hit = trig ? (N.lf[k] & LF_WAS_COLLIDING) != 0 : np > 0;

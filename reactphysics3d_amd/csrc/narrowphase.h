@@ -5,6 +5,18 @@
 
 namespace rp3dg {
 
+// The triangle tests of one step's convex vs concave pairs, pair by pair in middle-phase order.
+struct ConcaveTests {
+    int* pair;                 // pair slot
+    int* tri;                  // triangle id (mesh: triangle index; height field: HeightField::computeTriangleShapeId)
+    uint32_t* lf;              // the triangle's LastFrameCollisionInfo after the test
+    float4* lfaxis;
+    int* hit;                  // NarrowPhaseInfo::isColliding
+    int2* pts;                 // its contact points in the point pool (first, count)
+    int cap;
+    int* count;
+};
+
 struct NpArgs {
     DevBodies B;
     DevColliders C;
@@ -27,6 +39,20 @@ struct NpArgs {
     int report;                // NP_REPORT_NO_TRIGGER, NP_REPORT_ALL or NP_REPORT_NONE (reportContacts)
     int clip_prev;             // SATAlgorithm's clipWithPreviousAxisIfStillColliding (1 in steps, 0 in queries)
     int lf_update;             // processPotentialContacts updates wasColliding / isValid (0: overlap queries)
+    // convex vs concave pairs (concave.cpp arenas; null pointers when the batch has no concave collider)
+    const ConcaveInst* cinst;
+    const float4* cverts;      // mesh vertices (unscaled)
+    const int4* ctris;         // mesh triangles: vertex indices
+    const float4* cnorm;       // scaled vertex normals of each concave collider
+    const float4* cnodes;      // mesh AABB tree: (min, child0) (max, child1); a leaf has child0 = -1, child1 = triangle
+    const float* cheights;     // height field samples
+    const float4* hull_bounds; // local bounds of convex meshes (the convex shape's AABB in the concave space)
+    ConcaveTests CT, CTo;      // this step's triangle tests, the previous step's (the pairs' last-frame infos)
+    float4 *cp_nd, *cp_l1, *cp_l2;   // contact points of the tests: normal + depth, local points
+    int cp_cap;
+    int* cp_count;
+    int tail_base, tail_cap;   // tail entries of P: [tail_base, tail_base + tail_cap)
+    int* tail_count;
 };
 constexpr int NP_STEP = 0, NP_QUERY_ALL = 1, NP_QUERY_BODIES = 2;
 constexpr int NP_REPORT_NO_TRIGGER = 0, NP_REPORT_ALL = 1, NP_REPORT_NONE = 2;
@@ -56,5 +82,11 @@ __global__ void np_poly_poly(NpArgs A, const int* list, const int* count);
 constexpr int kNpBoxThreads = 128;
 __global__ void np_box_box(NpArgs A, const int* list, const int* count);
 void upload_box_topology(const int* edges4, const int* face_edge, const int* face_verts);
+void upload_triangle_topology(const int* edges4, const int* face_edge, const int* face_verts);
+// convex vs concave pairs: middle phase (lists the triangle tests), the tests, then per pair the potential
+// manifolds, their reduction and the warm-start transfer
+__global__ void np_concave_mid(NpArgs A, const int* list, const int* count);
+__global__ void np_concave_tri(NpArgs A);
+__global__ void np_concave_reduce(NpArgs A, const int* list, const int* count);
 
 }  // namespace rp3dg

@@ -14,6 +14,7 @@
 
 #include "kcommon.h"
 #include "narrowphase.h"
+#include "refit_dev.h"
 #include "rp3d_internal.h"
 
 namespace rp3dg {
@@ -21,13 +22,19 @@ namespace rp3dg {
 __constant__ int4 c_box_edges[24];   // vertex, twin, face, next (HalfEdgeStructure of the box, PhysicsCommon.cpp:73-109)
 __constant__ int c_box_face_edge[6];
 __constant__ int c_box_face_verts[24];
+__constant__ int4 c_tri_edges[6];     // TriangleShape's half-edges (PhysicsCommon.cpp initTriangleShapeHalfEdgeStructure)
+__constant__ int c_tri_face_edge[2];
+__constant__ int c_tri_face_verts[6];
 
 // ------------------------------------------------------------------------------------------------
-// Polyhedron view: BoxShape (implicit) or ConvexMeshShape (hull arena).
+// Polyhedron view: BoxShape (implicit), ConvexMeshShape (hull arena) or TriangleShape (a concave
+// shape's triangle, vertices held here; TriangleShape.h).
 // ------------------------------------------------------------------------------------------------
+constexpr int kTriPoly = -2;
 struct Poly {
-    int hull;          // -1 for a box
+    int hull;          // -1 for a box, kTriPoly for a triangle
     V3 he;             // box half-extents
+    V3 t0, t1, t2, tn; // triangle vertices and unit normal
     HullHeader h;
     const float4* hv;  // hull vertices
     const float4* hn;  // hull face normals
@@ -36,29 +43,45 @@ struct Poly {
     const int* hfverts;   // [2*f] = start, [2*f+1] = count into hfvidx
     const int* hfvidx;
 
-    __device__ int nfaces() const { return hull < 0 ? 6 : h.nf; }
-    __device__ int nhalfedges() const { return hull < 0 ? 24 : h.ne; }
+    __device__ bool tri() const { return hull == kTriPoly; }
+    __device__ int nfaces() const { return tri() ? 2 : (hull < 0 ? 6 : h.nf); }
+    __device__ int nhalfedges() const { return tri() ? 6 : (hull < 0 ? 24 : h.ne); }
     // BoxShape.h:186-205 (vertices 0-3 on +z, 4-7 on -z; +x for 1, 2, 5, 6; +y for 2, 3, 6, 7), as
     // selects so that lanes holding different indices do not branch
     __device__ V3 vertex(int i) const {
         if (hull >= 0) return xyz(hv[h.vbase + i]);
+        if (tri()) return i == 0 ? t0 : (i == 1 ? t1 : t2);
         return v3(((i + 1) & 2) ? he.x : -he.x, (i & 2) ? he.y : -he.y, i < 4 ? he.z : -he.z);
     }
     // BoxShape.h:208-221 (faces +z, +x, -z, -x, -y, +y)
     __device__ V3 face_normal(int f) const {
         if (hull >= 0) return xyz(hn[h.fbase + f]);
+        if (tri()) return f == 0 ? tn : -tn;   // TriangleShape.h getFaceNormal
         return v3(f == 1 ? 1.0f : f == 3 ? -1.0f : 0.0f, f == 5 ? 1.0f : f == 4 ? -1.0f : 0.0f,
                   f == 0 ? 1.0f : f == 2 ? -1.0f : 0.0f);
     }
-    __device__ int4 edge(int e) const { return hull < 0 ? c_box_edges[e] : hedges[h.ebase + e]; }
-    __device__ int face_edge(int f) const { return hull < 0 ? c_box_face_edge[f] : hfedge[h.fbase + f]; }
-    __device__ int face_nverts(int f) const { return hull < 0 ? 4 : hfverts[2 * (h.fbase + f) + 1]; }
+    __device__ int4 edge(int e) const {
+        return tri() ? c_tri_edges[e] : (hull < 0 ? c_box_edges[e] : hedges[h.ebase + e]);
+    }
+    __device__ int face_edge(int f) const {
+        return tri() ? c_tri_face_edge[f] : (hull < 0 ? c_box_face_edge[f] : hfedge[h.fbase + f]);
+    }
+    __device__ int face_nverts(int f) const { return tri() ? 3 : (hull < 0 ? 4 : hfverts[2 * (h.fbase + f) + 1]); }
     __device__ int face_vertex(int f, int k) const {
+        if (tri()) return c_tri_face_verts[3 * f + k];
         return hull < 0 ? c_box_face_verts[4 * f + k] : hfvidx[hfverts[2 * (h.fbase + f)] + k];
     }
-    __device__ V3 centroid() const { return hull < 0 ? v3zero() : v3(h.cx, h.cy, h.cz); }
+    __device__ V3 centroid() const {
+        if (tri()) return (t0 + t1 + t2) / 3.0f;   // TriangleShape.h getCentroid
+        return hull < 0 ? v3zero() : v3(h.cx, h.cy, h.cz);
+    }
     // BoxShape.h:167-172, ConvexMeshShape.cpp:80-102
     __device__ V3 support(V3 d) const {
+        if (tri()) {   // TriangleShape.h getLocalSupportPointWithoutMargin: Vector3::getMaxAxis of the dots
+            const float a = dot(d, t0), b = dot(d, t1), c = dot(d, t2);
+            const int k = a < b ? (b < c ? 2 : 1) : (a < c ? 2 : 0);
+            return k == 0 ? t0 : (k == 1 ? t1 : t2);
+        }
         if (hull < 0)
             return v3(d.x < 0.0f ? -he.x : he.x, d.y < 0.0f ? -he.y : he.y, d.z < 0.0f ? -he.z : he.z);
         float best = kDecimalSmallest;
@@ -107,16 +130,58 @@ __device__ inline Poly make_poly(const NpArgs& A, int collider) {
 // ------------------------------------------------------------------------------------------------
 // N = kMaxRawContacts in general; box-box tests produce at most 8 points (a quad clipped by four
 // planes), so their kernel uses a buffer of 8 and the reference's cap of 16 is never reached.
+// TriangleShape::computeSmoothTriangleMeshContact (TriangleShape.h:330-350, TriangleShape.cpp:85-114): a
+// contact with a concave shape's triangle takes the mesh's interpolated vertex normal at the point on the
+// triangle, and the point on the other shape moves along it by the penetration depth.  Applied to every
+// contact of a triangle test right before it is added, as the reference does at each contact site.
+struct TriSmooth {
+    bool tri1;                 // the triangle is shape 1
+    V3 p0, p1, p2, n;          // triangle (concave shape space) and its normal
+    V3 vn0, vn1, vn2;          // vertex normals
+    Tf triToWorld, worldToOther;
+    __device__ void apply(V3& normal, float depth, V3& l1, V3& l2) const {
+        const V3 lp = tri1 ? l1 : l2;
+        // computeBarycentricCoordinatesInTriangle (mathematics_functions.h:203-219)
+        const V3 e0 = p1 - p0, e1 = p2 - p0, e2 = lp - p0;
+        const float d00 = dot(e0, e0), d01 = dot(e0, e1), d11 = dot(e1, e1), d20 = dot(e2, e0), d21 = dot(e2, e1);
+        const float denom = d00 * d11 - d01 * d01;
+        const float v = (d11 * d20 - d01 * d21) / denom;
+        const float w = (d00 * d21 - d01 * d20) / denom;
+        const float u = 1.0f - v - w;
+        // computeSmoothLocalContactNormalForTriangle (TriangleShape.h:310-327)
+        V3 nl = n;
+        if (!(u > kMachineEps && v > kMachineEps && w > kMachineEps)) {
+            const V3 in = u * vn0 + v * vn1 + w * vn2;
+            if (!(len2(in) < kMachineEps)) nl = unit(in);
+        }
+        V3 nw = qrot(triToWorld.q, nl);
+        const V3 pen = tri1 ? normal : -normal;
+        if (dot(nw, pen) < 0.0f) {
+            nw = -nw;
+            nl = -nl;
+        }
+        normal = tri1 ? nw : -nw;
+        const V3 other = tapply(tmul(worldToOther, triToWorld), lp - nl * depth);
+        if (tri1) l2 = other;
+        else l1 = other;
+    }
+};
+
 template <int N>
 struct RawBuf {
     static constexpr int kN = N;
     int n;
     bool clip_full = false;   // a clipped polygon had more vertices than the clip buffer holds
+    const TriSmooth* smooth = nullptr;   // set for the tests of a concave shape's triangle
     V3 nrm[N];
     float depth[N];
     V3 l1[N];
     V3 l2[N];
-    __device__ void add(V3 normal, float d, V3 a, V3 b) {  // NarrowPhaseInfoBatch::addContactPoint
+    __device__ void add(V3 normal, float d, V3 a, V3 b) { add_carry(normal, d, a, b); }
+    // a contact site whose normal variable the smoothing rewrites for the next contact of the same loop
+    // (SATAlgorithm::computeCapsulePolyhedronFaceContactPoints takes it by reference)
+    __device__ void add_carry(V3& normal, float d, V3 a, V3 b) {  // NarrowPhaseInfoBatch::addContactPoint
+        if (smooth) smooth->apply(normal, d, a, b);
         if (n < N) {
             nrm[n] = normal; depth[n] = d; l1[n] = a; l2[n] = b;
             n++;
@@ -252,11 +317,11 @@ __device__ inline bool sat_minkowski_face(const Poly& P1, int4 e1, const Poly& P
     return cba * dba < 0.0f && adc * bdc < 0.0f && cba * bdc > 0.0f;
 }
 
-// :1040-1083 (shape 1 is never a triangle here)
-__device__ inline float sat_edge_distance(V3 e1a, V3 e2a, V3 c1, V3 e1d, V3 e2d, V3& axis_out) {
+// :1040-1083; when shape 1 is a triangle the axis is oriented with shape 2's centroid c2
+__device__ inline float sat_edge_distance(V3 e1a, V3 e2a, V3 c1, V3 e1d, V3 e2d, V3& axis_out, const Poly* tri2 = nullptr) {
     if (parallel_vectors(e1d, e2d)) return kDecimalLargest;
     V3 axis = unit(cross(e1d, e2d));
-    const float dp = dot(axis, c1 - e1a);
+    const float dp = tri2 ? dot(axis, e2a - tri2->centroid()) : dot(axis, c1 - e1a);
     if (dp > 0.0f) axis = -axis;
     axis_out = axis;
     return -dot(axis, e2a - e1a);
@@ -557,9 +622,9 @@ __device__ bool capsule_capsule(float r1, float h1, float r2, float h2, const Tf
 // ------------------------------------------------------------------------------------------------
 
 // CollisionDetectionSystem::reduceContactPoints (:1483-1662): picks 4 of n > 4 raw points.
-template <class R>
+template <class R, class L = IdxList>
 __device__ void reduce_points(const R& raw, const Tf& T1, int* keep) {
-    IdxList cand;
+    L cand;
     int nc = raw.n;
     for (int i = 0; i < nc; i++) cand.set(i, i);
     const Tf w2s = tinv(T1);
@@ -741,6 +806,10 @@ __device__ inline uint32_t lf_after_test(uint32_t lf, bool colliding) {
 // pairs into those ranges.
 __device__ __forceinline__ int np_bin(const NpArgs& A, int k, bool count_pass) {
     const int2 c = A.P.col[k];
+    if (count_pass && A.P.xm) {   // no triangle tests and no extra manifolds unless the concave kernels list them
+        A.P.ctn[k] = 0;
+        A.P.xm[k] = 0;
+    }
     const int f1 = A.B.flags[A.C.body[c.x]], f2 = A.B.flags[A.C.body[c.y]];
     // tested only if a body is enabled and both colliders are simulation colliders or triggers (CD.cpp:386-400);
     // a world query tests the pairs of world-query colliders (:386), a query about bodies the pairs of those
@@ -905,7 +974,7 @@ __device__ __forceinline__ int sat_begin(const Poly& P1, const Poly& P2, const T
             if (sat_minkowski_face(P1, e1, P2, e2, t12)) {
                 V3 axis;
                 const V3 c1 = tapply(t12, P1.centroid());
-                const float pen = sat_edge_distance(e1a, e2a, c1, e1d, e2d, axis);
+                const float pen = sat_edge_distance(e1a, e2a, c1, e1d, e2d, axis, P1.tri() ? &P2 : nullptr);
                 if (!wasColliding && pen <= 0.0f) return 0;
                 if (clipAgain && pen > 0.0f && pen < kDecimalLargest) {
                     V3 cp1, cp2;
@@ -954,7 +1023,7 @@ __device__ __forceinline__ bool sat_edge_pair(const Poly& P1, const Poly& P2, co
     const V3 e2a = P2.vertex(e2.x);
     const V3 e2b = P2.vertex(P2.edge(e2.w).x);
     V3 axis = v3zero();
-    pen = sat_edge_distance(e1a, e2a, tapply(t12, P1.centroid()), e1b - e1a, e2b - e2a, axis);
+    pen = sat_edge_distance(e1a, e2a, tapply(t12, P1.centroid()), e1b - e1a, e2b - e2a, axis, P1.tri() ? &P2 : nullptr);
     if (axis_out) *axis_out = axis;
     return true;
 }
@@ -1181,6 +1250,11 @@ void upload_box_topology(const int* edges4, const int* face_edge, const int* fac
     hipMemcpyToSymbol(HIP_SYMBOL(c_box_edges), edges4, sizeof(int4) * 24);
     hipMemcpyToSymbol(HIP_SYMBOL(c_box_face_edge), face_edge, sizeof(int) * 6);
     hipMemcpyToSymbol(HIP_SYMBOL(c_box_face_verts), face_verts, sizeof(int) * 24);
+}
+void upload_triangle_topology(const int* edges4, const int* face_edge, const int* face_verts) {
+    hipMemcpyToSymbol(HIP_SYMBOL(c_tri_edges), edges4, sizeof(int4) * 6);
+    hipMemcpyToSymbol(HIP_SYMBOL(c_tri_face_edge), face_edge, sizeof(int) * 2);
+    hipMemcpyToSymbol(HIP_SYMBOL(c_tri_face_verts), face_verts, sizeof(int) * 6);
 }
 
 }  // namespace rp3dg
@@ -1577,7 +1651,7 @@ __device__ bool capsule_face_contacts(int face, float r, const Poly& P, float pe
             found = true;
             const V3 cpP = clip[i] + delta;
             const V3 cpC = tapply(p2c, clip[i]) - sepAxisCap * r;
-            raw.add(normalWorld, pen, capsule1 ? cpC : cpP, capsule1 ? cpP : cpC);
+            raw.add_carry(normalWorld, pen, capsule1 ? cpC : cpP, capsule1 ? cpP : cpC);
         }
     }
     return found;
@@ -1646,6 +1720,24 @@ __device__ bool sat_capsule_poly(bool capsule1, float r, float hh, const Poly& P
 }
 
 // SphereVsConvexPolyhedronAlgorithm.cpp:38-103
+__device__ __forceinline__ bool sphere_poly_test(const Conv& S1, const Conv& S2, const Tf& T1, const Tf& T2, uint32_t& lf,
+                                                 float4& axis, Raw& raw) {
+    const int g = gjk(S1, S2, T1, T2, lf, axis, raw);
+    lf |= LF_WAS_GJK;
+    lf &= ~LF_WAS_SAT;
+    bool hit = false;
+    if (g == GJK_COLLIDE_IN_MARGIN) hit = true;
+    else if (g == GJK_INTERPENETRATE) {
+        const bool s1 = S1.type == SHAPE_SPHERE;
+        const Conv& Ss = s1 ? S1 : S2;
+        const Conv& Sp = s1 ? S2 : S1;
+        hit = sat_sphere_poly(s1, Ss.margin, Sp.P, s1 ? T1 : T2, s1 ? T2 : T1, raw);
+        lf &= ~LF_WAS_GJK;
+        lf |= LF_WAS_SAT;
+    }
+    return hit;
+}
+
 __global__ void np_sphere_poly(NpArgs A, const int* list, const int* count) {
     RP3DG_GRID_STRIDE(t, *count) {
         const int k = list[t];
@@ -1656,19 +1748,7 @@ __global__ void np_sphere_poly(NpArgs A, const int* list, const int* count) {
         uint32_t lf = A.P.lf[k];
         float4 axis = A.P.lfaxis[k];
         Raw raw; raw.n = 0;
-        const int g = gjk(S1, S2, T1, T2, lf, axis, raw);
-        lf |= LF_WAS_GJK;
-        lf &= ~LF_WAS_SAT;
-        bool hit = false;
-        if (g == GJK_COLLIDE_IN_MARGIN) hit = true;
-        else if (g == GJK_INTERPENETRATE) {
-            const bool s1 = S1.type == SHAPE_SPHERE;
-            const Conv& Ss = s1 ? S1 : S2;
-            const Conv& Sp = s1 ? S2 : S1;
-            hit = sat_sphere_poly(s1, Ss.margin, Sp.P, s1 ? T1 : T2, s1 ? T2 : T1, raw);
-            lf &= ~LF_WAS_GJK;
-            lf |= LF_WAS_SAT;
-        }
+        const bool hit = sphere_poly_test(S1, S2, T1, T2, lf, axis, raw);
         A.P.lfaxis[k] = axis;
         A.P.lf[k] = A.lf_update ? lf_after_test(lf, hit) : lf;
         contact_epilogue(A, k, hit, raw, c.x);
@@ -1676,6 +1756,51 @@ __global__ void np_sphere_poly(NpArgs A, const int* list, const int* count) {
 }
 
 // CapsuleVsConvexPolyhedronAlgorithm.cpp:42-177
+__device__ __forceinline__ bool capsule_poly_test(const Conv& S1, const Conv& S2, const Tf& T1, const Tf& T2, uint32_t& lf,
+                                                  float4& axis, Raw& raw, bool report) {
+    const int g = gjk(S1, S2, T1, T2, lf, axis, raw);
+    lf |= LF_WAS_GJK;
+    lf &= ~LF_WAS_SAT;
+    bool hit = false;
+    const bool cap1 = S1.type == SHAPE_CAPSULE;
+    const Conv& Sc = cap1 ? S1 : S2;
+    const Conv& Sp = cap1 ? S2 : S1;
+    const Tf& Tc = cap1 ? T1 : T2;
+    const Tf& Tp = cap1 ? T2 : T1;
+    if (g == GJK_COLLIDE_IN_MARGIN && !report) {
+        hit = true;   // the face-contact search only adds points (CapsuleVsConvexPolyhedronAlgorithm.cpp:83)
+    } else if (g == GJK_COLLIDE_IN_MARGIN) {
+        const V3 cn = raw.nrm[0];
+        const float cpen = raw.depth[0];
+        const float height = Sc.hh + Sc.hh;
+        const int nf = Sp.P.nfaces();
+        for (int f = 0; f < nf; f++) {
+            const V3 fn = Sp.P.face_normal(f);
+            V3 fnw = qrot(Tp.q, fn);
+            const V3 sa = v3(0.0f, -height * 0.5f, 0.0f);
+            const V3 sb = v3(0.0f, height * 0.5f, 0.0f);
+            const V3 sdir = normalized(qrot(Tc.q, sb - sa));
+            const bool inDir = dot(fnw, cn) > 0.0f;
+            const bool inContactDir = (cap1 && !inDir) || (!cap1 && inDir);
+            if (inContactDir && fabsf(dot(fnw, sdir)) < 0.001f && parallel_vectors(fnw, cn)) {
+                const Tf p2c = tmul(tinv(Tc), Tp);
+                const Tf c2p = tinv(p2c);
+                const V3 saP = tapply(c2p, sa);
+                const V3 sbP = tapply(c2p, sb);
+                const V3 sepCap = qrot(p2c.q, fn);
+                if (cap1) fnw = -fnw;
+                if (capsule_face_contacts(f, Sc.margin, Sp.P, cpen, p2c, fnw, sepCap, saP, sbP, cap1, raw)) break;
+            }
+        }
+        hit = true;
+    } else if (g == GJK_INTERPENETRATE) {
+        hit = sat_capsule_poly(cap1, Sc.margin, Sc.hh, Sp.P, Tc, Tp, raw, report);
+        lf &= ~LF_WAS_GJK;
+        lf |= LF_WAS_SAT;
+    }
+    return hit;
+}
+
 __global__ void np_capsule_poly(NpArgs A, const int* list, const int* count) {
     RP3DG_GRID_STRIDE(t, *count) {
         const int k = list[t];
@@ -1686,50 +1811,443 @@ __global__ void np_capsule_poly(NpArgs A, const int* list, const int* count) {
         uint32_t lf = A.P.lf[k];
         float4 axis = A.P.lfaxis[k];
         Raw raw; raw.n = 0;
-        const int g = gjk(S1, S2, T1, T2, lf, axis, raw);
-        lf |= LF_WAS_GJK;
-        lf &= ~LF_WAS_SAT;
-        bool hit = false;
-        const bool cap1 = S1.type == SHAPE_CAPSULE;
-        const Conv& Sc = cap1 ? S1 : S2;
-        const Conv& Sp = cap1 ? S2 : S1;
-        const Tf& Tc = cap1 ? T1 : T2;
-        const Tf& Tp = cap1 ? T2 : T1;
-        const bool report = pair_reports(A, k);
-        if (g == GJK_COLLIDE_IN_MARGIN && !report) {
-            hit = true;   // the face-contact search only adds points (CapsuleVsConvexPolyhedronAlgorithm.cpp:83)
-        } else if (g == GJK_COLLIDE_IN_MARGIN) {
-            const V3 cn = raw.nrm[0];
-            const float cpen = raw.depth[0];
-            const float height = Sc.hh + Sc.hh;
-            const int nf = Sp.P.nfaces();
-            for (int f = 0; f < nf; f++) {
-                const V3 fn = Sp.P.face_normal(f);
-                V3 fnw = qrot(Tp.q, fn);
-                const V3 sa = v3(0.0f, -height * 0.5f, 0.0f);
-                const V3 sb = v3(0.0f, height * 0.5f, 0.0f);
-                const V3 sdir = normalized(qrot(Tc.q, sb - sa));
-                const bool inDir = dot(fnw, cn) > 0.0f;
-                const bool inContactDir = (cap1 && !inDir) || (!cap1 && inDir);
-                if (inContactDir && fabsf(dot(fnw, sdir)) < 0.001f && parallel_vectors(fnw, cn)) {
-                    const Tf p2c = tmul(tinv(Tc), Tp);
-                    const Tf c2p = tinv(p2c);
-                    const V3 saP = tapply(c2p, sa);
-                    const V3 sbP = tapply(c2p, sb);
-                    const V3 sepCap = qrot(p2c.q, fn);
-                    if (cap1) fnw = -fnw;
-                    if (capsule_face_contacts(f, Sc.margin, Sp.P, cpen, p2c, fnw, sepCap, saP, sbP, cap1, raw)) break;
-                }
-            }
-            hit = true;
-        } else if (g == GJK_INTERPENETRATE) {
-            hit = sat_capsule_poly(cap1, Sc.margin, Sc.hh, Sp.P, Tc, Tp, raw, report);
-            lf &= ~LF_WAS_GJK;
-            lf |= LF_WAS_SAT;
-        }
+        const bool hit = capsule_poly_test(S1, S2, T1, T2, lf, axis, raw, pair_reports(A, k));
         A.P.lfaxis[k] = axis;
         A.P.lf[k] = A.lf_update ? lf_after_test(lf, hit) : lf;
         contact_epilogue(A, k, hit, raw, c.x);
+    }
+}
+
+}  // namespace rp3dg
+
+// ------------------------------------------------------------------------------------------------
+// Convex vs concave pairs (CollisionDetectionSystem::computeConvexVsConcaveMiddlePhase, CD.cpp:522-615;
+// processPotentialContacts for non-convex pairs, :1292-1385; reducePotentialContactManifolds, :1390-1458)
+// ------------------------------------------------------------------------------------------------
+namespace rp3dg {
+
+constexpr int kConcaveStack = 64;          // tree walk stack (a balanced tree of 2^31 triangles needs 45)
+constexpr int kConcavePairPoints = 64;     // contact points of one pair's tests that its reduction holds
+constexpr int kMaxPotentialManifolds = 64;   // NB_MAX_POTENTIAL_CONTACT_MANIFOLDS (configuration.h:140)
+constexpr int kMaxPairManifolds = 4;         // NB_MAX_CONTACT_MANIFOLDS (configuration.h:137)
+constexpr int kMaxPotentialManifoldPoints = 255;   // NB_MAX_CONTACT_POINTS_IN_POTENTIAL_MANIFOLD (:143)
+
+// convex collider cv and concave collider cc of a pair; convex1: the convex one is collider 1
+__device__ __forceinline__ void concave_sides(const NpArgs& A, int2 c, int& cv, int& cc, bool& convex1) {
+    convex1 = !shape_is_concave(A.C.type[c.x]);
+    cv = convex1 ? c.x : c.y;
+    cc = convex1 ? c.y : c.x;
+}
+
+// HeightField::getVertexAt (HeightField.cpp:399-409)
+__device__ __forceinline__ V3 hf_vertex(const NpArgs& A, const ConcaveInst& I, int x, int y) {
+    const float h = A.cheights[I.hbase + y * I.columns + x];
+    return v3(-I.width * 0.5f + (float)(uint32_t)x, I.origin + h, -I.length * 0.5f + (float)(uint32_t)y);
+}
+
+// HeightFieldShape::computeOverlappingTriangles -> HeightField::computeMinMaxGridCoordinates + clamps
+// (HeightFieldShape.cpp:46-52, HeightField.cpp:133-216): the cells [i0, i1) x [j0, j1)
+__device__ __forceinline__ void hf_cells(const ConcaveInst& I, V3 mn, V3 mx, int& i0, int& i1, int& j0, int& j1) {
+    const V3 is = v3(I.isx, I.isy, I.isz);
+    mn = cmul(mn, is);
+    mx = cmul(mx, is);
+    const V3 bmin = v3(I.bminx, I.bminy, I.bminz), bmax = v3(I.bmaxx, I.bmaxy, I.bmaxz);
+    V3 lo = vmin(vmax(mn, bmin), bmax);
+    V3 hi = vmax(vmin(mx, bmax), bmin);
+    const V3 tr = (bmax - bmin) * 0.5f;
+    lo = lo + tr;
+    hi = hi + tr;
+    const int a0 = (int)((double)lo.x + 0.5) - 1, a2 = (int)((double)lo.z + 0.5) - 1;
+    const int b0 = (int)((double)hi.x + 0.5) + 1, b2 = (int)((double)hi.z + 0.5) + 1;
+    i0 = min(max(a0, 0), I.columns - 1);
+    i1 = min(max(b0, 0), I.columns - 1);
+    j0 = min(max(a2, 0), I.rows - 1);
+    j1 = min(max(b2, 0), I.rows - 1);
+}
+
+// DynamicAABBTree::reportAllShapesOverlappingWithAABB (DynamicAABBTree.cpp:641-681) on a mesh's tree:
+// emit(index, triangle) for the overlapping leaves in the reference's order; false on a stack overflow
+template <class Emit>
+__device__ __forceinline__ bool mesh_walk(const NpArgs& A, const ConcaveInst& I, V3 mn, V3 mx, Emit emit) {
+    int stack[kConcaveStack];
+    int sp = 0, n = 0;
+    stack[sp++] = I.root;
+    while (sp > 0) {
+        const int id = stack[--sp];
+        if (id < 0) continue;
+        const float4 a = A.cnodes[2 * (I.node_base + id)];
+        const float4 b = A.cnodes[2 * (I.node_base + id) + 1];
+        if (mx.x < a.x || b.x < mn.x || mx.y < a.y || b.y < mn.y || mx.z < a.z || b.z < mn.z) continue;
+        const int c0 = __float_as_int(a.w), c1 = __float_as_int(b.w);
+        if (c0 < 0) {
+            emit(n++, c1);
+        } else {
+            if (sp + 2 > kConcaveStack) return false;
+            stack[sp++] = c0;
+            stack[sp++] = c1;
+        }
+    }
+    return true;
+}
+
+// Middle phase: the convex shape's AABB in the concave shape's space and the triangles it overlaps
+// (ConcaveShape::computeOverlappingTriangles), listed as this step's tests of the pair, contiguous and in
+// order.
+__global__ void np_concave_mid(NpArgs A, const int* list, const int* count) {
+    RP3DG_GRID_STRIDE(t, *count) {
+        const int k = list[t];
+        const int2 c = A.P.col[k];
+        int cv, cc;
+        bool convex1;
+        concave_sides(A, c, cv, cc, convex1);
+        const Tf Tv = tf(ld3(A.C.wpos, cv), ldq(A.C.wq, cv));
+        const Tf Tc = tf(ld3(A.C.wpos, cc), ldq(A.C.wq, cc));
+        V3 mn, mx;
+        transformed_aabb(A.C.type[cv], A.C.shape[cv], A.hull_bounds, tmul(tinv(Tc), Tv), mn, mx);
+        const ConcaveInst I = A.cinst[(int)A.C.shape[cc].y];
+        int n = 0, base = 0;
+        bool ok = true, reserved = false;
+        if (I.kind == SHAPE_CONCAVE_MESH) {
+            const V3 is = v3(I.isx, I.isy, I.isz);   // AABB::applyScale(1 / scale)
+            mn = cmul(mn, is);
+            mx = cmul(mx, is);
+            ok = mesh_walk(A, I, mn, mx, [&](int i, int) { n = i + 1; });
+            if (ok) {
+                base = n ? atomicAdd(A.CT.count, n) : 0;
+                reserved = n > 0;
+                if (base + n > A.CT.cap) ok = false;
+                else mesh_walk(A, I, mn, mx, [&](int i, int tri) {
+                    A.CT.pair[base + i] = k;
+                    A.CT.tri[base + i] = tri;
+                });
+            }
+        } else {
+            int i0, i1, j0, j1;
+            hf_cells(I, mn, mx, i0, i1, j0, j1);
+            n = (i1 > i0 && j1 > j0) ? 2 * (i1 - i0) * (j1 - j0) : 0;
+            base = n ? atomicAdd(A.CT.count, n) : 0;
+            reserved = n > 0;
+            if (base + n > A.CT.cap) ok = false;
+            else {
+                int q = base;
+                for (int i = i0; i < i1; i++) {
+                    for (int j = j0; j < j1; j++) {
+                        const int id = (j * (I.columns - 1) + i) * 2;   // HeightField::computeTriangleShapeId
+                        A.CT.pair[q] = k; A.CT.tri[q] = id; q++;
+                        A.CT.pair[q] = k; A.CT.tri[q] = id + 1; q++;
+                    }
+                }
+            }
+        }
+        if (!ok) {
+            atomicOr(A.status, STATUS_CONCAVE_OVERFLOW);
+            if (reserved)   // slots reserved but not listed
+                for (int q = base; q < A.CT.cap && q < base + n; q++) A.CT.pair[q] = -1;
+            n = 0;
+        }
+        A.P.ct0[k] = base;
+        A.P.ctn[k] = n;
+    }
+}
+
+// One triangle test: the TriangleShape of the middle phase (TriangleShape.cpp:40-62) against the convex
+// shape with the pair's algorithm, the triangle's LastFrameCollisionInfo (kept by the pair for the
+// triangles it tested in the last step, OverlappingPairs.h:231-294) and smoothed contacts.
+__global__ __launch_bounds__(128) void np_concave_tri(NpArgs A) {
+    const int n = min(*A.CT.count, A.CT.cap);
+    RP3DG_GRID_STRIDE(t, n) {
+        const int k = A.CT.pair[t];
+        if (k < 0) continue;
+        const int id = A.CT.tri[t];
+        const int2 c = A.P.col[k];
+        int cv, cc;
+        bool convex1;
+        concave_sides(A, c, cv, cc, convex1);
+        const ConcaveInst I = A.cinst[(int)A.C.shape[cc].y];
+        const V3 sc = v3(I.sx, I.sy, I.sz);
+        TriSmooth sm;
+        if (I.kind == SHAPE_CONCAVE_MESH) {
+            const int4 ix = A.ctris[I.tbase + id];
+            sm.p0 = cmul(xyz(A.cverts[I.vbase + ix.x]), sc);
+            sm.p1 = cmul(xyz(A.cverts[I.vbase + ix.y]), sc);
+            sm.p2 = cmul(xyz(A.cverts[I.vbase + ix.z]), sc);
+            sm.vn0 = xyz(A.cnorm[I.nbase + ix.x]);
+            sm.vn1 = xyz(A.cnorm[I.nbase + ix.y]);
+            sm.vn2 = xyz(A.cnorm[I.nbase + ix.z]);
+        } else {   // HeightField::computeOverlappingTriangles (:150-176)
+            const int cell = id >> 1;
+            const int i = cell % (I.columns - 1), j = cell / (I.columns - 1);
+            const V3 q1 = cmul(hf_vertex(A, I, i, j), sc), q2 = cmul(hf_vertex(A, I, i, j + 1), sc);
+            const V3 q3 = cmul(hf_vertex(A, I, i + 1, j), sc), q4 = cmul(hf_vertex(A, I, i + 1, j + 1), sc);
+            V3 fnrm;
+            if ((id & 1) == 0) {
+                sm.p0 = q1; sm.p1 = q2; sm.p2 = q3;
+                fnrm = unit(cross(q2 - q1, q3 - q1));
+            } else {
+                sm.p0 = q3; sm.p1 = q2; sm.p2 = q4;
+                fnrm = unit(cross(q2 - q3, q4 - q3));
+            }
+            sm.vn0 = sm.vn1 = sm.vn2 = fnrm;
+        }
+        sm.n = normalized(cross(sm.p1 - sm.p0, sm.p2 - sm.p0));
+        const Tf Tv = tf(ld3(A.C.wpos, cv), ldq(A.C.wq, cv));
+        const Tf Tc = tf(ld3(A.C.wpos, cc), ldq(A.C.wq, cc));
+        sm.tri1 = !convex1;
+        sm.triToWorld = Tc;
+        sm.worldToOther = tinv(Tv);
+        Conv Sv = make_conv(A, cv);
+        Conv St;
+        St.type = SHAPE_HULL;
+        St.margin = 0.0f;
+        St.hh = 0.0f;
+        St.P = Poly{};
+        St.P.hull = kTriPoly;
+        St.P.t0 = sm.p0; St.P.t1 = sm.p1; St.P.t2 = sm.p2; St.P.tn = sm.n;
+        const Conv& S1 = convex1 ? Sv : St;
+        const Conv& S2 = convex1 ? St : Sv;
+        const Tf& T1 = convex1 ? Tv : Tc;
+        const Tf& T2 = convex1 ? Tc : Tv;
+        // the triangle's last-frame info: from the pair's tests of the last step, else the default
+        uint32_t lf = 0u;
+        float4 axis = make_float4(0.f, 1.f, 0.f, 0.f);
+        const int prev = A.P.prev[k];
+        if (prev >= 0) {
+            const int b = A.O.ct0[prev], m = A.O.ctn[prev];
+            for (int q = 0; q < m; q++) {
+                if (A.CTo.tri[b + q] == id) {
+                    lf = A.CTo.lf[b + q];
+                    axis = A.CTo.lfaxis[b + q];
+                    break;
+                }
+            }
+        }
+        Raw raw;
+        raw.n = 0;
+        raw.smooth = &sm;
+        const int vt = Sv.type;
+        bool hit = false;
+        if (vt == SHAPE_SPHERE) {
+            hit = sphere_poly_test(S1, S2, T1, T2, lf, axis, raw);
+        } else if (vt == SHAPE_CAPSULE) {
+            hit = capsule_poly_test(S1, S2, T1, T2, lf, axis, raw, pair_reports(A, k));
+        } else {   // ConvexPolyhedronVsConvexPolyhedronAlgorithm: SAT, one lane per test
+            const Tf t12 = tmul(tinv(T2), T1), t21 = tinv(t12);
+            float minPen0 = 0.0f;
+            int minFace = 0;
+            bool isMinFace1 = false;
+            const int stage = sat_begin<false, true>(S1.P, S2.P, T1, T2, t12, t21, lf, raw, minPen0, minFace, isMinFace1,
+                                                     A.clip_prev != 0);
+            hit = stage == 1;
+            if (stage == 2) {
+                EdgeSearch es;
+                es.init();
+                int h2;
+                const int np = sat_edge_pairs(S1.P, S2.P, h2);
+                for (int kk = 0; kk < np; kk++) {
+                    float p = 0.0f;
+                    const bool v = sat_edge_pair(S1.P, S2.P, t12, kk, h2, p);
+                    es.visit(kk, v, p);
+                    if (es.sep != INT_MAX) break;
+                }
+                hit = sat_finish<false>(S1.P, S2.P, T1, T2, t12, t21, lf, raw, minPen0, minFace, isMinFace1, es);
+            }
+            lf |= LF_WAS_SAT;
+            lf &= ~LF_WAS_GJK;
+        }
+        if (raw.clip_full) atomicOr(A.status, STATUS_CLIP_OVERFLOW);
+        A.CT.lf[t] = A.lf_update ? lf_after_test(lf, hit) : lf;
+        A.CT.lfaxis[t] = axis;
+        A.CT.hit[t] = hit ? 1 : 0;
+        int2 pts = make_int2(0, 0);
+        if (hit && raw.n > 0 && pair_reports(A, k)) {
+            const int base = atomicAdd(A.cp_count, raw.n);
+            if (base + raw.n > A.cp_cap) {
+                atomicOr(A.status, STATUS_CONCAVE_OVERFLOW);
+            } else {
+                for (int q = 0; q < raw.n; q++) {
+                    A.cp_nd[base + q] = f4(raw.nrm[q], raw.depth[q]);
+                    A.cp_l1[base + q] = f4(raw.l1[q], 0.f);
+                    A.cp_l2[base + q] = f4(raw.l2[q], 0.f);
+                }
+                pts = make_int2(base, raw.n);
+            }
+        }
+        A.CT.pts[t] = pts;
+    }
+}
+
+// The points of one manifold of a concave pair, as reduce_points reads them
+struct PoolPoints {
+    const float4 *nd, *l1, *l2;
+    const int* idx;   // pool index of each point
+    int n;
+    __device__ V3 normal(int r) const { return xyz(nd[idx[r]]); }
+    __device__ V3 p1(int r) const { return xyz(l1[idx[r]]); }
+};
+template <int N>
+struct IntList {
+    int v[N];
+    __device__ int get(int i) const { return v[i]; }
+    __device__ void set(int i, int x) { v[i] = x; }
+};
+
+// Per pair: the potential manifolds of its triangle tests' points (contact points go to the first
+// manifold of a similar normal, CD.cpp:1325-1380), at most NB_MAX_CONTACT_MANIFOLDS kept (the one of the
+// smallest largest depth removed first, swapped with the last), each reduced to four points and without
+// duplicates, then the warm-start transfer over the pair's previous manifolds (initContactsWithPreviousOnes,
+// :1069-1160) and the cache write: manifold 0 in the pair's entry, the others in tail entries.
+__global__ void np_concave_reduce(NpArgs A, const int* list, const int* count) {
+    RP3DG_GRID_STRIDE(t, *count) {
+        const int k = list[t];
+        DevPairs P = A.P;
+        const DevPairs& O = A.O;
+        const int b = P.ct0[k], nt = P.ctn[k];
+        bool colliding = false;
+        int pidx[kConcavePairPoints];
+        int np = 0;
+        for (int q = 0; q < nt; q++) {
+            colliding |= A.CT.hit[b + q] != 0;
+            const int2 r = A.CT.pts[b + q];
+            for (int p = 0; p < r.y; p++) {
+                if (np < kConcavePairPoints) pidx[np++] = r.x + p;
+                else atomicOr(A.status, STATUS_CONCAVE_OVERFLOW);
+            }
+        }
+        // the pair's wasColliding (trigger pairs report their overlap from it)
+        uint32_t plf = P.lf[k] | LF_VALID;
+        plf = colliding ? (plf | LF_WAS_COLLIDING) : (plf & ~LF_WAS_COLLIDING);
+        if (A.lf_update) P.lf[k] = plf;
+        P.xm[k] = 0;
+        P.xs[k] = 0;
+        if (!colliding || np == 0 || !pair_reports(A, k)) {
+            P.ncp[k] = 0;
+            continue;
+        }
+        // potential manifolds in creation order
+        int pm[kConcavePairPoints];   // manifold of each point (-1: dropped)
+        int mfirst[kMaxPotentialManifolds], mcnt[kMaxPotentialManifolds];
+        int nm = 0;
+        for (int q = 0; q < np; q++) {
+            const V3 nq = xyz(A.cp_nd[pidx[q]]);
+            int found = -1;
+            for (int m = 0; m < nm; m++) {
+                if (mcnt[m] < kMaxPotentialManifoldPoints &&
+                    dot(xyz(A.cp_nd[pidx[mfirst[m]]]), nq) >= A.cos_similar) {
+                    found = m;
+                    break;
+                }
+            }
+            if (found < 0 && nm < kMaxPotentialManifolds) {
+                found = nm++;
+                mfirst[found] = q;
+                mcnt[found] = 0;
+            }
+            pm[q] = found;
+            if (found >= 0) mcnt[found]++;
+        }
+        // at most four manifolds (ContactPair::removePotentialManifoldAtIndex swaps with the last)
+        int ml[kMaxPotentialManifolds];
+        for (int m = 0; m < nm; m++) ml[m] = m;
+        int L = nm;
+        while (L > kMaxPairManifolds) {
+            float minDepth = kDecimalLargest;
+            int mi = -1;
+            for (int j = 0; j < L; j++) {
+                float largest = 0.0f;
+                for (int q = 0; q < np; q++)
+                    if (pm[q] == ml[j] && A.cp_nd[pidx[q]].w > largest) largest = A.cp_nd[pidx[q]].w;
+                if (largest < minDepth) {
+                    minDepth = largest;
+                    mi = j;
+                }
+            }
+            ml[mi] = ml[L - 1];
+            L--;
+        }
+        // previous manifolds of the pair: its entry, then its tail entries
+        int prevSlots[kMaxPairManifolds];
+        int nprev = 0;
+        const int prev = P.prev[k];
+        if (prev >= 0 && O.ncp[prev] > 0) {
+            prevSlots[nprev++] = prev;
+            const int pxm = O.xm[prev], pxs = O.xs[prev];
+            for (int j = 0; j < pxm && nprev < kMaxPairManifolds; j++) prevSlots[nprev++] = pxs + j;
+        }
+        const int tbase = L > 1 ? atomicAdd(A.tail_count, L - 1) : 0;
+        if (L > 1 && tbase + L - 1 > A.tail_cap) {
+            atomicOr(A.status, STATUS_CONCAVE_OVERFLOW);
+            L = 1;
+        }
+        const Tf T1 = tf(ld3(A.C.wpos, P.col[k].x), ldq(A.C.wq, P.col[k].x));
+        const float pt2 = A.persistent_threshold * A.persistent_threshold;
+        const float thr2 = 0.01f * 0.01f;
+        for (int j = 0; j < L; j++) {
+            const int slot = j == 0 ? k : A.tail_base + tbase + j - 1;
+            int mp[kConcavePairPoints];
+            int n = 0;
+            for (int q = 0; q < np; q++)
+                if (pm[q] == ml[j]) mp[n++] = pidx[q];
+            IntList<kConcavePairPoints> idx;
+            for (int i = 0; i < n; i++) idx.set(i, mp[i]);
+            if (n > kMaxManifoldPoints) {   // reduceContactPoints
+                PoolPoints view{A.cp_nd, A.cp_l1, A.cp_l2, mp, n};
+                int keep[4];
+                reduce_points<PoolPoints, IntList<kConcavePairPoints>>(view, T1, keep);
+                for (int i = 0; i < 4; i++) idx.set(i, mp[keep[i]]);
+                n = 4;
+            }
+            // removeDuplicatedContactPointsInManifold (:1673-1705)
+            for (int i = 0; i < n; i++) {
+                for (int m = i + 1; m < n; m++) {
+                    if (len2(xyz(A.cp_l1[idx.get(m)]) - xyz(A.cp_l1[idx.get(i)])) < thr2) {
+                        idx.set(m, idx.get(n - 1));
+                        n--;
+                        m--;
+                    }
+                }
+            }
+            // friction of the first previous manifold of a similar normal
+            float4 fv1 = make_float4(0.f, 0.f, 0.f, 0.f), fv2 = fv1, fimp = fv1;
+            const V3 n0 = xyz(A.cp_nd[idx.get(0)]);
+            for (int p = 0; p < nprev; p++) {
+                const float4 on = O.pn[4 * prevSlots[p]];
+                if (dot(v3(on.x, on.y, on.z), n0) >= A.cos_similar) {
+                    fv1 = O.fv1[prevSlots[p]];
+                    fv2 = O.fv2[prevSlots[p]];
+                    fimp = O.fimp[prevSlots[p]];
+                    break;
+                }
+            }
+            for (int i = 0; i < n; i++) {
+                const int r = idx.get(i);
+                const V3 l1 = xyz(A.cp_l1[r]);
+                float imp = 0.0f, resting = 0.0f;
+                bool matched = false;
+                for (int p = 0; p < nprev && !matched; p++) {
+                    const int os = prevSlots[p];
+                    const int on = O.ncp[os];
+                    for (int o = 0; o < on; o++) {
+                        const float4 ol1 = O.pl1[4 * os + o];
+                        if (len2(l1 - xyz(ol1)) <= pt2) {
+                            imp = ol1.w;
+                            resting = O.pl2[4 * os + o].w;
+                            matched = true;
+                            break;
+                        }
+                    }
+                }
+                P.pn[4 * slot + i] = A.cp_nd[r];
+                P.pl1[4 * slot + i] = f4(l1, imp);
+                P.pl2[4 * slot + i] = f4(xyz(A.cp_l2[r]), resting);
+            }
+            P.fv1[slot] = fv1;
+            P.fv2[slot] = fv2;
+            P.fimp[slot] = fimp;
+            P.ncp[slot] = n;
+            if (j > 0) P.col[slot] = P.col[k];
+        }
+        P.xm[k] = L - 1;
+        P.xs[k] = A.tail_base + tbase;
     }
 }
 

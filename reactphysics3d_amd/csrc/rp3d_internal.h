@@ -35,7 +35,8 @@ enum Algo : int {
     ALGO_SPHERE_POLY = 4,
     ALGO_CAPSULE_POLY = 5,
     ALGO_POLY_POLY = 6,
-    ALGO_COUNT = 7
+    ALGO_CONCAVE = 7,      // convex vs concave: middle phase over the triangles (CD.cpp:522-615)
+    ALGO_COUNT = 8
 };
 
 // Body flags
@@ -141,6 +142,25 @@ struct DevPairs {
     float4 *pl2;               // [4*slot+k] local point on shape 2, w = resting flag (0/1)
     int *evf;                  // EV_CUR / EV_PREV: OverlappingPair::collidingInCurrentFrame / InPreviousFrame
     int *kept;                 // set by the merge on every entry of the previous table that survives
+    // convex vs concave pairs (null when the batch has no concave collider): the step's triangle tests
+    // [ct0, ct0 + ctn) of the test list, and the contact manifolds beyond the first one (a concave pair
+    // has up to NB_MAX_CONTACT_MANIFOLDS = 4): xm of them in the table's tail entries [xs, xs + xm),
+    // which carry the pair's col and their own contact cache
+    int *ct0, *ctn, *xm, *xs;
+};
+
+// A concave collider's shape (ConcaveMeshShape / HeightFieldShape) with its scale; the collider's
+// shape.x is its local-bounds slot (after the hulls' slots), shape.y the index of this record.
+struct ConcaveInst {
+    int kind;                  // SHAPE_CONCAVE_MESH / SHAPE_HEIGHTFIELD
+    int vbase, tbase, nbase;   // mesh: first vertex, triangle and scaled vertex normal in the arenas
+    int node_base, root;       // mesh: its AABB tree nodes (two float4 each) and root
+    int columns, rows, hbase;  // height field: grid and first height
+    int pad;
+    float sx, sy, sz;          // scale
+    float isx, isy, isz;       // decimal(1.0) / scale
+    float width, length, origin;              // height field (HeightField.cpp:56-99)
+    float bminx, bminy, bminz, bmaxx, bmaxy, bmaxz;   // height field bounds, unscaled
 };
 
 constexpr int EV_CUR = 1;
@@ -206,6 +226,9 @@ struct DevCounters {
     int nb_hull_tests;         // the rest of nb_tests[ALGO_POLY_POLY] (a convex mesh involved)
     int nb_walk_again;         // colliders with more partners than the traversal slab holds
     int nb_events;             // contact events of the step (may exceed the event buffer)
+    int nb_concave_tests;      // triangle tests of the convex vs concave pairs
+    int nb_concave_points;     // contact points of those tests
+    int nb_tails;              // contact manifolds of concave pairs beyond their first
 };
 
 constexpr int STATUS_PAIR_OVERFLOW = 1;
@@ -213,5 +236,6 @@ constexpr int STATUS_COLOR_OVERFLOW = 2;
 constexpr int STATUS_STACK_OVERFLOW = 4;
 constexpr int STATUS_COLOR_STALL = 8;
 constexpr int STATUS_CLIP_OVERFLOW = 16;   // a face clip produced more vertices than its buffer holds
+constexpr int STATUS_CONCAVE_OVERFLOW = 32;   // concave pairs: triangle tests, their points or manifolds over capacity
 
 }  // namespace rp3dg

@@ -26,9 +26,9 @@ struct SvArgs {
 __global__ void dyn_integrate_velocities(DevBodies B, Settings S, float dt);
 __global__ void dyn_integrate_positions(DevBodies B, float dt, float split_factor);
 __global__ void dyn_update_state(DevBodies B);
-__global__ void sv_flag_manifolds(const int* ncp, const int* nb_pairs, int* flag);
-__global__ void sv_compact_manifolds(const int* flag, const int* offs, const int* nb_pairs, const int* ncp, int* list,
-                                     int* counters_manifolds, int* counters_points,
+__global__ void sv_flag_manifolds(const int* ncp, const int* xm, const int* nb_pairs, int* flag);
+__global__ void sv_compact_manifolds(const int* flag, const int* offs, const int* nb_pairs, const int* ncp, const int* xm,
+                                     const int* xs, int* list, int* counters_manifolds, int* counters_points,
                                      unsigned long long* total_points);
 __global__ void sv_adj_count(DevBodies B, DevColliders C, DevPairs P, const int* list, const int* nm_ptr, int* cnt);
 __global__ void sv_adj_fill(DevBodies B, DevColliders C, DevPairs P, const int* list, const int* nm_ptr, const int* offs,

@@ -57,13 +57,14 @@ __global__ void sv_store(SvArgs A) {
 // Manifold list: pairs with at least one contact point, in pair order
 // ------------------------------------------------------------------------------------------------
 
-__global__ void sv_flag_manifolds(const int* ncp, const int* nb_pairs, int* flag) {
+// a pair's manifolds: its entry, then (concave pairs, xm non-null) its xm tail entries from xs
+__global__ void sv_flag_manifolds(const int* ncp, const int* xm, const int* nb_pairs, int* flag) {
     const int np = *nb_pairs;
-    RP3DG_GRID_STRIDE(k, np) flag[k] = ncp[k] > 0 ? 1 : 0;
+    RP3DG_GRID_STRIDE(k, np) flag[k] = ncp[k] > 0 ? 1 + (xm ? xm[k] : 0) : 0;
 }
 
-__global__ void sv_compact_manifolds(const int* flag, const int* offs, const int* nb_pairs, const int* ncp, int* list,
-                                     int* counters_manifolds, int* counters_points,
+__global__ void sv_compact_manifolds(const int* flag, const int* offs, const int* nb_pairs, const int* ncp, const int* xm,
+                                     const int* xs, int* list, int* counters_manifolds, int* counters_points,
                                      unsigned long long* total_points) {
     const int np = *nb_pairs;
     if (blockIdx.x == 0 && threadIdx.x == 0) *counters_manifolds = offs[np];
@@ -72,6 +73,11 @@ __global__ void sv_compact_manifolds(const int* flag, const int* offs, const int
         if (flag[k]) {
             list[offs[k]] = k;
             pts += ncp[k];
+            for (int j = 1; j < flag[k]; j++) {
+                const int slot = xs[k] + j - 1;
+                list[offs[k] + j] = slot;
+                pts += ncp[slot];
+            }
         }
     }
     // one atomic per workgroup: same-address atomics from every wave would serialise in the L2

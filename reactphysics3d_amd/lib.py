@@ -56,7 +56,8 @@ class ContactView(C.Structure):
     _fields_ = [("colliders", C.POINTER(C.c_int32)), ("nb_points", C.POINTER(C.c_int32)),
                 ("color", C.POINTER(C.c_int32)), ("normal_depth", C.POINTER(C.c_float)),
                 ("local1", C.POINTER(C.c_float)), ("local2", C.POINTER(C.c_float)),
-                ("impulse", C.POINTER(C.c_float)), ("friction", C.POINTER(C.c_float))]
+                ("impulse", C.POINTER(C.c_float)), ("friction", C.POINTER(C.c_float)),
+                ("manifold", C.POINTER(C.c_int32))]
 
 
 # Every symbol the header declares (the not-GPU test checks that each is exported).
@@ -561,9 +562,11 @@ class Batch:
             "color": np.zeros(nm, np.int32), "normal_depth": np.zeros((nm, 4, 4), np.float32),
             "local1": np.zeros((nm, 4, 3), np.float32), "local2": np.zeros((nm, 4, 3), np.float32),
             "impulse": np.zeros((nm, 4), np.float32), "friction": np.zeros((nm, 3), np.float32),
+            "manifold": np.zeros(nm, np.int32),
         }
         v = ContactView(_ip(out["colliders"]), _ip(out["nb_points"]), _ip(out["color"]), _fp(out["normal_depth"]),
-                        _fp(out["local1"]), _fp(out["local2"]), _fp(out["impulse"]), _fp(out["friction"]))
+                        _fp(out["local1"]), _fp(out["local2"]), _fp(out["impulse"]), _fp(out["friction"]),
+                        _ip(out["manifold"]))
         n = _check(load().rp3dg_batch_get_contacts(self.h, C.byref(v), nm), "get_contacts")
         return {k: a[:n] for k, a in out.items()}
 

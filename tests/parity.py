@@ -7,9 +7,14 @@ def pair_set(c1, c2):
 
 
 def contacts_by_pair(ct):
+    """Manifolds keyed by (collider 1, collider 2), and by their index among the pair's manifolds when the
+    pair has several (a convex vs concave pair has up to four)."""
     out = {}
+    sub = ct.get("manifold")
     for m in range(len(ct["nb_points"])):
         key = (int(ct["colliders"][m, 0]), int(ct["colliders"][m, 1]))
+        if sub is not None and int(sub[m]) > 0:
+            key = key + (int(sub[m]),)
         n = int(ct["nb_points"][m])
         out[key] = dict(n=n, nd=ct["normal_depth"][m, :n], l1=ct["local1"][m, :n], l2=ct["local2"][m, :n])
     return out
