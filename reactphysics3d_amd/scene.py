@@ -535,17 +535,20 @@ def terrain_mesh(nx=12, ny=12, cell=1.0, amp=0.35, seed=9):
 
 
 def concave_scene(kind="mesh", n_bodies=40, shapes=(SPHERE, CAPSULE, BOX, HULL), nx=10, scale=(1.0, 1.0, 1.0),
-                  seed=9, height=(1.5, 6.0), normals=False):
+                  seed=9, height=(1.5, 6.0), normals=False, ground_type=STATIC, ground_lin=(0.0, 0.0, 0.0),
+                  ground_ang=(0.0, 0.0, 0.0), sleeping=False, trigger_every=0):
     """Dynamic bodies of the given shapes (cycled) dropped at seeded positions and orientations onto a
-    static concave ground: a bumpy triangle mesh (kind "mesh", z up; normals=True hands the mesh its
-    own vertex normals instead of letting the library compute them) or a height field (kind
-    "heightfield", y up, gravity -y) of nx x nx cells, with the given shape scale."""
+    concave ground: a bumpy triangle mesh (kind "mesh", z up; normals=True hands the mesh its own
+    vertex normals instead of letting the library compute them) or a height field (kind "heightfield",
+    y up, gravity -y) of nx x nx cells, with the given shape scale.  The ground is static, or kinematic
+    with the given velocities; sleeping turns the WorldSettings' sleeping on; trigger_every = n makes
+    every n-th falling body's collider a trigger."""
     rng = SplitMix64(seed)
     up = 2 if kind == "mesh" else 1
     g = [0.0, 0.0, 0.0]
     g[up] = -9.81
-    sc = Scene(f"concave_{kind}_{n_bodies}", make_settings(gravity=tuple(g)))
-    ground = sc.add_body(STATIC, (0.0, 0.0, 0.0))
+    sc = Scene(f"concave_{kind}_{n_bodies}", make_settings(gravity=tuple(g), sleeping=1 if sleeping else 0))
+    ground = sc.add_body(ground_type, (0.0, 0.0, 0.0), lin=ground_lin, ang=ground_ang)
     if kind == "mesh":
         v, t = terrain_mesh(nx, nx, seed=seed + 1)
         n = None
@@ -566,15 +569,16 @@ def concave_scene(kind="mesh", n_bodies=40, shapes=(SPHERE, CAPSULE, BOX, HULL),
         pos = [rng.uniform(-half, half), rng.uniform(-half, half), rng.uniform(-half, half)]
         pos[up] = rng.uniform(height[0], height[1])
         b = sc.add_body(DYNAMIC, tuple(pos), rng.quaternion())
+        trig = trigger_every > 0 and k % trigger_every == trigger_every - 1
         if shape == SPHERE:
-            sc.add_collider(b, SPHERE, (rng.uniform(0.2, 0.5),))
+            sc.add_collider(b, SPHERE, (rng.uniform(0.2, 0.5),), trigger=trig)
         elif shape == CAPSULE:
-            sc.add_collider(b, CAPSULE, (rng.uniform(0.15, 0.3), rng.uniform(0.3, 0.8)))
+            sc.add_collider(b, CAPSULE, (rng.uniform(0.15, 0.3), rng.uniform(0.3, 0.8)), trigger=trig)
         elif shape == BOX:
-            sc.add_collider(b, BOX, (rng.uniform(0.2, 0.5), rng.uniform(0.2, 0.5), rng.uniform(0.2, 0.5)))
+            sc.add_collider(b, BOX, (rng.uniform(0.2, 0.5), rng.uniform(0.2, 0.5), rng.uniform(0.2, 0.5)), trigger=trig)
         else:
             hh = sc.add_hull_from_points(ellipsoid_points(rng, 16, 0.25, 0.5))
-            sc.add_collider(b, HULL, (float(hh),))
+            sc.add_collider(b, HULL, (float(hh),), trigger=trig)
     return sc
 
 
