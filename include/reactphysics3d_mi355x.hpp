@@ -202,6 +202,96 @@ class ConvexMeshShape : public CollisionShape {
     ConvexMesh* mMesh;
 };
 
+// TriangleVertexArray (collision/TriangleVertexArray.h:45-200): strided float or double vertices (and
+// optional vertex normals) and int or short triangle indices, copied as floats and int32.
+class TriangleVertexArray {
+   public:
+    enum class VertexDataType { VERTEX_FLOAT_TYPE, VERTEX_DOUBLE_TYPE };
+    enum class NormalDataType { NORMAL_FLOAT_TYPE, NORMAL_DOUBLE_TYPE };
+    enum class IndexDataType { INDEX_INTEGER_TYPE, INDEX_SHORT_TYPE };
+    TriangleVertexArray(uint32 nbVertices, const void* verticesStart, uint32 verticesStride, uint32 nbTriangles,
+                        const void* indexesStart, uint32 indexesStride, VertexDataType vertexDataType,
+                        IndexDataType indexDataType) {
+        copy_vertices(mVertices, nbVertices, verticesStart, verticesStride, vertexDataType == VertexDataType::VERTEX_DOUBLE_TYPE);
+        copy_indices(nbTriangles, indexesStart, indexesStride, indexDataType == IndexDataType::INDEX_SHORT_TYPE);
+    }
+    TriangleVertexArray(uint32 nbVertices, const void* verticesStart, uint32 verticesStride, const void* normalsStart,
+                        uint32 normalsStride, uint32 nbTriangles, const void* indexesStart, uint32 indexesStride,
+                        VertexDataType vertexDataType, NormalDataType normalDataType, IndexDataType indexDataType) {
+        copy_vertices(mVertices, nbVertices, verticesStart, verticesStride, vertexDataType == VertexDataType::VERTEX_DOUBLE_TYPE);
+        copy_vertices(mNormals, nbVertices, normalsStart, normalsStride, normalDataType == NormalDataType::NORMAL_DOUBLE_TYPE);
+        copy_indices(nbTriangles, indexesStart, indexesStride, indexDataType == IndexDataType::INDEX_SHORT_TYPE);
+    }
+    uint32 getNbVertices() const { return (uint32)(mVertices.size() / 3); }
+    uint32 getNbTriangles() const { return (uint32)(mIndices.size() / 3); }
+    bool getHasNormals() const { return !mNormals.empty(); }
+    std::vector<float> mVertices, mNormals;
+    std::vector<int32_t> mIndices;
+
+   private:
+    static void copy_vertices(std::vector<float>& out, uint32 n, const void* start, uint32 stride, bool dbl) {
+        out.resize(3 * (size_t)n);
+        for (uint32 i = 0; i < n; i++) {
+            const unsigned char* p = static_cast<const unsigned char*>(start) + (size_t)i * stride;
+            for (int k = 0; k < 3; k++)
+                out[3 * i + k] = dbl ? (float)reinterpret_cast<const double*>(p)[k] : reinterpret_cast<const float*>(p)[k];
+        }
+    }
+    void copy_indices(uint32 n, const void* start, uint32 stride, bool shrt) {
+        mIndices.resize(3 * (size_t)n);
+        for (uint32 i = 0; i < n; i++) {
+            const unsigned char* p = static_cast<const unsigned char*>(start) + (size_t)i * stride;
+            for (int k = 0; k < 3; k++)
+                mIndices[3 * i + k] = shrt ? (int32_t)reinterpret_cast<const unsigned short*>(p)[k]
+                                           : (int32_t)reinterpret_cast<const uint32*>(p)[k];
+        }
+    }
+};
+// TriangleMesh (collision/TriangleMesh.h); each collider that uses it builds it in its world.
+class TriangleMesh {
+   public:
+    explicit TriangleMesh(TriangleVertexArray a) : mArray(std::move(a)) {}
+    uint32 getNbVertices() const { return mArray.getNbVertices(); }
+    uint32 getNbTriangles() const { return mArray.getNbTriangles(); }
+    TriangleVertexArray mArray;
+};
+class ConcaveMeshShape : public CollisionShape {
+   public:
+    ConcaveMeshShape(TriangleMesh* mesh, const Vector3& scaling) : mMesh(mesh), mScale(scaling) {
+        mType = RP3DG_SHAPE_CONCAVE_MESH;
+        mParams[1] = scaling.x; mParams[2] = scaling.y; mParams[3] = scaling.z;
+    }
+    TriangleMesh* mesh() const { return mMesh; }
+    const Vector3& getScale() const { return mScale; }
+
+   private:
+    TriangleMesh* mMesh;
+    Vector3 mScale;
+};
+// HeightField (collision/HeightField.h): float, double or integer samples (HeightField.cpp:72-102), kept as
+// floats; heights[y * columns + x].
+class HeightField {
+   public:
+    enum class HeightDataType { HEIGHT_FLOAT_TYPE, HEIGHT_DOUBLE_TYPE, HEIGHT_INT_TYPE };
+    int getNbColumns() const { return mColumns; }
+    int getNbRows() const { return mRows; }
+    int mColumns = 0, mRows = 0;
+    std::vector<float> mHeights;
+};
+class HeightFieldShape : public CollisionShape {
+   public:
+    HeightFieldShape(HeightField* field, const Vector3& scaling) : mField(field), mScale(scaling) {
+        mType = RP3DG_SHAPE_HEIGHTFIELD;
+        mParams[1] = scaling.x; mParams[2] = scaling.y; mParams[3] = scaling.z;
+    }
+    HeightField* field() const { return mField; }
+    const Vector3& getScale() const { return mScale; }
+
+   private:
+    HeightField* mField;
+    Vector3 mScale;
+};
+
 class PhysicsWorld;
 
 class Material {
@@ -263,6 +353,18 @@ class RigidBody {
             const int32_t h = rp3dg_hull_create(mWorld, a.mVertices.data(), (int32_t)(a.mVertices.size() / 3),
                                                 a.mCounts.data(), a.mIndices.data(), (int32_t)a.getNbFaces());
             check(h, "createConvexMesh");
+            params[0] = (float)h;
+        } else if (shape->type() == RP3DG_SHAPE_CONCAVE_MESH) {
+            const TriangleVertexArray& a = static_cast<ConcaveMeshShape*>(shape)->mesh()->mArray;
+            const int32_t m = rp3dg_triangle_mesh_create(mWorld, a.mVertices.data(), (int32_t)a.getNbVertices(),
+                                                         a.mIndices.data(), (int32_t)a.getNbTriangles(),
+                                                         a.getHasNormals() ? a.mNormals.data() : nullptr, nullptr);
+            check(m, "createTriangleMesh");
+            params[0] = (float)m;
+        } else if (shape->type() == RP3DG_SHAPE_HEIGHTFIELD) {
+            const HeightField& f = *static_cast<HeightFieldShape*>(shape)->field();
+            const int32_t h = rp3dg_heightfield_create(mWorld, f.mColumns, f.mRows, f.mHeights.data());
+            check(h, "createHeightField");
             params[0] = (float)h;
         }
         const int32_t c = rp3dg_collider_add(mWorld, mIndex, shape->type(), params, p, o);
@@ -914,6 +1016,48 @@ class PhysicsCommon {
             throw std::runtime_error("createConvexMeshShape: scaled convex meshes are not supported");
         return keep(std::make_unique<ConvexMeshShape>(m));
     }
+    // PhysicsCommon::createTriangleMesh(TriangleVertexArray, messages) (TriangleMesh.cpp:39-200): nullptr
+    // and an error message when the reference refuses the array; a warning per dropped degenerate triangle.
+    TriangleMesh* createTriangleMesh(const TriangleVertexArray& a, std::vector<Message>& messages) {
+        rp3dg_world* probe = rp3dg_world_create(nullptr);
+        int32_t skipped = 0;
+        const int32_t m = rp3dg_triangle_mesh_create(probe, a.mVertices.data(), (int32_t)a.getNbVertices(),
+                                                     a.mIndices.data(), (int32_t)a.getNbTriangles(),
+                                                     a.getHasNormals() ? a.mNormals.data() : nullptr, &skipped);
+        if (m < 0) messages.emplace_back(std::string("Error when creating a TriangleMesh: ") + rp3dg_last_error());
+        rp3dg_world_destroy(probe);
+        if (m < 0) return nullptr;
+        for (int32_t i = 0; i < skipped; i++)
+            messages.emplace_back("A face with an almost zero area or edge is not part of the final collision shape",
+                                  Message::Type::Warning);
+        mTriMeshes.push_back(std::make_unique<TriangleMesh>(a));
+        return mTriMeshes.back().get();
+    }
+    ConcaveMeshShape* createConcaveMeshShape(TriangleMesh* m, const Vector3& scaling = Vector3(1, 1, 1)) {
+        return keep(std::make_unique<ConcaveMeshShape>(m, scaling));
+    }
+    // PhysicsCommon::createHeightField (HeightField.cpp:43-130): integer samples times integerHeightScale
+    HeightField* createHeightField(int columns, int rows, const void* data, HeightField::HeightDataType type,
+                                   std::vector<Message>& messages, decimal integerHeightScale = 1.0f) {
+        if (columns < 2 || rows < 2) {
+            messages.emplace_back("The number of grid columns and grid rows must be at least two", Message::Type::Error);
+            return nullptr;
+        }
+        auto f = std::make_unique<HeightField>();
+        f->mColumns = columns;
+        f->mRows = rows;
+        f->mHeights.resize((size_t)columns * rows);
+        for (size_t i = 0; i < f->mHeights.size(); i++) {
+            if (type == HeightField::HeightDataType::HEIGHT_FLOAT_TYPE) f->mHeights[i] = static_cast<const float*>(data)[i];
+            else if (type == HeightField::HeightDataType::HEIGHT_DOUBLE_TYPE) f->mHeights[i] = (float)static_cast<const double*>(data)[i];
+            else f->mHeights[i] = (float)(static_cast<const int*>(data)[i] * integerHeightScale);
+        }
+        mHeightFields.push_back(std::move(f));
+        return mHeightFields.back().get();
+    }
+    HeightFieldShape* createHeightFieldShape(HeightField* f, const Vector3& scaling = Vector3(1, 1, 1)) {
+        return keep(std::make_unique<HeightFieldShape>(f, scaling));
+    }
 
    private:
     template <typename T>
@@ -925,6 +1069,8 @@ class PhysicsCommon {
     std::vector<std::unique_ptr<PhysicsWorld>> mWorlds;
     std::vector<std::unique_ptr<CollisionShape>> mShapes;
     std::vector<std::unique_ptr<ConvexMesh>> mMeshes;
+    std::vector<std::unique_ptr<TriangleMesh>> mTriMeshes;
+    std::vector<std::unique_ptr<HeightField>> mHeightFields;
 };
 
 }  // namespace reactphysics3d

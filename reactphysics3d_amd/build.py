@@ -78,7 +78,7 @@ def build(verbose=False, jobs=None, asan=False):
     if asan:
         return lib
     # the C++ API examples (host code only: g++ against the C++ mirror header and the library)
-    for name in ("hello_world", "joints", "contacts", "driven", "meshes", "queries"):
+    for name in ("hello_world", "joints", "contacts", "driven", "meshes", "queries", "concave"):
         ex = os.path.join(LIB_DIR, name)
         cmd = ["g++", "-std=c++17", "-O2", "-I", os.path.join(os.path.dirname(HERE), "include"),
                os.path.join(HERE, "examples", name + ".cpp"), "-L", LIB_DIR, "-lrp3d_mi355x",

@@ -343,3 +343,68 @@ def test_world_queries_through_the_cpp_mirror(ref_available):
     err = np.max(np.abs(st - np.concatenate([p, q, v, w], 1)))
     print("max |device - reference|", err)
     assert err <= 1e-4
+
+
+CONCAVE_EXE = os.path.join(ROOT, "reactphysics3d_amd", "_lib", "concave")
+
+
+def _concave_scene():
+    """The world of examples/concave.cpp (WorldSettings defaults, gravity -z)."""
+    f = np.float32
+    sc = S.Scene("concave_example", S.make_settings(gravity=(0.0, 0.0, -9.81), sleeping=1))
+    n = 8
+    v = np.array([(f(i - n // 2), f(j - n // 2), f(0.1) * f((7 * i + 3 * j) % 5) - f(0.2))
+                  for j in range(n + 1) for i in range(n + 1)], np.float32)
+    t = []
+    for j in range(n):
+        for i in range(n):
+            a = j * (n + 1) + i
+            b, c, d = a + 1, a + n + 2, a + n + 1
+            t += [(a, b, c), (a, c, d)] if (i + j) % 2 == 0 else [(a, b, d), (b, c, d)]
+    m = sc.add_triangle_mesh(v, np.array(t, np.int32))
+    g = sc.add_body(S.STATIC, (0.0, 0.0, 0.0))
+    sc.add_collider(g, S.CONCAVE_MESH, (float(m), 1.0, 1.0, 1.0))
+    h = np.array([[f(0.05) * f((x + 2 * y) % 4) for x in range(9)] for y in range(9)], np.float32)
+    hf = sc.add_heightfield(h)
+    hb = sc.add_body(S.STATIC, (12.0, 0.0, 0.0))
+    sc.add_collider(hb, S.HEIGHTFIELD, (float(hf), 1.0, 1.5, 1.0), lquat=(0.70710677, 0.0, 0.0, 0.70710677))
+    drops = [(-2, -1, 2), (1, 2, 3), (2, -2, 2.5), (-1, 1.5, 4), (11, 1, 2), (13, -1, 3)]
+    for k, p in enumerate(drops):
+        b = sc.add_body(S.DYNAMIC, tuple(float(x) for x in p))
+        if k < 2:
+            sc.add_collider(b, S.SPHERE, (0.4,))
+        elif k == 3:
+            sc.add_collider(b, S.CAPSULE, (0.25, 0.8))
+        else:
+            sc.add_collider(b, S.BOX, (0.4, 0.3, 0.35))
+    return sc
+
+
+def test_concave_example_builds_and_creates_world():
+    if not os.path.exists(CONCAVE_EXE):
+        pytest.skip("example not built (python -m reactphysics3d_amd.build)")
+    out = subprocess.run([CONCAVE_EXE, "0"], capture_output=True, text=True, timeout=120)
+    assert out.returncode == 0, out.stderr
+    rows = [list(map(float, line.split()[2:])) for line in out.stdout.splitlines() if line.startswith("body")]
+    assert len(rows) == 8 and rows[2][:3] == [-2.0, -1.0, 2.0]
+
+
+@pytest.mark.gpu
+def test_concave_example_steps_like_the_reference(ref_available):
+    """TriangleVertexArray / createTriangleMesh / createConcaveMeshShape and createHeightField /
+    createHeightFieldShape through the C++ mirror: the example matches the reference world built with the
+    same data over 150 steps."""
+    from oracle.ref import RefWorld
+    out = subprocess.run([CONCAVE_EXE, "150"], capture_output=True, text=True, timeout=120)
+    assert out.returncode == 0, out.stderr
+    st = np.array([list(map(float, line.split()[2:])) for line in out.stdout.splitlines() if line.startswith("body")],
+                  np.float32)
+    ref = RefWorld(_concave_scene())
+    for _ in range(150):
+        ref.step(1.0 / 60.0)
+    p, q, v, w = ref.bodies()
+    want = np.concatenate([p, q, v, w], 1)
+    err = np.max(np.abs(st - want))
+    print("max |device - reference|", err, "resting heights", want[2:, 2])
+    assert want[2:, 2].max() < 1.0   # everything landed
+    assert err <= 1e-4
