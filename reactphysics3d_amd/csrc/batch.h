@@ -149,6 +149,7 @@ struct rp3dg_batch {
     float* d_cheights = nullptr;
     rp3dg::ConcaveTests CTa{}, CTb{};
     float4 *cp_nd = nullptr, *cp_l1 = nullptr, *cp_l2 = nullptr;
+    rp3dg::TriArena tri_arena{};   // the step's triangles as hulls, one slot per concave test
     int cp_cap = 0;
     int tail_cap = 0;
     int man_cap = 0;           // contact manifolds of a step: pair_cap + tail_cap

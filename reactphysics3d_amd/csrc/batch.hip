@@ -896,7 +896,16 @@ int rp3dg_batch::upload(rp3dg_world* const* ws, int nw, const rp3dg_batch_option
         for (auto& e : tri.edges) { t4.push_back(e.vertex); t4.push_back(e.twin); t4.push_back(e.face); t4.push_back(e.next); }
         for (int f = 0; f < 2; f++) tfe[f] = tri.face_edge[f];
         for (auto& f : tri.faces) for (int vv : f) tfv.push_back(vv);
-        upload_triangle_topology(t4.data(), tfe.data(), tfv.data());
+        TriArena& TA = tri_arena;
+        TA.verts = dalloc<float4>(this, 3 * ct_cap); TA.normals = dalloc<float4>(this, 2 * ct_cap);
+        TA.face_edge = dalloc<int>(this, 2 * ct_cap); TA.face_verts = dalloc<int>(this, 4 * ct_cap);
+        int4* d_edges = dalloc<int4>(this, 6);
+        int* d_vidx = dalloc<int>(this, 6);
+        if (!TA.face_verts || !d_vidx) { set_error("device allocation failed (concave shapes)"); return RP3DG_EHIP; }
+        hipMemcpy(d_edges, t4.data(), sizeof(int4) * 6, hipMemcpyHostToDevice);
+        hipMemcpy(d_vidx, tfv.data(), sizeof(int) * 6, hipMemcpyHostToDevice);
+        TA.edges = d_edges; TA.vidx = d_vidx;
+        TA.face_edge0[0] = tfe[0]; TA.face_edge0[1] = tfe[1];
     }
     const int nn = std::max(1, 2 * nc);
     bp_keys = dalloc<uint64_t>(this, nc); bp_vals = dalloc<int>(this, nc);
@@ -1178,6 +1187,7 @@ NpArgs rp3dg_batch::np_args(const DevPairs& P, const DevPairs& O, int query, int
     A.CTo = na ? CTb : CTa;
     A.CT.count = &counters->nb_concave_tests;
     A.CTo.count = nullptr;   // the last step's tests are read through the previous table's ranges
+    A.tri = tri_arena;
     A.cp_nd = cp_nd; A.cp_l1 = cp_l1; A.cp_l2 = cp_l2;
     A.cp_cap = cp_cap;
     A.cp_count = &counters->nb_concave_points;
@@ -1361,7 +1371,8 @@ int rp3dg_batch::enqueue_step(float dt, bool capturing) {
         } else {
             F.feats = feat_buf;
             F.phase = 0;
-            LAUNCH_TS("sv_world_prepare", ngroups, fused_block, 0, sv_world_solve, F);
+            if (settings.split_impulse) LAUNCH_TS("sv_world_prepare", ngroups, fused_block, 0, sv_world_solve<true>, F);
+            else LAUNCH_TS("sv_world_prepare", ngroups, fused_block, 0, sv_world_solve<false>, F);
             if (SA.learned) {
                 const int rows = capturing ? feat_cap : std::min(feat_cap, last.nb_new_points + last.nb_new_points / 4 + 1024);
                 const int grid = std::max(1, std::min(2048, (rows + 63) / 64));
@@ -1458,7 +1469,8 @@ int rp3dg_batch::launch_world_classes(const char* name, const FusedArgs& F0, boo
         }
         const size_t shmem = fused_lds_bytes(F.lds_bodies, F.lds_residual != 0);
         hipStream_t s = c == 0 ? stream : side_streams[c - 1];
-        sv_world_solve<<<wc.count, wc.block, shmem, s>>>(F);
+        if (settings.split_impulse) sv_world_solve<true><<<wc.count, wc.block, shmem, s>>>(F);
+        else sv_world_solve<false><<<wc.count, wc.block, shmem, s>>>(F);
     }
     if (nclass > 1) {
         for (int c = 1; c < nclass; c++) {

@@ -49,6 +49,8 @@ constexpr int kFusedMaxLds = 512;     // bodies per group staged in LDS at most 
 constexpr int kGroupMaxWorlds = 64;   // worlds per group at most (one bit each in the convergence mask)
 constexpr int kGroupShift = 16;       // staged body flags carry the body's world index in its group here
 
+// one instance per contact position-correction technique: SPLIT = split impulses, else Baumgarte contacts
+template <bool SPLIT>
 __global__ void sv_world_solve(FusedArgs F);
 __global__ void world_position(FusedArgs F);
 // Appends (features, converged impulses) of this step's feature rows to the sample log at *logged.

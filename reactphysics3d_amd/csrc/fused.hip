@@ -44,6 +44,7 @@ __device__ __forceinline__ float block_maxf(float v, float* red) {
 // state in LDS (A.B's velocity, inertia and flag arrays point into the workgroup's LDS, so every access
 // compiles to an LDS instruction) and with it in HBM (groups larger than the LDS budget).
 // PV / PW: the velocities after the previous sweep (residual mode), indexed by body - b0, in LDS or HBM.
+template <bool SPLIT>
 __device__ __forceinline__ void group_sweeps(const FusedArgs& F, const SvArgs& A, const DevBodies& B, bool staged,
                                              int w0, int ngw, int b0, int b1, int m1, const int* seg, const int2* jr,
                                              float* red, unsigned int* s_wres, unsigned long long* s_done,
@@ -102,7 +103,7 @@ __device__ __forceinline__ void group_sweeps(const FusedArgs& F, const SvArgs& A
             if (lo == m1) break;
             for (int i = lo + tid; i < hi; i += nt) {
                 if (masked && ((done >> wk(A.S.bodies[i].x)) & 1ull)) continue;
-                solve_manifold(A, i, F.beta_dt);
+                solve_manifold<SPLIT>(A, i, F.beta_dt);
             }
             __syncthreads();
         }
@@ -169,6 +170,7 @@ __device__ __forceinline__ void group_sweeps(const FusedArgs& F, const SvArgs& A
         }
 }
 
+template <bool SPLIT>
 __global__ __launch_bounds__(kFusedMaxBlock) void sv_world_solve(FusedArgs F) {
     __shared__ float red[16];
     __shared__ unsigned int s_wres[kGroupMaxWorlds];   // residual of each world of the group (float bits)
@@ -264,11 +266,13 @@ __global__ __launch_bounds__(kFusedMaxBlock) void sv_world_solve(FusedArgs F) {
         float4* lpv = F.lds_residual ? reinterpret_cast<float4*>(lfl + L) : F.pv + b0;
         float4* lpw = F.lds_residual ? lpv + L : F.pw + b0;
         __syncthreads();
-        group_sweeps(F, A, B, true, w0, ngw, b0, b1, m1, seg, jr, red, s_wres, &s_done, lpv, lpw);
+        group_sweeps<SPLIT>(F, A, B, true, w0, ngw, b0, b1, m1, seg, jr, red, s_wres, &s_done, lpv, lpw);
     } else {
-        group_sweeps(F, AG, B, false, w0, ngw, b0, b1, m1, seg, jr, red, s_wres, &s_done, F.pv + b0, F.pw + b0);
+        group_sweeps<SPLIT>(F, AG, B, false, w0, ngw, b0, b1, m1, seg, jr, red, s_wres, &s_done, F.pv + b0, F.pw + b0);
     }
 }
+template __global__ void sv_world_solve<true>(FusedArgs F);
+template __global__ void sv_world_solve<false>(FusedArgs F);
 
 __global__ __launch_bounds__(kFusedMaxBlock) void world_position(FusedArgs F) {
     const DevBodies& B = F.A.B;

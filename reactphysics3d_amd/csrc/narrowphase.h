@@ -17,6 +17,19 @@ struct ConcaveTests {
     int* count;
 };
 
+// The triangles tested this step as hulls: test t's TriangleShape has vertices [3t, 3t+3) and faces
+// [2t, 2t+2) (normal, opposite normal), over the six half-edges and face vertex lists that every triangle
+// shares (PhysicsCommon.cpp initTriangleShapeHalfEdgeStructure; face_edge0 = each face's first half-edge).
+struct TriArena {
+    float4* verts;       // 3 per test slot
+    float4* normals;     // 2 per test slot
+    int* face_edge;      // 2 per test slot
+    int* face_verts;     // 2 x 2 per test slot: (offset into vidx, 3)
+    const int4* edges;   // 6 shared half-edges
+    const int* vidx;     // 6 shared face vertex indices
+    int face_edge0[2];
+};
+
 struct NpArgs {
     DevBodies B;
     DevColliders C;
@@ -51,6 +64,7 @@ struct NpArgs {
     float4 *cp_nd, *cp_l1, *cp_l2;   // contact points of the tests: normal + depth, local points
     int cp_cap;
     int* cp_count;
+    TriArena tri;
     int tail_base, tail_cap;   // tail entries of P: [tail_base, tail_base + tail_cap)
     int* tail_count;
 };
@@ -82,7 +96,6 @@ __global__ void np_poly_poly(NpArgs A, const int* list, const int* count);
 constexpr int kNpBoxThreads = 128;
 __global__ void np_box_box(NpArgs A, const int* list, const int* count);
 void upload_box_topology(const int* edges4, const int* face_edge, const int* face_verts);
-void upload_triangle_topology(const int* edges4, const int* face_edge, const int* face_verts);
 // convex vs concave pairs: middle phase (lists the triangle tests), the tests, then per pair the potential
 // manifolds, their reduction and the warm-start transfer
 __global__ void np_concave_mid(NpArgs A, const int* list, const int* count);

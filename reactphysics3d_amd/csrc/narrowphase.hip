@@ -22,19 +22,14 @@ namespace rp3dg {
 __constant__ int4 c_box_edges[24];   // vertex, twin, face, next (HalfEdgeStructure of the box, PhysicsCommon.cpp:73-109)
 __constant__ int c_box_face_edge[6];
 __constant__ int c_box_face_verts[24];
-__constant__ int4 c_tri_edges[6];     // TriangleShape's half-edges (PhysicsCommon.cpp initTriangleShapeHalfEdgeStructure)
-__constant__ int c_tri_face_edge[2];
-__constant__ int c_tri_face_verts[6];
 
 // ------------------------------------------------------------------------------------------------
-// Polyhedron view: BoxShape (implicit), ConvexMeshShape (hull arena) or TriangleShape (a concave
-// shape's triangle, vertices held here; TriangleShape.h).
+// Polyhedron view: BoxShape (implicit) or ConvexMeshShape (hull arena).  A concave shape's triangle
+// (TriangleShape.h) is a hull of the arena's triangle slots, written by its test (header pad = 1).
 // ------------------------------------------------------------------------------------------------
-constexpr int kTriPoly = -2;
 struct Poly {
-    int hull;          // -1 for a box, kTriPoly for a triangle
+    int hull;          // -1 for a box
     V3 he;             // box half-extents
-    V3 t0, t1, t2, tn; // triangle vertices and unit normal
     HullHeader h;
     const float4* hv;  // hull vertices
     const float4* hn;  // hull face normals
@@ -43,45 +38,31 @@ struct Poly {
     const int* hfverts;   // [2*f] = start, [2*f+1] = count into hfvidx
     const int* hfvidx;
 
-    __device__ bool tri() const { return hull == kTriPoly; }
-    __device__ int nfaces() const { return tri() ? 2 : (hull < 0 ? 6 : h.nf); }
-    __device__ int nhalfedges() const { return tri() ? 6 : (hull < 0 ? 24 : h.ne); }
+    __device__ bool tri() const { return hull >= 0 && h.pad != 0.0f; }
+    __device__ int nfaces() const { return hull < 0 ? 6 : h.nf; }
+    __device__ int nhalfedges() const { return hull < 0 ? 24 : h.ne; }
     // BoxShape.h:186-205 (vertices 0-3 on +z, 4-7 on -z; +x for 1, 2, 5, 6; +y for 2, 3, 6, 7), as
     // selects so that lanes holding different indices do not branch
     __device__ V3 vertex(int i) const {
         if (hull >= 0) return xyz(hv[h.vbase + i]);
-        if (tri()) return i == 0 ? t0 : (i == 1 ? t1 : t2);
         return v3(((i + 1) & 2) ? he.x : -he.x, (i & 2) ? he.y : -he.y, i < 4 ? he.z : -he.z);
     }
     // BoxShape.h:208-221 (faces +z, +x, -z, -x, -y, +y)
     __device__ V3 face_normal(int f) const {
         if (hull >= 0) return xyz(hn[h.fbase + f]);
-        if (tri()) return f == 0 ? tn : -tn;   // TriangleShape.h getFaceNormal
         return v3(f == 1 ? 1.0f : f == 3 ? -1.0f : 0.0f, f == 5 ? 1.0f : f == 4 ? -1.0f : 0.0f,
                   f == 0 ? 1.0f : f == 2 ? -1.0f : 0.0f);
     }
-    __device__ int4 edge(int e) const {
-        return tri() ? c_tri_edges[e] : (hull < 0 ? c_box_edges[e] : hedges[h.ebase + e]);
-    }
-    __device__ int face_edge(int f) const {
-        return tri() ? c_tri_face_edge[f] : (hull < 0 ? c_box_face_edge[f] : hfedge[h.fbase + f]);
-    }
-    __device__ int face_nverts(int f) const { return tri() ? 3 : (hull < 0 ? 4 : hfverts[2 * (h.fbase + f) + 1]); }
+    __device__ int4 edge(int e) const { return hull < 0 ? c_box_edges[e] : hedges[h.ebase + e]; }
+    __device__ int face_edge(int f) const { return hull < 0 ? c_box_face_edge[f] : hfedge[h.fbase + f]; }
+    __device__ int face_nverts(int f) const { return hull < 0 ? 4 : hfverts[2 * (h.fbase + f) + 1]; }
     __device__ int face_vertex(int f, int k) const {
-        if (tri()) return c_tri_face_verts[3 * f + k];
         return hull < 0 ? c_box_face_verts[4 * f + k] : hfvidx[hfverts[2 * (h.fbase + f)] + k];
     }
-    __device__ V3 centroid() const {
-        if (tri()) return (t0 + t1 + t2) / 3.0f;   // TriangleShape.h getCentroid
-        return hull < 0 ? v3zero() : v3(h.cx, h.cy, h.cz);
-    }
+    __device__ V3 centroid() const { return hull < 0 ? v3zero() : v3(h.cx, h.cy, h.cz); }
     // BoxShape.h:167-172, ConvexMeshShape.cpp:80-102
+    // (a triangle's getMaxAxis of its three dot products picks the first maximum, as this walk does)
     __device__ V3 support(V3 d) const {
-        if (tri()) {   // TriangleShape.h getLocalSupportPointWithoutMargin: Vector3::getMaxAxis of the dots
-            const float a = dot(d, t0), b = dot(d, t1), c = dot(d, t2);
-            const int k = a < b ? (b < c ? 2 : 1) : (a < c ? 2 : 0);
-            return k == 0 ? t0 : (k == 1 ? t1 : t2);
-        }
         if (hull < 0)
             return v3(d.x < 0.0f ? -he.x : he.x, d.y < 0.0f ? -he.y : he.y, d.z < 0.0f ? -he.z : he.z);
         float best = kDecimalSmallest;
@@ -317,11 +298,12 @@ __device__ inline bool sat_minkowski_face(const Poly& P1, int4 e1, const Poly& P
     return cba * dba < 0.0f && adc * bdc < 0.0f && cba * bdc > 0.0f;
 }
 
-// :1040-1083; when shape 1 is a triangle the axis is oriented with shape 2's centroid c2
-__device__ inline float sat_edge_distance(V3 e1a, V3 e2a, V3 c1, V3 e1d, V3 e2d, V3& axis_out, const Poly* tri2 = nullptr) {
+// :1040-1083; when shape 1 is a triangle (tri1) the axis is oriented with shape 2's centroid c2
+__device__ inline float sat_edge_distance(V3 e1a, V3 e2a, V3 c1, V3 e1d, V3 e2d, V3& axis_out, bool tri1 = false,
+                                          V3 c2 = V3{0.f, 0.f, 0.f}) {
     if (parallel_vectors(e1d, e2d)) return kDecimalLargest;
     V3 axis = unit(cross(e1d, e2d));
-    const float dp = tri2 ? dot(axis, e2a - tri2->centroid()) : dot(axis, c1 - e1a);
+    const float dp = tri1 ? dot(axis, e2a - c2) : dot(axis, c1 - e1a);
     if (dp > 0.0f) axis = -axis;
     axis_out = axis;
     return -dot(axis, e2a - e1a);
@@ -974,7 +956,7 @@ __device__ __forceinline__ int sat_begin(const Poly& P1, const Poly& P2, const T
             if (sat_minkowski_face(P1, e1, P2, e2, t12)) {
                 V3 axis;
                 const V3 c1 = tapply(t12, P1.centroid());
-                const float pen = sat_edge_distance(e1a, e2a, c1, e1d, e2d, axis, P1.tri() ? &P2 : nullptr);
+                const float pen = sat_edge_distance(e1a, e2a, c1, e1d, e2d, axis, P1.tri(), P1.tri() ? P2.centroid() : v3zero());
                 if (!wasColliding && pen <= 0.0f) return 0;
                 if (clipAgain && pen > 0.0f && pen < kDecimalLargest) {
                     V3 cp1, cp2;
@@ -1023,7 +1005,8 @@ __device__ __forceinline__ bool sat_edge_pair(const Poly& P1, const Poly& P2, co
     const V3 e2a = P2.vertex(e2.x);
     const V3 e2b = P2.vertex(P2.edge(e2.w).x);
     V3 axis = v3zero();
-    pen = sat_edge_distance(e1a, e2a, tapply(t12, P1.centroid()), e1b - e1a, e2b - e2a, axis, P1.tri() ? &P2 : nullptr);
+    pen = sat_edge_distance(e1a, e2a, tapply(t12, P1.centroid()), e1b - e1a, e2b - e2a, axis, P1.tri(),
+                            P1.tri() ? P2.centroid() : v3zero());
     if (axis_out) *axis_out = axis;
     return true;
 }
@@ -1140,6 +1123,10 @@ __device__ __forceinline__ void sat_pair_list(const NpArgs& A, const int* list, 
         int k = 0;
         int2 c = make_int2(0, 0);
         Poly P1{}, P2{};
+        if constexpr (BOX) {   // inactive lanes too: every access folds to the box's implicit topology
+            P1.hull = -1;
+            P2.hull = -1;
+        }
         Tf T1{}, T2{}, t12{}, t21{};
         R raw;
         raw.n = 0;
@@ -1251,11 +1238,7 @@ void upload_box_topology(const int* edges4, const int* face_edge, const int* fac
     hipMemcpyToSymbol(HIP_SYMBOL(c_box_face_edge), face_edge, sizeof(int) * 6);
     hipMemcpyToSymbol(HIP_SYMBOL(c_box_face_verts), face_verts, sizeof(int) * 24);
 }
-void upload_triangle_topology(const int* edges4, const int* face_edge, const int* face_verts) {
-    hipMemcpyToSymbol(HIP_SYMBOL(c_tri_edges), edges4, sizeof(int4) * 6);
-    hipMemcpyToSymbol(HIP_SYMBOL(c_tri_face_edge), face_edge, sizeof(int) * 2);
-    hipMemcpyToSymbol(HIP_SYMBOL(c_tri_face_verts), face_verts, sizeof(int) * 6);
-}
+
 
 }  // namespace rp3dg
 
@@ -1996,13 +1979,37 @@ __global__ __launch_bounds__(128) void np_concave_tri(NpArgs A) {
         sm.triToWorld = Tc;
         sm.worldToOther = tinv(Tv);
         Conv Sv = make_conv(A, cv);
+        // the TriangleShape as a hull in the test's slots of the triangle arena: its three vertices and
+        // two faces (the normal and its opposite) over the half-edge structure every triangle shares
+        const TriArena& TA = A.tri;
+        HullHeader h;
+        h.vbase = 3 * t; h.nv = 3;
+        h.fbase = 2 * t; h.nf = 2;
+        h.ebase = 0; h.ne = 6;
+        h.fvbase = 0; h.nfv = 6;
+        const V3 cen = (sm.p0 + sm.p1 + sm.p2) / 3.0f;   // TriangleShape.h getCentroid
+        h.cx = cen.x; h.cy = cen.y; h.cz = cen.z;
+        h.pad = 1.0f;
+        TA.verts[h.vbase] = f4(sm.p0, 0.f);
+        TA.verts[h.vbase + 1] = f4(sm.p1, 0.f);
+        TA.verts[h.vbase + 2] = f4(sm.p2, 0.f);
+        TA.normals[h.fbase] = f4(sm.n, 0.f);
+        TA.normals[h.fbase + 1] = f4(-sm.n, 0.f);
+        TA.face_edge[h.fbase] = TA.face_edge0[0];
+        TA.face_edge[h.fbase + 1] = TA.face_edge0[1];
+        TA.face_verts[2 * h.fbase] = 0;
+        TA.face_verts[2 * h.fbase + 1] = 3;
+        TA.face_verts[2 * h.fbase + 2] = 3;
+        TA.face_verts[2 * h.fbase + 3] = 3;
         Conv St;
         St.type = SHAPE_HULL;
         St.margin = 0.0f;
         St.hh = 0.0f;
-        St.P = Poly{};
-        St.P.hull = kTriPoly;
-        St.P.t0 = sm.p0; St.P.t1 = sm.p1; St.P.t2 = sm.p2; St.P.tn = sm.n;
+        St.P.hull = t;
+        St.P.he = v3zero();
+        St.P.h = h;
+        St.P.hv = TA.verts; St.P.hn = TA.normals; St.P.hedges = TA.edges;
+        St.P.hfedge = TA.face_edge; St.P.hfverts = TA.face_verts; St.P.hfvidx = TA.vidx;
         const Conv& S1 = convex1 ? Sv : St;
         const Conv& S2 = convex1 ? St : Sv;
         const Tf& T1 = convex1 ? Tv : Tc;

@@ -45,7 +45,11 @@ __global__ void sv_warmstart(SvArgs A, int color) {
 __global__ void sv_solve_color(SvArgs A, int color, float beta_dt) {
     if (color >= *A.ncolors) return;
     const int lo = A.cstart[color], hi = A.cstart[color + 1];
-    RP3DG_GRID_STRIDE(t, hi - lo) solve_manifold(A, lo + t, beta_dt);
+    if (A.split) {
+        RP3DG_GRID_STRIDE(t, hi - lo) solve_manifold<true>(A, lo + t, beta_dt);
+    } else {
+        RP3DG_GRID_STRIDE(t, hi - lo) solve_manifold<false>(A, lo + t, beta_dt);
+    }
 }
 
 __global__ void sv_store(SvArgs A) {

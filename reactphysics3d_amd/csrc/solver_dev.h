@@ -370,8 +370,10 @@ __device__ __forceinline__ void sv_point_features(const SvArgs& A, int i, int p,
 // ContactSolverSystem::solve (CS.cpp:479-759), for one colour: with split impulses the penetration bias
 // drives the split row; with Baumgarte contacts it joins the restitution bias in the main row
 // LK = false: both bodies have unit lock factors, and every multiplication by a factor (x * 1 = x
-// exactly) is left out; LK = true multiplies as the reference does.
-template <bool LK>
+// exactly) is left out; LK = true multiplies as the reference does.  SPLIT: split impulses (the
+// default technique) or Baumgarte contacts, a separate copy each so that the default's sweep keeps
+// its own registers and schedule.
+template <bool LK, bool SPLIT>
 __device__ __forceinline__ void solve_manifold_t(const SvArgs& A, int i, float beta_dt) {
     const DevBodies& B = A.B;
     const DevSolver& S = A.S;
@@ -403,7 +405,7 @@ __device__ __forceinline__ void solve_manifold_t(const SvArgs& A, int i, float b
         const float Jv = dv.x * n.x + dv.y * n.y + dv.z * n.z;
         float biasPen = 0.0f;
         if (pn.w > 0.01f) biasPen = -(beta_dt) * fmax_s(0.0f, pn.w - 0.01f);
-        float dl = A.split ? -(Jv + rb) * ipm : -(Jv + (biasPen + rb)) * ipm;
+        float dl = SPLIT ? -(Jv + rb) * ipm : -(Jv + (biasPen + rb)) * ipm;
         float lt = lam.x;
         lam.x = fmax_s(lam.x + dl, 0.0f);
         dl = lam.x - lt;
@@ -413,7 +415,7 @@ __device__ __forceinline__ void solve_manifold_t(const SvArgs& A, int i, float b
         v2.x += lk<LK>(im2 * li.x, ll2.x); v2.y += lk<LK>(im2 * li.y, ll2.y); v2.z += lk<LK>(im2 * li.z, ll2.z);
         w2.x += lk<LK>(i2n.x, al2.x) * dl; w2.y += lk<LK>(i2n.y, al2.y) * dl; w2.z += lk<LK>(i2n.z, al2.z) * dl;
         sumPen += lam.x;
-        if (!A.split) {
+        if constexpr (!SPLIT) {
             S.lam[q] = lam;
             continue;
         }
@@ -502,10 +504,12 @@ __device__ __forceinline__ void solve_manifold_t(const SvArgs& A, int i, float b
     }
 }
 
+// SPLIT = A.split, fixed by the caller's kernel instance (the fused solver kernel has one per technique)
+template <bool SPLIT>
 __device__ __forceinline__ void solve_manifold(const SvArgs& A, int i, float beta_dt) {
     const int2 bb = A.S.bodies[i];
-    if (body_flags(A.B, bb.x) & body_flags(A.B, bb.y) & BF_UNIT_LOCKS) solve_manifold_t<false>(A, i, beta_dt);
-    else solve_manifold_t<true>(A, i, beta_dt);
+    if (body_flags(A.B, bb.x) & body_flags(A.B, bb.y) & BF_UNIT_LOCKS) solve_manifold_t<false, SPLIT>(A, i, beta_dt);
+    else solve_manifold_t<true, SPLIT>(A, i, beta_dt);
 }
 
 // ContactSolverSystem::storeImpulses (CS.cpp:763-785); initializeForIsland marks every point resting (:161)
