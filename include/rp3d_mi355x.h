@@ -553,6 +553,28 @@ int64_t rp3dg_batch_get_log(rp3dg_batch* batch, float* features, float* targets,
  * the count when pred is NULL). */
 int64_t rp3dg_batch_get_warm_start_predictions(rp3dg_batch* batch, float* pred, int64_t capacity);
 
+/* ---- Snapshot and restore of a batch.  No counterpart in the reference, which keeps no checkpoint; a
+ * snapshot covers the state that PhysicsWorld::update evolves: the bodies (RigidBodyComponents.h:70-160),
+ * the broadphase's fat AABBs, the overlapping pairs with their LastFrameCollisionInfo
+ * (OverlappingPairs.h:55-93), the previous step's contact cache (CollisionDetectionSystem.cpp:881-903),
+ * joints' accumulated impulses and limit states, sleep timers and islands. ---- */
+typedef struct rp3dg_snapshot rp3dg_snapshot;
+/* Copies the batch's state into a new snapshot held in device memory (RigidBody calls still queued are
+ * applied first): every device array of the batch and the host-side state of the step sequence (pair
+ * table parity, counters of the last step, step count).  RP3DG_EHIP when device memory runs out,
+ * RP3DG_ESTATE during a graph capture. */
+int rp3dg_batch_snapshot(rp3dg_batch* batch, rp3dg_snapshot** snapshot);
+/* Puts the batch back into a snapshot taken from it: the steps that follow equal, bit for bit, those
+ * that followed the snapshot, and captured step graphs stay valid.  RigidBody calls queued since are
+ * dropped; body and joint properties changed since return to the snapshot's.  Batch-level settings
+ * (iterations, position correction techniques, residual mode, warm start, contact events on or off)
+ * and the sample log stay as they are now.  RP3DG_EINVAL for a snapshot of another batch.  After a step
+ * that failed with RP3DG_EOVERFLOW, a restore gives the batch back in the state of the snapshot. */
+int rp3dg_batch_restore(rp3dg_batch* batch, const rp3dg_snapshot* snapshot);
+/* Device bytes held by a snapshot. */
+int64_t rp3dg_snapshot_bytes(const rp3dg_snapshot* snapshot);
+void rp3dg_snapshot_destroy(rp3dg_snapshot* snapshot);
+
 const char* rp3dg_last_error(void);
 const char* rp3dg_version(void);
 

@@ -25,7 +25,9 @@ struct rp3dg_batch {
     std::vector<rp3dg_world*> worlds;
     std::vector<int> body_base, collider_base;
     std::vector<void*> allocs;
+    std::vector<size_t> alloc_bytes;   // size of every allocation in allocs
     size_t bytes = 0;
+    uint64_t uid = 0;                  // unique over the process (snapshots name their batch by it)
     hipStream_t stream = nullptr;
     int nb = 0, nc = 0, nworlds = 0, world_bits = 0;
     rp3dg::Settings settings{};

@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <chrono>
 #include <cstdio>
 #include <cstring>
@@ -200,6 +201,7 @@ static T* dalloc(rp3dg_batch* b, size_t n) {
     if (n == 0) n = 1;
     if (hipMalloc(&p, n * sizeof(T)) != hipSuccess) return nullptr;
     b->allocs.push_back(p);
+    b->alloc_bytes.push_back(n * sizeof(T));
     b->bytes += n * sizeof(T);
     return (T*)p;
 }
@@ -1017,6 +1019,7 @@ int rp3dg_batch::upload(rp3dg_world* const* ws, int nw, const rp3dg_batch_option
     sort_offs = dalloc<int>(this, sort_hist_cap + 1);
     HIPCHECK(hipMalloc(&counters, sizeof(DevCounters)));
     allocs.push_back(counters);
+    alloc_bytes.push_back(sizeof(DevCounters));
     HIPCHECK(hipHostMalloc(&hcounters, sizeof(DevCounters), hipHostMallocDefault));
     HIPCHECK(hipMemsetAsync(counters, 0, sizeof(DevCounters), stream));
     if (!sort_offs || !S.lam) { set_error("device allocation failed (pair arenas)"); return RP3DG_EHIP; }
@@ -1680,7 +1683,9 @@ rp3dg_batch* rp3dg_batch_create_ex(rp3dg_world* const* worlds, int32_t n, const 
         if (!worlds[i]) { set_error("null world"); return nullptr; }
         if (worlds[i]->uploaded) { set_error("a world belongs to a batch already"); return nullptr; }
     }
+    static std::atomic<uint64_t> next_uid{1};
     rp3dg_batch* b = new rp3dg_batch();
+    b->uid = next_uid++;
     const int r = b->upload(worlds, n, opt);
     if (r) { delete b; return nullptr; }
     for (int i = 0; i < n; i++) worlds[i]->uploaded = true;
@@ -1707,6 +1712,110 @@ int rp3dg_batch_synchronize(rp3dg_batch* b) {
     if (r) return r;
     HIPCHECK(hipStreamSynchronize(b->stream));
     return RP3DG_OK;
+}
+
+// Snapshot: a device copy of every array the batch allocated (its whole simulation state lives in
+// them, the step counters included), and the host-side state of the step sequence.  Arrays allocated after the snapshot (contact
+// events or sample logging enabled later) are left as they are by a restore.
+struct rp3dg_snapshot {
+    uint64_t batch_uid = 0;
+    size_t nallocs = 0;
+    std::vector<size_t> offs, sizes;
+    char* dev = nullptr;
+    size_t bytes = 0;
+    DevCounters last{};
+    bool cur_is_a = true, contacts_is_a = true, ev_pending = false, query_points = false;
+    int contacts_np = 0, hint_pairs = 0, hint_manifolds = 0, hint_rounds = 0;
+    int64_t steps_done = 0;
+    std::vector<int> body_type, coll_kind;
+};
+
+static int sync_all_streams(rp3dg_batch* b) {
+    HIPCHECK(hipStreamSynchronize(b->stream));
+    for (hipStream_t st : b->side_streams) HIPCHECK(hipStreamSynchronize(st));
+    return RP3DG_OK;
+}
+
+int rp3dg_batch_snapshot(rp3dg_batch* b, rp3dg_snapshot** out) {
+    if (!b || !out) { set_error("null batch or output"); return RP3DG_EINVAL; }
+    *out = nullptr;
+    if (b->capture_parity >= 0) { set_error("snapshot while a step graph is being captured"); return RP3DG_ESTATE; }
+    int r = b->flush_api();
+    if (r) return r;
+    r = sync_all_streams(b);
+    if (r) return r;
+    if (b->alloc_bytes.size() != b->allocs.size()) { set_error("allocation sizes not tracked"); return RP3DG_EINVAL; }
+    rp3dg_snapshot* s = new rp3dg_snapshot();
+    s->batch_uid = b->uid;
+    s->nallocs = b->allocs.size();
+    for (size_t i = 0; i < s->nallocs; i++) {
+        s->offs.push_back(s->bytes);
+        s->sizes.push_back(b->alloc_bytes[i]);
+        s->bytes += (b->alloc_bytes[i] + 255) & ~(size_t)255;
+    }
+    if (hipMalloc((void**)&s->dev, std::max<size_t>(s->bytes, 1)) != hipSuccess) {
+        (void)hipGetLastError();
+        delete s;
+        set_error("device allocation failed (snapshot of " + std::to_string(b->bytes) + " bytes)");
+        return RP3DG_EHIP;
+    }
+    for (size_t i = 0; i < s->nallocs; i++)
+        if (hipMemcpyAsync(s->dev + s->offs[i], b->allocs[i], s->sizes[i], hipMemcpyDeviceToDevice, b->stream) != hipSuccess) {
+            rp3dg_snapshot_destroy(s);
+            set_error("snapshot copy failed");
+            return RP3DG_EHIP;
+        }
+    if (hipStreamSynchronize(b->stream) != hipSuccess) {
+        rp3dg_snapshot_destroy(s);
+        set_error("snapshot copy failed");
+        return RP3DG_EHIP;
+    }
+    s->last = b->last;
+    s->cur_is_a = b->cur_is_a; s->contacts_is_a = b->contacts_is_a;
+    s->ev_pending = b->ev_pending; s->query_points = b->query_points;
+    s->contacts_np = b->contacts_np;
+    s->hint_pairs = b->hint_pairs; s->hint_manifolds = b->hint_manifolds; s->hint_rounds = b->hint_rounds;
+    s->steps_done = b->steps_done;
+    s->body_type = b->body_type; s->coll_kind = b->coll_kind;
+    *out = s;
+    return RP3DG_OK;
+}
+
+int rp3dg_batch_restore(rp3dg_batch* b, const rp3dg_snapshot* s) {
+    if (!b || !s) { set_error("null batch or snapshot"); return RP3DG_EINVAL; }
+    if (s->batch_uid != b->uid || s->nallocs > b->allocs.size()) {
+        set_error("the snapshot was taken from another batch");
+        return RP3DG_EINVAL;
+    }
+    for (size_t i = 0; i < s->nallocs; i++)
+        if (s->sizes[i] != b->alloc_bytes[i]) { set_error("snapshot layout differs from the batch"); return RP3DG_EINVAL; }
+    if (b->capture_parity >= 0) { set_error("restore while a step graph is being captured"); return RP3DG_ESTATE; }
+    int r = sync_all_streams(b);
+    if (r) return r;
+    b->api_ops.clear();   // calls queued since the snapshot belong to the state it replaces
+    b->api_sleep_op = false;
+    for (size_t i = 0; i < s->nallocs; i++) {
+        if (b->allocs[i] == (void*)b->log_count) continue;   // the sample log is not simulation state
+        HIPCHECK(hipMemcpyAsync(b->allocs[i], s->dev + s->offs[i], s->sizes[i], hipMemcpyDeviceToDevice, b->stream));
+    }
+    HIPCHECK(hipStreamSynchronize(b->stream));
+    b->last = s->last;
+    b->cur_is_a = s->cur_is_a; b->contacts_is_a = s->contacts_is_a;
+    b->ev_pending = s->ev_pending; b->query_points = s->query_points;
+    b->contacts_np = s->contacts_np;
+    b->hint_pairs = s->hint_pairs; b->hint_manifolds = s->hint_manifolds; b->hint_rounds = s->hint_rounds;
+    b->steps_done = s->steps_done;
+    b->body_type = s->body_type; b->coll_kind = s->coll_kind;
+    for (rp3dg_world* w : b->worlds) w->mirror_dirty = true;
+    return RP3DG_OK;
+}
+
+int64_t rp3dg_snapshot_bytes(const rp3dg_snapshot* s) { return s ? (int64_t)s->bytes : 0; }
+
+void rp3dg_snapshot_destroy(rp3dg_snapshot* s) {
+    if (!s) return;
+    if (s->dev) (void)hipFree(s->dev);
+    delete s;
 }
 
 rp3dg_batch* rp3dg_world_upload(rp3dg_world* w) {

@@ -91,7 +91,8 @@ EXPORTED = [
     "rp3dg_batch_raycast_all", "rp3dg_batch_test_overlap_bodies", "rp3dg_batch_test_overlap",
     "rp3dg_batch_test_collision", "rp3dg_batch_get_query_results", "rp3dg_world_upload",
     "rp3dg_batch_set_group_order", "rp3dg_world_set_contacts_position_correction",
-    "rp3dg_batch_set_contacts_position_correction",
+    "rp3dg_batch_set_contacts_position_correction", "rp3dg_batch_snapshot", "rp3dg_batch_restore",
+    "rp3dg_snapshot_bytes", "rp3dg_snapshot_destroy",
 ]
 
 # RigidBody call codes of Batch.body_ops (the order of the C-ABI's per-body calls)
@@ -161,6 +162,10 @@ def load():
         "rp3dg_batch_step": (C.c_int, [vp, f, i32]),
         "rp3dg_batch_step_graph": (C.c_int, [vp, f, i32]),
         "rp3dg_batch_synchronize": (C.c_int, [vp]),
+        "rp3dg_batch_snapshot": (C.c_int, [vp, C.POINTER(vp)]),
+        "rp3dg_batch_restore": (C.c_int, [vp, vp]),
+        "rp3dg_snapshot_bytes": (C.c_int64, [vp]),
+        "rp3dg_snapshot_destroy": (None, [vp]),
         "rp3dg_batch_get_counts": (C.c_int, [vp, C.POINTER(Counts)]),
         "rp3dg_batch_get_bodies": (C.c_int, [vp, fptr, fptr, fptr, fptr]),
         "rp3dg_batch_set_bodies": (C.c_int, [vp, fptr, fptr, fptr, fptr]),
@@ -330,6 +335,28 @@ def quickhull(points):
     return v[:nv.value].copy(), c[:nf.value].copy(), i[:int(c[:nf.value].sum())].copy()
 
 
+class Snapshot:
+    """A batch's state held in device memory (freed with the object)."""
+
+    def __init__(self, h):
+        self.h = h
+
+    @property
+    def nbytes(self) -> int:
+        return int(load().rp3dg_snapshot_bytes(self.h))
+
+    def close(self):
+        if self.h:
+            load().rp3dg_snapshot_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
 class Batch:
     """One or many worlds stepped together on the GPU."""
 
@@ -377,6 +404,16 @@ class Batch:
 
     def sync(self):
         _check(load().rp3dg_batch_synchronize(self.h), "synchronize")
+
+    def snapshot(self) -> "Snapshot":
+        """The batch's state as a device copy (rp3dg_batch_snapshot)."""
+        h = C.c_void_p()
+        _check(load().rp3dg_batch_snapshot(self.h, C.byref(h)), "batch_snapshot")
+        return Snapshot(h.value)
+
+    def restore(self, snap: "Snapshot"):
+        """Puts the batch back into the state of a snapshot taken from it (rp3dg_batch_restore)."""
+        _check(load().rp3dg_batch_restore(self.h, snap.h), "batch_restore")
 
     def counts(self) -> dict:
         c = Counts()
