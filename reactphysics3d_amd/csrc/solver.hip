@@ -256,18 +256,67 @@ __global__ __launch_bounds__(kWave) void sv_group_color(const int2* mb, const in
     for (int i = lane; i < nbg; i += kWave) s_mask[i] = 0ull;
     s_hist[lane] = 0;
     __syncthreads();
-    // greedy colouring, one world per lane; the body pairs are fetched eight manifolds ahead
+    // greedy colouring, one world per lane.  The chain of colour decisions is serial; the body pairs of
+    // the next kAhead manifolds are loaded while the current kAhead are coloured, so a global load's
+    // latency is paid once per world rather than once per batch.
+    constexpr int kAhead = 16;
     int cmax = -1;
-    if (lane < w1 - w0) {
+    if (w1 - w0 == 1 && mz - ma > 2 * kWave) {
+        // A group of one large world: the wave colours 64 manifolds at a time.  A manifold's colour is
+        // the smallest one used neither at its bodies by earlier chunks (the masks) nor by the earlier
+        // manifolds of its chunk that share a body with it.  Those dependencies point backwards, so
+        // iterating every lane's choice to a fixed point (at most one round per manifold of the chunk,
+        // plus one that sees no change) gives the colours of the sequential greedy pass.
+        for (int i0 = ma; i0 < mz; i0 += kWave) {
+            const int i = i0 + lane;
+            const int2 bb = i < mz ? mb[i] : make_int2(-1, -1);
+            unsigned long long pre = 0ull;
+            if (bb.x >= 0) pre |= s_mask[bb.x - b0];
+            if (bb.y >= 0) pre |= s_mask[bb.y - b0];
+            unsigned long long dep = 0ull;   // earlier lanes of the chunk sharing a dynamic body
+            for (int k = 0; k < kWave; k++) {
+                const int kx = __shfl(bb.x, k, kWave), ky = __shfl(bb.y, k, kWave);
+                const bool share = (kx >= 0 && (kx == bb.x || kx == bb.y)) || (ky >= 0 && (ky == bb.x || ky == bb.y));
+                if (k < lane && share) dep |= 1ull << k;
+            }
+            const int nd = wave_max(__popcll(dep));
+            unsigned long long used = pre;
+            int c = ~used ? __ffsll(~used) - 1 : kMaxColors - 1;
+            for (int round = 0; round <= kWave; round++) {
+                used = pre;
+                unsigned long long d = dep;
+                for (int t = 0; t < nd; t++) {   // every lane takes part in each shuffle
+                    const int k = d ? __ffsll(d) - 1 : lane;
+                    const int ck = __shfl(c, k, kWave);
+                    if (d) { used |= 1ull << ck; d &= d - 1; }
+                }
+                const int cn = ~used ? __ffsll(~used) - 1 : kMaxColors - 1;
+                const bool changed = cn != c;
+                c = cn;
+                if (!__ballot(changed)) break;
+            }
+            if (i < mz) {
+                if (!~used) atomicOr(status, STATUS_COLOR_OVERFLOW);
+                const unsigned long long bit = 1ull << c;
+                if (bb.x >= 0) atomicOr(&s_mask[bb.x - b0], bit);
+                if (bb.y >= 0) atomicOr(&s_mask[bb.y - b0], bit);
+                color[i] = c;
+                cmax = c > cmax ? c : cmax;
+            }
+            __syncthreads();
+        }
+    } else if (lane < w1 - w0) {
         const int s = wmstart[w0 + lane], e = wmstart[w0 + lane + 1];
-        for (int i = s; i < e; i += 8) {
-            int2 pre[8];
+        int2 cur[kAhead], nxt[kAhead];
 #pragma unroll
-            for (int u = 0; u < 8; u++) pre[u] = i + u < e ? mb[i + u] : make_int2(-1, -1);
+        for (int u = 0; u < kAhead; u++) cur[u] = s + u < e ? mb[s + u] : make_int2(-1, -1);
+        for (int i = s; i < e; i += kAhead) {
 #pragma unroll
-            for (int u = 0; u < 8; u++) {
+            for (int u = 0; u < kAhead; u++) nxt[u] = i + kAhead + u < e ? mb[i + kAhead + u] : make_int2(-1, -1);
+#pragma unroll
+            for (int u = 0; u < kAhead; u++) {
                 if (i + u >= e) break;
-                const int2 bb = pre[u];
+                const int2 bb = cur[u];
                 unsigned long long used = 0ull;
                 if (bb.x >= 0) used |= s_mask[bb.x - b0];
                 if (bb.y >= 0) used |= s_mask[bb.y - b0];
@@ -282,6 +331,8 @@ __global__ __launch_bounds__(kWave) void sv_group_color(const int2* mb, const in
                 color[i + u] = c;
                 cmax = c > cmax ? c : cmax;
             }
+#pragma unroll
+            for (int u = 0; u < kAhead; u++) cur[u] = nxt[u];
         }
     }
     cmax = wave_max(cmax);
