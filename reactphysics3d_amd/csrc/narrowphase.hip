@@ -95,6 +95,7 @@ __device__ inline Poly make_poly(const NpArgs& A, int collider) {
     } else {
         P.hull = (int)s.x;
         P.h = A.hulls[P.hull];
+        P.h.pad = 0.0f;   // a collider's hull is never a triangle: lets the triangle branches fold away
         P.hv = A.hull_verts;
         P.hn = A.hull_normals;
         P.hedges = A.hull_edges;
