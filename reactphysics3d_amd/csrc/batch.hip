@@ -43,6 +43,9 @@ __global__ void bp_traverse(DevColliders C, const int* sorted, int n, const int2
                             const float4* nmax, const int* root, NoColLists nc, int* count,
                             const int* offset, uint64_t* out_keys, uint64_t* slab, int capacity, int pass,
                             int* walk_list, int* walk_count, int* status);
+__global__ void bp_walk_wave(DevColliders C, const int* sorted, int n, const int2* child, const float4* nmin,
+                             const float4* nmax, const int* root, NoColLists ncl, const int* offset, uint64_t* out_keys,
+                             int capacity, const int* walk_list, const int* walk_count, int* status);
 __global__ void bp_copy_slabs(DevColliders C, int n, const int* count, const int* offset, const uint64_t* slab,
                               uint64_t* out_keys, int capacity, int* status);
 __global__ void bp_world_scan(DevColliders C, const int* gworld, const int* wcoll, const int* block_group, NoColLists L,
@@ -1155,9 +1158,9 @@ int rp3dg_batch::enqueue_broadphase(bool capturing, const DevEvents& E, int pg) 
         LAUNCH_T("bp_copy_pairs", grid_for(nc * kTravSlab, 256, 1 << 20), 256, bp_copy_slabs, C, nc, bp_count, bp_offs,
                  bp_slab, bp_out, pair_cap, dstatus);
         const int hw = capturing ? nc : std::min(nc, last.nb_walk_again + last.nb_walk_again / 4 + 256);
-        LAUNCH_T("bp_walk_again", grid_for(hw, kTravBlock, 1 << 20), kTravBlock, bp_traverse, C, vls, nc, bp_child,
-                 bp_nmin, bp_nmax, bp_root, nocol_lists, bp_count, bp_offs, bp_out, bp_slab, pair_cap, 1, bp_walk,
-                 &counters->nb_walk_again, dstatus);
+        LAUNCH_T("bp_walk_again", std::max(1, std::min(hw, 1 << 16)), kWave, bp_walk_wave, C, vls, nc, bp_child,
+                 bp_nmin, bp_nmax, bp_root, nocol_lists, bp_offs, bp_out, pair_cap, bp_walk, &counters->nb_walk_again,
+                 dstatus);
     }
     if (events_on) fill_int_n<<<pg, 256, 0, stream>>>(O.kept, pair_cap, old_count, 0);
     LAUNCH_T("bp_merge", pg, 256, bp_merge, C, bp_out, bp_offs + nc, pair_cap, O, old_count, old_offs, N, new_count,

@@ -428,6 +428,111 @@ __global__ __launch_bounds__(kTravBlock) void bp_traverse(DevColliders C, const 
     }
 }
 
+// Second traversal pass, one wavefront per collider whose partners overflowed its slab (a ground or
+// wall overlapping hundreds of bodies).  The overlapping top of the tree is expanded breadth-first
+// into at most 64 subtrees, each lane walks one with the filters of bp_traverse, the keys go to the
+// collider's segment, and the segment is sorted: in LDS by a bitonic network when it fits, else by
+// lane 0.  Keys are unique (each leaf is reached once), so the segment equals the one pass 1 of
+// bp_traverse produces.
+constexpr int kWalkSort = 4096;   // keys sorted in LDS (32 KB)
+__global__ __launch_bounds__(kWave) void bp_walk_wave(DevColliders C, const int* sorted, int n, const int2* child,
+                                                      const float4* nmin, const float4* nmax, const int* root,
+                                                      NoColLists ncl, const int* offset, uint64_t* out_keys,
+                                                      int capacity, const int* walk_list, const int* walk_count,
+                                                      int* status) {
+    __shared__ uint64_t s_keys[kWalkSort];
+    __shared__ int s_front[2 * kWave];
+    __shared__ int s_nf, s_cnt;
+    const int lane = threadIdx.x;
+    const int nw = *walk_count;
+    for (int s = blockIdx.x; s < nw; s += gridDim.x) {
+        const int c = walk_list[s];
+        const int base = offset[c];
+        const float4 amn = C.amin[c], amx = C.amax[c];
+        if (lane == 0) { s_front[0] = root[C.world[c]]; s_nf = 1; s_cnt = 0; }
+        __syncthreads();
+        // breadth-first expansion while the frontier holds at most kWave / 2 nodes and some are internal
+        for (int level = 0; level < 64; level++) {
+            const int nf = s_nf;
+            const int node = lane < nf ? s_front[lane] : -1;
+            const bool internal = node >= 0 && node < n - 1;
+            if (nf > kWave / 2 || !__ballot(internal)) break;
+            const bool ov = node >= 0 && aabb_overlap(amn, amx, nmin[node], nmax[node]);
+            const int emit = !ov ? 0 : (internal ? 2 : 1);
+            const int incl = wave_incl_scan(emit);
+            const int off = incl - emit;
+            const int tot = __shfl(incl, kWave - 1, kWave);
+            __syncthreads();   // every lane has read the frontier
+            if (emit == 1) s_front[off] = node;
+            if (emit == 2) {
+                const int2 ch = child[node];
+                s_front[off] = ch.x;
+                s_front[off + 1] = ch.y;
+            }
+            if (lane == 0) s_nf = tot;
+            __syncthreads();
+        }
+        const int nf = s_nf;
+        for (int f = lane; f < nf; f += kWave) {
+            int stack[kTravStack];
+            int sp = 0;
+            stack[sp++] = s_front[f];
+            while (sp > 0) {
+                const int node = stack[--sp];
+                if (!aabb_overlap(amn, amx, nmin[node], nmax[node])) continue;
+                if (node >= n - 1) {
+                    const int o = sorted[node - (n - 1)];
+                    if (o == c) continue;
+                    if (!C.is_static[o] && o < c) continue;
+                    if (!pair_allowed(C, c, o, ncl)) continue;
+                    const uint64_t key = ((uint64_t)(uint32_t)c << 32) | (uint32_t)o;
+                    const int pos = atomicAdd(&s_cnt, 1);
+                    if (pos < kWalkSort) s_keys[pos] = key;
+                    if (base + pos < capacity) out_keys[base + pos] = key;
+                    else atomicOr(status, STATUS_PAIR_OVERFLOW);
+                } else {
+                    if (sp + 2 > kTravStack) { atomicOr(status, STATUS_STACK_OVERFLOW); break; }
+                    const int2 ch = child[node];
+                    stack[sp++] = ch.x;
+                    stack[sp++] = ch.y;
+                }
+            }
+        }
+        __syncthreads();
+        const int nk = s_cnt;
+        const int nsort = min(nk, capacity - base);   // keys present in the segment
+        if (nk <= kWalkSort) {
+            int P = 1;
+            while (P < nk) P <<= 1;
+            for (int i = nk + lane; i < P; i += kWave) s_keys[i] = ~0ull;
+            __syncthreads();
+            for (int k = 2; k <= P; k <<= 1)
+                for (int j = k >> 1; j > 0; j >>= 1) {
+                    for (int i = lane; i < P; i += kWave) {
+                        const int ixj = i ^ j;
+                        if (ixj > i) {
+                            const uint64_t a = s_keys[i], b = s_keys[ixj];
+                            if ((a > b) == ((i & k) == 0)) { s_keys[i] = b; s_keys[ixj] = a; }
+                        }
+                    }
+                    __syncthreads();
+                }
+            for (int i = lane; i < nsort; i += kWave) out_keys[base + i] = s_keys[i];
+        } else if (lane == 0) {
+            for (int i = 1; i < nsort; i++) {
+                const uint64_t key = out_keys[base + i];
+                int pos = base + i;
+                while (pos > base && out_keys[pos - 1] > key) {
+                    out_keys[pos] = out_keys[pos - 1];
+                    pos--;
+                }
+                out_keys[pos] = key;
+            }
+        }
+        __syncthreads();   // the frontier and keys of this collider are done before the next one
+    }
+}
+
 // Second traversal pass for the colliders with at most kTravSlab partners: their sorted partners are
 // in the slab; eight lanes per collider copy them to the collider's segment of the key array.  Static
 // colliders copy their static-static partner list.
